@@ -1,0 +1,161 @@
+/*
+ * zav1_dsp.h — C-ABI of the MI355X (gfx950) backend for zenav1-svt's
+ * data-parallel intra-encode DSP kernels.
+ *
+ * Two forms share one set of HIP kernels:
+ *
+ *  1. The PLAIN form: a versioned function table (zav1_dsp_v1) whose entries
+ *     mirror, argument for argument, the reference's per-kernel entry points
+ *     that its encoder calls through archmage tier dispatch.  Pointers are
+ *     HOST pointers (caller-owned slices, never retained); each call copies
+ *     its inputs to HBM, runs one launch on the calling thread's stream and
+ *     copies the result back.  This is what a drop-in FFI crate binds
+ *     (see INTEGRATION.md for the Rust binding).
+ *
+ *  2. The BATCHED form: zav1_*_batch() entry points taking DEVICE pointers to
+ *     many jobs at once, ordered on a caller-supplied HIP stream.  The loop
+ *     filter, candidate-RD and serving drivers build on these.
+ *
+ * Status codes (map onto the reference's EncodeError, typ/error.rs:22-40):
+ *   ZAV1_OK 0, ZAV1_EINVAL bad argument, ZAV1_ENOTSUP unsupported combination
+ *   (e.g. ADST on a 32/64 dimension — the reference's `false` return),
+ *   ZAV1_ENOMEM device allocation / HBM budget, ZAV1_ECANCELED, ZAV1_EIO HIP
+ *   runtime failure.  No exception crosses the ABI.  Every entry point is
+ *   thread-safe; the plain form uses the per-thread default stream.
+ *
+ * Enumerations follow the reference (C SVT-AV1) order:
+ *   tx_size: rust/crates/svtav1-types/src/transform.rs:7-29 (19 sizes)
+ *   tx_type: rust/crates/svtav1-types/src/transform.rs:84-101 (16 types)
+ */
+#ifndef ZAV1_DSP_H
+#define ZAV1_DSP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZAV1_ABI_VERSION 1u
+
+#define ZAV1_OK 0
+#define ZAV1_EINVAL (-22)
+#define ZAV1_ENOTSUP (-95)
+#define ZAV1_ENOMEM (-12)
+#define ZAV1_ECANCELED (-125)
+#define ZAV1_EIO (-5)
+
+typedef void *zav1_stream_t; /* a hipStream_t; NULL = the per-thread default stream */
+
+/* ------------------------------------------------------------------------ */
+/* Runtime plumbing (lets hosts without their own GPU runtime drive HBM).    */
+/* ------------------------------------------------------------------------ */
+int zav1_init(int device);                   /* select + initialise a device */
+int zav1_device_count(void);
+int zav1_malloc(void **dptr, size_t bytes);  /* device memory */
+int zav1_free(void *dptr);
+int zav1_host_alloc(void **hptr, size_t bytes); /* pinned host memory */
+int zav1_host_free(void *hptr);
+int zav1_memcpy_h2d(void *dst, const void *src, size_t bytes, zav1_stream_t s);
+int zav1_memcpy_d2h(void *dst, const void *src, size_t bytes, zav1_stream_t s);
+int zav1_memcpy_d2d(void *dst, const void *src, size_t bytes, zav1_stream_t s);
+int zav1_memset(void *dst, int value, size_t bytes, zav1_stream_t s);
+int zav1_stream_create(zav1_stream_t *s);
+int zav1_stream_destroy(zav1_stream_t s);
+int zav1_stream_sync(zav1_stream_t s);
+int zav1_device_sync(void);
+/* Event timing on a stream: record start, run work, record stop, elapsed. */
+int zav1_event_create(void **ev);
+int zav1_event_destroy(void *ev);
+int zav1_event_record(void *ev, zav1_stream_t s);
+int zav1_event_elapsed_ms(void *start, void *stop, float *ms);
+const char *zav1_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Quantizer rows (host side, no device work).                               */
+/* ------------------------------------------------------------------------ */
+typedef struct zav1_quant_row {
+  int32_t zbin[2], round[2], quant[2], quant_shift[2];
+  int32_t round_fp[2], quant_fp[2], dequant[2];
+} zav1_quant_row;
+
+/* Replaces build_quant_table_bd (rust/crates/svtav1-encoder/src/quant.rs:180),
+ * sharpness 0.  bd in {8, 10}.  [0] = DC, [1] = AC. */
+int zav1_build_quant_row(int qindex, int bd, zav1_quant_row *row);
+
+/* ------------------------------------------------------------------------ */
+/* PLAIN form: function table, host pointers, one launch per call.           */
+/* ------------------------------------------------------------------------ */
+typedef struct zav1_dsp_v1 {
+  uint32_t abi_version; /* ZAV1_ABI_VERSION */
+  uint32_t backend;     /* 1 = HIP gfx950 */
+
+  /* fwd_txfm2d_dispatch (rust/crates/svtav1-dsp/src/txfm_dispatch.rs:18).
+   * in: w x h residual with row stride `stride`; out: w x h raster.
+   * Returns 1 on success, 0 when no kernel exists (the reference's false),
+   * negative on runtime failure. */
+  int (*fwd_txfm2d)(const int32_t *in, int32_t *out, size_t stride, uint8_t tx_size,
+                    uint8_t tx_type);
+
+  /* inv_txfm2d_dispatch_bd (rust/crates/svtav1-dsp/src/txfm_dispatch.rs:55).
+   * in: coefficients with row stride `stride` (top-left 32x32 read for 64-dim);
+   * out: residual with row stride `stride`. bd in {8, 10}. 1 / 0 / negative. */
+  int (*inv_txfm2d_bd)(const int32_t *in, int32_t *out, size_t stride, uint8_t tx_size,
+                       uint8_t tx_type, uint8_t bd);
+
+  /* quantize_fp_raster (rust/crates/svtav1-dsp/src/quant_coding.rs:58), bd8:
+   * `rounding` is the pre-shifted _fp round row. */
+  int (*quantize_fp_raster)(const int32_t *coeffs, int32_t *qcoeff, int32_t *dqcoeff, size_t n,
+                            const int32_t rounding[2], const int32_t quant_fp[2],
+                            const int32_t dequant[2], int32_t log_scale);
+
+  /* quantize_b_raster (rust/crates/svtav1-dsp/src/quant_coding.rs:336), bd8:
+   * zbins/round pre-shifted by log_scale as the caller (quant.rs:255) does. */
+  int (*quantize_b_raster)(const int32_t *coeffs, int32_t *qcoeff, int32_t *dqcoeff, size_t n,
+                           const int32_t zbins[2], const int32_t round[2],
+                           const int32_t quant[2], const int32_t quant_shift[2],
+                           const int32_t dequant[2], int32_t log_scale);
+
+  /* sse_i32 / sq_sum_i32 (rust/crates/svtav1-dsp/src/residual.rs:245/:364):
+   * wrapping u64 result through *out. b == NULL gives sq_sum. */
+  int (*sse_i32)(const int32_t *a, const int32_t *b, size_t n, uint64_t *out);
+
+  /* aom_hadamard_{4x4,8x8,16x16,32x32} + aom_satd
+   * (rust/crates/svtav1-dsp/src/hadamard.rs:691,719,885,913,942).
+   * src_diff: n x n i16 residual, row stride `stride`; coeff (n*n, may be
+   * NULL) receives the reference's coefficient order; *satd the SATD. */
+  int (*hadamard_satd)(const int16_t *src_diff, size_t stride, int n, int32_t *coeff,
+                       int32_t *satd);
+} zav1_dsp_v1;
+
+/* Returns the table for `backend` (1 = HIP), or NULL if no GPU is usable. */
+const zav1_dsp_v1 *zav1_dsp_get(int backend);
+
+/* ------------------------------------------------------------------------ */
+/* BATCHED form: device pointers, stream-ordered, many jobs per launch.      */
+/* Jobs are packed back to back (job j at offset j * w * h).                 */
+/* ------------------------------------------------------------------------ */
+
+/* n_jobs forward 2D transforms of one tx_size; types[j] per job (device). */
+int zav1_fwd_txfm2d_batch(const int32_t *d_in, int32_t *d_out, const uint8_t *d_types,
+                          int tx_size, int n_jobs, zav1_stream_t s);
+/* n_jobs inverse 2D transforms (residual out, HIGHBD_WRAPLOW at bd). */
+int zav1_inv_txfm2d_batch(const int32_t *d_in, int32_t *d_out, const uint8_t *d_types,
+                          int tx_size, int n_jobs, int bd, zav1_stream_t s);
+/* Quantize n_jobs blocks of n_per_job coefficients; d_qidx[j] selects the
+ * quantizer row of qindex d_qidx[j] at `bd`.  kind 0 = fp, 1 = b. */
+int zav1_quantize_batch(const int32_t *d_coeff, int32_t *d_qcoeff, int32_t *d_dqcoeff,
+                        const uint8_t *d_qidx, int n_per_job, int n_jobs, int log_scale, int bd,
+                        int kind, zav1_stream_t s);
+/* d_out[j] = sum over job j of (a - b)^2 (wrapping u64); d_b NULL = sq_sum. */
+int zav1_sse_i32_batch(const int32_t *d_a, const int32_t *d_b, uint64_t *d_out, int n_per_job,
+                       int n_jobs, zav1_stream_t s);
+/* n x n Hadamard + SATD of packed i16 tiles; d_coeff may be NULL. */
+int zav1_satd_batch(const int16_t *d_res, int32_t *d_coeff, int32_t *d_satd, int n, int n_jobs,
+                    zav1_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,88 @@
+/*
+ * zav1 oracle — CPU restatement of zenav1-svt's intra-encode DSP kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, its
+ * C-ABI, the host drivers) may include, link or call this code.  Only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark use it,
+ * and only as the checker.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the zenav1-svt checkout):
+ *   dsp/ = rust/crates/svtav1-dsp/src/
+ *   enc/ = rust/crates/svtav1-encoder/src/
+ *   lf/  = rust/crates/svtav1-encoder/src/leaf_funnel/
+ *
+ * Integer semantics: the reference computes butterflies with i64 products
+ * and i32 sums; this file is compiled with -fwrapv so every i32 wrap is
+ * defined and equals the reference's two's-complement result.
+ *
+ * Parity pinning (how each group is checked, see tests/):
+ *   1D transforms       golden vectors extracted from C SVT-AV1 by the
+ *                       reference (svtav1/tests/golden_parity.rs:36-720),
+ *                       cos/sin table closed form, float-DCT closeness.
+ *   2D transforms       structure pinned by the 1D goldens; 2D composition
+ *                       pinned by separability/flip invariants.
+ *   quantizers          per-coefficient brute force of the quoted formulas,
+ *                       dequant bounds; table rows by closed form.
+ *   hadamard / SATD     brute-force Walsh matrix on small-range input.
+ */
+#ifndef ZAV1_ORACLE_H
+#define ZAV1_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums in C order (typ/transform.rs:7-29, :84-101) ---------------- */
+enum {
+  OR_TX_4X4 = 0, OR_TX_8X8, OR_TX_16X16, OR_TX_32X32, OR_TX_64X64,
+  OR_TX_4X8, OR_TX_8X4, OR_TX_8X16, OR_TX_16X8, OR_TX_16X32, OR_TX_32X16,
+  OR_TX_32X64, OR_TX_64X32, OR_TX_4X16, OR_TX_16X4, OR_TX_8X32, OR_TX_32X8,
+  OR_TX_16X64, OR_TX_64X16, OR_TX_SIZES_ALL
+};
+
+/* ---- transforms ------------------------------------------------------- */
+void or_tx_size_dims(int tx_size, int *w, int *h);
+/* 1D kernels, exposed for the golden-vector tests.  kind: 0 DCT, 1 ADST,
+ * 3 identity.  inverse uses INV_COS_BIT 12 and `range` clamps. */
+int or_fwd_txfm1d(int kind, int n, const int32_t *in, int32_t *out, int cos_bit);
+int or_inv_txfm1d(int kind, int n, const int32_t *in, int32_t *out, int range);
+/* dsp/txfm_dispatch.rs:18 fwd_txfm2d_dispatch: returns 1 ok, 0 unsupported. */
+int or_fwd_txfm2d(const int32_t *in, int32_t *out, int stride, int tx_size, int tx_type);
+/* dsp/txfm_dispatch.rs:55 inv_txfm2d_dispatch_bd (residual out, stride). */
+int or_inv_txfm2d(const int32_t *in, int32_t *out, int stride, int tx_size, int tx_type, int bd);
+const int32_t *or_cospi_row(int cos_bit);
+const int32_t *or_sinpi_row(int cos_bit);
+
+/* ---- quantizers ------------------------------------------------------- */
+typedef struct {
+  int32_t zbin[2], round[2], quant[2], quant_shift[2];
+  int32_t round_fp[2], quant_fp[2], dequant[2];
+} or_quant_row;
+/* enc/quant.rs:132-236 build_quant_table(_bd) */
+void or_build_quant_row(int qindex, int bd, or_quant_row *row);
+/* dsp/quant_coding.rs:80 fp_one / enc/quant.rs:286,335 (hbd drops clamp) */
+void or_quantize_fp(const int32_t *coeff, int n, const or_quant_row *t, int log_scale, int bd,
+                    int32_t *qcoeff, int32_t *dqcoeff);
+/* dsp/quant_coding.rs:365 b_one / enc/quant.rs:247,391 */
+void or_quantize_b(const int32_t *coeff, int n, const or_quant_row *t, int log_scale, int bd,
+                   int32_t *qcoeff, int32_t *dqcoeff);
+int or_tx_scale(int tx_size); /* enc/quant.rs:238 TX_SCALE_TAB */
+
+/* ---- residual / distortion ------------------------------------------- */
+uint64_t or_sse_i32(const int32_t *a, const int32_t *b, int n);      /* dsp/residual.rs:245 */
+uint64_t or_sq_sum_i32(const int32_t *a, int n);                     /* dsp/residual.rs:364 */
+uint64_t or_sse_pix(const void *src, int src_stride, const void *ref, int ref_stride,
+                    int w, int h, int hbd);                          /* dsp/variance.rs:124 */
+void or_hadamard(int n, const int16_t *src, int stride, int32_t *coeff); /* dsp/hadamard.rs:691-913 */
+int32_t or_satd(const int32_t *coeff, int n);                           /* dsp/hadamard.rs:942 */
+/* lf/predict.rs:139/185 hadamard_satd(_hbd): block SATD over square tiles. */
+uint64_t or_block_satd(const void *src, int src_stride, const void *pred, int pred_stride,
+                       int w, int h, int hbd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+"""zav1: MI355X (gfx950) backend for zenav1-svt's intra-encode DSP kernels.
+
+Python host mirror of the C-ABI in include/zav1_dsp.h (ctypes, no torch).
+The HIP library must be built (build.py); if it is missing, every entry
+point raises — there is no CPU fallback in this package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, 'libzav1_hip.so')
+HEADER = os.path.join(ROOT, 'include', 'zav1_dsp.h')
+
+ZAV1_OK = 0
+ZAV1_EINVAL = -22
+ZAV1_ENOTSUP = -95
+ZAV1_ENOMEM = -12
+ZAV1_ECANCELED = -125
+ZAV1_EIO = -5
+
+TX_DIMS = [(4, 4), (8, 8), (16, 16), (32, 32), (64, 64), (4, 8), (8, 4), (8, 16), (16, 8), (16, 32),
+           (32, 16), (32, 64), (64, 32), (4, 16), (16, 4), (8, 32), (32, 8), (16, 64), (64, 16)]
+
+
+class Zav1Error(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__('%s failed with %d: %s' % (what, code, last_error()))
+        self.code = code
+
+
+def build():
+    from importlib import util
+    spec = util.spec_from_file_location('zav1_build', os.path.join(HERE, 'build.py'))
+    mod = util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.build()
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int
+_lib = None
+
+
+class QuantRow(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32 * 2) for n in
+                ('zbin', 'round', 'quant', 'quant_shift', 'round_fp', 'quant_fp', 'dequant')]
+
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I16P = ctypes.POINTER(ctypes.c_int16)
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+
+
+class DspV1(ctypes.Structure):
+    _fields_ = [
+        ('abi_version', ctypes.c_uint32),
+        ('backend', ctypes.c_uint32),
+        ('fwd_txfm2d', ctypes.CFUNCTYPE(_i32, _I32P, _I32P, ctypes.c_size_t, ctypes.c_uint8, ctypes.c_uint8)),
+        ('inv_txfm2d_bd', ctypes.CFUNCTYPE(_i32, _I32P, _I32P, ctypes.c_size_t, ctypes.c_uint8, ctypes.c_uint8,
+                                           ctypes.c_uint8)),
+        ('quantize_fp_raster', ctypes.CFUNCTYPE(_i32, _I32P, _I32P, _I32P, ctypes.c_size_t, _I32P, _I32P, _I32P,
+                                                ctypes.c_int32)),
+        ('quantize_b_raster', ctypes.CFUNCTYPE(_i32, _I32P, _I32P, _I32P, ctypes.c_size_t, _I32P, _I32P, _I32P,
+                                               _I32P, _I32P, ctypes.c_int32)),
+        ('sse_i32', ctypes.CFUNCTYPE(_i32, _I32P, _I32P, ctypes.c_size_t, _U64P)),
+        ('hadamard_satd', ctypes.CFUNCTYPE(_i32, _I16P, ctypes.c_size_t, _i32, _I32P, _I32P)),
+    ]
+
+
+# (name, argtypes) of every exported C-ABI symbol; mirrors include/zav1_dsp.h.
+EXPORTS = {
+    'zav1_init': [_i32],
+    'zav1_device_count': [],
+    'zav1_malloc': [ctypes.POINTER(_vp), ctypes.c_size_t],
+    'zav1_free': [_vp],
+    'zav1_host_alloc': [ctypes.POINTER(_vp), ctypes.c_size_t],
+    'zav1_host_free': [_vp],
+    'zav1_memcpy_h2d': [_vp, _vp, ctypes.c_size_t, _vp],
+    'zav1_memcpy_d2h': [_vp, _vp, ctypes.c_size_t, _vp],
+    'zav1_memcpy_d2d': [_vp, _vp, ctypes.c_size_t, _vp],
+    'zav1_memset': [_vp, _i32, ctypes.c_size_t, _vp],
+    'zav1_stream_create': [ctypes.POINTER(_vp)],
+    'zav1_stream_destroy': [_vp],
+    'zav1_stream_sync': [_vp],
+    'zav1_device_sync': [],
+    'zav1_event_create': [ctypes.POINTER(_vp)],
+    'zav1_event_destroy': [_vp],
+    'zav1_event_record': [_vp, _vp],
+    'zav1_event_elapsed_ms': [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    'zav1_last_error': [],
+    'zav1_build_quant_row': [_i32, _i32, ctypes.POINTER(QuantRow)],
+    'zav1_dsp_get': [_i32],
+    'zav1_fwd_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
+    'zav1_inv_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _i32, _vp],
+    'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
+    'zav1_sse_i32_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
+    'zav1_satd_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
+}
+
+
+def lib():
+    """Load libzav1_hip.so (raises if it was never built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError('zav1: %s is not built (run zenav1-svt_amd/build.py); '
+                               'there is no CPU fallback' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _i32
+        L.zav1_last_error.restype = ctypes.c_char_p
+        L.zav1_dsp_get.restype = ctypes.POINTER(DspV1)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    try:
+        return lib().zav1_last_error().decode()
+    except Exception:  # noqa: BLE001 - diagnostics only
+        return ''
+
+
+def check(rc, what):
+    if rc < 0:
+        raise Zav1Error(rc, what)
+    return rc
+
+
+def init(device=0):
+    check(lib().zav1_init(device), 'zav1_init')
+
+
+def device_count():
+    return lib().zav1_device_count()
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype/shape, owned by Python."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        check(lib().zav1_malloc(ctypes.byref(p), self.nbytes), 'zav1_malloc')
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.shape, arr.dtype)
+        d.upload(arr, stream)
+        return d
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes
+        check(lib().zav1_memcpy_h2d(self.ptr, arr.ctypes.data, self.nbytes, stream), 'memcpy_h2d')
+        check(lib().zav1_stream_sync(stream), 'stream_sync')
+
+    def to_host(self, stream=None):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().zav1_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes, stream), 'memcpy_d2h')
+        return out
+
+    def zero(self, stream=None):
+        check(lib().zav1_memset(self.ptr, 0, self.nbytes, stream), 'memset')
+
+    def free(self):
+        if self.ptr:
+            lib().zav1_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001 - interpreter teardown
+            pass
+
+
+def sync(stream=None):
+    check(lib().zav1_stream_sync(stream), 'stream_sync')
+
+
+def quant_row(qindex, bd):
+    r = QuantRow()
+    check(lib().zav1_build_quant_row(qindex, bd, ctypes.byref(r)), 'build_quant_row')
+    return r
+
+
+def dsp_table():
+    p = lib().zav1_dsp_get(1)
+    if not p:
+        raise RuntimeError('zav1: no usable GPU for the HIP backend')
+    return p.contents
+
+
+# ---- batched wrappers (device arrays) --------------------------------------
+def fwd_txfm2d_batch(d_in, d_out, d_types, tx_size, n_jobs, stream=None):
+    check(lib().zav1_fwd_txfm2d_batch(d_in.ptr, d_out.ptr, d_types.ptr, tx_size, n_jobs, stream),
+          'fwd_txfm2d_batch')
+
+
+def inv_txfm2d_batch(d_in, d_out, d_types, tx_size, n_jobs, bd, stream=None):
+    check(lib().zav1_inv_txfm2d_batch(d_in.ptr, d_out.ptr, d_types.ptr, tx_size, n_jobs, bd, stream),
+          'inv_txfm2d_batch')
+
+
+def quantize_batch(d_coeff, d_q, d_dq, d_qidx, n_per_job, n_jobs, log_scale, bd, kind, stream=None):
+    check(lib().zav1_quantize_batch(d_coeff.ptr, d_q.ptr, d_dq.ptr, d_qidx.ptr, n_per_job, n_jobs,
+                                    log_scale, bd, kind, stream), 'quantize_batch')
+
+
+def sse_i32_batch(d_a, d_b, d_out, n_per_job, n_jobs, stream=None):
+    check(lib().zav1_sse_i32_batch(d_a.ptr, d_b.ptr if d_b is not None else None, d_out.ptr, n_per_job,
+                                   n_jobs, stream), 'sse_i32_batch')
+
+
+def satd_batch(d_res, d_coeff, d_satd, n, n_jobs, stream=None):
+    check(lib().zav1_satd_batch(d_res.ptr, d_coeff.ptr if d_coeff is not None else None, d_satd.ptr, n,
+                                n_jobs, stream), 'satd_batch')

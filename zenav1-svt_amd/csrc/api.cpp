@@ -1,0 +1,409 @@
+// C-ABI of the zav1 gfx950 backend: runtime plumbing, quantizer rows, the
+// plain per-call function table and the batched entry points.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/zav1_dsp.h"
+#include "av1_qlookup_tables.h"
+#include "zav1_dev.h"
+#include "zav1_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(hipError_t e, const char* where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? ZAV1_ENOMEM : ZAV1_EIO;
+}
+#define ZCHECK(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return fail(e_, #call);      \
+  } while (0)
+
+hipStream_t S(zav1_stream_t s) { return s ? (hipStream_t)s : hipStreamPerThread; }
+
+// Per-device table of quantizer rows: [bd index][qindex], bd index 0 = 8, 1 = 10.
+struct DeviceState {
+  zav1::QuantRow* rows = nullptr;
+};
+std::mutex g_mu;
+std::vector<DeviceState> g_dev;
+
+void build_row(int qindex, int bd, zav1::QuantRow* t) {
+  // rust/crates/svtav1-encoder/src/quant.rs:180 build_quant_table_bd_sharp (sharpness 0);
+  // qzbin per rust/crates/svtav1-encoder/src/bd10.rs:52; invert_quant per quant.rs:69.
+  const int32_t dc = bd == 10 ? DC_QLOOKUP_10[qindex] : DC_QLOOKUP_8[qindex];
+  const int32_t ac = bd == 10 ? AC_QLOOKUP_10[qindex] : AC_QLOOKUP_8[qindex];
+  const int th = bd == 10 ? 592 : 148;
+  const int qzbin = qindex == 0 ? 64 : (dc < th ? 84 : 80);
+  const int qround = qindex == 0 ? 64 : 48;
+  const int32_t q[2] = {dc, ac};
+  for (int i = 0; i < 2; i++) {
+    uint32_t tt = (uint32_t)q[i];
+    int l = 0;
+    while (tt > 1) { tt >>= 1; l++; }
+    const int64_t m = 1 + ((int64_t)1 << (16 + l)) / q[i];
+    t->quant[i] = (int32_t)(m - (1 << 16));
+    t->quant_shift[i] = 1 << (16 - l);
+    t->zbin[i] = (qzbin * q[i] + 64) >> 7;
+    t->round[i] = (qround * q[i]) >> 7;
+    t->quant_fp[i] = (1 << 16) / q[i];
+    t->round_fp[i] = (64 * q[i]) >> 7;
+    t->dequant[i] = q[i];
+  }
+}
+
+// Device pointer to the 2 x 256 quantizer-row table of the current device.
+int quant_rows(const zav1::QuantRow** out) {
+  int dev = 0;
+  ZCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+  if (!g_dev[dev].rows) {
+    std::vector<zav1::QuantRow> h(512);
+    for (int b = 0; b < 2; b++)
+      for (int q = 0; q < 256; q++) build_row(q, b ? 10 : 8, &h[b * 256 + q]);
+    zav1::QuantRow* d = nullptr;
+    ZCHECK(hipMalloc(&d, sizeof(zav1::QuantRow) * 512));
+    ZCHECK(hipMemcpy(d, h.data(), sizeof(zav1::QuantRow) * 512, hipMemcpyHostToDevice));
+    g_dev[dev].rows = d;
+  }
+  *out = g_dev[dev].rows;
+  return ZAV1_OK;
+}
+
+// Per-thread device staging for the plain form (grown on demand, kept).
+struct Staging {
+  char* d = nullptr;
+  size_t cap = 0;
+  int dev = -1;
+  ~Staging() {
+    if (d) (void)hipFree(d);
+  }
+};
+thread_local Staging t_stage;
+
+int staging(size_t bytes, char** out) {
+  int dev = 0;
+  ZCHECK(hipGetDevice(&dev));
+  if (t_stage.cap < bytes || t_stage.dev != dev) {
+    if (t_stage.d) (void)hipFree(t_stage.d);
+    t_stage.d = nullptr;
+    t_stage.cap = 0;
+    size_t cap = bytes < (1u << 20) ? (1u << 20) : bytes;
+    ZCHECK(hipMalloc(&t_stage.d, cap));
+    t_stage.cap = cap;
+    t_stage.dev = dev;
+  }
+  *out = t_stage.d;
+  return ZAV1_OK;
+}
+
+bool valid_tx(int tx_size, int tx_type) { return tx_size >= 0 && tx_size < 19 && tx_type >= 0 && tx_type < 16; }
+bool tx_supported(int tx_size, int tx_type) {
+  const zav1::TypeCfg c = zav1::type_cfg(tx_type);
+  return zav1::kind_ok(c.col, zav1::tx_h(tx_size)) && zav1::kind_ok(c.row, zav1::tx_w(tx_size));
+}
+
+// ---- plain form ------------------------------------------------------------
+int plain_fwd_txfm2d(const int32_t* in, int32_t* out, size_t stride, uint8_t tx_size, uint8_t tx_type) {
+  if (!in || !out || !valid_tx(tx_size, tx_type)) return ZAV1_EINVAL;
+  if (!tx_supported(tx_size, tx_type)) return 0;
+  const int w = zav1::tx_w(tx_size), h = zav1::tx_h(tx_size), n = w * h;
+  if (stride < (size_t)w) return ZAV1_EINVAL;
+  std::vector<int32_t> packed((size_t)n);
+  for (int r = 0; r < h; r++) std::memcpy(&packed[(size_t)r * w], in + r * stride, sizeof(int32_t) * w);
+  char* st = nullptr;
+  int rc = staging(sizeof(int32_t) * 2 * n + 64, &st);
+  if (rc) return rc;
+  int32_t* din = (int32_t*)st;
+  int32_t* dout = din + n;
+  uint8_t* dty = (uint8_t*)(dout + n);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(din, packed.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(dty, &tx_type, 1, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::fwd_txfm2d_batch(din, dout, dty, tx_size, 1, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return 1;
+}
+
+int plain_inv_txfm2d_bd(const int32_t* in, int32_t* out, size_t stride, uint8_t tx_size, uint8_t tx_type,
+                        uint8_t bd) {
+  if (!in || !out || !valid_tx(tx_size, tx_type) || (bd != 8 && bd != 10)) return ZAV1_EINVAL;
+  if (!tx_supported(tx_size, tx_type)) return 0;
+  const int w = zav1::tx_w(tx_size), h = zav1::tx_h(tx_size), n = w * h;
+  if (stride < (size_t)(w < 32 ? w : 32)) return ZAV1_EINVAL;
+  std::vector<int32_t> packed((size_t)n, 0);
+  for (int r = 0; r < h && r < 32; r++)
+    for (int c = 0; c < w && c < 32; c++) packed[(size_t)r * w + c] = in[r * stride + c];
+  char* st = nullptr;
+  int rc = staging(sizeof(int32_t) * 2 * n + 64, &st);
+  if (rc) return rc;
+  int32_t* din = (int32_t*)st;
+  int32_t* dout = din + n;
+  uint8_t* dty = (uint8_t*)(dout + n);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(din, packed.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(dty, &tx_type, 1, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::inv_txfm2d_batch(din, dout, dty, tx_size, 1, bd, s));
+  ZCHECK(hipMemcpyAsync(packed.data(), dout, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  for (int r = 0; r < h; r++) std::memcpy(out + r * stride, &packed[(size_t)r * w], sizeof(int32_t) * w);
+  return 1;
+}
+
+int plain_quant(const int32_t* coeffs, int32_t* q, int32_t* dq, size_t n, const zav1::PlainQuant& p) {
+  if (!coeffs || !q || !dq) return ZAV1_EINVAL;
+  if (n == 0) return ZAV1_OK;
+  if (p.log_scale < 0 || p.log_scale > 2) return ZAV1_EINVAL;
+  char* st = nullptr;
+  int rc = staging(sizeof(int32_t) * 3 * n, &st);
+  if (rc) return rc;
+  int32_t* dc = (int32_t*)st;
+  int32_t* dqq = dc + n;
+  int32_t* ddq = dqq + n;
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dc, coeffs, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::quantize_plain(dc, dqq, ddq, (int)n, p, s));
+  ZCHECK(hipMemcpyAsync(q, dqq, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipMemcpyAsync(dq, ddq, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+
+int plain_quantize_fp_raster(const int32_t* coeffs, int32_t* qcoeff, int32_t* dqcoeff, size_t n,
+                             const int32_t rounding[2], const int32_t quant_fp[2], const int32_t dequant[2],
+                             int32_t log_scale) {
+  if (!rounding || !quant_fp || !dequant) return ZAV1_EINVAL;
+  zav1::PlainQuant p{};
+  p.kind = 0;
+  p.log_scale = log_scale;
+  for (int i = 0; i < 2; i++) {
+    p.round[i] = rounding[i];
+    p.quant[i] = quant_fp[i];
+    p.dequant[i] = dequant[i];
+  }
+  return plain_quant(coeffs, qcoeff, dqcoeff, n, p);
+}
+
+int plain_quantize_b_raster(const int32_t* coeffs, int32_t* qcoeff, int32_t* dqcoeff, size_t n,
+                            const int32_t zbins[2], const int32_t round[2], const int32_t quant[2],
+                            const int32_t quant_shift[2], const int32_t dequant[2], int32_t log_scale) {
+  if (!zbins || !round || !quant || !quant_shift || !dequant) return ZAV1_EINVAL;
+  zav1::PlainQuant p{};
+  p.kind = 1;
+  p.log_scale = log_scale;
+  for (int i = 0; i < 2; i++) {
+    p.zbin[i] = zbins[i];
+    p.round[i] = round[i];
+    p.quant[i] = quant[i];
+    p.quant_shift[i] = quant_shift[i];
+    p.dequant[i] = dequant[i];
+  }
+  return plain_quant(coeffs, qcoeff, dqcoeff, n, p);
+}
+
+int plain_sse_i32(const int32_t* a, const int32_t* b, size_t n, uint64_t* out) {
+  if (!a || !out) return ZAV1_EINVAL;
+  char* st = nullptr;
+  int rc = staging(sizeof(int32_t) * 2 * n + 64, &st);
+  if (rc) return rc;
+  int32_t* da = (int32_t*)st;
+  int32_t* db = b ? da + n : nullptr;
+  uint64_t* dout = (uint64_t*)(((uintptr_t)(da + 2 * n) + 7) & ~(uintptr_t)7);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(da, a, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  if (b) ZCHECK(hipMemcpyAsync(db, b, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::sse_i32_batch(da, db, dout, (int)n, 1, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+
+int plain_hadamard_satd(const int16_t* src, size_t stride, int n, int32_t* coeff, int32_t* satd) {
+  if (!src || !satd || (n != 4 && n != 8 && n != 16 && n != 32) || stride < (size_t)n) return ZAV1_EINVAL;
+  std::vector<int16_t> packed((size_t)n * n);
+  for (int r = 0; r < n; r++) std::memcpy(&packed[(size_t)r * n], src + r * stride, sizeof(int16_t) * n);
+  char* st = nullptr;
+  const size_t nn = (size_t)n * n;
+  int rc = staging(sizeof(int16_t) * nn + sizeof(int32_t) * (nn + 2) + 64, &st);
+  if (rc) return rc;
+  int16_t* dres = (int16_t*)st;
+  int32_t* dco = (int32_t*)(st + ((sizeof(int16_t) * nn + 15) & ~(size_t)15));
+  int32_t* dsatd = dco + nn;
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dres, packed.data(), sizeof(int16_t) * nn, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::satd_batch(dres, dco, dsatd, n, 1, s));
+  if (coeff) ZCHECK(hipMemcpyAsync(coeff, dco, sizeof(int32_t) * nn, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipMemcpyAsync(satd, dsatd, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+
+const zav1_dsp_v1 g_table = {
+    ZAV1_ABI_VERSION,        1u,
+    plain_fwd_txfm2d,        plain_inv_txfm2d_bd,
+    plain_quantize_fp_raster, plain_quantize_b_raster,
+    plain_sse_i32,           plain_hadamard_satd,
+};
+
+}  // namespace
+
+extern "C" {
+
+int zav1_init(int device) {
+  int n = 0;
+  ZCHECK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return ZAV1_EINVAL;
+  ZCHECK(hipSetDevice(device));
+  ZCHECK(hipFree(nullptr));
+  return ZAV1_OK;
+}
+
+int zav1_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int zav1_malloc(void** p, size_t bytes) {
+  if (!p) return ZAV1_EINVAL;
+  ZCHECK(hipMalloc(p, bytes ? bytes : 1));
+  return ZAV1_OK;
+}
+int zav1_free(void* p) {
+  ZCHECK(hipFree(p));
+  return ZAV1_OK;
+}
+int zav1_host_alloc(void** p, size_t bytes) {
+  if (!p) return ZAV1_EINVAL;
+  ZCHECK(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+  return ZAV1_OK;
+}
+int zav1_host_free(void* p) {
+  ZCHECK(hipHostFree(p));
+  return ZAV1_OK;
+}
+int zav1_memcpy_h2d(void* dst, const void* src, size_t bytes, zav1_stream_t s) {
+  ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s)));
+  return ZAV1_OK;
+}
+int zav1_memcpy_d2h(void* dst, const void* src, size_t bytes, zav1_stream_t s) {
+  ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(s)));
+  ZCHECK(hipStreamSynchronize(S(s)));
+  return ZAV1_OK;
+}
+int zav1_memcpy_d2d(void* dst, const void* src, size_t bytes, zav1_stream_t s) {
+  ZCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(s)));
+  return ZAV1_OK;
+}
+int zav1_memset(void* dst, int value, size_t bytes, zav1_stream_t s) {
+  ZCHECK(hipMemsetAsync(dst, value, bytes, S(s)));
+  return ZAV1_OK;
+}
+int zav1_stream_create(zav1_stream_t* s) {
+  if (!s) return ZAV1_EINVAL;
+  hipStream_t h = nullptr;
+  ZCHECK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+  *s = (zav1_stream_t)h;
+  return ZAV1_OK;
+}
+int zav1_stream_destroy(zav1_stream_t s) {
+  if (s) ZCHECK(hipStreamDestroy((hipStream_t)s));
+  return ZAV1_OK;
+}
+int zav1_stream_sync(zav1_stream_t s) {
+  ZCHECK(hipStreamSynchronize(S(s)));
+  return ZAV1_OK;
+}
+int zav1_device_sync(void) {
+  ZCHECK(hipDeviceSynchronize());
+  return ZAV1_OK;
+}
+int zav1_event_create(void** ev) {
+  if (!ev) return ZAV1_EINVAL;
+  hipEvent_t e;
+  ZCHECK(hipEventCreate(&e));
+  *ev = (void*)e;
+  return ZAV1_OK;
+}
+int zav1_event_destroy(void* ev) {
+  ZCHECK(hipEventDestroy((hipEvent_t)ev));
+  return ZAV1_OK;
+}
+int zav1_event_record(void* ev, zav1_stream_t s) {
+  ZCHECK(hipEventRecord((hipEvent_t)ev, S(s)));
+  return ZAV1_OK;
+}
+int zav1_event_elapsed_ms(void* a, void* b, float* ms) {
+  if (!ms) return ZAV1_EINVAL;
+  ZCHECK(hipEventSynchronize((hipEvent_t)b));
+  ZCHECK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return ZAV1_OK;
+}
+const char* zav1_last_error(void) { return g_err.c_str(); }
+
+int zav1_build_quant_row(int qindex, int bd, zav1_quant_row* row) {
+  if (!row || qindex < 0 || qindex > 255 || (bd != 8 && bd != 10)) return ZAV1_EINVAL;
+  static_assert(sizeof(zav1_quant_row) == sizeof(zav1::QuantRow), "row layout");
+  build_row(qindex, bd, reinterpret_cast<zav1::QuantRow*>(row));
+  return ZAV1_OK;
+}
+
+const zav1_dsp_v1* zav1_dsp_get(int backend) {
+  if (backend != 1) return nullptr;
+  if (zav1_device_count() <= 0) return nullptr;
+  return &g_table;
+}
+
+int zav1_fwd_txfm2d_batch(const int32_t* d_in, int32_t* d_out, const uint8_t* d_types, int tx_size, int n_jobs,
+                          zav1_stream_t s) {
+  if (tx_size < 0 || tx_size >= 19 || n_jobs < 0 || (n_jobs && (!d_in || !d_out || !d_types))) return ZAV1_EINVAL;
+  ZCHECK(zav1::fwd_txfm2d_batch(d_in, d_out, d_types, tx_size, n_jobs, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_inv_txfm2d_batch(const int32_t* d_in, int32_t* d_out, const uint8_t* d_types, int tx_size, int n_jobs,
+                          int bd, zav1_stream_t s) {
+  if (tx_size < 0 || tx_size >= 19 || n_jobs < 0 || (bd != 8 && bd != 10) ||
+      (n_jobs && (!d_in || !d_out || !d_types)))
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::inv_txfm2d_batch(d_in, d_out, d_types, tx_size, n_jobs, bd, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_quantize_batch(const int32_t* d_coeff, int32_t* d_qcoeff, int32_t* d_dqcoeff, const uint8_t* d_qidx,
+                        int n_per_job, int n_jobs, int log_scale, int bd, int kind, zav1_stream_t s) {
+  if (n_per_job <= 0 || n_jobs < 0 || log_scale < 0 || log_scale > 2 || (bd != 8 && bd != 10) ||
+      (kind != 0 && kind != 1))
+    return ZAV1_EINVAL;
+  const zav1::QuantRow* rows = nullptr;
+  int rc = quant_rows(&rows);
+  if (rc) return rc;
+  ZCHECK(zav1::quantize_batch(d_coeff, d_qcoeff, d_dqcoeff, rows + (bd == 10 ? 256 : 0), d_qidx, n_per_job,
+                              n_jobs, log_scale, bd, kind, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_sse_i32_batch(const int32_t* d_a, const int32_t* d_b, uint64_t* d_out, int n_per_job, int n_jobs,
+                       zav1_stream_t s) {
+  if (n_per_job <= 0 || n_jobs < 0) return ZAV1_EINVAL;
+  ZCHECK(zav1::sse_i32_batch(d_a, d_b, d_out, n_per_job, n_jobs, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_satd_batch(const int16_t* d_res, int32_t* d_coeff, int32_t* d_satd, int n, int n_jobs, zav1_stream_t s) {
+  if ((n != 4 && n != 8 && n != 16 && n != 32) || n_jobs < 0) return ZAV1_EINVAL;
+  ZCHECK(zav1::satd_batch(d_res, d_coeff, d_satd, n, n_jobs, S(s)));
+  return ZAV1_OK;
+}
+
+}  // extern "C"

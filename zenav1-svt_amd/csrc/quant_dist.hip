@@ -1,0 +1,332 @@
+// Batched quantizers, frequency-domain SSE and Hadamard SATD for gfx950.
+//
+//   quantize_fp / quantize_b   rust/crates/svtav1-dsp/src/quant_coding.rs:80,365
+//                              (bd8 INT16 clamp; bd10 per
+//                              rust/crates/svtav1-encoder/src/quant.rs:335,391)
+//   sse_i32 / sq_sum_i32       rust/crates/svtav1-dsp/src/residual.rs:245,364
+//   hadamard 4/8/16/32 + satd  rust/crates/svtav1-dsp/src/hadamard.rs:643-948
+#include "zav1_dev.h"
+#include "zav1_kernels.h"
+
+namespace zav1 {
+
+// ---------------------------------------------------------------------------
+// quantizers: one lane per coefficient, rows from a small device table.
+// ---------------------------------------------------------------------------
+template <int KIND, int BD>
+__global__ __launch_bounds__(256) void k_quantize(const int32_t* __restrict__ coeff,
+                                                  int32_t* __restrict__ qcoeff,
+                                                  int32_t* __restrict__ dqcoeff,
+                                                  const QuantRow* __restrict__ rows,
+                                                  const uint8_t* __restrict__ qidx, int n_per_job,
+                                                  long total, int ls) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  const int job = (int)(i / n_per_job);
+  const int rc = (int)(i - (long)job * n_per_job);
+  const int iz = rc != 0;
+  const QuantRow& t = rows[qidx[job]];
+  const int32_t c = coeff[i];
+  const int32_t sgn = c >> 31;
+  const int32_t a = (c ^ sgn) - sgn;
+  int32_t q = 0;
+  if constexpr (KIND == 0) {
+    const int32_t rounding = (t.round_fp[iz] + ((1 << ls) >> 1)) >> ls;
+    if (((int64_t)a << (1 + ls)) >= t.dequant[iz]) {
+      int64_t x = (int64_t)a + rounding;
+      if constexpr (BD == 8) x = x > 32767 ? 32767 : x;
+      q = (int32_t)((x * t.quant_fp[iz]) >> (16 - ls));
+    }
+  } else {
+    const int32_t zbin = (t.zbin[iz] + ((1 << ls) >> 1)) >> ls;
+    const int32_t rnd = (t.round[iz] + ((1 << ls) >> 1)) >> ls;
+    if (a >= zbin) {
+      int64_t x = (int64_t)a + rnd;
+      if constexpr (BD == 8) x = x > 32767 ? 32767 : x;
+      q = (int32_t)(((((x * t.quant[iz]) >> 16) + x) * t.quant_shift[iz]) >> (16 - ls));
+    }
+  }
+  int32_t qo = 0, dqo = 0;
+  if (q != 0) {
+    qo = (q ^ sgn) - sgn;
+    const int32_t adq = (int32_t)(((int64_t)q * t.dequant[iz]) >> ls);
+    dqo = (adq ^ sgn) - sgn;
+  }
+  qcoeff[i] = qo;
+  dqcoeff[i] = dqo;
+}
+
+hipError_t quantize_batch(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff,
+                          const QuantRow* rows, const uint8_t* qidx, int n_per_job, int n_jobs,
+                          int log_scale, int bd, int kind, hipStream_t st) {
+  const long total = (long)n_per_job * n_jobs;
+  if (total <= 0) return hipSuccess;
+  const int grid = (int)((total + 255) / 256);
+  if (kind == 0 && bd == 8)
+    hipLaunchKernelGGL((k_quantize<0, 8>), dim3(grid), dim3(256), 0, st, coeff, qcoeff, dqcoeff, rows, qidx, n_per_job, total, log_scale);
+  else if (kind == 0)
+    hipLaunchKernelGGL((k_quantize<0, 10>), dim3(grid), dim3(256), 0, st, coeff, qcoeff, dqcoeff, rows, qidx, n_per_job, total, log_scale);
+  else if (bd == 8)
+    hipLaunchKernelGGL((k_quantize<1, 8>), dim3(grid), dim3(256), 0, st, coeff, qcoeff, dqcoeff, rows, qidx, n_per_job, total, log_scale);
+  else
+    hipLaunchKernelGGL((k_quantize<1, 10>), dim3(grid), dim3(256), 0, st, coeff, qcoeff, dqcoeff, rows, qidx, n_per_job, total, log_scale);
+  return hipGetLastError();
+}
+
+// Plain-form quantizer: caller-supplied, already log_scale-adjusted rows
+// (the arguments of quantize_fp_raster / quantize_b_raster), bd8 clamp.
+__global__ __launch_bounds__(256) void k_quantize_plain(const int32_t* __restrict__ coeff,
+                                                        int32_t* __restrict__ qcoeff,
+                                                        int32_t* __restrict__ dqcoeff, int n,
+                                                        PlainQuant p) {
+  const int rc = blockIdx.x * 256 + threadIdx.x;
+  if (rc >= n) return;
+  const int iz = rc != 0;
+  const int32_t c = coeff[rc];
+  const int32_t sgn = c >> 31;
+  const int32_t a = (c ^ sgn) - sgn;
+  int32_t q = 0;
+  if (p.kind == 0) {
+    if (((int64_t)a << (1 + p.log_scale)) >= p.dequant[iz]) {
+      int64_t x = (int64_t)a + p.round[iz];
+      x = x > 32767 ? 32767 : x;
+      q = (int32_t)((x * p.quant[iz]) >> (16 - p.log_scale));
+    }
+  } else {
+    if (a >= p.zbin[iz]) {
+      int64_t x = (int64_t)a + p.round[iz];
+      x = x > 32767 ? 32767 : x;
+      q = (int32_t)(((((x * p.quant[iz]) >> 16) + x) * p.quant_shift[iz]) >> (16 - p.log_scale));
+    }
+  }
+  int32_t qo = 0, dqo = 0;
+  if (q != 0) {
+    qo = (q ^ sgn) - sgn;
+    const int32_t adq = (int32_t)(((int64_t)q * p.dequant[iz]) >> p.log_scale);
+    dqo = (adq ^ sgn) - sgn;
+  }
+  qcoeff[rc] = qo;
+  dqcoeff[rc] = dqo;
+}
+
+hipError_t quantize_plain(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff, int n,
+                          const PlainQuant& p, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_quantize_plain, dim3((n + 255) / 256), dim3(256), 0, st, coeff, qcoeff,
+                     dqcoeff, n, p);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// SSE: one wavefront per job, lanes stride the job, DPP/shuffle reduction.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_sse_i32(const int32_t* __restrict__ a,
+                                                 const int32_t* __restrict__ b,
+                                                 uint64_t* __restrict__ out, int n, int n_jobs) {
+  const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= n_jobs) return;
+  const int32_t* pa = a + (size_t)wave * n;
+  const int32_t* pb = b ? b + (size_t)wave * n : nullptr;
+  uint64_t acc = 0;
+  for (int i = lane; i < n; i += 64) {
+    const int64_t e = (int64_t)pa[i] - (pb ? (int64_t)pb[i] : 0);
+    acc += (uint64_t)e * (uint64_t)e;
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0) out[wave] = acc;
+}
+
+hipError_t sse_i32_batch(const int32_t* a, const int32_t* b, uint64_t* out, int n_per_job,
+                         int n_jobs, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  const int grid = (n_jobs + 3) / 4;
+  hipLaunchKernelGGL(k_sse_i32, dim3(grid), dim3(256), 0, st, a, b, out, n_per_job, n_jobs);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Hadamard / SATD.  Each lane owns one 8x8 (or 4x4) sub-tile in registers;
+// 16x16 and 32x32 combine their quadrants through LDS with the 16-bit wrap /
+// saturation rules of the kernels the reference reproduces.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void had8_col(const int32_t* s, int16_t* o) {
+  // 8-point butterfly on s[0..7]; C output permutation, 16-bit store.
+  const int32_t b0 = s[0] + s[1], b1 = s[0] - s[1], b2 = s[2] + s[3], b3 = s[2] - s[3];
+  const int32_t b4 = s[4] + s[5], b5 = s[4] - s[5], b6 = s[6] + s[7], b7 = s[6] - s[7];
+  const int32_t c0 = b0 + b2, c1 = b1 + b3, c2 = b0 - b2, c3 = b1 - b3;
+  const int32_t c4 = b4 + b6, c5 = b5 + b7, c6 = b4 - b6, c7 = b5 - b7;
+  o[0] = (int16_t)(c0 + c4); o[7] = (int16_t)(c1 + c5); o[3] = (int16_t)(c2 + c6);
+  o[4] = (int16_t)(c3 + c7); o[2] = (int16_t)(c0 - c4); o[6] = (int16_t)(c1 - c5);
+  o[1] = (int16_t)(c2 - c6); o[5] = (int16_t)(c3 - c7);
+}
+
+// 8x8 tile at src (row stride `st`) -> 64 coefficients (i16-valued).
+__device__ __forceinline__ void had8_tile(const int16_t* __restrict__ src, int st, int32_t* out) {
+  int16_t buf[64];
+  int32_t col[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) col[r] = src[r * st + c];
+    had8_col(col, buf + c * 8);
+  }
+  int16_t o[8];
+#pragma unroll
+  for (int idx = 0; idx < 8; idx++) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) col[k] = buf[idx + 8 * k];
+    had8_col(col, o);
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[idx * 8 + k] = o[k];
+  }
+}
+
+__device__ __forceinline__ void had4_tile(const int16_t* __restrict__ src, int st, int32_t* out) {
+  int16_t b1[16], b2[16];
+  auto col4 = [](const int16_t* s, int stride, int16_t* o) {
+    const int16_t x0 = (int16_t)((int16_t)(s[0] + s[stride]) >> 1);
+    const int16_t x1 = (int16_t)((int16_t)(s[0] - s[stride]) >> 1);
+    const int16_t x2 = (int16_t)((int16_t)(s[2 * stride] + s[3 * stride]) >> 1);
+    const int16_t x3 = (int16_t)((int16_t)(s[2 * stride] - s[3 * stride]) >> 1);
+    o[0] = (int16_t)(x0 + x2); o[1] = (int16_t)(x1 + x3);
+    o[2] = (int16_t)(x0 - x2); o[3] = (int16_t)(x1 - x3);
+  };
+  int16_t tile[16];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) tile[r * 4 + c] = src[r * st + c];
+#pragma unroll
+  for (int i = 0; i < 4; i++) col4(tile + i, 4, b1 + 4 * i);
+#pragma unroll
+  for (int i = 0; i < 4; i++) col4(b1 + i, 4, b2 + 4 * i);
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[i * 4 + j] = b2[j * 4 + i];
+}
+
+__device__ __forceinline__ int16_t sat16(int32_t v) {
+  return (int16_t)(v < -32768 ? -32768 : (v > 32767 ? 32767 : v));
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_satd(const int16_t* __restrict__ res,
+                                              int32_t* __restrict__ coeff_out,
+                                              int32_t* __restrict__ satd_out, int n_jobs) {
+  constexpr int TPJ = N <= 8 ? 1 : (N / 8) * (N / 8);  // lanes per job
+  constexpr int JPW = 256 / TPJ;
+  constexpr int NN = N * N;
+  const int jl = threadIdx.x / TPJ, s = threadIdx.x - jl * TPJ;
+  const int job = blockIdx.x * JPW + jl;
+  const bool active = job < n_jobs;
+  const int16_t* src = res + (size_t)job * NN;
+  if constexpr (N <= 8) {
+    if (!active) return;
+    int32_t c[64];
+    if constexpr (N == 4) had4_tile(src, 4, c);
+    else had8_tile(src, 8, c);
+    int32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+      sum += c[i] < 0 ? -c[i] : c[i];
+      if (coeff_out) coeff_out[(size_t)job * NN + i] = c[i];
+    }
+    satd_out[job] = sum;
+  } else {
+    __shared__ int32_t lds[JPW * NN];
+    __shared__ int32_t part[256];
+    int32_t* L = lds + jl * NN;
+    // sub-tile s: for N == 16 quadrant q = s; for N == 32, 16x16 quadrant Q = s / 4
+    // and 8x8 quadrant q = s % 4 inside it (coefficient layout of the reference).
+    int base_r, base_c, dst;
+    if constexpr (N == 16) {
+      base_r = (s >> 1) * 8; base_c = (s & 1) * 8; dst = s * 64;
+    } else {
+      const int Q = s >> 2, q = s & 3;
+      base_r = (Q >> 1) * 16 + (q >> 1) * 8;
+      base_c = (Q & 1) * 16 + (q & 1) * 8;
+      dst = Q * 256 + q * 64;
+    }
+    if (active) {
+      int32_t c[64];
+      had8_tile(src + base_r * N + base_c, N, c);
+#pragma unroll
+      for (int i = 0; i < 64; i++) L[dst + i] = c[i];
+    }
+    __syncthreads();
+    // 16x16 combine (int16 wrap, >> 1): per 16x16 group g, positions 0..63.
+    constexpr int G16 = N == 16 ? 1 : 4;
+    constexpr int PER = 64 * G16 / TPJ;  // positions per lane
+    if (active) {
+      for (int k = 0; k < PER; k++) {
+        const int p = s * PER + k;
+        const int g = p >> 6, i = p & 63;
+        int32_t* B = L + g * 256;
+        const int16_t a0 = (int16_t)B[i], a1 = (int16_t)B[i + 64];
+        const int16_t a2 = (int16_t)B[i + 128], a3 = (int16_t)B[i + 192];
+        const int16_t b0 = (int16_t)((int16_t)(a0 + a1) >> 1), b1 = (int16_t)((int16_t)(a0 - a1) >> 1);
+        const int16_t b2 = (int16_t)((int16_t)(a2 + a3) >> 1), b3 = (int16_t)((int16_t)(a2 - a3) >> 1);
+        B[i] = (int16_t)(b0 + b2);
+        B[i + 64] = (int16_t)(b1 + b3);
+        B[i + 128] = (int16_t)(b0 - b2);
+        B[i + 192] = (int16_t)(b1 - b3);
+      }
+    }
+    if constexpr (N == 32) {
+      __syncthreads();
+      if (active) {
+        for (int k = 0; k < 16; k++) {
+          const int i = s * 16 + k;
+          const int32_t a0 = (int16_t)L[i], a1 = (int16_t)L[i + 256];
+          const int32_t a2 = (int16_t)L[i + 512], a3 = (int16_t)L[i + 768];
+          const int16_t b0 = sat16((a0 + a1) >> 2), b1 = sat16((a0 - a1) >> 2);
+          const int16_t b2 = sat16((a2 + a3) >> 2), b3 = sat16((a2 - a3) >> 2);
+          L[i] = (int16_t)(b0 + b2);
+          L[i + 256] = (int16_t)(b1 + b3);
+          L[i + 512] = (int16_t)(b0 - b2);
+          L[i + 768] = (int16_t)(b1 - b3);
+        }
+      }
+    }
+    __syncthreads();
+    int32_t sum = 0;
+    if (active) {
+      for (int i = s; i < NN; i += TPJ) {
+        const int32_t v = L[i];
+        sum += v < 0 ? -v : v;
+        if (coeff_out) coeff_out[(size_t)job * NN + i] = v;
+      }
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (active && s == 0) {
+      int32_t tot = 0;
+      for (int k = 0; k < TPJ; k++) tot += part[jl * TPJ + k];
+      satd_out[job] = tot;
+    }
+  }
+}
+
+hipError_t satd_batch(const int16_t* res, int32_t* coeff, int32_t* satd, int n, int n_jobs,
+                      hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  auto grid_for = [&](int tpj) { return (n_jobs + (256 / tpj) - 1) / (256 / tpj); };
+  switch (n) {
+    case 4: hipLaunchKernelGGL((k_satd<4>), dim3(grid_for(1)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
+    case 8: hipLaunchKernelGGL((k_satd<8>), dim3(grid_for(1)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
+    case 16: hipLaunchKernelGGL((k_satd<16>), dim3(grid_for(4)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
+    case 32: hipLaunchKernelGGL((k_satd<32>), dim3(grid_for(16)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace zav1

@@ -1,0 +1,202 @@
+// Device-side shared definitions for the zav1 HIP kernels (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "txfm1d_gen.h"
+
+namespace zav1 {
+
+// TxSize in C order (rust/crates/svtav1-types/src/transform.rs:7-29).
+__host__ __device__ constexpr int tx_w(int s) {
+  return s == 0 ? 4 : s == 1 ? 8 : s == 2 ? 16 : s == 3 ? 32 : s == 4 ? 64 : s == 5 ? 4 : s == 6 ? 8
+       : s == 7 ? 8 : s == 8 ? 16 : s == 9 ? 16 : s == 10 ? 32 : s == 11 ? 32 : s == 12 ? 64
+       : s == 13 ? 4 : s == 14 ? 16 : s == 15 ? 8 : s == 16 ? 32 : s == 17 ? 16 : 64;
+}
+__host__ __device__ constexpr int tx_h(int s) {
+  return s == 0 ? 4 : s == 1 ? 8 : s == 2 ? 16 : s == 3 ? 32 : s == 4 ? 64 : s == 5 ? 8 : s == 6 ? 4
+       : s == 7 ? 16 : s == 8 ? 8 : s == 9 ? 32 : s == 10 ? 16 : s == 11 ? 64 : s == 12 ? 32
+       : s == 13 ? 16 : s == 14 ? 4 : s == 15 ? 32 : s == 16 ? 8 : s == 17 ? 64 : 16;
+}
+__host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+// Forward cos bits per (log2 w - 2, log2 h - 2)
+// (rust/crates/svtav1-dsp/src/fwd_txfm.rs:91 / :101).
+__host__ __device__ constexpr int fwd_cb_col(int w, int h) {
+  constexpr int8_t T[5][5] = {{13, 13, 13, 0, 0}, {13, 13, 13, 12, 0}, {13, 13, 13, 12, 13},
+                              {0, 13, 13, 12, 13}, {0, 0, 13, 12, 13}};
+  return T[ilog2c(w) - 2][ilog2c(h) - 2];
+}
+__host__ __device__ constexpr int fwd_cb_row(int w, int h) {
+  constexpr int8_t T[5][5] = {{13, 13, 12, 0, 0}, {13, 13, 13, 12, 0}, {13, 13, 12, 13, 12},
+                              {0, 12, 13, 12, 11}, {0, 0, 12, 11, 10}};
+  return T[ilog2c(w) - 2][ilog2c(h) - 2];
+}
+// Forward per-pass shifts (fwd_txfm.rs:110): {left shift in, round-right
+// after column pass, round-right after row pass}.
+struct FwdShift { int s0, s1, s2; };
+__host__ __device__ constexpr FwdShift fwd_shift(int w, int h) {
+  return (w == 64 && h == 64) ? FwdShift{0, 2, 2}
+       : (w == 32 && h == 64) ? FwdShift{0, 2, 2}
+       : (w == 64 && h == 32) ? FwdShift{2, 4, 2}
+       : (w == 16 && h == 64) ? FwdShift{0, 2, 0}
+       : (w == 64 && h == 16) ? FwdShift{2, 4, 0}
+       : (w == 4 && h == 4) ? FwdShift{2, 0, 0}
+       : (w == 8 && h == 8) ? FwdShift{2, 1, 0}
+       : (w == 16 && h == 16) ? FwdShift{2, 2, 0}
+       : (w == 32 && h == 32) ? FwdShift{2, 4, 0}
+       : ((w == 4 && h == 8) || (w == 8 && h == 4)) ? FwdShift{2, 1, 0}
+       : ((w == 8 && h == 16) || (w == 16 && h == 8)) ? FwdShift{2, 2, 0}
+       : ((w == 16 && h == 32) || (w == 32 && h == 16)) ? FwdShift{2, 4, 0}
+       : ((w == 4 && h == 16) || (w == 16 && h == 4)) ? FwdShift{2, 1, 0}
+       : FwdShift{2, 2, 0};  // 8x32 / 32x8
+}
+// Inverse shifts (inv_transforms shift table, inv_txfm.rs:1888): round-right
+// after the row pass and after the column pass.
+struct InvShift { int r0, r1; };
+__host__ __device__ constexpr InvShift inv_shift(int w, int h) {
+  return (w == h) ? InvShift{w == 4 ? 0 : (w == 8 ? 1 : 2), 4}
+       : ((w == 2 * h) || (h == 2 * w)) ? InvShift{(w == 4 || h == 4) ? 0 : 1, 4}
+       : InvShift{(w == 4 || h == 4) ? 1 : 2, 4};
+}
+
+// TxType -> (col kind, row kind, ud flip, lr flip); kinds 0 DCT, 1 ADST,
+// 2 FLIPADST (ADST kernel, flip outside), 3 identity
+// (rust/crates/svtav1-dsp/src/txfm_dispatch.rs:89,:99).
+struct TypeCfg { int col, row, ud, lr; };
+__host__ __device__ inline TypeCfg type_cfg(int t) {
+  const int COL[16] = {0, 1, 0, 1, 2, 0, 2, 1, 2, 3, 0, 3, 1, 3, 2, 3};
+  const int ROW[16] = {0, 0, 1, 1, 0, 2, 2, 2, 1, 3, 3, 0, 3, 1, 3, 2};
+  TypeCfg c;
+  c.col = COL[t];
+  c.row = ROW[t];
+  c.ud = (t == 4 || t == 6 || t == 8 || t == 14);
+  c.lr = (t == 5 || t == 6 || t == 7 || t == 15);
+  return c;
+}
+// A 1D kernel exists for (kind, n): ADST only up to 16 points.
+__host__ __device__ constexpr bool kind_ok(int kind, int n) { return !((kind == 1 || kind == 2) && n > 16); }
+
+__device__ __forceinline__ int32_t rshift_round(int32_t v, int bit) {
+  return bit == 0 ? v : (int32_t)(((int64_t)v + (1 << (bit - 1))) >> bit);
+}
+__device__ __forceinline__ int32_t rshift64r(int64_t v, int bit) {
+  return (int32_t)((v + ((int64_t)1 << (bit - 1))) >> bit);
+}
+
+// ---------------------------------------------------------------------------
+// 1D forward kernel dispatch on (N, cos_bit) compile-time, kind at run time.
+// ---------------------------------------------------------------------------
+template <int N, int CB>
+__device__ __forceinline__ void fdct_n(int32_t* x) {
+#define ZAV1_FDCT_PICK(NN)                                   \
+  if constexpr (N == NN) {                                   \
+    if constexpr (CB == 10) fdct##NN##_cb10(x);              \
+    else if constexpr (CB == 11) fdct##NN##_cb11(x);         \
+    else if constexpr (CB == 12) fdct##NN##_cb12(x);         \
+    else fdct##NN##_cb13(x);                                 \
+  }
+  ZAV1_FDCT_PICK(4)
+  ZAV1_FDCT_PICK(8)
+  ZAV1_FDCT_PICK(16)
+  ZAV1_FDCT_PICK(32)
+  ZAV1_FDCT_PICK(64)
+#undef ZAV1_FDCT_PICK
+}
+
+template <int N, int CB>
+__device__ __forceinline__ void fadst_n(int32_t* x) {
+  if constexpr (N == 4) {
+    const int32_t* s = kSinpi[CB - 10];
+    const int32_t x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
+    if ((x0 | x1 | x2 | x3) == 0) return;
+    const int64_t s0 = (int64_t)s[1] * x0, s1 = (int64_t)s[4] * x0, s2 = (int64_t)s[2] * x1;
+    const int64_t s3 = (int64_t)s[1] * x1, s4 = (int64_t)s[3] * x2, s5 = (int64_t)s[4] * x3;
+    const int64_t s6 = (int64_t)s[2] * x3;
+    const int64_t s7 = (int64_t)(int32_t)(x0 + x1) - x3;
+    const int64_t y0 = s0 + s2 + s5, y1 = (int64_t)s[3] * s7, y2 = s1 - s3 + s6, y3 = s4;
+    x[0] = rshift64r(y0 + y3, CB);
+    x[1] = rshift64r(y1, CB);
+    x[2] = rshift64r(y2 - y3, CB);
+    x[3] = rshift64r(y2 - y0 + y3, CB);
+  } else if constexpr (N == 8) {
+    if constexpr (CB == 10) fadst8_cb10(x);
+    else if constexpr (CB == 11) fadst8_cb11(x);
+    else if constexpr (CB == 12) fadst8_cb12(x);
+    else fadst8_cb13(x);
+  } else if constexpr (N == 16) {
+    if constexpr (CB == 10) fadst16_cb10(x);
+    else if constexpr (CB == 11) fadst16_cb11(x);
+    else if constexpr (CB == 12) fadst16_cb12(x);
+    else fadst16_cb13(x);
+  }
+}
+
+// identity scalings: x sqrt2 (4), x2 (8), x2sqrt2 (16), x4 (32), x4sqrt2 (64)
+template <int N>
+__device__ __forceinline__ void fidentity_n(int32_t* x) {
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    if constexpr (N == 4) x[i] = rshift64r((int64_t)x[i] * 5793, 12);
+    else if constexpr (N == 8) x[i] = x[i] * 2;
+    else if constexpr (N == 16) x[i] = rshift64r((int64_t)x[i] * 2 * 5793, 12);
+    else if constexpr (N == 32) x[i] = x[i] * 4;
+    else x[i] = rshift64r((int64_t)x[i] * 4 * 5793, 12);
+  }
+}
+
+template <int N, int CB>
+__device__ __forceinline__ void fwd1d(int32_t* x, int kind) {
+  if (kind == 0) {
+    fdct_n<N, CB>(x);
+  } else if (kind == 3) {
+    fidentity_n<N>(x);
+  } else {
+    if constexpr (N <= 16) fadst_n<N, CB>(x);
+  }
+}
+
+// 1D inverse, INV_COS_BIT 12, stage clamp `rng`.
+template <int N>
+__device__ __forceinline__ void inv1d(int32_t* x, int kind, int rng) {
+  if (kind == 0) {
+    if constexpr (N == 4) idct4(x, rng);
+    else if constexpr (N == 8) idct8(x, rng);
+    else if constexpr (N == 16) idct16(x, rng);
+    else if constexpr (N == 32) idct32(x, rng);
+    else idct64(x, rng);
+  } else if (kind == 3) {
+    fidentity_n<N>(x);  // iidentity scalings equal the forward ones
+  } else {
+    if constexpr (N == 4) {
+      const int32_t* s = kSinpi[2];
+      const int32_t x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
+      if ((x0 | x1 | x2 | x3) == 0) return;
+      // i32 products with two's-complement wrap, as the reference computes them
+      auto m = [](int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); };
+      int32_t s0 = m(s[1], x0), s1 = m(s[2], x0), s2 = m(s[3], x1), s3 = m(s[4], x2);
+      int32_t s4 = m(s[1], x2), s5 = m(s[2], x3), s6 = m(s[4], x3);
+      int32_t s7 = (int32_t)((uint32_t)(x0 - x2) + (uint32_t)x3);
+      s0 = (int32_t)((uint32_t)s0 + (uint32_t)s3);
+      s1 = (int32_t)((uint32_t)s1 - (uint32_t)s4);
+      s3 = s2;
+      s2 = m(s[3], s7);
+      s0 = (int32_t)((uint32_t)s0 + (uint32_t)s5);
+      s1 = (int32_t)((uint32_t)s1 - (uint32_t)s6);
+      const int32_t y0 = (int32_t)((uint32_t)s0 + (uint32_t)s3);
+      const int32_t y1 = (int32_t)((uint32_t)s1 + (uint32_t)s3);
+      const int32_t y2 = s2;
+      const int32_t y3 = (int32_t)((uint32_t)s0 + (uint32_t)s1 - (uint32_t)s3);
+      x[0] = rshift64r(y0, 12);
+      x[1] = rshift64r(y1, 12);
+      x[2] = rshift64r(y2, 12);
+      x[3] = rshift64r(y3, 12);
+    } else if constexpr (N == 8) {
+      iadst8(x, rng);
+    } else if constexpr (N == 16) {
+      iadst16(x, rng);
+    }
+  }
+}
+
+}  // namespace zav1

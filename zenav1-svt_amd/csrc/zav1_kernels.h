@@ -1,0 +1,44 @@
+// Internal launch entry points of the zav1 HIP kernels (device pointers,
+// stream-ordered).  The public C-ABI in include/zav1_dsp.h wraps these.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zav1 {
+
+// One quantizer row (DC = [0], AC = [1]) as built by
+// rust/crates/svtav1-encoder/src/quant.rs:132-236.
+struct QuantRow {
+  int32_t zbin[2], round[2], quant[2], quant_shift[2];
+  int32_t round_fp[2], quant_fp[2], dequant[2];
+};
+
+// Plain-form quantizer parameters: kind 0 uses round (= pre-shifted _fp
+// rounding), quant (= quant_fp), dequant; kind 1 uses zbin, round, quant,
+// quant_shift, dequant (zbin/round pre-shifted).
+struct PlainQuant {
+  int32_t kind, log_scale;
+  int32_t zbin[2], round[2], quant[2], quant_shift[2], dequant[2];
+};
+hipError_t quantize_plain(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff, int n,
+                          const PlainQuant& p, hipStream_t st);
+
+hipError_t fwd_txfm2d_batch(const int32_t* in, int32_t* out, const uint8_t* types, int tx_size,
+                            int n_jobs, hipStream_t st);
+hipError_t inv_txfm2d_batch(const int32_t* in, int32_t* out, const uint8_t* types, int tx_size,
+                            int n_jobs, int bd, hipStream_t st);
+
+// kind: 0 = fp (RDOQ initial), 1 = b (dead-zone).  rows: device array of
+// QuantRow indexed by qidx[job].
+hipError_t quantize_batch(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff,
+                          const QuantRow* rows, const uint8_t* qidx, int n_per_job, int n_jobs,
+                          int log_scale, int bd, int kind, hipStream_t st);
+// out[job] = sum (a-b)^2 (wrapping u64); b == nullptr gives sum a^2.
+hipError_t sse_i32_batch(const int32_t* a, const int32_t* b, uint64_t* out, int n_per_job,
+                         int n_jobs, hipStream_t st);
+// Hadamard of n x n i16 residual tiles (packed) -> SATD per tile; coeff may
+// be nullptr.  n in {4, 8, 16, 32}.
+hipError_t satd_batch(const int16_t* res, int32_t* coeff, int32_t* satd, int n, int n_jobs,
+                      hipStream_t st);
+
+}  // namespace zav1
