@@ -1,0 +1,214 @@
+"""Benchmark: batched candidate RD (tx-type search) on one MI355X.
+
+Workload (one step): the 2048x2048 10-bit 4:2:0 synthetic still (luma: 1/f^2
+spectral noise in [64, 940] + 30 hard lines, seed 2000), at qp 20 and qp 40
+(qindex 80 / 160).  For each qp, all 19 transform sizes tile the frame and
+every block is evaluated with every transform type legal for its size
+(DC prediction from the source neighbours -> residual -> forward transform
+-> 64-point fold -> fp quantize -> distortion / eob / level sum), one fused
+launch per (size, qp): 38 launches per step, ~4.4 M candidates.
+
+value = luma megapixels x qps swept per second (inputs resident in HBM).
+Also reported: the plain per-call GPU form (host pointers, one launch per
+function-table call: fwd + quantize + sse per candidate) on a bounded sample,
+the roofline of the dominant kernel from HIP events over the timed region, and
+the oracle (CPU restatement) timed on a bounded crop of the same workload.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, 'zenav1-svt_amd')
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 39.3  # 256 CU x 64 int32 lanes/clk x 2.4 GHz
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    m = importlib.util.module_from_spec(spec)
+    sys.modules[name] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def candidate_ops(ts_mod, tx_size, tx_type, opsd):
+    """Algorithmic integer ops of one fused candidate."""
+    w, h = ts_mod.TX_DIMS[tx_size]
+    col = [0, 1, 0, 1, 2, 0, 2, 1, 2, 3, 0, 3, 1, 3, 2, 3][tx_type]
+    row = [0, 0, 1, 1, 0, 2, 2, 2, 1, 3, 3, 0, 3, 1, 3, 2][tx_type]
+
+    def one(kind, n):
+        return opsd[('dct' if kind == 0 else ('identity' if kind == 3 else 'adst')) + str(n)]
+    n = w * h
+    rect = 3 if (w == 2 * h or h == 2 * w) else 0
+    ops = (w + h)                 # DC prediction sums
+    ops += n * 2                  # residual + input shift
+    ops += w * one(col, h) + h * one(row, w)
+    ops += n * (2 + 2 + rect)     # two round shifts (+ sqrt2 scale)
+    ops += n * 12                 # fp quantize + dequant (abs, compare, add, mul, shift, sign x2)
+    ops += n * 3                  # distortion (sub, mul, add)
+    ops += n * 3                  # eob / level sum
+    return ops
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--size', type=int, default=2048)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-plain', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build measures one GPU; multi-GPU runs are not part of it '
+              '(requested --gpus %d)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    zav1 = _load('zav1', os.path.join(PKG, '__init__.py'))
+    synth = _load('zav1_synth', os.path.join(PKG, 'synth.py'))
+    tsm = _load('zav1_txsearch', os.path.join(PKG, 'txsearch.py'))
+    opsd = json.load(open(os.path.join(PKG, 'csrc', 'txfm1d_ops.json')))
+    zav1.init(0)
+    L = zav1.lib()
+
+    S = args.size
+    plane = synth.config2_luma(S, S, seed=2000)
+    qps = (20, 40)
+    qis = [synth.qp_to_qindex(q) for q in qps]
+    sweep = tsm.TxSearchSweep(zav1, plane, 10, qis)
+    n_cand = sweep.candidates_per_step()
+
+    # explicit stream: kernels and events share it
+    st = ctypes.c_void_p()
+    zav1.check(L.zav1_stream_create(ctypes.byref(st)), 'stream_create')
+    stream = st.value
+
+    for _ in range(args.warmup):
+        sweep.run(stream)
+    zav1.sync(stream)
+
+    # per-size HIP events inside the timed region
+    evs = {}
+    for ts in range(19):
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        L.zav1_event_create(ctypes.byref(a))
+        L.zav1_event_create(ctypes.byref(b))
+        evs[ts] = (a.value, b.value)
+    per_size_ms = {ts: 0.0 for ts in range(19)}
+
+    zav1.sync(stream)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for ts in range(19):
+            a, b = evs[ts]
+            L.zav1_event_record(a, stream)
+            sweep.run(stream, sizes=(ts,))
+            L.zav1_event_record(b, stream)
+        # accumulate per-size kernel time (events complete in order)
+        for ts in range(19):
+            ms = ctypes.c_float()
+            a, b = evs[ts]
+            L.zav1_event_elapsed_ms(a, b, ctypes.byref(ms))
+            per_size_ms[ts] += ms.value
+    zav1.sync(stream)
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    mp = S * S / 1e6 * len(qps)
+    value = mp / (ms_per_step / 1000.0)
+
+    # dominant kernel: the tx size with the largest event time
+    dom = max(per_size_ms, key=per_size_ms.get)
+    types = tsm.legal_types(dom)
+    w, h = tsm.TX_DIMS[dom]
+    nblk = (S // w) * (S // h)
+    launches = args.steps * len(qps)
+    avg_launch_s = per_size_ms[dom] / 1000.0 / launches
+    ops_launch = nblk * sum(candidate_ops(tsm, dom, t, opsd) for t in types)
+    bytes_launch = nblk * (w * h * 2 + (w + h) * 2 + len(types) * 16)
+    achieved_tops = ops_launch / avg_launch_s / 1e12
+
+    out = {
+        'metric': 'MP/s (luma megapixels x qps) through batched candidate RD, byte-identical to the oracle',
+        'value': round(value, 2), 'unit': 'MP/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'int32', 'data': 'synthetic',
+        'config': {'workload': 'config2 tx-type search: %dx%d 10-bit luma, qp 20+40, 19 tx sizes x legal '
+                               'types, fused DC-pred/fwd/fold/fp-quant/dist' % (S, S),
+                   'candidates_per_step': n_cand, 'launches_per_step': 19 * len(qps)},
+        'per_size_ms_per_step': {str(tsm.TX_DIMS[t]): round(per_size_ms[t] / args.steps, 4) for t in range(19)},
+        'roofline': {'bound': 'valu', 'kernel': 'k_txsearch<%d,%d>' % (w, h), 'achieved': round(achieved_tops, 3),
+                     'peak': VALU_PEAK_TOPS, 'unit': 'Tops/s', 'frac': round(achieved_tops / VALU_PEAK_TOPS, 4),
+                     'traffic': None, 'avg_launch_us': round(avg_launch_s * 1e6, 2),
+                     'algorithmic_ops_per_launch': int(ops_launch),
+                     'hbm_algorithmic_GBs': round(bytes_launch / avg_launch_s / 1e9, 2),
+                     'hbm_peak_GBs': HBM_PEAK_GBS},
+    }
+
+    if not args.no_plain:
+        # plain per-call form: function table with host pointers, one launch per call
+        tab = zav1.dsp_table()
+        rng = np.random.default_rng(1)
+        n = 64
+        blocks = rng.integers(-1023, 1024, (n, 16, 16)).astype(np.int32)
+        row = zav1.quant_row(qis[0], 10)
+        rounding = np.array(list(row.round_fp), np.int32)
+        qfp = np.array(list(row.quant_fp), np.int32)
+        deq = np.array(list(row.dequant), np.int32)
+        P = ctypes.POINTER(ctypes.c_int32)
+        co = np.zeros(256, np.int32)
+        q = np.zeros(256, np.int32)
+        dq = np.zeros(256, np.int32)
+        res = ctypes.c_uint64()
+        tp0 = time.perf_counter()
+        calls = 0
+        for j in range(n):
+            for t in range(16):
+                tab.fwd_txfm2d(blocks[j].ctypes.data_as(P), co.ctypes.data_as(P), 16, 2, t)
+                tab.quantize_fp_raster(co.ctypes.data_as(P), q.ctypes.data_as(P), dq.ctypes.data_as(P), 256,
+                                       rounding.ctypes.data_as(P), qfp.ctypes.data_as(P),
+                                       deq.ctypes.data_as(P), 0)
+                tab.sse_i32(co.ctypes.data_as(P), dq.ctypes.data_as(P), 256, ctypes.byref(res))
+                calls += 3
+        tp1 = time.perf_counter()
+        per_cand = (tp1 - tp0) / (n * 16)
+        plain_ms_step = per_cand * n_cand * 1000.0
+        out['plain_gpu'] = {'value': round(mp / (plain_ms_step / 1000.0), 4), 'unit': 'MP/s',
+                            'sample': '%d 16x16 blocks x 16 types, 3 table calls each (%d calls), '
+                                      'extrapolated per candidate to the step' % (n, calls),
+                            'us_per_candidate': round(per_cand * 1e6, 2),
+                            'batched_over_plain': round(value / (mp / (plain_ms_step / 1000.0)), 1)}
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        crop = 1024
+        sub = np.ascontiguousarray(plane[:crop, :crop])
+        threads = min(os.cpu_count() or 1, 16)
+        os.environ['OMP_NUM_THREADS'] = str(threads)
+        tc0 = time.perf_counter()
+        for qi in qis:
+            for ts in range(19):
+                xy = tsm.tile_xy(crop, crop, ts)
+                O.txsearch_plane(sub, 10, ts, xy, np.array(tsm.legal_types(ts), np.uint8), qi)
+        tc1 = time.perf_counter()
+        cpu_mp = crop * crop / 1e6 * len(qps) / (tc1 - tc0)
+        out['cpu_baseline'] = {'value': round(cpu_mp, 4), 'unit': 'MP/s', 'cores': threads, 'kind': 'port',
+                               'sample': '%dx%d crop of the same frame, qp 20+40, all sizes/types '
+                                         '(oracle C, OpenMP over blocks)' % (crop, crop),
+                               'seconds': round(tc1 - tc0, 2)}
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

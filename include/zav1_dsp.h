@@ -155,6 +155,26 @@ int zav1_sse_i32_batch(const int32_t *d_a, const int32_t *d_b, uint64_t *d_out, 
 int zav1_satd_batch(const int16_t *d_res, int32_t *d_coeff, int32_t *d_satd, int n, int n_jobs,
                     zav1_stream_t s);
 
+/* AV1 coefficient scan of tx_size's packed (<= 32x32) block for a scan
+ * class (0 default diagonal, 1 row, 2 column; the reference's
+ * TX_TYPE_TO_SCAN_INDEX classes).  Replaces scan()
+ * (rust/crates/svtav1-entropy/src/scan_tables.rs:1640).  Host only; returns
+ * the scan length or a negative status. */
+int zav1_scan_order(int tx_size, int scan_class, uint16_t *scan);
+
+/* Fused transform-unit evaluation of the tx-type search (the per-candidate
+ * body of tx_unit_inner, rust/crates/svtav1-encoder/src/leaf_funnel/
+ * tx_pipeline.rs:268, without RDOQ / rate): for every block (d_xy[2j],
+ * d_xy[2j+1]) of tx_size on the 16-bit plane (8-bit samples widened), DC
+ * prediction from the source neighbours, residual, forward transform,
+ * 64-point fold, fp quantization at qindex, frequency distortion.  For
+ * each block j and host-side candidate type types[k] (n_types <= 16) writes
+ * d_dist[j*n_types+k] (u64), d_eob[...] (scan order), d_lsum[...] (sum of
+ * |level|). */
+int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size, const int32_t *d_xy,
+                        int n_blocks, const uint8_t *types, int n_types, int qindex, uint64_t *d_dist,
+                        int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

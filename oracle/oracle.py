@@ -142,3 +142,47 @@ def block_satd(src, pred, hbd):
     p = np.ascontiguousarray(pred)
     h, w = s.shape
     return lib().or_block_satd(s.ctypes.data, w, p.ctypes.data, w, w, h, int(hbd))
+
+
+def _bind_txsearch(L):
+    L.or_scan.argtypes = [ctypes.c_int, ctypes.c_int, np.ctypeslib.ndpointer(dtype=np.uint16, flags='C_CONTIGUOUS')]
+    L.or_scan_class.argtypes = [ctypes.c_int]
+    L.or_tx_unit.argtypes = [_i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_uint32)]
+    L.or_txsearch_plane.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    _vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+
+
+def scan(tx_size, scan_class):
+    L = lib()
+    _bind_txsearch(L)
+    buf = np.zeros(1024, np.uint16)
+    n = L.or_scan(tx_size, scan_class, buf)
+    return buf[:n].copy()
+
+
+def tx_unit(residual, tx_size, tx_type, qindex, bd, use_b=False):
+    L = lib()
+    _bind_txsearch(L)
+    r = np.ascontiguousarray(residual, dtype=np.int32).ravel()
+    d, e, s = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint32()
+    ok = L.or_tx_unit(r, tx_size, tx_type, qindex, bd, int(use_b), None, ctypes.byref(d), ctypes.byref(e),
+                      ctypes.byref(s))
+    return (d.value, e.value, s.value) if ok else None
+
+
+def txsearch_plane(plane, bd, tx_size, xy, types, qindex):
+    """Reference tx-type search over blocks xy (n,2) of a uint16 plane."""
+    L = lib()
+    _bind_txsearch(L)
+    p = np.ascontiguousarray(plane, dtype=np.uint16)
+    xy = np.ascontiguousarray(xy, dtype=np.int32)
+    t = np.ascontiguousarray(types, dtype=np.uint8)
+    nb, nt = xy.shape[0], t.size
+    dist = np.zeros(nb * nt, np.uint64)
+    eob = np.zeros(nb * nt, np.int32)
+    ls = np.zeros(nb * nt, np.uint32)
+    L.or_txsearch_plane(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], bd, tx_size, xy.ctypes.data, nb,
+                        t.ctypes.data, nt, qindex, dist.ctypes.data, eob.ctypes.data, ls.ctypes.data)
+    return dist, eob, ls

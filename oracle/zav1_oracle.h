@@ -82,6 +82,20 @@ int32_t or_satd(const int32_t *coeff, int n);                           /* dsp/h
 uint64_t or_block_satd(const void *src, int src_stride, const void *pred, int pred_stride,
                        int w, int h, int hbd);
 
+/* ---- scans / eob / fused transform unit (txsearch.c) ------------------- */
+int or_scan_class(int tx_type);
+int or_scan(int tx_size, int scan_class, uint16_t *scan);
+int or_eob(const int32_t *qcoeff, const uint16_t *scan, int n);
+/* tx_pipeline.rs:268 tx_unit_inner (freq-domain distortion, no RDOQ):
+ * residual w x h -> dist (after the log_scale shift), eob, sum |level|. */
+int or_tx_unit(const int32_t *residual, int tx_size, int tx_type, int qindex, int bd, int use_b,
+               int32_t *qcoeff_out, uint64_t *dist, int *eob, uint32_t *level_sum);
+int32_t or_dc_value(const int32_t *above, const int32_t *left, int w, int h, int has_above,
+                    int has_left, int bd);
+void or_txsearch_plane(const uint16_t *plane, int stride, int width, int height, int bd, int tx_size,
+                       const int32_t *xy, int n_blocks, const uint8_t *types, int n_types, int qindex,
+                       uint64_t *dist, int32_t *eob, uint32_t *lsum);
+
 #ifdef __cplusplus
 }
 #endif

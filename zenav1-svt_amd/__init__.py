@@ -98,6 +98,8 @@ EXPORTS = {
     'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
     'zav1_sse_i32_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_satd_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
+    'zav1_scan_order': [_i32, _i32, _vp],
+    'zav1_txsearch_batch': [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp],
 }
 
 
@@ -225,3 +227,16 @@ def sse_i32_batch(d_a, d_b, d_out, n_per_job, n_jobs, stream=None):
 def satd_batch(d_res, d_coeff, d_satd, n, n_jobs, stream=None):
     check(lib().zav1_satd_batch(d_res.ptr, d_coeff.ptr if d_coeff is not None else None, d_satd.ptr, n,
                                 n_jobs, stream), 'satd_batch')
+
+
+def scan_order(tx_size, scan_class):
+    buf = np.zeros(1024, np.uint16)
+    n = check(lib().zav1_scan_order(tx_size, scan_class, buf.ctypes.data), 'scan_order')
+    return buf[:n].copy()
+
+
+def txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, types, qindex, d_dist, d_eob, d_lsum,
+                   stream=None):
+    t = np.ascontiguousarray(types, dtype=np.uint8)
+    check(lib().zav1_txsearch_batch(d_plane.ptr, stride, bd, tx_size, d_xy.ptr, n_blocks, t.ctypes.data, t.size,
+                                    qindex, d_dist.ptr, d_eob.ptr, d_lsum.ptr, stream), 'txsearch_batch')

@@ -32,6 +32,7 @@ hipStream_t S(zav1_stream_t s) { return s ? (hipStream_t)s : hipStreamPerThread;
 // Per-device table of quantizer rows: [bd index][qindex], bd index 0 = 8, 1 = 10.
 struct DeviceState {
   zav1::QuantRow* rows = nullptr;
+  uint16_t* iscan = nullptr;  // [19][3][1024] inverse scans
 };
 std::mutex g_mu;
 std::vector<DeviceState> g_dev;
@@ -76,6 +77,55 @@ int quant_rows(const zav1::QuantRow** out) {
     g_dev[dev].rows = d;
   }
   *out = g_dev[dev].rows;
+  return ZAV1_OK;
+}
+
+// AV1 coefficient scans of the packed (<= 32x32) block: class 0 runs the
+// anti-diagonals (square blocks alternate direction starting up-right, wide
+// blocks always run up-right, tall blocks down-left), class 1 is raster,
+// class 2 column-major.  Same orders as the C-extracted tables of
+// rust/crates/svtav1-entropy/src/scan_tables.rs.
+void build_scan(int tx_size, int cls, uint16_t* scan) {
+  int w = zav1::tx_w(tx_size), h = zav1::tx_h(tx_size);
+  w = w > 32 ? 32 : w;
+  h = h > 32 ? 32 : h;
+  int k = 0;
+  if (cls == 1) {
+    for (int i = 0; i < w * h; i++) scan[k++] = (uint16_t)i;
+  } else if (cls == 2) {
+    for (int c = 0; c < w; c++)
+      for (int r = 0; r < h; r++) scan[k++] = (uint16_t)(r * w + c);
+  } else {
+    for (int d = 0; d < w + h - 1; d++) {
+      const bool up = (w == h) ? ((d & 1) == 0) : (w > h);
+      for (int i = 0; i < h; i++) {
+        const int r = up ? (h - 1 - i) : i, c = d - r;
+        if (c >= 0 && c < w) scan[k++] = (uint16_t)(r * w + c);
+      }
+    }
+  }
+}
+
+int iscan_table(const uint16_t** out) {
+  int dev = 0;
+  ZCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+  if (!g_dev[dev].iscan) {
+    std::vector<uint16_t> h(19 * 3 * 1024, 0), scan(1024);
+    for (int s = 0; s < 19; s++)
+      for (int c = 0; c < 3; c++) {
+        build_scan(s, c, scan.data());
+        int w = zav1::tx_w(s), hh = zav1::tx_h(s);
+        const int n = (w > 32 ? 32 : w) * (hh > 32 ? 32 : hh);
+        for (int i = 0; i < n; i++) h[(s * 3 + c) * 1024 + scan[i]] = (uint16_t)i;
+      }
+    uint16_t* d = nullptr;
+    ZCHECK(hipMalloc(&d, h.size() * sizeof(uint16_t)));
+    ZCHECK(hipMemcpy(d, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    g_dev[dev].iscan = d;
+  }
+  *out = g_dev[dev].iscan;
   return ZAV1_OK;
 }
 
@@ -403,6 +453,36 @@ int zav1_sse_i32_batch(const int32_t* d_a, const int32_t* d_b, uint64_t* d_out, 
 int zav1_satd_batch(const int16_t* d_res, int32_t* d_coeff, int32_t* d_satd, int n, int n_jobs, zav1_stream_t s) {
   if ((n != 4 && n != 8 && n != 16 && n != 32) || n_jobs < 0) return ZAV1_EINVAL;
   ZCHECK(zav1::satd_batch(d_res, d_coeff, d_satd, n, n_jobs, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_scan_order(int tx_size, int scan_class, uint16_t* scan) {
+  if (tx_size < 0 || tx_size >= 19 || scan_class < 0 || scan_class > 2 || !scan) return ZAV1_EINVAL;
+  build_scan(tx_size, scan_class, scan);
+  int w = zav1::tx_w(tx_size), h = zav1::tx_h(tx_size);
+  return (w > 32 ? 32 : w) * (h > 32 ? 32 : h);
+}
+
+int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size, const int32_t* d_xy,
+                        int n_blocks, const uint8_t* types, int n_types, int qindex, uint64_t* d_dist,
+                        int32_t* d_eob, uint32_t* d_lsum, zav1_stream_t s) {
+  if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_blocks < 0 || n_types < 1 || n_types > 16 ||
+      !types || qindex < 0 || qindex > 255 || stride <= 0)
+    return ZAV1_EINVAL;
+  zav1::TypeList tl{};
+  tl.n = n_types;
+  for (int i = 0; i < n_types; i++) {
+    if (types[i] > 15 || !tx_supported(tx_size, types[i])) return ZAV1_ENOTSUP;
+    tl.t[i] = types[i];
+  }
+  const zav1::QuantRow* rows = nullptr;
+  const uint16_t* iscan = nullptr;
+  int rc = quant_rows(&rows);
+  if (rc) return rc;
+  rc = iscan_table(&iscan);
+  if (rc) return rc;
+  ZCHECK(zav1::txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, tl, rows + (bd == 10 ? 256 : 0) + qindex,
+                              iscan, d_dist, d_eob, d_lsum, S(s)));
   return ZAV1_OK;
 }
 

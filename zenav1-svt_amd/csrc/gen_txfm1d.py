@@ -348,6 +348,26 @@ def main():
     parts.append('static __device__ __constant__ const int32_t kSinpi[7][5] = {\n  %s};\n' % rows)
     with open(os.path.join(HERE, 'txfm1d_gen.h'), 'w') as f:
         f.write('\n'.join(parts))
+    # Algorithmic integer-op counts per 1D transform (the roofline numerator):
+    # add/sub/neg = 1, clamp = 2, half butterfly = 2 mul + add + round add +
+    # shift = 5.  Identity: 1 (x2, x4) or 3 (x sqrt2 scalings) per sample;
+    # 4-point ADST: 7 mul + 10 add + 4 round-shift pairs.
+    ops = {}
+    for name, n, fn in (('dct', 4, gen_fdct), ('dct', 8, gen_fdct), ('dct', 16, gen_fdct),
+                        ('dct', 32, gen_fdct), ('dct', 64, gen_fdct), ('adst', 8, gen_fadst8),
+                        ('adst', 16, gen_fadst16)):
+        em = Emitter(12)
+        fn(em, [em.tmp('x[%d]' % i) for i in range(n)])
+        cnt = 0
+        for line in em.lines[n:]:
+            cnt += 5 if 'zbtf' in line else (2 if 'zclamp' in line else 1)
+        ops['%s%d' % (name, n)] = cnt
+    ops['adst4'] = 7 + 10 + 8
+    for n in (4, 8, 16, 32, 64):
+        ops['identity%d' % n] = n * (3 if n in (4, 16, 64) else 1)
+    import json
+    with open(os.path.join(HERE, 'txfm1d_ops.json'), 'w') as f:
+        json.dump(ops, f, indent=1, sort_keys=True)
     print('wrote txfm1d_gen.h')
 
 

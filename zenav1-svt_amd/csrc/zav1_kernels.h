@@ -41,4 +41,16 @@ hipError_t sse_i32_batch(const int32_t* a, const int32_t* b, uint64_t* out, int 
 hipError_t satd_batch(const int16_t* res, int32_t* coeff, int32_t* satd, int n, int n_jobs,
                       hipStream_t st);
 
+// Candidate tx types of one search launch (passed by value).
+struct TypeList {
+  int32_t n;
+  uint8_t t[16];
+};
+// Fused tx-type search over blocks of one tx size on a 16-bit plane (8-bit
+// content is widened by the caller).  iscan: device table of inverse scans,
+// [tx_size][class][1024] u16.  Outputs per (block, type) pair.
+hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
+                          int n_blocks, const TypeList& types, const QuantRow* row, const uint16_t* iscan,
+                          uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st);
+
 }  // namespace zav1
