@@ -175,6 +175,50 @@ int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size
                         int n_blocks, const uint8_t *types, int n_types, int qindex, uint64_t *d_dist,
                         int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
 
+/* ------------------------------------------------------------------------ */
+/* Intra prediction (batched).                                               */
+/* ------------------------------------------------------------------------ */
+/* One transform unit.  Replaces the per-unit predictor routing of
+ * predict_unit (rust/crates/svtav1-encoder/src/leaf_funnel/predict.rs:49) and
+ * its u16 twin predict_unit_hbd (tx_pipeline.rs:667): the neighbour fill of
+ * extract_neighbors_tiled (partition.rs:250), the directional edge assembly,
+ * corner / edge filters and upsampling of dr_predict (intra_edge.rs:910) and
+ * the predictors of rust/crates/svtav1-dsp/src/intra_pred.rs:18-1060 /
+ * hbd.rs:169-850.  The caller supplies the availability counts (C
+ * n_top_px / n_topright_px / n_left_px / n_bottomleft_px, i.e. real reference
+ * samples after the frame-extent and coding-order rules of
+ * intra_edge.rs:355/436); the device reads every sample from the plane. */
+typedef struct zav1_intra_job {
+  int32_t x, y;        /* unit origin in the plane (samples) */
+  uint8_t w, h;        /* 4..64 */
+  uint8_t mode;        /* 0 DC, 1 V, 2 H, 3 D45, 4 D135, 5 D113, 6 D157, 7 D203,
+                          8 D67, 9 SMOOTH, 10 SMOOTH_V, 11 SMOOTH_H, 12 PAETH */
+  int8_t delta;        /* angle delta -3..3 (3-degree steps) */
+  uint8_t fi_mode;     /* filter intra 0..4 (w, h <= 32), 5 = none */
+  uint8_t edge_filter; /* sequence enable_intra_edge_filter */
+  uint8_t filt_type;   /* 1 if the above or left block uses a smooth mode */
+  uint8_t pad0;
+  int16_t n_top_px, n_topright_px, n_left_px, n_bottomleft_px;
+  uint32_t out_off;    /* d_out offset (samples) of this unit's w*h raster */
+  uint32_t pad1;
+} zav1_intra_job;
+
+/* d_plane: 8-bit (bd 8) or 16-bit (bd 10) reconstruction, `stride` samples;
+ * d_out: u16 predictions. */
+int zav1_intra_pred_batch(const void *d_plane, int stride, int bd, const zav1_intra_job *d_jobs, int n_jobs,
+                          uint16_t *d_out, zav1_stream_t s);
+
+/* Chroma-from-luma: cfl_luma_subsampling_420 + cfl_subtract_average +
+ * cfl_predict (rust/crates/svtav1-dsp/src/intra_pred.rs:1088,1110,1137;
+ * hbd.rs:855,889).  Per job: the 2cw x 2ch co-located luma at (lx, ly) of
+ * d_luma, the cw x ch chroma DC prediction at d_dc + dc_off, alpha in Q3. */
+typedef struct zav1_cfl_job {
+  int32_t lx, ly, cw, ch, alpha_q3;
+  uint32_t dc_off, out_off, pad;
+} zav1_cfl_job;
+int zav1_cfl_batch(const void *d_luma, int stride, int bd, const zav1_cfl_job *d_jobs, int n_jobs,
+                   const uint16_t *d_dc, uint16_t *d_out, zav1_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

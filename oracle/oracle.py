@@ -186,3 +186,30 @@ def txsearch_plane(plane, bd, tx_size, xy, types, qindex):
     L.or_txsearch_plane(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], bd, tx_size, xy.ctypes.data, nb,
                         t.ctypes.data, nt, qindex, dist.ctypes.data, eob.ctypes.data, ls.ctypes.data)
     return dist, eob, ls
+
+
+INTRA_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('w', 'u1'), ('h', 'u1'), ('mode', 'u1'), ('delta', 'i1'),
+                      ('fi_mode', 'u1'), ('edge_filter', 'u1'), ('filt_type', 'u1'), ('pad0', 'u1'),
+                      ('n_top_px', '<i2'), ('n_topright_px', '<i2'), ('n_left_px', '<i2'),
+                      ('n_bottomleft_px', '<i2'), ('out_off', '<u4'), ('pad1', '<u4')])
+
+
+def intra_predict(plane, bd, job):
+    """Predict one unit; job is a length-1 INTRA_JOB record array."""
+    L = lib()
+    L.or_intra_predict.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
+    p = np.ascontiguousarray(plane)
+    j = np.ascontiguousarray(job, dtype=INTRA_JOB).reshape(1)
+    out = np.zeros(int(j['w'][0]) * int(j['h'][0]), np.int32)
+    ok = L.or_intra_predict(p.ctypes.data, p.shape[1], bd, j.ctypes.data, out.ctypes.data)
+    return out.reshape(int(j['h'][0]), int(j['w'][0])) if ok else None
+
+
+def cfl_predict(luma, bd, cw, ch, dc, alpha_q3):
+    L = lib()
+    L.or_cfl_predict.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp]
+    lm = np.ascontiguousarray(luma)
+    d = np.ascontiguousarray(dc, dtype=np.int32)
+    out = np.zeros(cw * ch, np.int32)
+    L.or_cfl_predict(lm.ctypes.data, lm.shape[1], bd, cw, ch, d.ctypes.data, alpha_q3, out.ctypes.data)
+    return out.reshape(ch, cw)

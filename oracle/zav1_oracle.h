@@ -96,6 +96,32 @@ void or_txsearch_plane(const uint16_t *plane, int stride, int width, int height,
                        const int32_t *xy, int n_blocks, const uint8_t *types, int n_types, int qindex,
                        uint64_t *dist, int32_t *eob, uint32_t *lsum);
 
+/* ---- intra prediction (intra.c) ----------------------------------------- */
+/* One transform unit to predict.  The host computes the availability counts
+ * (C n_top_px / n_topright_px / n_left_px / n_bottomleft_px: real reference
+ * samples, capped by the aligned frame / tile extent and the coding-order
+ * top-right / bottom-left rules); everything sample-dependent is here. */
+typedef struct {
+  int32_t x, y;          /* tx-unit origin in the plane (samples) */
+  uint8_t w, h;          /* tx dims 4..64 */
+  uint8_t mode;          /* 0 DC,1 V,2 H,3 D45,4 D135,5 D113,6 D157,7 D203,8 D67,
+                            9 SMOOTH,10 SMOOTH_V,11 SMOOTH_H,12 PAETH */
+  int8_t delta;          /* angle delta, -3..3 (x 3 degrees) */
+  uint8_t fi_mode;       /* filter-intra mode 0..4, 5 = none */
+  uint8_t edge_filter;   /* sequence enable_intra_edge_filter */
+  uint8_t filt_type;     /* 1 when an above/left neighbour block is smooth */
+  uint8_t pad0;
+  int16_t n_top_px, n_topright_px, n_left_px, n_bottomleft_px;
+  uint32_t out_off;      /* output offset (samples) */
+  uint32_t pad1;
+} or_intra_job;
+int or_edge_filter_strength(int bs0, int bs1, int delta, int filt_type);
+int or_use_upsample(int bs0, int bs1, int delta, int filt_type);
+/* lf/predict.rs:49 predict_unit (+ hbd twins): w*h samples into dst (raster). */
+int or_intra_predict(const void *plane, int stride, int bd, const or_intra_job *job, int32_t *dst);
+void or_cfl_predict(const void *luma, int luma_stride, int bd, int cw, int ch, const int32_t *dc, int alpha_q3,
+                    int32_t *dst);
+
 #ifdef __cplusplus
 }
 #endif

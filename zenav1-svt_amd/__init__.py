@@ -100,7 +100,17 @@ EXPORTS = {
     'zav1_satd_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_scan_order': [_i32, _i32, _vp],
     'zav1_txsearch_batch': [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp],
+    'zav1_intra_pred_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
+    'zav1_cfl_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
 }
+
+# numpy layouts of the job structs (match include/zav1_dsp.h)
+INTRA_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('w', 'u1'), ('h', 'u1'), ('mode', 'u1'), ('delta', 'i1'),
+                      ('fi_mode', 'u1'), ('edge_filter', 'u1'), ('filt_type', 'u1'), ('pad0', 'u1'),
+                      ('n_top_px', '<i2'), ('n_topright_px', '<i2'), ('n_left_px', '<i2'),
+                      ('n_bottomleft_px', '<i2'), ('out_off', '<u4'), ('pad1', '<u4')])
+CFL_JOB = np.dtype([('lx', '<i4'), ('ly', '<i4'), ('cw', '<i4'), ('ch', '<i4'), ('alpha_q3', '<i4'),
+                    ('dc_off', '<u4'), ('out_off', '<u4'), ('pad', '<u4')])
 
 
 def lib():
@@ -240,3 +250,12 @@ def txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, types, qindex, 
     t = np.ascontiguousarray(types, dtype=np.uint8)
     check(lib().zav1_txsearch_batch(d_plane.ptr, stride, bd, tx_size, d_xy.ptr, n_blocks, t.ctypes.data, t.size,
                                     qindex, d_dist.ptr, d_eob.ptr, d_lsum.ptr, stream), 'txsearch_batch')
+
+
+def intra_pred_batch(d_plane, stride, bd, d_jobs, n_jobs, d_out, stream=None):
+    check(lib().zav1_intra_pred_batch(d_plane.ptr, stride, bd, d_jobs.ptr, n_jobs, d_out.ptr, stream),
+          'intra_pred_batch')
+
+
+def cfl_batch(d_luma, stride, bd, d_jobs, n_jobs, d_dc, d_out, stream=None):
+    check(lib().zav1_cfl_batch(d_luma.ptr, stride, bd, d_jobs.ptr, n_jobs, d_dc.ptr, d_out.ptr, stream), 'cfl_batch')

@@ -486,4 +486,24 @@ int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size
   return ZAV1_OK;
 }
 
+int zav1_intra_pred_batch(const void* d_plane, int stride, int bd, const zav1_intra_job* d_jobs, int n_jobs,
+                          uint16_t* d_out, zav1_stream_t s) {
+  static_assert(sizeof(zav1_intra_job) == sizeof(zav1::IntraJob), "intra job layout");
+  if ((bd != 8 && bd != 10) || n_jobs < 0 || stride <= 0 || (n_jobs && (!d_plane || !d_jobs || !d_out)))
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::intra_pred_batch(d_plane, stride, bd, reinterpret_cast<const zav1::IntraJob*>(d_jobs), n_jobs, d_out,
+                                S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_cfl_batch(const void* d_luma, int stride, int bd, const zav1_cfl_job* d_jobs, int n_jobs,
+                   const uint16_t* d_dc, uint16_t* d_out, zav1_stream_t s) {
+  static_assert(sizeof(zav1_cfl_job) == sizeof(zav1::CflJob), "cfl job layout");
+  if ((bd != 8 && bd != 10) || n_jobs < 0 || stride <= 0 || (n_jobs && (!d_luma || !d_jobs || !d_dc || !d_out)))
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::cfl_batch(d_luma, stride, bd, reinterpret_cast<const zav1::CflJob*>(d_jobs), n_jobs, d_dc, d_out,
+                         S(s)));
+  return ZAV1_OK;
+}
+
 }  // extern "C"

@@ -53,4 +53,24 @@ hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size
                           int n_blocks, const TypeList& types, const QuantRow* row, const uint16_t* iscan,
                           uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st);
 
+// One intra transform unit (layout of zav1_intra_job in include/zav1_dsp.h).
+struct IntraJob {
+  int32_t x, y;
+  uint8_t w, h, mode;
+  int8_t delta;
+  uint8_t fi_mode, edge_filter, filt_type, pad0;
+  int16_t n_top_px, n_topright_px, n_left_px, n_bottomleft_px;
+  uint32_t out_off, pad1;
+};
+struct CflJob {
+  int32_t lx, ly;       // luma origin of the 2cw x 2ch co-located luma
+  int32_t cw, ch;       // chroma block dims (4..32)
+  int32_t alpha_q3;
+  uint32_t dc_off, out_off, pad;
+};
+hipError_t intra_pred_batch(const void* plane, int stride, int bd, const IntraJob* jobs, int n_jobs, uint16_t* out,
+                            hipStream_t st);
+hipError_t cfl_batch(const void* luma, int stride, int bd, const CflJob* jobs, int n_jobs, const uint16_t* dc,
+                     uint16_t* out, hipStream_t st);
+
 }  // namespace zav1
