@@ -1,0 +1,417 @@
+/*
+ * Oracle: deblocking filters + frame edge walk, and the CDEF direction /
+ * filter / strength-SSE search (TEST INFRASTRUCTURE ONLY — see
+ * zav1_oracle.h).
+ *
+ *   lf_thresholds                dsp/loop_filter.rs:50
+ *   masks, filter4/6/8/14        dsp/loop_filter.rs:76-286; hbd twins
+ *                                dsp/hbd.rs:1040-1330 (8-bit is the bd=8 case:
+ *                                bias 0x80, clamp [-128,127], no shift)
+ *   lpf_params / filter_plane    enc/deblock.rs:698 / :811 (all vertical
+ *                                edges, then all horizontal edges)
+ *   cdef_find_dir                dsp/cdef.rs:143
+ *   constrain / adjust_strength  dsp/cdef.rs:113 / :125
+ *   cdef_filter_block            dsp/cdef.rs:931 (hbd dsp/hbd.rs:1765)
+ *   build_src / filter_fb_packed / dist / cdef_search_still(_hbd)
+ *                                enc/cdef.rs:517 / :952,:1045 / :1128 /
+ *                                :1166,:1374
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "zav1_oracle.h"
+
+static int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static int32_t iabs(int32_t v) { return v < 0 ? -v : v; }
+
+/* ------------------------------------------------------------------ */
+/* deblocking                                                          */
+/* ------------------------------------------------------------------ */
+void or_lf_thresholds(int level, int sharpness, int *mblim, int *lim, int *hev) {
+  int bil = level >> ((sharpness > 0) + (sharpness > 4));
+  if (sharpness > 0 && bil > 9 - sharpness) bil = 9 - sharpness;
+  if (bil < 1) bil = 1;
+  *mblim = 2 * (level + 2) + bil;
+  *lim = bil;
+  *hev = level >> 4;
+}
+
+typedef struct { int32_t lim, mblim, hev, shift, bd; } thr_t;
+
+static int32_t sclamp(int32_t t, int bd) {
+  int32_t lo = -(128 << (bd - 8)), hi = (128 << (bd - 8)) - 1;
+  return clampi(t, lo, hi);
+}
+
+/* filter_mask over n samples w[0..n) (n = 4, 6, 8): limit on neighbour
+ * steps, blimit on the p0/q0 step. */
+static int mask_n(const thr_t *t, const int32_t *w, int n) {
+  int32_t l = t->lim << t->shift, bl = t->mblim << t->shift;
+  int h = n / 2, bad = 0;
+  for (int k = 0; k + 1 < h; k++) bad |= iabs(w[k] - w[k + 1]) > l;          /* p side */
+  for (int k = h; k + 1 < n; k++) bad |= iabs(w[k] - w[k + 1]) > l;          /* q side */
+  bad |= iabs(w[h - 1] - w[h]) * 2 + iabs(w[h - 2] - w[h + 1]) / 2 > bl;
+  return !bad;
+}
+
+static int flat_n(int32_t thr, int shift, const int32_t *p, const int32_t *q, int m) {
+  /* |p_k - p0| and |q_k - q0| for k = 1..m-1 against thr << shift */
+  int32_t t = thr << shift;
+  int bad = 0;
+  for (int k = 1; k < m; k++) bad |= iabs(p[k] - p[0]) > t || iabs(q[k] - q[0]) > t;
+  return !bad;
+}
+
+static void filter4(int mask, const thr_t *t, int32_t *w) {
+  int32_t bias = 0x80 << t->shift;
+  int32_t ps1 = w[0] - bias, ps0 = w[1] - bias, qs0 = w[2] - bias, qs1 = w[3] - bias;
+  int32_t th = t->hev << t->shift;
+  int32_t hev = (iabs(w[0] - w[1]) > th || iabs(w[3] - w[2]) > th) ? -1 : 0;
+  int32_t m = mask ? -1 : 0;
+  int32_t f = sclamp(ps1 - qs1, t->bd) & hev;
+  f = sclamp(f + 3 * (qs0 - ps0), t->bd) & m;
+  int32_t f1 = sclamp(f + 4, t->bd) >> 3, f2 = sclamp(f + 3, t->bd) >> 3;
+  w[2] = sclamp(qs0 - f1, t->bd) + bias;
+  w[1] = sclamp(ps0 + f2, t->bd) + bias;
+  int32_t o = ((f1 + 1) >> 1) & ~hev;
+  w[3] = sclamp(qs1 - o, t->bd) + bias;
+  w[0] = sclamp(ps1 + o, t->bd) + bias;
+}
+
+#define R3(x) (((x) + 4) >> 3)
+#define R4(x) (((x) + 8) >> 4)
+
+/* One line across an edge: w holds `len` samples centred on the edge. */
+static void lpf_line(int len, const thr_t *t, int32_t *w) {
+  if (len == 4) {
+    filter4(mask_n(t, w, 4), t, w);
+  } else if (len == 6) {
+    int m = mask_n(t, w, 6);
+    int32_t p[3] = {w[2], w[1], w[0]}, q[3] = {w[3], w[4], w[5]};
+    int fl = flat_n(1, t->shift, p, q, 3);
+    if (fl && m) {
+      int32_t p2 = w[0], p1 = w[1], p0 = w[2], q0 = w[3], q1 = w[4], q2 = w[5];
+      w[1] = R3(p2 * 3 + p1 * 2 + p0 * 2 + q0);
+      w[2] = R3(p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1);
+      w[3] = R3(p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2);
+      w[4] = R3(p0 + q0 * 2 + q1 * 2 + q2 * 3);
+    } else {
+      filter4(m, t, w + 1);
+    }
+  } else if (len == 8) {
+    int m = mask_n(t, w, 8);
+    int32_t p[4] = {w[3], w[2], w[1], w[0]}, q[4] = {w[4], w[5], w[6], w[7]};
+    int fl = flat_n(1, t->shift, p, q, 4);
+    if (fl && m) {
+      int32_t p3 = w[0], p2 = w[1], p1 = w[2], p0 = w[3], q0 = w[4], q1 = w[5], q2 = w[6], q3 = w[7];
+      w[1] = R3(p3 + p3 + p3 + 2 * p2 + p1 + p0 + q0);
+      w[2] = R3(p3 + p3 + p2 + 2 * p1 + p0 + q0 + q1);
+      w[3] = R3(p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2);
+      w[4] = R3(p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3);
+      w[5] = R3(p1 + p0 + q0 + 2 * q1 + q2 + q3 + q3);
+      w[6] = R3(p0 + q0 + q1 + 2 * q2 + q3 + q3 + q3);
+    } else {
+      filter4(m, t, w + 2);
+    }
+  } else {
+    int m = mask_n(t, w + 3, 8);
+    int32_t p[4] = {w[6], w[5], w[4], w[3]}, q[4] = {w[7], w[8], w[9], w[10]};
+    int fl = flat_n(1, t->shift, p, q, 4);
+    int32_t p2a[4] = {w[6], w[2], w[1], w[0]}, q2a[4] = {w[7], w[11], w[12], w[13]};
+    int fl2 = flat_n(1, t->shift, p2a, q2a, 4);
+    if (fl2 && fl && m) {
+      int32_t p6 = w[0], p5 = w[1], p4 = w[2], p3 = w[3], p2 = w[4], p1 = w[5], p0 = w[6];
+      int32_t q0 = w[7], q1 = w[8], q2 = w[9], q3 = w[10], q4 = w[11], q5 = w[12], q6 = w[13];
+      w[1] = R4(p6 * 7 + p5 * 2 + p4 * 2 + p3 + p2 + p1 + p0 + q0);
+      w[2] = R4(p6 * 5 + p5 * 2 + p4 * 2 + p3 * 2 + p2 + p1 + p0 + q0 + q1);
+      w[3] = R4(p6 * 4 + p5 + p4 * 2 + p3 * 2 + p2 * 2 + p1 + p0 + q0 + q1 + q2);
+      w[4] = R4(p6 * 3 + p5 + p4 + p3 * 2 + p2 * 2 + p1 * 2 + p0 + q0 + q1 + q2 + q3);
+      w[5] = R4(p6 * 2 + p5 + p4 + p3 + p2 * 2 + p1 * 2 + p0 * 2 + q0 + q1 + q2 + q3 + q4);
+      w[6] = R4(p6 + p5 + p4 + p3 + p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1 + q2 + q3 + q4 + q5);
+      w[7] = R4(p5 + p4 + p3 + p2 + p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2 + q3 + q4 + q5 + q6);
+      w[8] = R4(p4 + p3 + p2 + p1 + p0 + q0 * 2 + q1 * 2 + q2 * 2 + q3 + q4 + q5 + q6 * 2);
+      w[9] = R4(p3 + p2 + p1 + p0 + q0 + q1 * 2 + q2 * 2 + q3 * 2 + q4 + q5 + q6 * 3);
+      w[10] = R4(p2 + p1 + p0 + q0 + q1 + q2 * 2 + q3 * 2 + q4 * 2 + q5 + q6 * 4);
+      w[11] = R4(p1 + p0 + q0 + q1 + q2 + q3 * 2 + q4 * 2 + q5 * 2 + q6 * 5);
+      w[12] = R4(p0 + q0 + q1 + q2 + q3 + q4 * 2 + q5 * 2 + q6 * 7);
+    } else if (fl && m) {
+      int32_t p3 = w[3], p2 = w[4], p1 = w[5], p0 = w[6], q0 = w[7], q1 = w[8], q2 = w[9], q3 = w[10];
+      w[4] = R3(p3 + p3 + p3 + 2 * p2 + p1 + p0 + q0);
+      w[5] = R3(p3 + p3 + p2 + 2 * p1 + p0 + q0 + q1);
+      w[6] = R3(p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2);
+      w[7] = R3(p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3);
+      w[8] = R3(p1 + p0 + q0 + 2 * q1 + q2 + q3 + q3);
+      w[9] = R3(p0 + q0 + q1 + 2 * q2 + q3 + q3 + q3);
+    } else {
+      filter4(m, t, w + 5);
+    }
+  }
+}
+
+static int32_t ld(const void *p, int hbd, long i) { return hbd ? ((const uint16_t *)p)[i] : ((const uint8_t *)p)[i]; }
+static void st(void *p, int hbd, long i, int32_t v) {
+  if (hbd) ((uint16_t *)p)[i] = (uint16_t)v;
+  else ((uint8_t *)p)[i] = (uint8_t)v;
+}
+
+/* lpf_{horizontal,vertical}_{4,6,8,14}: 4 lines at sample offset `off`. */
+void or_lpf(void *buf, int hbd, int bd, long off, int pitch, int vertical, int len, int level, int sharpness) {
+  thr_t t;
+  or_lf_thresholds(level, sharpness, &t.mblim, &t.lim, &t.hev);
+  t.shift = bd - 8;
+  t.bd = bd;
+  int32_t w[14];
+  for (int i = 0; i < 4; i++) {
+    long base = vertical ? off + (long)i * pitch : off + i;
+    long step = vertical ? 1 : pitch;
+    long start = base - (len / 2) * step;
+    for (int k = 0; k < len; k++) w[k] = ld(buf, hbd, start + k * step);
+    lpf_line(len, &t, w);
+    for (int k = 0; k < len; k++) st(buf, hbd, start + k * step, w[k]);
+  }
+}
+
+/* enc/deblock.rs:698 lpf_params: (advance in mi, filter length or 0). */
+static int lpf_params(const or_deblock_geom *g, int vert, int x, int y, int plane, int ss, int level, int *len) {
+  *len = 0;
+  if (x >= (g->true_w >> ss) || y >= (g->true_h >> ss)) return 1;
+  int mi_row = ss | ((y << ss) >> 2), mi_col = ss | ((x << ss) >> 2);
+  long idx = (long)mi_row * g->mi_cols + mi_col;
+#define TXW(i) (plane == 0 ? g->tw[i] : ((g->bw[i] >> 1) > 4 ? (g->bw[i] >> 1) : 4))
+#define TXH(i) (plane == 0 ? g->th[i] : ((g->bh[i] >> 1) > 4 ? (g->bh[i] >> 1) : 4))
+  int ts = vert ? TXW(idx) : TXH(idx);
+  int adv = ts >> 2;
+  int coord = vert ? x : y;
+  if (coord & (ts - 1)) return adv;
+  if (coord == 0) return adv;
+  long prev = vert ? idx - (1 << ss) : idx - ((long)g->mi_cols << ss);
+  int pv = vert ? TXW(prev) : TXH(prev);
+  int cs = g->skip ? g->skip[idx] : 0, ps = g->skip ? g->skip[prev] : 0;
+  int pu_edge = g->block_id[prev] != g->block_id[idx];
+  if (level != 0 && (!ps || !cs || pu_edge)) {
+    int md = ts < pv ? ts : pv;
+    *len = md == 4 ? 4 : (plane != 0 ? 6 : (md == 8 ? 8 : 14));
+  }
+#undef TXW
+#undef TXH
+  return adv;
+}
+
+/* enc/deblock.rs:811 filter_plane / :875 filter_plane_hbd. */
+void or_deblock_plane(void *buf, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
+                      int level_h, int sharpness, const or_deblock_geom *g) {
+  int hbd = bd > 8, mc = plane_w / 4, mr = plane_h / 4, len;
+  for (int my = 0; my < mr; my++)
+    for (int mx = 0; mx < mc;) {
+      int adv = lpf_params(g, 1, mx * 4, my * 4, plane, ss, level_v, &len);
+      if (len) or_lpf(buf, hbd, bd, (long)my * 4 * stride + mx * 4, stride, 1, len, level_v, sharpness);
+      mx += adv;
+    }
+  for (int mx = 0; mx < mc; mx++)
+    for (int my = 0; my < mr;) {
+      int adv = lpf_params(g, 0, mx * 4, my * 4, plane, ss, level_h, &len);
+      if (len) or_lpf(buf, hbd, bd, (long)my * 4 * stride + mx * 4, stride, 0, len, level_h, sharpness);
+      my += adv;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* CDEF                                                                */
+/* ------------------------------------------------------------------ */
+#define BSTRIDE 144
+#define VB 3
+#define HB 8
+#define VERY_LARGE 0x7f7f
+
+static int msb(uint32_t n) { return 31 - __builtin_clz(n); }
+
+int or_cdef_constrain(int diff, int threshold, int damping) {
+  if (threshold == 0) return 0;
+  int shift = damping - msb((uint32_t)threshold);
+  if (shift < 0) shift = 0;
+  int ad = iabs(diff), m = threshold - (ad >> shift);
+  if (m < 0) m = 0;
+  int v = ad < m ? ad : m;
+  return diff < 0 ? -v : v;
+}
+
+int or_cdef_adjust_strength(int strength, int var) {
+  int i = (var >> 6) ? msb((uint32_t)(var >> 6)) : 0;
+  if (i > 12) i = 12;
+  return var ? (strength * (4 + i) + 8) >> 4 : 0;
+}
+
+/* dsp/cdef.rs:143 */
+int or_cdef_find_dir(const uint16_t *img, int stride, int coeff_shift, int *var_out) {
+  static const int DIV[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+  int32_t cost[8] = {0}, partial[8][15];
+  memset(partial, 0, sizeof(partial));
+  for (int i = 0; i < 8; i++)
+    for (int j = 0; j < 8; j++) {
+      int x = (img[i * stride + j] >> coeff_shift) - 128;
+      partial[0][i + j] += x;
+      partial[1][i + j / 2] += x;
+      partial[2][i] += x;
+      partial[3][3 + i - j / 2] += x;
+      partial[4][7 + i - j] += x;
+      partial[5][3 - i / 2 + j] += x;
+      partial[6][j] += x;
+      partial[7][i / 2 + j] += x;
+    }
+  for (int i = 0; i < 8; i++) {
+    cost[2] += partial[2][i] * partial[2][i];
+    cost[6] += partial[6][i] * partial[6][i];
+  }
+  cost[2] *= DIV[8];
+  cost[6] *= DIV[8];
+  for (int i = 0; i < 7; i++) {
+    cost[0] += (partial[0][i] * partial[0][i] + partial[0][14 - i] * partial[0][14 - i]) * DIV[i + 1];
+    cost[4] += (partial[4][i] * partial[4][i] + partial[4][14 - i] * partial[4][14 - i]) * DIV[i + 1];
+  }
+  cost[0] += partial[0][7] * partial[0][7] * DIV[8];
+  cost[4] += partial[4][7] * partial[4][7] * DIV[8];
+  for (int i = 1; i < 8; i += 2) {
+    for (int j = 0; j < 5; j++) cost[i] += partial[i][3 + j] * partial[i][3 + j];
+    cost[i] *= DIV[8];
+    for (int j = 0; j < 3; j++)
+      cost[i] += (partial[i][j] * partial[i][j] + partial[i][10 - j] * partial[i][10 - j]) * DIV[2 * j + 2];
+  }
+  int32_t best = 0;
+  int bd = 0;
+  for (int i = 0; i < 8; i++)
+    if (cost[i] > best) { best = cost[i]; bd = i; }
+  *var_out = (best - cost[(bd + 4) & 7]) >> 10;
+  return bd;
+}
+
+/* dsp/cdef.rs:73 direction offsets in the padded buffer */
+static int cdef_dir_off(int dir, int k) {
+  static const int D[12][2] = {
+      {BSTRIDE, 2 * BSTRIDE},      {BSTRIDE, 2 * BSTRIDE - 1}, {-BSTRIDE + 1, -2 * BSTRIDE + 2},
+      {1, -BSTRIDE + 2},           {1, 2},                     {1, BSTRIDE + 2},
+      {BSTRIDE + 1, 2 * BSTRIDE + 2}, {BSTRIDE, 2 * BSTRIDE + 1}, {BSTRIDE, 2 * BSTRIDE},
+      {BSTRIDE, 2 * BSTRIDE - 1},  {-BSTRIDE + 1, -2 * BSTRIDE + 2}, {1, -BSTRIDE + 2}};
+  return D[dir + 2][k];
+}
+
+/* dsp/cdef.rs:931 / hbd.rs:1765: one 8x8 (or 4x4) block, rows stepped by
+ * `sub`, output into dst (dstride). */
+void or_cdef_filter_block(int32_t *dst, int dstride, const uint16_t *in, int pri, int sec, int dir, int pdamp,
+                          int sdamp, int bw, int bh, int coeff_shift, int sub) {
+  static const int PT[2][2] = {{4, 2}, {3, 3}};
+  const int *pt = PT[(pri >> coeff_shift) & 1];
+  const int stt[2] = {2, 1};
+  for (int i = 0; i < bh; i += sub)
+    for (int j = 0; j < bw; j++) {
+      const uint16_t *c = in + i * BSTRIDE + j;
+      int32_t x = (int16_t)c[0];
+      int32_t sum = 0, mx = x, mn = x;
+      for (int k = 0; k < 2; k++) {
+        int dk = cdef_dir_off(dir, k);
+        int32_t p0 = (int16_t)c[dk], p1 = (int16_t)c[-dk];
+        sum += pt[k] * or_cdef_constrain(p0 - x, pri, pdamp);
+        sum += pt[k] * or_cdef_constrain(p1 - x, pri, pdamp);
+        if ((uint16_t)p0 != VERY_LARGE && p0 > mx) mx = p0;
+        if ((uint16_t)p1 != VERY_LARGE && p1 > mx) mx = p1;
+        if (p0 < mn) mn = p0;
+        if (p1 < mn) mn = p1;
+        int da = cdef_dir_off(dir + 2, k), db = cdef_dir_off(dir - 2, k);
+        int32_t s[4] = {(int16_t)c[da], (int16_t)c[-da], (int16_t)c[db], (int16_t)c[-db]};
+        for (int m = 0; m < 4; m++) {
+          if ((uint16_t)s[m] != VERY_LARGE && s[m] > mx) mx = s[m];
+          if (s[m] < mn) mn = s[m];
+          sum += stt[k] * or_cdef_constrain(s[m] - x, sec, sdamp);
+        }
+      }
+      int32_t s16 = (int16_t)sum; /* the reference accumulates in wrapping i16 */
+      dst[i * dstride + j] = clampi(x + ((8 + s16 - (s16 < 0)) >> 4), mn, mx);
+    }
+}
+
+/* enc/cdef.rs:517 build_src: padded fb source, VERY_LARGE outside the plane. */
+static void build_src(uint16_t *src, const void *pre, int hbd, int pw, int ph, int y0, int x0, int vsize, int hsize) {
+  for (int r = 0; r < vsize + 2 * VB; r++) {
+    int gy = y0 + r - VB;
+    for (int c = 0; c < hsize + 2 * HB; c++) {
+      int gx = x0 + c - HB;
+      src[r * BSTRIDE + c] =
+          (gy < 0 || gy >= ph || gx < 0 || gx >= pw) ? VERY_LARGE : (uint16_t)ld(pre, hbd, (long)gy * pw + gx);
+    }
+  }
+}
+
+/* enc/cdef.rs:1166 cdef_search_still / :1374 _hbd for one 64x64 filter
+ * block (fbr, fbc), every candidate in fs[0..n_fs), every 8x8 in the fb
+ * (no skip map).  Writes mse_y[n_fs] and mse_uv[n_fs] (U + V accumulated);
+ * planes are tightly packed (stride = plane width).  chroma may be NULL. */
+void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, const void *src_y,
+                       const void *src_u, const void *src_v, int width, int height, int bd, int qindex,
+                       int subsampling, int fbr, int fbc, const int *fs, int n_fs, uint64_t *mse_y,
+                       uint64_t *mse_uv) {
+  const int hbd = bd > 8, cs = bd - 8;
+  const int damping = 3 + (qindex >> 6);
+  const int sub_y = subsampling < 4 ? subsampling : 4, sub_uv = subsampling < 1 ? subsampling : 1;
+  int vsize = height - fbr * 64 < 64 ? height - fbr * 64 : 64;
+  int hsize = width - fbc * 64 < 64 ? width - fbc * 64 : 64;
+  uint16_t *pad = (uint16_t *)malloc(sizeof(uint16_t) * BSTRIDE * (128 + 2 * VB));
+  int32_t blk[64];
+  int dir[8][8], var[8][8];
+  const int base = VB * BSTRIDE + HB;
+  build_src(pad, rec_y, hbd, width, height, fbr * 64, fbc * 64, vsize, hsize);
+  for (int by = 0; by < vsize / 8; by++)
+    for (int bx = 0; bx < hsize / 8; bx++)
+      dir[by][bx] = or_cdef_find_dir(pad + base + by * 8 * BSTRIDE + bx * 8, BSTRIDE, cs, &var[by][bx]);
+  for (int g = 0; g < n_fs; g++) {
+    int pri = (fs[g] / 4) << cs, s = fs[g] % 4, sec = (s + (s == 3)) << cs;
+    int dmp = damping + cs;
+    uint64_t d = 0;
+    for (int by = 0; by < vsize / 8; by++)
+      for (int bx = 0; bx < hsize / 8; bx++) {
+        const uint16_t *in = pad + base + by * 8 * BSTRIDE + bx * 8;
+        if (fs[g] == 0) {
+          for (int i = 0; i < 8; i += sub_y)
+            for (int j = 0; j < 8; j++) blk[i * 8 + j] = in[i * BSTRIDE + j];
+        } else {
+          int t = or_cdef_adjust_strength(pri, var[by][bx]);
+          or_cdef_filter_block(blk, 8, in, t, sec, pri ? dir[by][bx] : 0, dmp, dmp, 8, 8, cs, sub_y);
+        }
+        for (int i = 0; i < 8; i += sub_y)
+          for (int j = 0; j < 8; j++) {
+            int32_t e = ld(src_y, hbd, (long)(fbr * 64 + by * 8 + i) * width + fbc * 64 + bx * 8 + j) - blk[i * 8 + j];
+            d += (uint64_t)(e * e);
+          }
+      }
+    mse_y[g] = (d >> (2 * cs)) * (uint64_t)sub_y;
+  }
+  if (rec_u && mse_uv) {
+    int cw = width / 2, ch = height / 2;
+    for (int g = 0; g < n_fs; g++) mse_uv[g] = 0;
+    const void *recs[2] = {rec_u, rec_v}, *srcs[2] = {src_u, src_v};
+    for (int pl = 0; pl < 2; pl++) {
+      build_src(pad, recs[pl], hbd, cw, ch, fbr * 32, fbc * 32, vsize / 2, hsize / 2);
+      for (int g = 0; g < n_fs; g++) {
+        int pri = (fs[g] / 4) << cs, s = fs[g] % 4, sec = (s + (s == 3)) << cs;
+        int dmp = damping + cs - 1;
+        uint64_t d = 0;
+        for (int by = 0; by < vsize / 8; by++)
+          for (int bx = 0; bx < hsize / 8; bx++) {
+            const uint16_t *in = pad + base + by * 4 * BSTRIDE + bx * 4;
+            if (fs[g] == 0) {
+              for (int i = 0; i < 4; i += sub_uv)
+                for (int j = 0; j < 4; j++) blk[i * 4 + j] = in[i * BSTRIDE + j];
+            } else {
+              or_cdef_filter_block(blk, 4, in, pri, sec, pri ? dir[by][bx] : 0, dmp, dmp, 4, 4, cs, sub_uv);
+            }
+            for (int i = 0; i < 4; i += sub_uv)
+              for (int j = 0; j < 4; j++) {
+                int32_t e = ld(srcs[pl], hbd, (long)(fbr * 32 + by * 4 + i) * cw + fbc * 32 + bx * 4 + j) -
+                            blk[i * 4 + j];
+                d += (uint64_t)(e * e);
+              }
+          }
+        mse_uv[g] += (d >> (2 * cs)) * (uint64_t)sub_uv;
+      }
+    }
+  }
+  free(pad);
+}

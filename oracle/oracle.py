@@ -213,3 +213,97 @@ def cfl_predict(luma, bd, cw, ch, dc, alpha_q3):
     out = np.zeros(cw * ch, np.int32)
     L.or_cfl_predict(lm.ctypes.data, lm.shape[1], bd, cw, ch, d.ctypes.data, alpha_q3, out.ctypes.data)
     return out.reshape(ch, cw)
+
+
+class DeblockGeom(ctypes.Structure):
+    _fields_ = [('mi_cols', ctypes.c_int), ('mi_rows', ctypes.c_int), ('true_w', ctypes.c_int),
+                ('true_h', ctypes.c_int), ('block_id', _vp), ('bw', _vp), ('bh', _vp), ('tw', _vp), ('th', _vp),
+                ('skip', _vp)]
+
+
+def _bind_lf(L):
+    L.or_cdef_constrain.argtypes = [ctypes.c_int] * 3
+    L.or_cdef_adjust_strength.argtypes = [ctypes.c_int] * 2
+    L.or_cdef_find_dir.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.or_cdef_filter_block.argtypes = [_vp, ctypes.c_int, _vp] + [ctypes.c_int] * 9
+    L.or_lpf.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_long] + [ctypes.c_int] * 5
+    L.or_deblock_plane.argtypes = [_vp] + [ctypes.c_int] * 9 + [ctypes.POINTER(DeblockGeom)]
+    L.or_cdef_search_fb.argtypes = [_vp] * 6 + [ctypes.c_int] * 7 + [_vp, ctypes.c_int, _vp, _vp]
+    L.or_lf_thresholds.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+
+
+def lflib():
+    L = lib()
+    _bind_lf(L)
+    return L
+
+
+def cdef_find_dir(block8x8, coeff_shift=0):
+    b = np.ascontiguousarray(block8x8, dtype=np.uint16)
+    v = ctypes.c_int()
+    d = lflib().or_cdef_find_dir(b.ctypes.data, b.shape[1], coeff_shift, ctypes.byref(v))
+    return d, v.value
+
+
+def deblock_plane(plane, bd, plane_idx, ss, level_v, level_h, sharpness, geom_arrays, mi_cols, mi_rows,
+                  true_w, true_h):
+    """In-place deblock of a copy of `plane`; geom_arrays: dict of numpy arrays."""
+    p = np.ascontiguousarray(plane).copy()
+    g = DeblockGeom(mi_cols, mi_rows, true_w, true_h, geom_arrays['block_id'].ctypes.data,
+                    geom_arrays['bw'].ctypes.data, geom_arrays['bh'].ctypes.data, geom_arrays['tw'].ctypes.data,
+                    geom_arrays['th'].ctypes.data, geom_arrays['skip'].ctypes.data if 'skip' in geom_arrays else None)
+    lflib().or_deblock_plane(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], bd, plane_idx, ss, level_v,
+                             level_h, sharpness, ctypes.byref(g))
+    return p
+
+
+def cdef_search_fb(rec, src, width, height, bd, qindex, sub, fbr, fbc, fs):
+    """rec/src: (y, u, v) tuples of planes (u/v may be None)."""
+    fsa = np.ascontiguousarray(fs, dtype=np.int32)
+    my = np.zeros(fsa.size, np.uint64)
+    muv = np.zeros(fsa.size, np.uint64)
+    ptr = lambda a: a.ctypes.data if a is not None else None
+    lflib().or_cdef_search_fb(ptr(rec[0]), ptr(rec[1]), ptr(rec[2]), ptr(src[0]), ptr(src[1]), ptr(src[2]),
+                              width, height, bd, qindex, sub, fbr, fbc, fsa.ctypes.data, fsa.size, my.ctypes.data,
+                              muv.ctypes.data if rec[1] is not None else None)
+    return my, muv
+
+
+def _bind_lr(L):
+    L.or_count_units.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.or_unit_limits.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int)] * 4
+    L.or_wiener_stats.argtypes = [ctypes.c_int, _vp, _vp] + [ctypes.c_int] * 7 + [_vp, _vp]
+    L.or_sse_region.argtypes = [_vp, _vp] + [ctypes.c_int] * 6
+    L.or_sse_region.restype = ctypes.c_uint64
+    L.or_sgr_param.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.or_sgr_unit_stats.argtypes = [_vp, _vp] + [ctypes.c_int] * 8 + [_vp]
+
+
+def lrlib():
+    L = lib()
+    _bind_lr(L)
+    return L
+
+
+def unit_limits(unit_size, pw, ph, ss_y, ux, uy):
+    v = [ctypes.c_int() for _ in range(4)]
+    lrlib().or_unit_limits(unit_size, pw, ph, ss_y, ux, uy, *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def wiener_stats(win, dgd, src, bd, lim):
+    d = np.ascontiguousarray(dgd)
+    s = np.ascontiguousarray(src)
+    M = np.zeros(win * win, np.int64)
+    H = np.zeros(win ** 4, np.int64)
+    lrlib().or_wiener_stats(win, d.ctypes.data, s.ctypes.data, d.shape[1], d.shape[0], bd, *lim, M.ctypes.data,
+                            H.ctypes.data)
+    return M, H.reshape(win * win, win * win)
+
+
+def sgr_unit_stats(dgd, src, bd, lim, pset):
+    d = np.ascontiguousarray(dgd)
+    s = np.ascontiguousarray(src)
+    out = np.zeros(5, np.int64)
+    lrlib().or_sgr_unit_stats(d.ctypes.data, s.ctypes.data, d.shape[1], d.shape[0], bd, *lim, pset, out.ctypes.data)
+    return out

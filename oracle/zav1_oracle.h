@@ -122,6 +122,40 @@ int or_intra_predict(const void *plane, int stride, int bd, const or_intra_job *
 void or_cfl_predict(const void *luma, int luma_stride, int bd, int cw, int ch, const int32_t *dc, int alpha_q3,
                     int32_t *dst);
 
+/* ---- deblocking / CDEF (lf.c) ------------------------------------------ */
+/* Per-4x4 (mi) frame geometry (enc/deblock.rs:532 DeblockGeom). */
+typedef struct {
+  int mi_cols, mi_rows;     /* luma mi grid */
+  int true_w, true_h;       /* coded (unpadded) luma frame size */
+  const uint32_t *block_id; /* covering block id per mi */
+  const uint8_t *bw, *bh;   /* covering block dims (px) */
+  const uint8_t *tw, *th;   /* luma tx dims (px) */
+  const uint8_t *skip;      /* inter skip per mi, may be NULL (all-intra) */
+} or_deblock_geom;
+void or_lf_thresholds(int level, int sharpness, int *mblim, int *lim, int *hev);
+void or_lpf(void *buf, int hbd, int bd, long off, int pitch, int vertical, int len, int level, int sharpness);
+void or_deblock_plane(void *buf, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
+                      int level_h, int sharpness, const or_deblock_geom *g);
+int or_cdef_constrain(int diff, int threshold, int damping);
+int or_cdef_adjust_strength(int strength, int var);
+int or_cdef_find_dir(const uint16_t *img, int stride, int coeff_shift, int *var_out);
+void or_cdef_filter_block(int32_t *dst, int dstride, const uint16_t *in, int pri, int sec, int dir, int pdamp,
+                          int sdamp, int bw, int bh, int coeff_shift, int sub);
+void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, const void *src_y,
+                       const void *src_u, const void *src_v, int width, int height, int bd, int qindex,
+                       int subsampling, int fbr, int fbc, const int *fs, int n_fs, uint64_t *mse_y,
+                       uint64_t *mse_uv);
+
+/* ---- loop restoration statistics (restoration.c) ------------------------ */
+int or_count_units(int unit_size, int size);
+void or_unit_limits(int unit_size, int pw, int ph, int ss_y, int ux, int uy, int *hs, int *he, int *vs, int *ve);
+void or_wiener_stats(int win, const void *dgd, const void *src, int pw, int ph, int bd, int hs, int he, int vs,
+                     int ve, int64_t *M, int64_t *H);
+uint64_t or_sse_region(const void *a, const void *b, int pw, int bd, int hs, int he, int vs, int ve);
+int or_sgr_param(int set, int k);
+void or_sgr_unit_stats(const void *dgd, const void *src, int pw, int ph, int bd, int hs, int he, int vs, int ve,
+                       int set, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif
