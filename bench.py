@@ -42,7 +42,8 @@ def _load(name, path):
 
 
 def candidate_ops(ts_mod, tx_size, tx_type, opsd):
-    """Algorithmic integer ops of one fused candidate."""
+    """Algorithmic integer ops of one fused candidate, from the straight-line
+    1D transform op counts the generator records."""
     w, h = ts_mod.TX_DIMS[tx_size]
     col = [0, 1, 0, 1, 2, 0, 2, 1, 2, 3, 0, 3, 1, 3, 2, 3][tx_type]
     row = [0, 0, 1, 1, 0, 2, 2, 2, 1, 3, 3, 0, 3, 1, 3, 2][tx_type]

@@ -9,8 +9,7 @@
  *     that its encoder calls through archmage tier dispatch.  Pointers are
  *     HOST pointers (caller-owned slices, never retained); each call copies
  *     its inputs to HBM, runs one launch on the calling thread's stream and
- *     copies the result back.  This is what a drop-in FFI crate binds
- *     (see INTEGRATION.md for the Rust binding).
+ *     copies the result back.  This is what a drop-in FFI crate binds.
  *
  *  2. The BATCHED form: zav1_*_batch() entry points taking DEVICE pointers to
  *     many jobs at once, ordered on a caller-supplied HIP stream.  The loop
@@ -218,6 +217,79 @@ typedef struct zav1_cfl_job {
 } zav1_cfl_job;
 int zav1_cfl_batch(const void *d_luma, int stride, int bd, const zav1_cfl_job *d_jobs, int n_jobs,
                    const uint16_t *d_dc, uint16_t *d_out, zav1_stream_t s);
+
+/* ------------------------------------------------------------------------ */
+/* Loop-filter stage.  Planes are tight (stride = plane width) u8 at bd 8,   */
+/* u16 at bd 10; chroma is 4:2:0.                                            */
+/* ------------------------------------------------------------------------ */
+/* Per-4x4 (mi) luma-grid geometry, all pointers device memory, row-major
+ * mi_rows x mi_cols (the reference's DeblockGeom,
+ * rust/crates/svtav1-encoder/src/deblock.rs:532).  Transforms and blocks
+ * must be aligned to their own size (as AV1 partitions are). */
+typedef struct zav1_deblock_geom {
+  int32_t mi_cols, mi_rows;   /* luma mi grid */
+  int32_t true_w, true_h;     /* coded (unpadded) luma frame size */
+  const uint32_t *block_id;   /* covering block id per mi */
+  const uint8_t *bw, *bh;     /* covering block dims (px) */
+  const uint8_t *tw, *th;     /* luma transform dims (px) */
+  const uint8_t *skip;        /* inter skip per mi; NULL = all-intra */
+} zav1_deblock_geom;
+
+/* In-place deblocking of one plane (plane 0 luma, 1/2 chroma with ss = 1):
+ * every vertical edge, then every horizontal edge, at the given levels.
+ * Replaces filter_plane / filter_plane_hbd
+ * (rust/crates/svtav1-encoder/src/deblock.rs:811 / :875) with lpf_params
+ * (:698) and the lpf_{vertical,horizontal}_{4,6,8,14} filters
+ * (rust/crates/svtav1-dsp/src/loop_filter.rs:50-405, hbd.rs:1040-1433).
+ * geom is a host struct holding device pointers.  Luma plane dimensions must
+ * be multiples of 8 and chroma multiples of 4 (the padded frame), so every
+ * filter tap stays inside the plane; else -EINVAL. */
+int zav1_deblock_plane(void *d_plane, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
+                       int level_h, int sharpness, const zav1_deblock_geom *geom, zav1_stream_t s);
+
+/* CDEF strength search of a whole frame: for every 64x64 filter block
+ * (raster, ceil(w/64) x ceil(h/64)) and every strength index g = 4 * pri +
+ * sec (pri 0..15, sec 0..3 -> 0,1,2,4), the SSE of the filtered luma
+ * (d_mse_y[fb*64 + g]) and of U + V (d_mse_uv, may be NULL) against the
+ * source, with the reference's per-plane shift and subsampling scale.
+ * d_dirvar[(fb*64 + by*8 + bx)*2 + {0,1}] receives each 8x8's direction and
+ * variance (required when d_mse_uv is given, else may be NULL).  Damping is
+ * 3 + (qindex >> 6); rows are sampled every `subsampling` (luma, at most 4)
+ * / min(subsampling, 1) (chroma).  Replaces cdef_search_still /
+ * cdef_search_still_hbd (rust/crates/svtav1-encoder/src/cdef.rs:1166 /
+ * :1374) over every candidate, with cdef_find_dir and cdef_filter_block
+ * (rust/crates/svtav1-dsp/src/cdef.rs:143 / :931). */
+int zav1_cdef_search(const void *d_rec_y, const void *d_rec_u, const void *d_rec_v, const void *d_src_y,
+                     const void *d_src_u, const void *d_src_v, int width, int height, int bd, int qindex,
+                     int subsampling, int32_t *d_dirvar, uint64_t *d_mse_y, uint64_t *d_mse_uv, zav1_stream_t s);
+
+/* Restoration-unit grid of a plane (count_units_in_tile,
+ * rust/crates/svtav1-dsp/src/restoration.rs:1388).  Host only. */
+int zav1_rest_unit_grid(int unit_size, int plane_w, int plane_h, int *n_h, int *n_v);
+
+/* Wiener statistics of every restoration unit (raster order of the grid
+ * above; unit limits of foreach_rest_unit_in_tile, restoration.rs:1395,
+ * with the 8 >> ss_y row offset), d_M[u*win^2 + k], d_H[u*win^4 + k*win^2
+ * + l]; d_dgd is read with edge replication beyond the plane.  win 7
+ * (luma) or 5 (chroma); unit_size <= 256.  Replaces compute_stats /
+ * compute_stats_hbd (restoration.rs:232 / :1738) with find_average(_hbd)
+ * (:198 / :1707). */
+int zav1_wiener_stats(const void *d_dgd, const void *d_src, int plane_w, int plane_h, int bd, int win, int unit_size,
+                      int ss_y, int64_t *d_M, int64_t *d_H, zav1_stream_t s);
+
+/* SSE of two planes over every restoration unit (d_out[u]).  Replaces
+ * sse_region / sse_region_hbd (restoration.rs:1666 / :1991). */
+int zav1_unit_sse(const void *d_a, const void *d_b, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
+                  uint64_t *d_out, zav1_stream_t s);
+
+/* Self-guided projection statistics of every restoration unit and each of
+ * the 16 AV1 parameter sets: d_out[(u*16 + set)*5 + {0..4}] = H00, H01,
+ * H11, C0, C1 of the projection normal equations, each summed over the unit
+ * and divided by its area.  No counterpart in the reference (its encoder
+ * never searches self-guided restoration); offered for an encoder that
+ * adds that search. */
+int zav1_sgr_stats(const void *d_dgd, const void *d_src, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
+                   int64_t *d_out, zav1_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -2,8 +2,8 @@
  * Oracle: loop-restoration statistics (TEST INFRASTRUCTURE ONLY — see
  * zav1_oracle.h).
  *
- *   find_average(_hbd)        dsp/restoration.rs:198 / :1704
- *   compute_stats(_hbd)       dsp/restoration.rs:567 (bd8) / :1723 (hbd: i64
+ *   find_average(_hbd)        dsp/restoration.rs:198 / :1707
+ *   compute_stats(_hbd)       dsp/restoration.rs:232 + scalar core :567 (bd8) / :1738 (hbd: i64
  *                             products, / 4 at bd10 after accumulation)
  *   unit iteration            dsp/restoration.rs:1388 count_units_in_tile,
  *                             :1395 foreach_rest_unit_in_tile

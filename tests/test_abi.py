@@ -47,3 +47,23 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     assert L.zav1_build_quant_row(300, 8, ctypes.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
     assert L.zav1_build_quant_row(10, 12, ctypes.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
     assert L.zav1_fwd_txfm2d_batch(None, None, None, 25, 1, None) == zav1.ZAV1_EINVAL
+    # loop-filter entry points validate before touching the device
+    g = zav1.DeblockGeom()
+    assert L.zav1_deblock_plane(None, 64, 64, 64, 8, 0, 0, 10, 10, 0, ctypes.byref(g), None) == zav1.ZAV1_EINVAL
+    fake = ctypes.c_void_p(1)
+    assert L.zav1_deblock_plane(fake, 68, 68, 64, 8, 0, 0, 10, 10, 0, ctypes.byref(g), None) == zav1.ZAV1_EINVAL
+    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 0, None, fake, None,
+                              None) == zav1.ZAV1_EINVAL
+    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 1, None, fake, fake,
+                              None) == zav1.ZAV1_EINVAL
+    assert L.zav1_wiener_stats(fake, fake, 64, 64, 8, 6, 64, 0, fake, fake, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_wiener_stats(fake, fake, 64, 64, 8, 7, 512, 0, fake, fake, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_sgr_stats(fake, fake, 64, 64, 12, 64, 0, fake, None) == zav1.ZAV1_EINVAL
+
+
+def test_rest_unit_grid_matches_oracle(zav1, oracle):
+    L = oracle.lrlib()
+    for us in (32, 64, 128, 256):
+        for size in (1, 17, 63, 64, 95, 96, 97, 200, 383, 384, 385, 4096):
+            nh, nv = zav1.rest_unit_grid(us, size, 4096 - size + 1)
+            assert (nh, nv) == (L.or_count_units(us, size), L.or_count_units(us, 4096 - size + 1))

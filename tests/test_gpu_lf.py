@@ -1,0 +1,180 @@
+"""GPU parity of the loop-filter stage against the oracle, bit-exact:
+deblocking (all planes, partial frames, skip maps, levels, sharpness),
+the CDEF direction + 64-strength SSE tables per filter block (luma and joint
+chroma, row subsampling, partial blocks), Wiener M/H, unit SSE and
+self-guided projection statistics of every restoration unit; and the
+whole stage at the 4096x4096 8-bit workload size with sampled checks."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _mod(name):
+    path = os.path.join(ROOT, 'zenav1-svt_amd', name + '.py')
+    spec = importlib.util.spec_from_file_location('zav1_' + name, path)
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+synth = _mod('synth')
+
+
+def blocky(h, w, bd, seed, blk=8):
+    """Noise with flattened blocks and block-edge steps (deblock/CDEF food)."""
+    rng = np.random.default_rng(seed)
+    f = synth.spectral_noise(h, w, 1.0, seed)
+    mx = (1 << bd) - 1
+    src = np.clip(np.round(f * mx * 0.8 + mx * 0.1), 0, mx).astype(np.int32)
+    rec = src.copy()
+    for y in range(0, h - blk + 1, blk):
+        for x in range(0, w - blk + 1, blk):
+            if rng.uniform() < 0.4:
+                rec[y:y + blk, x:x + blk] = int(rec[y:y + blk, x:x + blk].mean())
+    rec = rec + rng.integers(-3 << (bd - 8), (3 << (bd - 8)) + 1, rec.shape)
+    dt = np.uint8 if bd == 8 else np.uint16
+    return src.astype(dt), np.clip(rec, 0, mx).astype(dt)
+
+
+@pytest.mark.parametrize('bd,skip,sharp', [(8, 0.0, 0), (10, 0.0, 0), (8, 0.4, 3), (10, 0.4, 5)])
+def test_deblock_planes_bitexact(gpu, oracle, bd, skip, sharp):
+    z = gpu
+    H, W = 136, 200
+    mi_rows, mi_cols = H // 4, W // 4
+    true_w, true_h = W - 2, H - 2
+    geom = synth.partition_geometry(mi_rows, mi_cols, 4000 + bd, skip_frac=skip)
+    gd = {k: z.DeviceArray.from_host(v) for k, v in geom.items()}
+    g = z.deblock_geom(mi_cols, mi_rows, true_w, true_h, gd)
+    for plane, (ph, pw, ss) in enumerate([(H, W, 0), (H // 2, W // 2, 1), (H // 2, W // 2, 1)]):
+        _, rec = blocky(ph, pw, bd, 50 + plane, blk=8 if plane == 0 else 4)
+        for lv, lh in [(40, 24), (63, 63), (0, 17), (9, 0)]:
+            d = z.DeviceArray.from_host(rec)
+            z.deblock_plane(d, pw, pw, ph, bd, plane, ss, lv, lh, sharp, g)
+            got = d.to_host()
+            ref = oracle.deblock_plane(rec, bd, plane, ss, lv, lh, sharp, geom, mi_cols, mi_rows, true_w, true_h)
+            assert not np.array_equal(ref, rec) or (lv == 0 and lh == 0)
+            if not np.array_equal(got, ref):
+                bad = np.argwhere(got != ref)[:4]
+                pytest.fail('plane %d levels %d/%d: %d mismatches, first %s' % (
+                    plane, lv, lh, int((got != ref).sum()), bad.tolist()))
+
+
+@pytest.mark.parametrize('bd,sub,qindex', [(8, 1, 100), (10, 1, 200), (8, 2, 40), (10, 4, 255)])
+def test_cdef_search_bitexact(gpu, oracle, bd, sub, qindex):
+    z = gpu
+    H, W = 136, 200  # 4 x 3 filter blocks, the last row / column partial
+    sy, ry = blocky(H, W, bd, 70, 8)
+    su, ru = blocky(H // 2, W // 2, bd, 71, 4)
+    sv, rv = blocky(H // 2, W // 2, bd, 72, 4)
+    D = z.DeviceArray
+    rec = [D.from_host(a) for a in (ry, ru, rv)]
+    src = [D.from_host(a) for a in (sy, su, sv)]
+    nhfb, nvfb = (W + 63) // 64, (H + 63) // 64
+    dv = D((nhfb * nvfb, 64, 2), np.int32)
+    my = D((nhfb * nvfb, 64), np.uint64)
+    muv = D((nhfb * nvfb, 64), np.uint64)
+    z.cdef_search(rec, src, W, H, bd, qindex, sub, dv, my, muv)
+    gy, guv, gdv = my.to_host(), muv.to_host(), dv.to_host()
+    fs = np.arange(64)
+    for fbr in range(nvfb):
+        for fbc in range(nhfb):
+            fb = fbr * nhfb + fbc
+            ey, euv = oracle.cdef_search_fb((ry, ru, rv), (sy, su, sv), W, H, bd, qindex, sub, fbr, fbc, fs)
+            assert np.array_equal(gy[fb], ey), (fb, np.nonzero(gy[fb] != ey)[0][:8], gy[fb][:4], ey[:4])
+            assert np.array_equal(guv[fb], euv), (fb, np.nonzero(guv[fb] != euv)[0][:8])
+            vs, hs = min(64, H - fbr * 64), min(64, W - fbc * 64)
+            for by in range(vs // 8):
+                for bx in range(hs // 8):
+                    blk = ry[fbr * 64 + by * 8:fbr * 64 + by * 8 + 8, fbc * 64 + bx * 8:fbc * 64 + bx * 8 + 8]
+                    d, v = oracle.cdef_find_dir(blk, bd - 8)
+                    assert tuple(gdv[fb, by * 8 + bx]) == (d, v), (fb, by, bx)
+    # the table is not degenerate: some strength beats no filtering
+    assert (gy.min(axis=1) < gy[:, 0]).any()
+
+
+def _units(oracle, us, pw, ph, ss):
+    nh = oracle.lrlib().or_count_units(us, pw)
+    nv = oracle.lrlib().or_count_units(us, ph)
+    return nh, nv, [oracle.unit_limits(us, pw, ph, ss, ux, uy) for uy in range(nv) for ux in range(nh)]
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+@pytest.mark.parametrize('win,us,ss,dims', [(7, 64, 0, (200, 300)), (5, 32, 1, (100, 150)), (7, 128, 0, (136, 200))])
+def test_wiener_stats_and_unit_sse_bitexact(gpu, oracle, bd, win, us, ss, dims):
+    z = gpu
+    ph, pw = dims
+    src, rec = blocky(ph, pw, bd, 90 + win, 8)
+    nh, nv, lims = _units(oracle, us, pw, ph, ss)
+    assert z.rest_unit_grid(us, pw, ph) == (nh, nv)
+    D = z.DeviceArray
+    dd, ds = D.from_host(rec), D.from_host(src)
+    w2 = win * win
+    M, Hm, sse = D((nh * nv, w2), np.int64), D((nh * nv, w2, w2), np.int64), D((nh * nv,), np.uint64)
+    z.wiener_stats(dd, ds, pw, ph, bd, win, us, ss, M, Hm)
+    z.unit_sse(dd, ds, pw, ph, bd, us, ss, sse)
+    gm, gh, gs = M.to_host(), Hm.to_host(), sse.to_host()
+    L = oracle.lrlib()
+    for u, lim in enumerate(lims):
+        em, eh = oracle.wiener_stats(win, rec, src, bd, lim)
+        assert np.array_equal(gm[u], em), (u, lim)
+        assert np.array_equal(gh[u], eh), (u, lim, np.argwhere(gh[u] != eh)[:4].tolist())
+        es = L.or_sse_region(rec.ctypes.data, src.ctypes.data, pw, bd, *lim)
+        assert int(gs[u]) == es, (u, lim)
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_sgr_stats_bitexact(gpu, oracle, bd):
+    z = gpu
+    ph, pw, us = 100, 160, 64
+    src, rec = blocky(ph, pw, bd, 120 + bd, 8)
+    nh, nv, lims = _units(oracle, us, pw, ph, 0)
+    D = z.DeviceArray
+    out = D((nh * nv, 16, 5), np.int64)
+    z.sgr_stats(D.from_host(rec), D.from_host(src), pw, ph, bd, us, 0, out)
+    got = out.to_host()
+    for u, lim in enumerate(lims):
+        for s in range(16):
+            ref = oracle.sgr_unit_stats(rec, src, bd, lim, s)
+            assert np.array_equal(got[u, s], ref), (u, s, got[u, s], ref)
+
+
+def test_loop_filter_stage_full_size_sampled(gpu, oracle):
+    """The 4096x4096 8-bit workload end to end; deblocked planes checked in
+    full, the CDEF / restoration tables on sampled blocks and units."""
+    z = gpu
+    lfm = _mod('loopfilter')
+    src_y, rec_y = synth.config3_recon(4096, 4096, seed=3000)
+    (su, ru), (sv, rv) = synth.config3_chroma(2048, 2048, seed=3000)
+    geom = synth.partition_geometry(1024, 1024, 3001)
+    qindex = synth.qp_to_qindex(32)
+    levels = synth.key_frame_lf_levels(int(z.quant_row(qindex, 8).dequant[1]), 8)
+    st = lfm.LoopFilterStage(z, {'src': (src_y, su, sv), 'rec': (rec_y, ru, rv)}, geom, 8, qindex, levels)
+    st.run()
+    z.sync()
+    dy = [d.to_host() for d in st.dgd]
+    ref = [oracle.deblock_plane(rec_y, 8, 0, 0, levels[0], levels[1], 0, geom, 1024, 1024, 4096, 4096),
+           oracle.deblock_plane(ru, 8, 1, 1, levels[2], levels[2], 0, geom, 1024, 1024, 4096, 4096),
+           oracle.deblock_plane(rv, 8, 2, 1, levels[3], levels[3], 0, geom, 1024, 1024, 4096, 4096)]
+    for p in range(3):
+        assert np.array_equal(dy[p], ref[p]), p
+    gy, guv = st.mse_y.to_host(), st.mse_uv.to_host()
+    rng = np.random.default_rng(7)
+    for fb in list(rng.integers(0, 4096, 6)) + [0, 4095]:
+        fbr, fbc = divmod(int(fb), 64)
+        ey, euv = oracle.cdef_search_fb(tuple(ref), (src_y, su, sv), 4096, 4096, 8, qindex, 1, fbr, fbc,
+                                        np.arange(64))
+        assert np.array_equal(gy[fb], ey) and np.array_equal(guv[fb], euv), fb
+    for p, (pw, ss, srcp) in enumerate([(4096, 0, src_y), (2048, 1, su), (2048, 1, sv)]):
+        nh, nv = st.grid[p]
+        gm, gh, gsgr = st.M[p].to_host(), st.H[p].to_host(), st.sgrs[p].to_host()
+        for u in [0, nh * nv - 1, int(rng.integers(0, nh * nv))]:
+            lim = oracle.unit_limits(st.ru[p], pw, pw, ss, u % nh, u // nh)
+            em, eh = oracle.wiener_stats(st.win[p], dy[p], srcp, 8, lim)
+            assert np.array_equal(gm[u], em) and np.array_equal(gh[u], eh), (p, u)
+            for s in (0, 10, 14):
+                assert np.array_equal(gsgr[u, s], oracle.sgr_unit_stats(dy[p], srcp, 8, lim, s)), (p, u, s)

@@ -102,6 +102,12 @@ EXPORTS = {
     'zav1_txsearch_batch': [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp],
     'zav1_intra_pred_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
     'zav1_cfl_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
+    'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
+    'zav1_cdef_search': [_vp] * 6 + [_i32] * 5 + [_vp, _vp, _vp, _vp],
+    'zav1_rest_unit_grid': [_i32, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
+    'zav1_wiener_stats': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _vp],
+    'zav1_unit_sse': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
+    'zav1_sgr_stats': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
 }
 
 # numpy layouts of the job structs (match include/zav1_dsp.h)
@@ -259,3 +265,48 @@ def intra_pred_batch(d_plane, stride, bd, d_jobs, n_jobs, d_out, stream=None):
 
 def cfl_batch(d_luma, stride, bd, d_jobs, n_jobs, d_dc, d_out, stream=None):
     check(lib().zav1_cfl_batch(d_luma.ptr, stride, bd, d_jobs.ptr, n_jobs, d_dc.ptr, d_out.ptr, stream), 'cfl_batch')
+
+
+class DeblockGeom(ctypes.Structure):
+    """zav1_deblock_geom: per-mi geometry, device pointers."""
+    _fields_ = [('mi_cols', _i32), ('mi_rows', _i32), ('true_w', _i32), ('true_h', _i32), ('block_id', _vp),
+                ('bw', _vp), ('bh', _vp), ('tw', _vp), ('th', _vp), ('skip', _vp)]
+
+
+def deblock_geom(mi_cols, mi_rows, true_w, true_h, d):
+    """d: dict of DeviceArrays block_id (u32), bw, bh, tw, th, optional skip (u8)."""
+    return DeblockGeom(mi_cols, mi_rows, true_w, true_h, d['block_id'].ptr, d['bw'].ptr, d['bh'].ptr, d['tw'].ptr,
+                       d['th'].ptr, d['skip'].ptr if 'skip' in d else None)
+
+
+def deblock_plane(d_plane, stride, plane_w, plane_h, bd, plane, ss, level_v, level_h, sharpness, geom, stream=None):
+    check(lib().zav1_deblock_plane(d_plane.ptr, stride, plane_w, plane_h, bd, plane, ss, level_v, level_h, sharpness,
+                                   ctypes.byref(geom), stream), 'deblock_plane')
+
+
+def cdef_search(rec, src, width, height, bd, qindex, subsampling, d_dirvar, d_mse_y, d_mse_uv, stream=None):
+    """rec/src: (y, u, v) DeviceArrays (u/v may be None)."""
+    ptr = lambda a: a.ptr if a is not None else None  # noqa: E731
+    check(lib().zav1_cdef_search(ptr(rec[0]), ptr(rec[1]), ptr(rec[2]), ptr(src[0]), ptr(src[1]), ptr(src[2]), width,
+                                 height, bd, qindex, subsampling, ptr(d_dirvar), d_mse_y.ptr, ptr(d_mse_uv), stream),
+          'cdef_search')
+
+
+def rest_unit_grid(unit_size, plane_w, plane_h):
+    nh, nv = _i32(), _i32()
+    check(lib().zav1_rest_unit_grid(unit_size, plane_w, plane_h, ctypes.byref(nh), ctypes.byref(nv)), 'rest_unit_grid')
+    return nh.value, nv.value
+
+
+def wiener_stats(d_dgd, d_src, plane_w, plane_h, bd, win, unit_size, ss_y, d_M, d_H, stream=None):
+    check(lib().zav1_wiener_stats(d_dgd.ptr, d_src.ptr, plane_w, plane_h, bd, win, unit_size, ss_y, d_M.ptr, d_H.ptr,
+                                  stream), 'wiener_stats')
+
+
+def unit_sse(d_a, d_b, plane_w, plane_h, bd, unit_size, ss_y, d_out, stream=None):
+    check(lib().zav1_unit_sse(d_a.ptr, d_b.ptr, plane_w, plane_h, bd, unit_size, ss_y, d_out.ptr, stream), 'unit_sse')
+
+
+def sgr_stats(d_dgd, d_src, plane_w, plane_h, bd, unit_size, ss_y, d_out, stream=None):
+    check(lib().zav1_sgr_stats(d_dgd.ptr, d_src.ptr, plane_w, plane_h, bd, unit_size, ss_y, d_out.ptr, stream),
+          'sgr_stats')

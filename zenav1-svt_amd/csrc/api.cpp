@@ -506,4 +506,79 @@ int zav1_cfl_batch(const void* d_luma, int stride, int bd, const zav1_cfl_job* d
   return ZAV1_OK;
 }
 
+int zav1_deblock_plane(void* d_plane, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
+                       int level_h, int sharpness, const zav1_deblock_geom* geom, zav1_stream_t s) {
+  if ((bd != 8 && bd != 10) || plane < 0 || plane > 2 || ss < 0 || ss > 1 || (plane == 0 && ss != 0) ||
+      plane_w < 0 || plane_h < 0 || stride < plane_w || level_v < 0 || level_v > 63 || level_h < 0 ||
+      level_h > 63 || sharpness < 0 || sharpness > 7 || !geom || !d_plane)
+    return ZAV1_EINVAL;
+  // A 14-tap edge reaches 6 samples past a 16-aligned coordinate: luma planes
+  // padded to 8 samples (chroma to 4) keep every filter inside the plane.
+  const int align = plane == 0 ? 8 : 4;
+  if (plane_w % align || plane_h % align) return ZAV1_EINVAL;
+  if (!geom->block_id || !geom->bw || !geom->bh || !geom->tw || !geom->th || geom->mi_cols <= 0 ||
+      geom->mi_rows <= 0 || ((plane_w << ss) + 3) / 4 > geom->mi_cols || ((plane_h << ss) + 3) / 4 > geom->mi_rows)
+    return ZAV1_EINVAL;
+  zav1::DeblockGeomDev g{geom->mi_cols, geom->mi_rows, geom->true_w, geom->true_h, geom->block_id,
+                         geom->bw,      geom->bh,      geom->tw,     geom->th,     geom->skip};
+  ZCHECK(zav1::deblock_plane(d_plane, stride, plane_w, plane_h, bd, plane, ss, level_v, level_h, sharpness, g, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_cdef_search(const void* d_rec_y, const void* d_rec_u, const void* d_rec_v, const void* d_src_y,
+                     const void* d_src_u, const void* d_src_v, int width, int height, int bd, int qindex,
+                     int subsampling, int32_t* d_dirvar, uint64_t* d_mse_y, uint64_t* d_mse_uv, zav1_stream_t s) {
+  if ((bd != 8 && bd != 10) || width <= 0 || height <= 0 || qindex < 0 || qindex > 255 || subsampling < 1 ||
+      !d_rec_y || !d_src_y || !d_mse_y)
+    return ZAV1_EINVAL;
+  if (d_mse_uv && (!d_rec_u || !d_rec_v || !d_src_u || !d_src_v || !d_dirvar)) return ZAV1_EINVAL;
+  const int damping = 3 + (qindex >> 6);
+  ZCHECK(zav1::cdef_search(d_rec_y, d_rec_u, d_rec_v, d_src_y, d_src_u, d_src_v, width, height, bd, damping,
+                           subsampling < 4 ? subsampling : 4, 1, d_dirvar, d_mse_y, d_mse_uv, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_rest_unit_grid(int unit_size, int plane_w, int plane_h, int* n_h, int* n_v) {
+  if (unit_size < 8 || unit_size > 256 || (unit_size & (unit_size - 1)) || plane_w <= 0 || plane_h <= 0 || !n_h ||
+      !n_v)
+    return ZAV1_EINVAL;
+  auto count = [&](int size) {
+    const int n = (size + (unit_size >> 1)) / unit_size;
+    return n < 1 ? 1 : n;
+  };
+  *n_h = count(plane_w);
+  *n_v = count(plane_h);
+  return ZAV1_OK;
+}
+
+int zav1_wiener_stats(const void* d_dgd, const void* d_src, int plane_w, int plane_h, int bd, int win, int unit_size,
+                      int ss_y, int64_t* d_M, int64_t* d_H, zav1_stream_t s) {
+  int nh = 0, nv = 0;
+  if ((bd != 8 && bd != 10) || (win != 5 && win != 7) || ss_y < 0 || ss_y > 1 || !d_dgd || !d_src || !d_M || !d_H ||
+      zav1_rest_unit_grid(unit_size, plane_w, plane_h, &nh, &nv) != ZAV1_OK)
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::wiener_stats(d_dgd, d_src, plane_w, plane_h, bd, win, unit_size, ss_y, nh, nv, d_M, d_H, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_unit_sse(const void* d_a, const void* d_b, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
+                  uint64_t* d_out, zav1_stream_t s) {
+  int nh = 0, nv = 0;
+  if ((bd != 8 && bd != 10) || ss_y < 0 || ss_y > 1 || !d_a || !d_b || !d_out ||
+      zav1_rest_unit_grid(unit_size, plane_w, plane_h, &nh, &nv) != ZAV1_OK)
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::unit_sse(d_a, d_b, plane_w, plane_h, bd, unit_size, ss_y, nh, nv, d_out, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_sgr_stats(const void* d_dgd, const void* d_src, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
+                   int64_t* d_out, zav1_stream_t s) {
+  int nh = 0, nv = 0;
+  if ((bd != 8 && bd != 10) || ss_y < 0 || ss_y > 1 || !d_dgd || !d_src || !d_out ||
+      zav1_rest_unit_grid(unit_size, plane_w, plane_h, &nh, &nv) != ZAV1_OK)
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::sgr_stats(d_dgd, d_src, plane_w, plane_h, bd, unit_size, ss_y, nh, nv, d_out, S(s)));
+  return ZAV1_OK;
+}
+
 }  // extern "C"

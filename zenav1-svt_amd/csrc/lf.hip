@@ -1,0 +1,908 @@
+// Loop-filter stage kernels for gfx950: deblocking, CDEF direction +
+// strength-SSE search, Wiener covariance and self-guided projection
+// statistics per restoration unit, and the per-unit SSE.
+//
+// Semantics:
+//   deblock   rust/crates/svtav1-dsp/src/loop_filter.rs:50-405, hbd.rs:1040-1433,
+//             rust/crates/svtav1-encoder/src/deblock.rs:698 lpf_params, :811 filter_plane.
+//             All vertical edges of a plane, then all horizontal edges.  An
+//             edge's filter reaches at most half of the smaller of the two
+//             transforms it separates, so no two edges of one pass touch the
+//             same sample and each pass runs fully in parallel.
+//   CDEF      rust/crates/svtav1-dsp/src/cdef.rs:113-1228,
+//             rust/crates/svtav1-encoder/src/cdef.rs:517,952,1045,1128,1166,1374
+//   Wiener    rust/crates/svtav1-dsp/src/restoration.rs:198,232,567,1388,1395,1707,1738
+//   unit SSE  rust/crates/svtav1-dsp/src/restoration.rs:1666,1991
+//   SGR       self-guided filter + projection normal equations (no reference
+//             twin; AV1 self-guided filter process, fast r = 2 variant)
+#include "zav1_dev.h"
+#include "zav1_kernels.h"
+
+namespace zav1 {
+
+namespace {
+__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// ---------------------------------------------------------------------------
+// deblocking
+// ---------------------------------------------------------------------------
+struct Thr {
+  int32_t lim, mblim, hev, shift, bd;
+};
+
+__device__ __forceinline__ int32_t sclamp(int32_t t, int bd) {
+  const int32_t m = 128 << (bd - 8);
+  return clampi(t, -m, m - 1);
+}
+
+__device__ __forceinline__ void filter4(bool mask, const Thr& t, int32_t* w) {
+  const int32_t bias = 0x80 << t.shift;
+  const int32_t ps1 = w[0] - bias, ps0 = w[1] - bias, qs0 = w[2] - bias, qs1 = w[3] - bias;
+  const int32_t th = t.hev << t.shift;
+  const int32_t hev = (abs(w[0] - w[1]) > th || abs(w[3] - w[2]) > th) ? -1 : 0;
+  int32_t f = sclamp(ps1 - qs1, t.bd) & hev;
+  f = sclamp(f + 3 * (qs0 - ps0), t.bd) & (mask ? -1 : 0);
+  const int32_t f1 = sclamp(f + 4, t.bd) >> 3, f2 = sclamp(f + 3, t.bd) >> 3;
+  w[2] = sclamp(qs0 - f1, t.bd) + bias;
+  w[1] = sclamp(ps0 + f2, t.bd) + bias;
+  const int32_t o = ((f1 + 1) >> 1) & ~hev;
+  w[3] = sclamp(qs1 - o, t.bd) + bias;
+  w[0] = sclamp(ps1 + o, t.bd) + bias;
+}
+
+// Filter mask over N samples (N = 4, 6, 8) centred on the edge.
+template <int N>
+__device__ __forceinline__ bool mask_n(const Thr& t, const int32_t* w) {
+  const int32_t l = t.lim << t.shift, bl = t.mblim << t.shift;
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k + 1 < N / 2; k++) bad |= abs(w[k] - w[k + 1]) > l;
+#pragma unroll
+  for (int k = N / 2; k + 1 < N; k++) bad |= abs(w[k] - w[k + 1]) > l;
+  bad |= abs(w[N / 2 - 1] - w[N / 2]) * 2 + abs(w[N / 2 - 2] - w[N / 2 + 1]) / 2 > bl;
+  return !bad;
+}
+
+// Flatness with p0 = w[c-1], q0 = w[c]: taps k = K0 .. K1-1 on each side.
+template <int K0, int K1>
+__device__ __forceinline__ bool flat_at(int shift, const int32_t* w, int c) {
+  const int32_t t = 1 << shift;
+  bool bad = false;
+#pragma unroll
+  for (int k = K0; k < K1; k++) bad |= abs(w[c - 1 - k] - w[c - 1]) > t || abs(w[c + k] - w[c]) > t;
+  return !bad;
+}
+
+#define R3(x) (((x) + 4) >> 3)
+#define R4(x) (((x) + 8) >> 4)
+
+template <int LEN>
+__device__ __forceinline__ void lpf_line(const Thr& t, int32_t* w) {
+  if constexpr (LEN == 4) {
+    filter4(mask_n<4>(t, w), t, w);
+  } else if constexpr (LEN == 6) {
+    const bool m = mask_n<6>(t, w);
+    if (m && flat_at<1, 3>(t.shift, w, 3)) {
+      const int32_t p2 = w[0], p1 = w[1], p0 = w[2], q0 = w[3], q1 = w[4], q2 = w[5];
+      w[1] = R3(p2 * 3 + p1 * 2 + p0 * 2 + q0);
+      w[2] = R3(p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1);
+      w[3] = R3(p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2);
+      w[4] = R3(p0 + q0 * 2 + q1 * 2 + q2 * 3);
+    } else {
+      filter4(m, t, w + 1);
+    }
+  } else if constexpr (LEN == 8) {
+    const bool m = mask_n<8>(t, w);
+    if (m && flat_at<1, 4>(t.shift, w, 4)) {
+      const int32_t p3 = w[0], p2 = w[1], p1 = w[2], p0 = w[3], q0 = w[4], q1 = w[5], q2 = w[6], q3 = w[7];
+      w[1] = R3(p3 + p3 + p3 + 2 * p2 + p1 + p0 + q0);
+      w[2] = R3(p3 + p3 + p2 + 2 * p1 + p0 + q0 + q1);
+      w[3] = R3(p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2);
+      w[4] = R3(p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3);
+      w[5] = R3(p1 + p0 + q0 + 2 * q1 + q2 + q3 + q3);
+      w[6] = R3(p0 + q0 + q1 + 2 * q2 + q3 + q3 + q3);
+    } else {
+      filter4(m, t, w + 2);
+    }
+  } else {
+    const bool m = mask_n<8>(t, w + 3);
+    const bool fl = flat_at<1, 4>(t.shift, w, 7);
+    const bool fl2 = flat_at<4, 7>(t.shift, w, 7);
+    if (fl2 && fl && m) {
+      const int32_t p6 = w[0], p5 = w[1], p4 = w[2], p3 = w[3], p2 = w[4], p1 = w[5], p0 = w[6];
+      const int32_t q0 = w[7], q1 = w[8], q2 = w[9], q3 = w[10], q4 = w[11], q5 = w[12], q6 = w[13];
+      w[1] = R4(p6 * 7 + p5 * 2 + p4 * 2 + p3 + p2 + p1 + p0 + q0);
+      w[2] = R4(p6 * 5 + p5 * 2 + p4 * 2 + p3 * 2 + p2 + p1 + p0 + q0 + q1);
+      w[3] = R4(p6 * 4 + p5 + p4 * 2 + p3 * 2 + p2 * 2 + p1 + p0 + q0 + q1 + q2);
+      w[4] = R4(p6 * 3 + p5 + p4 + p3 * 2 + p2 * 2 + p1 * 2 + p0 + q0 + q1 + q2 + q3);
+      w[5] = R4(p6 * 2 + p5 + p4 + p3 + p2 * 2 + p1 * 2 + p0 * 2 + q0 + q1 + q2 + q3 + q4);
+      w[6] = R4(p6 + p5 + p4 + p3 + p2 + p1 * 2 + p0 * 2 + q0 * 2 + q1 + q2 + q3 + q4 + q5);
+      w[7] = R4(p5 + p4 + p3 + p2 + p1 + p0 * 2 + q0 * 2 + q1 * 2 + q2 + q3 + q4 + q5 + q6);
+      w[8] = R4(p4 + p3 + p2 + p1 + p0 + q0 * 2 + q1 * 2 + q2 * 2 + q3 + q4 + q5 + q6 * 2);
+      w[9] = R4(p3 + p2 + p1 + p0 + q0 + q1 * 2 + q2 * 2 + q3 * 2 + q4 + q5 + q6 * 3);
+      w[10] = R4(p2 + p1 + p0 + q0 + q1 + q2 * 2 + q3 * 2 + q4 * 2 + q5 + q6 * 4);
+      w[11] = R4(p1 + p0 + q0 + q1 + q2 + q3 * 2 + q4 * 2 + q5 * 2 + q6 * 5);
+      w[12] = R4(p0 + q0 + q1 + q2 + q3 + q4 * 2 + q5 * 2 + q6 * 7);
+    } else if (fl && m) {
+      const int32_t p3 = w[3], p2 = w[4], p1 = w[5], p0 = w[6], q0 = w[7], q1 = w[8], q2 = w[9], q3 = w[10];
+      w[4] = R3(p3 + p3 + p3 + 2 * p2 + p1 + p0 + q0);
+      w[5] = R3(p3 + p3 + p2 + 2 * p1 + p0 + q0 + q1);
+      w[6] = R3(p3 + p2 + p1 + 2 * p0 + q0 + q1 + q2);
+      w[7] = R3(p2 + p1 + p0 + 2 * q0 + q1 + q2 + q3);
+      w[8] = R3(p1 + p0 + q0 + 2 * q1 + q2 + q3 + q3);
+      w[9] = R3(p0 + q0 + q1 + 2 * q2 + q3 + q3 + q3);
+    } else {
+      filter4(m, t, w + 5);
+    }
+  }
+}
+
+// lpf_params (deblock.rs:698): filter length of the edge at (x, y) of a
+// plane (x or y a multiple of 4 along the walked axis), 0 = no filtering.
+// The reference walks each mi row jumping from transform start to transform
+// start; with size-aligned transforms that visits exactly the positions whose
+// coordinate is a multiple of the covering transform's size, which is the
+// test applied here per position.
+__device__ __forceinline__ int edge_len(const DeblockGeomDev& g, bool vert, int x, int y, int plane, int ss,
+                                        int level) {
+  if (level == 0 || x >= (g.true_w >> ss) || y >= (g.true_h >> ss)) return 0;
+  const int mi_row = ss | ((y << ss) >> 2), mi_col = ss | ((x << ss) >> 2);
+  const long idx = (long)mi_row * g.mi_cols + mi_col;
+  const int ts = vert ? (plane == 0 ? (int)g.tw[idx] : max((int)g.bw[idx] >> 1, 4))
+                      : (plane == 0 ? (int)g.th[idx] : max((int)g.bh[idx] >> 1, 4));
+  const int coord = vert ? x : y;
+  if ((coord & (ts - 1)) || coord == 0) return 0;
+  const long prev = vert ? idx - (1 << ss) : idx - ((long)g.mi_cols << ss);
+  const int pv = vert ? (plane == 0 ? (int)g.tw[prev] : max((int)g.bw[prev] >> 1, 4))
+                      : (plane == 0 ? (int)g.th[prev] : max((int)g.bh[prev] >> 1, 4));
+  const int cs = g.skip ? g.skip[idx] : 0, ps = g.skip ? g.skip[prev] : 0;
+  const bool pu_edge = g.block_id[prev] != g.block_id[idx];
+  if (!(!ps || !cs || pu_edge)) return 0;
+  const int md = min(ts, pv);
+  return md == 4 ? 4 : (plane != 0 ? 6 : (md == 8 ? 8 : 14));
+}
+
+template <typename Pix, int LEN>
+__device__ __forceinline__ void filter_at(Pix* buf, long start, long step, const Thr& t) {
+  int32_t w[LEN];
+#pragma unroll
+  for (int k = 0; k < LEN; k++) w[k] = buf[start + k * step];
+  lpf_line<LEN>(t, w);
+#pragma unroll
+  for (int k = 0; k < LEN; k++) buf[start + k * step] = (Pix)w[k];
+}
+
+}  // namespace
+
+// One thread per (edge position, line).  VERT: consecutive threads take
+// consecutive mi columns of one pixel row; HORZ: consecutive pixel columns
+// of one mi row.
+template <typename Pix, bool VERT>
+__global__ __launch_bounds__(256) void k_deblock(Pix* __restrict__ buf, int stride, int pw, int ph, int plane,
+                                                 int ss, int level, Thr t, DeblockGeomDev g) {
+  const long tid = (long)blockIdx.x * 256 + threadIdx.x;
+  int x, y;
+  if (VERT) {
+    const int nm = pw >> 2, nl = ph & ~3;
+    if (tid >= (long)nm * nl) return;
+    y = (int)(tid / nm);
+    x = (int)(tid - (long)y * nm) * 4;
+  } else {
+    const int nm = ph >> 2, nl = pw & ~3;
+    if (tid >= (long)nm * nl) return;
+    const int my = (int)(tid / nl);
+    x = (int)(tid - (long)my * nl);
+    y = my * 4;
+  }
+  const int len = edge_len(g, VERT, VERT ? x : (x & ~3), VERT ? (y & ~3) : y, plane, ss, level);
+  if (len == 0) return;
+  const long step = VERT ? 1 : stride;
+  const long start = (long)y * stride + x - (len / 2) * step;
+  if (len == 4) filter_at<Pix, 4>(buf, start, step, t);
+  else if (len == 6) filter_at<Pix, 6>(buf, start, step, t);
+  else if (len == 8) filter_at<Pix, 8>(buf, start, step, t);
+  else filter_at<Pix, 14>(buf, start, step, t);
+}
+
+// ---------------------------------------------------------------------------
+// CDEF
+// ---------------------------------------------------------------------------
+constexpr int kVB = 3, kHB = 8;  // vertical / horizontal border of the padded block
+constexpr uint16_t kLarge = 0x7f7f;
+constexpr int kTS = 80;          // LDS tile stride (64 + 2 * kHB)
+
+// Tap offsets (k = 0, 1) of the 8 directions in the LDS tile, row dir + 2;
+// the two extra rows at each end let dir - 2 and dir + 2 index directly.
+__constant__ int kCdefDir[12][2] = {
+    {kTS, 2 * kTS},  {kTS, 2 * kTS - 1},     {-kTS + 1, -2 * kTS + 2}, {1, -kTS + 2},
+    {1, 2},          {1, kTS + 2},           {kTS + 1, 2 * kTS + 2},   {kTS, 2 * kTS + 1},
+    {kTS, 2 * kTS},  {kTS, 2 * kTS - 1},     {-kTS + 1, -2 * kTS + 2}, {1, -kTS + 2}};
+
+namespace {
+__device__ __forceinline__ int msb32(uint32_t n) { return 31 - __clz(n); }
+
+// constrain(diff, thr, damping) for thr > 0, with sh = max(0, damping - msb(thr)).
+__device__ __forceinline__ int32_t constrain(int32_t diff, int32_t thr, int32_t sh) {
+  const int32_t ad = abs(diff);
+  const int32_t v = min(ad, max(0, thr - (ad >> sh)));
+  return diff < 0 ? -v : v;
+}
+
+__device__ __forceinline__ int adjust_strength(int s, int var) {
+  const int i = (var >> 6) ? min(msb32((uint32_t)(var >> 6)), 12) : 0;
+  return var ? (s * (4 + i) + 8) >> 4 : 0;
+}
+
+// cdef_find_dir (cdef.rs:143) on the 8x8 at t (LDS tile, stride kTS).
+__device__ int find_dir(const uint16_t* t, int cs, int* var_out) {
+  constexpr int DIV[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
+  int32_t partial[8][15];
+#pragma unroll
+  for (int a = 0; a < 8; a++)
+#pragma unroll
+    for (int b = 0; b < 15; b++) partial[a][b] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int x = (t[i * kTS + j] >> cs) - 128;
+      partial[0][i + j] += x;
+      partial[1][i + j / 2] += x;
+      partial[2][i] += x;
+      partial[3][3 + i - j / 2] += x;
+      partial[4][7 + i - j] += x;
+      partial[5][3 - i / 2 + j] += x;
+      partial[6][j] += x;
+      partial[7][i / 2 + j] += x;
+    }
+  int32_t cost[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    cost[2] += partial[2][i] * partial[2][i];
+    cost[6] += partial[6][i] * partial[6][i];
+  }
+  cost[2] *= DIV[8];
+  cost[6] *= DIV[8];
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    cost[0] += (partial[0][i] * partial[0][i] + partial[0][14 - i] * partial[0][14 - i]) * DIV[i + 1];
+    cost[4] += (partial[4][i] * partial[4][i] + partial[4][14 - i] * partial[4][14 - i]) * DIV[i + 1];
+  }
+  cost[0] += partial[0][7] * partial[0][7] * DIV[8];
+  cost[4] += partial[4][7] * partial[4][7] * DIV[8];
+#pragma unroll
+  for (int i = 1; i < 8; i += 2) {
+#pragma unroll
+    for (int j = 0; j < 5; j++) cost[i] += partial[i][3 + j] * partial[i][3 + j];
+    cost[i] *= DIV[8];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      cost[i] += (partial[i][j] * partial[i][j] + partial[i][10 - j] * partial[i][10 - j]) * DIV[2 * j + 2];
+  }
+  int32_t best = 0;
+  int bd = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (cost[i] > best) { best = cost[i]; bd = i; }
+  *var_out = (best - cost[(bd + 4) & 7]) >> 10;
+  return bd;
+}
+
+// Neighbour differences of one pixel for one tap geometry (dir) and the
+// clamp window of the filter output.
+struct PixTaps {
+  int32_t d_p[4];  // primary taps minus x: (k0 +, k0 -, k1 +, k1 -)
+  int32_t d_s[8];  // secondary taps minus x: k0 (4), k1 (4)
+  int32_t mn, mx;
+};
+
+__device__ __forceinline__ void gather_taps(const uint16_t* c, int dir, PixTaps& pt) {
+  const int32_t x = (int16_t)c[0];
+  int32_t mn = x, mx = x;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int dk = kCdefDir[dir + 2][k];
+    const int32_t p0 = (int16_t)c[dk], p1 = (int16_t)c[-dk];
+    pt.d_p[2 * k] = p0 - x;
+    pt.d_p[2 * k + 1] = p1 - x;
+    if ((uint16_t)p0 != kLarge) mx = max(mx, p0);
+    if ((uint16_t)p1 != kLarge) mx = max(mx, p1);
+    mn = min(mn, min(p0, p1));
+    const int da = kCdefDir[dir + 4][k], db = kCdefDir[dir][k];
+    const int32_t s[4] = {(int16_t)c[da], (int16_t)c[-da], (int16_t)c[db], (int16_t)c[-db]};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      if ((uint16_t)s[m] != kLarge) mx = max(mx, s[m]);
+      mn = min(mn, s[m]);
+      pt.d_s[4 * k + m] = s[m] - x;
+    }
+  }
+  pt.mn = mn;
+  pt.mx = mx;
+}
+
+// Secondary-tap sums of one geometry for the 4 secondary strengths.
+__device__ __forceinline__ void sec_sums(const PixTaps& t, int cs, int damping, int32_t out[4]) {
+  out[0] = 0;
+#pragma unroll
+  for (int S = 1; S < 4; S++) {
+    const int32_t st = (S + (S == 3)) << cs;
+    const int32_t sh = max(0, damping - msb32((uint32_t)st));
+    int32_t s = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) s += (m < 4 ? 2 : 1) * constrain(t.d_s[m], st, sh);
+    out[S] = s;
+  }
+}
+
+// Primary-tap sum for primary strength tq (> 0) with tap parity par.
+__device__ __forceinline__ int32_t pri_sum(const PixTaps& t, int32_t tq, int par, int damping) {
+  const int32_t sh = max(0, damping - msb32((uint32_t)tq));
+  const int t0w = par ? 3 : 4, t1w = par ? 3 : 2;
+  return t0w * (constrain(t.d_p[0], tq, sh) + constrain(t.d_p[1], tq, sh)) +
+         t1w * (constrain(t.d_p[2], tq, sh) + constrain(t.d_p[3], tq, sh));
+}
+
+__device__ __forceinline__ int32_t cdef_out(int32_t x, int32_t sum, int32_t mn, int32_t mx) {
+  const int32_t s16 = (int16_t)sum;  // the reference accumulates in wrapping i16
+  return clampi(x + ((8 + s16 - (s16 < 0)) >> 4), mn, mx);
+}
+
+// Sum a per-lane 64-entry strength row over each wave into red[wave].
+__device__ __forceinline__ void reduce_rows(const uint32_t* acc, uint32_t (*red)[64]) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int g = 0; g < 64; g++) {
+    uint32_t v = acc[g];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (lane == 0) red[wave][g] = v;
+  }
+}
+
+// SSE of every (P, S) strength pair at one pixel, added to acc[P * 4 + S].
+// adj(P) gives the primary strength used for P (adjusted for luma).
+template <typename Adj>
+__device__ __forceinline__ void cdef_pixel(const uint16_t* cp, int dir, int32_t s_v, int cs, int dmp, Adj adj,
+                                           uint32_t* acc) {
+  const int32_t x = (int16_t)cp[0];
+  PixTaps tb, t0;
+  gather_taps(cp, dir, tb);
+  gather_taps(cp, 0, t0);
+  int32_t secb[4], sec0[4];
+  sec_sums(tb, cs, dmp, secb);
+  sec_sums(t0, cs, dmp, sec0);
+  // P = 0 filters along direction 0 with no primary taps; (0, 0) is the
+  // unfiltered block
+  const int32_t e0 = s_v - x;
+  acc[0] += (uint32_t)(e0 * e0);
+#pragma unroll
+  for (int S = 1; S < 4; S++) {
+    const int32_t e = s_v - cdef_out(x, sec0[S], t0.mn, t0.mx);
+    acc[S] += (uint32_t)(e * e);
+  }
+#pragma unroll
+  for (int P = 1; P < 16; P++) {
+    const int32_t tq = adj(P);
+    const int32_t pri = tq ? pri_sum(tb, tq, (tq >> cs) & 1, dmp) : 0;
+#pragma unroll
+    for (int S = 0; S < 4; S++) {
+      const int32_t e = s_v - cdef_out(x, pri + secb[S], tb.mn, tb.mx);
+      acc[P * 4 + S] += (uint32_t)(e * e);
+    }
+  }
+}
+
+}  // namespace
+
+// One workgroup per 64x64 luma filter block: the 8x8 directions, then all
+// 64 (primary, secondary) strength pairs for every pixel; per-pair SSE
+// against the source reduced over the block.  dirvar (optional) receives
+// (dir, var) per 8x8 for the chroma pass.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, const Pix* __restrict__ src,
+                                                   int width, int height, int bd, int damping, int sub, int nhfb,
+                                                   uint64_t* __restrict__ mse, int32_t* __restrict__ dirvar) {
+  __shared__ uint16_t tile[70 * kTS];
+  __shared__ int32_t sdir[64];
+  __shared__ uint8_t sadj[64][16];
+  __shared__ uint32_t red[4][64];
+  const int fbr = blockIdx.x / nhfb, fbc = blockIdx.x - fbr * nhfb;
+  const int cs = bd - 8;
+  const int y0 = fbr * 64, x0 = fbc * 64;
+  const int vsize = min(64, height - y0), hsize = min(64, width - x0);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 70 * kTS; i += 256) {
+    const int r = i / kTS, c = i - r * kTS;
+    const int gy = y0 + r - kVB, gx = x0 + c - kHB;
+    const bool in = (r < vsize + 2 * kVB) && (c < hsize + 2 * kHB) && gy >= 0 && gy < height && gx >= 0 &&
+                    gx < width;
+    tile[i] = in ? (uint16_t)rec[(long)gy * width + gx] : kLarge;
+  }
+  __syncthreads();
+  const int nbx = hsize >> 3, nby = vsize >> 3;
+  if (tid < 64) {
+    const int by = tid >> 3, bx = tid & 7;
+    int d = 0, v = 0;
+    if (by < nby && bx < nbx) d = find_dir(tile + (kVB + by * 8) * kTS + kHB + bx * 8, cs, &v);
+    sdir[tid] = d;
+    // adjusted primary strength of every P (at most 60 at 10 bits)
+    for (int P = 0; P < 16; P++) sadj[tid][P] = (uint8_t)adjust_strength(P << cs, v);
+    if (dirvar) {
+      dirvar[((long)blockIdx.x * 64 + tid) * 2] = d;
+      dirvar[((long)blockIdx.x * 64 + tid) * 2 + 1] = v;
+    }
+  }
+  __syncthreads();
+
+  const int dmp = damping + cs;
+  uint32_t acc[64];
+#pragma unroll
+  for (int g = 0; g < 64; g++) acc[g] = 0;
+  for (int p = tid; p < 4096; p += 256) {
+    const int r = p >> 6, c = p & 63;
+    const int by = r >> 3, bx = c >> 3;
+    if (by >= nby || bx >= nbx || ((r & 7) % sub) != 0) continue;
+    const int blk = by * 8 + bx;
+    const uint8_t* adj = sadj[blk];
+    cdef_pixel(tile + (kVB + r) * kTS + kHB + c, sdir[blk], (int32_t)src[(long)(y0 + r) * width + x0 + c], cs, dmp,
+               [adj](int P) { return (int32_t)adj[P]; }, acc);
+  }
+  reduce_rows(acc, red);
+  __syncthreads();
+  if (tid < 64) {
+    const uint64_t tot = (uint64_t)red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    mse[(long)blockIdx.x * 64 + tid] = (tot >> (2 * cs)) * (uint64_t)sub;
+  }
+}
+
+// Chroma (4:2:0): one workgroup per filter block for both chroma planes;
+// 4x4 blocks with the co-located luma direction, unadjusted primary
+// strength, damping - 1.  mse receives the U + V row (each plane's sum
+// shifted on its own, as the reference does).
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec_u, const Pix* __restrict__ rec_v,
+                                                     const Pix* __restrict__ src_u, const Pix* __restrict__ src_v,
+                                                     int width, int height, int bd, int damping, int sub, int nhfb,
+                                                     const int32_t* __restrict__ dirvar,
+                                                     uint64_t* __restrict__ mse) {
+  __shared__ uint16_t tile[38 * kTS];
+  __shared__ uint32_t red[4][64];
+  __shared__ uint64_t total[64];
+  const int fbr = blockIdx.x / nhfb, fbc = blockIdx.x - fbr * nhfb;
+  const int cs = bd - 8, cw = width >> 1, ch = height >> 1;
+  const int y0 = fbr * 32, x0 = fbc * 32;
+  const int vsize = min(32, ch - y0), hsize = min(32, cw - x0);
+  const int nr = vsize & ~3, nc = hsize & ~3;
+  const int tid = threadIdx.x;
+  const int dmp = damping + cs - 1;
+  if (tid < 64) total[tid] = 0;
+  for (int pl = 0; pl < 2; pl++) {
+    const Pix* rec = pl ? rec_v : rec_u;
+    const Pix* srcp = pl ? src_v : src_u;
+    __syncthreads();  // previous plane's tile and red consumed
+    for (int i = tid; i < 38 * kTS; i += 256) {
+      const int r = i / kTS, c = i - r * kTS;
+      const int gy = y0 + r - kVB, gx = x0 + c - kHB;
+      const bool in = (r < vsize + 2 * kVB) && (c < hsize + 2 * kHB) && gy >= 0 && gy < ch && gx >= 0 && gx < cw;
+      tile[i] = in ? (uint16_t)rec[(long)gy * cw + gx] : kLarge;
+    }
+    __syncthreads();
+    uint32_t acc[64];
+#pragma unroll
+    for (int g = 0; g < 64; g++) acc[g] = 0;
+    for (int p = tid; p < 1024; p += 256) {
+      const int r = p >> 5, c = p & 31;
+      if (r >= nr || c >= nc || ((r & 3) % sub) != 0) continue;
+      const int dir = dirvar[((long)blockIdx.x * 64 + (r >> 2) * 8 + (c >> 2)) * 2];
+      cdef_pixel(tile + (kVB + r) * kTS + kHB + c, dir, (int32_t)srcp[(long)(y0 + r) * cw + x0 + c], cs, dmp,
+                 [cs](int P) { return (int32_t)(P << cs); }, acc);
+    }
+    reduce_rows(acc, red);
+    __syncthreads();
+    if (tid < 64) {
+      const uint64_t tot = (uint64_t)red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+      total[tid] += (tot >> (2 * cs)) * (uint64_t)sub;
+    }
+  }
+  __syncthreads();
+  if (tid < 64) mse[(long)blockIdx.x * 64 + tid] = total[tid];
+}
+
+// ---------------------------------------------------------------------------
+// Restoration units
+// ---------------------------------------------------------------------------
+constexpr int kMaxUW = 384;  // widest unit: 1.5 x 256
+
+namespace {
+// foreach_rest_unit_in_tile (restoration.rs:1395) for a whole plane.
+__device__ __forceinline__ void unit_limits(int unit_size, int pw, int ph, int ss_y, int ux, int uy, int& hs,
+                                            int& he, int& vs, int& ve) {
+  const int ext = unit_size * 3 / 2;
+  const int y0 = uy * unit_size, x0 = ux * unit_size;
+  const int h = (ph - y0 < ext) ? ph - y0 : unit_size;
+  const int w = (pw - x0 < ext) ? pw - x0 : unit_size;
+  const int voff = 8 >> ss_y;
+  vs = max(0, y0 - voff);
+  ve = y0 + h;
+  if (ve < ph) ve -= voff;
+  hs = x0;
+  he = x0 + w;
+}
+
+// Workgroup-wide u64 sum (256 threads); every thread gets the result.
+__device__ __forceinline__ uint64_t wg_sum_u64(uint64_t v, uint64_t* red) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+}  // namespace
+
+// Wiener statistics (compute_stats / compute_stats_hbd): one workgroup per
+// unit.  Rows are staged in strips of SR (+ 2*HALF replicated halo rows) as
+// mean-removed i16 samples; each lane owns up to CH consecutive entries of
+// one upper-triangle row of H (or of M), sums one pixel row of products in
+// i32 (at most 384 products of |v| <= 1023) and flushes it into i64.
+template <typename Pix, int WIN>
+__global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
+                                                      int pw, int ph, int bd, int unit_size, int ss_y, int nh,
+                                                      int64_t* __restrict__ out_m, int64_t* __restrict__ out_h) {
+  constexpr int W2 = WIN * WIN, HALF = WIN / 2, CH = WIN == 7 ? 7 : 2, SR = 16;
+  __shared__ int16_t Y[(SR + 2 * HALF) * (kMaxUW + 2 * HALF)];
+  __shared__ int16_t X[SR * kMaxUW];
+  __shared__ uint64_t red[4];
+  const int u = blockIdx.x, uy = u / nh, ux = u - uy * nh;
+  int hs, he, vs, ve;
+  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  const int uw = he - hs, uh = ve - vs, tw = uw + 2 * HALF;
+  const int tid = threadIdx.x;
+  uint64_t s = 0;
+  for (int i = tid; i < uw * uh; i += 256) {
+    const int r = i / uw, c = i - r * uw;
+    s += dgd[(long)(vs + r) * pw + hs + c];
+  }
+  const int32_t avg = (int32_t)(wg_sum_u64(s, red) / ((uint64_t)uw * uh));
+
+  // segment of this lane: row k (k == W2 selects M), entries l0 .. l0+n-1
+  int my_k = -1, my_l0 = 0, my_n = 0;
+  {
+    int seg = 0;
+    for (int k = 0; k <= W2 && my_k < 0; k++)
+      for (int l0 = (k == W2 ? 0 : k); l0 < W2; l0 += CH) {
+        if (seg == tid) {
+          my_k = k;
+          my_l0 = l0;
+          my_n = min(CH, W2 - l0);
+          break;
+        }
+        seg++;
+      }
+  }
+  const bool active = my_k >= 0;
+  const bool is_m = my_k == W2;
+  int offl[CH];
+#pragma unroll
+  for (int q = 0; q < CH; q++) {
+    const int l = my_l0 + (q < my_n ? q : 0);
+    offl[q] = (l % WIN) * tw + (l / WIN);
+  }
+  const int offk = (active && !is_m) ? (my_k % WIN) * tw + (my_k / WIN) : 0;
+  int64_t acc[CH];
+#pragma unroll
+  for (int q = 0; q < CH; q++) acc[q] = 0;
+
+  for (int s0 = 0; s0 < uh; s0 += SR) {
+    const int sr = min(SR, uh - s0);
+    __syncthreads();
+    for (int i = tid; i < (sr + 2 * HALF) * tw; i += 256) {
+      const int r = i / tw, c = i - r * tw;
+      const int gy = clampi(vs + s0 + r - HALF, 0, ph - 1), gx = clampi(hs + c - HALF, 0, pw - 1);
+      Y[i] = (int16_t)((int32_t)dgd[(long)gy * pw + gx] - avg);
+    }
+    for (int i = tid; i < sr * uw; i += 256) {
+      const int r = i / uw, c = i - r * uw;
+      X[i] = (int16_t)((int32_t)src[(long)(vs + s0 + r) * pw + hs + c] - avg);
+    }
+    __syncthreads();
+    if (active) {
+      for (int r = 0; r < sr; r++) {
+        int32_t row[CH];
+#pragma unroll
+        for (int q = 0; q < CH; q++) row[q] = 0;
+        const int16_t* yr = Y + r * tw;
+        const int16_t* av = is_m ? X + r * uw : yr + offk;
+        for (int c = 0; c < uw; c++) {
+          const int32_t a = av[c];
+#pragma unroll
+          for (int q = 0; q < CH; q++) row[q] += a * (int32_t)yr[offl[q] + c];
+        }
+#pragma unroll
+        for (int q = 0; q < CH; q++) acc[q] += row[q];
+      }
+    }
+  }
+  if (!active) return;
+  const int64_t div = bd == 10 ? 4 : 1;
+#pragma unroll
+  for (int q = 0; q < CH; q++) {
+    if (q < my_n) {
+      const int l = my_l0 + q;
+      const int64_t v = acc[q] / div;
+      if (is_m) {
+        out_m[(long)u * W2 + l] = v;
+      } else {
+        out_h[(long)u * W2 * W2 + my_k * W2 + l] = v;
+        out_h[(long)u * W2 * W2 + l * W2 + my_k] = v;
+      }
+    }
+  }
+}
+
+// Per-unit SSE of two planes over the unit limits.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_unit_sse(const Pix* __restrict__ a, const Pix* __restrict__ b, int pw,
+                                                  int ph, int unit_size, int ss_y, int nh,
+                                                  uint64_t* __restrict__ out) {
+  __shared__ uint64_t red[4];
+  const int u = blockIdx.x, uy = u / nh, ux = u - uy * nh;
+  int hs, he, vs, ve;
+  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  const int uw = he - hs, n = uw * (ve - vs);
+  uint64_t s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int r = i / uw, c = i - r * uw;
+    const long o = (long)(vs + r) * pw + hs + c;
+    const int64_t e = (int64_t)a[o] - b[o];
+    s += (uint64_t)(e * e);
+  }
+  const uint64_t t = wg_sum_u64(s, red);
+  if (threadIdx.x == 0) out[u] = t;
+}
+
+// Self-guided parameter sets {r0, s0, r1, s1} (AV1 Sgr_Params).
+__constant__ int kSgrParams[16][4] = {
+    {2, 140, 1, 3236}, {2, 112, 1, 2158}, {2, 93, 1, 1618}, {2, 80, 1, 1438}, {2, 70, 1, 1295}, {2, 58, 1, 1177},
+    {2, 47, 1, 1079},  {2, 37, 1, 996},   {2, 30, 1, 925},  {2, 25, 1, 863},  {0, -1, 1, 2589}, {0, -1, 1, 1618},
+    {0, -1, 1, 1177},  {0, -1, 1, 925},   {2, 56, 0, -1},   {2, 22, 0, -1}};
+
+namespace {
+__device__ __forceinline__ int32_t rpot32(int32_t v, int n) { return n == 0 ? v : (v + (1 << (n - 1))) >> n; }
+__device__ __forceinline__ int x_by_xplus1(int z) {
+  return z == 0 ? 1 : (z >= 255 ? 256 : (256 * z + (z + 1) / 2) / (z + 1));
+}
+}  // namespace
+
+// Self-guided projection statistics: one workgroup per (unit, parameter
+// set).  Strips of SR output rows: stage the dgd rows they need (radius-2
+// box + 1 row of A/B halo = 3 rows / columns of replicated border), compute
+// A/B of both passes at rows/cols -1 .. n of the strip, then the two
+// filtered outputs per pixel and the five projection sums.
+// out[(u * 16 + set) * 5 + {0..4}] = H00, H01, H11, C0, C1 (sums / unit area).
+// Every i32 intermediate is bounded: box sums <= 25 * 1023^2, A <= 256,
+// B <= 2^18, weighted A*d + B < 2^25.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
+                                                   int pw, int ph, int bd, int unit_size, int ss_y, int nh,
+                                                   int64_t* __restrict__ out) {
+  constexpr int SR = 8, DW = kMaxUW + 6, AW = kMaxUW + 2;
+  __shared__ uint16_t D[(SR + 6) * DW];
+  __shared__ uint16_t A[2][(SR + 2) * AW];
+  __shared__ int32_t B[2][(SR + 2) * AW];
+  __shared__ uint64_t red[4];
+  const int u = blockIdx.x >> 4, set = blockIdx.x & 15, uy = u / nh, ux = u - uy * nh;
+  int hs, he, vs, ve;
+  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  const int W = he - hs, Hh = ve - vs, dw = W + 6, aw = W + 2;
+  const int cs = bd - 8;
+  const int rr[2] = {kSgrParams[set][0], kSgrParams[set][2]};
+  const int sp[2] = {kSgrParams[set][1], kSgrParams[set][3]};
+  const int tid = threadIdx.x;
+  int64_t h00 = 0, h01 = 0, h11 = 0, c0 = 0, c1 = 0;
+  for (int s0 = 0; s0 < Hh; s0 += SR) {
+    const int sr = min(SR, Hh - s0);
+    __syncthreads();
+    // D row R <-> unit row s0 - 3 + R; column C <-> unit column C - 3
+    for (int i = tid; i < (sr + 6) * dw; i += 256) {
+      const int R = i / dw, C = i - R * dw;
+      const int gy = clampi(vs + s0 - 3 + R, 0, ph - 1), gx = clampi(hs + C - 3, 0, pw - 1);
+      D[R * DW + C] = (uint16_t)dgd[(long)gy * pw + gx];
+    }
+    __syncthreads();
+    // A/B row a <-> unit row s0 - 1 + a, column b <-> unit column b - 1
+    for (int p = 0; p < 2; p++) {
+      const int r = rr[p];
+      if (r == 0) continue;
+      const int n = (2 * r + 1) * (2 * r + 1);
+      const uint32_t obx = (uint32_t)((4096 + n / 2) / n);
+      for (int i = tid; i < (sr + 2) * aw; i += 256) {
+        const int a = i / aw, b = i - a * aw;
+        const int ur = s0 - 1 + a;
+        if (r == 2 && ((ur + 1) & 1)) continue;  // fast variant: odd unit rows (and row -1) only
+        int32_t sum = 0, sq = 0;
+        // D row of unit row ur + dy is a + 2 + dy; column of b - 1 + dx is b + 2 + dx
+        for (int dy = -r; dy <= r; dy++)
+          for (int dx = -r; dx <= r; dx++) {
+            const int32_t v = D[(a + 2 + dy) * DW + b + 2 + dx];
+            sum += v;
+            sq += v * v;
+          }
+        const int32_t av2 = rpot32(sq, 2 * cs), bv2 = rpot32(sum, cs);
+        int32_t pp = av2 * n - bv2 * bv2;
+        pp = pp < 0 ? 0 : pp;
+        const uint32_t z = (uint32_t)(((int64_t)pp * sp[p] + (1 << 19)) >> 20);
+        const int32_t xa = x_by_xplus1(z > 255 ? 255 : (int)z);
+        const int64_t bb = (int64_t)(uint32_t)(256 - xa) * (uint32_t)sum * obx;
+        A[p][a * AW + b] = (uint16_t)xa;
+        B[p][a * AW + b] = (int32_t)((bb + (1 << 11)) >> 12);
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < sr * W; i += 256) {
+      const int rl = i / W, j = i - rl * W;
+      const int ur = s0 + rl;
+      const int32_t d = D[(rl + 3) * DW + j + 3];
+      const int64_t uu = (int64_t)d << 4;
+      const int64_t sv = ((int64_t)src[(long)(vs + ur) * pw + hs + j] << 4) - uu;
+      int64_t f[2] = {0, 0};
+      for (int p = 0; p < 2; p++) {
+        const int r = rr[p];
+        if (r == 0) continue;
+        const uint16_t* Ap = A[p];
+        const int32_t* Bp = B[p];
+        const int a = rl + 1, b = j + 1;  // A/B position of this pixel
+#define AT(X, dy, dx) ((int32_t)X[(a + (dy)) * AW + b + (dx)])
+        int32_t wa, wb, sh;
+        if (r == 1) {
+          wa = (AT(Ap, 0, 0) + AT(Ap, 0, -1) + AT(Ap, 0, 1) + AT(Ap, -1, 0) + AT(Ap, 1, 0)) * 4 +
+               (AT(Ap, -1, -1) + AT(Ap, -1, 1) + AT(Ap, 1, -1) + AT(Ap, 1, 1)) * 3;
+          wb = (AT(Bp, 0, 0) + AT(Bp, 0, -1) + AT(Bp, 0, 1) + AT(Bp, -1, 0) + AT(Bp, 1, 0)) * 4 +
+               (AT(Bp, -1, -1) + AT(Bp, -1, 1) + AT(Bp, 1, -1) + AT(Bp, 1, 1)) * 3;
+          sh = 9;
+        } else if ((ur & 1) == 0) {
+          wa = (AT(Ap, -1, 0) + AT(Ap, 1, 0)) * 6 +
+               (AT(Ap, -1, -1) + AT(Ap, -1, 1) + AT(Ap, 1, -1) + AT(Ap, 1, 1)) * 5;
+          wb = (AT(Bp, -1, 0) + AT(Bp, 1, 0)) * 6 +
+               (AT(Bp, -1, -1) + AT(Bp, -1, 1) + AT(Bp, 1, -1) + AT(Bp, 1, 1)) * 5;
+          sh = 9;
+        } else {
+          wa = AT(Ap, 0, 0) * 6 + (AT(Ap, 0, -1) + AT(Ap, 0, 1)) * 5;
+          wb = AT(Bp, 0, 0) * 6 + (AT(Bp, 0, -1) + AT(Bp, 0, 1)) * 5;
+          sh = 8;
+        }
+#undef AT
+        f[p] = (int64_t)((wa * d + wb + (1 << (sh - 1))) >> sh) - uu;
+      }
+      h00 += f[0] * f[0];
+      h01 += f[0] * f[1];
+      h11 += f[1] * f[1];
+      c0 += f[0] * sv;
+      c1 += f[1] * sv;
+    }
+  }
+  const int64_t size = (int64_t)W * Hh;
+  const int64_t v0 = (int64_t)wg_sum_u64((uint64_t)h00, red);
+  const int64_t v1 = (int64_t)wg_sum_u64((uint64_t)h01, red);
+  const int64_t v2 = (int64_t)wg_sum_u64((uint64_t)h11, red);
+  const int64_t v3 = (int64_t)wg_sum_u64((uint64_t)c0, red);
+  const int64_t v4 = (int64_t)wg_sum_u64((uint64_t)c1, red);
+  if (tid == 0) {
+    int64_t* o = out + ((long)u * 16 + set) * 5;
+    o[0] = v0 / size;
+    o[1] = v1 / size;
+    o[2] = v2 / size;
+    o[3] = v3 / size;
+    o[4] = v4 / size;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static void lf_thresholds(int level, int sharpness, Thr& t) {
+  int bil = level >> ((sharpness > 0) + (sharpness > 4));
+  if (sharpness > 0 && bil > 9 - sharpness) bil = 9 - sharpness;
+  if (bil < 1) bil = 1;
+  t.mblim = 2 * (level + 2) + bil;
+  t.lim = bil;
+  t.hev = level >> 4;
+}
+
+hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plane, int ss, int level_v,
+                         int level_h, int sharpness, const DeblockGeomDev& g, hipStream_t st) {
+  Thr tv{}, th{};
+  lf_thresholds(level_v, sharpness, tv);
+  lf_thresholds(level_h, sharpness, th);
+  tv.shift = th.shift = bd - 8;
+  tv.bd = th.bd = bd;
+  const long nv = (long)(pw >> 2) * (ph & ~3), nhz = (long)(ph >> 2) * (pw & ~3);
+  if (level_v && nv > 0) {
+    const unsigned gr = (unsigned)((nv + 255) / 256);
+    if (bd == 8)
+      hipLaunchKernelGGL((k_deblock<uint8_t, true>), dim3(gr), dim3(256), 0, st, (uint8_t*)buf, stride, pw, ph,
+                         plane, ss, level_v, tv, g);
+    else
+      hipLaunchKernelGGL((k_deblock<uint16_t, true>), dim3(gr), dim3(256), 0, st, (uint16_t*)buf, stride, pw, ph,
+                         plane, ss, level_v, tv, g);
+  }
+  if (level_h && nhz > 0) {
+    const unsigned gr = (unsigned)((nhz + 255) / 256);
+    if (bd == 8)
+      hipLaunchKernelGGL((k_deblock<uint8_t, false>), dim3(gr), dim3(256), 0, st, (uint8_t*)buf, stride, pw, ph,
+                         plane, ss, level_h, th, g);
+    else
+      hipLaunchKernelGGL((k_deblock<uint16_t, false>), dim3(gr), dim3(256), 0, st, (uint16_t*)buf, stride, pw, ph,
+                         plane, ss, level_h, th, g);
+  }
+  return hipGetLastError();
+}
+
+hipError_t cdef_search(const void* rec_y, const void* rec_u, const void* rec_v, const void* src_y,
+                       const void* src_u, const void* src_v, int width, int height, int bd, int damping, int sub_y,
+                       int sub_uv, int32_t* dirvar, uint64_t* mse_y, uint64_t* mse_uv, hipStream_t st) {
+  const int nhfb = (width + 63) / 64, nvfb = (height + 63) / 64;
+  const unsigned gr = (unsigned)(nhfb * nvfb);
+  if (bd == 8) {
+    hipLaunchKernelGGL((k_cdef_luma<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)rec_y,
+                       (const uint8_t*)src_y, width, height, bd, damping, sub_y, nhfb, mse_y, dirvar);
+    if (mse_uv)
+      hipLaunchKernelGGL((k_cdef_chroma<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)rec_u,
+                         (const uint8_t*)rec_v, (const uint8_t*)src_u, (const uint8_t*)src_v, width, height, bd,
+                         damping, sub_uv, nhfb, (const int32_t*)dirvar, mse_uv);
+  } else {
+    hipLaunchKernelGGL((k_cdef_luma<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)rec_y,
+                       (const uint16_t*)src_y, width, height, bd, damping, sub_y, nhfb, mse_y, dirvar);
+    if (mse_uv)
+      hipLaunchKernelGGL((k_cdef_chroma<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)rec_u,
+                         (const uint16_t*)rec_v, (const uint16_t*)src_u, (const uint16_t*)src_v, width, height, bd,
+                         damping, sub_uv, nhfb, (const int32_t*)dirvar, mse_uv);
+  }
+  return hipGetLastError();
+}
+
+hipError_t wiener_stats(const void* dgd, const void* src, int pw, int ph, int bd, int win, int unit_size, int ss_y,
+                        int nh, int nv, int64_t* M, int64_t* H, hipStream_t st) {
+  const unsigned gr = (unsigned)(nh * nv);
+#define ZW(P, WN)                                                                                              \
+  hipLaunchKernelGGL((k_wiener_stats<P, WN>), dim3(gr), dim3(256), 0, st, (const P*)dgd, (const P*)src, pw, ph, \
+                     bd, unit_size, ss_y, nh, M, H)
+  if (bd == 8) {
+    if (win == 7) ZW(uint8_t, 7);
+    else ZW(uint8_t, 5);
+  } else {
+    if (win == 7) ZW(uint16_t, 7);
+    else ZW(uint16_t, 5);
+  }
+#undef ZW
+  return hipGetLastError();
+}
+
+hipError_t unit_sse(const void* a, const void* b, int pw, int ph, int bd, int unit_size, int ss_y, int nh, int nv,
+                    uint64_t* out, hipStream_t st) {
+  const unsigned gr = (unsigned)(nh * nv);
+  if (bd == 8)
+    hipLaunchKernelGGL((k_unit_sse<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)a, (const uint8_t*)b, pw,
+                       ph, unit_size, ss_y, nh, out);
+  else
+    hipLaunchKernelGGL((k_unit_sse<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)a, (const uint16_t*)b,
+                       pw, ph, unit_size, ss_y, nh, out);
+  return hipGetLastError();
+}
+
+hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, int unit_size, int ss_y, int nh,
+                     int nv, int64_t* out, hipStream_t st) {
+  const unsigned gr = (unsigned)(nh * nv * 16);
+  if (bd == 8)
+    hipLaunchKernelGGL((k_sgr_stats<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)dgd, (const uint8_t*)src,
+                       pw, ph, bd, unit_size, ss_y, nh, out);
+  else
+    hipLaunchKernelGGL((k_sgr_stats<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)dgd,
+                       (const uint16_t*)src, pw, ph, bd, unit_size, ss_y, nh, out);
+  return hipGetLastError();
+}
+
+}  // namespace zav1

@@ -73,4 +73,28 @@ hipError_t intra_pred_batch(const void* plane, int stride, int bd, const IntraJo
 hipError_t cfl_batch(const void* luma, int stride, int bd, const CflJob* jobs, int n_jobs, const uint16_t* dc,
                      uint16_t* out, hipStream_t st);
 
+// Per-4x4 (mi) frame geometry for deblocking (device pointers), the
+// reference's DeblockGeom (rust/crates/svtav1-encoder/src/deblock.rs:532).
+struct DeblockGeomDev {
+  int32_t mi_cols, mi_rows, true_w, true_h;
+  const uint32_t* block_id;
+  const uint8_t *bw, *bh, *tw, *th, *skip;  // skip may be nullptr (all-intra)
+};
+// Vertical then horizontal edges of one plane, in place (buf is u8 at
+// bd 8, u16 at bd 10).
+hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plane, int ss, int level_v, int level_h,
+                         int sharpness, const DeblockGeomDev& g, hipStream_t st);
+// CDEF strength search over every 64x64 filter block: mse_y / mse_uv rows of
+// 64 strengths per block (raster of blocks); dirvar [fb][64][2] (dir, var).
+hipError_t cdef_search(const void* rec_y, const void* rec_u, const void* rec_v, const void* src_y,
+                       const void* src_u, const void* src_v, int width, int height, int bd, int damping, int sub_y,
+                       int sub_uv, int32_t* dirvar, uint64_t* mse_y, uint64_t* mse_uv, hipStream_t st);
+// Restoration-unit statistics over an nh x nv unit grid (raster order).
+hipError_t wiener_stats(const void* dgd, const void* src, int pw, int ph, int bd, int win, int unit_size, int ss_y,
+                        int nh, int nv, int64_t* M, int64_t* H, hipStream_t st);
+hipError_t unit_sse(const void* a, const void* b, int pw, int ph, int bd, int unit_size, int ss_y, int nh, int nv,
+                    uint64_t* out, hipStream_t st);
+hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, int unit_size, int ss_y, int nh,
+                     int nv, int64_t* out, hipStream_t st);
+
 }  // namespace zav1

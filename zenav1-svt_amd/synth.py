@@ -125,3 +125,92 @@ def qp_to_qindex(qp):
     if qp < 62:
         return 4 * qp
     return 249 if qp == 62 else 255
+
+
+def config3_chroma(h=2048, w=2048, seed=3000):
+    """4:2:0 chroma planes of the loop-filter input: 1/f noise sources with
+    4x4 mean-flattening on 30% of the 4x4 blocks in the reconstructions."""
+    out = []
+    for k in range(2):
+        s = seed + 10 + 2 * k
+        f = spectral_noise(h, w, 1.0, s)
+        src = np.clip(np.round(64 + f * 128), 0, 255).astype(np.uint8)
+        rng = _np_rng(s + 1)
+        rec = src.astype(np.int32).copy()
+        bh, bw = h // 4, w // 4
+        mask = rng.uniform(size=(bh, bw)) < 0.30
+        means = np.round(rec[:bh * 4, :bw * 4].reshape(bh, 4, bw, 4).mean(axis=(1, 3))).astype(np.int32)
+        flat = np.repeat(np.repeat(means, 4, 0), 4, 1)
+        m = np.repeat(np.repeat(mask, 4, 0), 4, 1)
+        sub = rec[:bh * 4, :bw * 4]
+        sub[m] = flat[m]
+        rec = rec + rng.integers(-1, 2, rec.shape)
+        out.append((src, np.clip(rec, 0, 255).astype(np.uint8)))
+    return out
+
+
+def partition_geometry(mi_rows, mi_cols, seed, sb=64, skip_frac=0.0):
+    """Synthetic per-4x4 (mi) block / transform geometry of a frame: each
+    SB is split recursively (none / quad / horizontal / vertical halves) into
+    blocks aligned to their own size, and each block carries transforms of
+    its own size or half of it (capped at 64).  Returns a dict of row-major
+    mi_rows x mi_cols arrays: block_id u32, bw, bh, tw, th (px) and, when
+    skip_frac > 0, skip u8 (a block's skip flag)."""
+    rnd = XorShift64Star(seed)
+    g = {k: np.zeros((mi_rows, mi_cols), np.uint8) for k in ('bw', 'bh', 'tw', 'th', 'skip')}
+    g['block_id'] = np.zeros((mi_rows, mi_cols), np.uint32)
+    nid = [0]
+
+    def leaf(r, c, h, w):
+        tdep = rnd.randint(0, 1) if min(w, h) >= 8 else 0
+        tw, th = min(w >> tdep, 64), min(h >> tdep, 64)
+        sl = (slice(r // 4, (r + h) // 4), slice(c // 4, (c + w) // 4))
+        g['block_id'][sl] = nid[0]
+        g['bw'][sl] = min(w, 255)
+        g['bh'][sl] = min(h, 255)
+        g['tw'][sl] = tw
+        g['th'][sl] = th
+        g['skip'][sl] = 1 if rnd.uniform() < skip_frac else 0
+        nid[0] += 1
+
+    def split(r, c, s):
+        if r >= mi_rows * 4 or c >= mi_cols * 4:
+            return
+        u = rnd.uniform()
+        if s == 4:
+            leaf(r, c, 4, 4)
+        elif u < 0.30 or (s == 64 and u < 0.05):
+            leaf(r, c, s, s)
+        elif u < 0.70:
+            h = s // 2
+            for dr in (0, h):
+                for dc in (0, h):
+                    split(r + dr, c + dc, h)
+        elif u < 0.85:
+            leaf(r, c, s // 2, s)
+            if r + s // 2 < mi_rows * 4:
+                leaf(r + s // 2, c, s // 2, s)
+        else:
+            leaf(r, c, s, s // 2)
+            if c + s // 2 < mi_cols * 4:
+                leaf(r, c + s // 2, s, s // 2)
+
+    for r in range(0, mi_rows * 4, sb):
+        for c in range(0, mi_cols * 4, sb):
+            split(r, c, sb)
+    if skip_frac <= 0:
+        del g['skip']
+    return g
+
+
+def key_frame_lf_levels(ac_q, bd):
+    """Deblocking levels of a key frame from the AC quantizer step
+    (rust/crates/svtav1-encoder/src/deblock.rs:55): [luma V, luma H, U, V]."""
+    if bd == 8:
+        fg = (ac_q * 17563 - 421574 + (1 << 17)) >> 18
+    else:
+        fg = ((ac_q * 20723 + 4060632 + (1 << 19)) >> 20) - 4
+    fc = int(fg / 2) if fg >= 0 else -((-fg) // 2)
+    y = min(max(fg, 0), 63)
+    uv = min(max(fc, 0), 63)
+    return [y, y, uv, uv]
