@@ -14,6 +14,13 @@ function-table call: fwd + quantize + sse per candidate) on a bounded sample,
 the roofline of the dominant kernel from HIP events over the timed region, and
 the oracle (CPU restatement) timed on a bounded crop of the same workload.
 
+A second object, loopfilter_stage, times the loop-filter stage on the
+4096x4096 8-bit 4:2:0 reconstruction (1/f noise + 30% 8x8 mean-flattened
+blocks, seed 3000): deblocking of all planes, CDEF direction + 64-strength SSE
+tables per 64x64 block, Wiener M/H + unit SSE and self-guided statistics per
+restoration unit (128 luma / 64 chroma), with its own roofline and oracle
+baseline.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -62,6 +69,114 @@ def candidate_ops(ts_mod, tx_size, tx_type, opsd):
     return ops
 
 
+# Algorithmic integer ops per pixel of the self-guided statistics, per
+# parameter set kind: box sums + A/B per position (r = 2 on every other row),
+# the weighted A/B neighbourhood and output of each pass, and the five
+# projection products.
+SGR_OPS_R2 = (25 * 2 + 10) / 2 + 17
+SGR_OPS_R1 = 9 * 2 + 10 + 25
+SGR_OPS_PROJ = 13
+SGR_OPS_PER_PX = (10 * (SGR_OPS_R2 + SGR_OPS_R1 + SGR_OPS_PROJ) + 4 * (SGR_OPS_R1 + SGR_OPS_PROJ) +
+                  2 * (SGR_OPS_R2 + SGR_OPS_PROJ))
+
+
+def loopfilter_stage(zav1, synth, args, cpu=True):
+    L = zav1.lib()
+    lfm = _load('zav1_loopfilter', os.path.join(PKG, 'loopfilter.py'))
+    N = 4096
+    src_y, rec_y = synth.config3_recon(N, N, seed=3000)
+    (su, ru), (sv, rv) = synth.config3_chroma(N // 2, N // 2, seed=3000)
+    geom = synth.partition_geometry(N // 4, N // 4, 3001)
+    qi = synth.qp_to_qindex(32)
+    levels = synth.key_frame_lf_levels(int(zav1.quant_row(qi, 8).dequant[1]), 8)
+    st = lfm.LoopFilterStage(zav1, {'src': (src_y, su, sv), 'rec': (rec_y, ru, rv)}, geom, 8, qi, levels)
+    sp = ctypes.c_void_p()
+    zav1.check(L.zav1_stream_create(ctypes.byref(sp)), 'stream_create')
+    stream = sp.value
+    for _ in range(2):
+        st.run(stream)
+    zav1.sync(stream)
+    nph = len(st.PHASES)
+    ev = []
+    for _ in range(nph + 1):
+        e = ctypes.c_void_p()
+        L.zav1_event_create(ctypes.byref(e))
+        ev.append(e.value)
+    # luma self-guided launch alone, for the kernel roofline
+    e_sg = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in e_sg:
+        L.zav1_event_create(ctypes.byref(e))
+    phase_ms = np.zeros(nph)
+    sg_luma_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.lf_steps):
+        L.zav1_event_record(ev[0], stream)
+        for i, ph in enumerate(st.PHASES):
+            if ph == 'selfguided':
+                L.zav1_event_record(e_sg[0].value, stream)
+                zav1.sgr_stats(st.dgd[0], st.src[0], st.w, st.h, 8, st.ru[0], 0, st.sgrs[0], stream)
+                L.zav1_event_record(e_sg[1].value, stream)
+                for p in (1, 2):
+                    zav1.sgr_stats(st.dgd[p], st.src[p], st.cw, st.ch, 8, st.ru[p], 1, st.sgrs[p], stream)
+            else:
+                getattr(st, ph)(stream)
+            L.zav1_event_record(ev[i + 1], stream)
+        zav1.sync(stream)
+        for i in range(nph):
+            ms = ctypes.c_float()
+            L.zav1_event_elapsed_ms(ev[i], ev[i + 1], ctypes.byref(ms))
+            phase_ms[i] += ms.value
+        ms = ctypes.c_float()
+        L.zav1_event_elapsed_ms(e_sg[0].value, e_sg[1].value, ctypes.byref(ms))
+        sg_luma_ms += ms.value
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1000.0 / args.lf_steps
+    sg_s = sg_luma_ms / 1000.0 / args.lf_steps
+    sg_ops = N * N * SGR_OPS_PER_PX
+    out = {'workload': 'config3 loop-filter stage: 4096x4096 8-bit 4:2:0, deblock + CDEF 64-strength SSE tables '
+                       '+ Wiener M/H + unit SSE + self-guided stats (RU 128 luma / 64 chroma)',
+           'value': round(N * N / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s', 'steps': args.lf_steps,
+           'ms_per_step': round(ms_step, 3),
+           'phase_ms': {ph: round(phase_ms[i] / args.lf_steps, 4) for i, ph in enumerate(st.PHASES)},
+           'roofline': {'bound': 'valu', 'kernel': 'k_sgr_stats<uint8_t> (luma)', 'unit': 'Tops/s',
+                        'achieved': round(sg_ops / sg_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
+                        'frac': round(sg_ops / sg_s / 1e12 / VALU_PEAK_TOPS, 4), 'traffic': None,
+                        'avg_launch_us': round(sg_s * 1e6, 2), 'algorithmic_ops_per_launch': int(sg_ops),
+                        'hbm_algorithmic_GBs': round(2 * N * N / sg_s / 1e9, 2)}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        c = 2048
+        ys, yr = np.ascontiguousarray(src_y[:c, :c]), np.ascontiguousarray(rec_y[:c, :c])
+        us_, ur_ = np.ascontiguousarray(su[:c // 2, :c // 2]), np.ascontiguousarray(ru[:c // 2, :c // 2])
+        vs_, vr_ = np.ascontiguousarray(sv[:c // 2, :c // 2]), np.ascontiguousarray(rv[:c // 2, :c // 2])
+        g = {k: np.ascontiguousarray(v[:c // 4, :c // 4]) for k, v in geom.items()}
+        LR = O.lrlib()
+        tc0 = time.perf_counter()
+        d = [O.deblock_plane(yr, 8, 0, 0, levels[0], levels[1], 0, g, c // 4, c // 4, c, c),
+             O.deblock_plane(ur_, 8, 1, 1, levels[2], levels[2], 0, g, c // 4, c // 4, c, c),
+             O.deblock_plane(vr_, 8, 2, 1, levels[3], levels[3], 0, g, c // 4, c // 4, c, c)]
+        for fbr in range(c // 64):
+            for fbc in range(c // 64):
+                O.cdef_search_fb(tuple(d), (ys, us_, vs_), c, c, 8, qi, 1, fbr, fbc, np.arange(64))
+        for p, (pl, sp_, us, ss) in enumerate([(d[0], ys, 128, 0), (d[1], us_, 64, 1), (d[2], vs_, 64, 1)]):
+            pw = pl.shape[1]
+            n = LR.or_count_units(us, pw)
+            for uy in range(n):
+                for ux in range(n):
+                    lim = O.unit_limits(us, pw, pw, ss, ux, uy)
+                    O.wiener_stats(7 if p == 0 else 5, pl, sp_, 8, lim)
+                    LR.or_sse_region(pl.ctypes.data, sp_.ctypes.data, pw, 8, *lim)
+                    for s_ in range(16):
+                        O.sgr_unit_stats(pl, sp_, 8, lim, s_)
+        tc1 = time.perf_counter()
+        out['cpu_baseline'] = {'value': round(c * c / 1e6 / (tc1 - tc0), 4), 'unit': 'MP/s', 'cores': 1,
+                               'kind': 'port', 'sample': '2048x2048 crop of the same frame, whole stage '
+                                                         '(oracle C, one thread)',
+                               'seconds': round(tc1 - tc0, 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -70,6 +185,8 @@ def main():
     ap.add_argument('--size', type=int, default=2048)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-lf', action='store_true')
+    ap.add_argument('--lf-steps', type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; multi-GPU runs are not part of it '
@@ -193,7 +310,7 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
         import oracle as O
-        crop = 1024
+        crop = S
         sub = np.ascontiguousarray(plane[:crop, :crop])
         threads = min(os.cpu_count() or 1, 16)
         os.environ['OMP_NUM_THREADS'] = str(threads)
@@ -205,9 +322,11 @@ def main():
         tc1 = time.perf_counter()
         cpu_mp = crop * crop / 1e6 * len(qps) / (tc1 - tc0)
         out['cpu_baseline'] = {'value': round(cpu_mp, 4), 'unit': 'MP/s', 'cores': threads, 'kind': 'port',
-                               'sample': '%dx%d crop of the same frame, qp 20+40, all sizes/types '
+                               'sample': '%dx%d (whole) frame, qp 20+40, all sizes/types '
                                          '(oracle C, OpenMP over blocks)' % (crop, crop),
                                'seconds': round(tc1 - tc0, 2)}
+    if not args.no_lf:
+        out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=not args.no_cpu)
     print(json.dumps(out))
 
 

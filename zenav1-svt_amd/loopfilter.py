@@ -60,23 +60,45 @@ class LoopFilterStage:
     def pixels(self):
         return self.w * self.h
 
-    def run(self, stream=None):
-        z = self.z
-        L = z.lib()
+    def _dims(self, p):
+        return (self.w, self.h, 0) if p == 0 else (self.cw, self.ch, 1)
+
+    def copy_rec(self, stream=None):
+        """Fresh deblocking input: dgd <- rec (deblocking works in place)."""
+        L = self.z.lib()
         for p in range(3):
-            z.check(L.zav1_memcpy_d2d(self.dgd[p].ptr, self.rec[p].ptr, self.rec[p].nbytes, stream), 'memcpy_d2d')
-        dims = [(self.w, self.h, 0), (self.cw, self.ch, 1), (self.cw, self.ch, 1)]
+            self.z.check(L.zav1_memcpy_d2d(self.dgd[p].ptr, self.rec[p].ptr, self.rec[p].nbytes, stream),
+                         'memcpy_d2d')
+
+    def deblock(self, stream=None):
         lv = [(self.levels[0], self.levels[1]), (self.levels[2], self.levels[2]), (self.levels[3], self.levels[3])]
         for p in range(3):
-            pw, ph, ss = dims[p]
-            z.deblock_plane(self.dgd[p], pw, pw, ph, self.bd, p, ss, lv[p][0], lv[p][1], self.sharpness, self.geom,
-                            stream)
-        z.cdef_search(self.dgd, self.src, self.w, self.h, self.bd, self.qindex, self.cdef_sub, self.dirvar,
-                      self.mse_y, self.mse_uv, stream)
+            pw, ph, ss = self._dims(p)
+            self.z.deblock_plane(self.dgd[p], pw, pw, ph, self.bd, p, ss, lv[p][0], lv[p][1], self.sharpness,
+                                 self.geom, stream)
+
+    def cdef(self, stream=None):
+        self.z.cdef_search(self.dgd, self.src, self.w, self.h, self.bd, self.qindex, self.cdef_sub, self.dirvar,
+                           self.mse_y, self.mse_uv, stream)
+
+    def wiener(self, stream=None):
         for p in range(3):
-            pw, ph, ss = dims[p]
-            z.wiener_stats(self.dgd[p], self.src[p], pw, ph, self.bd, self.win[p], self.ru[p], ss, self.M[p],
-                           self.H[p], stream)
-            z.unit_sse(self.dgd[p], self.src[p], pw, ph, self.bd, self.ru[p], ss, self.sse[p], stream)
-            if self.sgr:
-                z.sgr_stats(self.dgd[p], self.src[p], pw, ph, self.bd, self.ru[p], ss, self.sgrs[p], stream)
+            pw, ph, ss = self._dims(p)
+            self.z.wiener_stats(self.dgd[p], self.src[p], pw, ph, self.bd, self.win[p], self.ru[p], ss, self.M[p],
+                                self.H[p], stream)
+            self.z.unit_sse(self.dgd[p], self.src[p], pw, ph, self.bd, self.ru[p], ss, self.sse[p], stream)
+
+    def selfguided(self, stream=None):
+        for p in range(3):
+            pw, ph, ss = self._dims(p)
+            self.z.sgr_stats(self.dgd[p], self.src[p], pw, ph, self.bd, self.ru[p], ss, self.sgrs[p], stream)
+
+    PHASES = ('copy_rec', 'deblock', 'cdef', 'wiener', 'selfguided')
+
+    def run(self, stream=None):
+        self.copy_rec(stream)
+        self.deblock(stream)
+        self.cdef(stream)
+        self.wiener(stream)
+        if self.sgr:
+            self.selfguided(stream)
