@@ -86,7 +86,7 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
     N = 4096
     src_y, rec_y = synth.config3_recon(N, N, seed=3000)
     (su, ru), (sv, rv) = synth.config3_chroma(N // 2, N // 2, seed=3000)
-    geom = synth.partition_geometry(N // 4, N // 4, 3001)
+    geom = synth.partition_geometry(N // 4, N // 4, 3001, skip_frac=0.2)
     qi = synth.qp_to_qindex(32)
     levels = synth.key_frame_lf_levels(int(zav1.quant_row(qi, 8).dequant[1]), 8)
     st = lfm.LoopFilterStage(zav1, {'src': (src_y, su, sv), 'rec': (rec_y, ru, rv)}, geom, 8, qi, levels)
@@ -134,7 +134,8 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
     sg_s = sg_luma_ms / 1000.0 / args.lf_steps
     sg_ops = N * N * SGR_OPS_PER_PX
     out = {'workload': 'config3 loop-filter stage: 4096x4096 8-bit 4:2:0, deblock + CDEF 64-strength SSE tables '
-                       '+ Wiener M/H + unit SSE + self-guided stats (RU 128 luma / 64 chroma)',
+                       '(20% of blocks signaled skip) + Wiener M/H + unit SSE + self-guided stats '
+                       '(RU 128 luma / 64 chroma)',
            'value': round(N * N / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s', 'steps': args.lf_steps,
            'ms_per_step': round(ms_step, 3),
            'phase_ms': {ph: round(phase_ms[i] / args.lf_steps, 4) for i, ph in enumerate(st.PHASES)},
@@ -150,7 +151,8 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
         ys, yr = np.ascontiguousarray(src_y[:c, :c]), np.ascontiguousarray(rec_y[:c, :c])
         us_, ur_ = np.ascontiguousarray(su[:c // 2, :c // 2]), np.ascontiguousarray(ru[:c // 2, :c // 2])
         vs_, vr_ = np.ascontiguousarray(sv[:c // 2, :c // 2]), np.ascontiguousarray(rv[:c // 2, :c // 2])
-        g = {k: np.ascontiguousarray(v[:c // 4, :c // 4]) for k, v in geom.items()}
+        g = {k: np.ascontiguousarray(v[:c // 4, :c // 4]) for k, v in geom.items() if k != 'skip'}
+        gskip = np.ascontiguousarray(geom['skip'][:c // 4, :c // 4])
         LR = O.lrlib()
         tc0 = time.perf_counter()
         d = [O.deblock_plane(yr, 8, 0, 0, levels[0], levels[1], 0, g, c // 4, c // 4, c, c),
@@ -158,7 +160,7 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
              O.deblock_plane(vr_, 8, 2, 1, levels[3], levels[3], 0, g, c // 4, c // 4, c, c)]
         for fbr in range(c // 64):
             for fbc in range(c // 64):
-                O.cdef_search_fb(tuple(d), (ys, us_, vs_), c, c, 8, qi, 1, fbr, fbc, np.arange(64))
+                O.cdef_search_fb(tuple(d), (ys, us_, vs_), c, c, 8, qi, 1, fbr, fbc, np.arange(64), skip=gskip)
         for p, (pl, sp_, us, ss) in enumerate([(d[0], ys, 128, 0), (d[1], us_, 64, 1), (d[2], vs_, 64, 1)]):
             pw = pl.shape[1]
             n = LR.or_count_units(us, pw)

@@ -253,7 +253,11 @@ int zav1_deblock_plane(void *d_plane, int stride, int plane_w, int plane_h, int 
  * (d_mse_y[fb*64 + g]) and of U + V (d_mse_uv, may be NULL) against the
  * source, with the reference's per-plane shift and subsampling scale.
  * d_dirvar[(fb*64 + by*8 + bx)*2 + {0,1}] receives each 8x8's direction and
- * variance (required when d_mse_uv is given, else may be NULL).  Damping is
+ * variance (required when d_mse_uv is given, else may be NULL).  d_skip
+ * (may be NULL) is the per-mi signaled skip flag, mi_rows x mi_cols (the
+ * reference's DeblockGeom::skip): an 8x8 whose mi are all skip is left out
+ * of the search (is_8x8_all_skip, rust/crates/svtav1-encoder/src/
+ * deblock.rs:652), and a block with none left has zero rows.  Damping is
  * 3 + (qindex >> 6); rows are sampled every `subsampling` (luma, at most 4)
  * / min(subsampling, 1) (chroma).  Replaces cdef_search_still /
  * cdef_search_still_hbd (rust/crates/svtav1-encoder/src/cdef.rs:1166 /
@@ -261,7 +265,8 @@ int zav1_deblock_plane(void *d_plane, int stride, int plane_w, int plane_h, int 
  * (rust/crates/svtav1-dsp/src/cdef.rs:143 / :931). */
 int zav1_cdef_search(const void *d_rec_y, const void *d_rec_u, const void *d_rec_v, const void *d_src_y,
                      const void *d_src_u, const void *d_src_v, int width, int height, int bd, int qindex,
-                     int subsampling, int32_t *d_dirvar, uint64_t *d_mse_y, uint64_t *d_mse_uv, zav1_stream_t s);
+                     int subsampling, const uint8_t *d_skip, int mi_cols, int mi_rows, int32_t *d_dirvar,
+                     uint64_t *d_mse_y, uint64_t *d_mse_uv, zav1_stream_t s);
 
 /* Restoration-unit grid of a plane (count_units_in_tile,
  * rust/crates/svtav1-dsp/src/restoration.rs:1388).  Host only. */

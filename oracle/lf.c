@@ -340,14 +340,26 @@ static void build_src(uint16_t *src, const void *pre, int hbd, int pw, int ph, i
   }
 }
 
+/* enc/deblock.rs:652 is_8x8_all_skip: all (up to) four mi of the 8x8 whose
+ * top-left mi is (mr, mc) carry the signaled skip flag. */
+static int all_skip_8x8(const uint8_t *skip, int mi_cols, int mi_rows, int mr, int mc) {
+  for (int r = mr; r < mr + 2 && r < mi_rows; r++)
+    for (int c = mc; c < mc + 2 && c < mi_cols; c++)
+      if (!skip[r * mi_cols + c]) return 0;
+  return 1;
+}
+
 /* enc/cdef.rs:1166 cdef_search_still / :1374 _hbd for one 64x64 filter
- * block (fbr, fbc), every candidate in fs[0..n_fs), every 8x8 in the fb
- * (no skip map).  Writes mse_y[n_fs] and mse_uv[n_fs] (U + V accumulated);
- * planes are tightly packed (stride = plane width).  chroma may be NULL. */
+ * block (fbr, fbc), every candidate in fs[0..n_fs).  The 8x8s searched are
+ * the fb's dlist (:1202-1210): all of them when skip is NULL, else those
+ * that are not all-skip per the per-mi signaled skip map (mi_cols x
+ * mi_rows).  Writes mse_y[n_fs] and mse_uv[n_fs] (U + V accumulated; zero
+ * rows for an empty dlist); planes are tightly packed (stride = plane
+ * width).  chroma may be NULL. */
 void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, const void *src_y,
                        const void *src_u, const void *src_v, int width, int height, int bd, int qindex,
                        int subsampling, int fbr, int fbc, const int *fs, int n_fs, uint64_t *mse_y,
-                       uint64_t *mse_uv) {
+                       uint64_t *mse_uv, const uint8_t *skip, int mi_cols, int mi_rows) {
   const int hbd = bd > 8, cs = bd - 8;
   const int damping = 3 + (qindex >> 6);
   const int sub_y = subsampling < 4 ? subsampling : 4, sub_uv = subsampling < 1 ? subsampling : 1;
@@ -355,18 +367,22 @@ void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, 
   int hsize = width - fbc * 64 < 64 ? width - fbc * 64 : 64;
   uint16_t *pad = (uint16_t *)malloc(sizeof(uint16_t) * BSTRIDE * (128 + 2 * VB));
   int32_t blk[64];
-  int dir[8][8], var[8][8];
+  int dir[8][8], var[8][8], act[8][8];
   const int base = VB * BSTRIDE + HB;
+  for (int by = 0; by < 8; by++)
+    for (int bx = 0; bx < 8; bx++)
+      act[by][bx] = !skip || !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + by * 2, fbc * 16 + bx * 2);
   build_src(pad, rec_y, hbd, width, height, fbr * 64, fbc * 64, vsize, hsize);
   for (int by = 0; by < vsize / 8; by++)
     for (int bx = 0; bx < hsize / 8; bx++)
-      dir[by][bx] = or_cdef_find_dir(pad + base + by * 8 * BSTRIDE + bx * 8, BSTRIDE, cs, &var[by][bx]);
+      if (act[by][bx]) dir[by][bx] = or_cdef_find_dir(pad + base + by * 8 * BSTRIDE + bx * 8, BSTRIDE, cs, &var[by][bx]);
   for (int g = 0; g < n_fs; g++) {
     int pri = (fs[g] / 4) << cs, s = fs[g] % 4, sec = (s + (s == 3)) << cs;
     int dmp = damping + cs;
     uint64_t d = 0;
     for (int by = 0; by < vsize / 8; by++)
       for (int bx = 0; bx < hsize / 8; bx++) {
+        if (!act[by][bx]) continue;
         const uint16_t *in = pad + base + by * 8 * BSTRIDE + bx * 8;
         if (fs[g] == 0) {
           for (int i = 0; i < 8; i += sub_y)
@@ -395,6 +411,7 @@ void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, 
         uint64_t d = 0;
         for (int by = 0; by < vsize / 8; by++)
           for (int bx = 0; bx < hsize / 8; bx++) {
+            if (!act[by][bx]) continue;
             const uint16_t *in = pad + base + by * 4 * BSTRIDE + bx * 4;
             if (fs[g] == 0) {
               for (int i = 0; i < 4; i += sub_uv)

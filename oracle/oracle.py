@@ -228,7 +228,8 @@ def _bind_lf(L):
     L.or_cdef_filter_block.argtypes = [_vp, ctypes.c_int, _vp] + [ctypes.c_int] * 9
     L.or_lpf.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_long] + [ctypes.c_int] * 5
     L.or_deblock_plane.argtypes = [_vp] + [ctypes.c_int] * 9 + [ctypes.POINTER(DeblockGeom)]
-    L.or_cdef_search_fb.argtypes = [_vp] * 6 + [ctypes.c_int] * 7 + [_vp, ctypes.c_int, _vp, _vp]
+    L.or_cdef_search_fb.argtypes = [_vp] * 6 + [ctypes.c_int] * 7 + [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                                                       ctypes.c_int]
     L.or_lf_thresholds.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
 
 
@@ -257,15 +258,18 @@ def deblock_plane(plane, bd, plane_idx, ss, level_v, level_h, sharpness, geom_ar
     return p
 
 
-def cdef_search_fb(rec, src, width, height, bd, qindex, sub, fbr, fbc, fs):
-    """rec/src: (y, u, v) tuples of planes (u/v may be None)."""
+def cdef_search_fb(rec, src, width, height, bd, qindex, sub, fbr, fbc, fs, skip=None):
+    """rec/src: (y, u, v) tuples of planes (u/v may be None); skip: optional
+    per-mi signaled-skip map (mi_rows x mi_cols u8)."""
+    sk = np.ascontiguousarray(skip, dtype=np.uint8) if skip is not None else None
     fsa = np.ascontiguousarray(fs, dtype=np.int32)
     my = np.zeros(fsa.size, np.uint64)
     muv = np.zeros(fsa.size, np.uint64)
     ptr = lambda a: a.ctypes.data if a is not None else None
     lflib().or_cdef_search_fb(ptr(rec[0]), ptr(rec[1]), ptr(rec[2]), ptr(src[0]), ptr(src[1]), ptr(src[2]),
                               width, height, bd, qindex, sub, fbr, fbc, fsa.ctypes.data, fsa.size, my.ctypes.data,
-                              muv.ctypes.data if rec[1] is not None else None)
+                              muv.ctypes.data if rec[1] is not None else None, ptr(sk),
+                              sk.shape[1] if sk is not None else 0, sk.shape[0] if sk is not None else 0)
     return my, muv
 
 

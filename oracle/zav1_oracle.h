@@ -144,7 +144,7 @@ void or_cdef_filter_block(int32_t *dst, int dstride, const uint16_t *in, int pri
 void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, const void *src_y,
                        const void *src_u, const void *src_v, int width, int height, int bd, int qindex,
                        int subsampling, int fbr, int fbc, const int *fs, int n_fs, uint64_t *mse_y,
-                       uint64_t *mse_uv);
+                       uint64_t *mse_uv, const uint8_t *skip, int mi_cols, int mi_rows);
 
 /* ---- loop restoration statistics (restoration.c) ------------------------ */
 int or_count_units(int unit_size, int size);

@@ -52,9 +52,11 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     assert L.zav1_deblock_plane(None, 64, 64, 64, 8, 0, 0, 10, 10, 0, ctypes.byref(g), None) == zav1.ZAV1_EINVAL
     fake = ctypes.c_void_p(1)
     assert L.zav1_deblock_plane(fake, 68, 68, 64, 8, 0, 0, 10, 10, 0, ctypes.byref(g), None) == zav1.ZAV1_EINVAL
-    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 0, None, fake, None,
+    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 0, None, 0, 0, None, fake, None,
                               None) == zav1.ZAV1_EINVAL
-    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 1, None, fake, fake,
+    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 1, None, 0, 0, None, fake, fake,
+                              None) == zav1.ZAV1_EINVAL
+    assert L.zav1_cdef_search(fake, None, None, fake, None, None, 64, 64, 8, 100, 1, fake, 8, 8, None, fake, None,
                               None) == zav1.ZAV1_EINVAL
     assert L.zav1_wiener_stats(fake, fake, 64, 64, 8, 6, 64, 0, fake, fake, None) == zav1.ZAV1_EINVAL
     assert L.zav1_wiener_stats(fake, fake, 64, 64, 8, 7, 512, 0, fake, fake, None) == zav1.ZAV1_EINVAL

@@ -64,10 +64,11 @@ def test_deblock_planes_bitexact(gpu, oracle, bd, skip, sharp):
                     plane, lv, lh, int((got != ref).sum()), bad.tolist()))
 
 
-@pytest.mark.parametrize('bd,sub,qindex', [(8, 1, 100), (10, 1, 200), (8, 2, 40), (10, 4, 255)])
-def test_cdef_search_bitexact(gpu, oracle, bd, sub, qindex):
+@pytest.mark.parametrize('bd,sub,qindex,skipf', [(8, 1, 100, 0.0), (10, 1, 200, 0.0), (8, 2, 40, 0.5), (10, 4, 255, 0.6)])
+def test_cdef_search_bitexact(gpu, oracle, bd, sub, qindex, skipf):
     z = gpu
     H, W = 136, 200  # 4 x 3 filter blocks, the last row / column partial
+    skip = synth.partition_geometry(H // 4, W // 4, 77, skip_frac=skipf)['skip'] if skipf else None
     sy, ry = blocky(H, W, bd, 70, 8)
     su, ru = blocky(H // 2, W // 2, bd, 71, 4)
     sv, rv = blocky(H // 2, W // 2, bd, 72, 4)
@@ -78,21 +79,29 @@ def test_cdef_search_bitexact(gpu, oracle, bd, sub, qindex):
     dv = D((nhfb * nvfb, 64, 2), np.int32)
     my = D((nhfb * nvfb, 64), np.uint64)
     muv = D((nhfb * nvfb, 64), np.uint64)
-    z.cdef_search(rec, src, W, H, bd, qindex, sub, dv, my, muv)
+    dsk = D.from_host(skip) if skip is not None else None
+    z.cdef_search(rec, src, W, H, bd, qindex, sub, dv, my, muv, d_skip=dsk, mi_cols=W // 4, mi_rows=H // 4)
     gy, guv, gdv = my.to_host(), muv.to_host(), dv.to_host()
+    n_skipped = 0
     fs = np.arange(64)
     for fbr in range(nvfb):
         for fbc in range(nhfb):
             fb = fbr * nhfb + fbc
-            ey, euv = oracle.cdef_search_fb((ry, ru, rv), (sy, su, sv), W, H, bd, qindex, sub, fbr, fbc, fs)
+            ey, euv = oracle.cdef_search_fb((ry, ru, rv), (sy, su, sv), W, H, bd, qindex, sub, fbr, fbc, fs, skip=skip)
             assert np.array_equal(gy[fb], ey), (fb, np.nonzero(gy[fb] != ey)[0][:8], gy[fb][:4], ey[:4])
             assert np.array_equal(guv[fb], euv), (fb, np.nonzero(guv[fb] != euv)[0][:8])
             vs, hs = min(64, H - fbr * 64), min(64, W - fbc * 64)
             for by in range(vs // 8):
                 for bx in range(hs // 8):
+                    mr, mc = fbr * 16 + by * 2, fbc * 16 + bx * 2
+                    if skip is not None and skip[mr:mr + 2, mc:mc + 2].all():
+                        n_skipped += 1
+                        assert tuple(gdv[fb, by * 8 + bx]) == (0, 0)
+                        continue
                     blk = ry[fbr * 64 + by * 8:fbr * 64 + by * 8 + 8, fbc * 64 + bx * 8:fbc * 64 + bx * 8 + 8]
                     d, v = oracle.cdef_find_dir(blk, bd - 8)
                     assert tuple(gdv[fb, by * 8 + bx]) == (d, v), (fb, by, bx)
+    assert (n_skipped > 0) == (skip is not None)
     # the table is not degenerate: some strength beats no filtering
     assert (gy.min(axis=1) < gy[:, 0]).any()
 
@@ -150,16 +159,17 @@ def test_loop_filter_stage_full_size_sampled(gpu, oracle):
     lfm = _mod('loopfilter')
     src_y, rec_y = synth.config3_recon(4096, 4096, seed=3000)
     (su, ru), (sv, rv) = synth.config3_chroma(2048, 2048, seed=3000)
-    geom = synth.partition_geometry(1024, 1024, 3001)
+    geom = synth.partition_geometry(1024, 1024, 3001, skip_frac=0.2)
+    deb = {k: v for k, v in geom.items() if k != 'skip'}
     qindex = synth.qp_to_qindex(32)
     levels = synth.key_frame_lf_levels(int(z.quant_row(qindex, 8).dequant[1]), 8)
     st = lfm.LoopFilterStage(z, {'src': (src_y, su, sv), 'rec': (rec_y, ru, rv)}, geom, 8, qindex, levels)
     st.run()
     z.sync()
     dy = [d.to_host() for d in st.dgd]
-    ref = [oracle.deblock_plane(rec_y, 8, 0, 0, levels[0], levels[1], 0, geom, 1024, 1024, 4096, 4096),
-           oracle.deblock_plane(ru, 8, 1, 1, levels[2], levels[2], 0, geom, 1024, 1024, 4096, 4096),
-           oracle.deblock_plane(rv, 8, 2, 1, levels[3], levels[3], 0, geom, 1024, 1024, 4096, 4096)]
+    ref = [oracle.deblock_plane(rec_y, 8, 0, 0, levels[0], levels[1], 0, deb, 1024, 1024, 4096, 4096),
+           oracle.deblock_plane(ru, 8, 1, 1, levels[2], levels[2], 0, deb, 1024, 1024, 4096, 4096),
+           oracle.deblock_plane(rv, 8, 2, 1, levels[3], levels[3], 0, deb, 1024, 1024, 4096, 4096)]
     for p in range(3):
         assert np.array_equal(dy[p], ref[p]), p
     gy, guv = st.mse_y.to_host(), st.mse_uv.to_host()
@@ -167,7 +177,7 @@ def test_loop_filter_stage_full_size_sampled(gpu, oracle):
     for fb in list(rng.integers(0, 4096, 6)) + [0, 4095]:
         fbr, fbc = divmod(int(fb), 64)
         ey, euv = oracle.cdef_search_fb(tuple(ref), (src_y, su, sv), 4096, 4096, 8, qindex, 1, fbr, fbc,
-                                        np.arange(64))
+                                        np.arange(64), skip=geom['skip'])
         assert np.array_equal(gy[fb], ey) and np.array_equal(guv[fb], euv), fb
     for p, (pw, ss, srcp) in enumerate([(4096, 0, src_y), (2048, 1, su), (2048, 1, sv)]):
         nh, nv = st.grid[p]

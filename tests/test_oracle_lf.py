@@ -102,3 +102,29 @@ def test_deblock_flat_unchanged_and_step_smoothed(oracle):
             L.or_lpf(q.ctypes.data, int(bd > 8), bd, 2 * 16 + 8, 16, 1, ln, 40, 0)
             assert q[2, 7] > (2 << (bd - 8)) and q[2, 8] < (6 << (bd - 8)), ln
             assert np.array_equal(q[6:], np.where(np.arange(16) >= 8, 6, 2)[None, :].repeat(10, 0) << (bd - 8))
+
+
+def test_cdef_search_skip_list_partitions_the_sse(oracle):
+    """The search list drops all-skip 8x8s (enc/deblock.rs:652): at 8 bits
+    (no per-plane shift) the rows of complementary skip maps sum to the
+    unskipped rows, and an all-skip map gives zero rows."""
+    rng = np.random.default_rng(5)
+    H, W = 72, 136
+    ry = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    sy = np.clip(ry.astype(int) + rng.integers(-9, 10, ry.shape), 0, 255).astype(np.uint8)
+    ru, rv = (rng.integers(0, 256, (H // 2, W // 2)).astype(np.uint8) for _ in range(2))
+    su, sv = (np.clip(p.astype(int) + rng.integers(-5, 6, p.shape), 0, 255).astype(np.uint8) for p in (ru, rv))
+    blk = rng.uniform(size=(H // 8, W // 8)) < 0.5
+    skip_a = np.repeat(np.repeat(blk, 2, 0), 2, 1).astype(np.uint8)
+    skip_b = (1 - skip_a).astype(np.uint8)
+    fs = np.arange(64)
+    for fbr in range(2):
+        for fbc in range(3):
+            args = ((ry, ru, rv), (sy, su, sv), W, H, 8, 120, 1, fbr, fbc, fs)
+            full = oracle.cdef_search_fb(*args)
+            a = oracle.cdef_search_fb(*args, skip=skip_a)
+            b = oracle.cdef_search_fb(*args, skip=skip_b)
+            none = oracle.cdef_search_fb(*args, skip=np.ones_like(skip_a))
+            assert np.array_equal(a[0] + b[0], full[0]) and np.array_equal(a[1] + b[1], full[1])
+            assert not none[0].any() and not none[1].any()
+            assert a[0].any() or b[0].any()

@@ -103,7 +103,7 @@ EXPORTS = {
     'zav1_intra_pred_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
     'zav1_cfl_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
     'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
-    'zav1_cdef_search': [_vp] * 6 + [_i32] * 5 + [_vp, _vp, _vp, _vp],
+    'zav1_cdef_search': [_vp] * 6 + [_i32] * 5 + [_vp, _i32, _i32, _vp, _vp, _vp, _vp],
     'zav1_rest_unit_grid': [_i32, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     'zav1_wiener_stats': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _vp],
     'zav1_unit_sse': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
@@ -284,12 +284,14 @@ def deblock_plane(d_plane, stride, plane_w, plane_h, bd, plane, ss, level_v, lev
                                    ctypes.byref(geom), stream), 'deblock_plane')
 
 
-def cdef_search(rec, src, width, height, bd, qindex, subsampling, d_dirvar, d_mse_y, d_mse_uv, stream=None):
-    """rec/src: (y, u, v) DeviceArrays (u/v may be None)."""
+def cdef_search(rec, src, width, height, bd, qindex, subsampling, d_dirvar, d_mse_y, d_mse_uv, stream=None,
+                d_skip=None, mi_cols=0, mi_rows=0):
+    """rec/src: (y, u, v) DeviceArrays (u/v may be None); d_skip: optional
+    per-mi signaled-skip map (u8, mi_rows x mi_cols)."""
     ptr = lambda a: a.ptr if a is not None else None  # noqa: E731
     check(lib().zav1_cdef_search(ptr(rec[0]), ptr(rec[1]), ptr(rec[2]), ptr(src[0]), ptr(src[1]), ptr(src[2]), width,
-                                 height, bd, qindex, subsampling, ptr(d_dirvar), d_mse_y.ptr, ptr(d_mse_uv), stream),
-          'cdef_search')
+                                 height, bd, qindex, subsampling, ptr(d_skip), mi_cols, mi_rows, ptr(d_dirvar),
+                                 d_mse_y.ptr, ptr(d_mse_uv), stream), 'cdef_search')
 
 
 def rest_unit_grid(unit_size, plane_w, plane_h):

@@ -527,14 +527,17 @@ int zav1_deblock_plane(void* d_plane, int stride, int plane_w, int plane_h, int 
 
 int zav1_cdef_search(const void* d_rec_y, const void* d_rec_u, const void* d_rec_v, const void* d_src_y,
                      const void* d_src_u, const void* d_src_v, int width, int height, int bd, int qindex,
-                     int subsampling, int32_t* d_dirvar, uint64_t* d_mse_y, uint64_t* d_mse_uv, zav1_stream_t s) {
+                     int subsampling, const uint8_t* d_skip, int mi_cols, int mi_rows, int32_t* d_dirvar,
+                     uint64_t* d_mse_y, uint64_t* d_mse_uv, zav1_stream_t s) {
   if ((bd != 8 && bd != 10) || width <= 0 || height <= 0 || qindex < 0 || qindex > 255 || subsampling < 1 ||
       !d_rec_y || !d_src_y || !d_mse_y)
     return ZAV1_EINVAL;
+  if (d_skip && (mi_cols < (width + 3) / 4 || mi_rows < (height + 3) / 4)) return ZAV1_EINVAL;
   if (d_mse_uv && (!d_rec_u || !d_rec_v || !d_src_u || !d_src_v || !d_dirvar)) return ZAV1_EINVAL;
   const int damping = 3 + (qindex >> 6);
   ZCHECK(zav1::cdef_search(d_rec_y, d_rec_u, d_rec_v, d_src_y, d_src_u, d_src_v, width, height, bd, damping,
-                           subsampling < 4 ? subsampling : 4, 1, d_dirvar, d_mse_y, d_mse_uv, S(s)));
+                           subsampling < 4 ? subsampling : 4, 1, d_skip, mi_cols, mi_rows, d_dirvar, d_mse_y,
+                           d_mse_uv, S(s)));
   return ZAV1_OK;
 }
 

@@ -397,16 +397,31 @@ __device__ __forceinline__ void cdef_pixel(const uint16_t* cp, int dir, int32_t 
 
 }  // namespace
 
+namespace {
+// is_8x8_all_skip (deblock.rs:652): every mi (up to 2 x 2) of the 8x8 at
+// luma mi (mr, mc) carries the signaled skip flag.  skip == nullptr: none.
+__device__ __forceinline__ bool all_skip_8x8(const uint8_t* skip, int mi_cols, int mi_rows, int mr, int mc) {
+  if (!skip) return false;
+  for (int r = mr; r < min(mr + 2, mi_rows); r++)
+    for (int c = mc; c < min(mc + 2, mi_cols); c++)
+      if (!skip[(long)r * mi_cols + c]) return false;
+  return true;
+}
+}  // namespace
+
 // One workgroup per 64x64 luma filter block: the 8x8 directions, then all
-// 64 (primary, secondary) strength pairs for every pixel; per-pair SSE
-// against the source reduced over the block.  dirvar (optional) receives
-// (dir, var) per 8x8 for the chroma pass.
+// 64 (primary, secondary) strength pairs for every pixel of the blocks in
+// the fb's search list (not all-skip); per-pair SSE against the source
+// reduced over the block.  dirvar (optional) receives (dir, var) per 8x8 for
+// the chroma pass (0, 0 for blocks outside the list).
 template <typename Pix>
 __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, const Pix* __restrict__ src,
                                                    int width, int height, int bd, int damping, int sub, int nhfb,
+                                                   const uint8_t* __restrict__ skip, int mi_cols, int mi_rows,
                                                    uint64_t* __restrict__ mse, int32_t* __restrict__ dirvar) {
   __shared__ uint16_t tile[70 * kTS];
   __shared__ int32_t sdir[64];
+  __shared__ uint8_t sact[64];
   __shared__ uint8_t sadj[64][16];
   __shared__ uint32_t red[4][64];
   const int fbr = blockIdx.x / nhfb, fbc = blockIdx.x - fbr * nhfb;
@@ -426,8 +441,10 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
   if (tid < 64) {
     const int by = tid >> 3, bx = tid & 7;
     int d = 0, v = 0;
-    if (by < nby && bx < nbx) d = find_dir(tile + (kVB + by * 8) * kTS + kHB + bx * 8, cs, &v);
+    const bool act = by < nby && bx < nbx && !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + by * 2, fbc * 16 + bx * 2);
+    if (act) d = find_dir(tile + (kVB + by * 8) * kTS + kHB + bx * 8, cs, &v);
     sdir[tid] = d;
+    sact[tid] = act;
     // adjusted primary strength of every P (at most 60 at 10 bits)
     for (int P = 0; P < 16; P++) sadj[tid][P] = (uint8_t)adjust_strength(P << cs, v);
     if (dirvar) {
@@ -443,9 +460,8 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
   for (int g = 0; g < 64; g++) acc[g] = 0;
   for (int p = tid; p < 4096; p += 256) {
     const int r = p >> 6, c = p & 63;
-    const int by = r >> 3, bx = c >> 3;
-    if (by >= nby || bx >= nbx || ((r & 7) % sub) != 0) continue;
-    const int blk = by * 8 + bx;
+    const int blk = (r >> 3) * 8 + (c >> 3);
+    if (!sact[blk] || ((r & 7) % sub) != 0) continue;
     const uint8_t* adj = sadj[blk];
     cdef_pixel(tile + (kVB + r) * kTS + kHB + c, sdir[blk], (int32_t)src[(long)(y0 + r) * width + x0 + c], cs, dmp,
                [adj](int P) { return (int32_t)adj[P]; }, acc);
@@ -466,11 +482,13 @@ template <typename Pix>
 __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec_u, const Pix* __restrict__ rec_v,
                                                      const Pix* __restrict__ src_u, const Pix* __restrict__ src_v,
                                                      int width, int height, int bd, int damping, int sub, int nhfb,
+                                                     const uint8_t* __restrict__ skip, int mi_cols, int mi_rows,
                                                      const int32_t* __restrict__ dirvar,
                                                      uint64_t* __restrict__ mse) {
   __shared__ uint16_t tile[38 * kTS];
   __shared__ uint32_t red[4][64];
   __shared__ uint64_t total[64];
+  __shared__ uint8_t sact[64];
   const int fbr = blockIdx.x / nhfb, fbc = blockIdx.x - fbr * nhfb;
   const int cs = bd - 8, cw = width >> 1, ch = height >> 1;
   const int y0 = fbr * 32, x0 = fbc * 32;
@@ -478,7 +496,10 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
   const int nr = vsize & ~3, nc = hsize & ~3;
   const int tid = threadIdx.x;
   const int dmp = damping + cs - 1;
-  if (tid < 64) total[tid] = 0;
+  if (tid < 64) {
+    total[tid] = 0;
+    sact[tid] = !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + (tid >> 3) * 2, fbc * 16 + (tid & 7) * 2);
+  }
   for (int pl = 0; pl < 2; pl++) {
     const Pix* rec = pl ? rec_v : rec_u;
     const Pix* srcp = pl ? src_v : src_u;
@@ -495,7 +516,7 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
     for (int g = 0; g < 64; g++) acc[g] = 0;
     for (int p = tid; p < 1024; p += 256) {
       const int r = p >> 5, c = p & 31;
-      if (r >= nr || c >= nc || ((r & 3) % sub) != 0) continue;
+      if (r >= nr || c >= nc || !sact[(r >> 2) * 8 + (c >> 2)] || ((r & 3) % sub) != 0) continue;
       const int dir = dirvar[((long)blockIdx.x * 64 + (r >> 2) * 8 + (c >> 2)) * 2];
       cdef_pixel(tile + (kVB + r) * kTS + kHB + c, dir, (int32_t)srcp[(long)(y0 + r) * cw + x0 + c], cs, dmp,
                  [cs](int P) { return (int32_t)(P << cs); }, acc);
@@ -843,23 +864,24 @@ hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plan
 
 hipError_t cdef_search(const void* rec_y, const void* rec_u, const void* rec_v, const void* src_y,
                        const void* src_u, const void* src_v, int width, int height, int bd, int damping, int sub_y,
-                       int sub_uv, int32_t* dirvar, uint64_t* mse_y, uint64_t* mse_uv, hipStream_t st) {
+                       int sub_uv, const uint8_t* skip, int mi_cols, int mi_rows, int32_t* dirvar, uint64_t* mse_y,
+                       uint64_t* mse_uv, hipStream_t st) {
   const int nhfb = (width + 63) / 64, nvfb = (height + 63) / 64;
   const unsigned gr = (unsigned)(nhfb * nvfb);
   if (bd == 8) {
     hipLaunchKernelGGL((k_cdef_luma<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)rec_y,
-                       (const uint8_t*)src_y, width, height, bd, damping, sub_y, nhfb, mse_y, dirvar);
+                       (const uint8_t*)src_y, width, height, bd, damping, sub_y, nhfb, skip, mi_cols, mi_rows, mse_y, dirvar);
     if (mse_uv)
       hipLaunchKernelGGL((k_cdef_chroma<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)rec_u,
                          (const uint8_t*)rec_v, (const uint8_t*)src_u, (const uint8_t*)src_v, width, height, bd,
-                         damping, sub_uv, nhfb, (const int32_t*)dirvar, mse_uv);
+                         damping, sub_uv, nhfb, skip, mi_cols, mi_rows, (const int32_t*)dirvar, mse_uv);
   } else {
     hipLaunchKernelGGL((k_cdef_luma<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)rec_y,
-                       (const uint16_t*)src_y, width, height, bd, damping, sub_y, nhfb, mse_y, dirvar);
+                       (const uint16_t*)src_y, width, height, bd, damping, sub_y, nhfb, skip, mi_cols, mi_rows, mse_y, dirvar);
     if (mse_uv)
       hipLaunchKernelGGL((k_cdef_chroma<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)rec_u,
                          (const uint16_t*)rec_v, (const uint16_t*)src_u, (const uint16_t*)src_v, width, height, bd,
-                         damping, sub_uv, nhfb, (const int32_t*)dirvar, mse_uv);
+                         damping, sub_uv, nhfb, skip, mi_cols, mi_rows, (const int32_t*)dirvar, mse_uv);
   }
   return hipGetLastError();
 }

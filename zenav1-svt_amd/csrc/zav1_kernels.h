@@ -86,9 +86,11 @@ hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plan
                          int sharpness, const DeblockGeomDev& g, hipStream_t st);
 // CDEF strength search over every 64x64 filter block: mse_y / mse_uv rows of
 // 64 strengths per block (raster of blocks); dirvar [fb][64][2] (dir, var).
+// skip: per-mi signaled skip (mi_rows x mi_cols) or nullptr.
 hipError_t cdef_search(const void* rec_y, const void* rec_u, const void* rec_v, const void* src_y,
                        const void* src_u, const void* src_v, int width, int height, int bd, int damping, int sub_y,
-                       int sub_uv, int32_t* dirvar, uint64_t* mse_y, uint64_t* mse_uv, hipStream_t st);
+                       int sub_uv, const uint8_t* skip, int mi_cols, int mi_rows, int32_t* dirvar, uint64_t* mse_y,
+                       uint64_t* mse_uv, hipStream_t st);
 // Restoration-unit statistics over an nh x nv unit grid (raster order).
 hipError_t wiener_stats(const void* dgd, const void* src, int pw, int ph, int bd, int win, int unit_size, int ss_y,
                         int nh, int nv, int64_t* M, int64_t* H, hipStream_t st);

@@ -16,7 +16,9 @@ class LoopFilterStage:
     """Holds source / reconstruction planes, geometry and every output in HBM.
 
     planes: dict with 'src' and 'rec' -> (y, u, v) numpy arrays (u8 at bd 8,
-    u16 at bd 10); geom: dict of mi arrays from synth.partition_geometry.
+    u16 at bd 10); geom: dict of mi arrays from synth.partition_geometry.  An
+    all-intra still has no inter-skip, so deblocking sees no skip map; the
+    geometry's 'skip' (signaled skip per mi) drives the CDEF search list.
     """
 
     def __init__(self, zav1, planes, geom, bd, qindex, levels, sharpness=0, cdef_sub=1, ru_luma=128,
@@ -36,8 +38,10 @@ class LoopFilterStage:
         self.rec = [D.from_host(p) for p in planes['rec']]
         self.dgd = [D(p.shape, p.dtype) for p in planes['rec']]
         mi_rows, mi_cols = geom['bw'].shape
-        self.geom_dev = {k: D.from_host(v) for k, v in geom.items()}
+        self.mi_rows, self.mi_cols = mi_rows, mi_cols
+        self.geom_dev = {k: D.from_host(v) for k, v in geom.items() if k != 'skip'}
         self.geom = zav1.deblock_geom(mi_cols, mi_rows, self.w, self.h, self.geom_dev)
+        self.cdef_skip = D.from_host(geom['skip']) if 'skip' in geom else None
         nfb = ((self.w + 63) // 64) * ((self.h + 63) // 64)
         self.nfb = nfb
         self.dirvar = D((nfb, 64, 2), np.int32)
@@ -79,7 +83,8 @@ class LoopFilterStage:
 
     def cdef(self, stream=None):
         self.z.cdef_search(self.dgd, self.src, self.w, self.h, self.bd, self.qindex, self.cdef_sub, self.dirvar,
-                           self.mse_y, self.mse_uv, stream)
+                           self.mse_y, self.mse_uv, stream, d_skip=self.cdef_skip, mi_cols=self.mi_cols,
+                           mi_rows=self.mi_rows)
 
     def wiener(self, stream=None):
         for p in range(3):
