@@ -23,8 +23,26 @@
  *   2D transforms       structure pinned by the 1D goldens; 2D composition
  *                       pinned by separability/flip invariants.
  *   quantizers          per-coefficient brute force of the quoted formulas,
- *                       dequant bounds; table rows by closed form.
+ *                       dequant bounds; table rows by closed form and a row
+ *                       captured from the reference's tests.
  *   hadamard / SATD     brute-force Walsh matrix on small-range input.
+ *   scans / eob         sha256 digests of the reference's C-extracted scan
+ *                       tables (all sizes x classes).
+ *   intra predictors    the reference's own known-answer cases, closed forms
+ *                       of the directional projections, constant-edge and
+ *                       availability fallbacks, spec edge-filter tables,
+ *                       CfL identities; tables generated from the reference's
+ *                       data files.
+ *   deblocking          thresholds closed form; flat lines unchanged, steps
+ *                       smoothed; p/q mirror symmetry of every filter.
+ *   CDEF                the reference's constrain / adjust_strength known
+ *                       values, find_dir spec semantics, direction table vs
+ *                       the spec, flat-block identity, skip-list additivity.
+ *   Wiener / unit SSE   numpy covariance of the windowed samples; unit SSE
+ *                       and unit limits against the reference's C captures
+ *                       (tests/golden/lr).
+ *   self-guided stats   PARITY UNPINNED: no reference twin; checked only by
+ *                       invariants (flat input, projection identities).
  */
 #ifndef ZAV1_ORACLE_H
 #define ZAV1_ORACLE_H

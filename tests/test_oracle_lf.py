@@ -128,3 +128,36 @@ def test_cdef_search_skip_list_partitions_the_sse(oracle):
             assert np.array_equal(a[0] + b[0], full[0]) and np.array_equal(a[1] + b[1], full[1])
             assert not none[0].any() and not none[1].any()
             assert a[0].any() or b[0].any()
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_deblock_lines_mirror_symmetric(oracle, bd):
+    """Every AV1 deblocking filter treats the p and q sides alike: filtering a
+    reversed line gives the reversed result.  Exact for the 6/8/14-tap
+    smoothing paths and filter4's inner taps (its rounding pair (f+4)>>3 /
+    (f+3)>>3 is antisymmetric away from the clamps); filter4's outer-tap
+    step ROUND_POWER_OF_TWO(f1, 1) rounds halves up, so p1 / q1 may differ
+    by one.  A swapped tap weight, side or index breaks this."""
+    L = oracle.lflib()
+    rng = np.random.default_rng(9 + bd)
+    dt = np.uint8 if bd == 8 else np.uint16
+    sh = bd - 8
+    hits = 0
+    for ln in (4, 6, 8, 14):
+        for trial in range(400):
+            base = int(rng.integers(40, 200)) << sh
+            step = int(rng.integers(-12, 13)) << sh
+            noise = rng.integers(-2, 3, ln) << sh if trial % 2 else np.zeros(ln, int)
+            line = np.array([base + (step if k >= ln // 2 else 0) for k in range(ln)]) + noise
+            buf = np.zeros((4, 16), dt)
+            buf[:, 8 - ln // 2:8 + ln // 2] = line
+            rev = buf[:, ::-1].copy()  # edge now between columns 7 and 8 of the mirrored row
+            level = int(rng.integers(8, 64))
+            a, b = buf.copy(), rev.copy()
+            L.or_lpf(a.ctypes.data, int(bd > 8), bd, 8, 16, 1, ln, level, 0)
+            L.or_lpf(b.ctypes.data, int(bd > 8), bd, 8, 16, 1, ln, level, 0)
+            diff = np.abs(a[:, ::-1].astype(int) - b.astype(int))
+            diff[:, [6, 9]] = np.maximum(diff[:, [6, 9]] - 1, 0)  # filter4 outer taps: +-1
+            assert not diff.any(), (ln, line.tolist(), level)
+            hits += int(not np.array_equal(a, buf))
+    assert hits > 200
