@@ -48,24 +48,32 @@ def _load(name, path):
     return m
 
 
-def candidate_ops(ts_mod, tx_size, tx_type, opsd):
-    """Algorithmic integer ops of one fused candidate, from the straight-line
-    1D transform op counts the generator records."""
+COL_KIND = [0, 1, 0, 1, 2, 0, 2, 1, 2, 3, 0, 3, 1, 3, 2, 3]
+ROW_KIND = [0, 0, 1, 1, 0, 2, 2, 2, 1, 3, 3, 0, 3, 1, 3, 2]
+UD_FLIP = {4, 6, 8, 14}
+
+
+def block_ops(ts_mod, tx_size, types, opsd):
+    """Algorithmic integer ops of evaluating one block with every type in
+    `types`: the DC prediction and residual once, each distinct column
+    transform (kind, up-down flip) once, every type's row transforms, round
+    shifts, fp quantize, distortion, eob and level sum.  1D transform counts
+    are the straight-line op counts the generator records."""
     w, h = ts_mod.TX_DIMS[tx_size]
-    col = [0, 1, 0, 1, 2, 0, 2, 1, 2, 3, 0, 3, 1, 3, 2, 3][tx_type]
-    row = [0, 0, 1, 1, 0, 2, 2, 2, 1, 3, 3, 0, 3, 1, 3, 2][tx_type]
 
     def one(kind, n):
         return opsd[('dct' if kind == 0 else ('identity' if kind == 3 else 'adst')) + str(n)]
     n = w * h
     rect = 3 if (w == 2 * h or h == 2 * w) else 0
-    ops = (w + h)                 # DC prediction sums
-    ops += n * 2                  # residual + input shift
-    ops += w * one(col, h) + h * one(row, w)
-    ops += n * (2 + 2 + rect)     # two round shifts (+ sqrt2 scale)
-    ops += n * 12                 # fp quantize + dequant (abs, compare, add, mul, shift, sign x2)
-    ops += n * 3                  # distortion (sub, mul, add)
-    ops += n * 3                  # eob / level sum
+    ops = (w + h) + n * 2                       # DC prediction sums; residual + input shift
+    variants = {(1 if COL_KIND[t] == 2 else COL_KIND[t], t in UD_FLIP) for t in types}
+    ops += sum(w * one(k, h) + n * 2 for k, _ in variants)   # column passes + round shift
+    for t in types:
+        ops += h * one(ROW_KIND[t], w)
+        ops += n * (2 + rect)                   # round shift (+ sqrt2 scale)
+        ops += n * 12                           # fp quantize + dequant (abs, compare, add, mul, shift, sign x2)
+        ops += n * 3                            # distortion (sub, mul, add)
+        ops += n * 3                            # eob / level sum
     return ops
 
 
@@ -254,7 +262,7 @@ def main():
     nblk = (S // w) * (S // h)
     launches = args.steps * len(qps)
     avg_launch_s = per_size_ms[dom] / 1000.0 / launches
-    ops_launch = nblk * sum(candidate_ops(tsm, dom, t, opsd) for t in types)
+    ops_launch = nblk * block_ops(tsm, dom, types, opsd)
     bytes_launch = nblk * (w * h * 2 + (w + h) * 2 + len(types) * 16)
     achieved_tops = ops_launch / avg_launch_s / 1e12
 

@@ -17,15 +17,6 @@
 
 namespace zav1 {
 
-template <int W, int H>
-struct TsGeom {
-  static constexpr int T = W > H ? W : H;
-  static constexpr int JPW = 256 / T;
-  static constexpr int LS = W + 1;
-  static constexpr int BLK = H * LS;
-  static constexpr int PW = W < 32 ? W : 32, PH = H < 32 ? H : 32;
-  static constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
-};
 
 // log_scale per TxSize (rust/crates/svtav1-encoder/src/quant.rs:238).
 __host__ __device__ constexpr int tx_log_scale(int w, int h) {
@@ -51,31 +42,48 @@ __device__ __forceinline__ int32_t group_max_i32(int32_t v) {
   return v;
 }
 
-template <int W, int H>
+// Fused search, one lane group per block (all candidate types of the block):
+//   DC prediction from the source neighbours (group reduction), then the
+//   distinct column transforms of the type list (at most 4: DCT, ADST,
+//   FLIPADST, identity) into LDS, items (variant, column) spread over the
+//   group's lanes; then the row pass of every type, items (type, row) spread
+//   over the P lanes (P / H types at once), each lane quantizing and measuring
+//   its row; per-type group reductions give dist / eob / level sum.
+// Types are processed in the plan's order (grouped by row kind, so the T/H
+// types of one round share their row transform); outputs go to the caller's
+// order via plan.out_k.
+template <int W, int H, int P>
+struct BlkGeom {
+  static constexpr int T = P;        // lanes per block (>= max(W, H), divides 64)
+  static constexpr int LS = W + 1;   // padded LDS row: conflict-free column writes
+  static constexpr int VS = H * LS;  // one column variant of one block
+  static constexpr int PW = W < 32 ? W : 32;
+  static constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
+};
+
+template <int W, int H, int P>
 __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ plane, int stride, int bd,
-                                                  const int32_t* __restrict__ xy, int n_blocks,
-                                                  TypeList types, const QuantRow* __restrict__ row_p,
+                                                  const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
+                                                  const QuantRow* __restrict__ row_p,
                                                   const uint16_t* __restrict__ iscan_all, int tx_size,
                                                   uint64_t* __restrict__ out_dist,
                                                   int32_t* __restrict__ out_eob,
                                                   uint32_t* __restrict__ out_lsum) {
-  using G = TsGeom<W, H>;
+  using G = BlkGeom<W, H, P>;
+  static_assert(P >= W && P >= H && 64 % P == 0, "lanes per block");
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr int LSC = tx_log_scale(W, H);
-  __shared__ int32_t lds[G::JPW * G::BLK];
+  extern __shared__ int32_t lds[];
 
-  const int jl = threadIdx.x / G::T, t = threadIdx.x - jl * G::T;
-  const long pair = (long)blockIdx.x * G::JPW + jl;
-  const long n_pairs = (long)n_blocks * types.n;
-  const bool active = pair < n_pairs;
-  const int blk_i = active ? (int)(pair / types.n) : 0;
-  const int k = active ? (int)(pair - (long)blk_i * types.n) : 0;
-  const int tx_type = types.t[k];
-  const TypeCfg cfg = type_cfg(tx_type);
-  const int x0 = xy[2 * blk_i], y0 = xy[2 * blk_i + 1];
+  const int g = threadIdx.x / G::T, t = threadIdx.x - g * G::T;
+  const int nvar = plan.n_var, ntyp = plan.n_types;
+  const long b = (long)blockIdx.x * (blockDim.x / G::T) + g;
+  const bool active = b < n_blocks;
+  const int bi = active ? (int)b : 0;
+  const int x0 = xy[2 * bi], y0 = xy[2 * bi + 1];
 
-  // ---- DC prediction from the source neighbours (group reduction) ----
+  // ---- DC prediction from the source neighbours ----
   uint32_t s_above = 0, s_left = 0;
   if (active && y0 > 0 && t < W) s_above = plane[(long)(y0 - 1) * stride + x0 + t];
   if (active && x0 > 0 && t < H) s_left = plane[(long)(y0 + t) * stride + x0 - 1];
@@ -87,91 +95,151 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   else if (x0 > 0) dc = (int32_t)((s_left + H / 2) / H);
   else dc = 1 << (bd - 1);
 
-  int32_t* blk = lds + jl * G::BLK;
-  int32_t v[G::T];
-  const bool do_col = active && t < W;
-  if (do_col) {
+  // ---- column variants -> LDS ----
+  int32_t* L = lds + g * nvar * G::VS;
+  for (int j = t; j < nvar * W; j += G::T) {
+    const int v = j / W, c = j - v * W;
+    const int ud = plan.var_ud[v];
+    int32_t col[H];
 #pragma unroll
     for (int r = 0; r < H; r++) {
-      const int rr = cfg.ud ? (H - 1 - r) : r;
-      v[r] = ((int32_t)plane[(long)(y0 + rr) * stride + x0 + t] - dc) << SH.s0;
+      const int rr = ud ? (H - 1 - r) : r;
+      col[r] = active ? ((int32_t)plane[(long)(y0 + rr) * stride + x0 + c] - dc) << SH.s0 : 0;
     }
-    fwd1d<H, CBC>(v, cfg.col);
+    fwd1d<H, CBC>(col, plan.var_kind[v]);
 #pragma unroll
-    for (int r = 0; r < H; r++) v[r] = rshift_round(v[r], SH.s1);
-    const int cc = cfg.lr ? (W - 1 - t) : t;
-#pragma unroll
-    for (int r = 0; r < H; r++) blk[r * G::LS + cc] = v[r];
+    for (int r = 0; r < H; r++) L[v * G::VS + r * G::LS + c] = rshift_round(col[r], SH.s1);
   }
   __syncthreads();
 
-  uint64_t dist = 0;
-  int32_t eob = 0, lsum = 0;
-  if (active && t < H) {
+  // ---- row passes: items (type, row) ----
+  constexpr int TPR = G::T / H;
+  const int sub = t / H, r = t - sub * H;
+  const QuantRow& qr = *row_p;
+  const int32_t rnd0 = (qr.round_fp[0] + ((1 << LSC) >> 1)) >> LSC;
+  const int32_t rnd1 = (qr.round_fp[1] + ((1 << LSC) >> 1)) >> LSC;
+  for (int base = 0; base < ntyp; base += TPR) {
+    const int ti = base + sub;
+    const bool live = active && ti < ntyp;
+    uint64_t dist = 0;
+    int32_t eob = 0, lsum = 0;
+    if (live) {
+      const int tx_type = plan.type[ti];
+      const TypeCfg cfg = type_cfg(tx_type);
+      const int32_t* src = L + plan.var_of[ti] * G::VS + r * G::LS;
+      int32_t v[W];
 #pragma unroll
-    for (int c = 0; c < W; c++) v[c] = blk[t * G::LS + c];
-    fwd1d<W, CBR>(v, cfg.row);
-    const QuantRow& qr = *row_p;
-    const int cls = (tx_type >= 10) ? (((tx_type - 10) & 1) ? 2 : 1) : 0;
-    const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024;
+      for (int c = 0; c < W; c++) v[c] = cfg.lr ? src[W - 1 - c] : src[c];
+      fwd1d<W, CBR>(v, cfg.row);
+      const int cls = (tx_type >= 10) ? (((tx_type - 10) & 1) ? 2 : 1) : 0;
+      const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + r * G::PW;
 #pragma unroll
-    for (int c = 0; c < W; c++) {
-      int32_t x = rshift_round(v[c], SH.s2);
-      if constexpr (G::RECT2) x = rshift64r((int64_t)x * 5793, 12);
-      if (t >= 32 || c >= 32) {
-        dist += (uint64_t)((int64_t)x * x);  // 64-point fold: discarded energy
-        continue;
+      for (int c = 0; c < W; c++) {
+        int32_t x = rshift_round(v[c], SH.s2);
+        if constexpr (G::RECT2) x = rshift64r((int64_t)x * 5793, 12);
+        if (r >= 32 || c >= 32) {
+          dist += (uint64_t)((int64_t)x * x);  // 64-point fold: discarded energy
+          continue;
+        }
+        const int iz = (r | c) != 0;
+        const int32_t sgn = x >> 31;
+        const int32_t a = (x ^ sgn) - sgn;
+        int32_t q = 0;
+        if (((int64_t)a << (1 + LSC)) >= qr.dequant[iz]) {
+          int64_t xx = (int64_t)a + (iz ? rnd1 : rnd0);
+          if (bd == 8) xx = xx > 32767 ? 32767 : xx;
+          q = (int32_t)((xx * qr.quant_fp[iz]) >> (16 - LSC));
+        }
+        int32_t dq = 0;
+        if (q != 0) {
+          dq = (int32_t)(((int64_t)q * qr.dequant[iz]) >> LSC);
+          lsum += q;
+          eob = max(eob, (int32_t)iscan[c] + 1);
+          dq = (dq ^ sgn) - sgn;
+        }
+        const int64_t e = (int64_t)x - dq;
+        dist += (uint64_t)e * (uint64_t)e;
       }
-      const int iz = (t | c) != 0;
-      const int32_t sgn = x >> 31;
-      const int32_t a = (x ^ sgn) - sgn;
-      int32_t q = 0;
-      const int32_t rounding = (qr.round_fp[iz] + ((1 << LSC) >> 1)) >> LSC;
-      if (((int64_t)a << (1 + LSC)) >= qr.dequant[iz]) {
-        int64_t xx = (int64_t)a + rounding;
-        if (bd == 8) xx = xx > 32767 ? 32767 : xx;
-        q = (int32_t)((xx * qr.quant_fp[iz]) >> (16 - LSC));
-      }
-      int32_t dq = 0;
-      if (q != 0) {
-        dq = (int32_t)(((int64_t)q * qr.dequant[iz]) >> LSC);
-        lsum += q;
-        eob = max(eob, (int32_t)iscan[t * G::PW + c] + 1);
-        dq = (dq ^ sgn) - sgn;
-      }
-      const int64_t e = (int64_t)x - dq;
-      dist += (uint64_t)e * (uint64_t)e;
     }
-  }
-  dist = group_sum_u64<G::T>(dist);
-  eob = group_max_i32<G::T>(eob);
-  lsum = group_sum_i32<G::T>(lsum);
-  if (active && t == 0) {
-    constexpr int SHF = (1 - LSC) * 2;
-    out_dist[pair] = SHF < 0 ? (dist << (-SHF)) : (dist >> SHF);
-    out_eob[pair] = eob;
-    out_lsum[pair] = (uint32_t)lsum;
+    dist = group_sum_u64<H>(dist);
+    eob = group_max_i32<H>(eob);
+    lsum = group_sum_i32<H>(lsum);
+    if (live && r == 0) {
+      constexpr int SHF = (1 - LSC) * 2;
+      const long o = (long)bi * ntyp + plan.out_k[ti];
+      out_dist[o] = SHF < 0 ? (dist << (-SHF)) : (dist >> SHF);
+      out_eob[o] = eob;
+      out_lsum[o] = (uint32_t)lsum;
+    }
   }
 }
 
+// Lanes per block, picked per size by measurement on MI355X (config-2
+// sweep): more lanes than max(W, H) only where the extra type slots of the
+// row pass beat the per-block overhead they add.
+constexpr int ts_lanes(int w, int h) {
+  return (w == 4 && h == 4) ? 8
+       : (w == 8 && h == 8) ? 32
+       : (w == 16 && h == 16) ? 64
+       : ((w == 8 && h == 16) || (w == 16 && h == 8)) ? 32
+       : (w > h ? w : h);
+}
 template <int S>
 static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int32_t* xy, int n_blocks,
-                            const TypeList& types, const QuantRow* row, const uint16_t* iscan,
+                            const TypeList& types, const TxPlan& plan, const QuantRow* row, const uint16_t* iscan,
                             uint64_t* d, int32_t* e, uint32_t* l, hipStream_t st) {
   constexpr int W = tx_w(S), H = tx_h(S);
-  const long pairs = (long)n_blocks * types.n;
-  const int grid = (int)((pairs + TsGeom<W, H>::JPW - 1) / TsGeom<W, H>::JPW);
-  hipLaunchKernelGGL((k_txsearch<W, H>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy, n_blocks,
-                     types, row, iscan, S, d, e, l);
+  constexpr int P = ts_lanes(W, H);
+  using G = BlkGeom<W, H, P>;
+  // groups per workgroup: up to 256 threads and 64 KB of column variants
+  const int per_group = plan.n_var * G::VS * (int)sizeof(int32_t);
+  int groups = 256 / G::T;
+  while (groups > 1 && groups * per_group > 65536) groups >>= 1;
+  const int grid = (n_blocks + groups - 1) / groups;
+  hipLaunchKernelGGL((k_txsearch<W, H, P>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane, stride,
+                     bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
+  (void)types;
   return hipGetLastError();
+}
+
+// Processing plan of a type list: types grouped by row kind (ADST and
+// FLIPADST rows share a kernel), distinct column variants (kind, up-down
+// flip) numbered in first-use order.
+static TxPlan make_plan(const TypeList& types) {
+  TxPlan p{};
+  p.n_types = types.n;
+  int order[16], n = 0;
+  for (int key = 0; key < 3; key++)
+    for (int i = 0; i < types.n; i++) {
+      const int rk = type_cfg(types.t[i]).row;
+      const int k2 = rk == 0 ? 0 : (rk == 3 ? 2 : 1);
+      if (k2 == key) order[n++] = i;
+    }
+  for (int i = 0; i < n; i++) {
+    const int k = order[i];
+    const TypeCfg c = type_cfg(types.t[k]);
+    const int kind = c.col == 2 ? 1 : c.col;
+    int v = 0;
+    while (v < p.n_var && !(p.var_kind[v] == kind && p.var_ud[v] == c.ud)) v++;
+    if (v == p.n_var) {
+      p.var_kind[v] = (uint8_t)kind;
+      p.var_ud[v] = (uint8_t)c.ud;
+      p.n_var++;
+    }
+    p.type[i] = types.t[k];
+    p.out_k[i] = (uint8_t)k;
+    p.var_of[i] = (uint8_t)v;
+  }
+  return p;
 }
 
 hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
                           int n_blocks, const TypeList& types, const QuantRow* row, const uint16_t* iscan,
                           uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st) {
   if (n_blocks <= 0 || types.n <= 0) return hipSuccess;
+  const TxPlan plan = make_plan(types);
   switch (tx_size) {
-#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, types, row, iscan, dist, eob, lsum, st);
+#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, types, plan, row, iscan, dist, eob, lsum, st);
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 #undef X
     default: return hipErrorInvalidValue;

@@ -46,6 +46,14 @@ struct TypeList {
   int32_t n;
   uint8_t t[16];
 };
+// Processing plan of a candidate type list (built on the host): types in
+// processing order with their index in the caller's list, and the distinct
+// column transforms (kind 0 DCT / 1 ADST / 3 identity, up-down flip).
+struct TxPlan {
+  int32_t n_types, n_var;
+  uint8_t type[16], out_k[16], var_of[16];
+  uint8_t var_kind[4], var_ud[4];
+};
 // Fused tx-type search over blocks of one tx size on a 16-bit plane (8-bit
 // content is widened by the caller).  iscan: device table of inverse scans,
 // [tx_size][class][1024] u16.  Outputs per (block, type) pair.
