@@ -377,11 +377,11 @@ __device__ __forceinline__ void cdef_pixel(const uint16_t* cp, int dir, int32_t 
   // P = 0 filters along direction 0 with no primary taps; (0, 0) is the
   // unfiltered block
   const int32_t e0 = s_v - x;
-  acc[0] += (uint32_t)(e0 * e0);
+  acc[0] += (uint32_t)__mul24(e0, e0);
 #pragma unroll
   for (int S = 1; S < 4; S++) {
     const int32_t e = s_v - cdef_out(x, sec0[S], t0.mn, t0.mx);
-    acc[S] += (uint32_t)(e * e);
+    acc[S] += (uint32_t)__mul24(e, e);
   }
 #pragma unroll
   for (int P = 1; P < 16; P++) {
@@ -390,7 +390,7 @@ __device__ __forceinline__ void cdef_pixel(const uint16_t* cp, int dir, int32_t 
 #pragma unroll
     for (int S = 0; S < 4; S++) {
       const int32_t e = s_v - cdef_out(x, pri + secb[S], tb.mn, tb.mx);
-      acc[P * 4 + S] += (uint32_t)(e * e);
+      acc[P * 4 + S] += (uint32_t)__mul24(e, e);
     }
   }
 }
@@ -566,100 +566,125 @@ __device__ __forceinline__ uint64_t wg_sum_u64(uint64_t v, uint64_t* red) {
 
 // Wiener statistics (compute_stats / compute_stats_hbd): one workgroup per
 // unit.  Rows are staged in strips of SR (+ 2*HALF replicated halo rows) as
-// mean-removed i16 samples; each lane owns up to CH consecutive entries of
-// one upper-triangle row of H (or of M), sums one pixel row of products in
-// i32 (at most 384 products of |v| <= 1023) and flushes it into i64.
+// mean-removed i16 samples.  H is split into segments (k, g): row k of H
+// against the WIN entries l = WIN*g .. WIN*g + WIN-1 of window column g (the
+// segments with g >= k / WIN cover the upper triangle); M adds WIN segments
+// with the source sample in place of y_k.  A lane owns one segment and walks
+// the pixel columns of its column range top to bottom, keeping the WIN
+// samples of window column g in a sliding register window: per pixel it
+// reads y_k and one new sample and does WIN multiply-adds.  A column's
+// products are summed in i32 (at most 16 products of |v| <= 1023) and then
+// added into i64, which reproduces the reference's i64 sums exactly.  With
+// the 5-tap window the 80 segments run in 3 column groups whose sums are
+// combined at the end.
 template <typename Pix, int WIN>
 __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
                                                       int pw, int ph, int bd, int unit_size, int ss_y, int nh,
                                                       int64_t* __restrict__ out_m, int64_t* __restrict__ out_h) {
-  constexpr int W2 = WIN * WIN, HALF = WIN / 2, CH = WIN == 7 ? 7 : 2, SR = 16;
-  __shared__ int16_t Y[(SR + 2 * HALF) * (kMaxUW + 2 * HALF)];
-  __shared__ int16_t X[SR * kMaxUW];
+  constexpr int W2 = WIN * WIN, HALF = WIN / 2, SR = 16;
+  constexpr int NSEG = WIN * (WIN * (WIN + 1) / 2) + WIN;
+  constexpr int NG = 256 / NSEG;
+  constexpr int YS = kMaxUW + 2 * HALF;  // LDS row stride of Y and X (constant: immediate offsets)
+  __shared__ int16_t Y[(SR + 2 * HALF) * YS];
+  __shared__ int16_t X[SR * YS];
   __shared__ uint64_t red[4];
+  __shared__ int64_t gpart[NG > 1 ? (NG - 1) * NSEG * WIN : 1];
   const int u = blockIdx.x, uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
   const int uw = he - hs, uh = ve - vs, tw = uw + 2 * HALF;
   const int tid = threadIdx.x;
+  const int lr0 = tid >> 6, lc0 = tid & 63;  // 4 rows x 64 columns per pass, no divisions
   uint64_t s = 0;
-  for (int i = tid; i < uw * uh; i += 256) {
-    const int r = i / uw, c = i - r * uw;
-    s += dgd[(long)(vs + r) * pw + hs + c];
-  }
+  for (int r = lr0; r < uh; r += 4)
+    for (int c = lc0; c < uw; c += 64) s += dgd[(long)(vs + r) * pw + hs + c];
   const int32_t avg = (int32_t)(wg_sum_u64(s, red) / ((uint64_t)uw * uh));
 
-  // segment of this lane: row k (k == W2 selects M), entries l0 .. l0+n-1
-  int my_k = -1, my_l0 = 0, my_n = 0;
+  const int seg = tid % NSEG, cg = tid / NSEG;
+  const bool active = cg < NG;
+  int my_k = 0, my_g = 0;
   {
-    int seg = 0;
-    for (int k = 0; k <= W2 && my_k < 0; k++)
-      for (int l0 = (k == W2 ? 0 : k); l0 < W2; l0 += CH) {
-        if (seg == tid) {
-          my_k = k;
-          my_l0 = l0;
-          my_n = min(CH, W2 - l0);
-          break;
-        }
-        seg++;
-      }
+    int r = seg, k = 0;
+    for (; k < W2; k++) {
+      const int ng = WIN - k / WIN;
+      if (r < ng) break;
+      r -= ng;
+    }
+    my_k = k;
+    my_g = k < W2 ? k / WIN + r : r;
   }
-  const bool active = my_k >= 0;
   const bool is_m = my_k == W2;
-  int offl[CH];
+  const int dyk = is_m ? 0 : my_k % WIN, dxk = is_m ? 0 : my_k / WIN;
+  const int c_lo = cg * uw / NG, c_hi = (cg + 1) * uw / NG;
+  int64_t acc[WIN];
 #pragma unroll
-  for (int q = 0; q < CH; q++) {
-    const int l = my_l0 + (q < my_n ? q : 0);
-    offl[q] = (l % WIN) * tw + (l / WIN);
-  }
-  const int offk = (active && !is_m) ? (my_k % WIN) * tw + (my_k / WIN) : 0;
-  int64_t acc[CH];
-#pragma unroll
-  for (int q = 0; q < CH; q++) acc[q] = 0;
+  for (int q = 0; q < WIN; q++) acc[q] = 0;
 
   for (int s0 = 0; s0 < uh; s0 += SR) {
     const int sr = min(SR, uh - s0);
     __syncthreads();
-    for (int i = tid; i < (sr + 2 * HALF) * tw; i += 256) {
-      const int r = i / tw, c = i - r * tw;
-      const int gy = clampi(vs + s0 + r - HALF, 0, ph - 1), gx = clampi(hs + c - HALF, 0, pw - 1);
-      Y[i] = (int16_t)((int32_t)dgd[(long)gy * pw + gx] - avg);
+    for (int r = lr0; r < sr + 2 * HALF; r += 4) {
+      const int gy = clampi(vs + s0 + r - HALF, 0, ph - 1);
+      for (int c = lc0; c < tw; c += 64) {
+        const int gx = clampi(hs + c - HALF, 0, pw - 1);
+        Y[r * YS + c] = (int16_t)((int32_t)dgd[(long)gy * pw + gx] - avg);
+      }
     }
-    for (int i = tid; i < sr * uw; i += 256) {
-      const int r = i / uw, c = i - r * uw;
-      X[i] = (int16_t)((int32_t)src[(long)(vs + s0 + r) * pw + hs + c] - avg);
-    }
+    for (int r = lr0; r < sr; r += 4)
+      for (int c = lc0; c < uw; c += 64)
+        X[r * YS + c] = (int16_t)((int32_t)src[(long)(vs + s0 + r) * pw + hs + c] - avg);
     __syncthreads();
     if (active) {
-      for (int r = 0; r < sr; r++) {
-        int32_t row[CH];
+      for (int c = c_lo; c < c_hi; c++) {
+        const int16_t* ycol = Y + c + my_g;  // window column g, row R at ycol[R * YS]
+        const int16_t* kcol = is_m ? X + c : Y + dyk * YS + c + dxk;
+        int32_t yl[WIN], part[WIN];
 #pragma unroll
-        for (int q = 0; q < CH; q++) row[q] = 0;
-        const int16_t* yr = Y + r * tw;
-        const int16_t* av = is_m ? X + r * uw : yr + offk;
-        for (int c = 0; c < uw; c++) {
-          const int32_t a = av[c];
+        for (int q = 0; q < WIN; q++) part[q] = 0;
 #pragma unroll
-          for (int q = 0; q < CH; q++) row[q] += a * (int32_t)yr[offl[q] + c];
+        for (int q = 0; q + 1 < WIN; q++) yl[q] = ycol[q * YS];
+        auto row = [&](int r) {
+          yl[WIN - 1] = ycol[(r + WIN - 1) * YS];
+          const int32_t a = kcol[r * YS];
+#pragma unroll
+          for (int q = 0; q < WIN; q++) part[q] += __mul24(a, yl[q]);  // |a|, |y| <= 1023
+#pragma unroll
+          for (int q = 0; q + 1 < WIN; q++) yl[q] = yl[q + 1];
+        };
+        if (sr == SR) {
+#pragma unroll
+          for (int r = 0; r < SR; r++) row(r);
+        } else {
+          for (int r = 0; r < sr; r++) row(r);
         }
 #pragma unroll
-        for (int q = 0; q < CH; q++) acc[q] += row[q];
+        for (int q = 0; q < WIN; q++) acc[q] += part[q];
       }
     }
   }
-  if (!active) return;
+  if constexpr (NG > 1) {
+    if (active && cg > 0) {
+#pragma unroll
+      for (int q = 0; q < WIN; q++) gpart[((cg - 1) * NSEG + seg) * WIN + q] = acc[q];
+    }
+    __syncthreads();
+    if (cg == 0) {
+#pragma unroll
+      for (int q = 0; q < WIN; q++)
+        for (int g = 1; g < NG; g++) acc[q] += gpart[((g - 1) * NSEG + seg) * WIN + q];
+    }
+  }
+  if (!active || cg != 0) return;
   const int64_t div = bd == 10 ? 4 : 1;
 #pragma unroll
-  for (int q = 0; q < CH; q++) {
-    if (q < my_n) {
-      const int l = my_l0 + q;
-      const int64_t v = acc[q] / div;
-      if (is_m) {
-        out_m[(long)u * W2 + l] = v;
-      } else {
-        out_h[(long)u * W2 * W2 + my_k * W2 + l] = v;
-        out_h[(long)u * W2 * W2 + l * W2 + my_k] = v;
-      }
+  for (int q = 0; q < WIN; q++) {
+    const int l = my_g * WIN + q;
+    const int64_t v = acc[q] / div;
+    if (is_m) {
+      out_m[(long)u * W2 + l] = v;
+    } else if (l >= my_k) {
+      out_h[(long)u * W2 * W2 + my_k * W2 + l] = v;
+      out_h[(long)u * W2 * W2 + l * W2 + my_k] = v;
     }
   }
 }
@@ -698,127 +723,183 @@ __device__ __forceinline__ int x_by_xplus1(int z) {
 }
 }  // namespace
 
-// Self-guided projection statistics: one workgroup per (unit, parameter
-// set).  Strips of SR output rows: stage the dgd rows they need (radius-2
-// box + 1 row of A/B halo = 3 rows / columns of replicated border), compute
-// A/B of both passes at rows/cols -1 .. n of the strip, then the two
-// filtered outputs per pixel and the five projection sums.
+// Self-guided projection statistics.  One workgroup per (unit, group of
+// kSgSets parameter sets).  The unit is walked in tiles of kSgSR output rows
+// x kSgTW output columns: the edge-replicated dgd samples of a tile (3-sample
+// border) are staged in LDS, the radius-1 and radius-2 box sums (value and
+// square) of the tile's A/B positions (rows / columns -1 .. n) are computed
+// once and shared by the group's sets, then per set: A/B of both passes
+// (x / (x + 1) from a 256-entry LDS table), the two filtered outputs of every
+// pixel and the five projection sums, kept per set in i64 registers.
 // out[(u * 16 + set) * 5 + {0..4}] = H00, H01, H11, C0, C1 (sums / unit area).
 // Every i32 intermediate is bounded: box sums <= 25 * 1023^2, A <= 256,
-// B <= 2^18, weighted A*d + B < 2^25.
+// (256 - A) * sum * one_by_x < 2^31, B <= (256 - A) * 1024 + 1, so the
+// weighted A*d + B <= 32 * 2^18 and each filtered output is <= 2^14 (as is
+// the source term), which bounds a lane's per-tile product sums by 2^30.
+constexpr int kSgTW = 64, kSgSR = 16, kSgSets = 4;
+
 template <typename Pix>
 __global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
                                                    int pw, int ph, int bd, int unit_size, int ss_y, int nh,
                                                    int64_t* __restrict__ out) {
-  constexpr int SR = 8, DW = kMaxUW + 6, AW = kMaxUW + 2;
-  __shared__ uint16_t D[(SR + 6) * DW];
-  __shared__ uint16_t A[2][(SR + 2) * AW];
-  __shared__ int32_t B[2][(SR + 2) * AW];
+  constexpr int DW = kSgTW + 6, AW = kSgTW + 2, DR = kSgSR + 6, AR = kSgSR + 2, AR2 = AR / 2;
+  __shared__ uint16_t D[DR * DW];
+  __shared__ int32_t S1[AR * AW], Q1[AR * AW];    // radius 1, every A/B row
+  __shared__ int32_t S2[AR2 * AW], Q2[AR2 * AW];  // radius 2, A/B rows at odd unit rows (even a)
+  __shared__ uint16_t A0[AR2 * AW], A1[AR * AW];
+  __shared__ int32_t B0[AR2 * AW], B1[AR * AW];
+  __shared__ uint16_t XBY[256];
   __shared__ uint64_t red[4];
-  const int u = blockIdx.x >> 4, set = blockIdx.x & 15, uy = u / nh, ux = u - uy * nh;
+  constexpr int NG = 16 / kSgSets;
+  const int u = blockIdx.x / NG, sg = blockIdx.x - u * NG, uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
-  const int W = he - hs, Hh = ve - vs, dw = W + 6, aw = W + 2;
+  const int W = he - hs, Hh = ve - vs;
   const int cs = bd - 8;
-  const int rr[2] = {kSgrParams[set][0], kSgrParams[set][2]};
-  const int sp[2] = {kSgrParams[set][1], kSgrParams[set][3]};
   const int tid = threadIdx.x;
-  int64_t h00 = 0, h01 = 0, h11 = 0, c0 = 0, c1 = 0;
-  for (int s0 = 0; s0 < Hh; s0 += SR) {
-    const int sr = min(SR, Hh - s0);
-    __syncthreads();
-    // D row R <-> unit row s0 - 3 + R; column C <-> unit column C - 3
-    for (int i = tid; i < (sr + 6) * dw; i += 256) {
-      const int R = i / dw, C = i - R * dw;
-      const int gy = clampi(vs + s0 - 3 + R, 0, ph - 1), gx = clampi(hs + C - 3, 0, pw - 1);
-      D[R * DW + C] = (uint16_t)dgd[(long)gy * pw + gx];
-    }
-    __syncthreads();
-    // A/B row a <-> unit row s0 - 1 + a, column b <-> unit column b - 1
-    for (int p = 0; p < 2; p++) {
-      const int r = rr[p];
-      if (r == 0) continue;
-      const int n = (2 * r + 1) * (2 * r + 1);
-      const uint32_t obx = (uint32_t)((4096 + n / 2) / n);
-      for (int i = tid; i < (sr + 2) * aw; i += 256) {
-        const int a = i / aw, b = i - a * aw;
-        const int ur = s0 - 1 + a;
-        if (r == 2 && ((ur + 1) & 1)) continue;  // fast variant: odd unit rows (and row -1) only
-        int32_t sum = 0, sq = 0;
-        // D row of unit row ur + dy is a + 2 + dy; column of b - 1 + dx is b + 2 + dx
-        for (int dy = -r; dy <= r; dy++)
-          for (int dx = -r; dx <= r; dx++) {
-            const int32_t v = D[(a + 2 + dy) * DW + b + 2 + dx];
-            sum += v;
-            sq += v * v;
-          }
-        const int32_t av2 = rpot32(sq, 2 * cs), bv2 = rpot32(sum, cs);
-        int32_t pp = av2 * n - bv2 * bv2;
-        pp = pp < 0 ? 0 : pp;
-        const uint32_t z = (uint32_t)(((int64_t)pp * sp[p] + (1 << 19)) >> 20);
-        const int32_t xa = x_by_xplus1(z > 255 ? 255 : (int)z);
-        const int64_t bb = (int64_t)(uint32_t)(256 - xa) * (uint32_t)sum * obx;
-        A[p][a * AW + b] = (uint16_t)xa;
-        B[p][a * AW + b] = (int32_t)((bb + (1 << 11)) >> 12);
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < sr * W; i += 256) {
-      const int rl = i / W, j = i - rl * W;
-      const int ur = s0 + rl;
-      const int32_t d = D[(rl + 3) * DW + j + 3];
-      const int64_t uu = (int64_t)d << 4;
-      const int64_t sv = ((int64_t)src[(long)(vs + ur) * pw + hs + j] << 4) - uu;
-      int64_t f[2] = {0, 0};
-      for (int p = 0; p < 2; p++) {
-        const int r = rr[p];
-        if (r == 0) continue;
-        const uint16_t* Ap = A[p];
-        const int32_t* Bp = B[p];
-        const int a = rl + 1, b = j + 1;  // A/B position of this pixel
-#define AT(X, dy, dx) ((int32_t)X[(a + (dy)) * AW + b + (dx)])
-        int32_t wa, wb, sh;
-        if (r == 1) {
-          wa = (AT(Ap, 0, 0) + AT(Ap, 0, -1) + AT(Ap, 0, 1) + AT(Ap, -1, 0) + AT(Ap, 1, 0)) * 4 +
-               (AT(Ap, -1, -1) + AT(Ap, -1, 1) + AT(Ap, 1, -1) + AT(Ap, 1, 1)) * 3;
-          wb = (AT(Bp, 0, 0) + AT(Bp, 0, -1) + AT(Bp, 0, 1) + AT(Bp, -1, 0) + AT(Bp, 1, 0)) * 4 +
-               (AT(Bp, -1, -1) + AT(Bp, -1, 1) + AT(Bp, 1, -1) + AT(Bp, 1, 1)) * 3;
-          sh = 9;
-        } else if ((ur & 1) == 0) {
-          wa = (AT(Ap, -1, 0) + AT(Ap, 1, 0)) * 6 +
-               (AT(Ap, -1, -1) + AT(Ap, -1, 1) + AT(Ap, 1, -1) + AT(Ap, 1, 1)) * 5;
-          wb = (AT(Bp, -1, 0) + AT(Bp, 1, 0)) * 6 +
-               (AT(Bp, -1, -1) + AT(Bp, -1, 1) + AT(Bp, 1, -1) + AT(Bp, 1, 1)) * 5;
-          sh = 9;
-        } else {
-          wa = AT(Ap, 0, 0) * 6 + (AT(Ap, 0, -1) + AT(Ap, 0, 1)) * 5;
-          wb = AT(Bp, 0, 0) * 6 + (AT(Bp, 0, -1) + AT(Bp, 0, 1)) * 5;
-          sh = 8;
+  XBY[tid] = (uint16_t)x_by_xplus1(tid);
+  int64_t acc[kSgSets][5];
+#pragma unroll
+  for (int k = 0; k < kSgSets; k++)
+#pragma unroll
+    for (int m = 0; m < 5; m++) acc[k][m] = 0;
+
+  for (int s0 = 0; s0 < Hh; s0 += kSgSR) {
+    const int sr = min(kSgSR, Hh - s0), ar = sr + 2;
+    for (int c0 = 0; c0 < W; c0 += kSgTW) {
+      const int tw = min(kSgTW, W - c0), aw = tw + 2, dw = tw + 6;
+      __syncthreads();
+      // Lanes walk 64 consecutive columns of 4 rows at a time (no divisions);
+      // the few columns past 64 (tile halos) go to a second, linear pass.
+      const int lr0 = tid >> 6, lc0 = tid & 63;
+      const int aex = aw - 64;  // 0..2 A/B columns past 64
+      // D row R <-> unit row s0 - 3 + R; column C <-> unit column c0 - 3 + C
+      for (int R = lr0; R < sr + 6; R += 4) {
+        const int gy = clampi(vs + s0 - 3 + R, 0, ph - 1);
+        for (int C = lc0; C < dw; C += 64) {
+          const int gx = clampi(hs + c0 - 3 + C, 0, pw - 1);
+          D[R * DW + C] = (uint16_t)dgd[(long)gy * pw + gx];
         }
-#undef AT
-        f[p] = (int64_t)((wa * d + wb + (1 << (sh - 1))) >> sh) - uu;
       }
-      h00 += f[0] * f[0];
-      h01 += f[0] * f[1];
-      h11 += f[1] * f[1];
-      c0 += f[0] * sv;
-      c1 += f[1] * sv;
+      __syncthreads();
+      // box sums; A/B position (a, b) <-> unit (s0 - 1 + a, c0 - 1 + b), whose
+      // box rows are D rows a + 2 + dy and columns b + 2 + dx
+      auto box = [&](int a, int b) {
+        int32_t s1 = 0, q1 = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+          for (int dx = -1; dx <= 1; dx++) {
+            const int32_t v = D[(a + 2 + dy) * DW + b + 2 + dx];
+            s1 += v;
+            q1 += __mul24(v, v);
+          }
+        S1[a * AW + b] = s1;
+        Q1[a * AW + b] = q1;
+        if ((a & 1) == 0) {  // unit row s0 - 1 + a is odd (s0 is even)
+          int32_t s2 = 0, q2 = 0;
+#pragma unroll
+          for (int dy = -2; dy <= 2; dy++)
+#pragma unroll
+            for (int dx = -2; dx <= 2; dx++) {
+              const int32_t v = D[(a + 2 + dy) * DW + b + 2 + dx];
+              s2 += v;
+              q2 += __mul24(v, v);
+            }
+          S2[(a >> 1) * AW + b] = s2;
+          Q2[(a >> 1) * AW + b] = q2;
+        }
+      };
+      for (int a = lr0; a < ar; a += 4)
+        if (lc0 < aw) box(a, lc0);
+      for (int i = tid; i < ar * aex; i += 256) box(aex == 2 ? i >> 1 : i, 64 + (aex == 2 ? (i & 1) : 0));
+#pragma unroll
+      for (int k = 0; k < kSgSets; k++) {
+        const int set = sg * kSgSets + k;
+        const int r0 = kSgrParams[set][0], sp0 = kSgrParams[set][1];
+        const int r1 = kSgrParams[set][2], sp1 = kSgrParams[set][3];
+        __syncthreads();  // box sums ready / previous set's A, B consumed
+        auto ab = [&](int a, int b) {
+          if (r1) {
+            const int32_t sm = S1[a * AW + b];
+            const int32_t pp = max(rpot32(Q1[a * AW + b], 2 * cs) * 9 - rpot32(sm, cs) * rpot32(sm, cs), 0);
+            const uint32_t z = (uint32_t)(((uint64_t)(uint32_t)pp * (uint32_t)sp1 + (1u << 19)) >> 20);
+            const uint32_t xa = XBY[z > 255 ? 255 : z];
+            A1[a * AW + b] = (uint16_t)xa;
+            B1[a * AW + b] = (int32_t)(((256u - xa) * (uint32_t)sm * 455u + 2048u) >> 12);
+          }
+          if (r0 && (a & 1) == 0) {
+            const int i2 = (a >> 1) * AW + b;
+            const int32_t sm = S2[i2];
+            const int32_t pp = max(rpot32(Q2[i2], 2 * cs) * 25 - rpot32(sm, cs) * rpot32(sm, cs), 0);
+            const uint32_t z = (uint32_t)(((uint64_t)(uint32_t)pp * (uint32_t)sp0 + (1u << 19)) >> 20);
+            const uint32_t xa = XBY[z > 255 ? 255 : z];
+            A0[i2] = (uint16_t)xa;
+            B0[i2] = (int32_t)(((256u - xa) * (uint32_t)sm * 164u + 2048u) >> 12);
+          }
+        };
+        for (int a = lr0; a < ar; a += 4)
+          if (lc0 < aw) ab(a, lc0);
+        for (int i = tid; i < ar * aex; i += 256) ab(aex == 2 ? i >> 1 : i, 64 + (aex == 2 ? (i & 1) : 0));
+        __syncthreads();
+        // a lane's pixels of one tile (at most 4) summed in i32: |f|, |sv| <= 2^14
+        static_assert(kSgSR * kSgTW <= 4 * 256, "i32 tile partials");
+        int32_t p00 = 0, p01 = 0, p11 = 0, pc0 = 0, pc1 = 0;
+        for (int rl = lr0; rl < sr; rl += 4) {
+          const int j = lc0;
+          if (j >= tw) break;
+          const int a = rl + 1, b = j + 1;  // A/B position of this pixel
+          const int32_t d = D[(rl + 3) * DW + j + 3];
+          const int32_t uu = d << 4;
+          const int32_t sv = ((int32_t)src[(long)(vs + s0 + rl) * pw + hs + c0 + j] << 4) - uu;
+          int32_t f0 = 0, f1 = 0;
+          if (r1) {
+#define AT(X, dy, dx) ((int32_t)X[(a + (dy)) * AW + b + (dx)])
+            const int32_t wa = (AT(A1, 0, 0) + AT(A1, 0, -1) + AT(A1, 0, 1) + AT(A1, -1, 0) + AT(A1, 1, 0)) * 4 +
+                               (AT(A1, -1, -1) + AT(A1, -1, 1) + AT(A1, 1, -1) + AT(A1, 1, 1)) * 3;
+            const int32_t wb = (AT(B1, 0, 0) + AT(B1, 0, -1) + AT(B1, 0, 1) + AT(B1, -1, 0) + AT(B1, 1, 0)) * 4 +
+                               (AT(B1, -1, -1) + AT(B1, -1, 1) + AT(B1, 1, -1) + AT(B1, 1, 1)) * 3;
+#undef AT
+            f1 = ((__mul24(wa, d) + wb + (1 << 8)) >> 9) - uu;
+          }
+          if (r0) {
+            // radius-2 rows live at even a: row a -> index a >> 1
+#define AT2(X, ra, dx) ((int32_t)X[((ra) >> 1) * AW + b + (dx)])
+            if ((rl & 1) == 0) {  // even unit row: rows a - 1 and a + 1 (both even a)
+              const int32_t wa = (AT2(A0, a - 1, 0) + AT2(A0, a + 1, 0)) * 6 +
+                                 (AT2(A0, a - 1, -1) + AT2(A0, a - 1, 1) + AT2(A0, a + 1, -1) + AT2(A0, a + 1, 1)) * 5;
+              const int32_t wb = (AT2(B0, a - 1, 0) + AT2(B0, a + 1, 0)) * 6 +
+                                 (AT2(B0, a - 1, -1) + AT2(B0, a - 1, 1) + AT2(B0, a + 1, -1) + AT2(B0, a + 1, 1)) * 5;
+              f0 = ((__mul24(wa, d) + wb + (1 << 8)) >> 9) - uu;
+            } else {
+              const int32_t wa = AT2(A0, a, 0) * 6 + (AT2(A0, a, -1) + AT2(A0, a, 1)) * 5;
+              const int32_t wb = AT2(B0, a, 0) * 6 + (AT2(B0, a, -1) + AT2(B0, a, 1)) * 5;
+              f0 = ((__mul24(wa, d) + wb + (1 << 7)) >> 8) - uu;
+            }
+#undef AT2
+          }
+          p00 += __mul24(f0, f0);
+          p01 += __mul24(f0, f1);
+          p11 += __mul24(f1, f1);
+          pc0 += __mul24(f0, sv);
+          pc1 += __mul24(f1, sv);
+        }
+        acc[k][0] += p00;
+        acc[k][1] += p01;
+        acc[k][2] += p11;
+        acc[k][3] += pc0;
+        acc[k][4] += pc1;
+      }
     }
   }
   const int64_t size = (int64_t)W * Hh;
-  const int64_t v0 = (int64_t)wg_sum_u64((uint64_t)h00, red);
-  const int64_t v1 = (int64_t)wg_sum_u64((uint64_t)h01, red);
-  const int64_t v2 = (int64_t)wg_sum_u64((uint64_t)h11, red);
-  const int64_t v3 = (int64_t)wg_sum_u64((uint64_t)c0, red);
-  const int64_t v4 = (int64_t)wg_sum_u64((uint64_t)c1, red);
-  if (tid == 0) {
-    int64_t* o = out + ((long)u * 16 + set) * 5;
-    o[0] = v0 / size;
-    o[1] = v1 / size;
-    o[2] = v2 / size;
-    o[3] = v3 / size;
-    o[4] = v4 / size;
-  }
+#pragma unroll
+  for (int k = 0; k < kSgSets; k++)
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const int64_t v = (int64_t)wg_sum_u64((uint64_t)acc[k][m], red);
+      if (tid == 0) out[((long)u * 16 + sg * kSgSets + k) * 5 + m] = v / size;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -917,7 +998,7 @@ hipError_t unit_sse(const void* a, const void* b, int pw, int ph, int bd, int un
 
 hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, int unit_size, int ss_y, int nh,
                      int nv, int64_t* out, hipStream_t st) {
-  const unsigned gr = (unsigned)(nh * nv * 16);
+  const unsigned gr = (unsigned)(nh * nv * (16 / kSgSets));
   if (bd == 8)
     hipLaunchKernelGGL((k_sgr_stats<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)dgd, (const uint8_t*)src,
                        pw, ph, bd, unit_size, ss_y, nh, out);
