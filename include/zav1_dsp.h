@@ -174,6 +174,38 @@ int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size
                         int n_blocks, const uint8_t *types, int n_types, int qindex, uint64_t *d_dist,
                         int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
 
+/* Fused transform units with a given prediction: for every job (one unit of
+ * tx_size at (x, y) of d_src, its w*h u16 prediction at d_pred + pred_off,
+ * its own tx type) residual -> forward 2D -> 64-point fold -> quantize
+ * (quantize_fp, or quantize_b with ZAV1_TXU_QUANT_B) -> eob (scan order),
+ * cul level byte, and distortion: frequency-domain (sse_i32 / sq_sum_i32 +
+ * folded energy, >> (1 - log_scale) * 2) or, with ZAV1_TXU_SPATIAL, the SSE
+ * of the reconstruction against the source over crop_w x crop_h, << 4.
+ * With ZAV1_TXU_RECON (implied by SPATIAL) the unit is reconstructed:
+ * inverse 2D of the dequantized levels + prediction, clipped to bd.
+ * Replaces tx_unit_inner / tx_unit_hbd
+ * (rust/crates/svtav1-encoder/src/leaf_funnel/tx_pipeline.rs:268 / :854)
+ * without the RDOQ trellis, quantization matrices and coefficient rate
+ * (the caller prices d_qcoeff), with compute_cul_level (:130).
+ * Per job: d_dist (u64), d_eob (i32; -1 = no kernel for the type at this
+ * size, the reference's `false`; -22 = unit or prediction out of bounds),
+ * d_cul (u8); optional d_qcoeff (packed <= 32x32 levels, job-major) and
+ * d_recon (w*h u16, job-major).  d_src is u8 at bd 8, u16 at bd 10. */
+typedef struct zav1_txunit_job {
+  int32_t x, y;            /* unit origin in d_src (samples) */
+  uint32_t pred_off;       /* d_pred offset (samples) of the w*h prediction */
+  uint8_t tx_type;         /* 0..15 */
+  uint8_t crop_w, crop_h;  /* spatial-distortion extent, <= w, h */
+  uint8_t pad;
+} zav1_txunit_job;
+#define ZAV1_TXU_QUANT_B 1u
+#define ZAV1_TXU_SPATIAL 2u
+#define ZAV1_TXU_RECON 4u
+int zav1_txunit_batch(const void *d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                      const uint16_t *d_pred, size_t pred_len, const zav1_txunit_job *d_jobs, int n_jobs, int qindex,
+                      unsigned flags, uint64_t *d_dist, int32_t *d_eob, uint8_t *d_cul, int32_t *d_qcoeff,
+                      uint16_t *d_recon, zav1_stream_t s);
+
 /* ------------------------------------------------------------------------ */
 /* Intra prediction (batched).                                               */
 /* ------------------------------------------------------------------------ */

@@ -221,6 +221,33 @@ class DeblockGeom(ctypes.Structure):
                 ('skip', _vp)]
 
 
+def _bind_txunit(L):
+    L.or_tx_unit_full.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp] + [ctypes.c_int] * 9 + [_vp, _vp, _vp, _vp,
+                                                                                               _vp]
+
+
+def tx_unit_full(src, pred, tx_size, tx_type, qindex, bd, use_b=False, need_recon=True, spatial=False, crop=None):
+    """One transform unit of tx_unit_inner / tx_unit_hbd on a block `src`
+    (h x w view of a plane, u8 or u16) with a u16 prediction.  Returns
+    (dist, eob, cul, qcoeff packed, recon or None)."""
+    L = lib()
+    _bind_txunit(L)
+    w, h = tx_dims(tx_size)
+    s = np.ascontiguousarray(src)
+    p = np.ascontiguousarray(pred, dtype=np.uint16)
+    pw, ph = min(w, 32), min(h, 32)
+    q = np.zeros(pw * ph, np.int32)
+    rec = np.zeros((h, w), np.uint16)
+    d, e, c = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint8()
+    cw, ch = crop if crop is not None else (w, h)
+    ok = L.or_tx_unit_full(s.ctypes.data, s.shape[1], int(s.dtype == np.uint16), p.ctypes.data, tx_size, tx_type,
+                           qindex, bd, int(use_b), int(need_recon), int(spatial), cw, ch, q.ctypes.data,
+                           rec.ctypes.data, ctypes.byref(d), ctypes.byref(e), ctypes.byref(c))
+    if not ok:
+        return None
+    return d.value, e.value, c.value, q.reshape(ph, pw), (rec if (need_recon or spatial) else None)
+
+
 def _bind_lf(L):
     L.or_cdef_constrain.argtypes = [ctypes.c_int] * 3
     L.or_cdef_adjust_strength.argtypes = [ctypes.c_int] * 2

@@ -149,3 +149,110 @@ void or_txsearch_plane(const uint16_t *plane, int stride, int width, int height,
   (void)width;
   (void)height;
 }
+
+/* tx_unit_inner (lf/tx_pipeline.rs:268) and its u16 twin tx_unit_hbd
+ * (:854) for one transform unit, without the RDOQ trellis, quantization
+ * matrices, noise normalization, tx-bias / psy terms and coefficient rate:
+ *   residual_i32 (dsp/residual.rs:26): src - pred
+ *   forward 2D, 64-point fold + three_quad energy (:335-365)
+ *   quantize: use_b ? quantize_b : quantize_fp (the RDOQ path's quantizer)
+ *   eob in scan order; cul level compute_cul_level (:130)
+ *   recon when need_recon or spatial (:432-470 / hbd :1004-1030):
+ *     eob > 0 ? clamp(pred + inverse 2D of the unfolded dqcoeff, 0, 2^bd - 1)
+ *             : pred
+ *   dist: spatial ? SSE(src, recon) over crop_w x crop_h << 4
+ *         (variance.rs:60 / hbd.rs:936)
+ *       : (sse_i32(packed, dq) or sq_sum_i32(packed)) + three_quad,
+ *         shifted by (1 - log_scale) * 2 (:567-577).
+ * src: u8 (hbd == 0) or u16 plane; pred: u16 w x h raster.  Outputs:
+ * qcoeff (packed pw x ph, may be NULL), recon (u16 w x h, NULL when not
+ * reconstructed). */
+int or_tx_unit_full(const void *src, int src_stride, int hbd, const uint16_t *pred, int tx_size, int tx_type,
+                    int qindex, int bd, int use_b, int need_recon, int spatial, int crop_w, int crop_h,
+                    int32_t *qcoeff_out, uint16_t *recon_out, uint64_t *dist, int *eob, uint8_t *cul) {
+  int w, h;
+  or_tx_size_dims(tx_size, &w, &h);
+  const int n = w * h;
+  int32_t *res = (int32_t *)malloc(sizeof(int32_t) * n);
+  for (int r = 0; r < h; r++)
+    for (int c = 0; c < w; c++) {
+      const long o = (long)r * src_stride + c;
+      const int32_t s = hbd ? ((const uint16_t *)src)[o] : ((const uint8_t *)src)[o];
+      res[r * w + c] = s - (int32_t)pred[r * w + c];
+    }
+  int32_t *coeff = (int32_t *)malloc(sizeof(int32_t) * n);
+  if (!or_fwd_txfm2d(res, coeff, w, tx_size, tx_type)) {
+    free(res);
+    free(coeff);
+    return 0;
+  }
+  const int pw = w < 32 ? w : 32, ph = h < 32 ? h : 32;
+  uint64_t tq = 0;
+  if (w == 64 && h == 64) tq = energy(coeff + 32, 64, 32, 32) + energy(coeff + 32 * 64, 64, 64, 32);
+  else if (w == 64) tq = energy(coeff + 32, 64, 32, h < 32 ? h : 32);
+  else if (h == 64) tq = energy(coeff + 32 * w, w, w, h - 32);
+  int32_t packed[1024], q[1024], dq[1024];
+  for (int r = 0; r < ph; r++)
+    for (int c = 0; c < pw; c++) packed[r * pw + c] = coeff[r * w + c];
+  or_quant_row row;
+  or_build_quant_row(qindex, bd, &row);
+  const int ls = or_tx_scale(tx_size);
+  if (use_b) or_quantize_b(packed, pw * ph, &row, ls, bd, q, dq);
+  else or_quantize_fp(packed, pw * ph, &row, ls, bd, q, dq);
+  uint16_t scan[1024];
+  const int ns = or_scan(tx_size, or_scan_class(tx_type), scan);
+  const int e = or_eob(q, scan, ns);
+  /* compute_cul_level: |q| in scan order up to eob, capped at 63, dc sign */
+  uint32_t cl = 0;
+  for (int i = 0; i < e; i++) {
+    const int32_t v = q[scan[i]];
+    cl += (uint32_t)(v < 0 ? -v : v);
+    if (cl >= 63) break;
+  }
+  if (cl > 63) cl = 63;
+  const int32_t dc = e > 0 ? q[0] : 0;
+  if (dc < 0) cl |= 1u << 6;
+  else if (dc > 0) cl += 2u << 6;
+  *cul = (uint8_t)cl;
+  *eob = e;
+  if (qcoeff_out) memcpy(qcoeff_out, q, sizeof(int32_t) * pw * ph);
+
+  uint16_t *rec = NULL;
+  if (need_recon || spatial) {
+    rec = recon_out ? recon_out : (uint16_t *)malloc(sizeof(uint16_t) * n);
+    if (e > 0) {
+      int32_t *dq_full = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+      for (int r = 0; r < ph; r++)
+        for (int c = 0; c < pw; c++) dq_full[r * w + c] = dq[r * pw + c];
+      or_inv_txfm2d(dq_full, res, w, tx_size, tx_type, bd);
+      const int32_t maxv = (1 << bd) - 1;
+      for (int i = 0; i < n; i++) {
+        const int32_t v = (int32_t)pred[i] + res[i];
+        rec[i] = (uint16_t)(v < 0 ? 0 : (v > maxv ? maxv : v));
+      }
+      free(dq_full);
+    } else {
+      memcpy(rec, pred, sizeof(uint16_t) * n);
+    }
+  }
+  if (spatial) {
+    uint64_t s = 0;
+    for (int r = 0; r < crop_h; r++)
+      for (int c = 0; c < crop_w; c++) {
+        const long o = (long)r * src_stride + c;
+        const int64_t sv = hbd ? ((const uint16_t *)src)[o] : ((const uint8_t *)src)[o];
+        const int64_t d = sv - rec[r * w + c];
+        s += (uint64_t)(d * d);
+      }
+    *dist = s << 4;
+  } else {
+    uint64_t d = e > 0 ? or_sse_i32(packed, dq, pw * ph) : or_sq_sum_i32(packed, pw * ph);
+    d += tq;
+    const int shift = (1 - ls) * 2;
+    *dist = shift < 0 ? (d << (-shift)) : (d >> shift);
+  }
+  if (rec && rec != recon_out) free(rec);
+  free(res);
+  free(coeff);
+  return 1;
+}

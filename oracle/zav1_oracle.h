@@ -108,6 +108,11 @@ int or_eob(const int32_t *qcoeff, const uint16_t *scan, int n);
  * residual w x h -> dist (after the log_scale shift), eob, sum |level|. */
 int or_tx_unit(const int32_t *residual, int tx_size, int tx_type, int qindex, int bd, int use_b,
                int32_t *qcoeff_out, uint64_t *dist, int *eob, uint32_t *level_sum);
+/* lf/tx_pipeline.rs:268 tx_unit_inner / :854 tx_unit_hbd for one unit with a
+ * given u16 prediction (no RDOQ / QM / rate): see txsearch.c. */
+int or_tx_unit_full(const void *src, int src_stride, int hbd, const uint16_t *pred, int tx_size, int tx_type,
+                    int qindex, int bd, int use_b, int need_recon, int spatial, int crop_w, int crop_h,
+                    int32_t *qcoeff_out, uint16_t *recon_out, uint64_t *dist, int *eob, uint8_t *cul);
 int32_t or_dc_value(const int32_t *above, const int32_t *left, int w, int h, int has_above,
                     int has_left, int bd);
 void or_txsearch_plane(const uint16_t *plane, int stride, int width, int height, int bd, int tx_size,

@@ -102,6 +102,8 @@ EXPORTS = {
     'zav1_txsearch_batch': [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp],
     'zav1_intra_pred_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
     'zav1_cfl_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
+    'zav1_txunit_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
+                          _vp, _vp, _vp, _vp, _vp, _vp],
     'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
     'zav1_cdef_search': [_vp] * 6 + [_i32] * 5 + [_vp, _i32, _i32, _vp, _vp, _vp, _vp],
     'zav1_rest_unit_grid': [_i32, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
@@ -115,6 +117,9 @@ INTRA_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('w', 'u1'), ('h', 'u1'), ('mo
                       ('fi_mode', 'u1'), ('edge_filter', 'u1'), ('filt_type', 'u1'), ('pad0', 'u1'),
                       ('n_top_px', '<i2'), ('n_topright_px', '<i2'), ('n_left_px', '<i2'),
                       ('n_bottomleft_px', '<i2'), ('out_off', '<u4'), ('pad1', '<u4')])
+TXUNIT_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('pred_off', '<u4'), ('tx_type', 'u1'), ('crop_w', 'u1'),
+                       ('crop_h', 'u1'), ('pad', 'u1')])
+TXU_QUANT_B, TXU_SPATIAL, TXU_RECON = 1, 2, 4
 CFL_JOB = np.dtype([('lx', '<i4'), ('ly', '<i4'), ('cw', '<i4'), ('ch', '<i4'), ('alpha_q3', '<i4'),
                     ('dc_off', '<u4'), ('out_off', '<u4'), ('pad', '<u4')])
 
@@ -312,3 +317,12 @@ def unit_sse(d_a, d_b, plane_w, plane_h, bd, unit_size, ss_y, d_out, stream=None
 def sgr_stats(d_dgd, d_src, plane_w, plane_h, bd, unit_size, ss_y, d_out, stream=None):
     check(lib().zav1_sgr_stats(d_dgd.ptr, d_src.ptr, plane_w, plane_h, bd, unit_size, ss_y, d_out.ptr, stream),
           'sgr_stats')
+
+
+def txunit_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, d_jobs, n_jobs, qindex, flags, d_dist, d_eob,
+                 d_cul, d_qcoeff=None, d_recon=None, stream=None):
+    """Fused transform units (zav1_txunit_batch); d_pred is a u16 DeviceArray."""
+    ptr = lambda a: a.ptr if a is not None else None  # noqa: E731
+    check(lib().zav1_txunit_batch(d_src.ptr, stride, plane_w, plane_h, bd, tx_size, d_pred.ptr,
+                                  d_pred.nbytes // 2, d_jobs.ptr, n_jobs, qindex, flags, d_dist.ptr, d_eob.ptr,
+                                  d_cul.ptr, ptr(d_qcoeff), ptr(d_recon), stream), 'txunit_batch')

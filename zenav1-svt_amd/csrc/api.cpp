@@ -486,6 +486,28 @@ int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size
   return ZAV1_OK;
 }
 
+int zav1_txunit_batch(const void* d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                      const uint16_t* d_pred, size_t pred_len, const zav1_txunit_job* d_jobs, int n_jobs, int qindex,
+                      unsigned flags, uint64_t* d_dist, int32_t* d_eob, uint8_t* d_cul, int32_t* d_qcoeff,
+                      uint16_t* d_recon, zav1_stream_t s) {
+  static_assert(sizeof(zav1_txunit_job) == sizeof(zav1::TxUnitJob), "txunit job layout");
+  if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_jobs < 0 || qindex < 0 || qindex > 255 ||
+      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~7u) ||
+      (n_jobs && (!d_src || !d_pred || !d_jobs || !d_dist || !d_eob || !d_cul)))
+    return ZAV1_EINVAL;
+  const zav1::QuantRow* rows = nullptr;
+  const uint16_t* iscan = nullptr;
+  int rc = quant_rows(&rows);
+  if (rc) return rc;
+  rc = iscan_table(&iscan);
+  if (rc) return rc;
+  ZCHECK(zav1::txunit_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, (long)pred_len,
+                            reinterpret_cast<const zav1::TxUnitJob*>(d_jobs), n_jobs,
+                            rows + (bd == 10 ? 256 : 0) + qindex, iscan, flags, d_dist, d_eob, d_cul, d_qcoeff,
+                            d_recon, S(s)));
+  return ZAV1_OK;
+}
+
 int zav1_intra_pred_batch(const void* d_plane, int stride, int bd, const zav1_intra_job* d_jobs, int n_jobs,
                           uint16_t* d_out, zav1_stream_t s) {
   static_assert(sizeof(zav1_intra_job) == sizeof(zav1::IntraJob), "intra job layout");

@@ -29,27 +29,11 @@ __global__ __launch_bounds__(256) void k_quantize(const int32_t* __restrict__ co
   const int32_t c = coeff[i];
   const int32_t sgn = c >> 31;
   const int32_t a = (c ^ sgn) - sgn;
-  int32_t q = 0;
-  if constexpr (KIND == 0) {
-    const int32_t rounding = (t.round_fp[iz] + ((1 << ls) >> 1)) >> ls;
-    if (((int64_t)a << (1 + ls)) >= t.dequant[iz]) {
-      int64_t x = (int64_t)a + rounding;
-      if constexpr (BD == 8) x = x > 32767 ? 32767 : x;
-      q = (int32_t)((x * t.quant_fp[iz]) >> (16 - ls));
-    }
-  } else {
-    const int32_t zbin = (t.zbin[iz] + ((1 << ls) >> 1)) >> ls;
-    const int32_t rnd = (t.round[iz] + ((1 << ls) >> 1)) >> ls;
-    if (a >= zbin) {
-      int64_t x = (int64_t)a + rnd;
-      if constexpr (BD == 8) x = x > 32767 ? 32767 : x;
-      q = (int32_t)(((((x * t.quant[iz]) >> 16) + x) * t.quant_shift[iz]) >> (16 - ls));
-    }
-  }
+  const int32_t q = KIND == 0 ? quant_fp_level(a, t, iz, ls, BD) : quant_b_level(a, t, iz, ls, BD);
   int32_t qo = 0, dqo = 0;
   if (q != 0) {
     qo = (q ^ sgn) - sgn;
-    const int32_t adq = (int32_t)(((int64_t)q * t.dequant[iz]) >> ls);
+    const int32_t adq = dequant_level(q, t, iz, ls);
     dqo = (adq ^ sgn) - sgn;
   }
   qcoeff[i] = qo;

@@ -1,0 +1,214 @@
+// Fused transform units for gfx950: for every job (one transform unit of one
+// tx size with its own type and prediction) residual -> forward 2D ->
+// 64-point fold -> quantize (fp or b) -> eob, cul level, frequency-domain
+// distortion; and, when reconstructing, inverse 2D -> recon clip -> spatial
+// distortion over the cropped extent.  Residual, coefficients and the
+// inverse stay in VGPRs / LDS; per job only the results (and optionally the
+// packed levels and the reconstruction) reach HBM.
+//
+// Semantics: rust/crates/svtav1-encoder/src/leaf_funnel/tx_pipeline.rs:268
+// tx_unit_inner (8-bit) and :854 tx_unit_hbd (10-bit), without the RDOQ
+// trellis, quantization matrices and coefficient rate; compute_cul_level
+// :130; recon_add_clamp rust/crates/svtav1-dsp/src/residual.rs:140; spatial
+// SSE rust/crates/svtav1-dsp/src/variance.rs:60 / hbd.rs:936.
+#include "zav1_dev.h"
+#include "zav1_kernels.h"
+
+namespace zav1 {
+
+template <int W, int H>
+struct TuGeom {
+  static constexpr int T = W > H ? W : H;  // lanes per job
+  static constexpr int JPW = 256 / T;      // jobs per workgroup
+  static constexpr int LS = W + 1;         // padded LDS row stride
+  static constexpr int BLK = H * LS;
+  static constexpr int PW = W < 32 ? W : 32, PH = H < 32 ? H : 32;
+  static constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
+};
+
+// A job whose (type, size) has no 1D kernel (ADST on a 32/64-point
+// dimension) gets eob = -1 and dist = 0 (the reference's `false` return); a
+// job whose unit or prediction lies outside the plane / prediction buffer
+// gets eob = -22 and is not read.
+template <int W, int H, typename Pix>
+__global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int stride, int plane_w, int plane_h,
+                                                int bd, const uint16_t* __restrict__ pred, long pred_len,
+                                                const TxUnitJob* __restrict__ jobs, int n_jobs,
+                                                const QuantRow* __restrict__ row_p,
+                                                const uint16_t* __restrict__ iscan_all, int tx_size, unsigned flags,
+                                                uint64_t* __restrict__ out_dist, int32_t* __restrict__ out_eob,
+                                                uint8_t* __restrict__ out_cul, int32_t* __restrict__ out_q,
+                                                uint16_t* __restrict__ out_rec) {
+  using G = TuGeom<W, H>;
+  constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
+  constexpr FwdShift FS = fwd_shift(W, H);
+  constexpr InvShift IS = inv_shift(W, H);
+  constexpr int LSC = tx_log_scale(W, H);
+  __shared__ int32_t lds[G::JPW * G::BLK];
+
+  const int jl = threadIdx.x / G::T, t = threadIdx.x - jl * G::T;
+  const int job = blockIdx.x * G::JPW + jl;
+  const TxUnitJob jb = jobs[job < n_jobs ? job : 0];
+  const TypeCfg cfg = type_cfg(jb.tx_type & 15);
+  const bool ok = jb.tx_type < 16 && kind_ok(cfg.col, H) && kind_ok(cfg.row, W);
+  const bool inside = jb.x >= 0 && jb.y >= 0 && jb.x + W <= plane_w && jb.y + H <= plane_h &&
+                      (long)jb.pred_off + W * H <= pred_len;
+  const bool active = job < n_jobs && ok && inside;
+  const uint16_t* pr = pred + jb.pred_off;
+  const Pix* sp = src + (long)jb.y * stride + jb.x;
+  int32_t* blk = lds + jl * G::BLK;
+
+  // ---- forward column pass: lane t < W transforms column t of the residual ----
+  if (active && t < W) {
+    int32_t v[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+      const int rr = cfg.ud ? (H - 1 - r) : r;
+      v[r] = ((int32_t)sp[(long)rr * stride + t] - (int32_t)pr[rr * W + t]) << FS.s0;
+    }
+    fwd1d<H, CBC>(v, cfg.col);
+    const int cc = cfg.lr ? (W - 1 - t) : t;
+#pragma unroll
+    for (int r = 0; r < H; r++) blk[r * G::LS + cc] = rshift_round(v[r], FS.s1);
+  }
+  __syncthreads();
+
+  // ---- forward row pass + fold + quantize: lane t < H owns row t ----
+  // The row's dequantized values replace it in LDS (0 where folded) as the
+  // inverse transform's input.
+  uint64_t fdist = 0;
+  int32_t eob = 0, lsum = 0, dcq = 0;
+  if (active && t < H) {
+    int32_t v[W];
+#pragma unroll
+    for (int c = 0; c < W; c++) v[c] = blk[t * G::LS + c];
+    fwd1d<W, CBR>(v, cfg.row);
+    const QuantRow& qr = *row_p;
+    const int cls = (jb.tx_type >= 10) ? (((jb.tx_type - 10) & 1) ? 2 : 1) : 0;
+    const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + t * G::PW;
+    const bool use_b = flags & kTxuQuantB;
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+      int32_t x = rshift_round(v[c], FS.s2);
+      if constexpr (G::RECT2) x = rshift64r((int64_t)x * 5793, 12);
+      if (t >= 32 || c >= 32) {
+        fdist += (uint64_t)((int64_t)x * x);  // energy of the folded-away coefficients
+        blk[t * G::LS + c] = 0;
+        continue;
+      }
+      const int iz = (t | c) != 0;
+      const int32_t sgn = x >> 31;
+      const int32_t a = (x ^ sgn) - sgn;
+      const int32_t q = use_b ? quant_b_level(a, qr, iz, LSC, bd) : quant_fp_level(a, qr, iz, LSC, bd);
+      int32_t dq = 0, qs = 0;
+      if (q != 0) {
+        dq = (dequant_level(q, qr, iz, LSC) ^ sgn) - sgn;
+        qs = (q ^ sgn) - sgn;
+        lsum += q;
+        eob = max(eob, (int32_t)iscan[c] + 1);
+      }
+      if (t == 0 && c == 0) dcq = qs;
+      const int64_t e = (int64_t)x - dq;
+      fdist += (uint64_t)e * (uint64_t)e;
+      blk[t * G::LS + c] = dq;
+      if (out_q) out_q[(long)job * (G::PW * G::PH) + t * G::PW + c] = qs;
+    }
+  }
+  fdist = group_sum_u64<G::T>(fdist);
+  eob = group_max_i32<G::T>(eob);
+  lsum = group_sum_i32<G::T>(lsum);
+
+  // ---- reconstruction: inverse 2D of the dequantized block + prediction ----
+  uint64_t sdist = 0;
+  const bool spatial = flags & kTxuSpatial;
+  if (flags & (kTxuRecon | kTxuSpatial)) {
+    __syncthreads();
+    const int row_rng = bd + 8, col_rng = (bd + 6) > 16 ? bd + 6 : 16;
+    const int32_t wrap_max = (1 << (7 + bd)) - 1 + (914 << (bd - 7));
+    if (active && t < H) {
+      int32_t v[W];
+#pragma unroll
+      for (int c = 0; c < W; c++) {
+        int32_t x = blk[t * G::LS + c];
+        if constexpr (G::RECT2) x = rshift64r((int64_t)x * 2896, 12);
+        v[c] = zclamp(x, row_rng);
+      }
+      inv1d<W>(v, cfg.row, row_rng);
+#pragma unroll
+      for (int c = 0; c < W; c++) blk[t * G::LS + c] = rshift_round(v[c], IS.r0);
+    }
+    __syncthreads();
+    if (active && t < W) {
+      const int cc = cfg.lr ? (W - 1 - t) : t;
+      int32_t v[H];
+#pragma unroll
+      for (int r = 0; r < H; r++) v[r] = zclamp(blk[r * G::LS + cc], col_rng);
+      inv1d<H>(v, cfg.col, col_rng);
+      const int32_t maxv = (1 << bd) - 1;
+#pragma unroll
+      for (int r = 0; r < H; r++) {
+        int32_t x = rshift_round(v[r], IS.r1);
+        x = x < -wrap_max - 1 ? -wrap_max - 1 : (x > wrap_max ? wrap_max : x);
+        const int rr = cfg.ud ? (H - 1 - r) : r;
+        int32_t rec = (int32_t)pr[rr * W + t] + x;
+        rec = rec < 0 ? 0 : (rec > maxv ? maxv : rec);
+        if (out_rec) out_rec[(long)job * (W * H) + rr * W + t] = (uint16_t)rec;
+        if (spatial && rr < jb.crop_h && t < jb.crop_w) {
+          const int32_t d = (int32_t)sp[(long)rr * stride + t] - rec;
+          sdist += (uint64_t)(uint32_t)__mul24(d, d);
+        }
+      }
+    }
+    sdist = group_sum_u64<G::T>(sdist);
+  }
+
+  if (job < n_jobs && t == 0) {
+    if (!ok || !inside) {
+      out_dist[job] = 0;
+      out_eob[job] = !inside ? -22 : -1;
+      out_cul[job] = 0;
+      return;
+    }
+    constexpr int SHF = (1 - LSC) * 2;
+    out_dist[job] = spatial ? (sdist << 4) : (SHF < 0 ? (fdist << (-SHF)) : (fdist >> SHF));
+    out_eob[job] = eob;
+    uint32_t cl = (uint32_t)(lsum < 63 ? lsum : 63);
+    if (dcq < 0) cl |= 1u << 6;
+    else if (dcq > 0) cl += 2u << 6;
+    out_cul[job] = (uint8_t)cl;
+  }
+}
+
+template <int S>
+static hipError_t launch_tu(const void* src, int stride, int pw, int ph, int bd, const uint16_t* pred, long pred_len,
+                            const TxUnitJob* jobs,
+                            int n_jobs, const QuantRow* row, const uint16_t* iscan, unsigned flags, uint64_t* dist,
+                            int32_t* eob, uint8_t* cul, int32_t* q, uint16_t* rec, hipStream_t st) {
+  constexpr int W = tx_w(S), H = tx_h(S);
+  const int grid = (n_jobs + TuGeom<W, H>::JPW - 1) / TuGeom<W, H>::JPW;
+  if (bd == 8)
+    hipLaunchKernelGGL((k_txunit<W, H, uint8_t>), dim3(grid), dim3(256), 0, st, (const uint8_t*)src, stride, pw, ph,
+                       bd, pred, pred_len, jobs, n_jobs, row, iscan, S, flags, dist, eob, cul, q, rec);
+  else
+    hipLaunchKernelGGL((k_txunit<W, H, uint16_t>), dim3(grid), dim3(256), 0, st, (const uint16_t*)src, stride, pw,
+                       ph, bd, pred, pred_len, jobs, n_jobs, row, iscan, S, flags, dist, eob, cul, q, rec);
+  return hipGetLastError();
+}
+
+hipError_t txunit_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                        const uint16_t* pred, long pred_len, const TxUnitJob* jobs, int n_jobs, const QuantRow* row,
+                        const uint16_t* iscan, unsigned flags, uint64_t* dist, int32_t* eob, uint8_t* cul,
+                        int32_t* qcoeff, uint16_t* recon, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  switch (tx_size) {
+#define X(S)                                                                                                    \
+  case S:                                                                                                       \
+    return launch_tu<S>(src, stride, plane_w, plane_h, bd, pred, pred_len, jobs, n_jobs, row, iscan, flags, dist, \
+                        eob, cul, qcoeff, recon, st);
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace zav1

@@ -199,4 +199,28 @@ __device__ __forceinline__ void inv1d(int32_t* x, int kind, int rng) {
   }
 }
 
+// log_scale per TxSize (rust/crates/svtav1-encoder/src/quant.rs:238).
+__host__ __device__ constexpr int tx_log_scale(int w, int h) {
+  return (w * h > 1024) ? 2 : ((w * h > 256) ? 1 : 0);
+}
+
+template <int T>
+__device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int m = 1; m < T; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+template <int T>
+__device__ __forceinline__ int32_t group_sum_i32(int32_t v) {
+#pragma unroll
+  for (int m = 1; m < T; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+template <int T>
+__device__ __forceinline__ int32_t group_max_i32(int32_t v) {
+#pragma unroll
+  for (int m = 1; m < T; m <<= 1) v = max(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
 }  // namespace zav1

@@ -13,6 +13,28 @@ struct QuantRow {
   int32_t round_fp[2], quant_fp[2], dequant[2];
 };
 
+// Level magnitude of one coefficient magnitude `a` (iz: 0 = DC, 1 = AC) at
+// log_scale ls.  fp: quant_coding.rs:80 fp_one (the INT16 clamp applies at 8
+// bits only, quant.rs:335); b: quant_coding.rs:365 b_one (quant.rs:391).
+__device__ __forceinline__ int32_t quant_fp_level(int32_t a, const QuantRow& t, int iz, int ls, int bd) {
+  const int32_t rounding = (t.round_fp[iz] + ((1 << ls) >> 1)) >> ls;
+  if (((int64_t)a << (1 + ls)) < t.dequant[iz]) return 0;
+  int64_t x = (int64_t)a + rounding;
+  if (bd == 8) x = x > 32767 ? 32767 : x;
+  return (int32_t)((x * t.quant_fp[iz]) >> (16 - ls));
+}
+__device__ __forceinline__ int32_t quant_b_level(int32_t a, const QuantRow& t, int iz, int ls, int bd) {
+  const int32_t zbin = (t.zbin[iz] + ((1 << ls) >> 1)) >> ls;
+  const int32_t rnd = (t.round[iz] + ((1 << ls) >> 1)) >> ls;
+  if (a < zbin) return 0;
+  int64_t x = (int64_t)a + rnd;
+  if (bd == 8) x = x > 32767 ? 32767 : x;
+  return (int32_t)(((((x * t.quant[iz]) >> 16) + x) * t.quant_shift[iz]) >> (16 - ls));
+}
+__device__ __forceinline__ int32_t dequant_level(int32_t q, const QuantRow& t, int iz, int ls) {
+  return (int32_t)(((int64_t)q * t.dequant[iz]) >> ls);
+}
+
 // Plain-form quantizer parameters: kind 0 uses round (= pre-shifted _fp
 // rounding), quant (= quant_fp), dequant; kind 1 uses zbin, round, quant,
 // quant_shift, dequant (zbin/round pre-shifted).
@@ -60,6 +82,21 @@ struct TxPlan {
 hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
                           int n_blocks, const TypeList& types, const QuantRow* row, const uint16_t* iscan,
                           uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st);
+
+// One transform unit of the fused unit kernel (layout of zav1_txunit_job).
+struct TxUnitJob {
+  int32_t x, y;
+  uint32_t pred_off;
+  uint8_t tx_type, crop_w, crop_h, pad;
+};
+enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u };
+// Fused transform units of one tx size: residual (src - pred) -> forward ->
+// fold -> quantize -> eob / cul level / frequency distortion, and, when
+// reconstructing, inverse -> recon clip -> spatial distortion.
+hipError_t txunit_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                        const uint16_t* pred, long pred_len, const TxUnitJob* jobs, int n_jobs, const QuantRow* row,
+                        const uint16_t* iscan, unsigned flags, uint64_t* dist, int32_t* eob, uint8_t* cul,
+                        int32_t* qcoeff, uint16_t* recon, hipStream_t st);
 
 // One intra transform unit (layout of zav1_intra_job in include/zav1_dsp.h).
 struct IntraJob {
