@@ -279,6 +279,28 @@ typedef struct zav1_deblock_geom {
 int zav1_deblock_plane(void *d_plane, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
                        int level_h, int sharpness, const zav1_deblock_geom *geom, zav1_stream_t s);
 
+/* Deblocking level trials: for each of n_levels (<= 64) candidate levels
+ * (host array, 0..63; the level applies to both edge directions, as on the
+ * still-picture path) a tight copy of d_recon in d_work (n_levels planes;
+ * work_bytes checked) is filtered and its SSE against d_src written to
+ * d_sse[i].  Replaces try_filter_plane + plane_sse
+ * (rust/crates/svtav1-encoder/src/deblock.rs:228 / :208).  Planes tight. */
+int zav1_deblock_level_sse(const void *d_recon, const void *d_src, int plane_w, int plane_h, int bd, int plane, int ss,
+                           int sharpness, const zav1_deblock_geom *geom, const uint8_t *levels, int n_levels,
+                           void *d_work, size_t work_bytes, uint64_t *d_sse, zav1_stream_t s);
+
+/* Deblocking level search of one plane: search_filter_level
+ * (rust/crates/svtav1-encoder/src/deblock.rs:337) on the still path (start
+ * level 0; luma both directions at one level, chroma per plane), with the
+ * trials of each round run on the GPU (at most two levels per round; d_work
+ * >= 2 planes, d_sse >= 2 entries).  conv_th is the early-exit convergence
+ * count (0 = full search).  Synchronizes `s` once per round.  Outputs the
+ * picked level and the SSEs at level 0 and at the pick. */
+int zav1_deblock_search_level(const void *d_recon, const void *d_src, int plane_w, int plane_h, int bd, int plane,
+                              int ss, int sharpness, const zav1_deblock_geom *geom, int conv_th, void *d_work,
+                              size_t work_bytes, uint64_t *d_sse, int *best_level, int64_t *zero_sse,
+                              int64_t *best_sse, zav1_stream_t s);
+
 /* CDEF strength search of a whole frame: for every 64x64 filter block
  * (raster, ceil(w/64) x ceil(h/64)) and every strength index g = 4 * pri +
  * sec (pri 0..15, sec 0..3 -> 0,1,2,4), the SSE of the filtered luma

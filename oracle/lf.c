@@ -432,3 +432,70 @@ void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, 
   }
   free(pad);
 }
+
+/* enc/deblock.rs:208 plane_sse + :228 try_filter_plane: SSE of the source
+ * against a scratch copy of the recon filtered at (level, level). */
+static int64_t try_level(const void *recon, const void *src, int pw, int ph, int bd, int plane, int ss, int sharpness,
+                         const or_deblock_geom *g, int level) {
+  const int hbd = bd > 8;
+  const size_t bytes = (size_t)pw * ph * (hbd ? 2 : 1);
+  void *scratch = malloc(bytes);
+  memcpy(scratch, recon, bytes);
+  if (level) or_deblock_plane(scratch, pw, pw, ph, bd, plane, ss, level, level, sharpness, g);
+  uint64_t s = 0;
+  for (long i = 0; i < (long)pw * ph; i++) {
+    const int64_t d = (int64_t)ld(src, hbd, i) - ld(scratch, hbd, i);
+    s += (uint64_t)(d * d);
+  }
+  free(scratch);
+  return (int64_t)s;
+}
+
+/* enc/deblock.rs:337 search_filter_level on the still path (last levels 0,
+ * luma dir 2 / chroma dir 0: one level for both edge directions), with a
+ * real trial filter per evaluated level.  Planes tight (stride = width). */
+void or_deblock_search_level(const void *recon, const void *src, int pw, int ph, int bd, int plane, int ss,
+                             int sharpness, const or_deblock_geom *g, int conv_th, int *best, int64_t *err0,
+                             int64_t *best_err) {
+  const int maxl = 63;
+  int64_t ss_err[64];
+  for (int i = 0; i < 64; i++) ss_err[i] = -1;
+  int filt_mid = 0, filt_dir = 0;
+  int step = filt_mid < 16 ? 4 : filt_mid / 4;
+  int64_t be = try_level(recon, src, pw, ph, bd, plane, ss, sharpness, g, filt_mid);
+  int fb = filt_mid;
+  ss_err[filt_mid] = be;
+  int conv = 0;
+  while (step > 0) {
+    const int hi = filt_mid + step > maxl ? maxl : filt_mid + step;
+    const int lo = filt_mid - step < 0 ? 0 : filt_mid - step;
+    int64_t bias = (be >> (15 - (filt_mid / 8))) * (int64_t)step;
+    bias >>= 1;
+    if (filt_dir <= 0 && lo != filt_mid) {
+      if (ss_err[lo] < 0) ss_err[lo] = try_level(recon, src, pw, ph, bd, plane, ss, sharpness, g, lo);
+      if (ss_err[lo] < be + bias) {
+        if (ss_err[lo] < be) be = ss_err[lo];
+        fb = lo;
+      }
+    }
+    if (filt_dir >= 0 && hi != filt_mid) {
+      if (ss_err[hi] < 0) ss_err[hi] = try_level(recon, src, pw, ph, bd, plane, ss, sharpness, g, hi);
+      if (ss_err[hi] < be - bias) {
+        be = ss_err[hi];
+        fb = hi;
+      }
+    }
+    if (fb == filt_mid) {
+      conv++;
+      if (conv == conv_th) step = 0;
+      else step /= 2;
+      filt_dir = 0;
+    } else {
+      filt_dir = fb < filt_mid ? -1 : 1;
+      filt_mid = fb;
+    }
+  }
+  *best = fb;
+  *err0 = ss_err[0];
+  *best_err = ss_err[fb];
+}

@@ -258,6 +258,8 @@ def _bind_lf(L):
     L.or_cdef_search_fb.argtypes = [_vp] * 6 + [ctypes.c_int] * 7 + [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
                                                                        ctypes.c_int]
     L.or_lf_thresholds.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+    L.or_deblock_search_level.argtypes = [_vp, _vp] + [ctypes.c_int] * 6 + [ctypes.POINTER(DeblockGeom), ctypes.c_int,
+                                                                              _vp, _vp, _vp]
 
 
 def lflib():
@@ -283,6 +285,20 @@ def deblock_plane(plane, bd, plane_idx, ss, level_v, level_h, sharpness, geom_ar
     lflib().or_deblock_plane(p.ctypes.data, p.shape[1], p.shape[1], p.shape[0], bd, plane_idx, ss, level_v,
                              level_h, sharpness, ctypes.byref(g))
     return p
+
+
+def deblock_search_level(recon, src, bd, plane_idx, ss, sharpness, geom_arrays, mi_cols, mi_rows, true_w, true_h,
+                         conv_th=2):
+    """search_filter_level with real trials: (best level, sse at level 0, best sse)."""
+    r, s_ = np.ascontiguousarray(recon), np.ascontiguousarray(src)
+    g = DeblockGeom(mi_cols, mi_rows, true_w, true_h, geom_arrays['block_id'].ctypes.data,
+                    geom_arrays['bw'].ctypes.data, geom_arrays['bh'].ctypes.data, geom_arrays['tw'].ctypes.data,
+                    geom_arrays['th'].ctypes.data, geom_arrays['skip'].ctypes.data if 'skip' in geom_arrays else None)
+    b, e0, be = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+    lflib().or_deblock_search_level(r.ctypes.data, s_.ctypes.data, r.shape[1], r.shape[0], bd, plane_idx, ss,
+                                    sharpness, ctypes.byref(g), conv_th, ctypes.byref(b), ctypes.byref(e0),
+                                    ctypes.byref(be))
+    return b.value, e0.value, be.value
 
 
 def cdef_search_fb(rec, src, width, height, bd, qindex, sub, fbr, fbc, fs, skip=None):

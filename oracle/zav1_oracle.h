@@ -159,6 +159,10 @@ void or_lf_thresholds(int level, int sharpness, int *mblim, int *lim, int *hev);
 void or_lpf(void *buf, int hbd, int bd, long off, int pitch, int vertical, int len, int level, int sharpness);
 void or_deblock_plane(void *buf, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
                       int level_h, int sharpness, const or_deblock_geom *g);
+/* enc/deblock.rs:337 search_filter_level (still path, real trials). */
+void or_deblock_search_level(const void *recon, const void *src, int pw, int ph, int bd, int plane, int ss,
+                             int sharpness, const or_deblock_geom *g, int conv_th, int *best, int64_t *err0,
+                             int64_t *best_err);
 int or_cdef_constrain(int diff, int threshold, int damping);
 int or_cdef_adjust_strength(int strength, int var);
 int or_cdef_find_dir(const uint16_t *img, int stride, int coeff_shift, int *var_out);

@@ -188,3 +188,43 @@ def test_loop_filter_stage_full_size_sampled(gpu, oracle):
             assert np.array_equal(gm[u], em) and np.array_equal(gh[u], eh), (p, u)
             for s in (0, 10, 14):
                 assert np.array_equal(gsgr[u, s], oracle.sgr_unit_stats(dy[p], srcp, 8, lim, s)), (p, u, s)
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_deblock_level_trials_and_search(gpu, oracle, bd):
+    """Trial SSE of every candidate level and the hill-climb pick equal the
+    oracle's sequential search with real trials, luma and chroma, both
+    convergence settings."""
+    z = gpu
+    H, W = 136, 200
+    mi_rows, mi_cols = H // 4, W // 4
+    geom = synth.partition_geometry(mi_rows, mi_cols, 600 + bd)
+    gd = {k: z.DeviceArray.from_host(v) for k, v in geom.items()}
+    g = z.deblock_geom(mi_cols, mi_rows, W, H, gd)
+    for plane, (ph, pw, ss) in enumerate([(H, W, 0), (H // 2, W // 2, 1)]):
+        # block means of a gentle ramp: small steps the filters can smooth
+        yy, xx = np.mgrid[0:ph, 0:pw]
+        dt = np.uint8 if bd == 8 else np.uint16
+        src = ((100 + 0.5 * xx + 0.25 * yy + 2 * np.sin(xx / 5.0)) * (1 << (bd - 8))).astype(dt)
+        blk = 8 if plane == 0 else 4
+        rec = src.astype(np.int64).copy()
+        for y in range(0, ph, blk):
+            for x in range(0, pw, blk):
+                rec[y:y + blk, x:x + blk] = rec[y:y + blk, x:x + blk].mean()
+        rec = rec.astype(dt)
+        d_rec, d_src = z.DeviceArray.from_host(rec), z.DeviceArray.from_host(src)
+        levels = [0, 3, 9, 17, 32, 47, 63]
+        d_work = z.DeviceArray((len(levels), ph, pw), src.dtype)
+        d_sse = z.DeviceArray(64, np.uint64)
+        z.deblock_level_sse(d_rec, d_src, pw, ph, bd, plane, ss, 2, g, levels, d_work, d_sse)
+        got = d_sse.to_host()[:len(levels)]
+        for i, lv in enumerate(levels):
+            f = oracle.deblock_plane(rec, bd, plane, ss, lv, lv, 2, geom, mi_cols, mi_rows, W, H) if lv else rec
+            ref = int(((f.astype(np.int64) - src.astype(np.int64)) ** 2).sum())
+            assert int(got[i]) == ref, (plane, lv, int(got[i]), ref)
+        d_work2 = z.DeviceArray((2, ph, pw), src.dtype)
+        for conv in (0, 1):
+            res = z.deblock_search_level(d_rec, d_src, pw, ph, bd, plane, ss, 2, g, conv, d_work2, d_sse)
+            ref = oracle.deblock_search_level(rec, src, bd, plane, ss, 2, geom, mi_cols, mi_rows, W, H, conv)
+            assert res == ref, (plane, conv, res, ref)
+            assert res[0] > 0

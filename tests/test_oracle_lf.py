@@ -161,3 +161,30 @@ def test_deblock_lines_mirror_symmetric(oracle, bd):
             assert not diff.any(), (ln, line.tolist(), level)
             hits += int(not np.array_equal(a, buf))
     assert hits > 200
+
+
+def test_deblock_search_level_endpoints(oracle):
+    """search_filter_level picks level 0 when the reconstruction already is
+    the source, and a positive level when the reconstruction is a blocky
+    version of a smooth source (deblocking removes the steps)."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        'zs', os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'zenav1-svt_amd', 'synth.py'))
+    synth = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(synth)
+    H, W = 64, 96
+    geom = synth.partition_geometry(H // 4, W // 4, 11)
+    yy, xx = np.mgrid[0:H, 0:W]
+    smooth = (100 + 0.5 * xx + 0.25 * yy + 2 * np.sin(xx / 5.0)).astype(np.uint8)
+    b, e0, be = oracle.deblock_search_level(smooth, smooth, 8, 0, 0, 0, geom, W // 4, H // 4, W, H)
+    assert (b, e0, be) == (0, 0, 0)
+    # 8x8 block means of a gentle ramp: small steps that the filters smooth
+    blocky = smooth.astype(np.int32).copy()
+    for y in range(0, H, 8):
+        for x in range(0, W, 8):
+            blocky[y:y + 8, x:x + 8] = blocky[y:y + 8, x:x + 8].mean()
+    blocky = blocky.astype(np.uint8)
+    for conv in (0, 1):
+        b, e0, be = oracle.deblock_search_level(blocky, smooth, 8, 0, 0, 0, geom, W // 4, H // 4, W, H, conv)
+        assert b > 0 and be < e0

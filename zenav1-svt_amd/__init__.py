@@ -105,6 +105,10 @@ EXPORTS = {
     'zav1_txunit_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
                           _vp, _vp, _vp, _vp, _vp, _vp],
     'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
+    'zav1_deblock_level_sse': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _i32, _vp, ctypes.c_size_t, _vp, _vp],
+    'zav1_deblock_search_level': [_vp, _vp] + [_i32] * 6 + [_vp, _i32, _vp, ctypes.c_size_t, _vp,
+                                                            ctypes.POINTER(_i32), ctypes.POINTER(ctypes.c_int64),
+                                                            ctypes.POINTER(ctypes.c_int64), _vp],
     'zav1_cdef_search': [_vp] * 6 + [_i32] * 5 + [_vp, _i32, _i32, _vp, _vp, _vp, _vp],
     'zav1_rest_unit_grid': [_i32, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)],
     'zav1_wiener_stats': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _vp],
@@ -326,3 +330,22 @@ def txunit_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, d_jobs, n
     check(lib().zav1_txunit_batch(d_src.ptr, stride, plane_w, plane_h, bd, tx_size, d_pred.ptr,
                                   d_pred.nbytes // 2, d_jobs.ptr, n_jobs, qindex, flags, d_dist.ptr, d_eob.ptr,
                                   d_cul.ptr, ptr(d_qcoeff), ptr(d_recon), stream), 'txunit_batch')
+
+
+def deblock_level_sse(d_recon, d_src, plane_w, plane_h, bd, plane, ss, sharpness, geom, levels, d_work, d_sse,
+                      stream=None):
+    lv = np.ascontiguousarray(levels, dtype=np.uint8)
+    check(lib().zav1_deblock_level_sse(d_recon.ptr, d_src.ptr, plane_w, plane_h, bd, plane, ss, sharpness,
+                                       ctypes.byref(geom), lv.ctypes.data, lv.size, d_work.ptr, d_work.nbytes,
+                                       d_sse.ptr, stream), 'deblock_level_sse')
+
+
+def deblock_search_level(d_recon, d_src, plane_w, plane_h, bd, plane, ss, sharpness, geom, conv_th, d_work, d_sse,
+                         stream=None):
+    """search_filter_level on the GPU: (best level, sse at 0, best sse)."""
+    b, e0, be = _i32(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().zav1_deblock_search_level(d_recon.ptr, d_src.ptr, plane_w, plane_h, bd, plane, ss, sharpness,
+                                          ctypes.byref(geom), conv_th, d_work.ptr, d_work.nbytes, d_sse.ptr,
+                                          ctypes.byref(b), ctypes.byref(e0), ctypes.byref(be), stream),
+          'deblock_search_level')
+    return b.value, e0.value, be.value

@@ -528,22 +528,111 @@ int zav1_cfl_batch(const void* d_luma, int stride, int bd, const zav1_cfl_job* d
   return ZAV1_OK;
 }
 
-int zav1_deblock_plane(void* d_plane, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
-                       int level_h, int sharpness, const zav1_deblock_geom* geom, zav1_stream_t s) {
-  if ((bd != 8 && bd != 10) || plane < 0 || plane > 2 || ss < 0 || ss > 1 || (plane == 0 && ss != 0) ||
-      plane_w < 0 || plane_h < 0 || stride < plane_w || level_v < 0 || level_v > 63 || level_h < 0 ||
-      level_h > 63 || sharpness < 0 || sharpness > 7 || !geom || !d_plane)
+static int geom_dev(const zav1_deblock_geom* geom, int plane, int ss, int plane_w, int plane_h,
+                    zav1::DeblockGeomDev* g) {
+  if (!geom || !geom->block_id || !geom->bw || !geom->bh || !geom->tw || !geom->th || geom->mi_cols <= 0 ||
+      geom->mi_rows <= 0 || ((plane_w << ss) + 3) / 4 > geom->mi_cols || ((plane_h << ss) + 3) / 4 > geom->mi_rows)
     return ZAV1_EINVAL;
   // A 14-tap edge reaches 6 samples past a 16-aligned coordinate: luma planes
   // padded to 8 samples (chroma to 4) keep every filter inside the plane.
   const int align = plane == 0 ? 8 : 4;
   if (plane_w % align || plane_h % align) return ZAV1_EINVAL;
-  if (!geom->block_id || !geom->bw || !geom->bh || !geom->tw || !geom->th || geom->mi_cols <= 0 ||
-      geom->mi_rows <= 0 || ((plane_w << ss) + 3) / 4 > geom->mi_cols || ((plane_h << ss) + 3) / 4 > geom->mi_rows)
+  *g = zav1::DeblockGeomDev{geom->mi_cols, geom->mi_rows, geom->true_w, geom->true_h, geom->block_id,
+                            geom->bw,      geom->bh,      geom->tw,     geom->th,     geom->skip};
+  return ZAV1_OK;
+}
+
+int zav1_deblock_plane(void* d_plane, int stride, int plane_w, int plane_h, int bd, int plane, int ss, int level_v,
+                       int level_h, int sharpness, const zav1_deblock_geom* geom, zav1_stream_t s) {
+  zav1::DeblockGeomDev g{};
+  if ((bd != 8 && bd != 10) || plane < 0 || plane > 2 || ss < 0 || ss > 1 || (plane == 0 && ss != 0) ||
+      plane_w <= 0 || plane_h <= 0 || stride < plane_w || level_v < 0 || level_v > 63 || level_h < 0 ||
+      level_h > 63 || sharpness < 0 || sharpness > 7 || !d_plane ||
+      geom_dev(geom, plane, ss, plane_w, plane_h, &g) != ZAV1_OK)
     return ZAV1_EINVAL;
-  zav1::DeblockGeomDev g{geom->mi_cols, geom->mi_rows, geom->true_w, geom->true_h, geom->block_id,
-                         geom->bw,      geom->bh,      geom->tw,     geom->th,     geom->skip};
   ZCHECK(zav1::deblock_plane(d_plane, stride, plane_w, plane_h, bd, plane, ss, level_v, level_h, sharpness, g, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_deblock_level_sse(const void* d_recon, const void* d_src, int plane_w, int plane_h, int bd, int plane, int ss,
+                           int sharpness, const zav1_deblock_geom* geom, const uint8_t* levels, int n_levels,
+                           void* d_work, size_t work_bytes, uint64_t* d_sse, zav1_stream_t s) {
+  zav1::DeblockGeomDev g{};
+  if ((bd != 8 && bd != 10) || plane < 0 || plane > 2 || ss < 0 || ss > 1 || (plane == 0 && ss != 0) ||
+      sharpness < 0 || sharpness > 7 || !d_recon || !d_src || !d_work || !d_sse || !levels || n_levels < 1 ||
+      n_levels > 64 || plane_w <= 0 || plane_h <= 0 || geom_dev(geom, plane, ss, plane_w, plane_h, &g) != ZAV1_OK)
+    return ZAV1_EINVAL;
+  for (int i = 0; i < n_levels; i++)
+    if (levels[i] > 63) return ZAV1_EINVAL;
+  const size_t need = (size_t)n_levels * plane_w * plane_h * (bd > 8 ? 2 : 1);
+  if (work_bytes < need) return ZAV1_EINVAL;
+  ZCHECK(zav1::deblock_trials(d_recon, d_src, d_work, plane_w, plane_h, bd, plane, ss, levels, n_levels, sharpness, g,
+                              d_sse, S(s)));
+  return ZAV1_OK;
+}
+
+// search_filter_level (rust/crates/svtav1-encoder/src/deblock.rs:337) on the
+// still path: start level 0, step 4, bias against raising the level, step
+// halving on convergence and an early exit after conv_th convergences (0 =
+// never).  Each round's (at most two) untried levels run as one GPU trial
+// batch; the decisions read the SSE table exactly as the sequential search
+// does, so the pick is identical.
+int zav1_deblock_search_level(const void* d_recon, const void* d_src, int plane_w, int plane_h, int bd, int plane,
+                              int ss, int sharpness, const zav1_deblock_geom* geom, int conv_th, void* d_work,
+                              size_t work_bytes, uint64_t* d_sse, int* best_level, int64_t* zero_sse,
+                              int64_t* best_sse, zav1_stream_t s) {
+  if (!best_level || !zero_sse || !best_sse || conv_th < 0) return ZAV1_EINVAL;
+  int64_t ss_err[64];
+  for (int i = 0; i < 64; i++) ss_err[i] = -1;
+  auto trial = [&](int a, int b) -> int {  // evaluate levels a (and b >= 0) not yet in the table
+    uint8_t lv[2];
+    int n = 0;
+    if (a >= 0 && ss_err[a] < 0) lv[n++] = (uint8_t)a;
+    if (b >= 0 && b != a && ss_err[b] < 0) lv[n++] = (uint8_t)b;
+    if (!n) return ZAV1_OK;
+    int rc = zav1_deblock_level_sse(d_recon, d_src, plane_w, plane_h, bd, plane, ss, sharpness, geom, lv, n, d_work,
+                                    work_bytes, d_sse, s);
+    if (rc) return rc;
+    uint64_t out[2];
+    ZCHECK(hipMemcpyAsync(out, d_sse, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, S(s)));
+    ZCHECK(hipStreamSynchronize(S(s)));
+    for (int i = 0; i < n; i++) ss_err[lv[i]] = (int64_t)out[i];
+    return ZAV1_OK;
+  };
+  int filt_mid = 0, filt_dir = 0, conv = 0;
+  int step = filt_mid < 16 ? 4 : filt_mid / 4;
+  int rc = trial(filt_mid, -1);
+  if (rc) return rc;
+  int64_t best_err = ss_err[filt_mid];
+  int filt_best = filt_mid;
+  while (step > 0) {
+    const int hi = filt_mid + step > 63 ? 63 : filt_mid + step;
+    const int lo = filt_mid - step < 0 ? 0 : filt_mid - step;
+    int64_t bias = (best_err >> (15 - (filt_mid / 8))) * (int64_t)step;
+    bias >>= 1;
+    const bool try_lo = filt_dir <= 0 && lo != filt_mid, try_hi = filt_dir >= 0 && hi != filt_mid;
+    rc = trial(try_lo ? lo : -1, try_hi ? hi : -1);
+    if (rc) return rc;
+    if (try_lo && ss_err[lo] < best_err + bias) {
+      if (ss_err[lo] < best_err) best_err = ss_err[lo];
+      filt_best = lo;
+    }
+    if (try_hi && ss_err[hi] < best_err - bias) {
+      best_err = ss_err[hi];
+      filt_best = hi;
+    }
+    if (filt_best == filt_mid) {
+      conv++;
+      step = (conv == conv_th) ? 0 : step / 2;
+      filt_dir = 0;
+    } else {
+      filt_dir = filt_best < filt_mid ? -1 : 1;
+      filt_mid = filt_best;
+    }
+  }
+  *best_level = filt_best;
+  *zero_sse = ss_err[0];
+  *best_sse = ss_err[filt_best];
   return ZAV1_OK;
 }
 

@@ -15,6 +15,8 @@
 //   unit SSE  rust/crates/svtav1-dsp/src/restoration.rs:1666,1991
 //   SGR       self-guided filter + projection normal equations (no reference
 //             twin; AV1 self-guided filter process, fast r = 2 variant)
+#include <algorithm>
+
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
 
@@ -30,6 +32,9 @@ __device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) {
 // ---------------------------------------------------------------------------
 struct Thr {
   int32_t lim, mblim, hev, shift, bd;
+};
+struct LevelList {
+  uint8_t l[64];
 };
 
 __device__ __forceinline__ int32_t sclamp(int32_t t, int bd) {
@@ -176,13 +181,13 @@ __device__ __forceinline__ void filter_at(Pix* buf, long start, long step, const
 
 }  // namespace
 
-// One thread per (edge position, line).  VERT: consecutive threads take
-// consecutive mi columns of one pixel row; HORZ: consecutive pixel columns
-// of one mi row.
+namespace {
+// One (edge position, line) of a deblocking pass: thread `tid` of the pass.
+// VERT: consecutive threads take consecutive mi columns of one pixel row;
+// HORZ: consecutive pixel columns of one mi row.
 template <typename Pix, bool VERT>
-__global__ __launch_bounds__(256) void k_deblock(Pix* __restrict__ buf, int stride, int pw, int ph, int plane,
-                                                 int ss, int level, Thr t, DeblockGeomDev g) {
-  const long tid = (long)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void deblock_line(Pix* __restrict__ buf, int stride, int pw, int ph, int plane, int ss,
+                                             int level, const Thr& t, const DeblockGeomDev& g, long tid) {
   int x, y;
   if (VERT) {
     const int nm = pw >> 2, nl = ph & ~3;
@@ -204,6 +209,64 @@ __global__ __launch_bounds__(256) void k_deblock(Pix* __restrict__ buf, int stri
   else if (len == 6) filter_at<Pix, 6>(buf, start, step, t);
   else if (len == 8) filter_at<Pix, 8>(buf, start, step, t);
   else filter_at<Pix, 14>(buf, start, step, t);
+}
+
+// lf_thresholds (loop_filter.rs:50) at bit depth bd.
+__host__ __device__ __forceinline__ Thr make_thr(int level, int sharpness, int bd) {
+  int bil = level >> ((sharpness > 0) + (sharpness > 4));
+  if (sharpness > 0 && bil > 9 - sharpness) bil = 9 - sharpness;
+  if (bil < 1) bil = 1;
+  Thr t;
+  t.mblim = 2 * (level + 2) + bil;
+  t.lim = bil;
+  t.hev = level >> 4;
+  t.shift = bd - 8;
+  t.bd = bd;
+  return t;
+}
+}  // namespace
+
+template <typename Pix, bool VERT>
+__global__ __launch_bounds__(256) void k_deblock(Pix* __restrict__ buf, int stride, int pw, int ph, int plane,
+                                                 int ss, int level, Thr t, DeblockGeomDev g) {
+  deblock_line<Pix, VERT>(buf, stride, pw, ph, plane, ss, level, t, g, (long)blockIdx.x * 256 + threadIdx.x);
+}
+
+// Level trials: blockIdx.y selects candidate level levels[y] and its own
+// tight plane copy work + y * pw * ph; the level applies to both directions.
+template <typename Pix, bool VERT>
+__global__ __launch_bounds__(256) void k_deblock_trials(Pix* __restrict__ work, int pw, int ph, int plane, int ss,
+                                                        LevelList levels, int sharpness, int bd, DeblockGeomDev g) {
+  const int level = levels.l[blockIdx.y];
+  if (level == 0) return;
+  deblock_line<Pix, VERT>(work + (long)blockIdx.y * pw * ph, pw, pw, ph, plane, ss, level,
+                          make_thr(level, sharpness, bd), g, (long)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_trial_copy(const Pix* __restrict__ recon, Pix* __restrict__ work, long n) {
+  Pix* dst = work + (long)blockIdx.y * n;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) dst[i] = recon[i];
+}
+
+__global__ __launch_bounds__(64) void k_zero_u64(uint64_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// SSE of each trial plane against the source (plane_sse, deblock.rs:208):
+// grid-stride partial sums, wave reduction, one atomic per wave.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_trial_sse(const Pix* __restrict__ src, const Pix* __restrict__ work, long n,
+                                                   unsigned long long* __restrict__ sse) {
+  const Pix* a = work + (long)blockIdx.y * n;
+  uint64_t s = 0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const int32_t d = (int32_t)src[i] - (int32_t)a[i];
+    s += (uint64_t)(uint32_t)__mul24(d, d);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&sse[blockIdx.y], (unsigned long long)s);
 }
 
 // ---------------------------------------------------------------------------
@@ -905,22 +968,9 @@ __global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-static void lf_thresholds(int level, int sharpness, Thr& t) {
-  int bil = level >> ((sharpness > 0) + (sharpness > 4));
-  if (sharpness > 0 && bil > 9 - sharpness) bil = 9 - sharpness;
-  if (bil < 1) bil = 1;
-  t.mblim = 2 * (level + 2) + bil;
-  t.lim = bil;
-  t.hev = level >> 4;
-}
-
 hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plane, int ss, int level_v,
                          int level_h, int sharpness, const DeblockGeomDev& g, hipStream_t st) {
-  Thr tv{}, th{};
-  lf_thresholds(level_v, sharpness, tv);
-  lf_thresholds(level_h, sharpness, th);
-  tv.shift = th.shift = bd - 8;
-  tv.bd = th.bd = bd;
+  const Thr tv = make_thr(level_v, sharpness, bd), th = make_thr(level_h, sharpness, bd);
   const long nv = (long)(pw >> 2) * (ph & ~3), nhz = (long)(ph >> 2) * (pw & ~3);
   if (level_v && nv > 0) {
     const unsigned gr = (unsigned)((nv + 255) / 256);
@@ -940,6 +990,40 @@ hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plan
       hipLaunchKernelGGL((k_deblock<uint16_t, false>), dim3(gr), dim3(256), 0, st, (uint16_t*)buf, stride, pw, ph,
                          plane, ss, level_h, th, g);
   }
+  return hipGetLastError();
+}
+
+template <typename Pix>
+static void trials_t(const Pix* recon, const Pix* src, Pix* work, int pw, int ph, int plane, int ss,
+                     const LevelList& levels, int n_levels, int sharpness, int bd, const DeblockGeomDev& g,
+                     uint64_t* sse, hipStream_t st) {
+  const long n = (long)pw * ph;
+  const unsigned cg = (unsigned)std::min<long>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL((k_trial_copy<Pix>), dim3(cg, n_levels), dim3(256), 0, st, recon, work, n);
+  const long nv = (long)(pw >> 2) * (ph & ~3), nhz = (long)(ph >> 2) * (pw & ~3);
+  if (nv > 0)
+    hipLaunchKernelGGL((k_deblock_trials<Pix, true>), dim3((unsigned)((nv + 255) / 256), n_levels), dim3(256), 0, st,
+                       work, pw, ph, plane, ss, levels, sharpness, bd, g);
+  if (nhz > 0)
+    hipLaunchKernelGGL((k_deblock_trials<Pix, false>), dim3((unsigned)((nhz + 255) / 256), n_levels), dim3(256), 0,
+                       st, work, pw, ph, plane, ss, levels, sharpness, bd, g);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, sse, n_levels);
+  hipLaunchKernelGGL((k_trial_sse<Pix>), dim3(cg, n_levels), dim3(256), 0, st, src, work, n,
+                     (unsigned long long*)sse);
+}
+
+hipError_t deblock_trials(const void* recon, const void* src, void* work, int pw, int ph, int bd, int plane, int ss,
+                          const uint8_t* levels_host, int n_levels, int sharpness, const DeblockGeomDev& g,
+                          uint64_t* sse, hipStream_t st) {
+  if (n_levels <= 0 || n_levels > 64) return hipErrorInvalidValue;
+  LevelList levels{};
+  for (int i = 0; i < n_levels; i++) levels.l[i] = levels_host[i];
+  if (bd == 8)
+    trials_t<uint8_t>((const uint8_t*)recon, (const uint8_t*)src, (uint8_t*)work, pw, ph, plane, ss, levels,
+                      n_levels, sharpness, bd, g, sse, st);
+  else
+    trials_t<uint16_t>((const uint16_t*)recon, (const uint16_t*)src, (uint16_t*)work, pw, ph, plane, ss, levels,
+                       n_levels, sharpness, bd, g, sse, st);
   return hipGetLastError();
 }
 

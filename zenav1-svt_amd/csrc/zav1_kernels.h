@@ -129,6 +129,13 @@ struct DeblockGeomDev {
 // bd 8, u16 at bd 10).
 hipError_t deblock_plane(void* buf, int stride, int pw, int ph, int bd, int plane, int ss, int level_v, int level_h,
                          int sharpness, const DeblockGeomDev& g, hipStream_t st);
+// Deblocking level trials: for each candidate level (host array of
+// n_levels <= 64 values in 0..63, both directions) a tight copy of recon in
+// work[i] is filtered and its SSE against src written to sse[i].  work
+// holds n_levels planes.
+hipError_t deblock_trials(const void* recon, const void* src, void* work, int pw, int ph, int bd, int plane, int ss,
+                          const uint8_t* levels_host, int n_levels, int sharpness, const DeblockGeomDev& g,
+                          uint64_t* sse, hipStream_t st);
 // CDEF strength search over every 64x64 filter block: mse_y / mse_uv rows of
 // 64 strengths per block (raster of blocks); dirvar [fb][64][2] (dir, var).
 // skip: per-mi signaled skip (mi_rows x mi_cols) or nullptr.
