@@ -230,3 +230,25 @@ def test_hadamard_matches_walsh(oracle, n):
         assert np.max(np.abs(got - ref)) <= 4 * scale, n
         if n == 8:
             assert np.array_equal(got, ref)
+
+
+def test_fused_kernel_coefficient_bound(oracle):
+    """The fused search / unit kernels quantize in 32-bit arithmetic under the
+    contract |coefficient| < 2^21 for residuals of bd <= 10 planes.  The
+    forward transform is linear up to per-stage rounding, so the largest
+    coefficient over the residual box [-1023, 1023] is the l1 norm of a
+    transform row times 1023 (measured from impulse responses, plus a margin
+    for the roundings): it must stay far below 2^21 for every size and type."""
+    worst = 0
+    for ts in range(19):
+        w, h = oracle.tx_dims(ts)
+        for t in range(16):
+            if oracle.fwd_txfm2d(np.zeros((h, w), np.int32), ts, t) is None:
+                continue
+            M = np.zeros((h * w, h * w))
+            for i in range(h * w):
+                e = np.zeros(h * w, np.int32)
+                e[i] = 1023
+                M[:, i] = oracle.fwd_txfm2d(e.reshape(h, w), ts, t).ravel()
+            worst = max(worst, np.abs(M).sum(axis=1).max() + 64 * h * w)
+    assert worst < (1 << 19), worst

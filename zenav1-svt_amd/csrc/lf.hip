@@ -807,6 +807,7 @@ __global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, 
                                                    int64_t* __restrict__ out) {
   constexpr int DW = kSgTW + 6, AW = kSgTW + 2, DR = kSgSR + 6, AR = kSgSR + 2, AR2 = AR / 2;
   __shared__ uint16_t D[DR * DW];
+  __shared__ uint16_t SRC[kSgSR * kSgTW];        // the tile's source samples
   __shared__ int32_t S1[AR * AW], Q1[AR * AW];    // radius 1, every A/B row
   __shared__ int32_t S2[AR2 * AW], Q2[AR2 * AW];  // radius 2, A/B rows at odd unit rows (even a)
   __shared__ uint16_t A0[AR2 * AW], A1[AR * AW];
@@ -844,6 +845,8 @@ __global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, 
           D[R * DW + C] = (uint16_t)dgd[(long)gy * pw + gx];
         }
       }
+      for (int R = lr0; R < sr; R += 4)
+        if (lc0 < tw) SRC[R * kSgTW + lc0] = (uint16_t)src[(long)(vs + s0 + R) * pw + hs + c0 + lc0];
       __syncthreads();
       // box sums; A/B position (a, b) <-> unit (s0 - 1 + a, c0 - 1 + b), whose
       // box rows are D rows a + 2 + dy and columns b + 2 + dx
@@ -914,7 +917,7 @@ __global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, 
           const int a = rl + 1, b = j + 1;  // A/B position of this pixel
           const int32_t d = D[(rl + 3) * DW + j + 3];
           const int32_t uu = d << 4;
-          const int32_t sv = ((int32_t)src[(long)(vs + s0 + rl) * pw + hs + c0 + j] << 4) - uu;
+          const int32_t sv = ((int32_t)SRC[rl * kSgTW + j] << 4) - uu;
           int32_t f0 = 0, f1 = 0;
           if (r1) {
 #define AT(X, dy, dx) ((int32_t)X[(a + (dy)) * AW + b + (dx)])

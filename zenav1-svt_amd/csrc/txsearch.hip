@@ -72,16 +72,16 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   else dc = 1 << (bd - 1);
 
   // ---- column variants -> LDS ----
+  // Loads are unconditional (an inactive group reads block 0 and its
+  // results are never written out), so they issue back to back.
   int32_t* L = lds + g * nvar * G::VS;
+  const uint16_t* pb = plane + (long)y0 * stride + x0;
   for (int j = t; j < nvar * W; j += G::T) {
     const int v = j / W, c = j - v * W;
     const int ud = plan.var_ud[v];
     int32_t col[H];
 #pragma unroll
-    for (int r = 0; r < H; r++) {
-      const int rr = ud ? (H - 1 - r) : r;
-      col[r] = active ? ((int32_t)plane[(long)(y0 + rr) * stride + x0 + c] - dc) << SH.s0 : 0;
-    }
+    for (int r = 0; r < H; r++) col[r] = ((int32_t)pb[(ud ? (H - 1 - r) : r) * stride + c] - dc) << SH.s0;
     fwd1d<H, CBC>(col, plan.var_kind[v]);
 #pragma unroll
     for (int r = 0; r < H; r++) L[v * G::VS + r * G::LS + c] = rshift_round(col[r], SH.s1);
@@ -92,8 +92,7 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   constexpr int TPR = G::T / H;
   const int sub = t / H, r = t - sub * H;
   const QuantRow& qr = *row_p;
-  const int32_t rnd0 = (qr.round_fp[0] + ((1 << LSC) >> 1)) >> LSC;
-  const int32_t rnd1 = (qr.round_fp[1] + ((1 << LSC) >> 1)) >> LSC;
+  const QuantFp qf = make_quant_fp(qr, LSC);
   for (int base = 0; base < ntyp; base += TPR) {
     const int ti = base + sub;
     const bool live = active && ti < ntyp;
@@ -117,24 +116,15 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
           dist += (uint64_t)((int64_t)x * x);  // 64-point fold: discarded energy
           continue;
         }
-        const int iz = (r | c) != 0;
-        const int32_t sgn = x >> 31;
-        const int32_t a = (x ^ sgn) - sgn;
-        int32_t q = 0;
-        if (((int64_t)a << (1 + LSC)) >= qr.dequant[iz]) {
-          int64_t xx = (int64_t)a + (iz ? rnd1 : rnd0);
-          if (bd == 8) xx = xx > 32767 ? 32767 : xx;
-          q = (int32_t)((xx * qr.quant_fp[iz]) >> (16 - LSC));
-        }
-        int32_t dq = 0;
-        if (q != 0) {
-          dq = (int32_t)(((int64_t)q * qr.dequant[iz]) >> LSC);
-          lsum += q;
-          eob = max(eob, (int32_t)iscan[c] + 1);
-          dq = (dq ^ sgn) - sgn;
-        }
-        const int64_t e = (int64_t)x - dq;
-        dist += (uint64_t)e * (uint64_t)e;
+        const int32_t a = x < 0 ? -x : x;
+        int32_t adq;
+        const int32_t q = quant_fp_fast<LSC>(a, (r | c) != 0, qf, bd, &adq);
+        const int32_t sp = (int32_t)iscan[c] + 1;
+        lsum += q;
+        eob = q ? max(eob, sp) : eob;
+        // x - dq = sign(x) * (a - |dq|): the squared error needs magnitudes only
+        const int32_t e = a - adq;
+        dist += (uint64_t)((int64_t)e * e);
       }
     }
     dist = group_sum_u64<H>(dist);

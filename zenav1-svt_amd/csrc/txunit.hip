@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
 #pragma unroll
     for (int r = 0; r < H; r++) {
       const int rr = cfg.ud ? (H - 1 - r) : r;
-      v[r] = ((int32_t)sp[(long)rr * stride + t] - (int32_t)pr[rr * W + t]) << FS.s0;
+      v[r] = ((int32_t)sp[rr * stride + t] - (int32_t)pr[rr * W + t]) << FS.s0;
     }
     fwd1d<H, CBC>(v, cfg.col);
     const int cc = cfg.lr ? (W - 1 - t) : t;
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
         rec = rec < 0 ? 0 : (rec > maxv ? maxv : rec);
         if (out_rec) out_rec[(long)job * (W * H) + rr * W + t] = (uint16_t)rec;
         if (spatial && rr < jb.crop_h && t < jb.crop_w) {
-          const int32_t d = (int32_t)sp[(long)rr * stride + t] - rec;
+          const int32_t d = (int32_t)sp[rr * stride + t] - rec;
           sdist += (uint64_t)(uint32_t)__mul24(d, d);
         }
       }

@@ -35,6 +35,40 @@ __device__ __forceinline__ int32_t dequant_level(int32_t q, const QuantRow& t, i
   return (int32_t)(((int64_t)q * t.dequant[iz]) >> ls);
 }
 
+// The fp quantizer of one row held in registers for a fused kernel (DC at
+// [0], AC at [1], rounding already shifted by log_scale).  Contract: the
+// coefficient magnitude a < 2^21.  The fused kernels only transform
+// residuals of bd <= 10 planes, whose coefficients are bounded by the l1
+// norm of the transform rows times 1023: <= 2^17 for every size and type
+// (pinned by a CPU test).  Within it, a << 3, a + rounding and q * dequant
+// stay below 2^31, so the level needs one 32x32->64 multiply and the
+// dequantized magnitude one 24-bit multiply, with no branches.
+struct QuantFp {
+  int32_t dq[2], rnd[2];
+  uint32_t qf[2];
+};
+__device__ __forceinline__ QuantFp make_quant_fp(const QuantRow& t, int ls) {
+  QuantFp f;
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    f.dq[i] = t.dequant[i];
+    f.rnd[i] = (t.round_fp[i] + ((1 << ls) >> 1)) >> ls;
+    f.qf[i] = (uint32_t)t.quant_fp[i];
+  }
+  return f;
+}
+// Level (>= 0) of |coefficient| a; *adq receives its dequantized magnitude.
+template <int LS>
+__device__ __forceinline__ int32_t quant_fp_fast(int32_t a, int iz, const QuantFp& f, int bd, int32_t* adq) {
+  const int32_t dqv = iz ? f.dq[1] : f.dq[0];
+  int32_t xx = a + (iz ? f.rnd[1] : f.rnd[0]);
+  if (bd == 8) xx = xx > 32767 ? 32767 : xx;
+  const uint32_t q = (uint32_t)(((uint64_t)(uint32_t)xx * (iz ? f.qf[1] : f.qf[0])) >> (16 - LS));
+  const int32_t qq = ((a << (1 + LS)) >= dqv) ? (int32_t)q : 0;
+  *adq = __mul24(qq, dqv) >> LS;
+  return qq;
+}
+
 // Plain-form quantizer parameters: kind 0 uses round (= pre-shifted _fp
 // rounding), quant (= quant_fp), dequant; kind 1 uses zbin, round, quant,
 // quant_shift, dequant (zbin/round pre-shifted).
