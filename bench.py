@@ -14,6 +14,11 @@ function-table call: fwd + quantize + sse per candidate) on a bounded sample,
 the roofline of the dominant kernel from HIP events over the timed region, and
 the oracle (CPU restatement) timed on a bounded crop of the same workload.
 
+A third object, md_funnel, times the intra mode-decision funnel's MDS0 ->
+MDS1 chain over every 16x16 unit of the config2 frame (61 candidates per
+unit: prediction, Hadamard SATD, fast cost and exchange-sort top-3 on the
+device, DCT_DCT quantize_b transform units of the survivors).
+
 A second object, loopfilter_stage, times the loop-filter stage on the
 4096x4096 8-bit 4:2:0 reconstruction (1/f noise + 30% 8x8 mean-flattened
 blocks, seed 3000): deblocking of all planes, CDEF direction + 64-strength SSE
@@ -187,6 +192,95 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
     return out
 
 
+def md_funnel_stage(zav1, synth, args, cpu=True):
+    """MDS0 -> MDS1 of the intra funnel over every 16x16 unit of the 2048x2048
+    10-bit frame: 61 candidates per unit (13 modes + angle deltas), SATD,
+    fast cost + exchange-sort survivors (k = 3), DCT_DCT quantize_b units."""
+    L = zav1.lib()
+    fm = _load('zav1_funnel', os.path.join(PKG, 'funnel.py'))
+    S, bw = 2048, 16
+    src = synth.config2_luma(S, S, seed=2000)
+    rng = np.random.default_rng(2001)
+    recon = np.clip(src.astype(np.int64) + rng.integers(-4, 5, src.shape), 0, 1023).astype(np.uint16)
+    cands = fm.angle_candidates()
+    nc = len(cands)
+    rates = rng.integers(100, 1500, (1, nc)).astype(np.uint32)
+    qi = synth.qp_to_qindex(40)
+    lam, k = 90000, 3
+    f = fm.IntraFunnel(zav1, src, recon, 10, bw, bw, qi, k=k, cands=cands, lam=lam, rates=rates)
+    sp = ctypes.c_void_p()
+    zav1.check(L.zav1_stream_create(ctypes.byref(sp)), 'stream_create')
+    stream = sp.value
+    for _ in range(2):
+        f.run(stream)
+    zav1.sync(stream)
+    phases = ('pred', 'satd', 'select', 'tx')
+    ev = []
+    for _ in range(len(phases) + 1):
+        e = ctypes.c_void_p()
+        L.zav1_event_create(ctypes.byref(e))
+        ev.append(e.value)
+    phase_ms = np.zeros(len(phases))
+    steps = args.lf_steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.zav1_event_record(ev[0], stream)
+        for i, ph in enumerate(phases):
+            f.run(stream, stages=(ph,))
+            L.zav1_event_record(ev[i + 1], stream)
+        zav1.sync(stream)
+        for i in range(len(phases)):
+            ms = ctypes.c_float()
+            L.zav1_event_elapsed_ms(ev[i], ev[i + 1], ctypes.byref(ms))
+            phase_ms[i] += ms.value
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1000.0 / steps
+    phase_ms /= steps
+    n = bw * bw
+    npred = f.predictions()
+    ntx = f.n_tx
+    # algorithmic HBM bytes per launch (16-bit samples; job records read once)
+    bytes_of = {'pred': npred * (2 * n + (2 * bw + 2 * bw + 1) * 2 + 32),
+                'satd': npred * (2 * n + 2 * n + 16 + 4),
+                'select': f.n_units * nc * 8 + ntx * 16 + f.n_units * k * 9,
+                'tx': ntx * (2 * n + 2 * n + 16 + 13)}
+    dom = phases[int(np.argmax(phase_ms))]
+    dom_s = phase_ms[phases.index(dom)] / 1000.0
+    kern = {'pred': 'k_intra_pred', 'satd': 'k_satd<16, UnitTiles<uint16_t>>', 'select': 'k_fast_select',
+            'tx': 'k_txunit<16,16,uint16_t>'}[dom]
+    gbs = bytes_of[dom] / dom_s / 1e9
+    out = {'workload': 'intra funnel MDS0 -> MDS1: 2048x2048 10-bit luma (config2), all 16x16 units, 61 '
+                       'candidates each (13 modes + angle deltas, synthetic neighbour canvas and mode rates), '
+                       'Hadamard SATD, fast cost + exchange-sort top-3, DCT_DCT quantize_b units at qp 40',
+           'value': round(S * S / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s', 'steps': steps,
+           'ms_per_step': round(ms_step, 3), 'predictions_per_step': int(npred), 'tx_units_per_step': int(ntx),
+           'phase_ms': {ph: round(float(phase_ms[i]), 4) for i, ph in enumerate(phases)},
+           'roofline': {'bound': 'hbm', 'kernel': kern, 'unit': 'GB/s', 'achieved': round(gbs, 1),
+                        'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 4), 'traffic': None,
+                        'avg_launch_us': round(dom_s * 1e6, 2), 'algorithmic_bytes_per_launch': int(bytes_of[dom])}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        c = S
+        xy = fm.unit_grid(c, c, bw, bw)
+        jobs = np.zeros(len(xy) * nc, O.INTRA_JOB)
+        for u, (x, y) in enumerate(xy.tolist()):
+            nt, ntr, nl, nbl = fm.raster_availability(x, y, bw, bw, c)
+            for m, (mode, delta, fi) in enumerate(cands):
+                jobs[u * nc + m] = (x, y, bw, bw, mode, delta, fi, 1, 0, 0, nt, ntr, nl, nbl, 0, 0)
+        threads = min(os.cpu_count() or 1, 16)
+        os.environ['OMP_NUM_THREADS'] = str(threads)
+        s_c, r_c = np.ascontiguousarray(src[:c, :c]), np.ascontiguousarray(recon[:c, :c])
+        tc0 = time.perf_counter()
+        O.md_funnel(s_c, r_c, 10, jobs, nc, rates, lam, k, f.tx_size, 0, qi)
+        tc1 = time.perf_counter()
+        out['cpu_baseline'] = {'value': round(c * c / 1e6 / (tc1 - tc0), 4), 'unit': 'MP/s', 'cores': threads,
+                               'kind': 'port', 'sample': '%dx%d (whole) frame, same chain (oracle C, OpenMP '
+                                                         'over units)' % (c, c),
+                               'seconds': round(tc1 - tc0, 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -196,6 +290,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-lf', action='store_true')
+    ap.add_argument('--no-funnel', action='store_true')
     ap.add_argument('--lf-steps', type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -337,6 +432,8 @@ def main():
                                'seconds': round(tc1 - tc0, 2)}
     if not args.no_lf:
         out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=not args.no_cpu)
+    if not args.no_funnel:
+        out['md_funnel'] = md_funnel_stage(zav1, synth, args, cpu=not args.no_cpu)
     print(json.dumps(out))
 
 

@@ -207,6 +207,39 @@ int zav1_txunit_batch(const void *d_src, int stride, int plane_w, int plane_h, i
                       uint16_t *d_recon, zav1_stream_t s);
 
 /* ------------------------------------------------------------------------ */
+/* Mode-decision funnel, MDS0 (batched).                                     */
+/* ------------------------------------------------------------------------ */
+/* MDS0 SATD of (unit, prediction) jobs of one w x h unit size (w, h powers
+ * of two in 4..64): residual d_src - prediction over square tiles of
+ * min(w, h, 32), raster order, aom Hadamard + SATD per tile, summed.
+ * Replaces hadamard_satd / hadamard_satd_hbd
+ * (rust/crates/svtav1-encoder/src/leaf_funnel/predict.rs:143 / :188) over
+ * every candidate of every unit of the fast loop (mod.rs:2084).  Jobs use
+ * zav1_txunit_job's x, y, pred_off (the other fields are ignored); d_satd
+ * (u32 per job) is 0xffffffff for a job outside the plane or d_pred.
+ * d_src is u8 at bd 8, u16 at bd 10; d_pred u16 (pred_len samples). */
+int zav1_pred_satd_batch(const void *d_src, int stride, int plane_w, int plane_h, int bd, int w, int h,
+                         const uint16_t *d_pred, size_t pred_len, const zav1_txunit_job *d_jobs, int n_jobs,
+                         uint32_t *d_satd, zav1_stream_t s);
+
+/* MDS0 survivor selection on the device.  d_satd / d_rate (nullable: rate
+ * 0) hold n_cands entries per unit, unit-major, in candidate generation
+ * order; the fast cost is rdcost(lambda, rate, satd << 4) =
+ * ((rate * lambda + 256) >> 9) + (satd << 11) (mod.rs:2181,
+ * rate_tables.rs:1033), an invalid SATD costing UINT64_MAX.  The first k
+ * positions of the reference's exchange sort over those costs
+ * (c_exchange_sort_by, rate_tables.rs:108, as sort_lane uses it at
+ * mod.rs:2994: not stable on ties) go to d_sel[u*k + r] (candidate index)
+ * and d_sel_cost[u*k + r] (nullable), and the transform-unit jobs of the
+ * survivors, one per host-side type types[t] (n_types <= 16), to
+ * d_jobs[(u*k + r)*n_types + t]: x, y from d_unit_xy[2u], [2u+1],
+ * pred_off = (u*n_cands + cand)*w*h, crop w x h, ready for
+ * zav1_txunit_batch.  1 <= k <= n_cands <= 64. */
+int zav1_md_fast_select(const uint32_t *d_satd, const uint32_t *d_rate, uint64_t lambda, int n_units, int n_cands,
+                        int k, const int32_t *d_unit_xy, int w, int h, const uint8_t *types, int n_types,
+                        uint8_t *d_sel, uint64_t *d_sel_cost, zav1_txunit_job *d_jobs, zav1_stream_t s);
+
+/* ------------------------------------------------------------------------ */
 /* Intra prediction (batched).                                               */
 /* ------------------------------------------------------------------------ */
 /* One transform unit.  Replaces the per-unit predictor routing of

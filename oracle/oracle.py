@@ -144,6 +144,64 @@ def block_satd(src, pred, hbd):
     return lib().or_block_satd(s.ctypes.data, w, p.ctypes.data, w, w, h, int(hbd))
 
 
+def _bind_funnel(L):
+    L.or_rdcost.argtypes = [ctypes.c_uint64] * 3
+    L.or_rdcost.restype = ctypes.c_uint64
+    L.or_exchange_sort.argtypes = [_vp, ctypes.c_int, _vp]
+    L.or_md_fast_select.argtypes = [_vp, _vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
+
+
+def rdcost(lam, rate, dist):
+    L = lib()
+    _bind_funnel(L)
+    return L.or_rdcost(lam, rate, dist)
+
+
+def exchange_sort(costs):
+    """Candidate order after the reference's exchange sort of `costs`."""
+    L = lib()
+    _bind_funnel(L)
+    c = np.ascontiguousarray(costs, dtype=np.uint64)
+    idx = np.zeros(len(c), np.int32)
+    L.or_exchange_sort(c.ctypes.data, len(c), idx.ctypes.data)
+    return idx
+
+
+def md_fast_select(satd, rate, lam, n_cands, k):
+    """(sel [n_units, k] u8, sel_cost [n_units, k] u64) of the MDS0 selection."""
+    L = lib()
+    _bind_funnel(L)
+    s = np.ascontiguousarray(satd, dtype=np.uint32).ravel()
+    r = None if rate is None else np.ascontiguousarray(rate, dtype=np.uint32).ravel()
+    n_units = s.size // n_cands
+    sel = np.zeros((n_units, k), np.uint8)
+    cost = np.zeros((n_units, k), np.uint64)
+    L.or_md_fast_select(s.ctypes.data, None if r is None else r.ctypes.data, lam, n_units, n_cands, k,
+                        sel.ctypes.data, cost.ctypes.data)
+    return sel, cost
+
+
+def md_funnel(src, recon, bd, intra_jobs, n_cands, rate, lam, k, tx_size, tx_type, qindex, use_b=True):
+    """MDS0 -> MDS1 chain over units (jobs unit-major).  Returns sel, sel_cost,
+    dist, eob, cul, each [n_units, k]."""
+    L = lib()
+    L.or_md_funnel.argtypes = ([_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                ctypes.c_uint64] + [ctypes.c_int] * 5 + [_vp] * 5)
+    s, r = np.ascontiguousarray(src), np.ascontiguousarray(recon)
+    j = np.ascontiguousarray(intra_jobs).view(INTRA_JOB)
+    nu = len(j) // n_cands
+    rt = None if rate is None else np.ascontiguousarray(np.broadcast_to(rate, (nu, n_cands)), dtype=np.uint32)
+    sel = np.zeros((nu, k), np.uint8)
+    cost = np.zeros((nu, k), np.uint64)
+    dist = np.zeros((nu, k), np.uint64)
+    eob = np.zeros((nu, k), np.int32)
+    cul = np.zeros((nu, k), np.uint8)
+    L.or_md_funnel(s.ctypes.data, r.ctypes.data, s.shape[1], bd, j.ctypes.data, nu, n_cands,
+                   None if rt is None else rt.ctypes.data, lam, k, tx_size, tx_type, qindex, int(use_b),
+                   sel.ctypes.data, cost.ctypes.data, dist.ctypes.data, eob.ctypes.data, cul.ctypes.data)
+    return sel, cost, dist, eob, cul
+
+
 def _bind_txsearch(L):
     L.or_scan.argtypes = [ctypes.c_int, ctypes.c_int, np.ctypeslib.ndpointer(dtype=np.uint16, flags='C_CONTIGUOUS')]
     L.or_scan_class.argtypes = [ctypes.c_int]

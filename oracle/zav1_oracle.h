@@ -41,6 +41,11 @@
  *   Wiener / unit SSE   numpy covariance of the windowed samples; unit SSE
  *                       and unit limits against the reference's C captures
  *                       (tests/golden/lr).
+ *   MDS0 selection      the exchange sort's documented behaviour
+ *                       (lf/rate_tables.rs:97-104): a permutation, costs
+ *                       non-decreasing, equal to a stable sort on distinct
+ *                       keys, and the stated tie displacement on a worked
+ *                       case; rdcost by hand-computed values.
  *   self-guided stats   PARITY UNPINNED: no reference twin; checked only by
  *                       invariants (flat input, projection identities).
  */
@@ -182,6 +187,22 @@ uint64_t or_sse_region(const void *a, const void *b, int pw, int bd, int hs, int
 int or_sgr_param(int set, int k);
 void or_sgr_unit_stats(const void *dgd, const void *src, int pw, int ph, int bd, int hs, int he, int vs, int ve,
                        int set, int64_t *out);
+
+/* ---- mode-decision funnel, MDS0 (funnel.c) ------------------------------ */
+/* lf/rate_tables.rs:1033 rdcost. */
+uint64_t or_rdcost(uint64_t lambda, uint64_t rate, uint64_t dist);
+/* lf/rate_tables.rs:108 c_exchange_sort_by over idx = 0..n-1. */
+void or_exchange_sort(uint64_t *cost_of, int n, int *idx);
+/* Per unit: fast costs rdcost(lambda, rate, satd << 4) (0xffffffff SATD ->
+ * UINT64_MAX), exchange-sorted; first k indices / costs out. n_cands <= 64. */
+void or_md_fast_select(const uint32_t *satd, const uint32_t *rate, uint64_t lambda, int n_units, int n_cands, int k,
+                       uint8_t *sel, uint64_t *sel_cost);
+/* MDS0 -> MDS1 over units: jobs unit-major (n_cands each, same w x h),
+ * predictions from recon, SATD vs src, fast select, survivors' transform
+ * units (tx_type, quantize_b when use_b, frequency distortion). */
+void or_md_funnel(const void *src, const void *recon, int stride, int bd, const or_intra_job *jobs, int n_units,
+                  int n_cands, const uint32_t *rate, uint64_t lambda, int k, int tx_size, int tx_type, int qindex,
+                  int use_b, uint8_t *sel, uint64_t *sel_cost, uint64_t *dist, int32_t *eob, uint8_t *cul);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,20 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     assert L.zav1_wiener_stats(fake, fake, 64, 64, 8, 6, 64, 0, fake, fake, None) == zav1.ZAV1_EINVAL
     assert L.zav1_wiener_stats(fake, fake, 64, 64, 8, 7, 512, 0, fake, fake, None) == zav1.ZAV1_EINVAL
     assert L.zav1_sgr_stats(fake, fake, 64, 64, 12, 64, 0, fake, None) == zav1.ZAV1_EINVAL
+    # MDS0 funnel: unit sides 4..64 powers of two, 1 <= k <= n_cands <= 64, types 0..15
+    assert L.zav1_pred_satd_batch(fake, 64, 64, 64, 8, 12, 8, fake, 64, fake, 1, fake, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_pred_satd_batch(fake, 64, 64, 64, 9, 8, 8, fake, 64, fake, 1, fake, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_pred_satd_batch(fake, 60, 64, 64, 8, 8, 8, fake, 64, fake, 1, fake, None) == zav1.ZAV1_EINVAL
+    ty = (ctypes.c_uint8 * 2)(0, 9)
+    bad_ty = (ctypes.c_uint8 * 1)(16)
+    assert L.zav1_md_fast_select(fake, None, 1, 4, 65, 3, fake, 8, 8, ty, 2, fake, None, fake,
+                                 None) == zav1.ZAV1_EINVAL
+    assert L.zav1_md_fast_select(fake, None, 1, 4, 13, 14, fake, 8, 8, ty, 2, fake, None, fake,
+                                 None) == zav1.ZAV1_EINVAL
+    assert L.zav1_md_fast_select(fake, None, 1, 4, 13, 3, fake, 8, 8, bad_ty, 1, fake, None, fake,
+                                 None) == zav1.ZAV1_EINVAL
+    assert L.zav1_md_fast_select(fake, None, 1, 4, 13, 3, fake, 8, 8, ty, 0, fake, None, fake,
+                                 None) == zav1.ZAV1_EINVAL
 
 
 def test_rest_unit_grid_matches_oracle(zav1, oracle):

@@ -1,0 +1,166 @@
+"""GPU parity of the mode-decision funnel's MDS0 stages against the oracle:
+the per-(unit, prediction) Hadamard SATD for every unit shape, the fast-cost
+survivor selection (exchange-sort order, ties, invalid jobs, with and
+without rates) and the MDS0 -> MDS1 chain of IntraFunnel end to end
+(bit-exact)."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_spec = importlib.util.spec_from_file_location('zav1_funnel', os.path.join(ROOT, 'zenav1-svt_amd', 'funnel.py'))
+funnel = importlib.util.module_from_spec(_spec)
+_spec.loader.exec_module(funnel)
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [4, 8, 16, 32, 64]
+
+
+def _plane(rng, bd, ph, pw):
+    mx = (1 << bd) - 1
+    yy, xx = np.mgrid[0:ph, 0:pw]
+    p = (np.sin(xx / 6.0) * np.cos(yy / 9.0) * 0.4 + 0.5) * mx + rng.normal(0, 10 << (bd - 8), (ph, pw))
+    return np.clip(p, 0, mx).astype(np.uint8 if bd == 8 else np.uint16)
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_pred_satd_batch_all_shapes(gpu, oracle, bd):
+    z = gpu
+    rng = np.random.default_rng(40 + bd)
+    mx = (1 << bd) - 1
+    PH, PW = 160, 176
+    plane = _plane(rng, bd, PH, PW)
+    d_src = z.DeviceArray.from_host(plane)
+    for w in DIMS:
+        for h in DIMS:
+            n = 12
+            jobs = np.zeros(n, z.TXUNIT_JOB)
+            preds = []
+            for j in range(n):
+                x, y = int(rng.integers(0, PW - w + 1)), int(rng.integers(0, PH - h + 1))
+                blk = plane[y:y + h, x:x + w].astype(np.int64)
+                if j % 3 == 0:   # full-range noise: exercises the 16-bit wrap / saturation rules
+                    p = rng.integers(0, mx + 1, blk.shape)
+                else:
+                    p = blk + rng.integers(-40, 41, blk.shape)
+                preds.append(np.clip(p, 0, mx).astype(np.uint16))
+                jobs[j] = (x, y, j * w * h, 0, w, h, 0)
+            jobs[3]['y'] = PH - h + 1            # out of the plane
+            jobs[5]['pred_off'] = n * w * h      # past the prediction buffer
+            pred = np.concatenate([p.ravel() for p in preds])
+            d_pred = z.DeviceArray.from_host(pred)
+            d_jobs = z.DeviceArray.from_host(jobs.view(np.uint8))
+            d_satd = z.DeviceArray(n, np.uint32)
+            z.pred_satd_batch(d_src, PW, PW, PH, bd, w, h, d_pred, d_jobs, n, d_satd)
+            got = d_satd.to_host()
+            for j in range(n):
+                if j in (3, 5):
+                    assert got[j] == 0xffffffff, (w, h, j)
+                    continue
+                x, y = int(jobs['x'][j]), int(jobs['y'][j])
+                blk = np.ascontiguousarray(plane[y:y + h, x:x + w])
+                pj = preds[j] if bd == 10 else preds[j].astype(np.uint8)
+                ref = oracle.block_satd(blk, pj, bd == 10)
+                assert int(got[j]) == ref, (w, h, bd, j, int(got[j]), ref)
+
+
+@pytest.mark.parametrize('with_rate', [False, True])
+def test_md_fast_select_matches_oracle(gpu, oracle, with_rate):
+    z = gpu
+    rng = np.random.default_rng(7 + with_rate)
+    nu, nc, k, w, h = 1000, 13, 3, 16, 8
+    satd = rng.integers(0, 6, (nu, nc)).astype(np.uint32)      # heavy ties
+    satd[nu // 2:] = rng.integers(0, 1 << 20, (nu - nu // 2, nc))
+    satd[::11, 2] = 0xffffffff
+    satd[5] = 0xffffffff                                        # a unit with no valid candidate
+    rate = rng.integers(0, 4000, (nu, nc)).astype(np.uint32) if with_rate else None
+    lam = 70000
+    xy = rng.integers(0, 1000, (nu, 2)).astype(np.int32)
+    types = np.array([0, 9, 3], np.uint8)
+    A = z.DeviceArray
+    d_satd, d_xy = A.from_host(satd), A.from_host(xy)
+    d_rate = A.from_host(rate) if with_rate else None
+    d_sel, d_cost = A(nu * k, np.uint8), A(nu * k, np.uint64)
+    d_jobs = A(nu * k * len(types) * z.TXUNIT_JOB.itemsize, np.uint8)
+    z.md_fast_select(d_satd, d_rate, lam, nu, nc, k, d_xy, w, h, types, d_sel, d_cost, d_jobs)
+    sel, cost = d_sel.to_host().reshape(nu, k), d_cost.to_host().reshape(nu, k)
+    jobs = d_jobs.to_host().view(z.TXUNIT_JOB).reshape(nu, k, len(types))
+    rsel, rcost = oracle.md_fast_select(satd, rate, lam, nc, k)
+    assert np.array_equal(sel, rsel)
+    assert np.array_equal(cost, rcost)
+    assert np.array_equal(jobs['x'], np.broadcast_to(xy[:, 0, None, None], jobs.shape))
+    assert np.array_equal(jobs['y'], np.broadcast_to(xy[:, 1, None, None], jobs.shape))
+    off = (np.arange(nu)[:, None] * nc + rsel.astype(np.int64)) * (w * h)
+    assert np.array_equal(jobs['pred_off'], np.broadcast_to(off[:, :, None], jobs.shape))
+    assert np.array_equal(jobs['tx_type'], np.broadcast_to(types, jobs.shape))
+    assert np.all(jobs['crop_w'] == w) and np.all(jobs['crop_h'] == h)
+
+
+@pytest.mark.parametrize('bd,w,h', [(8, 8, 8), (8, 16, 16), (10, 16, 8), (10, 32, 32), (8, 4, 16)])
+def test_intra_funnel_chain(gpu, oracle, bd, w, h):
+    z = gpu
+    rng = np.random.default_rng(bd * 1000 + w * 10 + h)
+    PH, PW = 128, 96
+    src = _plane(rng, bd, PH, PW)
+    # flat areas give exact SATD ties between modes
+    src[:32, :48] = src[0, 0]
+    mx = (1 << bd) - 1
+    recon = np.clip(src.astype(np.int64) + rng.integers(-3, 4, src.shape), 0, mx).astype(src.dtype)
+    nc = len(funnel.REGULAR_MODES)
+    rates = rng.integers(0, 600, (1, nc)).astype(np.uint32)
+    qindex = 120
+    f = funnel.IntraFunnel(z, src, recon, bd, w, h, qindex, k=3, rates=rates, lam=50000)
+    f.run()
+    r = f.results()
+    pred_all = f.d_pred.to_host()
+    n = w * h
+    oj = f.intra_jobs.view(oracle.INTRA_JOB)
+    for u in range(f.n_units):
+        x, y = f.xy[u].tolist()
+        blk = np.ascontiguousarray(src[y:y + h, x:x + w])
+        for m in range(nc):
+            j = u * nc + m
+            pred = pred_all[j * n:(j + 1) * n].reshape(h, w)
+            if u % 5 == 0:
+                assert np.array_equal(pred, oracle.intra_predict(recon, bd, oj[j:j + 1])), (u, m)
+            pj = pred if bd == 10 else pred.astype(np.uint8)
+            assert int(r['satd'][u, m]) == oracle.block_satd(blk, pj, bd == 10), (u, m)
+    rsel, rcost = oracle.md_fast_select(r['satd'], f.rates, f.lam, nc, 3)
+    assert np.array_equal(r['sel'], rsel)
+    assert np.array_equal(r['sel_cost'], rcost)
+    for u in range(f.n_units):
+        x, y = f.xy[u].tolist()
+        blk = np.ascontiguousarray(src[y:y + h, x:x + w])
+        for rank in range(3):
+            m = int(rsel[u, rank])
+            pred = pred_all[(u * nc + m) * n:(u * nc + m + 1) * n].reshape(h, w)
+            rd, re, rc, _, _ = oracle.tx_unit_full(blk, pred, f.tx_size, 0, qindex, bd, use_b=True, need_recon=False)
+            got = (int(r['dist'][u, rank, 0]), int(r['eob'][u, rank, 0]), int(r['cul'][u, rank, 0]))
+            assert got == (rd, re, rc), (u, rank, m, got, (rd, re, rc))
+    # and the oracle's one-call chain
+    osel, ocost, odist, oeob, ocul = oracle.md_funnel(src, recon, bd, f.intra_jobs.view(oracle.INTRA_JOB), nc,
+                                                      f.rates, f.lam, 3, f.tx_size, 0, qindex)
+    assert np.array_equal(r['sel'], osel) and np.array_equal(r['sel_cost'], ocost)
+    assert np.array_equal(r['dist'][:, :, 0], odist)
+    assert np.array_equal(r['eob'][:, :, 0], oeob) and np.array_equal(r['cul'][:, :, 0], ocul)
+
+
+def test_intra_funnel_angle_candidates(gpu, oracle):
+    """61 candidates per unit (regular modes + angle deltas) at 10 bits."""
+    z = gpu
+    rng = np.random.default_rng(61)
+    src = _plane(rng, 10, 64, 96)
+    recon = np.clip(src.astype(np.int64) + rng.integers(-5, 6, src.shape), 0, 1023).astype(np.uint16)
+    cands = funnel.angle_candidates()
+    f = funnel.IntraFunnel(z, src, recon, 10, 16, 16, 150, k=4, cands=cands, lam=90000,
+                           rates=rng.integers(0, 900, (1, len(cands))).astype(np.uint32))
+    f.run()
+    r = f.results()
+    osel, ocost, odist, oeob, ocul = oracle.md_funnel(src, recon, 10, f.intra_jobs.view(oracle.INTRA_JOB),
+                                                      len(cands), f.rates, f.lam, 4, f.tx_size, 0, 150)
+    assert np.array_equal(r['sel'], osel) and np.array_equal(r['sel_cost'], ocost)
+    assert np.array_equal(r['dist'][:, :, 0], odist)
+    assert np.array_equal(r['eob'][:, :, 0], oeob) and np.array_equal(r['cul'][:, :, 0], ocul)

@@ -1,0 +1,95 @@
+"""Oracle checks of the MDS0 selection (fast cost + the reference's exchange
+sort, lf/rate_tables.rs:97-108, :1033).  The exchange sort is pinned by the
+properties its reference documentation states: it yields a permutation with
+non-decreasing costs, equals a stable ascending sort when all costs differ,
+and on a tie displaces the first member of a tie group behind the rest of
+its group when a strictly smaller cost follows the group."""
+import numpy as np
+import pytest
+
+
+def test_rdcost_hand_values(oracle):
+    assert oracle.rdcost(0, 123, 7) == 7 << 7
+    assert oracle.rdcost(1000, 3, 0) == (3000 + 256) >> 9 == 6
+    assert oracle.rdcost(512, 1, 1) == 1 + 128          # (512 + 256) >> 9 = 1
+    assert oracle.rdcost(511, 1, 0) == 1                  # (511 + 256) >> 9 = 1
+    assert oracle.rdcost(255, 1, 0) == 0                  # (255 + 256) >> 9 = 0
+
+
+def test_exchange_sort_worked_ties(oracle):
+    # a strictly smaller cost after a tie group moves the group's first
+    # member to its position: [5, 5, 3] -> 2, 1, 0 (a stable sort gives 2, 0, 1)
+    assert oracle.exchange_sort([5, 5, 3]).tolist() == [2, 1, 0]
+    assert oracle.exchange_sort([4, 4, 4, 1]).tolist() == [3, 1, 2, 0]
+    # no smaller cost after the group: ties keep generation order
+    assert oracle.exchange_sort([1, 4, 4, 4]).tolist() == [0, 1, 2, 3]
+    assert oracle.exchange_sort([7, 7]).tolist() == [0, 1]
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_exchange_sort_properties(oracle, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 65))
+    distinct = rng.permutation(10 * n)[:n].astype(np.uint64)
+    idx = oracle.exchange_sort(distinct)
+    assert idx.tolist() == np.argsort(distinct, kind='stable').tolist()
+    tied = rng.integers(0, 4, n).astype(np.uint64)
+    idx = oracle.exchange_sort(tied)
+    assert sorted(idx.tolist()) == list(range(n))
+    assert np.all(np.diff(tied[idx].astype(np.int64)) >= 0)
+
+
+def test_md_fast_select_invalid_last_and_prefix(oracle):
+    rng = np.random.default_rng(5)
+    n_units, nc, k = 40, 13, 3
+    satd = rng.integers(0, 50, (n_units, nc)).astype(np.uint32)
+    satd[::7, 0] = 0xffffffff
+    rate = rng.integers(0, 3000, (n_units, nc)).astype(np.uint32)
+    lam = 12345
+    sel, cost = oracle.md_fast_select(satd, rate, lam, nc, k)
+    for u in range(n_units):
+        c = [(~np.uint64(0)) if satd[u, m] == 0xffffffff else oracle.rdcost(lam, int(rate[u, m]), int(satd[u, m]) << 4)
+             for m in range(nc)]
+        order = oracle.exchange_sort(np.array(c, np.uint64))
+        assert sel[u].tolist() == order[:k].tolist()
+        assert cost[u].tolist() == [int(c[m]) for m in order[:k]]
+        assert 0 not in sel[u].tolist() or satd[u, 0] != 0xffffffff
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_md_funnel_chain_composes(oracle, bd):
+    """The oracle's one-call chain equals its pieces composed by hand."""
+    rng = np.random.default_rng(bd)
+    mx = (1 << bd) - 1
+    dt = np.uint8 if bd == 8 else np.uint16
+    H = W = 48
+    src = rng.integers(0, mx + 1, (H, W)).astype(dt)
+    src[:16, :32] = mx // 3
+    recon = np.clip(src.astype(np.int64) + rng.integers(-2, 3, src.shape), 0, mx).astype(dt)
+    bw = bh = 16
+    nc, k = 13, 3
+    jobs = np.zeros(9 * nc, oracle.INTRA_JOB)
+    units = [(x, y) for y in range(0, H, bh) for x in range(0, W, bw)]
+    for u, (x, y) in enumerate(units):
+        for m in range(nc):
+            jobs[u * nc + m] = (x, y, bw, bh, m, 0, 5, 1, 0, 0, bw if y else 0,
+                                min(bw, W - x - bw) if y else 0, bh if x else 0, 0, 0, 0)
+    rate = rng.integers(0, 500, (len(units), nc)).astype(np.uint32)
+    lam, qi = 30000, 100
+    sel, cost, dist, eob, cul = oracle.md_funnel(src, recon, bd, jobs, nc, rate, lam, k, 2, 0, qi)
+    satd = np.zeros((len(units), nc), np.uint32)
+    preds = {}
+    for u, (x, y) in enumerate(units):
+        blk = np.ascontiguousarray(src[y:y + bh, x:x + bw])
+        for m in range(nc):
+            p = oracle.intra_predict(recon, bd, jobs[u * nc + m:u * nc + m + 1]).astype(np.uint16)
+            preds[u, m] = p
+            satd[u, m] = oracle.block_satd(blk, p if bd == 10 else p.astype(np.uint8), bd == 10)
+    rsel, rcost = oracle.md_fast_select(satd, rate, lam, nc, k)
+    assert np.array_equal(sel, rsel) and np.array_equal(cost, rcost)
+    for u, (x, y) in enumerate(units):
+        blk = np.ascontiguousarray(src[y:y + bh, x:x + bw])
+        for r in range(k):
+            d, e, c, _, _ = oracle.tx_unit_full(blk, preds[u, int(sel[u, r])], 2, 0, qi, bd, use_b=True,
+                                                need_recon=False)
+            assert (int(dist[u, r]), int(eob[u, r]), int(cul[u, r])) == (d, e, c)

@@ -104,6 +104,9 @@ EXPORTS = {
     'zav1_cfl_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
     'zav1_txunit_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
                           _vp, _vp, _vp, _vp, _vp, _vp],
+    'zav1_pred_satd_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _vp, _vp],
+    'zav1_md_fast_select': [_vp, _vp, ctypes.c_uint64, _i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp,
+                            _vp],
     'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
     'zav1_deblock_level_sse': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _i32, _vp, ctypes.c_size_t, _vp, _vp],
     'zav1_deblock_search_level': [_vp, _vp] + [_i32] * 6 + [_vp, _i32, _vp, ctypes.c_size_t, _vp,
@@ -330,6 +333,22 @@ def txunit_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, d_jobs, n
     check(lib().zav1_txunit_batch(d_src.ptr, stride, plane_w, plane_h, bd, tx_size, d_pred.ptr,
                                   d_pred.nbytes // 2, d_jobs.ptr, n_jobs, qindex, flags, d_dist.ptr, d_eob.ptr,
                                   d_cul.ptr, ptr(d_qcoeff), ptr(d_recon), stream), 'txunit_batch')
+
+
+def pred_satd_batch(d_src, stride, plane_w, plane_h, bd, w, h, d_pred, d_jobs, n_jobs, d_satd, stream=None):
+    """MDS0 SATD per (unit, prediction) job (zav1_pred_satd_batch); d_satd u32."""
+    check(lib().zav1_pred_satd_batch(d_src.ptr, stride, plane_w, plane_h, bd, w, h, d_pred.ptr, d_pred.nbytes // 2,
+                                     d_jobs.ptr, n_jobs, d_satd.ptr, stream), 'pred_satd_batch')
+
+
+def md_fast_select(d_satd, d_rate, lam, n_units, n_cands, k, d_unit_xy, w, h, types, d_sel, d_sel_cost, d_jobs,
+                   stream=None):
+    """MDS0 survivors per unit and their transform-unit jobs (zav1_md_fast_select)."""
+    tl = np.ascontiguousarray(types, dtype=np.uint8)
+    check(lib().zav1_md_fast_select(d_satd.ptr, d_rate.ptr if d_rate is not None else None, int(lam), n_units,
+                                    n_cands, k, d_unit_xy.ptr, w, h, tl.ctypes.data, len(tl), d_sel.ptr,
+                                    d_sel_cost.ptr if d_sel_cost is not None else None, d_jobs.ptr, stream),
+          'md_fast_select')
 
 
 def deblock_level_sse(d_recon, d_src, plane_w, plane_h, bd, plane, ss, sharpness, geom, levels, d_work, d_sse,

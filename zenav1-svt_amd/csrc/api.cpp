@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
+#include <climits>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -505,6 +508,39 @@ int zav1_txunit_batch(const void* d_src, int stride, int plane_w, int plane_h, i
                             reinterpret_cast<const zav1::TxUnitJob*>(d_jobs), n_jobs,
                             rows + (bd == 10 ? 256 : 0) + qindex, iscan, flags, d_dist, d_eob, d_cul, d_qcoeff,
                             d_recon, S(s)));
+  return ZAV1_OK;
+}
+
+static bool funnel_dim(int v) { return v == 4 || v == 8 || v == 16 || v == 32 || v == 64; }
+
+int zav1_pred_satd_batch(const void* d_src, int stride, int plane_w, int plane_h, int bd, int w, int h,
+                         const uint16_t* d_pred, size_t pred_len, const zav1_txunit_job* d_jobs, int n_jobs,
+                         uint32_t* d_satd, zav1_stream_t s) {
+  if (!funnel_dim(w) || !funnel_dim(h) || (bd != 8 && bd != 10) || n_jobs < 0 || plane_w <= 0 || plane_h <= 0 ||
+      stride < plane_w || (n_jobs && (!d_src || !d_pred || !d_jobs || !d_satd)))
+    return ZAV1_EINVAL;
+  const int t = std::min(std::min(w, h), 32);
+  if ((long)n_jobs * (w / t) * (h / t) > INT32_MAX) return ZAV1_EINVAL;
+  ZCHECK(zav1::pred_satd_batch(d_src, stride, plane_w, plane_h, bd, w, h, d_pred, (long)pred_len,
+                               reinterpret_cast<const zav1::TxUnitJob*>(d_jobs), n_jobs, d_satd, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_md_fast_select(const uint32_t* d_satd, const uint32_t* d_rate, uint64_t lambda, int n_units, int n_cands,
+                        int k, const int32_t* d_unit_xy, int w, int h, const uint8_t* types, int n_types,
+                        uint8_t* d_sel, uint64_t* d_sel_cost, zav1_txunit_job* d_jobs, zav1_stream_t s) {
+  if (n_units < 0 || n_cands < 1 || n_cands > 64 || k < 1 || k > n_cands || !funnel_dim(w) || !funnel_dim(h) ||
+      n_types < 1 || n_types > 16 || !types || (n_units && (!d_satd || !d_unit_xy || !d_sel || !d_jobs)))
+    return ZAV1_EINVAL;
+  if ((long)n_units * n_cands * w * h > UINT32_MAX) return ZAV1_EINVAL;  // pred_off range
+  zav1::TypeList tl{};
+  tl.n = n_types;
+  for (int i = 0; i < n_types; i++) {
+    if (types[i] > 15) return ZAV1_EINVAL;
+    tl.t[i] = types[i];
+  }
+  ZCHECK(zav1::fast_select(d_satd, d_rate, lambda, n_units, n_cands, k, d_unit_xy, w, h, tl, d_sel, d_sel_cost,
+                           reinterpret_cast<zav1::TxUnitJob*>(d_jobs), S(s)));
   return ZAV1_OK;
 }
 

@@ -151,14 +151,35 @@ __device__ __forceinline__ void had8_col(const int32_t* s, int16_t* o) {
   o[1] = (int16_t)(c2 - c6); o[5] = (int16_t)(c3 - c7);
 }
 
-// 8x8 tile at src (row stride `st`) -> 64 coefficients (i16-valued).
-__device__ __forceinline__ void had8_tile(const int16_t* __restrict__ src, int st, int32_t* out) {
+// Residual readers: a(r, c) is the i16 residual at row r, column c of a tile.
+// Packed: a stored i16 residual.  Diff: source minus prediction in 16 bits
+// (the reference's `src as i16 - pred as i16`).
+struct PackedRes {
+  const int16_t* p;
+  int st;
+  __device__ __forceinline__ int16_t operator()(int r, int c) const { return p[r * st + c]; }
+  __device__ __forceinline__ PackedRes at(int r0, int c0) const { return {p + r0 * st + c0, st}; }
+};
+template <typename Pix>
+struct DiffRes {
+  const Pix* s;
+  const uint16_t* q;
+  int ss, qs;
+  __device__ __forceinline__ int16_t operator()(int r, int c) const {
+    return (int16_t)((int16_t)s[r * ss + c] - (int16_t)q[r * qs + c]);
+  }
+  __device__ __forceinline__ DiffRes at(int r0, int c0) const { return {s + r0 * ss + c0, q + r0 * qs + c0, ss, qs}; }
+};
+
+// 8x8 tile of residual reader `a` -> 64 coefficients (i16-valued).
+template <class R>
+__device__ __forceinline__ void had8_tile(const R& a, int32_t* out) {
   int16_t buf[64];
   int32_t col[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
 #pragma unroll
-    for (int r = 0; r < 8; r++) col[r] = src[r * st + c];
+    for (int r = 0; r < 8; r++) col[r] = a(r, c);
     had8_col(col, buf + c * 8);
   }
   int16_t o[8];
@@ -172,7 +193,8 @@ __device__ __forceinline__ void had8_tile(const int16_t* __restrict__ src, int s
   }
 }
 
-__device__ __forceinline__ void had4_tile(const int16_t* __restrict__ src, int st, int32_t* out) {
+template <class R>
+__device__ __forceinline__ void had4_tile(const R& a, int32_t* out) {
   int16_t b1[16], b2[16];
   auto col4 = [](const int16_t* s, int stride, int16_t* o) {
     const int16_t x0 = (int16_t)((int16_t)(s[0] + s[stride]) >> 1);
@@ -186,7 +208,7 @@ __device__ __forceinline__ void had4_tile(const int16_t* __restrict__ src, int s
 #pragma unroll
   for (int r = 0; r < 4; r++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) tile[r * 4 + c] = src[r * st + c];
+    for (int c = 0; c < 4; c++) tile[r * 4 + c] = a(r, c);
 #pragma unroll
   for (int i = 0; i < 4; i++) col4(tile + i, 4, b1 + 4 * i);
 #pragma unroll
@@ -201,29 +223,70 @@ __device__ __forceinline__ int16_t sat16(int32_t v) {
   return (int16_t)(v < -32768 ? -32768 : (v > 32767 ? 32767 : v));
 }
 
-template <int N>
-__global__ __launch_bounds__(256) void k_satd(const int16_t* __restrict__ res,
-                                              int32_t* __restrict__ coeff_out,
-                                              int32_t* __restrict__ satd_out, int n_jobs) {
-  constexpr int TPJ = N <= 8 ? 1 : (N / 8) * (N / 8);  // lanes per job
+// Tile sources for k_satd: tile(i, &a) binds reader `a` to tile i and says
+// whether the tile is live; emit(i, satd) stores its SATD.
+struct PackedTiles {
+  using Reader = PackedRes;
+  const int16_t* res;
+  int32_t* satd;
+  int n, nn;  // tiles, coefficients per tile
+  __device__ __forceinline__ bool tile(int i, int st, PackedRes* a) const {
+    *a = {res + (size_t)i * nn, st};
+    return i < n;
+  }
+  __device__ __forceinline__ void emit(int i, int32_t v) const { satd[i] = v; }
+};
+
+// Square tiles of (unit, prediction) jobs: tile i is tile k = i % tpj (raster
+// order) of job i / tpj; the job's SATD is the sum over its tiles, added
+// into out[] (0, or kSatdInvalid for a job outside the plane or the
+// prediction buffer, as k_satd_init leaves it).
+template <typename Pix>
+struct UnitTiles {
+  using Reader = DiffRes<Pix>;
+  const Pix* src;
+  int stride, pw, ph;
+  const uint16_t* pred;
+  long pred_len;
+  const TxUnitJob* jobs;
+  int w, h, t, tpr, tpj, n;  // n = tiles in all
+  uint32_t* out;
+  __device__ __forceinline__ bool tile(int i, int, DiffRes<Pix>* a) const {
+    if (i >= n) return false;
+    const int j = i / tpj, k = i - j * tpj;
+    const TxUnitJob jb = jobs[j];
+    if (!unit_inside(jb, w, h, pw, ph, pred_len)) return false;
+    const int kr = k / tpr, r0 = kr * t, c0 = (k - kr * tpr) * t;
+    *a = {src + (long)(jb.y + r0) * stride + jb.x + c0, pred + jb.pred_off + r0 * w + c0, stride, w};
+    return true;
+  }
+  __device__ __forceinline__ void emit(int i, int32_t v) const {
+    if (tpj == 1) out[i] = (uint32_t)v;
+    else atomicAdd(out + i / tpj, (uint32_t)v);
+  }
+};
+
+template <int N, class Src>
+__global__ __launch_bounds__(256) void k_satd(Src tiles, int32_t* __restrict__ coeff_out) {
+  constexpr int TPJ = N <= 8 ? 1 : (N / 8) * (N / 8);  // lanes per tile
   constexpr int JPW = 256 / TPJ;
   constexpr int NN = N * N;
   const int jl = threadIdx.x / TPJ, s = threadIdx.x - jl * TPJ;
   const int job = blockIdx.x * JPW + jl;
-  const bool active = job < n_jobs;
-  const int16_t* src = res + (size_t)job * NN;
+  typename Src::Reader src{};
+  const bool active = tiles.tile(job, N, &src);
   if constexpr (N <= 8) {
     if (!active) return;
     int32_t c[64];
-    if constexpr (N == 4) had4_tile(src, 4, c);
-    else had8_tile(src, 8, c);
+    if constexpr (N == 4) had4_tile(src, c);
+    else had8_tile(src, c);
     int32_t sum = 0;
 #pragma unroll
     for (int i = 0; i < NN; i++) {
       sum += c[i] < 0 ? -c[i] : c[i];
       if (coeff_out) coeff_out[(size_t)job * NN + i] = c[i];
     }
-    satd_out[job] = sum;
+    tiles.emit(job, sum);
   } else {
     __shared__ int32_t lds[JPW * NN];
     __shared__ int32_t part[256];
@@ -241,7 +304,7 @@ __global__ __launch_bounds__(256) void k_satd(const int16_t* __restrict__ res,
     }
     if (active) {
       int32_t c[64];
-      had8_tile(src + base_r * N + base_c, N, c);
+      had8_tile(src.at(base_r, base_c), c);
 #pragma unroll
       for (int i = 0; i < 64; i++) L[dst + i] = c[i];
     }
@@ -294,7 +357,7 @@ __global__ __launch_bounds__(256) void k_satd(const int16_t* __restrict__ res,
     if (active && s == 0) {
       int32_t tot = 0;
       for (int k = 0; k < TPJ; k++) tot += part[jl * TPJ + k];
-      satd_out[job] = tot;
+      tiles.emit(job, tot);
     }
   }
 }
@@ -303,14 +366,58 @@ hipError_t satd_batch(const int16_t* res, int32_t* coeff, int32_t* satd, int n, 
                       hipStream_t st) {
   if (n_jobs <= 0) return hipSuccess;
   auto grid_for = [&](int tpj) { return (n_jobs + (256 / tpj) - 1) / (256 / tpj); };
+  const PackedTiles src{res, satd, n_jobs, n * n};
   switch (n) {
-    case 4: hipLaunchKernelGGL((k_satd<4>), dim3(grid_for(1)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
-    case 8: hipLaunchKernelGGL((k_satd<8>), dim3(grid_for(1)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
-    case 16: hipLaunchKernelGGL((k_satd<16>), dim3(grid_for(4)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
-    case 32: hipLaunchKernelGGL((k_satd<32>), dim3(grid_for(16)), dim3(256), 0, st, res, coeff, satd, n_jobs); break;
+    case 4: hipLaunchKernelGGL((k_satd<4, PackedTiles>), dim3(grid_for(1)), dim3(256), 0, st, src, coeff); break;
+    case 8: hipLaunchKernelGGL((k_satd<8, PackedTiles>), dim3(grid_for(1)), dim3(256), 0, st, src, coeff); break;
+    case 16: hipLaunchKernelGGL((k_satd<16, PackedTiles>), dim3(grid_for(4)), dim3(256), 0, st, src, coeff); break;
+    case 32: hipLaunchKernelGGL((k_satd<32, PackedTiles>), dim3(grid_for(16)), dim3(256), 0, st, src, coeff); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// Per job: 0 (its tiles add into it) or kSatdInvalid.
+__global__ __launch_bounds__(256) void k_satd_init(const TxUnitJob* __restrict__ jobs, int n_jobs, int w, int h,
+                                                   int pw, int ph, long pred_len, uint32_t* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n_jobs) return;
+  out[j] = unit_inside(jobs[j], w, h, pw, ph, pred_len) ? 0u : kSatdInvalid;
+}
+
+template <typename Pix>
+static hipError_t launch_unit_satd(const UnitTiles<Pix>& u, hipStream_t st) {
+  auto grid_for = [&](int tpj) { return (u.n + (256 / tpj) - 1) / (256 / tpj); };
+  switch (u.t) {
+    case 4: hipLaunchKernelGGL((k_satd<4, UnitTiles<Pix>>), dim3(grid_for(1)), dim3(256), 0, st, u, nullptr); break;
+    case 8: hipLaunchKernelGGL((k_satd<8, UnitTiles<Pix>>), dim3(grid_for(1)), dim3(256), 0, st, u, nullptr); break;
+    case 16:
+      hipLaunchKernelGGL((k_satd<16, UnitTiles<Pix>>), dim3(grid_for(4)), dim3(256), 0, st, u, nullptr);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_satd<32, UnitTiles<Pix>>), dim3(grid_for(16)), dim3(256), 0, st, u, nullptr);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t pred_satd_batch(const void* src, int stride, int pw, int ph, int bd, int w, int h, const uint16_t* pred,
+                           long pred_len, const TxUnitJob* jobs, int n_jobs, uint32_t* satd, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  int t = w < h ? w : h;
+  t = t > 32 ? 32 : t;
+  const int tpr = w / t, tpj = tpr * (h / t);
+  if ((long)n_jobs * tpj > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_satd_init, dim3((n_jobs + 255) / 256), dim3(256), 0, st, jobs, n_jobs, w, h, pw, ph, pred_len,
+                     satd);
+  if (bd == 8)
+    return launch_unit_satd(UnitTiles<uint8_t>{(const uint8_t*)src, stride, pw, ph, pred, pred_len, jobs, w, h, t,
+                                               tpr, tpj, n_jobs * tpj, satd},
+                            st);
+  return launch_unit_satd(UnitTiles<uint16_t>{(const uint16_t*)src, stride, pw, ph, pred, pred_len, jobs, w, h, t,
+                                              tpr, tpj, n_jobs * tpj, satd},
+                          st);
 }
 
 }  // namespace zav1

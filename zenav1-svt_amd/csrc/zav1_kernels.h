@@ -124,6 +124,26 @@ struct TxUnitJob {
   uint8_t tx_type, crop_w, crop_h, pad;
 };
 enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u };
+// A w x h unit job reads plane rows y..y+h-1, columns x..x+w-1 and
+// prediction samples pred_off..pred_off+w*h-1.
+__host__ __device__ inline bool unit_inside(const TxUnitJob& j, int w, int h, int pw, int ph, long pred_len) {
+  return j.x >= 0 && j.y >= 0 && j.x <= pw - w && j.y <= ph - h && (long)j.pred_off + (long)w * h <= pred_len;
+}
+// SATD reported for a job outside the plane or the prediction buffer.
+constexpr uint32_t kSatdInvalid = 0xffffffffu;
+// MDS0 SATD of (unit, prediction) jobs of one w x h unit size: square
+// tiles of min(w, h, 32), Hadamard + SATD per tile, summed per job.
+hipError_t pred_satd_batch(const void* src, int stride, int pw, int ph, int bd, int w, int h, const uint16_t* pred,
+                           long pred_len, const TxUnitJob* jobs, int n_jobs, uint32_t* satd, hipStream_t st);
+// MDS0 selection: per unit, fast cost rdcost(lambda, rate, satd << 4) of
+// each of its n_cands candidates (kSatdInvalid -> UINT64_MAX), the first k
+// positions of the exchange sort over them into sel / sel_cost
+// [unit * k + rank], and the transform-unit jobs of those candidates for
+// every type of `types` into jobs[(unit * k + rank) * types.n + ti]
+// (prediction of candidate m of unit u at (u * n_cands + m) * w * h).
+hipError_t fast_select(const uint32_t* satd, const uint32_t* rate, uint64_t lambda, int n_units, int n_cands, int k,
+                       const int32_t* unit_xy, int w, int h, const TypeList& types, uint8_t* sel, uint64_t* sel_cost,
+                       TxUnitJob* jobs, hipStream_t st);
 // Fused transform units of one tx size: residual (src - pred) -> forward ->
 // fold -> quantize -> eob / cul level / frequency distortion, and, when
 // reconstructing, inverse -> recon clip -> spatial distortion.
