@@ -164,3 +164,44 @@ def test_intra_funnel_angle_candidates(gpu, oracle):
     assert np.array_equal(r['sel'], osel) and np.array_equal(r['sel_cost'], ocost)
     assert np.array_equal(r['dist'][:, :, 0], odist)
     assert np.array_equal(r['eob'][:, :, 0], oeob) and np.array_equal(r['cul'][:, :, 0], ocul)
+
+
+def test_mds0_matches_c_captures_at_origin(gpu):
+    """The device chain (predict -> SATD -> fast select) on the C encoder's
+    captured origin blocks of the 8-bit gradient: SATD = MDS0 dist, fast
+    cost = fc, survivors in POST0 order."""
+    import json
+    z = gpu
+    g = json.load(open(os.path.join(ROOT, 'tests', 'golden', 'funnel_mds0.json')))
+    A = z.DeviceArray
+    checked = 0
+    for f in g['files']:
+        r, c = np.mgrid[0:f['h'], 0:f['w']]
+        img = (((r * 255) // f['h']).astype(np.uint8) ^ ((c * 3) & 0x3f).astype(np.uint8)).astype(np.uint8)
+        d_img = A.from_host(img)
+        for b in f['blocks']:
+            if b['org'] != [0, 0]:
+                continue
+            s, cands = b['sq'], b['mds0']
+            nc = len(cands)
+            ij = np.zeros(nc, z.INTRA_JOB)
+            sj = np.zeros(nc, z.TXUNIT_JOB)
+            for m, (buf, mode, fi, dist, flr, fcr, fc) in enumerate(cands):
+                ij[m] = (0, 0, s, s, mode, 0, fi, 1, 0, 0, 0, 0, 0, 0, m * s * s, 0)
+                sj[m] = (0, 0, m * s * s, 0, s, s, 0)
+            d_pred = A(nc * s * s, np.uint16)
+            z.intra_pred_batch(d_img, f['w'], 8, A.from_host(ij.view(np.uint8)), nc, d_pred)
+            d_satd = A(nc, np.uint32)
+            z.pred_satd_batch(d_img, f['w'], f['w'], f['h'], 8, s, s, d_pred, A.from_host(sj.view(np.uint8)), nc,
+                              d_satd)
+            assert d_satd.to_host().tolist() == [cd[3] for cd in cands], (f['name'], b['org'])
+            k = len(b['order'])
+            rate = np.array([[cd[4] + cd[5] for cd in cands]], np.uint32)
+            d_sel, d_cost = A(k, np.uint8), A(k, np.uint64)
+            d_tj = A(k * z.TXUNIT_JOB.itemsize, np.uint8)
+            z.md_fast_select(d_satd, A.from_host(rate), b['lambda'], 1, nc, k, A.from_host(np.zeros(2, np.int32)),
+                             s, s, [0], d_sel, d_cost, d_tj)
+            assert d_sel.to_host().tolist() == b['order']
+            assert d_cost.to_host().tolist() == [cands[i][6] for i in b['order']]
+            checked += 1
+    assert checked == 6

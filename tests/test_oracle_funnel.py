@@ -93,3 +93,59 @@ def test_md_funnel_chain_composes(oracle, bd):
             d, e, c, _, _ = oracle.tx_unit_full(blk, preds[u, int(sel[u, r])], 2, 0, qi, bd, use_b=True,
                                                 need_recon=False)
             assert (int(dist[u, r]), int(eob[u, r]), int(cul[u, r])) == (d, e, c)
+
+
+def _golden():
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    return json.load(open(os.path.join(here, 'golden', 'funnel_mds0.json')))
+
+
+def _gradient(w, h):
+    r, c = np.mgrid[0:h, 0:w]
+    return (((r * 255) // h).astype(np.uint8) ^ ((c * 3) & 0x3f).astype(np.uint8)).astype(np.uint8)
+
+
+def test_fast_cost_and_order_match_c_captures(oracle):
+    """Every MDS0 record of the C encoder's captures: fc = rdcost(lambda,
+    flr + fcr, dist << 4), and the POST0 survivor order is the exchange sort
+    of the fast costs in buffer order."""
+    g = _golden()
+    n = 0
+    for f in g['files']:
+        for b in f['blocks']:
+            lam = b['lambda']
+            c = np.array(b['mds0'], np.int64)
+            for buf, mode, fi, dist, flr, fcr, fc in c.tolist():
+                assert oracle.rdcost(lam, flr + fcr, dist << 4) == fc, (f['name'], b['org'], buf)
+                n += 1
+            order = oracle.exchange_sort(c[:, 6].astype(np.uint64)).tolist()
+            assert order[:len(b['order'])] == b['order'], (f['name'], b['org'])
+            k = len(b['order'])
+            sel, cost = oracle.md_fast_select(c[:, 3].astype(np.uint32)[None], (c[:, 4] + c[:, 5]).astype(np.uint32)[None],
+                                              lam, len(c), k)
+            assert sel[0].tolist() == b['order'] and cost[0].tolist() == [int(c[i, 6]) for i in b['order']]
+    assert n >= 150
+
+
+def test_origin_block_satd_matches_c_captures(oracle):
+    """Blocks at the frame origin have no coded neighbours: each candidate's
+    prediction follows from the spec's fallbacks, and its Hadamard SATD must
+    equal the C encoder's MDS0 dist."""
+    g = _golden()
+    seen = 0
+    for f in g['files']:
+        img = _gradient(f['w'], f['h'])
+        for b in f['blocks']:
+            if b['org'] != [0, 0]:
+                continue
+            s = b['sq']
+            blk = np.ascontiguousarray(img[:s, :s])
+            for buf, mode, fi, dist, flr, fcr, fc in b['mds0']:
+                j = np.zeros(1, oracle.INTRA_JOB)
+                j[0] = (0, 0, s, s, mode, 0, fi, 1, 0, 0, 0, 0, 0, 0, 0, 0)
+                p = oracle.intra_predict(img, 8, j).astype(np.uint8)
+                assert oracle.block_satd(blk, p, False) == dist, (f['name'], mode, fi)
+                seen += 1
+    assert seen >= 20
