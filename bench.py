@@ -192,6 +192,30 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
     return out
 
 
+# Per-sample int ops of one candidate in the fused fast loop, by prediction
+# class (counted from the predictor arithmetic): DC / V / H copy 1, SMOOTH 10,
+# SMOOTH_V / _H 5, PAETH 10, directional interpolation 9, filter intra 16;
+# plus the residual (1) and the Hadamard + absolute sum (8 per sample for
+# 4x4 / 8x8 tiles, +3 with the 16x16 combine, +6 with the 32x32 one).
+def md_fast_ops_per_unit(cands, w, h):
+    t = min(w, h, 32)
+    had = 8 + (3 if t >= 16 else 0) + (3 if t >= 32 else 0)
+    tot = 0
+    for mode, delta, fi in cands:
+        if fi < 5:
+            p = 16
+        elif mode in (3, 4, 5, 6, 7, 8) or (mode in (1, 2) and delta != 0):
+            p = 9
+        elif mode == 9 or mode == 12:
+            p = 10
+        elif mode in (10, 11):
+            p = 5
+        else:
+            p = 1
+        tot += (p + 1 + had) * w * h
+    return tot
+
+
 def md_funnel_stage(zav1, synth, args, cpu=True):
     """MDS0 -> MDS1 of the intra funnel over every 16x16 unit of the 2048x2048
     10-bit frame: 61 candidates per unit (13 modes + angle deltas), SATD,
@@ -214,7 +238,7 @@ def md_funnel_stage(zav1, synth, args, cpu=True):
     for _ in range(2):
         f.run(stream)
     zav1.sync(stream)
-    phases = ('pred', 'satd', 'select', 'tx')
+    phases = f.stages()
     ev = []
     for _ in range(len(phases) + 1):
         e = ctypes.c_void_p()
@@ -239,25 +263,48 @@ def md_funnel_stage(zav1, synth, args, cpu=True):
     n = bw * bw
     npred = f.predictions()
     ntx = f.n_tx
-    # algorithmic HBM bytes per launch (16-bit samples; job records read once)
-    bytes_of = {'pred': npred * (2 * n + (2 * bw + 2 * bw + 1) * 2 + 32),
-                'satd': npred * (2 * n + 2 * n + 16 + 4),
-                'select': f.n_units * nc * 8 + ntx * 16 + f.n_units * k * 9,
-                'tx': ntx * (2 * n + 2 * n + 16 + 13)}
+    nu = f.n_units
+    # fused fast loop: algorithmic int ops per launch (per candidate sample:
+    # prediction by mode class + residual + Hadamard/SATD), and compulsory
+    # HBM bytes (source block + reference samples per unit, u32 SATD per candidate)
+    ops_unit = md_fast_ops_per_unit(cands, bw, bw)
+    satd_bytes = nu * (2 * n + (2 * bw + 2 * bw + 1) * 2 + 20) + npred * 4
     dom = phases[int(np.argmax(phase_ms))]
     dom_s = phase_ms[phases.index(dom)] / 1000.0
-    kern = {'pred': 'k_intra_pred', 'satd': 'k_satd<16, UnitTiles<uint16_t>>', 'select': 'k_fast_select',
-            'tx': 'k_txunit<16,16,uint16_t>'}[dom]
-    gbs = bytes_of[dom] / dom_s / 1e9
+    if dom == 'satd':
+        roof = {'bound': 'valu', 'kernel': 'k_md_fast_satd<uint16_t,16>', 'unit': 'Tops/s',
+                'achieved': round(nu * ops_unit / dom_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
+                'frac': round(nu * ops_unit / dom_s / 1e12 / VALU_PEAK_TOPS, 4), 'traffic': None,
+                'avg_launch_us': round(dom_s * 1e6, 2), 'algorithmic_ops_per_launch': int(nu * ops_unit),
+                'hbm_algorithmic_GBs': round(satd_bytes / dom_s / 1e9, 2)}
+    else:
+        bytes_of = {'pred': nu * k * (2 * n + (4 * bw + 1) * 2 + 20),
+                    'select': nu * nc * 8 + ntx * 16 + nu * k * 9,
+                    'tx': ntx * (2 * n + 2 * n + 16 + 13)}
+        gbs = bytes_of[dom] / dom_s / 1e9
+        roof = {'bound': 'hbm', 'kernel': {'pred': 'k_intra_pred<uint16_t,SurvivorJobs>', 'select': 'k_fast_select',
+                                           'tx': 'k_txunit<16,16,uint16_t>'}[dom],
+                'unit': 'GB/s', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 4),
+                'traffic': None, 'avg_launch_us': round(dom_s * 1e6, 2),
+                'algorithmic_bytes_per_launch': int(bytes_of[dom])}
     out = {'workload': 'intra funnel MDS0 -> MDS1: 2048x2048 10-bit luma (config2), all 16x16 units, 61 '
                        'candidates each (13 modes + angle deltas, synthetic neighbour canvas and mode rates), '
-                       'Hadamard SATD, fast cost + exchange-sort top-3, DCT_DCT quantize_b units at qp 40',
+                       'fused prediction + Hadamard SATD on chip, fast cost + exchange-sort top-3, survivors '
+                       'predicted again, DCT_DCT quantize_b units at qp 40',
            'value': round(S * S / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s', 'steps': steps,
-           'ms_per_step': round(ms_step, 3), 'predictions_per_step': int(npred), 'tx_units_per_step': int(ntx),
+           'ms_per_step': round(ms_step, 3), 'candidates_per_step': int(npred), 'tx_units_per_step': int(ntx),
            'phase_ms': {ph: round(float(phase_ms[i]), 4) for i, ph in enumerate(phases)},
-           'roofline': {'bound': 'hbm', 'kernel': kern, 'unit': 'GB/s', 'achieved': round(gbs, 1),
-                        'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 4), 'traffic': None,
-                        'avg_launch_us': round(dom_s * 1e6, 2), 'algorithmic_bytes_per_launch': int(bytes_of[dom])}}
+           'roofline': roof}
+    # the two-pass form (every prediction through HBM), same results, for comparison
+    f2 = fm.IntraFunnel(zav1, src, recon, 10, bw, bw, qi, k=k, cands=cands, lam=lam, rates=rates, fused=False)
+    f2.run(stream)
+    zav1.sync(stream)
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        f2.run(stream)
+    zav1.sync(stream)
+    out['two_pass_ms_per_step'] = round((time.perf_counter() - t2) * 1000.0 / steps, 3)
+    del f2
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, 'oracle'))
         import oracle as O

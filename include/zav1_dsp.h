@@ -232,12 +232,49 @@ int zav1_pred_satd_batch(const void *d_src, int stride, int plane_w, int plane_h
  * mod.rs:2994: not stable on ties) go to d_sel[u*k + r] (candidate index)
  * and d_sel_cost[u*k + r] (nullable), and the transform-unit jobs of the
  * survivors, one per host-side type types[t] (n_types <= 16), to
- * d_jobs[(u*k + r)*n_types + t]: x, y from d_unit_xy[2u], [2u+1],
- * pred_off = (u*n_cands + cand)*w*h, crop w x h, ready for
- * zav1_txunit_batch.  1 <= k <= n_cands <= 64. */
+ * d_jobs[(u*k + r)*n_types + t]: x, y = d_unit_xy[u*xy_stride + 0 / 1]
+ * (xy_stride 2 for (x, y) pairs, 5 for zav1_md_unit records), crop w x h,
+ * pred_off = (u*n_cands + cand)*w*h, or (u*k + r)*w*h when survivors_only
+ * (the layout of zav1_md_predict_survivors), ready for zav1_txunit_batch.
+ * 1 <= k <= n_cands <= 64. */
 int zav1_md_fast_select(const uint32_t *d_satd, const uint32_t *d_rate, uint64_t lambda, int n_units, int n_cands,
-                        int k, const int32_t *d_unit_xy, int w, int h, const uint8_t *types, int n_types,
-                        uint8_t *d_sel, uint64_t *d_sel_cost, zav1_txunit_job *d_jobs, zav1_stream_t s);
+                        int k, const int32_t *d_unit_xy, int xy_stride, int w, int h, const uint8_t *types,
+                        int n_types, int survivors_only, uint8_t *d_sel, uint64_t *d_sel_cost,
+                        zav1_txunit_job *d_jobs, zav1_stream_t s);
+
+/* Fused MDS0 fast loop.  A unit (x, y, availability counts as for
+ * zav1_intra_job, edge-filter flags) and a candidate list shared by every
+ * unit of the call (host array, n_cands <= 64, generation order):
+ * mode 0..12, angle delta -3..3 (directional modes), filter-intra mode
+ * 0..4 with mode 0 and w, h <= 32, or 5 = none. */
+typedef struct zav1_md_unit {
+  int32_t x, y;
+  int16_t n_top_px, n_topright_px, n_left_px, n_bottomleft_px;
+  uint8_t edge_filter, filt_type, pad0, pad1;
+} zav1_md_unit;
+typedef struct zav1_md_cand {
+  uint8_t mode;
+  int8_t delta;
+  uint8_t fi_mode, pad;
+} zav1_md_cand;
+/* For every unit of one w x h size (an AV1 block size: powers of two 4..64,
+ * aspect at most 4:1) and every
+ * candidate: the prediction from d_recon (predict_unit, lf/predict.rs:49)
+ * and its Hadamard SATD against d_src (hadamard_satd, lf/predict.rs:143),
+ * the fast loop of mod.rs:2060-2185, into d_satd[u*n_cands + m].  The
+ * predictions stay on chip; the result equals zav1_intra_pred_batch +
+ * zav1_pred_satd_batch.  A unit whose reference reads (availability
+ * counts) would leave the plane gets 0xffffffff for every candidate.  d_src
+ * and d_recon share stride and sample type (u8 at bd 8, u16 at bd 10). */
+int zav1_md_fast_satd(const void *d_src, const void *d_recon, int stride, int plane_w, int plane_h, int bd, int w,
+                      int h, const zav1_md_unit *d_units, int n_units, const zav1_md_cand *cands, int n_cands,
+                      uint32_t *d_satd, zav1_stream_t s);
+/* Predictions of the survivors for the next stage: candidate d_sel[u*k + r]
+ * of unit u into d_pred + (u*k + r)*w*h (u16; zeros for a unit that
+ * zav1_md_fast_satd reports invalid). */
+int zav1_md_predict_survivors(const void *d_recon, int stride, int plane_w, int plane_h, int bd, int w, int h,
+                              const zav1_md_unit *d_units, int n_units, const zav1_md_cand *cands, int n_cands,
+                              const uint8_t *d_sel, int k, uint16_t *d_pred, zav1_stream_t s);
 
 /* ------------------------------------------------------------------------ */
 /* Intra prediction (batched).                                               */

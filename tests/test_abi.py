@@ -5,6 +5,8 @@ import os
 import re
 import subprocess
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -67,14 +69,35 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     assert L.zav1_pred_satd_batch(fake, 60, 64, 64, 8, 8, 8, fake, 64, fake, 1, fake, None) == zav1.ZAV1_EINVAL
     ty = (ctypes.c_uint8 * 2)(0, 9)
     bad_ty = (ctypes.c_uint8 * 1)(16)
-    assert L.zav1_md_fast_select(fake, None, 1, 4, 65, 3, fake, 8, 8, ty, 2, fake, None, fake,
-                                 None) == zav1.ZAV1_EINVAL
-    assert L.zav1_md_fast_select(fake, None, 1, 4, 13, 14, fake, 8, 8, ty, 2, fake, None, fake,
-                                 None) == zav1.ZAV1_EINVAL
-    assert L.zav1_md_fast_select(fake, None, 1, 4, 13, 3, fake, 8, 8, bad_ty, 1, fake, None, fake,
-                                 None) == zav1.ZAV1_EINVAL
-    assert L.zav1_md_fast_select(fake, None, 1, 4, 13, 3, fake, 8, 8, ty, 0, fake, None, fake,
-                                 None) == zav1.ZAV1_EINVAL
+
+    def sel(n_cands, k, types, n_types, xy_stride=2, surv=0):
+        return L.zav1_md_fast_select(fake, None, 1, 4, n_cands, k, fake, xy_stride, 8, 8, types, n_types, surv, fake,
+                                     None, fake, None)
+    assert sel(65, 3, ty, 2) == zav1.ZAV1_EINVAL
+    assert sel(13, 14, ty, 2) == zav1.ZAV1_EINVAL
+    assert sel(13, 3, bad_ty, 1) == zav1.ZAV1_EINVAL
+    assert sel(13, 3, ty, 0) == zav1.ZAV1_EINVAL
+    assert sel(13, 3, ty, 2, xy_stride=1) == zav1.ZAV1_EINVAL
+    assert sel(13, 3, ty, 2, surv=2) == zav1.ZAV1_EINVAL
+    # fused fast loop: candidate list rules (mode, delta, filter intra only with DC and sides <= 32)
+    C = zav1.MD_CAND
+
+    def fast(cands, w=16, h=16, bd=8):
+        c = np.array(cands, C)
+        return L.zav1_md_fast_satd(fake, fake, 64, 64, 64, bd, w, h, fake, 1, c.ctypes.data, len(c), fake, None)
+    assert fast([(13, 0, 5, 0)]) == zav1.ZAV1_EINVAL
+    assert fast([(1, 4, 5, 0)]) == zav1.ZAV1_EINVAL
+    assert fast([(1, 0, 0, 0)]) == zav1.ZAV1_EINVAL
+    assert fast([(0, 0, 0, 0)], w=64) == zav1.ZAV1_EINVAL
+    assert fast([(0, 0, 6, 0)]) == zav1.ZAV1_EINVAL
+    assert fast([(0, 0, 5, 0)], bd=12) == zav1.ZAV1_EINVAL
+    assert fast([(0, 0, 5, 0)], w=12) == zav1.ZAV1_EINVAL
+    assert fast([(0, 0, 5, 0)] * 65) == zav1.ZAV1_EINVAL
+    c1 = np.array([(0, 0, 5, 0)], C)
+    assert L.zav1_md_predict_survivors(fake, 64, 64, 64, 8, 16, 16, fake, 4, c1.ctypes.data, 1, fake, 2, fake,
+                                       None) == zav1.ZAV1_EINVAL
+    assert L.zav1_md_predict_survivors(fake, 60, 64, 64, 8, 16, 16, fake, 4, c1.ctypes.data, 1, fake, 1, fake,
+                                       None) == zav1.ZAV1_EINVAL
 
 
 def test_rest_unit_grid_matches_oracle(zav1, oracle):

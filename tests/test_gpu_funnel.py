@@ -99,11 +99,12 @@ def test_md_fast_select_matches_oracle(gpu, oracle, with_rate):
     assert np.all(jobs['crop_w'] == w) and np.all(jobs['crop_h'] == h)
 
 
-@pytest.mark.parametrize('bd,w,h', [(8, 8, 8), (8, 16, 16), (10, 16, 8), (10, 32, 32), (8, 4, 16)])
-def test_intra_funnel_chain(gpu, oracle, bd, w, h):
+@pytest.mark.parametrize('fused', [True, False])
+@pytest.mark.parametrize('bd,w,h', [(8, 8, 8), (8, 16, 16), (10, 16, 8), (10, 32, 32), (8, 4, 16), (10, 64, 64)])
+def test_intra_funnel_chain(gpu, oracle, bd, w, h, fused):
     z = gpu
     rng = np.random.default_rng(bd * 1000 + w * 10 + h)
-    PH, PW = 128, 96
+    PH, PW = 128, 192 if w == 64 else 96
     src = _plane(rng, bd, PH, PW)
     # flat areas give exact SATD ties between modes
     src[:32, :48] = src[0, 0]
@@ -112,21 +113,19 @@ def test_intra_funnel_chain(gpu, oracle, bd, w, h):
     nc = len(funnel.REGULAR_MODES)
     rates = rng.integers(0, 600, (1, nc)).astype(np.uint32)
     qindex = 120
-    f = funnel.IntraFunnel(z, src, recon, bd, w, h, qindex, k=3, rates=rates, lam=50000)
+    f = funnel.IntraFunnel(z, src, recon, bd, w, h, qindex, k=3, rates=rates, lam=50000, fused=fused)
     f.run()
     r = f.results()
-    pred_all = f.d_pred.to_host()
-    n = w * h
     oj = f.intra_jobs.view(oracle.INTRA_JOB)
+    refp = {}
     for u in range(f.n_units):
         x, y = f.xy[u].tolist()
         blk = np.ascontiguousarray(src[y:y + h, x:x + w])
         for m in range(nc):
             j = u * nc + m
-            pred = pred_all[j * n:(j + 1) * n].reshape(h, w)
-            if u % 5 == 0:
-                assert np.array_equal(pred, oracle.intra_predict(recon, bd, oj[j:j + 1])), (u, m)
-            pj = pred if bd == 10 else pred.astype(np.uint8)
+            p = oracle.intra_predict(recon, bd, oj[j:j + 1]).astype(np.uint16)
+            refp[u, m] = p
+            pj = p if bd == 10 else p.astype(np.uint8)
             assert int(r['satd'][u, m]) == oracle.block_satd(blk, pj, bd == 10), (u, m)
     rsel, rcost = oracle.md_fast_select(r['satd'], f.rates, f.lam, nc, 3)
     assert np.array_equal(r['sel'], rsel)
@@ -136,8 +135,9 @@ def test_intra_funnel_chain(gpu, oracle, bd, w, h):
         blk = np.ascontiguousarray(src[y:y + h, x:x + w])
         for rank in range(3):
             m = int(rsel[u, rank])
-            pred = pred_all[(u * nc + m) * n:(u * nc + m + 1) * n].reshape(h, w)
-            rd, re, rc, _, _ = oracle.tx_unit_full(blk, pred, f.tx_size, 0, qindex, bd, use_b=True, need_recon=False)
+            assert np.array_equal(f.survivor_pred(u, rank), refp[u, m]), (u, rank, m)
+            rd, re, rc, _, _ = oracle.tx_unit_full(blk, refp[u, m], f.tx_size, 0, qindex, bd, use_b=True,
+                                                   need_recon=False)
             got = (int(r['dist'][u, rank, 0]), int(r['eob'][u, rank, 0]), int(r['cul'][u, rank, 0]))
             assert got == (rd, re, rc), (u, rank, m, got, (rd, re, rc))
     # and the oracle's one-call chain
@@ -146,6 +146,75 @@ def test_intra_funnel_chain(gpu, oracle, bd, w, h):
     assert np.array_equal(r['sel'], osel) and np.array_equal(r['sel_cost'], ocost)
     assert np.array_equal(r['dist'][:, :, 0], odist)
     assert np.array_equal(r['eob'][:, :, 0], oeob) and np.array_equal(r['cul'][:, :, 0], ocul)
+
+
+def _md_cands(w, h):
+    c = [(m, 0, 5) for m in range(13)]
+    c += [(m, d, 5) for m in range(1, 9) for d in (-3, -1, 2)]
+    if w <= 32 and h <= 32:
+        c += [(0, 0, fi) for fi in range(5)]
+    return c
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_md_fast_satd_matches_two_pass(gpu, bd):
+    """The fused fast loop equals intra_pred_batch + pred_satd_batch for
+    every unit shape, random availability, edge filter on/off, smooth
+    neighbours, filter intra and angle deltas; inconsistent units are
+    reported invalid."""
+    z = gpu
+    rng = np.random.default_rng(500 + bd)
+    PH, PW = 200, 232
+    plane = _plane(rng, bd, PH, PW)
+    mx = (1 << bd) - 1
+    recon = np.clip(plane.astype(np.int64) + rng.integers(-20, 21, plane.shape), 0, mx).astype(plane.dtype)
+    A = z.DeviceArray
+    d_src, d_rec = A.from_host(plane), A.from_host(recon)
+    for w in DIMS:
+        for h in DIMS:
+            if max(w, h) > 4 * min(w, h):
+                continue
+            cands = _md_cands(w, h)
+            nc = len(cands)
+            nu = 24
+            units = np.zeros(nu, z.MD_UNIT)
+            for u in range(nu):
+                x = int(rng.integers(1, PW - 2 * w)) if u % 6 else 0
+                y = int(rng.integers(1, PH - 2 * h)) if u % 7 else 0
+                nt = 0 if y == 0 else (w if rng.uniform() < 0.7 else int(rng.integers(1, w + 1)))
+                ntr = int(rng.integers(0, w + 1)) if nt == w and rng.uniform() < 0.6 else 0
+                nl = 0 if x == 0 else (h if rng.uniform() < 0.7 else int(rng.integers(1, h + 1)))
+                nbl = int(rng.integers(0, h + 1)) if nl == h and rng.uniform() < 0.4 else 0
+                units[u] = (x, y, nt, ntr, nl, nbl, int(rng.uniform() < 0.7), int(rng.uniform() < 0.3), 0, 0)
+            # inconsistent availability: reads above the plane / right of it
+            units[3]['y'], units[3]['n_top_px'] = 0, w
+            units[5]['x'], units[5]['n_topright_px'], units[5]['n_top_px'] = PW - w, w, w
+            units[5]['y'] = max(int(units[5]['y']), 1)
+            d_units = A.from_host(units.view(np.uint8))
+            mc = np.array([(m, d, fi, 0) for (m, d, fi) in cands], z.MD_CAND)
+            d_fs = A(nu * nc, np.uint32)
+            z.md_fast_satd(d_src, d_rec, PW, PW, PH, bd, w, h, d_units, nu, mc, d_fs)
+            fused = d_fs.to_host().reshape(nu, nc)
+            ij = np.zeros(nu * nc, z.INTRA_JOB)
+            sj = np.zeros(nu * nc, z.TXUNIT_JOB)
+            ok = [u for u in range(nu) if u not in (3, 5)]
+            for u in ok:
+                U = units[u]
+                for m, (mode, d, fi) in enumerate(cands):
+                    j = u * nc + m
+                    ij[j] = (U['x'], U['y'], w, h, mode, d, fi, U['edge_filter'], U['filt_type'], 0, U['n_top_px'],
+                             U['n_topright_px'], U['n_left_px'], U['n_bottomleft_px'], j * w * h, 0)
+                    sj[j] = (U['x'], U['y'], j * w * h, 0, w, h, 0)
+            sel = np.array([u * nc + m for u in ok for m in range(nc)])
+            d_pred = A(nu * nc * w * h, np.uint16)
+            d_ij = A.from_host(ij[sel].view(np.uint8))
+            z.intra_pred_batch(d_rec, PW, bd, d_ij, len(sel), d_pred)
+            d_ts = A(len(sel), np.uint32)
+            z.pred_satd_batch(d_src, PW, PW, PH, bd, w, h, d_pred, A.from_host(sj[sel].view(np.uint8)), len(sel),
+                              d_ts)
+            two = d_ts.to_host()
+            assert np.array_equal(fused[ok].ravel(), two), (w, h, bd, np.argwhere(fused[ok].ravel() != two)[:4])
+            assert np.all(fused[3] == 0xffffffff) and np.all(fused[5] == 0xffffffff), (w, h)
 
 
 def test_intra_funnel_angle_candidates(gpu, oracle):
@@ -157,6 +226,7 @@ def test_intra_funnel_angle_candidates(gpu, oracle):
     cands = funnel.angle_candidates()
     f = funnel.IntraFunnel(z, src, recon, 10, 16, 16, 150, k=4, cands=cands, lam=90000,
                            rates=rng.integers(0, 900, (1, len(cands))).astype(np.uint32))
+    assert f.fused
     f.run()
     r = f.results()
     osel, ocost, odist, oeob, ocul = oracle.md_funnel(src, recon, 10, f.intra_jobs.view(oracle.INTRA_JOB),

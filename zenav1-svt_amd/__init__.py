@@ -105,8 +105,10 @@ EXPORTS = {
     'zav1_txunit_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
                           _vp, _vp, _vp, _vp, _vp, _vp],
     'zav1_pred_satd_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _vp, _vp],
-    'zav1_md_fast_select': [_vp, _vp, ctypes.c_uint64, _i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp,
-                            _vp],
+    'zav1_md_fast_select': [_vp, _vp, ctypes.c_uint64, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _i32, _vp,
+                            _vp, _vp, _vp],
+    'zav1_md_fast_satd': [_vp, _vp] + [_i32] * 6 + [_vp, _i32, _vp, _i32, _vp, _vp],
+    'zav1_md_predict_survivors': [_vp] + [_i32] * 6 + [_vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp],
     'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
     'zav1_deblock_level_sse': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _i32, _vp, ctypes.c_size_t, _vp, _vp],
     'zav1_deblock_search_level': [_vp, _vp] + [_i32] * 6 + [_vp, _i32, _vp, ctypes.c_size_t, _vp,
@@ -342,13 +344,35 @@ def pred_satd_batch(d_src, stride, plane_w, plane_h, bd, w, h, d_pred, d_jobs, n
 
 
 def md_fast_select(d_satd, d_rate, lam, n_units, n_cands, k, d_unit_xy, w, h, types, d_sel, d_sel_cost, d_jobs,
-                   stream=None):
+                   stream=None, xy_stride=2, survivors_only=False):
     """MDS0 survivors per unit and their transform-unit jobs (zav1_md_fast_select)."""
     tl = np.ascontiguousarray(types, dtype=np.uint8)
     check(lib().zav1_md_fast_select(d_satd.ptr, d_rate.ptr if d_rate is not None else None, int(lam), n_units,
-                                    n_cands, k, d_unit_xy.ptr, w, h, tl.ctypes.data, len(tl), d_sel.ptr,
+                                    n_cands, k, d_unit_xy.ptr, xy_stride, w, h, tl.ctypes.data, len(tl),
+                                    int(survivors_only), d_sel.ptr,
                                     d_sel_cost.ptr if d_sel_cost is not None else None, d_jobs.ptr, stream),
           'md_fast_select')
+
+
+MD_UNIT = np.dtype([('x', '<i4'), ('y', '<i4'), ('n_top_px', '<i2'), ('n_topright_px', '<i2'), ('n_left_px', '<i2'),
+                    ('n_bottomleft_px', '<i2'), ('edge_filter', 'u1'), ('filt_type', 'u1'), ('pad0', 'u1'),
+                    ('pad1', 'u1')])
+MD_CAND = np.dtype([('mode', 'u1'), ('delta', 'i1'), ('fi_mode', 'u1'), ('pad', 'u1')])
+
+
+def md_fast_satd(d_src, d_recon, stride, plane_w, plane_h, bd, w, h, d_units, n_units, cands, d_satd, stream=None):
+    """Fused MDS0 fast loop (zav1_md_fast_satd); cands: host MD_CAND array."""
+    c = np.ascontiguousarray(cands, dtype=MD_CAND)
+    check(lib().zav1_md_fast_satd(d_src.ptr, d_recon.ptr, stride, plane_w, plane_h, bd, w, h, d_units.ptr, n_units,
+                                  c.ctypes.data, len(c), d_satd.ptr, stream), 'md_fast_satd')
+
+
+def md_predict_survivors(d_recon, stride, plane_w, plane_h, bd, w, h, d_units, n_units, cands, d_sel, k, d_pred,
+                         stream=None):
+    c = np.ascontiguousarray(cands, dtype=MD_CAND)
+    check(lib().zav1_md_predict_survivors(d_recon.ptr, stride, plane_w, plane_h, bd, w, h, d_units.ptr, n_units,
+                                          c.ctypes.data, len(c), d_sel.ptr, k, d_pred.ptr, stream),
+          'md_predict_survivors')
 
 
 def deblock_level_sse(d_recon, d_src, plane_w, plane_h, bd, plane, ss, sharpness, geom, levels, d_work, d_sse,

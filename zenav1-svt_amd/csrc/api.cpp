@@ -527,20 +527,69 @@ int zav1_pred_satd_batch(const void* d_src, int stride, int plane_w, int plane_h
 }
 
 int zav1_md_fast_select(const uint32_t* d_satd, const uint32_t* d_rate, uint64_t lambda, int n_units, int n_cands,
-                        int k, const int32_t* d_unit_xy, int w, int h, const uint8_t* types, int n_types,
-                        uint8_t* d_sel, uint64_t* d_sel_cost, zav1_txunit_job* d_jobs, zav1_stream_t s) {
+                        int k, const int32_t* d_unit_xy, int xy_stride, int w, int h, const uint8_t* types,
+                        int n_types, int survivors_only, uint8_t* d_sel, uint64_t* d_sel_cost,
+                        zav1_txunit_job* d_jobs, zav1_stream_t s) {
   if (n_units < 0 || n_cands < 1 || n_cands > 64 || k < 1 || k > n_cands || !funnel_dim(w) || !funnel_dim(h) ||
-      n_types < 1 || n_types > 16 || !types || (n_units && (!d_satd || !d_unit_xy || !d_sel || !d_jobs)))
+      n_types < 1 || n_types > 16 || !types || xy_stride < 2 || (survivors_only != 0 && survivors_only != 1) ||
+      (n_units && (!d_satd || !d_unit_xy || !d_sel || !d_jobs)))
     return ZAV1_EINVAL;
-  if ((long)n_units * n_cands * w * h > UINT32_MAX) return ZAV1_EINVAL;  // pred_off range
+  if ((long)n_units * (survivors_only ? k : n_cands) * w * h > UINT32_MAX) return ZAV1_EINVAL;  // pred_off range
   zav1::TypeList tl{};
   tl.n = n_types;
   for (int i = 0; i < n_types; i++) {
     if (types[i] > 15) return ZAV1_EINVAL;
     tl.t[i] = types[i];
   }
-  ZCHECK(zav1::fast_select(d_satd, d_rate, lambda, n_units, n_cands, k, d_unit_xy, w, h, tl, d_sel, d_sel_cost,
-                           reinterpret_cast<zav1::TxUnitJob*>(d_jobs), S(s)));
+  ZCHECK(zav1::fast_select(d_satd, d_rate, lambda, n_units, n_cands, k, d_unit_xy, xy_stride, w, h, tl,
+                           survivors_only, d_sel, d_sel_cost, reinterpret_cast<zav1::TxUnitJob*>(d_jobs), S(s)));
+  return ZAV1_OK;
+}
+
+static int md_cand_list(const zav1_md_cand* cands, int n_cands, int w, int h, zav1::MdCandList* cl) {
+  static_assert(sizeof(zav1_md_cand) == sizeof(zav1::MdCand), "md cand layout");
+  if (!cands || n_cands < 1 || n_cands > 64) return ZAV1_EINVAL;
+  *cl = zav1::MdCandList{};
+  cl->n = n_cands;
+  for (int i = 0; i < n_cands; i++) {
+    const zav1_md_cand c = cands[i];
+    if (c.mode > 12 || c.delta < -3 || c.delta > 3 || c.fi_mode > 5) return ZAV1_EINVAL;
+    if (c.fi_mode < 5 && (c.mode != 0 || w > 32 || h > 32)) return ZAV1_EINVAL;
+    cl->c[i] = zav1::MdCand{c.mode, c.delta, c.fi_mode, 0};
+  }
+  return ZAV1_OK;
+}
+
+int zav1_md_fast_satd(const void* d_src, const void* d_recon, int stride, int plane_w, int plane_h, int bd, int w,
+                      int h, const zav1_md_unit* d_units, int n_units, const zav1_md_cand* cands, int n_cands,
+                      uint32_t* d_satd, zav1_stream_t s) {
+  static_assert(sizeof(zav1_md_unit) == sizeof(zav1::MdUnit), "md unit layout");
+  if (!funnel_dim(w) || !funnel_dim(h) || w > 4 * h || h > 4 * w || (bd != 8 && bd != 10) || n_units < 0 ||
+      plane_w <= 0 || plane_h <= 0 ||
+      stride < plane_w || (n_units && (!d_src || !d_recon || !d_units || !d_satd)))
+    return ZAV1_EINVAL;
+  zav1::MdCandList cl;
+  const int rc = md_cand_list(cands, n_cands, w, h, &cl);
+  if (rc) return rc;
+  ZCHECK(zav1::md_fast_satd(d_src, d_recon, stride, plane_w, plane_h, bd, w, h,
+                            reinterpret_cast<const zav1::MdUnit*>(d_units), n_units, cl, d_satd, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_md_predict_survivors(const void* d_recon, int stride, int plane_w, int plane_h, int bd, int w, int h,
+                              const zav1_md_unit* d_units, int n_units, const zav1_md_cand* cands, int n_cands,
+                              const uint8_t* d_sel, int k, uint16_t* d_pred, zav1_stream_t s) {
+  if (!funnel_dim(w) || !funnel_dim(h) || (bd != 8 && bd != 10) || n_units < 0 || plane_w <= 0 || plane_h <= 0 ||
+      stride < plane_w || k < 1 ||
+      k > n_cands || (n_units && (!d_recon || !d_units || !d_sel || !d_pred)))
+    return ZAV1_EINVAL;
+  if ((long)n_units * k > INT32_MAX || (long)n_units * k * w * h > UINT32_MAX) return ZAV1_EINVAL;
+  zav1::MdCandList cl;
+  const int rc = md_cand_list(cands, n_cands, w, h, &cl);
+  if (rc) return rc;
+  ZCHECK(zav1::md_predict_survivors(d_recon, stride, plane_w, plane_h, bd, w, h,
+                                    reinterpret_cast<const zav1::MdUnit*>(d_units), n_units, cl, d_sel, k, d_pred,
+                                    S(s)));
   return ZAV1_OK;
 }
 

@@ -23,8 +23,9 @@ constexpr int kMaxCands = 64;
 __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict__ satd,
                                                      const uint32_t* __restrict__ rate, uint64_t lambda,
                                                      int n_units, int n_cands, int k,
-                                                     const int32_t* __restrict__ unit_xy, int w, int h,
-                                                     TypeList types, uint8_t* __restrict__ sel,
+                                                     const int32_t* __restrict__ unit_xy, int unit_stride, int w,
+                                                     int h, TypeList types, int survivors_only,
+                                                     uint8_t* __restrict__ sel,
                                                      uint64_t* __restrict__ sel_cost,
                                                      TxUnitJob* __restrict__ jobs_out) {
   __shared__ uint8_t ord[kMaxCands * 256];
@@ -38,7 +39,7 @@ __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict_
     return ((r * lambda + 256) >> 9) + ((uint64_t)s << 11);
   };
   for (int m = 0; m < n_cands; m++) ord[m * 256 + threadIdx.x] = (uint8_t)m;
-  const int ux = unit_xy[2 * u], uy = unit_xy[2 * u + 1];
+  const int ux = unit_xy[(long)u * unit_stride], uy = unit_xy[(long)u * unit_stride + 1];
   for (int i = 0; i < k; i++) {
     int mi = ord[i * 256 + threadIdx.x];
     uint64_t ci = cost(mi);
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict_
       TxUnitJob jb;
       jb.x = ux;
       jb.y = uy;
-      jb.pred_off = (uint32_t)((base + mi) * (long)(w * h));
+      jb.pred_off = (uint32_t)((survivors_only ? (long)u * k + i : base + mi) * (long)(w * h));
       jb.tx_type = types.t[ti];
       jb.crop_w = (uint8_t)(w > 255 ? 255 : w);
       jb.crop_h = (uint8_t)(h > 255 ? 255 : h);
@@ -69,12 +70,12 @@ __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict_
 }
 
 hipError_t fast_select(const uint32_t* satd, const uint32_t* rate, uint64_t lambda, int n_units, int n_cands, int k,
-                       const int32_t* unit_xy, int w, int h, const TypeList& types, uint8_t* sel, uint64_t* sel_cost,
-                       TxUnitJob* jobs, hipStream_t st) {
+                       const int32_t* unit_xy, int unit_stride, int w, int h, const TypeList& types,
+                       int survivors_only, uint8_t* sel, uint64_t* sel_cost, TxUnitJob* jobs, hipStream_t st) {
   if (n_units <= 0) return hipSuccess;
   if (n_cands < 1 || n_cands > kMaxCands || k < 1 || k > n_cands) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fast_select, dim3((n_units + 255) / 256), dim3(256), 0, st, satd, rate, lambda, n_units,
-                     n_cands, k, unit_xy, w, h, types, sel, sel_cost, jobs);
+                     n_cands, k, unit_xy, unit_stride, w, h, types, survivors_only, sel, sel_cost, jobs);
   return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 // (directional edge assembly + filters), rust/crates/svtav1-dsp/src/
 // intra_pred.rs:18-1180 and hbd.rs:169-935 (predictors, CfL).
 #include "av1_intra_tables.h"
+#include "hadamard.h"
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
 
@@ -58,34 +59,50 @@ __device__ __forceinline__ const uint8_t* sm_w(int n) {
   return SM_WEIGHTS_ALL + (n == 4 ? 0 : n == 8 ? 4 : n == 16 ? 12 : n == 32 ? 28 : 60);
 }
 
+// Barriers of the predictor: a workgroup of one wave uses the workgroup
+// barrier; waves predicting different candidates of one workgroup order
+// their own LDS traffic with a wave-level fence + barrier.
+struct BlockSync {
+  __device__ __forceinline__ void operator()() const { __syncthreads(); }
+};
+struct WaveSync {
+  __device__ __forceinline__ void operator()() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
 // 5-tap smoothing of e[start .. start+sz) in place (tmp = scratch copy).
-__device__ void filter_edge_wave(int32_t* e, int32_t* tmp, int start, int sz, int strength, int lane) {
+template <class SYNC>
+__device__ void filter_edge_wave(int16_t* e, int16_t* tmp, int start, int sz, int strength, int lane, SYNC sync) {
   if (strength == 0) return;  // wave-uniform
   for (int i = lane; i < sz; i += 64) tmp[i] = e[start + i];
-  __syncthreads();
+  sync();
   const int k0 = strength == 1 ? 0 : (strength == 2 ? 0 : 2);
   const int k1 = strength == 1 ? 4 : (strength == 2 ? 5 : 4);
   const int k2 = strength == 1 ? 8 : (strength == 2 ? 6 : 4);
   for (int i = 1 + lane; i < sz; i += 64) {
     auto at = [&](int k) { return tmp[clampi(k, 0, sz - 1)]; };
     const int32_t s = at(i - 2) * k0 + at(i - 1) * k1 + at(i) * k2 + at(i + 1) * k1 + at(i + 2) * k0;
-    e[start + i] = (s + 8) >> 4;
+    e[start + i] = (int16_t)((s + 8) >> 4);
   }
-  __syncthreads();
+  sync();
 }
 
-__device__ void upsample_wave(int32_t* p, int32_t* tmp, int sz, int maxv, int lane) {
+template <class SYNC>
+__device__ void upsample_wave(int16_t* p, int16_t* tmp, int sz, int maxv, int lane, SYNC sync) {
   // tmp[0..sz+3) = {p[o-1], p[o-1], p[o..o+sz), p[o+sz-1]}
   for (int i = lane; i < sz + 3; i += 64)
     tmp[i] = (i < 2) ? p[kO - 1] : (i < sz + 2 ? p[kO + i - 2] : p[kO + sz - 1]);
-  __syncthreads();
+  sync();
   if (lane == 0) p[kO - 2] = tmp[0];
   for (int i = lane; i < sz; i += 64) {
     const int32_t s = -tmp[i] + 9 * tmp[i + 1] + 9 * tmp[i + 2] - tmp[i + 3];
-    p[kO + 2 * i - 1] = clampi((s + 8) >> 4, 0, maxv);
+    p[kO + 2 * i - 1] = (int16_t)clampi((s + 8) >> 4, 0, maxv);
     p[kO + 2 * i] = tmp[i + 2];
   }
-  __syncthreads();
+  sync();
 }
 
 __device__ __forceinline__ int32_t wave_sum(int32_t v) {
@@ -95,20 +112,17 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) {
 }
 }  // namespace
 
-template <typename Pix>
-__global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane, int stride, int bd,
-                                                   const IntraJob* __restrict__ jobs, int n_jobs,
-                                                   uint16_t* __restrict__ out) {
-  __shared__ int32_t above[kEdge], left[kEdge], tmp[kEdge];
-  __shared__ int32_t fbuf[33 * 33];
-  const int lane = threadIdx.x;
-  const int ji = blockIdx.x;
-  if (ji >= n_jobs) return;  // whole workgroup exits together
-  const IntraJob j = jobs[ji];
-  const int w = j.w, h = j.h, n = w * h;
+// One unit's prediction by one wave: P(x, y) reads a reference sample of
+// the plane (only the row above, the column left and the top-left corner
+// of the unit, within the job's availability counts); out(i, v) receives
+// sample i (raster) of the w x h prediction.  above / left / tmp hold kEdge
+// samples, fbuf (h + 1) x 33 (filter intra only), all private to the wave.
+// W, H > 0 fix the unit size at compile time (j.w, j.h otherwise).
+template <int W = 0, int H = 0, class NB, class OUT, class SYNC>
+__device__ void predict_wave(const IntraJob& j, int bd, const NB& P, const OUT& out, SYNC sync, int16_t* above,
+                             int16_t* left, int16_t* tmp, int16_t* fbuf, int lane) {
+  const int w = W > 0 ? W : j.w, h = H > 0 ? H : j.h, n = w * h;
   const int base = 128 << (bd - 8), maxv = (1 << bd) - 1;
-  uint16_t* dst = out + j.out_off;
-  auto P = [&](int x, int y) -> int32_t { return (int32_t)plane[(long)y * stride + x]; };
 
   const bool directional = (j.mode >= 3 && j.mode <= 8) || ((j.mode == 1 || j.mode == 2) && j.delta != 0);
   if (directional) {
@@ -119,14 +133,14 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
     if ((!need_above && n_left == 0) || (!need_left && n_top == 0)) {
       const int32_t v = need_left ? (n_top > 0 ? P(j.x, j.y - 1) : base + 1)
                                   : (n_left > 0 ? P(j.x - 1, j.y) : base - 1);
-      for (int i = lane; i < n; i += 64) dst[i] = (uint16_t)v;
+      for (int i = lane; i < n; i += 64) out(i, v);
       return;
     }
     for (int i = lane; i < kEdge; i += 64) {
       above[i] = base - 1;
       left[i] = base + 1;
     }
-    __syncthreads();
+    sync();
     if (need_left) {
       const int num_left = h + (need_bottom ? w : 0);
       if (n_left > 0) {
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
       above[kO - 1] = tl;
       left[kO - 1] = tl;
     }
-    __syncthreads();
+    sync();
     bool up_a = false, up_l = false;
     if (j.edge_filter) {
       if (pa != 90 && pa != 180) {
@@ -166,19 +180,19 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
             above[kO - 1] = s;
             left[kO - 1] = s;
           }
-          __syncthreads();
+          sync();
         }
         if (need_above && n_top > 0)
           filter_edge_wave(above, tmp, kO - 1, n_top + 1 + (need_right ? h : 0),
-                           edge_strength(w, h, pa - 90, j.filt_type), lane);
+                           edge_strength(w, h, pa - 90, j.filt_type), lane, sync);
         if (need_left && n_left > 0)
           filter_edge_wave(left, tmp, kO - 1, n_left + 1 + (need_bottom ? w : 0),
-                           edge_strength(h, w, pa - 180, j.filt_type), lane);
+                           edge_strength(h, w, pa - 180, j.filt_type), lane, sync);
       }
       up_a = use_upsample(w, h, pa - 90, j.filt_type);
-      if (need_above && up_a) upsample_wave(above, tmp, w + (need_right ? h : 0), maxv, lane);
+      if (need_above && up_a) upsample_wave(above, tmp, w + (need_right ? h : 0), maxv, lane, sync);
       up_l = use_upsample(h, w, pa - 180, j.filt_type);
-      if (need_left && up_l) upsample_wave(left, tmp, h + (need_bottom ? w : 0), maxv, lane);
+      if (need_left && up_l) upsample_wave(left, tmp, h + (need_bottom ? w : 0), maxv, lane, sync);
     }
     const int dx = dr_dx(pa), dy = dr_dy(pa);
     const int ua = up_a ? 1 : 0, ul = up_l ? 1 : 0;
@@ -211,7 +225,7 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
         if (b >= max_base) v = left[kO + max_base];
         else v = clampi((left[kO + b] * (32 - shift) + left[kO + b + 1] * shift + 16) >> 5, 0, maxv);
       }
-      dst[i] = (uint16_t)v;
+      out(i, v);
     }
     return;
   }
@@ -225,16 +239,16 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
   for (int i = lane; i < h; i += 64)
     left[i] = has_left ? P(j.x - 1, j.y + (i < j.n_left_px ? i : j.n_left_px - 1)) : (has_above ? a0 : base + 1);
   const int32_t tl = (has_above && has_left) ? P(j.x - 1, j.y - 1) : (has_above ? a0 : (has_left ? l0 : base));
-  __syncthreads();
+  sync();
 
   if (j.fi_mode < 5) {
     // 33x33 staging buffer; sub-block (i, k) of 4x2 samples at rows 1+2i, cols 1+4k.
-    for (int i = lane; i < 33 * 33; i += 64) fbuf[i] = 0;
-    __syncthreads();
-    if (lane == 0) fbuf[0] = tl;
-    for (int c = lane; c < w; c += 64) fbuf[c + 1] = above[c];
-    for (int r = lane; r < h; r += 64) fbuf[(r + 1) * 33] = left[r];
-    __syncthreads();
+    for (int i = lane; i < (h + 1) * 33; i += 64) fbuf[i] = 0;
+    sync();
+    if (lane == 0) fbuf[0] = (int16_t)tl;
+    for (int c = lane; c < w; c += 64) fbuf[c + 1] = (int16_t)above[c];
+    for (int r = lane; r < h; r += 64) fbuf[(r + 1) * 33] = (int16_t)left[r];
+    sync();
     const int8_t* taps = FILTER_INTRA_TAPS + j.fi_mode * 64;
     const int sbr = h / 2, sbc = w / 4;
     for (int d = 0; d < sbr + sbc - 1; d++) {
@@ -250,13 +264,13 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
         const int8_t* t = taps + k * 8;
         const int32_t v = t[0] * p0 + t[1] * p1 + t[2] * p2 + t[3] * p3 + t[4] * p4 + t[5] * p5 + t[6] * p6;
         const int32_t rv = v < 0 ? -((-v + 8) >> 4) : ((v + 8) >> 4);
-        fbuf[(r + (k >> 2)) * 33 + c + (k & 3)] = clampi(rv, 0, maxv);
+        fbuf[(r + (k >> 2)) * 33 + c + (k & 3)] = (int16_t)clampi(rv, 0, maxv);
       }
-      __syncthreads();
+      sync();
     }
     for (int i = lane; i < n; i += 64) {
       const int r = i / w, c = i - r * w;
-      dst[i] = (uint16_t)fbuf[(r + 1) * 33 + c + 1];
+      out(i, (int32_t)fbuf[(r + 1) * 33 + c + 1]);
     }
     return;
   }
@@ -308,8 +322,357 @@ __global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane
         break;
       }
     }
-    dst[i] = (uint16_t)v;
+    out(i, v);
   }
+}
+
+// Reference samples straight from the plane.
+template <typename Pix>
+struct PlaneNb {
+  const Pix* plane;
+  int stride;
+  __device__ __forceinline__ int32_t operator()(int x, int y) const { return (int32_t)plane[(long)y * stride + x]; }
+};
+struct GlobalOut {
+  uint16_t* dst;
+  __device__ __forceinline__ void operator()(int i, int32_t v) const { dst[i] = (uint16_t)v; }
+};
+
+// Jobs of intra_pred_batch: a table of IntraJob records.
+struct JobTable {
+  const IntraJob* jobs;
+  __device__ __forceinline__ IntraJob operator()(int i) const { return jobs[i]; }
+  __device__ __forceinline__ bool valid(const IntraJob&) const { return true; }
+};
+
+template <typename Pix, class JOBS>
+__global__ __launch_bounds__(64) void k_intra_pred(const Pix* __restrict__ plane, int stride, int bd, JOBS jobs,
+                                                   int n_jobs, uint16_t* __restrict__ out) {
+  __shared__ int16_t above[kEdge], left[kEdge], tmp[kEdge];
+  __shared__ int16_t fbuf[33 * 33];
+  const int ji = blockIdx.x;
+  if (ji >= n_jobs) return;  // whole workgroup exits together
+  const IntraJob j = jobs(ji);
+  if (!jobs.valid(j)) {  // a survivor of a unit whose reference reads would leave the plane
+    for (int i = threadIdx.x; i < j.w * j.h; i += 64) out[j.out_off + i] = 0;
+    return;
+  }
+  predict_wave(j, bd, PlaneNb<Pix>{plane, stride}, GlobalOut{out + j.out_off}, BlockSync{}, above, left, tmp, fbuf,
+               (int)threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Fused MDS0 fast loop: one workgroup (4 waves) per unit.  The unit's source
+// block and reference samples are loaded into LDS once.  Each wave then works
+// alone on batches of candidates: it predicts a batch into its private LDS
+// residual buffer (source - prediction, 16-bit, tile-major) and runs the
+// Hadamard SATD of the batch with 8 lanes per 8x8 sub-tile (a column pass
+// and a row pass through LDS), adding each tile's SATD into its candidate's
+// LDS counter.  Only wave-level barriers after the loads; predictions never
+// leave LDS.  Bit-exact with intra_pred_batch followed by pred_satd_batch.
+// ---------------------------------------------------------------------------
+
+// Reference samples of one unit cached in LDS: row[0] = top-left, row[1 + i]
+// = above sample i, col[i] = left sample i.
+struct CacheNb {
+  const int32_t* row;
+  const int32_t* col;
+  int x0, y0;
+  __device__ __forceinline__ int32_t operator()(int x, int y) const {
+    return y == y0 - 1 ? row[x - x0 + 1] : col[y - y0];
+  }
+};
+
+// Residual store of one candidate: sample i of the W x H raster goes to its
+// T x T tile (tile-major, raster inside the tile).
+template <int W, int H, int T>
+struct TileResOut {
+  int16_t* res;
+  const int16_t* src;
+  __device__ __forceinline__ void operator()(int i, int32_t v) const {
+    const int r = i / W, c = i % W;
+    const int k = (r / T) * (W / T) + c / T;
+    res[k * (T * T) + (r % T) * T + (c % T)] = (int16_t)(src[i] - (int16_t)v);
+  }
+};
+
+// Geometry of the fused kernel for one unit size.
+template <int W, int H>
+struct MdGeom {
+  static constexpr int N = W * H;
+  static constexpr int T = (W < H ? W : H) < 32 ? (W < H ? W : H) : 32;  // Hadamard tile side
+  static constexpr int TPJ = N / (T * T);                                 // tiles per candidate
+  // lanes one candidate's SATD occupies: 1 per 4x4 tile, 8 per 8x8 sub-tile
+  static constexpr int LANES = T == 4 ? TPJ : TPJ * (T / 8) * (T / 8) * 8;
+  static constexpr int BATCH = LANES >= 64 ? 1 : 64 / LANES;  // candidates per wave batch
+  static constexpr int RES = BATCH * N;                       // int16 residual samples per wave
+  static constexpr bool FI = W <= 32 && H <= 32;              // filter intra possible
+};
+
+// SATD of nt T x T residual tiles at res (contiguous, tile-major) by one
+// wave; tile i adds into acc[i / tpj].  The tiles are overwritten.
+template <int T>
+__device__ void wave_satd_tiles(int16_t* res, int nt, int tpj, uint32_t* acc, int lane) {
+  const WaveSync wsync{};
+  if constexpr (T == 4) {
+    for (int i = lane; i < nt; i += 64) {
+      int32_t c[16];
+      had4_tile(PackedRes{res + i * 16, 4}, c);
+      int32_t sum = 0;
+#pragma unroll
+      for (int k = 0; k < 16; k++) sum += c[k] < 0 ? -c[k] : c[k];
+      atomicAdd(acc + i / tpj, (uint32_t)sum);
+    }
+  } else {
+    constexpr int SUB = (T / 8) * (T / 8);        // 8x8 sub-tiles per tile
+    constexpr int LT = SUB * 8;                   // lane slots per tile
+    constexpr int TPR = LT >= 64 ? 1 : 64 / LT;   // tiles per round
+    constexpr int J = LT > 64 ? LT / 64 : 1;      // slots per lane
+    for (int t0 = 0; t0 < nt; t0 += TPR) {
+      int16_t* base[J];
+      int q[J], L[J], ti[J];
+      bool ok[J];
+      int16_t o[J][8];
+#pragma unroll
+      for (int jj = 0; jj < J; jj++) {
+        const int sl = lane + 64 * jj;
+        ti[jj] = t0 + sl / LT;
+        ok[jj] = ti[jj] < nt;
+        q[jj] = (sl % LT) >> 3;
+        L[jj] = sl & 7;
+        base[jj] = res + (ok[jj] ? ti[jj] : 0) * (T * T);
+      }
+      // column pass: slot (q, L) transforms column L of sub-tile q
+#pragma unroll
+      for (int jj = 0; jj < J; jj++) {
+        if (!ok[jj]) continue;
+        int r0, c0;
+        if constexpr (T == 8) {
+          r0 = 0; c0 = 0;
+        } else if constexpr (T == 16) {
+          r0 = (q[jj] >> 1) * 8; c0 = (q[jj] & 1) * 8;
+        } else {
+          const int Q = q[jj] >> 2, qq = q[jj] & 3;
+          r0 = (Q >> 1) * 16 + (qq >> 1) * 8; c0 = (Q & 1) * 16 + (qq & 1) * 8;
+        }
+        int32_t col[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) col[r] = base[jj][(r0 + r) * T + c0 + L[jj]];
+        had8_col(col, o[jj]);
+      }
+      wsync();
+#pragma unroll
+      for (int jj = 0; jj < J; jj++)
+        if (ok[jj]) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) base[jj][q[jj] * 64 + L[jj] * 8 + k] = o[jj][k];
+        }
+      wsync();
+      // row pass: slot (q, L) produces coefficients L*8 .. L*8+7 of sub-tile q
+      int32_t sum = 0;
+#pragma unroll
+      for (int jj = 0; jj < J; jj++) {
+        if (!ok[jj]) continue;
+        int32_t col[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) col[k] = base[jj][q[jj] * 64 + L[jj] + 8 * k];
+        had8_col(col, o[jj]);
+        if constexpr (T == 8) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) sum += abs((int32_t)o[jj][k]);
+        }
+      }
+      if constexpr (T >= 16) {
+        wsync();
+#pragma unroll
+        for (int jj = 0; jj < J; jj++)
+          if (ok[jj]) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) base[jj][q[jj] * 64 + L[jj] * 8 + k] = o[jj][k];
+          }
+        wsync();
+        // 16x16 combine (int16 wrap, >> 1): each slot takes 2 (group, position) pairs
+#pragma unroll
+        for (int jj = 0; jj < J; jj++) {
+          if (!ok[jj]) continue;
+          const int sl = (lane + 64 * jj) % LT;
+#pragma unroll
+          for (int e = 0; e < 2; e++) {
+            const int pidx = sl * 2 + e;  // 0 .. 64 * SUB / 4 - 1
+            int16_t* B = base[jj] + (pidx >> 6) * 256;
+            const int ii = pidx & 63;
+            const int16_t a0 = B[ii], a1 = B[ii + 64], a2 = B[ii + 128], a3 = B[ii + 192];
+            const int16_t b0 = (int16_t)((int16_t)(a0 + a1) >> 1), b1 = (int16_t)((int16_t)(a0 - a1) >> 1);
+            const int16_t b2 = (int16_t)((int16_t)(a2 + a3) >> 1), b3 = (int16_t)((int16_t)(a2 - a3) >> 1);
+            const int16_t o0 = (int16_t)(b0 + b2), o1 = (int16_t)(b1 + b3);
+            const int16_t o2 = (int16_t)(b0 - b2), o3 = (int16_t)(b1 - b3);
+            if constexpr (T == 16) {
+              sum += abs((int32_t)o0) + abs((int32_t)o1) + abs((int32_t)o2) + abs((int32_t)o3);
+            } else {
+              B[ii] = o0; B[ii + 64] = o1; B[ii + 128] = o2; B[ii + 192] = o3;
+            }
+          }
+        }
+        if constexpr (T == 32) {
+          wsync();
+          // 32x32 combine (saturating >> 2): each slot takes 2 of the 256 positions
+#pragma unroll
+          for (int jj = 0; jj < J; jj++) {
+            if (!ok[jj]) continue;
+            const int sl = (lane + 64 * jj) % LT;
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+              const int ii = sl * 2 + e;
+              const int16_t* Lb = base[jj];
+              const int32_t a0 = Lb[ii], a1 = Lb[ii + 256], a2 = Lb[ii + 512], a3 = Lb[ii + 768];
+              const int16_t b0 = sat16((a0 + a1) >> 2), b1 = sat16((a0 - a1) >> 2);
+              const int16_t b2 = sat16((a2 + a3) >> 2), b3 = sat16((a2 - a3) >> 2);
+              sum += abs((int32_t)(int16_t)(b0 + b2)) + abs((int32_t)(int16_t)(b1 + b3)) +
+                     abs((int32_t)(int16_t)(b0 - b2)) + abs((int32_t)(int16_t)(b1 - b3));
+            }
+          }
+        }
+      }
+      // every slot of a tile belongs to tile ti[0] (J > 1 only when a tile spans the wave)
+      if (ok[0] && sum != 0) atomicAdd(acc + ti[0] / tpj, (uint32_t)sum);
+      wsync();
+    }
+  }
+}
+
+// A unit whose reference reads would leave the plane (availability counts
+// inconsistent with its position) gets kSatdInvalid for every candidate.
+__device__ __forceinline__ bool md_unit_ok(const MdUnit& u, int w, int h, int pw, int ph, int* na, int* nl) {
+  const int nt = u.n_top_px, ntr = u.n_topright_px, nlf = u.n_left_px, nbl = u.n_bottomleft_px;
+  if (nt < 0 || ntr < 0 || nlf < 0 || nbl < 0) return false;
+  *na = nt > 0 ? max(nt, ntr > 0 ? w + ntr : 0) : 0;
+  *nl = nlf > 0 ? max(nlf, nbl > 0 ? h + nbl : 0) : 0;
+  return u.x >= 0 && u.y >= 0 && u.x <= pw - w && u.y <= ph - h && (nt == 0 || u.y >= 1) && (nlf == 0 || u.x >= 1) &&
+         u.x + *na <= pw && u.y + *nl <= ph && *na <= kEdge - 1 && *nl <= kEdge;
+}
+
+template <typename Pix, int W, int H>
+__global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ src, const Pix* __restrict__ recon,
+                                                      int stride, int pw, int ph, int bd,
+                                                      const MdUnit* __restrict__ units, int n_units, MdCandList cl,
+                                                      uint32_t* __restrict__ satd_out) {
+  using G = MdGeom<W, H>;
+  __shared__ int16_t res[4][G::RES];
+  __shared__ int16_t srcb[G::N];
+  __shared__ int32_t rowc[kEdge], colc[kEdge];
+  __shared__ int16_t above[4][kEdge], left[4][kEdge], tmp[4][kEdge];
+  __shared__ int16_t fbuf[4][G::FI ? (H + 1) * 33 : 1];
+  __shared__ uint32_t acc[64];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int u = blockIdx.x;
+  if (u >= n_units) return;
+  const MdUnit U = units[u];
+  const int nc = cl.n;
+  int na = 0, nl = 0;
+  if (!md_unit_ok(U, W, H, pw, ph, &na, &nl)) {
+    for (int m = tid; m < nc; m += 256) satd_out[(long)u * nc + m] = kSatdInvalid;
+    return;
+  }
+  for (int i = tid; i < G::N; i += 256) srcb[i] = (int16_t)src[(long)(U.y + i / W) * stride + U.x + i % W];
+  if (tid == 0) rowc[0] = (U.n_top_px > 0 && U.n_left_px > 0) ? (int32_t)recon[(long)(U.y - 1) * stride + U.x - 1] : 0;
+  for (int i = tid; i < na; i += 256) rowc[1 + i] = recon[(long)(U.y - 1) * stride + U.x + i];
+  for (int i = tid; i < nl; i += 256) colc[i] = recon[(long)(U.y + i) * stride + U.x - 1];
+  for (int m = tid; m < nc; m += 256) acc[m] = 0;
+  __syncthreads();
+
+  const CacheNb nb{rowc, colc, U.x, U.y};
+  // batches of at most BATCH candidates, spread so every wave gets work
+  const int per = (nc + 3) / 4;
+  const int bsz = per < G::BATCH ? per : G::BATCH;
+  for (int m0 = wave * bsz; m0 < nc; m0 += 4 * bsz) {
+    const int nb_c = min(bsz, nc - m0);
+    for (int b = 0; b < nb_c; b++) {
+      const MdCand cd = cl.c[m0 + b];
+      IntraJob j;
+      j.x = U.x; j.y = U.y; j.w = (uint8_t)W; j.h = (uint8_t)H;
+      j.mode = cd.mode; j.delta = cd.delta; j.fi_mode = G::FI ? cd.fi_mode : (uint8_t)5;
+      j.edge_filter = U.edge_filter; j.filt_type = U.filt_type; j.pad0 = 0;
+      j.n_top_px = U.n_top_px; j.n_topright_px = U.n_topright_px;
+      j.n_left_px = U.n_left_px; j.n_bottomleft_px = U.n_bottomleft_px;
+      j.out_off = 0; j.pad1 = 0;
+      predict_wave<W, H>(j, bd, nb, TileResOut<W, H, G::T>{res[wave] + b * G::N, srcb}, WaveSync{}, above[wave],
+                         left[wave], tmp[wave], fbuf[wave], lane);
+      WaveSync{}();
+    }
+    wave_satd_tiles<G::T>(res[wave], nb_c * G::TPJ, G::TPJ, acc + m0, lane);
+  }
+  __syncthreads();
+  for (int m = tid; m < nc; m += 256) satd_out[(long)u * nc + m] = acc[m];
+}
+
+// Survivor predictions: job i = (unit i / k, rank i % k) predicts candidate
+// sel[i] of its unit into out + i * w * h.
+struct SurvivorJobs {
+  const MdUnit* units;
+  MdCandList cl;
+  const uint8_t* sel;
+  int k, w, h, pw, ph;
+  __device__ __forceinline__ bool valid(const IntraJob& j) const {
+    MdUnit u;
+    u.x = j.x; u.y = j.y;
+    u.n_top_px = j.n_top_px; u.n_topright_px = j.n_topright_px;
+    u.n_left_px = j.n_left_px; u.n_bottomleft_px = j.n_bottomleft_px;
+    int na, nl;
+    return md_unit_ok(u, w, h, pw, ph, &na, &nl);
+  }
+  __device__ __forceinline__ IntraJob operator()(int i) const {
+    const MdUnit U = units[i / k];
+    const MdCand cd = cl.c[sel[i] < cl.n ? sel[i] : 0];
+    IntraJob j;
+    j.x = U.x; j.y = U.y; j.w = (uint8_t)w; j.h = (uint8_t)h;
+    j.mode = cd.mode; j.delta = cd.delta; j.fi_mode = cd.fi_mode;
+    j.edge_filter = U.edge_filter; j.filt_type = U.filt_type; j.pad0 = 0;
+    j.n_top_px = U.n_top_px; j.n_topright_px = U.n_topright_px;
+    j.n_left_px = U.n_left_px; j.n_bottomleft_px = U.n_bottomleft_px;
+    j.out_off = (uint32_t)i * (uint32_t)(w * h); j.pad1 = 0;
+    return j;
+  }
+};
+
+template <typename Pix>
+static hipError_t launch_md_satd(const Pix* src, const Pix* recon, int stride, int pw, int ph, int bd, int w, int h,
+                                 const MdUnit* units, int n_units, const MdCandList& cl, uint32_t* satd,
+                                 hipStream_t st) {
+#define X(W, H)                                                                                                  \
+  if (w == W && h == H) {                                                                                        \
+    hipLaunchKernelGGL((k_md_fast_satd<Pix, W, H>), dim3(n_units), dim3(256), 0, st, src, recon, stride, pw, ph, \
+                       bd, units, n_units, cl, satd);                                                            \
+    return hipGetLastError();                                                                                    \
+  }
+  X(4, 4) X(8, 8) X(16, 16) X(32, 32) X(64, 64) X(4, 8) X(8, 4) X(8, 16) X(16, 8) X(16, 32) X(32, 16) X(32, 64)
+  X(64, 32) X(4, 16) X(16, 4) X(8, 32) X(32, 8) X(16, 64) X(64, 16)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t md_fast_satd(const void* src, const void* recon, int stride, int pw, int ph, int bd, int w, int h,
+                        const MdUnit* units, int n_units, const MdCandList& cl, uint32_t* satd, hipStream_t st) {
+  if (n_units <= 0) return hipSuccess;
+  if (bd == 8)
+    return launch_md_satd((const uint8_t*)src, (const uint8_t*)recon, stride, pw, ph, bd, w, h, units, n_units, cl,
+                          satd, st);
+  return launch_md_satd((const uint16_t*)src, (const uint16_t*)recon, stride, pw, ph, bd, w, h, units, n_units, cl,
+                        satd, st);
+}
+
+hipError_t md_predict_survivors(const void* recon, int stride, int pw, int ph, int bd, int w, int h,
+                                const MdUnit* units, int n_units, const MdCandList& cl, const uint8_t* sel, int k,
+                                uint16_t* out, hipStream_t st) {
+  const long n = (long)n_units * k;
+  if (n <= 0) return hipSuccess;
+  const SurvivorJobs sj{units, cl, sel, k, w, h, pw, ph};
+  if (bd == 8)
+    hipLaunchKernelGGL((k_intra_pred<uint8_t, SurvivorJobs>), dim3((unsigned)n), dim3(64), 0, st,
+                       (const uint8_t*)recon, stride, bd, sj, (int)n, out);
+  else
+    hipLaunchKernelGGL((k_intra_pred<uint16_t, SurvivorJobs>), dim3((unsigned)n), dim3(64), 0, st,
+                       (const uint16_t*)recon, stride, bd, sj, (int)n, out);
+  return hipGetLastError();
 }
 
 // CfL: one wave per chroma block.  luma: 2cw x 2ch reconstructed luma at
@@ -347,11 +710,11 @@ hipError_t intra_pred_batch(const void* plane, int stride, int bd, const IntraJo
                             hipStream_t st) {
   if (n_jobs <= 0) return hipSuccess;
   if (bd == 8)
-    hipLaunchKernelGGL((k_intra_pred<uint8_t>), dim3(n_jobs), dim3(64), 0, st, (const uint8_t*)plane, stride, bd,
-                       jobs, n_jobs, out);
+    hipLaunchKernelGGL((k_intra_pred<uint8_t, JobTable>), dim3(n_jobs), dim3(64), 0, st, (const uint8_t*)plane,
+                       stride, bd, JobTable{jobs}, n_jobs, out);
   else
-    hipLaunchKernelGGL((k_intra_pred<uint16_t>), dim3(n_jobs), dim3(64), 0, st, (const uint16_t*)plane, stride,
-                       bd, jobs, n_jobs, out);
+    hipLaunchKernelGGL((k_intra_pred<uint16_t, JobTable>), dim3(n_jobs), dim3(64), 0, st, (const uint16_t*)plane,
+                       stride, bd, JobTable{jobs}, n_jobs, out);
   return hipGetLastError();
 }
 

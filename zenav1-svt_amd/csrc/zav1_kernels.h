@@ -140,10 +140,11 @@ hipError_t pred_satd_batch(const void* src, int stride, int pw, int ph, int bd, 
 // positions of the exchange sort over them into sel / sel_cost
 // [unit * k + rank], and the transform-unit jobs of those candidates for
 // every type of `types` into jobs[(unit * k + rank) * types.n + ti]
-// (prediction of candidate m of unit u at (u * n_cands + m) * w * h).
+// (x, y = unit_xy[unit * unit_stride + 0 / 1]; prediction at
+// (u * n_cands + m) * w * h, or (u * k + rank) * w * h when survivors_only).
 hipError_t fast_select(const uint32_t* satd, const uint32_t* rate, uint64_t lambda, int n_units, int n_cands, int k,
-                       const int32_t* unit_xy, int w, int h, const TypeList& types, uint8_t* sel, uint64_t* sel_cost,
-                       TxUnitJob* jobs, hipStream_t st);
+                       const int32_t* unit_xy, int unit_stride, int w, int h, const TypeList& types,
+                       int survivors_only, uint8_t* sel, uint64_t* sel_cost, TxUnitJob* jobs, hipStream_t st);
 // Fused transform units of one tx size: residual (src - pred) -> forward ->
 // fold -> quantize -> eob / cul level / frequency distortion, and, when
 // reconstructing, inverse -> recon clip -> spatial distortion.
@@ -169,6 +170,31 @@ struct CflJob {
 };
 hipError_t intra_pred_batch(const void* plane, int stride, int bd, const IntraJob* jobs, int n_jobs, uint16_t* out,
                             hipStream_t st);
+// Mode-decision units and candidates (C-ABI zav1_md_unit / zav1_md_cand).
+struct MdUnit {
+  int32_t x, y;
+  int16_t n_top_px, n_topright_px, n_left_px, n_bottomleft_px;
+  uint8_t edge_filter, filt_type, pad0, pad1;
+};
+struct MdCand {
+  uint8_t mode;
+  int8_t delta;
+  uint8_t fi_mode, pad;
+};
+struct MdCandList {
+  int32_t n;
+  MdCand c[64];
+};
+// Fused MDS0 fast loop: per unit, every candidate predicted from recon and
+// its Hadamard SATD against src; satd[unit * n + cand] (kSatdInvalid for a
+// unit whose reference reads would leave the plane).
+hipError_t md_fast_satd(const void* src, const void* recon, int stride, int pw, int ph, int bd, int w, int h,
+                        const MdUnit* units, int n_units, const MdCandList& cl, uint32_t* satd, hipStream_t st);
+// Predictions of the selected candidates: out + (unit * k + rank) * w * h
+// (zeros for a unit whose reference reads would leave the plane).
+hipError_t md_predict_survivors(const void* recon, int stride, int pw, int ph, int bd, int w, int h,
+                                const MdUnit* units, int n_units, const MdCandList& cl, const uint8_t* sel, int k,
+                                uint16_t* out, hipStream_t st);
 hipError_t cfl_batch(const void* luma, int stride, int bd, const CflJob* jobs, int n_jobs, const uint16_t* dc,
                      uint16_t* out, hipStream_t st);
 

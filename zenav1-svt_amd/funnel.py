@@ -56,11 +56,18 @@ def raster_availability(x, y, w, h, width):
 
 
 class IntraFunnel:
-    """MDS0 -> MDS1 over every w x h unit of a plane, device resident."""
+    """MDS0 -> MDS1 over every w x h unit of a plane, device resident.
+
+    fused=True (default): one fused launch predicts every candidate on chip
+    and returns its SATD (zav1_md_fast_satd); after the selection only the
+    k survivors per unit are predicted again, for the transform units.
+    fused=False: every candidate's prediction is written to HBM
+    (zav1_intra_pred_batch) and read back by zav1_pred_satd_batch."""
 
     def __init__(self, zav1, src, recon, bd, w, h, qindex, k=3, cands=None, lam=40000, rates=None, types=(0,),
-                 flags=None, edge_filter=1):
+                 flags=None, edge_filter=1, fused=True):
         self.zav1 = zav1
+        self.fused = fused
         self.bd, self.w, self.h, self.qindex, self.k = bd, w, h, qindex, k
         self.tx_size = tx_size_of(w, h)
         dt = np.uint8 if bd == 8 else np.uint16
@@ -86,6 +93,11 @@ class IntraFunnel:
                 ij[j] = (x, y, w, h, mode, delta, fi, edge_filter, 0, 0, nt, ntr, nl, nbl, j * n, 0)
                 sj[j] = (x, y, j * n, 0, w, h, 0)
         self.intra_jobs, self.satd_jobs = ij, sj
+        mu = np.zeros(nu, zav1.MD_UNIT)
+        for u, (x, y) in enumerate(self.xy.tolist()):
+            mu[u] = (x, y) + raster_availability(x, y, w, h, self.pw) + (edge_filter, 0, 0, 0)
+        self.md_units = mu
+        self.md_cands = np.array([(m, d, fi, 0) for (m, d, fi) in self.cands], zav1.MD_CAND)
         if rates is None:
             rates = np.zeros((nu, nc), np.uint32)
         self.rates = np.ascontiguousarray(np.broadcast_to(rates, (nu, nc)), dtype=np.uint32)
@@ -96,7 +108,8 @@ class IntraFunnel:
         self.d_sj = A.from_host(sj.view(np.uint8))
         self.d_xy = A.from_host(self.xy)
         self.d_rate = A.from_host(self.rates)
-        self.d_pred = A(nu * nc * n, np.uint16)
+        self.d_units = A.from_host(mu.view(np.uint8))
+        self.d_pred = A(nu * (k if fused else nc) * n, np.uint16)
         self.d_satd = A(nu * nc, np.uint32)
         self.d_sel = A(nu * k, np.uint8)
         self.d_cost = A(nu * k, np.uint64)
@@ -110,8 +123,30 @@ class IntraFunnel:
     def predictions(self):
         return self.n_units * self.n_cands
 
-    def run(self, stream=None, stages=('pred', 'satd', 'select', 'tx')):
+    def stages(self):
+        return ('satd', 'select', 'pred', 'tx') if self.fused else ('pred', 'satd', 'select', 'tx')
+
+    def run(self, stream=None, stages=None):
         z = self.zav1
+        stages = self.stages() if stages is None else stages
+        if self.fused:
+            for st in stages:
+                if st == 'satd':
+                    z.md_fast_satd(self.d_src, self.d_recon, self.pw, self.pw, self.ph, self.bd, self.w, self.h,
+                                   self.d_units, self.n_units, self.md_cands, self.d_satd, stream)
+                elif st == 'select':
+                    z.md_fast_select(self.d_satd, self.d_rate, self.lam, self.n_units, self.n_cands, self.k,
+                                     self.d_units, self.w, self.h, self.types, self.d_sel, self.d_cost, self.d_tj,
+                                     stream, xy_stride=self.zav1.MD_UNIT.itemsize // 4, survivors_only=True)
+                elif st == 'pred':
+                    z.md_predict_survivors(self.d_recon, self.pw, self.pw, self.ph, self.bd, self.w, self.h,
+                                           self.d_units, self.n_units, self.md_cands, self.d_sel, self.k,
+                                           self.d_pred, stream)
+                elif st == 'tx':
+                    z.txunit_batch(self.d_src, self.pw, self.pw, self.ph, self.bd, self.tx_size, self.d_pred,
+                                   self.d_tj, self.n_tx, self.qindex, self.flags, self.d_dist, self.d_eob,
+                                   self.d_cul, None, None, stream)
+            return
         if 'pred' in stages:
             z.intra_pred_batch(self.d_recon, self.pw, self.bd, self.d_ij, self.predictions(), self.d_pred, stream)
         if 'satd' in stages:
@@ -134,7 +169,11 @@ class IntraFunnel:
                     eob=self.d_eob.to_host().reshape(nu, k, nty),
                     cul=self.d_cul.to_host().reshape(nu, k, nty))
 
-    def pred_of(self, u, m):
+    def survivor_pred(self, u, rank):
+        """Prediction of survivor `rank` of unit u (after a full run)."""
         n = self.w * self.h
-        off = (u * self.n_cands + m) * n
+        if self.fused:
+            off = (u * self.k + rank) * n
+        else:
+            off = (u * self.n_cands + int(self.d_sel.to_host()[u * self.k + rank])) * n
         return self.d_pred.to_host()[off:off + n].reshape(self.h, self.w)
