@@ -15,10 +15,10 @@
 
 namespace zav1 {
 
-// One lane per unit.  Position p of the exchange sort is final after its
-// outer pass p, so the first k passes give the k survivors in the order the
-// sort leaves them; the candidate order of the unit lives in LDS, transposed
-// (position-major) so a wave's lanes touch consecutive bytes.
+// One wave per unit (grid-stride over units).  The lanes compute the unit's
+// fast costs into LDS; lane 0 runs the first k outer passes of the exchange
+// sort (position p is final after pass p) over the candidates in generation
+// order; the lanes then write the survivors' transform-unit jobs.
 constexpr int kMaxCands = 64;
 __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict__ satd,
                                                      const uint32_t* __restrict__ rate, uint64_t lambda,
@@ -28,44 +28,53 @@ __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict_
                                                      uint8_t* __restrict__ sel,
                                                      uint64_t* __restrict__ sel_cost,
                                                      TxUnitJob* __restrict__ jobs_out) {
-  __shared__ uint8_t ord[kMaxCands * 256];
-  const int u = blockIdx.x * 256 + threadIdx.x;
-  if (u >= n_units) return;
-  const long base = (long)u * n_cands;
-  auto cost = [&](int m) -> uint64_t {
-    const uint32_t s = satd[base + m];
-    if (s == kSatdInvalid) return ~0ull;
-    const uint64_t r = rate ? rate[base + m] : 0u;
-    return ((r * lambda + 256) >> 9) + ((uint64_t)s << 11);
-  };
-  for (int m = 0; m < n_cands; m++) ord[m * 256 + threadIdx.x] = (uint8_t)m;
-  const int ux = unit_xy[(long)u * unit_stride], uy = unit_xy[(long)u * unit_stride + 1];
-  for (int i = 0; i < k; i++) {
-    int mi = ord[i * 256 + threadIdx.x];
-    uint64_t ci = cost(mi);
-    for (int j = i + 1; j < n_cands; j++) {
-      const int mj = ord[j * 256 + threadIdx.x];
-      const uint64_t cj = cost(mj);
-      if (cj < ci) {  // swap positions i and j
-        ord[j * 256 + threadIdx.x] = (uint8_t)mi;
-        mi = mj;
-        ci = cj;
+  __shared__ uint64_t cst[4][kMaxCands];
+  __shared__ uint8_t ord[4][kMaxCands];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t* C = cst[wave];
+  uint8_t* O = ord[wave];
+  for (int u = blockIdx.x * 4 + wave; u < n_units; u += gridDim.x * 4) {
+    const long base = (long)u * n_cands;
+    if (lane < n_cands) {
+      const uint32_t s = satd[base + lane];
+      const uint64_t r = rate ? rate[base + lane] : 0u;
+      C[lane] = s == kSatdInvalid ? ~0ull : ((r * lambda + 256) >> 9) + ((uint64_t)s << 11);
+      O[lane] = (uint8_t)lane;
+    }
+    wave_sync();
+    if (lane == 0) {
+      for (int i = 0; i < k; i++) {
+        int mi = O[i];
+        uint64_t ci = C[mi];
+        for (int j = i + 1; j < n_cands; j++) {
+          const int mj = O[j];
+          const uint64_t cj = C[mj];
+          if (cj < ci) {  // swap positions i and j
+            O[j] = (uint8_t)mi;
+            mi = mj;
+            ci = cj;
+          }
+        }
+        O[i] = (uint8_t)mi;
+        sel[(long)u * k + i] = (uint8_t)mi;
+        if (sel_cost) sel_cost[(long)u * k + i] = ci;
       }
     }
-    ord[i * 256 + threadIdx.x] = (uint8_t)mi;
-    sel[(long)u * k + i] = (uint8_t)mi;
-    if (sel_cost) sel_cost[(long)u * k + i] = ci;
-    for (int ti = 0; ti < types.n; ti++) {
+    wave_sync();
+    const int ux = unit_xy[(long)u * unit_stride], uy = unit_xy[(long)u * unit_stride + 1];
+    for (int t = lane; t < k * types.n; t += 64) {
+      const int i = t / types.n, ti = t - i * types.n;
       TxUnitJob jb;
       jb.x = ux;
       jb.y = uy;
-      jb.pred_off = (uint32_t)((survivors_only ? (long)u * k + i : base + mi) * (long)(w * h));
+      jb.pred_off = (uint32_t)((survivors_only ? (long)u * k + i : base + O[i]) * (long)(w * h));
       jb.tx_type = types.t[ti];
-      jb.crop_w = (uint8_t)(w > 255 ? 255 : w);
-      jb.crop_h = (uint8_t)(h > 255 ? 255 : h);
+      jb.crop_w = (uint8_t)w;
+      jb.crop_h = (uint8_t)h;
       jb.pad = 0;
       jobs_out[((long)u * k + i) * types.n + ti] = jb;
     }
+    wave_sync();
   }
 }
 
@@ -74,7 +83,8 @@ hipError_t fast_select(const uint32_t* satd, const uint32_t* rate, uint64_t lamb
                        int survivors_only, uint8_t* sel, uint64_t* sel_cost, TxUnitJob* jobs, hipStream_t st) {
   if (n_units <= 0) return hipSuccess;
   if (n_cands < 1 || n_cands > kMaxCands || k < 1 || k > n_cands) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_fast_select, dim3((n_units + 255) / 256), dim3(256), 0, st, satd, rate, lambda, n_units,
+  const int grid = (n_units + 3) / 4 < 8192 ? (n_units + 3) / 4 : 8192;
+  hipLaunchKernelGGL(k_fast_select, dim3(grid), dim3(256), 0, st, satd, rate, lambda, n_units,
                      n_cands, k, unit_xy, unit_stride, w, h, types, survivors_only, sel, sel_cost, jobs);
   return hipGetLastError();
 }

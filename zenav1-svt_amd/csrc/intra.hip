@@ -66,11 +66,7 @@ struct BlockSync {
   __device__ __forceinline__ void operator()() const { __syncthreads(); }
 };
 struct WaveSync {
-  __device__ __forceinline__ void operator()() const {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
+  __device__ __forceinline__ void operator()() const { wave_sync(); }
 };
 
 // 5-tap smoothing of e[start .. start+sz) in place (tmp = scratch copy).

@@ -7,6 +7,14 @@
 
 namespace zav1 {
 
+// Orders one wave's LDS traffic between its lanes (a wave-level fence and
+// barrier; lanes of other waves are not waited for).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // TxSize in C order (rust/crates/svtav1-types/src/transform.rs:7-29).
 __host__ __device__ constexpr int tx_w(int s) {
   return s == 0 ? 4 : s == 1 ? 8 : s == 2 ? 16 : s == 3 ? 32 : s == 4 ? 64 : s == 5 ? 4 : s == 6 ? 8
