@@ -93,6 +93,31 @@ SGR_OPS_PER_PX = (10 * (SGR_OPS_R2 + SGR_OPS_R1 + SGR_OPS_PROJ) + 4 * (SGR_OPS_R
                   2 * (SGR_OPS_R2 + SGR_OPS_PROJ))
 
 
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor the
+    same counter shows on a device-to-device copy of known size), or None."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    for name, k in d.get('kernels', {}).items():
+        if name.startswith(kernel_prefix):
+            return int(k['fetch_kib'] * 1024 * d['fetch_scale'] + k['write_kib'] * 1024 * d['write_scale'])
+    return None
+
+
+def copy_calibration(zav1):
+    """Device-to-device copies of 64 MiB for the PMC byte-counter scale."""
+    n = 64 << 20
+    a, b = zav1.DeviceArray(n, np.uint8), zav1.DeviceArray(n, np.uint8)
+    a.zero()
+    for _ in range(3):
+        zav1.check(zav1.lib().zav1_memcpy_d2d(b.ptr, a.ptr, n, None), 'memcpy_d2d')
+    zav1.sync(None)
+    print(json.dumps({'copy_calibration_bytes': n, 'copies': 3}))
+
+
 def loopfilter_stage(zav1, synth, args, cpu=True):
     L = zav1.lib()
     lfm = _load('zav1_loopfilter', os.path.join(PKG, 'loopfilter.py'))
@@ -154,7 +179,8 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
            'phase_ms': {ph: round(phase_ms[i] / args.lf_steps, 4) for i, ph in enumerate(st.PHASES)},
            'roofline': {'bound': 'valu', 'kernel': 'k_sgr_stats<uint8_t> (luma)', 'unit': 'Tops/s',
                         'achieved': round(sg_ops / sg_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
-                        'frac': round(sg_ops / sg_s / 1e12 / VALU_PEAK_TOPS, 4), 'traffic': None,
+                        'frac': round(sg_ops / sg_s / 1e12 / VALU_PEAK_TOPS, 4),
+                        'traffic': measured_traffic('k_sgr_stats'),
                         'avg_launch_us': round(sg_s * 1e6, 2), 'algorithmic_ops_per_launch': int(sg_ops),
                         'hbm_algorithmic_GBs': round(2 * N * N / sg_s / 1e9, 2)}}
     if cpu:
@@ -274,7 +300,8 @@ def md_funnel_stage(zav1, synth, args, cpu=True):
     if dom == 'satd':
         roof = {'bound': 'valu', 'kernel': 'k_md_fast_satd<uint16_t,16>', 'unit': 'Tops/s',
                 'achieved': round(nu * ops_unit / dom_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
-                'frac': round(nu * ops_unit / dom_s / 1e12 / VALU_PEAK_TOPS, 4), 'traffic': None,
+                'frac': round(nu * ops_unit / dom_s / 1e12 / VALU_PEAK_TOPS, 4),
+                'traffic': measured_traffic('k_md_fast_satd<unsigned short, 16, 16>'),
                 'avg_launch_us': round(dom_s * 1e6, 2), 'algorithmic_ops_per_launch': int(nu * ops_unit),
                 'hbm_algorithmic_GBs': round(satd_bytes / dom_s / 1e9, 2)}
     else:
@@ -338,6 +365,7 @@ def main():
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-lf', action='store_true')
     ap.add_argument('--no-funnel', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--lf-steps', type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -346,6 +374,10 @@ def main():
         sys.exit(2)
 
     zav1 = _load('zav1', os.path.join(PKG, '__init__.py'))
+    if args.copy_calibration:
+        zav1.init(0)
+        copy_calibration(zav1)
+        return
     synth = _load('zav1_synth', os.path.join(PKG, 'synth.py'))
     tsm = _load('zav1_txsearch', os.path.join(PKG, 'txsearch.py'))
     opsd = json.load(open(os.path.join(PKG, 'csrc', 'txfm1d_ops.json')))
@@ -419,7 +451,8 @@ def main():
         'per_size_ms_per_step': {str(tsm.TX_DIMS[t]): round(per_size_ms[t] / args.steps, 4) for t in range(19)},
         'roofline': {'bound': 'valu', 'kernel': 'k_txsearch<%d,%d>' % (w, h), 'achieved': round(achieved_tops, 3),
                      'peak': VALU_PEAK_TOPS, 'unit': 'Tops/s', 'frac': round(achieved_tops / VALU_PEAK_TOPS, 4),
-                     'traffic': None, 'avg_launch_us': round(avg_launch_s * 1e6, 2),
+                     'traffic': measured_traffic('k_txsearch<%d, %d, ' % (w, h)),
+                     'avg_launch_us': round(avg_launch_s * 1e6, 2),
                      'algorithmic_ops_per_launch': int(ops_launch),
                      'hbm_algorithmic_GBs': round(bytes_launch / avg_launch_s / 1e9, 2),
                      'hbm_peak_GBs': HBM_PEAK_GBS},
