@@ -560,7 +560,7 @@ __global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ sr
   __shared__ int16_t fbuf[4][G::FI ? (H + 1) * 33 : 1];
   __shared__ uint32_t acc[64];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int u = blockIdx.x;
+  const int u = xcd_chunked(blockIdx.x, gridDim.x);
   if (u >= n_units) return;
   const MdUnit U = units[u];
   const int nc = cl.n;

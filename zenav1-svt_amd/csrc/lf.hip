@@ -487,7 +487,8 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
   __shared__ uint8_t sact[64];
   __shared__ uint8_t sadj[64][16];
   __shared__ uint32_t red[4][64];
-  const int fbr = blockIdx.x / nhfb, fbc = blockIdx.x - fbr * nhfb;
+  const int fb = xcd_chunked(blockIdx.x, gridDim.x);
+  const int fbr = fb / nhfb, fbc = fb - fbr * nhfb;
   const int cs = bd - 8;
   const int y0 = fbr * 64, x0 = fbc * 64;
   const int vsize = min(64, height - y0), hsize = min(64, width - x0);
@@ -511,8 +512,8 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
     // adjusted primary strength of every P (at most 60 at 10 bits)
     for (int P = 0; P < 16; P++) sadj[tid][P] = (uint8_t)adjust_strength(P << cs, v);
     if (dirvar) {
-      dirvar[((long)blockIdx.x * 64 + tid) * 2] = d;
-      dirvar[((long)blockIdx.x * 64 + tid) * 2 + 1] = v;
+      dirvar[((long)fb * 64 + tid) * 2] = d;
+      dirvar[((long)fb * 64 + tid) * 2 + 1] = v;
     }
   }
   __syncthreads();
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
   __syncthreads();
   if (tid < 64) {
     const uint64_t tot = (uint64_t)red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-    mse[(long)blockIdx.x * 64 + tid] = (tot >> (2 * cs)) * (uint64_t)sub;
+    mse[(long)fb * 64 + tid] = (tot >> (2 * cs)) * (uint64_t)sub;
   }
 }
 
@@ -552,7 +553,8 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
   __shared__ uint32_t red[4][64];
   __shared__ uint64_t total[64];
   __shared__ uint8_t sact[64];
-  const int fbr = blockIdx.x / nhfb, fbc = blockIdx.x - fbr * nhfb;
+  const int fb = xcd_chunked(blockIdx.x, gridDim.x);
+  const int fbr = fb / nhfb, fbc = fb - fbr * nhfb;
   const int cs = bd - 8, cw = width >> 1, ch = height >> 1;
   const int y0 = fbr * 32, x0 = fbc * 32;
   const int vsize = min(32, ch - y0), hsize = min(32, cw - x0);
@@ -580,7 +582,7 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
     for (int p = tid; p < 1024; p += 256) {
       const int r = p >> 5, c = p & 31;
       if (r >= nr || c >= nc || !sact[(r >> 2) * 8 + (c >> 2)] || ((r & 3) % sub) != 0) continue;
-      const int dir = dirvar[((long)blockIdx.x * 64 + (r >> 2) * 8 + (c >> 2)) * 2];
+      const int dir = dirvar[((long)fb * 64 + (r >> 2) * 8 + (c >> 2)) * 2];
       cdef_pixel(tile + (kVB + r) * kTS + kHB + c, dir, (int32_t)srcp[(long)(y0 + r) * cw + x0 + c], cs, dmp,
                  [cs](int P) { return (int32_t)(P << cs); }, acc);
     }
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
     }
   }
   __syncthreads();
-  if (tid < 64) mse[(long)blockIdx.x * 64 + tid] = total[tid];
+  if (tid < 64) mse[(long)fb * 64 + tid] = total[tid];
 }
 
 // ---------------------------------------------------------------------------
@@ -652,7 +654,7 @@ __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dg
   __shared__ int16_t X[SR * YS];
   __shared__ uint64_t red[4];
   __shared__ int64_t gpart[NG > 1 ? (NG - 1) * NSEG * WIN : 1];
-  const int u = blockIdx.x, uy = u / nh, ux = u - uy * nh;
+  const int u = xcd_chunked(blockIdx.x, gridDim.x), uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
   const int uw = he - hs, uh = ve - vs, tw = uw + 2 * HALF;
@@ -758,7 +760,7 @@ __global__ __launch_bounds__(256) void k_unit_sse(const Pix* __restrict__ a, con
                                                   int ph, int unit_size, int ss_y, int nh,
                                                   uint64_t* __restrict__ out) {
   __shared__ uint64_t red[4];
-  const int u = blockIdx.x, uy = u / nh, ux = u - uy * nh;
+  const int u = xcd_chunked(blockIdx.x, gridDim.x), uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
   const int uw = he - hs, n = uw * (ve - vs);
@@ -815,7 +817,8 @@ __global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, 
   __shared__ uint16_t XBY[256];
   __shared__ uint64_t red[4];
   constexpr int NG = 16 / kSgSets;
-  const int u = blockIdx.x / NG, sg = blockIdx.x - u * NG, uy = u / nh, ux = u - uy * nh;
+  const int lb = xcd_chunked(blockIdx.x, gridDim.x);
+  const int u = lb / NG, sg = lb - u * NG, uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
   const int W = he - hs, Hh = ve - vs;

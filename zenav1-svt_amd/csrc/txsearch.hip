@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
 
   const int g = threadIdx.x / G::T, t = threadIdx.x - g * G::T;
   const int nvar = plan.n_var, ntyp = plan.n_types;
-  const long b = (long)blockIdx.x * (blockDim.x / G::T) + g;
+  const long b = (long)xcd_chunked(blockIdx.x, gridDim.x) * (blockDim.x / G::T) + g;
   const bool active = b < n_blocks;
   const int bi = active ? (int)b : 0;
   const int x0 = xy[2 * bi], y0 = xy[2 * bi + 1];

@@ -7,6 +7,16 @@
 
 namespace zav1 {
 
+// Logical index of workgroup b of a grid of G with the logical range cut
+// into 8 contiguous chunks, one per XCD: the dispatcher hands consecutive
+// workgroups round-robin to the 8 XCDs (b runs on XCD b % 8), so chunking
+// keeps neighbouring work items, which share halo and source lines, in one
+// XCD's L2.  A bijection on [0, G).
+__device__ __forceinline__ int xcd_chunked(int b, int G) {
+  const int x = b & 7, i = b >> 3, per = G >> 3, rem = G & 7;
+  return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
 // Orders one wave's LDS traffic between its lanes (a wave-level fence and
 // barrier; lanes of other waves are not waited for).
 __device__ __forceinline__ void wave_sync() {
