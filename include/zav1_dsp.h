@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define ZAV1_ABI_VERSION 1u
+#define ZAV1_ABI_VERSION 2u
 
 #define ZAV1_OK 0
 #define ZAV1_EINVAL (-22)
@@ -126,6 +126,73 @@ typedef struct zav1_dsp_v1 {
    * NULL) receives the reference's coefficient order; *satd the SATD. */
   int (*hadamard_satd)(const int16_t *src_diff, size_t stride, int n, int32_t *coeff,
                        int32_t *satd);
+
+  /* ---- ABI 2 (appended: an ABI 1 caller reads the prefix unchanged) ---- */
+  /* In the entries below 8-bit samples are uint8_t and 10-bit uint16_t; the
+   * void * sample buffers follow bd (8: uint8_t, 10: uint16_t).  Outputs are
+   * written only where the reference writes. */
+
+  /* residual_i32 (rust/crates/svtav1-dsp/src/residual.rs:26) and its 16-bit
+   * twin: out[r*w + c] = src - pred, w x h (1..128). */
+  int (*residual_i32)(const uint8_t *src, size_t src_stride, const uint8_t *pred, size_t pred_stride,
+                      size_t w, size_t h, int32_t *out);
+  int (*residual_i32_u16)(const uint16_t *src, size_t src_stride, const uint16_t *pred, size_t pred_stride,
+                          size_t w, size_t h, int32_t *out);
+  /* recon_add_clamp (residual.rs:140): out[r*w + c] = clip(pred + inv, 0, 255);
+   * the 16-bit twin clips to 2^bd - 1 (highbd_clip_pixel_add, hbd.rs:145). */
+  int (*recon_add_clamp)(const uint8_t *pred, size_t pred_stride, const int32_t *inv, size_t w, size_t h,
+                         uint8_t *out);
+  int (*recon_add_clamp_u16)(const uint16_t *pred, size_t pred_stride, const int32_t *inv, size_t w, size_t h,
+                             uint16_t *out, int bd);
+  /* variance::sse (variance.rs:60) / full_distortion_kernel16_bits
+   * (hbd.rs:936): sum of squared differences over w x h into *out. */
+  int (*sse_u8)(const uint8_t *a, size_t a_stride, const uint8_t *b, size_t b_stride, size_t w, size_t h,
+                uint64_t *out);
+  int (*sse_u16)(const uint16_t *a, size_t a_stride, const uint16_t *b, size_t b_stride, size_t w, size_t h,
+                 uint64_t *out);
+  /* predict_dc / _v / _h / _smooth / _smooth_v / _smooth_h / _paeth
+   * (intra_pred.rs:18-154; hbd.rs:169-361): mode 0 DC (has_above / has_left
+   * as the reference's flags), 1 V, 2 H, 9 SMOOTH, 10 SMOOTH_V, 11 SMOOTH_H,
+   * 12 PAETH (top_left used).  above[0..w), left[0..h); w, h in 4..64. */
+  int (*intra_pred)(int mode, void *dst, size_t dst_stride, const void *above, const void *left, int top_left,
+                    int w, int h, int has_above, int has_left, int bd);
+  /* dr_predictor_edged (intra_pred.rs:810; hbd.rs:436-700): final (filtered,
+   * upsampled) edge buffers of edge_len samples with sample 0 of the row /
+   * column at `origin` and the corner at origin - 1; angle = a directional
+   * mode's nominal angle (45, 67, 90, 113, 135, 157, 180, 203) + 3 * delta,
+   * |delta| <= 3; upsampling only for w + h <= 16. */
+  int (*dr_predictor_edged)(void *dst, size_t dst_stride, const void *above, const void *left, size_t edge_len,
+                            size_t origin, int upsample_above, int upsample_left, int w, int h, int angle, int bd);
+  /* predict_filter_intra (intra_pred.rs:988; hbd.rs:784): above[0] the
+   * top-left sample, above[1..=w] the row above; mode 0..4; w, h <= 32. */
+  int (*predict_filter_intra)(void *dst, size_t dst_stride, const void *above, const void *left, int w, int h,
+                              int mode, int bd);
+  /* cfl_predict_lbd (intra_pred.rs:1137; hbd.rs:889): ac_q3 with row stride
+   * 32 (CFL_BUF_LINE), pred the DC prediction; dst = clip(pred + alpha*ac). */
+  int (*cfl_predict)(const int16_t *ac_q3, const void *pred, size_t pred_stride, void *dst, size_t dst_stride,
+                     int alpha_q3, int w, int h, int bd);
+  /* lpf_{horizontal,vertical}_{4,6,8,14} (loop_filter.rs:307-405) and the
+   * _hbd twins (hbd.rs:1335-1433): filters the 4 lines of the call at sample
+   * offset `off` of buf (buf_len samples) in place; thresholds as
+   * lf_thresholds (loop_filter.rs:50) gives them at 8 bits. */
+  int (*lpf)(void *buf, size_t buf_len, size_t off, size_t pitch, int vertical, int taps, uint8_t mblim,
+             uint8_t lim, uint8_t hev_thr, int bd);
+  /* cdef_find_dir (cdef.rs:143): returns the direction 0..7 of the 8x8 u16
+   * block (negative on error), *var its variance. */
+  int (*cdef_find_dir)(const uint16_t *img, size_t stride, int coeff_shift, int32_t *var);
+  /* cdef_filter_block (cdef.rs:236) / cdef_filter_block_hbd (hbd.rs:1533):
+   * `in` the padded u16 source (row stride 144 = CDEF_BSTRIDE, in_len
+   * samples, 3-row / 8-column border around the block at in + ioff); bsize
+   * 0 4x4, 1 4x8, 2 8x4, 3 8x8; rows stepped by subsampling_factor. */
+  int (*cdef_filter_block)(void *dst, size_t dst_stride, const uint16_t *in, size_t in_len, size_t ioff,
+                           int pri_strength, int sec_strength, int dir, int pri_damping, int sec_damping,
+                           int bsize, int coeff_shift, size_t subsampling_factor, int bd);
+  /* compute_stats (restoration.rs:232) / compute_stats_hbd (:1738): Wiener
+   * M (win^2) and H (win^4) over [h_start, h_end) x [v_start, v_end), dgd
+   * with win/2 samples of margin; window sides <= 384. */
+  int (*compute_stats)(int wiener_win, const void *dgd, size_t dgd_len, size_t dgd_origin, size_t dgd_stride,
+                       const void *src, size_t src_len, size_t src_origin, size_t src_stride, int h_start,
+                       int h_end, int v_start, int v_end, int64_t *M, int64_t *H, int bd);
 } zav1_dsp_v1;
 
 /* Returns the table for `backend` (1 = HIP), or NULL if no GPU is usable. */

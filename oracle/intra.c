@@ -162,6 +162,22 @@ static void dr_z3(int32_t *dst, int bw, int bh, const int32_t *left, int up, int
   }
 }
 
+/* dsp/intra_pred.rs:810 dr_predictor_edged: final edge buffers with the
+ * sample 0 of the row / column at EDGE_ORIGIN; angle in (0, 270). */
+void or_dr_predict_edges(const int32_t *above, const int32_t *left, int w, int h, int angle, int up_a, int up_l,
+                         int bd, int32_t *dst) {
+  const int maxv = (1 << bd) - 1;
+  if (angle > 0 && angle < 90) dr_z1(dst, w, h, above, up_a, get_dx(angle), maxv);
+  else if (angle > 90 && angle < 180) dr_z2(dst, w, h, above, left, up_a, up_l, get_dx(angle), get_dy(angle), maxv);
+  else if (angle > 180 && angle < 270) dr_z3(dst, w, h, left, up_l, get_dy(angle), maxv);
+  else if (angle == 90)
+    for (int r = 0; r < h; r++)
+      for (int c = 0; c < w; c++) dst[r * w + c] = above[EDGE_ORIGIN + c];
+  else if (angle == 180)
+    for (int r = 0; r < h; r++)
+      for (int c = 0; c < w; c++) dst[r * w + c] = left[EDGE_ORIGIN + r];
+}
+
 /* Plane sample fetch: 8-bit or 16-bit plane. */
 static int32_t S(const void *p, int stride, int hbd, int x, int y) {
   return hbd ? ((const uint16_t *)p)[(long)y * stride + x] : ((const uint8_t *)p)[(long)y * stride + x];

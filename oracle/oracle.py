@@ -202,6 +202,25 @@ def md_funnel(src, recon, bd, intra_jobs, n_cands, rate, lam, k, tx_size, tx_typ
     return sel, cost, dist, eob, cul
 
 
+def dr_predict_edges(above, left, w, h, angle, up_a, up_l, bd):
+    """Directional prediction from final 160-sample edges (origin 16)."""
+    L = lib()
+    L.or_dr_predict_edges.argtypes = [_vp, _vp] + [ctypes.c_int] * 6 + [_vp]
+    a = np.ascontiguousarray(above, dtype=np.int32)
+    lf = np.ascontiguousarray(left, dtype=np.int32)
+    out = np.zeros((h, w), np.int32)
+    L.or_dr_predict_edges(a.ctypes.data, lf.ctypes.data, w, h, angle, int(up_a), int(up_l), bd, out.ctypes.data)
+    return out
+
+
+def lf_thresholds(level, sharpness):
+    L = lib()
+    L.or_lf_thresholds.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 3
+    m, li, hv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.or_lf_thresholds(level, sharpness, ctypes.byref(m), ctypes.byref(li), ctypes.byref(hv))
+    return m.value, li.value, hv.value
+
+
 def _bind_txsearch(L):
     L.or_scan.argtypes = [ctypes.c_int, ctypes.c_int, np.ctypeslib.ndpointer(dtype=np.uint16, flags='C_CONTIGUOUS')]
     L.or_scan_class.argtypes = [ctypes.c_int]

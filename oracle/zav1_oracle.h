@@ -147,6 +147,10 @@ int or_edge_filter_strength(int bs0, int bs1, int delta, int filt_type);
 int or_use_upsample(int bs0, int bs1, int delta, int filt_type);
 /* lf/predict.rs:49 predict_unit (+ hbd twins): w*h samples into dst (raster). */
 int or_intra_predict(const void *plane, int stride, int bd, const or_intra_job *job, int32_t *dst);
+/* dsp/intra_pred.rs:810 dr_predictor_edged on final edges (160 samples,
+ * origin 16). */
+void or_dr_predict_edges(const int32_t *above, const int32_t *left, int w, int h, int angle, int up_a, int up_l,
+                         int bd, int32_t *dst);
 void or_cfl_predict(const void *luma, int luma_stride, int bd, int cw, int ch, const int32_t *dc, int alpha_q3,
                     int32_t *dst);
 

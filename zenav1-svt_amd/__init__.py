@@ -52,6 +52,9 @@ class QuantRow(ctypes.Structure):
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _I16P = ctypes.POINTER(ctypes.c_int16)
 _U64P = ctypes.POINTER(ctypes.c_uint64)
+_U16P = ctypes.POINTER(ctypes.c_uint16)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_sz = ctypes.c_size_t
 
 
 class DspV1(ctypes.Structure):
@@ -67,6 +70,25 @@ class DspV1(ctypes.Structure):
                                                _I32P, _I32P, ctypes.c_int32)),
         ('sse_i32', ctypes.CFUNCTYPE(_i32, _I32P, _I32P, ctypes.c_size_t, _U64P)),
         ('hadamard_satd', ctypes.CFUNCTYPE(_i32, _I16P, ctypes.c_size_t, _i32, _I32P, _I32P)),
+        # ABI 2
+        ('residual_i32', ctypes.CFUNCTYPE(_i32, _vp, _sz, _vp, _sz, _sz, _sz, _I32P)),
+        ('residual_i32_u16', ctypes.CFUNCTYPE(_i32, _vp, _sz, _vp, _sz, _sz, _sz, _I32P)),
+        ('recon_add_clamp', ctypes.CFUNCTYPE(_i32, _vp, _sz, _I32P, _sz, _sz, _vp)),
+        ('recon_add_clamp_u16', ctypes.CFUNCTYPE(_i32, _vp, _sz, _I32P, _sz, _sz, _vp, _i32)),
+        ('sse_u8', ctypes.CFUNCTYPE(_i32, _vp, _sz, _vp, _sz, _sz, _sz, _U64P)),
+        ('sse_u16', ctypes.CFUNCTYPE(_i32, _vp, _sz, _vp, _sz, _sz, _sz, _U64P)),
+        ('intra_pred', ctypes.CFUNCTYPE(_i32, _i32, _vp, _sz, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32)),
+        ('dr_predictor_edged', ctypes.CFUNCTYPE(_i32, _vp, _sz, _vp, _vp, _sz, _sz, _i32, _i32, _i32, _i32, _i32,
+                                                _i32)),
+        ('predict_filter_intra', ctypes.CFUNCTYPE(_i32, _vp, _sz, _vp, _vp, _i32, _i32, _i32, _i32)),
+        ('cfl_predict', ctypes.CFUNCTYPE(_i32, _I16P, _vp, _sz, _vp, _sz, _i32, _i32, _i32, _i32)),
+        ('lpf', ctypes.CFUNCTYPE(_i32, _vp, _sz, _sz, _sz, _i32, _i32, ctypes.c_uint8, ctypes.c_uint8,
+                                 ctypes.c_uint8, _i32)),
+        ('cdef_find_dir', ctypes.CFUNCTYPE(_i32, _U16P, _sz, _i32, _I32P)),
+        ('cdef_filter_block', ctypes.CFUNCTYPE(_i32, _vp, _sz, _U16P, _sz, _sz, _i32, _i32, _i32, _i32, _i32, _i32,
+                                               _i32, _sz, _i32)),
+        ('compute_stats', ctypes.CFUNCTYPE(_i32, _i32, _vp, _sz, _sz, _sz, _vp, _sz, _sz, _sz, _i32, _i32, _i32,
+                                           _i32, _I64P, _I64P, _i32)),
     ]
 
 

@@ -301,11 +301,438 @@ int plain_hadamard_satd(const int16_t* src, size_t stride, int n, int32_t* coeff
   return ZAV1_OK;
 }
 
+// ---- plain form, pixel-domain and loop-filter entries --------------------
+// Staging layout helper: consecutive 256-byte aligned regions of one buffer.
+struct Carve {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    T* p = (T*)(base + off);
+    off += (sizeof(T) * count + 255) & ~(size_t)255;
+    return p;
+  }
+};
+constexpr size_t kCarveMax = 1u << 20;
+
+// Packs a strided w x h block of `es`-byte samples into dst.
+void pack2d(void* dst, const void* src, size_t stride, size_t w, size_t h, size_t es) {
+  for (size_t r = 0; r < h; r++) std::memcpy((char*)dst + r * w * es, (const char*)src + r * stride * es, w * es);
+}
+void unpack2d(void* dst, size_t stride, const void* src, size_t w, size_t h, size_t es) {
+  for (size_t r = 0; r < h; r++) std::memcpy((char*)dst + r * stride * es, (const char*)src + r * w * es, w * es);
+}
+bool plain_dims(size_t w, size_t h) { return w >= 1 && h >= 1 && w <= 128 && h <= 128; }
+
+int residual_common(const void* src, size_t ss, const void* pred, size_t ps, size_t w, size_t h, int32_t* out,
+                    int hbd) {
+  if (!src || !pred || !out || !plain_dims(w, h) || ss < w || ps < w) return ZAV1_EINVAL;
+  const size_t n = w * h, es = hbd ? 2 : 1;
+  std::vector<char> a(n * es), b(n * es);
+  pack2d(a.data(), src, ss, w, h, es);
+  pack2d(b.data(), pred, ps, w, h, es);
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  char* da = c.take<char>(n * es);
+  char* db = c.take<char>(n * es);
+  int32_t* dout = c.take<int32_t>(n);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(da, a.data(), n * es, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(db, b.data(), n * es, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_residual(da, db, (int)n, hbd, dout, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+int plain_residual_i32(const uint8_t* src, size_t ss, const uint8_t* pred, size_t ps, size_t w, size_t h,
+                       int32_t* out) {
+  return residual_common(src, ss, pred, ps, w, h, out, 0);
+}
+int plain_residual_i32_u16(const uint16_t* src, size_t ss, const uint16_t* pred, size_t ps, size_t w, size_t h,
+                           int32_t* out) {
+  return residual_common(src, ss, pred, ps, w, h, out, 1);
+}
+
+int recon_common(const void* pred, size_t ps, const int32_t* inv, size_t w, size_t h, void* out, int bd, int hbd) {
+  if (!pred || !inv || !out || !plain_dims(w, h) || ps < w || (bd != 8 && bd != 10) || (!hbd && bd != 8))
+    return ZAV1_EINVAL;
+  const size_t n = w * h, es = hbd ? 2 : 1;
+  std::vector<char> a(n * es), o(n * es);
+  pack2d(a.data(), pred, ps, w, h, es);
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  char* dp = c.take<char>(n * es);
+  int32_t* di = c.take<int32_t>(n);
+  char* dout = c.take<char>(n * es);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dp, a.data(), n * es, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(di, inv, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_recon(dp, di, (int)n, bd, hbd, dout, s));
+  ZCHECK(hipMemcpyAsync(out, dout, n * es, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+int plain_recon_add_clamp(const uint8_t* pred, size_t ps, const int32_t* inv, size_t w, size_t h, uint8_t* out) {
+  return recon_common(pred, ps, inv, w, h, out, 8, 0);
+}
+int plain_recon_add_clamp_u16(const uint16_t* pred, size_t ps, const int32_t* inv, size_t w, size_t h, uint16_t* out,
+                              int bd) {
+  return recon_common(pred, ps, inv, w, h, out, bd, 1);
+}
+
+int sse_common(const void* a, size_t as, const void* b, size_t bs, size_t w, size_t h, uint64_t* out, int hbd) {
+  if (!a || !b || !out || !plain_dims(w, h) || as < w || bs < w) return ZAV1_EINVAL;
+  const size_t n = w * h, es = hbd ? 2 : 1;
+  std::vector<char> pa(n * es), pb(n * es);
+  pack2d(pa.data(), a, as, w, h, es);
+  pack2d(pb.data(), b, bs, w, h, es);
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  char* da = c.take<char>(n * es);
+  char* db = c.take<char>(n * es);
+  uint64_t* dout = c.take<uint64_t>(1);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(da, pa.data(), n * es, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(db, pb.data(), n * es, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_sse(da, db, (int)n, hbd, dout, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+int plain_sse_u8(const uint8_t* a, size_t as, const uint8_t* b, size_t bs, size_t w, size_t h, uint64_t* out) {
+  return sse_common(a, as, b, bs, w, h, out, 0);
+}
+int plain_sse_u16(const uint16_t* a, size_t as, const uint16_t* b, size_t bs, size_t w, size_t h, uint64_t* out) {
+  return sse_common(a, as, b, bs, w, h, out, 1);
+}
+
+bool pred_dims(int w, int h) {
+  auto ok = [](int v) { return v == 4 || v == 8 || v == 16 || v == 32 || v == 64; };
+  return ok(w) && ok(h);
+}
+
+// Runs one plain prediction and writes the w x h result into dst (stride).
+int run_plain_intra(const zav1::PlainIntra& p, const std::vector<int32_t>& row, const std::vector<int32_t>& col,
+                    const std::vector<int16_t>& above, const std::vector<int16_t>& left, void* dst,
+                    size_t dst_stride) {
+  const size_t n = (size_t)p.w * p.h, es = p.bd > 8 ? 2 : 1;
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  int32_t* drow = c.take<int32_t>(160);
+  int32_t* dcol = c.take<int32_t>(160);
+  int16_t* dab = c.take<int16_t>(160);
+  int16_t* dlf = c.take<int16_t>(160);
+  char* dout = c.take<char>(n * es);
+  hipStream_t s = hipStreamPerThread;
+  if (!row.empty()) ZCHECK(hipMemcpyAsync(drow, row.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, s));
+  if (!col.empty()) ZCHECK(hipMemcpyAsync(dcol, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice, s));
+  if (!above.empty())
+    ZCHECK(hipMemcpyAsync(dab, above.data(), sizeof(int16_t) * above.size(), hipMemcpyHostToDevice, s));
+  if (!left.empty()) ZCHECK(hipMemcpyAsync(dlf, left.data(), sizeof(int16_t) * left.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_intra(p, drow, dcol, dab, dlf, dout, s));
+  std::vector<char> out(n * es);
+  ZCHECK(hipMemcpyAsync(out.data(), dout, n * es, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  unpack2d(dst, dst_stride, out.data(), p.w, p.h, es);
+  return ZAV1_OK;
+}
+
+int32_t sample_at(const void* p, int hbd, size_t i) {
+  return hbd ? ((const uint16_t*)p)[i] : ((const uint8_t*)p)[i];
+}
+
+// predict_dc / _v / _h / _smooth / _smooth_v / _smooth_h / _paeth
+// (intra_pred.rs:18-154, hbd.rs:169-361): mode 0, 1, 2, 9, 10, 11, 12.
+int plain_intra_pred(int mode, void* dst, size_t dst_stride, const void* above, const void* left, int top_left, int w,
+                     int h, int has_above, int has_left, int bd) {
+  if (!dst || !pred_dims(w, h) || dst_stride < (size_t)w || (bd != 8 && bd != 10)) return ZAV1_EINVAL;
+  if (!(mode == 0 || mode == 1 || mode == 2 || (mode >= 9 && mode <= 12))) return ZAV1_EINVAL;
+  const bool dc = mode == 0;
+  const bool need_a = dc ? has_above != 0 : mode != 2, need_l = dc ? has_left != 0 : mode != 1;
+  if ((need_a && !above) || (need_l && !left)) return ZAV1_EINVAL;
+  const int hbd = bd > 8;
+  std::vector<int32_t> row(1 + w, 0), col(h, 0);
+  row[0] = top_left;
+  for (int i = 0; i < w && above; i++) row[1 + i] = sample_at(above, hbd, i);
+  for (int i = 0; i < h && left; i++) col[i] = sample_at(left, hbd, i);
+  zav1::PlainIntra p{};
+  p.kind = 0;
+  p.mode = mode;
+  p.fi_mode = 5;
+  p.w = w;
+  p.h = h;
+  p.bd = bd;
+  p.has_above = dc ? (has_above != 0) : 1;
+  p.has_left = dc ? (has_left != 0) : 1;
+  return run_plain_intra(p, row, col, {}, {}, dst, dst_stride);
+}
+
+// predict_filter_intra (intra_pred.rs:988, hbd.rs:784): above[0] = top-left,
+// above[1..=w] the row above; left[0..h).
+int plain_filter_intra(void* dst, size_t dst_stride, const void* above, const void* left, int w, int h, int mode,
+                       int bd) {
+  if (!dst || !above || !left || !pred_dims(w, h) || w > 32 || h > 32 || dst_stride < (size_t)w || mode < 0 ||
+      mode > 4 || (bd != 8 && bd != 10))
+    return ZAV1_EINVAL;
+  const int hbd = bd > 8;
+  std::vector<int32_t> row(1 + w), col(h);
+  for (int i = 0; i <= w; i++) row[i] = sample_at(above, hbd, i);
+  for (int i = 0; i < h; i++) col[i] = sample_at(left, hbd, i);
+  zav1::PlainIntra p{};
+  p.kind = 0;
+  p.mode = 0;
+  p.fi_mode = mode;
+  p.w = w;
+  p.h = h;
+  p.bd = bd;
+  p.has_above = 1;
+  p.has_left = 1;
+  return run_plain_intra(p, row, col, {}, {}, dst, dst_stride);
+}
+
+// dr_predictor_edged (intra_pred.rs:810, hbd.rs:700): above / left edge
+// buffers of edge_len samples, `origin` = index of sample 0 of the row /
+// column (the corner at origin - 1).
+int plain_dr_predictor_edged(void* dst, size_t dst_stride, const void* above, const void* left, size_t edge_len,
+                             size_t origin, int upsample_above, int upsample_left, int w, int h, int angle, int bd) {
+  if (!dst || !above || !left || !pred_dims(w, h) || dst_stride < (size_t)w || origin >= edge_len ||
+      (bd != 8 && bd != 10))
+    return ZAV1_EINVAL;
+  // the angles the encoder forms: a directional mode's nominal angle +- 3, 6, 9
+  // degrees; upsampled edges exist only for w + h <= 16 (use_intra_edge_upsample)
+  static const int kNominal[8] = {45, 67, 90, 113, 135, 157, 180, 203};
+  bool angle_ok = false;
+  for (int m = 0; m < 8; m++)
+    for (int d = -3; d <= 3; d++) angle_ok |= angle == kNominal[m] + 3 * d;
+  if (!angle_ok || ((upsample_above || upsample_left) && w + h > 16)) return ZAV1_EINVAL;
+  const int hbd = bd > 8;
+  std::vector<int16_t> ab(160, 0), lf(160, 0);
+  for (int k = 0; k < 160; k++) {
+    const long i = (long)origin - 16 + k;
+    if (i >= 0 && i < (long)edge_len) {
+      ab[k] = (int16_t)sample_at(above, hbd, (size_t)i);
+      lf[k] = (int16_t)sample_at(left, hbd, (size_t)i);
+    }
+  }
+  zav1::PlainIntra p{};
+  p.kind = 1;
+  p.w = w;
+  p.h = h;
+  p.bd = bd;
+  p.angle = angle;
+  p.ua = upsample_above ? 1 : 0;
+  p.ul = upsample_left ? 1 : 0;
+  return run_plain_intra(p, {}, {}, ab, lf, dst, dst_stride);
+}
+
+// cfl_predict_lbd (intra_pred.rs:1137) / cfl_predict_hbd (hbd.rs:889): ac in
+// Q3 with row stride 32 (CFL_BUF_LINE).
+int plain_cfl_predict(const int16_t* ac_q3, const void* pred, size_t pred_stride, void* dst, size_t dst_stride,
+                      int alpha_q3, int w, int h, int bd) {
+  if (!ac_q3 || !pred || !dst || w < 4 || h < 4 || w > 32 || h > 32 || pred_stride < (size_t)w ||
+      dst_stride < (size_t)w || (bd != 8 && bd != 10))
+    return ZAV1_EINVAL;
+  const size_t n = (size_t)w * h, es = bd > 8 ? 2 : 1;
+  std::vector<int16_t> ac(n);
+  for (int r = 0; r < h; r++) std::memcpy(&ac[(size_t)r * w], ac_q3 + r * 32, sizeof(int16_t) * w);
+  std::vector<char> pp(n * es), out(n * es);
+  pack2d(pp.data(), pred, pred_stride, w, h, es);
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  int16_t* dac = c.take<int16_t>(n);
+  char* dp = c.take<char>(n * es);
+  char* dout = c.take<char>(n * es);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dac, ac.data(), sizeof(int16_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(dp, pp.data(), n * es, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_cfl(dac, dp, alpha_q3, (int)n, bd, dout, s));
+  ZCHECK(hipMemcpyAsync(out.data(), dout, n * es, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  unpack2d(dst, dst_stride, out.data(), w, h, es);
+  return ZAV1_OK;
+}
+
+// lpf_{horizontal,vertical}_{4,6,8,14} (loop_filter.rs:307-405) and the
+// _hbd twins (hbd.rs:1335-1433): the 4 lines at sample offset `off`.
+int plain_lpf(void* buf, size_t buf_len, size_t off, size_t pitch, int vertical, int taps, uint8_t mblim,
+              uint8_t lim, uint8_t hev_thr, int bd) {
+  if (!buf || (taps != 4 && taps != 6 && taps != 8 && taps != 14) || (bd != 8 && bd != 10) || pitch == 0)
+    return ZAV1_EINVAL;
+  const int hbd = bd > 8;
+  const size_t es = hbd ? 2 : 1;
+  const long half = taps / 2;
+  std::vector<long> idx(4 * (size_t)taps);
+  for (int i = 0; i < 4; i++) {
+    const long base = vertical ? (long)off + (long)i * (long)pitch : (long)off + i;
+    const long step = vertical ? 1 : (long)pitch;
+    for (int k = 0; k < taps; k++) {
+      const long j = base + (k - half) * step;
+      if (j < 0 || j >= (long)buf_len) return ZAV1_EINVAL;
+      idx[(size_t)i * taps + k] = j;
+    }
+  }
+  std::vector<char> seg(idx.size() * es);
+  for (size_t k = 0; k < idx.size(); k++) std::memcpy(&seg[k * es], (char*)buf + idx[k] * es, es);
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(st, seg.data(), seg.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_lpf(st, hbd, taps, mblim, lim, hev_thr, bd, s));
+  ZCHECK(hipMemcpyAsync(seg.data(), st, seg.size(), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  for (size_t k = 0; k < idx.size(); k++) std::memcpy((char*)buf + idx[k] * es, &seg[k * es], es);
+  return ZAV1_OK;
+}
+
+// cdef_find_dir (cdef.rs:143): 8x8 of u16 samples; returns the direction
+// 0..7 (negative on error), *var the variance.
+int plain_cdef_find_dir(const uint16_t* img, size_t stride, int coeff_shift, int32_t* var) {
+  if (!img || !var || stride < 8 || coeff_shift < 0 || coeff_shift > 4) return ZAV1_EINVAL;
+  uint16_t blk[64];
+  for (int r = 0; r < 8; r++) std::memcpy(blk + r * 8, img + r * stride, 16);
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  uint16_t* db = c.take<uint16_t>(64);
+  int32_t* dout = c.take<int32_t>(2);
+  hipStream_t s = hipStreamPerThread;
+  int32_t out[2];
+  ZCHECK(hipMemcpyAsync(db, blk, sizeof(blk), hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_cdef_dir(db, coeff_shift, dout, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(out), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  *var = out[1];
+  return out[0];
+}
+
+// cdef_filter_block (cdef.rs:236) / cdef_filter_block_hbd (hbd.rs:1533):
+// `in` is the padded u16 source with row stride 144 (CDEF_BSTRIDE), the
+// block at in + ioff; dst u8 at bd 8, u16 at bd 10.  bsize 0 4x4, 1 4x8,
+// 2 8x4, 3 8x8; rows stepped by subsampling_factor.
+int plain_cdef_filter_block(void* dst, size_t dst_stride, const uint16_t* in, size_t in_len, size_t ioff,
+                            int pri_strength, int sec_strength, int dir, int pri_damping, int sec_damping, int bsize,
+                            int coeff_shift, size_t subsampling_factor, int bd) {
+  constexpr long kBS = 144, kTS = 80, kVB = 3, kHB = 8;
+  if (!dst || !in || bsize < 0 || bsize > 3 || dir < 0 || dir > 7 || pri_strength < 0 || sec_strength < 0 ||
+      subsampling_factor < 1 || coeff_shift < 0 || coeff_shift > 4 || (bd != 8 && bd != 10))
+    return ZAV1_EINVAL;
+  if ((pri_strength && pri_damping < 0) || (sec_strength && sec_damping < 0)) return ZAV1_EINVAL;
+  const int rows = (bsize == 3 || bsize == 1) ? 8 : 4, cols = (bsize == 3 || bsize == 2) ? 8 : 4;
+  if (dst_stride < (size_t)cols) return ZAV1_EINVAL;
+  const long lo = (long)ioff - kVB * kBS - kHB, hi = (long)ioff + (rows + kVB - 1) * kBS + cols + kHB - 1;
+  if (lo < 0 || hi >= (long)in_len) return ZAV1_EINVAL;
+  std::vector<uint16_t> tile((size_t)(rows + 2 * kVB) * kTS, 0);
+  for (long r = 0; r < rows + 2 * kVB; r++)
+    std::memcpy(&tile[(size_t)(r * kTS)], in + ioff + (r - kVB) * kBS - kHB, sizeof(uint16_t) * (cols + 2 * kHB));
+  zav1::PlainCdef p{};
+  p.pri = pri_strength;
+  p.sec = sec_strength;
+  p.dir = dir;
+  p.pdamp = pri_damping;
+  p.sdamp = sec_damping;
+  p.rows = rows;
+  p.cols = cols;
+  p.cs = coeff_shift;
+  p.sub = (int)subsampling_factor;
+  char* st = nullptr;
+  int rc = staging(kCarveMax, &st);
+  if (rc) return rc;
+  Carve c{st};
+  uint16_t* dt = c.take<uint16_t>(tile.size());
+  int32_t* dout = c.take<int32_t>(64);
+  int32_t out[64];
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dt, tile.data(), sizeof(uint16_t) * tile.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_cdef_filter(dt, p, dout, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(int32_t) * rows * cols, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  for (int i = 0; i < rows; i += (int)subsampling_factor)
+    for (int j = 0; j < cols; j++) {
+      if (bd > 8) ((uint16_t*)dst)[i * dst_stride + j] = (uint16_t)out[i * cols + j];
+      else ((uint8_t*)dst)[i * dst_stride + j] = (uint8_t)out[i * cols + j];
+    }
+  return ZAV1_OK;
+}
+
+// compute_stats (restoration.rs:232) / compute_stats_hbd (:1738): dgd needs
+// wiener_win / 2 samples of margin around the window (as the reference's
+// search provides); samples u8 at bd 8, u16 at bd 10.
+int plain_compute_stats(int wiener_win, const void* dgd, size_t dgd_len, size_t dgd_origin, size_t dgd_stride,
+                        const void* src, size_t src_len, size_t src_origin, size_t src_stride, int h_start, int h_end,
+                        int v_start, int v_end, int64_t* M, int64_t* H, int bd) {
+  if (!dgd || !src || !M || !H || (wiener_win != 5 && wiener_win != 7) || (bd != 8 && bd != 10) ||
+      h_end <= h_start || v_end <= v_start || h_end - h_start > 384 || v_end - v_start > 384)
+    return ZAV1_EINVAL;
+  const int half = wiener_win / 2, uw = h_end - h_start, uh = v_end - v_start;
+  const int pw = uw + 2 * half, ph = uh + 2 * half;
+  const size_t es = bd > 8 ? 2 : 1;
+  auto at = [](size_t origin, size_t stride, long r, long c) { return (long)origin + r * (long)stride + c; };
+  if (at(dgd_origin, dgd_stride, v_start - half, h_start - half) < 0 ||
+      at(dgd_origin, dgd_stride, v_end + half - 1, h_end + half - 1) >= (long)dgd_len ||
+      at(src_origin, src_stride, v_start, h_start) < 0 ||
+      at(src_origin, src_stride, v_end - 1, h_end - 1) >= (long)src_len)
+    return ZAV1_EINVAL;
+  std::vector<char> pd((size_t)pw * ph * es), psrc((size_t)pw * ph * es, 0);
+  for (int r = 0; r < ph; r++)
+    std::memcpy(&pd[(size_t)r * pw * es],
+                (const char*)dgd + at(dgd_origin, dgd_stride, v_start - half + r, h_start - half) * es, pw * es);
+  for (int r = 0; r < uh; r++)
+    std::memcpy(&psrc[((size_t)(r + half) * pw + half) * es],
+                (const char*)src + at(src_origin, src_stride, v_start + r, h_start) * es, uw * es);
+  const int w2 = wiener_win * wiener_win;
+  char* st = nullptr;
+  int rc = staging(kCarveMax + 2 * pd.size(), &st);
+  if (rc) return rc;
+  Carve c{st};
+  char* dd = c.take<char>(pd.size());
+  char* ds = c.take<char>(psrc.size());
+  int64_t* dm = c.take<int64_t>(w2);
+  int64_t* dh = c.take<int64_t>((size_t)w2 * w2);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dd, pd.data(), pd.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(ds, psrc.data(), psrc.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::plain_wiener_window(dd, ds, pw, ph, bd, wiener_win, half, half + uw, half, half + uh, dm, dh, s));
+  ZCHECK(hipMemcpyAsync(M, dm, sizeof(int64_t) * w2, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipMemcpyAsync(H, dh, sizeof(int64_t) * w2 * w2, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
+}
+
 const zav1_dsp_v1 g_table = {
-    ZAV1_ABI_VERSION,        1u,
-    plain_fwd_txfm2d,        plain_inv_txfm2d_bd,
-    plain_quantize_fp_raster, plain_quantize_b_raster,
-    plain_sse_i32,           plain_hadamard_satd,
+    ZAV1_ABI_VERSION,
+    1u,
+    plain_fwd_txfm2d,
+    plain_inv_txfm2d_bd,
+    plain_quantize_fp_raster,
+    plain_quantize_b_raster,
+    plain_sse_i32,
+    plain_hadamard_satd,
+    plain_residual_i32,
+    plain_residual_i32_u16,
+    plain_recon_add_clamp,
+    plain_recon_add_clamp_u16,
+    plain_sse_u8,
+    plain_sse_u16,
+    plain_intra_pred,
+    plain_dr_predictor_edged,
+    plain_filter_intra,
+    plain_cfl_predict,
+    plain_lpf,
+    plain_cdef_find_dir,
+    plain_cdef_filter_block,
+    plain_compute_stats,
 };
 
 }  // namespace

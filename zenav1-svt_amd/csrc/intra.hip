@@ -108,6 +108,52 @@ __device__ __forceinline__ int32_t wave_sum(int32_t v) {
 }
 }  // namespace
 
+// Directional prediction from final edges (dr_predictor_edged,
+// rust/crates/svtav1-dsp/src/intra_pred.rs:810): above[kO + i] / left[kO + i]
+// are the (filtered, possibly upsampled: ua / ul) above row and left column,
+// above[kO - 1] the corner; angle pa in (0, 270).  Lanes stride the w x h
+// samples; out(i, v) receives sample i.
+template <class OUT>
+__device__ __forceinline__ void dr_predict_edges(int pa, const int16_t* above, const int16_t* left, int w, int h,
+                                                 int ua, int ul, int maxv, const OUT& out, int lane) {
+  const int dx = dr_dx(pa), dy = dr_dy(pa), n = w * h;
+  for (int i = lane; i < n; i += 64) {
+    const int r = i / w, c = i - r * w;
+    int32_t v;
+    if (pa < 90) {  // zone 1: along the above edge
+      const int max_base = (w + h - 1) << ua;
+      const int x = dx * (r + 1);
+      const int b0 = x >> (6 - ua), shift = ((x << ua) & 0x3F) >> 1;
+      const int b = b0 + (c << ua);
+      if (b0 >= max_base || b >= max_base) v = above[kO + max_base];
+      else v = clampi((above[kO + b] * (32 - shift) + above[kO + b + 1] * shift + 16) >> 5, 0, maxv);
+    } else if (pa == 90) {
+      v = above[kO + c];
+    } else if (pa < 180) {  // zone 2
+      const int x = (c << 6) - (r + 1) * dx, b = x >> (6 - ua);
+      int32_t s;
+      if (b >= -(1 << ua)) {
+        const int sh = ((x * (1 << ua)) & 0x3F) >> 1;
+        s = above[kO + b] * (32 - sh) + above[kO + b + 1] * sh;
+      } else {
+        const int y = (r << 6) - (c + 1) * dy, b2 = y >> (6 - ul);
+        const int sh = ((y * (1 << ul)) & 0x3F) >> 1;
+        s = left[kO + b2] * (32 - sh) + left[kO + b2 + 1] * sh;
+      }
+      v = clampi((s + 16) >> 5, 0, maxv);
+    } else if (pa == 180) {
+      v = left[kO + r];
+    } else {  // zone 3: along the left edge
+      const int max_base = (w + h - 1) << ul;
+      const int y = dy * (c + 1);
+      const int b = (y >> (6 - ul)) + (r << ul), shift = ((y << ul) & 0x3F) >> 1;
+      if (b >= max_base) v = left[kO + max_base];
+      else v = clampi((left[kO + b] * (32 - shift) + left[kO + b + 1] * shift + 16) >> 5, 0, maxv);
+    }
+    out(i, v);
+  }
+}
+
 // One unit's prediction by one wave: P(x, y) reads a reference sample of
 // the plane (only the row above, the column left and the top-left corner
 // of the unit, within the job's availability counts); out(i, v) receives
@@ -190,39 +236,7 @@ __device__ void predict_wave(const IntraJob& j, int bd, const NB& P, const OUT& 
       up_l = use_upsample(h, w, pa - 180, j.filt_type);
       if (need_left && up_l) upsample_wave(left, tmp, h + (need_bottom ? w : 0), maxv, lane, sync);
     }
-    const int dx = dr_dx(pa), dy = dr_dy(pa);
-    const int ua = up_a ? 1 : 0, ul = up_l ? 1 : 0;
-    for (int i = lane; i < n; i += 64) {
-      const int r = i / w, c = i - r * w;
-      int32_t v;
-      if (pa < 90) {  // zone 1: along the above edge
-        const int max_base = (w + h - 1) << ua;
-        const int x = dx * (r + 1);
-        const int b0 = x >> (6 - ua), shift = ((x << ua) & 0x3F) >> 1;
-        const int b = b0 + (c << ua);
-        if (b0 >= max_base || b >= max_base) v = above[kO + max_base];
-        else v = clampi((above[kO + b] * (32 - shift) + above[kO + b + 1] * shift + 16) >> 5, 0, maxv);
-      } else if (pa < 180) {  // zone 2
-        const int x = (c << 6) - (r + 1) * dx, b = x >> (6 - ua);
-        int32_t s;
-        if (b >= -(1 << ua)) {
-          const int sh = ((x * (1 << ua)) & 0x3F) >> 1;
-          s = above[kO + b] * (32 - sh) + above[kO + b + 1] * sh;
-        } else {
-          const int y = (r << 6) - (c + 1) * dy, b2 = y >> (6 - ul);
-          const int sh = ((y * (1 << ul)) & 0x3F) >> 1;
-          s = left[kO + b2] * (32 - sh) + left[kO + b2 + 1] * sh;
-        }
-        v = clampi((s + 16) >> 5, 0, maxv);
-      } else {  // zone 3: along the left edge
-        const int max_base = (w + h - 1) << ul;
-        const int y = dy * (c + 1);
-        const int b = (y >> (6 - ul)) + (r << ul), shift = ((y << ul) & 0x3F) >> 1;
-        if (b >= max_base) v = left[kO + max_base];
-        else v = clampi((left[kO + b] * (32 - shift) + left[kO + b + 1] * shift + 16) >> 5, 0, maxv);
-      }
-      out(i, v);
-    }
+    dr_predict_edges(pa, above, left, w, h, up_a ? 1 : 0, up_l ? 1 : 0, maxv, out, lane);
     return;
   }
 
@@ -668,6 +682,77 @@ hipError_t md_predict_survivors(const void* recon, int stride, int pw, int ph, i
   else
     hipLaunchKernelGGL((k_intra_pred<uint16_t, SurvivorJobs>), dim3((unsigned)n), dim3(64), 0, st,
                        (const uint16_t*)recon, stride, bd, sj, (int)n, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Plain form: one unit predicted from edges the caller supplies.
+// kind 0 (predict_dc / _v / _h / _smooth* / _paeth, predict_filter_intra):
+// the predictor reads its reference samples from the cached row / column;
+// kind 1 (dr_predictor_edged): the final edge buffers, origin kO.
+// ---------------------------------------------------------------------------
+template <typename Pix>
+struct PackedOut {
+  Pix* dst;
+  __device__ __forceinline__ void operator()(int i, int32_t v) const { dst[i] = (Pix)v; }
+};
+
+template <typename Pix>
+__global__ __launch_bounds__(64) void k_plain_intra(PlainIntra p, const int32_t* __restrict__ row,
+                                                    const int32_t* __restrict__ col,
+                                                    const int16_t* __restrict__ above_e,
+                                                    const int16_t* __restrict__ left_e, Pix* __restrict__ out) {
+  __shared__ int16_t above[kEdge], left[kEdge], tmp[kEdge];
+  __shared__ int16_t fbuf[33 * 33];
+  const int lane = threadIdx.x;
+  const int maxv = (1 << p.bd) - 1;
+  if (p.kind == 1) {
+    for (int i = lane; i < kEdge; i += 64) {
+      above[i] = above_e[i];
+      left[i] = left_e[i];
+    }
+    __syncthreads();
+    dr_predict_edges(p.angle, above, left, p.w, p.h, p.ua, p.ul, maxv, PackedOut<Pix>{out}, lane);
+    return;
+  }
+  IntraJob j;
+  j.x = 1; j.y = 1; j.w = (uint8_t)p.w; j.h = (uint8_t)p.h;
+  j.mode = (uint8_t)p.mode; j.delta = 0; j.fi_mode = (uint8_t)p.fi_mode;
+  j.edge_filter = 0; j.filt_type = 0; j.pad0 = 0;
+  j.n_top_px = (int16_t)(p.has_above ? p.w : 0); j.n_topright_px = 0;
+  j.n_left_px = (int16_t)(p.has_left ? p.h : 0); j.n_bottomleft_px = 0;
+  j.out_off = 0; j.pad1 = 0;
+  predict_wave(j, p.bd, CacheNb{row, col, 1, 1}, PackedOut<Pix>{out}, BlockSync{}, above, left, tmp, fbuf, lane);
+}
+
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_plain_cfl(const int16_t* __restrict__ ac, const Pix* __restrict__ pred,
+                                                   int alpha_q3, int n, int maxv, Pix* __restrict__ out) {
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int32_t sc = alpha_q3 * ac[i];
+    const int32_t q0 = sc < 0 ? -((-sc + 32) >> 6) : ((sc + 32) >> 6);
+    out[i] = (Pix)clampi(q0 + pred[i], 0, maxv);
+  }
+}
+
+hipError_t plain_intra(const PlainIntra& p, const int32_t* row, const int32_t* col, const int16_t* above,
+                       const int16_t* left, void* out, hipStream_t st) {
+  if (p.bd > 8)
+    hipLaunchKernelGGL((k_plain_intra<uint16_t>), dim3(1), dim3(64), 0, st, p, row, col, above, left,
+                       (uint16_t*)out);
+  else
+    hipLaunchKernelGGL((k_plain_intra<uint8_t>), dim3(1), dim3(64), 0, st, p, row, col, above, left, (uint8_t*)out);
+  return hipGetLastError();
+}
+
+hipError_t plain_cfl(const int16_t* ac, const void* pred, int alpha_q3, int n, int bd, void* out, hipStream_t st) {
+  const int maxv = (1 << bd) - 1;
+  if (bd > 8)
+    hipLaunchKernelGGL((k_plain_cfl<uint16_t>), dim3(1), dim3(256), 0, st, ac, (const uint16_t*)pred, alpha_q3, n,
+                       maxv, (uint16_t*)out);
+  else
+    hipLaunchKernelGGL((k_plain_cfl<uint8_t>), dim3(1), dim3(256), 0, st, ac, (const uint8_t*)pred, alpha_q3, n,
+                       maxv, (uint8_t*)out);
   return hipGetLastError();
 }
 

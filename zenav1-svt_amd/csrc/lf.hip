@@ -298,7 +298,8 @@ __device__ __forceinline__ int adjust_strength(int s, int var) {
   return var ? (s * (4 + i) + 8) >> 4 : 0;
 }
 
-// cdef_find_dir (cdef.rs:143) on the 8x8 at t (LDS tile, stride kTS).
+// cdef_find_dir (cdef.rs:143) on the 8x8 at t (row stride ST).
+template <int ST = kTS>
 __device__ int find_dir(const uint16_t* t, int cs, int* var_out) {
   constexpr int DIV[9] = {0, 840, 420, 280, 210, 168, 140, 120, 105};
   int32_t partial[8][15];
@@ -310,7 +311,7 @@ __device__ int find_dir(const uint16_t* t, int cs, int* var_out) {
   for (int i = 0; i < 8; i++)
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const int x = (t[i * kTS + j] >> cs) - 128;
+      const int x = (t[i * ST + j] >> cs) - 128;
       partial[0][i + j] += x;
       partial[1][i + j / 2] += x;
       partial[2][i] += x;
@@ -645,7 +646,8 @@ __device__ __forceinline__ uint64_t wg_sum_u64(uint64_t v, uint64_t* red) {
 template <typename Pix, int WIN>
 __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
                                                       int pw, int ph, int bd, int unit_size, int ss_y, int nh,
-                                                      int64_t* __restrict__ out_m, int64_t* __restrict__ out_h) {
+                                                      int4 lim, int64_t* __restrict__ out_m,
+                                                      int64_t* __restrict__ out_h) {
   constexpr int W2 = WIN * WIN, HALF = WIN / 2, SR = 16;
   constexpr int NSEG = WIN * (WIN * (WIN + 1) / 2) + WIN;
   constexpr int NG = 256 / NSEG;
@@ -656,7 +658,11 @@ __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dg
   __shared__ int64_t gpart[NG > 1 ? (NG - 1) * NSEG * WIN : 1];
   const int u = xcd_chunked(blockIdx.x, gridDim.x), uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
-  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  if (lim.y > lim.x) {  // one window given explicitly (the plain compute_stats)
+    hs = lim.x; he = lim.y; vs = lim.z; ve = lim.w;
+  } else {
+    unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  }
   const int uw = he - hs, uh = ve - vs, tw = uw + 2 * HALF;
   const int tid = threadIdx.x;
   const int lr0 = tid >> 6, lc0 = tid & 63;  // 4 rows x 64 columns per pass, no divisions
@@ -1062,7 +1068,7 @@ hipError_t wiener_stats(const void* dgd, const void* src, int pw, int ph, int bd
   const unsigned gr = (unsigned)(nh * nv);
 #define ZW(P, WN)                                                                                              \
   hipLaunchKernelGGL((k_wiener_stats<P, WN>), dim3(gr), dim3(256), 0, st, (const P*)dgd, (const P*)src, pw, ph, \
-                     bd, unit_size, ss_y, nh, M, H)
+                     bd, unit_size, ss_y, nh, make_int4(0, 0, 0, 0), M, H)
   if (bd == 8) {
     if (win == 7) ZW(uint8_t, 7);
     else ZW(uint8_t, 5);
@@ -1095,6 +1101,92 @@ hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, i
   else
     hipLaunchKernelGGL((k_sgr_stats<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)dgd,
                        (const uint16_t*)src, pw, ph, bd, unit_size, ss_y, nh, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Plain form: one call of the reference's per-block kernels on staged data.
+// ---------------------------------------------------------------------------
+// lpf_{horizontal,vertical}_{4,6,8,14}(_hbd): the 4 lines of the call packed
+// line-major (len samples each, p side first).
+template <typename Pix>
+__global__ __launch_bounds__(64) void k_plain_lpf(Pix* __restrict__ seg, int len, Thr t) {
+  const int i = threadIdx.x;
+  if (i >= 4) return;
+  if (len == 4) filter_at<Pix, 4>(seg, (long)i * len, 1, t);
+  else if (len == 6) filter_at<Pix, 6>(seg, (long)i * len, 1, t);
+  else if (len == 8) filter_at<Pix, 8>(seg, (long)i * len, 1, t);
+  else filter_at<Pix, 14>(seg, (long)i * len, 1, t);
+}
+
+// cdef_find_dir on one packed 8x8: out = {dir, var}.
+__global__ __launch_bounds__(64) void k_plain_cdef_dir(const uint16_t* __restrict__ blk, int cs,
+                                                       int32_t* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  int var = 0;
+  out[0] = find_dir<8>(blk, cs, &var);
+  out[1] = var;
+}
+
+// cdef_filter_block(_hbd): tile = the call's padded input, rows -kVB .. and
+// columns -kHB .. around the block, row stride kTS; one thread per output
+// sample of the rows the row step keeps.
+__global__ __launch_bounds__(64) void k_plain_cdef_filter(const uint16_t* __restrict__ tile, PlainCdef p,
+                                                          int32_t* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t >= p.rows * p.cols) return;
+  const int i = t / p.cols, j = t - (t / p.cols) * p.cols;
+  if (i % p.sub) return;
+  const uint16_t* cp = tile + (i + kVB) * kTS + j + kHB;
+  PixTaps pt;
+  gather_taps(cp, p.dir, pt);
+  int32_t sum = p.pri ? pri_sum(pt, p.pri, (p.pri >> p.cs) & 1, p.pdamp) : 0;
+  if (p.sec) {
+    const int32_t sh = max(0, p.sdamp - msb32((uint32_t)p.sec));
+#pragma unroll
+    for (int m = 0; m < 8; m++) sum += (m < 4 ? 2 : 1) * constrain(pt.d_s[m], p.sec, sh);
+  }
+  out[t] = cdef_out((int16_t)cp[0], sum, pt.mn, pt.mx);
+}
+
+hipError_t plain_lpf(void* seg, int hbd, int len, int mblim, int lim, int hev, int bd, hipStream_t st) {
+  Thr t;
+  t.mblim = mblim;
+  t.lim = lim;
+  t.hev = hev;
+  t.shift = bd - 8;
+  t.bd = bd;
+  if (hbd)
+    hipLaunchKernelGGL((k_plain_lpf<uint16_t>), dim3(1), dim3(64), 0, st, (uint16_t*)seg, len, t);
+  else
+    hipLaunchKernelGGL((k_plain_lpf<uint8_t>), dim3(1), dim3(64), 0, st, (uint8_t*)seg, len, t);
+  return hipGetLastError();
+}
+
+hipError_t plain_cdef_dir(const uint16_t* blk, int cs, int32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_plain_cdef_dir, dim3(1), dim3(64), 0, st, blk, cs, out);
+  return hipGetLastError();
+}
+
+hipError_t plain_cdef_filter(const uint16_t* tile, const PlainCdef& p, int32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_plain_cdef_filter, dim3(1), dim3(64), 0, st, tile, p, out);
+  return hipGetLastError();
+}
+
+hipError_t plain_wiener_window(const void* dgd, const void* src, int pw, int ph, int bd, int win, int hs, int he,
+                               int vs, int ve, int64_t* M, int64_t* H, hipStream_t st) {
+  const int4 lim = make_int4(hs, he, vs, ve);
+#define ZP(P, WN)                                                                                              \
+  hipLaunchKernelGGL((k_wiener_stats<P, WN>), dim3(1), dim3(256), 0, st, (const P*)dgd, (const P*)src, pw, ph, \
+                     bd, 64, 0, 1, lim, M, H)
+  if (bd == 8) {
+    if (win == 7) ZP(uint8_t, 7);
+    else ZP(uint8_t, 5);
+  } else {
+    if (win == 7) ZP(uint16_t, 7);
+    else ZP(uint16_t, 5);
+  }
+#undef ZP
   return hipGetLastError();
 }
 

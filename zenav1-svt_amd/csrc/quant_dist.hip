@@ -338,4 +338,77 @@ hipError_t pred_satd_batch(const void* src, int stride, int pw, int ph, int bd, 
                           st);
 }
 
+// ---------------------------------------------------------------------------
+// Plain form: residual_i32 / recon_add_clamp / variance::sse (and their 16-bit
+// twins) over one staged block of n packed samples.
+// ---------------------------------------------------------------------------
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_plain_residual(const Pix* __restrict__ src, const Pix* __restrict__ pred,
+                                                        int n, int32_t* __restrict__ out) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) out[i] = (int32_t)src[i] - (int32_t)pred[i];
+}
+
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_plain_recon(const Pix* __restrict__ pred, const int32_t* __restrict__ inv,
+                                                     int n, int32_t maxv, Pix* __restrict__ out) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int32_t v = (int32_t)pred[i] + inv[i];
+    out[i] = (Pix)(v < 0 ? 0 : (v > maxv ? maxv : v));
+  }
+}
+
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_plain_sse(const Pix* __restrict__ a, const Pix* __restrict__ b, int n,
+                                                   unsigned long long* __restrict__ out) {
+  uint64_t acc = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int32_t d = (int32_t)a[i] - (int32_t)b[i];
+    acc += (uint64_t)((int64_t)d * d);
+  }
+  acc = wave_sum_u64(acc);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)acc);
+}
+
+static unsigned plain_grid(int n) {
+  const int g = (n + 255) / 256;
+  return (unsigned)(g < 1 ? 1 : (g > 1024 ? 1024 : g));
+}
+
+hipError_t plain_residual(const void* src, const void* pred, int n, int hbd, int32_t* out, hipStream_t st) {
+  if (hbd)
+    hipLaunchKernelGGL((k_plain_residual<uint16_t>), dim3(plain_grid(n)), dim3(256), 0, st, (const uint16_t*)src,
+                       (const uint16_t*)pred, n, out);
+  else
+    hipLaunchKernelGGL((k_plain_residual<uint8_t>), dim3(plain_grid(n)), dim3(256), 0, st, (const uint8_t*)src,
+                       (const uint8_t*)pred, n, out);
+  return hipGetLastError();
+}
+
+hipError_t plain_recon(const void* pred, const int32_t* inv, int n, int bd, int hbd, void* out, hipStream_t st) {
+  const int32_t maxv = (1 << bd) - 1;
+  if (hbd)
+    hipLaunchKernelGGL((k_plain_recon<uint16_t>), dim3(plain_grid(n)), dim3(256), 0, st, (const uint16_t*)pred, inv,
+                       n, maxv, (uint16_t*)out);
+  else
+    hipLaunchKernelGGL((k_plain_recon<uint8_t>), dim3(plain_grid(n)), dim3(256), 0, st, (const uint8_t*)pred, inv, n,
+                       maxv, (uint8_t*)out);
+  return hipGetLastError();
+}
+
+// out must be zeroed on the stream first (a fill kernel, not a memset node)
+__global__ void k_zero_u64_one(unsigned long long* p) {
+  if (threadIdx.x == 0) *p = 0;
+}
+
+hipError_t plain_sse(const void* a, const void* b, int n, int hbd, uint64_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64_one, dim3(1), dim3(64), 0, st, (unsigned long long*)out);
+  if (hbd)
+    hipLaunchKernelGGL((k_plain_sse<uint16_t>), dim3(plain_grid(n)), dim3(256), 0, st, (const uint16_t*)a,
+                       (const uint16_t*)b, n, (unsigned long long*)out);
+  else
+    hipLaunchKernelGGL((k_plain_sse<uint8_t>), dim3(plain_grid(n)), dim3(256), 0, st, (const uint8_t*)a,
+                       (const uint8_t*)b, n, (unsigned long long*)out);
+  return hipGetLastError();
+}
+
 }  // namespace zav1

@@ -231,4 +231,31 @@ hipError_t unit_sse(const void* a, const void* b, int pw, int ph, int bd, int un
 hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, int unit_size, int ss_y, int nh,
                      int nv, int64_t* out, hipStream_t st);
 
+// ---- plain form: single calls of the reference's per-block kernels ----
+struct PlainCdef {
+  int32_t pri, sec, dir, pdamp, sdamp, rows, cols, cs, sub;
+};
+// residual (src - pred), recon (clip(pred + inv) to bd), SSE over n packed samples
+hipError_t plain_residual(const void* src, const void* pred, int n, int hbd, int32_t* out, hipStream_t st);
+hipError_t plain_recon(const void* pred, const int32_t* inv, int n, int bd, int hbd, void* out, hipStream_t st);
+hipError_t plain_sse(const void* a, const void* b, int n, int hbd, uint64_t* out, hipStream_t st);
+// 4 lines of one deblocking call (line-major, len samples each), in place
+hipError_t plain_lpf(void* seg, int hbd, int len, int mblim, int lim, int hev, int bd, hipStream_t st);
+hipError_t plain_cdef_dir(const uint16_t* blk8x8, int cs, int32_t* out_dir_var, hipStream_t st);
+// tile: padded input of one block (3 rows / 8 columns border, row stride 80)
+hipError_t plain_cdef_filter(const uint16_t* tile, const PlainCdef& p, int32_t* out, hipStream_t st);
+// compute_stats over one explicit window [hs, he) x [vs, ve) of pw x ph planes
+hipError_t plain_wiener_window(const void* dgd, const void* src, int pw, int ph, int bd, int win, int hs, int he,
+                               int vs, int ve, int64_t* M, int64_t* H, hipStream_t st);
+struct PlainIntra {
+  int32_t kind;  // 0: predictor from cached edges (non-directional / filter intra), 1: directional from final edges
+  int32_t mode, fi_mode, w, h, bd, has_above, has_left, angle, ua, ul;
+};
+// kind 0: row[0] = top-left, row[1 + i] = above[i], col[i] = left[i] (int32);
+// kind 1: above / left int16 edge buffers of 160 samples, origin 16.
+hipError_t plain_intra(const PlainIntra& p, const int32_t* row, const int32_t* col, const int16_t* above,
+                       const int16_t* left, void* out, hipStream_t st);
+// CfL: out = clip(pred + round(alpha * ac / 64)) over n packed samples
+hipError_t plain_cfl(const int16_t* ac, const void* pred, int alpha_q3, int n, int bd, void* out, hipStream_t st);
+
 }  // namespace zav1
