@@ -344,6 +344,64 @@ int zav1_md_predict_survivors(const void *d_recon, int stride, int plane_w, int 
                               const uint8_t *d_sel, int k, uint16_t *d_pred, zav1_stream_t s);
 
 /* ------------------------------------------------------------------------ */
+/* Contexts and MD waves: an HBM budget, image upload and a queue of        */
+/* mode-decision waves, collected when the host needs the results.           */
+/* ------------------------------------------------------------------------ */
+/* A context owns one stream and one device arena of hbm_budget bytes
+ * (allocated once: ZAV1_ENOMEM maps to EncodeError::AllocFailed,
+ * rust/crates/svtav1-types/src/error.rs:22-40).  Allocations from the arena
+ * are 256-byte aligned and live until zav1_ctx_reset or zav1_ctx_destroy. */
+typedef struct zav1_ctx zav1_ctx;
+int zav1_ctx_create(int device, size_t hbm_budget, zav1_ctx **ctx);
+int zav1_ctx_destroy(zav1_ctx *ctx);
+int zav1_ctx_alloc(zav1_ctx *ctx, size_t bytes, void **d_ptr);
+/* Bytes in use / free in the arena. */
+int zav1_ctx_usage(const zav1_ctx *ctx, size_t *used, size_t *capacity);
+/* Drops every arena allocation (call after zav1_wave_collect). */
+int zav1_ctx_reset(zav1_ctx *ctx);
+zav1_stream_t zav1_ctx_stream(zav1_ctx *ctx);
+/* Copies a w x h plane (host rows `host_stride` samples apart; u8 at bd 8,
+ * u16 at bd 10) into a tight plane allocated from the arena. */
+int zav1_image_upload(zav1_ctx *ctx, const void *host, size_t host_stride, int w, int h, int bd, void **d_plane);
+
+/* One MD wave: the units of one size whose neighbours are reconstructed
+ * (an SB-diagonal wavefront's leaves), through MDS0 and MDS1 — the fused fast
+ * loop (zav1_md_fast_satd), survivor selection (zav1_md_fast_select),
+ * survivor prediction (zav1_md_predict_survivors) and the survivors'
+ * transform units of each type in `types` (zav1_txunit_batch) — with every
+ * intermediate allocated from the context arena at submit time.  Outputs
+ * (device, caller-owned) per (unit, rank[, type]): sel, sel_cost
+ * (nullable), dist / eob / cul (unit-major, then rank, then type) and the
+ * packed levels (nullable, <= 32x32 per job). */
+typedef struct zav1_md_wave {
+  const void *d_src, *d_recon;
+  int stride, plane_w, plane_h, bd, w, h;
+  const zav1_md_unit *d_units;
+  int n_units;
+  const zav1_md_cand *cands; /* host, n_cands <= 64 */
+  int n_cands;
+  const uint32_t *d_rate;    /* nullable */
+  uint64_t lambda;
+  int k;
+  const uint8_t *types;      /* host, n_types <= 16 */
+  int n_types;
+  int qindex;
+  unsigned txu_flags;        /* ZAV1_TXU_* (RECON / SPATIAL need d_recon_out) */
+  uint8_t *d_sel;
+  uint64_t *d_sel_cost;
+  uint64_t *d_dist;
+  int32_t *d_eob;
+  uint8_t *d_cul;
+  int32_t *d_qcoeff;
+  uint16_t *d_recon_out;
+} zav1_md_wave;
+/* Enqueues one wave on the context stream; fails with ZAV1_ENOMEM (nothing
+ * enqueued) when the arena cannot hold its intermediates. */
+int zav1_md_wave_submit(zav1_ctx *ctx, const zav1_md_wave *wave);
+/* Waits for every submitted wave; returns the first error any of them met. */
+int zav1_wave_collect(zav1_ctx *ctx);
+
+/* ------------------------------------------------------------------------ */
 /* Intra prediction (batched).                                               */
 /* ------------------------------------------------------------------------ */
 /* One transform unit.  Replaces the per-unit predictor routing of

@@ -106,3 +106,15 @@ def test_rest_unit_grid_matches_oracle(zav1, oracle):
         for size in (1, 17, 63, 64, 95, 96, 97, 200, 383, 384, 385, 4096):
             nh, nv = zav1.rest_unit_grid(us, size, 4096 - size + 1)
             assert (nh, nv) == (L.or_count_units(us, size), L.or_count_units(us, 4096 - size + 1))
+
+
+def test_context_entry_points_validate_without_gpu(zav1):
+    L = zav1.lib()
+    h = ctypes.c_void_p()
+    assert L.zav1_ctx_create(0, 0, ctypes.byref(h)) == zav1.ZAV1_EINVAL
+    assert L.zav1_ctx_create(0, 1 << 20, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_ctx_alloc(None, 16, ctypes.byref(h)) == zav1.ZAV1_EINVAL
+    assert L.zav1_ctx_reset(None) == zav1.ZAV1_EINVAL
+    assert L.zav1_wave_collect(None) == zav1.ZAV1_EINVAL
+    assert L.zav1_md_wave_submit(None, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_ctx_stream(None) is None

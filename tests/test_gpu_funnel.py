@@ -275,3 +275,78 @@ def test_mds0_matches_c_captures_at_origin(gpu):
             assert d_cost.to_host().tolist() == [cands[i][6] for i in b['order']]
             checked += 1
     assert checked == 6
+
+
+def test_md_wave_through_context(gpu, oracle):
+    """A context (arena budget, image upload) running MD waves: results equal
+    the oracle chain for DCT_DCT and the oracle transform unit for a second
+    type; the arena accounts the wave's intermediates; a budget too small for
+    a wave is reported as ZAV1_ENOMEM with nothing enqueued."""
+    z = gpu
+    rng = np.random.default_rng(77)
+    bd, w, h, k = 10, 16, 16, 3
+    src = _plane(rng, bd, 96, 128)
+    recon = np.clip(src.astype(np.int64) + rng.integers(-4, 5, src.shape), 0, 1023).astype(np.uint16)
+    cands = funnel.angle_candidates()
+    rates = rng.integers(0, 800, (1, len(cands))).astype(np.uint32)
+    f = funnel.IntraFunnel(z, src, recon, bd, w, h, 140, k=k, cands=cands, lam=70000, rates=rates)  # unit records
+    ctx = z.Context(0, 64 << 20)
+    try:
+        d_src, d_rec = ctx.upload(src, bd), ctx.upload(recon, bd)
+        nu, nc = f.n_units, len(cands)
+        d_units = ctx.alloc(f.md_units.nbytes)
+        d_rate = ctx.alloc(f.rates.nbytes)
+        z.check(z.lib().zav1_memcpy_h2d(d_units, f.md_units.ctypes.data, f.md_units.nbytes, ctx.stream()), 'h2d')
+        z.check(z.lib().zav1_memcpy_h2d(d_rate, f.rates.ctypes.data, f.rates.nbytes, ctx.stream()), 'h2d')
+        types = np.array([0, 9], np.uint8)
+        nj = nu * k * len(types)
+        outs = {n: ctx.alloc(sz) for n, sz in (('sel', nu * k), ('cost', nu * k * 8), ('dist', nj * 8),
+                                                 ('eob', nj * 4), ('cul', nj))}
+        mc = f.md_cands
+        wave = z.MdWave(d_src, d_rec, 128, 128, 96, bd, w, h, d_units, nu, mc.ctypes.data, nc, d_rate, f.lam, k,
+                        types.ctypes.data, len(types), 140, z.TXU_QUANT_B, outs['sel'], outs['cost'], outs['dist'],
+                        outs['eob'], outs['cul'], None, None)
+        used0, cap = ctx.usage()
+        assert ctx.submit_md_wave(wave) == 0
+        used1, _ = ctx.usage()
+        assert used1 > used0
+        ctx.collect()
+
+        def get(ptr, n, dt):
+            a = np.zeros(n, dt)
+            z.check(z.lib().zav1_memcpy_d2h(a.ctypes.data, ptr, a.nbytes, ctx.stream()), 'd2h')
+            return a
+        sel = get(outs['sel'], nu * k, np.uint8).reshape(nu, k)
+        cost = get(outs['cost'], nu * k, np.uint64).reshape(nu, k)
+        dist = get(outs['dist'], nj, np.uint64).reshape(nu, k, 2)
+        eob = get(outs['eob'], nj, np.int32).reshape(nu, k, 2)
+        cul = get(outs['cul'], nj, np.uint8).reshape(nu, k, 2)
+        osel, ocost, odist, oeob, ocul = oracle.md_funnel(src, recon, bd, f.intra_jobs.view(oracle.INTRA_JOB), nc,
+                                                          f.rates, f.lam, k, f.tx_size, 0, 140)
+        assert np.array_equal(sel, osel) and np.array_equal(cost, ocost)
+        assert np.array_equal(dist[:, :, 0], odist) and np.array_equal(eob[:, :, 0], oeob)
+        assert np.array_equal(cul[:, :, 0], ocul)
+        oj = f.intra_jobs.view(oracle.INTRA_JOB)
+        for u in range(0, nu, 5):
+            x, y = f.xy[u].tolist()
+            blk = np.ascontiguousarray(src[y:y + h, x:x + w])
+            for r in range(k):
+                m = int(sel[u, r])
+                p = oracle.intra_predict(recon, bd, oj[u * nc + m:u * nc + m + 1]).astype(np.uint16)
+                rd, re, rc, _, _ = oracle.tx_unit_full(blk, p, f.tx_size, 9, 140, bd, use_b=True, need_recon=False)
+                assert (int(dist[u, r, 1]), int(eob[u, r, 1]), int(cul[u, r, 1])) == (rd, re, rc)
+    finally:
+        ctx.close()
+    # a budget that holds the inputs (2 x 8 KiB) but not the wave's first
+    # intermediate (the 4 KiB SATD table): nothing may be enqueued
+    small = z.Context(0, 2 * 64 * 64 * 2 + 1024)
+    try:
+        d_src, d_rec = small.upload(src[:64, :64], bd), small.upload(recon[:64, :64], bd)
+        used0, _ = small.usage()
+        wave.d_src, wave.d_recon, wave.stride, wave.plane_w, wave.plane_h = d_src, d_rec, 64, 64, 64
+        wave.n_units = 16
+        assert small.submit_md_wave(wave) == z.ZAV1_ENOMEM
+        assert small.usage()[0] == used0
+        small.collect()
+    finally:
+        small.close()

@@ -232,23 +232,77 @@ def test_hadamard_matches_walsh(oracle, n):
             assert np.array_equal(got, ref)
 
 
+_WORST_L1 = {}
+
+
+def _coefficient_bound(oracle, amp):
+    """Largest |coefficient| any size / type can produce from residuals in
+    [-amp, amp]: the forward transform is linear up to per-stage rounding, so
+    it is the l1 norm of a transform row times amp (measured from impulse
+    responses of amplitude 1023, scaled), plus a margin for the roundings."""
+    if 'l1' not in _WORST_L1:
+        worst, margin = 0.0, 0
+        for ts in range(19):
+            w, h = oracle.tx_dims(ts)
+            for t in range(16):
+                if oracle.fwd_txfm2d(np.zeros((h, w), np.int32), ts, t) is None:
+                    continue
+                M = np.zeros((h * w, h * w))
+                for i in range(h * w):
+                    e = np.zeros(h * w, np.int32)
+                    e[i] = 1023
+                    M[:, i] = oracle.fwd_txfm2d(e.reshape(h, w), ts, t).ravel()
+                worst = max(worst, np.abs(M).sum(axis=1).max() / 1023.0)
+                margin = max(margin, 64 * h * w)
+        _WORST_L1['l1'] = (worst, margin)
+    worst, margin = _WORST_L1['l1']
+    return int(np.ceil(worst * amp)) + margin
+
+
 def test_fused_kernel_coefficient_bound(oracle):
     """The fused search / unit kernels quantize in 32-bit arithmetic under the
-    contract |coefficient| < 2^21 for residuals of bd <= 10 planes.  The
-    forward transform is linear up to per-stage rounding, so the largest
-    coefficient over the residual box [-1023, 1023] is the l1 norm of a
-    transform row times 1023 (measured from impulse responses, plus a margin
-    for the roundings): it must stay far below 2^21 for every size and type."""
-    worst = 0
-    for ts in range(19):
-        w, h = oracle.tx_dims(ts)
-        for t in range(16):
-            if oracle.fwd_txfm2d(np.zeros((h, w), np.int32), ts, t) is None:
-                continue
-            M = np.zeros((h * w, h * w))
-            for i in range(h * w):
-                e = np.zeros(h * w, np.int32)
-                e[i] = 1023
-                M[:, i] = oracle.fwd_txfm2d(e.reshape(h, w), ts, t).ravel()
-            worst = max(worst, np.abs(M).sum(axis=1).max() + 64 * h * w)
+    contract |coefficient| < 2^21 for residuals of bd <= 10 planes: the
+    bound must stay far below 2^21 for every size and type."""
+    worst = _coefficient_bound(oracle, 1023)
     assert worst < (1 << 19), worst
+
+
+def test_fused_fp_quantizer_error_and_level_bounds(oracle):
+    """The fused kernels' 32-bit fp quantizer relies on two facts about every
+    quantizer row (bd 8 and 10, all 256 qindex): quant_fp <= 2^14, so the
+    level (x * quant_fp) >> (16 - ls) is the high word of
+    (x << ls) * (quant_fp << 16) with both factors below 2^32; and the
+    quantization error |coef| - |dq| of every reachable magnitude stays below
+    2^15, so its square is a 24-bit-operand product that fits 32 bits.
+    The error bound is checked on the oracle quantizer (DC and AC parameters,
+    every log_scale) over every magnitude a residual of the bit depth can
+    produce: l1 * (2^bd - 1) plus a rounding allowance (the largest row l1
+    norm is 128.13; the sign-aligned worst residual measures 32648 at 8 bits
+    against the linear 32672, so rounding moves a coefficient by far less
+    than the 64 / 256 allowed), i.e. below the 8-bit INT16 clamp and below
+    2^17 + 2^12 at 10 bits."""
+    _coefficient_bound(oracle, 1)
+    l1 = _WORST_L1['l1'][0]
+    reach = {8: 32767, 10: (1 << 17) + (1 << 12)}
+    assert l1 * 255 + 64 <= reach[8] and l1 * 1023 + 256 <= reach[10], l1
+    for bd in (8, 10):
+        a = np.arange(0, reach[bd] + 1, dtype=np.int32)
+        for qi in range(256):
+            row = oracle.quant_row(qi, bd)
+            assert max(row.quant_fp[0], row.quant_fp[1]) <= (1 << 14), (bd, qi)
+            for which in (0, 1):
+                r = oracle.QuantRow()
+                for f, _ in oracle.QuantRow._fields_:
+                    v = getattr(row, f)[which]
+                    getattr(r, f)[0] = v
+                    getattr(r, f)[1] = v
+                dqv = row.dequant[which]
+                assert dqv < (1 << 13)
+                for ls in (0, 1, 2):
+                    q, dq = oracle.quantize('fp', a, r, ls, bd)
+                    e = a.astype(np.int64) - np.abs(dq).astype(np.int64)
+                    # one step, plus the truncation of quant_fp = 2^16 / dequant (a * dequant / 2^16)
+                    # and the rounding offset where the 8-bit clamp bites
+                    lim = dqv + row.round_fp[which] + (int(a[-1]) * dqv >> 16) + 1
+                    assert e.max() <= lim and e.min() >= -lim, (bd, qi, which, ls, e.min(), e.max())
+                    assert np.abs(e).max() < (1 << 15)

@@ -130,6 +130,15 @@ EXPORTS = {
     'zav1_md_fast_select': [_vp, _vp, ctypes.c_uint64, _i32, _i32, _i32, _vp, _i32, _i32, _i32, _vp, _i32, _i32, _vp,
                             _vp, _vp, _vp],
     'zav1_md_fast_satd': [_vp, _vp] + [_i32] * 6 + [_vp, _i32, _vp, _i32, _vp, _vp],
+    'zav1_ctx_create': [_i32, ctypes.c_size_t, ctypes.POINTER(_vp)],
+    'zav1_ctx_destroy': [_vp],
+    'zav1_ctx_alloc': [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)],
+    'zav1_ctx_usage': [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)],
+    'zav1_ctx_reset': [_vp],
+    'zav1_ctx_stream': [_vp],
+    'zav1_image_upload': [_vp, _vp, ctypes.c_size_t, _i32, _i32, _i32, ctypes.POINTER(_vp)],
+    'zav1_md_wave_submit': [_vp, _vp],
+    'zav1_wave_collect': [_vp],
     'zav1_md_predict_survivors': [_vp] + [_i32] * 6 + [_vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp],
     'zav1_deblock_plane': [_vp] + [_i32] * 9 + [_vp, _vp],
     'zav1_deblock_level_sse': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _i32, _vp, ctypes.c_size_t, _vp, _vp],
@@ -169,6 +178,7 @@ def lib():
             fn.restype = _i32
         L.zav1_last_error.restype = ctypes.c_char_p
         L.zav1_dsp_get.restype = ctypes.POINTER(DspV1)
+        L.zav1_ctx_stream.restype = _vp
         _lib = L
     return _lib
 
@@ -387,6 +397,60 @@ def md_fast_satd(d_src, d_recon, stride, plane_w, plane_h, bd, w, h, d_units, n_
     c = np.ascontiguousarray(cands, dtype=MD_CAND)
     check(lib().zav1_md_fast_satd(d_src.ptr, d_recon.ptr, stride, plane_w, plane_h, bd, w, h, d_units.ptr, n_units,
                                   c.ctypes.data, len(c), d_satd.ptr, stream), 'md_fast_satd')
+
+
+class MdWave(ctypes.Structure):
+    """zav1_md_wave (include/zav1_dsp.h)."""
+    _fields_ = [('d_src', _vp), ('d_recon', _vp), ('stride', _i32), ('plane_w', _i32), ('plane_h', _i32),
+                ('bd', _i32), ('w', _i32), ('h', _i32), ('d_units', _vp), ('n_units', _i32), ('cands', _vp),
+                ('n_cands', _i32), ('d_rate', _vp), ('lam', ctypes.c_uint64), ('k', _i32), ('types', _vp),
+                ('n_types', _i32), ('qindex', _i32), ('txu_flags', ctypes.c_uint), ('d_sel', _vp),
+                ('d_sel_cost', _vp), ('d_dist', _vp), ('d_eob', _vp), ('d_cul', _vp), ('d_qcoeff', _vp),
+                ('d_recon_out', _vp)]
+
+
+class Context:
+    """A device context: one stream and an arena of `budget` bytes."""
+
+    def __init__(self, device=0, budget=1 << 30):
+        h = _vp()
+        check(lib().zav1_ctx_create(device, budget, ctypes.byref(h)), 'ctx_create')
+        self.h = h.value
+
+    def alloc(self, nbytes):
+        p = _vp()
+        check(lib().zav1_ctx_alloc(self.h, nbytes, ctypes.byref(p)), 'ctx_alloc')
+        return p.value
+
+    def usage(self):
+        u, c = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().zav1_ctx_usage(self.h, ctypes.byref(u), ctypes.byref(c)), 'ctx_usage')
+        return u.value, c.value
+
+    def reset(self):
+        check(lib().zav1_ctx_reset(self.h), 'ctx_reset')
+
+    def stream(self):
+        return lib().zav1_ctx_stream(self.h)
+
+    def upload(self, plane, bd):
+        a = np.ascontiguousarray(plane)
+        p = _vp()
+        check(lib().zav1_image_upload(self.h, a.ctypes.data, a.shape[1], a.shape[1], a.shape[0], bd,
+                                      ctypes.byref(p)), 'image_upload')
+        return p.value
+
+    def submit_md_wave(self, wave):
+        """Returns the status code (ZAV1_ENOMEM when the arena is too small)."""
+        return lib().zav1_md_wave_submit(self.h, ctypes.byref(wave))
+
+    def collect(self):
+        check(lib().zav1_wave_collect(self.h), 'wave_collect')
+
+    def close(self):
+        if self.h:
+            check(lib().zav1_ctx_destroy(self.h), 'ctx_destroy')
+            self.h = None
 
 
 def md_predict_survivors(d_recon, stride, plane_w, plane_h, bd, w, h, d_units, n_units, cands, d_sel, k, d_pred,

@@ -1207,4 +1207,154 @@ int zav1_sgr_stats(const void* d_dgd, const void* d_src, int plane_w, int plane_
   return ZAV1_OK;
 }
 
+// ---- contexts and MD waves -------------------------------------------------
+}  // extern "C"
+
+struct zav1_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  char* arena = nullptr;
+  size_t cap = 0, used = 0;
+  int err = ZAV1_OK;  // first error of the submitted waves
+};
+
+namespace {
+int ctx_take(zav1_ctx* c, size_t bytes, void** p) {
+  const size_t b = (bytes + 255) & ~(size_t)255;
+  if (b > c->cap - c->used) return ZAV1_ENOMEM;
+  *p = c->arena + c->used;
+  c->used += b;
+  return ZAV1_OK;
+}
+int tx_size_of(int w, int h) {
+  for (int s = 0; s < 19; s++)
+    if (zav1::tx_w(s) == w && zav1::tx_h(s) == h) return s;
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+
+int zav1_ctx_create(int device, size_t hbm_budget, zav1_ctx** ctx) {
+  if (!ctx || hbm_budget == 0) return ZAV1_EINVAL;
+  *ctx = nullptr;
+  int n = 0;
+  ZCHECK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return ZAV1_EINVAL;
+  ZCHECK(hipSetDevice(device));
+  zav1_ctx* c = new zav1_ctx();
+  c->device = device;
+  c->cap = (hbm_budget + 255) & ~(size_t)255;
+  hipError_t e = hipMalloc((void**)&c->arena, c->cap);
+  if (e != hipSuccess) {
+    delete c;
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? ZAV1_ENOMEM : fail(e, "hipMalloc(arena)");
+  }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipFree(c->arena);
+    delete c;
+    return fail(e, "hipStreamCreateWithFlags");
+  }
+  *ctx = c;
+  return ZAV1_OK;
+}
+
+int zav1_ctx_destroy(zav1_ctx* c) {
+  if (!c) return ZAV1_EINVAL;
+  (void)hipSetDevice(c->device);
+  hipError_t e1 = hipStreamSynchronize(c->stream);
+  hipError_t e2 = hipStreamDestroy(c->stream);
+  hipError_t e3 = hipFree(c->arena);
+  delete c;
+  ZCHECK(e1);
+  ZCHECK(e2);
+  ZCHECK(e3);
+  return ZAV1_OK;
+}
+
+int zav1_ctx_alloc(zav1_ctx* c, size_t bytes, void** p) {
+  if (!c || !p) return ZAV1_EINVAL;
+  return ctx_take(c, bytes, p);
+}
+
+int zav1_ctx_usage(const zav1_ctx* c, size_t* used, size_t* capacity) {
+  if (!c) return ZAV1_EINVAL;
+  if (used) *used = c->used;
+  if (capacity) *capacity = c->cap;
+  return ZAV1_OK;
+}
+
+int zav1_ctx_reset(zav1_ctx* c) {
+  if (!c) return ZAV1_EINVAL;
+  ZCHECK(hipStreamSynchronize(c->stream));
+  c->used = 0;
+  return ZAV1_OK;
+}
+
+zav1_stream_t zav1_ctx_stream(zav1_ctx* c) { return c ? (zav1_stream_t)c->stream : nullptr; }
+
+int zav1_image_upload(zav1_ctx* c, const void* host, size_t host_stride, int w, int h, int bd, void** d_plane) {
+  if (!c || !host || !d_plane || w <= 0 || h <= 0 || host_stride < (size_t)w || (bd != 8 && bd != 10))
+    return ZAV1_EINVAL;
+  const size_t es = bd > 8 ? 2 : 1;
+  void* p = nullptr;
+  const int rc = ctx_take(c, (size_t)w * h * es, &p);
+  if (rc) return rc;
+  ZCHECK(hipSetDevice(c->device));
+  ZCHECK(hipMemcpy2DAsync(p, (size_t)w * es, host, host_stride * es, (size_t)w * es, (size_t)h,
+                          hipMemcpyHostToDevice, c->stream));
+  *d_plane = p;
+  return ZAV1_OK;
+}
+
+int zav1_md_wave_submit(zav1_ctx* c, const zav1_md_wave* v) {
+  if (!c || !v) return ZAV1_EINVAL;
+  const int ts = tx_size_of(v->w, v->h);
+  if (ts < 0 || v->n_units < 0 || v->k < 1 || v->k > v->n_cands || v->n_types < 1 || v->n_types > 16 || !v->types ||
+      !v->d_sel || !v->d_dist || !v->d_eob || !v->d_cul || (v->txu_flags & ~7u) ||
+      ((v->txu_flags & (ZAV1_TXU_RECON | ZAV1_TXU_SPATIAL)) && !v->d_recon_out) || v->qindex < 0 || v->qindex > 255)
+    return ZAV1_EINVAL;
+  if (v->n_units == 0) return ZAV1_OK;
+  const long n_jobs = (long)v->n_units * v->k * v->n_types;
+  if (n_jobs > INT32_MAX) return ZAV1_EINVAL;
+  // intermediates from the arena; on ENOMEM nothing is enqueued and the arena is left as it was
+  const size_t mark = c->used;
+  void *satd = nullptr, *jobs = nullptr, *pred = nullptr;
+  int rc = ctx_take(c, sizeof(uint32_t) * (size_t)v->n_units * v->n_cands, &satd);
+  if (!rc) rc = ctx_take(c, sizeof(zav1_txunit_job) * (size_t)n_jobs, &jobs);
+  if (!rc) rc = ctx_take(c, sizeof(uint16_t) * (size_t)v->n_units * v->k * v->w * v->h, &pred);
+  if (rc) {
+    c->used = mark;
+    return rc;
+  }
+  ZCHECK(hipSetDevice(c->device));
+  zav1_stream_t s = (zav1_stream_t)c->stream;
+  rc = zav1_md_fast_satd(v->d_src, v->d_recon, v->stride, v->plane_w, v->plane_h, v->bd, v->w, v->h, v->d_units,
+                         v->n_units, v->cands, v->n_cands, (uint32_t*)satd, s);
+  if (!rc)
+    rc = zav1_md_fast_select((const uint32_t*)satd, v->d_rate, v->lambda, v->n_units, v->n_cands, v->k,
+                             (const int32_t*)v->d_units, (int)(sizeof(zav1_md_unit) / 4), v->w, v->h, v->types,
+                             v->n_types, 1, v->d_sel, v->d_sel_cost, (zav1_txunit_job*)jobs, s);
+  if (!rc)
+    rc = zav1_md_predict_survivors(v->d_recon, v->stride, v->plane_w, v->plane_h, v->bd, v->w, v->h, v->d_units,
+                                   v->n_units, v->cands, v->n_cands, v->d_sel, v->k, (uint16_t*)pred, s);
+  if (!rc)
+    rc = zav1_txunit_batch(v->d_src, v->stride, v->plane_w, v->plane_h, v->bd, ts, (const uint16_t*)pred,
+                           (size_t)v->n_units * v->k * v->w * v->h, (const zav1_txunit_job*)jobs, (int)n_jobs,
+                           v->qindex, v->txu_flags, v->d_dist, v->d_eob, v->d_cul, v->d_qcoeff, v->d_recon_out, s);
+  if (rc && c->err == ZAV1_OK) c->err = rc;
+  return rc;
+}
+
+int zav1_wave_collect(zav1_ctx* c) {
+  if (!c) return ZAV1_EINVAL;
+  ZCHECK(hipSetDevice(c->device));
+  ZCHECK(hipStreamSynchronize(c->stream));
+  const int rc = c->err;
+  c->err = ZAV1_OK;
+  return rc;
+}
+
 }  // extern "C"

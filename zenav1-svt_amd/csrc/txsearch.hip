@@ -123,8 +123,8 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
         lsum += q;
         eob = q ? max(eob, sp) : eob;
         // x - dq = sign(x) * (a - |dq|): the squared error needs magnitudes only
-        const int32_t e = a - adq;
-        dist += (uint64_t)((int64_t)e * e);
+        const int32_t e = a - adq;  // |e| < 2^15 (see QuantFp)
+        dist += (uint32_t)__mul24(e, e);
       }
     }
     dist = group_sum_u64<H>(dist);

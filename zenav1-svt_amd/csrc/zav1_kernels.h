@@ -41,11 +41,15 @@ __device__ __forceinline__ int32_t dequant_level(int32_t q, const QuantRow& t, i
 // residuals of bd <= 10 planes, whose coefficients are bounded by the l1
 // norm of the transform rows times 1023: <= 2^17 for every size and type
 // (pinned by a CPU test).  Within it, a << 3, a + rounding and q * dequant
-// stay below 2^31, so the level needs one 32x32->64 multiply and the
-// dequantized magnitude one 24-bit multiply, with no branches.
+// stay below 2^31, and the level (x * quant_fp) >> (16 - LS) equals the high
+// word of (x << LS) * (quant_fp << 16): quant_fp = 2^16 / dequant <= 2^14
+// (dequant >= 4), so both factors fit 32 bits.  One 32-bit high multiply
+// for the level and one 24-bit multiply for the dequantized magnitude, no
+// branches.  The error a - |dq| stays below 2^15 in magnitude (one step plus
+// the truncation of quant_fp; pinned), so its square is one 24-bit multiply.
 struct QuantFp {
   int32_t dq[2], rnd[2];
-  uint32_t qf[2];
+  uint32_t qf16[2];  // quant_fp << 16
 };
 __device__ __forceinline__ QuantFp make_quant_fp(const QuantRow& t, int ls) {
   QuantFp f;
@@ -53,7 +57,7 @@ __device__ __forceinline__ QuantFp make_quant_fp(const QuantRow& t, int ls) {
   for (int i = 0; i < 2; i++) {
     f.dq[i] = t.dequant[i];
     f.rnd[i] = (t.round_fp[i] + ((1 << ls) >> 1)) >> ls;
-    f.qf[i] = (uint32_t)t.quant_fp[i];
+    f.qf16[i] = (uint32_t)t.quant_fp[i] << 16;
   }
   return f;
 }
@@ -63,7 +67,7 @@ __device__ __forceinline__ int32_t quant_fp_fast(int32_t a, int iz, const QuantF
   const int32_t dqv = iz ? f.dq[1] : f.dq[0];
   int32_t xx = a + (iz ? f.rnd[1] : f.rnd[0]);
   if (bd == 8) xx = xx > 32767 ? 32767 : xx;
-  const uint32_t q = (uint32_t)(((uint64_t)(uint32_t)xx * (iz ? f.qf[1] : f.qf[0])) >> (16 - LS));
+  const uint32_t q = __umulhi((uint32_t)xx << LS, iz ? f.qf16[1] : f.qf16[0]);
   const int32_t qq = ((a << (1 + LS)) >= dqv) ? (int32_t)q : 0;
   *adq = __mul24(qq, dqv) >> LS;
   return qq;
