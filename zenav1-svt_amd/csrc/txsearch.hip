@@ -101,13 +101,19 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
     if (live) {
       const int tx_type = plan.type[ti];
       const TypeCfg cfg = type_cfg(tx_type);
+      // scan positions of the row's kept coefficients, loaded ahead of the
+      // transform so their latency overlaps it (rows >= 32 keep none)
+      const int cls = (tx_type >= 10) ? (((tx_type - 10) & 1) ? 2 : 1) : 0;
+      const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + (r < 32 ? r : 0) * G::PW;
+      constexpr int KW = W < 32 ? W : 32;
+      int32_t sp[KW];
+#pragma unroll
+      for (int c = 0; c < KW; c++) sp[c] = (int32_t)iscan[c] + 1;
       const int32_t* src = L + plan.var_of[ti] * G::VS + r * G::LS;
       int32_t v[W];
 #pragma unroll
       for (int c = 0; c < W; c++) v[c] = cfg.lr ? src[W - 1 - c] : src[c];
       fwd1d<W, CBR>(v, cfg.row);
-      const int cls = (tx_type >= 10) ? (((tx_type - 10) & 1) ? 2 : 1) : 0;
-      const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + r * G::PW;
 #pragma unroll
       for (int c = 0; c < W; c++) {
         int32_t x = rshift_round(v[c], SH.s2);
@@ -119,9 +125,8 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
         const int32_t a = x < 0 ? -x : x;
         int32_t adq;
         const int32_t q = quant_fp_fast<LSC>(a, (r | c) != 0, qf, bd, &adq);
-        const int32_t sp = (int32_t)iscan[c] + 1;
         lsum += q;
-        eob = q ? max(eob, sp) : eob;
+        eob = max(eob, q ? sp[c < KW ? c : 0] : 0);
         // x - dq = sign(x) * (a - |dq|): the squared error needs magnitudes only
         const int32_t e = a - adq;  // |e| < 2^15 (see QuantFp)
         dist += (uint32_t)__mul24(e, e);

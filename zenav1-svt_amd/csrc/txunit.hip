@@ -85,7 +85,10 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
     fwd1d<W, CBR>(v, cfg.row);
     const QuantRow& qr = *row_p;
     const int cls = (jb.tx_type >= 10) ? (((jb.tx_type - 10) & 1) ? 2 : 1) : 0;
-    const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + t * G::PW;
+    const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + (t < 32 ? t : 0) * G::PW;
+    int32_t sp[G::PW];  // scan positions, loaded unconditionally (no load behind the level test)
+#pragma unroll
+    for (int c = 0; c < G::PW; c++) sp[c] = (int32_t)iscan[c] + 1;
     const bool use_b = flags & kTxuQuantB;
 #pragma unroll
     for (int c = 0; c < W; c++) {
@@ -105,7 +108,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
         dq = (dequant_level(q, qr, iz, LSC) ^ sgn) - sgn;
         qs = (q ^ sgn) - sgn;
         lsum += q;
-        eob = max(eob, (int32_t)iscan[c] + 1);
+        eob = max(eob, sp[c]);
       }
       if (t == 0 && c == 0) dcq = qs;
       const int64_t e = (int64_t)x - dq;
