@@ -236,7 +236,8 @@ int zav1_scan_order(int tx_size, int scan_class, uint16_t *scan);
  * 64-point fold, fp quantization at qindex, frequency distortion.  For
  * each block j and host-side candidate type types[k] (n_types <= 16) writes
  * d_dist[j*n_types+k] (u64), d_eob[...] (scan order), d_lsum[...] (sum of
- * |level|). */
+ * |level|). Samples must lie in [0, 2^bd - 1] (the fused transforms size their
+ * arithmetic for residuals of at most 10-bit samples). */
 int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size, const int32_t *d_xy,
                         int n_blocks, const uint8_t *types, int n_types, int qindex, uint64_t *d_dist,
                         int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
@@ -257,7 +258,7 @@ int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size
  * Per job: d_dist (u64), d_eob (i32; -1 = no kernel for the type at this
  * size, the reference's `false`; -22 = unit or prediction out of bounds),
  * d_cul (u8); optional d_qcoeff (packed <= 32x32 levels, job-major) and
- * d_recon (w*h u16, job-major).  d_src is u8 at bd 8, u16 at bd 10. */
+ * d_recon (w*h u16, job-major).  d_src is u8 at bd 8, u16 at bd 10. Source and prediction samples must lie in [0, 2^bd - 1]. */
 typedef struct zav1_txunit_job {
   int32_t x, y;            /* unit origin in d_src (samples) */
   uint32_t pred_off;       /* d_pred offset (samples) of the w*h prediction */

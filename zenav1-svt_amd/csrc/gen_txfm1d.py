@@ -40,17 +40,39 @@ def bitrev(x, bits):
 
 
 class Emitter:
-    """SSA emitter: every op creates a fresh temporary."""
+    """SSA emitter: every op creates a fresh temporary.
 
-    def __init__(self, cb):
+    Alongside the code it runs an interval analysis over magnitudes: every
+    temporary gets a bound g * B + c for inputs bounded by B in magnitude,
+    and every half butterfly records the linear constraints under which its
+    24-bit-operand, 32-bit-sum form (zbtf24) is exact: both operands below
+    2^23 and |w0 * a| + |w1 * b| + 2^(cb - 1) below 2^31.  `bmax()` is the
+    largest input bound meeting all of them (cancellations are ignored, so
+    the bound is conservative)."""
+
+    def __init__(self, cb, narrow=False):
         self.cb = cb
+        self.narrow = narrow
         self.lines = []
         self.n = 0
+        self.bnd = {}
+        self.cons = []  # (alpha, beta, limit): alpha * B + beta <= limit
 
-    def tmp(self, expr):
+    def _b(self, name):
+        return self.bnd[name[1:]] if name.startswith('-') else self.bnd[name]
+
+    def tmp(self, expr, bound=None):
         name = 't%d' % self.n
         self.n += 1
         self.lines.append('  const int32_t %s = %s;' % (name, expr))
+        if bound is None:
+            if expr.startswith('x['):
+                bound = (1.0, 0.0)
+            elif expr.startswith('-'):
+                bound = self._b(expr[1:])
+            else:
+                raise ValueError('no bound for ' + expr)
+        self.bnd[name] = bound
         return name
 
     def c(self, k):
@@ -58,15 +80,28 @@ class Emitter:
         return -v if k < 0 else v
 
     def btf(self, w0, a, w1, b):
-        return self.tmp('zbtf(%d, %s, %d, %s, %d)' % (w0, a, w1, b, self.cb))
+        (ga, ca), (gb, cb_) = self._b(a), self._b(b)
+        r = 1 << (self.cb - 1)
+        self.cons.append((ga, ca, (1 << 23) - 1))
+        self.cons.append((gb, cb_, (1 << 23) - 1))
+        self.cons.append((abs(w0) * ga + abs(w1) * gb, abs(w0) * ca + abs(w1) * cb_ + r, (1 << 31) - 1))
+        g = (abs(w0) * ga + abs(w1) * gb) / float(1 << self.cb)
+        c = (abs(w0) * ca + abs(w1) * cb_ + r) / float(1 << self.cb) + 1.0
+        f = 'zbtf24' if self.narrow else 'zbtf'
+        return self.tmp('%s(%d, %s, %d, %s, %d)' % (f, w0, a, w1, b, self.cb), (g, c))
 
     def add(self, a, b, clamp):
         e = '%s + %s' % (a, b)
-        return self.tmp('zclamp(%s, rng)' % e if clamp else e)
+        (g1, c1), (g2, c2) = self._b(a), self._b(b)
+        return self.tmp('zclamp(%s, rng)' % e if clamp else e, (g1 + g2, c1 + c2))
 
     def sub(self, a, b, clamp):
         e = '%s - %s' % (a, b)
-        return self.tmp('zclamp(%s, rng)' % e if clamp else e)
+        (g1, c1), (g2, c2) = self._b(a), self._b(b)
+        return self.tmp('zclamp(%s, rng)' % e if clamp else e, (g1 + g2, c1 + c2))
+
+    def bmax(self):
+        return min(int((lim - beta) // alpha) for alpha, beta, lim in self.cons if alpha > 0)
 
 
 # ----------------------------------------------------------------------------
@@ -298,8 +333,10 @@ def gen_iadst16(em, x):
     return [v[s] if i % 2 == 0 else em.tmp('-' + v[s]) for i, s in enumerate(src)]
 
 
-def emit_fn(name, n, body_fn, cb, needs_rng=False):
-    em = Emitter(cb)
+def emit_fn(name, n, body_fn, cb, needs_rng=False, narrow=False):
+    """Returns (code, bmax, (g, c)): the input bound under which the narrow
+    form is exact and the output bound g * B + c."""
+    em = Emitter(cb, narrow)
     ins = ['x[%d]' % i for i in range(n)]
     em.lines.append('  // load')
     loaded = [em.tmp(v) for v in ins]
@@ -307,11 +344,9 @@ def emit_fn(name, n, body_fn, cb, needs_rng=False):
     sig = 'static __device__ __forceinline__ void %s(int32_t* __restrict__ x%s)' % (
         name, ', int rng' if needs_rng else '')
     store = ['  x[%d] = %s;' % (i, o) for i, o in enumerate(outs)]
-    if needs_rng:
-        pre = []
-    else:
-        pre = []
-    return '\n'.join([sig + ' {'] + pre + em.lines + store + ['}', ''])
+    g = max(em._b(o)[0] for o in outs)
+    c = max(em._b(o)[1] for o in outs)
+    return '\n'.join([sig + ' {'] + em.lines + store + ['}', '']), em.bmax(), (g, c)
 
 
 def main():
@@ -326,6 +361,12 @@ def main():
         '  const int64_t r = (int64_t)w0 * a + (int64_t)w1 * b;',
         '  return (int32_t)((r + ((int64_t)1 << (cb - 1))) >> cb);',
         '}',
+        '// The same butterfly with 24-bit-operand multiplies and a 32-bit sum:',
+        '// exact when |a|, |b| < 2^23 and |w0 a| + |w1 b| + 2^(cb-1) < 2^31, which',
+        '// the generator proves for the input bound of each *_n variant.',
+        'static __device__ __forceinline__ int32_t zbtf24(int32_t w0, int32_t a, int32_t w1, int32_t b, int cb) {',
+        '  return (__mul24(w0, a) + __mul24(w1, b) + (1 << (cb - 1))) >> cb;',
+        '}',
         '// signed clamp to `bit` bits (no-op when bit <= 0).',
         'static __device__ __forceinline__ int32_t zclamp(int32_t v, int bit) {',
         '  if (bit <= 0) return v;',
@@ -334,15 +375,35 @@ def main():
         '}',
         '',
     ]
+    bounds = []  # (kind, n, cb, bmax, g, c)
     for cb in (10, 11, 12, 13):
-        for n in (4, 8, 16, 32, 64):
-            parts.append(emit_fn('fdct%d_cb%d' % (n, cb), n, gen_fdct, cb))
-        parts.append(emit_fn('fadst8_cb%d' % cb, 8, gen_fadst8, cb))
-        parts.append(emit_fn('fadst16_cb%d' % cb, 16, gen_fadst16, cb))
+        for kind, fn, sizes in (('fdct', gen_fdct, (4, 8, 16, 32, 64)), ('fadst', gen_fadst8, (8,)),
+                                ('fadst', gen_fadst16, (16,))):
+            for n in sizes:
+                code, bmax, (g, c) = emit_fn('%s%d_cb%d' % (kind, n, cb), n, fn, cb)
+                parts.append(code)
+                parts.append(emit_fn('%s%d_cb%d_n' % (kind, n, cb), n, fn, cb, narrow=True)[0])
+                bounds.append((kind, n, cb, bmax, g, c))
     for n in (4, 8, 16, 32, 64):
-        parts.append(emit_fn('idct%d' % n, n, gen_idct, 12, needs_rng=True))
-    parts.append(emit_fn('iadst8', 8, gen_iadst8, 12, needs_rng=True))
-    parts.append(emit_fn('iadst16', 16, gen_iadst16, 12, needs_rng=True))
+        parts.append(emit_fn('idct%d' % n, n, gen_idct, 12, needs_rng=True)[0])
+    parts.append(emit_fn('iadst8', 8, gen_iadst8, 12, needs_rng=True)[0])
+    parts.append(emit_fn('iadst16', 16, gen_iadst16, 12, needs_rng=True)[0])
+    # input bound of each narrow forward variant and the output bound g * B + c
+    parts.append('// Interval analysis of the forward transforms: the *_n form is exact for')
+    parts.append('// inputs |x| <= bmax; outputs satisfy |y| <= g * B + c for inputs |x| <= B.')
+    parts.append('struct ZTxBound {')
+    parts.append('  long long bmax;')
+    parts.append('  double g, c;')
+    parts.append('};')
+    parts.append('// kind 0 DCT, 1 ADST (8 and 16 points)')
+    parts.append('constexpr ZTxBound ztx_bound(int kind, int n, int cb) {')
+    for kind, n, cb, bmax, g, c in bounds:
+        k = 0 if kind == 'fdct' else 1
+        parts.append('  if (kind == %d && n == %d && cb == %d) return ZTxBound{%dLL, %.17g, %.17g};' % (
+            k, n, cb, bmax, g, c))
+    parts.append('  return ZTxBound{-1LL, 0.0, 0.0};')
+    parts.append('}')
+    parts.append('')
     # sinpi rows for the 4-point ADST, indexed by cos_bit - 10
     rows = ',\n  '.join('{%s}' % ', '.join(str(v) for v in r) for r in SINPI)
     parts.append('static __device__ __constant__ const int32_t kSinpi[7][5] = {\n  %s};\n' % rows)

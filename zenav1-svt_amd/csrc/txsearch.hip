@@ -49,6 +49,10 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   static_assert(P >= W && P >= H && 64 % P == 0, "lanes per block");
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
+  // input bounds of the 1D passes for residuals of <= 10-bit planes: the
+  // passes whose bound allows it run with 24-bit-operand butterflies
+  constexpr long long kBCol = fwd_col_bound(SH.s0);
+  constexpr long long kBRow = fwd_row_bound(H, CBC, SH.s1, kBCol);
   constexpr int LSC = tx_log_scale(W, H);
   extern __shared__ int32_t lds[];
 
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
     int32_t col[H];
 #pragma unroll
     for (int r = 0; r < H; r++) col[r] = ((int32_t)pb[(ud ? (H - 1 - r) : r) * stride + c] - dc) << SH.s0;
-    fwd1d<H, CBC>(col, plan.var_kind[v]);
+    fwd1d<H, CBC, kBCol>(col, plan.var_kind[v]);
 #pragma unroll
     for (int r = 0; r < H; r++) L[v * G::VS + r * G::LS + c] = rshift_round(col[r], SH.s1);
   }
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
       int32_t v[W];
 #pragma unroll
       for (int c = 0; c < W; c++) v[c] = cfg.lr ? src[W - 1 - c] : src[c];
-      fwd1d<W, CBR>(v, cfg.row);
+      fwd1d<W, CBR, kBRow>(v, cfg.row);
 #pragma unroll
       for (int c = 0; c < W; c++) {
         int32_t x = rshift_round(v[c], SH.s2);

@@ -42,6 +42,10 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   using G = TuGeom<W, H>;
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift FS = fwd_shift(W, H);
+  // input bounds of the 1D passes for residuals of <= 10-bit planes: the
+  // passes whose bound allows it run with 24-bit-operand butterflies
+  constexpr long long kBCol = fwd_col_bound(FS.s0);
+  constexpr long long kBRow = fwd_row_bound(H, CBC, FS.s1, kBCol);
   constexpr InvShift IS = inv_shift(W, H);
   constexpr int LSC = tx_log_scale(W, H);
   __shared__ int32_t lds[G::JPW * G::BLK];
@@ -66,7 +70,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       const int rr = cfg.ud ? (H - 1 - r) : r;
       v[r] = ((int32_t)sp[rr * stride + t] - (int32_t)pr[rr * W + t]) << FS.s0;
     }
-    fwd1d<H, CBC>(v, cfg.col);
+    fwd1d<H, CBC, kBCol>(v, cfg.col);
     const int cc = cfg.lr ? (W - 1 - t) : t;
 #pragma unroll
     for (int r = 0; r < H; r++) blk[r * G::LS + cc] = rshift_round(v[r], FS.s1);
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
     int32_t v[W];
 #pragma unroll
     for (int c = 0; c < W; c++) v[c] = blk[t * G::LS + c];
-    fwd1d<W, CBR>(v, cfg.row);
+    fwd1d<W, CBR, kBRow>(v, cfg.row);
     const QuantRow& qr = *row_p;
     const int cls = (jb.tx_type >= 10) ? (((jb.tx_type - 10) & 1) ? 2 : 1) : 0;
     const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + (t < 32 ? t : 0) * G::PW;

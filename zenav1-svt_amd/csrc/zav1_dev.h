@@ -123,6 +123,110 @@ __device__ __forceinline__ void fdct_n(int32_t* x) {
 }
 
 template <int N, int CB>
+__device__ __forceinline__ void fdct_n_narrow(int32_t* x) {
+#define ZAV1_FDCT_PICK(NN)                                   \
+  if constexpr (N == NN) {                                   \
+    if constexpr (CB == 10) fdct##NN##_cb10_n(x);            \
+    else if constexpr (CB == 11) fdct##NN##_cb11_n(x);       \
+    else if constexpr (CB == 12) fdct##NN##_cb12_n(x);       \
+    else fdct##NN##_cb13_n(x);                               \
+  }
+  ZAV1_FDCT_PICK(4)
+  ZAV1_FDCT_PICK(8)
+  ZAV1_FDCT_PICK(16)
+  ZAV1_FDCT_PICK(32)
+  ZAV1_FDCT_PICK(64)
+#undef ZAV1_FDCT_PICK
+}
+
+// Input-range bookkeeping for the narrow (24-bit operand, 32-bit sum)
+// forward transforms.  The 4-point ADST's largest partial sum is
+// y2 - y0 + y3 = (s4 - s1) x0 - (s1 + s2) x1 + s3 x2 + (s2 - s4) x3 with
+// the products formed separately: at most (2 (s1 + s2 + s4) + s3) * B.
+constexpr int kSinpiHost[4][5] = {{0, 330, 621, 836, 951}, {0, 660, 1241, 1672, 1901}, {0, 1321, 2482, 3344, 3803},
+                                  {0, 2642, 4964, 6689, 7606}};
+__host__ __device__ constexpr long long adst4_sum_gain(int cb) {
+  return 2LL * (kSinpiHost[cb - 10][1] + kSinpiHost[cb - 10][2] + kSinpiHost[cb - 10][4]) + kSinpiHost[cb - 10][3];
+}
+__host__ __device__ constexpr long long identity_mult(int n) {
+  return n == 4 ? 5793 : n == 8 ? 2 * 4096 : n == 16 ? 2 * 5793 : n == 32 ? 4 * 4096 : 4 * 5793;
+}
+// kind: 0 DCT, 1 ADST, 3 identity.  True when the narrow form of the 1D
+// forward transform is exact for every input with |x| <= B.
+__host__ __device__ constexpr bool fwd_narrow_ok(int kind, int n, int cb, long long B) {
+  if (B < 0) return false;
+  if (kind == 3) return B < (1LL << 23) && B * identity_mult(n) + 2048 < (1LL << 31);
+  if (kind == 1 && n == 4) return 3 * B < (1LL << 23) && adst4_sum_gain(cb) * B + (1LL << (cb - 1)) < (1LL << 31);
+  return B <= ztx_bound(kind == 0 ? 0 : 1, n, cb).bmax;
+}
+// Bound on the outputs of the 1D forward transform for inputs |x| <= B.
+__host__ __device__ constexpr long long fwd_out_bound(int kind, int n, int cb, long long B) {
+  if (kind == 3) return (B * identity_mult(n) + 2048) / 4096 + 1;
+  if (kind == 1 && n == 4) return (adst4_sum_gain(cb) * B + (1LL << (cb - 1))) / (1LL << cb) + 1;
+  const ZTxBound b = ztx_bound(kind == 0 ? 0 : 1, n, cb);
+  return (long long)(b.g * (double)B + b.c) + 1;
+}
+// Input bound of the row pass of a w x h forward 2D transform whose column
+// inputs are bounded by bcol (max over the column kinds the size allows).
+__host__ __device__ constexpr long long fwd_row_bound(int h, int cbc, int s1, long long bcol) {
+  long long m = fwd_out_bound(0, h, cbc, bcol);
+  const long long mi = fwd_out_bound(3, h, cbc, bcol);
+  if (mi > m) m = mi;
+  if (h <= 16) {
+    const long long ma = fwd_out_bound(1, h, cbc, bcol);
+    if (ma > m) m = ma;
+  }
+  return s1 > 0 ? (m + (1LL << (s1 - 1))) >> s1 : m;
+}
+// Column-input bound of the fused kernels: residuals of planes of at most
+// 10 bits, |r| <= 1023, shifted left by s0.
+__host__ __device__ constexpr long long fwd_col_bound(int s0) { return 1023LL << s0; }
+
+template <int N, int CB>
+__device__ __forceinline__ void fadst4_narrow(int32_t* x) {
+  const int32_t* s = kSinpi[CB - 10];
+  const int32_t x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
+  if ((x0 | x1 | x2 | x3) == 0) return;
+  const int32_t s0 = __mul24(s[1], x0), s1 = __mul24(s[4], x0), s2 = __mul24(s[2], x1);
+  const int32_t s3 = __mul24(s[1], x1), s4 = __mul24(s[3], x2), s5 = __mul24(s[4], x3);
+  const int32_t s6 = __mul24(s[2], x3);
+  const int32_t s7 = x0 + x1 - x3;
+  const int32_t y0 = s0 + s2 + s5, y1 = __mul24(s[3], s7), y2 = s1 - s3 + s6, y3 = s4;
+  constexpr int32_t R = 1 << (CB - 1);
+  x[0] = (y0 + y3 + R) >> CB;
+  x[1] = (y1 + R) >> CB;
+  x[2] = (y2 - y3 + R) >> CB;
+  x[3] = (y2 - y0 + y3 + R) >> CB;
+}
+
+template <int N, int CB>
+__device__ __forceinline__ void fadst_n_narrow(int32_t* x) {
+  if constexpr (N == 4) {
+    fadst4_narrow<N, CB>(x);
+  } else if constexpr (N == 8) {
+    if constexpr (CB == 10) fadst8_cb10_n(x);
+    else if constexpr (CB == 11) fadst8_cb11_n(x);
+    else if constexpr (CB == 12) fadst8_cb12_n(x);
+    else fadst8_cb13_n(x);
+  } else if constexpr (N == 16) {
+    if constexpr (CB == 10) fadst16_cb10_n(x);
+    else if constexpr (CB == 11) fadst16_cb11_n(x);
+    else if constexpr (CB == 12) fadst16_cb12_n(x);
+    else fadst16_cb13_n(x);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void fidentity_narrow(int32_t* x) {
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    if constexpr (N == 8) x[i] = x[i] * 2;
+    else if constexpr (N == 32) x[i] = x[i] * 4;
+    else x[i] = (__mul24(x[i], (int32_t)identity_mult(N)) + 2048) >> 12;
+  }
+}
+
+template <int N, int CB>
 __device__ __forceinline__ void fadst_n(int32_t* x) {
   if constexpr (N == 4) {
     const int32_t* s = kSinpi[CB - 10];
@@ -163,14 +267,21 @@ __device__ __forceinline__ void fidentity_n(int32_t* x) {
   }
 }
 
-template <int N, int CB>
+// B >= 0: every input satisfies |x| <= B, and each kind whose narrow form is
+// exact at that bound uses it; B < 0 (no bound known): 64-bit butterflies.
+template <int N, int CB, long long B = -1>
 __device__ __forceinline__ void fwd1d(int32_t* x, int kind) {
   if (kind == 0) {
-    fdct_n<N, CB>(x);
+    if constexpr (fwd_narrow_ok(0, N, CB, B)) fdct_n_narrow<N, CB>(x);
+    else fdct_n<N, CB>(x);
   } else if (kind == 3) {
-    fidentity_n<N>(x);
+    if constexpr (fwd_narrow_ok(3, N, CB, B)) fidentity_narrow<N>(x);
+    else fidentity_n<N>(x);
   } else {
-    if constexpr (N <= 16) fadst_n<N, CB>(x);
+    if constexpr (N <= 16) {
+      if constexpr (fwd_narrow_ok(1, N, CB, B)) fadst_n_narrow<N, CB>(x);
+      else fadst_n<N, CB>(x);
+    }
   }
 }
 
