@@ -46,6 +46,17 @@ def _run(lines, x, check=False):
             elif re.match(r'(t\d+) ([+-]) (t\d+)$', e):
                 a, op, b = re.match(r'(t\d+) ([+-]) (t\d+)$', e).groups()
                 v = env[a] + env[b] if op == '+' else env[a] - env[b]
+            elif e.startswith('zbtf24m('):
+                mm = re.match(r'zbtf24m\((-?\d+), (t\d+) ([+-]) (t\d+), (\d+)\)$', e)
+                assert mm, e
+                w, a, op, b, cb = mm.groups()
+                w, cb = int(w), int(cb)
+                ab = env[a] + env[b] if op == '+' else env[a] - env[b]
+                s = w * ab + (1 << (cb - 1))
+                if check:
+                    assert abs(ab) < (1 << 23), (ln, ab)
+                    assert -(1 << 31) <= s < (1 << 31), (ln, s)
+                v = s >> cb
             else:
                 mm = re.match(r'(zbtf24|zbtf)\((-?\d+), (t\d+), (-?\d+), (t\d+), (\d+)\)$', e)
                 assert mm, e

@@ -87,6 +87,12 @@ class Emitter:
         self.cons.append((abs(w0) * ga + abs(w1) * gb, abs(w0) * ca + abs(w1) * cb_ + r, (1 << 31) - 1))
         g = (abs(w0) * ga + abs(w1) * gb) / float(1 << self.cb)
         c = (abs(w0) * ca + abs(w1) * cb_ + r) / float(1 << self.cb) + 1.0
+        if self.narrow and abs(w0) == abs(w1):
+            # equal weights: w0 * (a +- b) with one 24-bit multiply; the
+            # operand a +- b is bounded by the sum of the operand bounds
+            self.cons.append((ga + gb, ca + cb_, (1 << 23) - 1))
+            op = '+' if w0 == w1 else '-'
+            return self.tmp('zbtf24m(%d, %s %s %s, %d)' % (w0, a, op, b, self.cb), (g, c))
         f = 'zbtf24' if self.narrow else 'zbtf'
         return self.tmp('%s(%d, %s, %d, %s, %d)' % (f, w0, a, w1, b, self.cb), (g, c))
 
@@ -366,6 +372,10 @@ def main():
         '// the generator proves for the input bound of each *_n variant.',
         'static __device__ __forceinline__ int32_t zbtf24(int32_t w0, int32_t a, int32_t w1, int32_t b, int cb) {',
         '  return (__mul24(w0, a) + __mul24(w1, b) + (1 << (cb - 1))) >> cb;',
+        '}',
+        '// Equal-magnitude weights: w * (a +- b), the operand a +- b bounded like a and b.',
+        'static __device__ __forceinline__ int32_t zbtf24m(int32_t w, int32_t ab, int cb) {',
+        '  return (__mul24(w, ab) + (1 << (cb - 1))) >> cb;',
         '}',
         '// signed clamp to `bit` bits (no-op when bit <= 0).',
         'static __device__ __forceinline__ int32_t zclamp(int32_t v, int bit) {',

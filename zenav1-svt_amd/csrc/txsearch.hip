@@ -37,7 +37,7 @@ struct BlkGeom {
   static constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
 };
 
-template <int W, int H, int P>
+template <int W, int H, int P, int BD>
 __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ plane, int stride, int bd,
                                                   const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
                                                   const QuantRow* __restrict__ row_p,
@@ -78,17 +78,22 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   // ---- column variants -> LDS ----
   // Loads are unconditional (an inactive group reads block 0 and its
   // results are never written out), so they issue back to back.
+  // Sample offsets are 32-bit from the plane base (planes hold < 2^31
+  // samples), so each load needs no 64-bit address arithmetic.  Column
+  // outputs are bounded (see fwd_row_bound), so the rounding shift is 32-bit.
   int32_t* L = lds + g * nvar * G::VS;
-  const uint16_t* pb = plane + (long)y0 * stride + x0;
+  const uint32_t o0 = (uint32_t)y0 * (uint32_t)stride + (uint32_t)x0;
   for (int j = t; j < nvar * W; j += G::T) {
     const int v = j / W, c = j - v * W;
     const int ud = plan.var_ud[v];
+    const uint32_t oc = o0 + (uint32_t)c + (ud ? (uint32_t)(H - 1) * (uint32_t)stride : 0u);
+    const int32_t step = ud ? -stride : stride;
     int32_t col[H];
 #pragma unroll
-    for (int r = 0; r < H; r++) col[r] = ((int32_t)pb[(ud ? (H - 1 - r) : r) * stride + c] - dc) << SH.s0;
+    for (int r = 0; r < H; r++) col[r] = ((int32_t)plane[oc + (uint32_t)(r * step)] - dc) << SH.s0;
     fwd1d<H, CBC, kBCol>(col, plan.var_kind[v]);
 #pragma unroll
-    for (int r = 0; r < H; r++) L[v * G::VS + r * G::LS + c] = rshift_round(col[r], SH.s1);
+    for (int r = 0; r < H; r++) L[v * G::VS + r * G::LS + c] = rshift_round32(col[r], SH.s1);
   }
   __syncthreads();
 
@@ -118,22 +123,31 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
 #pragma unroll
       for (int c = 0; c < W; c++) v[c] = cfg.lr ? src[W - 1 - c] : src[c];
       fwd1d<W, CBR, kBRow>(v, cfg.row);
+      // squared errors: each < 2^30 (|e| < 2^15, see QuantFp), so four of
+      // them sum in 32 bits before one 64-bit add
+      uint32_t sq4 = 0;
 #pragma unroll
       for (int c = 0; c < W; c++) {
-        int32_t x = rshift_round(v[c], SH.s2);
-        if constexpr (G::RECT2) x = rshift64r((int64_t)x * 5793, 12);
+        int32_t x = rshift_round32(v[c], SH.s2);
+        // 2:1 rectangles: x NewSqrt2 / 2^12; |x| stays below 2^18 (the pinned
+        // coefficient bound before this scaling), so the product fits 32 bits
+        if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
         if (r >= 32 || c >= 32) {
           dist += (uint64_t)((int64_t)x * x);  // 64-point fold: discarded energy
           continue;
         }
         const int32_t a = x < 0 ? -x : x;
         int32_t adq;
-        const int32_t q = quant_fp_fast<LSC>(a, (r | c) != 0, qf, bd, &adq);
+        const int32_t q = quant_fp_fast<LSC>(a, (r | c) != 0, qf, BD, &adq);
         lsum += q;
         eob = max(eob, q ? sp[c < KW ? c : 0] : 0);
         // x - dq = sign(x) * (a - |dq|): the squared error needs magnitudes only
-        const int32_t e = a - adq;  // |e| < 2^15 (see QuantFp)
-        dist += (uint32_t)__mul24(e, e);
+        const int32_t e = a - adq;
+        sq4 += (uint32_t)__mul24(e, e);
+        if ((c & 3) == 3 || c == W - 1 || c == 31) {
+          dist += sq4;
+          sq4 = 0;
+        }
       }
     }
     dist = group_sum_u64<H>(dist);
@@ -171,8 +185,12 @@ static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int
   int groups = 256 / G::T;
   while (groups > 1 && groups * per_group > 65536) groups >>= 1;
   const int grid = (n_blocks + groups - 1) / groups;
-  hipLaunchKernelGGL((k_txsearch<W, H, P>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane, stride,
-                     bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
+  if (bd == 8)
+    hipLaunchKernelGGL((k_txsearch<W, H, P, 8>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
+                       stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
+  else
+    hipLaunchKernelGGL((k_txsearch<W, H, P, 10>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
+                       stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
   (void)types;
   return hipGetLastError();
 }

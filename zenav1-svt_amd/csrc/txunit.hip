@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
     fwd1d<H, CBC, kBCol>(v, cfg.col);
     const int cc = cfg.lr ? (W - 1 - t) : t;
 #pragma unroll
-    for (int r = 0; r < H; r++) blk[r * G::LS + cc] = rshift_round(v[r], FS.s1);
+    for (int r = 0; r < H; r++) blk[r * G::LS + cc] = rshift_round32(v[r], FS.s1);
   }
   __syncthreads();
 
@@ -96,8 +96,10 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
     const bool use_b = flags & kTxuQuantB;
 #pragma unroll
     for (int c = 0; c < W; c++) {
-      int32_t x = rshift_round(v[c], FS.s2);
-      if constexpr (G::RECT2) x = rshift64r((int64_t)x * 5793, 12);
+      int32_t x = rshift_round32(v[c], FS.s2);
+      // 2:1 rectangles: x NewSqrt2 / 2^12; |x| stays below 2^18 (the pinned
+      // coefficient bound before this scaling), so the product fits 32 bits
+      if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
       if (t >= 32 || c >= 32) {
         fdist += (uint64_t)((int64_t)x * x);  // energy of the folded-away coefficients
         blk[t * G::LS + c] = 0;

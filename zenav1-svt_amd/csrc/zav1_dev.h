@@ -98,6 +98,11 @@ __host__ __device__ constexpr bool kind_ok(int kind, int n) { return !((kind == 
 __device__ __forceinline__ int32_t rshift_round(int32_t v, int bit) {
   return bit == 0 ? v : (int32_t)(((int64_t)v + (1 << (bit - 1))) >> bit);
 }
+// rshift_round for values known to satisfy |v| < 2^31 - 2^(bit - 1) (the
+// fused kernels' bounded intermediates): the rounding add stays in 32 bits.
+__device__ __forceinline__ int32_t rshift_round32(int32_t v, int bit) {
+  return bit == 0 ? v : (v + (1 << (bit - 1))) >> bit;
+}
 __device__ __forceinline__ int32_t rshift64r(int64_t v, int bit) {
   return (int32_t)((v + ((int64_t)1 << (bit - 1))) >> bit);
 }
