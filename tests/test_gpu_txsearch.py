@@ -60,3 +60,39 @@ def test_txsearch_ragged_and_edges(gpu, oracle):
         rd, re, rl = oracle.txsearch_plane(plane, 10, tx_size, sub, types, 255)
         assert np.array_equal(dd.to_host(), rd) and np.array_equal(de.to_host(), re)
         assert np.array_equal(dl.to_host(), rl)
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_txsearch_type_orders_and_extremes(gpu, oracle, bd):
+    """Caller type lists in any order (one type, all 16 reversed, a subset
+    without some column variants) on extreme content: checkerboards and
+    full-range steps at qindex 0 (largest levels and errors) and 255."""
+    zav1 = gpu
+    rng = np.random.default_rng(bd)
+    top = (1 << bd) - 1
+    plane = rng.integers(0, top + 1, (64, 128)).astype(np.uint16)
+    yy, xx = np.mgrid[0:64, 0:128]
+    plane[:32, :64] = np.where((yy[:32, :64] + xx[:32, :64]) % 2 == 0, top, 0)
+    plane[32:, :64] = np.where(xx[32:, :64] % 8 < 4, top, 0)
+    d_plane = zav1.DeviceArray.from_host(plane)
+    lists = [[9], list(range(15, -1, -1)), [4, 6, 8, 14, 1], [10, 11, 12, 13, 14, 15]]
+    for tx_size in range(19):
+        w, h = zav1.TX_DIMS[tx_size]
+        ys, xs = np.mgrid[0:64 - h + 1:h, 0:128 - w + 1:w]
+        xy = np.stack([xs.ravel(), ys.ravel()], axis=1).astype(np.int32)
+        d_xy = zav1.DeviceArray.from_host(xy)
+        for tl in lists:
+            if max(w, h) == 64:
+                tl = [0]
+            elif max(w, h) == 32:
+                tl = [t for t in tl if t in (0, 9, 10, 11)] or [10, 0]
+            types = np.array(tl, np.uint8)
+            for qi in (0, 255):
+                n = xy.shape[0] * types.size
+                dd, de, dl = (zav1.DeviceArray(n, np.uint64), zav1.DeviceArray(n, np.int32),
+                              zav1.DeviceArray(n, np.uint32))
+                zav1.txsearch_batch(d_plane, 128, bd, tx_size, d_xy, xy.shape[0], types, qi, dd, de, dl)
+                rd, re, rl = oracle.txsearch_plane(plane, bd, tx_size, xy, types, qi)
+                assert np.array_equal(dd.to_host(), rd), (tx_size, tl, qi)
+                assert np.array_equal(de.to_host(), re), (tx_size, tl, qi)
+                assert np.array_equal(dl.to_host(), rl), (tx_size, tl, qi)

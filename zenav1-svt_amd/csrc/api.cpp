@@ -83,32 +83,6 @@ int quant_rows(const zav1::QuantRow** out) {
   return ZAV1_OK;
 }
 
-// AV1 coefficient scans of the packed (<= 32x32) block: class 0 runs the
-// anti-diagonals (square blocks alternate direction starting up-right, wide
-// blocks always run up-right, tall blocks down-left), class 1 is raster,
-// class 2 column-major.  Same orders as the C-extracted tables of
-// rust/crates/svtav1-entropy/src/scan_tables.rs.
-void build_scan(int tx_size, int cls, uint16_t* scan) {
-  int w = zav1::tx_w(tx_size), h = zav1::tx_h(tx_size);
-  w = w > 32 ? 32 : w;
-  h = h > 32 ? 32 : h;
-  int k = 0;
-  if (cls == 1) {
-    for (int i = 0; i < w * h; i++) scan[k++] = (uint16_t)i;
-  } else if (cls == 2) {
-    for (int c = 0; c < w; c++)
-      for (int r = 0; r < h; r++) scan[k++] = (uint16_t)(r * w + c);
-  } else {
-    for (int d = 0; d < w + h - 1; d++) {
-      const bool up = (w == h) ? ((d & 1) == 0) : (w > h);
-      for (int i = 0; i < h; i++) {
-        const int r = up ? (h - 1 - i) : i, c = d - r;
-        if (c >= 0 && c < w) scan[k++] = (uint16_t)(r * w + c);
-      }
-    }
-  }
-}
-
 int iscan_table(const uint16_t** out) {
   int dev = 0;
   ZCHECK(hipGetDevice(&dev));
@@ -118,7 +92,7 @@ int iscan_table(const uint16_t** out) {
     std::vector<uint16_t> h(19 * 3 * 1024, 0), scan(1024);
     for (int s = 0; s < 19; s++)
       for (int c = 0; c < 3; c++) {
-        build_scan(s, c, scan.data());
+        zav1::build_scan(s, c, scan.data());
         int w = zav1::tx_w(s), hh = zav1::tx_h(s);
         const int n = (w > 32 ? 32 : w) * (hh > 32 ? 32 : hh);
         for (int i = 0; i < n; i++) h[(s * 3 + c) * 1024 + scan[i]] = (uint16_t)i;
@@ -888,7 +862,7 @@ int zav1_satd_batch(const int16_t* d_res, int32_t* d_coeff, int32_t* d_satd, int
 
 int zav1_scan_order(int tx_size, int scan_class, uint16_t* scan) {
   if (tx_size < 0 || tx_size >= 19 || scan_class < 0 || scan_class > 2 || !scan) return ZAV1_EINVAL;
-  build_scan(tx_size, scan_class, scan);
+  zav1::build_scan(tx_size, scan_class, scan);
   int w = zav1::tx_w(tx_size), h = zav1::tx_h(tx_size);
   return (w > 32 ? 32 : w) * (h > 32 ? 32 : h);
 }

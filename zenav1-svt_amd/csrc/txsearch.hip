@@ -163,6 +163,220 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   }
 }
 
+// Lane-per-block form for small blocks: one lane holds a whole block and
+// evaluates every candidate type of it in registers.  All lanes of a wave
+// take the same column variant and the same type at the same time, so the
+// kinds, the scan class and the quantizer parameters are wave-uniform
+// (scalar registers, scalar branches), each transform is specialised on its
+// kind at compile time and no lane exchanges data: no LDS, no barriers, no
+// cross-lane reductions.  Types are processed grouped by column variant
+// (plan.var_start), so only the current variant's column outputs are live.
+// Scan position + 1 of every coefficient of a block of <= 64 samples, per
+// scan class (0 default, 1 raster for V_*, 2 column-major for H_*).
+struct LaneScan {
+  uint32_t pos[3][16];  // four positions per dword, raster order
+};
+
+// Passes the inputs of one arm of a wave-uniform selection through an
+// empty volatile asm.  The arms' transforms depend only on values that are
+// invariant in the variant / type loops, so without this the compiler
+// hoists every arm out of the loop (all four kinds computed up front:
+// registers for four, and the 24-bit multiply forms lost).  No instruction
+// is emitted.
+template <int N>
+__device__ __forceinline__ void opaque(int32_t (&x)[N]) {
+  static_assert(N % 4 == 0, "groups of four");
+#pragma unroll
+  for (int i = 0; i < N; i += 4) asm volatile("" : "+v"(x[i]), "+v"(x[i + 1]), "+v"(x[i + 2]), "+v"(x[i + 3]));
+}
+
+// Column pass of one variant from the block's residual, held packed as
+// 16-bit pairs ((sample - dc) << s0 fits 16 bits for <= 10-bit content).
+// The words of each column pair pass through opaque(), so the pass stays in
+// the variant loop.
+template <int W, int H, int CK>
+__device__ __forceinline__ void ts_lane_cols(const uint32_t (&rp)[H][W / 2], int32_t (&co)[H][W]) {
+  constexpr int CBC = fwd_cb_col(W, H);
+  constexpr FwdShift SH = fwd_shift(W, H);
+  constexpr long long kBCol = fwd_col_bound(SH.s0);
+#pragma unroll
+  for (int k = 0; k < W / 2; k++) {
+    int32_t wv[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) wv[r] = (int32_t)rp[r][k];
+    opaque(wv);
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      int32_t x[H];
+#pragma unroll
+      for (int r = 0; r < H; r++) {
+        const int32_t w = wv[CK == 2 ? H - 1 - r : r];  // FLIPADST: up-down flip
+        x[r] = half ? (w >> 16) : (int32_t)(int16_t)(w & 0xffff);
+      }
+      fwd1d<H, CBC, kBCol>(x, CK);
+#pragma unroll
+      for (int r = 0; r < H; r++) co[r][2 * k + half] = rshift_round32(x[r], SH.s1);
+    }
+  }
+}
+
+template <int W, int H, int RK, int BD>
+__device__ __forceinline__ void ts_lane_rows(const int32_t (&co)[H][W], const uint32_t* __restrict__ pos,
+                                             const QuantFp& qf, uint64_t& dist_out, int32_t& eob_out,
+                                             uint32_t& lsum_out) {
+  constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
+  constexpr FwdShift SH = fwd_shift(W, H);
+  constexpr long long kBCol = fwd_col_bound(SH.s0);
+  constexpr long long kBRow = fwd_row_bound(H, CBC, SH.s1, kBCol);
+  constexpr int LSC = tx_log_scale(W, H);
+  constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
+  uint64_t dist = 0;
+  int32_t eob = 0, lsum = 0;
+  uint32_t sq4 = 0;
+  // the type's scan positions, read before the transform and held in scalar
+  // registers: a select on a freshly loaded value would otherwise become a
+  // branch around the load, per coefficient
+  uint32_t pw[W * H / 4];
+#pragma unroll
+  for (int k = 0; k < W * H / 4; k++) {
+    pw[k] = pos[k];
+    asm volatile("" : "+s"(pw[k]));
+  }
+#pragma unroll
+  for (int r = 0; r < H; r++) {
+    int32_t v[W];
+#pragma unroll
+    for (int c = 0; c < W; c++) v[c] = RK == 2 ? co[r][W - 1 - c] : co[r][c];  // FLIPADST: left-right flip
+    opaque(v);
+    fwd1d<W, CBR, kBRow>(v, RK);
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+      int32_t x = rshift_round32(v[c], SH.s2);
+      // 2:1 rectangles: x NewSqrt2 / 2^12 (|x| < 2^18, the product fits 32 bits)
+      if constexpr (RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
+      const int32_t a = x < 0 ? -x : x;
+      int32_t e;  // a - |dequantized|, |e| < 2^15: four squares sum in 32 bits
+      const int32_t q = quant_fp_err<LSC, BD>(a, (r | c) != 0, qf, &e);
+      lsum += q;
+      // scan position + 1 of this coefficient (kernel argument: scalar)
+      const int32_t p = (int32_t)((pw[(r * W + c) >> 2] >> (8 * ((r * W + c) & 3))) & 0xffu);
+      eob = max(eob, q ? p : 0);
+      sq4 = (uint32_t)mad_i24(e, e, (int32_t)sq4);
+      if (((r * W + c) & 3) == 3) {
+        dist += sq4;
+        sq4 = 0;
+      }
+    }
+  }
+  constexpr int SHF = (1 - LSC) * 2;
+  dist_out = SHF < 0 ? (dist << (-SHF)) : (dist >> SHF);
+  eob_out = eob;
+  lsum_out = (uint32_t)lsum;
+}
+
+constexpr int kLaneBlocks = 64;  // blocks per workgroup: one per lane of each wave
+constexpr int kStageRow = 17;    // staging row: 16 types + 1 (spreads the blocks' rows over the banks)
+
+// Workgroup of 4 waves over the same 64 blocks; wave v takes column variant
+// v (all of its types) for every block, lane l block l.  Results meet in a
+// per-workgroup staging block and leave as contiguous rows.
+template <int W, int H, int BD>
+__global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restrict__ plane, int stride, int bd,
+                                                       const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
+                                                       const QuantRow* __restrict__ row_p, LaneScan scan,
+                                                       uint64_t* __restrict__ out_dist,
+                                                       int32_t* __restrict__ out_eob,
+                                                       uint32_t* __restrict__ out_lsum) {
+  static_assert(W * H % 4 == 0 && W <= 16 && H <= 16, "lane-per-block sizes");
+  __shared__ uint64_t st_d[kLaneBlocks * kStageRow];
+  __shared__ int32_t st_e[kLaneBlocks * kStageRow];
+  __shared__ uint32_t st_l[kLaneBlocks * kStageRow];
+  constexpr FwdShift SH = fwd_shift(W, H);
+  constexpr int LSC = tx_log_scale(W, H);
+  const int lane = threadIdx.x & 63;
+  const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // this wave's variant
+  const long b0 = (long)xcd_chunked(blockIdx.x, gridDim.x) * kLaneBlocks;
+  const int ntyp = plan.n_types;
+
+  if (v < plan.n_var) {
+    const long b = b0 + lane;
+    const int bi = b < n_blocks ? (int)b : 0;  // a lane past the end computes block 0, never copied out
+    const int x0 = xy[2 * bi], y0 = xy[2 * bi + 1];
+    const uint32_t o0 = (uint32_t)y0 * (uint32_t)stride + (uint32_t)x0;
+
+    // DC prediction from the source neighbours
+    uint32_t sa = 0, sl = 0;
+    if (y0 > 0) {
+#pragma unroll
+      for (int c = 0; c < W; c++) sa += plane[o0 - (uint32_t)stride + (uint32_t)c];
+    }
+    if (x0 > 0) {
+#pragma unroll
+      for (int r = 0; r < H; r++) sl += plane[o0 + (uint32_t)(r * stride) - 1u];
+    }
+    int32_t dc;
+    if (y0 > 0 && x0 > 0) dc = (int32_t)((sa + sl + (W + H) / 2) / (W + H));
+    else if (y0 > 0) dc = (int32_t)((sa + W / 2) / W);
+    else if (x0 > 0) dc = (int32_t)((sl + H / 2) / H);
+    else dc = 1 << (BD - 1);
+
+    // residual, packed: (sample - dc) << s0 as 16-bit pairs
+    uint32_t rp[H][W / 2];
+#pragma unroll
+    for (int r = 0; r < H; r++)
+#pragma unroll
+      for (int k = 0; k < W / 2; k++) {
+        const uint32_t o = o0 + (uint32_t)(r * stride + 2 * k);
+        const int32_t lo = ((int32_t)plane[o] - dc) << SH.s0, hi = ((int32_t)plane[o + 1] - dc) << SH.s0;
+        rp[r][k] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
+      }
+
+    const QuantFp qf = make_quant_fp(*row_p, LSC);
+    const uint32_t vi = plan.lane_var[v];
+    const int ck = (int)(vi & 0xff);
+    int32_t co[H][W];
+    if (ck == 0) ts_lane_cols<W, H, 0>(rp, co);
+    else if (ck == 1) ts_lane_cols<W, H, 1>(rp, co);
+    else if (ck == 2) ts_lane_cols<W, H, 2>(rp, co);
+    else ts_lane_cols<W, H, 3>(rp, co);
+    const int i_end = (int)(vi >> 16);
+    for (int i = (int)((vi >> 8) & 0xff); i < i_end; i++) {
+      const uint32_t ti = plan.lane_type[i];
+      const uint32_t* pos = scan.pos[(ti >> 16) & 0xff];
+      uint64_t d;
+      int32_t e;
+      uint32_t l;
+      const int rk = (int)((ti >> 8) & 0xff);
+      if (rk == 0) ts_lane_rows<W, H, 0, BD>(co, pos, qf, d, e, l);
+      else if (rk == 1) ts_lane_rows<W, H, 1, BD>(co, pos, qf, d, e, l);
+      else if (rk == 2) ts_lane_rows<W, H, 2, BD>(co, pos, qf, d, e, l);
+      else ts_lane_rows<W, H, 3, BD>(co, pos, qf, d, e, l);
+      const int k = (int)(ti & 0xff);
+      st_d[lane * kStageRow + k] = d;
+      st_e[lane * kStageRow + k] = e;
+      st_l[lane * kStageRow + k] = l;
+    }
+  }
+  __syncthreads();
+
+  // The 64 blocks own one contiguous range of each output array: copy it out
+  // from the staging rows, consecutive threads on consecutive entries (entry
+  // j -> block j / ntyp by a multiply-shift, exact for j < 1024, ntyp <= 16).
+  const long nb = n_blocks - b0 < kLaneBlocks ? n_blocks - b0 : kLaneBlocks;
+  const uint32_t mg = (65536u + (uint32_t)ntyp - 1u) / (uint32_t)ntyp;
+  for (int j = threadIdx.x; j < nb * ntyp; j += blockDim.x) {
+    const int blk = (int)(((uint32_t)j * mg) >> 16), k = j - blk * ntyp;
+    const int src = blk * kStageRow + k;
+    out_dist[b0 * ntyp + j] = st_d[src];
+    out_eob[b0 * ntyp + j] = st_e[src];
+    out_lsum[b0 * ntyp + j] = st_l[src];
+  }
+}
+
+// Sizes evaluated by the lane-per-block kernel (by measurement: 16-wide rows
+// hold too many registers per lane; 16x4 stays on the group form).
+constexpr bool ts_lane_size(int w, int h) { return w * h <= 64 && w <= 8; }
+
 // Lanes per block, picked per size by measurement on MI355X (config-2
 // sweep): more lanes than max(W, H) only where the extra type slots of the
 // row pass beat the per-block overhead they add.
@@ -173,26 +387,72 @@ constexpr int ts_lanes(int w, int h) {
        : ((w == 8 && h == 16) || (w == 16 && h == 8)) ? 32
        : (w > h ? w : h);
 }
+static TxPlan plan_by_variant(const TxPlan& a);
+
 template <int S>
 static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int32_t* xy, int n_blocks,
-                            const TypeList& types, const TxPlan& plan, const QuantRow* row, const uint16_t* iscan,
-                            uint64_t* d, int32_t* e, uint32_t* l, hipStream_t st) {
+                            const TxPlan& plan, const QuantRow* row, const uint16_t* iscan, uint64_t* d, int32_t* e,
+                            uint32_t* l, hipStream_t st) {
   constexpr int W = tx_w(S), H = tx_h(S);
-  constexpr int P = ts_lanes(W, H);
-  using G = BlkGeom<W, H, P>;
-  // groups per workgroup: up to 256 threads and 64 KB of column variants
-  const int per_group = plan.n_var * G::VS * (int)sizeof(int32_t);
-  int groups = 256 / G::T;
-  while (groups > 1 && groups * per_group > 65536) groups >>= 1;
-  const int grid = (n_blocks + groups - 1) / groups;
-  if (bd == 8)
-    hipLaunchKernelGGL((k_txsearch<W, H, P, 8>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
-                       stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
-  else
-    hipLaunchKernelGGL((k_txsearch<W, H, P, 10>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
-                       stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
-  (void)types;
+  if constexpr (ts_lane_size(W, H)) {
+    const TxPlan lp = plan_by_variant(plan);
+    LaneScan sc{};
+    uint16_t order[1024];
+    for (int c = 0; c < 3; c++) {
+      build_scan(S, c, order);
+      for (int i = 0; i < W * H; i++) sc.pos[c][order[i] >> 2] |= (uint32_t)(i + 1) << (8 * (order[i] & 3));
+    }
+    // one wave per column variant: 4 waves (the plans here have <= 4 variants)
+    const int grid = (n_blocks + kLaneBlocks - 1) / kLaneBlocks;
+    if (bd == 8)
+      hipLaunchKernelGGL((k_txsearch_lane<W, H, 8>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy, n_blocks,
+                         lp, row, sc, d, e, l);
+    else
+      hipLaunchKernelGGL((k_txsearch_lane<W, H, 10>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+                         n_blocks, lp, row, sc, d, e, l);
+    (void)iscan;
+  } else {
+    constexpr int P = ts_lanes(W, H);
+    using G = BlkGeom<W, H, P>;
+    // groups per workgroup: up to 256 threads and 64 KB of column variants
+    const int per_group = plan.n_var * G::VS * (int)sizeof(int32_t);
+    int groups = 256 / G::T;
+    while (groups > 1 && groups * per_group > 65536) groups >>= 1;
+    const int grid = (n_blocks + groups - 1) / groups;
+    if (bd == 8)
+      hipLaunchKernelGGL((k_txsearch<W, H, P, 8>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
+                         stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
+    else
+      hipLaunchKernelGGL((k_txsearch<W, H, P, 10>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
+                         stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
+  }
   return hipGetLastError();
+}
+
+// AV1 coefficient scans of the packed (<= 32x32) block: class 0 runs the
+// anti-diagonals (square blocks alternate direction starting up-right, wide
+// blocks always run up-right, tall blocks down-left), class 1 is raster,
+// class 2 column-major.  Same orders as the C-extracted tables of
+// rust/crates/svtav1-entropy/src/scan_tables.rs.
+void build_scan(int tx_size, int cls, uint16_t* scan) {
+  int w = tx_w(tx_size), h = tx_h(tx_size);
+  w = w > 32 ? 32 : w;
+  h = h > 32 ? 32 : h;
+  int k = 0;
+  if (cls == 1) {
+    for (int i = 0; i < w * h; i++) scan[k++] = (uint16_t)i;
+  } else if (cls == 2) {
+    for (int c = 0; c < w; c++)
+      for (int r = 0; r < h; r++) scan[k++] = (uint16_t)(r * w + c);
+  } else {
+    for (int d = 0; d < w + h - 1; d++) {
+      const bool up = (w == h) ? ((d & 1) == 0) : (w > h);
+      for (int i = 0; i < h; i++) {
+        const int r = up ? (h - 1 - i) : i, c = d - r;
+        if (c >= 0 && c < w) scan[k++] = (uint16_t)(r * w + c);
+      }
+    }
+  }
 }
 
 // Processing plan of a type list: types grouped by row kind (ADST and
@@ -222,6 +482,36 @@ static TxPlan make_plan(const TypeList& types) {
     p.type[i] = types.t[k];
     p.out_k[i] = (uint8_t)k;
     p.var_of[i] = (uint8_t)v;
+    p.row_kind[i] = (uint8_t)c.row;
+    const int t = types.t[k];
+    p.scan_cls[i] = (uint8_t)((t >= 10) ? (((t - 10) & 1) ? 2 : 1) : 0);
+  }
+  return p;
+}
+
+// The same plan with its types grouped by column variant (stable), for the
+// lane-per-block kernel.
+static TxPlan plan_by_variant(const TxPlan& a) {
+  TxPlan p = a;
+  int n = 0;
+  for (int v = 0; v < a.n_var; v++) {
+    p.var_start[v] = (uint8_t)n;
+    for (int i = 0; i < a.n_types; i++)
+      if (a.var_of[i] == v) {
+        p.type[n] = a.type[i];
+        p.out_k[n] = a.out_k[i];
+        p.var_of[n] = a.var_of[i];
+        p.row_kind[n] = a.row_kind[i];
+        p.scan_cls[n] = a.scan_cls[i];
+        n++;
+      }
+  }
+  p.var_start[a.n_var] = (uint8_t)n;
+  for (int i = 0; i < n; i++)
+    p.lane_type[i] = (uint32_t)p.out_k[i] | (uint32_t)p.row_kind[i] << 8 | (uint32_t)p.scan_cls[i] << 16;
+  for (int v = 0; v < a.n_var; v++) {
+    const uint32_t ck = p.var_ud[v] ? 2u : p.var_kind[v];
+    p.lane_var[v] = ck | (uint32_t)p.var_start[v] << 8 | (uint32_t)p.var_start[v + 1] << 16;
   }
   return p;
 }
@@ -232,7 +522,7 @@ hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size
   if (n_blocks <= 0 || types.n <= 0) return hipSuccess;
   const TxPlan plan = make_plan(types);
   switch (tx_size) {
-#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, types, plan, row, iscan, dist, eob, lsum, st);
+#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, plan, row, iscan, dist, eob, lsum, st);
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 #undef X
     default: return hipErrorInvalidValue;

@@ -103,6 +103,7 @@ __device__ __forceinline__ int32_t rshift_round(int32_t v, int bit) {
 __device__ __forceinline__ int32_t rshift_round32(int32_t v, int bit) {
   return bit == 0 ? v : (v + (1 << (bit - 1))) >> bit;
 }
+
 __device__ __forceinline__ int32_t rshift64r(int64_t v, int bit) {
   return (int32_t)((v + ((int64_t)1 << (bit - 1))) >> bit);
 }
@@ -191,7 +192,7 @@ template <int N, int CB>
 __device__ __forceinline__ void fadst4_narrow(int32_t* x) {
   const int32_t* s = kSinpi[CB - 10];
   const int32_t x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
-  if ((x0 | x1 | x2 | x3) == 0) return;
+  // (no early exit for an all-zero input: every output is then R >> CB = 0)
   const int32_t s0 = __mul24(s[1], x0), s1 = __mul24(s[4], x0), s2 = __mul24(s[2], x1);
   const int32_t s3 = __mul24(s[1], x1), s4 = __mul24(s[3], x2), s5 = __mul24(s[4], x3);
   const int32_t s6 = __mul24(s[2], x3);

@@ -61,6 +61,21 @@ __device__ __forceinline__ QuantFp make_quant_fp(const QuantRow& t, int ls) {
   }
   return f;
 }
+// The 24-bit multiplier forms, written out: x * y (x, y < 2^24, product
+// < 2^32) and x * y + a (|x|, |y| < 2^23, result in 32 bits).  Instruction
+// selection only picks them when it sees the operands' range in the same
+// block, which a loop-invariant factor hoisted out of the loop hides.
+__device__ __forceinline__ uint32_t mul_u24(uint32_t x, uint32_t y) {
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+  return r;
+}
+__device__ __forceinline__ int32_t mad_i24(int32_t x, int32_t y, int32_t a) {
+  int32_t r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(a));
+  return r;
+}
+
 // Level (>= 0) of |coefficient| a; *adq receives its dequantized magnitude.
 template <int LS>
 __device__ __forceinline__ int32_t quant_fp_fast(int32_t a, int iz, const QuantFp& f, int bd, int32_t* adq) {
@@ -69,7 +84,20 @@ __device__ __forceinline__ int32_t quant_fp_fast(int32_t a, int iz, const QuantF
   if (bd == 8) xx = xx > 32767 ? 32767 : xx;
   const uint32_t q = __umulhi((uint32_t)xx << LS, iz ? f.qf16[1] : f.qf16[0]);
   const int32_t qq = ((a << (1 + LS)) >= dqv) ? (int32_t)q : 0;
-  *adq = __mul24(qq, dqv) >> LS;
+  *adq = (int32_t)mul_u24((uint32_t)qq, (uint32_t)dqv) >> LS;  // both < 2^24, product < 2^31
+  return qq;
+}
+// The same level; *err receives a - |dequantized| (|err| < 2^15), for
+// LS == 0 in one multiply-add.
+template <int LS, int BD>
+__device__ __forceinline__ int32_t quant_fp_err(int32_t a, int iz, const QuantFp& f, int32_t* err) {
+  const int32_t dqv = iz ? f.dq[1] : f.dq[0];
+  int32_t xx = a + (iz ? f.rnd[1] : f.rnd[0]);
+  if constexpr (BD == 8) xx = xx > 32767 ? 32767 : xx;
+  const uint32_t q = __umulhi((uint32_t)xx << LS, iz ? f.qf16[1] : f.qf16[0]);
+  const int32_t qq = ((a << (1 + LS)) >= dqv) ? (int32_t)q : 0;
+  if constexpr (LS == 0) *err = mad_i24(qq, -dqv, a);
+  else *err = a - ((int32_t)mul_u24((uint32_t)qq, (uint32_t)dqv) >> LS);
   return qq;
 }
 
@@ -109,10 +137,20 @@ struct TypeList {
 // Processing plan of a candidate type list (built on the host): types in
 // processing order with their index in the caller's list, and the distinct
 // column transforms (kind 0 DCT / 1 ADST / 3 identity, up-down flip).
+// Coefficient scan of a packed (<= 32x32) block of tx_size, class 0
+// default / 1 raster / 2 column-major: scan[i] = raster index (host).
+void build_scan(int tx_size, int cls, uint16_t* scan);
+
 struct TxPlan {
   int32_t n_types, n_var;
   uint8_t type[16], out_k[16], var_of[16];
+  uint8_t row_kind[16], scan_cls[16];  // per type: row kind 0..3, scan class
   uint8_t var_kind[4], var_ud[4];
+  uint8_t var_start[5];  // types of variant v: [var_start[v], var_start[v+1]) when grouped by variant
+  // the same, one dword per entry (uniform loads of whole dwords stay scalar):
+  // lane_type[i] = out_k | row_kind << 8 | scan_cls << 16 (types grouped by
+  // variant), lane_var[v] = column kind 0..3 | first type << 8 | end << 16
+  uint32_t lane_type[16], lane_var[4];
 };
 // Fused tx-type search over blocks of one tx size on a 16-bit plane (8-bit
 // content is widened by the caller).  iscan: device table of inverse scans,
