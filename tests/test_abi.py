@@ -118,3 +118,34 @@ def test_context_entry_points_validate_without_gpu(zav1):
     assert L.zav1_wave_collect(None) == zav1.ZAV1_EINVAL
     assert L.zav1_md_wave_submit(None, None) == zav1.ZAV1_EINVAL
     assert L.zav1_ctx_stream(None) is None
+
+
+def test_fused_fp_level_forms_pinned(zav1):
+    """The fused search kernels' fp level: (1) the dead zone (level 0 when
+    2a < dequant) is implied by every row's own round_fp / quant_fp, so the
+    kernels skip the test; (2) the 24-bit form, high word of
+    ((a + round_fp) << S) * (quant_fp << (16 - S)) with S = 8 (8 bits, after
+    the INT16 clamp) or 6 (10 bits, quant_fp < 2^14), equals the 32-bit form
+    (a + round_fp) * quant_fp >> 16 for every |coefficient| < 2^17."""
+    rng = np.random.default_rng(5)
+    for bd in (8, 10):
+        for q in range(256):
+            row = zav1.quant_row(q, bd)
+            for i in (0, 1):
+                dq, rf, qf = row.dequant[i], row.round_fp[i], row.quant_fp[i]
+                a = np.arange(0, (dq + 1) // 2, dtype=np.int64)  # every a with 2a < dq
+                xx = a + rf
+                if bd == 8:
+                    xx = np.minimum(xx, 32767)
+                assert not np.any((xx * qf) >> 16), (bd, q, i)
+                S = 8 if bd == 8 else 6
+                if bd == 10 and qf >= 1 << 14:
+                    assert q == 0  # only qindex 0 takes the 32-bit form
+                    continue
+                a = np.concatenate([rng.integers(0, 1 << 17, 4000), [0, 1, (1 << 17) - 1]]).astype(np.int64)
+                xx = a + rf
+                if bd == 8:
+                    xx = np.minimum(xx, 32767)
+                f1, f2 = xx << S, np.int64(qf) << (16 - S)
+                assert f1.max() < (1 << 24) and f2 < (1 << 24), (bd, q, i)
+                assert np.array_equal((f1 * f2) >> 32, (xx * qf) >> 16), (bd, q, i)

@@ -885,8 +885,11 @@ int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size
   if (rc) return rc;
   rc = iscan_table(&iscan);
   if (rc) return rc;
+  zav1::QuantRow hr;
+  build_row(qindex, bd, &hr);
+  const bool q24 = bd == 8 || (hr.quant_fp[0] < 16384 && hr.quant_fp[1] < 16384);
   ZCHECK(zav1::txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, tl, rows + (bd == 10 ? 256 : 0) + qindex,
-                              iscan, d_dist, d_eob, d_lsum, S(s)));
+                              q24, iscan, d_dist, d_eob, d_lsum, S(s)));
   return ZAV1_OK;
 }
 

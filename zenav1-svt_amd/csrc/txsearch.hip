@@ -220,9 +220,9 @@ __device__ __forceinline__ void ts_lane_cols(const uint32_t (&rp)[H][W / 2], int
   }
 }
 
-template <int W, int H, int RK, int BD>
+template <int W, int H, int RK, int BD, bool Q24>
 __device__ __forceinline__ void ts_lane_rows(const int32_t (&co)[H][W], const uint32_t* __restrict__ pos,
-                                             const QuantFp& qf, uint64_t& dist_out, int32_t& eob_out,
+                                             const QuantFpL& qf, uint64_t& dist_out, int32_t& eob_out,
                                              uint32_t& lsum_out) {
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
@@ -256,11 +256,12 @@ __device__ __forceinline__ void ts_lane_rows(const int32_t (&co)[H][W], const ui
       if constexpr (RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
       const int32_t a = x < 0 ? -x : x;
       int32_t e;  // a - |dequantized|, |e| < 2^15: four squares sum in 32 bits
-      const int32_t q = quant_fp_err<LSC, BD>(a, (r | c) != 0, qf, &e);
+      const int32_t q = quant_fpl<BD, Q24, true>(a, (r | c) != 0, qf, &e);
       lsum += q;
-      // scan position + 1 of this coefficient (kernel argument: scalar)
+      // scan position + 1 of this coefficient (<= 64, scalar); min(q << 7, p)
+      // is p for a nonzero level, else 0
       const int32_t p = (int32_t)((pw[(r * W + c) >> 2] >> (8 * ((r * W + c) & 3))) & 0xffu);
-      eob = max(eob, q ? p : 0);
+      eob = max(eob, min(q << 7, p));
       sq4 = (uint32_t)mad_i24(e, e, (int32_t)sq4);
       if (((r * W + c) & 3) == 3) {
         dist += sq4;
@@ -280,7 +281,7 @@ constexpr int kStageRow = 17;    // staging row: 16 types + 1 (spreads the block
 // Workgroup of 4 waves over the same 64 blocks; wave v takes column variant
 // v (all of its types) for every block, lane l block l.  Results meet in a
 // per-workgroup staging block and leave as contiguous rows.
-template <int W, int H, int BD>
+template <int W, int H, int BD, bool Q24>
 __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restrict__ plane, int stride, int bd,
                                                        const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
                                                        const QuantRow* __restrict__ row_p, LaneScan scan,
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
         rp[r][k] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
       }
 
-    const QuantFp qf = make_quant_fp(*row_p, LSC);
+    const QuantFpL qf = make_quant_fpl<BD, Q24>(*row_p);
     const uint32_t vi = plan.lane_var[v];
     const int ck = (int)(vi & 0xff);
     int32_t co[H][W];
@@ -347,10 +348,10 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
       int32_t e;
       uint32_t l;
       const int rk = (int)((ti >> 8) & 0xff);
-      if (rk == 0) ts_lane_rows<W, H, 0, BD>(co, pos, qf, d, e, l);
-      else if (rk == 1) ts_lane_rows<W, H, 1, BD>(co, pos, qf, d, e, l);
-      else if (rk == 2) ts_lane_rows<W, H, 2, BD>(co, pos, qf, d, e, l);
-      else ts_lane_rows<W, H, 3, BD>(co, pos, qf, d, e, l);
+      if (rk == 0) ts_lane_rows<W, H, 0, BD, Q24>(co, pos, qf, d, e, l);
+      else if (rk == 1) ts_lane_rows<W, H, 1, BD, Q24>(co, pos, qf, d, e, l);
+      else if (rk == 2) ts_lane_rows<W, H, 2, BD, Q24>(co, pos, qf, d, e, l);
+      else ts_lane_rows<W, H, 3, BD, Q24>(co, pos, qf, d, e, l);
       const int k = (int)(ti & 0xff);
       st_d[lane * kStageRow + k] = d;
       st_e[lane * kStageRow + k] = e;
@@ -373,6 +374,199 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
   }
 }
 
+// Row form for the 16-point sizes (16x16, 8x16, 16x8, 16x4): one lane per
+// row, 64 / H blocks per wave, and every lane of a wave on the same column
+// variant / the same type at a time (kinds, scan class and type data are
+// wave-uniform and scalar).  Per variant: the lanes' columns (residual held
+// packed from the start) are transformed into a per-wave LDS tile; per type
+// each lane transforms and quantizes its row with its scan positions
+// (preloaded bytes for the three classes), and the block's H lanes reduce.
+// A block lives in one wave, so only wave-level ordering is needed.
+template <int W, int H, int RK, int BD, bool Q24>
+__device__ __forceinline__ void ts_row_arm(const int32_t* __restrict__ src, const uint32_t (&pc)[W / 4], int r,
+                                           const QuantFpL& qf, uint64_t& dist, int32_t& m, int32_t& lsum) {
+  constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
+  constexpr FwdShift SH = fwd_shift(W, H);
+  constexpr long long kBCol = fwd_col_bound(SH.s0);
+  constexpr long long kBRow = fwd_row_bound(H, CBC, SH.s1, kBCol);
+  constexpr int LSC = tx_log_scale(W, H);
+  constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
+  int32_t v[W];
+#pragma unroll
+  for (int c = 0; c < W; c++) v[c] = src[RK == 2 ? W - 1 - c : c];  // FLIPADST: left-right flip
+  opaque(v);
+  fwd1d<W, CBR, kBRow>(v, RK);
+  uint32_t sq4 = 0;
+#pragma unroll
+  for (int c = 0; c < W; c++) {
+    int32_t x = rshift_round32(v[c], SH.s2);
+    if constexpr (RECT2) x = (__mul24(x, 5793) + 2048) >> 12;  // |x| < 2^18
+    const int32_t a = x < 0 ? -x : x;
+    int32_t e;
+    // column 0: DC or AC by the lane's row; other columns AC (uniform)
+    const int32_t q = c == 0 ? quant_fpl<BD, Q24, false>(a, r != 0, qf, &e) : quant_fpl<BD, Q24, true>(a, 1, qf, &e);
+    lsum += q;
+    const int32_t p = (int32_t)((pc[c >> 2] >> (8 * (c & 3))) & 0xffu);  // scan index
+    m = max(m, q ? p : -1);
+    sq4 = (uint32_t)mad_i24(e, e, (int32_t)sq4);
+    if ((c & 3) == 3) {
+      dist += sq4;
+      sq4 = 0;
+    }
+  }
+}
+
+template <int W, int H, int CK>
+__device__ __forceinline__ void ts_row_cols(const uint32_t (&rp)[(W >= H ? W / H : 1)][H / 2], int r,
+                                            int32_t* __restrict__ tile) {
+  constexpr int CBC = fwd_cb_col(W, H);
+  constexpr FwdShift SH = fwd_shift(W, H);
+  constexpr long long kBCol = fwd_col_bound(SH.s0);
+  constexpr int NC = W >= H ? W / H : 1, LS = W + 1;
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    int32_t wv[H / 2];
+#pragma unroll
+    for (int i = 0; i < H / 2; i++) wv[i] = (int32_t)rp[k][i];
+    if constexpr (H / 2 >= 4) opaque(wv);
+    else asm volatile("" : "+v"(wv[0]), "+v"(wv[1]));
+    int32_t x[H];
+#pragma unroll
+    for (int rr = 0; rr < H; rr++) {
+      const int s = CK == 2 ? H - 1 - rr : rr;  // FLIPADST: up-down flip
+      x[rr] = (s & 1) ? (wv[s >> 1] >> 16) : (int32_t)(int16_t)(wv[s >> 1] & 0xffff);
+    }
+    fwd1d<H, CBC, kBCol>(x, CK);
+    const int c = r + H * k;
+    if (c < W) {
+#pragma unroll
+      for (int rr = 0; rr < H; rr++) tile[rr * LS + c] = rshift_round32(x[rr], SH.s1);
+    }
+  }
+}
+
+template <int W, int H, int BD, bool Q24>
+__global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restrict__ plane, int stride, int bd,
+                                                       const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
+                                                       const QuantRow* __restrict__ row_p,
+                                                       const uint16_t* __restrict__ iscan_all, int tx_size,
+                                                       uint64_t* __restrict__ out_dist,
+                                                       int32_t* __restrict__ out_eob,
+                                                       uint32_t* __restrict__ out_lsum) {
+  static_assert((W == 16 || H == 16) && W >= 8 && W * H <= 256, "row-form sizes");
+  constexpr int BPW = 64 / H;                 // blocks per wave
+  constexpr int LS = W + 1;                   // padded tile row
+  constexpr int NC = W >= H ? W / H : 1;      // columns per lane in the column pass
+  constexpr FwdShift SH = fwd_shift(W, H);
+  constexpr int LSC = tx_log_scale(W, H);
+  __shared__ int32_t tiles[4][BPW * H * LS];
+  __shared__ uint64_t st_d[4][BPW * kStageRow];
+  __shared__ int32_t st_e[4][BPW * kStageRow];
+  __shared__ uint32_t st_l[4][BPW * kStageRow];
+
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int g = lane / H, r = lane - g * H;  // block within the wave, row / column slot
+  const long bw0 = ((long)xcd_chunked(blockIdx.x, gridDim.x) * 4 + wv) * BPW;
+  const long b = bw0 + g;
+  const int bi = b < n_blocks ? (int)b : 0;  // past the end: block 0, never copied out
+  const int x0 = xy[2 * bi], y0 = xy[2 * bi + 1];
+  const uint32_t o0 = (uint32_t)y0 * (uint32_t)stride + (uint32_t)x0;
+
+  // DC prediction from the source neighbours (block's H lanes reduce)
+  uint32_t sa = 0, sl = 0;
+  if (y0 > 0) {
+#pragma unroll
+    for (int k = 0; k < NC; k++)
+      if (r + H * k < W) sa += plane[o0 - (uint32_t)stride + (uint32_t)(r + H * k)];
+  }
+  if (x0 > 0) sl = plane[o0 + (uint32_t)(r * stride) - 1u];
+  sa = (uint32_t)group_sum_i32<H>((int32_t)sa);
+  sl = (uint32_t)group_sum_i32<H>((int32_t)sl);
+  int32_t dc;
+  if (y0 > 0 && x0 > 0) dc = (int32_t)((sa + sl + (W + H) / 2) / (W + H));
+  else if (y0 > 0) dc = (int32_t)((sa + W / 2) / W);
+  else if (x0 > 0) dc = (int32_t)((sl + H / 2) / H);
+  else dc = 1 << (BD - 1);
+
+  // the lane's columns of the residual, packed: (sample - dc) << s0 in 16-bit pairs
+  uint32_t rp[NC][H / 2];
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    const int c = (r + H * k) % W;  // lanes without a column (W < H) read a valid one, never stored
+#pragma unroll
+    for (int i = 0; i < H / 2; i++) {
+      const uint32_t o = o0 + (uint32_t)(2 * i * stride + c);
+      const int32_t lo = ((int32_t)plane[o] - dc) << SH.s0;
+      const int32_t hi = ((int32_t)plane[o + (uint32_t)stride] - dc) << SH.s0;
+      rp[k][i] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
+    }
+  }
+  // scan index of each coefficient of the lane's row, three classes, bytes
+  uint32_t pw[3][W / 4];
+#pragma unroll
+  for (int cls = 0; cls < 3; cls++) {
+    const uint16_t* is = iscan_all + (tx_size * 3 + cls) * 1024 + r * W;
+#pragma unroll
+    for (int k = 0; k < W / 4; k++)
+      pw[cls][k] = (uint32_t)is[4 * k] | (uint32_t)is[4 * k + 1] << 8 | (uint32_t)is[4 * k + 2] << 16 |
+                   (uint32_t)is[4 * k + 3] << 24;
+  }
+
+  const QuantFpL qf = make_quant_fpl<BD, Q24>(*row_p);
+  const int ntyp = plan.n_types;
+  int32_t* tile = tiles[wv] + g * H * LS;
+  for (int v = 0; v < plan.n_var; v++) {
+    const uint32_t vi = plan.lane_var[v];
+    const int ck = (int)(vi & 0xff);
+    wave_sync();  // the previous variant's rows have been read
+    if (ck == 0) ts_row_cols<W, H, 0>(rp, r, tile);
+    else if (ck == 1) ts_row_cols<W, H, 1>(rp, r, tile);
+    else if (ck == 2) ts_row_cols<W, H, 2>(rp, r, tile);
+    else ts_row_cols<W, H, 3>(rp, r, tile);
+    wave_sync();
+    const int i_end = (int)(vi >> 16);
+    for (int i = (int)((vi >> 8) & 0xff); i < i_end; i++) {
+      const uint32_t ti = plan.lane_type[i];
+      const int cls = (int)((ti >> 16) & 0xff), rk = (int)((ti >> 8) & 0xff);
+      uint32_t pc[W / 4];
+#pragma unroll
+      for (int k = 0; k < W / 4; k++) pc[k] = cls == 0 ? pw[0][k] : (cls == 1 ? pw[1][k] : pw[2][k]);
+      uint64_t dist = 0;
+      int32_t m = -1, lsum = 0;
+      const int32_t* src = tile + r * LS;
+      if (rk == 0) ts_row_arm<W, H, 0, BD, Q24>(src, pc, r, qf, dist, m, lsum);
+      else if (rk == 1) ts_row_arm<W, H, 1, BD, Q24>(src, pc, r, qf, dist, m, lsum);
+      else if (rk == 2) ts_row_arm<W, H, 2, BD, Q24>(src, pc, r, qf, dist, m, lsum);
+      else ts_row_arm<W, H, 3, BD, Q24>(src, pc, r, qf, dist, m, lsum);
+      dist = group_sum_u64<H>(dist);
+      m = group_max_i32<H>(m);
+      lsum = group_sum_i32<H>(lsum);
+      if (r == 0) {
+        const int k = (int)(ti & 0xff);
+        st_d[wv][g * kStageRow + k] = dist >> 2;  // (1 - log scale) * 2 with log scale 0
+        st_e[wv][g * kStageRow + k] = m + 1;
+        st_l[wv][g * kStageRow + k] = (uint32_t)lsum;
+      }
+    }
+  }
+  wave_sync();
+
+  // the wave's blocks own one contiguous range of each output array
+  const long nb = n_blocks - bw0 < BPW ? n_blocks - bw0 : BPW;
+  const uint32_t mg = (65536u + (uint32_t)ntyp - 1u) / (uint32_t)ntyp;
+  for (int j = lane; j < nb * ntyp; j += 64) {
+    const int blk = (int)(((uint32_t)j * mg) >> 16), k = j - blk * ntyp;
+    const int src = blk * kStageRow + k;
+    out_dist[bw0 * ntyp + j] = st_d[wv][src];
+    out_eob[bw0 * ntyp + j] = st_e[wv][src];
+    out_lsum[bw0 * ntyp + j] = st_l[wv][src];
+  }
+  (void)bd;
+}
+
+// Sizes evaluated by the row form.
+constexpr bool ts_row_size(int w, int h) { return (w == 16 || h == 16) && w >= 8 && w * h <= 256; }
+
 // Sizes evaluated by the lane-per-block kernel (by measurement: 16-wide rows
 // hold too many registers per lane; 16x4 stays on the group form).
 constexpr bool ts_lane_size(int w, int h) { return w * h <= 64 && w <= 8; }
@@ -391,8 +585,8 @@ static TxPlan plan_by_variant(const TxPlan& a);
 
 template <int S>
 static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int32_t* xy, int n_blocks,
-                            const TxPlan& plan, const QuantRow* row, const uint16_t* iscan, uint64_t* d, int32_t* e,
-                            uint32_t* l, hipStream_t st) {
+                            const TxPlan& plan, const QuantRow* row, bool q24, const uint16_t* iscan, uint64_t* d,
+                            int32_t* e, uint32_t* l, hipStream_t st) {
   constexpr int W = tx_w(S), H = tx_h(S);
   if constexpr (ts_lane_size(W, H)) {
     const TxPlan lp = plan_by_variant(plan);
@@ -405,12 +599,27 @@ static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int
     // one wave per column variant: 4 waves (the plans here have <= 4 variants)
     const int grid = (n_blocks + kLaneBlocks - 1) / kLaneBlocks;
     if (bd == 8)
-      hipLaunchKernelGGL((k_txsearch_lane<W, H, 8>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy, n_blocks,
-                         lp, row, sc, d, e, l);
+      hipLaunchKernelGGL((k_txsearch_lane<W, H, 8, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+                         n_blocks, lp, row, sc, d, e, l);
+    else if (q24)
+      hipLaunchKernelGGL((k_txsearch_lane<W, H, 10, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+                         n_blocks, lp, row, sc, d, e, l);
     else
-      hipLaunchKernelGGL((k_txsearch_lane<W, H, 10>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+      hipLaunchKernelGGL((k_txsearch_lane<W, H, 10, false>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
                          n_blocks, lp, row, sc, d, e, l);
     (void)iscan;
+  } else if constexpr (ts_row_size(W, H)) {
+    const TxPlan lp = plan_by_variant(plan);
+    const int grid = (n_blocks + 4 * (64 / H) - 1) / (4 * (64 / H));
+    if (bd == 8)
+      hipLaunchKernelGGL((k_txsearch_rows<W, H, 8, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+                         n_blocks, lp, row, iscan, S, d, e, l);
+    else if (q24)
+      hipLaunchKernelGGL((k_txsearch_rows<W, H, 10, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+                         n_blocks, lp, row, iscan, S, d, e, l);
+    else
+      hipLaunchKernelGGL((k_txsearch_rows<W, H, 10, false>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
+                         n_blocks, lp, row, iscan, S, d, e, l);
   } else {
     constexpr int P = ts_lanes(W, H);
     using G = BlkGeom<W, H, P>;
@@ -517,12 +726,12 @@ static TxPlan plan_by_variant(const TxPlan& a) {
 }
 
 hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
-                          int n_blocks, const TypeList& types, const QuantRow* row, const uint16_t* iscan,
+                          int n_blocks, const TypeList& types, const QuantRow* row, bool q24, const uint16_t* iscan,
                           uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st) {
   if (n_blocks <= 0 || types.n <= 0) return hipSuccess;
   const TxPlan plan = make_plan(types);
   switch (tx_size) {
-#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, plan, row, iscan, dist, eob, lsum, st);
+#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, plan, row, q24, iscan, dist, eob, lsum, st);
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 #undef X
     default: return hipErrorInvalidValue;

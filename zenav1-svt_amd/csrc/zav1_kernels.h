@@ -87,6 +87,63 @@ __device__ __forceinline__ int32_t quant_fp_fast(int32_t a, int iz, const QuantF
   *adq = (int32_t)mul_u24((uint32_t)qq, (uint32_t)dqv) >> LS;  // both < 2^24, product < 2^31
   return qq;
 }
+// fp quantizer of the fused search kernels (log scale 0), level on the
+// 24-bit multiplier when Q24: level = (xx * quant_fp) >> 16 = high word of
+// (xx << S) * (quant_fp << (16 - S)), exact when both factors are below
+// 2^24.  8 bits: xx <= 32767 (the INT16 clamp), S = 8.  10 bits: xx = a +
+// round_fp < 2^18 (|coefficient| < 2^17, pinned), S = 6, which needs
+// quant_fp < 2^14, i.e. dequant > 4: the caller takes the Q24 form only then
+// (everywhere but qindex 0).  Otherwise the 32-bit high multiply, S = 0.
+// The reference's dead zone (level 0 when 2a < dequant) needs no test here:
+// the rows hold round_fp = floor(dequant / 2) and quant_fp =
+// floor(2^16 / dequant), so 2a < dequant gives a + round_fp <= dequant - 1
+// and (a + round_fp) * quant_fp < 2^16, a zero level (pinned on the CPU for
+// every row).  UNI: the quantizer operands are wave-uniform (scalar
+// registers feed the multiplies directly).
+struct QuantFpL {
+  int32_t ndq[2];
+  uint32_t rs[2], qt[2];
+};
+__host__ __device__ constexpr int quant_fpl_shift(int bd, bool q24) { return q24 ? (bd == 8 ? 8 : 6) : 0; }
+template <int BD, bool Q24>
+__device__ __forceinline__ QuantFpL make_quant_fpl(const QuantRow& t) {
+  constexpr int S = quant_fpl_shift(BD, Q24);
+  QuantFpL f;
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    f.ndq[i] = -t.dequant[i];
+    f.rs[i] = (uint32_t)t.round_fp[i] << S;
+    f.qt[i] = (uint32_t)t.quant_fp[i] << (16 - S);
+  }
+  return f;
+}
+template <bool UNI>
+__device__ __forceinline__ uint32_t mul_hi_u24(uint32_t x, uint32_t y) {
+  uint32_t r;
+  if constexpr (UNI) asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(r) : "s"(y), "v"(x));
+  else asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+  return r;
+}
+template <bool UNI>
+__device__ __forceinline__ int32_t mad_i24u(int32_t x, int32_t y, int32_t a) {
+  int32_t r;
+  if constexpr (UNI) asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "s"(y), "v"(a));
+  else asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(a));
+  return r;
+}
+template <int BD, bool Q24, bool UNI>
+__device__ __forceinline__ int32_t quant_fpl(int32_t a, int iz, const QuantFpL& f, int32_t* err) {
+  constexpr int S = quant_fpl_shift(BD, Q24);
+  uint32_t xs = ((uint32_t)a << S) + (iz ? f.rs[1] : f.rs[0]);
+  if constexpr (BD == 8) xs = xs < (32767u << S) ? xs : (32767u << S);
+  const uint32_t qt = iz ? f.qt[1] : f.qt[0];
+  uint32_t q;
+  if constexpr (Q24) q = mul_hi_u24<UNI>(xs, qt);
+  else q = __umulhi(xs, qt);
+  *err = mad_i24u<UNI>((int32_t)q, iz ? f.ndq[1] : f.ndq[0], a);
+  return (int32_t)q;
+}
+
 // The same level; *err receives a - |dequantized| (|err| < 2^15), for
 // LS == 0 in one multiply-add.
 template <int LS, int BD>
@@ -155,8 +212,9 @@ struct TxPlan {
 // Fused tx-type search over blocks of one tx size on a 16-bit plane (8-bit
 // content is widened by the caller).  iscan: device table of inverse scans,
 // [tx_size][class][1024] u16.  Outputs per (block, type) pair.
+// q24: the row's quant_fp values allow the 24-bit level (see QuantFpL).
 hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
-                          int n_blocks, const TypeList& types, const QuantRow* row, const uint16_t* iscan,
+                          int n_blocks, const TypeList& types, const QuantRow* row, bool q24, const uint16_t* iscan,
                           uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st);
 
 // One transform unit of the fused unit kernel (layout of zav1_txunit_job).
