@@ -192,19 +192,25 @@ __device__ __forceinline__ void opaque(int32_t (&x)[N]) {
 
 // Column pass of one variant from the block's residual, held packed as
 // 16-bit pairs ((sample - dc) << s0 fits 16 bits for <= 10-bit content).
-// The words of each column pair pass through opaque(), so the pass stays in
-// the variant loop.
+// Where the type loop below keeps its inputs loop-carried (LOOPC, the
+// 64-sample sizes: measured faster there, slower for the smaller ones) the
+// arms only need to stay branches; otherwise each arm's inputs pass through
+// opaque().
+template <int W, int H>
+constexpr bool ts_lane_loopc() { return W * H == 64; }
+
 template <int W, int H, int CK>
 __device__ __forceinline__ void ts_lane_cols(const uint32_t (&rp)[H][W / 2], int32_t (&co)[H][W]) {
   constexpr int CBC = fwd_cb_col(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr long long kBCol = fwd_col_bound(SH.s0);
+  if constexpr (ts_lane_loopc<W, H>()) asm volatile("");  // no if-conversion of the four arms
 #pragma unroll
   for (int k = 0; k < W / 2; k++) {
     int32_t wv[H];
 #pragma unroll
     for (int r = 0; r < H; r++) wv[r] = (int32_t)rp[r][k];
-    opaque(wv);
+    if constexpr (!ts_lane_loopc<W, H>()) opaque(wv);
 #pragma unroll
     for (int half = 0; half < 2; half++) {
       int32_t x[H];
@@ -247,7 +253,7 @@ __device__ __forceinline__ void ts_lane_rows(const int32_t (&co)[H][W], const ui
     int32_t v[W];
 #pragma unroll
     for (int c = 0; c < W; c++) v[c] = RK == 2 ? co[r][W - 1 - c] : co[r][c];  // FLIPADST: left-right flip
-    opaque(v);
+    if constexpr (!ts_lane_loopc<W, H>()) opaque(v);
     fwd1d<W, CBR, kBRow>(v, RK);
 #pragma unroll
     for (int c = 0; c < W; c++) {
@@ -304,16 +310,18 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
     const int bi = b < n_blocks ? (int)b : 0;  // a lane past the end computes block 0, never copied out
     const int x0 = xy[2 * bi], y0 = xy[2 * bi + 1];
     const uint32_t o0 = (uint32_t)y0 * (uint32_t)stride + (uint32_t)x0;
+    const PlaneBuf pb(plane);
+    const uint32_t rb = 2u * (uint32_t)stride;  // row pitch in bytes
 
     // DC prediction from the source neighbours
     uint32_t sa = 0, sl = 0;
     if (y0 > 0) {
 #pragma unroll
-      for (int c = 0; c < W; c++) sa += plane[o0 - (uint32_t)stride + (uint32_t)c];
+      for (int c = 0; c < W; c++) sa += (uint32_t)pb.at(2u * (o0 - (uint32_t)stride) + 2u * c, 0);
     }
     if (x0 > 0) {
 #pragma unroll
-      for (int r = 0; r < H; r++) sl += plane[o0 + (uint32_t)(r * stride) - 1u];
+      for (int r = 0; r < H; r++) sl += (uint32_t)pb.at(2u * (o0 - 1u), rb * r);
     }
     int32_t dc;
     if (y0 > 0 && x0 > 0) dc = (int32_t)((sa + sl + (W + H) / 2) / (W + H));
@@ -327,8 +335,8 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
     for (int r = 0; r < H; r++)
 #pragma unroll
       for (int k = 0; k < W / 2; k++) {
-        const uint32_t o = o0 + (uint32_t)(r * stride + 2 * k);
-        const int32_t lo = ((int32_t)plane[o] - dc) << SH.s0, hi = ((int32_t)plane[o + 1] - dc) << SH.s0;
+        const int32_t lo = (pb.at(2u * o0 + 4u * k, rb * r) - dc) << SH.s0;
+        const int32_t hi = (pb.at(2u * o0 + 4u * k + 2u, rb * r) - dc) << SH.s0;
         rp[r][k] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
       }
 
@@ -356,6 +364,12 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
       st_d[lane * kStageRow + k] = d;
       st_e[lane * kStageRow + k] = e;
       st_l[lane * kStageRow + k] = l;
+      if constexpr (ts_lane_loopc<W, H>()) {
+        // the column outputs become loop-carried: the row arms stay in the
+        // type loop and read co in place
+#pragma unroll
+        for (int r = 0; r < H; r++) opaque(co[r]);
+      }
     }
   }
   __syncthreads();
@@ -471,15 +485,17 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
   const int bi = b < n_blocks ? (int)b : 0;  // past the end: block 0, never copied out
   const int x0 = xy[2 * bi], y0 = xy[2 * bi + 1];
   const uint32_t o0 = (uint32_t)y0 * (uint32_t)stride + (uint32_t)x0;
+  const PlaneBuf pb(plane);
+  const uint32_t rb = 2u * (uint32_t)stride;  // row pitch in bytes
 
   // DC prediction from the source neighbours (block's H lanes reduce)
   uint32_t sa = 0, sl = 0;
   if (y0 > 0) {
 #pragma unroll
     for (int k = 0; k < NC; k++)
-      if (r + H * k < W) sa += plane[o0 - (uint32_t)stride + (uint32_t)(r + H * k)];
+      if (r + H * k < W) sa += (uint32_t)pb.at(2u * (o0 - (uint32_t)stride + (uint32_t)(r + H * k)), 0);
   }
-  if (x0 > 0) sl = plane[o0 + (uint32_t)(r * stride) - 1u];
+  if (x0 > 0) sl = (uint32_t)pb.at(2u * (o0 + (uint32_t)(r * stride) - 1u), 0);
   sa = (uint32_t)group_sum_i32<H>((int32_t)sa);
   sl = (uint32_t)group_sum_i32<H>((int32_t)sl);
   int32_t dc;
@@ -495,9 +511,8 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
     const int c = (r + H * k) % W;  // lanes without a column (W < H) read a valid one, never stored
 #pragma unroll
     for (int i = 0; i < H / 2; i++) {
-      const uint32_t o = o0 + (uint32_t)(2 * i * stride + c);
-      const int32_t lo = ((int32_t)plane[o] - dc) << SH.s0;
-      const int32_t hi = ((int32_t)plane[o + (uint32_t)stride] - dc) << SH.s0;
+      const int32_t lo = (pb.at(2u * (o0 + (uint32_t)c), rb * (2 * i)) - dc) << SH.s0;
+      const int32_t hi = (pb.at(2u * (o0 + (uint32_t)c), rb * (2 * i + 1)) - dc) << SH.s0;
       rp[k][i] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
     }
   }
@@ -524,6 +539,7 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
     else if (ck == 2) ts_row_cols<W, H, 2>(rp, r, tile);
     else ts_row_cols<W, H, 3>(rp, r, tile);
     wave_sync();
+    const int32_t* src = tile + r * LS;
     const int i_end = (int)(vi >> 16);
     for (int i = (int)((vi >> 8) & 0xff); i < i_end; i++) {
       const uint32_t ti = plan.lane_type[i];
@@ -533,7 +549,6 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
       for (int k = 0; k < W / 4; k++) pc[k] = cls == 0 ? pw[0][k] : (cls == 1 ? pw[1][k] : pw[2][k]);
       uint64_t dist = 0;
       int32_t m = -1, lsum = 0;
-      const int32_t* src = tile + r * LS;
       if (rk == 0) ts_row_arm<W, H, 0, BD, Q24>(src, pc, r, qf, dist, m, lsum);
       else if (rk == 1) ts_row_arm<W, H, 1, BD, Q24>(src, pc, r, qf, dist, m, lsum);
       else if (rk == 2) ts_row_arm<W, H, 2, BD, Q24>(src, pc, r, qf, dist, m, lsum);
@@ -547,6 +562,8 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
         st_e[wv][g * kStageRow + k] = m + 1;
         st_l[wv][g * kStageRow + k] = (uint32_t)lsum;
       }
+      // the row's address becomes loop-carried: its loads and the row arms
+      // stay in the type loop instead of being hoisted out of it, all four
     }
   }
   wave_sync();

@@ -104,6 +104,20 @@ __device__ __forceinline__ int32_t rshift_round32(int32_t v, int bit) {
   return bit == 0 ? v : (v + (1 << (bit - 1))) >> bit;
 }
 
+// Raw buffer view of a u16 plane: loads take a 32-bit per-lane byte offset,
+// a wave-uniform byte offset and an immediate, so they need no 64-bit
+// address arithmetic.  Offsets must stay below 2^32 bytes (planes of
+// < 2^31 samples) and are non-negative.
+struct PlaneBuf {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ explicit PlaneBuf(const uint16_t* p)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(p), (short)0, (int)0xffffffff, 0x00020000)) {}
+  // sample at byte offset vbyte (per lane) + sbyte (uniform)
+  __device__ __forceinline__ int32_t at(uint32_t vbyte, uint32_t sbyte) const {
+    return (int32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)vbyte, (int)sbyte, 0);
+  }
+};
+
 __device__ __forceinline__ int32_t rshift64r(int64_t v, int bit) {
   return (int32_t)((v + ((int64_t)1 << (bit - 1))) >> bit);
 }
