@@ -42,7 +42,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, 'zenav1-svt_amd')
 
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_TOPS = 39.3  # 256 CU x 64 int32 lanes/clk x 2.4 GHz
+# int32 VALU peak in the op-count model's units: 256 CU x 64 lanes x 2.4 GHz
+# instructions, 2 ops each at best (the fused 3-operand forms the kernels
+# issue, v_mad_*24 / v_add3 / v_lshl_add / v_max3, are 2 model ops, as an
+# FMA is 2 flops).  The hardware view, VALU instructions issued per lane
+# and clock, comes from the committed PMC counts (valu_issue_frac).
+VALU_ISSUE_PEAK = 39.3e12
+VALU_PEAK_TOPS = 78.6
 
 
 def _load(name, path):
@@ -58,12 +64,13 @@ ROW_KIND = [0, 0, 1, 1, 0, 2, 2, 2, 1, 3, 3, 0, 3, 1, 3, 2]
 UD_FLIP = {4, 6, 8, 14}
 
 
-def block_ops(ts_mod, tx_size, types, opsd):
+def block_ops(ts_mod, tx_size, types, opsd, n_q=1):
     """Algorithmic integer ops of evaluating one block with every type in
-    `types`: the DC prediction and residual once, each distinct column
-    transform (kind, up-down flip) once, every type's row transforms, round
-    shifts, fp quantize, distortion, eob and level sum.  1D transform counts
-    are the straight-line op counts the generator records."""
+    `types` at n_q quantizers: the DC prediction and residual once, each
+    distinct column transform (kind, up-down flip) once, every type's row
+    transforms and round shifts once, and per quantizer the fp quantize,
+    distortion, eob and level sum.  1D transform counts are the
+    straight-line op counts the generator records."""
     w, h = ts_mod.TX_DIMS[tx_size]
 
     def one(kind, n):
@@ -76,9 +83,9 @@ def block_ops(ts_mod, tx_size, types, opsd):
     for t in types:
         ops += h * one(ROW_KIND[t], w)
         ops += n * (2 + rect)                   # round shift (+ sqrt2 scale)
-        ops += n * 12                           # fp quantize + dequant (abs, compare, add, mul, shift, sign x2)
-        ops += n * 3                            # distortion (sub, mul, add)
-        ops += n * 3                            # eob / level sum
+        ops += n_q * n * 12                     # fp quantize + dequant (abs, compare, add, mul, shift, sign x2)
+        ops += n_q * n * 3                      # distortion (sub, mul, add)
+        ops += n_q * n * 3                      # eob / level sum
     return ops
 
 
@@ -91,6 +98,18 @@ SGR_OPS_R1 = 9 * 2 + 10 + 25
 SGR_OPS_PROJ = 13
 SGR_OPS_PER_PX = (10 * (SGR_OPS_R2 + SGR_OPS_R1 + SGR_OPS_PROJ) + 4 * (SGR_OPS_R1 + SGR_OPS_PROJ) +
                   2 * (SGR_OPS_R2 + SGR_OPS_PROJ))
+
+
+def measured_valu_insts(kernel_prefix):
+    """VALU instructions per launch of a kernel (SQ_INSTS_VALU, wave-level,
+    from the committed PMC summary), or None."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
+    if not os.path.exists(path):
+        return None
+    for name, k in json.load(open(path)).get('kernels', {}).items():
+        if name.startswith(kernel_prefix) and 'valu_insts' in k:
+            return k['valu_insts']
+    return None
 
 
 def measured_traffic(kernel_prefix):
@@ -429,16 +448,23 @@ def main():
     mp = S * S / 1e6 * len(qps)
     value = mp / (ms_per_step / 1000.0)
 
-    # dominant kernel: the tx size with the largest event time
+    # dominant kernel: the tx size with the largest event time.  The lane /
+    # row forms evaluate both quantizers in one launch (transforms shared);
+    # their algorithmic work is still counted per quantizer, as the
+    # defining computation does it.
     dom = max(per_size_ms, key=per_size_ms.get)
     types = tsm.legal_types(dom)
     w, h = tsm.TX_DIMS[dom]
     nblk = (S // w) * (S // h)
-    launches = args.steps * len(qps)
-    avg_launch_s = per_size_ms[dom] / 1000.0 / launches
-    ops_launch = nblk * block_ops(tsm, dom, types, opsd)
-    bytes_launch = nblk * (w * h * 2 + (w + h) * 2 + len(types) * 16)
+    form = tsm.kernel_form(dom)
+    n_launch = tsm.launches(dom, len(qps))
+    q_per_launch = len(qps) // n_launch
+    avg_launch_s = per_size_ms[dom] / 1000.0 / (args.steps * n_launch)
+    ops_launch = nblk * block_ops(tsm, dom, types, opsd, q_per_launch)
+    ops_launch_per_candidate = nblk * block_ops(tsm, dom, types, opsd) * q_per_launch
+    bytes_launch = nblk * (w * h * 2 + (w + h) * 2 + len(types) * 16 * q_per_launch)
     achieved_tops = ops_launch / avg_launch_s / 1e12
+    kname = {'lane': 'k_txsearch_lane', 'rows': 'k_txsearch_rows', 'group': 'k_txsearch'}[form]
 
     out = {
         'metric': 'MP/s (luma megapixels x qps) through batched candidate RD, byte-identical to the oracle',
@@ -447,13 +473,19 @@ def main():
         'dtype': 'int32', 'data': 'synthetic',
         'config': {'workload': 'config2 tx-type search: %dx%d 10-bit luma, qp 20+40, 19 tx sizes x legal '
                                'types, fused DC-pred/fwd/fold/fp-quant/dist' % (S, S),
-                   'candidates_per_step': n_cand, 'launches_per_step': 19 * len(qps)},
+                   'candidates_per_step': n_cand,
+                   'launches_per_step': sum(tsm.launches(t, len(qps)) for t in range(19))},
         'per_size_ms_per_step': {str(tsm.TX_DIMS[t]): round(per_size_ms[t] / args.steps, 4) for t in range(19)},
-        'roofline': {'bound': 'valu', 'kernel': 'k_txsearch<%d,%d>' % (w, h), 'achieved': round(achieved_tops, 3),
+        'roofline': {'bound': 'valu', 'kernel': '%s<%d,%d>' % (kname, w, h), 'achieved': round(achieved_tops, 3),
                      'peak': VALU_PEAK_TOPS, 'unit': 'Tops/s', 'frac': round(achieved_tops / VALU_PEAK_TOPS, 4),
-                     'traffic': measured_traffic('k_txsearch<%d, %d, ' % (w, h)),
+                     'traffic': measured_traffic('%s<%d, %d, ' % (kname, w, h)),
+                     'valu_issue_frac': (None if measured_valu_insts('%s<%d, %d, ' % (kname, w, h)) is None else
+                                         round(measured_valu_insts('%s<%d, %d, ' % (kname, w, h)) * 64 /
+                                               avg_launch_s / VALU_ISSUE_PEAK, 4)),
+                     'quantizers_per_launch': q_per_launch,
                      'avg_launch_us': round(avg_launch_s * 1e6, 2),
                      'algorithmic_ops_per_launch': int(ops_launch),
+                     'ops_per_launch_transform_per_quantizer': int(ops_launch_per_candidate),
                      'hbm_algorithmic_GBs': round(bytes_launch / avg_launch_s / 1e9, 2),
                      'hbm_peak_GBs': HBM_PEAK_GBS},
     }

@@ -241,6 +241,15 @@ int zav1_scan_order(int tx_size, int scan_class, uint16_t *scan);
 int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size, const int32_t *d_xy,
                         int n_blocks, const uint8_t *types, int n_types, int qindex, uint64_t *d_dist,
                         int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
+/* The same search at n_q quantizers (qindices[q], 1 <= n_q <= 8): each
+ * block's transforms are computed once and quantized at every qindex.
+ * Outputs per (q, block j, type k) at q*n_blocks*n_types + j*n_types + k;
+ * identical to n_q calls of zav1_txsearch_batch (the evaluation of one
+ * candidate set at several quantizers, as a quantizer or rate-control
+ * search asks for). */
+int zav1_txsearch_batch_q(const uint16_t *d_plane, int stride, int bd, int tx_size, const int32_t *d_xy,
+                          int n_blocks, const uint8_t *types, int n_types, const int *qindices, int n_q,
+                          uint64_t *d_dist, int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
 
 /* Fused transform units with a given prediction: for every job (one unit of
  * tx_size at (x, y) of d_src, its w*h u16 prediction at d_pred + pred_off,

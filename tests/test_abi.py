@@ -49,6 +49,11 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     assert L.zav1_build_quant_row(300, 8, ctypes.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
     assert L.zav1_build_quant_row(10, 12, ctypes.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
     assert L.zav1_fwd_txfm2d_batch(None, None, None, 25, 1, None) == zav1.ZAV1_EINVAL
+    qs = (ctypes.c_int * 3)(10, 300, 20)
+    assert L.zav1_txsearch_batch_q(None, 64, 10, 0, None, 1, ctypes.byref(ctypes.c_uint8(0)), 1, qs, 3, None, None,
+                                   None, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_txsearch_batch_q(None, 64, 10, 0, None, 1, ctypes.byref(ctypes.c_uint8(0)), 1, qs, 9, None, None,
+                                   None, None) == zav1.ZAV1_EINVAL
     # loop-filter entry points validate before touching the device
     g = zav1.DeblockGeom()
     assert L.zav1_deblock_plane(None, 64, 64, 64, 8, 0, 0, 10, 10, 0, ctypes.byref(g), None) == zav1.ZAV1_EINVAL

@@ -96,3 +96,35 @@ def test_txsearch_type_orders_and_extremes(gpu, oracle, bd):
                 assert np.array_equal(dd.to_host(), rd), (tx_size, tl, qi)
                 assert np.array_equal(de.to_host(), re), (tx_size, tl, qi)
                 assert np.array_equal(dl.to_host(), rl), (tx_size, tl, qi)
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_txsearch_multi_quantizer_matches_single(gpu, oracle, bd):
+    """The several-quantizer entry point (transforms shared, one launch per
+    pair of quantizers) equals one call per qindex and the oracle, including
+    qindex 0 (the 32-bit level form at 10 bits) mixed with others and an odd
+    quantizer count."""
+    zav1 = gpu
+    synth, ts = _mod('synth'), _mod('txsearch')
+    plane = synth.config2_luma(128, 128, seed=77 + bd)
+    if bd == 8:
+        plane = (plane >> 2).astype(np.uint16)
+    qis = [0, 60, 255]
+    d_plane = zav1.DeviceArray.from_host(plane)
+    for tx_size in range(19):
+        xy = ts.tile_xy(128, 128, tx_size)
+        types = np.array(ts.legal_types(tx_size), np.uint8)
+        d_xy = zav1.DeviceArray.from_host(xy)
+        n = xy.shape[0] * types.size
+        dd, de, dl = (zav1.DeviceArray(3 * n, np.uint64), zav1.DeviceArray(3 * n, np.int32),
+                      zav1.DeviceArray(3 * n, np.uint32))
+        zav1.txsearch_batch_q(d_plane, 128, bd, tx_size, d_xy, xy.shape[0], types, qis, dd, de, dl)
+        md, me, ml = dd.to_host(), de.to_host(), dl.to_host()
+        for qp, qi in enumerate(qis):
+            sd, se, sl = zav1.DeviceArray(n, np.uint64), zav1.DeviceArray(n, np.int32), zav1.DeviceArray(n, np.uint32)
+            zav1.txsearch_batch(d_plane, 128, bd, tx_size, d_xy, xy.shape[0], types, qi, sd, se, sl)
+            rd, re, rl = oracle.txsearch_plane(plane, bd, tx_size, xy, types, qi)
+            sl_ = slice(qp * n, (qp + 1) * n)
+            assert np.array_equal(md[sl_], rd) and np.array_equal(sd.to_host(), rd), (tx_size, qi)
+            assert np.array_equal(me[sl_], re) and np.array_equal(se.to_host(), re), (tx_size, qi)
+            assert np.array_equal(ml[sl_], rl) and np.array_equal(sl.to_host(), rl), (tx_size, qi)

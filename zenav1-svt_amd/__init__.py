@@ -122,6 +122,7 @@ EXPORTS = {
     'zav1_satd_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_scan_order': [_i32, _i32, _vp],
     'zav1_txsearch_batch': [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp],
+    'zav1_txsearch_batch_q': [_vp, _i32, _i32, _i32, _vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _vp],
     'zav1_intra_pred_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp],
     'zav1_cfl_batch': [_vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
     'zav1_txunit_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
@@ -302,6 +303,16 @@ def txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, types, qindex, 
     t = np.ascontiguousarray(types, dtype=np.uint8)
     check(lib().zav1_txsearch_batch(d_plane.ptr, stride, bd, tx_size, d_xy.ptr, n_blocks, t.ctypes.data, t.size,
                                     qindex, d_dist.ptr, d_eob.ptr, d_lsum.ptr, stream), 'txsearch_batch')
+
+
+def txsearch_batch_q(d_plane, stride, bd, tx_size, d_xy, n_blocks, types, qindices, d_dist, d_eob, d_lsum,
+                     stream=None):
+    """The search at several qindices; outputs [q][block][type]."""
+    t = np.ascontiguousarray(types, dtype=np.uint8)
+    q = np.ascontiguousarray(qindices, dtype=np.int32)
+    check(lib().zav1_txsearch_batch_q(d_plane.ptr, stride, bd, tx_size, d_xy.ptr, n_blocks, t.ctypes.data, t.size,
+                                      q.ctypes.data, q.size, d_dist.ptr, d_eob.ptr, d_lsum.ptr, stream),
+          'txsearch_batch_q')
 
 
 def intra_pred_batch(d_plane, stride, bd, d_jobs, n_jobs, d_out, stream=None):

@@ -867,12 +867,14 @@ int zav1_scan_order(int tx_size, int scan_class, uint16_t* scan) {
   return (w > 32 ? 32 : w) * (h > 32 ? 32 : h);
 }
 
-int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size, const int32_t* d_xy,
-                        int n_blocks, const uint8_t* types, int n_types, int qindex, uint64_t* d_dist,
-                        int32_t* d_eob, uint32_t* d_lsum, zav1_stream_t s) {
+int zav1_txsearch_batch_q(const uint16_t* d_plane, int stride, int bd, int tx_size, const int32_t* d_xy,
+                          int n_blocks, const uint8_t* types, int n_types, const int* qindices, int n_q,
+                          uint64_t* d_dist, int32_t* d_eob, uint32_t* d_lsum, zav1_stream_t s) {
   if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_blocks < 0 || n_types < 1 || n_types > 16 ||
-      !types || qindex < 0 || qindex > 255 || stride <= 0)
+      !types || !qindices || n_q < 1 || n_q > 8 || stride <= 0)
     return ZAV1_EINVAL;
+  for (int q = 0; q < n_q; q++)
+    if (qindices[q] < 0 || qindices[q] > 255) return ZAV1_EINVAL;
   zav1::TypeList tl{};
   tl.n = n_types;
   for (int i = 0; i < n_types; i++) {
@@ -885,12 +887,22 @@ int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size
   if (rc) return rc;
   rc = iscan_table(&iscan);
   if (rc) return rc;
-  zav1::QuantRow hr;
-  build_row(qindex, bd, &hr);
-  const bool q24 = bd == 8 || (hr.quant_fp[0] < 16384 && hr.quant_fp[1] < 16384);
-  ZCHECK(zav1::txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, tl, rows + (bd == 10 ? 256 : 0) + qindex,
-                              q24, iscan, d_dist, d_eob, d_lsum, S(s)));
+  const zav1::QuantRow* dev_rows[8];
+  zav1::QuantRow host_rows[8];
+  for (int q = 0; q < n_q; q++) {
+    dev_rows[q] = rows + (bd == 10 ? 256 : 0) + qindices[q];
+    build_row(qindices[q], bd, &host_rows[q]);
+  }
+  ZCHECK(zav1::txsearch_batch(d_plane, stride, bd, tx_size, d_xy, n_blocks, tl, dev_rows, host_rows, n_q, iscan,
+                              d_dist, d_eob, d_lsum, S(s)));
   return ZAV1_OK;
+}
+
+int zav1_txsearch_batch(const uint16_t* d_plane, int stride, int bd, int tx_size, const int32_t* d_xy,
+                        int n_blocks, const uint8_t* types, int n_types, int qindex, uint64_t* d_dist,
+                        int32_t* d_eob, uint32_t* d_lsum, zav1_stream_t s) {
+  return zav1_txsearch_batch_q(d_plane, stride, bd, tx_size, d_xy, n_blocks, types, n_types, &qindex, 1, d_dist,
+                               d_eob, d_lsum, s);
 }
 
 int zav1_txunit_batch(const void* d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,

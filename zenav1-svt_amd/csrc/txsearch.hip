@@ -226,19 +226,21 @@ __device__ __forceinline__ void ts_lane_cols(const uint32_t (&rp)[H][W / 2], int
   }
 }
 
-template <int W, int H, int RK, int BD, bool Q24>
+template <int W, int H, int RK, int BD, bool Q24, int NQ>
 __device__ __forceinline__ void ts_lane_rows(const int32_t (&co)[H][W], const uint32_t* __restrict__ pos,
-                                             const QuantFpL& qf, uint64_t& dist_out, int32_t& eob_out,
-                                             uint32_t& lsum_out) {
+                                             const QuantFpSet& qs, uint64_t (&dist_out)[NQ], int32_t (&eob_out)[NQ],
+                                             uint32_t (&lsum_out)[NQ]) {
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr long long kBCol = fwd_col_bound(SH.s0);
   constexpr long long kBRow = fwd_row_bound(H, CBC, SH.s1, kBCol);
   constexpr int LSC = tx_log_scale(W, H);
   constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
-  uint64_t dist = 0;
-  int32_t eob = 0, lsum = 0;
-  uint32_t sq4 = 0;
+  uint64_t dist[NQ];
+  int32_t eob[NQ], lsum[NQ];
+  uint32_t sq4[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) dist[q] = 0, eob[q] = 0, lsum[q] = 0, sq4[q] = 0;
   // the type's scan positions, read before the transform and held in scalar
   // registers: a select on a freshly loaded value would otherwise become a
   // branch around the load, per coefficient
@@ -261,24 +263,30 @@ __device__ __forceinline__ void ts_lane_rows(const int32_t (&co)[H][W], const ui
       // 2:1 rectangles: x NewSqrt2 / 2^12 (|x| < 2^18, the product fits 32 bits)
       if constexpr (RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
       const int32_t a = x < 0 ? -x : x;
-      int32_t e;  // a - |dequantized|, |e| < 2^15: four squares sum in 32 bits
-      const int32_t q = quant_fpl<BD, Q24, true>(a, (r | c) != 0, qf, &e);
-      lsum += q;
-      // scan position + 1 of this coefficient (<= 64, scalar); min(q << 7, p)
+      // scan position + 1 of this coefficient (<= 64, scalar); min(level << 7, p)
       // is p for a nonzero level, else 0
       const int32_t p = (int32_t)((pw[(r * W + c) >> 2] >> (8 * ((r * W + c) & 3))) & 0xffu);
-      eob = max(eob, min(q << 7, p));
-      sq4 = (uint32_t)mad_i24(e, e, (int32_t)sq4);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        int32_t e;  // a - |dequantized|, |e| < 2^15: four squares sum in 32 bits
+        const int32_t l = quant_fpl<BD, Q24, true>(a, (r | c) != 0, qs.q[q], &e);
+        lsum[q] += l;
+        eob[q] = max(eob[q], min(l << 7, p));
+        sq4[q] = (uint32_t)mad_i24(e, e, (int32_t)sq4[q]);
+      }
       if (((r * W + c) & 3) == 3) {
-        dist += sq4;
-        sq4 = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) dist[q] += sq4[q], sq4[q] = 0;
       }
     }
   }
   constexpr int SHF = (1 - LSC) * 2;
-  dist_out = SHF < 0 ? (dist << (-SHF)) : (dist >> SHF);
-  eob_out = eob;
-  lsum_out = (uint32_t)lsum;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    dist_out[q] = SHF < 0 ? (dist[q] << (-SHF)) : (dist[q] >> SHF);
+    eob_out[q] = eob[q];
+    lsum_out[q] = (uint32_t)lsum[q];
+  }
 }
 
 constexpr int kLaneBlocks = 64;  // blocks per workgroup: one per lane of each wave
@@ -287,17 +295,17 @@ constexpr int kStageRow = 17;    // staging row: 16 types + 1 (spreads the block
 // Workgroup of 4 waves over the same 64 blocks; wave v takes column variant
 // v (all of its types) for every block, lane l block l.  Results meet in a
 // per-workgroup staging block and leave as contiguous rows.
-template <int W, int H, int BD, bool Q24>
+template <int W, int H, int BD, bool Q24, int NQ>
 __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restrict__ plane, int stride, int bd,
                                                        const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
-                                                       const QuantRow* __restrict__ row_p, LaneScan scan,
+                                                       QuantFpSet qs, LaneScan scan,
                                                        uint64_t* __restrict__ out_dist,
                                                        int32_t* __restrict__ out_eob,
                                                        uint32_t* __restrict__ out_lsum) {
   static_assert(W * H % 4 == 0 && W <= 16 && H <= 16, "lane-per-block sizes");
-  __shared__ uint64_t st_d[kLaneBlocks * kStageRow];
-  __shared__ int32_t st_e[kLaneBlocks * kStageRow];
-  __shared__ uint32_t st_l[kLaneBlocks * kStageRow];
+  __shared__ uint64_t st_d[NQ][kLaneBlocks * kStageRow];
+  __shared__ int32_t st_e[NQ][kLaneBlocks * kStageRow];
+  __shared__ uint32_t st_l[NQ][kLaneBlocks * kStageRow];
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr int LSC = tx_log_scale(W, H);
   const int lane = threadIdx.x & 63;
@@ -340,7 +348,6 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
         rp[r][k] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
       }
 
-    const QuantFpL qf = make_quant_fpl<BD, Q24>(*row_p);
     const uint32_t vi = plan.lane_var[v];
     const int ck = (int)(vi & 0xff);
     int32_t co[H][W];
@@ -352,18 +359,21 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
     for (int i = (int)((vi >> 8) & 0xff); i < i_end; i++) {
       const uint32_t ti = plan.lane_type[i];
       const uint32_t* pos = scan.pos[(ti >> 16) & 0xff];
-      uint64_t d;
-      int32_t e;
-      uint32_t l;
+      uint64_t d[NQ];
+      int32_t e[NQ];
+      uint32_t l[NQ];
       const int rk = (int)((ti >> 8) & 0xff);
-      if (rk == 0) ts_lane_rows<W, H, 0, BD, Q24>(co, pos, qf, d, e, l);
-      else if (rk == 1) ts_lane_rows<W, H, 1, BD, Q24>(co, pos, qf, d, e, l);
-      else if (rk == 2) ts_lane_rows<W, H, 2, BD, Q24>(co, pos, qf, d, e, l);
-      else ts_lane_rows<W, H, 3, BD, Q24>(co, pos, qf, d, e, l);
+      if (rk == 0) ts_lane_rows<W, H, 0, BD, Q24, NQ>(co, pos, qs, d, e, l);
+      else if (rk == 1) ts_lane_rows<W, H, 1, BD, Q24, NQ>(co, pos, qs, d, e, l);
+      else if (rk == 2) ts_lane_rows<W, H, 2, BD, Q24, NQ>(co, pos, qs, d, e, l);
+      else ts_lane_rows<W, H, 3, BD, Q24, NQ>(co, pos, qs, d, e, l);
       const int k = (int)(ti & 0xff);
-      st_d[lane * kStageRow + k] = d;
-      st_e[lane * kStageRow + k] = e;
-      st_l[lane * kStageRow + k] = l;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        st_d[q][lane * kStageRow + k] = d[q];
+        st_e[q][lane * kStageRow + k] = e[q];
+        st_l[q][lane * kStageRow + k] = l[q];
+      }
       if constexpr (ts_lane_loopc<W, H>()) {
         // the column outputs become loop-carried: the row arms stay in the
         // type loop and read co in place
@@ -374,17 +384,22 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
   }
   __syncthreads();
 
-  // The 64 blocks own one contiguous range of each output array: copy it out
-  // from the staging rows, consecutive threads on consecutive entries (entry
-  // j -> block j / ntyp by a multiply-shift, exact for j < 1024, ntyp <= 16).
+  // The 64 blocks own one contiguous range of each output array (per
+  // quantizer): copy it out from the staging rows, consecutive threads on
+  // consecutive entries (entry j -> block j / ntyp by a multiply-shift,
+  // exact for j < 1024, ntyp <= 16).
   const long nb = n_blocks - b0 < kLaneBlocks ? n_blocks - b0 : kLaneBlocks;
   const uint32_t mg = (65536u + (uint32_t)ntyp - 1u) / (uint32_t)ntyp;
   for (int j = threadIdx.x; j < nb * ntyp; j += blockDim.x) {
     const int blk = (int)(((uint32_t)j * mg) >> 16), k = j - blk * ntyp;
     const int src = blk * kStageRow + k;
-    out_dist[b0 * ntyp + j] = st_d[src];
-    out_eob[b0 * ntyp + j] = st_e[src];
-    out_lsum[b0 * ntyp + j] = st_l[src];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const long o = (long)q * n_blocks * ntyp + b0 * ntyp + j;
+      out_dist[o] = st_d[q][src];
+      out_eob[o] = st_e[q][src];
+      out_lsum[o] = st_l[q][src];
+    }
   }
 }
 
@@ -396,36 +411,42 @@ __global__ __launch_bounds__(256) void k_txsearch_lane(const uint16_t* __restric
 // each lane transforms and quantizes its row with its scan positions
 // (preloaded bytes for the three classes), and the block's H lanes reduce.
 // A block lives in one wave, so only wave-level ordering is needed.
-template <int W, int H, int RK, int BD, bool Q24>
+template <int W, int H, int RK, int BD, bool Q24, int NQ>
 __device__ __forceinline__ void ts_row_arm(const int32_t* __restrict__ src, const uint32_t (&pc)[W / 4], int r,
-                                           const QuantFpL& qf, uint64_t& dist, int32_t& m, int32_t& lsum) {
+                                           const QuantFpSet& qs, uint64_t (&dist)[NQ], int32_t (&m)[NQ],
+                                           int32_t (&lsum)[NQ]) {
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr long long kBCol = fwd_col_bound(SH.s0);
   constexpr long long kBRow = fwd_row_bound(H, CBC, SH.s1, kBCol);
-  constexpr int LSC = tx_log_scale(W, H);
   constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
   int32_t v[W];
 #pragma unroll
   for (int c = 0; c < W; c++) v[c] = src[RK == 2 ? W - 1 - c : c];  // FLIPADST: left-right flip
   opaque(v);
   fwd1d<W, CBR, kBRow>(v, RK);
-  uint32_t sq4 = 0;
+  uint32_t sq4[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) sq4[q] = 0;
 #pragma unroll
   for (int c = 0; c < W; c++) {
     int32_t x = rshift_round32(v[c], SH.s2);
     if constexpr (RECT2) x = (__mul24(x, 5793) + 2048) >> 12;  // |x| < 2^18
     const int32_t a = x < 0 ? -x : x;
-    int32_t e;
-    // column 0: DC or AC by the lane's row; other columns AC (uniform)
-    const int32_t q = c == 0 ? quant_fpl<BD, Q24, false>(a, r != 0, qf, &e) : quant_fpl<BD, Q24, true>(a, 1, qf, &e);
-    lsum += q;
     const int32_t p = (int32_t)((pc[c >> 2] >> (8 * (c & 3))) & 0xffu);  // scan index
-    m = max(m, q ? p : -1);
-    sq4 = (uint32_t)mad_i24(e, e, (int32_t)sq4);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      int32_t e;
+      // column 0: DC or AC by the lane's row; other columns AC (uniform)
+      const int32_t l = c == 0 ? quant_fpl<BD, Q24, false>(a, r != 0, qs.q[q], &e)
+                               : quant_fpl<BD, Q24, true>(a, 1, qs.q[q], &e);
+      lsum[q] += l;
+      m[q] = max(m[q], l ? p : -1);
+      sq4[q] = (uint32_t)mad_i24(e, e, (int32_t)sq4[q]);
+    }
     if ((c & 3) == 3) {
-      dist += sq4;
-      sq4 = 0;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) dist[q] += sq4[q], sq4[q] = 0;
     }
   }
 }
@@ -459,10 +480,10 @@ __device__ __forceinline__ void ts_row_cols(const uint32_t (&rp)[(W >= H ? W / H
   }
 }
 
-template <int W, int H, int BD, bool Q24>
+template <int W, int H, int BD, bool Q24, int NQ>
 __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restrict__ plane, int stride, int bd,
                                                        const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
-                                                       const QuantRow* __restrict__ row_p,
+                                                       QuantFpSet qs,
                                                        const uint16_t* __restrict__ iscan_all, int tx_size,
                                                        uint64_t* __restrict__ out_dist,
                                                        int32_t* __restrict__ out_eob,
@@ -474,9 +495,9 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr int LSC = tx_log_scale(W, H);
   __shared__ int32_t tiles[4][BPW * H * LS];
-  __shared__ uint64_t st_d[4][BPW * kStageRow];
-  __shared__ int32_t st_e[4][BPW * kStageRow];
-  __shared__ uint32_t st_l[4][BPW * kStageRow];
+  __shared__ uint64_t st_d[NQ][4][BPW * kStageRow];
+  __shared__ int32_t st_e[NQ][4][BPW * kStageRow];
+  __shared__ uint32_t st_l[NQ][4][BPW * kStageRow];
 
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int g = lane / H, r = lane - g * H;  // block within the wave, row / column slot
@@ -527,7 +548,6 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
                    (uint32_t)is[4 * k + 3] << 24;
   }
 
-  const QuantFpL qf = make_quant_fpl<BD, Q24>(*row_p);
   const int ntyp = plan.n_types;
   int32_t* tile = tiles[wv] + g * H * LS;
   for (int v = 0; v < plan.n_var; v++) {
@@ -547,20 +567,25 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
       uint32_t pc[W / 4];
 #pragma unroll
       for (int k = 0; k < W / 4; k++) pc[k] = cls == 0 ? pw[0][k] : (cls == 1 ? pw[1][k] : pw[2][k]);
-      uint64_t dist = 0;
-      int32_t m = -1, lsum = 0;
-      if (rk == 0) ts_row_arm<W, H, 0, BD, Q24>(src, pc, r, qf, dist, m, lsum);
-      else if (rk == 1) ts_row_arm<W, H, 1, BD, Q24>(src, pc, r, qf, dist, m, lsum);
-      else if (rk == 2) ts_row_arm<W, H, 2, BD, Q24>(src, pc, r, qf, dist, m, lsum);
-      else ts_row_arm<W, H, 3, BD, Q24>(src, pc, r, qf, dist, m, lsum);
-      dist = group_sum_u64<H>(dist);
-      m = group_max_i32<H>(m);
-      lsum = group_sum_i32<H>(lsum);
-      if (r == 0) {
-        const int k = (int)(ti & 0xff);
-        st_d[wv][g * kStageRow + k] = dist >> 2;  // (1 - log scale) * 2 with log scale 0
-        st_e[wv][g * kStageRow + k] = m + 1;
-        st_l[wv][g * kStageRow + k] = (uint32_t)lsum;
+      uint64_t dist[NQ];
+      int32_t m[NQ], lsum[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) dist[q] = 0, m[q] = -1, lsum[q] = 0;
+      if (rk == 0) ts_row_arm<W, H, 0, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+      else if (rk == 1) ts_row_arm<W, H, 1, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+      else if (rk == 2) ts_row_arm<W, H, 2, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+      else ts_row_arm<W, H, 3, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+      const int k = (int)(ti & 0xff);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const uint64_t dq = group_sum_u64<H>(dist[q]);
+        const int32_t mq = group_max_i32<H>(m[q]);
+        const int32_t lq = group_sum_i32<H>(lsum[q]);
+        if (r == 0) {
+          st_d[q][wv][g * kStageRow + k] = dq >> 2;  // (1 - log scale) * 2 with log scale 0
+          st_e[q][wv][g * kStageRow + k] = mq + 1;
+          st_l[q][wv][g * kStageRow + k] = (uint32_t)lq;
+        }
       }
       // the row's address becomes loop-carried: its loads and the row arms
       // stay in the type loop instead of being hoisted out of it, all four
@@ -574,9 +599,13 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
   for (int j = lane; j < nb * ntyp; j += 64) {
     const int blk = (int)(((uint32_t)j * mg) >> 16), k = j - blk * ntyp;
     const int src = blk * kStageRow + k;
-    out_dist[bw0 * ntyp + j] = st_d[wv][src];
-    out_eob[bw0 * ntyp + j] = st_e[wv][src];
-    out_lsum[bw0 * ntyp + j] = st_l[wv][src];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const long o = (long)q * n_blocks * ntyp + bw0 * ntyp + j;
+      out_dist[o] = st_d[q][wv][src];
+      out_eob[o] = st_e[q][wv][src];
+      out_lsum[o] = st_l[q][wv][src];
+    }
   }
   (void)bd;
 }
@@ -600,43 +629,66 @@ constexpr int ts_lanes(int w, int h) {
 }
 static TxPlan plan_by_variant(const TxPlan& a);
 
+// Fused (lane / row form) launch for quantizers [q0, q0 + nq) of the list,
+// nq <= 2, at the bit depth and level form of the rows.
+template <int W, int H, template <int, int, int, bool, int> class K, typename... Args>
+static void launch_fused(int bd, bool q24, int nq, dim3 grid, hipStream_t st, Args... args) {
+  if (bd == 8) {
+    if (nq == 2) hipLaunchKernelGGL((K<W, H, 8, true, 2>::fn), grid, dim3(256), 0, st, args...);
+    else hipLaunchKernelGGL((K<W, H, 8, true, 1>::fn), grid, dim3(256), 0, st, args...);
+  } else if (q24) {
+    if (nq == 2) hipLaunchKernelGGL((K<W, H, 10, true, 2>::fn), grid, dim3(256), 0, st, args...);
+    else hipLaunchKernelGGL((K<W, H, 10, true, 1>::fn), grid, dim3(256), 0, st, args...);
+  } else {
+    if (nq == 2) hipLaunchKernelGGL((K<W, H, 10, false, 2>::fn), grid, dim3(256), 0, st, args...);
+    else hipLaunchKernelGGL((K<W, H, 10, false, 1>::fn), grid, dim3(256), 0, st, args...);
+  }
+}
+template <int W, int H, int BD, bool Q24, int NQ>
+struct LaneK {
+  static constexpr auto fn = k_txsearch_lane<W, H, BD, Q24, NQ>;
+};
+template <int W, int H, int BD, bool Q24, int NQ>
+struct RowK {
+  static constexpr auto fn = k_txsearch_rows<W, H, BD, Q24, NQ>;
+};
+
 template <int S>
 static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int32_t* xy, int n_blocks,
-                            const TxPlan& plan, const QuantRow* row, bool q24, const uint16_t* iscan, uint64_t* d,
-                            int32_t* e, uint32_t* l, hipStream_t st) {
+                            const TxPlan& plan, const QuantRow* const* dev_rows, const QuantRow* host_rows, int n_q,
+                            const uint16_t* iscan, uint64_t* d, int32_t* e, uint32_t* l, hipStream_t st) {
   constexpr int W = tx_w(S), H = tx_h(S);
-  if constexpr (ts_lane_size(W, H)) {
+  const long per_q = (long)n_blocks * plan.n_types;
+  if constexpr (ts_lane_size(W, H) || ts_row_size(W, H)) {
     const TxPlan lp = plan_by_variant(plan);
     LaneScan sc{};
-    uint16_t order[1024];
-    for (int c = 0; c < 3; c++) {
-      build_scan(S, c, order);
-      for (int i = 0; i < W * H; i++) sc.pos[c][order[i] >> 2] |= (uint32_t)(i + 1) << (8 * (order[i] & 3));
+    if constexpr (ts_lane_size(W, H)) {
+      uint16_t order[1024];
+      for (int c = 0; c < 3; c++) {
+        build_scan(S, c, order);
+        for (int i = 0; i < W * H; i++) sc.pos[c][order[i] >> 2] |= (uint32_t)(i + 1) << (8 * (order[i] & 3));
+      }
     }
-    // one wave per column variant: 4 waves (the plans here have <= 4 variants)
-    const int grid = (n_blocks + kLaneBlocks - 1) / kLaneBlocks;
-    if (bd == 8)
-      hipLaunchKernelGGL((k_txsearch_lane<W, H, 8, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
-                         n_blocks, lp, row, sc, d, e, l);
-    else if (q24)
-      hipLaunchKernelGGL((k_txsearch_lane<W, H, 10, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
-                         n_blocks, lp, row, sc, d, e, l);
-    else
-      hipLaunchKernelGGL((k_txsearch_lane<W, H, 10, false>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
-                         n_blocks, lp, row, sc, d, e, l);
-    (void)iscan;
-  } else if constexpr (ts_row_size(W, H)) {
-    const TxPlan lp = plan_by_variant(plan);
-    const int grid = (n_blocks + 4 * (64 / H) - 1) / (4 * (64 / H));
-    if (bd == 8)
-      hipLaunchKernelGGL((k_txsearch_rows<W, H, 8, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
-                         n_blocks, lp, row, iscan, S, d, e, l);
-    else if (q24)
-      hipLaunchKernelGGL((k_txsearch_rows<W, H, 10, true>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
-                         n_blocks, lp, row, iscan, S, d, e, l);
-    else
-      hipLaunchKernelGGL((k_txsearch_rows<W, H, 10, false>), dim3(grid), dim3(256), 0, st, plane, stride, bd, xy,
-                         n_blocks, lp, row, iscan, S, d, e, l);
+    for (int q0 = 0; q0 < n_q; q0 += 2) {
+      const int nq = n_q - q0 < 2 ? n_q - q0 : 2;
+      bool q24 = true;  // the 24-bit level form (exact when quant_fp < 2^14 at 10 bits)
+      QuantFpSet qs{};
+      for (int i = 0; i < nq; i++)
+        q24 = q24 && (bd == 8 || (host_rows[q0 + i].quant_fp[0] < 16384 && host_rows[q0 + i].quant_fp[1] < 16384));
+      for (int i = 0; i < nq; i++) qs.q[i] = make_quant_fpl(host_rows[q0 + i], bd, q24);
+      uint64_t* dq = d + q0 * per_q;
+      int32_t* eq = e + q0 * per_q;
+      uint32_t* lq = l + q0 * per_q;
+      if constexpr (ts_lane_size(W, H)) {
+        // one wave per column variant: 4 waves (the plans here have <= 4 variants)
+        const dim3 grid((n_blocks + kLaneBlocks - 1) / kLaneBlocks);
+        launch_fused<W, H, LaneK>(bd, q24, nq, grid, st, plane, stride, bd, xy, n_blocks, lp, qs, sc, dq, eq, lq);
+      } else {
+        const dim3 grid((n_blocks + 4 * (64 / H) - 1) / (4 * (64 / H)));
+        launch_fused<W, H, RowK>(bd, q24, nq, grid, st, plane, stride, bd, xy, n_blocks, lp, qs, iscan, S, dq, eq,
+                                 lq);
+      }
+    }
   } else {
     constexpr int P = ts_lanes(W, H);
     using G = BlkGeom<W, H, P>;
@@ -645,12 +697,17 @@ static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int
     int groups = 256 / G::T;
     while (groups > 1 && groups * per_group > 65536) groups >>= 1;
     const int grid = (n_blocks + groups - 1) / groups;
-    if (bd == 8)
-      hipLaunchKernelGGL((k_txsearch<W, H, P, 8>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
-                         stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
-    else
-      hipLaunchKernelGGL((k_txsearch<W, H, P, 10>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
-                         stride, bd, xy, n_blocks, plan, row, iscan, S, d, e, l);
+    for (int q = 0; q < n_q; q++) {
+      if (bd == 8)
+        hipLaunchKernelGGL((k_txsearch<W, H, P, 8>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
+                           stride, bd, xy, n_blocks, plan, dev_rows[q], iscan, S, d + q * per_q, e + q * per_q,
+                           l + q * per_q);
+      else
+        hipLaunchKernelGGL((k_txsearch<W, H, P, 10>), dim3(grid), dim3(groups * G::T), groups * per_group, st,
+                           plane, stride, bd, xy, n_blocks, plan, dev_rows[q], iscan, S, d + q * per_q,
+                           e + q * per_q, l + q * per_q);
+    }
+    (void)host_rows;
   }
   return hipGetLastError();
 }
@@ -743,12 +800,14 @@ static TxPlan plan_by_variant(const TxPlan& a) {
 }
 
 hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
-                          int n_blocks, const TypeList& types, const QuantRow* row, bool q24, const uint16_t* iscan,
-                          uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st) {
+                          int n_blocks, const TypeList& types, const QuantRow* const* dev_rows,
+                          const QuantRow* host_rows, int n_q, const uint16_t* iscan, uint64_t* dist, int32_t* eob,
+                          uint32_t* lsum, hipStream_t st) {
   if (n_blocks <= 0 || types.n <= 0) return hipSuccess;
   const TxPlan plan = make_plan(types);
   switch (tx_size) {
-#define X(S) case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, plan, row, q24, iscan, dist, eob, lsum, st);
+#define X(S) \
+  case S: return launch_ts<S>(plane, stride, bd, xy, n_blocks, plan, dev_rows, host_rows, n_q, iscan, dist, eob, lsum, st);
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 #undef X
     default: return hipErrorInvalidValue;

@@ -105,11 +105,9 @@ struct QuantFpL {
   uint32_t rs[2], qt[2];
 };
 __host__ __device__ constexpr int quant_fpl_shift(int bd, bool q24) { return q24 ? (bd == 8 ? 8 : 6) : 0; }
-template <int BD, bool Q24>
-__device__ __forceinline__ QuantFpL make_quant_fpl(const QuantRow& t) {
-  constexpr int S = quant_fpl_shift(BD, Q24);
+__host__ __device__ inline QuantFpL make_quant_fpl(const QuantRow& t, int bd, bool q24) {
+  const int S = quant_fpl_shift(bd, q24);
   QuantFpL f;
-#pragma unroll
   for (int i = 0; i < 2; i++) {
     f.ndq[i] = -t.dequant[i];
     f.rs[i] = (uint32_t)t.round_fp[i] << S;
@@ -117,6 +115,11 @@ __device__ __forceinline__ QuantFpL make_quant_fpl(const QuantRow& t) {
   }
   return f;
 }
+// Up to two quantizers evaluated by one launch of the fused search kernels
+// (kernel argument: scalar registers).
+struct QuantFpSet {
+  QuantFpL q[2];
+};
 template <bool UNI>
 __device__ __forceinline__ uint32_t mul_hi_u24(uint32_t x, uint32_t y) {
   uint32_t r;
@@ -212,10 +215,13 @@ struct TxPlan {
 // Fused tx-type search over blocks of one tx size on a 16-bit plane (8-bit
 // content is widened by the caller).  iscan: device table of inverse scans,
 // [tx_size][class][1024] u16.  Outputs per (block, type) pair.
-// q24: the row's quant_fp values allow the 24-bit level (see QuantFpL).
+// n_q quantizers: dev_rows[q] (device rows) and host_rows[q] (the same
+// rows on the host); outputs per (q, block, type) at
+// q * n_blocks * types.n + block * types.n + k.
 hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size, const int32_t* xy,
-                          int n_blocks, const TypeList& types, const QuantRow* row, bool q24, const uint16_t* iscan,
-                          uint64_t* dist, int32_t* eob, uint32_t* lsum, hipStream_t st);
+                          int n_blocks, const TypeList& types, const QuantRow* const* dev_rows,
+                          const QuantRow* host_rows, int n_q, const uint16_t* iscan, uint64_t* dist, int32_t* eob,
+                          uint32_t* lsum, hipStream_t st);
 
 // One transform unit of the fused unit kernel (layout of zav1_txunit_job).
 struct TxUnitJob {

@@ -24,6 +24,25 @@ def legal_types(tx_size):
     return list(range(16))
 
 
+def kernel_form(tx_size):
+    """Which fused kernel evaluates a size (as the library picks it): 'lane'
+    (one lane per block: blocks up to 8 wide and 64 samples), 'rows' (one
+    lane per row: the 16-point sizes) or 'group' (lane groups: 32- and
+    64-point sizes).  The lane and row forms evaluate two quantizers per
+    launch; the group form one."""
+    w, h = TX_DIMS[tx_size]
+    if w * h <= 64 and w <= 8:
+        return 'lane'
+    if (w == 16 or h == 16) and w >= 8 and w * h <= 256:
+        return 'rows'
+    return 'group'
+
+
+def launches(tx_size, n_q):
+    """Kernel launches of one zav1_txsearch_batch_q call."""
+    return (n_q + 1) // 2 if kernel_form(tx_size) != 'group' else n_q
+
+
 def tile_xy(width, height, tx_size):
     w, h = TX_DIMS[tx_size]
     ys, xs = np.mgrid[0:height - h + 1:h, 0:width - w + 1:w]
@@ -42,16 +61,16 @@ class TxSearchSweep:
         self.sizes = list(range(19)) if sizes is None else list(sizes)
         self.d_plane = zav1.DeviceArray.from_host(self.plane)
         self.jobs = []
+        nq = len(self.qindices)
         for ts in self.sizes:
             xy = tile_xy(self.w, self.h, ts)
             types = np.array(legal_types(ts), np.uint8)
             d_xy = zav1.DeviceArray.from_host(xy)
-            per_q = []
-            for _ in self.qindices:
-                n = xy.shape[0] * types.size
-                per_q.append((zav1.DeviceArray(n, np.uint64), zav1.DeviceArray(n, np.int32),
-                              zav1.DeviceArray(n, np.uint32)))
-            self.jobs.append((ts, xy, types, d_xy, per_q))
+            n = xy.shape[0] * types.size
+            # outputs [q][block][type], all quantizers in one call
+            out = (zav1.DeviceArray(nq * n, np.uint64), zav1.DeviceArray(nq * n, np.int32),
+                   zav1.DeviceArray(nq * n, np.uint32))
+            self.jobs.append((ts, xy, types, d_xy, out))
 
     def candidates_per_step(self):
         return sum(xy.shape[0] * t.size for _, xy, t, _, _ in self.jobs) * len(self.qindices)
@@ -64,16 +83,16 @@ class TxSearchSweep:
         return tot * len(self.qindices)
 
     def run(self, stream=None, sizes=None):
-        for ts, xy, types, d_xy, per_q in self.jobs:
+        for ts, xy, types, d_xy, (dd, de, dl) in self.jobs:
             if sizes is not None and ts not in sizes:
                 continue
-            for qi, (dd, de, dl) in zip(self.qindices, per_q):
-                self.zav1.txsearch_batch(self.d_plane, self.w, self.bd, ts, d_xy, xy.shape[0], types, qi, dd, de,
-                                         dl, stream)
+            self.zav1.txsearch_batch_q(self.d_plane, self.w, self.bd, ts, d_xy, xy.shape[0], types, self.qindices,
+                                       dd, de, dl, stream)
 
     def results(self, tx_size, q_pos=0):
-        for ts, xy, types, d_xy, per_q in self.jobs:
+        for ts, xy, types, d_xy, (dd, de, dl) in self.jobs:
             if ts == tx_size:
-                dd, de, dl = per_q[q_pos]
-                return xy, types, dd.to_host(), de.to_host(), dl.to_host()
+                n = xy.shape[0] * types.size
+                sl = slice(q_pos * n, (q_pos + 1) * n)
+                return xy, types, dd.to_host()[sl], de.to_host()[sl], dl.to_host()[sl]
         raise KeyError(tx_size)
