@@ -451,13 +451,23 @@ __device__ __forceinline__ void ts_row_arm(const int32_t* __restrict__ src, cons
   }
 }
 
-template <int W, int H, int CK>
-__device__ __forceinline__ void ts_row_cols(const uint32_t (&rp)[(W >= H ? W / H : 1)][H / 2], int r,
-                                            int32_t* __restrict__ tile) {
+// Rows per lane of the row form, by measurement: 2 for 8x16 (its 8 lanes per
+// block then each own a column in the column pass), 1 otherwise (two rows per
+// lane measured no faster for the 16-wide rows, and costs LDS).
+template <int W, int H>
+constexpr int ts_rows_r() {
+  return (W == 8 && H == 16) ? 2 : 1;
+}
+
+// Column pass of one variant: the lane's columns c = t + LP k (packed
+// residual words, laundered per variant so the pass stays in the loop) into
+// the block's LDS tile.
+template <int W, int H, int CK, int LP, int NC>
+__device__ __forceinline__ void ts_row_cols(const uint32_t (&rp)[NC][H / 2], int t, int32_t* __restrict__ tile) {
   constexpr int CBC = fwd_cb_col(W, H);
   constexpr FwdShift SH = fwd_shift(W, H);
   constexpr long long kBCol = fwd_col_bound(SH.s0);
-  constexpr int NC = W >= H ? W / H : 1, LS = W + 1;
+  constexpr int LS = W + 1;
 #pragma unroll
   for (int k = 0; k < NC; k++) {
     int32_t wv[H / 2];
@@ -472,7 +482,7 @@ __device__ __forceinline__ void ts_row_cols(const uint32_t (&rp)[(W >= H ? W / H
       x[rr] = (s & 1) ? (wv[s >> 1] >> 16) : (int32_t)(int16_t)(wv[s >> 1] & 0xffff);
     }
     fwd1d<H, CBC, kBCol>(x, CK);
-    const int c = r + H * k;
+    const int c = t + LP * k;
     if (c < W) {
 #pragma unroll
       for (int rr = 0; rr < H; rr++) tile[rr * LS + c] = rshift_round32(x[rr], SH.s1);
@@ -489,18 +499,20 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
                                                        int32_t* __restrict__ out_eob,
                                                        uint32_t* __restrict__ out_lsum) {
   static_assert((W == 16 || H == 16) && W >= 8 && W * H <= 256, "row-form sizes");
-  constexpr int BPW = 64 / H;                 // blocks per wave
-  constexpr int LS = W + 1;                   // padded tile row
-  constexpr int NC = W >= H ? W / H : 1;      // columns per lane in the column pass
+  constexpr int R = ts_rows_r<W, H>();       // rows per lane
+  constexpr int LP = H / R;                  // lanes per block
+  constexpr int BPW = 64 / LP;               // blocks per wave
+  constexpr int LS = W + 1;                  // padded tile row
+  constexpr int NC = W >= LP ? W / LP : 1;   // columns per lane in the column pass
   constexpr FwdShift SH = fwd_shift(W, H);
-  constexpr int LSC = tx_log_scale(W, H);
+  static_assert(4 * (BPW * H * LS * 4 + NQ * BPW * kStageRow * 16) <= 65536, "LDS of a 4-wave workgroup");
   __shared__ int32_t tiles[4][BPW * H * LS];
   __shared__ uint64_t st_d[NQ][4][BPW * kStageRow];
   __shared__ int32_t st_e[NQ][4][BPW * kStageRow];
   __shared__ uint32_t st_l[NQ][4][BPW * kStageRow];
 
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int g = lane / H, r = lane - g * H;  // block within the wave, row / column slot
+  const int g = lane / LP, t = lane - g * LP;  // block within the wave, lane within the block
   const long bw0 = ((long)xcd_chunked(blockIdx.x, gridDim.x) * 4 + wv) * BPW;
   const long b = bw0 + g;
   const int bi = b < n_blocks ? (int)b : 0;  // past the end: block 0, never copied out
@@ -509,16 +521,19 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
   const PlaneBuf pb(plane);
   const uint32_t rb = 2u * (uint32_t)stride;  // row pitch in bytes
 
-  // DC prediction from the source neighbours (block's H lanes reduce)
+  // DC prediction from the source neighbours (the block's LP lanes reduce)
   uint32_t sa = 0, sl = 0;
   if (y0 > 0) {
 #pragma unroll
     for (int k = 0; k < NC; k++)
-      if (r + H * k < W) sa += (uint32_t)pb.at(2u * (o0 - (uint32_t)stride + (uint32_t)(r + H * k)), 0);
+      if (t + LP * k < W) sa += (uint32_t)pb.at(2u * (o0 - (uint32_t)stride + (uint32_t)(t + LP * k)), 0);
   }
-  if (x0 > 0) sl = (uint32_t)pb.at(2u * (o0 + (uint32_t)(r * stride) - 1u), 0);
-  sa = (uint32_t)group_sum_i32<H>((int32_t)sa);
-  sl = (uint32_t)group_sum_i32<H>((int32_t)sl);
+  if (x0 > 0) {
+#pragma unroll
+    for (int i = 0; i < R; i++) sl += (uint32_t)pb.at(2u * (o0 - 1u), rb * (uint32_t)(t * R + i));
+  }
+  sa = (uint32_t)dpp_sum_i32<LP>((int32_t)sa);
+  sl = (uint32_t)dpp_sum_i32<LP>((int32_t)sl);
   int32_t dc;
   if (y0 > 0 && x0 > 0) dc = (int32_t)((sa + sl + (W + H) / 2) / (W + H));
   else if (y0 > 0) dc = (int32_t)((sa + W / 2) / W);
@@ -529,7 +544,7 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
   uint32_t rp[NC][H / 2];
 #pragma unroll
   for (int k = 0; k < NC; k++) {
-    const int c = (r + H * k) % W;  // lanes without a column (W < H) read a valid one, never stored
+    const int c = (t + LP * k) % W;  // lanes without a column (W < LP) read a valid one, never stored
 #pragma unroll
     for (int i = 0; i < H / 2; i++) {
       const int32_t lo = (pb.at(2u * (o0 + (uint32_t)c), rb * (2 * i)) - dc) << SH.s0;
@@ -537,16 +552,18 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
       rp[k][i] = ((uint32_t)lo & 0xffffu) | ((uint32_t)hi << 16);
     }
   }
-  // scan index of each coefficient of the lane's row, three classes, bytes
-  uint32_t pw[3][W / 4];
+  // scan index of each coefficient of the lane's rows, three classes, bytes
+  uint32_t pw[R][3][W / 4];
 #pragma unroll
-  for (int cls = 0; cls < 3; cls++) {
-    const uint16_t* is = iscan_all + (tx_size * 3 + cls) * 1024 + r * W;
+  for (int i = 0; i < R; i++)
 #pragma unroll
-    for (int k = 0; k < W / 4; k++)
-      pw[cls][k] = (uint32_t)is[4 * k] | (uint32_t)is[4 * k + 1] << 8 | (uint32_t)is[4 * k + 2] << 16 |
-                   (uint32_t)is[4 * k + 3] << 24;
-  }
+    for (int cls = 0; cls < 3; cls++) {
+      const uint16_t* is = iscan_all + (tx_size * 3 + cls) * 1024 + (t * R + i) * W;
+#pragma unroll
+      for (int k = 0; k < W / 4; k++)
+        pw[i][cls][k] = (uint32_t)is[4 * k] | (uint32_t)is[4 * k + 1] << 8 | (uint32_t)is[4 * k + 2] << 16 |
+                        (uint32_t)is[4 * k + 3] << 24;
+    }
 
   const int ntyp = plan.n_types;
   int32_t* tile = tiles[wv] + g * H * LS;
@@ -554,41 +571,43 @@ __global__ __launch_bounds__(256) void k_txsearch_rows(const uint16_t* __restric
     const uint32_t vi = plan.lane_var[v];
     const int ck = (int)(vi & 0xff);
     wave_sync();  // the previous variant's rows have been read
-    if (ck == 0) ts_row_cols<W, H, 0>(rp, r, tile);
-    else if (ck == 1) ts_row_cols<W, H, 1>(rp, r, tile);
-    else if (ck == 2) ts_row_cols<W, H, 2>(rp, r, tile);
-    else ts_row_cols<W, H, 3>(rp, r, tile);
+    if (ck == 0) ts_row_cols<W, H, 0, LP, NC>(rp, t, tile);
+    else if (ck == 1) ts_row_cols<W, H, 1, LP, NC>(rp, t, tile);
+    else if (ck == 2) ts_row_cols<W, H, 2, LP, NC>(rp, t, tile);
+    else ts_row_cols<W, H, 3, LP, NC>(rp, t, tile);
     wave_sync();
-    const int32_t* src = tile + r * LS;
     const int i_end = (int)(vi >> 16);
     for (int i = (int)((vi >> 8) & 0xff); i < i_end; i++) {
       const uint32_t ti = plan.lane_type[i];
       const int cls = (int)((ti >> 16) & 0xff), rk = (int)((ti >> 8) & 0xff);
-      uint32_t pc[W / 4];
-#pragma unroll
-      for (int k = 0; k < W / 4; k++) pc[k] = cls == 0 ? pw[0][k] : (cls == 1 ? pw[1][k] : pw[2][k]);
       uint64_t dist[NQ];
       int32_t m[NQ], lsum[NQ];
 #pragma unroll
       for (int q = 0; q < NQ; q++) dist[q] = 0, m[q] = -1, lsum[q] = 0;
-      if (rk == 0) ts_row_arm<W, H, 0, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
-      else if (rk == 1) ts_row_arm<W, H, 1, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
-      else if (rk == 2) ts_row_arm<W, H, 2, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
-      else ts_row_arm<W, H, 3, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int r = t * R + j;
+        uint32_t pc[W / 4];
+#pragma unroll
+        for (int k = 0; k < W / 4; k++) pc[k] = cls == 0 ? pw[j][0][k] : (cls == 1 ? pw[j][1][k] : pw[j][2][k]);
+        const int32_t* src = tile + r * LS;
+        if (rk == 0) ts_row_arm<W, H, 0, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+        else if (rk == 1) ts_row_arm<W, H, 1, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+        else if (rk == 2) ts_row_arm<W, H, 2, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+        else ts_row_arm<W, H, 3, BD, Q24, NQ>(src, pc, r, qs, dist, m, lsum);
+      }
       const int k = (int)(ti & 0xff);
 #pragma unroll
       for (int q = 0; q < NQ; q++) {
-        const uint64_t dq = group_sum_u64<H>(dist[q]);
-        const int32_t mq = group_max_i32<H>(m[q]);
-        const int32_t lq = group_sum_i32<H>(lsum[q]);
-        if (r == 0) {
+        const uint64_t dq = dpp_sum_u64<LP>(dist[q]);
+        const int32_t mq = dpp_max_i32<LP>(m[q]);
+        const int32_t lq = dpp_sum_i32<LP>(lsum[q]);
+        if (t == 0) {
           st_d[q][wv][g * kStageRow + k] = dq >> 2;  // (1 - log scale) * 2 with log scale 0
           st_e[q][wv][g * kStageRow + k] = mq + 1;
           st_l[q][wv][g * kStageRow + k] = (uint32_t)lq;
         }
       }
-      // the row's address becomes loop-carried: its loads and the row arms
-      // stay in the type loop instead of being hoisted out of it, all four
     }
   }
   wave_sync();
@@ -684,7 +703,8 @@ static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int
         const dim3 grid((n_blocks + kLaneBlocks - 1) / kLaneBlocks);
         launch_fused<W, H, LaneK>(bd, q24, nq, grid, st, plane, stride, bd, xy, n_blocks, lp, qs, sc, dq, eq, lq);
       } else {
-        const dim3 grid((n_blocks + 4 * (64 / H) - 1) / (4 * (64 / H)));
+        constexpr int BPW = 64 / (H / ts_rows_r<W, H>());  // blocks per wave
+        const dim3 grid((n_blocks + 4 * BPW - 1) / (4 * BPW));
         launch_fused<W, H, RowK>(bd, q24, nq, grid, st, plane, stride, bd, xy, n_blocks, lp, qs, iscan, S, dq, eq,
                                  lq);
       }

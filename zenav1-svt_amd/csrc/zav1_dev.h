@@ -353,6 +353,59 @@ __host__ __device__ constexpr int tx_log_scale(int w, int h) {
   return (w * h > 1024) ? 2 : ((w * h > 256) ? 1 : 0);
 }
 
+// Sum / max over aligned groups of G = 1, 2, 4, 8 or 16 lanes, every lane
+// of a group receiving the result, through DPP lane permutations (no LDS
+// crossbar): G = 16 rotates within the 16-lane row by 8, 4, 2, 1; G = 8
+// mirrors the half-row, then swaps lane pairs within quads; G = 4 swaps
+// within quads; G = 2 swaps neighbours.
+template <int G>
+__device__ __forceinline__ int32_t dpp_step(int32_t v, int k) {
+  // k-th permutation of the group's reduction
+  if constexpr (G == 2) {
+    return __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);
+  } else if constexpr (G == 16) {
+    return k == 0 ? __builtin_amdgcn_update_dpp(0, v, 0x128, 0xf, 0xf, false)
+         : k == 1 ? __builtin_amdgcn_update_dpp(0, v, 0x124, 0xf, 0xf, false)
+         : k == 2 ? __builtin_amdgcn_update_dpp(0, v, 0x122, 0xf, 0xf, false)
+                  : __builtin_amdgcn_update_dpp(0, v, 0x121, 0xf, 0xf, false);
+  } else if constexpr (G == 8) {
+    return k == 0 ? __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false)
+         : k == 1 ? __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false)
+                  : __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);
+  } else {
+    return k == 0 ? __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false)
+                  : __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);
+  }
+}
+template <int G>
+constexpr int dpp_steps() {
+  return G == 16 ? 4 : (G == 8 ? 3 : (G == 4 ? 2 : (G == 2 ? 1 : 0)));
+}
+template <int G>
+__device__ __forceinline__ int32_t dpp_sum_i32(int32_t v) {
+  static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16, "group");
+  constexpr int NS = dpp_steps<G>();
+#pragma unroll
+  for (int k = 0; k < NS; k++) v += dpp_step<G>(v, k);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ int32_t dpp_max_i32(int32_t v) {
+  constexpr int NS = dpp_steps<G>();
+#pragma unroll
+  for (int k = 0; k < NS; k++) v = max(v, dpp_step<G>(v, k));
+  return v;
+}
+// For per-lane values below 2^52: the low 24 bits and the rest are summed
+// as two 32-bit values (16 lanes of each stay below 2^32), so every step is
+// one fused 32-bit DPP add.
+template <int G>
+__device__ __forceinline__ uint64_t dpp_sum_u64(uint64_t v) {
+  const int32_t lo = dpp_sum_i32<G>((int32_t)((uint32_t)v & 0xffffffu));
+  const int32_t hi = dpp_sum_i32<G>((int32_t)(uint32_t)(v >> 24));
+  return ((uint64_t)(uint32_t)hi << 24) + (uint32_t)lo;
+}
+
 template <int T>
 __device__ __forceinline__ uint64_t group_sum_u64(uint64_t v) {
 #pragma unroll
