@@ -37,10 +37,10 @@ struct BlkGeom {
   static constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
 };
 
-template <int W, int H, int P, int BD>
+template <int W, int H, int P, int BD, int NQ>
 __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ plane, int stride, int bd,
                                                   const int32_t* __restrict__ xy, int n_blocks, TxPlan plan,
-                                                  const QuantRow* __restrict__ row_p,
+                                                  QuantFpPair qp,
                                                   const uint16_t* __restrict__ iscan_all, int tx_size,
                                                   uint64_t* __restrict__ out_dist,
                                                   int32_t* __restrict__ out_eob,
@@ -97,16 +97,16 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
   }
   __syncthreads();
 
-  // ---- row passes: items (type, row) ----
+  // ---- row passes: items (type, row), every quantizer of the launch ----
   constexpr int TPR = G::T / H;
   const int sub = t / H, r = t - sub * H;
-  const QuantRow& qr = *row_p;
-  const QuantFp qf = make_quant_fp(qr, LSC);
   for (int base = 0; base < ntyp; base += TPR) {
     const int ti = base + sub;
     const bool live = active && ti < ntyp;
-    uint64_t dist = 0;
-    int32_t eob = 0, lsum = 0;
+    uint64_t dist[NQ], fold = 0;
+    int32_t eob[NQ], lsum[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) dist[q] = 0, eob[q] = 0, lsum[q] = 0;
     if (live) {
       const int tx_type = plan.type[ti];
       const TypeCfg cfg = type_cfg(tx_type);
@@ -125,7 +125,9 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
       fwd1d<W, CBR, kBRow>(v, cfg.row);
       // squared errors: each < 2^30 (|e| < 2^15, see QuantFp), so four of
       // them sum in 32 bits before one 64-bit add
-      uint32_t sq4 = 0;
+      uint32_t sq4[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) sq4[q] = 0;
 #pragma unroll
       for (int c = 0; c < W; c++) {
         int32_t x = rshift_round32(v[c], SH.s2);
@@ -133,32 +135,40 @@ __global__ __launch_bounds__(256) void k_txsearch(const uint16_t* __restrict__ p
         // coefficient bound before this scaling), so the product fits 32 bits
         if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
         if (r >= 32 || c >= 32) {
-          dist += (uint64_t)((int64_t)x * x);  // 64-point fold: discarded energy
+          // 64-point fold: discarded energy (|x| < 2^18: the 48-bit square
+          // from the 24-bit multiplier's low and high words)
+          fold += ((uint64_t)(uint32_t)mul_hi_i24(x, x) << 32) | (uint32_t)__mul24(x, x);
           continue;
         }
         const int32_t a = x < 0 ? -x : x;
-        int32_t adq;
-        const int32_t q = quant_fp_fast<LSC>(a, (r | c) != 0, qf, BD, &adq);
-        lsum += q;
-        eob = max(eob, q ? sp[c < KW ? c : 0] : 0);
-        // x - dq = sign(x) * (a - |dq|): the squared error needs magnitudes only
-        const int32_t e = a - adq;
-        sq4 += (uint32_t)__mul24(e, e);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          int32_t adq;
+          const int32_t l = quant_fp_fast<LSC>(a, (r | c) != 0, qp.q[q], BD, &adq);
+          lsum[q] += l;
+          eob[q] = max(eob[q], l ? sp[c < KW ? c : 0] : 0);
+          // x - dq = sign(x) * (a - |dq|): the squared error needs magnitudes only
+          const int32_t e = a - adq;
+          sq4[q] += (uint32_t)__mul24(e, e);
+        }
         if ((c & 3) == 3 || c == W - 1 || c == 31) {
-          dist += sq4;
-          sq4 = 0;
+#pragma unroll
+          for (int q = 0; q < NQ; q++) dist[q] += sq4[q], sq4[q] = 0;
         }
       }
     }
-    dist = group_sum_u64<H>(dist);
-    eob = group_max_i32<H>(eob);
-    lsum = group_sum_i32<H>(lsum);
-    if (live && r == 0) {
-      constexpr int SHF = (1 - LSC) * 2;
-      const long o = (long)bi * ntyp + plan.out_k[ti];
-      out_dist[o] = SHF < 0 ? (dist << (-SHF)) : (dist >> SHF);
-      out_eob[o] = eob;
-      out_lsum[o] = (uint32_t)lsum;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const uint64_t dq = group_sum_u64<H>(dist[q] + fold);
+      const int32_t eq = group_max_i32<H>(eob[q]);
+      const int32_t lq = group_sum_i32<H>(lsum[q]);
+      if (live && r == 0) {
+        constexpr int SHF = (1 - LSC) * 2;
+        const long o = (long)q * n_blocks * ntyp + (long)bi * ntyp + plan.out_k[ti];
+        out_dist[o] = SHF < 0 ? (dq << (-SHF)) : (dq >> SHF);
+        out_eob[o] = eq;
+        out_lsum[o] = (uint32_t)lq;
+      }
     }
   }
 }
@@ -717,17 +727,33 @@ static hipError_t launch_ts(const uint16_t* plane, int stride, int bd, const int
     int groups = 256 / G::T;
     while (groups > 1 && groups * per_group > 65536) groups >>= 1;
     const int grid = (n_blocks + groups - 1) / groups;
-    for (int q = 0; q < n_q; q++) {
-      if (bd == 8)
-        hipLaunchKernelGGL((k_txsearch<W, H, P, 8>), dim3(grid), dim3(groups * G::T), groups * per_group, st, plane,
-                           stride, bd, xy, n_blocks, plan, dev_rows[q], iscan, S, d + q * per_q, e + q * per_q,
-                           l + q * per_q);
-      else
-        hipLaunchKernelGGL((k_txsearch<W, H, P, 10>), dim3(grid), dim3(groups * G::T), groups * per_group, st,
-                           plane, stride, bd, xy, n_blocks, plan, dev_rows[q], iscan, S, d + q * per_q,
-                           e + q * per_q, l + q * per_q);
+    constexpr int LSC = tx_log_scale(W, H);
+    for (int q0 = 0; q0 < n_q; q0 += 2) {
+      const int nq = n_q - q0 < 2 ? n_q - q0 : 2;
+      QuantFpPair qp{};
+      for (int i = 0; i < nq; i++) qp.q[i] = make_quant_fp(host_rows[q0 + i], LSC);
+      uint64_t* dq = d + q0 * per_q;
+      int32_t* eq = e + q0 * per_q;
+      uint32_t* lq = l + q0 * per_q;
+      const dim3 gr(grid), bl(groups * G::T);
+      const size_t sh = (size_t)groups * per_group;
+      if (bd == 8) {
+        if (nq == 2)
+          hipLaunchKernelGGL((k_txsearch<W, H, P, 8, 2>), gr, bl, sh, st, plane, stride, bd, xy, n_blocks, plan, qp,
+                             iscan, S, dq, eq, lq);
+        else
+          hipLaunchKernelGGL((k_txsearch<W, H, P, 8, 1>), gr, bl, sh, st, plane, stride, bd, xy, n_blocks, plan, qp,
+                             iscan, S, dq, eq, lq);
+      } else {
+        if (nq == 2)
+          hipLaunchKernelGGL((k_txsearch<W, H, P, 10, 2>), gr, bl, sh, st, plane, stride, bd, xy, n_blocks, plan, qp,
+                             iscan, S, dq, eq, lq);
+        else
+          hipLaunchKernelGGL((k_txsearch<W, H, P, 10, 1>), gr, bl, sh, st, plane, stride, bd, xy, n_blocks, plan, qp,
+                             iscan, S, dq, eq, lq);
+      }
     }
-    (void)host_rows;
+    (void)dev_rows;
   }
   return hipGetLastError();
 }

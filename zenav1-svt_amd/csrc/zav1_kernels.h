@@ -51,7 +51,7 @@ struct QuantFp {
   int32_t dq[2], rnd[2];
   uint32_t qf16[2];  // quant_fp << 16
 };
-__device__ __forceinline__ QuantFp make_quant_fp(const QuantRow& t, int ls) {
+__host__ __device__ inline QuantFp make_quant_fp(const QuantRow& t, int ls) {
   QuantFp f;
 #pragma unroll
   for (int i = 0; i < 2; i++) {
@@ -120,11 +120,21 @@ __host__ __device__ inline QuantFpL make_quant_fpl(const QuantRow& t, int bd, bo
 struct QuantFpSet {
   QuantFpL q[2];
 };
+struct QuantFpPair {
+  QuantFp q[2];
+};
 template <bool UNI>
 __device__ __forceinline__ uint32_t mul_hi_u24(uint32_t x, uint32_t y) {
   uint32_t r;
   if constexpr (UNI) asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(r) : "s"(y), "v"(x));
   else asm("v_mul_hi_u32_u24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+  return r;
+}
+// High 16 bits (sign-extended) of the 48-bit product of two signed 24-bit
+// values.
+__device__ __forceinline__ int32_t mul_hi_i24(int32_t x, int32_t y) {
+  int32_t r;
+  asm("v_mul_hi_i32_i24 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
   return r;
 }
 template <bool UNI>

@@ -28,8 +28,7 @@ def kernel_form(tx_size):
     """Which fused kernel evaluates a size (as the library picks it): 'lane'
     (one lane per block: blocks up to 8 wide and 64 samples), 'rows' (one
     lane per row: the 16-point sizes) or 'group' (lane groups: 32- and
-    64-point sizes).  The lane and row forms evaluate two quantizers per
-    launch; the group form one."""
+    64-point sizes).  Every form evaluates two quantizers per launch."""
     w, h = TX_DIMS[tx_size]
     if w * h <= 64 and w <= 8:
         return 'lane'
@@ -40,7 +39,7 @@ def kernel_form(tx_size):
 
 def launches(tx_size, n_q):
     """Kernel launches of one zav1_txsearch_batch_q call."""
-    return (n_q + 1) // 2 if kernel_form(tx_size) != 'group' else n_q
+    return (n_q + 1) // 2
 
 
 def tile_xy(width, height, tx_size):
