@@ -386,6 +386,7 @@ def main():
     ap.add_argument('--no-funnel', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--lf-steps', type=int, default=10)
+    ap.add_argument('--streams', type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; multi-GPU runs are not part of it '
@@ -443,20 +444,36 @@ def main():
             per_size_ms[ts] += elapsed(*evs[ts]) / cal_steps
     dom = max(per_size_ms, key=per_size_ms.get)
 
-    # timed region: the steps back to back on the stream; HIP events bracket
-    # the dominant size's launch of every step (read after the region)
+    # timed region: the steps back to back, the other sizes spread over
+    # args.streams streams (kernels of different sizes overlap, filling each
+    # other's tails).  The dominant size runs alone at the start of every
+    # step (all streams join before it and fork after it), bracketed by HIP
+    # events on its stream, so its measured duration is its own (read after
+    # the region).
+    streams = [stream]
+    for _ in range(args.streams - 1):
+        sx = ctypes.c_void_p()
+        zav1.check(L.zav1_stream_create(ctypes.byref(sx)), 'stream_create')
+        streams.append(sx.value)
     dom_ev = [(event(), event()) for _ in range(args.steps)]
-    zav1.sync(stream)
+    join_ev = [[event() for _ in streams] for _ in range(args.steps)]
+    for sx in streams:
+        zav1.sync(sx)
     t0 = time.perf_counter()
     for k in range(args.steps):
+        for j, sx in enumerate(streams[1:], 1):  # join: the stream of the dominant waits for the others
+            L.zav1_event_record(join_ev[k][j], sx)
+            L.zav1_stream_wait_event(stream, join_ev[k][j])
+        L.zav1_event_record(dom_ev[k][0], stream)
+        sweep.run(stream, sizes=(dom,))
+        L.zav1_event_record(dom_ev[k][1], stream)
+        for sx in streams[1:]:  # fork: the others start after it
+            L.zav1_stream_wait_event(sx, dom_ev[k][1])
         for ts in range(19):
-            if ts == dom:
-                L.zav1_event_record(dom_ev[k][0], stream)
-                sweep.run(stream, sizes=(ts,))
-                L.zav1_event_record(dom_ev[k][1], stream)
-            else:
-                sweep.run(stream, sizes=(ts,))
-    zav1.sync(stream)
+            if ts != dom:
+                sweep.run(streams, sizes=(ts,))
+    for sx in streams:
+        zav1.sync(sx)
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     mp = S * S / 1e6 * len(qps)
@@ -486,7 +503,8 @@ def main():
         'config': {'workload': 'config2 tx-type search: %dx%d 10-bit luma, qp 20+40, 19 tx sizes x legal '
                                'types, fused DC-pred/fwd/fold/fp-quant/dist' % (S, S),
                    'candidates_per_step': n_cand,
-                   'launches_per_step': sum(tsm.launches(t, len(qps)) for t in range(19))},
+                   'launches_per_step': sum(tsm.launches(t, len(qps)) for t in range(19)),
+                   'streams': args.streams},
         'per_size_ms_per_step': {str(tsm.TX_DIMS[t]): round(per_size_ms[t], 4) for t in range(19)},
         'roofline': {'bound': 'valu', 'kernel': '%s<%d,%d>' % (kname, w, h), 'achieved': round(achieved_tops, 3),
                      'peak': VALU_PEAK_TOPS, 'unit': 'Tops/s', 'frac': round(achieved_tops / VALU_PEAK_TOPS, 4),

@@ -69,6 +69,9 @@ int zav1_event_create(void **ev);
 int zav1_event_destroy(void *ev);
 int zav1_event_record(void *ev, zav1_stream_t s);
 int zav1_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Work enqueued on s after this call waits for ev's recorded work (joins two
+ * streams on the device). */
+int zav1_stream_wait_event(zav1_stream_t s, void *ev);
 const char *zav1_last_error(void);
 
 /* ------------------------------------------------------------------------ */

@@ -112,6 +112,7 @@ EXPORTS = {
     'zav1_event_destroy': [_vp],
     'zav1_event_record': [_vp, _vp],
     'zav1_event_elapsed_ms': [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    'zav1_stream_wait_event': [_vp, _vp],
     'zav1_last_error': [],
     'zav1_build_quant_row': [_i32, _i32, ctypes.POINTER(QuantRow)],
     'zav1_dsp_get': [_i32],

@@ -803,6 +803,12 @@ int zav1_event_elapsed_ms(void* a, void* b, float* ms) {
   ZCHECK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
   return ZAV1_OK;
 }
+
+int zav1_stream_wait_event(zav1_stream_t s, void* ev) {
+  if (!ev) return ZAV1_EINVAL;
+  ZCHECK(hipStreamWaitEvent(S(s), (hipEvent_t)ev, 0));
+  return ZAV1_OK;
+}
 const char* zav1_last_error(void) { return g_err.c_str(); }
 
 int zav1_build_quant_row(int qindex, int bd, zav1_quant_row* row) {

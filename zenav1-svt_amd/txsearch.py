@@ -81,12 +81,24 @@ class TxSearchSweep:
             tot += xy.shape[0] * t.size * w * h
         return tot * len(self.qindices)
 
+    def stream_of(self, tx_size, streams):
+        """The stream a size runs on when the sweep is spread over several:
+        sizes in decreasing order of work, dealt round-robin (each size
+        always on the same stream, so repeated runs stay ordered per size;
+        different sizes write different outputs)."""
+        if not isinstance(streams, (list, tuple)):
+            return streams
+        order = sorted(range(19), key=lambda t: (-len(legal_types(t)), -TX_DIMS[t][0] * TX_DIMS[t][1], t))
+        return streams[order.index(tx_size) % len(streams)]
+
     def run(self, stream=None, sizes=None):
+        """One pass over the frame; `stream` may be a list of streams, over
+        which the sizes are spread (see stream_of)."""
         for ts, xy, types, d_xy, (dd, de, dl) in self.jobs:
             if sizes is not None and ts not in sizes:
                 continue
             self.zav1.txsearch_batch_q(self.d_plane, self.w, self.bd, ts, d_xy, xy.shape[0], types, self.qindices,
-                                       dd, de, dl, stream)
+                                       dd, de, dl, self.stream_of(ts, stream))
 
     def results(self, tx_size, q_pos=0):
         for ts, xy, types, d_xy, (dd, de, dl) in self.jobs:
