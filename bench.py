@@ -374,6 +374,61 @@ def md_funnel_stage(zav1, synth, args, cpu=True):
     return out
 
 
+def config4_stage(zav1, synth, tsm, args, streams):
+    """Config 4 at its per-GPU shape: one tile column (2048 wide, 4096 high)
+    of the 8192x4096 10-bit frame with screen regions, tx-type search at
+    qp 28 (one quantizer), sizes spread over the bench's streams."""
+    plane = synth.config4_tile_column(4096, 2048, seed=4000)
+    sw = tsm.TxSearchSweep(zav1, plane, 10, [synth.qp_to_qindex(28)])
+    for _ in range(2):
+        sw.run(streams)
+    for sx in streams:
+        zav1.sync(sx)
+    steps = max(5, args.steps // 5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sw.run(streams)
+    for sx in streams:
+        zav1.sync(sx)
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    mp = plane.shape[0] * plane.shape[1] / 1e6
+    return {'metric': 'MP/s (luma megapixels x qps) through the tx-type search', 'value': round(mp / ms * 1000.0, 2),
+            'unit': 'MP/s', 'ms_per_step': round(ms, 3), 'steps': steps,
+            'config': {'workload': 'config4 tile column: 2048x4096 10-bit luma (1/f noise, 25% screen regions), '
+                                   'qp 28, 19 tx sizes x legal types', 'streams': len(streams)}}
+
+
+def serving_stage(zav1, synth, tsm, args):
+    """Serving mix at a fixed Poisson arrival rate: per-request latency of
+    the tx-type search stage (upload, grids, every size at the request's
+    qp, results resident in HBM) with an HBM admission pool of contexts.
+    The rate is set to args.serve_load of the pool's measured back-to-back
+    throughput."""
+    srv = _load('zav1_serving', os.path.join(PKG, 'serving.py'))
+    reqs = synth.serving_mix(args.serve_n, seed=5000)
+    imgs = synth.serving_images(reqs, seed=5000)
+    loop = srv.ServingLoop(zav1, tsm, synth, n_ctx=args.serve_ctx)
+    # back-to-back throughput (every request arrives at once), twice: the
+    # first pass warms up
+    for _ in range(2):
+        lat0, mp0, wall0 = loop.run(reqs, imgs, rate_hz=1e9)
+    mp_per_s = mp0 / wall0
+    rate = args.serve_load * mp_per_s / (mp0 / len(reqs))
+    lat, mp, wall = loop.run(reqs, imgs, rate_hz=rate, seed=5002)
+    loop.close()
+    return {'metric': 'per-image latency (ms) of the tx-type search stage, Poisson arrivals',
+            'p50_ms': round(float(np.percentile(lat, 50)) * 1e3, 3),
+            'p99_ms': round(float(np.percentile(lat, 99)) * 1e3, 3),
+            'mean_ms': round(float(lat.mean()) * 1e3, 3),
+            'value': round(mp / wall, 2), 'unit': 'MP/s', 'requests': len(reqs), 'megapixels': round(mp, 2),
+            'arrival_rate_hz': round(rate, 1), 'load': args.serve_load,
+            'back_to_back_MPs': round(mp_per_s, 2),
+            'admission': {'contexts': args.serve_ctx, 'hbm_per_context_MB': round(loop.budget / 1e6, 1)},
+            'config': {'workload': 'serving mix: area log-uniform 256^2..4096x3072, aspect 1:1..16:9, 50/50 '
+                                   '8/10-bit, qp U{20..50}; per request upload + tx-type search over every size '
+                                   '(one quantizer)'}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -387,6 +442,11 @@ def main():
     ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--lf-steps', type=int, default=10)
     ap.add_argument('--streams', type=int, default=2)
+    ap.add_argument('--no-config4', action='store_true')
+    ap.add_argument('--no-serving', action='store_true')
+    ap.add_argument('--serve-n', type=int, default=96)
+    ap.add_argument('--serve-ctx', type=int, default=4)
+    ap.add_argument('--serve-load', type=float, default=0.6)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build measures one GPU; multi-GPU runs are not part of it '
@@ -572,6 +632,10 @@ def main():
                                'sample': '%dx%d (whole) frame, qp 20+40, all sizes/types '
                                          '(oracle C, OpenMP over blocks)' % (crop, crop),
                                'seconds': round(tc1 - tc0, 2)}
+    if not args.no_config4:
+        out['config4_tile'] = config4_stage(zav1, synth, tsm, args, streams)
+    if not args.no_serving:
+        out['serving'] = serving_stage(zav1, synth, tsm, args)
     if not args.no_lf:
         out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=not args.no_cpu)
     if not args.no_funnel:

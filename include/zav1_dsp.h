@@ -72,6 +72,8 @@ int zav1_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Work enqueued on s after this call waits for ev's recorded work (joins two
  * streams on the device). */
 int zav1_stream_wait_event(zav1_stream_t s, void *ev);
+/* 1 when ev's recorded work has completed, 0 while it is pending. */
+int zav1_event_query(void *ev);
 const char *zav1_last_error(void);
 
 /* ------------------------------------------------------------------------ */
@@ -244,6 +246,10 @@ int zav1_scan_order(int tx_size, int scan_class, uint16_t *scan);
 int zav1_txsearch_batch(const uint16_t *d_plane, int stride, int bd, int tx_size, const int32_t *d_xy,
                         int n_blocks, const uint8_t *types, int n_types, int qindex, uint64_t *d_dist,
                         int32_t *d_eob, uint32_t *d_lsum, zav1_stream_t s);
+/* Origins of the full grid of bw x bh blocks covering a plane_w x plane_h
+ * plane in raster order (x, y pairs, the layout d_xy of the search takes):
+ * writes (plane_w / bw) * (plane_h / bh) pairs and returns that count. */
+int zav1_grid_xy(int32_t *d_xy, int plane_w, int plane_h, int bw, int bh, zav1_stream_t s);
 /* The same search at n_q quantizers (qindices[q], 1 <= n_q <= 8): each
  * block's transforms are computed once and quantized at every qindex.
  * Outputs per (q, block j, type k) at q*n_blocks*n_types + j*n_types + k;
@@ -408,6 +414,25 @@ typedef struct zav1_md_wave {
   int32_t *d_qcoeff;
   uint16_t *d_recon_out;
 } zav1_md_wave;
+/* The tx-type search over a whole plane: every tx size's full block grid
+ * (zav1_grid_xy), each with the legal types of its size (all 16 up to 16
+ * points per side; DCT_DCT, IDTX, V_DCT, H_DCT at 32; DCT_DCT at 64), at
+ * n_q quantizers (zav1_txsearch_batch_q), on the context stream; grids and
+ * outputs are allocated from the context arena (ZAV1_ENOMEM, nothing
+ * enqueued, when they do not fit).  d_plane: u16 plane of w x h (tight
+ * rows).  out receives per size the block and type counts, the types and
+ * the device outputs ([q][block][type]; NULL for sizes larger than the
+ * plane).  The search body of a frame's MD in one call. */
+typedef struct zav1_frame_search {
+  int32_t n_blocks[19], n_types[19];
+  uint8_t types[19][16];
+  uint64_t *dist[19];
+  int32_t *eob[19];
+  uint32_t *lsum[19];
+} zav1_frame_search;
+int zav1_txsearch_frame(zav1_ctx *ctx, const uint16_t *d_plane, int w, int h, int bd, const int *qindices, int n_q,
+                        zav1_frame_search *out);
+
 /* Enqueues one wave on the context stream; fails with ZAV1_ENOMEM (nothing
  * enqueued) when the arena cannot hold its intermediates. */
 int zav1_md_wave_submit(zav1_ctx *ctx, const zav1_md_wave *wave);

@@ -154,3 +154,27 @@ def test_fused_fp_level_forms_pinned(zav1):
                 f1, f2 = xx << S, np.int64(qf) << (16 - S)
                 assert f1.max() < (1 << 24) and f2 < (1 << 24), (bd, q, i)
                 assert np.array_equal((f1 * f2) >> 32, (xx * qf) >> 16), (bd, q, i)
+
+
+def test_serving_mix_properties():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location('zav1_synth_t', os.path.join(ROOT, 'zenav1-svt_amd', 'synth.py'))
+    synth = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(synth)
+    reqs = synth.serving_mix(400, seed=5000)
+    assert reqs == synth.serving_mix(400, seed=5000)  # deterministic
+    for h, w, bd, qp in reqs:
+        assert h % 64 == 0 and w % 64 == 0 and 64 <= h <= 4096 and 64 <= w <= 4096
+        assert bd in (8, 10) and 20 <= qp <= 50
+    bds = [r[2] for r in reqs]
+    assert 0.4 < bds.count(8) / len(bds) < 0.6
+    areas = np.log([h * w for h, w, _, _ in reqs])
+    assert areas.min() < np.log(300 * 300) and areas.max() > np.log(3000 * 2500)
+
+
+def test_grid_and_event_entry_points_validate_without_gpu(zav1):
+    L = zav1.lib()
+    assert L.zav1_grid_xy(None, 64, 64, 8, 8, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_grid_xy(ctypes.c_void_p(1), 64, 64, 0, 8, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_event_query(None) == zav1.ZAV1_EINVAL
+    assert L.zav1_stream_wait_event(None, None) == zav1.ZAV1_EINVAL

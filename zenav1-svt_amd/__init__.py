@@ -113,6 +113,8 @@ EXPORTS = {
     'zav1_event_record': [_vp, _vp],
     'zav1_event_elapsed_ms': [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
     'zav1_stream_wait_event': [_vp, _vp],
+    'zav1_event_query': [_vp],
+    'zav1_grid_xy': [_vp, _i32, _i32, _i32, _i32, _vp],
     'zav1_last_error': [],
     'zav1_build_quant_row': [_i32, _i32, ctypes.POINTER(QuantRow)],
     'zav1_dsp_get': [_i32],
@@ -139,6 +141,7 @@ EXPORTS = {
     'zav1_ctx_reset': [_vp],
     'zav1_ctx_stream': [_vp],
     'zav1_image_upload': [_vp, _vp, ctypes.c_size_t, _i32, _i32, _i32, ctypes.POINTER(_vp)],
+    'zav1_txsearch_frame': [_vp, _vp, _i32, _i32, _i32, _vp, _i32, _vp],
     'zav1_md_wave_submit': [_vp, _vp],
     'zav1_wave_collect': [_vp],
     'zav1_md_predict_survivors': [_vp] + [_i32] * 6 + [_vp, _i32, _vp, _i32, _vp, _i32, _vp, _vp],
@@ -421,6 +424,14 @@ class MdWave(ctypes.Structure):
                 ('d_recon_out', _vp)]
 
 
+class FrameSearch(ctypes.Structure):
+    """zav1_frame_search: per tx size block / type counts, types and the
+    device outputs of zav1_txsearch_frame."""
+    _fields_ = [('n_blocks', ctypes.c_int32 * 19), ('n_types', ctypes.c_int32 * 19),
+                ('types', (ctypes.c_uint8 * 16) * 19), ('dist', ctypes.c_void_p * 19), ('eob', ctypes.c_void_p * 19),
+                ('lsum', ctypes.c_void_p * 19)]
+
+
 class Context:
     """A device context: one stream and an arena of `budget` bytes."""
 
@@ -451,6 +462,15 @@ class Context:
         check(lib().zav1_image_upload(self.h, a.ctypes.data, a.shape[1], a.shape[1], a.shape[0], bd,
                                       ctypes.byref(p)), 'image_upload')
         return p.value
+
+    def txsearch_frame(self, d_plane, w, h, bd, qindices):
+        """The tx-type search over every size of a plane (zav1_txsearch_frame);
+        returns the FrameSearch record."""
+        q = np.ascontiguousarray(qindices, dtype=np.int32)
+        fs = FrameSearch()
+        check(lib().zav1_txsearch_frame(self.h, d_plane, w, h, bd, q.ctypes.data, q.size, ctypes.byref(fs)),
+              'txsearch_frame')
+        return fs
 
     def submit_md_wave(self, wave):
         """Returns the status code (ZAV1_ENOMEM when the arena is too small)."""

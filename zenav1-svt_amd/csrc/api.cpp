@@ -804,6 +804,22 @@ int zav1_event_elapsed_ms(void* a, void* b, float* ms) {
   return ZAV1_OK;
 }
 
+int zav1_event_query(void* ev) {
+  if (!ev) return ZAV1_EINVAL;
+  const hipError_t e = hipEventQuery((hipEvent_t)ev);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return fail(e, "event_query");
+}
+
+int zav1_grid_xy(int32_t* d_xy, int plane_w, int plane_h, int bw, int bh, zav1_stream_t s) {
+  if (!d_xy || plane_w <= 0 || plane_h <= 0 || bw <= 0 || bh <= 0) return ZAV1_EINVAL;
+  const long n = (long)(plane_w / bw) * (plane_h / bh);
+  if (n > INT_MAX) return ZAV1_EINVAL;
+  if (n > 0) ZCHECK(zav1::grid_xy(d_xy, plane_w / bw, (int)n, bw, bh, S(s)));
+  return (int)n;
+}
+
 int zav1_stream_wait_event(zav1_stream_t s, void* ev) {
   if (!ev) return ZAV1_EINVAL;
   ZCHECK(hipStreamWaitEvent(S(s), (hipEvent_t)ev, 0));
@@ -1266,6 +1282,57 @@ int zav1_ctx_destroy(zav1_ctx* c) {
   ZCHECK(e1);
   ZCHECK(e2);
   ZCHECK(e3);
+  return ZAV1_OK;
+}
+
+int zav1_txsearch_frame(zav1_ctx* c, const uint16_t* d_plane, int w, int h, int bd, const int* qindices, int n_q,
+                        zav1_frame_search* out) {
+  if (!c || !d_plane || !out || w <= 0 || h <= 0 || (bd != 8 && bd != 10) || !qindices || n_q < 1 || n_q > 8)
+    return ZAV1_EINVAL;
+  for (int q = 0; q < n_q; q++)
+    if (qindices[q] < 0 || qindices[q] > 255) return ZAV1_EINVAL;
+  // reserve every grid and output first: all or nothing
+  const size_t mark = c->used;
+  std::memset(out, 0, sizeof(*out));
+  void* xy[19] = {};
+  for (int ts = 0; ts < 19; ts++) {
+    const int bw = zav1::tx_w(ts), bh = zav1::tx_h(ts), mx = bw > bh ? bw : bh;
+    const long nb = (long)(w / bw) * (h / bh);
+    if (nb > INT_MAX / 16) {
+      c->used = mark;
+      return ZAV1_EINVAL;
+    }
+    int nt = 0;
+    if (mx == 64) out->types[ts][nt++] = 0;
+    else if (mx == 32) {
+      const uint8_t t32[4] = {0, 9, 10, 11};
+      for (int i = 0; i < 4; i++) out->types[ts][nt++] = t32[i];
+    } else
+      for (int t = 0; t < 16; t++) out->types[ts][nt++] = (uint8_t)t;
+    out->n_blocks[ts] = (int)nb;
+    out->n_types[ts] = nt;
+    if (nb == 0) continue;
+    const size_t n = (size_t)nb * nt * n_q;
+    void *d, *e, *l;
+    if (ctx_take(c, 8 * (size_t)nb, &xy[ts]) || ctx_take(c, 8 * n, &d) || ctx_take(c, 4 * n, &e) ||
+        ctx_take(c, 4 * n, &l)) {
+      c->used = mark;
+      std::memset(out, 0, sizeof(*out));
+      return ZAV1_ENOMEM;
+    }
+    out->dist[ts] = (uint64_t*)d;
+    out->eob[ts] = (int32_t*)e;
+    out->lsum[ts] = (uint32_t*)l;
+  }
+  for (int ts = 0; ts < 19; ts++) {
+    if (out->n_blocks[ts] == 0) continue;
+    int rc = zav1_grid_xy((int32_t*)xy[ts], w, h, zav1::tx_w(ts), zav1::tx_h(ts), (zav1_stream_t)c->stream);
+    if (rc < 0) return rc;
+    rc = zav1_txsearch_batch_q(d_plane, w, bd, ts, (const int32_t*)xy[ts], out->n_blocks[ts], out->types[ts],
+                               out->n_types[ts], qindices, n_q, out->dist[ts], out->eob[ts], out->lsum[ts],
+                               (zav1_stream_t)c->stream);
+    if (rc) return rc;
+  }
   return ZAV1_OK;
 }
 

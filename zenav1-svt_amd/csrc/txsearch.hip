@@ -860,4 +860,17 @@ hipError_t txsearch_batch(const uint16_t* plane, int stride, int bd, int tx_size
   }
 }
 
+__global__ void k_grid_xy(int32_t* __restrict__ xy, int nx, int n, int bw, int bh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int by = i / nx, bx = i - by * nx;
+  xy[2 * i] = bx * bw;
+  xy[2 * i + 1] = by * bh;
+}
+
+hipError_t grid_xy(int32_t* xy, int nx, int n, int bw, int bh, hipStream_t st) {
+  hipLaunchKernelGGL(k_grid_xy, dim3((n + 255) / 256), dim3(256), 0, st, xy, nx, n, bw, bh);
+  return hipGetLastError();
+}
+
 }  // namespace zav1

@@ -225,6 +225,9 @@ struct TxPlan {
 // Fused tx-type search over blocks of one tx size on a 16-bit plane (8-bit
 // content is widened by the caller).  iscan: device table of inverse scans,
 // [tx_size][class][1024] u16.  Outputs per (block, type) pair.
+// Raster grid of n blocks (nx per row) of bw x bh: xy[2i], xy[2i+1].
+hipError_t grid_xy(int32_t* xy, int nx, int n, int bw, int bh, hipStream_t st);
+
 // n_q quantizers: dev_rows[q] (device rows) and host_rows[q] (the same
 // rows on the host); outputs per (q, block, type) at
 // q * n_blocks * types.n + block * types.n + k.

@@ -120,6 +120,72 @@ def config3_recon(h=4096, w=4096, seed=3000):
     return src, np.clip(rec, 0, 255).astype(np.uint8)
 
 
+def config4_tile_column(h=4096, w=2048, seed=4000):
+    """One tile column (2048 wide, 4096 high) of the 8192x4096 10-bit frame:
+    1/f noise scaled to [64, 940] with 25% of the area as screen regions
+    (256x256 cells: flat fills from a palette of at most 8 colours, with
+    1-px glyph strokes in palette colours)."""
+    f = spectral_noise(h, w, 1.0, seed)
+    img = np.round(64 + f * (940 - 64)).astype(np.int32)
+    rnd = XorShift64Star(seed + 1)
+    cs = 256
+    for cy in range(0, h, cs):
+        for cx in range(0, w, cs):
+            if rnd.uniform() >= 0.25:
+                continue
+            pal = [rnd.randint(64, 940) for _ in range(rnd.randint(2, 8))]
+            cell = img[cy:cy + cs, cx:cx + cs]
+            cell[:] = pal[0]
+            for _ in range(rnd.randint(4, 12)):  # flat fills
+                y0, x0 = rnd.randint(0, cs - 16), rnd.randint(0, cs - 16)
+                y1, x1 = min(cs, y0 + rnd.randint(8, 96)), min(cs, x0 + rnd.randint(8, 128))
+                cell[y0:y1, x0:x1] = pal[rnd.randint(0, len(pal) - 1)]
+            for _ in range(rnd.randint(20, 60)):  # glyph strokes, 1 px wide
+                y0, x0 = rnd.randint(0, cs - 12), rnd.randint(0, cs - 12)
+                v = pal[rnd.randint(0, len(pal) - 1)]
+                if rnd.uniform() < 0.5:
+                    cell[y0, x0:x0 + rnd.randint(3, 12)] = v
+                else:
+                    cell[y0:y0 + rnd.randint(3, 12), x0] = v
+    return np.clip(img, 0, 1023).astype(np.uint16)
+
+
+def serving_mix(n, seed=5000):
+    """Requests of the serving mix: (height, width, bd, qp), area log-uniform
+    in 256^2 .. 4096x3072, aspect 1:1 .. 16:9 (either orientation), 50/50
+    8 / 10 bits, qp uniform in 20..50; sides rounded to multiples of 64."""
+    rnd = XorShift64Star(seed)
+    out = []
+    lo, hi = np.log(256.0 * 256.0), np.log(4096.0 * 3072.0)
+    for _ in range(n):
+        area = np.exp(lo + (hi - lo) * rnd.uniform())
+        aspect = 1.0 + (16.0 / 9.0 - 1.0) * rnd.uniform()
+        w = np.sqrt(area * aspect)
+        h = area / w
+        if rnd.uniform() < 0.5:
+            w, h = h, w
+        w = int(max(64, min(4096, round(w / 64) * 64)))
+        h = int(max(64, min(4096, round(h / 64) * 64)))
+        bd = 8 if rnd.uniform() < 0.5 else 10
+        qp = rnd.randint(20, 50)
+        out.append((h, w, bd, qp))
+    return out
+
+
+def serving_images(requests, seed=5000):
+    """Content of the serving requests: crops, at seeded offsets, of one
+    4096x4096 config-2 style field (1/f^2 noise + lines), at each request's
+    size and bit depth (8-bit content as the 10-bit field >> 2)."""
+    pool = config2_luma(4096, 4096, seed=seed)
+    rnd = XorShift64Star(seed + 7)
+    out = []
+    for h, w, bd, _ in requests:
+        y0, x0 = rnd.randint(0, 4096 - h), rnd.randint(0, 4096 - w)
+        img = np.ascontiguousarray(pool[y0:y0 + h, x0:x0 + w])
+        out.append((img >> 2).astype(np.uint16) if bd == 8 else img)
+    return out
+
+
 def qp_to_qindex(qp):
     """CQP qp -> qindex (rust/crates/svtav1-encoder/src/rate_control.rs:129)."""
     if qp < 62:
