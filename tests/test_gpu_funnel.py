@@ -186,6 +186,11 @@ def test_md_fast_satd_matches_two_pass(gpu, bd):
                 nl = 0 if x == 0 else (h if rng.uniform() < 0.7 else int(rng.integers(1, h + 1)))
                 nbl = int(rng.integers(0, h + 1)) if nl == h and rng.uniform() < 0.4 else 0
                 units[u] = (x, y, nt, ntr, nl, nbl, int(rng.uniform() < 0.7), int(rng.uniform() < 0.3), 0, 0)
+            # an extension beside a short edge (not from AV1's rules, accepted):
+            # the fused kernel gives such units their own edges
+            units[8]['x'], units[8]['y'] = max(int(units[8]['x']), 1), max(int(units[8]['y']), 1)
+            units[8]['n_top_px'], units[8]['n_topright_px'] = max(1, w // 2), min(4, w)
+            units[8]['n_left_px'], units[8]['n_bottomleft_px'] = max(1, h // 2), min(4, h)
             # inconsistent availability: reads above the plane / right of it
             units[3]['y'], units[3]['n_top_px'] = 0, w
             units[5]['x'], units[5]['n_topright_px'], units[5]['n_top_px'] = PW - w, w, w

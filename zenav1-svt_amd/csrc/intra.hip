@@ -561,6 +561,70 @@ __device__ __forceinline__ bool md_unit_ok(const MdUnit& u, int w, int h, int pw
          u.x + *na <= pw && u.y + *nl <= ph && *na <= kEdge - 1 && *nl <= kEdge;
 }
 
+// Shared directional edges of a unit.  A directional candidate's final
+// edges depend on its angle only through its zone (which edges and
+// extensions it needs) and its corner-filter / edge-strength / upsample
+// choices, so the unit's candidates fall into a few edge configurations.
+// The unit's raw edges are assembled once (with the top-right and
+// bottom-left extensions: a candidate that needs no extension never reads
+// past the unextended length, and the edge filters stop at their own
+// counts), then every configuration's filtered / upsampled edges once; the
+// candidates then only predict.  Key: zone (1..3) | corner << 2 |
+// above strength << 3 | left strength << 5 | upsample above << 7 |
+// upsample left << 8.
+constexpr int kMdSlots = 16;
+constexpr uint16_t kKeyNonDir = 0xffff, kKeyConst = 0xfffe;
+
+template <int W, int H>
+__device__ __forceinline__ uint16_t md_edge_key(const MdCand& cd, const MdUnit& U, int fi_ok) {
+  const bool dir = (cd.mode >= 3 && cd.mode <= 8) || ((cd.mode == 1 || cd.mode == 2) && cd.delta != 0);
+  if (!dir || (fi_ok && cd.fi_mode < 5)) return kKeyNonDir;
+  // an extension without the full edge beside it (not produced by AV1's
+  // availability rules) makes the extended fill differ by zone: own edges
+  if ((U.n_topright_px > 0 && U.n_top_px < W) || (U.n_bottomleft_px > 0 && U.n_left_px < H)) return kKeyNonDir;
+  const int pa = MODE_TO_ANGLE[cd.mode] + 3 * cd.delta;
+  const bool na = pa < 180, nl = pa > 90;
+  if ((!na && U.n_left_px == 0) || (!nl && U.n_top_px == 0)) return kKeyConst;
+  const int zone = pa < 90 ? 1 : (pa < 180 ? 2 : 3);
+  int cf = 0, sa = 0, sl = 0, ua = 0, ul = 0;
+  if (U.edge_filter) {
+    cf = na && nl && (W + H >= 24);
+    if (na && U.n_top_px > 0) sa = edge_strength(W, H, pa - 90, U.filt_type);
+    if (nl && U.n_left_px > 0) sl = edge_strength(H, W, pa - 180, U.filt_type);
+    ua = na && use_upsample(W, H, pa - 90, U.filt_type);
+    ul = nl && use_upsample(H, W, pa - 180, U.filt_type);
+  }
+  return (uint16_t)(zone | cf << 2 | sa << 3 | sl << 5 | ua << 7 | ul << 8);
+}
+
+// Filtered / upsampled edges of configuration `key` from the raw edges, by
+// one wave (tmp: the wave's scratch).
+template <int W, int H>
+__device__ void md_build_slot(uint16_t key, const int16_t* rawA, const int16_t* rawL, int16_t* A, int16_t* Lf,
+                              int16_t* tmp, const MdUnit& U, int maxv, int lane) {
+  const WaveSync sync{};
+  const int zone = key & 3;
+  const bool na = zone <= 2, nl = zone >= 2, nr = zone == 1, nbm = zone == 3;
+  const int cf = (key >> 2) & 1, sa = (key >> 3) & 3, sl = (key >> 5) & 3, ua = (key >> 7) & 1, ul = (key >> 8) & 1;
+  for (int i = lane; i < kEdge; i += 64) {
+    A[i] = rawA[i];
+    Lf[i] = rawL[i];
+  }
+  sync();
+  if (cf) {
+    if (lane == 0) {
+      const int32_t v = (Lf[kO] * 5 + A[kO - 1] * 6 + A[kO] * 5 + 8) >> 4;
+      A[kO - 1] = (int16_t)v;
+      Lf[kO - 1] = (int16_t)v;
+    }
+    sync();
+  }
+  if (na && sa) filter_edge_wave(A, tmp, kO - 1, U.n_top_px + 1 + (nr ? H : 0), sa, lane, sync);
+  if (nl && sl) filter_edge_wave(Lf, tmp, kO - 1, U.n_left_px + 1 + (nbm ? W : 0), sl, lane, sync);
+  if (na && ua) upsample_wave(A, tmp, W + (nr ? H : 0), maxv, lane, sync);
+  if (nl && ul) upsample_wave(Lf, tmp, H + (nbm ? W : 0), maxv, lane, sync);
+}
+
 template <typename Pix, int W, int H>
 __global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ src, const Pix* __restrict__ recon,
                                                       int stride, int pw, int ph, int bd,
@@ -573,6 +637,11 @@ __global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ sr
   __shared__ int16_t above[4][kEdge], left[4][kEdge], tmp[4][kEdge];
   __shared__ int16_t fbuf[4][G::FI ? (H + 1) * 33 : 1];
   __shared__ uint32_t acc[64];
+  __shared__ int16_t rawA[kEdge], rawL[kEdge];
+  __shared__ int16_t slotA[kMdSlots][kEdge], slotL[kMdSlots][kEdge];
+  __shared__ uint16_t skey[kMdSlots];
+  __shared__ uint8_t cslot[64];  // candidate -> slot; 0xff: own edges, 0xfe: constant fill
+  __shared__ int nslot_sh;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int u = xcd_chunked(blockIdx.x, gridDim.x);
   if (u >= n_units) return;
@@ -591,6 +660,53 @@ __global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ sr
   __syncthreads();
 
   const CacheNb nb{rowc, colc, U.x, U.y};
+  const int base = 128 << (bd - 8), maxv = (1 << bd) - 1;
+  // ---- the unit's raw directional edges (with both extensions) ----
+  {
+    const int n_top = U.n_top_px, n_tr = U.n_topright_px, n_left = U.n_left_px, n_bl = U.n_bottomleft_px;
+    for (int i = tid; i < kEdge; i += 256) {
+      int32_t a = base - 1, l = base + 1;
+      const int k = i - kO;
+      if (k >= 0 && k < W + H) {
+        if (n_top > 0) a = nb(U.x + min(k, (n_tr > 0 ? W + n_tr : n_top) - 1), U.y - 1);
+        else if (n_left > 0) a = nb(U.x - 1, U.y);
+        if (n_left > 0) l = nb(U.x - 1, U.y + min(k, (n_bl > 0 ? H + n_bl : n_left) - 1));
+        else if (n_top > 0) l = nb(U.x, U.y - 1);
+      } else if (k == -1) {
+        const int32_t tl = (n_top > 0 && n_left > 0) ? nb(U.x - 1, U.y - 1)
+                         : n_top > 0 ? nb(U.x, U.y - 1) : (n_left > 0 ? nb(U.x - 1, U.y) : base);
+        a = tl;
+        l = tl;
+      }
+      rawA[i] = (int16_t)a;
+      rawL[i] = (int16_t)l;
+    }
+  }
+  // ---- edge configurations: candidate keys, distinct keys -> slots ----
+  if (wave == 0) {
+    const uint16_t key = lane < nc ? md_edge_key<W, H>(cl.c[lane], U, G::FI) : kKeyNonDir;
+    uint64_t todo = __ballot(key < kKeyConst);
+    int ns = 0;
+    uint8_t my = key == kKeyConst ? 0xfe : 0xff;
+    while (todo) {
+      const int lead = __ffsll((unsigned long long)todo) - 1;
+      const uint16_t k = (uint16_t)__shfl((int)key, lead, 64);
+      const bool mine = key == k;
+      if (ns < kMdSlots) {
+        if (mine) my = (uint8_t)ns;
+        if (lane == 0) skey[ns] = k;
+        ns++;
+      }
+      todo &= ~__ballot(mine);
+    }
+    if (lane < nc) cslot[lane] = my;
+    if (lane == 0) nslot_sh = ns;
+  }
+  __syncthreads();
+  const int nslot = nslot_sh;
+  for (int sl = wave; sl < nslot; sl += 4)
+    md_build_slot<W, H>(skey[sl], rawA, rawL, slotA[sl], slotL[sl], tmp[wave], U, maxv, lane);
+  __syncthreads();
   // batches of at most BATCH candidates, spread so every wave gets work
   const int per = (nc + 3) / 4;
   const int bsz = per < G::BATCH ? per : G::BATCH;
@@ -598,6 +714,21 @@ __global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ sr
     const int nb_c = min(bsz, nc - m0);
     for (int b = 0; b < nb_c; b++) {
       const MdCand cd = cl.c[m0 + b];
+      const TileResOut<W, H, G::T> ro{res[wave] + b * G::N, srcb};
+      const int cs = cslot[m0 + b];
+      if (cs < kMdSlots) {  // directional, shared edges
+        const uint16_t key = skey[cs];
+        dr_predict_edges(MODE_TO_ANGLE[cd.mode] + 3 * cd.delta, slotA[cs], slotL[cs], W, H, (key >> 7) & 1,
+                         (key >> 8) & 1, maxv, ro, lane);
+        continue;
+      }
+      if (cs == 0xfe) {  // directional without the edge it needs: one value
+        const int pa = MODE_TO_ANGLE[cd.mode] + 3 * cd.delta;
+        const int32_t v = pa > 90 ? (U.n_top_px > 0 ? nb(U.x, U.y - 1) : base + 1)
+                                  : (U.n_left_px > 0 ? nb(U.x - 1, U.y) : base - 1);
+        for (int i = lane; i < G::N; i += 64) ro(i, v);
+        continue;
+      }
       IntraJob j;
       j.x = U.x; j.y = U.y; j.w = (uint8_t)W; j.h = (uint8_t)H;
       j.mode = cd.mode; j.delta = cd.delta; j.fi_mode = G::FI ? cd.fi_mode : (uint8_t)5;
@@ -605,10 +736,10 @@ __global__ __launch_bounds__(256) void k_md_fast_satd(const Pix* __restrict__ sr
       j.n_top_px = U.n_top_px; j.n_topright_px = U.n_topright_px;
       j.n_left_px = U.n_left_px; j.n_bottomleft_px = U.n_bottomleft_px;
       j.out_off = 0; j.pad1 = 0;
-      predict_wave<W, H>(j, bd, nb, TileResOut<W, H, G::T>{res[wave] + b * G::N, srcb}, WaveSync{}, above[wave],
-                         left[wave], tmp[wave], fbuf[wave], lane);
+      predict_wave<W, H>(j, bd, nb, ro, WaveSync{}, above[wave], left[wave], tmp[wave], fbuf[wave], lane);
       WaveSync{}();
     }
+    WaveSync{}();
     wave_satd_tiles<G::T>(res[wave], nb_c * G::TPJ, G::TPJ, acc + m0, lane);
   }
   __syncthreads();
