@@ -196,10 +196,10 @@ def loopfilter_stage(zav1, synth, args, cpu=True):
            'value': round(N * N / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s', 'steps': args.lf_steps,
            'ms_per_step': round(ms_step, 3),
            'phase_ms': {ph: round(phase_ms[i] / args.lf_steps, 4) for i, ph in enumerate(st.PHASES)},
-           'roofline': {'bound': 'valu', 'kernel': 'k_sgr_stats<uint8_t> (luma)', 'unit': 'Tops/s',
+           'roofline': {'bound': 'valu', 'kernel': 'k_sgr_strip<uint8_t> (luma)', 'unit': 'Tops/s',
                         'achieved': round(sg_ops / sg_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
                         'frac': round(sg_ops / sg_s / 1e12 / VALU_PEAK_TOPS, 4),
-                        'traffic': measured_traffic('k_sgr_stats'),
+                        'traffic': measured_traffic('k_sgr_strip'),
                         'avg_launch_us': round(sg_s * 1e6, 2), 'algorithmic_ops_per_launch': int(sg_ops),
                         'hbm_algorithmic_GBs': round(2 * N * N / sg_s / 1e9, 2)}}
     if cpu:

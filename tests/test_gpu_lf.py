@@ -152,6 +152,50 @@ def test_sgr_stats_bitexact(gpu, oracle, bd):
             assert np.array_equal(got[u, s], ref), (u, s, got[u, s], ref)
 
 
+def _sgr_content(kind, h, w, bd, seed):
+    rng = np.random.default_rng(seed)
+    mx = (1 << bd) - 1
+    dt = np.uint8 if bd == 8 else np.uint16
+    if kind == 'extreme':
+        # 0 / max in 2x2 cells: box variances at the largest the bit depth
+        # allows (the u32 products of the scale step at their bound)
+        cells = rng.integers(0, 2, ((h + 1) // 2, (w + 1) // 2)) * mx
+        rec = np.kron(cells, np.ones((2, 2), np.int64))[:h, :w]
+        src = rng.integers(0, mx + 1, (h, w))
+    elif kind == 'noise':
+        rec, src = rng.integers(0, mx + 1, (h, w)), rng.integers(0, mx + 1, (h, w))
+    else:  # flat halves next to noise: z = 0 and saturated z side by side
+        rec = np.full((h, w), mx // 3)
+        rec[:, w // 2:] = rng.integers(0, mx + 1, (h, w - w // 2))
+        src = np.clip(rec + rng.integers(-9, 10, (h, w)), 0, mx)
+    return src.astype(dt), rec.astype(dt)
+
+
+@pytest.mark.parametrize('pw,ph,us,ss,bd,kind', [
+    (200, 70, 32, 1, 10, 'extreme'),   # strips spanning three units, chroma offsets
+    (200, 70, 32, 1, 8, 'extreme'),
+    (130, 9, 64, 0, 8, 'noise'),       # one short unit row
+    (300, 140, 128, 0, 10, 'flat'),
+    (59, 61, 256, 0, 10, 'noise'),     # one unit narrower than a strip
+    (117, 200, 64, 0, 8, 'flat'),      # last unit row extended, odd height
+])
+def test_sgr_stats_shapes_and_extremes(gpu, oracle, pw, ph, us, ss, bd, kind):
+    """All 16 parameter sets of every unit equal the oracle across unit sizes
+    smaller and larger than a 58-column strip, ragged planes, chroma unit
+    offsets and contents at the arithmetic bounds."""
+    z = gpu
+    src, rec = _sgr_content(kind, ph, pw, bd, pw * 7 + ph)
+    nh, nv, lims = _units(oracle, us, pw, ph, ss)
+    D = z.DeviceArray
+    out = D((nh * nv, 16, 5), np.int64)
+    z.sgr_stats(D.from_host(rec), D.from_host(src), pw, ph, bd, us, ss, out)
+    got = out.to_host()
+    for u, lim in enumerate(lims):
+        for s in range(16):
+            ref = oracle.sgr_unit_stats(rec, src, bd, lim, s)
+            assert np.array_equal(got[u, s], ref), (u, lim, s, got[u, s], ref)
+
+
 def test_loop_filter_stage_full_size_sampled(gpu, oracle):
     """The 4096x4096 8-bit workload end to end; deblocked planes checked in
     full, the CDEF / restoration tables on sampled blocks and units."""

@@ -16,6 +16,7 @@
 //   SGR       self-guided filter + projection normal equations (no reference
 //             twin; AV1 self-guided filter process, fast r = 2 variant)
 #include <algorithm>
+#include <type_traits>
 
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
@@ -788,193 +789,272 @@ __constant__ int kSgrParams[16][4] = {
     {0, -1, 1, 1177},  {0, -1, 1, 925},   {2, 56, 0, -1},   {2, 22, 0, -1}};
 
 namespace {
-__device__ __forceinline__ int32_t rpot32(int32_t v, int n) { return n == 0 ? v : (v + (1 << (n - 1))) >> n; }
 __device__ __forceinline__ int x_by_xplus1(int z) {
   return z == 0 ? 1 : (z >= 255 ? 256 : (256 * z + (z + 1) / 2) / (z + 1));
 }
 }  // namespace
 
-// Self-guided projection statistics.  One workgroup per (unit, group of
-// kSgSets parameter sets).  The unit is walked in tiles of kSgSR output rows
-// x kSgTW output columns: the edge-replicated dgd samples of a tile (3-sample
-// border) are staged in LDS, the radius-1 and radius-2 box sums (value and
-// square) of the tile's A/B positions (rows / columns -1 .. n) are computed
-// once and shared by the group's sets, then per set: A/B of both passes
-// (x / (x + 1) from a 256-entry LDS table), the two filtered outputs of every
-// pixel and the five projection sums, kept per set in i64 registers.
-// out[(u * 16 + set) * 5 + {0..4}] = H00, H01, H11, C0, C1 (sums / unit area).
-// Every i32 intermediate is bounded: box sums <= 25 * 1023^2, A <= 256,
-// (256 - A) * sum * one_by_x < 2^31, B <= (256 - A) * 1024 + 1, so the
-// weighted A*d + B <= 32 * 2^18 and each filtered output is <= 2^14 (as is
-// the source term), which bounds a lane's per-tile product sums by 2^30.
-constexpr int kSgTW = 64, kSgSR = 16, kSgSets = 4;
+// Self-guided projection statistics, strip form.
+//
+// One wave walks one restoration-unit row of one 64-column strip of the
+// plane (lane L <-> column x0 - 3 + L, clamped to the plane like every
+// other read), carrying four parameter sets; a workgroup is the four waves
+// of one strip, which between them carry all 16 sets.  Everything a row
+// needs stays in registers: the last five dgd rows (and squares), the
+// radius-1 and radius-2 box sums of the row they centre on (vertical sums
+// in the lane, horizontal neighbours by DPP wave shifts), per set the last
+// three rows of the radius-1 pass's A/B and the last two odd rows of the
+// radius-2 pass's, and the set's five projection sums.  Output columns are
+// lanes 3 .. 60 (every lane a filter reaches is then inside the wave), so
+// strips advance by 58 columns and may cross unit boundaries: each lane
+// knows its unit, and at the end of the unit row the wave adds its lanes'
+// sums per unit into `out` with 64-bit atomics (integer, so the order does
+// not matter).  A zero pass before and a division pass after turn the sums
+// into the sums / unit area the projection solver reads.
+//
+// The filter is evaluated through T = 256 - A (a 256-entry u8 table in LDS,
+// indexed by z) and B' = B + 8: with weights w summing to 2^(sh - 4),
+//   rpot(sum w A * d + sum w B, sh) - 16 d = (sum w B' - (sum w T) * d) >> sh
+// exactly, since sum w A * d = 16 d * 2^sh - (sum w T) * d and the +8 per
+// weight unit is the rounding term 2^(sh - 1).
+//
+// Bounds (bd <= 10, samples inside the bit depth): box sums S1 <= 9207,
+// S2 <= 25575; the variance terms p1 <= 1.32e6, p2 <= 1.03e7 (< 2^24) and
+// p * s + 2^19 < 2^32 (the reference's own u32 product), T * S < 2^23 and
+// T * S * one_by_x + 34816 < 2^31; sum w T <= 8160, sum w B' < 2^23, so the
+// filter is one i24 multiply-add; |f|, |sv| <= 16368, so a product is below
+// 2^28 and a lane's i32 partial holds 8 rows at 10 bits (64 at 8 bits)
+// before it is folded into the i64 sum.
+constexpr int kSgHalo = 3, kSgStrip = 64 - 2 * kSgHalo;
+constexpr int kSgBoth = 0, kSgR1 = 1, kSgR0 = 2;
 
-template <typename Pix>
-__global__ __launch_bounds__(256) void k_sgr_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
-                                                   int pw, int ph, int bd, int unit_size, int ss_y, int nh,
-                                                   int64_t* __restrict__ out) {
-  constexpr int DW = kSgTW + 6, AW = kSgTW + 2, DR = kSgSR + 6, AR = kSgSR + 2, AR2 = AR / 2;
-  __shared__ uint16_t D[DR * DW];
-  __shared__ uint16_t SRC[kSgSR * kSgTW];        // the tile's source samples
-  __shared__ int32_t S1[AR * AW], Q1[AR * AW];    // radius 1, every A/B row
-  __shared__ int32_t S2[AR2 * AW], Q2[AR2 * AW];  // radius 2, A/B rows at odd unit rows (even a)
-  __shared__ uint16_t A0[AR2 * AW], A1[AR * AW];
-  __shared__ int32_t B0[AR2 * AW], B1[AR * AW];
-  __shared__ uint16_t XBY[256];
-  __shared__ uint64_t red[4];
-  constexpr int NG = 16 / kSgSets;
-  const int lb = xcd_chunked(blockIdx.x, gridDim.x);
-  const int u = lb / NG, sg = lb - u * NG, uy = u / nh, ux = u - uy * nh;
-  int hs, he, vs, ve;
-  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
-  const int W = he - hs, Hh = ve - vs;
-  const int cs = bd - 8;
-  const int tid = threadIdx.x;
-  XBY[tid] = (uint16_t)x_by_xplus1(tid);
-  int64_t acc[kSgSets][5];
-#pragma unroll
-  for (int k = 0; k < kSgSets; k++)
-#pragma unroll
-    for (int m = 0; m < 5; m++) acc[k][m] = 0;
+namespace {
+// lane i receives lane i - 1 / i + 1 of the wave (DPP wave_shr:1 / wave_shl:1;
+// the end lanes receive 0 and are never output lanes)
+__device__ __forceinline__ int32_t lane_prev(int32_t v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int32_t lane_next(int32_t v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int32_t sides(int32_t v) { return lane_prev(v) + lane_next(v); }
+__device__ __forceinline__ uint32_t mad_u24(uint32_t x, uint32_t y, uint32_t a) {
+  uint32_t r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(a));
+  return r;
+}
 
-  for (int s0 = 0; s0 < Hh; s0 += kSgSR) {
-    const int sr = min(kSgSR, Hh - s0), ar = sr + 2;
-    for (int c0 = 0; c0 < W; c0 += kSgTW) {
-      const int tw = min(kSgTW, W - c0), aw = tw + 2, dw = tw + 6;
-      __syncthreads();
-      // Lanes walk 64 consecutive columns of 4 rows at a time (no divisions);
-      // the few columns past 64 (tile halos) go to a second, linear pass.
-      const int lr0 = tid >> 6, lc0 = tid & 63;
-      const int aex = aw - 64;  // 0..2 A/B columns past 64
-      // D row R <-> unit row s0 - 3 + R; column C <-> unit column c0 - 3 + C
-      for (int R = lr0; R < sr + 6; R += 4) {
-        const int gy = clampi(vs + s0 - 3 + R, 0, ph - 1);
-        for (int C = lc0; C < dw; C += 64) {
-          const int gx = clampi(hs + c0 - 3 + C, 0, pw - 1);
-          D[R * DW + C] = (uint16_t)dgd[(long)gy * pw + gx];
+template <int KIND>
+struct SgSet {
+  static constexpr bool R1 = KIND != kSgR0, R0 = KIND != kSgR1;
+  uint32_t s0 = 0, s1 = 0;                                    // scales of the two passes
+  int32_t t1a = 0, t1b = 0, t1c = 0, b1a = 0, b1b = 0, b1c = 0;  // radius 1: rows m-2, m-1, m
+  int32_t t0a = 0, t0b = 0, b0a = 0, b0b = 0;                 // radius 2: older / newer odd row
+  int32_t p00 = 0, p01 = 0, p11 = 0, pc0 = 0, pc1 = 0;        // i32 partials
+  int64_t a00 = 0, a01 = 0, a11 = 0, ac0 = 0, ac1 = 0;
+
+  __device__ __forceinline__ void fold() {
+    if constexpr (R0) { a00 += (uint32_t)p00; ac0 += pc0; p00 = pc0 = 0; }
+    if constexpr (R1) { a11 += (uint32_t)p11; ac1 += pc1; p11 = pc1 = 0; }
+    if constexpr (R0 && R1) { a01 += p01; p01 = 0; }
+  }
+  // One row step centred on row m (ODD: m is an odd row of the unit, where
+  // the radius-2 pass has A/B); with OUT, output row m - 1 (sample d, source
+  // term sv, nd = -d).
+  template <bool ODD, bool OUT>
+  __device__ __forceinline__ void step(const uint8_t* tab, uint32_t p1, uint32_t S1, uint32_t p2, uint32_t S2,
+                                       int32_t nd, int32_t sv) {
+    if constexpr (R1) {
+      const uint32_t z = min(mad_u24(p1, s1, 1u << 19) >> 20, 255u);
+      const int32_t T = tab[z];
+      const int32_t B = (int32_t)(mad_u24(mul_u24((uint32_t)T, S1), 455u, 34816u) >> 12);
+      t1a = t1b; t1b = t1c; t1c = T;
+      b1a = b1b; b1b = b1c; b1c = B;
+    }
+    if constexpr (R0 && ODD) {
+      const uint32_t z = min(mad_u24(p2, s0, 1u << 19) >> 20, 255u);
+      const int32_t T = tab[z];
+      const int32_t B = (int32_t)(mad_u24(mul_u24((uint32_t)T, S2), 164u, 34816u) >> 12);
+      t0a = t0b; t0b = T;
+      b0a = b0b; b0b = B;
+    }
+    if constexpr (OUT) {
+      int32_t f0 = 0, f1 = 0;
+      if constexpr (R1) {  // weights 4 (centre, cross) and 3 (corners), sum 32
+        const int32_t vT = t1a + t1b + t1c, vB = b1a + b1b + b1c;
+        const int32_t wT = 4 * vT + 3 * sides(vT) + sides(t1b);
+        const int32_t wB = 4 * vB + 3 * sides(vB) + sides(b1b);
+        f1 = mad_i24(wT, nd, wB) >> 9;
+      }
+      if constexpr (R0) {
+        if constexpr (ODD) {  // even output row: the odd rows above and below, 6 / 5, sum 32
+          const int32_t vT = t0a + t0b, vB = b0a + b0b;
+          f0 = mad_i24(6 * vT + 5 * sides(vT), nd, 6 * vB + 5 * sides(vB)) >> 9;
+        } else {  // odd output row: its own row, 6 / 5, sum 16
+          f0 = mad_i24(6 * t0b + 5 * sides(t0b), nd, 6 * b0b + 5 * sides(b0b)) >> 8;
         }
       }
-      for (int R = lr0; R < sr; R += 4)
-        if (lc0 < tw) SRC[R * kSgTW + lc0] = (uint16_t)src[(long)(vs + s0 + R) * pw + hs + c0 + lc0];
-      __syncthreads();
-      // box sums; A/B position (a, b) <-> unit (s0 - 1 + a, c0 - 1 + b), whose
-      // box rows are D rows a + 2 + dy and columns b + 2 + dx
-      auto box = [&](int a, int b) {
-        int32_t s1 = 0, q1 = 0;
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-          for (int dx = -1; dx <= 1; dx++) {
-            const int32_t v = D[(a + 2 + dy) * DW + b + 2 + dx];
-            s1 += v;
-            q1 += __mul24(v, v);
-          }
-        S1[a * AW + b] = s1;
-        Q1[a * AW + b] = q1;
-        if ((a & 1) == 0) {  // unit row s0 - 1 + a is odd (s0 is even)
-          int32_t s2 = 0, q2 = 0;
-#pragma unroll
-          for (int dy = -2; dy <= 2; dy++)
-#pragma unroll
-            for (int dx = -2; dx <= 2; dx++) {
-              const int32_t v = D[(a + 2 + dy) * DW + b + 2 + dx];
-              s2 += v;
-              q2 += __mul24(v, v);
-            }
-          S2[(a >> 1) * AW + b] = s2;
-          Q2[(a >> 1) * AW + b] = q2;
-        }
-      };
-      for (int a = lr0; a < ar; a += 4)
-        if (lc0 < aw) box(a, lc0);
-      for (int i = tid; i < ar * aex; i += 256) box(aex == 2 ? i >> 1 : i, 64 + (aex == 2 ? (i & 1) : 0));
-#pragma unroll
-      for (int k = 0; k < kSgSets; k++) {
-        const int set = sg * kSgSets + k;
-        const int r0 = kSgrParams[set][0], sp0 = kSgrParams[set][1];
-        const int r1 = kSgrParams[set][2], sp1 = kSgrParams[set][3];
-        __syncthreads();  // box sums ready / previous set's A, B consumed
-        auto ab = [&](int a, int b) {
-          if (r1) {
-            const int32_t sm = S1[a * AW + b];
-            const int32_t pp = max(rpot32(Q1[a * AW + b], 2 * cs) * 9 - rpot32(sm, cs) * rpot32(sm, cs), 0);
-            const uint32_t z = (uint32_t)(((uint64_t)(uint32_t)pp * (uint32_t)sp1 + (1u << 19)) >> 20);
-            const uint32_t xa = XBY[z > 255 ? 255 : z];
-            A1[a * AW + b] = (uint16_t)xa;
-            B1[a * AW + b] = (int32_t)(((256u - xa) * (uint32_t)sm * 455u + 2048u) >> 12);
-          }
-          if (r0 && (a & 1) == 0) {
-            const int i2 = (a >> 1) * AW + b;
-            const int32_t sm = S2[i2];
-            const int32_t pp = max(rpot32(Q2[i2], 2 * cs) * 25 - rpot32(sm, cs) * rpot32(sm, cs), 0);
-            const uint32_t z = (uint32_t)(((uint64_t)(uint32_t)pp * (uint32_t)sp0 + (1u << 19)) >> 20);
-            const uint32_t xa = XBY[z > 255 ? 255 : z];
-            A0[i2] = (uint16_t)xa;
-            B0[i2] = (int32_t)(((256u - xa) * (uint32_t)sm * 164u + 2048u) >> 12);
-          }
-        };
-        for (int a = lr0; a < ar; a += 4)
-          if (lc0 < aw) ab(a, lc0);
-        for (int i = tid; i < ar * aex; i += 256) ab(aex == 2 ? i >> 1 : i, 64 + (aex == 2 ? (i & 1) : 0));
-        __syncthreads();
-        // a lane's pixels of one tile (at most 4) summed in i32: |f|, |sv| <= 2^14
-        static_assert(kSgSR * kSgTW <= 4 * 256, "i32 tile partials");
-        int32_t p00 = 0, p01 = 0, p11 = 0, pc0 = 0, pc1 = 0;
-        for (int rl = lr0; rl < sr; rl += 4) {
-          const int j = lc0;
-          if (j >= tw) break;
-          const int a = rl + 1, b = j + 1;  // A/B position of this pixel
-          const int32_t d = D[(rl + 3) * DW + j + 3];
-          const int32_t uu = d << 4;
-          const int32_t sv = ((int32_t)SRC[rl * kSgTW + j] << 4) - uu;
-          int32_t f0 = 0, f1 = 0;
-          if (r1) {
-#define AT(X, dy, dx) ((int32_t)X[(a + (dy)) * AW + b + (dx)])
-            const int32_t wa = (AT(A1, 0, 0) + AT(A1, 0, -1) + AT(A1, 0, 1) + AT(A1, -1, 0) + AT(A1, 1, 0)) * 4 +
-                               (AT(A1, -1, -1) + AT(A1, -1, 1) + AT(A1, 1, -1) + AT(A1, 1, 1)) * 3;
-            const int32_t wb = (AT(B1, 0, 0) + AT(B1, 0, -1) + AT(B1, 0, 1) + AT(B1, -1, 0) + AT(B1, 1, 0)) * 4 +
-                               (AT(B1, -1, -1) + AT(B1, -1, 1) + AT(B1, 1, -1) + AT(B1, 1, 1)) * 3;
-#undef AT
-            f1 = ((__mul24(wa, d) + wb + (1 << 8)) >> 9) - uu;
-          }
-          if (r0) {
-            // radius-2 rows live at even a: row a -> index a >> 1
-#define AT2(X, ra, dx) ((int32_t)X[((ra) >> 1) * AW + b + (dx)])
-            if ((rl & 1) == 0) {  // even unit row: rows a - 1 and a + 1 (both even a)
-              const int32_t wa = (AT2(A0, a - 1, 0) + AT2(A0, a + 1, 0)) * 6 +
-                                 (AT2(A0, a - 1, -1) + AT2(A0, a - 1, 1) + AT2(A0, a + 1, -1) + AT2(A0, a + 1, 1)) * 5;
-              const int32_t wb = (AT2(B0, a - 1, 0) + AT2(B0, a + 1, 0)) * 6 +
-                                 (AT2(B0, a - 1, -1) + AT2(B0, a - 1, 1) + AT2(B0, a + 1, -1) + AT2(B0, a + 1, 1)) * 5;
-              f0 = ((__mul24(wa, d) + wb + (1 << 8)) >> 9) - uu;
-            } else {
-              const int32_t wa = AT2(A0, a, 0) * 6 + (AT2(A0, a, -1) + AT2(A0, a, 1)) * 5;
-              const int32_t wb = AT2(B0, a, 0) * 6 + (AT2(B0, a, -1) + AT2(B0, a, 1)) * 5;
-              f0 = ((__mul24(wa, d) + wb + (1 << 7)) >> 8) - uu;
-            }
-#undef AT2
-          }
-          p00 += __mul24(f0, f0);
-          p01 += __mul24(f0, f1);
-          p11 += __mul24(f1, f1);
-          pc0 += __mul24(f0, sv);
-          pc1 += __mul24(f1, sv);
-        }
-        acc[k][0] += p00;
-        acc[k][1] += p01;
-        acc[k][2] += p11;
-        acc[k][3] += pc0;
-        acc[k][4] += pc1;
-      }
+      if constexpr (R0) { p00 = mad_i24(f0, f0, p00); pc0 = mad_i24(f0, sv, pc0); }
+      if constexpr (R1) { p11 = mad_i24(f1, f1, p11); pc1 = mad_i24(f1, sv, pc1); }
+      if constexpr (R0 && R1) p01 = mad_i24(f0, f1, p01);
     }
   }
-  const int64_t size = (int64_t)W * Hh;
-#pragma unroll
-  for (int k = 0; k < kSgSets; k++)
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-      const int64_t v = (int64_t)wg_sum_u64((uint64_t)acc[k][m], red);
-      if (tid == 0) out[((long)u * 16 + sg * kSgSets + k) * 5 + m] = v / size;
+};
+
+// Wave sum of a signed 64-bit value whose lanes are below 2^40 in magnitude:
+// the low 24 bits and the rest are summed as 32-bit values over each 16-lane
+// row (DPP) and the four row results read into scalars.
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+  const int32_t lo = dpp_sum_i32<16>((int32_t)((uint32_t)v & 0xffffffu));
+  const int32_t hi = dpp_sum_i32<16>((int32_t)(v >> 24));
+  const int64_t l = (int64_t)__builtin_amdgcn_readlane(lo, 0) + __builtin_amdgcn_readlane(lo, 16) +
+                    __builtin_amdgcn_readlane(lo, 32) + __builtin_amdgcn_readlane(lo, 48);
+  const int64_t h = (int64_t)__builtin_amdgcn_readlane(hi, 0) + __builtin_amdgcn_readlane(hi, 16) +
+                    __builtin_amdgcn_readlane(hi, 32) + __builtin_amdgcn_readlane(hi, 48);
+  return h * (1 << 24) + l;
+}
+}  // namespace
+
+template <typename Pix, int KA, int KB, int KC, int KD>
+__device__ __forceinline__ void sgr_strip_wave(const Pix* __restrict__ dgd, const Pix* __restrict__ src, int pw,
+                                               int ph, int x0, int vs, int ve, int uy, int nh, int unit_size,
+                                               const uint8_t* tab, int4 sets, int64_t* __restrict__ out) {
+  constexpr int BD = sizeof(Pix) == 1 ? 8 : 10, CS = BD - 8;
+  constexpr int FOLD = BD == 8 ? 32 : 4;  // row pairs per i32 partial
+  const int lane = threadIdx.x & 63;
+  const int col = x0 - kSgHalo + lane, gx = clampi(col, 0, pw - 1);
+  SgSet<KA> sa;
+  SgSet<KB> sb;
+  SgSet<KC> sc;
+  SgSet<KD> sd;
+  sa.s0 = kSgrParams[sets.x][1], sa.s1 = kSgrParams[sets.x][3];
+  sb.s0 = kSgrParams[sets.y][1], sb.s1 = kSgrParams[sets.y][3];
+  sc.s0 = kSgrParams[sets.z][1], sc.s1 = kSgrParams[sets.z][3];
+  sd.s0 = kSgrParams[sets.w][1], sd.s1 = kSgrParams[sets.w][3];
+  auto dg = [&](int y) -> int32_t { return dgd[(long)clampi(y, 0, ph - 1) * pw + gx]; };
+  auto sr = [&](int y) -> int32_t { return src[(long)clampi(y, 0, ph - 1) * pw + gx]; };
+  // dgd rows m-2 .. m+2 of the current step (d0..d4) and their squares
+  int32_t d0 = 0, d1 = dg(vs - 3), d2 = dg(vs - 2), d3 = dg(vs - 1), d4 = dg(vs);
+  uint32_t q0 = 0, q1 = mul_u24(d1, d1), q2 = mul_u24(d2, d2), q3 = mul_u24(d3, d3), q4 = mul_u24(d4, d4);
+  auto step = [&](auto odd_c, auto out_c, int32_t nrow, int32_t srow) {
+    constexpr bool ODD = decltype(odd_c)::value, OUT = decltype(out_c)::value;
+    d0 = d1; d1 = d2; d2 = d3; d3 = d4; d4 = nrow;
+    q0 = q1; q1 = q2; q2 = q3; q3 = q4; q4 = mul_u24(nrow, nrow);
+    const int32_t c3s = d1 + d2 + d3, c3q = (int32_t)(q1 + q2 + q3);
+    const int32_t S1 = c3s + sides(c3s), Q1 = c3q + sides(c3q);
+    const uint32_t a1 = CS ? (uint32_t)(Q1 + (1 << (2 * CS - 1))) >> (2 * CS) : (uint32_t)Q1;
+    const uint32_t e1 = CS ? (uint32_t)(S1 + (1 << (CS - 1))) >> CS : (uint32_t)S1;
+    const uint32_t p1 = (uint32_t)max((int32_t)mad_u24(a1, 9u, 0u - mul_u24(e1, e1)), 0);
+    uint32_t p2 = 0, S2 = 0;
+    if constexpr (ODD) {
+      const int32_t c5s = c3s + d0 + d4, c5q = c3q + (int32_t)(q0 + q4);
+      const int32_t s5 = sides(c5s + sides(c5s)) - c5s, q5 = sides(c5q + sides(c5q)) - c5q;
+      const uint32_t a2 = CS ? (uint32_t)(q5 + (1 << (2 * CS - 1))) >> (2 * CS) : (uint32_t)q5;
+      const uint32_t e2 = CS ? (uint32_t)(s5 + (1 << (CS - 1))) >> CS : (uint32_t)s5;
+      p2 = (uint32_t)max((int32_t)mad_u24(a2, 25u, 0u - mul_u24(e2, e2)), 0);
+      S2 = (uint32_t)s5;
     }
+    int32_t nd = 0, sv = 0;
+    if constexpr (OUT) {
+      nd = -d1;
+      sv = (srow - d1) * 16;
+    }
+    sa.template step<ODD, OUT>(tab, p1, (uint32_t)S1, p2, S2, nd, sv);
+    sb.template step<ODD, OUT>(tab, p1, (uint32_t)S1, p2, S2, nd, sv);
+    sc.template step<ODD, OUT>(tab, p1, (uint32_t)S1, p2, S2, nd, sv);
+    sd.template step<ODD, OUT>(tab, p1, (uint32_t)S1, p2, S2, nd, sv);
+  };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  // pair 0: centres vs - 1 (odd) and vs, no output yet
+  int32_t n0 = dg(vs + 1), n1 = dg(vs + 2);
+  int32_t m0 = dg(vs + 3), m1 = dg(vs + 4), r0 = sr(vs), r1 = sr(vs + 1);
+  step(T_{}, F_{}, n0, 0);
+  step(F_{}, F_{}, n1, 0);
+  int cnt = 0;
+  // pair p >= 1: centres m = vs - 1 + 2p (odd) and m + 1, output rows m - 1, m
+  for (int m = vs + 1; m <= ve; m += 2) {
+    n0 = m0, n1 = m1;
+    const int32_t s0 = r0, s1 = r1;
+    m0 = dg(m + 4), m1 = dg(m + 5), r0 = sr(m + 1), r1 = sr(m + 2);  // next pair
+    step(T_{}, T_{}, n0, s0);
+    if (m + 1 > ve) break;
+    step(F_{}, T_{}, n1, s1);
+    if (++cnt == FOLD) {
+      cnt = 0;
+      sa.fold(), sb.fold(), sc.fold(), sd.fold();
+    }
+  }
+  sa.fold(), sb.fold(), sc.fold(), sd.fold();
+  // per unit the strip's output columns reach: wave sums into `out`
+  const bool on = lane >= kSgHalo && lane < 64 - kSgHalo && col < pw;
+  const int last = min(x0 + kSgStrip, pw) - 1;
+  const int ux_lane = on ? min(col / unit_size, nh - 1) : -1;
+  const int ux_hi = min(last / unit_size, nh - 1);
+  for (int ux = min(x0 / unit_size, nh - 1); ux <= ux_hi; ux++) {
+    const bool mine = ux_lane == ux;
+    int64_t* o = out + (long)(uy * nh + ux) * 16 * 5;
+    auto put = [&](int set, int m, int64_t v) {
+      const int64_t t = wave_sum_i64(mine ? v : 0);
+      if (lane == 0) atomicAdd((unsigned long long*)&o[set * 5 + m], (unsigned long long)t);
+    };
+    auto emit = [&](auto& s, int set) {
+      using S = std::remove_reference_t<decltype(s)>;
+      if constexpr (S::R0) { put(set, 0, s.a00); put(set, 3, s.ac0); }
+      if constexpr (S::R1) { put(set, 2, s.a11); put(set, 4, s.ac1); }
+      if constexpr (S::R0 && S::R1) put(set, 1, s.a01);
+    };
+    emit(sa, sets.x);
+    emit(sb, sets.y);
+    emit(sc, sets.z);
+    emit(sd, sets.w);
+  }
+}
+
+// Grid: one workgroup of four waves per (unit row, 58-column strip); the
+// waves carry sets {0,1,2,14}, {3,4,5,15}, {6,7,10,11}, {8,9,12,13}, which
+// evens out their work (three two-pass sets and one radius-2 set against two
+// two-pass and two radius-1 sets).
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_sgr_strip(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
+                                                   int pw, int ph, int unit_size, int ss_y, int nh, int nstrips,
+                                                   int64_t* __restrict__ out) {
+  __shared__ uint8_t tab[256];  // 256 - x_by_xplus1(z)
+  tab[threadIdx.x] = (uint8_t)(256 - x_by_xplus1(threadIdx.x));
+  __syncthreads();
+  const int b = xcd_chunked(blockIdx.x, gridDim.x);
+  const int uy = b / nstrips, sx = b - uy * nstrips;
+  int hs, he, vs, ve;
+  unit_limits(unit_size, pw, ph, ss_y, 0, uy, hs, he, vs, ve);
+  const int x0 = sx * kSgStrip, w = threadIdx.x >> 6;
+  if (w == 0)
+    sgr_strip_wave<Pix, kSgBoth, kSgBoth, kSgBoth, kSgR0>(dgd, src, pw, ph, x0, vs, ve, uy, nh, unit_size, tab,
+                                                          make_int4(0, 1, 2, 14), out);
+  else if (w == 1)
+    sgr_strip_wave<Pix, kSgBoth, kSgBoth, kSgBoth, kSgR0>(dgd, src, pw, ph, x0, vs, ve, uy, nh, unit_size, tab,
+                                                          make_int4(3, 4, 5, 15), out);
+  else if (w == 2)
+    sgr_strip_wave<Pix, kSgBoth, kSgBoth, kSgR1, kSgR1>(dgd, src, pw, ph, x0, vs, ve, uy, nh, unit_size, tab,
+                                                        make_int4(6, 7, 10, 11), out);
+  else
+    sgr_strip_wave<Pix, kSgBoth, kSgBoth, kSgR1, kSgR1>(dgd, src, pw, ph, x0, vs, ve, uy, nh, unit_size, tab,
+                                                        make_int4(8, 9, 12, 13), out);
+}
+
+__global__ __launch_bounds__(256) void k_fill_i64(int64_t* p, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += gridDim.x * 256L) p[i] = 0;
+}
+
+// Sums / unit area (C division, toward zero); an empty unit reads 0.
+__global__ __launch_bounds__(256) void k_sgr_finalize(int64_t* out, int pw, int ph, int unit_size, int ss_y, int nh,
+                                                      int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int u = i / 80, uy = u / nh, ux = u - uy * nh;
+  int hs, he, vs, ve;
+  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  const int64_t size = (int64_t)(he - hs) * (ve - vs);
+  out[i] = size > 0 ? out[i] / size : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1094,13 +1174,17 @@ hipError_t unit_sse(const void* a, const void* b, int pw, int ph, int bd, int un
 
 hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, int unit_size, int ss_y, int nh,
                      int nv, int64_t* out, hipStream_t st) {
-  const unsigned gr = (unsigned)(nh * nv * (16 / kSgSets));
+  const int nstrips = (pw + kSgStrip - 1) / kSgStrip, n = nh * nv * 80;
+  hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)std::min((n + 255) / 256, 1024)), dim3(256), 0, st, out, (long)n);
+  const unsigned gr = (unsigned)(nv * nstrips);
   if (bd == 8)
-    hipLaunchKernelGGL((k_sgr_stats<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)dgd, (const uint8_t*)src,
-                       pw, ph, bd, unit_size, ss_y, nh, out);
+    hipLaunchKernelGGL((k_sgr_strip<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)dgd, (const uint8_t*)src,
+                       pw, ph, unit_size, ss_y, nh, nstrips, out);
   else
-    hipLaunchKernelGGL((k_sgr_stats<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)dgd,
-                       (const uint16_t*)src, pw, ph, bd, unit_size, ss_y, nh, out);
+    hipLaunchKernelGGL((k_sgr_strip<uint16_t>), dim3(gr), dim3(256), 0, st, (const uint16_t*)dgd,
+                       (const uint16_t*)src, pw, ph, unit_size, ss_y, nh, nstrips, out);
+  hipLaunchKernelGGL(k_sgr_finalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, pw, ph, unit_size,
+                     ss_y, nh, n);
   return hipGetLastError();
 }
 
