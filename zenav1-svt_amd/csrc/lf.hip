@@ -602,7 +602,6 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
 // ---------------------------------------------------------------------------
 // Restoration units
 // ---------------------------------------------------------------------------
-constexpr int kMaxUW = 384;  // widest unit: 1.5 x 256
 
 namespace {
 // foreach_rest_unit_in_tile (restoration.rs:1395) for a whole plane.
@@ -631,32 +630,64 @@ __device__ __forceinline__ uint64_t wg_sum_u64(uint64_t v, uint64_t* red) {
 }
 }  // namespace
 
-// Wiener statistics (compute_stats / compute_stats_hbd): one workgroup per
-// unit.  Rows are staged in strips of SR (+ 2*HALF replicated halo rows) as
-// mean-removed i16 samples.  H is split into segments (k, g): row k of H
-// against the WIN entries l = WIN*g .. WIN*g + WIN-1 of window column g (the
-// segments with g >= k / WIN cover the upper triangle); M adds WIN segments
-// with the source sample in place of y_k.  A lane owns one segment and walks
-// the pixel columns of its column range top to bottom, keeping the WIN
-// samples of window column g in a sliding register window: per pixel it
-// reads y_k and one new sample and does WIN multiply-adds.  A column's
-// products are summed in i32 (at most 16 products of |v| <= 1023) and then
-// added into i64, which reproduces the reference's i64 sums exactly.  With
-// the 5-tap window the 80 segments run in 3 column groups whose sums are
-// combined at the end.
+// Wiener statistics (compute_stats / compute_stats_hbd) on the matrix cores.
+//
+// With y_p the WIN x WIN window of mean-removed dgd samples around pixel p
+// (entry k = dx * WIN + dy) and x_p its mean-removed source sample,
+// H = sum_p y_p y_p^T and M = sum_p y_p x_p: one Gram matrix Z^T Z over the
+// unit's pixels, Z's row p holding y_p and x_p.  Each |value| <= 1023 is
+// split into two int8 digits, v = 32 * vh + vl (vh in [-32, 31], vl in
+// [0, 31]), so Z has 2 * WIN^2 + 2 columns (yh, yl, xh, xl) and Z^T Z is
+// accumulated exactly in i32 by v_mfma_i32_16x16x64_i8 over 16-column
+// blocks, upper-triangle block pairs only (28 for WIN = 7, 10 for WIN = 5);
+// H = 1024 HH + 32 (HL + HL^T) + LL and M likewise, in i64, at the end.
+//
+// A workgroup (four waves) owns one unit (or one explicit window) and walks
+// it in tiles of 16 rows x 128 columns.  A tile is staged in LDS as digit
+// planes, one per (digit, dx) with its columns shifted by dx, so that the
+// 16 samples of feature (dx, dy) at 16 consecutive pixels of a row are one
+// aligned 16-byte read; pixel columns past the tile read as 0 in every
+// plane.  A K step is 4 rows x 16 columns (64 pixels): lane group g (lanes
+// 16g .. 16g + 15) covers row g, and a lane's fragment of block b (feature
+// 16b + lane % 16) is at once its A operand (row of Z^T) and its B operand
+// (column of Z), both MFMA operands taking K from the lane group and byte in
+// the same way.  Wave w takes row quad w of the tile.  The four waves'
+// accumulators meet in LDS (ds_add) and the final H / M are formed from it.
+// The i32 sums stay exact: |digit products| <= 1024 over < 2^15 pixels.
+constexpr int kWnRB = 16, kWnCT = 128;
+typedef int32_t wn_v4i __attribute__((ext_vector_type(4)));
+
+template <int WIN>
+struct WnGeom {
+  static constexpr int W2 = WIN * WIN, HALF = WIN / 2, NF = 2 * W2 + 2, NB = (NF + 15) / 16;
+  static constexpr int NT = NB * (NB + 1) / 2;
+  static constexpr int PR = kWnRB + 2 * HALF, PLANE = PR * kWnCT, XOFF = 2 * WIN * PLANE;
+  static constexpr int LDS = XOFF + 2 * kWnRB * kWnCT;
+  static_assert(NT * 256 * 4 <= LDS, "reduction fits the staging area");
+};
+
+namespace {
+// bytes [0, vb) of a 16-byte chunk kept, the rest cleared
+__device__ __forceinline__ wn_v4i keep_bytes(wn_v4i v, int vb) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int n = vb - 4 * q;
+    const uint32_t m = n >= 4 ? 0xffffffffu : (n <= 0 ? 0u : (1u << (8 * n)) - 1u);
+    v[q] &= (int32_t)m;
+  }
+  return v;
+}
+}  // namespace
+
 template <typename Pix, int WIN>
 __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
-                                                      int pw, int ph, int bd, int unit_size, int ss_y, int nh,
-                                                      int4 lim, int64_t* __restrict__ out_m,
-                                                      int64_t* __restrict__ out_h) {
-  constexpr int W2 = WIN * WIN, HALF = WIN / 2, SR = 16;
-  constexpr int NSEG = WIN * (WIN * (WIN + 1) / 2) + WIN;
-  constexpr int NG = 256 / NSEG;
-  constexpr int YS = kMaxUW + 2 * HALF;  // LDS row stride of Y and X (constant: immediate offsets)
-  __shared__ int16_t Y[(SR + 2 * HALF) * YS];
-  __shared__ int16_t X[SR * YS];
+                                                      int pw, int ph, int unit_size, int ss_y, int nh, int4 lim,
+                                                      int64_t* __restrict__ out_m, int64_t* __restrict__ out_h) {
+  using G = WnGeom<WIN>;
+  constexpr int W2 = G::W2, HALF = G::HALF, NB = G::NB;
+  constexpr int64_t DIV = sizeof(Pix) == 1 ? 1 : 4;  // compute_stats_hbd scales by 1/4
+  __shared__ __align__(16) uint8_t lds[G::LDS];
   __shared__ uint64_t red[4];
-  __shared__ int64_t gpart[NG > 1 ? (NG - 1) * NSEG * WIN : 1];
   const int u = xcd_chunked(blockIdx.x, gridDim.x), uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   if (lim.y > lim.x) {  // one window given explicitly (the plain compute_stats)
@@ -664,99 +695,147 @@ __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dg
   } else {
     unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
   }
-  const int uw = he - hs, uh = ve - vs, tw = uw + 2 * HALF;
-  const int tid = threadIdx.x;
-  const int lr0 = tid >> 6, lc0 = tid & 63;  // 4 rows x 64 columns per pass, no divisions
+  const int uw = he - hs, uh = ve - vs;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4;
   uint64_t s = 0;
-  for (int r = lr0; r < uh; r += 4)
-    for (int c = lc0; c < uw; c += 64) s += dgd[(long)(vs + r) * pw + hs + c];
-  const int32_t avg = (int32_t)(wg_sum_u64(s, red) / ((uint64_t)uw * uh));
+  for (int r = w; r < uh; r += 4)
+    for (int c = lane; c < uw; c += 64) s += dgd[(long)(vs + r) * pw + hs + c];
+  const uint64_t area = (uint64_t)uw * (uint64_t)uh;
+  const int32_t avg = (int32_t)(area ? wg_sum_u64(s, red) / area : 0);
 
-  const int seg = tid % NSEG, cg = tid / NSEG;
-  const bool active = cg < NG;
-  int my_k = 0, my_g = 0;
-  {
-    int r = seg, k = 0;
-    for (; k < W2; k++) {
-      const int ng = WIN - k / WIN;
-      if (r < ng) break;
-      r -= ng;
+  // LDS offset of this lane's feature in block b (row g of a K step); the
+  // padding features past the last real one read feature 0 (never used)
+  int addr[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    int f = 16 * b + (lane & 15);
+    if (f >= G::NF) f = 0;
+    if (f < 2 * W2) {
+      const int d = f >= W2, k = f - d * W2, dx = k / WIN, dy = k - dx * WIN;
+      addr[b] = (d * WIN + dx) * G::PLANE + (dy + g) * kWnCT;
+    } else {
+      addr[b] = G::XOFF + (f - 2 * W2) * kWnRB * kWnCT + g * kWnCT;
     }
-    my_k = k;
-    my_g = k < W2 ? k / WIN + r : r;
   }
-  const bool is_m = my_k == W2;
-  const int dyk = is_m ? 0 : my_k % WIN, dxk = is_m ? 0 : my_k / WIN;
-  const int c_lo = cg * uw / NG, c_hi = (cg + 1) * uw / NG;
-  int64_t acc[WIN];
+  wn_v4i acc[G::NT];
 #pragma unroll
-  for (int q = 0; q < WIN; q++) acc[q] = 0;
+  for (int t = 0; t < G::NT; t++) acc[t] = wn_v4i{0, 0, 0, 0};
 
-  for (int s0 = 0; s0 < uh; s0 += SR) {
-    const int sr = min(SR, uh - s0);
-    __syncthreads();
-    for (int r = lr0; r < sr + 2 * HALF; r += 4) {
-      const int gy = clampi(vs + s0 + r - HALF, 0, ph - 1);
-      for (int c = lc0; c < tw; c += 64) {
-        const int gx = clampi(hs + c - HALF, 0, pw - 1);
-        Y[r * YS + c] = (int16_t)((int32_t)dgd[(long)gy * pw + gx] - avg);
-      }
-    }
-    for (int r = lr0; r < sr; r += 4)
-      for (int c = lc0; c < uw; c += 64)
-        X[r * YS + c] = (int16_t)((int32_t)src[(long)(vs + s0 + r) * pw + hs + c] - avg);
-    __syncthreads();
-    if (active) {
-      for (int c = c_lo; c < c_hi; c++) {
-        const int16_t* ycol = Y + c + my_g;  // window column g, row R at ycol[R * YS]
-        const int16_t* kcol = is_m ? X + c : Y + dyk * YS + c + dxk;
-        int32_t yl[WIN], part[WIN];
+  for (int r0 = 0; r0 < uh; r0 += kWnRB) {
+    const int br = min(kWnRB, uh - r0);
+    for (int ct0 = 0; ct0 < uw; ct0 += kWnCT) {
+      const int tw = min(kWnCT, uw - ct0), nc = (tw + 15) >> 4;
+      __syncthreads();  // previous tile's fragments read
+      const int ny = G::PR * nc, nx = br * nc;
+      for (int task = tid; task < ny + nx; task += 256) {
+        if (task < ny) {
+          // plane row rr, chunk j: samples at columns 16j - HALF .. 16j + 15 + HALF
+          const int rr = task / nc, j = task - rr * nc;
+          const Pix* row = dgd + (long)clampi(vs + r0 - HALF + rr, 0, ph - 1) * pw;
+          const int cb = hs + ct0 + 16 * j - HALF;
+          uint32_t wh[6], wl[6];
 #pragma unroll
-        for (int q = 0; q < WIN; q++) part[q] = 0;
+          for (int q = 0; q < 6; q++) {
+            uint32_t h = 0, l = 0;
 #pragma unroll
-        for (int q = 0; q + 1 < WIN; q++) yl[q] = ycol[q * YS];
-        auto row = [&](int r) {
-          yl[WIN - 1] = ycol[(r + WIN - 1) * YS];
-          const int32_t a = kcol[r * YS];
+            for (int e = 0; e < 4; e++) {
+              const int idx = 4 * q + e;
+              if (idx < 16 + 2 * HALF) {
+                const int32_t y = (int32_t)row[clampi(cb + idx, 0, pw - 1)] - avg;
+                h |= (uint32_t)(uint8_t)(y >> 5) << (8 * e);
+                l |= (uint32_t)(y & 31) << (8 * e);
+              }
+            }
+            wh[q] = h, wl[q] = l;
+          }
+          const int vb = tw - 16 * j;
 #pragma unroll
-          for (int q = 0; q < WIN; q++) part[q] += __mul24(a, yl[q]);  // |a|, |y| <= 1023
+          for (int dx = 0; dx < WIN; dx++) {
+            wn_v4i vh, vl;
 #pragma unroll
-          for (int q = 0; q + 1 < WIN; q++) yl[q] = yl[q + 1];
-        };
-        if (sr == SR) {
-#pragma unroll
-          for (int r = 0; r < SR; r++) row(r);
+            for (int q = 0; q < 4; q++) {
+              const int n = q + (dx >> 2), sh = dx & 3;
+              vh[q] = (int32_t)(sh ? __builtin_amdgcn_alignbyte(wh[n + 1], wh[n], sh) : wh[n]);
+              vl[q] = (int32_t)(sh ? __builtin_amdgcn_alignbyte(wl[n + 1], wl[n], sh) : wl[n]);
+            }
+            if (vb < 16) vh = keep_bytes(vh, vb), vl = keep_bytes(vl, vb);
+            *(wn_v4i*)(lds + dx * G::PLANE + rr * kWnCT + 16 * j) = vh;
+            *(wn_v4i*)(lds + (WIN + dx) * G::PLANE + rr * kWnCT + 16 * j) = vl;
+          }
         } else {
-          for (int r = 0; r < sr; r++) row(r);
-        }
+          const int t2 = task - ny, rr = t2 / nc, j = t2 - rr * nc;
+          const Pix* row = src + (long)(vs + r0 + rr) * pw;
+          const int cb = hs + ct0 + 16 * j;
+          wn_v4i vh, vl;
 #pragma unroll
-        for (int q = 0; q < WIN; q++) acc[q] += part[q];
+          for (int q = 0; q < 4; q++) {
+            uint32_t h = 0, l = 0;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              const int32_t x = (int32_t)row[min(cb + 4 * q + e, pw - 1)] - avg;
+              h |= (uint32_t)(uint8_t)(x >> 5) << (8 * e);
+              l |= (uint32_t)(x & 31) << (8 * e);
+            }
+            vh[q] = (int32_t)h, vl[q] = (int32_t)l;
+          }
+          const int vb = tw - 16 * j;
+          if (vb < 16) vh = keep_bytes(vh, vb), vl = keep_bytes(vl, vb);
+          *(wn_v4i*)(lds + G::XOFF + rr * kWnCT + 16 * j) = vh;
+          *(wn_v4i*)(lds + G::XOFF + kWnRB * kWnCT + rr * kWnCT + 16 * j) = vl;
+        }
+      }
+      __syncthreads();
+      if (4 * w < br) {  // wave w: tile rows 4w .. 4w + 3
+        const bool rowok = 4 * w + g < br, ragged = 4 * w + 4 > br;
+        for (int j = 0; j < nc; j++) {
+          const int off = 4 * w * kWnCT + 16 * j;
+          wn_v4i fr[NB];
+#pragma unroll
+          for (int b = 0; b < NB; b++) fr[b] = *(const wn_v4i*)(lds + addr[b] + off);
+          if (ragged) {
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+              if (!rowok) fr[b] = wn_v4i{0, 0, 0, 0};
+          }
+          int t = 0;
+#pragma unroll
+          for (int bi = 0; bi < NB; bi++)
+#pragma unroll
+            for (int bj = bi; bj < NB; bj++, t++)
+              acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fr[bi], fr[bj], acc[t], 0, 0, 0);
+        }
       }
     }
   }
-  if constexpr (NG > 1) {
-    if (active && cg > 0) {
+  // the four waves' blocks summed in LDS: block t, row i = 4g + r, column lane % 16
+  __syncthreads();
+  int32_t* R = (int32_t*)lds;
+  for (int i = tid; i < G::NT * 256; i += 256) R[i] = 0;
+  __syncthreads();
 #pragma unroll
-      for (int q = 0; q < WIN; q++) gpart[((cg - 1) * NSEG + seg) * WIN + q] = acc[q];
+  for (int t = 0; t < G::NT; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) atomicAdd(&R[t * 256 + (4 * g + r) * 16 + (lane & 15)], acc[t][r]);
+  __syncthreads();
+  auto at = [&](int i, int j) -> int64_t {  // (Z^T Z)[i][j] from the upper block triangle
+    if ((i >> 4) > (j >> 4)) {
+      const int x = i;
+      i = j, j = x;
     }
-    __syncthreads();
-    if (cg == 0) {
-#pragma unroll
-      for (int q = 0; q < WIN; q++)
-        for (int g = 1; g < NG; g++) acc[q] += gpart[((g - 1) * NSEG + seg) * WIN + q];
-    }
-  }
-  if (!active || cg != 0) return;
-  const int64_t div = bd == 10 ? 4 : 1;
-#pragma unroll
-  for (int q = 0; q < WIN; q++) {
-    const int l = my_g * WIN + q;
-    const int64_t v = acc[q] / div;
-    if (is_m) {
-      out_m[(long)u * W2 + l] = v;
-    } else if (l >= my_k) {
-      out_h[(long)u * W2 * W2 + my_k * W2 + l] = v;
-      out_h[(long)u * W2 * W2 + l * W2 + my_k] = v;
+    const int bi = i >> 4, bj = j >> 4, t = bi * NB - bi * (bi - 1) / 2 + (bj - bi);
+    return R[t * 256 + (i & 15) * 16 + (j & 15)];
+  };
+  auto join = [&](int i, int j) -> int64_t {  // digits (h, l) of i and j recombined
+    return 1024 * at(i, j) + 32 * (at(i, W2 + j) + at(j, W2 + i)) + at(W2 + i, W2 + j);
+  };
+  for (int i = tid; i < W2 * W2 + W2; i += 256) {
+    if (i < W2 * W2) {
+      const int k = i / W2, l = i - k * W2;
+      out_h[(long)u * W2 * W2 + i] = join(k, l) / DIV;
+    } else {
+      const int k = i - W2 * W2, x = 2 * W2;
+      const int64_t m = 1024 * at(k, x) + 32 * (at(k, x + 1) + at(W2 + k, x)) + at(W2 + k, x + 1);
+      out_m[(long)u * W2 + k] = m / DIV;
     }
   }
 }
@@ -1148,7 +1227,7 @@ hipError_t wiener_stats(const void* dgd, const void* src, int pw, int ph, int bd
   const unsigned gr = (unsigned)(nh * nv);
 #define ZW(P, WN)                                                                                              \
   hipLaunchKernelGGL((k_wiener_stats<P, WN>), dim3(gr), dim3(256), 0, st, (const P*)dgd, (const P*)src, pw, ph, \
-                     bd, unit_size, ss_y, nh, make_int4(0, 0, 0, 0), M, H)
+                     unit_size, ss_y, nh, make_int4(0, 0, 0, 0), M, H)
   if (bd == 8) {
     if (win == 7) ZW(uint8_t, 7);
     else ZW(uint8_t, 5);
@@ -1262,7 +1341,7 @@ hipError_t plain_wiener_window(const void* dgd, const void* src, int pw, int ph,
   const int4 lim = make_int4(hs, he, vs, ve);
 #define ZP(P, WN)                                                                                              \
   hipLaunchKernelGGL((k_wiener_stats<P, WN>), dim3(1), dim3(256), 0, st, (const P*)dgd, (const P*)src, pw, ph, \
-                     bd, 64, 0, 1, lim, M, H)
+                     64, 0, 1, lim, M, H)
   if (bd == 8) {
     if (win == 7) ZP(uint8_t, 7);
     else ZP(uint8_t, 5);
