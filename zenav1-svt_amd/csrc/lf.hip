@@ -474,21 +474,89 @@ __device__ __forceinline__ bool all_skip_8x8(const uint8_t* skip, int mi_cols, i
 }
 }  // namespace
 
+namespace {
+__device__ __forceinline__ int32_t med3_i32(int32_t a, int32_t b, int32_t c) {
+  int32_t r;
+  asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+// constrain(d, thr, damping) with |d| = ad and sh = max(0, damping - msb(thr))
+// wave-uniform: sign(d) * min(|d|, max(0, thr - (|d| >> sh))) as a clamp of d
+// to +-max(0, thr - (|d| >> sh)).
+__device__ __forceinline__ int32_t cons_u(int32_t d, uint32_t ad, uint32_t thr, uint32_t sh) {
+  const int32_t m = (int32_t)__builtin_elementwise_sub_sat(thr, ad >> sh);
+  return med3_i32(d, -m, m);
+}
+// x + round(sum / 16) (ties away from zero) clamped to [mn, mx], as the error
+// s_v - out: sx = s_v - x, [lo, hi] = [mn - x, mx - x].  The tap sums stay
+// far inside i16 (|sum| <= 2 * 4 * 60 + 2 * 2 * 60 + 12 * 16 = 912), so the
+// reference's wrapping i16 accumulation never wraps.
+__device__ __forceinline__ int32_t cdef_err(int32_t sum, int32_t sx, int32_t lo, int32_t hi) {
+  return sx - med3_i32((sum + 8 + (sum >> 31)) >> 4, lo, hi);
+}
+
+// Differences of one pixel's taps from x for one direction (uniform), the
+// clamp window relative to x, and the tap magnitudes.
+struct TapsU {
+  int32_t dp[4], ds[8];
+  uint32_t ap[4], as[8];
+  int32_t lo, hi;
+};
+__device__ __forceinline__ void gather_u(const uint16_t* c, int dir, int32_t x, TapsU& t) {
+  int32_t mn = x, mx = x;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int dk = kCdefDir[dir + 2][k], da = kCdefDir[dir + 4][k], db = kCdefDir[dir][k];
+    const int32_t v[6] = {(int16_t)c[dk], (int16_t)c[-dk], (int16_t)c[da], (int16_t)c[-da], (int16_t)c[db],
+                          (int16_t)c[-db]};
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      mx = max(mx, (uint16_t)v[m] != kLarge ? v[m] : x);
+      mn = min(mn, v[m]);
+      const int32_t d = v[m] - x;
+      if (m < 2) t.dp[2 * k + m] = d, t.ap[2 * k + m] = (uint32_t)abs(d);
+      else t.ds[4 * k + m - 2] = d, t.as[4 * k + m - 2] = (uint32_t)abs(d);
+    }
+  }
+  t.lo = mn - x;
+  t.hi = mx - x;
+}
+// Secondary sums for S = 1..3 (strengths 1, 2, 4 << cs); out[0] = 0.
+__device__ __forceinline__ void sec_u(const TapsU& t, int cs, int dmp, int32_t out[4]) {
+  out[0] = 0;
+#pragma unroll
+  for (int S = 1; S < 4; S++) {
+    const uint32_t st = (uint32_t)((S + (S == 3)) << cs);
+    const uint32_t sh = (uint32_t)max(0, dmp - msb32(st));
+    int32_t a = 0, b = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) a += cons_u(t.ds[m], t.as[m], st, sh), b += cons_u(t.ds[4 + m], t.as[4 + m], st, sh);
+    out[S] = 2 * a + b;
+  }
+}
+}  // namespace
+
 // One workgroup per 64x64 luma filter block: the 8x8 directions, then all
 // 64 (primary, secondary) strength pairs for every pixel of the blocks in
 // the fb's search list (not all-skip); per-pair SSE against the source
 // reduced over the block.  dirvar (optional) receives (dir, var) per 8x8 for
 // the chroma pass (0, 0 for blocks outside the list).
+//
+// A wave filters one 8x8 block at a time (lane = pixel), so the block's
+// direction, variance and every adjusted primary strength are wave-uniform
+// (scalar registers and scalar tap-offset loads).  Consecutive strengths P
+// whose adjusted primary strength repeats give the same outputs, so their
+// squared errors are computed once.  The per-pair sums of a wave's 16 blocks
+// are reduced over each 16-lane row by DPP and added in LDS.
 template <typename Pix>
 __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, const Pix* __restrict__ src,
                                                    int width, int height, int bd, int damping, int sub, int nhfb,
                                                    const uint8_t* __restrict__ skip, int mi_cols, int mi_rows,
                                                    uint64_t* __restrict__ mse, int32_t* __restrict__ dirvar) {
   __shared__ uint16_t tile[70 * kTS];
-  __shared__ int32_t sdir[64];
+  __shared__ int32_t sdir[64], svar[64];
   __shared__ uint8_t sact[64];
-  __shared__ uint8_t sadj[64][16];
-  __shared__ uint32_t red[4][64];
+  __shared__ uint32_t tot[4][64];
   const int fb = xcd_chunked(blockIdx.x, gridDim.x);
   const int fbr = fb / nhfb, fbc = fb - fbr * nhfb;
   const int cs = bd - 8;
@@ -510,9 +578,8 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
     const bool act = by < nby && bx < nbx && !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + by * 2, fbc * 16 + bx * 2);
     if (act) d = find_dir(tile + (kVB + by * 8) * kTS + kHB + bx * 8, cs, &v);
     sdir[tid] = d;
+    svar[tid] = v;
     sact[tid] = act;
-    // adjusted primary strength of every P (at most 60 at 10 bits)
-    for (int P = 0; P < 16; P++) sadj[tid][P] = (uint8_t)adjust_strength(P << cs, v);
     if (dirvar) {
       dirvar[((long)fb * 64 + tid) * 2] = d;
       dirvar[((long)fb * 64 + tid) * 2 + 1] = v;
@@ -521,22 +588,70 @@ __global__ __launch_bounds__(256) void k_cdef_luma(const Pix* __restrict__ rec, 
   __syncthreads();
 
   const int dmp = damping + cs;
+  const int wave = tid >> 6, lane = tid & 63, pr = lane >> 3, pc = lane & 7;
   uint32_t acc[64];
 #pragma unroll
   for (int g = 0; g < 64; g++) acc[g] = 0;
-  for (int p = tid; p < 4096; p += 256) {
-    const int r = p >> 6, c = p & 63;
-    const int blk = (r >> 3) * 8 + (c >> 3);
-    if (!sact[blk] || ((r & 7) % sub) != 0) continue;
-    const uint8_t* adj = sadj[blk];
-    cdef_pixel(tile + (kVB + r) * kTS + kHB + c, sdir[blk], (int32_t)src[(long)(y0 + r) * width + x0 + c], cs, dmp,
-               [adj](int P) { return (int32_t)adj[P]; }, acc);
+  for (int blk = wave; blk < 64; blk += 4) {
+    if (!__builtin_amdgcn_readfirstlane(sact[blk])) continue;
+    const int dir = __builtin_amdgcn_readfirstlane(sdir[blk]);
+    const int var = __builtin_amdgcn_readfirstlane(svar[blk]);
+    const int r = (blk >> 3) * 8 + pr, c = (blk & 7) * 8 + pc;
+    const uint16_t* cp = tile + (kVB + r) * kTS + kHB + c;
+    const int32_t x = (int16_t)cp[0];
+    TapsU tb, t0;
+    gather_u(cp, dir, x, tb);
+    gather_u(cp, 0, x, t0);
+    int32_t sx = (int32_t)src[(long)(y0 + r) * width + x0 + c] - x;
+    if ((pr % sub) != 0) sx = 0, tb.lo = tb.hi = t0.lo = t0.hi = 0;  // row not sampled: every error 0
+    int32_t secb[4], sec0[4];
+    sec_u(tb, cs, dmp, secb);
+    sec_u(t0, cs, dmp, sec0);
+    // P = 0 filters along direction 0 with no primary taps; (0, 0) is the
+    // unfiltered block
+    acc[0] = (uint32_t)mad_i24(sx, sx, (int32_t)acc[0]);
+#pragma unroll
+    for (int S = 1; S < 4; S++) {
+      const int32_t e = cdef_err(sec0[S], sx, t0.lo, t0.hi);
+      acc[S] = (uint32_t)mad_i24(e, e, (int32_t)acc[S]);
+    }
+    int tq_prev = -1;
+    int32_t e2[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int P = 1; P < 16; P++) {
+      const int tq = adjust_strength(P << cs, var);
+      if (tq != tq_prev) {
+        tq_prev = tq;
+        int32_t pri = 0;
+        if (tq) {
+          const uint32_t sh = (uint32_t)max(0, dmp - msb32((uint32_t)tq));
+          const int par = (tq >> cs) & 1, t0w = par ? 3 : 4, t1w = par ? 3 : 2;
+          pri = mad_i24(t0w, cons_u(tb.dp[0], tb.ap[0], tq, sh) + cons_u(tb.dp[1], tb.ap[1], tq, sh),
+                        __mul24(t1w, cons_u(tb.dp[2], tb.ap[2], tq, sh) + cons_u(tb.dp[3], tb.ap[3], tq, sh)));
+        }
+#pragma unroll
+        for (int S = 0; S < 4; S++) {
+          const int32_t e = cdef_err(pri + secb[S], sx, tb.lo, tb.hi);
+          e2[S] = __mul24(e, e);
+        }
+      }
+#pragma unroll
+      for (int S = 0; S < 4; S++) acc[P * 4 + S] += (uint32_t)e2[S];
+    }
   }
-  reduce_rows(acc, red);
+  // per-pair sums: 16-lane rows by DPP, the wave's four rows in scalars, one
+  // LDS row per wave
+#pragma unroll
+  for (int g = 0; g < 64; g++) {
+    const int32_t v = dpp_sum_i32<16>((int32_t)acc[g]);
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readlane(v, 0) + (uint32_t)__builtin_amdgcn_readlane(v, 16) +
+                       (uint32_t)__builtin_amdgcn_readlane(v, 32) + (uint32_t)__builtin_amdgcn_readlane(v, 48);
+    if (lane == 0) tot[wave][g] = t;
+  }
   __syncthreads();
   if (tid < 64) {
-    const uint64_t tot = (uint64_t)red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-    mse[(long)fb * 64 + tid] = (tot >> (2 * cs)) * (uint64_t)sub;
+    const uint64_t t = (uint64_t)tot[0][tid] + tot[1][tid] + tot[2][tid] + tot[3][tid];
+    mse[(long)fb * 64 + tid] = (t >> (2 * cs)) * (uint64_t)sub;
   }
 }
 
