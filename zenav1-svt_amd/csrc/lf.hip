@@ -388,20 +388,6 @@ __device__ __forceinline__ void gather_taps(const uint16_t* c, int dir, PixTaps&
   pt.mx = mx;
 }
 
-// Secondary-tap sums of one geometry for the 4 secondary strengths.
-__device__ __forceinline__ void sec_sums(const PixTaps& t, int cs, int damping, int32_t out[4]) {
-  out[0] = 0;
-#pragma unroll
-  for (int S = 1; S < 4; S++) {
-    const int32_t st = (S + (S == 3)) << cs;
-    const int32_t sh = max(0, damping - msb32((uint32_t)st));
-    int32_t s = 0;
-#pragma unroll
-    for (int m = 0; m < 8; m++) s += (m < 4 ? 2 : 1) * constrain(t.d_s[m], st, sh);
-    out[S] = s;
-  }
-}
-
 // Primary-tap sum for primary strength tq (> 0) with tap parity par.
 __device__ __forceinline__ int32_t pri_sum(const PixTaps& t, int32_t tq, int par, int damping) {
   const int32_t sh = max(0, damping - msb32((uint32_t)tq));
@@ -413,51 +399,6 @@ __device__ __forceinline__ int32_t pri_sum(const PixTaps& t, int32_t tq, int par
 __device__ __forceinline__ int32_t cdef_out(int32_t x, int32_t sum, int32_t mn, int32_t mx) {
   const int32_t s16 = (int16_t)sum;  // the reference accumulates in wrapping i16
   return clampi(x + ((8 + s16 - (s16 < 0)) >> 4), mn, mx);
-}
-
-// Sum a per-lane 64-entry strength row over each wave into red[wave].
-__device__ __forceinline__ void reduce_rows(const uint32_t* acc, uint32_t (*red)[64]) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int g = 0; g < 64; g++) {
-    uint32_t v = acc[g];
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    if (lane == 0) red[wave][g] = v;
-  }
-}
-
-// SSE of every (P, S) strength pair at one pixel, added to acc[P * 4 + S].
-// adj(P) gives the primary strength used for P (adjusted for luma).
-template <typename Adj>
-__device__ __forceinline__ void cdef_pixel(const uint16_t* cp, int dir, int32_t s_v, int cs, int dmp, Adj adj,
-                                           uint32_t* acc) {
-  const int32_t x = (int16_t)cp[0];
-  PixTaps tb, t0;
-  gather_taps(cp, dir, tb);
-  gather_taps(cp, 0, t0);
-  int32_t secb[4], sec0[4];
-  sec_sums(tb, cs, dmp, secb);
-  sec_sums(t0, cs, dmp, sec0);
-  // P = 0 filters along direction 0 with no primary taps; (0, 0) is the
-  // unfiltered block
-  const int32_t e0 = s_v - x;
-  acc[0] += (uint32_t)__mul24(e0, e0);
-#pragma unroll
-  for (int S = 1; S < 4; S++) {
-    const int32_t e = s_v - cdef_out(x, sec0[S], t0.mn, t0.mx);
-    acc[S] += (uint32_t)__mul24(e, e);
-  }
-#pragma unroll
-  for (int P = 1; P < 16; P++) {
-    const int32_t tq = adj(P);
-    const int32_t pri = tq ? pri_sum(tb, tq, (tq >> cs) & 1, dmp) : 0;
-#pragma unroll
-    for (int S = 0; S < 4; S++) {
-      const int32_t e = s_v - cdef_out(x, pri + secb[S], tb.mn, tb.mx);
-      acc[P * 4 + S] += (uint32_t)__mul24(e, e);
-    }
-  }
 }
 
 }  // namespace
@@ -700,10 +641,41 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
       const int r = p >> 5, c = p & 31;
       if (r >= nr || c >= nc || !sact[(r >> 2) * 8 + (c >> 2)] || ((r & 3) % sub) != 0) continue;
       const int dir = dirvar[((long)fb * 64 + (r >> 2) * 8 + (c >> 2)) * 2];
-      cdef_pixel(tile + (kVB + r) * kTS + kHB + c, dir, (int32_t)srcp[(long)(y0 + r) * cw + x0 + c], cs, dmp,
-                 [cs](int P) { return (int32_t)(P << cs); }, acc);
+      const uint16_t* cp = tile + (kVB + r) * kTS + kHB + c;
+      const int32_t x = (int16_t)cp[0];
+      TapsU tb, t0;
+      gather_u(cp, dir, x, tb);
+      gather_u(cp, 0, x, t0);
+      const int32_t sx = (int32_t)srcp[(long)(y0 + r) * cw + x0 + c] - x;
+      int32_t secb[4], sec0[4];
+      sec_u(tb, cs, dmp, secb);
+      sec_u(t0, cs, dmp, sec0);
+      acc[0] = (uint32_t)mad_i24(sx, sx, (int32_t)acc[0]);
+#pragma unroll
+      for (int S = 1; S < 4; S++) {
+        const int32_t e = cdef_err(sec0[S], sx, t0.lo, t0.hi);
+        acc[S] = (uint32_t)mad_i24(e, e, (int32_t)acc[S]);
+      }
+#pragma unroll
+      for (int P = 1; P < 16; P++) {  // unadjusted primary strength P << cs, tap parity P & 1
+        const uint32_t tq = (uint32_t)(P << cs), sh = (uint32_t)max(0, dmp - msb32(tq));
+        const int t0w = (P & 1) ? 3 : 4, t1w = (P & 1) ? 3 : 2;
+        const int32_t pri = t0w * (cons_u(tb.dp[0], tb.ap[0], tq, sh) + cons_u(tb.dp[1], tb.ap[1], tq, sh)) +
+                            t1w * (cons_u(tb.dp[2], tb.ap[2], tq, sh) + cons_u(tb.dp[3], tb.ap[3], tq, sh));
+#pragma unroll
+        for (int S = 0; S < 4; S++) {
+          const int32_t e = cdef_err(pri + secb[S], sx, tb.lo, tb.hi);
+          acc[P * 4 + S] = (uint32_t)mad_i24(e, e, (int32_t)acc[P * 4 + S]);
+        }
+      }
     }
-    reduce_rows(acc, red);
+#pragma unroll
+    for (int g = 0; g < 64; g++) {
+      const int32_t v = dpp_sum_i32<16>((int32_t)acc[g]);
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readlane(v, 0) + (uint32_t)__builtin_amdgcn_readlane(v, 16) +
+                         (uint32_t)__builtin_amdgcn_readlane(v, 32) + (uint32_t)__builtin_amdgcn_readlane(v, 48);
+      if ((tid & 63) == 0) red[tid >> 6][g] = t;
+    }
     __syncthreads();
     if (tid < 64) {
       const uint64_t tot = (uint64_t)red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
