@@ -226,6 +226,57 @@ int zav1_sse_i32_batch(const int32_t *d_a, const int32_t *d_b, uint64_t *d_out, 
 int zav1_satd_batch(const int16_t *d_res, int32_t *d_coeff, int32_t *d_satd, int n, int n_jobs,
                     zav1_stream_t s);
 
+/* Coefficient rate tables (1/512 bit) the RDOQ trellis prices with, in
+ * the reference's layout: one (txs_ctx, plane) of TxbCosts
+ * (rust/crates/svtav1-encoder/src/quant.rs:459, C LvMapCoeffCost), one
+ * (eob_multi_size, plane) of EobCosts (:469, LvMapEobCost), and a frame's
+ * complete set, CoeffCostTables (:475): txb[txs_ctx * 2 + plane],
+ * eob[eob_multi_size][plane].  Built by the caller from its CDFs
+ * (build_coeff_cost_tables, :514). */
+typedef struct {
+  int32_t txb_skip_cost[13][2];
+  int32_t base_eob_cost[4][3];
+  int32_t base_cost[42][8];
+  int32_t eob_extra_cost[9][2];
+  int32_t dc_sign_cost[3][2];
+  int32_t lps_cost[21][26];
+} zav1_txb_costs;
+typedef struct {
+  int32_t eob_cost[2][11];
+} zav1_eob_costs;
+typedef struct {
+  zav1_txb_costs txb[10];
+  zav1_eob_costs eob[7][2];
+} zav1_coeff_costs;
+
+/* One block of an RDOQ batch: the per-call fields of OptimizeCtx
+ * (quant.rs:887). */
+typedef struct {
+  int64_t rdmult;       /* RDCOST multiplier (rdoq_rdmult_full, quant.rs:923) */
+  uint32_t coeff;       /* element offset of the block's packed coefficients */
+  uint8_t tx_class;     /* 0 2D, 1 horizontal, 2 vertical (TX_TYPE_TO_CLASS) */
+  uint8_t plane_type;   /* 0 luma, 1 chroma: selects the cost tables */
+  uint8_t txb_skip_ctx; /* < 13 */
+  uint8_t dc_sign_ctx;  /* < 3 */
+  uint8_t sharpness;    /* sharpness_flag: no eob shortening, no skip */
+  uint8_t pad[7];
+} zav1_rdoq_job;
+
+/* RDOQ: the trellis requantization of n_jobs blocks of tx_size after fp
+ * quantization at (qindex, bd), in place on d_qcoeff / d_dqcoeff (packed,
+ * coded size <= 32x32, raster) with d_eob[j] in (scan order) and out.
+ * Replaces optimize_b (rust/crates/svtav1-encoder/src/quant.rs:1312) on the
+ * path without quantization matrices.  d_costs is the frame's
+ * zav1_coeff_costs; d_scans holds tx_size's three scans (zav1_scan_order
+ * classes 0, 1, 2, each of the coded block's length).  Blocks with eob 0 are
+ * left as they are.  cut_off_num / cut_off_denum: rdoq_cutoffs (:1551),
+ * 0 / 0 for the full trellis.  A job with a field out of range is left as it
+ * is. */
+int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, int32_t *d_qcoeff,
+                    int32_t *d_dqcoeff, uint16_t *d_eob, const zav1_rdoq_job *d_jobs, int n_jobs,
+                    const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,
+                    uint32_t cut_off_denum, zav1_stream_t s);
+
 /* AV1 coefficient scan of tx_size's packed (<= 32x32) block for a scan
  * class (0 default diagonal, 1 row, 2 column; the reference's
  * TX_TYPE_TO_SCAN_INDEX classes).  Replaces scan()

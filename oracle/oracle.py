@@ -431,3 +431,46 @@ def sgr_unit_stats(dgd, src, bd, lim, pset):
     out = np.zeros(5, np.int64)
     lrlib().or_sgr_unit_stats(d.ctypes.data, s.ctypes.data, d.shape[1], d.shape[0], bd, *lim, pset, out.ctypes.data)
     return out
+
+
+def _bind_rdoq(L):
+    L.or_optimize_b.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.or_optimize_b.restype = ctypes.c_int
+    L.or_rdoq_eob_cost.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int]
+    for f in ('or_adjusted_tx_size', 'or_txs_ctx', 'or_eob_multi_size'):
+        getattr(L, f).argtypes = [ctypes.c_int]
+
+
+def optimize_b(tcoeff, qcoeff, dqcoeff, eob, scan_, dequant, txb_costs, eob_costs, rdmult, tx_size, tx_class,
+               txb_skip_ctx=0, dc_sign_ctx=0, sharpness=0, cut_off_num=0, cut_off_denum=0):
+    """enc/quant.rs:1312 optimize_b on one packed block; returns (q, dq, eob).
+    txb_costs: int32[944] (one TxbCosts), eob_costs: int32[22] (one EobCosts)."""
+    L = lib()
+    _bind_rdoq(L)
+    t = np.ascontiguousarray(tcoeff, dtype=np.int32)
+    q = np.array(qcoeff, dtype=np.int32)
+    d = np.array(dqcoeff, dtype=np.int32)
+    sc = np.ascontiguousarray(scan_, dtype=np.uint16)
+    dqv = np.ascontiguousarray(dequant, dtype=np.int32)
+    tc = np.ascontiguousarray(txb_costs, dtype=np.int32)
+    ec = np.ascontiguousarray(eob_costs, dtype=np.int32)
+    e = L.or_optimize_b(t.ctypes.data, q.ctypes.data, d.ctypes.data, int(eob), sc.ctypes.data, dqv.ctypes.data,
+                        tc.ctypes.data, ec.ctypes.data, int(rdmult), tx_size, tx_class, txb_skip_ctx, dc_sign_ctx,
+                        sharpness, cut_off_num, cut_off_denum)
+    return q, d, e
+
+
+def rdoq_eob_cost(eob, txb_costs, eob_costs, tx_class):
+    L = lib()
+    _bind_rdoq(L)
+    tc = np.ascontiguousarray(txb_costs, dtype=np.int32)
+    ec = np.ascontiguousarray(eob_costs, dtype=np.int32)
+    return L.or_rdoq_eob_cost(eob, ec.ctypes.data, tc.ctypes.data, tx_class)
+
+
+def rdoq_size_ctx(tx_size):
+    """(adjusted tx size, txs_ctx, eob multi size) of coeff_c.rs:69/:119/:63."""
+    L = lib()
+    _bind_rdoq(L)
+    return L.or_adjusted_tx_size(tx_size), L.or_txs_ctx(tx_size), L.or_eob_multi_size(tx_size)

@@ -48,6 +48,14 @@
  *                       case; rdcost by hand-computed values.
  *   self-guided stats   PARITY UNPINNED: no reference twin; checked only by
  *                       invariants (flat input, projection identities).
+ *   RDOQ trellis        the reference's own invariant test (dequant mirror,
+ *                       nonzero last coefficient, enc/quant.rs:1852); brute
+ *                       force of the single-coefficient block (every keep /
+ *                       lower / skip decision priced by hand from the
+ *                       tables); level-only lowering, sign and eob
+ *                       monotonicity; rdmult = 0 and prohibitive-rate
+ *                       limits.  The multi-coefficient context chain is
+ *                       pinned only by these invariants (no captures).
  */
 #ifndef ZAV1_ORACLE_H
 #define ZAV1_ORACLE_H
@@ -207,6 +215,31 @@ void or_md_fast_select(const uint32_t *satd, const uint32_t *rate, uint64_t lamb
 void or_md_funnel(const void *src, const void *recon, int stride, int bd, const or_intra_job *jobs, int n_units,
                   int n_cands, const uint32_t *rate, uint64_t lambda, int k, int tx_size, int tx_type, int qindex,
                   int use_b, uint8_t *sel, uint64_t *sel_cost, uint64_t *dist, int32_t *eob, uint8_t *cul);
+
+/* Coefficient rate tables (1/512 bit): one (txs_ctx, plane) of C
+ * LvMapCoeffCost (enc/quant.rs:459 TxbCosts) and one (eob_multi_size,
+ * plane) of LvMapEobCost (:469 EobCosts). */
+typedef struct {
+  int32_t txb_skip_cost[13][2];
+  int32_t base_eob_cost[4][3];
+  int32_t base_cost[42][8];
+  int32_t eob_extra_cost[9][2];
+  int32_t dc_sign_cost[3][2];
+  int32_t lps_cost[21][26];
+} or_txb_costs;
+typedef struct {
+  int32_t eob_cost[2][11];
+} or_eob_costs;
+int or_adjusted_tx_size(int tx_size);
+int or_txs_ctx(int tx_size);
+int or_eob_multi_size(int tx_size);
+int or_rdoq_eob_cost(int eob, const or_eob_costs *e, const or_txb_costs *c, int tx_class);
+/* enc/quant.rs:1312 optimize_b without quantization matrices on one packed
+ * block (raster, adjusted size); eob > 0; returns the new eob. */
+int or_optimize_b(const int32_t *tcoeff, int32_t *qcoeff, int32_t *dqcoeff, int eob, const uint16_t *scan,
+                  const int32_t dequant[2], const or_txb_costs *c, const or_eob_costs *e, int64_t rdmult,
+                  int tx_size, int tx_class, int txb_skip_ctx, int dc_sign_ctx, int sharpness_flag,
+                  int cut_off_num, int cut_off_denum);
 
 #ifdef __cplusplus
 }

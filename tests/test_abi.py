@@ -49,6 +49,14 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     assert L.zav1_build_quant_row(300, 8, ctypes.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
     assert L.zav1_build_quant_row(10, 12, ctypes.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
     assert L.zav1_fwd_txfm2d_batch(None, None, None, 25, 1, None) == zav1.ZAV1_EINVAL
+    # RDOQ: size, depth, qindex, cut-off and pointer validation before any device work
+    fake = ctypes.c_void_p(1)
+    assert zav1.RDOQ_JOB.itemsize == 24 and zav1.COEFF_COSTS_WORDS == 10 * 944 + 14 * 22
+    assert L.zav1_rdoq_batch(19, 100, 8, fake, fake, fake, fake, fake, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_rdoq_batch(0, 100, 12, fake, fake, fake, fake, fake, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_rdoq_batch(0, 256, 8, fake, fake, fake, fake, fake, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_rdoq_batch(0, 100, 8, fake, fake, fake, fake, fake, 1, fake, fake, 60, 0, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_rdoq_batch(0, 100, 8, fake, fake, fake, fake, None, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL
     qs = (ctypes.c_int * 3)(10, 300, 20)
     assert L.zav1_txsearch_batch_q(None, 64, 10, 0, None, 1, ctypes.byref(ctypes.c_uint8(0)), 1, qs, 3, None, None,
                                    None, None) == zav1.ZAV1_EINVAL

@@ -1,0 +1,70 @@
+"""GPU parity of the RDOQ trellis (zav1_rdoq_batch) against the oracle's
+optimize_b: every tx size, the three tx classes, both plane types, 8 and
+10 bits, random rdmult / contexts / sharpness, the full trellis and the
+level-2 / level-3 cut-offs; levels, dequantized values and eobs bit-exact."""
+import numpy as np
+import pytest
+
+import rdoq_cases as RC
+import rdoq_tables as RT
+
+pytestmark = pytest.mark.gpu
+
+
+def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70):
+    w, h = RC.coded_dims(tx_size)
+    n = w * h
+    costs = RT.coeff_costs(seed)
+    _, txs, ems = oracle.rdoq_size_ctx(tx_size)
+    ts, qs, dqs, eobs, jobs_l = [], [], [], [], []
+    scans = np.concatenate([oracle.scan(tx_size, c) for c in (0, 1, 2)]).astype(np.uint16)
+    rng = np.random.default_rng(seed)
+    dqv = None
+    for cls in (0, 1, 2):
+        t, q, dq, eob, scan, dqv = RC.blocks(oracle, tx_size, cls, n_per_class, seed * 3 + cls, qindex, bd)
+        for b in range(n_per_class):
+            j = np.zeros(1, z.RDOQ_JOB)[0]
+            j['rdmult'] = int(rng.choice([0, 109013, 1054880, 6493388, int(rng.integers(1, 20_000_000))]))
+            j['coeff'] = len(ts) * n
+            j['tx_class'] = cls
+            j['plane_type'] = int(rng.integers(0, 2))
+            j['txb_skip_ctx'] = int(rng.integers(0, 13))
+            j['dc_sign_ctx'] = int(rng.integers(0, 3))
+            j['sharpness'] = int(rng.random() < 0.15)
+            jobs_l.append(j)
+            ts.append(t[b]), qs.append(q[b]), dqs.append(dq[b]), eobs.append(eob[b])
+    jobs = np.array(jobs_l, dtype=z.RDOQ_JOB)
+    T, Q, DQ = np.stack(ts), np.stack(qs), np.stack(dqs)
+    E = np.array(eobs, np.uint16)
+    D = z.DeviceArray
+    dt, dq_, ddq, de = D.from_host(T), D.from_host(Q), D.from_host(DQ), D.from_host(E)
+    z.rdoq_batch(tx_size, qindex, bd, dt, dq_, ddq, de, D.from_host(jobs), len(jobs), D.from_host(costs),
+                 D.from_host(scans), *cut)
+    gq, gdq, ge = dq_.to_host(), ddq.to_host(), de.to_host()
+    changed = 0
+    for b, j in enumerate(jobs):
+        cls = int(j['tx_class'])
+        scan = scans[RC.SCAN_CLASS[cls] * n:(RC.SCAN_CLASS[cls] + 1) * n]
+        if E[b] == 0:
+            assert ge[b] == 0 and np.array_equal(gq[b], Q[b]) and np.array_equal(gdq[b], DQ[b])
+            continue
+        tc, ec = RT.tables_for(costs, txs, ems, int(j['plane_type']))
+        rq, rdq, re = oracle.optimize_b(T[b], Q[b], DQ[b], int(E[b]), scan, dqv, tc, ec, int(j['rdmult']), tx_size,
+                                        cls, int(j['txb_skip_ctx']), int(j['dc_sign_ctx']), int(j['sharpness']), *cut)
+        assert int(ge[b]) == re, (tx_size, b, int(ge[b]), re)
+        assert np.array_equal(gq[b], rq), (tx_size, b, np.nonzero(gq[b] != rq)[0][:6])
+        assert np.array_equal(gdq[b], rdq), (tx_size, b)
+        changed += int((rq != Q[b]).any())
+    assert changed > len(jobs) // 4  # the trellis did work on most blocks
+
+
+@pytest.mark.parametrize('tx_size', list(range(19)))
+def test_rdoq_batch_bitexact(gpu, oracle, tx_size):
+    run_case(gpu, oracle, tx_size, 8, 100, (0, 0), 500 + tx_size)
+
+
+@pytest.mark.parametrize('tx_size,bd,qindex,cut', [(0, 10, 40, (60, 100)), (2, 10, 200, (80, 100)),
+                                                   (3, 8, 20, (60, 100)), (9, 10, 120, (0, 0)),
+                                                   (16, 8, 255, (80, 100)), (4, 10, 0, (0, 0))])
+def test_rdoq_batch_depths_and_cutoffs(gpu, oracle, tx_size, bd, qindex, cut):
+    run_case(gpu, oracle, tx_size, bd, qindex, cut, 900 + tx_size * 7 + bd)

@@ -121,6 +121,8 @@ EXPORTS = {
     'zav1_fwd_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_inv_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _i32, _vp],
     'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
+    'zav1_rdoq_batch': [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
+                        _vp],
     'zav1_sse_i32_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_satd_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_scan_order': [_i32, _i32, _vp],
@@ -284,6 +286,21 @@ def inv_txfm2d_batch(d_in, d_out, d_types, tx_size, n_jobs, bd, stream=None):
 def quantize_batch(d_coeff, d_q, d_dq, d_qidx, n_per_job, n_jobs, log_scale, bd, kind, stream=None):
     check(lib().zav1_quantize_batch(d_coeff.ptr, d_q.ptr, d_dq.ptr, d_qidx.ptr, n_per_job, n_jobs,
                                     log_scale, bd, kind, stream), 'quantize_batch')
+
+
+# zav1_rdoq_job / zav1_coeff_costs as numpy layouts (include/zav1_dsp.h)
+RDOQ_JOB = np.dtype([('rdmult', '<i8'), ('coeff', '<u4'), ('tx_class', 'u1'), ('plane_type', 'u1'),
+                     ('txb_skip_ctx', 'u1'), ('dc_sign_ctx', 'u1'), ('sharpness', 'u1'), ('pad', 'u1', 7)])
+TXB_COST_WORDS, EOB_COST_WORDS = 944, 22
+COEFF_COSTS_WORDS = 10 * TXB_COST_WORDS + 14 * EOB_COST_WORDS
+
+
+def rdoq_batch(tx_size, qindex, bd, d_tcoeff, d_qcoeff, d_dqcoeff, d_eob, d_jobs, n_jobs, d_costs, d_scans,
+               cut_off_num=0, cut_off_denum=0, stream=None):
+    """RDOQ of n_jobs blocks in place (zav1_rdoq_batch)."""
+    check(lib().zav1_rdoq_batch(tx_size, qindex, bd, d_tcoeff.ptr, d_qcoeff.ptr, d_dqcoeff.ptr, d_eob.ptr,
+                                d_jobs.ptr, n_jobs, d_costs.ptr, d_scans.ptr, cut_off_num, cut_off_denum, stream),
+          'rdoq_batch')
 
 
 def sse_i32_batch(d_a, d_b, d_out, n_per_job, n_jobs, stream=None):

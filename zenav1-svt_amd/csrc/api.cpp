@@ -869,6 +869,25 @@ int zav1_quantize_batch(const int32_t* d_coeff, int32_t* d_qcoeff, int32_t* d_dq
   return ZAV1_OK;
 }
 
+static_assert(sizeof(zav1_rdoq_job) == sizeof(zav1::RdoqJob), "rdoq job layout");
+static_assert(sizeof(zav1_txb_costs) == 944 * 4 && sizeof(zav1_eob_costs) == 22 * 4, "cost table layout");
+static_assert(sizeof(zav1_coeff_costs) == (10 * 944 + 14 * 22) * 4, "cost tables layout");
+
+int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, int32_t* d_qcoeff, int32_t* d_dqcoeff,
+                    uint16_t* d_eob, const zav1_rdoq_job* d_jobs, int n_jobs, const zav1_coeff_costs* d_costs,
+                    const uint16_t* d_scans, uint32_t cut_off_num, uint32_t cut_off_denum, zav1_stream_t s) {
+  if (tx_size < 0 || tx_size >= 19 || qindex < 0 || qindex > 255 || (bd != 8 && bd != 10) || n_jobs < 0 ||
+      (cut_off_num != 0 && cut_off_denum == 0) || (n_jobs > 0 && (!d_tcoeff || !d_qcoeff || !d_dqcoeff || !d_eob ||
+                                                                  !d_jobs || !d_costs || !d_scans)))
+    return ZAV1_EINVAL;
+  zav1::QuantRow row;
+  build_row(qindex, bd, &row);
+  ZCHECK(zav1::rdoq_batch(tx_size, d_tcoeff, d_qcoeff, d_dqcoeff, d_eob, reinterpret_cast<const zav1::RdoqJob*>(d_jobs),
+                          n_jobs, reinterpret_cast<const int32_t*>(d_costs), d_scans, row.dequant[0],
+                          row.dequant[1], cut_off_num, cut_off_denum, S(s)));
+  return ZAV1_OK;
+}
+
 int zav1_sse_i32_batch(const int32_t* d_a, const int32_t* d_b, uint64_t* d_out, int n_per_job, int n_jobs,
                        zav1_stream_t s) {
   if (n_per_job <= 0 || n_jobs < 0) return ZAV1_EINVAL;

@@ -1,0 +1,465 @@
+// RDOQ trellis for gfx950: svt_av1_optimize_b
+// (rust/crates/svtav1-encoder/src/quant.rs:1312) on the path without
+// quantization matrices, with the rate helpers of quant.rs:636-:906 and the
+// coefficient contexts of rust/crates/svtav1-entropy/src/coeff_c.rs.
+//
+// The trellis is a sequential reverse scan per transform block (each step
+// prices a coefficient against the levels of the coefficients after it,
+// which earlier steps may have lowered), so the parallel axis is the block:
+// one lane per block, 64 blocks per workgroup.  A lane's level map lives in
+// LDS as 4-bit levels min(|q|, 15), eight per dword, dwords interleaved
+// across the 64 lanes.  Capping at 15 instead of the reference's 127 leaves
+// every context unchanged: nz_mag clips each level at 3, and br_ctx's
+// min((l1 + l2 + l3 + 1) >> 1, 6) saturates once the sum reaches 11.
+// Reads past the block's right or bottom edge return 0, which is what the
+// reference's zero padding (4 columns, >= 6 rows) holds.  The cost tables of
+// the launch's size (both plane types) and its three scans are staged in
+// LDS once per workgroup.
+#include "zav1_dev.h"
+#include "zav1_kernels.h"
+
+namespace zav1 {
+
+namespace {
+// one TxbCosts table (quant.rs:459) as 944 words, one EobCosts (:469) as 22
+constexpr int kTxbWords = 944, kEobWords = 22;
+constexpr int kSkip = 0, kBaseEob = 26, kBase = 38, kEobExtra = 374, kDcSign = 392, kLps = 398;
+constexpr int kCls2D = 0, kClsHoriz = 1, kClsVert = 2;
+
+constexpr int c_txw(int t) { return tx_w(t) > 32 ? 32 : tx_w(t); }
+constexpr int c_txh(int t) { return tx_h(t) > 32 ? 32 : tx_h(t); }
+constexpr int c_log2(int v) { return v <= 1 ? 0 : 1 + c_log2(v >> 1); }
+constexpr int c_sqr(int t) { return c_log2(tx_w(t) < tx_h(t) ? tx_w(t) : tx_h(t)) - 2; }
+constexpr int c_sqr_up(int t) { return c_log2(tx_w(t) > tx_h(t) ? tx_w(t) : tx_h(t)) - 2; }
+// coeff_c.rs:119 txsize_entropy_ctx; :63 TXSIZE_LOG2_MINUS4, which follows
+// the coded (32-capped) dimensions
+constexpr int c_txs_ctx(int t) { return (c_sqr(t) + c_sqr_up(t) + 1) >> 1; }
+constexpr int c_eob_ms(int t) { return c_log2(c_txw(t)) + c_log2(c_txh(t)) - 4; }
+// quant.rs:238 TX_SCALE_TAB
+constexpr int c_scale(int t) {
+  return tx_w(t) * tx_h(t) > 1024 ? 2 : (tx_w(t) * tx_h(t) > 256 ? 1 : 0);
+}
+
+__device__ __forceinline__ int msb(uint32_t v) { return 31 - __clz(v); }
+__device__ __forceinline__ int64_t rdcost(int64_t rdmult, int64_t rate, int64_t dist) {
+  return ((rate * rdmult + 256) >> 9) + (dist << 7);
+}
+__device__ __forceinline__ int64_t coeff_dist(int32_t t, int32_t dq, int shift) {
+  const int64_t d = ((int64_t)t - dq) * ((int64_t)1 << shift);
+  return d * d;
+}
+__device__ __forceinline__ int golomb_cost(int a) {
+  return a >= 15 ? 512 * (2 * (msb((uint32_t)(a - 14)) + 1) - 1) : 0;
+}
+
+template <int TXS>
+struct Trellis {
+  static constexpr int AW = c_txw(TXS), AH = c_txh(TXS), BWL = c_log2(AW), N = AW * AH, NW = (N + 7) / 8;
+  static constexpr int SHIFT = c_scale(TXS);
+  const int32_t* tc;   // this block's coefficients
+  int32_t* qc;
+  int32_t* dq;
+  uint32_t* lv;        // level words of lane 0; lane stride 1, word stride 64
+  const int32_t* cost; // TxbCosts of the block's plane type
+  const int32_t* ecost;
+  const uint16_t* scan;
+  int64_t rdmult;
+  int32_t dqv[2];
+  int cls, dc_ctx;
+
+  __device__ __forceinline__ int level(int r, int c) const {
+    if (r >= AH || c >= AW) return 0;
+    const int p = r * AW + c;
+    return (lv[(p >> 3) * 64] >> ((p & 7) * 4)) & 15;
+  }
+  __device__ __forceinline__ void set_level(int p, int v) {
+    uint32_t& w = lv[(p >> 3) * 64];
+    const int sh = (p & 7) * 4;
+    w = (w & ~(15u << sh)) | ((uint32_t)(v < 15 ? v : 15) << sh);
+  }
+  // coeff_c.rs:344 nz_map_ctx_offset_2d: shape from the unadjusted size
+  __device__ __forceinline__ static int offset_2d(int ci) {
+    if (ci == 0) return 0;
+    const int row = ci >> BWL, col = ci - (row << BWL);
+    if (tx_w(TXS) < tx_h(TXS)) {
+      if (row < 2) return 11;
+    } else if (tx_w(TXS) > tx_h(TXS) && col < 2) {
+      return 16;
+    }
+    if (row + col < 2) return 1;
+    if (row + col < 4) return 6;
+    return 21;
+  }
+  // coeff_c.rs:406 nz_map_ctx (+ :306 nz_mag, :380 from_stats)
+  __device__ int lower_ctx(int ci, int si, bool is_last) const {
+    if (is_last) {
+      if (si == 0) return 0;
+      if (si <= N / 8) return 1;
+      if (si <= N / 4) return 2;
+      return 3;
+    }
+    const int r = ci >> BWL, c = ci & (AW - 1);
+    auto c3 = [](int v) { return v < 3 ? v : 3; };
+    int mag = c3(level(r, c + 1)) + c3(level(r + 1, c));
+    if (cls == kCls2D) {
+      mag += c3(level(r + 1, c + 1)) + c3(level(r, c + 2)) + c3(level(r + 2, c));
+    } else if (cls == kClsVert) {
+      mag += c3(level(r + 2, c)) + c3(level(r + 3, c)) + c3(level(r + 4, c));
+    } else {
+      mag += c3(level(r, c + 2)) + c3(level(r, c + 3)) + c3(level(r, c + 4));
+    }
+    if ((cls | ci) == 0) return 0;
+    const int ctx = min((mag + 1) >> 1, 4);
+    if (cls == kCls2D) return ctx + offset_2d(ci);
+    const int pos = cls == kClsHoriz ? c : r;
+    return ctx + (pos == 0 ? 26 : (pos == 1 ? 31 : 36));
+  }
+  // coeff_c.rs:456 br_ctx_eob
+  __device__ __forceinline__ int br_ctx_eob(int ci) const {
+    const int r = ci >> BWL, c = ci & (AW - 1);
+    if (ci == 0) return 0;
+    if ((cls == kCls2D && r < 2 && c < 2) || (cls == kClsHoriz && c == 0) || (cls == kClsVert && r == 0)) return 7;
+    return 14;
+  }
+  // coeff_c.rs:537 br_ctx
+  __device__ int br_ctx(int ci) const {
+    const int r = ci >> BWL, c = ci & (AW - 1);
+    int mag = level(r, c + 1) + level(r + 1, c);
+    mag += cls == kCls2D ? level(r + 1, c + 1) : (cls == kClsHoriz ? level(r, c + 2) : level(r + 2, c));
+    mag = min((mag + 1) >> 1, 6);
+    if (ci == 0) return mag;
+    if ((cls == kCls2D && r < 2 && c < 2) || (cls == kClsHoriz && c == 0) || (cls == kClsVert && r == 0))
+      return mag + 7;
+    return mag + 14;
+  }
+  __device__ __forceinline__ int base_cost(int ctx, int k) const { return cost[kBase + ctx * 8 + k]; }
+  __device__ __forceinline__ int lps(int ctx, int k) const { return cost[kLps + ctx * 26 + k]; }
+  __device__ __forceinline__ int br_cost(int level_, int ctx) const {
+    return lps(ctx, min(level_ - 3, 12)) + golomb_cost(level_);
+  }
+  // quant.rs:758 get_coeff_cost_general
+  __device__ int cost_general(bool is_last, int ci, int a, int sign, int ctx) const {
+    int c = is_last ? cost[kBaseEob + ctx * 3 + min(a, 3) - 1] : base_cost(ctx, min(a, 3));
+    if (a) {
+      c += ci == 0 ? cost[kDcSign + dc_ctx * 2 + sign] : 512;
+      if (a > 2) c += br_cost(a, is_last ? br_ctx_eob(ci) : br_ctx(ci));
+    }
+    return c;
+  }
+  // quant.rs:795 get_coeff_cost_eob
+  __device__ int cost_eob(int ci, int a, int sign, int ctx) const {
+    int c = cost[kBaseEob + ctx * 3 + min(a, 3) - 1];
+    if (a) {
+      c += ci == 0 ? cost[kDcSign + dc_ctx * 2 + sign] : 512;
+      if (a > 2) c += br_cost(a, br_ctx_eob(ci));
+    }
+    return c;
+  }
+  // quant.rs:822 get_two_coeff_cost_simple (+ :736 br_cost_with_diff with
+  // the two Golomb tables in closed form)
+  __device__ int cost_simple(int ci, int a, int ctx, int* cost_low) const {
+    int c = base_cost(ctx, min(a, 3));
+    int diff = a <= 3 ? base_cost(ctx, a + 4) : 0;
+    if (a) {
+      c += 512;
+      if (a > 2) {
+        const int b = br_ctx(ci), br = min(a - 3, 12);
+        int gb = 0;
+        if (a <= 15) diff += lps(b, br + 13);
+        if (a >= 15) {
+          const int r = a - 14;
+          gb = 512 * (2 * (msb((uint32_t)r) + 1) - 1);
+          diff += r == 1 ? 512 : ((r & (r - 1)) == 0 ? 1024 : 0);
+        }
+        c += lps(b, br) + gb;
+      }
+    }
+    *cost_low = c - diff;
+    return c;
+  }
+  // quant.rs:851 get_eob_cost (coeff_c.rs:606 eob_pos_token)
+  __device__ int eob_cost(int eob) const {
+    int t;
+    if (eob < 33) {
+      t = eob <= 2 ? eob : (eob <= 4 ? 3 : (eob <= 8 ? 4 : (eob <= 16 ? 5 : 6)));
+    } else {
+      const int e = min((eob - 1) >> 5, 16);
+      t = e == 0 ? 6 : (e == 1 ? 7 : (e <= 3 ? 8 : (e <= 7 ? 9 : (e <= 15 ? 10 : 11))));
+    }
+    const int start = t == 0 ? 0 : (t == 1 ? 1 : (1 << (t - 2)) + 1);
+    int c = ecost[(cls != kCls2D) * 11 + t - 1];
+    if (t > 2) {
+      const int cnt = t - 3, bit = ((eob - start) >> cnt) & 1;
+      c += cost[kEobExtra + cnt * 2 + bit] + 512 * cnt;
+    }
+    return c;
+  }
+  __device__ __forceinline__ void low(int a, int sign, int dv, int32_t* q, int32_t* d) const {
+    const int lo = a - 1;
+    *q = (-sign ^ lo) + sign;
+    const int32_t dl = (int32_t)(((int64_t)lo * dv) >> SHIFT);
+    *d = (-sign ^ dl) + sign;
+  }
+
+  // quant.rs:966 update_coeff_general
+  __device__ void update_general(int* accu_rate, int64_t* accu_dist, int si, int eob) {
+    const int ci = scan[si], dv = dqv[ci != 0], qv = qc[ci];
+    const bool is_last = si == eob - 1;
+    const int ctx = lower_ctx(ci, si, is_last);
+    if (qv == 0) {
+      *accu_rate += base_cost(ctx, 0);
+      return;
+    }
+    const int sign = qv < 0, a = qv < 0 ? -qv : qv;
+    const int32_t t = tc[ci], d = dq[ci];
+    const int64_t dist = coeff_dist(t, d, SHIFT), dist0 = coeff_dist(t, 0, SHIFT);
+    const int rate = cost_general(is_last, ci, a, sign, ctx);
+    const int64_t rd = rdcost(rdmult, rate, dist);
+    int32_t q_low = 0, d_low = 0;
+    int rate_low;
+    int64_t dist_low;
+    if (a == 1) {
+      dist_low = dist0;
+      rate_low = base_cost(ctx, 0);
+    } else {
+      low(a, sign, dv, &q_low, &d_low);
+      dist_low = coeff_dist(t, d_low, SHIFT);
+      rate_low = cost_general(is_last, ci, a - 1, sign, ctx);
+    }
+    if (rdcost(rdmult, rate_low, dist_low) < rd) {
+      qc[ci] = q_low;
+      dq[ci] = d_low;
+      set_level(ci, a - 1);
+      *accu_rate += rate_low;
+      *accu_dist += dist_low - dist0;
+    } else {
+      *accu_rate += rate;
+      *accu_dist += dist - dist0;
+    }
+  }
+  // quant.rs:1056 update_coeff_simple
+  __device__ void update_simple(int* accu_rate, int si) {
+    const int ci = scan[si], dv = dqv[ci != 0], qv = qc[ci];
+    const int ctx = lower_ctx(ci, si, false);
+    if (qv == 0) {
+      *accu_rate += base_cost(ctx, 0);
+      return;
+    }
+    const int a = qv < 0 ? -qv : qv;
+    const int32_t at = abs(tc[ci]), ad = abs(dq[ci]);
+    int rate_low = 0;
+    const int rate = cost_simple(ci, a, ctx, &rate_low);
+    if (ad < at) {
+      *accu_rate += rate;
+      return;
+    }
+    const int64_t rd = rdcost(rdmult, rate, coeff_dist(at, ad, SHIFT));
+    const int32_t ad_low = (int32_t)(((int64_t)(a - 1) * dv) >> SHIFT);
+    if (rdcost(rdmult, rate_low, coeff_dist(at, ad_low, SHIFT)) < rd) {
+      const int sign = qv < 0;
+      qc[ci] = (-sign ^ (a - 1)) + sign;
+      dq[ci] = (-sign ^ ad_low) + sign;
+      set_level(ci, a - 1);
+      *accu_rate += rate_low;
+    } else {
+      *accu_rate += rate;
+    }
+  }
+  // quant.rs:1120 update_coeff_eob; nz_ci[0 .. nz_num) are the nonzero
+  // positions since the eob (at most 5)
+  __device__ void update_eob(int* accu_rate, int64_t* accu_dist, int* eob, int* nz_num, int* nz_ci, int si,
+                             bool sharp) {
+    const int ci = scan[si], dv = dqv[ci != 0], qv = qc[ci];
+    const int ctx = lower_ctx(ci, si, false);
+    if (qv == 0) {
+      *accu_rate += base_cost(ctx, 0);
+      return;
+    }
+    bool lower = false;
+    const int a = qv < 0 ? -qv : qv, sign = qv < 0;
+    const int32_t t = tc[ci], d = dq[ci];
+    const int64_t dist0 = coeff_dist(t, 0, SHIFT);
+    int64_t dist = coeff_dist(t, d, SHIFT) - dist0;
+    int rate = cost_general(false, ci, a, sign, ctx);
+    const int64_t rd = rdcost(rdmult, *accu_rate + rate, *accu_dist + dist);
+    int32_t q_low = 0, d_low = 0;
+    int rate_low;
+    int64_t dist_low, rd_low;
+    if (a == 1) {
+      dist_low = 0;
+      rate_low = base_cost(ctx, 0);
+      rd_low = rdcost(rdmult, *accu_rate + rate_low, *accu_dist);
+    } else {
+      low(a, sign, dv, &q_low, &d_low);
+      dist_low = coeff_dist(t, d_low, SHIFT) - dist0;
+      rate_low = cost_general(false, ci, a - 1, sign, ctx);
+      rd_low = rdcost(rdmult, *accu_rate + rate_low, *accu_dist + dist_low);
+    }
+    bool lower_ne = false;
+    const int ctx_ne = lower_ctx(ci, si, true);
+    const int ne_cost = eob_cost(si + 1);
+    int rate_ce = ne_cost + cost_eob(ci, a, sign, ctx_ne);
+    int64_t dist_ne = dist;
+    int64_t rd_ne = rdcost(rdmult, rate_ce, dist_ne);
+    if (a - 1 > 0) {
+      const int rate_ce_low = ne_cost + cost_eob(ci, a - 1, sign, ctx_ne);
+      const int64_t rd_ne_low = rdcost(rdmult, rate_ce_low, dist_low);
+      if (rd_ne_low < rd_ne) {
+        lower_ne = true;
+        rd_ne = rd_ne_low;
+        rate_ce = rate_ce_low;
+        dist_ne = dist_low;
+      }
+    }
+    if (rd_low < rd) {
+      lower = true;
+      rate = rate_low;
+      dist = dist_low;
+    }
+    if (!sharp && rd_ne < (rd_low < rd ? rd_low : rd)) {
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+        if (k < *nz_num) {
+          set_level(nz_ci[k], 0);
+          qc[nz_ci[k]] = 0;
+          dq[nz_ci[k]] = 0;
+        }
+      *eob = si + 1;
+      *nz_num = 0;
+      *accu_rate = rate_ce;
+      *accu_dist = dist_ne;
+      lower = lower_ne;
+    } else {
+      *accu_rate += rate;
+      *accu_dist += dist;
+    }
+    if (lower) {
+      qc[ci] = q_low;
+      dq[ci] = d_low;
+      set_level(ci, a - 1);
+    }
+    if (qc[ci] != 0) {
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+        if (k == *nz_num) nz_ci[k] = ci;
+      ++*nz_num;
+    }
+  }
+};
+}  // namespace
+
+template <int TXS>
+__global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff,
+                                             int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
+                                             const RdoqJob* __restrict__ jobs, int n_jobs,
+                                             const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans,
+                                             int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
+  using T = Trellis<TXS>;
+  constexpr int N = T::N;
+  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
+  __shared__ uint16_t sc[3 * N];
+  __shared__ uint32_t lv[T::NW * 64];
+  const int lane = threadIdx.x;
+  // CoeffCostTables (quant.rs:475): txb[txs_ctx * 2 + plane], then eob[ms][plane]
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = lane; i < 2 * kTxbWords; i += 64) tab[i] = txb_src[i];
+  for (int i = lane; i < 2 * kEobWords; i += 64) tab[2 * kTxbWords + i] = eob_src[i];
+  for (int i = lane; i < 3 * N; i += 64) sc[i] = scans[i];
+  __syncthreads();
+  const int j = blockIdx.x * 64 + lane;
+  if (j >= n_jobs) return;
+  const RdoqJob job = jobs[j];
+  int eob = eobs[j];
+  // out-of-range fields would index past the staged tables: such blocks are left as they are
+  if (eob <= 0 || eob > N || job.tx_class > 2 || job.plane_type > 1 || job.txb_skip_ctx >= 13 ||
+      job.dc_sign_ctx >= 3)
+    return;
+  T s;
+  s.tc = tcoeff + job.coeff;
+  s.qc = qcoeff + job.coeff;
+  s.dq = dqcoeff + job.coeff;
+  s.lv = lv + lane;
+  s.cost = tab + (job.plane_type ? kTxbWords : 0);
+  s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
+  // scan classes: 0 diagonal (2D), 1 row (vertical class), 2 column (horizontal class)
+  s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
+  s.rdmult = job.rdmult;
+  s.dqv[0] = dq0, s.dqv[1] = dq1;
+  s.cls = job.tx_class;
+  s.dc_ctx = job.dc_sign_ctx;
+  const bool sharp = job.sharpness != 0;
+  // txb_init_levels (coeff_c.rs:278)
+  for (int w = 0; w < T::NW; w++) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int p = 8 * w + k;
+      if (p < N) v |= (uint32_t)min(abs(s.qc[p]), 15) << (4 * k);
+    }
+    s.lv[w * 64] = v;
+  }
+  const int non_skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2], skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2 + 1];
+  int accu_rate = s.eob_cost(eob);
+  int64_t accu_dist = 0;
+  int si = eob - 1;
+  {
+    const int ci = s.scan[si], qv = s.qc[ci];
+    const int a = qv < 0 ? -qv : qv, sign = qv < 0;
+    if (a >= 2) {
+      s.update_general(&accu_rate, &accu_dist, si, eob);
+    } else {
+      accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
+      accu_dist += coeff_dist(s.tc[ci], s.dq[ci], T::SHIFT) - coeff_dist(s.tc[ci], 0, T::SHIFT);
+    }
+    si--;
+  }
+  int nz_num = 1, nz_ci[5] = {s.scan[eob - 1], 0, 0, 0, 0};
+  while (si >= 0 && nz_num <= 4) {
+    s.update_eob(&accu_rate, &accu_dist, &eob, &nz_num, nz_ci, si, sharp);
+    si--;
+  }
+  if (si == -1 && nz_num <= 4) {  // update_skip (quant.rs:1281)
+    const int64_t rd = rdcost(s.rdmult, accu_rate + non_skip_cost, accu_dist);
+    if (!sharp && rdcost(s.rdmult, skip_cost, 0) < rd) {
+#pragma unroll
+      for (int k = 0; k < 5; k++)
+        if (k < nz_num) s.qc[nz_ci[k]] = 0, s.dq[nz_ci[k]] = 0;
+      accu_rate = 0;
+      eob = 0;
+    }
+  }
+  int si_end = 1;
+  if (cut_num != 0) {
+    const uint32_t cut = max((uint32_t)(N >> 7), (uint32_t)eob * cut_num / cut_den);
+    si_end = max(eob - (int)cut, 1);
+  }
+  for (; si >= si_end; si--) s.update_simple(&accu_rate, si);
+  if (si == 0) {
+    int64_t dummy = 0;
+    s.update_general(&accu_rate, &dummy, 0, eob);
+  }
+  eobs[j] = (uint16_t)eob;
+}
+
+hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                      const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((n_jobs + 63) / 64));
+#define ZR(T)                                                                                                   \
+  case T:                                                                                                       \
+    hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, \
+                       scans, dq0, dq1, cut_num, cut_den);                                                      \
+    break;
+  switch (tx_size) {
+    ZR(0) ZR(1) ZR(2) ZR(3) ZR(4) ZR(5) ZR(6) ZR(7) ZR(8) ZR(9) ZR(10) ZR(11) ZR(12) ZR(13) ZR(14) ZR(15) ZR(16)
+    ZR(17) ZR(18)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef ZR
+  return hipGetLastError();
+}
+
+}  // namespace zav1

@@ -188,6 +188,17 @@ hipError_t inv_txfm2d_batch(const int32_t* in, int32_t* out, const uint8_t* type
 
 // kind: 0 = fp (RDOQ initial), 1 = b (dead-zone).  rows: device array of
 // QuantRow indexed by qidx[job].
+// One block of the RDOQ batch (same layout as zav1_rdoq_job).
+struct RdoqJob {
+  int64_t rdmult;
+  uint32_t coeff;
+  uint8_t tx_class, plane_type, txb_skip_ctx, dc_sign_ctx, sharpness;
+  uint8_t pad[7];
+};
+static_assert(sizeof(RdoqJob) == 24, "RdoqJob layout");
+hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                      const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st);
 hipError_t quantize_batch(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff,
                           const QuantRow* rows, const uint8_t* qidx, int n_per_job, int n_jobs,
                           int log_scale, int bd, int kind, hipStream_t st);
