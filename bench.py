@@ -374,6 +374,119 @@ def md_funnel_stage(zav1, synth, args, cpu=True):
     return out
 
 
+def rdoq_stage(zav1, synth, args, cpu=True):
+    """RDOQ (zav1_rdoq_batch) of every tx-type candidate of every 16x16 block
+    of the config-2 frame (2048x2048 10-bit): 16 384 blocks x 16 types =
+    262 144 trellis runs, as the tx-type search asks for them.  Residual =
+    block minus its rounded mean; each type's forward transform on the
+    device, fp quantization at qp 20, eob from the type's scan (all
+    untimed); the timed launch requantizes every candidate in place (the
+    quantized inputs restored by a device copy before each step, outside the
+    events).  Rate tables: synthetic (synth.coeff_costs), rdmult 1054880."""
+    L = zav1.lib()
+    N, ts, n, nt = 2048, 2, 256, 16
+    img = synth.config2_luma(N, N, seed=2000).astype(np.int32)
+    blk = img.reshape(N // 16, 16, N // 16, 16).transpose(0, 2, 1, 3).reshape(-1, n)
+    res = (blk - (blk.sum(axis=1, keepdims=True) + n // 2) // n).astype(np.int32)
+    nb = res.shape[0]
+    nj = nb * nt
+    types = np.tile(np.arange(nt, dtype=np.uint8), nb)  # job j: block j // 16, type j % 16
+    qi = synth.qp_to_qindex(20)
+    D = zav1.DeviceArray
+    d_res = D.from_host(np.repeat(res, nt, axis=0))
+    d_coef = D((nj, n), np.int32)
+    zav1.fwd_txfm2d_batch(d_res, d_coef, D.from_host(types), ts, nj)
+    del d_res
+    d_q, d_dq = D((nj, n), np.int32), D((nj, n), np.int32)
+    zav1.quantize_batch(d_coef, d_q, d_dq, D.from_host(np.full(nj, qi, np.uint8)), n, nj, 0, 10, 0)
+    scans = np.concatenate([zav1.scan_order(ts, c) for c in (0, 1, 2)]).astype(np.uint16)
+    q0, dq0 = d_q.to_host(), d_dq.to_host()
+    # tx class of each type (V_* vertical, H_* horizontal) and its scan
+    cls_of = np.array([0] * 10 + [2, 1, 2, 1, 2, 1], np.uint8)
+    scan_of = {0: scans[:n], 2: scans[n:2 * n], 1: scans[2 * n:]}
+    eob0 = np.zeros(nj, np.uint16)
+    for c in (0, 1, 2):
+        sel = np.nonzero(cls_of[types] == c)[0]
+        nzq = q0[sel][:, scan_of[c]] != 0
+        eob0[sel] = np.where(nzq.any(axis=1), n - np.argmax(nzq[:, ::-1], axis=1), 0)
+    jobs = np.zeros(nj, zav1.RDOQ_JOB)
+    jobs['rdmult'] = 1054880
+    jobs['coeff'] = np.arange(nj, dtype=np.uint32) * n
+    jobs['tx_class'] = cls_of[types]
+    costs = synth.coeff_costs(2020)
+    d_jobs, d_costs, d_scans = D.from_host(jobs), D.from_host(costs), D.from_host(scans)
+    d_eob, d_q_keep, d_dq_keep, d_eob_keep = D.from_host(eob0), D.from_host(q0), D.from_host(dq0), D.from_host(eob0)
+    st = ctypes.c_void_p()
+    zav1.check(L.zav1_stream_create(ctypes.byref(st)), 'stream_create')
+    stream = st.value
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        L.zav1_event_create(ctypes.byref(e))
+
+    def restore():
+        for dst, src in ((d_q, d_q_keep), (d_dq, d_dq_keep), (d_eob, d_eob_keep)):
+            zav1.check(L.zav1_memcpy_d2d(dst.ptr, src.ptr, src.nbytes, stream), 'memcpy_d2d')
+
+    def launch():
+        zav1.rdoq_batch(ts, qi, 10, d_coef, d_q, d_dq, d_eob, d_jobs, nj, d_costs, d_scans, stream=stream)
+
+    steps = max(5, args.steps // 5)
+    for _ in range(2):
+        restore()
+        launch()
+    zav1.sync(stream)
+    tot_ms = 0.0
+    for _ in range(steps):
+        restore()
+        L.zav1_event_record(ev[0].value, stream)
+        launch()
+        L.zav1_event_record(ev[1].value, stream)
+        zav1.sync(stream)
+        ms = ctypes.c_float()
+        L.zav1_event_elapsed_ms(ev[0].value, ev[1].value, ctypes.byref(ms))
+        tot_ms += ms.value
+    ms_step = tot_ms / steps
+    eob1 = d_eob.to_host()
+    q1 = d_q.to_host()
+    steps_coef = int(eob0.astype(np.int64).sum())
+    # algorithmic HBM bytes: tcoeff, qcoeff, dqcoeff read, changed levels written back, jobs and eobs
+    changed = int((q1 != q0).sum())
+    alg_bytes = nj * n * 12 + changed * 8 + nj * (24 + 4)
+    out = {'workload': 'RDOQ of every tx-type candidate (16) of the 16 384 16x16 blocks of the 2048x2048 10-bit '
+                       'config-2 frame at qp 20 (block-mean residual, fp quantization; synthetic rate tables)',
+           'value': round(N * N / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s (all 16 candidates per pixel)',
+           'steps': steps, 'ms_per_step': round(ms_step, 4), 'trellis_runs': nj,
+           'trellis_runs_per_s': round(nj / (ms_step / 1000.0)),
+           'trellis_steps_per_s': round(steps_coef / (ms_step / 1000.0)),
+           'mean_eob_in': round(float(eob0.mean()), 2), 'mean_eob_out': round(float(eob1.mean()), 2),
+           'levels_changed_frac': round(changed / max(1, int((q0 != 0).sum())), 4),
+           'roofline': {'bound': 'latency', 'kernel': 'k_rdoq<TX_16X16>', 'unit': 'GB/s',
+                        'achieved': round(alg_bytes / (ms_step / 1000.0) / 1e9, 2), 'peak': 8000.0,
+                        'frac': round(alg_bytes / (ms_step / 1000.0) / 1e9 / 8000.0, 4),
+                        'traffic': measured_traffic('k_rdoq<2>'), 'avg_launch_us': round(ms_step * 1000.0, 2),
+                        'algorithmic_bytes_per_launch': alg_bytes}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import oracle as O
+        k = 16384
+        coef = d_coef.to_host()
+        row = O.quant_row(qi, 10)
+        dqv = [row.dequant[0], row.dequant[1]]
+        tc, ec = synth.tables_for(costs, 2, 4, 0)
+        tc0 = time.perf_counter()
+        for j in range(k):
+            if eob0[j]:
+                c = int(cls_of[types[j]])
+                O.optimize_b(coef[j], q0[j], dq0[j], int(eob0[j]), scan_of[c], dqv, tc, ec, 1054880, ts, c)
+        tc1 = time.perf_counter()
+        out['cpu_baseline'] = {'value': round(k / nt * n / 1e6 / (tc1 - tc0), 4),
+                               'unit': 'MP/s (all 16 candidates per pixel)', 'cores': 1, 'kind': 'port',
+                               'sample': 'the first %d candidates (1024 blocks x 16 types; oracle C optimize_b, '
+                                         'one thread, ctypes call per block)' % k,
+                               'seconds': round(tc1 - tc0, 3)}
+    return out
+
+
 def config4_stage(zav1, synth, tsm, args, streams):
     """Config 4 at its per-GPU shape: one tile column (2048 wide, 4096 high)
     of the 8192x4096 10-bit frame with screen regions, tx-type search at
@@ -444,6 +557,7 @@ def main():
     ap.add_argument('--streams', type=int, default=2)
     ap.add_argument('--no-config4', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
+    ap.add_argument('--no-rdoq', action='store_true')
     ap.add_argument('--serve-n', type=int, default=96)
     ap.add_argument('--serve-ctx', type=int, default=4)
     ap.add_argument('--serve-load', type=float, default=0.6)
@@ -640,6 +754,8 @@ def main():
         out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=not args.no_cpu)
     if not args.no_funnel:
         out['md_funnel'] = md_funnel_stage(zav1, synth, args, cpu=not args.no_cpu)
+    if not args.no_rdoq:
+        out['rdoq'] = rdoq_stage(zav1, synth, args, cpu=not args.no_cpu)
     print(json.dumps(out))
 
 
