@@ -277,6 +277,26 @@ int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, in
                     const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,
                     uint32_t cut_off_denum, zav1_stream_t s);
 
+/* One block of a coefficient-rate batch (cost_coeffs_txb's arguments,
+ * rust/crates/svtav1-encoder/src/leaf_funnel/coeff_rate.rs:60). */
+typedef struct {
+  uint32_t coeff;       /* element offset of the block's packed levels */
+  int32_t txt_rate;     /* the tx type's rate (MdRates::txt_rate), added for luma */
+  uint8_t tx_class;     /* 0 2D, 1 horizontal, 2 vertical */
+  uint8_t plane_type;   /* 0 luma, 1 chroma */
+  uint8_t txb_skip_ctx; /* < 13 */
+  uint8_t dc_sign_ctx;  /* < 3 */
+} zav1_rate_job;
+
+/* Coefficient rate (1/512 bit) of n_jobs quantized blocks of tx_size
+ * (packed, coded size, raster; d_eob[j] in scan order): d_rate[j] =
+ * cost_coeffs_txb (coeff_rate.rs:60, the full loop) priced with the frame's
+ * zav1_coeff_costs and tx_size's three scans (as for zav1_rdoq_batch).
+ * Blocks with eob 0, or a job field out of range, get 0. */
+int zav1_coeff_rate_batch(int tx_size, const int32_t *d_qcoeff, const uint16_t *d_eob,
+                          const zav1_rate_job *d_jobs, int n_jobs, const zav1_coeff_costs *d_costs,
+                          const uint16_t *d_scans, int32_t *d_rate, zav1_stream_t s);
+
 /* AV1 coefficient scan of tx_size's packed (<= 32x32) block for a scan
  * class (0 default diagonal, 1 row, 2 column; the reference's
  * TX_TYPE_TO_SCAN_INDEX classes).  Replaces scan()

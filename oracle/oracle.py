@@ -438,6 +438,7 @@ def _bind_rdoq(L):
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.or_optimize_b.restype = ctypes.c_int
     L.or_rdoq_eob_cost.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_int]
+    L.or_cost_coeffs_txb.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp] + [ctypes.c_int] * 6
     for f in ('or_adjusted_tx_size', 'or_txs_ctx', 'or_eob_multi_size'):
         getattr(L, f).argtypes = [ctypes.c_int]
 
@@ -474,3 +475,16 @@ def rdoq_size_ctx(tx_size):
     L = lib()
     _bind_rdoq(L)
     return L.or_adjusted_tx_size(tx_size), L.or_txs_ctx(tx_size), L.or_eob_multi_size(tx_size)
+
+
+def cost_coeffs_txb(qcoeff, eob, scan_, txb_costs, eob_costs, tx_size, tx_class, plane_type=0, txb_skip_ctx=0,
+                    dc_sign_ctx=0, txt_rate=0):
+    """lf/coeff_rate.rs:60 cost_coeffs_txb of one packed block (eob > 0)."""
+    L = lib()
+    _bind_rdoq(L)
+    q = np.ascontiguousarray(qcoeff, dtype=np.int32)
+    sc = np.ascontiguousarray(scan_, dtype=np.uint16)
+    tc = np.ascontiguousarray(txb_costs, dtype=np.int32)
+    ec = np.ascontiguousarray(eob_costs, dtype=np.int32)
+    return L.or_cost_coeffs_txb(q.ctypes.data, int(eob), sc.ctypes.data, tc.ctypes.data, ec.ctypes.data, tx_size,
+                                tx_class, plane_type, txb_skip_ctx, dc_sign_ctx, txt_rate)

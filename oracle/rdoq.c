@@ -7,8 +7,9 @@
  * optimize_b :1312) and the coefficient contexts of
  * rust/crates/svtav1-entropy/src/coeff_c.rs (levels map :236-:298, nz_mag :306,
  * nz_map_ctx_offset_2d :344, nz_map_ctx_from_stats :380, nz_map_ctx :406,
- * br_ctx_eob :456, br_ctx :537, eob_pos_token :606).  The cost tables are
- * inputs (built by the caller from its CDFs, enc/quant.rs:514-:632).
+ * br_ctx_eob :456, br_ctx :537, eob_pos_token :606), and the coefficient
+ * rate cost_coeffs_txb (lf/coeff_rate.rs:60).  The cost tables are inputs
+ * (built by the caller from its CDFs, enc/quant.rs:514-:632).
  */
 #include <stdint.h>
 #include <string.h>
@@ -468,4 +469,72 @@ int or_optimize_b(const int32_t *tcoeff, int32_t *qcoeff, int32_t *dqcoeff, int 
     update_general(s, &accu_rate, &dummy, 0, eob);
   }
   return eob;
+}
+
+/* lf/coeff_rate.rs:60 cost_coeffs_txb at the full middle loop: the rate of
+ * one quantized block (raster, coded size) with eob > 0; txt_rate is the
+ * caller's tx-type rate (added for luma), the contexts those of the final
+ * levels (coeff_c.rs:492 get_nz_map_contexts). */
+int or_cost_coeffs_txb(const int32_t *qcoeff, int eob, const uint16_t *scan, const or_txb_costs *c,
+                       const or_eob_costs *e, int tx_size, int tx_class, int plane_type, int txb_skip_ctx,
+                       int dc_sign_ctx, int txt_rate) {
+  const int adj = or_adjusted_tx_size(tx_size);
+  const int w = TXW[adj], h = TXH[adj], bwl = log2i(w);
+  static __thread uint8_t lv[LEVELS_LEN];
+  memset(lv, 0, sizeof(lv));
+  if (eob > 1) {
+    const int st = w + PAD_HOR;
+    for (int r = 0; r < h; r++)
+      for (int col = 0; col < w; col++) {
+        const int32_t v = qcoeff[r * w + col];
+        const uint32_t a = v < 0 ? -(uint32_t)v : (uint32_t)v;
+        lv[PAD_TOP * st + r * st + col] = (uint8_t)(a < 127 ? a : 127);
+      }
+  }
+  int cost = c->txb_skip_cost[txb_skip_ctx][0];
+  if (plane_type == 0) cost += txt_rate;
+  cost += or_rdoq_eob_cost(eob, e, c, tx_class);
+  /* one position's level cost (the closure of coeff_rate.rs:117) */
+#define LEVEL_COST(pos, v, is_eob_pos, is_dc, ctx)                                              \
+  do {                                                                                           \
+    const int lvl = (v) < 0 ? -(v) : (v);                                                        \
+    cost += (is_eob_pos) ? c->base_eob_cost[ctx][(lvl < 3 ? lvl : 3) - 1]                        \
+                         : c->base_cost[ctx][lvl < 3 ? lvl : 3];                                 \
+    if ((v) != 0) {                                                                              \
+      cost += (is_dc) ? c->dc_sign_cost[dc_sign_ctx][(v) < 0] : 512;                             \
+      if (lvl > 2) {                                                                             \
+        const int b = br_ctx(lv, pos, bwl, tx_class);                                            \
+        cost += c->lps_cost[b][lvl - 3 < 12 ? lvl - 3 : 12];                                     \
+        if (lvl >= 15) cost += golomb_cost(lvl);                                                 \
+      }                                                                                          \
+    }                                                                                            \
+  } while (0)
+  if (eob == 1) {
+    LEVEL_COST(0, qcoeff[0], 1, 1, 0);
+    return cost;
+  }
+  {
+    const int pos = scan[eob - 1];
+    const int ctx = lower_ctx(lv, pos, bwl, h, eob - 1, 1, tx_size, tx_class);
+    LEVEL_COST(pos, qcoeff[pos], 1, 0, ctx);
+  }
+  {
+    const int ctx = lower_ctx(lv, 0, bwl, h, 0, 0, tx_size, tx_class);
+    LEVEL_COST(0, qcoeff[0], 0, 1, ctx);
+  }
+#undef LEVEL_COST
+  for (int i = eob - 2; i >= 1; i--) {
+    const int pos = scan[i], v = qcoeff[pos], lvl = v < 0 ? -v : v;
+    const int ctx = lower_ctx(lv, pos, bwl, h, i, 0, tx_size, tx_class);
+    if (v != 0) cost += 512;
+    if (lvl > 2) {
+      const int b = br_ctx(lv, pos, bwl, tx_class);
+      cost += c->base_cost[ctx][3];
+      if (lvl - 3 < 12) cost += c->lps_cost[b][lvl - 3];
+      else cost += golomb_cost(lvl) + c->lps_cost[b][12];
+    } else {
+      cost += c->base_cost[ctx][lvl];
+    }
+  }
+  return cost;
 }

@@ -240,6 +240,10 @@ int or_optimize_b(const int32_t *tcoeff, int32_t *qcoeff, int32_t *dqcoeff, int 
                   const int32_t dequant[2], const or_txb_costs *c, const or_eob_costs *e, int64_t rdmult,
                   int tx_size, int tx_class, int txb_skip_ctx, int dc_sign_ctx, int sharpness_flag,
                   int cut_off_num, int cut_off_denum);
+/* lf/coeff_rate.rs:60 cost_coeffs_txb (full loop) of one block, eob > 0. */
+int or_cost_coeffs_txb(const int32_t *qcoeff, int eob, const uint16_t *scan, const or_txb_costs *c,
+                       const or_eob_costs *e, int tx_size, int tx_class, int plane_type, int txb_skip_ctx,
+                       int dc_sign_ctx, int txt_rate);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,9 @@ def test_invalid_arguments_rejected_without_gpu(zav1):
     # RDOQ: size, depth, qindex, cut-off and pointer validation before any device work
     fake = ctypes.c_void_p(1)
     assert zav1.RDOQ_JOB.itemsize == 24 and zav1.COEFF_COSTS_WORDS == 10 * 944 + 14 * 22
+    assert zav1.RATE_JOB.itemsize == 12
+    assert L.zav1_coeff_rate_batch(19, fake, fake, fake, 1, fake, fake, fake, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_coeff_rate_batch(0, fake, fake, fake, 1, None, fake, fake, None) == zav1.ZAV1_EINVAL
     assert L.zav1_rdoq_batch(19, 100, 8, fake, fake, fake, fake, fake, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL
     assert L.zav1_rdoq_batch(0, 100, 12, fake, fake, fake, fake, fake, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL
     assert L.zav1_rdoq_batch(0, 256, 8, fake, fake, fake, fake, fake, 1, fake, fake, 0, 0, None) == zav1.ZAV1_EINVAL

@@ -68,3 +68,45 @@ def test_rdoq_batch_bitexact(gpu, oracle, tx_size):
                                                    (16, 8, 255, (80, 100)), (4, 10, 0, (0, 0))])
 def test_rdoq_batch_depths_and_cutoffs(gpu, oracle, tx_size, bd, qindex, cut):
     run_case(gpu, oracle, tx_size, bd, qindex, cut, 900 + tx_size * 7 + bd)
+
+
+@pytest.mark.parametrize('tx_size', list(range(19)))
+def test_coeff_rate_batch_bitexact(gpu, oracle, tx_size):
+    """zav1_coeff_rate_batch == cost_coeffs_txb for fp-quantized blocks of
+    every class and both plane types, before and after the trellis."""
+    z = gpu
+    w, h = RC.coded_dims(tx_size)
+    n = w * h
+    costs = RT.coeff_costs(700 + tx_size)
+    _, txs, ems = oracle.rdoq_size_ctx(tx_size)
+    scans = np.concatenate([oracle.scan(tx_size, c) for c in (0, 1, 2)]).astype(np.uint16)
+    rng = np.random.default_rng(tx_size)
+    qs, eobs, jobs = [], [], []
+    for cls in (0, 1, 2):
+        t, q, dq, eob, scan, dqv = RC.blocks(oracle, tx_size, cls, 40, 70 + tx_size * 3 + cls, 60, 10)
+        for b in range(len(q)):
+            qb, eb = q[b], int(eob[b])
+            plane = int(rng.integers(0, 2))
+            if b % 2 and eb:  # half the blocks after the trellis
+                tc, ec = RT.tables_for(costs, txs, ems, plane)
+                qb, _, eb = oracle.optimize_b(t[b], q[b], dq[b], eb, scan, dqv, tc, ec, 1054880, tx_size, cls)
+            j = np.zeros(1, z.RATE_JOB)[0]
+            j['coeff'], j['txt_rate'], j['tx_class'], j['plane_type'] = len(qs) * n, int(rng.integers(0, 5000)), cls, plane
+            j['txb_skip_ctx'], j['dc_sign_ctx'] = int(rng.integers(0, 13)), int(rng.integers(0, 3))
+            qs.append(qb), eobs.append(eb), jobs.append(j)
+    Q, E, J = np.stack(qs), np.array(eobs, np.uint16), np.array(jobs, dtype=z.RATE_JOB)
+    D = z.DeviceArray
+    d_rate = D((len(J),), np.int32)
+    z.coeff_rate_batch(tx_size, D.from_host(Q), D.from_host(E), D.from_host(J), len(J), D.from_host(costs),
+                       D.from_host(scans), d_rate)
+    got = d_rate.to_host()
+    for b, j in enumerate(J):
+        cls = int(j['tx_class'])
+        if E[b] == 0:
+            assert got[b] == 0
+            continue
+        tc, ec = RT.tables_for(costs, txs, ems, int(j['plane_type']))
+        sc = scans[RC.SCAN_CLASS[cls] * n:(RC.SCAN_CLASS[cls] + 1) * n]
+        want = oracle.cost_coeffs_txb(Q[b], int(E[b]), sc, tc, ec, tx_size, cls, int(j['plane_type']),
+                                      int(j['txb_skip_ctx']), int(j['dc_sign_ctx']), int(j['txt_rate']))
+        assert int(got[b]) == want, (tx_size, b, int(got[b]), want)

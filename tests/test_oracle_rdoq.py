@@ -163,3 +163,46 @@ def test_trellis_invariants(oracle, tx_size):
                     if rdmult == 0:
                         err = lambda d: int((((t[b].astype(np.int64) - d) << shift) ** 2).sum())
                         assert err(d2) <= err(dq[b])
+
+
+@pytest.mark.parametrize('tx_size', [0, 2, 3, 5, 9, 13, 16, 18])
+def test_coeff_rate_with_empty_tables_is_literal_bits(oracle, tx_size):
+    """With every table entry 0 the coefficient rate is its literal part:
+    one bit per nonzero AC sign, Golomb bits of levels >= 15, the eob's
+    eob_pt - 3 raw offset bits, and the luma tx-type rate."""
+    w, h = RC.coded_dims(tx_size)
+    zt, ze = np.zeros(RT.TXB_WORDS, np.int32), np.zeros(RT.EOB_WORDS, np.int32)
+    for cls in (0, 1, 2):
+        t, q, dq, eob, scan, _ = RC.blocks(oracle, tx_size, cls, 8, 60 + tx_size + cls, 30, 8)
+        for b in range(len(q)):
+            if eob[b] == 0:
+                continue
+            lv = np.abs(q[b][scan[:eob[b]]]).astype(np.int64)
+            ac_nz = int((lv[1:] != 0).sum())
+            gol = sum(golomb(int(a)) for a in lv)
+            pt = eob[b] if eob[b] <= 2 else int(np.ceil(np.log2(eob[b]))) + 1
+            want = 512 * ac_nz + gol + 512 * max(0, pt - 3) + 777 * (b % 2 == 0)
+            got = oracle.cost_coeffs_txb(q[b], eob[b], scan, zt, ze, tx_size, cls, b % 2, 0, 0, 777)
+            assert got == want, (tx_size, cls, b)
+
+
+def test_coeff_rate_dc_only_priced_by_hand(oracle):
+    rng = np.random.default_rng(5)
+    costs = RT.coeff_costs(55)
+    for tx_size in (0, 1, 2, 3, 4, 9):
+        w, h = RC.coded_dims(tx_size)
+        _, txs, ems = oracle.rdoq_size_ctx(tx_size)
+        scan = oracle.scan(tx_size, 0)
+        for _ in range(50):
+            plane = int(rng.integers(0, 2))
+            tc, ec = RT.tables_for(costs, txs, ems, plane)
+            a = int(rng.choice([1, 2, 3, 9, 15, 16, 50, 300]))
+            sign = int(rng.integers(0, 2))
+            q = np.zeros(w * h, np.int32)
+            q[0] = -a if sign else a
+            skip, dc, txt = int(rng.integers(0, 13)), int(rng.integers(0, 3)), int(rng.integers(0, 4000))
+            want = int(tc[RT.SKIP + 2 * skip]) + (txt if plane == 0 else 0) + eob_cost_closed_form(1, tc, ec, 0)
+            want += int(tc[RT.BASE_EOB + min(a, 3) - 1]) + int(tc[RT.DC_SIGN + 2 * dc + sign])
+            if a > 2:
+                want += int(tc[RT.LPS + min(a - 3, 12)]) + golomb(a)
+            assert oracle.cost_coeffs_txb(q, 1, scan, tc, ec, tx_size, 0, plane, skip, dc, txt) == want

@@ -121,6 +121,7 @@ EXPORTS = {
     'zav1_fwd_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_inv_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _i32, _vp],
     'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
+    'zav1_coeff_rate_batch': [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp],
     'zav1_rdoq_batch': [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                         _vp],
     'zav1_sse_i32_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
@@ -293,6 +294,16 @@ RDOQ_JOB = np.dtype([('rdmult', '<i8'), ('coeff', '<u4'), ('tx_class', 'u1'), ('
                      ('txb_skip_ctx', 'u1'), ('dc_sign_ctx', 'u1'), ('sharpness', 'u1'), ('pad', 'u1', 7)])
 TXB_COST_WORDS, EOB_COST_WORDS = 944, 22
 COEFF_COSTS_WORDS = 10 * TXB_COST_WORDS + 14 * EOB_COST_WORDS
+
+
+RATE_JOB = np.dtype([('coeff', '<u4'), ('txt_rate', '<i4'), ('tx_class', 'u1'), ('plane_type', 'u1'),
+                     ('txb_skip_ctx', 'u1'), ('dc_sign_ctx', 'u1')])
+
+
+def coeff_rate_batch(tx_size, d_qcoeff, d_eob, d_jobs, n_jobs, d_costs, d_scans, d_rate, stream=None):
+    """Coefficient rate of n_jobs blocks (zav1_coeff_rate_batch)."""
+    check(lib().zav1_coeff_rate_batch(tx_size, d_qcoeff.ptr, d_eob.ptr, d_jobs.ptr, n_jobs, d_costs.ptr,
+                                      d_scans.ptr, d_rate.ptr, stream), 'coeff_rate_batch')
 
 
 def rdoq_batch(tx_size, qindex, bd, d_tcoeff, d_qcoeff, d_dqcoeff, d_eob, d_jobs, n_jobs, d_costs, d_scans,

@@ -873,6 +873,19 @@ static_assert(sizeof(zav1_rdoq_job) == sizeof(zav1::RdoqJob), "rdoq job layout")
 static_assert(sizeof(zav1_txb_costs) == 944 * 4 && sizeof(zav1_eob_costs) == 22 * 4, "cost table layout");
 static_assert(sizeof(zav1_coeff_costs) == (10 * 944 + 14 * 22) * 4, "cost tables layout");
 
+static_assert(sizeof(zav1_rate_job) == sizeof(zav1::RateJob), "rate job layout");
+
+int zav1_coeff_rate_batch(int tx_size, const int32_t* d_qcoeff, const uint16_t* d_eob, const zav1_rate_job* d_jobs,
+                          int n_jobs, const zav1_coeff_costs* d_costs, const uint16_t* d_scans, int32_t* d_rate,
+                          zav1_stream_t s) {
+  if (tx_size < 0 || tx_size >= 19 || n_jobs < 0 ||
+      (n_jobs > 0 && (!d_qcoeff || !d_eob || !d_jobs || !d_costs || !d_scans || !d_rate)))
+    return ZAV1_EINVAL;
+  ZCHECK(zav1::coeff_rate_batch(tx_size, d_qcoeff, d_eob, reinterpret_cast<const zav1::RateJob*>(d_jobs), n_jobs,
+                                reinterpret_cast<const int32_t*>(d_costs), d_scans, d_rate, S(s)));
+  return ZAV1_OK;
+}
+
 int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, int32_t* d_qcoeff, int32_t* d_dqcoeff,
                     uint16_t* d_eob, const zav1_rdoq_job* d_jobs, int n_jobs, const zav1_coeff_costs* d_costs,
                     const uint16_t* d_scans, uint32_t cut_off_num, uint32_t cut_off_denum, zav1_stream_t s) {

@@ -52,6 +52,58 @@ __device__ __forceinline__ int golomb_cost(int a) {
   return a >= 15 ? 512 * (2 * (msb((uint32_t)(a - 14)) + 1) - 1) : 0;
 }
 
+// Coefficient contexts of a block of size TXS with class cls, over a level
+// accessor lv(r, c) that returns 0 past the block's right / bottom edge.
+// coeff_c.rs:406 nz_map_ctx (+ :306 nz_mag, :344 nz_map_ctx_offset_2d,
+// :380 from_stats); si = scan index; the 2D offsets take their shape from
+// the unadjusted size.
+template <int TXS, class LV>
+__device__ int nz_ctx(const LV& lv, int ci, int si, bool is_last, int cls) {
+  constexpr int AW = c_txw(TXS), BWL = c_log2(AW), N = AW * c_txh(TXS);
+  if (is_last) {
+    if (si == 0) return 0;
+    if (si <= N / 8) return 1;
+    if (si <= N / 4) return 2;
+    return 3;
+  }
+  const int r = ci >> BWL, c = ci & (AW - 1);
+  auto c3 = [](int v) { return v < 3 ? v : 3; };
+  int mag = c3(lv(r, c + 1)) + c3(lv(r + 1, c));
+  if (cls == kCls2D) {
+    mag += c3(lv(r + 1, c + 1)) + c3(lv(r, c + 2)) + c3(lv(r + 2, c));
+  } else if (cls == kClsVert) {
+    mag += c3(lv(r + 2, c)) + c3(lv(r + 3, c)) + c3(lv(r + 4, c));
+  } else {
+    mag += c3(lv(r, c + 2)) + c3(lv(r, c + 3)) + c3(lv(r, c + 4));
+  }
+  if ((cls | ci) == 0) return 0;
+  const int ctx = min((mag + 1) >> 1, 4);
+  if (cls == kCls2D) {
+    if (ci == 0) return ctx;
+    if (tx_w(TXS) < tx_h(TXS)) {
+      if (r < 2) return ctx + 11;
+    } else if (tx_w(TXS) > tx_h(TXS) && c < 2) {
+      return ctx + 16;
+    }
+    return ctx + (r + c < 2 ? 1 : (r + c < 4 ? 6 : 21));
+  }
+  const int pos = cls == kClsHoriz ? c : r;
+  return ctx + (pos == 0 ? 26 : (pos == 1 ? 31 : 36));
+}
+// coeff_c.rs:537 br_ctx
+template <int TXS, class LV>
+__device__ int br_ctx_of(const LV& lv, int ci, int cls) {
+  constexpr int AW = c_txw(TXS), BWL = c_log2(AW);
+  const int r = ci >> BWL, c = ci & (AW - 1);
+  int mag = lv(r, c + 1) + lv(r + 1, c);
+  mag += cls == kCls2D ? lv(r + 1, c + 1) : (cls == kClsHoriz ? lv(r, c + 2) : lv(r + 2, c));
+  mag = min((mag + 1) >> 1, 6);
+  if (ci == 0) return mag;
+  if ((cls == kCls2D && r < 2 && c < 2) || (cls == kClsHoriz && c == 0) || (cls == kClsVert && r == 0))
+    return mag + 7;
+  return mag + 14;
+}
+
 template <int TXS>
 struct Trellis {
   static constexpr int AW = c_txw(TXS), AH = c_txh(TXS), BWL = c_log2(AW), N = AW * AH, NW = (N + 7) / 8;
@@ -77,42 +129,8 @@ struct Trellis {
     const int sh = (p & 7) * 4;
     w = (w & ~(15u << sh)) | ((uint32_t)(v < 15 ? v : 15) << sh);
   }
-  // coeff_c.rs:344 nz_map_ctx_offset_2d: shape from the unadjusted size
-  __device__ __forceinline__ static int offset_2d(int ci) {
-    if (ci == 0) return 0;
-    const int row = ci >> BWL, col = ci - (row << BWL);
-    if (tx_w(TXS) < tx_h(TXS)) {
-      if (row < 2) return 11;
-    } else if (tx_w(TXS) > tx_h(TXS) && col < 2) {
-      return 16;
-    }
-    if (row + col < 2) return 1;
-    if (row + col < 4) return 6;
-    return 21;
-  }
-  // coeff_c.rs:406 nz_map_ctx (+ :306 nz_mag, :380 from_stats)
-  __device__ int lower_ctx(int ci, int si, bool is_last) const {
-    if (is_last) {
-      if (si == 0) return 0;
-      if (si <= N / 8) return 1;
-      if (si <= N / 4) return 2;
-      return 3;
-    }
-    const int r = ci >> BWL, c = ci & (AW - 1);
-    auto c3 = [](int v) { return v < 3 ? v : 3; };
-    int mag = c3(level(r, c + 1)) + c3(level(r + 1, c));
-    if (cls == kCls2D) {
-      mag += c3(level(r + 1, c + 1)) + c3(level(r, c + 2)) + c3(level(r + 2, c));
-    } else if (cls == kClsVert) {
-      mag += c3(level(r + 2, c)) + c3(level(r + 3, c)) + c3(level(r + 4, c));
-    } else {
-      mag += c3(level(r, c + 2)) + c3(level(r, c + 3)) + c3(level(r, c + 4));
-    }
-    if ((cls | ci) == 0) return 0;
-    const int ctx = min((mag + 1) >> 1, 4);
-    if (cls == kCls2D) return ctx + offset_2d(ci);
-    const int pos = cls == kClsHoriz ? c : r;
-    return ctx + (pos == 0 ? 26 : (pos == 1 ? 31 : 36));
+  __device__ __forceinline__ int lower_ctx(int ci, int si, bool is_last) const {
+    return nz_ctx<TXS>([this](int r, int c) { return level(r, c); }, ci, si, is_last, cls);
   }
   // coeff_c.rs:456 br_ctx_eob
   __device__ __forceinline__ int br_ctx_eob(int ci) const {
@@ -121,16 +139,8 @@ struct Trellis {
     if ((cls == kCls2D && r < 2 && c < 2) || (cls == kClsHoriz && c == 0) || (cls == kClsVert && r == 0)) return 7;
     return 14;
   }
-  // coeff_c.rs:537 br_ctx
-  __device__ int br_ctx(int ci) const {
-    const int r = ci >> BWL, c = ci & (AW - 1);
-    int mag = level(r, c + 1) + level(r + 1, c);
-    mag += cls == kCls2D ? level(r + 1, c + 1) : (cls == kClsHoriz ? level(r, c + 2) : level(r + 2, c));
-    mag = min((mag + 1) >> 1, 6);
-    if (ci == 0) return mag;
-    if ((cls == kCls2D && r < 2 && c < 2) || (cls == kClsHoriz && c == 0) || (cls == kClsVert && r == 0))
-      return mag + 7;
-    return mag + 14;
+  __device__ __forceinline__ int br_ctx(int ci) const {
+    return br_ctx_of<TXS>([this](int r, int c) { return level(r, c); }, ci, cls);
   }
   __device__ __forceinline__ int base_cost(int ctx, int k) const { return cost[kBase + ctx * 8 + k]; }
   __device__ __forceinline__ int lps(int ctx, int k) const { return cost[kLps + ctx * 26 + k]; }
@@ -440,6 +450,114 @@ __global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff,
     s.update_general(&accu_rate, &dummy, 0, eob);
   }
   eobs[j] = (uint16_t)eob;
+}
+
+// Coefficient rate (cost_coeffs_txb, lf/coeff_rate.rs:60, full middle loop)
+// of quantized blocks.  Without a trellis every scanned position's cost is a
+// function of the final level map, so the positions of a block are priced in
+// parallel: a group of G = min(64, N) lanes per block (64 / G blocks per
+// wave) walks the scan indices below the eob with stride G, reading levels
+// straight from the block (min(|q|, 127), 0 past its edges), and the group's
+// sum is reduced across its lanes; the group's first lane adds the skip-flag,
+// tx-type and eob costs.
+template <int TXS>
+__global__ __launch_bounds__(256) void k_coeff_rate(const int32_t* __restrict__ qcoeff,
+                                                    const uint16_t* __restrict__ eobs,
+                                                    const RateJob* __restrict__ jobs, int n_jobs,
+                                                    const int32_t* __restrict__ costs,
+                                                    const uint16_t* __restrict__ scans, int32_t* __restrict__ rate) {
+  constexpr int AW = c_txw(TXS), AH = c_txh(TXS), N = AW * AH;
+  constexpr int G = N < 64 ? N : 64, PER_WG = 256 / G;
+  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
+  __shared__ uint16_t sc[3 * N];
+  const int tid = threadIdx.x;
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = tid; i < 2 * kTxbWords; i += 256) tab[i] = txb_src[i];
+  for (int i = tid; i < 2 * kEobWords; i += 256) tab[2 * kTxbWords + i] = eob_src[i];
+  for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
+  __syncthreads();
+  const int g = tid % G, j = blockIdx.x * PER_WG + tid / G;
+  const bool live = j < n_jobs;
+  RateJob job{};
+  int eob = 0;
+  if (live) {
+    job = jobs[j];
+    eob = eobs[j];
+    if (eob > N || job.tx_class > 2 || job.plane_type > 1 || job.txb_skip_ctx >= 13 || job.dc_sign_ctx >= 3)
+      eob = -1;  // invalid fields: the block is not priced
+  }
+  const int32_t* q = qcoeff + (live ? job.coeff : 0);
+  const int32_t* cost = tab + (job.plane_type ? kTxbWords : 0);
+  const int32_t* ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
+  const uint16_t* scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
+  const int cls = job.tx_class;
+  auto lv = [q](int r, int c) -> int {
+    if (r >= AH || c >= AW) return 0;
+    return min(abs(q[r * AW + c]), 127);
+  };
+  int32_t sum = 0;
+  for (int i = g; i < eob; i += G) {
+    const int pos = scan[i], v = q[pos], l = abs(v);
+    const bool is_eob = i == eob - 1;
+    const int ctx = nz_ctx<TXS>(lv, pos, i, is_eob, cls);
+    if (is_eob) {
+      sum += cost[kBaseEob + ctx * 3 + max(min(l, 3) - 1, 0)];
+    } else if (l > 2 && i > 0) {
+      sum += cost[kBase + ctx * 8 + 3];
+    } else {
+      sum += cost[kBase + ctx * 8 + min(l, 3)];
+    }
+    if (v != 0) {
+      sum += pos == 0 ? cost[kDcSign + job.dc_sign_ctx * 2 + (v < 0)] : 512;
+      if (l > 2) sum += cost[kLps + br_ctx_of<TXS>(lv, pos, cls) * 26 + min(l - 3, 12)] + golomb_cost(l);
+    }
+  }
+  // group sum: 16-lane rows by DPP, then across rows
+  sum = dpp_sum_i32<(G < 16 ? G : 16)>(sum);
+  if constexpr (G >= 32) sum += __shfl_xor(sum, 16, 64);
+  if constexpr (G >= 64) sum += __shfl_xor(sum, 32, 64);
+  if (live && g == 0) {
+    int32_t r = 0;
+    if (eob > 0) {
+      r = sum + cost[kSkip + job.txb_skip_ctx * 2] + (job.plane_type == 0 ? job.txt_rate : 0);
+      // get_eob_cost (quant.rs:851)
+      int t;
+      if (eob < 33) {
+        t = eob <= 2 ? eob : (eob <= 4 ? 3 : (eob <= 8 ? 4 : (eob <= 16 ? 5 : 6)));
+      } else {
+        const int e = min((eob - 1) >> 5, 16);
+        t = e == 0 ? 6 : (e == 1 ? 7 : (e <= 3 ? 8 : (e <= 7 ? 9 : (e <= 15 ? 10 : 11))));
+      }
+      const int start = t == 0 ? 0 : (t == 1 ? 1 : (1 << (t - 2)) + 1);
+      r += ecost[(cls != kCls2D) * 11 + t - 1];
+      if (t > 2) {
+        const int cnt = t - 3, bit = ((eob - start) >> cnt) & 1;
+        r += cost[kEobExtra + cnt * 2 + bit] + 512 * cnt;
+      }
+    }
+    rate[j] = r;
+  }
+}
+
+hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* eobs, const RateJob* jobs,
+                            int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t* rate, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+#define ZC(T)                                                                                                \
+  case T: {                                                                                                  \
+    constexpr int N = c_txw(T) * c_txh(T), PER_WG = 256 / (N < 64 ? N : 64);                               \
+    hipLaunchKernelGGL((k_coeff_rate<T>), dim3((unsigned)((n_jobs + PER_WG - 1) / PER_WG)), dim3(256), 0, st, \
+                       qcoeff, eobs, jobs, n_jobs, costs, scans, rate);                                      \
+    break;                                                                                                   \
+  }
+  switch (tx_size) {
+    ZC(0) ZC(1) ZC(2) ZC(3) ZC(4) ZC(5) ZC(6) ZC(7) ZC(8) ZC(9) ZC(10) ZC(11) ZC(12) ZC(13) ZC(14) ZC(15) ZC(16)
+    ZC(17) ZC(18)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef ZC
+  return hipGetLastError();
 }
 
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,

@@ -196,6 +196,15 @@ struct RdoqJob {
   uint8_t pad[7];
 };
 static_assert(sizeof(RdoqJob) == 24, "RdoqJob layout");
+// One block of the coefficient-rate batch (same layout as zav1_rate_job).
+struct RateJob {
+  uint32_t coeff;
+  int32_t txt_rate;
+  uint8_t tx_class, plane_type, txb_skip_ctx, dc_sign_ctx;
+};
+static_assert(sizeof(RateJob) == 12, "RateJob layout");
+hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* eobs, const RateJob* jobs,
+                            int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t* rate, hipStream_t st);
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st);
