@@ -446,6 +446,21 @@ def rdoq_stage(zav1, synth, args, cpu=True):
         L.zav1_event_elapsed_ms(ev[0].value, ev[1].value, ctypes.byref(ms))
         tot_ms += ms.value
     ms_step = tot_ms / steps
+    # coefficient rate of every candidate after the trellis (one launch)
+    rjobs = np.zeros(nj, zav1.RATE_JOB)
+    rjobs['coeff'] = jobs['coeff']
+    rjobs['tx_class'] = jobs['tx_class']
+    d_rjobs, d_rate = D.from_host(rjobs), D((nj,), np.int32)
+    rate_ms = 0.0
+    for it in range(steps + 1):
+        L.zav1_event_record(ev[0].value, stream)
+        zav1.coeff_rate_batch(ts, d_q, d_eob, d_rjobs, nj, d_costs, d_scans, d_rate, stream=stream)
+        L.zav1_event_record(ev[1].value, stream)
+        zav1.sync(stream)
+        ms = ctypes.c_float()
+        L.zav1_event_elapsed_ms(ev[0].value, ev[1].value, ctypes.byref(ms))
+        rate_ms += ms.value if it else 0.0
+    rate_ms /= steps
     eob1 = d_eob.to_host()
     q1 = d_q.to_host()
     steps_coef = int(eob0.astype(np.int64).sum())
@@ -460,6 +475,7 @@ def rdoq_stage(zav1, synth, args, cpu=True):
            'trellis_steps_per_s': round(steps_coef / (ms_step / 1000.0)),
            'mean_eob_in': round(float(eob0.mean()), 2), 'mean_eob_out': round(float(eob1.mean()), 2),
            'levels_changed_frac': round(changed / max(1, int((q0 != 0).sum())), 4),
+           'coeff_rate_ms': round(rate_ms, 4), 'coeff_rate_blocks_per_s': round(nj / (rate_ms / 1000.0)),
            'roofline': {'bound': 'latency', 'kernel': 'k_rdoq<TX_16X16>', 'unit': 'GB/s',
                         'achieved': round(alg_bytes / (ms_step / 1000.0) / 1e9, 2), 'peak': 8000.0,
                         'frac': round(alg_bytes / (ms_step / 1000.0) / 1e9 / 8000.0, 4),
