@@ -271,7 +271,10 @@ typedef struct {
  * classes 0, 1, 2, each of the coded block's length).  Blocks with eob 0 are
  * left as they are.  cut_off_num / cut_off_denum: rdoq_cutoffs (:1551),
  * 0 / 0 for the full trellis.  A job with a field out of range is left as it
- * is. */
+ * is.  d_dqcoeff must hold the fp quantizer's dequantized values,
+ * sign((|q| * dequant) >> log_scale), which the trellis recomputes from the
+ * levels; the call takes a temporary of n_jobs * 8 bytes per coefficient
+ * from the stream-ordered allocator. */
 int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, int32_t *d_qcoeff,
                     int32_t *d_dqcoeff, uint16_t *d_eob, const zav1_rdoq_job *d_jobs, int n_jobs,
                     const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,

@@ -15,6 +15,8 @@
 // reference's zero padding (4 columns, >= 6 rows) holds.  The cost tables of
 // the launch's size (both plane types) and its three scans are staged in
 // LDS once per workgroup.
+#include <algorithm>
+
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
 
@@ -108,8 +110,8 @@ template <int TXS>
 struct Trellis {
   static constexpr int AW = c_txw(TXS), AH = c_txh(TXS), BWL = c_log2(AW), N = AW * AH, NW = (N + 7) / 8;
   static constexpr int SHIFT = c_scale(TXS);
-  const int32_t* tc;   // this block's coefficients
-  int32_t* qc;
+  const int2* work;    // this block's (t, q) pairs in scan order
+  int32_t* qc;         // levels and dequantized values, raster (written on change)
   int32_t* dq;
   uint32_t* lv;        // level words of lane 0; lane stride 1, word stride 64
   const int32_t* cost; // TxbCosts of the block's plane type
@@ -204,6 +206,16 @@ struct Trellis {
     }
     return c;
   }
+  // coefficient of scan index si (a placeholder for si < 0); the
+  // dequantized value follows the level as fp quantization sets it,
+  // sign((|q| * dequant) >> log_scale)
+  __device__ __forceinline__ void fetch(int si, int& ci, int32_t& t, int32_t& q, int32_t& d) const {
+    ci = scan[si];
+    const int2 v = work[si];
+    t = v.x, q = v.y;
+    const int32_t a = (int32_t)(((int64_t)abs(q) * dqv[ci != 0]) >> SHIFT);
+    d = q < 0 ? -a : a;
+  }
   __device__ __forceinline__ void low(int a, int sign, int dv, int32_t* q, int32_t* d) const {
     const int lo = a - 1;
     *q = (-sign ^ lo) + sign;
@@ -213,7 +225,10 @@ struct Trellis {
 
   // quant.rs:966 update_coeff_general
   __device__ void update_general(int* accu_rate, int64_t* accu_dist, int si, int eob) {
-    const int ci = scan[si], dv = dqv[ci != 0], qv = qc[ci];
+    int ci;
+    int32_t t, qv, d;
+    fetch(si, ci, t, qv, d);
+    const int dv = dqv[ci != 0];
     const bool is_last = si == eob - 1;
     const int ctx = lower_ctx(ci, si, is_last);
     if (qv == 0) {
@@ -221,7 +236,6 @@ struct Trellis {
       return;
     }
     const int sign = qv < 0, a = qv < 0 ? -qv : qv;
-    const int32_t t = tc[ci], d = dq[ci];
     const int64_t dist = coeff_dist(t, d, SHIFT), dist0 = coeff_dist(t, 0, SHIFT);
     const int rate = cost_general(is_last, ci, a, sign, ctx);
     const int64_t rd = rdcost(rdmult, rate, dist);
@@ -249,14 +263,17 @@ struct Trellis {
   }
   // quant.rs:1056 update_coeff_simple
   __device__ void update_simple(int* accu_rate, int si) {
-    const int ci = scan[si], dv = dqv[ci != 0], qv = qc[ci];
+    int ci;
+    int32_t t, qv, d;
+    fetch(si, ci, t, qv, d);
+    const int dv = dqv[ci != 0];
     const int ctx = lower_ctx(ci, si, false);
     if (qv == 0) {
       *accu_rate += base_cost(ctx, 0);
       return;
     }
     const int a = qv < 0 ? -qv : qv;
-    const int32_t at = abs(tc[ci]), ad = abs(dq[ci]);
+    const int32_t at = abs(t), ad = abs(d);
     int rate_low = 0;
     const int rate = cost_simple(ci, a, ctx, &rate_low);
     if (ad < at) {
@@ -279,7 +296,10 @@ struct Trellis {
   // positions since the eob (at most 5)
   __device__ void update_eob(int* accu_rate, int64_t* accu_dist, int* eob, int* nz_num, int* nz_ci, int si,
                              bool sharp) {
-    const int ci = scan[si], dv = dqv[ci != 0], qv = qc[ci];
+    int ci;
+    int32_t t, qv, d;
+    fetch(si, ci, t, qv, d);
+    const int dv = dqv[ci != 0];
     const int ctx = lower_ctx(ci, si, false);
     if (qv == 0) {
       *accu_rate += base_cost(ctx, 0);
@@ -287,7 +307,6 @@ struct Trellis {
     }
     bool lower = false;
     const int a = qv < 0 ? -qv : qv, sign = qv < 0;
-    const int32_t t = tc[ci], d = dq[ci];
     const int64_t dist0 = coeff_dist(t, 0, SHIFT);
     int64_t dist = coeff_dist(t, d, SHIFT) - dist0;
     int rate = cost_general(false, ci, a, sign, ctx);
@@ -348,7 +367,7 @@ struct Trellis {
       dq[ci] = d_low;
       set_level(ci, a - 1);
     }
-    if (qc[ci] != 0) {
+    if (lower ? q_low != 0 : true) {  // the level kept is nonzero
 #pragma unroll
       for (int k = 0; k < 5; k++)
         if (k == *nz_num) nz_ci[k] = ci;
@@ -359,7 +378,7 @@ struct Trellis {
 }  // namespace
 
 template <int TXS>
-__global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff,
+__global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, int32_t* __restrict__ qcoeff,
                                              int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
                                              const RdoqJob* __restrict__ jobs, int n_jobs,
                                              const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans,
@@ -386,7 +405,7 @@ __global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff,
       job.dc_sign_ctx >= 3)
     return;
   T s;
-  s.tc = tcoeff + job.coeff;
+  s.work = work + (size_t)j * N;
   s.qc = qcoeff + job.coeff;
   s.dq = dqcoeff + job.coeff;
   s.lv = lv + lane;
@@ -414,13 +433,15 @@ __global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff,
   int64_t accu_dist = 0;
   int si = eob - 1;
   {
-    const int ci = s.scan[si], qv = s.qc[ci];
+    int ci;
+    int32_t t, qv, d;
+    s.fetch(si, ci, t, qv, d);
     const int a = qv < 0 ? -qv : qv, sign = qv < 0;
     if (a >= 2) {
       s.update_general(&accu_rate, &accu_dist, si, eob);
     } else {
       accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
-      accu_dist += coeff_dist(s.tc[ci], s.dq[ci], T::SHIFT) - coeff_dist(s.tc[ci], 0, T::SHIFT);
+      accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
     }
     si--;
   }
@@ -560,24 +581,57 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
   return hipGetLastError();
 }
 
+// (t, q) of every scan index of every block, in scan order: the trellis then
+// reads consecutive pairs as its scan index walks down, instead of one
+// cache line of a raster row per step.
+template <int TXS>
+__global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__ tcoeff,
+                                                     const int32_t* __restrict__ qcoeff,
+                                                     const uint16_t* __restrict__ eobs,
+                                                     const RdoqJob* __restrict__ jobs, int n_jobs,
+                                                     const uint16_t* __restrict__ scans, int2* __restrict__ work) {
+  constexpr int N = c_txw(TXS) * c_txh(TXS);
+  __shared__ uint16_t sc[3 * N];
+  for (int i = threadIdx.x; i < 3 * N; i += 256) sc[i] = scans[i];
+  __syncthreads();
+  const long k = (long)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (long)n_jobs * N) return;
+  const int j = (int)(k / N), i = (int)(k - (long)j * N);
+  if (eobs[j] == 0) return;  // a block the trellis leaves alone is not read
+  const RdoqJob job = jobs[j];
+  const int cls = job.tx_class > 2 ? 0 : job.tx_class;
+  const int ci = sc[(cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N + i];
+  work[k] = make_int2(tcoeff[job.coeff + ci], qcoeff[job.coeff + ci]);
+}
+
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
   if (n_jobs <= 0) return hipSuccess;
+  const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
+  int2* work = nullptr;
+  hipError_t e = hipMallocAsync((void**)&work, (size_t)n_jobs * n * sizeof(int2), st);
+  if (e != hipSuccess) return e;
   const dim3 grid((unsigned)((n_jobs + 63) / 64));
+  const dim3 ggrid((unsigned)(((long)n_jobs * n + 255) / 256));
 #define ZR(T)                                                                                                   \
   case T:                                                                                                       \
-    hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, \
+    hipLaunchKernelGGL((k_rdoq_gather<T>), ggrid, dim3(256), 0, st, tcoeff, qcoeff, eobs, jobs, n_jobs, scans,  \
+                       work);                                                                               \
+    hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, work, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs,   \
                        scans, dq0, dq1, cut_num, cut_den);                                                      \
     break;
   switch (tx_size) {
     ZR(0) ZR(1) ZR(2) ZR(3) ZR(4) ZR(5) ZR(6) ZR(7) ZR(8) ZR(9) ZR(10) ZR(11) ZR(12) ZR(13) ZR(14) ZR(15) ZR(16)
     ZR(17) ZR(18)
     default:
+      hipFreeAsync(work, st);
       return hipErrorInvalidValue;
   }
 #undef ZR
-  return hipGetLastError();
+  e = hipGetLastError();
+  const hipError_t f = hipFreeAsync(work, st);
+  return e != hipSuccess ? e : f;
 }
 
 }  // namespace zav1
