@@ -742,14 +742,19 @@ __device__ __forceinline__ uint64_t wg_sum_u64(uint64_t v, uint64_t* red) {
 // accumulators meet in LDS (ds_add) and the final H / M are formed from it.
 // The i32 sums stay exact: |digit products| <= 1024 over < 2^15 pixels.
 constexpr int kWnRB = 16, kWnCT = 128;
+// LDS row stride of the digit planes: 9 x 16 bytes, so the seven dy rows a
+// lane group reads start 36 dwords apart and spread over the banks
+constexpr int kWnRS = kWnCT + 16;
 typedef int32_t wn_v4i __attribute__((ext_vector_type(4)));
 
 template <int WIN>
 struct WnGeom {
   static constexpr int W2 = WIN * WIN, HALF = WIN / 2, NF = 2 * W2 + 2, NB = (NF + 15) / 16;
   static constexpr int NT = NB * (NB + 1) / 2;
-  static constexpr int PR = kWnRB + 2 * HALF, PLANE = PR * kWnCT, XOFF = 2 * WIN * PLANE;
-  static constexpr int LDS = XOFF + 2 * kWnRB * kWnCT;
+  // planes 16 bytes longer than their rows: the dx planes of a lane group
+  // then land on different banks as well (two-way conflicts at most)
+  static constexpr int PR = kWnRB + 2 * HALF, PLANE = PR * kWnRS + 16, XOFF = 2 * WIN * PLANE;
+  static constexpr int LDS = XOFF + 2 * kWnRB * kWnRS;
   static_assert(NT * 256 * 4 <= LDS, "reduction fits the staging area");
 };
 
@@ -799,9 +804,9 @@ __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dg
     if (f >= G::NF) f = 0;
     if (f < 2 * W2) {
       const int d = f >= W2, k = f - d * W2, dx = k / WIN, dy = k - dx * WIN;
-      addr[b] = (d * WIN + dx) * G::PLANE + (dy + g) * kWnCT;
+      addr[b] = (d * WIN + dx) * G::PLANE + (dy + g) * kWnRS;
     } else {
-      addr[b] = G::XOFF + (f - 2 * W2) * kWnRB * kWnCT + g * kWnCT;
+      addr[b] = G::XOFF + (f - 2 * W2) * kWnRB * kWnRS + g * kWnRS;
     }
   }
   wn_v4i acc[G::NT];
@@ -846,8 +851,8 @@ __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dg
               vl[q] = (int32_t)(sh ? __builtin_amdgcn_alignbyte(wl[n + 1], wl[n], sh) : wl[n]);
             }
             if (vb < 16) vh = keep_bytes(vh, vb), vl = keep_bytes(vl, vb);
-            *(wn_v4i*)(lds + dx * G::PLANE + rr * kWnCT + 16 * j) = vh;
-            *(wn_v4i*)(lds + (WIN + dx) * G::PLANE + rr * kWnCT + 16 * j) = vl;
+            *(wn_v4i*)(lds + dx * G::PLANE + rr * kWnRS + 16 * j) = vh;
+            *(wn_v4i*)(lds + (WIN + dx) * G::PLANE + rr * kWnRS + 16 * j) = vl;
           }
         } else {
           const int t2 = task - ny, rr = t2 / nc, j = t2 - rr * nc;
@@ -867,15 +872,15 @@ __global__ __launch_bounds__(256) void k_wiener_stats(const Pix* __restrict__ dg
           }
           const int vb = tw - 16 * j;
           if (vb < 16) vh = keep_bytes(vh, vb), vl = keep_bytes(vl, vb);
-          *(wn_v4i*)(lds + G::XOFF + rr * kWnCT + 16 * j) = vh;
-          *(wn_v4i*)(lds + G::XOFF + kWnRB * kWnCT + rr * kWnCT + 16 * j) = vl;
+          *(wn_v4i*)(lds + G::XOFF + rr * kWnRS + 16 * j) = vh;
+          *(wn_v4i*)(lds + G::XOFF + kWnRB * kWnRS + rr * kWnRS + 16 * j) = vl;
         }
       }
       __syncthreads();
       if (4 * w < br) {  // wave w: tile rows 4w .. 4w + 3
         const bool rowok = 4 * w + g < br, ragged = 4 * w + 4 > br;
         for (int j = 0; j < nc; j++) {
-          const int off = 4 * w * kWnCT + 16 * j;
+          const int off = 4 * w * kWnRS + 16 * j;
           wn_v4i fr[NB];
 #pragma unroll
           for (int b = 0; b < NB; b++) fr[b] = *(const wn_v4i*)(lds + addr[b] + off);
