@@ -100,6 +100,18 @@ SGR_OPS_PER_PX = (10 * (SGR_OPS_R2 + SGR_OPS_R1 + SGR_OPS_PROJ) + 4 * (SGR_OPS_R
                   2 * (SGR_OPS_R2 + SGR_OPS_PROJ))
 
 
+def pmc_kernel(kernel_prefix, bd=None, nq=None):
+    """The committed PMC summary's name of a fused-search instance: the key
+    starting with kernel_prefix whose template arguments carry the bit depth
+    bd and end in nq quantizers per launch (the first match otherwise)."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
+    if not os.path.exists(path):
+        return kernel_prefix
+    names = [k for k in json.load(open(path)).get('kernels', {}) if k.startswith(kernel_prefix)]
+    exact = [k for k in names if (bd is None or ', %d, ' % bd in k) and (nq is None or k.endswith(', %d>' % nq))]
+    return (exact or names or [kernel_prefix])[0]
+
+
 def measured_valu_insts(kernel_prefix):
     """VALU instructions per launch of a kernel (SQ_INSTS_VALU, wave-level,
     from the committed PMC summary), or None."""
@@ -685,6 +697,7 @@ def main():
     achieved_tops = ops_launch / avg_launch_s / 1e12
     kname = {'lane': 'k_txsearch_lane', 'rows': 'k_txsearch_rows', 'group': 'k_txsearch'}[form]
 
+    dom_pmc = pmc_kernel('%s<%d, %d, ' % (kname, w, h), bd=10, nq=q_per_launch)
     out = {
         'metric': 'MP/s (luma megapixels x qps) through batched candidate RD, byte-identical to the oracle',
         'value': round(value, 2), 'unit': 'MP/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
@@ -698,10 +711,11 @@ def main():
         'per_size_ms_per_step': {str(tsm.TX_DIMS[t]): round(per_size_ms[t], 4) for t in range(19)},
         'roofline': {'bound': 'valu', 'kernel': '%s<%d,%d>' % (kname, w, h), 'achieved': round(achieved_tops, 3),
                      'peak': VALU_PEAK_TOPS, 'unit': 'Tops/s', 'frac': round(achieved_tops / VALU_PEAK_TOPS, 4),
-                     'traffic': measured_traffic('%s<%d, %d, ' % (kname, w, h)),
-                     'valu_issue_frac': (None if measured_valu_insts('%s<%d, %d, ' % (kname, w, h)) is None else
-                                         round(measured_valu_insts('%s<%d, %d, ' % (kname, w, h)) * 64 /
-                                               avg_launch_s / VALU_ISSUE_PEAK, 4)),
+                     'traffic': measured_traffic(dom_pmc),
+                     'valu_issue_frac': (None if measured_valu_insts(dom_pmc) is None else
+                                         round(measured_valu_insts(dom_pmc) * 64 / avg_launch_s / VALU_ISSUE_PEAK,
+                                               4)),
+                     'pmc_kernel': dom_pmc,
                      'quantizers_per_launch': q_per_launch,
                      'avg_launch_us': round(avg_launch_s * 1e6, 2),
                      'algorithmic_ops_per_launch': int(ops_launch),
