@@ -491,6 +491,7 @@ __global__ __launch_bounds__(256) void k_coeff_rate(const int32_t* __restrict__ 
   constexpr int G = N < 64 ? N : 64, PER_WG = 256 / G;
   __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
   __shared__ uint16_t sc[3 * N];
+  __shared__ int32_t qs[PER_WG * N];  // each group's block, raster
   const int tid = threadIdx.x;
   const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
   const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
@@ -508,7 +509,11 @@ __global__ __launch_bounds__(256) void k_coeff_rate(const int32_t* __restrict__ 
     if (eob > N || job.tx_class > 2 || job.plane_type > 1 || job.txb_skip_ctx >= 13 || job.dc_sign_ctx >= 3)
       eob = -1;  // invalid fields: the block is not priced
   }
-  const int32_t* q = qcoeff + (live ? job.coeff : 0);
+  // the group's block into LDS, coalesced; the group is one wave's lanes
+  int32_t* q = qs + (tid / G) * N;
+  if (eob > 0)
+    for (int p = g; p < N; p += G) q[p] = qcoeff[job.coeff + p];
+  wave_sync();
   const int32_t* cost = tab + (job.plane_type ? kTxbWords : 0);
   const int32_t* ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
   const uint16_t* scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
@@ -583,25 +588,36 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
 
 // (t, q) of every scan index of every block, in scan order: the trellis then
 // reads consecutive pairs as its scan index walks down, instead of one
-// cache line of a raster row per step.
+// cache line of a raster row per step.  A workgroup reads whole blocks in
+// raster order (coalesced) into LDS and writes them out permuted.
 template <int TXS>
 __global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__ tcoeff,
                                                      const int32_t* __restrict__ qcoeff,
                                                      const uint16_t* __restrict__ eobs,
                                                      const RdoqJob* __restrict__ jobs, int n_jobs,
                                                      const uint16_t* __restrict__ scans, int2* __restrict__ work) {
-  constexpr int N = c_txw(TXS) * c_txh(TXS);
+  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = N < 256 ? 256 / N : 1;
   __shared__ uint16_t sc[3 * N];
-  for (int i = threadIdx.x; i < 3 * N; i += 256) sc[i] = scans[i];
+  __shared__ int32_t st[BPW * N], sq[BPW * N];
+  const int tid = threadIdx.x, j0 = blockIdx.x * BPW;
+  for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
+  for (int k = tid; k < BPW * N; k += 256) {
+    const int b = k / N, p = k - b * N, j = j0 + b;
+    if (j < n_jobs && eobs[j] != 0) {  // a block the trellis leaves alone is not read
+      const uint32_t off = jobs[j].coeff;
+      st[k] = tcoeff[off + p];
+      sq[k] = qcoeff[off + p];
+    }
+  }
   __syncthreads();
-  const long k = (long)blockIdx.x * 256 + threadIdx.x;
-  if (k >= (long)n_jobs * N) return;
-  const int j = (int)(k / N), i = (int)(k - (long)j * N);
-  if (eobs[j] == 0) return;  // a block the trellis leaves alone is not read
-  const RdoqJob job = jobs[j];
-  const int cls = job.tx_class > 2 ? 0 : job.tx_class;
-  const int ci = sc[(cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N + i];
-  work[k] = make_int2(tcoeff[job.coeff + ci], qcoeff[job.coeff + ci]);
+  for (int k = tid; k < BPW * N; k += 256) {
+    const int b = k / N, i = k - b * N, j = j0 + b;
+    if (j < n_jobs && eobs[j] != 0) {
+      const int cls = jobs[j].tx_class > 2 ? 0 : jobs[j].tx_class;
+      const int ci = sc[(cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N + i];
+      work[(long)j * N + i] = make_int2(st[b * N + ci], sq[b * N + ci]);
+    }
+  }
 }
 
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
@@ -613,7 +629,8 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
   hipError_t e = hipMallocAsync((void**)&work, (size_t)n_jobs * n * sizeof(int2), st);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)((n_jobs + 63) / 64));
-  const dim3 ggrid((unsigned)(((long)n_jobs * n + 255) / 256));
+  const int bpw = n < 256 ? 256 / n : 1;
+  const dim3 ggrid((unsigned)((n_jobs + bpw - 1) / bpw));
 #define ZR(T)                                                                                                   \
   case T:                                                                                                       \
     hipLaunchKernelGGL((k_rdoq_gather<T>), ggrid, dim3(256), 0, st, tcoeff, qcoeff, eobs, jobs, n_jobs, scans,  \
