@@ -443,11 +443,14 @@ struct TapsU {
   uint32_t ap[4], as[8];
   int32_t lo, hi;
 };
-__device__ __forceinline__ void gather_u(const uint16_t* c, int dir, int32_t x, TapsU& t) {
+// dirs: the 12 x 2 tap-offset table (kCdefDir, or its LDS copy when the
+// direction differs between lanes)
+__device__ __forceinline__ void gather_u(const uint16_t* c, int dir, int32_t x, TapsU& t,
+                                         const int (*dirs)[2] = kCdefDir) {
   int32_t mn = x, mx = x;
 #pragma unroll
   for (int k = 0; k < 2; k++) {
-    const int dk = kCdefDir[dir + 2][k], da = kCdefDir[dir + 4][k], db = kCdefDir[dir][k];
+    const int dk = dirs[dir + 2][k], da = dirs[dir + 4][k], db = dirs[dir][k];
     const int32_t v[6] = {(int16_t)c[dk], (int16_t)c[-dk], (int16_t)c[da], (int16_t)c[-da], (int16_t)c[db],
                           (int16_t)c[-db]};
 #pragma unroll
@@ -611,6 +614,7 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
   __shared__ uint32_t red[4][64];
   __shared__ uint64_t total[64];
   __shared__ uint8_t sact[64];
+  __shared__ int sdirs[12][2];  // per-lane directions: tap offsets from LDS, not per-lane constant loads
   const int fb = xcd_chunked(blockIdx.x, gridDim.x);
   const int fbr = fb / nhfb, fbc = fb - fbr * nhfb;
   const int cs = bd - 8, cw = width >> 1, ch = height >> 1;
@@ -619,6 +623,7 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
   const int nr = vsize & ~3, nc = hsize & ~3;
   const int tid = threadIdx.x;
   const int dmp = damping + cs - 1;
+  if (tid < 24) sdirs[tid >> 1][tid & 1] = kCdefDir[tid >> 1][tid & 1];
   if (tid < 64) {
     total[tid] = 0;
     sact[tid] = !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + (tid >> 3) * 2, fbc * 16 + (tid & 7) * 2);
@@ -644,7 +649,7 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
       const uint16_t* cp = tile + (kVB + r) * kTS + kHB + c;
       const int32_t x = (int16_t)cp[0];
       TapsU tb, t0;
-      gather_u(cp, dir, x, tb);
+      gather_u(cp, dir, x, tb, sdirs);
       gather_u(cp, 0, x, t0);
       const int32_t sx = (int32_t)srcp[(long)(y0 + r) * cw + x0 + c] - x;
       int32_t secb[4], sec0[4];
