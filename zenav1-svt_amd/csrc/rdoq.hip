@@ -596,26 +596,39 @@ __global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__
                                                      const uint16_t* __restrict__ eobs,
                                                      const RdoqJob* __restrict__ jobs, int n_jobs,
                                                      const uint16_t* __restrict__ scans, int2* __restrict__ work) {
-  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = N < 256 ? 256 / N : 1;
+  // 1024 coefficients per workgroup (four per thread): enough loads in
+  // flight per workgroup to hide their latency
+  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = N < 1024 ? 1024 / N : 1;
   __shared__ uint16_t sc[3 * N];
   __shared__ int32_t st[BPW * N], sq[BPW * N];
+  __shared__ uint32_t boff[BPW];  // per block: coefficient offset, or ~0 when it is not read
+  __shared__ int bscan[BPW];      // per block: its scan's offset in sc
   const int tid = threadIdx.x, j0 = blockIdx.x * BPW;
   for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
+  if (tid < BPW) {
+    const int j = j0 + tid;
+    // a block the trellis leaves alone (eob 0) is not read
+    const bool use = j < n_jobs && eobs[j] != 0;
+    const RdoqJob job = use ? jobs[j] : RdoqJob{};
+    const int cls = job.tx_class > 2 ? 0 : job.tx_class;
+    boff[tid] = use ? job.coeff : ~0u;
+    bscan[tid] = (cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N;
+  }
+  __syncthreads();
   for (int k = tid; k < BPW * N; k += 256) {
-    const int b = k / N, p = k - b * N, j = j0 + b;
-    if (j < n_jobs && eobs[j] != 0) {  // a block the trellis leaves alone is not read
-      const uint32_t off = jobs[j].coeff;
+    const int b = k / N, p = k - b * N;
+    const uint32_t off = boff[b];
+    if (off != ~0u) {
       st[k] = tcoeff[off + p];
       sq[k] = qcoeff[off + p];
     }
   }
   __syncthreads();
   for (int k = tid; k < BPW * N; k += 256) {
-    const int b = k / N, i = k - b * N, j = j0 + b;
-    if (j < n_jobs && eobs[j] != 0) {
-      const int cls = jobs[j].tx_class > 2 ? 0 : jobs[j].tx_class;
-      const int ci = sc[(cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N + i];
-      work[(long)j * N + i] = make_int2(st[b * N + ci], sq[b * N + ci]);
+    const int b = k / N, i = k - b * N;
+    if (boff[b] != ~0u) {
+      const int ci = sc[bscan[b] + i];
+      work[(long)(j0 + b) * N + i] = make_int2(st[b * N + ci], sq[b * N + ci]);
     }
   }
 }
@@ -629,7 +642,7 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
   hipError_t e = hipMallocAsync((void**)&work, (size_t)n_jobs * n * sizeof(int2), st);
   if (e != hipSuccess) return e;
   const dim3 grid((unsigned)((n_jobs + 63) / 64));
-  const int bpw = n < 256 ? 256 / n : 1;
+  const int bpw = n < 1024 ? 1024 / n : 1;
   const dim3 ggrid((unsigned)((n_jobs + bpw - 1) / bpw));
 #define ZR(T)                                                                                                   \
   case T:                                                                                                       \
