@@ -378,7 +378,8 @@ struct Trellis {
 }  // namespace
 
 template <int TXS>
-__global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, int32_t* __restrict__ qcoeff,
+__global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, const uint32_t* __restrict__ lvwork,
+                                             int32_t* __restrict__ qcoeff,
                                              int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
                                              const RdoqJob* __restrict__ jobs, int n_jobs,
                                              const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans,
@@ -418,16 +419,8 @@ __global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, int3
   s.cls = job.tx_class;
   s.dc_ctx = job.dc_sign_ctx;
   const bool sharp = job.sharpness != 0;
-  // txb_init_levels (coeff_c.rs:278)
-  for (int w = 0; w < T::NW; w++) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const int p = 8 * w + k;
-      if (p < N) v |= (uint32_t)min(abs(s.qc[p]), 15) << (4 * k);
-    }
-    s.lv[w * 64] = v;
-  }
+  // txb_init_levels (coeff_c.rs:278), packed by the gather
+  for (int w = 0; w < T::NW; w++) s.lv[w * 64] = lvwork[(long)j * T::NW + w];
   const int non_skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2], skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2 + 1];
   int accu_rate = s.eob_cost(eob);
   int64_t accu_dist = 0;
@@ -595,7 +588,8 @@ __global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__
                                                      const int32_t* __restrict__ qcoeff,
                                                      const uint16_t* __restrict__ eobs,
                                                      const RdoqJob* __restrict__ jobs, int n_jobs,
-                                                     const uint16_t* __restrict__ scans, int2* __restrict__ work) {
+                                                     const uint16_t* __restrict__ scans, int2* __restrict__ work,
+                                                     uint32_t* __restrict__ lvwork) {
   // 1024 coefficients per workgroup (four per thread): enough loads in
   // flight per workgroup to hide their latency
   constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = N < 1024 ? 1024 / N : 1;
@@ -631,6 +625,19 @@ __global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__
       work[(long)(j0 + b) * N + i] = make_int2(st[b * N + ci], sq[b * N + ci]);
     }
   }
+  // the blocks' level maps, as the trellis keeps them: min(|q|, 15) in
+  // nibbles, eight raster positions per word
+  constexpr int NW = (N + 7) / 8;
+  for (int k = tid; k < BPW * NW; k += 256) {
+    const int b = k / NW, w = k - b * NW;
+    if (boff[b] != ~0u) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (8 * w + e < N) v |= (uint32_t)min(abs(sq[b * N + 8 * w + e]), 15) << (4 * e);
+      lvwork[(long)(j0 + b) * NW + w] = v;
+    }
+  }
 }
 
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
@@ -638,17 +645,20 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
   if (n_jobs <= 0) return hipSuccess;
   const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
-  int2* work = nullptr;
-  hipError_t e = hipMallocAsync((void**)&work, (size_t)n_jobs * n * sizeof(int2), st);
+  const int nw = (n + 7) / 8;
+  int2* work = nullptr;  // scan-order pairs, then the packed level maps
+  hipError_t e = hipMallocAsync((void**)&work, (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t)), st);
   if (e != hipSuccess) return e;
+  uint32_t* lvwork = reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n);
   const dim3 grid((unsigned)((n_jobs + 63) / 64));
   const int bpw = n < 1024 ? 1024 / n : 1;
   const dim3 ggrid((unsigned)((n_jobs + bpw - 1) / bpw));
 #define ZR(T)                                                                                                   \
   case T:                                                                                                       \
     hipLaunchKernelGGL((k_rdoq_gather<T>), ggrid, dim3(256), 0, st, tcoeff, qcoeff, eobs, jobs, n_jobs, scans,  \
-                       work);                                                                               \
-    hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, work, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs,   \
+                       work, lvwork);                                                                       \
+    hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, work, lvwork, qcoeff, dqcoeff, eobs, jobs, n_jobs,  \
+                       costs,                                                                               \
                        scans, dq0, dq1, cut_num, cut_den);                                                      \
     break;
   switch (tx_size) {
