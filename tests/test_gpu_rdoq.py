@@ -110,3 +110,42 @@ def test_coeff_rate_batch_bitexact(gpu, oracle, tx_size):
         want = oracle.cost_coeffs_txb(Q[b], int(E[b]), sc, tc, ec, tx_size, cls, int(j['plane_type']),
                                       int(j['txb_skip_ctx']), int(j['dc_sign_ctx']), int(j['txt_rate']))
         assert int(got[b]) == want, (tx_size, b, int(got[b]), want)
+
+
+def test_rdoq_and_rate_leave_invalid_jobs_alone(gpu, oracle):
+    """Blocks whose job fields are out of range (tx class, plane type,
+    contexts) are left as they are by the trellis and priced 0 by the rate;
+    their neighbours in the batch are processed normally."""
+    z = gpu
+    tx_size, n = 1, 64
+    costs = RT.coeff_costs(31)
+    _, txs, ems = oracle.rdoq_size_ctx(tx_size)
+    scans = np.concatenate([oracle.scan(tx_size, c) for c in (0, 1, 2)]).astype(np.uint16)
+    t, q, dq, eob, scan, dqv = RC.blocks(oracle, tx_size, 0, 8, 77, 40, 8)
+    jobs = np.zeros(8, z.RDOQ_JOB)
+    jobs['rdmult'] = 1054880
+    jobs['coeff'] = np.arange(8) * n
+    bad = {1: ('tx_class', 3), 3: ('plane_type', 2), 5: ('txb_skip_ctx', 13), 7: ('dc_sign_ctx', 3)}
+    for b, (f, v) in bad.items():
+        jobs[b][f] = v
+    rjobs = np.zeros(8, z.RATE_JOB)
+    rjobs['coeff'] = jobs['coeff']
+    for b, (f, v) in bad.items():
+        rjobs[b][f] = v
+    D = z.DeviceArray
+    dq_, ddq, de = D.from_host(q), D.from_host(dq), D.from_host(eob.astype(np.uint16))
+    dcosts, dscans = D.from_host(costs), D.from_host(scans)
+    z.rdoq_batch(tx_size, 40, 8, D.from_host(t), dq_, ddq, de, D.from_host(jobs), 8, dcosts, dscans)
+    gq, ge = dq_.to_host(), de.to_host()
+    d_rate = D((8,), np.int32)
+    z.coeff_rate_batch(tx_size, dq_, de, D.from_host(rjobs), 8, dcosts, dscans, d_rate)
+    rate = d_rate.to_host()
+    tc, ec = RT.tables_for(costs, txs, ems, 0)
+    for b in range(8):
+        if b in bad:
+            assert np.array_equal(gq[b], q[b]) and ge[b] == eob[b], b
+            assert rate[b] == 0, b
+        elif eob[b]:
+            rq, _, re = oracle.optimize_b(t[b], q[b], dq[b], int(eob[b]), scan, dqv, tc, ec, 1054880, tx_size, 0)
+            assert np.array_equal(gq[b], rq) and ge[b] == re, b
+            assert rate[b] == (oracle.cost_coeffs_txb(rq, re, scan, tc, ec, tx_size, 0) if re else 0), b
