@@ -972,7 +972,8 @@ __device__ __forceinline__ int x_by_xplus1(int z) {
 
 // Self-guided projection statistics, strip form.
 //
-// One wave walks one restoration-unit row of one 64-column strip of the
+// One wave walks one half of a restoration-unit row (an even number of
+// rows, then the rest) of one 64-column strip of the
 // plane (lane L <-> column x0 - 3 + L, clamped to the plane like every
 // other read), carrying four parameter sets; a workgroup is the four waves
 // of one strip, which between them carry all 16 sets.  Everything a row
@@ -983,7 +984,7 @@ __device__ __forceinline__ int x_by_xplus1(int z) {
 // radius-2 pass's, and the set's five projection sums.  Output columns are
 // lanes 3 .. 60 (every lane a filter reaches is then inside the wave), so
 // strips advance by 58 columns and may cross unit boundaries: each lane
-// knows its unit, and at the end of the unit row the wave adds its lanes'
+// knows its unit, and at the end of its rows the wave adds its lanes'
 // sums per unit into `out` with 64-bit atomics (integer, so the order does
 // not matter).  A zero pass before and a division pass after turn the sums
 // into the sums / unit area the projection solver reads.
@@ -1187,7 +1188,7 @@ __device__ __forceinline__ void sgr_strip_wave(const Pix* __restrict__ dgd, cons
   }
 }
 
-// Grid: one workgroup of four waves per (unit row, 58-column strip); the
+// Grid: one workgroup of four waves per (unit row half, 58-column strip); the
 // waves carry sets {0,1,2,14}, {3,4,5,15}, {6,7,10,11}, {8,9,12,13}, which
 // evens out their work (three two-pass sets and one radius-2 set against two
 // two-pass and two radius-1 sets).
@@ -1198,10 +1199,16 @@ __global__ __launch_bounds__(256) void k_sgr_strip(const Pix* __restrict__ dgd, 
   __shared__ uint8_t tab[256];  // 256 - x_by_xplus1(z)
   tab[threadIdx.x] = (uint8_t)(256 - x_by_xplus1(threadIdx.x));
   __syncthreads();
+  // each unit row in two chunks of rows (the first an even number of rows
+  // long, so the radius-2 pass keeps its odd rows): twice the waves, a
+  // shorter tail
   const int b = xcd_chunked(blockIdx.x, gridDim.x);
-  const int uy = b / nstrips, sx = b - uy * nstrips;
-  int hs, he, vs, ve;
-  unit_limits(unit_size, pw, ph, ss_y, 0, uy, hs, he, vs, ve);
+  const int uy = b / (2 * nstrips), rem = b - uy * 2 * nstrips, half = rem / nstrips, sx = rem - half * nstrips;
+  int hs, he, vs0, ve0;
+  unit_limits(unit_size, pw, ph, ss_y, 0, uy, hs, he, vs0, ve0);
+  const int mid = vs0 + (((ve0 - vs0) >> 1) & ~1);
+  const int vs = half ? mid : vs0, ve = half ? ve0 : mid;
+  if (vs >= ve) return;
   const int x0 = sx * kSgStrip, w = threadIdx.x >> 6;
   if (w == 0)
     sgr_strip_wave<Pix, kSgBoth, kSgBoth, kSgBoth, kSgR0>(dgd, src, pw, ph, x0, vs, ve, uy, nh, unit_size, tab,
@@ -1352,7 +1359,7 @@ hipError_t sgr_stats(const void* dgd, const void* src, int pw, int ph, int bd, i
                      int nv, int64_t* out, hipStream_t st) {
   const int nstrips = (pw + kSgStrip - 1) / kSgStrip, n = nh * nv * 80;
   hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)std::min((n + 255) / 256, 1024)), dim3(256), 0, st, out, (long)n);
-  const unsigned gr = (unsigned)(nv * nstrips);
+  const unsigned gr = (unsigned)(nv * nstrips * 2);
   if (bd == 8)
     hipLaunchKernelGGL((k_sgr_strip<uint8_t>), dim3(gr), dim3(256), 0, st, (const uint8_t*)dgd, (const uint8_t*)src,
                        pw, ph, unit_size, ss_y, nh, nstrips, out);
