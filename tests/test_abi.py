@@ -189,3 +189,25 @@ def test_grid_and_event_entry_points_validate_without_gpu(zav1):
     assert L.zav1_grid_xy(ctypes.c_void_p(1), 64, 64, 0, 8, None) == zav1.ZAV1_EINVAL
     assert L.zav1_event_query(None) == zav1.ZAV1_EINVAL
     assert L.zav1_stream_wait_event(None, None) == zav1.ZAV1_EINVAL
+
+
+def test_header_is_plain_c_and_layouts_match(tmp_path, zav1):
+    """include/zav1_dsp.h compiles as C99 on its own (the FFI boundary needs
+    no C++), and the C compiler's sizes of the batch structs equal the ones
+    the Python mirror and the kernels use."""
+    import shutil
+    import subprocess
+    cc = shutil.which('gcc') or shutil.which('cc')
+    if cc is None:
+        pytest.skip('no C compiler')
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / 'probe.c'
+    src.write_text('#include <stdio.h>\n#include "zav1_dsp.h"\nint main(void) {\n'
+                   '  printf("%zu %zu %zu %zu %zu\\n", sizeof(zav1_rdoq_job), sizeof(zav1_rate_job),\n'
+                   '         sizeof(zav1_txb_costs), sizeof(zav1_eob_costs), sizeof(zav1_coeff_costs));\n'
+                   '  return 0;\n}\n')
+    exe = tmp_path / 'probe'
+    subprocess.run([cc, '-std=c99', '-Wall', '-Werror', '-I', os.path.join(root, 'include'), str(src), '-o', str(exe)],
+                   check=True, capture_output=True)
+    sizes = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert sizes == [zav1.RDOQ_JOB.itemsize, zav1.RATE_JOB.itemsize, 944 * 4, 22 * 4, zav1.COEFF_COSTS_WORDS * 4]
