@@ -273,8 +273,9 @@ typedef struct {
  * 0 / 0 for the full trellis.  A job with a field out of range is left as it
  * is.  d_dqcoeff must hold the fp quantizer's dequantized values,
  * sign((|q| * dequant) >> log_scale), which the trellis recomputes from the
- * levels; the call takes a temporary of n_jobs * 8 bytes per coefficient
- * from the stream-ordered allocator. */
+ * levels; the call takes a temporary of n_jobs * 8.5 bytes per coefficient
+ * from a library-owned stream-ordered pool (one per device, keeping up to
+ * 1 GiB between calls until the process ends). */
 int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, int32_t *d_qcoeff,
                     int32_t *d_dqcoeff, uint16_t *d_eob, const zav1_rdoq_job *d_jobs, int n_jobs,
                     const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,

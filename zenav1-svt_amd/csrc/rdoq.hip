@@ -16,6 +16,7 @@
 // the launch's size (both plane types) and its three scans are staged in
 // LDS once per workgroup.
 #include <algorithm>
+#include <mutex>
 
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
@@ -168,26 +169,30 @@ struct Trellis {
     return c;
   }
   // quant.rs:822 get_two_coeff_cost_simple (+ :736 br_cost_with_diff with
-  // the two Golomb tables in closed form)
-  __device__ int cost_simple(int ci, int a, int ctx, int* cost_low) const {
+  // the two Golomb tables in closed form), in two halves: the level's cost
+  // (b: its base-range context, for the second half), then the cost one
+  // level lower, which only a coefficient that may be lowered needs
+  __device__ int cost_simple(int ci, int a, int ctx, int* b) const {
     int c = base_cost(ctx, min(a, 3));
-    int diff = a <= 3 ? base_cost(ctx, a + 4) : 0;
     if (a) {
       c += 512;
       if (a > 2) {
-        const int b = br_ctx(ci), br = min(a - 3, 12);
-        int gb = 0;
-        if (a <= 15) diff += lps(b, br + 13);
-        if (a >= 15) {
-          const int r = a - 14;
-          gb = 512 * (2 * (msb((uint32_t)r) + 1) - 1);
-          diff += r == 1 ? 512 : ((r & (r - 1)) == 0 ? 1024 : 0);
-        }
-        c += lps(b, br) + gb;
+        *b = br_ctx(ci);
+        c += lps(*b, min(a - 3, 12)) + golomb_cost(a);
       }
     }
-    *cost_low = c - diff;
     return c;
+  }
+  __device__ int cost_simple_low(int a, int ctx, int b, int c) const {
+    int diff = a <= 3 ? base_cost(ctx, a + 4) : 0;
+    if (a > 2) {
+      if (a <= 15) diff += lps(b, min(a - 3, 12) + 13);
+      if (a >= 15) {
+        const int r = a - 14;
+        diff += r == 1 ? 512 : ((r & (r - 1)) == 0 ? 1024 : 0);
+      }
+    }
+    return c - diff;
   }
   // quant.rs:851 get_eob_cost (coeff_c.rs:606 eob_pos_token)
   __device__ int eob_cost(int eob) const {
@@ -274,12 +279,13 @@ struct Trellis {
     }
     const int a = qv < 0 ? -qv : qv;
     const int32_t at = abs(t), ad = abs(d);
-    int rate_low = 0;
-    const int rate = cost_simple(ci, a, ctx, &rate_low);
+    int b = 0;
+    const int rate = cost_simple(ci, a, ctx, &b);
     if (ad < at) {
       *accu_rate += rate;
       return;
     }
+    const int rate_low = cost_simple_low(a, ctx, b, rate);
     const int64_t rd = rdcost(rdmult, rate, coeff_dist(at, ad, SHIFT));
     const int32_t ad_low = (int32_t)(((int64_t)(a - 1) * dv) >> SHIFT);
     if (rdcost(rdmult, rate_low, coeff_dist(at, ad_low, SHIFT)) < rd) {
@@ -640,6 +646,33 @@ __global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__
   }
 }
 
+// The library's stream-ordered pool for the trellis's temporaries, one per
+// device, keeping up to 1 GiB between calls (an encoder calls the trellis
+// wave after wave; without it every call would map its temporary afresh).
+// Null if the runtime refuses a pool: the default pool serves then.
+static hipMemPool_t rdoq_pool() {
+  static std::mutex m;
+  static hipMemPool_t pools[64] = {};
+  static bool tried[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> g(m);
+  if (!tried[dev]) {
+    tried[dev] = true;
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    hipMemPool_t pool = nullptr;
+    if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
+      uint64_t keep = 1ull << 30;
+      hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+      pools[dev] = pool;
+    }
+  }
+  return pools[dev];
+}
+
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
@@ -647,7 +680,9 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
   const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
   const int nw = (n + 7) / 8;
   int2* work = nullptr;  // scan-order pairs, then the packed level maps
-  hipError_t e = hipMallocAsync((void**)&work, (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t)), st);
+  const size_t bytes = (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t));
+  hipMemPool_t pool = rdoq_pool();
+  hipError_t e = pool ? hipMallocFromPoolAsync((void**)&work, bytes, pool, st) : hipMallocAsync((void**)&work, bytes, st);
   if (e != hipSuccess) return e;
   uint32_t* lvwork = reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n);
   const dim3 grid((unsigned)((n_jobs + 63) / 64));
