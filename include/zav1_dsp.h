@@ -308,6 +308,21 @@ int zav1_coeff_rate_batch(int tx_size, const int32_t *d_qcoeff, const uint16_t *
  * the scan length or a negative status. */
 int zav1_scan_order(int tx_size, int scan_class, uint16_t *scan);
 
+/* The frame's coefficient rate tables from the AV1 default CDFs of
+ * base_qindex's q context.  Replaces build_coeff_cost_tables
+ * (rust/crates/svtav1-encoder/src/quant.rs:514, over
+ * rust/crates/svtav1-entropy/src/coeff_c.rs:651 default_for_qindex).  Host
+ * only. */
+int zav1_build_coeff_costs(int base_qindex, zav1_coeff_costs *out);
+
+/* Rate (1/512 bit) of signalling tx_type for an intra luma transform of
+ * tx_size whose (filter-intra mapped) prediction direction is intra_dir
+ * (0..12), from the default intra ext-tx CDFs; 0 where the size's intra set
+ * holds one type or tx_type is not in it.  Replaces MdRates::txt_rate
+ * (rust/crates/svtav1-encoder/src/leaf_funnel/rate_tables.rs:291, intra).
+ * Host only; negative on a bad argument. */
+int zav1_intra_txt_rate(int tx_size, int intra_dir, int tx_type);
+
 /* Fused transform-unit evaluation of the tx-type search (the per-candidate
  * body of tx_unit_inner, rust/crates/svtav1-encoder/src/leaf_funnel/
  * tx_pipeline.rs:268, without RDOQ / rate): for every block (d_xy[2j],
@@ -366,6 +381,52 @@ int zav1_txunit_batch(const void *d_src, int stride, int plane_w, int plane_h, i
                       const uint16_t *d_pred, size_t pred_len, const zav1_txunit_job *d_jobs, int n_jobs, int qindex,
                       unsigned flags, uint64_t *d_dist, int32_t *d_eob, uint8_t *d_cul, int32_t *d_qcoeff,
                       uint16_t *d_recon, zav1_stream_t s);
+
+/* The full-RD transform unit of the mode decision's full loop: the body of
+ * tx_unit_inner with RDOQ and the exact coefficient rate
+ * (rust/crates/svtav1-encoder/src/leaf_funnel/tx_pipeline.rs:268-640,
+ * RateMode::Exact; tx_unit_hbd :854 at 10 bits), for n_jobs units of one
+ * tx_size.  Per job: residual -> forward 2D -> 64-point fold -> quantize_b
+ * (ZAV1_TXU_QUANT_B) or quantize_fp, then with ZAV1_TXU_RDOQ the trellis
+ * (optimize_b, rust/crates/svtav1-encoder/src/quant.rs:1312, cut-offs
+ * rdoq_cutoffs :1551) -> eob -> coefficient rate cost_coeffs_txb
+ * (leaf_funnel/coeff_rate.rs:60) including the job's tx-type rate, or the
+ * skip cost txb_skip_cost[ctx][1] when the unit ends with eob 0 -> cul level
+ * -> distortion: frequency-domain (sse of the final levels + folded energy,
+ * >> (1 - log_scale) * 2) or, with ZAV1_TXU_SPATIAL, SSE << 4 of the
+ * reconstruction (inverse of the final levels + prediction) over the crop.
+ * d_costs: the frame's tables (zav1_build_coeff_costs).  d_work: a device
+ * workspace of at least zav1_txunit_rd_workspace(tx_size, n_jobs) bytes,
+ * the caller's (the call allocates nothing); calls sharing a workspace must
+ * be ordered on one stream.  Outputs per job: d_dist, d_eob (-1 no kernel
+ * for the type, -22 unit out of bounds: then rate 0), d_rate (1/512 bit),
+ * d_cul; optional d_qcoeff (final packed levels) and d_recon (w*h u16). */
+#define ZAV1_TXU_RDOQ 8u
+typedef struct zav1_txunit_rd_job {
+  int32_t x, y;           /* unit origin in d_src (samples) */
+  uint32_t pred_off;      /* d_pred offset (samples) of the w*h prediction */
+  uint8_t tx_type;        /* 0..15 */
+  uint8_t crop_w, crop_h; /* spatial-distortion extent, <= w, h */
+  uint8_t plane_type;     /* 0 luma, 1 chroma: selects the cost tables */
+  uint8_t txb_skip_ctx;   /* < 13 */
+  uint8_t dc_sign_ctx;    /* < 3 */
+  uint8_t sharpness;      /* sharpness_flag of the trellis */
+  uint8_t pad;
+  int32_t txt_rate;       /* tx-type rate added to the coefficient rate (luma) */
+  int64_t rdmult;         /* trellis RD multiplier (rdoq_rdmult_full, quant.rs:923) */
+} zav1_txunit_rd_job;
+int zav1_txunit_rd_workspace(int tx_size, int n_jobs, size_t *bytes);
+int zav1_txunit_rd_batch(const void *d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                         const uint16_t *d_pred, size_t pred_len, const zav1_txunit_rd_job *d_jobs, int n_jobs,
+                         int qindex, unsigned flags, uint32_t cut_off_num, uint32_t cut_off_denum,
+                         const zav1_coeff_costs *d_costs, void *d_work, size_t work_bytes, uint64_t *d_dist,
+                         int32_t *d_eob, int32_t *d_rate, uint8_t *d_cul, int32_t *d_qcoeff, uint16_t *d_recon,
+                         zav1_stream_t s);
+
+/* Returns the memory zav1_rdoq_batch's per-device pool keeps between calls
+ * (up to 1 GiB) to the device.  No counterpart in the reference: the
+ * trellis's temporaries are this library's. */
+int zav1_rdoq_pool_release(void);
 
 /* ------------------------------------------------------------------------ */
 /* Mode-decision funnel, MDS0 (batched).                                     */

@@ -488,3 +488,36 @@ def cost_coeffs_txb(qcoeff, eob, scan_, txb_costs, eob_costs, tx_size, tx_class,
     ec = np.ascontiguousarray(eob_costs, dtype=np.int32)
     return L.or_cost_coeffs_txb(q.ctypes.data, int(eob), sc.ctypes.data, tc.ctypes.data, ec.ctypes.data, tx_size,
                                 tx_class, plane_type, txb_skip_ctx, dc_sign_ctx, txt_rate)
+
+
+TXB_WORDS, EOB_WORDS = 944, 22
+
+
+def build_coeff_costs(base_qindex):
+    """enc/quant.rs:514 build_coeff_cost_tables(base_qindex) from the default
+    CDFs: (txb int32[10, 944] indexed txs_ctx*2 + plane, eob int32[7, 2, 22])."""
+    L = lib()
+    L.or_build_coeff_costs.argtypes = [ctypes.c_int, _vp, _vp]
+    txb = np.zeros((10, TXB_WORDS), np.int32)
+    eob = np.zeros((7, 2, EOB_WORDS), np.int32)
+    L.or_build_coeff_costs(int(base_qindex), txb.ctypes.data, eob.ctypes.data)
+    return txb, eob
+
+
+def cost_symbol(p15):
+    L = lib()
+    L.or_cost_symbol.argtypes = [ctypes.c_uint]
+    return L.or_cost_symbol(int(p15))
+
+
+def intra_txt_rate(tx_size, intra_dir, tx_type):
+    """leaf_funnel/rate_tables.rs:291 txt_rate (intra) from the default CDFs."""
+    L = lib()
+    L.or_intra_txt_rate.argtypes = [ctypes.c_int] * 3
+    return L.or_intra_txt_rate(tx_size, intra_dir, tx_type)
+
+
+def intra_tx_type_used(tx_size, tx_type):
+    L = lib()
+    L.or_intra_tx_type_used.argtypes = [ctypes.c_int] * 2
+    return bool(L.or_intra_tx_type_used(tx_size, tx_type))

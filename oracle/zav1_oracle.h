@@ -245,6 +245,17 @@ int or_cost_coeffs_txb(const int32_t *qcoeff, int eob, const uint16_t *scan, con
                        const or_eob_costs *e, int tx_size, int tx_class, int plane_type, int txb_skip_ctx,
                        int dc_sign_ctx, int txt_rate);
 
+/* Rate tables from the AV1 default CDFs (rate_tables.c):
+ * enc/quant.rs:514 build_coeff_cost_tables(base_qindex) into txb[txs_ctx*2 +
+ * plane] / eob[eob_multi_size][plane]; context.rs:388 av1_cost_symbol;
+ * leaf_funnel/rate_tables.rs:291 txt_rate for an intra tx type. */
+int or_coef_q_ctx(int base_qindex);
+int or_cost_symbol(unsigned p15);
+void or_build_coeff_costs(int base_qindex, or_txb_costs txb[10], or_eob_costs eob[7][2]);
+int or_intra_ext_tx_set_type(int tx_size);
+int or_intra_tx_type_used(int tx_size, int tx_type);
+int or_intra_txt_rate(int tx_size, int intra_dir, int tx_type);
+
 #ifdef __cplusplus
 }
 #endif

@@ -211,3 +211,42 @@ def test_header_is_plain_c_and_layouts_match(tmp_path, zav1):
                    check=True, capture_output=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert sizes == [zav1.RDOQ_JOB.itemsize, zav1.RATE_JOB.itemsize, 944 * 4, 22 * 4, zav1.COEFF_COSTS_WORDS * 4]
+
+
+def test_coeff_cost_tables_host_match_oracle(zav1, oracle):
+    """zav1_build_coeff_costs / zav1_intra_txt_rate (host, no device) equal the
+    oracle's restatement for every q context boundary and every
+    (size, direction, type)."""
+    for qi in (0, 20, 21, 60, 61, 120, 121, 200, 255):
+        t, e = oracle.build_coeff_costs(qi)
+        assert np.array_equal(zav1.build_coeff_costs(qi), np.concatenate([t.ravel(), e.ravel()])), qi
+    for tx in range(19):
+        for d in range(13):
+            for tt in range(16):
+                assert zav1.intra_txt_rate(tx, d, tt) == oracle.intra_txt_rate(tx, d, tt), (tx, d, tt)
+    L = zav1.lib()
+    assert L.zav1_build_coeff_costs(256, None) == zav1.ZAV1_EINVAL
+    assert L.zav1_intra_txt_rate(19, 0, 0) == zav1.ZAV1_EINVAL
+    assert L.zav1_intra_txt_rate(0, 13, 0) == zav1.ZAV1_EINVAL
+
+
+def test_txunit_rd_validation_without_gpu(zav1):
+    """The full-RD unit rejects bad arguments before any device work: RDOQ
+    with quantize_b, unknown flags, a missing pointer, a cut-off with a zero
+    denominator, a workspace smaller than its query."""
+    L = zav1.lib()
+    f = ctypes.c_void_p(16)
+    ws = ctypes.c_size_t()
+    assert L.zav1_txunit_rd_workspace(2, 100, ctypes.byref(ws)) == 0 and ws.value > 100 * 256 * 12
+    assert L.zav1_txunit_rd_workspace(19, 1, ctypes.byref(ws)) == zav1.ZAV1_EINVAL
+    args = lambda flags, work, nbytes, cut=(0, 0), costs=f: (f, 64, 64, 64, 8, 2, f, 1 << 20, f, 1, 100, flags,  # noqa
+                                                             cut[0], cut[1], costs, work, nbytes, f, f, f, f, None,
+                                                             None, None)
+    big = 1 << 30
+    assert L.zav1_txunit_rd_batch(*args(1 | 8, f, big)) == zav1.ZAV1_EINVAL
+    assert L.zav1_txunit_rd_batch(*args(16, f, big)) == zav1.ZAV1_EINVAL
+    assert L.zav1_txunit_rd_batch(*args(8, f, big, (60, 0))) == zav1.ZAV1_EINVAL
+    assert L.zav1_txunit_rd_batch(*args(8, f, big, costs=None)) == zav1.ZAV1_EINVAL
+    assert L.zav1_txunit_rd_batch(*args(8, None, big)) == zav1.ZAV1_EINVAL
+    assert L.zav1_txunit_rd_batch(*args(8, f, 16)) == zav1.ZAV1_EINVAL
+    assert zav1.TXUNIT_RD_JOB.itemsize == 32

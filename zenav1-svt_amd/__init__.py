@@ -158,6 +158,13 @@ EXPORTS = {
     'zav1_wiener_stats': [_vp, _vp] + [_i32] * 6 + [_vp, _vp, _vp],
     'zav1_unit_sse': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
     'zav1_sgr_stats': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
+    'zav1_build_coeff_costs': [_i32, _vp],
+    'zav1_intra_txt_rate': [_i32, _i32, _i32],
+    'zav1_txunit_rd_workspace': [_i32, _i32, ctypes.POINTER(ctypes.c_size_t)],
+    'zav1_txunit_rd_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
+                             ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp,
+                             _vp, _vp],
+    'zav1_rdoq_pool_release': [],
 }
 
 # numpy layouts of the job structs (match include/zav1_dsp.h)
@@ -167,7 +174,10 @@ INTRA_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('w', 'u1'), ('h', 'u1'), ('mo
                       ('n_bottomleft_px', '<i2'), ('out_off', '<u4'), ('pad1', '<u4')])
 TXUNIT_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('pred_off', '<u4'), ('tx_type', 'u1'), ('crop_w', 'u1'),
                        ('crop_h', 'u1'), ('pad', 'u1')])
-TXU_QUANT_B, TXU_SPATIAL, TXU_RECON = 1, 2, 4
+TXU_QUANT_B, TXU_SPATIAL, TXU_RECON, TXU_RDOQ = 1, 2, 4, 8
+TXUNIT_RD_JOB = np.dtype([('x', '<i4'), ('y', '<i4'), ('pred_off', '<u4'), ('tx_type', 'u1'), ('crop_w', 'u1'),
+                          ('crop_h', 'u1'), ('plane_type', 'u1'), ('txb_skip_ctx', 'u1'), ('dc_sign_ctx', 'u1'),
+                          ('sharpness', 'u1'), ('pad', 'u1'), ('txt_rate', '<i4'), ('rdmult', '<i8')])
 CFL_JOB = np.dtype([('lx', '<i4'), ('ly', '<i4'), ('cw', '<i4'), ('ch', '<i4'), ('alpha_q3', '<i4'),
                     ('dc_off', '<u4'), ('out_off', '<u4'), ('pad', '<u4')])
 
@@ -410,6 +420,38 @@ def txunit_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, d_jobs, n
     check(lib().zav1_txunit_batch(d_src.ptr, stride, plane_w, plane_h, bd, tx_size, d_pred.ptr,
                                   d_pred.nbytes // 2, d_jobs.ptr, n_jobs, qindex, flags, d_dist.ptr, d_eob.ptr,
                                   d_cul.ptr, ptr(d_qcoeff), ptr(d_recon), stream), 'txunit_batch')
+
+
+def build_coeff_costs(base_qindex):
+    """The frame's coefficient cost tables from the default CDFs (host, no
+    device): int32[COEFF_COSTS_WORDS] in the zav1_coeff_costs layout."""
+    out = np.zeros(COEFF_COSTS_WORDS, np.int32)
+    check(lib().zav1_build_coeff_costs(int(base_qindex), out.ctypes.data), 'build_coeff_costs')
+    return out
+
+
+def intra_txt_rate(tx_size, intra_dir, tx_type):
+    r = lib().zav1_intra_txt_rate(tx_size, intra_dir, tx_type)
+    if r < 0:
+        check(r, 'intra_txt_rate')
+    return r
+
+
+def txunit_rd_workspace(tx_size, n_jobs):
+    b = ctypes.c_size_t()
+    check(lib().zav1_txunit_rd_workspace(tx_size, n_jobs, ctypes.byref(b)), 'txunit_rd_workspace')
+    return b.value
+
+
+def txunit_rd_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, d_jobs, n_jobs, qindex, flags, d_costs,
+                    d_work, d_dist, d_eob, d_rate, d_cul, d_qcoeff=None, d_recon=None, cut_off=(0, 0),
+                    stream=None):
+    """Full-RD transform units (zav1_txunit_rd_batch); d_pred is u16."""
+    ptr = lambda a: a.ptr if a is not None else None  # noqa: E731
+    check(lib().zav1_txunit_rd_batch(d_src.ptr, stride, plane_w, plane_h, bd, tx_size, d_pred.ptr,
+                                     d_pred.nbytes // 2, d_jobs.ptr, n_jobs, qindex, flags, cut_off[0], cut_off[1],
+                                     d_costs.ptr, d_work.ptr, d_work.nbytes, d_dist.ptr, d_eob.ptr, d_rate.ptr,
+                                     d_cul.ptr, ptr(d_qcoeff), ptr(d_recon), stream), 'txunit_rd_batch')
 
 
 def pred_satd_batch(d_src, stride, plane_w, plane_h, bd, w, h, d_pred, d_jobs, n_jobs, d_satd, stream=None):

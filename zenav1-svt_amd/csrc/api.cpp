@@ -36,6 +36,7 @@ hipStream_t S(zav1_stream_t s) { return s ? (hipStream_t)s : hipStreamPerThread;
 struct DeviceState {
   zav1::QuantRow* rows = nullptr;
   uint16_t* iscan = nullptr;  // [19][3][1024] inverse scans
+  uint16_t* fscan = nullptr;  // [19][3 * 1024]: per tx size its three scans, class c at c * n
 };
 std::mutex g_mu;
 std::vector<DeviceState> g_dev;
@@ -103,6 +104,29 @@ int iscan_table(const uint16_t** out) {
     g_dev[dev].iscan = d;
   }
   *out = g_dev[dev].iscan;
+  return ZAV1_OK;
+}
+
+// Forward scans of tx_size on the current device, as the trellis and the
+// coefficient rate read them: the three classes back to back, each of the
+// coded block's length.
+int fscan_table(int tx_size, const uint16_t** out) {
+  int dev = 0;
+  ZCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((int)g_dev.size() <= dev) g_dev.resize(dev + 1);
+  if (!g_dev[dev].fscan) {
+    std::vector<uint16_t> h(19 * 3 * 1024, 0);
+    for (int sz = 0; sz < 19; sz++) {
+      const int n = std::min(zav1::tx_w(sz), 32) * std::min(zav1::tx_h(sz), 32);
+      for (int c = 0; c < 3; c++) zav1::build_scan(sz, c, h.data() + sz * 3072 + c * n);
+    }
+    uint16_t* d = nullptr;
+    ZCHECK(hipMalloc(&d, h.size() * sizeof(uint16_t)));
+    ZCHECK(hipMemcpy(d, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    g_dev[dev].fscan = d;
+  }
+  *out = g_dev[dev].fscan + tx_size * 3072;
   return ZAV1_OK;
 }
 
@@ -925,7 +949,8 @@ int zav1_txsearch_batch_q(const uint16_t* d_plane, int stride, int bd, int tx_si
                           int n_blocks, const uint8_t* types, int n_types, const int* qindices, int n_q,
                           uint64_t* d_dist, int32_t* d_eob, uint32_t* d_lsum, zav1_stream_t s) {
   if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_blocks < 0 || n_types < 1 || n_types > 16 ||
-      !types || !qindices || n_q < 1 || n_q > 8 || stride <= 0)
+      !types || !qindices || n_q < 1 || n_q > 8 || stride <= 0 ||
+      (n_blocks > 0 && (!d_plane || !d_xy || !d_dist || !d_eob || !d_lsum)))
     return ZAV1_EINVAL;
   for (int q = 0; q < n_q; q++)
     if (qindices[q] < 0 || qindices[q] > 255) return ZAV1_EINVAL;
@@ -978,6 +1003,51 @@ int zav1_txunit_batch(const void* d_src, int stride, int plane_w, int plane_h, i
                             reinterpret_cast<const zav1::TxUnitJob*>(d_jobs), n_jobs,
                             rows + (bd == 10 ? 256 : 0) + qindex, iscan, flags, d_dist, d_eob, d_cul, d_qcoeff,
                             d_recon, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_txunit_rd_workspace(int tx_size, int n_jobs, size_t* bytes) {
+  if (tx_size < 0 || tx_size >= 19 || n_jobs < 0 || !bytes) return ZAV1_EINVAL;
+  *bytes = n_jobs ? zav1::txunit_rd_workspace(tx_size, n_jobs) : 0;
+  return ZAV1_OK;
+}
+
+int zav1_txunit_rd_batch(const void* d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                         const uint16_t* d_pred, size_t pred_len, const zav1_txunit_rd_job* d_jobs, int n_jobs,
+                         int qindex, unsigned flags, uint32_t cut_off_num, uint32_t cut_off_denum,
+                         const zav1_coeff_costs* d_costs, void* d_work, size_t work_bytes, uint64_t* d_dist,
+                         int32_t* d_eob, int32_t* d_rate, uint8_t* d_cul, int32_t* d_qcoeff, uint16_t* d_recon,
+                         zav1_stream_t s) {
+  static_assert(sizeof(zav1_txunit_rd_job) == sizeof(zav1::TxUnitRdJob), "rd job layout");
+  if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_jobs < 0 || qindex < 0 || qindex > 255 ||
+      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~15u) ||
+      ((flags & ZAV1_TXU_RDOQ) && (flags & ZAV1_TXU_QUANT_B)) || (cut_off_num != 0 && cut_off_denum == 0) ||
+      (n_jobs && (!d_src || !d_pred || !d_jobs || !d_costs || !d_work || !d_dist || !d_eob || !d_rate || !d_cul)))
+    return ZAV1_EINVAL;
+  if (n_jobs == 0) return ZAV1_OK;
+  if (work_bytes < zav1::txunit_rd_workspace(tx_size, n_jobs)) return ZAV1_EINVAL;
+  const zav1::QuantRow* rows = nullptr;
+  const uint16_t *iscan = nullptr, *fscan = nullptr;
+  int rc = quant_rows(&rows);
+  if (rc) return rc;
+  rc = iscan_table(&iscan);
+  if (rc) return rc;
+  rc = fscan_table(tx_size, &fscan);
+  if (rc) return rc;
+  // the device row for the kernels, its dequantizers by value for the trellis
+  zav1::QuantRow row;
+  build_row(qindex, bd, &row);
+  const zav1::QuantRow* drow = rows + (bd == 10 ? 256 : 0) + qindex;
+  ZCHECK(zav1::txunit_rd_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, (long)pred_len,
+                               reinterpret_cast<const zav1::TxUnitRdJob*>(d_jobs), n_jobs, drow, iscan, fscan,
+                               reinterpret_cast<const int32_t*>(d_costs), flags, cut_off_num, cut_off_denum, d_work,
+                               d_dist, d_eob, d_rate, d_cul, d_qcoeff, d_recon, S(s), row.dequant[0],
+                               row.dequant[1]));
+  return ZAV1_OK;
+}
+
+int zav1_rdoq_pool_release(void) {
+  ZCHECK(zav1::rdoq_pool_trim());
   return ZAV1_OK;
 }
 
@@ -1334,13 +1404,15 @@ int zav1_txsearch_frame(zav1_ctx* c, const uint16_t* d_plane, int w, int h, int 
       c->used = mark;
       return ZAV1_EINVAL;
     }
+    // the intra ext-tx set of the size (coeff_c.rs:181-212): DCT only once a
+    // side reaches 32 (square-up size >= 32x32), five types at 16x16, seven
+    // (with V_DCT / H_DCT) at every other size
     int nt = 0;
-    if (mx == 64) out->types[ts][nt++] = 0;
-    else if (mx == 32) {
-      const uint8_t t32[4] = {0, 9, 10, 11};
-      for (int i = 0; i < 4; i++) out->types[ts][nt++] = t32[i];
-    } else
-      for (int t = 0; t < 16; t++) out->types[ts][nt++] = (uint8_t)t;
+    if (mx >= 32) out->types[ts][nt++] = 0;
+    else {
+      const uint8_t set[7] = {0, 1, 2, 3, 9, 10, 11};
+      for (int i = 0; i < (bw == 16 && bh == 16 ? 5 : 7); i++) out->types[ts][nt++] = set[i];
+    }
     out->n_blocks[ts] = (int)nb;
     out->n_types[ts] = nt;
     if (nb == 0) continue;
@@ -1359,11 +1431,18 @@ int zav1_txsearch_frame(zav1_ctx* c, const uint16_t* d_plane, int w, int h, int 
   for (int ts = 0; ts < 19; ts++) {
     if (out->n_blocks[ts] == 0) continue;
     int rc = zav1_grid_xy((int32_t*)xy[ts], w, h, zav1::tx_w(ts), zav1::tx_h(ts), (zav1_stream_t)c->stream);
-    if (rc < 0) return rc;
-    rc = zav1_txsearch_batch_q(d_plane, w, bd, ts, (const int32_t*)xy[ts], out->n_blocks[ts], out->types[ts],
-                               out->n_types[ts], qindices, n_q, out->dist[ts], out->eob[ts], out->lsum[ts],
-                               (zav1_stream_t)c->stream);
-    if (rc) return rc;
+    if (rc >= 0)
+      rc = zav1_txsearch_batch_q(d_plane, w, bd, ts, (const int32_t*)xy[ts], out->n_blocks[ts], out->types[ts],
+                                 out->n_types[ts], qindices, n_q, out->dist[ts], out->eob[ts], out->lsum[ts],
+                                 (zav1_stream_t)c->stream);
+    if (rc < 0) {
+      // all or nothing: let the enqueued work drain, then release the
+      // reservation and the partly written outputs
+      (void)hipStreamSynchronize(c->stream);
+      c->used = mark;
+      std::memset(out, 0, sizeof(*out));
+      return rc;
+    }
   }
   return ZAV1_OK;
 }

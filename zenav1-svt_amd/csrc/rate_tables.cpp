@@ -1,0 +1,118 @@
+// Host side: coefficient rate tables and the intra tx-type rate from the AV1
+// default CDFs.  No device work.
+//
+//   zav1_build_coeff_costs  rust/crates/svtav1-encoder/src/quant.rs:514
+//                           build_coeff_cost_tables (-> :523 ..._from_fc over
+//                           svtav1-entropy/src/coeff_c.rs:651 default_for_qindex)
+//   zav1_intra_txt_rate     rust/crates/svtav1-encoder/src/leaf_funnel/rate_tables.rs:291
+//                           MdRates::txt_rate, intra arm (ext-tx sets
+//                           svtav1-entropy/src/coeff_c.rs:153-212)
+// Symbol costs follow av1_cost_symbol (svtav1-entropy/src/context.rs:388).
+#include <array>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/zav1_dsp.h"
+#include "av1_coef_cdfs.h"
+
+namespace {
+
+// Cost in 1/512 bit of a symbol of probability p15 / 32768: normalise p15 into
+// [2^14, 2^15), round it to an 8-bit probability, table lookup + whole bits.
+int symbol_cost(uint32_t p15) {
+  p15 = p15 < 1 ? 1 : (p15 > 32767 ? 32767 : p15);
+  const int whole = __builtin_clz(p15) - 17;  // 14 - msb
+  uint32_t p8 = ((p15 << whole) * 256u + 16384u) >> 15;
+  p8 = p8 < 1 ? 1 : (p8 > 255 ? 255 : p8);
+  return AV1_PROB_COST[p8 - 128] + 512 * whole;
+}
+
+// Per-symbol costs of one stored inverse CDF: symbol k's probability is the
+// step between consecutive stored values (floored at 4); the symbol whose
+// stored value is 0 is the last.
+template <size_t K>
+void cdf_costs(int32_t* out, const uint16_t (&icdf)[K], int n_max) {
+  uint32_t prev = 32768;
+  for (int k = 0; k < n_max; k++) {
+    const uint32_t step = prev - icdf[k];
+    out[k] = symbol_cost(step < 4 ? 4 : step);
+    prev = icdf[k];
+    if (icdf[k] == 0) return;
+  }
+}
+
+int q_context(int qindex) { return qindex > 120 ? 3 : qindex > 60 ? 2 : qindex > 20 ? 1 : 0; }
+
+// intra ext-tx set type per tx size (DCT only from 32 points; 16x16 the
+// 5-type set, every other size the 7-type set), the set's tx types and each
+// type's symbol index within its set
+constexpr uint8_t kSqr[19] = {0, 1, 2, 3, 4, 0, 0, 1, 1, 2, 2, 3, 3, 0, 0, 1, 1, 2, 2};
+constexpr uint8_t kSqrUp[19] = {0, 1, 2, 3, 4, 1, 1, 2, 2, 3, 3, 4, 4, 2, 2, 3, 3, 4, 4};
+int intra_set(int tx_size) { return kSqrUp[tx_size] >= 3 ? 0 : (kSqr[tx_size] == 2 ? 2 : 3); }
+// set 2 (DTT4_IDTX): DCT_DCT, ADST_DCT, DCT_ADST, ADST_ADST, IDTX;
+// set 3 (DTT4_IDTX_1DDCT): those and V_DCT, H_DCT
+constexpr int8_t kSym2[16] = {1, 3, 4, 2, -1, -1, -1, -1, -1, 0, -1, -1, -1, -1, -1, -1};
+constexpr int8_t kSym3[16] = {1, 5, 6, 4, -1, -1, -1, -1, -1, 0, 2, 3, -1, -1, -1, -1};
+
+}  // namespace
+
+extern "C" {
+
+int zav1_build_coeff_costs(int base_qindex, zav1_coeff_costs* out) {
+  if (base_qindex < 0 || base_qindex > 255 || !out) return ZAV1_EINVAL;
+  const int q = q_context(base_qindex);
+  std::memset(out, 0, sizeof(*out));
+  for (int p = 0; p < 2; p++)
+    for (int c = 0; c < 2; c++) {
+      cdf_costs(out->eob[0][p].eob_cost[c], AV1_EOB_FLAG_CDF16[q][p][c], 6);
+      cdf_costs(out->eob[1][p].eob_cost[c], AV1_EOB_FLAG_CDF32[q][p][c], 7);
+      cdf_costs(out->eob[2][p].eob_cost[c], AV1_EOB_FLAG_CDF64[q][p][c], 8);
+      cdf_costs(out->eob[3][p].eob_cost[c], AV1_EOB_FLAG_CDF128[q][p][c], 9);
+      cdf_costs(out->eob[4][p].eob_cost[c], AV1_EOB_FLAG_CDF256[q][p][c], 10);
+      cdf_costs(out->eob[5][p].eob_cost[c], AV1_EOB_FLAG_CDF512[q][p][c], 11);
+      cdf_costs(out->eob[6][p].eob_cost[c], AV1_EOB_FLAG_CDF1024[q][p][c], 11);
+    }
+  for (int ts = 0; ts < 5; ts++)
+    for (int p = 0; p < 2; p++) {
+      zav1_txb_costs& t = out->txb[ts * 2 + p];
+      for (int c = 0; c < 13; c++) cdf_costs(t.txb_skip_cost[c], AV1_TXB_SKIP_CDF[q][ts][c], 2);
+      for (int c = 0; c < 4; c++) cdf_costs(t.base_eob_cost[c], AV1_COEFF_BASE_EOB_CDF[q][ts][p][c], 3);
+      for (int c = 0; c < 42; c++) {
+        int32_t* b = t.base_cost[c];
+        cdf_costs(b, AV1_COEFF_BASE_CDF[q][ts][p][c], 4);
+        b[4] = 0;
+        b[5] = b[1] - b[0] + 512;  // level 1 vs 0, with the sign bit
+        b[6] = b[2] - b[1];
+        b[7] = b[3] - b[2];
+      }
+      for (int c = 0; c < 9; c++) cdf_costs(t.eob_extra_cost[c], AV1_EOB_EXTRA_CDF[q][ts][p][c], 2);
+      for (int c = 0; c < 3; c++) cdf_costs(t.dc_sign_cost[c], AV1_DC_SIGN_CDF[q][p][c], 2);
+      for (int c = 0; c < 21; c++) {
+        std::array<int32_t, 4> br{};
+        cdf_costs(br.data(), AV1_COEFF_BR_CDF[q][ts < 3 ? ts : 3][p][c], 4);
+        int32_t* l = t.lps_cost[c];
+        // the base range is coded in up to four br symbols of three
+        // increments each; br[3] continues to the next symbol
+        for (int k = 0; k <= 12; k++) l[k] = (k / 3) * br[3] + (k < 12 ? br[k % 3] : 0);
+        l[13] = l[0];
+        for (int k = 1; k <= 12; k++) l[13 + k] = l[k] - l[k - 1];
+      }
+    }
+  return ZAV1_OK;
+}
+
+int zav1_intra_txt_rate(int tx_size, int intra_dir, int tx_type) {
+  if (tx_size < 0 || tx_size > 18 || intra_dir < 0 || intra_dir > 12 || tx_type < 0 || tx_type > 15)
+    return ZAV1_EINVAL;
+  const int set = intra_set(tx_size);
+  if (set == 0) return 0;
+  const int sym = set == 2 ? kSym2[tx_type] : kSym3[tx_type];
+  if (sym < 0) return 0;
+  // ext_tx_set_index for intra: set 2 -> 2, set 3 -> 1
+  const int eset = set == 2 ? 2 : 1;
+  int32_t cost[16] = {0};
+  cdf_costs(cost, AV1_INTRA_EXT_TX_CDF[eset][kSqr[tx_size]][intra_dir], 16);
+  return cost[sym];
+}
+
+}  // extern "C"

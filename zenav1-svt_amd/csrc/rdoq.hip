@@ -435,6 +435,9 @@ __global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, cons
     int ci;
     int32_t t, qv, d;
     s.fetch(si, ci, t, qv, d);
+    // an eob that is not tight (level 0 at scan[eob - 1]) is not a block the
+    // trellis is defined on: left as it is
+    if (qv == 0) return;
     const int a = qv < 0 ? -qv : qv, sign = qv < 0;
     if (a >= 2) {
       s.update_general(&accu_rate, &accu_dist, si, eob);
@@ -673,17 +676,20 @@ static hipMemPool_t rdoq_pool() {
   return pools[dev];
 }
 
-hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
-                      const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
-  if (n_jobs <= 0) return hipSuccess;
+size_t rdoq_workspace(int tx_size, int n_jobs) {
   const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
   const int nw = (n + 7) / 8;
-  int2* work = nullptr;  // scan-order pairs, then the packed level maps
-  const size_t bytes = (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t));
-  hipMemPool_t pool = rdoq_pool();
-  hipError_t e = pool ? hipMallocFromPoolAsync((void**)&work, bytes, pool, st) : hipMallocAsync((void**)&work, bytes, st);
-  if (e != hipSuccess) return e;
+  return (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t));
+}
+
+// The trellis over a caller's workspace of rdoq_workspace(tx_size, n_jobs)
+// bytes (scan-order pairs, then the packed level maps).
+hipError_t rdoq_batch_ws(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                         const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
+                         int32_t dq1, uint32_t cut_num, uint32_t cut_den, void* ws, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
+  int2* work = static_cast<int2*>(ws);
   uint32_t* lvwork = reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n);
   const dim3 grid((unsigned)((n_jobs + 63) / 64));
   const int bpw = n < 1024 ? 1024 / n : 1;
@@ -700,13 +706,33 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
     ZR(0) ZR(1) ZR(2) ZR(3) ZR(4) ZR(5) ZR(6) ZR(7) ZR(8) ZR(9) ZR(10) ZR(11) ZR(12) ZR(13) ZR(14) ZR(15) ZR(16)
     ZR(17) ZR(18)
     default:
-      hipFreeAsync(work, st);
       return hipErrorInvalidValue;
   }
 #undef ZR
-  e = hipGetLastError();
+  return hipGetLastError();
+}
+
+hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                      const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
+  if (n_jobs <= 0) return hipSuccess;
+  if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
+  void* work = nullptr;
+  const size_t bytes = rdoq_workspace(tx_size, n_jobs);
+  hipMemPool_t pool = rdoq_pool();
+  hipError_t e = pool ? hipMallocFromPoolAsync(&work, bytes, pool, st) : hipMallocAsync(&work, bytes, st);
+  if (e != hipSuccess) return e;
+  e = rdoq_batch_ws(tx_size, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
+                    work, st);
   const hipError_t f = hipFreeAsync(work, st);
   return e != hipSuccess ? e : f;
+}
+
+// Returns the trellis pool's cached memory to the device (memory of calls
+// still in flight stays reserved until they complete).
+hipError_t rdoq_pool_trim() {
+  hipMemPool_t pool = rdoq_pool();
+  return pool ? hipMemPoolTrimTo(pool, 0) : hipSuccess;
 }
 
 }  // namespace zav1

@@ -26,6 +26,56 @@ struct TuGeom {
   static constexpr bool RECT2 = (W == 2 * H) || (H == 2 * W);
 };
 
+// Inverse 2D of the dequantized block in LDS (0 where folded) + prediction,
+// clipped to bd; rec (w*h, may be null) receives the reconstruction; returns
+// this lane's part of the spatial SSE over crop_w x crop_h (0 unless
+// `spatial`).  Lanes t < H run the row pass, t < W the column pass; the caller
+// reduces the SSE over its lane group and synchronised the LDS block.
+template <int W, int H, typename Pix>
+__device__ uint64_t tu_recon(int32_t* blk, int t, bool active, const TypeCfg& cfg, int bd, const uint16_t* pr,
+                             const Pix* sp, int stride, int crop_w, int crop_h, bool spatial, uint16_t* rec_out) {
+  using G = TuGeom<W, H>;
+  constexpr InvShift IS = inv_shift(W, H);
+  uint64_t sdist = 0;
+  const int row_rng = bd + 8, col_rng = (bd + 6) > 16 ? bd + 6 : 16;
+  const int32_t wrap_max = (1 << (7 + bd)) - 1 + (914 << (bd - 7));
+  if (active && t < H) {
+    int32_t v[W];
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+      int32_t x = blk[t * G::LS + c];
+      if constexpr (G::RECT2) x = rshift64r((int64_t)x * 2896, 12);
+      v[c] = zclamp(x, row_rng);
+    }
+    inv1d<W>(v, cfg.row, row_rng);
+#pragma unroll
+    for (int c = 0; c < W; c++) blk[t * G::LS + c] = rshift_round(v[c], IS.r0);
+  }
+  __syncthreads();
+  if (active && t < W) {
+    const int cc = cfg.lr ? (W - 1 - t) : t;
+    int32_t v[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) v[r] = zclamp(blk[r * G::LS + cc], col_rng);
+    inv1d<H>(v, cfg.col, col_rng);
+    const int32_t maxv = (1 << bd) - 1;
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+      int32_t x = rshift_round(v[r], IS.r1);
+      x = x < -wrap_max - 1 ? -wrap_max - 1 : (x > wrap_max ? wrap_max : x);
+      const int rr = cfg.ud ? (H - 1 - r) : r;
+      int32_t rec = (int32_t)pr[rr * W + t] + x;
+      rec = rec < 0 ? 0 : (rec > maxv ? maxv : rec);
+      if (rec_out) rec_out[rr * W + t] = (uint16_t)rec;
+      if (spatial && rr < crop_h && t < crop_w) {
+        const int32_t d = (int32_t)sp[rr * stride + t] - rec;
+        sdist += (uint64_t)(uint32_t)__mul24(d, d);
+      }
+    }
+  }
+  return group_sum_u64<G::T>(sdist);
+}
+
 // A job whose (type, size) has no 1D kernel (ADST on a 32/64-point
 // dimension) gets eob = -1 and dist = 0 (the reference's `false` return); a
 // job whose unit or prediction lies outside the plane / prediction buffer
@@ -38,7 +88,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
                                                 const uint16_t* __restrict__ iscan_all, int tx_size, unsigned flags,
                                                 uint64_t* __restrict__ out_dist, int32_t* __restrict__ out_eob,
                                                 uint8_t* __restrict__ out_cul, int32_t* __restrict__ out_q,
-                                                uint16_t* __restrict__ out_rec) {
+                                                uint16_t* __restrict__ out_rec, const TuKeep keep) {
   using G = TuGeom<W, H>;
   constexpr int CBC = fwd_cb_col(W, H), CBR = fwd_cb_row(W, H);
   constexpr FwdShift FS = fwd_shift(W, H);
@@ -46,7 +96,6 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   // passes whose bound allows it run with 24-bit-operand butterflies
   constexpr long long kBCol = fwd_col_bound(FS.s0);
   constexpr long long kBRow = fwd_row_bound(H, CBC, FS.s1, kBCol);
-  constexpr InvShift IS = inv_shift(W, H);
   constexpr int LSC = tx_log_scale(W, H);
   __shared__ int32_t lds[G::JPW * G::BLK];
 
@@ -80,7 +129,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   // ---- forward row pass + fold + quantize: lane t < H owns row t ----
   // The row's dequantized values replace it in LDS (0 where folded) as the
   // inverse transform's input.
-  uint64_t fdist = 0;
+  uint64_t fdist = 0, fold = 0;
   int32_t eob = 0, lsum = 0, dcq = 0;
   if (active && t < H) {
     int32_t v[W];
@@ -101,7 +150,9 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       // coefficient bound before this scaling), so the product fits 32 bits
       if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
       if (t >= 32 || c >= 32) {
-        fdist += (uint64_t)((int64_t)x * x);  // energy of the folded-away coefficients
+        const uint64_t e2 = (uint64_t)((int64_t)x * x);  // energy of the folded-away coefficients
+        fdist += e2;
+        fold += e2;
         blk[t * G::LS + c] = 0;
         continue;
       }
@@ -120,10 +171,16 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       const int64_t e = (int64_t)x - dq;
       fdist += (uint64_t)e * (uint64_t)e;
       blk[t * G::LS + c] = dq;
-      if (out_q) out_q[(long)job * (G::PW * G::PH) + t * G::PW + c] = qs;
+      const long pi = (long)job * (G::PW * G::PH) + t * G::PW + c;
+      if (out_q) out_q[pi] = qs;
+      if (keep.t) {
+        keep.t[pi] = x;
+        keep.dq[pi] = dq;
+      }
     }
   }
   fdist = group_sum_u64<G::T>(fdist);
+  if constexpr (W == 64 || H == 64) fold = group_sum_u64<G::T>(fold);
   eob = group_max_i32<G::T>(eob);
   lsum = group_sum_i32<G::T>(lsum);
 
@@ -132,43 +189,8 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   const bool spatial = flags & kTxuSpatial;
   if (flags & (kTxuRecon | kTxuSpatial)) {
     __syncthreads();
-    const int row_rng = bd + 8, col_rng = (bd + 6) > 16 ? bd + 6 : 16;
-    const int32_t wrap_max = (1 << (7 + bd)) - 1 + (914 << (bd - 7));
-    if (active && t < H) {
-      int32_t v[W];
-#pragma unroll
-      for (int c = 0; c < W; c++) {
-        int32_t x = blk[t * G::LS + c];
-        if constexpr (G::RECT2) x = rshift64r((int64_t)x * 2896, 12);
-        v[c] = zclamp(x, row_rng);
-      }
-      inv1d<W>(v, cfg.row, row_rng);
-#pragma unroll
-      for (int c = 0; c < W; c++) blk[t * G::LS + c] = rshift_round(v[c], IS.r0);
-    }
-    __syncthreads();
-    if (active && t < W) {
-      const int cc = cfg.lr ? (W - 1 - t) : t;
-      int32_t v[H];
-#pragma unroll
-      for (int r = 0; r < H; r++) v[r] = zclamp(blk[r * G::LS + cc], col_rng);
-      inv1d<H>(v, cfg.col, col_rng);
-      const int32_t maxv = (1 << bd) - 1;
-#pragma unroll
-      for (int r = 0; r < H; r++) {
-        int32_t x = rshift_round(v[r], IS.r1);
-        x = x < -wrap_max - 1 ? -wrap_max - 1 : (x > wrap_max ? wrap_max : x);
-        const int rr = cfg.ud ? (H - 1 - r) : r;
-        int32_t rec = (int32_t)pr[rr * W + t] + x;
-        rec = rec < 0 ? 0 : (rec > maxv ? maxv : rec);
-        if (out_rec) out_rec[(long)job * (W * H) + rr * W + t] = (uint16_t)rec;
-        if (spatial && rr < jb.crop_h && t < jb.crop_w) {
-          const int32_t d = (int32_t)sp[rr * stride + t] - rec;
-          sdist += (uint64_t)(uint32_t)__mul24(d, d);
-        }
-      }
-    }
-    sdist = group_sum_u64<G::T>(sdist);
+    sdist = tu_recon<W, H>(blk, t, active, cfg, bd, pr, sp, stride, jb.crop_w, jb.crop_h, spatial,
+                           out_rec ? out_rec + (long)job * (W * H) : nullptr);
   }
 
   if (job < n_jobs && t == 0) {
@@ -176,11 +198,19 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       out_dist[job] = 0;
       out_eob[job] = !inside ? -22 : -1;
       out_cul[job] = 0;
+      if (keep.t) {
+        keep.eob16[job] = 0;
+        keep.fold[job] = 0;
+      }
       return;
     }
     constexpr int SHF = (1 - LSC) * 2;
     out_dist[job] = spatial ? (sdist << 4) : (SHF < 0 ? (fdist << (-SHF)) : (fdist >> SHF));
     out_eob[job] = eob;
+    if (keep.t) {
+      keep.eob16[job] = (uint16_t)eob;
+      keep.fold[job] = fold;
+    }
     uint32_t cl = (uint32_t)(lsum < 63 ? lsum : 63);
     if (dcq < 0) cl |= 1u << 6;
     else if (dcq > 0) cl += 2u << 6;
@@ -190,30 +220,216 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
 
 template <int S>
 static hipError_t launch_tu(const void* src, int stride, int pw, int ph, int bd, const uint16_t* pred, long pred_len,
-                            const TxUnitJob* jobs,
-                            int n_jobs, const QuantRow* row, const uint16_t* iscan, unsigned flags, uint64_t* dist,
-                            int32_t* eob, uint8_t* cul, int32_t* q, uint16_t* rec, hipStream_t st) {
+                            const TxUnitJob* jobs, int n_jobs, const QuantRow* row, const uint16_t* iscan,
+                            unsigned flags, uint64_t* dist, int32_t* eob, uint8_t* cul, int32_t* q, uint16_t* rec,
+                            hipStream_t st, TuKeep keep) {
   constexpr int W = tx_w(S), H = tx_h(S);
   const int grid = (n_jobs + TuGeom<W, H>::JPW - 1) / TuGeom<W, H>::JPW;
   if (bd == 8)
     hipLaunchKernelGGL((k_txunit<W, H, uint8_t>), dim3(grid), dim3(256), 0, st, (const uint8_t*)src, stride, pw, ph,
-                       bd, pred, pred_len, jobs, n_jobs, row, iscan, S, flags, dist, eob, cul, q, rec);
+                       bd, pred, pred_len, jobs, n_jobs, row, iscan, S, flags, dist, eob, cul, q, rec, keep);
   else
     hipLaunchKernelGGL((k_txunit<W, H, uint16_t>), dim3(grid), dim3(256), 0, st, (const uint16_t*)src, stride, pw,
-                       ph, bd, pred, pred_len, jobs, n_jobs, row, iscan, S, flags, dist, eob, cul, q, rec);
+                       ph, bd, pred, pred_len, jobs, n_jobs, row, iscan, S, flags, dist, eob, cul, q, rec, keep);
   return hipGetLastError();
 }
 
 hipError_t txunit_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
                         const uint16_t* pred, long pred_len, const TxUnitJob* jobs, int n_jobs, const QuantRow* row,
                         const uint16_t* iscan, unsigned flags, uint64_t* dist, int32_t* eob, uint8_t* cul,
-                        int32_t* qcoeff, uint16_t* recon, hipStream_t st) {
+                        int32_t* qcoeff, uint16_t* recon, hipStream_t st, TuKeep keep) {
   if (n_jobs <= 0) return hipSuccess;
   switch (tx_size) {
 #define X(S)                                                                                                    \
   case S:                                                                                                       \
     return launch_tu<S>(src, stride, plane_w, plane_h, bd, pred, pred_len, jobs, n_jobs, row, iscan, flags, dist, \
-                        eob, cul, qcoeff, recon, st);
+                        eob, cul, qcoeff, recon, st, keep);
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Full-RD transform unit (MDS3): forward path, RDOQ, rate, distortion.
+// Semantics: tx_unit_inner with do_rdoq and RateMode::Exact
+// (rust/crates/svtav1-encoder/src/leaf_funnel/tx_pipeline.rs:268-640).
+// ---------------------------------------------------------------------------
+
+// tx class (TX_TYPE_TO_CLASS): V_* types are the vertical class (2), H_* the
+// horizontal one (1), the rest 2D (0)
+__device__ inline uint8_t tx_class_of(uint8_t tt) { return tt >= 10 ? ((tt & 1) ? 1 : 2) : 0; }
+
+__global__ __launch_bounds__(256) void k_rd_prep(const TxUnitRdJob* __restrict__ jobs, int n_jobs, int np,
+                                                 TxUnitJob* __restrict__ tj, RdoqJob* __restrict__ rj,
+                                                 RateJob* __restrict__ cj) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n_jobs) return;
+  const TxUnitRdJob a = jobs[j];
+  TxUnitJob u;
+  u.x = a.x, u.y = a.y, u.pred_off = a.pred_off, u.tx_type = a.tx_type, u.crop_w = a.crop_w, u.crop_h = a.crop_h;
+  u.pad = 0;
+  tj[j] = u;
+  const uint8_t cls = tx_class_of(a.tx_type & 15);
+  RdoqJob r{};
+  r.rdmult = a.rdmult, r.coeff = (uint32_t)j * (uint32_t)np, r.tx_class = cls, r.plane_type = a.plane_type;
+  r.txb_skip_ctx = a.txb_skip_ctx, r.dc_sign_ctx = a.dc_sign_ctx, r.sharpness = a.sharpness;
+  rj[j] = r;
+  RateJob c{};
+  c.coeff = (uint32_t)j * (uint32_t)np, c.txt_rate = a.txt_rate, c.tx_class = cls, c.plane_type = a.plane_type;
+  c.txb_skip_ctx = a.txb_skip_ctx, c.dc_sign_ctx = a.dc_sign_ctx;
+  cj[j] = c;
+}
+
+// Per job after the trellis and the rate: frequency distortion over the
+// final levels (+ the fold's energy), or the spatial one after the inverse;
+// the recon; the cul level byte; the skip cost for blocks that end with
+// eob 0; the packed levels.  A job the forward pass rejected keeps its
+// status (eob -1 / -22) and gets rate 0.
+template <int W, int H, typename Pix>
+__global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src, int stride, int bd,
+                                                     const uint16_t* __restrict__ pred,
+                                                     const TxUnitRdJob* __restrict__ jobs, int n_jobs,
+                                                     const int32_t* __restrict__ tq, const int32_t* __restrict__ tt,
+                                                     const int32_t* __restrict__ tdq,
+                                                     const uint16_t* __restrict__ eob16,
+                                                     const uint64_t* __restrict__ fold,
+                                                     const int32_t* __restrict__ costs, int txs_ctx, unsigned flags,
+                                                     uint64_t* __restrict__ out_dist, int32_t* __restrict__ out_eob,
+                                                     int32_t* __restrict__ out_rate, uint8_t* __restrict__ out_cul,
+                                                     int32_t* __restrict__ out_q, uint16_t* __restrict__ out_rec) {
+  using G = TuGeom<W, H>;
+  constexpr int LSC = tx_log_scale(W, H);
+  constexpr int NP = G::PW * G::PH;
+  __shared__ int32_t lds[G::JPW * G::BLK];
+  const int jl = threadIdx.x / G::T, t = threadIdx.x - jl * G::T;
+  const int job = blockIdx.x * G::JPW + jl;
+  const bool live = job < n_jobs;
+  const bool active = live && out_eob[job] >= 0;
+  const TxUnitRdJob jb = jobs[live ? job : 0];
+  const TypeCfg cfg = type_cfg(jb.tx_type & 15);
+  int32_t* blk = lds + jl * G::BLK;
+  const long base = (long)job * NP;
+  uint64_t fd = 0;
+  int32_t lsum = 0;
+  if (active && t < H) {
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+      int32_t d = 0;
+      if (t < G::PH && c < G::PW) {
+        const long pi = base + t * G::PW + c;
+        d = tdq[pi];
+        const int64_t e = (int64_t)tt[pi] - d;
+        fd += (uint64_t)(e * e);
+        const int32_t q = tq[pi];
+        lsum += q < 0 ? -q : q;
+        if (out_q) out_q[pi] = q;
+      }
+      blk[t * G::LS + c] = d;
+    }
+  }
+  fd = group_sum_u64<G::T>(fd);
+  lsum = group_sum_i32<G::T>(lsum);
+  uint64_t sdist = 0;
+  const bool spatial = flags & kTxuSpatial;
+  if (flags & (kTxuRecon | kTxuSpatial)) {
+    __syncthreads();
+    sdist = tu_recon<W, H>(blk, t, active, cfg, bd, pred + jb.pred_off, src + (long)jb.y * stride + jb.x, stride,
+                           jb.crop_w, jb.crop_h, spatial, (out_rec && active) ? out_rec + (long)job * (W * H) : nullptr);
+  }
+  if (!live || t != 0) return;
+  if (!active) {
+    out_rate[job] = 0;
+    return;
+  }
+  const int eob = eob16[job];
+  constexpr int SHF = (1 - LSC) * 2;
+  const uint64_t f = fd + fold[job];
+  out_dist[job] = spatial ? (sdist << 4) : (SHF < 0 ? (f << (-SHF)) : (f >> SHF));
+  out_eob[job] = eob;
+  // cost_skip_txb: txb_skip_cost[ctx][1] of the unit's (txs_ctx, plane) tables
+  if (eob == 0) out_rate[job] = costs[(txs_ctx * 2 + (jb.plane_type ? 1 : 0)) * 944 + jb.txb_skip_ctx * 2 + 1];
+  uint32_t cl = (uint32_t)(lsum < 63 ? lsum : 63);
+  const int32_t dcq = tq[base];
+  if (dcq < 0) cl |= 1u << 6;
+  else if (dcq > 0) cl += 2u << 6;
+  out_cul[job] = (uint8_t)cl;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+size_t txunit_rd_workspace(int tx_size, int n_jobs) {
+  if (tx_size < 0 || tx_size > 18 || n_jobs <= 0) return 0;
+  const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
+  return 3 * align256(n * np * 4) + align256(n * 2) + align256(n * 8) + align256(n * sizeof(TxUnitJob)) +
+         align256(n * sizeof(RdoqJob)) + align256(n * sizeof(RateJob)) + align256(rdoq_workspace(tx_size, n_jobs));
+}
+
+template <int S>
+static hipError_t launch_finish(const void* src, int stride, int bd, const uint16_t* pred, const TxUnitRdJob* jobs,
+                                int n_jobs, const int32_t* tq, const int32_t* tt, const int32_t* tdq,
+                                const uint16_t* eob16, const uint64_t* fold, const int32_t* costs, int txs_ctx,
+                                unsigned flags, uint64_t* dist, int32_t* eob, int32_t* rate, uint8_t* cul,
+                                int32_t* q, uint16_t* rec, hipStream_t st) {
+  constexpr int W = tx_w(S), H = tx_h(S);
+  const int grid = (n_jobs + TuGeom<W, H>::JPW - 1) / TuGeom<W, H>::JPW;
+  if (bd == 8)
+    hipLaunchKernelGGL((k_unit_finish<W, H, uint8_t>), dim3(grid), dim3(256), 0, st, (const uint8_t*)src, stride, bd,
+                       pred, jobs, n_jobs, tq, tt, tdq, eob16, fold, costs, txs_ctx, flags, dist, eob, rate, cul, q,
+                       rec);
+  else
+    hipLaunchKernelGGL((k_unit_finish<W, H, uint16_t>), dim3(grid), dim3(256), 0, st, (const uint16_t*)src, stride,
+                       bd, pred, jobs, n_jobs, tq, tt, tdq, eob16, fold, costs, txs_ctx, flags, dist, eob, rate, cul,
+                       q, rec);
+  return hipGetLastError();
+}
+
+hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                           const uint16_t* pred, long pred_len, const TxUnitRdJob* jobs, int n_jobs,
+                           const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
+                           unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
+                           int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
+                           hipStream_t st, int32_t dq0, int32_t dq1) {
+  if (n_jobs <= 0) return hipSuccess;
+  if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
+  const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
+  char* w = static_cast<char*>(work);
+  auto carve = [&w](size_t bytes) {
+    char* p = w;
+    w += align256(bytes);
+    return p;
+  };
+  TuKeep keep;
+  int32_t* tq = reinterpret_cast<int32_t*>(carve(n * np * 4));
+  keep.t = reinterpret_cast<int32_t*>(carve(n * np * 4));
+  keep.dq = reinterpret_cast<int32_t*>(carve(n * np * 4));
+  keep.eob16 = reinterpret_cast<uint16_t*>(carve(n * 2));
+  keep.fold = reinterpret_cast<uint64_t*>(carve(n * 8));
+  TxUnitJob* tj = reinterpret_cast<TxUnitJob*>(carve(n * sizeof(TxUnitJob)));
+  RdoqJob* rj = reinterpret_cast<RdoqJob*>(carve(n * sizeof(RdoqJob)));
+  RateJob* cj = reinterpret_cast<RateJob*>(carve(n * sizeof(RateJob)));
+  void* rdoq_ws = carve(rdoq_workspace(tx_size, n_jobs));
+  hipLaunchKernelGGL(k_rd_prep, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, jobs, n_jobs, (int)np, tj,
+                     rj, cj);
+  hipError_t e = txunit_batch(src, stride, plane_w, plane_h, bd, tx_size, pred, pred_len, tj, n_jobs, row, iscan,
+                              flags & kTxuQuantB, dist, eob, cul, tq, nullptr, st, keep);
+  if (e != hipSuccess) return e;
+  if (flags & kTxuRdoq) {
+    e = rdoq_batch_ws(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num,
+                      cut_den, rdoq_ws, st);
+    if (e != hipSuccess) return e;
+  }
+  e = coeff_rate_batch(tx_size, tq, keep.eob16, cj, n_jobs, costs, scans, rate, st);
+  if (e != hipSuccess) return e;
+  // the tx size's entropy context: mean of its square-down and square-up sizes
+  constexpr int kSqr[19] = {0, 1, 2, 3, 4, 0, 0, 1, 1, 2, 2, 3, 3, 0, 0, 1, 1, 2, 2};
+  constexpr int kSqrUp[19] = {0, 1, 2, 3, 4, 1, 1, 2, 2, 3, 3, 4, 4, 2, 2, 3, 3, 4, 4};
+  const int txs_ctx = (kSqr[tx_size] + kSqrUp[tx_size] + 1) >> 1;
+  switch (tx_size) {
+#define X(S)                                                                                                        \
+  case S:                                                                                                           \
+    return launch_finish<S>(src, stride, bd, pred, jobs, n_jobs, tq, keep.t, keep.dq, keep.eob16, keep.fold, costs, \
+                            txs_ctx, flags, dist, eob, rate, cul, qcoeff, recon, st);
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 #undef X
     default: return hipErrorInvalidValue;

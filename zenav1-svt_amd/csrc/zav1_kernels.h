@@ -208,6 +208,11 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st);
+size_t rdoq_workspace(int tx_size, int n_jobs);
+hipError_t rdoq_batch_ws(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                         const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
+                         int32_t dq1, uint32_t cut_num, uint32_t cut_den, void* ws, hipStream_t st);
+hipError_t rdoq_pool_trim();
 hipError_t quantize_batch(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff,
                           const QuantRow* rows, const uint8_t* qidx, int n_per_job, int n_jobs,
                           int log_scale, int bd, int kind, hipStream_t st);
@@ -262,7 +267,26 @@ struct TxUnitJob {
   uint32_t pred_off;
   uint8_t tx_type, crop_w, crop_h, pad;
 };
-enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u };
+enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u, kTxuRdoq = 8u };
+// Intermediates a transform unit keeps for the RDOQ trellis and the
+// finishing pass (all null for the plain fused unit): the packed pre-quant
+// coefficients and dequantized values (min(w,32) x min(h,32) per job), the
+// eob as u16 and the 64-point fold's discarded energy per job.
+struct TuKeep {
+  int32_t* t = nullptr;
+  int32_t* dq = nullptr;
+  uint16_t* eob16 = nullptr;
+  uint64_t* fold = nullptr;
+};
+// One full-RD transform unit (layout of zav1_txunit_rd_job).
+struct TxUnitRdJob {
+  int32_t x, y;
+  uint32_t pred_off;
+  uint8_t tx_type, crop_w, crop_h, plane_type;
+  uint8_t txb_skip_ctx, dc_sign_ctx, sharpness, pad;
+  int32_t txt_rate;
+  int64_t rdmult;
+};
 // A w x h unit job reads plane rows y..y+h-1, columns x..x+w-1 and
 // prediction samples pred_off..pred_off+w*h-1.
 __host__ __device__ inline bool unit_inside(const TxUnitJob& j, int w, int h, int pw, int ph, long pred_len) {
@@ -290,7 +314,20 @@ hipError_t fast_select(const uint32_t* satd, const uint32_t* rate, uint64_t lamb
 hipError_t txunit_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
                         const uint16_t* pred, long pred_len, const TxUnitJob* jobs, int n_jobs, const QuantRow* row,
                         const uint16_t* iscan, unsigned flags, uint64_t* dist, int32_t* eob, uint8_t* cul,
-                        int32_t* qcoeff, uint16_t* recon, hipStream_t st);
+                        int32_t* qcoeff, uint16_t* recon, hipStream_t st, TuKeep keep = TuKeep{});
+// Full-RD transform units (the MDS3 unit): the fused unit's forward path
+// (quantize_b, or quantize_fp + the RDOQ trellis with kTxuRdoq), then the
+// coefficient rate (with the tx-type rate, skip cost at eob 0) and the
+// distortion (frequency, or spatial after inverse + recon).  work holds
+// txunit_rd_workspace(tx_size, n_jobs) bytes; scans are tx_size's three
+// forward scans, costs the frame's coefficient cost tables.
+size_t txunit_rd_workspace(int tx_size, int n_jobs);
+hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                           const uint16_t* pred, long pred_len, const TxUnitRdJob* jobs, int n_jobs,
+                           const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
+                           unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
+                           int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
+                           hipStream_t st, int32_t dq0, int32_t dq1);
 
 // One intra transform unit (layout of zav1_intra_job in include/zav1_dsp.h).
 struct IntraJob {

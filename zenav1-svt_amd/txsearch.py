@@ -2,9 +2,11 @@
 sizes and types) — the caller side of the fused transform-unit kernel.
 
 For a frame, every one of the 19 transform sizes tiles the frame, and every
-block is evaluated with every transform type the encoder may signal for that
-size (AV1: all 16 up to 16 points per side; DCT/IDTX and the 1D-DCT/identity
-mixes when a side is 32; DCT only when a side is 64).  This is the
+block is evaluated with every transform type an intra block may signal at
+that size: the intra ext-tx sets (rust/crates/svtav1-entropy/src/
+coeff_c.rs:181-212) -- DCT_DCT only once a side reaches 32, {DCT, ADST x DCT,
+ADST_ADST, IDTX} at 16x16, those and V_DCT / H_DCT at every other size.  This
+is the
 exhaustive form of the MDS3 tx-type search
 (rust/crates/svtav1-encoder/src/leaf_funnel/mod.rs:7140-7330) over every tx
 size at once; one launch per (size, qindex) covers the whole frame.
@@ -16,12 +18,13 @@ TX_DIMS = [(4, 4), (8, 8), (16, 16), (32, 32), (64, 64), (4, 8), (8, 4), (8, 16)
 
 
 def legal_types(tx_size):
+    """The intra ext-tx set of tx_size, in tx-type order."""
     w, h = TX_DIMS[tx_size]
-    if max(w, h) == 64:
+    if max(w, h) >= 32:
         return [0]
-    if max(w, h) == 32:
-        return [0, 9, 10, 11]  # DCT_DCT, IDTX, V_DCT, H_DCT
-    return list(range(16))
+    if w == 16 and h == 16:
+        return [0, 1, 2, 3, 9]                 # DTT4_IDTX
+    return [0, 1, 2, 3, 9, 10, 11]             # DTT4_IDTX_1DDCT
 
 
 def kernel_form(tx_size):
