@@ -423,6 +423,25 @@ int zav1_txunit_rd_batch(const void *d_src, int stride, int plane_w, int plane_h
                          int32_t *d_eob, int32_t *d_rate, uint8_t *d_cul, int32_t *d_qcoeff, uint16_t *d_recon,
                          zav1_stream_t s);
 
+/* zav1_txunit_rd_batch without a workspace argument and without any device
+ * allocation: no counterpart in the reference.  For callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The temporaries live in the library's fixed
+ * per-GPU scratch of zav1_noalloc_scratch_bytes() (32 MiB, held by each GPU
+ * the library runs on until the process ends); zav1_init(device) must have
+ * run on the device first.  A call whose n_jobs need more than the scratch
+ * (zav1_txunit_rd_workspace) returns ZAV1_EINVAL with the sizes in
+ * zav1_last_error().  Every call on one GPU uses the same scratch: calls
+ * that overlap on one GPU (different streams or host threads without
+ * ordering) race.  Nothing in the scratch needs to start at zero. */
+int zav1_txunit_rd_batch_noalloc(const void *d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                                 const uint16_t *d_pred, size_t pred_len, const zav1_txunit_rd_job *d_jobs,
+                                 int n_jobs, int qindex, unsigned flags, uint32_t cut_off_num,
+                                 uint32_t cut_off_denum, const zav1_coeff_costs *d_costs, uint64_t *d_dist,
+                                 int32_t *d_eob, int32_t *d_rate, uint8_t *d_cul, int32_t *d_qcoeff,
+                                 uint16_t *d_recon, zav1_stream_t s);
+size_t zav1_noalloc_scratch_bytes(void);
+
 /* Returns the memory zav1_rdoq_batch's per-device pool keeps between calls
  * (up to 1 GiB) to the device.  No counterpart in the reference: the
  * trellis's temporaries are this library's. */

@@ -165,6 +165,10 @@ EXPORTS = {
                              ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp,
                              _vp, _vp],
     'zav1_rdoq_pool_release': [],
+    'zav1_txunit_rd_batch_noalloc': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32,
+                                     ctypes.c_uint, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     _vp, _vp],
+    'zav1_noalloc_scratch_bytes': [],
 }
 
 # numpy layouts of the job structs (match include/zav1_dsp.h)
@@ -197,6 +201,7 @@ def lib():
         L.zav1_last_error.restype = ctypes.c_char_p
         L.zav1_dsp_get.restype = ctypes.POINTER(DspV1)
         L.zav1_ctx_stream.restype = _vp
+        L.zav1_noalloc_scratch_bytes.restype = ctypes.c_size_t
         _lib = L
     return _lib
 

@@ -130,6 +130,23 @@ int fscan_table(int tx_size, const uint16_t** out) {
   return ZAV1_OK;
 }
 
+// The current device's constant tables if they were built (zav1_init
+// builds them), without building them: for entry points that must not
+// allocate.
+int tables_ready(const zav1::QuantRow** rows, const uint16_t** iscan, const uint16_t** fscan_base) {
+  int dev = 0;
+  ZCHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((int)g_dev.size() <= dev || !g_dev[dev].rows || !g_dev[dev].iscan || !g_dev[dev].fscan) {
+    g_err = "device not initialised: call zav1_init(device) before an entry point that does not allocate";
+    return ZAV1_EINVAL;
+  }
+  *rows = g_dev[dev].rows;
+  *iscan = g_dev[dev].iscan;
+  *fscan_base = g_dev[dev].fscan;
+  return ZAV1_OK;
+}
+
 // Per-thread device staging for the plain form (grown on demand, kept).
 struct Staging {
   char* d = nullptr;
@@ -743,6 +760,15 @@ int zav1_init(int device) {
   if (device < 0 || device >= n) return ZAV1_EINVAL;
   ZCHECK(hipSetDevice(device));
   ZCHECK(hipFree(nullptr));
+  // the device's constant tables and the address of its fixed scratch, so
+  // that entry points that must not allocate find them ready
+  const zav1::QuantRow* rows = nullptr;
+  const uint16_t* tab = nullptr;
+  int rc = quant_rows(&rows);
+  if (!rc) rc = iscan_table(&tab);
+  if (!rc) rc = fscan_table(0, &tab);
+  if (rc) return rc;
+  ZCHECK(zav1::noalloc_setup(device));
   return ZAV1_OK;
 }
 
@@ -1045,6 +1071,48 @@ int zav1_txunit_rd_batch(const void* d_src, int stride, int plane_w, int plane_h
                                row.dequant[1]));
   return ZAV1_OK;
 }
+
+int zav1_txunit_rd_batch_noalloc(const void* d_src, int stride, int plane_w, int plane_h, int bd, int tx_size,
+                                 const uint16_t* d_pred, size_t pred_len, const zav1_txunit_rd_job* d_jobs,
+                                 int n_jobs, int qindex, unsigned flags, uint32_t cut_off_num,
+                                 uint32_t cut_off_denum, const zav1_coeff_costs* d_costs, uint64_t* d_dist,
+                                 int32_t* d_eob, int32_t* d_rate, uint8_t* d_cul, int32_t* d_qcoeff,
+                                 uint16_t* d_recon, zav1_stream_t s) {
+  if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_jobs < 0 || qindex < 0 || qindex > 255 ||
+      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~15u) ||
+      ((flags & ZAV1_TXU_RDOQ) && (flags & ZAV1_TXU_QUANT_B)) || (cut_off_num != 0 && cut_off_denum == 0) ||
+      (n_jobs && (!d_src || !d_pred || !d_jobs || !d_costs || !d_dist || !d_eob || !d_rate || !d_cul)))
+    return ZAV1_EINVAL;
+  if (n_jobs == 0) return ZAV1_OK;
+  const size_t need = zav1::txunit_rd_workspace(tx_size, n_jobs);
+  if (need > zav1::noalloc_scratch_bytes()) {
+    g_err = "zav1_txunit_rd_batch_noalloc: " + std::to_string(n_jobs) + " jobs need " + std::to_string(need) +
+            " bytes of scratch, more than the fixed " + std::to_string(zav1::noalloc_scratch_bytes());
+    return ZAV1_EINVAL;
+  }
+  int dev = 0;
+  ZCHECK(hipGetDevice(&dev));
+  void* work = zav1::noalloc_scratch(dev);
+  const zav1::QuantRow* rows = nullptr;
+  const uint16_t *iscan = nullptr, *fscan = nullptr;
+  int rc = tables_ready(&rows, &iscan, &fscan);
+  if (rc) return rc;
+  if (!work) {
+    g_err = "device not initialised: call zav1_init(device) before zav1_txunit_rd_batch_noalloc";
+    return ZAV1_EINVAL;
+  }
+  zav1::QuantRow row;
+  build_row(qindex, bd, &row);
+  ZCHECK(zav1::txunit_rd_batch(d_src, stride, plane_w, plane_h, bd, tx_size, d_pred, (long)pred_len,
+                               reinterpret_cast<const zav1::TxUnitRdJob*>(d_jobs), n_jobs,
+                               rows + (bd == 10 ? 256 : 0) + qindex, iscan, fscan + tx_size * 3072,
+                               reinterpret_cast<const int32_t*>(d_costs), flags, cut_off_num, cut_off_denum, work,
+                               d_dist, d_eob, d_rate, d_cul, d_qcoeff, d_recon, S(s), row.dequant[0],
+                               row.dequant[1]));
+  return ZAV1_OK;
+}
+
+size_t zav1_noalloc_scratch_bytes(void) { return zav1::noalloc_scratch_bytes(); }
 
 int zav1_rdoq_pool_release(void) {
   ZCHECK(zav1::rdoq_pool_trim());

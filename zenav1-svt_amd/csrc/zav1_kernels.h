@@ -322,6 +322,11 @@ hipError_t txunit_batch(const void* src, int stride, int plane_w, int plane_h, i
 // txunit_rd_workspace(tx_size, n_jobs) bytes; scans are tx_size's three
 // forward scans, costs the frame's coefficient cost tables.
 size_t txunit_rd_workspace(int tx_size, int n_jobs);
+// The fixed per-device scratch of the no-allocation entry points
+// (noalloc.hip): set up once per device with that device current.
+hipError_t noalloc_setup(int dev);
+void* noalloc_scratch(int dev);
+size_t noalloc_scratch_bytes();
 hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h, int bd, int tx_size,
                            const uint16_t* pred, long pred_len, const TxUnitRdJob* jobs, int n_jobs,
                            const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
