@@ -14,8 +14,10 @@
  * Self-guided statistics have NO counterpart in the reference
  * (enc/restoration.rs:5-9); the restatement below follows the algorithm
  * text of specs/08-loop-filters.md:371-420 and the AV1 specification's
- * self-guided filter process.  PARITY UNPINNED: checked only by the
- * mathematical invariants in tests/, and GPU == this file.
+ * self-guided filter process.  No reference artifact pins it: it is checked
+ * against the specification's parameter sets and lookup tables and a
+ * whole-array numpy brute force of every step
+ * (tests/test_oracle_pin_lr_cdef.py), and GPU == this file.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -110,6 +112,9 @@ static int one_by_x(int n) { return (4096 + n / 2) / n; }
 static int32_t rpot(int64_t v, int n) { return n == 0 ? (int32_t)v : (int32_t)((v + ((int64_t)1 << (n - 1))) >> n); }
 
 int or_sgr_param(int set, int k) { return SGR_PARAMS[set][k]; }
+/* the two lookup tables of the self-guided filter: 0 = x_by_xplus1[z]
+ * (z < 256), 1 = one_by_x[n - 1] (n = 1..25) */
+int or_sgr_table(int which, int i) { return which == 0 ? x_by_xplus1(i) : one_by_x(i + 1); }
 
 /* One self-guided pass of radius r (1 or 2) and scale s over the unit
  * [hs,he) x [vs,ve) of the replicated-border plane; flt is (he-hs) x (ve-vs)

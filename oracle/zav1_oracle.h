@@ -20,8 +20,10 @@
  *   1D transforms       golden vectors extracted from C SVT-AV1 by the
  *                       reference (svtav1/tests/golden_parity.rs:36-720),
  *                       cos/sin table closed form, float-DCT closeness.
- *   2D transforms       structure pinned by the 1D goldens; 2D composition
- *                       pinned by separability/flip invariants.
+ *   2D transforms       structure pinned by the 1D goldens; separability/flip
+ *                       invariants; the 16x16 / 32x32 / 64x64 DCT_DCT (with
+ *                       the 64-fold) end to end by the C encoder's captured
+ *                       full-loop records (tests/test_oracle_captures.py).
  *   quantizers          per-coefficient brute force of the quoted formulas,
  *                       dequant bounds; table rows by closed form and a row
  *                       captured from the reference's tests.
@@ -37,7 +39,9 @@
  *                       smoothed; p/q mirror symmetry of every filter.
  *   CDEF                the reference's constrain / adjust_strength known
  *                       values, find_dir spec semantics, direction table vs
- *                       the spec, flat-block identity, skip-list additivity.
+ *                       the spec, flat-block identity, skip-list additivity;
+ *                       single impulses isolating every primary / secondary
+ *                       tap weight and offset of every direction.
  *   Wiener / unit SSE   numpy covariance of the windowed samples; unit SSE
  *                       and unit limits against the reference's C captures
  *                       (tests/golden/lr).
@@ -46,16 +50,20 @@
  *                       non-decreasing, equal to a stable sort on distinct
  *                       keys, and the stated tie displacement on a worked
  *                       case; rdcost by hand-computed values.
- *   self-guided stats   PARITY UNPINNED: no reference twin; checked only by
- *                       invariants (flat input, projection identities).
- *   RDOQ trellis        the reference's own invariant test (dequant mirror,
- *                       nonzero last coefficient, enc/quant.rs:1852); brute
- *                       force of the single-coefficient block (every keep /
- *                       lower / skip decision priced by hand from the
- *                       tables); level-only lowering, sign and eob
- *                       monotonicity; rdmult = 0 and prohibitive-rate
- *                       limits.  The multi-coefficient context chain is
- *                       pinned only by these invariants (no captures).
+ *   self-guided stats   no reference twin (nothing in the reference pins it):
+ *                       the specification's Sgr_Params and lookup tables,
+ *                       and a whole-array numpy brute force of box sums,
+ *                       A/B, both passes and the projection sums.
+ *   RDOQ trellis /      the reference's own invariant test (dequant mirror,
+ *   coefficient rate    nonzero last coefficient, enc/quant.rs:1852); brute
+ *                       force of the single-coefficient block; and end to
+ *                       end by the C encoder's captured MDS1 / MDS3 records
+ *                       at the frame origin (eob, rate, distortion of
+ *                       32x32 / 64x64 / 16x16 blocks at rdoq levels 0, 2, 3
+ *                       with the default-CDF tables): every record exact.
+ *   rate tables         built from the AV1 default CDFs (fixture from the
+ *                       reference's default_cdfs.rs); pinned by the captures
+ *                       above and av1_cost_symbol's closed form.
  */
 #ifndef ZAV1_ORACLE_H
 #define ZAV1_ORACLE_H
@@ -197,6 +205,7 @@ void or_wiener_stats(int win, const void *dgd, const void *src, int pw, int ph, 
                      int ve, int64_t *M, int64_t *H);
 uint64_t or_sse_region(const void *a, const void *b, int pw, int bd, int hs, int he, int vs, int ve);
 int or_sgr_param(int set, int k);
+int or_sgr_table(int which, int i);
 void or_sgr_unit_stats(const void *dgd, const void *src, int pw, int ph, int bd, int hs, int he, int vs, int ve,
                        int set, int64_t *out);
 
