@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define ZAV1_ABI_VERSION 2u
+#define ZAV1_ABI_VERSION 3u
 
 #define ZAV1_OK 0
 #define ZAV1_EINVAL (-22)
@@ -87,6 +87,11 @@ typedef struct zav1_quant_row {
 /* Replaces build_quant_table_bd (rust/crates/svtav1-encoder/src/quant.rs:180),
  * sharpness 0.  bd in {8, 10}.  [0] = DC, [1] = AC. */
 int zav1_build_quant_row(int qindex, int bd, zav1_quant_row *row);
+/* The same row with the quantizer sharpness adjustment (-7..7; 0 = the row
+ * above).  Replaces build_quant_table_bd_sharp
+ * (rust/crates/svtav1-encoder/src/quant.rs:195, apply_quant_sharpness_factors
+ * :105). */
+int zav1_build_quant_row_sharp(int qindex, int bd, int sharpness, zav1_quant_row *row);
 
 /* ------------------------------------------------------------------------ */
 /* PLAIN form: function table, host pointers, one launch per call.           */
@@ -198,6 +203,26 @@ typedef struct zav1_dsp_v1 {
   int (*compute_stats)(int wiener_win, const void *dgd, size_t dgd_len, size_t dgd_origin, size_t dgd_stride,
                        const void *src, size_t src_len, size_t src_origin, size_t src_stride, int h_start,
                        int h_end, int v_start, int v_end, int64_t *M, int64_t *H, int bd);
+
+  /* ---- ABI 3 (appended) ---- */
+  /* quantize_fp_hbd / quantize_b_hbd (rust/crates/svtav1-encoder/src/
+   * quant.rs:335 / :391): the bd > 8 quantizers on n (<= 1024) coefficients
+   * in raster order with their scan (a permutation of 0..n-1), the row as
+   * zav1_build_quant_row(_sharp) builds it (log_scale applied inside), no
+   * INT16 clamp.  Returns the eob (>= 0) or a negative status. */
+  int (*quantize_fp_hbd)(const int32_t *coeffs, const uint16_t *scan, size_t n, const zav1_quant_row *t,
+                         int32_t log_scale, int32_t *qcoeff, int32_t *dqcoeff);
+  int (*quantize_b_hbd)(const int32_t *coeffs, const uint16_t *scan, size_t n, const zav1_quant_row *t,
+                        int32_t log_scale, int32_t *qcoeff, int32_t *dqcoeff);
+  /* compute_cdef_dist_8bit / _16bit (rust/crates/svtav1-dsp/src/cdef.rs:1192
+   * / :1160): SSE of the filtered blocks `packed` (back to back, bsize 0 4x4,
+   * 1 4x8, 2 8x4, 3 8x8) against the source plane (plane_len samples, block
+   * b at plane_off + (by << log2 bh) * pstride + (bx << log2 bw) for dlist's
+   * (by, bx) byte pairs), rows stepped by subsampling_factor, >>
+   * 2 * coeff_shift, into *out.  Samples are u8 at bd 8, u16 at bd 10. */
+  int (*cdef_dist)(const void *plane, size_t plane_len, size_t plane_off, size_t pstride, const void *packed,
+                   size_t packed_len, const uint8_t *dlist, size_t n_dlist, int bsize, int coeff_shift,
+                   size_t subsampling_factor, int bd, uint64_t *out);
 } zav1_dsp_v1;
 
 /* Returns the table for `backend` (1 = HIP), or NULL if no GPU is usable. */

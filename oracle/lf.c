@@ -328,6 +328,26 @@ void or_cdef_filter_block(int32_t *dst, int dstride, const uint16_t *in, int pri
     }
 }
 
+/* dsp/cdef.rs:1160 compute_cdef_dist_16bit / :1192 _8bit: SSE of the
+ * filtered blocks packed back to back against the source plane, over the
+ * dlist (by, bx) blocks of bsize (0 4x4, 1 4x8, 2 8x4, 3 8x8), rows stepped by
+ * sub, >> 2 * coeff_shift. */
+uint64_t or_cdef_dist(const void *plane, int hbd, long plane_off, int pstride, const void *packed,
+                      const uint8_t *dlist, int n, int bsize, int coeff_shift, int sub) {
+  const int bw = (bsize == 3 || bsize == 2) ? 8 : 4, bh = (bsize == 3 || bsize == 1) ? 8 : 4;
+  uint64_t sum = 0;
+  for (int b = 0; b < n; b++) {
+    const long p0 = plane_off + (long)dlist[2 * b] * bh * pstride + (long)dlist[2 * b + 1] * bw;
+    const long k0 = (long)b * bw * bh;
+    for (int i = 0; i < bh; i += sub)
+      for (int j = 0; j < bw; j++) {
+        const int32_t e = ld(plane, hbd, p0 + (long)i * pstride + j) - ld(packed, hbd, k0 + (long)i * bw + j);
+        sum += (uint64_t)(e * e);
+      }
+  }
+  return sum >> (2 * coeff_shift);
+}
+
 /* enc/cdef.rs:517 build_src: padded fb source, VERY_LARGE outside the plane. */
 static void build_src(uint16_t *src, const void *pre, int hbd, int pw, int ph, int y0, int x0, int vsize, int hsize) {
   for (int r = 0; r < vsize + 2 * VB; r++) {

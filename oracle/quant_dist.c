@@ -26,6 +26,39 @@ static void invert_quant(int32_t d, int32_t *quant, int32_t *shift) {
 
 /* enc/quant.rs:132-236 build_quant_table(_bd) with sharpness 0;
  * qzbin factor per enc/bd10.rs:52 (threshold 148 at bd8, 592 at bd10). */
+/* enc/quant.rs:105 apply_quant_sharpness_factors (base qindex 31, :90):
+ * sharpness s > 0 below the base (s < 0 above it) moves the zbin / rounding
+ * factors by an offset of the qindex distance, clamped to [1, 256]. */
+static void sharp_factors(int *qzbin, int *qround, int qindex, int s) {
+  const int diff = qindex - 31;
+  if (!((s > 0 && diff < 0) || (s < 0 && diff > 0))) return;
+  const int ad = diff < 0 ? -diff : diff;
+  const int off = s > 0 ? (2 * s > ad ? 2 * s : ad) : (-2 * s < diff ? -2 * s : diff);
+  int z = s > 0 ? *qzbin - off : *qzbin + off;
+  int r = s > 0 ? *qround + off : *qround - off;
+  *qzbin = z < 1 ? 1 : (z > 256 ? 256 : z);
+  *qround = r < 1 ? 1 : (r > 256 ? 256 : r);
+}
+
+/* enc/quant.rs:195 build_quant_table_bd_sharp */
+void or_build_quant_row_sharp(int qindex, int bd, int sharpness, or_quant_row *t) {
+  int32_t dc = bd == 10 ? DC_QLOOKUP_10[qindex] : DC_QLOOKUP_8[qindex];
+  int32_t ac = bd == 10 ? AC_QLOOKUP_10[qindex] : AC_QLOOKUP_8[qindex];
+  int th = bd == 10 ? 592 : 148;
+  int qzbin = qindex == 0 ? 64 : (dc < th ? 84 : 80);
+  int qround = qindex == 0 ? 64 : 48;
+  sharp_factors(&qzbin, &qround, qindex, sharpness);
+  int32_t q[2] = {dc, ac};
+  for (int i = 0; i < 2; i++) {
+    invert_quant(q[i], &t->quant[i], &t->quant_shift[i]);
+    t->zbin[i] = (qzbin * q[i] + 64) >> 7;
+    t->round[i] = (qround * q[i]) >> 7;
+    t->quant_fp[i] = (1 << 16) / q[i];
+    t->round_fp[i] = (64 * q[i]) >> 7;
+    t->dequant[i] = q[i];
+  }
+}
+
 void or_build_quant_row(int qindex, int bd, or_quant_row *t) {
   int32_t dc = bd == 10 ? DC_QLOOKUP_10[qindex] : DC_QLOOKUP_8[qindex];
   int32_t ac = bd == 10 ? AC_QLOOKUP_10[qindex] : AC_QLOOKUP_8[qindex];

@@ -102,6 +102,8 @@ typedef struct {
 } or_quant_row;
 /* enc/quant.rs:132-236 build_quant_table(_bd) */
 void or_build_quant_row(int qindex, int bd, or_quant_row *row);
+/* enc/quant.rs:195 build_quant_table_bd_sharp (sharpness -7..7) */
+void or_build_quant_row_sharp(int qindex, int bd, int sharpness, or_quant_row *t);
 /* dsp/quant_coding.rs:80 fp_one / enc/quant.rs:286,335 (hbd drops clamp) */
 void or_quantize_fp(const int32_t *coeff, int n, const or_quant_row *t, int log_scale, int bd,
                     int32_t *qcoeff, int32_t *dqcoeff);
@@ -191,6 +193,9 @@ void or_deblock_search_level(const void *recon, const void *src, int pw, int ph,
 int or_cdef_constrain(int diff, int threshold, int damping);
 int or_cdef_adjust_strength(int strength, int var);
 int or_cdef_find_dir(const uint16_t *img, int stride, int coeff_shift, int *var_out);
+/* dsp/cdef.rs:1160 / :1192 compute_cdef_dist_16bit / _8bit */
+uint64_t or_cdef_dist(const void *plane, int hbd, long plane_off, int pstride, const void *packed,
+                      const uint8_t *dlist, int n, int bsize, int coeff_shift, int sub);
 void or_cdef_filter_block(int32_t *dst, int dstride, const uint16_t *in, int pri, int sec, int dir, int pdamp,
                           int sdamp, int bw, int bh, int coeff_shift, int sub);
 void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, const void *src_y,

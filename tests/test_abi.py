@@ -203,14 +203,16 @@ def test_header_is_plain_c_and_layouts_match(tmp_path, zav1):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     src = tmp_path / 'probe.c'
     src.write_text('#include <stdio.h>\n#include "zav1_dsp.h"\nint main(void) {\n'
-                   '  printf("%zu %zu %zu %zu %zu\\n", sizeof(zav1_rdoq_job), sizeof(zav1_rate_job),\n'
-                   '         sizeof(zav1_txb_costs), sizeof(zav1_eob_costs), sizeof(zav1_coeff_costs));\n'
+                   '  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(zav1_rdoq_job), sizeof(zav1_rate_job),\n'
+                   '         sizeof(zav1_txb_costs), sizeof(zav1_eob_costs), sizeof(zav1_coeff_costs),\n'
+                   '         sizeof(zav1_txunit_rd_job), sizeof(zav1_dsp_v1));\n'
                    '  return 0;\n}\n')
     exe = tmp_path / 'probe'
     subprocess.run([cc, '-std=c99', '-Wall', '-Werror', '-I', os.path.join(root, 'include'), str(src), '-o', str(exe)],
                    check=True, capture_output=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
-    assert sizes == [zav1.RDOQ_JOB.itemsize, zav1.RATE_JOB.itemsize, 944 * 4, 22 * 4, zav1.COEFF_COSTS_WORDS * 4]
+    assert sizes == [zav1.RDOQ_JOB.itemsize, zav1.RATE_JOB.itemsize, 944 * 4, 22 * 4, zav1.COEFF_COSTS_WORDS * 4,
+                     zav1.TXUNIT_RD_JOB.itemsize, ctypes.sizeof(zav1.DspV1)]
 
 
 def test_coeff_cost_tables_host_match_oracle(zav1, oracle):
@@ -250,3 +252,62 @@ def test_txunit_rd_validation_without_gpu(zav1):
     assert L.zav1_txunit_rd_batch(*args(8, None, big)) == zav1.ZAV1_EINVAL
     assert L.zav1_txunit_rd_batch(*args(8, f, 16)) == zav1.ZAV1_EINVAL
     assert zav1.TXUNIT_RD_JOB.itemsize == 32
+
+
+def test_sharp_quant_rows_match_oracle(zav1, oracle):
+    """zav1_build_quant_row_sharp equals the oracle's restatement for every
+    qindex, bd and sharpness; sharpness 0 is the plain row; hand-worked
+    adjustments (base qindex 31): qindex 10, +2 -> offset max(4, 21) = 21,
+    zbin factor 84 - 21, rounding 48 + 21; qindex 200, -3 -> offset
+    min(6, 169) = 6, zbin factor 80 + 6, rounding 48 - 6."""
+    import ctypes as C
+    L = zav1.lib()
+    ol = oracle.lib()
+    ol.or_build_quant_row_sharp.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(oracle.QuantRow)]
+    for bd in (8, 10):
+        for q in range(256):
+            for sh in range(-7, 8):
+                a, b = zav1.QuantRow(), oracle.QuantRow()
+                assert L.zav1_build_quant_row_sharp(q, bd, sh, C.byref(a)) == 0
+                ol.or_build_quant_row_sharp(q, bd, sh, C.byref(b))
+                for f, _ in zav1.QuantRow._fields_:
+                    assert list(getattr(a, f)) == list(getattr(b, f)), (bd, q, sh, f)
+                if sh == 0:
+                    z = zav1.quant_row(q, bd)
+                    assert all(list(getattr(a, f)) == list(getattr(z, f)) for f, _ in zav1.QuantRow._fields_)
+    for q, bd, sh, fz, fr in ((10, 8, 2, 84 - 21, 48 + 21), (200, 8, -3, 80 + 6, 48 - 6), (200, 8, 3, 80, 48),
+                              (10, 8, -3, 84, 48)):
+        a = zav1.QuantRow()
+        L.zav1_build_quant_row_sharp(q, bd, sh, C.byref(a))
+        d = a.dequant[1]
+        assert a.zbin[1] == (fz * d + 64) >> 7 and a.round[1] == (fr * d) >> 7, (q, sh)
+    assert L.zav1_build_quant_row_sharp(10, 8, 8, C.byref(zav1.QuantRow())) == zav1.ZAV1_EINVAL
+
+
+def test_oracle_cdef_dist_closed_form(oracle):
+    """or_cdef_dist (compute_cdef_dist, dsp/cdef.rs:1160/:1192) equals a numpy
+    sum over the listed blocks' subsampled rows, shifted by 2 * coeff_shift."""
+    import ctypes as C
+    L = oracle.lflib()
+    L.or_cdef_dist.argtypes = [C.c_void_p, C.c_int, C.c_long, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                               C.c_int, C.c_int]
+    L.or_cdef_dist.restype = C.c_uint64
+    rng = np.random.default_rng(9)
+    for hbd in (0, 1):
+        dt = np.uint16 if hbd else np.uint8
+        mx = 1023 if hbd else 255
+        plane = rng.integers(0, mx + 1, (40, 48)).astype(dt)
+        for bsize, (bw, bh) in enumerate(((4, 4), (4, 8), (8, 4), (8, 8))):
+            dl = np.array([[0, 0], [1, 2], [3, 1]], np.uint8)
+            packed = rng.integers(0, mx + 1, (3, bh, bw)).astype(dt)
+            for sub in (1, 2):
+                for cs in (0, 2):
+                    off = 2 * 48 + 3
+                    ref = 0
+                    for b, (by, bx) in enumerate(dl):
+                        blk = plane.ravel()[off + int(by) * bh * 48 + int(bx) * bw:][:bh * 48].reshape(bh, 48)[:, :bw]
+                        e = blk.astype(np.int64) - packed[b].astype(np.int64)
+                        ref += int((e[::sub] ** 2).sum())
+                    got = L.or_cdef_dist(plane.ctypes.data, hbd, off, 48, packed.ctypes.data, dl.ctypes.data, 3,
+                                         bsize, cs, sub)
+                    assert got == ref >> (2 * cs)

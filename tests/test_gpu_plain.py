@@ -22,7 +22,7 @@ def _ptr(a, t=ctypes.c_void_p):
 @pytest.fixture(scope='module')
 def tab(gpu):
     t = gpu.dsp_table()
-    assert t.abi_version == 2
+    assert t.abi_version == 3
     return t
 
 
@@ -235,3 +235,58 @@ def test_compute_stats(tab, oracle, bd):
     h = np.zeros(49 * 49, np.int64)
     assert tab.compute_stats(7, _ptr(dgd), dgd.size, 0, pw, _ptr(src), src.size, 0, pw, 0, 10, 1, 15,
                              m.ctypes.data_as(P(ctypes.c_int64)), h.ctypes.data_as(P(ctypes.c_int64)), bd) == -22
+
+
+@pytest.mark.parametrize('kind', ['fp', 'b'])
+def test_quantize_hbd_plain(tab, gpu, oracle, kind):
+    """quantize_fp_hbd / quantize_b_hbd (ABI 3) against the oracle's 10-bit
+    quantizers (no INT16 clamp) and the eob over the scan."""
+    rng = np.random.default_rng(7 if kind == 'fp' else 8)
+    f = tab.quantize_fp_hbd if kind == 'fp' else tab.quantize_b_hbd
+    for tx_size, n, ls in ((0, 16, 0), (2, 256, 0), (3, 1024, 1), (4, 1024, 2), (9, 512, 1)):
+        for sharp in (0, 3, -2):
+            qi = int(rng.integers(1, 256))
+            row = gpu.QuantRow()
+            assert gpu.lib().zav1_build_quant_row_sharp(qi, 10, sharp, ctypes.byref(row)) == 0
+            coef = (rng.normal(0, 1, n) * rng.choice([50, 800, 40000], n)).astype(np.int32)
+            coef[rng.random(n) < 0.3] = 0
+            scan = oracle.scan(tx_size, int(rng.integers(0, 3))).astype(np.uint16)
+            q = np.zeros(n, np.int32)
+            dq = np.zeros(n, np.int32)
+            eob = f(coef.ctypes.data_as(P(ctypes.c_int32)), scan.ctypes.data_as(P(ctypes.c_uint16)), n,
+                    ctypes.byref(row), ls, q.ctypes.data_as(P(ctypes.c_int32)), dq.ctypes.data_as(P(ctypes.c_int32)))
+            orow = oracle.QuantRow()
+            for fld, _ in oracle.QuantRow._fields_:
+                setattr(orow, fld, type(getattr(orow, fld))(*getattr(row, fld)))
+            rq, rdq = oracle.quantize(kind, coef, orow, ls, 10)
+            nz = np.nonzero(rq[scan])[0]
+            assert eob == (int(nz[-1]) + 1 if nz.size else 0), (tx_size, sharp)
+            assert np.array_equal(q, rq) and np.array_equal(dq, rdq), (tx_size, sharp)
+
+
+@pytest.mark.parametrize('bd', [8, 10])
+def test_cdef_dist_plain(tab, oracle, bd):
+    """cdef_dist (ABI 3) against the oracle's compute_cdef_dist."""
+    L = oracle.lflib()
+    L.or_cdef_dist.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_int, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.or_cdef_dist.restype = ctypes.c_uint64
+    rng = np.random.default_rng(bd)
+    dt = np.uint8 if bd == 8 else np.uint16
+    mx = (1 << bd) - 1
+    W = 72
+    plane = rng.integers(0, mx + 1, (64, W)).astype(dt)
+    for bsize, (bw, bh) in enumerate(((4, 4), (4, 8), (8, 4), (8, 8))):
+        nb = int(rng.integers(1, 17))
+        dl = np.stack([rng.integers(0, 48 // bh, nb), rng.integers(0, 64 // bw, nb)], 1).astype(np.uint8)
+        packed = np.clip(rng.integers(-20, 21, (nb, bh, bw)) + mx // 2, 0, mx).astype(dt)
+        for sub in (1, 2):
+            cs = bd - 8
+            out = ctypes.c_uint64()
+            off = 3 * W + 5
+            rc = tab.cdef_dist(_ptr(plane), plane.size, off, W, _ptr(packed), packed.size, _ptr(dl), nb, bsize, cs,
+                               sub, bd, ctypes.byref(out))
+            assert rc == 0
+            ref = L.or_cdef_dist(plane.ctypes.data, int(bd > 8), off, W, packed.ctypes.data, dl.ctypes.data, nb,
+                                 bsize, cs, sub)
+            assert out.value == ref, (bd, bsize, sub)

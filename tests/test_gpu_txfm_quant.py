@@ -137,7 +137,7 @@ def test_plain_table_matches_oracle(gpu, oracle):
     """The per-call function table (host pointers, one launch per call)."""
     zav1 = gpu
     tab = zav1.dsp_table()
-    assert tab.abi_version == 2 and tab.backend == 1
+    assert tab.abi_version == 3 and tab.backend == 1
     rng = np.random.default_rng(11)
     i32 = ctypes.c_int32
     for tx_size in (0, 2, 5, 9, 13, 16, 4):

@@ -89,6 +89,12 @@ class DspV1(ctypes.Structure):
                                                _i32, _sz, _i32)),
         ('compute_stats', ctypes.CFUNCTYPE(_i32, _i32, _vp, _sz, _sz, _sz, _vp, _sz, _sz, _sz, _i32, _i32, _i32,
                                            _i32, _I64P, _I64P, _i32)),
+        # ABI 3
+        ('quantize_fp_hbd', ctypes.CFUNCTYPE(_i32, _I32P, _U16P, _sz, ctypes.POINTER(QuantRow), ctypes.c_int32,
+                                             _I32P, _I32P)),
+        ('quantize_b_hbd', ctypes.CFUNCTYPE(_i32, _I32P, _U16P, _sz, ctypes.POINTER(QuantRow), ctypes.c_int32,
+                                            _I32P, _I32P)),
+        ('cdef_dist', ctypes.CFUNCTYPE(_i32, _vp, _sz, _sz, _sz, _vp, _sz, _vp, _sz, _i32, _i32, _sz, _i32, _U64P)),
     ]
 
 
@@ -117,6 +123,7 @@ EXPORTS = {
     'zav1_grid_xy': [_vp, _i32, _i32, _i32, _i32, _vp],
     'zav1_last_error': [],
     'zav1_build_quant_row': [_i32, _i32, ctypes.POINTER(QuantRow)],
+    'zav1_build_quant_row_sharp': [_i32, _i32, _i32, ctypes.POINTER(QuantRow)],
     'zav1_dsp_get': [_i32],
     'zav1_fwd_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
     'zav1_inv_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _i32, _vp],

@@ -41,14 +41,25 @@ struct DeviceState {
 std::mutex g_mu;
 std::vector<DeviceState> g_dev;
 
-void build_row(int qindex, int bd, zav1::QuantRow* t) {
-  // rust/crates/svtav1-encoder/src/quant.rs:180 build_quant_table_bd_sharp (sharpness 0);
+void build_row_sharp(int qindex, int bd, int sharpness, zav1::QuantRow* t) {
+  // rust/crates/svtav1-encoder/src/quant.rs:195 build_quant_table_bd_sharp;
   // qzbin per rust/crates/svtav1-encoder/src/bd10.rs:52; invert_quant per quant.rs:69.
   const int32_t dc = bd == 10 ? DC_QLOOKUP_10[qindex] : DC_QLOOKUP_8[qindex];
   const int32_t ac = bd == 10 ? AC_QLOOKUP_10[qindex] : AC_QLOOKUP_8[qindex];
   const int th = bd == 10 ? 592 : 148;
-  const int qzbin = qindex == 0 ? 64 : (dc < th ? 84 : 80);
-  const int qround = qindex == 0 ? 64 : 48;
+  int qzbin = qindex == 0 ? 64 : (dc < th ? 84 : 80);
+  int qround = qindex == 0 ? 64 : 48;
+  // sharpness (apply_quant_sharpness_factors, quant.rs:105): a positive
+  // sharpness below qindex 31 narrows the dead zone and raises the rounding,
+  // a negative one above it does the opposite, by at least 2 |sharpness| and
+  // at most the qindex distance
+  const int dist = qindex - 31;
+  if (sharpness != 0 && ((sharpness > 0) == (dist < 0)) && dist != 0) {
+    const int off = sharpness > 0 ? std::max(2 * sharpness, -dist) : std::min(-2 * sharpness, dist);
+    const int dir = sharpness > 0 ? 1 : -1;
+    qzbin = std::clamp(qzbin - dir * off, 1, 256);
+    qround = std::clamp(qround + dir * off, 1, 256);
+  }
   const int32_t q[2] = {dc, ac};
   for (int i = 0; i < 2; i++) {
     uint32_t tt = (uint32_t)q[i];
@@ -64,6 +75,8 @@ void build_row(int qindex, int bd, zav1::QuantRow* t) {
     t->dequant[i] = q[i];
   }
 }
+
+void build_row(int qindex, int bd, zav1::QuantRow* t) { build_row_sharp(qindex, bd, 0, t); }
 
 // Device pointer to the 2 x 256 quantizer-row table of the current device.
 int quant_rows(const zav1::QuantRow** out) {
@@ -277,6 +290,87 @@ int plain_quantize_b_raster(const int32_t* coeffs, int32_t* qcoeff, int32_t* dqc
     p.dequant[i] = dequant[i];
   }
   return plain_quant(coeffs, qcoeff, dqcoeff, n, p);
+}
+
+int plain_quant_hbd(const int32_t* coeffs, const uint16_t* scan, size_t n, const zav1_quant_row* t,
+                    int32_t log_scale, int32_t* qcoeff, int32_t* dqcoeff, int kind) {
+  if (!coeffs || !scan || !t || !qcoeff || !dqcoeff || n == 0 || n > 1024 || log_scale < 0 || log_scale > 2)
+    return ZAV1_EINVAL;
+  for (size_t i = 0; i < n; i++)
+    if (scan[i] >= n) return ZAV1_EINVAL;
+  zav1::QuantRow row;
+  static_assert(sizeof(row) == sizeof(*t), "quant row layout");
+  std::memcpy(&row, t, sizeof(row));
+  for (int i = 0; i < 2; i++)
+    if (row.dequant[i] <= 0) return ZAV1_EINVAL;
+  char* st = nullptr;
+  int rc = staging(sizeof(int32_t) * 3 * n + sizeof(uint16_t) * n + 64, &st);
+  if (rc) return rc;
+  int32_t* dc = (int32_t*)st;
+  int32_t* dq = dc + n;
+  int32_t* ddq = dq + n;
+  int32_t* deob = ddq + n;
+  uint16_t* dscan = (uint16_t*)(deob + 4);
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dc, coeffs, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(dscan, scan, sizeof(uint16_t) * n, hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::quantize_hbd_plain(dc, dscan, (int)n, row, log_scale, kind, dq, ddq, deob, s));
+  int32_t eob = 0;
+  ZCHECK(hipMemcpyAsync(qcoeff, dq, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipMemcpyAsync(dqcoeff, ddq, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+  ZCHECK(hipMemcpyAsync(&eob, deob, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return eob;
+}
+
+int plain_quantize_fp_hbd(const int32_t* coeffs, const uint16_t* scan, size_t n, const zav1_quant_row* t,
+                          int32_t log_scale, int32_t* qcoeff, int32_t* dqcoeff) {
+  return plain_quant_hbd(coeffs, scan, n, t, log_scale, qcoeff, dqcoeff, 0);
+}
+
+int plain_quantize_b_hbd(const int32_t* coeffs, const uint16_t* scan, size_t n, const zav1_quant_row* t,
+                         int32_t log_scale, int32_t* qcoeff, int32_t* dqcoeff) {
+  return plain_quant_hbd(coeffs, scan, n, t, log_scale, qcoeff, dqcoeff, 1);
+}
+
+int plain_cdef_dist(const void* plane, size_t plane_len, size_t plane_off, size_t pstride, const void* packed,
+                    size_t packed_len, const uint8_t* dlist, size_t n_dlist, int bsize, int coeff_shift,
+                    size_t subsampling_factor, int bd, uint64_t* out) {
+  if (!plane || !packed || !out || (n_dlist && !dlist) || bsize < 0 || bsize > 3 || coeff_shift < 0 ||
+      coeff_shift > 4 || (bd != 8 && bd != 10) || n_dlist > 64)
+    return ZAV1_EINVAL;
+  const int bw = (bsize == 3 || bsize == 2) ? 8 : 4, bh = (bsize == 3 || bsize == 1) ? 8 : 4;
+  if (subsampling_factor < 1 || bh % (int)subsampling_factor) return ZAV1_EINVAL;
+  if (packed_len < n_dlist * bw * bh) return ZAV1_EINVAL;
+  const size_t per = (size_t)bw * bh;
+  std::vector<uint16_t> src(n_dlist * per), pk(n_dlist * per);
+  for (size_t b = 0; b < n_dlist; b++) {
+    const size_t p0 = plane_off + (size_t)dlist[2 * b] * bh * pstride + (size_t)dlist[2 * b + 1] * bw;
+    if (p0 + (bh - 1) * pstride + bw > plane_len) return ZAV1_EINVAL;
+    for (int i = 0; i < bh; i++)
+      for (int j = 0; j < bw; j++) {
+        const size_t k = b * per + i * bw + j, pi = p0 + i * pstride + j;
+        src[k] = bd > 8 ? ((const uint16_t*)plane)[pi] : ((const uint8_t*)plane)[pi];
+        pk[k] = bd > 8 ? ((const uint16_t*)packed)[b * per + i * bw + j] : ((const uint8_t*)packed)[b * per + i * bw + j];
+      }
+  }
+  if (n_dlist == 0) {
+    *out = 0;
+    return ZAV1_OK;
+  }
+  char* st = nullptr;
+  int rc = staging(sizeof(uint16_t) * 2 * src.size() + 64, &st);
+  if (rc) return rc;
+  uint64_t* dout = (uint64_t*)st;
+  uint16_t* dsrc = (uint16_t*)(st + 64);
+  uint16_t* dpk = dsrc + src.size();
+  hipStream_t s = hipStreamPerThread;
+  ZCHECK(hipMemcpyAsync(dsrc, src.data(), sizeof(uint16_t) * src.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(hipMemcpyAsync(dpk, pk.data(), sizeof(uint16_t) * pk.size(), hipMemcpyHostToDevice, s));
+  ZCHECK(zav1::cdef_dist_plain(dsrc, dpk, (int)src.size(), bw, (int)subsampling_factor, coeff_shift, dout, s));
+  ZCHECK(hipMemcpyAsync(out, dout, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  ZCHECK(hipStreamSynchronize(s));
+  return ZAV1_OK;
 }
 
 int plain_sse_i32(const int32_t* a, const int32_t* b, size_t n, uint64_t* out) {
@@ -748,6 +842,9 @@ const zav1_dsp_v1 g_table = {
     plain_cdef_find_dir,
     plain_cdef_filter_block,
     plain_compute_stats,
+    plain_quantize_fp_hbd,
+    plain_quantize_b_hbd,
+    plain_cdef_dist,
 };
 
 }  // namespace
@@ -876,6 +973,15 @@ int zav1_stream_wait_event(zav1_stream_t s, void* ev) {
   return ZAV1_OK;
 }
 const char* zav1_last_error(void) { return g_err.c_str(); }
+
+int zav1_build_quant_row_sharp(int qindex, int bd, int sharpness, zav1_quant_row* row) {
+  if (qindex < 0 || qindex > 255 || (bd != 8 && bd != 10) || sharpness < -7 || sharpness > 7 || !row)
+    return ZAV1_EINVAL;
+  zav1::QuantRow r;
+  build_row_sharp(qindex, bd, sharpness, &r);
+  std::memcpy(row, &r, sizeof(r));
+  return ZAV1_OK;
+}
 
 int zav1_build_quant_row(int qindex, int bd, zav1_quant_row* row) {
   if (!row || qindex < 0 || qindex > 255 || (bd != 8 && bd != 10)) return ZAV1_EINVAL;

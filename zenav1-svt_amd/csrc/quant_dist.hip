@@ -102,6 +102,87 @@ hipError_t quantize_plain(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoef
   return hipGetLastError();
 }
 
+// Plain-form high-bit-depth quantizers (quantize_fp_hbd / quantize_b_hbd,
+// rust/crates/svtav1-encoder/src/quant.rs:335 / :391): the quantizer row as
+// built (log_scale applied here), no INT16 clamp, levels written in raster
+// order through the scan, eob = 1 + the last scan index with a level.  One
+// workgroup walks the scan; the eob is a workgroup max.
+__global__ __launch_bounds__(256) void k_quantize_hbd_plain(const int32_t* __restrict__ coeff,
+                                                            const uint16_t* __restrict__ scan, int n, QuantRow t,
+                                                            int ls, int kind, int32_t* __restrict__ qcoeff,
+                                                            int32_t* __restrict__ dqcoeff, int32_t* __restrict__ eob) {
+  __shared__ int32_t best[256];
+  const int half = (1 << ls) >> 1;
+  int32_t last = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const int rc = scan[i], iz = rc != 0;
+    const int32_t c = coeff[rc];
+    const int32_t sgn = c >> 31;
+    const int64_t a = (int64_t)((c ^ sgn) - sgn);
+    int32_t lv = 0;
+    if (kind == 0) {
+      if ((a << (1 + ls)) >= t.dequant[iz])
+        lv = (int32_t)(((a + ((t.round_fp[iz] + half) >> ls)) * t.quant_fp[iz]) >> (16 - ls));
+    } else if (a >= ((t.zbin[iz] + half) >> ls)) {
+      const int64_t x = a + ((t.round[iz] + half) >> ls);
+      lv = (int32_t)(((((x * t.quant[iz]) >> 16) + x) * t.quant_shift[iz]) >> (16 - ls));
+    }
+    int32_t qo = 0, dqo = 0;
+    if (lv != 0) {
+      qo = (lv ^ sgn) - sgn;
+      const int32_t adq = (int32_t)(((int64_t)lv * t.dequant[iz]) >> ls);
+      dqo = (adq ^ sgn) - sgn;
+      last = i + 1;
+    }
+    qcoeff[rc] = qo;
+    dqcoeff[rc] = dqo;
+  }
+  best[threadIdx.x] = last;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) best[threadIdx.x] = max(best[threadIdx.x], best[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *eob = best[0];
+}
+
+hipError_t quantize_hbd_plain(const int32_t* coeff, const uint16_t* scan, int n, const QuantRow& row, int ls,
+                              int kind, int32_t* qcoeff, int32_t* dqcoeff, int32_t* eob, hipStream_t st) {
+  hipLaunchKernelGGL(k_quantize_hbd_plain, dim3(1), dim3(256), 0, st, coeff, scan, n, row, ls, kind, qcoeff, dqcoeff,
+                     eob);
+  return hipGetLastError();
+}
+
+// Plain-form CDEF distortion (compute_cdef_dist_8bit / _16bit,
+// rust/crates/svtav1-dsp/src/cdef.rs:1192 / :1160): the SSE of packed
+// filtered blocks against their source blocks (both gathered tight, bw x bh
+// per block, block-major), rows stepped by sub, >> 2 * coeff_shift.
+__global__ __launch_bounds__(256) void k_cdef_dist_plain(const uint16_t* __restrict__ src,
+                                                         const uint16_t* __restrict__ packed, int n_samples, int bw,
+                                                         int sub, int coeff_shift, uint64_t* __restrict__ out) {
+  __shared__ uint64_t part[256];
+  uint64_t acc = 0;
+  for (int k = threadIdx.x; k < n_samples; k += 256) {
+    const int row = k / bw;  // row index over the stacked blocks; every block starts at a multiple of sub
+    if (row % sub) continue;
+    const int32_t e = (int32_t)src[k] - (int32_t)packed[k];
+    acc += (uint64_t)(uint32_t)(e * e);
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0] >> (2 * coeff_shift);
+}
+
+hipError_t cdef_dist_plain(const uint16_t* src, const uint16_t* packed, int n_samples, int bw, int sub,
+                           int coeff_shift, uint64_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cdef_dist_plain, dim3(1), dim3(256), 0, st, src, packed, n_samples, bw, sub, coeff_shift, out);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // SSE: one wavefront per job, lanes stride the job, DPP/shuffle reduction.
 // ---------------------------------------------------------------------------

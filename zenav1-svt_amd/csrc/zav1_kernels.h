@@ -180,6 +180,10 @@ struct PlainQuant {
 };
 hipError_t quantize_plain(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff, int n,
                           const PlainQuant& p, hipStream_t st);
+hipError_t quantize_hbd_plain(const int32_t* coeff, const uint16_t* scan, int n, const QuantRow& row, int ls,
+                              int kind, int32_t* qcoeff, int32_t* dqcoeff, int32_t* eob, hipStream_t st);
+hipError_t cdef_dist_plain(const uint16_t* src, const uint16_t* packed, int n_samples, int bw, int sub,
+                           int coeff_shift, uint64_t* out, hipStream_t st);
 
 hipError_t fwd_txfm2d_batch(const int32_t* in, int32_t* out, const uint8_t* types, int tx_size,
                             int n_jobs, hipStream_t st);
