@@ -348,6 +348,17 @@ int zav1_build_coeff_costs(int base_qindex, zav1_coeff_costs *out);
  * Host only; negative on a bad argument. */
 int zav1_intra_txt_rate(int tx_size, int intra_dir, int tx_type);
 
+/* Luma rate (1/512 bit) of an intra candidate (mode 0..12, angle delta
+ * -3..3, filter-intra mode 0..4 or 5 = none) of a w x h key-frame block under
+ * the above / left mode contexts (0..4, intra_mode_context of the neighbour
+ * modes, 0 where unavailable), from the default CDFs: the kf y-mode symbol,
+ * the angle delta of a directional mode on blocks of at least 8x8 and, for
+ * DC_PRED up to 32x32, the filter-intra flag and mode.  Replaces the luma
+ * terms of build_md_rates
+ * (rust/crates/svtav1-encoder/src/leaf_funnel/rate_tables.rs:146) as the
+ * fast loop sums them.  Host only; negative on a bad argument. */
+int zav1_intra_mode_rate(int w, int h, int above_ctx, int left_ctx, int mode, int delta, int fi_mode);
+
 /* Fused transform-unit evaluation of the tx-type search (the per-candidate
  * body of tx_unit_inner, rust/crates/svtav1-encoder/src/leaf_funnel/
  * tx_pipeline.rs:268, without RDOQ / rate): for every block (d_xy[2j],
@@ -541,6 +552,79 @@ int zav1_md_fast_satd(const void *d_src, const void *d_recon, int stride, int pl
 int zav1_md_predict_survivors(const void *d_recon, int stride, int plane_w, int plane_h, int bd, int w, int h,
                               const zav1_md_unit *d_units, int n_units, const zav1_md_cand *cands, int n_cands,
                               const uint8_t *d_sel, int k, uint16_t *d_pred, zav1_stream_t s);
+
+/* ------------------------------------------------------------------------ */
+/* Superblock-wavefront mode decision (batched form of the leaf funnel).     */
+/* ------------------------------------------------------------------------ */
+/* Every leaf of a uniform partition (leaf x leaf blocks, transform = block)
+ * of n_images equal-size luma planes, stacked vertically in one device
+ * plane, decided with the reference's funnel (evaluate_leaf,
+ * rust/crates/svtav1-encoder/src/leaf_funnel/mod.rs:1007): MDS0 over the
+ * candidate list predicted from the reconstruction with AV1 neighbour
+ * availability (enc/intra_edge.rs:355 / :436 within the tile), fast cost
+ * with the default-CDF mode rates under the neighbours' mode contexts, the
+ * exchange sort's first k1 into MDS1 (DCT_DCT, quantize_b, frequency
+ * distortion, exact rate), the post-MDS1 exchange sort's first k3 into MDS3
+ * (the intra set's tx types in TX_TYPE_GROUPS order, quantize_fp + RDOQ at
+ * rdoq_level or quantize_b at level 0, spatial distortion; the tx-type rate
+ * gate txt_rate_th, the coefficient-SATD early exit txt_satd_th
+ * (qp-scaled), no signalable eob-0 non-DCT), the least full cost (mode rate
+ * + coefficient rate, distortion) first in order; its reconstruction is
+ * committed before the leaves that read it.  Superblocks run in
+ * anti-diagonal order within each tile (every tile and image of the batch
+ * together), leaves in Z-order within a superblock.  Rate tables are the
+ * frame's default-CDF tables (no per-superblock CDF adaptation); luma only. */
+typedef struct zav1_wf_cfg {
+  int32_t width, height;          /* per image, multiples of leaf */
+  int32_t n_images;               /* images of the batch, stacked in one plane */
+  int32_t bd;                     /* 8 or 10 */
+  int32_t leaf;                   /* 8, 16 or 32 */
+  int32_t sb;                     /* superblock 64 or 128 */
+  int32_t tile_cols, tile_rows;   /* uniform tiles (in superblocks) */
+  int32_t qindex;
+  int32_t k1, k3;                 /* candidates into MDS1 / MDS3 (k3 <= k1 <= n_cands) */
+  int32_t rdoq_level;             /* 0 quantize_b; 1, 2, 3 quantize_fp + trellis (rdoq_cutoffs) */
+  int32_t txt_groups;             /* TX_TYPE_GROUPS searched (1..6) */
+  int32_t txt_rate_th;            /* txt_rate_cost_th */
+  int32_t txt_satd_th;            /* qp-scaled SATD early-exit threshold (0 = off) */
+  uint64_t lambda;                /* full lambda */
+  int32_t n_cands;                /* <= 64 */
+  int32_t pad;
+  zav1_md_cand cands[64];
+} zav1_wf_cfg;
+typedef struct zav1_wf_decision {
+  uint8_t cand, tx_type, mode, fi_mode; /* candidate index, its tx type, mode and filter-intra mode */
+  int32_t eob, rate, mode_rate;         /* coefficient rate incl. tx-type rate; the mode rate */
+  uint64_t dist, cost;                  /* spatial distortion (SSE << 4), full cost */
+} zav1_wf_decision;
+typedef struct zav1_wf_info_t {
+  void *d_src;                    /* stacked source plane (u8 / u16, stride width): the caller fills it */
+  void *d_recon;                  /* stacked reconstruction plane, written by a run */
+  zav1_wf_decision *d_decisions;  /* per leaf, in leaf order */
+  const zav1_md_unit *d_units;    /* per leaf: origin in the stacked plane, availability */
+  int32_t n_leaves, n_steps, max_leaves_per_step, n_types, stacked_height;
+  uint8_t types[16];              /* MDS3 tx types in search order */
+  size_t bytes;                   /* device memory of the object */
+} zav1_wf_info_t;
+typedef struct zav1_wf zav1_wf;
+/* Device bytes zav1_wf_create would take for cfg (admission by footprint). */
+int zav1_wf_footprint(const zav1_wf_cfg *cfg, size_t *bytes);
+/* Plans the steps and takes the device memory (one allocation); zav1_init
+ * must have run on the current device. */
+int zav1_wf_create(const zav1_wf_cfg *cfg, zav1_wf **out);
+int zav1_wf_info(const zav1_wf *wf, zav1_wf_info_t *info);
+/* Decides every leaf: use_graph 0 enqueues the steps' launches on s; 1
+ * captures them once into a HIP graph (s must be a created stream) and
+ * replays it. */
+int zav1_wf_run(zav1_wf *wf, int use_graph, zav1_stream_t s);
+/* The plain per-call form of the same decisions for leaves
+ * [first_leaf, first_leaf + n_leaves) in leaf order (earlier leaves must be
+ * decided): one launch per candidate prediction + SATD, per survivor
+ * prediction and transform unit, per MDS3 tx type, with the selections on
+ * the host from the results read back after each stage; synchronous. */
+int zav1_wf_run_plain(zav1_wf *wf, int first_leaf, int n_leaves, zav1_stream_t s);
+int zav1_wf_destroy(zav1_wf *wf);
+const char *zav1_wf_last_error(void);
 
 /* ------------------------------------------------------------------------ */
 /* Contexts and MD waves: an HBM budget, image upload and a queue of        */

@@ -521,3 +521,21 @@ def intra_tx_type_used(tx_size, tx_type):
     L = lib()
     L.or_intra_tx_type_used.argtypes = [ctypes.c_int] * 2
     return bool(L.or_intra_tx_type_used(tx_size, tx_type))
+
+
+def wf_frame(cfg, images):
+    """or_wf_frame over (n_images, height, width) samples with a zav1_wf_cfg
+    record (numpy, same layout); returns (recon, decisions[img][row][col])."""
+    L = lib()
+    L.or_wf_frame.argtypes = [_vp, _vp, _vp, _vp]
+    c = cfg[0]
+    n, H, W, leaf = int(c['n_images']), int(c['height']), int(c['width']), int(c['leaf'])
+    src = np.ascontiguousarray(images)
+    rec = np.zeros_like(src)
+    dec = np.zeros((n, H // leaf, W // leaf), WF_DECISION)
+    L.or_wf_frame(np.ascontiguousarray(cfg).ctypes.data, src.ctypes.data, rec.ctypes.data, dec.ctypes.data)
+    return rec, dec
+
+
+WF_DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
+                        ('rate', '<i4'), ('mode_rate', '<i4'), ('dist', '<u8'), ('cost', '<u8')])

@@ -134,3 +134,39 @@ int or_intra_txt_rate(int tx_size, int intra_dir, int tx_type) {
   syntax_rate(cost, AV1_INTRA_EXT_TX_CDF[eset][SQR[tx_size]][intra_dir], 17);
   return cost[EXT_TX_IND[set][tx_type]];
 }
+
+/* BLOCK_SIZES index of a w x h block (4x4 .. 128x128, then the 1:4 sizes) */
+int or_bsize_index(int w, int h) {
+  static const int W[22] = {4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128, 128, 4, 16, 8, 32, 16, 64};
+  static const int H[22] = {4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8, 64, 16};
+  for (int i = 0; i < 22; i++)
+    if (W[i] == w && H[i] == h) return i;
+  return -1;
+}
+
+/* Luma rate (1/512 bit) of an intra candidate of a key frame from the
+ * default CDFs (leaf_funnel/rate_tables.rs:146 build_md_rates, as the fast
+ * loop adds it, mod.rs:2060-2185): the kf y-mode symbol under the above /
+ * left mode contexts, the angle delta of a directional mode on blocks of at
+ * least 8x8, and for DC_PRED on blocks up to 32x32 the filter-intra flag
+ * (+ its mode). */
+int or_intra_mode_rate(int w, int h, int actx, int lctx, int mode, int delta, int fi) {
+  int32_t c[16] = {0};
+  syntax_rate(c, AV1_KF_Y_CDF[actx][lctx], 13);
+  int r = c[mode];
+  if (mode >= 1 && mode <= 8 && w >= 8 && h >= 8) {
+    int32_t a[8] = {0};
+    syntax_rate(a, AV1_ANGLE_DELTA_CDF[mode - 1], 7);
+    r += a[delta + 3];
+  }
+  if (mode == 0 && w <= 32 && h <= 32) {
+    int32_t f[2] = {0}, m[5] = {0};
+    syntax_rate(f, AV1_FILTER_INTRA_CDF[or_bsize_index(w, h)], 2);
+    r += f[fi != 5];
+    if (fi != 5) {
+      syntax_rate(m, AV1_FILTER_INTRA_MODE_CDF, 5);
+      r += m[fi];
+    }
+  }
+  return r;
+}

@@ -269,6 +269,31 @@ void or_build_coeff_costs(int base_qindex, or_txb_costs txb[10], or_eob_costs eo
 int or_intra_ext_tx_set_type(int tx_size);
 int or_intra_tx_type_used(int tx_size, int tx_type);
 int or_intra_txt_rate(int tx_size, int intra_dir, int tx_type);
+int or_bsize_index(int w, int h);
+/* key-frame intra luma candidate rate (rate_tables.rs:146 build_md_rates) */
+int or_intra_mode_rate(int w, int h, int actx, int lctx, int mode, int delta, int fi);
+
+/* ---- superblock-wavefront leaf funnel (wavefront.c) --------------------- */
+typedef struct {
+  uint8_t mode;
+  int8_t delta;
+  uint8_t fi_mode, pad;
+} or_wf_cand;
+typedef struct { /* same layout as zav1_wf_cfg */
+  int32_t width, height, n_images, bd, leaf, sb, tile_cols, tile_rows, qindex, k1, k3, rdoq_level, txt_groups,
+      txt_rate_th, txt_satd_th;
+  uint64_t lambda;
+  int32_t n_cands, pad;
+  or_wf_cand cands[64];
+} or_wf_cfg;
+typedef struct {
+  uint8_t cand, tx_type, mode, fi_mode;
+  int32_t eob, rate, mode_rate;
+  uint64_t dist, cost;
+} or_wf_decision;
+/* Every leaf of n_images stacked planes (src, recon: n_images * height rows);
+ * dec[img][leaf row][leaf col]. */
+void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,7 @@
 """Write the AV1 default coefficient CDFs and the tx-type CDFs as a data
-fixture (coef_cdfs.json) and as C headers for the oracle and the library.
+fixture (coef_cdfs.json) and as C headers for the oracle and the library,
+with the key-frame luma mode, angle-delta and filter-intra CDFs the intra
+mode rates come from.
 
 Source (read as text in the build container only):
   rust/crates/svtav1-entropy/src/default_cdfs.rs   (C-layout default CDFs,
@@ -16,6 +18,10 @@ arrays in the reference's C layout:
   COEFF_BASE   [q][txs_ctx 5][plane 2][ctx 42][5]
   COEFF_BR     [q][txs_ctx 4][plane 2][ctx 21][5]
   INTRA_EXT_TX [eset 3][sq_tx 4][intra_dir 13][17]
+  KF_Y         [above ctx 5][left ctx 5][14]      (key-frame luma mode)
+  ANGLE_DELTA  [directional mode 8][8]
+  FILTER_INTRA [block size 22][3]                 (use_filter_intra flag)
+  FILTER_INTRA_MODE [6]
 Every CDF row is an inverse CDF (32768 - cumulative), 0 marking the last
 symbol, the trailing entry the adaptation counter.
 """
@@ -30,7 +36,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 
 TABLES = ['TXB_SKIP_CDF', 'EOB_EXTRA_CDF', 'DC_SIGN_CDF', 'EOB_FLAG_CDF16', 'EOB_FLAG_CDF32', 'EOB_FLAG_CDF64',
           'EOB_FLAG_CDF128', 'EOB_FLAG_CDF256', 'EOB_FLAG_CDF512', 'EOB_FLAG_CDF1024', 'COEFF_BASE_EOB_CDF',
-          'COEFF_BASE_CDF', 'COEFF_BR_CDF', 'INTRA_EXT_TX_CDF']
+          'COEFF_BASE_CDF', 'COEFF_BR_CDF', 'INTRA_EXT_TX_CDF', 'KF_Y_CDF', 'ANGLE_DELTA_CDF', 'FILTER_INTRA_CDF',
+          'FILTER_INTRA_MODE_CDF']
 
 
 def rust_array(text, name):
@@ -58,14 +65,6 @@ def dims(a):
         d.append(len(a))
         a = a[0]
     return d
-
-
-def flat(a):
-    if isinstance(a, list):
-        for x in a:
-            yield from flat(x)
-    else:
-        yield a
 
 
 def nested(a):

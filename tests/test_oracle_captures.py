@@ -214,3 +214,46 @@ def test_intra_txt_rate_sets(oracle):
                 assert used == (t in allowed16) and (r > 0) == used
             else:
                 assert used == (t in allowed8) and (r > 0) == used, (tx, t)
+
+
+def test_intra_mode_rates_match_captures(oracle):
+    """The key-frame candidate rates from the default CDFs equal every
+    origin block's captured MDS0 luma rate `flr` (no coded neighbour: both
+    mode contexts 0), and the reference's pinned entries (leaf_funnel/
+    mod.rs:7924-7960: kf_y DC 547 / SMOOTH 1556, V / H with the zero angle
+    delta 2874 / 2555, filter-intra flag off 281 and on + FILTER_DC 1803 at
+    32x32)."""
+    import ctypes
+    L = oracle.lib()
+    L.or_intra_mode_rate.argtypes = [ctypes.c_int] * 7
+    mds0 = json.load(open(os.path.join(ROOT, 'tests', 'golden', 'funnel_mds0.json')))
+    n = 0
+    for f in mds0['files']:
+        for b in f['blocks']:
+            if b['org'] != [0, 0]:
+                continue
+            s = b['sq']
+            for (buf, mode, fi, dist, flr, fcr, fc) in b['mds0']:
+                assert L.or_intra_mode_rate(s, s, 0, 0, mode, 0, fi) == flr, (f['name'], mode, fi)
+                n += 1
+    assert n >= 20
+    assert L.or_intra_mode_rate(64, 64, 0, 0, 0, 0, 5) == 547
+    assert L.or_intra_mode_rate(64, 64, 0, 0, 9, 0, 5) == 1556
+    assert L.or_intra_mode_rate(32, 32, 0, 0, 0, 0, 5) - 547 == 281
+    assert L.or_intra_mode_rate(32, 32, 0, 0, 0, 0, 0) - 547 == 1803
+
+
+def test_intra_mode_rates_product_matches_oracle(zav1, oracle):
+    import ctypes
+    L = oracle.lib()
+    L.or_intra_mode_rate.argtypes = [ctypes.c_int] * 7
+    P = zav1.lib()
+    for (w, h) in ((4, 4), (8, 8), (16, 16), (32, 32), (64, 64), (8, 32), (16, 4)):
+        for a in range(5):
+            for lc in range(5):
+                for m in range(13):
+                    for d in ((-3, -1, 0, 2, 3) if 1 <= m <= 8 else (0,)):
+                        for fi in ((5, 0, 3) if (m == 0 and max(w, h) <= 32) else (5,)):
+                            assert P.zav1_intra_mode_rate(w, h, a, lc, m, d, fi) == \
+                                L.or_intra_mode_rate(w, h, a, lc, m, d, fi), (w, h, a, lc, m, d, fi)
+    assert P.zav1_intra_mode_rate(64, 64, 0, 0, 0, 0, 0) == zav1.ZAV1_EINVAL

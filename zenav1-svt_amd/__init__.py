@@ -176,6 +176,14 @@ EXPORTS = {
                                      ctypes.c_uint, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp, _vp],
     'zav1_noalloc_scratch_bytes': [],
+    'zav1_intra_mode_rate': [_i32] * 7,
+    'zav1_wf_footprint': [_vp, ctypes.POINTER(ctypes.c_size_t)],
+    'zav1_wf_create': [_vp, ctypes.POINTER(_vp)],
+    'zav1_wf_info': [_vp, _vp],
+    'zav1_wf_run': [_vp, _i32, _vp],
+    'zav1_wf_run_plain': [_vp, _i32, _i32, _vp],
+    'zav1_wf_destroy': [_vp],
+    'zav1_wf_last_error': [],
 }
 
 # numpy layouts of the job structs (match include/zav1_dsp.h)
@@ -209,6 +217,7 @@ def lib():
         L.zav1_dsp_get.restype = ctypes.POINTER(DspV1)
         L.zav1_ctx_stream.restype = _vp
         L.zav1_noalloc_scratch_bytes.restype = ctypes.c_size_t
+        L.zav1_wf_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -280,6 +289,20 @@ class DeviceArray:
 
 def sync(stream=None):
     check(lib().zav1_stream_sync(stream), 'stream_sync')
+
+
+def stream_create():
+    s = ctypes.c_void_p()
+    check(lib().zav1_stream_create(ctypes.byref(s)), 'stream_create')
+    return s
+
+
+def stream_sync(s):
+    check(lib().zav1_stream_sync(s), 'stream_sync')
+
+
+def stream_destroy(s):
+    check(lib().zav1_stream_destroy(s), 'stream_destroy')
 
 
 def quant_row(qindex, bd):

@@ -160,6 +160,20 @@ int tables_ready(const zav1::QuantRow** rows, const uint16_t** iscan, const uint
   return ZAV1_OK;
 }
 
+}  // namespace
+
+namespace zav1 {
+int device_tables(const QuantRow** row, const uint16_t** iscan, const uint16_t** fscan_base, int row_index) {
+  const QuantRow* rows = nullptr;
+  const int rc = tables_ready(&rows, iscan, fscan_base);
+  if (rc) return rc;
+  *row = rows + row_index;
+  return ZAV1_OK;
+}
+}  // namespace zav1
+
+namespace {
+
 // Per-thread device staging for the plain form (grown on demand, kept).
 struct Staging {
   char* d = nullptr;

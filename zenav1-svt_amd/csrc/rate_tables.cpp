@@ -116,3 +116,36 @@ int zav1_intra_txt_rate(int tx_size, int intra_dir, int tx_type) {
 }
 
 }  // extern "C"
+
+extern "C" int zav1_intra_mode_rate(int w, int h, int above_ctx, int left_ctx, int mode, int delta, int fi_mode) {
+  // rust/crates/svtav1-encoder/src/leaf_funnel/rate_tables.rs:146 build_md_rates,
+  // key-frame luma terms as the fast loop sums them (mod.rs:2060-2185)
+  static const int kBw[22] = {4, 4, 8, 8, 8, 16, 16, 16, 32, 32, 32, 64, 64, 64, 128, 128, 4, 16, 8, 32, 16, 64};
+  static const int kBh[22] = {4, 8, 4, 8, 16, 8, 16, 32, 16, 32, 64, 32, 64, 128, 64, 128, 16, 4, 32, 8, 64, 16};
+  int bsize = -1;
+  for (int i = 0; i < 22 && bsize < 0; i++)
+    if (kBw[i] == w && kBh[i] == h) bsize = i;
+  if (bsize < 0 || above_ctx < 0 || above_ctx > 4 || left_ctx < 0 || left_ctx > 4 || mode < 0 || mode > 12 ||
+      delta < -3 || delta > 3 || fi_mode < 0 || fi_mode > 5 || (fi_mode < 5 && (mode != 0 || w > 32 || h > 32)))
+    return ZAV1_EINVAL;
+  int32_t y[16] = {0};
+  cdf_costs(y, AV1_KF_Y_CDF[above_ctx][left_ctx], 13);
+  int rate = y[mode];
+  const bool directional = mode >= 1 && mode <= 8;
+  if (directional && w >= 8 && h >= 8) {
+    int32_t a[8] = {0};
+    cdf_costs(a, AV1_ANGLE_DELTA_CDF[mode - 1], 7);
+    rate += a[delta + 3];
+  }
+  if (mode == 0 && w <= 32 && h <= 32) {  // the use_filter_intra flag (+ its mode)
+    int32_t flag[2] = {0};
+    cdf_costs(flag, AV1_FILTER_INTRA_CDF[bsize], 2);
+    rate += flag[fi_mode == 5 ? 0 : 1];
+    if (fi_mode != 5) {
+      int32_t m[5] = {0};
+      cdf_costs(m, AV1_FILTER_INTRA_MODE_CDF, 5);
+      rate += m[fi_mode];
+    }
+  }
+  return rate;
+}

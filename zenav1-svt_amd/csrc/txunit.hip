@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   // ---- forward row pass + fold + quantize: lane t < H owns row t ----
   // The row's dequantized values replace it in LDS (0 where folded) as the
   // inverse transform's input.
-  uint64_t fdist = 0, fold = 0;
+  uint64_t fdist = 0, fold = 0, csatd = 0;
   int32_t eob = 0, lsum = 0, dcq = 0;
   if (active && t < H) {
     int32_t v[W];
@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       // 2:1 rectangles: x NewSqrt2 / 2^12; |x| stays below 2^18 (the pinned
       // coefficient bound before this scaling), so the product fits 32 bits
       if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
+      if (keep.csatd) csatd += (uint64_t)(x < 0 ? -(int64_t)x : (int64_t)x);
       if (t >= 32 || c >= 32) {
         const uint64_t e2 = (uint64_t)((int64_t)x * x);  // energy of the folded-away coefficients
         fdist += e2;
@@ -181,6 +182,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   }
   fdist = group_sum_u64<G::T>(fdist);
   if constexpr (W == 64 || H == 64) fold = group_sum_u64<G::T>(fold);
+  if (keep.csatd) csatd = group_sum_u64<G::T>(csatd);
   eob = group_max_i32<G::T>(eob);
   lsum = group_sum_i32<G::T>(lsum);
 
@@ -202,6 +204,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
         keep.eob16[job] = 0;
         keep.fold[job] = 0;
       }
+      if (keep.csatd) keep.csatd[job] = 0;
       return;
     }
     constexpr int SHF = (1 - LSC) * 2;
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       keep.eob16[job] = (uint16_t)eob;
       keep.fold[job] = fold;
     }
+    if (keep.csatd) keep.csatd[job] = csatd;
     uint32_t cl = (uint32_t)(lsum < 63 ? lsum : 63);
     if (dcq < 0) cl |= 1u << 6;
     else if (dcq > 0) cl += 2u << 6;
@@ -389,7 +393,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
-                           hipStream_t st, int32_t dq0, int32_t dq1) {
+                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
@@ -405,6 +409,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   keep.dq = reinterpret_cast<int32_t*>(carve(n * np * 4));
   keep.eob16 = reinterpret_cast<uint16_t*>(carve(n * 2));
   keep.fold = reinterpret_cast<uint64_t*>(carve(n * 8));
+  keep.csatd = csatd;  // written by the forward pass straight into the caller's output
   TxUnitJob* tj = reinterpret_cast<TxUnitJob*>(carve(n * sizeof(TxUnitJob)));
   RdoqJob* rj = reinterpret_cast<RdoqJob*>(carve(n * sizeof(RdoqJob)));
   RateJob* cj = reinterpret_cast<RateJob*>(carve(n * sizeof(RateJob)));

@@ -1,0 +1,853 @@
+// Superblock-wavefront intra mode decision for gfx950: every leaf of a
+// uniform partition of one or more still images (and their tiles) decided
+// with the reference's funnel -- MDS0 over every intra candidate predicted
+// from the real reconstruction, the exchange-sorted survivors into MDS1
+// (DCT_DCT, quantize_b, frequency distortion, exact coefficient rate), the
+// best of those into MDS3 (the tx-type search with RDOQ and spatial
+// distortion, in TX_TYPE_GROUPS order with the rate and SATD gates), the
+// winner by full cost, committed to the reconstruction the next leaves read.
+//
+// Reference: leaf_funnel/mod.rs:1007 evaluate_leaf (MDS0 :2060-2185, the
+// exchange sort :2994, MDS1 :3067-3110, the post-MDS1 sort :3236, MDS3's
+// tx-type search :7140-7330), rates rate_tables.rs:146 / :291, the coding
+// order enc/pipeline.rs:7588-7598 (superblocks in raster order, tiles
+// independent) and the neighbour availability of enc/intra_edge.rs:355 /
+// :436 (has_top_right / has_bottom_left) -- all paths relative to
+// rust/crates/svtav1-encoder/src.
+//
+// Order: a superblock needs its left, top and top-right neighbours, so the
+// superblocks of one anti-diagonal (column + 2 * row within a tile) are
+// independent; within a superblock the leaves follow Z-order.  A step is
+// (anti-diagonal, leaf index): every superblock of that anti-diagonal, in
+// every tile and every image of the batch, decides that leaf together, one
+// launch per stage over all of them.  Each stage writes the next stage's
+// job list on the device (survivor lists, transform-unit jobs), so a step is
+// a fixed chain of launches over device-resident queues; the whole frame's
+// chain is captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/zav1_dsp.h"
+#include "zav1_dev.h"
+#include "zav1_kernels.h"
+
+namespace zav1 {
+namespace {
+
+struct WfCand {
+  uint8_t mode;
+  int8_t delta;
+  uint8_t fi;   // 0..4 filter intra, 5 none
+  uint8_t dir;  // intra direction for the tx-type rate (filter intra maps to DC, V, H, D157, DC)
+};
+
+struct WfDecision {  // layout of zav1_wf_decision
+  uint8_t cand, tx_type, mode, fi;
+  int32_t eob, rate, mode_rate;
+  uint64_t dist, cost;
+};
+static_assert(sizeof(WfDecision) == 32, "decision layout");
+
+__host__ __device__ inline uint64_t rdc(uint64_t lambda, uint64_t rate, uint64_t dist) {
+  return ((rate * lambda + 256) >> 9) + (dist << 7);
+}
+
+// intra_mode_context (AV1): the kf y-mode context of a neighbour's mode
+__host__ __device__ inline int mode_ctx(int m) {
+  constexpr int8_t c[13] = {0, 1, 2, 3, 4, 4, 4, 4, 3, 0, 1, 2, 0};
+  return c[m];
+}
+__host__ __device__ inline bool is_smooth(int m) { return m >= 9 && m <= 11; }
+
+// The first k positions of the reference's exchange sort (swap whenever a
+// later candidate costs strictly less), over n costs in generation order.
+__host__ __device__ inline void exchange_first_k(const uint64_t* cost, int n, int k, uint8_t* ord) {
+  for (int i = 0; i < n; i++) ord[i] = (uint8_t)i;
+  for (int i = 0; i < k; i++) {
+    int mi = ord[i];
+    uint64_t ci = cost[mi];
+    for (int j = i + 1; j < n; j++) {
+      const int mj = ord[j];
+      if (cost[mj] < ci) {
+        ord[j] = (uint8_t)mi;
+        mi = mj;
+        ci = cost[mj];
+      }
+    }
+    ord[i] = (uint8_t)mi;
+  }
+}
+
+// The MDS3 tx-type search over one candidate's evaluated types (in search
+// order): the rate gate against the DCT cost, the SATD early exit, the
+// unsignalable eob-0 rule, the first strictly smaller cost wins
+// (leaf_funnel/mod.rs:7228-7390).  Returns the type index and its cost.
+__host__ __device__ inline int txt_pick(int nt, const uint8_t* types, const int32_t* txt_row, const uint64_t* dist,
+                                        const int32_t* rate, const int32_t* eob, const uint64_t* csatd,
+                                        uint64_t lambda, uint64_t rate_th, int64_t satd_th, uint64_t* best_cost) {
+  uint64_t best = ~0ull, dct_cost = ~0ull;
+  int64_t best_satd = INT64_MAX;
+  int pick = 0;
+  for (int t = 0; t < nt; t++) {
+    const int tt = types[t];
+    if (tt != 0 && dct_cost != ~0ull && rdc(lambda, (uint64_t)txt_row[tt], 0) * 1000 > dct_cost * rate_th) continue;
+    if (satd_th > 0) {
+      const int64_t sd = (int64_t)csatd[t];
+      if (sd < best_satd) best_satd = sd;
+      else if ((sd - best_satd) * 100 > best_satd * satd_th) continue;
+    }
+    if (eob[t] == 0 && tt != 0) continue;
+    const uint64_t c = rdc(lambda, (uint64_t)(uint32_t)rate[t], dist[t]);
+    if (c < best) {
+      best = c;
+      pick = t;
+      if (tt == 0) dct_cost = c;
+    } else if (tt == 0) {
+      dct_cost = c;
+    }
+  }
+  *best_cost = best;
+  return pick;
+}
+
+// ---- per-step kernels ------------------------------------------------------
+
+// Mode contexts and the edge-filter type of every unit from its coded
+// neighbours, and each candidate's luma rate under those contexts.
+__global__ __launch_bounds__(256) void k_wf_prep(MdUnit* __restrict__ units, const int32_t* __restrict__ nbr, int n,
+                                                 const uint8_t* __restrict__ mode_map,
+                                                 const uint32_t* __restrict__ rate_tab, int nc,
+                                                 uint32_t* __restrict__ rate_out) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)n * nc) return;
+  const int u = (int)(i / nc), c = (int)(i - (long)u * nc);
+  const int a = nbr[2 * u], l = nbr[2 * u + 1];
+  const int am = a >= 0 ? mode_map[a] : 0, lm = l >= 0 ? mode_map[l] : 0;
+  if (c == 0) units[u].filt_type = (uint8_t)((a >= 0 && is_smooth(am)) || (l >= 0 && is_smooth(lm)));
+  rate_out[i] = rate_tab[(mode_ctx(am) * 5 + mode_ctx(lm)) * nc + c];
+}
+
+// MDS1 jobs: every survivor's DCT_DCT unit (quantize_b, frequency distortion).
+__global__ __launch_bounds__(256) void k_wf_mds1_jobs(const MdUnit* __restrict__ units, int n,
+                                                      const uint8_t* __restrict__ sel, int k1,
+                                                      const WfCand* __restrict__ cands,
+                                                      const int32_t* __restrict__ txt, int L, int64_t rdmult,
+                                                      TxUnitRdJob* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n * k1) return;
+  const int u = i / k1;
+  const WfCand c = cands[sel[i]];
+  TxUnitRdJob j{};
+  j.x = units[u].x, j.y = units[u].y, j.pred_off = (uint32_t)i * (uint32_t)(L * L);
+  j.tx_type = 0, j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.plane_type = 0;
+  j.txt_rate = txt[c.dir * 16 + 0];
+  j.rdmult = rdmult;
+  out[i] = j;
+}
+
+// After MDS1: full costs of the survivors, the exchange sort's first k3 into
+// MDS3, and their tx-type search jobs (every type of the intra set, in
+// search order).  One thread per unit.
+__global__ __launch_bounds__(64) void k_wf_mds3_jobs(const MdUnit* __restrict__ units, int n,
+                                                     const uint8_t* __restrict__ sel, int k1, int k3,
+                                                     const WfCand* __restrict__ cands,
+                                                     const uint32_t* __restrict__ mode_rate, int nc,
+                                                     const uint64_t* __restrict__ dist1,
+                                                     const int32_t* __restrict__ rate1, uint64_t lambda,
+                                                     const uint8_t* __restrict__ types, int nt,
+                                                     const int32_t* __restrict__ txt, int L, int64_t rdmult,
+                                                     uint8_t* __restrict__ m3, TxUnitRdJob* __restrict__ out) {
+  const int u = blockIdx.x * 64 + threadIdx.x;
+  if (u >= n) return;
+  uint64_t cost[64];
+  uint8_t ord[64];
+  for (int r = 0; r < k1; r++) {
+    const int c = sel[u * k1 + r];
+    cost[r] = rdc(lambda, (uint64_t)mode_rate[u * nc + c] + (uint32_t)rate1[u * k1 + r], dist1[u * k1 + r]);
+  }
+  exchange_first_k(cost, k1, k3, ord);
+  for (int i = 0; i < k3; i++) {
+    const int r = ord[i];
+    m3[u * k3 + i] = (uint8_t)r;
+    const WfCand c = cands[sel[u * k1 + r]];
+    for (int t = 0; t < nt; t++) {
+      TxUnitRdJob j{};
+      j.x = units[u].x, j.y = units[u].y, j.pred_off = (uint32_t)(u * k1 + r) * (uint32_t)(L * L);
+      j.tx_type = types[t], j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.plane_type = 0;
+      j.txt_rate = txt[c.dir * 16 + types[t]];
+      j.rdmult = rdmult;
+      out[(u * k3 + i) * nt + t] = j;
+    }
+  }
+}
+
+// The leaf's winner: per MDS3 candidate the tx-type search, then the least
+// full cost (mode rate + coefficient rate, spatial distortion), first in
+// order; its reconstruction is written into the plane, its decision and mode
+// recorded.  One workgroup per unit; `pick_in` (host-chosen winner job) is
+// used instead of the search when not null.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_wf_commit(const MdUnit* __restrict__ units, int n,
+                                                   const int32_t* __restrict__ unit_id,
+                                                   const uint8_t* __restrict__ m3, const uint8_t* __restrict__ sel,
+                                                   int k1, int k3, const WfCand* __restrict__ cands,
+                                                   const uint32_t* __restrict__ mode_rate, int nc,
+                                                   const uint8_t* __restrict__ types, int nt,
+                                                   const int32_t* __restrict__ txt,
+                                                   const uint64_t* __restrict__ dist3,
+                                                   const int32_t* __restrict__ rate3,
+                                                   const int32_t* __restrict__ eob3,
+                                                   const uint64_t* __restrict__ csatd3,
+                                                   const uint16_t* __restrict__ rec3, uint64_t lambda,
+                                                   uint64_t rate_th, int64_t satd_th, Pix* __restrict__ recon,
+                                                   int stride, int L, WfDecision* __restrict__ dec,
+                                                   uint8_t* __restrict__ mode_map) {
+  __shared__ int win_job;
+  const int u = blockIdx.x;
+  if (u >= n) return;
+  if (threadIdx.x == 0) {
+    uint64_t best_full = ~0ull;
+    int bi = 0, bt = 0;
+    uint64_t bcost = 0;
+    for (int i = 0; i < k3; i++) {
+      const int r = m3[u * k3 + i];
+      const int c = sel[u * k1 + r];
+      const long j0 = (long)(u * k3 + i) * nt;
+      uint64_t tc;
+      const int t = txt_pick(nt, types, txt + cands[c].dir * 16, dist3 + j0, rate3 + j0, eob3 + j0, csatd3 + j0,
+                             lambda, rate_th, satd_th, &tc);
+      const uint64_t full = rdc(lambda, (uint64_t)mode_rate[u * nc + c] + (uint32_t)rate3[j0 + t], dist3[j0 + t]);
+      if (full < best_full) {
+        best_full = full;
+        bi = i;
+        bt = t;
+        bcost = full;
+      }
+    }
+    const long jw = (long)(u * k3 + bi) * nt + bt;
+    win_job = (int)jw;
+    const int r = m3[u * k3 + bi];
+    const int c = sel[u * k1 + r];
+    WfDecision d;
+    d.cand = (uint8_t)c, d.tx_type = types[bt], d.mode = cands[c].mode, d.fi = cands[c].fi;
+    d.eob = eob3[jw], d.rate = rate3[jw], d.mode_rate = (int32_t)mode_rate[u * nc + c];
+    d.dist = dist3[jw], d.cost = bcost;
+    const int id = unit_id[u];
+    dec[id] = d;
+    mode_map[id] = cands[c].mode;
+  }
+  __syncthreads();
+  const uint16_t* src = rec3 + (long)win_job * L * L;
+  const long x0 = units[u].x, y0 = units[u].y;
+  for (int i = threadIdx.x; i < L * L; i += 256) {
+    const int r = i / L, c = i - r * L;
+    recon[(y0 + r) * stride + x0 + c] = (Pix)src[i];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host plan
+// ---------------------------------------------------------------------------
+
+struct Wavefront {
+  zav1_wf_cfg cfg{};
+  int L = 0, nl = 0, lps = 0, tx = 0, nt = 0, nc = 0, k1 = 0, k3 = 0;
+  int stacked_h = 0, n_units = 0, n_steps = 0, max_u = 0;
+  size_t bytes = 0;
+  std::vector<int> step_off;
+  std::vector<MdUnit> h_units;
+  std::vector<int32_t> h_nbr;
+  uint8_t types[16] = {};
+  MdCandList cl{};
+  std::vector<WfCand> h_cands;
+  uint64_t rate_th = 0;
+  int64_t satd_th = 0, rdmult = 0;
+  unsigned flags3 = 0;
+  uint32_t cut_num = 0, cut_den = 0;
+  int32_t dq0 = 0, dq1 = 0;
+  const QuantRow* row = nullptr;
+  const uint16_t *iscan = nullptr, *fscan = nullptr;
+  // device memory (one allocation, carved)
+  char* base = nullptr;
+  void *src = nullptr, *recon = nullptr;
+  MdUnit* units = nullptr;
+  int32_t *nbr = nullptr, *uid = nullptr;
+  WfDecision* dec = nullptr;
+  uint8_t* mode_map = nullptr;
+  uint32_t* rate_tab = nullptr;
+  int32_t* txt = nullptr;
+  WfCand* cands = nullptr;
+  uint8_t* d_types = nullptr;
+  int32_t* costs = nullptr;
+  uint32_t *rate = nullptr, *satd = nullptr;
+  uint8_t *sel = nullptr, *m3 = nullptr;
+  uint64_t* sel_cost = nullptr;
+  TxUnitJob* fs_jobs = nullptr;
+  uint16_t *pred = nullptr, *rec3 = nullptr;
+  TxUnitRdJob *jobs1 = nullptr, *jobs3 = nullptr;
+  uint64_t *dist1 = nullptr, *dist3 = nullptr, *csatd3 = nullptr;
+  int32_t *eob1 = nullptr, *rate1 = nullptr, *eob3 = nullptr, *rate3 = nullptr;
+  uint8_t *cul1 = nullptr, *cul3 = nullptr;
+  void *ws1 = nullptr, *ws3 = nullptr;
+  // plain per-call form scratch
+  uint8_t* one_sel = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+int zorder(int i, int j) {  // bit interleave: row bits odd, column bits even
+  int z = 0;
+  for (int b = 0; b < 8; b++) z |= (((j >> b) & 1) << (2 * b)) | (((i >> b) & 1) << (2 * b + 1));
+  return z;
+}
+
+size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// The intra ext-tx set of the leaf's transform in TX_TYPE_GROUPS order
+// (leaf_funnel/mod.rs:7189: {DCT}, {V_DCT, H_DCT}, {ADST_ADST},
+// {ADST_DCT, DCT_ADST}, {FLIPADST_FLIPADST, IDTX}, the rest), limited to the
+// first `groups` groups.
+static int search_types(int tx_size, int groups, uint8_t* out) {
+  static const int8_t kGroups[6][8] = {{0, -1}, {10, 11, -1}, {3, -1}, {1, 2, -1}, {6, 9, -1},
+                                       {4, 5, 7, 8, 12, 13, 14, 15}};
+  static const int8_t kGroupLen[6] = {1, 2, 1, 2, 2, 8};
+  const int w = tx_w(tx_size), h = tx_h(tx_size);
+  const bool dct_only = std::max(w, h) >= 32;
+  const bool sq16 = w == 16 && h == 16;
+  auto used = [&](int t) {
+    if (t == 0) return true;
+    if (dct_only) return false;
+    if (t == 1 || t == 2 || t == 3 || t == 9) return true;
+    return !sq16 && (t == 10 || t == 11);
+  };
+  int n = 0;
+  for (int g = 0; g < std::min(groups, 6); g++)
+    for (int k = 0; k < kGroupLen[g]; k++)
+      if (used(kGroups[g][k])) out[n++] = (uint8_t)kGroups[g][k];
+  return n;
+}
+
+static int plan(Wavefront* w) {
+  const zav1_wf_cfg& c = w->cfg;
+  w->L = c.leaf;
+  w->nl = c.sb / c.leaf;
+  w->lps = w->nl * w->nl;
+  for (int s = 0; s < 19; s++)
+    if (tx_w(s) == c.leaf && tx_h(s) == c.leaf) w->tx = s;
+  w->nt = search_types(w->tx, c.txt_groups, w->types);
+  w->nc = c.n_cands;
+  w->k1 = c.k1;
+  w->k3 = c.k3;
+  w->stacked_h = c.height * c.n_images;
+  const int sbc = (c.width + c.sb - 1) / c.sb, sbr = (c.height + c.sb - 1) / c.sb;
+  const int tcols = std::max(1, c.tile_cols), trows = std::max(1, c.tile_rows);
+  // uniform tiles in superblock units (the last ones take the remainder)
+  auto tile_edges = [](int n_sb, int n_t, std::vector<int>& e) {
+    e.assign(n_t + 1, 0);
+    const int per = (n_sb + n_t - 1) / n_t;
+    for (int t = 0; t <= n_t; t++) e[t] = std::min(n_sb, t * per);
+  };
+  std::vector<int> tc, tr;
+  tile_edges(sbc, tcols, tc);
+  tile_edges(sbr, trows, tr);
+  struct Leaf {
+    int step, img, x, y, tile, tx0, tx1, ty0, ty1;
+  };
+  std::vector<Leaf> leaves;
+  const int L = w->L, nl = w->nl;
+  for (int img = 0; img < c.n_images; img++)
+    for (int ty = 0; ty < trows; ty++)
+      for (int tx = 0; tx < tcols; tx++) {
+        const int x0 = tc[tx] * c.sb, x1 = std::min(c.width, tc[tx + 1] * c.sb);
+        const int y0 = tr[ty] * c.sb, y1 = std::min(c.height, tr[ty + 1] * c.sb);
+        for (int r = tr[ty]; r < tr[ty + 1]; r++)
+          for (int q = tc[tx]; q < tc[tx + 1]; q++) {
+            const int d = (q - tc[tx]) + 2 * (r - tr[ty]);
+            for (int z = 0; z < w->lps; z++) {
+              int i = 0, j = 0;
+              for (int b = 0; b < 8; b++) {
+                j |= ((z >> (2 * b)) & 1) << b;
+                i |= ((z >> (2 * b + 1)) & 1) << b;
+              }
+              const int x = q * c.sb + j * L, y = r * c.sb + i * L;
+              if (x >= c.width || y >= c.height) continue;
+              leaves.push_back({d * w->lps + z, img, x, y, ty * tcols + tx, x0, x1, y0, y1});
+            }
+          }
+      }
+  std::stable_sort(leaves.begin(), leaves.end(), [](const Leaf& a, const Leaf& b) { return a.step < b.step; });
+  const int n = (int)leaves.size();
+  w->n_units = n;
+  // leaf lookup for neighbour ids: (img, y / L, x / L) -> unit index
+  const int gw = c.width / L, gh = c.height / L;
+  std::vector<int32_t> at((size_t)c.n_images * gw * gh, -1);
+  for (int u = 0; u < n; u++) at[((size_t)leaves[u].img * gh + leaves[u].y / L) * gw + leaves[u].x / L] = u;
+  w->h_units.resize(n);
+  w->h_nbr.resize(2 * (size_t)n);
+  w->step_off.clear();
+  int last = -1;
+  for (int u = 0; u < n; u++) {
+    const Leaf& f = leaves[u];
+    while (last < f.step) {
+      w->step_off.push_back(u);
+      last++;
+    }
+    const int i = (f.y % c.sb) / L, j = (f.x % c.sb) / L, zi = zorder(i, j);
+    const bool up = f.y > f.ty0, lf = f.x > f.tx0;
+    // top-right: the row above; inside the superblock only when that leaf
+    // precedes this one in Z-order, never in the superblock to the right
+    bool tr_ok;
+    if (!up) tr_ok = false;
+    else if (i == 0) tr_ok = true;
+    else tr_ok = j + 1 < nl && zorder(i - 1, j + 1) < zi;
+    // bottom-left: the left superblock's rows of this superblock row, or an
+    // earlier leaf of this superblock
+    bool bl_ok;
+    if (!lf) bl_ok = false;
+    else if (j == 0) bl_ok = i + 1 < nl;
+    else bl_ok = i + 1 < nl && zorder(i + 1, j - 1) < zi;
+    MdUnit mu{};
+    mu.x = f.x;
+    mu.y = f.y + f.img * c.height;
+    mu.n_top_px = (int16_t)(up ? L : 0);
+    mu.n_topright_px = (int16_t)(tr_ok ? std::max(0, std::min(L, f.tx1 - (f.x + L))) : 0);
+    mu.n_left_px = (int16_t)(lf ? L : 0);
+    mu.n_bottomleft_px = (int16_t)(bl_ok ? std::max(0, std::min(L, f.ty1 - (f.y + L))) : 0);
+    mu.edge_filter = 1;
+    w->h_units[u] = mu;
+    const size_t g = (size_t)f.img * gw * gh;
+    w->h_nbr[2 * u] = up ? at[g + (size_t)(f.y / L - 1) * gw + f.x / L] : -1;
+    w->h_nbr[2 * u + 1] = lf ? at[g + (size_t)(f.y / L) * gw + f.x / L - 1] : -1;
+  }
+  w->step_off.push_back(n);
+  w->n_steps = (int)w->step_off.size() - 1;
+  w->max_u = 0;
+  for (int s = 0; s < w->n_steps; s++) w->max_u = std::max(w->max_u, w->step_off[s + 1] - w->step_off[s]);
+  return ZAV1_OK;
+}
+
+static size_t footprint(const Wavefront& w) {
+  const zav1_wf_cfg& c = w.cfg;
+  const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w.stacked_h * es, U = (size_t)w.max_u;
+  const size_t n = (size_t)w.n_units, LL = (size_t)w.L * w.L;
+  size_t b = 2 * al(plane) + al(n * sizeof(MdUnit)) + al(n * 8) + al(n * 4) + al(n * sizeof(WfDecision)) + al(n);
+  b += al(25 * w.nc * 4) + al(13 * 16 * 4) + al(64 * sizeof(WfCand)) + al(16) + al(sizeof(zav1_coeff_costs));
+  b += 2 * al(U * w.nc * 4) + 2 * al(U * w.k1) + al(U * w.k1 * 8) + al(U * w.k1 * sizeof(TxUnitJob));
+  b += al(U * w.k1 * LL * 2) + al(U * w.k1 * sizeof(TxUnitRdJob)) + al(U * w.k1 * 8) + 2 * al(U * w.k1 * 4) +
+       al(U * w.k1);
+  const size_t n3 = U * w.k3 * w.nt;
+  b += al(n3 * sizeof(TxUnitRdJob)) + 2 * al(n3 * 8) + 2 * al(n3 * 4) + al(n3) + al(n3 * LL * 2);
+  b += al(txunit_rd_workspace(w.tx, (int)(U * w.k1))) + al(txunit_rd_workspace(w.tx, (int)n3)) + al(64);
+  return b;
+}
+
+static hipError_t step_chain(Wavefront* w, int s, hipStream_t st) {
+  const int u0 = w->step_off[s], U = w->step_off[s + 1] - u0;
+  if (U == 0) return hipSuccess;
+  const zav1_wf_cfg& c = w->cfg;
+  const int L = w->L, W = c.width, H = w->stacked_h;
+  MdUnit* units = w->units + u0;
+  hipLaunchKernelGGL(k_wf_prep, dim3((unsigned)((U * w->nc + 255) / 256)), dim3(256), 0, st, units, w->nbr + 2 * u0,
+                     U, w->mode_map, w->rate_tab, w->nc, w->rate);
+  hipError_t e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, units, U, w->cl, w->satd, st);
+  if (e != hipSuccess) return e;
+  TypeList dct{};
+  dct.n = 1;
+  e = fast_select(w->satd, w->rate, c.lambda, U, w->nc, w->k1, reinterpret_cast<const int32_t*>(units), 5, L, L, dct,
+                  1, w->sel, w->sel_cost, w->fs_jobs, st);
+  if (e != hipSuccess) return e;
+  e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, units, U, w->cl, w->sel, w->k1, w->pred, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wf_mds1_jobs, dim3((unsigned)((U * w->k1 + 255) / 256)), dim3(256), 0, st, units, U, w->sel,
+                     w->k1, w->cands, w->txt, L, w->rdmult, w->jobs1);
+  e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs1, U * w->k1, w->row,
+                      w->iscan, w->fscan, w->costs, kTxuQuantB, 0, 0, w->ws1, w->dist1, w->eob1, w->rate1, w->cul1,
+                      nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_wf_mds3_jobs, dim3((unsigned)((U + 63) / 64)), dim3(64), 0, st, units, U, w->sel, w->k1, w->k3,
+                     w->cands, w->rate, w->nc, w->dist1, w->rate1, c.lambda, w->d_types, w->nt, w->txt, L, w->rdmult,
+                     w->m3, w->jobs3);
+  const int n3 = U * w->k3 * w->nt;
+  e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs3, n3, w->row, w->iscan,
+                      w->fscan, w->costs, w->flags3, w->cut_num, w->cut_den, w->ws3, w->dist3, w->eob3, w->rate3,
+                      w->cul3, nullptr, w->rec3, st, w->dq0, w->dq1, w->csatd3);
+  if (e != hipSuccess) return e;
+  if (c.bd == 8)
+    hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, w->m3,
+                       w->sel, w->k1, w->k3, w->cands, w->rate, w->nc, w->d_types, w->nt, w->txt, w->dist3, w->rate3,
+                       w->eob3, w->csatd3, w->rec3, c.lambda, w->rate_th, w->satd_th, (uint8_t*)w->recon, W, L, w->dec,
+                       w->mode_map);
+  else
+    hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, w->m3,
+                       w->sel, w->k1, w->k3, w->cands, w->rate, w->nc, w->d_types, w->nt, w->txt, w->dist3, w->rate3,
+                       w->eob3, w->csatd3, w->rec3, c.lambda, w->rate_th, w->satd_th, (uint16_t*)w->recon, W, L,
+                       w->dec, w->mode_map);
+  return hipGetLastError();
+}
+
+// The plain form of one unit: one launch per elementary call with one job
+// each (every candidate's prediction + SATD, every survivor's prediction and
+// transform unit, every MDS3 type's unit), the selections on the host from
+// the results read back after each stage -- what a per-call binding of the
+// same kernels does.
+static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
+  const zav1_wf_cfg& c = w->cfg;
+  const int L = w->L, W = c.width, H = w->stacked_h, nc = w->nc, k1 = w->k1, k3 = w->k3, nt = w->nt;
+  MdUnit* unit = w->units + u;
+  hipLaunchKernelGGL(k_wf_prep, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, unit, w->nbr + 2 * u, 1,
+                     w->mode_map, w->rate_tab, nc, w->rate);
+  hipError_t e;
+  for (int m = 0; m < nc; m++) {
+    MdCandList one{};
+    one.n = 1;
+    one.c[0] = w->cl.c[m];
+    e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, unit, 1, one, w->satd + m, st);
+    if (e != hipSuccess) return e;
+  }
+  std::vector<uint32_t> satd(nc), rate(nc);
+  if ((e = hipMemcpyAsync(satd.data(), w->satd, 4 * nc, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(rate.data(), w->rate, 4 * nc, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  std::vector<uint64_t> fc(nc);
+  for (int m = 0; m < nc; m++)
+    fc[m] = satd[m] == kSatdInvalid ? ~0ull : rdc(c.lambda, rate[m], (uint64_t)satd[m] << 4);
+  uint8_t ord[64];
+  exchange_first_k(fc.data(), nc, k1, ord);
+  std::vector<uint8_t> sel(ord, ord + k1);
+  if ((e = hipMemcpyAsync(w->sel, sel.data(), k1, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  MdUnit hu = w->h_units[u];
+  std::vector<uint64_t> d1(k1);
+  std::vector<int32_t> r1(k1);
+  for (int r = 0; r < k1; r++) {
+    e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, unit, 1, w->cl, w->sel + r, 1, w->pred + r * L * L, st);
+    if (e != hipSuccess) return e;
+  }
+  // MDS1 jobs on the device as the batched chain builds them, then one
+  // transform-unit call per survivor
+  hipLaunchKernelGGL(k_wf_mds1_jobs, dim3(1), dim3(256), 0, st, unit, 1, w->sel, k1, w->cands, w->txt, L, w->rdmult,
+                     w->jobs1);
+  for (int r = 0; r < k1; r++) {
+    e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)k1 * L * L, w->jobs1 + r, 1, w->row, w->iscan,
+                        w->fscan, w->costs, kTxuQuantB, 0, 0, w->ws1, w->dist1 + r, w->eob1 + r, w->rate1 + r,
+                        w->cul1 + r, nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
+    if (e != hipSuccess) return e;
+  }
+  if ((e = hipMemcpyAsync(d1.data(), w->dist1, 8 * k1, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(r1.data(), w->rate1, 4 * k1, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  std::vector<uint64_t> c1(k1);
+  for (int r = 0; r < k1; r++) c1[r] = rdc(c.lambda, (uint64_t)rate[sel[r]] + (uint32_t)r1[r], d1[r]);
+  uint8_t ord3[64];
+  exchange_first_k(c1.data(), k1, k3, ord3);
+  std::vector<TxUnitRdJob> j3((size_t)k3 * nt);
+  std::vector<int32_t> htxt(13 * 16);
+  if ((e = hipMemcpyAsync(htxt.data(), w->txt, 4 * 13 * 16, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  for (int i = 0; i < k3; i++) {
+    const WfCand cd = w->h_cands[sel[ord3[i]]];
+    for (int t = 0; t < nt; t++) {
+      TxUnitRdJob& j = j3[(size_t)i * nt + t];
+      j = TxUnitRdJob{};
+      j.x = hu.x, j.y = hu.y, j.pred_off = (uint32_t)(ord3[i] * L * L), j.tx_type = w->types[t];
+      j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.txt_rate = htxt[cd.dir * 16 + w->types[t]], j.rdmult = w->rdmult;
+    }
+  }
+  if ((e = hipMemcpyAsync(w->jobs3, j3.data(), j3.size() * sizeof(TxUnitRdJob), hipMemcpyHostToDevice, st)) !=
+      hipSuccess)
+    return e;
+  for (int q = 0; q < k3 * nt; q++) {
+    e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)k1 * L * L, w->jobs3 + q, 1, w->row, w->iscan,
+                        w->fscan, w->costs, w->flags3, w->cut_num, w->cut_den, w->ws3, w->dist3 + q, w->eob3 + q,
+                        w->rate3 + q, w->cul3 + q, nullptr, w->rec3 + (size_t)q * L * L, st, w->dq0, w->dq1,
+                        w->csatd3 + q);
+    if (e != hipSuccess) return e;
+  }
+  // the winner from the results, then one call storing its reconstruction
+  std::vector<uint64_t> d3((size_t)k3 * nt), s3((size_t)k3 * nt);
+  std::vector<int32_t> r3((size_t)k3 * nt), e3((size_t)k3 * nt);
+  if ((e = hipMemcpyAsync(d3.data(), w->dist3, 8 * d3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(s3.data(), w->csatd3, 8 * s3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(r3.data(), w->rate3, 4 * r3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(e3.data(), w->eob3, 4 * e3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  uint64_t best_full = ~0ull;
+  int bi = 0;
+  for (int i = 0; i < k3; i++) {
+    const int cand = sel[ord3[i]];
+    uint64_t tc;
+    const size_t j0 = (size_t)i * nt;
+    const int t = txt_pick(nt, w->types, htxt.data() + w->h_cands[cand].dir * 16, d3.data() + j0, r3.data() + j0,
+                           e3.data() + j0, s3.data() + j0, c.lambda, w->rate_th, w->satd_th, &tc);
+    const uint64_t full = rdc(c.lambda, (uint64_t)rate[cand] + (uint32_t)r3[j0 + t], d3[j0 + t]);
+    if (full < best_full) {
+      best_full = full;
+      bi = i;
+    }
+  }
+  // the store: the commit kernel restricted to the chosen candidate (its
+  // search over that candidate's types repeats the host's)
+  uint8_t m3one = ord3[bi];
+  if ((e = hipMemcpyAsync(w->m3, &m3one, 1, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  const size_t j0 = (size_t)bi * nt;
+  if (c.bd == 8)
+    hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, w->m3, w->sel, k1, 1,
+                       w->cands, w->rate, nc, w->d_types, nt, w->txt, w->dist3 + j0, w->rate3 + j0, w->eob3 + j0,
+                       w->csatd3 + j0, w->rec3 + j0 * L * L, c.lambda, w->rate_th, w->satd_th, (uint8_t*)w->recon, W,
+                       L, w->dec, w->mode_map);
+  else
+    hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, w->m3, w->sel, k1, 1,
+                       w->cands, w->rate, nc, w->d_types, nt, w->txt, w->dist3 + j0, w->rate3 + j0, w->eob3 + j0,
+                       w->csatd3 + j0, w->rec3 + j0 * L * L, c.lambda, w->rate_th, w->satd_th, (uint16_t*)w->recon,
+                       W, L, w->dec, w->mode_map);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
+}  // namespace zav1
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::string g_wf_err;
+int wf_fail(hipError_t e, const char* where) {
+  g_wf_err = std::string(where) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? ZAV1_ENOMEM : ZAV1_EIO;
+}
+#define WCHECK(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return wf_fail(e_, #call);   \
+  } while (0)
+
+int check_cfg(const zav1_wf_cfg* c) {
+  if (!c || (c->bd != 8 && c->bd != 10) || (c->leaf != 8 && c->leaf != 16 && c->leaf != 32) ||
+      (c->sb != 64 && c->sb != 128) || c->width <= 0 || c->height <= 0 || c->width % c->leaf ||
+      c->height % c->leaf || c->n_images < 1 || c->tile_cols < 1 || c->tile_rows < 1 || c->qindex < 0 ||
+      c->qindex > 255 || c->n_cands < 1 || c->n_cands > 64 || c->k1 < 1 || c->k1 > c->n_cands || c->k3 < 1 ||
+      c->k3 > c->k1 || c->rdoq_level < 0 || c->rdoq_level > 3 || c->txt_groups < 1 || c->txt_groups > 6 ||
+      c->txt_rate_th < 0 || c->txt_satd_th < 0 || c->lambda == 0 ||
+      (long)c->width * c->height * c->n_images > (1L << 31))
+    return ZAV1_EINVAL;
+  for (int i = 0; i < c->n_cands; i++) {
+    const zav1_md_cand m = c->cands[i];
+    if (m.mode > 12 || m.delta < -3 || m.delta > 3 || m.fi_mode > 5) return ZAV1_EINVAL;
+    if (m.fi_mode < 5 && (m.mode != 0 || c->leaf > 32)) return ZAV1_EINVAL;
+  }
+  return ZAV1_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int zav1_wf_footprint(const zav1_wf_cfg* cfg, size_t* bytes) {
+  if (!bytes) return ZAV1_EINVAL;
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  zav1::Wavefront w;
+  w.cfg = *cfg;
+  zav1::plan(&w);
+  *bytes = zav1::footprint(w);
+  return ZAV1_OK;
+}
+
+int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
+  if (!out) return ZAV1_EINVAL;
+  *out = nullptr;
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  auto* w = new zav1::Wavefront();
+  w->cfg = *cfg;
+  zav1::plan(w);
+  const zav1_wf_cfg& c = w->cfg;
+  // tables of the current device (built by zav1_init) and the host-side rates
+  rc = zav1::device_tables(&w->row, &w->iscan, &w->fscan, c.bd == 10 ? 256 + c.qindex : c.qindex);
+  if (rc) {
+    delete w;
+    return rc;
+  }
+  w->fscan += w->tx * 3072;
+  zav1_quant_row qr;
+  zav1_build_quant_row(c.qindex, c.bd, &qr);
+  w->dq0 = qr.dequant[0], w->dq1 = qr.dequant[1];
+  w->rdmult = ((int64_t)c.lambda * 17 + 2) >> 2;  // rdoq_rdmult_full, luma, sharpness <= 2
+  w->rate_th = (uint64_t)c.txt_rate_th;
+  w->satd_th = w->nt > 1 ? c.txt_satd_th : 0;
+  if (c.rdoq_level == 0) w->flags3 = ZAV1_TXU_QUANT_B | ZAV1_TXU_SPATIAL | ZAV1_TXU_RECON;
+  else {
+    w->flags3 = ZAV1_TXU_RDOQ | ZAV1_TXU_SPATIAL | ZAV1_TXU_RECON;
+    w->cut_num = c.rdoq_level == 1 ? 0 : (c.rdoq_level == 2 ? 80 : 60);
+    w->cut_den = c.rdoq_level == 1 ? 0 : 100;
+  }
+  w->cl.n = c.n_cands;
+  static const uint8_t kFiDir[5] = {0, 1, 2, 6, 0};  // FIMODE_TO_INTRADIR
+  for (int i = 0; i < c.n_cands; i++) {
+    const zav1_md_cand m = c.cands[i];
+    w->cl.c[i] = zav1::MdCand{m.mode, m.delta, m.fi_mode, 0};
+    w->h_cands.push_back(zav1::WfCand{m.mode, m.delta, m.fi_mode, (uint8_t)(m.fi_mode < 5 ? kFiDir[m.fi_mode] : m.mode)});
+  }
+  std::vector<uint32_t> rtab(25 * (size_t)c.n_cands);
+  for (int a = 0; a < 5; a++)
+    for (int l = 0; l < 5; l++)
+      for (int i = 0; i < c.n_cands; i++)
+        rtab[(a * 5 + l) * c.n_cands + i] = (uint32_t)zav1_intra_mode_rate(c.leaf, c.leaf, a, l, c.cands[i].mode,
+                                                                           c.cands[i].delta, c.cands[i].fi_mode);
+  std::vector<int32_t> txt(13 * 16);
+  for (int d = 0; d < 13; d++)
+    for (int t = 0; t < 16; t++) txt[d * 16 + t] = zav1_intra_txt_rate(w->tx, d, t);
+  zav1_coeff_costs costs;
+  zav1_build_coeff_costs(c.qindex, &costs);
+  // one device allocation, carved
+  w->bytes = zav1::footprint(*w);
+  if (hipMalloc(&w->base, w->bytes) != hipSuccess) {
+    g_wf_err = "zav1_wf_create: device allocation of " + std::to_string(w->bytes) + " bytes failed";
+    delete w;
+    return ZAV1_ENOMEM;
+  }
+  char* p = w->base;
+  auto take = [&p](size_t b) {
+    char* q = p;
+    p += zav1::al(b);
+    return (void*)q;
+  };
+  const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w->stacked_h * es, U = (size_t)w->max_u;
+  const size_t n = (size_t)w->n_units, LL = (size_t)w->L * w->L, n3 = U * w->k3 * w->nt;
+  w->src = take(plane);
+  w->recon = take(plane);
+  w->units = (zav1::MdUnit*)take(n * sizeof(zav1::MdUnit));
+  w->nbr = (int32_t*)take(n * 8);
+  w->uid = (int32_t*)take(n * 4);
+  w->dec = (zav1::WfDecision*)take(n * sizeof(zav1::WfDecision));
+  w->mode_map = (uint8_t*)take(n);
+  w->rate_tab = (uint32_t*)take(25 * w->nc * 4);
+  w->txt = (int32_t*)take(13 * 16 * 4);
+  w->cands = (zav1::WfCand*)take(64 * sizeof(zav1::WfCand));
+  w->d_types = (uint8_t*)take(16);
+  w->costs = (int32_t*)take(sizeof(zav1_coeff_costs));
+  w->rate = (uint32_t*)take(U * w->nc * 4);
+  w->satd = (uint32_t*)take(U * w->nc * 4);
+  w->sel = (uint8_t*)take(U * w->k1);
+  w->m3 = (uint8_t*)take(U * w->k1);
+  w->sel_cost = (uint64_t*)take(U * w->k1 * 8);
+  w->fs_jobs = (zav1::TxUnitJob*)take(U * w->k1 * sizeof(zav1::TxUnitJob));
+  w->pred = (uint16_t*)take(U * w->k1 * LL * 2);
+  w->jobs1 = (zav1::TxUnitRdJob*)take(U * w->k1 * sizeof(zav1::TxUnitRdJob));
+  w->dist1 = (uint64_t*)take(U * w->k1 * 8);
+  w->eob1 = (int32_t*)take(U * w->k1 * 4);
+  w->rate1 = (int32_t*)take(U * w->k1 * 4);
+  w->cul1 = (uint8_t*)take(U * w->k1);
+  w->jobs3 = (zav1::TxUnitRdJob*)take(n3 * sizeof(zav1::TxUnitRdJob));
+  w->dist3 = (uint64_t*)take(n3 * 8);
+  w->csatd3 = (uint64_t*)take(n3 * 8);
+  w->eob3 = (int32_t*)take(n3 * 4);
+  w->rate3 = (int32_t*)take(n3 * 4);
+  w->cul3 = (uint8_t*)take(n3);
+  w->rec3 = (uint16_t*)take(n3 * LL * 2);
+  w->ws1 = take(zav1::txunit_rd_workspace(w->tx, (int)(U * w->k1)));
+  w->ws3 = take(zav1::txunit_rd_workspace(w->tx, (int)n3));
+  std::vector<int32_t> uid(n);
+  for (size_t i = 0; i < n; i++) uid[i] = (int32_t)i;
+  hipError_t e = hipSuccess;
+  auto up = [&e](void* d, const void* h, size_t b) {
+    if (e == hipSuccess && b) e = hipMemcpy(d, h, b, hipMemcpyHostToDevice);
+  };
+  up(w->units, w->h_units.data(), n * sizeof(zav1::MdUnit));
+  up(w->nbr, w->h_nbr.data(), n * 8);
+  up(w->uid, uid.data(), n * 4);
+  up(w->rate_tab, rtab.data(), rtab.size() * 4);
+  up(w->txt, txt.data(), txt.size() * 4);
+  up(w->cands, w->h_cands.data(), w->h_cands.size() * sizeof(zav1::WfCand));
+  up(w->d_types, w->types, 16);
+  up(w->costs, &costs, sizeof(costs));
+  if (e != hipSuccess) {
+    const int r = wf_fail(e, "zav1_wf_create upload");
+    (void)hipFree(w->base);
+    delete w;
+    return r;
+  }
+  *out = reinterpret_cast<zav1_wf*>(w);
+  return ZAV1_OK;
+}
+
+int zav1_wf_info(const zav1_wf* wf, zav1_wf_info_t* info) {
+  if (!wf || !info) return ZAV1_EINVAL;
+  const auto* w = reinterpret_cast<const zav1::Wavefront*>(wf);
+  info->d_src = w->src;
+  info->d_recon = w->recon;
+  info->d_decisions = reinterpret_cast<zav1_wf_decision*>(w->dec);
+  info->d_units = reinterpret_cast<const zav1_md_unit*>(w->units);
+  info->n_leaves = w->n_units;
+  info->n_steps = w->n_steps;
+  info->max_leaves_per_step = w->max_u;
+  info->n_types = w->nt;
+  info->stacked_height = w->stacked_h;
+  info->bytes = w->bytes;
+  for (int i = 0; i < 16; i++) info->types[i] = w->types[i];
+  return ZAV1_OK;
+}
+
+int zav1_wf_run(zav1_wf* wf, int use_graph, zav1_stream_t s) {
+  if (!wf || (use_graph != 0 && use_graph != 1)) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
+  if (!use_graph) {
+    for (int i = 0; i < w->n_steps; i++) WCHECK(zav1::step_chain(w, i, st));
+    return ZAV1_OK;
+  }
+  if (st == hipStreamPerThread) {
+    g_wf_err = "zav1_wf_run: graph replay needs a created stream";
+    return ZAV1_EINVAL;
+  }
+  if (!w->graph || w->graph_stream != st) {
+    if (w->graph) {
+      WCHECK(hipGraphExecDestroy(w->graph));
+      w->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    WCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < w->n_steps && e == hipSuccess; i++) e = zav1::step_chain(w, i, st);
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess) return wf_fail(e, "zav1_wf_run capture");
+    if (ec != hipSuccess) return wf_fail(ec, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return wf_fail(e, "hipGraphInstantiate");
+    w->graph_stream = st;
+  }
+  WCHECK(hipGraphLaunch(w->graph, st));
+  return ZAV1_OK;
+}
+
+int zav1_wf_run_plain(zav1_wf* wf, int first_leaf, int n_leaves, zav1_stream_t s) {
+  if (!wf) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  if (first_leaf < 0 || n_leaves < 0 || first_leaf + n_leaves > w->n_units) return ZAV1_EINVAL;
+  hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
+  for (int u = first_leaf; u < first_leaf + n_leaves; u++) WCHECK(zav1::plain_unit(w, u, st));
+  return ZAV1_OK;
+}
+
+int zav1_wf_destroy(zav1_wf* wf) {
+  if (!wf) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  if (w->graph) (void)hipGraphExecDestroy(w->graph);
+  if (w->base) (void)hipFree(w->base);
+  delete w;
+  return ZAV1_OK;
+}
+
+const char* zav1_wf_last_error(void) { return g_wf_err.c_str(); }
+
+}  // extern "C"

@@ -281,6 +281,7 @@ struct TuKeep {
   int32_t* dq = nullptr;
   uint16_t* eob16 = nullptr;
   uint64_t* fold = nullptr;
+  uint64_t* csatd = nullptr;  // sum of |coefficient| over the whole transform (nullable)
 };
 // One full-RD transform unit (layout of zav1_txunit_rd_job).
 struct TxUnitRdJob {
@@ -336,7 +337,10 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
-                           hipStream_t st, int32_t dq0, int32_t dq1);
+                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr);
+// The current device's quantizer row `row_index` (bd index * 256 + qindex),
+// inverse scans and forward-scan base, as built by zav1_init (no building).
+int device_tables(const QuantRow** row, const uint16_t** iscan, const uint16_t** fscan_base, int row_index);
 
 // One intra transform unit (layout of zav1_intra_job in include/zav1_dsp.h).
 struct IntraJob {

@@ -1,0 +1,145 @@
+"""Host driver of the superblock-wavefront mode decision (zav1_wf_*): the
+batched form of the reference's leaf funnel over every leaf of a batch of
+still images (rust/crates/svtav1-encoder/src/leaf_funnel/mod.rs:1007
+evaluate_leaf, in the coding order of enc/pipeline.rs:7588-7598).
+
+`WavefrontMD` plans a batch (n_images equal-size luma planes, uniform
+leaves, tiles), uploads sources into the stacked device plane, runs the
+decision (eager launches or a replayed HIP graph, or the plain per-call
+form for a range of leaves) and returns the reconstruction and the per-leaf
+decisions.
+"""
+import ctypes
+
+import numpy as np
+
+CAND = np.dtype([('mode', 'u1'), ('delta', 'i1'), ('fi_mode', 'u1'), ('pad', 'u1')])
+CFG = np.dtype([('width', '<i4'), ('height', '<i4'), ('n_images', '<i4'), ('bd', '<i4'), ('leaf', '<i4'),
+                ('sb', '<i4'), ('tile_cols', '<i4'), ('tile_rows', '<i4'), ('qindex', '<i4'), ('k1', '<i4'),
+                ('k3', '<i4'), ('rdoq_level', '<i4'), ('txt_groups', '<i4'), ('txt_rate_th', '<i4'),
+                ('txt_satd_th', '<i4'), ('pad0', '<i4'), ('lambda', '<u8'), ('n_cands', '<i4'), ('pad', '<i4'),
+                ('cands', CAND, 64)])
+DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
+                     ('rate', '<i4'), ('mode_rate', '<i4'), ('dist', '<u8'), ('cost', '<u8')])
+MD_UNIT = np.dtype([('x', '<i4'), ('y', '<i4'), ('n_top_px', '<i2'), ('n_topright_px', '<i2'),
+                    ('n_left_px', '<i2'), ('n_bottomleft_px', '<i2'), ('edge_filter', 'u1'), ('filt_type', 'u1'),
+                    ('pad0', 'u1'), ('pad1', 'u1')])
+
+
+class WfInfo(ctypes.Structure):
+    _fields_ = [('d_src', ctypes.c_void_p), ('d_recon', ctypes.c_void_p), ('d_decisions', ctypes.c_void_p),
+                ('d_units', ctypes.c_void_p), ('n_leaves', ctypes.c_int32), ('n_steps', ctypes.c_int32),
+                ('max_leaves_per_step', ctypes.c_int32), ('n_types', ctypes.c_int32),
+                ('stacked_height', ctypes.c_int32), ('types', ctypes.c_uint8 * 16), ('bytes', ctypes.c_size_t)]
+
+
+def preset_candidates(fi=True):
+    """The intra candidates of the mode decision at presets 0-4 (mode_end
+    PAETH, angular level 1: every delta of the 8 directional modes) with
+    FILTER_DC the only filter-intra mode (fi_max 0): 13 + 48 + 1 = 62."""
+    c = [(m, 0, 5) for m in range(13)]
+    for m in range(1, 9):
+        for d in (-3, -2, -1, 1, 2, 3):
+            c.append((m, d, 5))
+    if fi:
+        c.append((0, 0, 0))
+    return c
+
+
+def qp_scaled_satd_th(th, cli_qp):
+    """div_round(th * w, d) with svt_aom_get_qp_based_th_scaling_factors
+    (rust/crates/svtav1-encoder/src/pd0.rs:185, leaf_funnel/mod.rs:7210)."""
+    import math
+    if cli_qp >= 46:
+        w, d = int((1.05 - math.exp(-(max(cli_qp, 40) - 35.0) / 10.0)) * 10000.0), 10000
+    else:
+        w, d = max(cli_qp, 10), 63
+    return (th * w + d // 2) // d
+
+
+def make_cfg(width, height, bd, qindex, lam, *, n_images=1, leaf=16, sb=64, tiles=(1, 1), k1=15, k3=4,
+             rdoq_level=1, txt_groups=6, txt_rate_th=250, txt_satd_th=0, cands=None):
+    cands = preset_candidates() if cands is None else cands
+    cfg = np.zeros(1, CFG)
+    c = cfg[0]
+    c['width'], c['height'], c['n_images'], c['bd'], c['leaf'], c['sb'] = width, height, n_images, bd, leaf, sb
+    c['tile_cols'], c['tile_rows'], c['qindex'], c['k1'], c['k3'] = tiles[0], tiles[1], qindex, k1, k3
+    c['rdoq_level'], c['txt_groups'], c['txt_rate_th'], c['txt_satd_th'] = rdoq_level, txt_groups, txt_rate_th, \
+        txt_satd_th
+    c['lambda'], c['n_cands'] = lam, len(cands)
+    for i, (m, d, f) in enumerate(cands):
+        c['cands'][i] = (m, d, f, 0)
+    return cfg
+
+
+class WavefrontMD:
+    def __init__(self, zav1, cfg):
+        self.z = zav1
+        self.cfg = np.ascontiguousarray(cfg, CFG)
+        L = zav1.lib()
+        h = ctypes.c_void_p()
+        rc = L.zav1_wf_create(self.cfg.ctypes.data, ctypes.byref(h))
+        if rc:
+            raise zav1.Zav1Error(rc, 'zav1_wf_create (%s)' % L.zav1_wf_last_error().decode())
+        self.h = h
+        self.info = WfInfo()
+        zav1.check(L.zav1_wf_info(h, ctypes.byref(self.info)), 'zav1_wf_info')
+        c = self.cfg[0]
+        self.W, self.H, self.n, self.bd = int(c['width']), int(c['height']), int(c['n_images']), int(c['bd'])
+        self.dt = np.uint8 if self.bd == 8 else np.uint16
+
+    @staticmethod
+    def footprint(zav1, cfg):
+        b = ctypes.c_size_t()
+        zav1.check(zav1.lib().zav1_wf_footprint(np.ascontiguousarray(cfg, CFG).ctypes.data, ctypes.byref(b)),
+                   'zav1_wf_footprint')
+        return b.value
+
+    def upload(self, images, stream=None):
+        """images: (n_images, height, width) samples."""
+        a = np.ascontiguousarray(images, self.dt).reshape(self.n * self.H, self.W)
+        self.z.check(self.z.lib().zav1_memcpy_h2d(self.info.d_src, a.ctypes.data, a.nbytes, stream), 'upload')
+
+    def run(self, use_graph=False, stream=None):
+        rc = self.z.lib().zav1_wf_run(self.h, int(use_graph), stream)
+        if rc:
+            raise self.z.Zav1Error(rc, 'zav1_wf_run (%s)' % self.z.lib().zav1_wf_last_error().decode())
+
+    def run_plain(self, first, count, stream=None):
+        rc = self.z.lib().zav1_wf_run_plain(self.h, first, count, stream)
+        if rc:
+            raise self.z.Zav1Error(rc, 'zav1_wf_run_plain (%s)' % self.z.lib().zav1_wf_last_error().decode())
+
+    def recon(self):
+        out = np.zeros((self.n * self.H, self.W), self.dt)
+        self.z.check(self.z.lib().zav1_memcpy_d2h(out.ctypes.data, self.info.d_recon, out.nbytes, None), 'recon')
+        return out.reshape(self.n, self.H, self.W)
+
+    def units(self):
+        n = self.info.n_leaves
+        out = np.zeros(n, MD_UNIT)
+        self.z.check(self.z.lib().zav1_memcpy_d2h(out.ctypes.data, self.info.d_units, out.nbytes, None), 'units')
+        return out
+
+    def decisions(self):
+        """Per leaf grid position: (n_images, height / leaf, width / leaf) decision records."""
+        n = self.info.n_leaves
+        d = np.zeros(n, DECISION)
+        self.z.check(self.z.lib().zav1_memcpy_d2h(d.ctypes.data, self.info.d_decisions, d.nbytes, None), 'dec')
+        u = self.units()
+        L = int(self.cfg[0]['leaf'])
+        grid = np.zeros((self.n, self.H // L, self.W // L), DECISION)
+        img = u['y'] // self.H
+        grid[img, (u['y'] - img * self.H) // L, u['x'] // L] = d
+        return grid
+
+    def close(self):
+        if self.h:
+            self.z.lib().zav1_wf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter teardown
+            pass
