@@ -359,6 +359,12 @@ int zav1_intra_txt_rate(int tx_size, int intra_dir, int tx_type);
  * fast loop sums them.  Host only; negative on a bad argument. */
 int zav1_intra_mode_rate(int w, int h, int above_ctx, int left_ctx, int mode, int delta, int fi_mode);
 
+/* The key-frame full mode-decision lambda of an allintra still at qindex
+ * (CLI qp cli_qp, bd 8 or 10).  Replaces kf_full_lambda_8bit / _bd10
+ * (rust/crates/svtav1-encoder/src/pd0.rs:223 / :294).  Host only; 0 on a bad
+ * argument. */
+uint64_t zav1_kf_full_lambda(int qindex, int cli_qp, int bd);
+
 /* Fused transform-unit evaluation of the tx-type search (the per-candidate
  * body of tx_unit_inner, rust/crates/svtav1-encoder/src/leaf_funnel/
  * tx_pipeline.rs:268, without RDOQ / rate): for every block (d_xy[2j],

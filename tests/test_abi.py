@@ -311,3 +311,19 @@ def test_oracle_cdef_dist_closed_form(oracle):
                     got = L.or_cdef_dist(plane.ctypes.data, hbd, off, 48, packed.ctypes.data, dl.ctypes.data, 3,
                                          bsize, cs, sub)
                     assert got == ref >> (2 * cs)
+
+
+def test_kf_full_lambda_matches_instrumented_values(zav1):
+    """zav1_kf_full_lambda against the values the reference instrumented
+    from the C library (pd0.rs:210-214: 25650 / 248207 / 1527856 at qindex
+    80 / 160 / 220, CLI qp 20 / 40 / 55; and the captures' FLC lambdas)."""
+    L = zav1.lib()
+    assert [L.zav1_kf_full_lambda(q, p, 8) for q, p in ((80, 20), (160, 40), (220, 55))] == [25650, 248207, 1527856]
+    # no lambda weight below qp 16, 175 from 56
+    base = L.zav1_kf_full_lambda(40, 10, 8)
+    assert L.zav1_kf_full_lambda(40, 16, 8) == (base * 150) >> 7
+    assert L.zav1_kf_full_lambda(40, 56, 8) == (base * 175) >> 7
+    # 10 bits: ~16x the 8-bit scale, a multiple of 16
+    l10 = L.zav1_kf_full_lambda(160, 40, 10)
+    assert l10 % 16 == 0 and 8 < l10 / 248207 < 24
+    assert L.zav1_kf_full_lambda(256, 40, 8) == 0

@@ -177,6 +177,7 @@ EXPORTS = {
                                      _vp, _vp],
     'zav1_noalloc_scratch_bytes': [],
     'zav1_intra_mode_rate': [_i32] * 7,
+    'zav1_kf_full_lambda': [_i32, _i32, _i32],
     'zav1_wf_footprint': [_vp, ctypes.POINTER(ctypes.c_size_t)],
     'zav1_wf_create': [_vp, ctypes.POINTER(_vp)],
     'zav1_wf_info': [_vp, _vp],
@@ -218,6 +219,7 @@ def lib():
         L.zav1_ctx_stream.restype = _vp
         L.zav1_noalloc_scratch_bytes.restype = ctypes.c_size_t
         L.zav1_wf_last_error.restype = ctypes.c_char_p
+        L.zav1_kf_full_lambda.restype = ctypes.c_uint64
         _lib = L
     return _lib
 

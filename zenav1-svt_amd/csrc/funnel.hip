@@ -15,10 +15,11 @@
 
 namespace zav1 {
 
-// One wave per unit (grid-stride over units).  The lanes compute the unit's
-// fast costs into LDS; lane 0 runs the first k outer passes of the exchange
-// sort (position p is final after pass p) over the candidates in generation
-// order; the lanes then write the survivors' transform-unit jobs.
+// One wave per unit (grid-stride over units).  Lane m computes candidate m's
+// fast cost; the first k outer passes of the exchange sort (position p is
+// final after pass p) run lane-parallel over the candidates in generation
+// order (wave_exchange_first_k); the lanes then write the survivors'
+// transform-unit jobs.
 constexpr int kMaxCands = 64;
 __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict__ satd,
                                                      const uint32_t* __restrict__ rate, uint64_t lambda,
@@ -28,38 +29,20 @@ __global__ __launch_bounds__(256) void k_fast_select(const uint32_t* __restrict_
                                                      uint8_t* __restrict__ sel,
                                                      uint64_t* __restrict__ sel_cost,
                                                      TxUnitJob* __restrict__ jobs_out) {
-  __shared__ uint64_t cst[4][kMaxCands];
   __shared__ uint8_t ord[4][kMaxCands];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint64_t* C = cst[wave];
   uint8_t* O = ord[wave];
   for (int u = blockIdx.x * 4 + wave; u < n_units; u += gridDim.x * 4) {
     const long base = (long)u * n_cands;
+    int idx = lane;
+    uint64_t cost = ~0ull;
     if (lane < n_cands) {
       const uint32_t s = satd[base + lane];
       const uint64_t r = rate ? rate[base + lane] : 0u;
-      C[lane] = s == kSatdInvalid ? ~0ull : ((r * lambda + 256) >> 9) + ((uint64_t)s << 11);
-      O[lane] = (uint8_t)lane;
+      cost = s == kSatdInvalid ? ~0ull : ((r * lambda + 256) >> 9) + ((uint64_t)s << 11);
     }
-    wave_sync();
-    if (lane == 0) {
-      for (int i = 0; i < k; i++) {
-        int mi = O[i];
-        uint64_t ci = C[mi];
-        for (int j = i + 1; j < n_cands; j++) {
-          const int mj = O[j];
-          const uint64_t cj = C[mj];
-          if (cj < ci) {  // swap positions i and j
-            O[j] = (uint8_t)mi;
-            mi = mj;
-            ci = cj;
-          }
-        }
-        O[i] = (uint8_t)mi;
-        sel[(long)u * k + i] = (uint8_t)mi;
-        if (sel_cost) sel_cost[(long)u * k + i] = ci;
-      }
-    }
+    wave_exchange_first_k(idx, cost, n_cands, k, lane, sel + (long)u * k, sel_cost ? sel_cost + (long)u * k : nullptr);
+    O[lane] = (uint8_t)idx;
     wave_sync();
     const int ux = unit_xy[(long)u * unit_stride], uy = unit_xy[(long)u * unit_stride + 1];
     for (int t = lane; t < k * types.n; t += 64) {

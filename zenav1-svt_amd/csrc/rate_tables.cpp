@@ -8,6 +8,7 @@
 //                           MdRates::txt_rate, intra arm (ext-tx sets
 //                           svtav1-entropy/src/coeff_c.rs:153-212)
 // Symbol costs follow av1_cost_symbol (svtav1-entropy/src/context.rs:388).
+#include <algorithm>
 #include <array>
 #include <cstdint>
 #include <cstring>
@@ -148,4 +149,23 @@ extern "C" int zav1_intra_mode_rate(int w, int h, int above_ctx, int left_ctx, i
     }
   }
   return rate;
+}
+
+// The key-frame full mode-decision lambda of an allintra still
+// (rust/crates/svtav1-encoder/src/pd0.rs:223 kf_full_lambda_8bit and :294
+// kf_full_lambda_bd10): rdmult (3.3 + 0.0015 q) q^2 from the frame's DC
+// quantizer, then at 8 bits the key-frame factor 150/128, at 10 bits the
+// rounding >> 4 and factor 128/128; the PSNR lambda weight (150 from CLI qp
+// 16, 175 from 56; none below 16) / 128; x16 at 10 bits.
+#include "av1_qlookup_tables.h"
+extern "C" uint64_t zav1_kf_full_lambda(int qindex, int cli_qp, int bd) {
+  if (qindex < 0 || qindex > 255 || (bd != 8 && bd != 10)) return 0;
+  const double q = bd == 10 ? DC_QLOOKUP_10[qindex] : DC_QLOOKUP_8[qindex];
+  int64_t rdmult = (int64_t)((3.3 + 0.0015 * q) * q * q);
+  if (bd == 10) rdmult = std::max<int64_t>((rdmult + 8) >> 4, 1);
+  rdmult = (rdmult * (bd == 10 ? 128 : 150)) >> 7;
+  uint64_t lambda = (uint32_t)rdmult;
+  const uint64_t weight = cli_qp >= 56 ? 175 : (cli_qp >= 16 ? 150 : 0);
+  if (weight) lambda = (lambda * weight) >> 7;
+  return bd == 10 ? lambda * 16 : lambda;
 }

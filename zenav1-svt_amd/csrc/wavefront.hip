@@ -151,8 +151,8 @@ __global__ __launch_bounds__(256) void k_wf_mds1_jobs(const MdUnit* __restrict__
 }
 
 // After MDS1: full costs of the survivors, the exchange sort's first k3 into
-// MDS3, and their tx-type search jobs (every type of the intra set, in
-// search order).  One thread per unit.
+// MDS3 (lane-parallel, one wave per unit, lane = survivor), and their
+// tx-type search jobs (every type of the intra set, in search order).
 __global__ __launch_bounds__(64) void k_wf_mds3_jobs(const MdUnit* __restrict__ units, int n,
                                                      const uint8_t* __restrict__ sel, int k1, int k3,
                                                      const WfCand* __restrict__ cands,
@@ -162,27 +162,29 @@ __global__ __launch_bounds__(64) void k_wf_mds3_jobs(const MdUnit* __restrict__ 
                                                      const uint8_t* __restrict__ types, int nt,
                                                      const int32_t* __restrict__ txt, int L, int64_t rdmult,
                                                      uint8_t* __restrict__ m3, TxUnitRdJob* __restrict__ out) {
-  const int u = blockIdx.x * 64 + threadIdx.x;
+  __shared__ uint8_t ord[64];
+  const int u = blockIdx.x, lane = threadIdx.x;
   if (u >= n) return;
-  uint64_t cost[64];
-  uint8_t ord[64];
-  for (int r = 0; r < k1; r++) {
-    const int c = sel[u * k1 + r];
-    cost[r] = rdc(lambda, (uint64_t)mode_rate[u * nc + c] + (uint32_t)rate1[u * k1 + r], dist1[u * k1 + r]);
+  int idx = lane;
+  uint64_t cost = ~0ull;
+  if (lane < k1) {
+    const int c = sel[u * k1 + lane];
+    cost = rdc(lambda, (uint64_t)mode_rate[u * nc + c] + (uint32_t)rate1[u * k1 + lane], dist1[u * k1 + lane]);
   }
-  exchange_first_k(cost, k1, k3, ord);
-  for (int i = 0; i < k3; i++) {
+  wave_exchange_first_k(idx, cost, k1, k3, lane, m3 + u * k3, nullptr);
+  ord[lane] = (uint8_t)idx;
+  wave_sync();
+  const int32_t x = units[u].x, y = units[u].y;
+  for (int q = lane; q < k3 * nt; q += 64) {
+    const int i = q / nt, t = q - i * nt;
     const int r = ord[i];
-    m3[u * k3 + i] = (uint8_t)r;
     const WfCand c = cands[sel[u * k1 + r]];
-    for (int t = 0; t < nt; t++) {
-      TxUnitRdJob j{};
-      j.x = units[u].x, j.y = units[u].y, j.pred_off = (uint32_t)(u * k1 + r) * (uint32_t)(L * L);
-      j.tx_type = types[t], j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.plane_type = 0;
-      j.txt_rate = txt[c.dir * 16 + types[t]];
-      j.rdmult = rdmult;
-      out[(u * k3 + i) * nt + t] = j;
-    }
+    TxUnitRdJob j{};
+    j.x = x, j.y = y, j.pred_off = (uint32_t)(u * k1 + r) * (uint32_t)(L * L);
+    j.tx_type = types[t], j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.plane_type = 0;
+    j.txt_rate = txt[c.dir * 16 + types[t]];
+    j.rdmult = rdmult;
+    out[(u * k3 + i) * nt + t] = j;
   }
 }
 
@@ -475,7 +477,7 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st) {
                       w->iscan, w->fscan, w->costs, kTxuQuantB, 0, 0, w->ws1, w->dist1, w->eob1, w->rate1, w->cul1,
                       nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_wf_mds3_jobs, dim3((unsigned)((U + 63) / 64)), dim3(64), 0, st, units, U, w->sel, w->k1, w->k3,
+  hipLaunchKernelGGL(k_wf_mds3_jobs, dim3((unsigned)U), dim3(64), 0, st, units, U, w->sel, w->k1, w->k3,
                      w->cands, w->rate, w->nc, w->dist1, w->rate1, c.lambda, w->d_types, w->nt, w->txt, L, w->rdmult,
                      w->m3, w->jobs3);
   const int n3 = U * w->k3 * w->nt;

@@ -425,4 +425,54 @@ __device__ __forceinline__ int32_t group_max_i32(int32_t v) {
   return v;
 }
 
+// The first k passes of the reference's exchange sort
+// (c_exchange_sort_by, rust/crates/svtav1-encoder/src/leaf_funnel/
+// rate_tables.rs:108: for position i, every later position j whose cost is
+// strictly below the running minimum swaps with position i), lane-parallel
+// in one wave: lane j holds position j's candidate `idx` and its `cost`
+// (n <= 64 lanes live, the rest hold ~0 costs and never move).  Pass i's
+// swaps move the running-minimum records: with the records at positions
+// i = r0 < r1 < ... < rm (the positions whose cost is below every cost in
+// [i, position)), position i ends with rm's candidate and each r(q) with
+// r(q-1)'s -- the sequential scan's result, found by an exclusive prefix
+// minimum over the lanes and a ballot.  After pass i, position i is final;
+// `out[i]` (lane-uniform) receives it.  Wave-uniform control flow.
+__device__ inline void wave_exchange_first_k(int& idx, uint64_t& cost, int n, int k, int lane, uint8_t* out,
+                                             uint64_t* out_cost) {
+  for (int i = 0; i < k; i++) {
+    const bool live = lane >= i && lane < n;
+    const uint64_t c = live ? cost : ~0ull;
+    // exclusive prefix minimum over lanes i .. lane-1
+    uint64_t pre = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t lo = __shfl_up((uint32_t)pre, d, 64), hi = __shfl_up((uint32_t)(pre >> 32), d, 64);
+      const uint64_t o = ((uint64_t)hi << 32) | lo;
+      if (lane >= d) pre = o < pre ? o : pre;
+    }
+    uint64_t ex;
+    {
+      const uint32_t lo = __shfl_up((uint32_t)pre, 1, 64), hi = __shfl_up((uint32_t)(pre >> 32), 1, 64);
+      ex = lane >= 1 ? (((uint64_t)hi << 32) | lo) : ~0ull;
+    }
+    const bool rec = live && (lane == i || c < ex);
+    const uint64_t mask = __ballot(rec);
+    const int last = 63 - __builtin_clzll(mask);
+    // the previous record below this lane
+    const uint64_t below = mask & ((lane ? (~0ull >> (64 - lane)) : 0ull));
+    const int prev = below ? 63 - __builtin_clzll(below) : lane;
+    const int from = lane == i ? last : (rec ? prev : lane);
+    const int ni = __shfl(idx, from, 64);
+    const uint32_t clo = __shfl((uint32_t)cost, from, 64), chi = __shfl((uint32_t)(cost >> 32), from, 64);
+    idx = ni;
+    cost = ((uint64_t)chi << 32) | clo;
+    const int fi = __shfl(idx, i, 64);
+    const uint32_t flo = __shfl((uint32_t)cost, i, 64), fhi = __shfl((uint32_t)(cost >> 32), i, 64);
+    if (lane == 0) {
+      out[i] = (uint8_t)fi;
+      if (out_cost) out_cost[i] = ((uint64_t)fhi << 32) | flo;
+    }
+  }
+}
+
 }  // namespace zav1
