@@ -1,32 +1,32 @@
-"""Benchmark: batched candidate RD (tx-type search) on one MI355X.
+"""Benchmark: the superblock-wavefront intra mode decision on one MI355X.
 
-Workload (one step): the 2048x2048 10-bit 4:2:0 synthetic still (luma: 1/f^2
-spectral noise in [64, 940] + 30 hard lines, seed 2000), at qp 20 and qp 40
-(qindex 80 / 160).  For each qp, all 19 transform sizes tile the frame and
-every block is evaluated with every transform type legal for its size
-(DC prediction from the source neighbours -> residual -> forward transform
--> 64-point fold -> fp quantize -> distortion / eob / level sum), one fused
-launch per (size, qp): 38 launches per step, ~4.4 M candidates.
+Workload (one step): a batch of N synthetic 4096x4096 10-bit luma planes
+(1/f^2 spectral noise in [64, 940] + 30 hard lines, seeds 2000 + i) in flight
+together, qp 40 (qindex 160), preset-4 funnel: 62 intra candidates per 16x16
+leaf (13 modes, 8 x 6 angle deltas, FILTER_DC) -> prediction + Hadamard SATD
+-> fast cost + exchange sort (15 survivors) -> DCT_DCT quantize_b units with
+exact coefficient rate (MDS1) -> 4 survivors x the intra tx-type groups with
+the rate and SATD gates -> RDOQ, exact rate, spatial distortion (MDS3) ->
+full-cost winner, reconstruction committed for the next leaves.  Every SB
+anti-diagonal step of every image runs as one chain of launches; the whole
+batch is one HIP graph, replayed.  Default-CDF rate tables, key-frame
+lambda.  Decisions and reconstruction are byte-identical to the oracle's
+sequential restatement (tests) and to the plain per-call form (checked here
+on a sample of leaves).
 
-value = luma megapixels x qps swept per second (inputs resident in HBM).
-Also reported: the plain per-call GPU form (host pointers, one launch per
-function-table call: fwd + quantize + sse per candidate) on a bounded sample,
-the roofline of the dominant kernel from HIP events over the timed region, and
-the oracle (CPU restatement) timed on a bounded crop of the same workload.
+value = luma megapixels decided per second over the batch (N x 16.78 MP per
+step; inputs resident in HBM).  Also reported: the plain per-call form on a
+bounded sample of leaves (one launch per elementary call, selections on the
+host), the per-stage device time of an eager pass (HIP events), the
+roofline of the dominant kernel, and the oracle (CPU restatement) on a
+bounded sample, multi-threaded over crops.
 
-A third object, md_funnel, times the intra mode-decision funnel's MDS0 ->
-MDS1 chain over every 16x16 unit of the config2 frame (61 candidates per
-unit: prediction, Hadamard SATD, fast cost and exchange-sort top-3 on the
-device, DCT_DCT quantize_b transform units of the survivors).
+Extra stages (--extras): lf (loop-filter stage on the 4096x4096 8-bit
+reconstruction), rdoq (the trellis over 262 144 16x16 candidates), txsearch
+(exhaustive DC-predicted tx-type search, 2048x2048 10-bit), config4 (one
+tile column of the 8192x4096 frame through the tx-type search), serving.
 
-A second object, loopfilter_stage, times the loop-filter stage on the
-4096x4096 8-bit 4:2:0 reconstruction (1/f noise + 30% 8x8 mean-flattened
-blocks, seed 3000): deblocking of all planes, CDEF direction + 64-strength SSE
-tables per 64x64 block, Wiener M/H + unit SSE and self-guided statistics per
-restoration unit (128 luma / 64 chroma), with its own roofline and oracle
-baseline.
-
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--images N] [--no-graph]
 """
 import argparse
 import ctypes
@@ -136,6 +136,66 @@ def measured_traffic(kernel_prefix):
         if name.startswith(kernel_prefix):
             return int(k['fetch_kib'] * 1024 * d['fetch_scale'] + k['write_kib'] * 1024 * d['write_scale'])
     return None
+
+
+def _short_kernel_name(name):
+    """'void zav1::k_rdoq_wave<2>(int const*, ...)' -> 'k_rdoq_wave<2>'."""
+    for ns in ('(anonymous namespace)::', 'zav1::'):
+        name = name.replace(ns, '')
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):
+        if ch == '<':
+            depth += 1
+        elif ch == '>':
+            depth -= 1
+        elif ch == '(' and depth == 0:
+            cut = i
+            break
+    n = name[:cut]
+    if n.startswith('void '):
+        n = n[5:]
+    return n.strip()
+
+
+def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, merge_into=None):
+    """Per-kernel mean FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from
+    rocprofv3 --pmc CSV output directories, with the byte scale of each
+    counter from the 64 MiB device copy runs (--copy-calibration); merged
+    over the kernels of an existing summary when given."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        acc = {}
+        for f in glob.glob(os.path.join(d, '**', '*counter_collection.csv'), recursive=True):
+            for row in csv.DictReader(open(f)):
+                if row.get('Counter_Name') != counter:
+                    continue
+                k = _short_kernel_name(row['Kernel_Name'])
+                a = acc.setdefault(k, [0.0, 0, 0])
+                a[0] += float(row['Counter_Value'])
+                a[1] += 1
+                a[2] = max(a[2], int(float(row.get('Grid_Size', 0) or 0)))
+        return {k: (v[0] / v[1], v[1], v[2]) for k, v in acc.items()}
+
+    fe, wr = per_kernel(fetch_dir, 'FETCH_SIZE'), per_kernel(write_dir, 'WRITE_SIZE')
+    cf, cw = per_kernel(cal_fetch_dir, 'FETCH_SIZE'), per_kernel(cal_write_dir, 'WRITE_SIZE')
+    copy = [k for k in cf if 'copyBuffer' in k]
+    if not copy:
+        raise SystemExit('no copy kernel in the calibration runs')
+    n_kib = (64 << 20) / 1024.0
+    out = json.load(open(merge_into)) if merge_into else {'kernels': {}}
+    out['fetch_scale'] = round(n_kib / cf[copy[0]][0], 4)
+    out['write_scale'] = round(n_kib / cw[[k for k in cw if 'copyBuffer' in k][0]][0], 4)
+    out['calibration'] = '__amd_rocclr_copyBuffer of 64 MiB: FETCH_SIZE %.1f KiB, WRITE_SIZE %.1f KiB per dispatch' % (
+        cf[copy[0]][0], cw[[k for k in cw if 'copyBuffer' in k][0]][0])
+    for k in fe:
+        if k in wr and 'copyBuffer' not in k:
+            e = out['kernels'].get(k, {})
+            e.update({'fetch_kib': round(fe[k][0], 1), 'write_kib': round(wr[k][0], 1), 'dispatches': fe[k][1],
+                      'grid_size': fe[k][2], 'per_dispatch': 'mean'})
+            out['kernels'][k] = e
+    return out
 
 
 def copy_calibration(zav1):
@@ -273,119 +333,6 @@ def md_fast_ops_per_unit(cands, w, h):
     return tot
 
 
-def md_funnel_stage(zav1, synth, args, cpu=True):
-    """MDS0 -> MDS1 of the intra funnel over every 16x16 unit of the 2048x2048
-    10-bit frame: 61 candidates per unit (13 modes + angle deltas), SATD,
-    fast cost + exchange-sort survivors (k = 3), DCT_DCT quantize_b units."""
-    L = zav1.lib()
-    fm = _load('zav1_funnel', os.path.join(PKG, 'funnel.py'))
-    S, bw = 2048, 16
-    src = synth.config2_luma(S, S, seed=2000)
-    rng = np.random.default_rng(2001)
-    recon = np.clip(src.astype(np.int64) + rng.integers(-4, 5, src.shape), 0, 1023).astype(np.uint16)
-    cands = fm.angle_candidates()
-    nc = len(cands)
-    rates = rng.integers(100, 1500, (1, nc)).astype(np.uint32)
-    qi = synth.qp_to_qindex(40)
-    lam, k = 90000, 3
-    f = fm.IntraFunnel(zav1, src, recon, 10, bw, bw, qi, k=k, cands=cands, lam=lam, rates=rates)
-    sp = ctypes.c_void_p()
-    zav1.check(L.zav1_stream_create(ctypes.byref(sp)), 'stream_create')
-    stream = sp.value
-    for _ in range(2):
-        f.run(stream)
-    zav1.sync(stream)
-    phases = f.stages()
-    ev = []
-    for _ in range(len(phases) + 1):
-        e = ctypes.c_void_p()
-        L.zav1_event_create(ctypes.byref(e))
-        ev.append(e.value)
-    phase_ms = np.zeros(len(phases))
-    steps = args.lf_steps
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        L.zav1_event_record(ev[0], stream)
-        for i, ph in enumerate(phases):
-            f.run(stream, stages=(ph,))
-            L.zav1_event_record(ev[i + 1], stream)
-        zav1.sync(stream)
-        for i in range(len(phases)):
-            ms = ctypes.c_float()
-            L.zav1_event_elapsed_ms(ev[i], ev[i + 1], ctypes.byref(ms))
-            phase_ms[i] += ms.value
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1000.0 / steps
-    phase_ms /= steps
-    n = bw * bw
-    npred = f.predictions()
-    ntx = f.n_tx
-    nu = f.n_units
-    # fused fast loop: algorithmic int ops per launch (per candidate sample:
-    # prediction by mode class + residual + Hadamard/SATD), and compulsory
-    # HBM bytes (source block + reference samples per unit, u32 SATD per candidate)
-    ops_unit = md_fast_ops_per_unit(cands, bw, bw)
-    satd_bytes = nu * (2 * n + (2 * bw + 2 * bw + 1) * 2 + 20) + npred * 4
-    dom = phases[int(np.argmax(phase_ms))]
-    dom_s = phase_ms[phases.index(dom)] / 1000.0
-    if dom == 'satd':
-        roof = {'bound': 'valu', 'kernel': 'k_md_fast_satd<uint16_t,16>', 'unit': 'Tops/s',
-                'achieved': round(nu * ops_unit / dom_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
-                'frac': round(nu * ops_unit / dom_s / 1e12 / VALU_PEAK_TOPS, 4),
-                'traffic': measured_traffic('k_md_fast_satd<unsigned short, 16, 16>'),
-                'avg_launch_us': round(dom_s * 1e6, 2), 'algorithmic_ops_per_launch': int(nu * ops_unit),
-                'hbm_algorithmic_GBs': round(satd_bytes / dom_s / 1e9, 2)}
-    else:
-        bytes_of = {'pred': nu * k * (2 * n + (4 * bw + 1) * 2 + 20),
-                    'select': nu * nc * 8 + ntx * 16 + nu * k * 9,
-                    'tx': ntx * (2 * n + 2 * n + 16 + 13)}
-        gbs = bytes_of[dom] / dom_s / 1e9
-        roof = {'bound': 'hbm', 'kernel': {'pred': 'k_intra_pred<uint16_t,SurvivorJobs>', 'select': 'k_fast_select',
-                                           'tx': 'k_txunit<16,16,uint16_t>'}[dom],
-                'unit': 'GB/s', 'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 4),
-                'traffic': None, 'avg_launch_us': round(dom_s * 1e6, 2),
-                'algorithmic_bytes_per_launch': int(bytes_of[dom])}
-    out = {'workload': 'intra funnel MDS0 -> MDS1: 2048x2048 10-bit luma (config2), all 16x16 units, 61 '
-                       'candidates each (13 modes + angle deltas, synthetic neighbour canvas and mode rates), '
-                       'fused prediction + Hadamard SATD on chip, fast cost + exchange-sort top-3, survivors '
-                       'predicted again, DCT_DCT quantize_b units at qp 40',
-           'value': round(S * S / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s', 'steps': steps,
-           'ms_per_step': round(ms_step, 3), 'candidates_per_step': int(npred), 'tx_units_per_step': int(ntx),
-           'phase_ms': {ph: round(float(phase_ms[i]), 4) for i, ph in enumerate(phases)},
-           'roofline': roof}
-    # the two-pass form (every prediction through HBM), same results, for comparison
-    f2 = fm.IntraFunnel(zav1, src, recon, 10, bw, bw, qi, k=k, cands=cands, lam=lam, rates=rates, fused=False)
-    f2.run(stream)
-    zav1.sync(stream)
-    t2 = time.perf_counter()
-    for _ in range(steps):
-        f2.run(stream)
-    zav1.sync(stream)
-    out['two_pass_ms_per_step'] = round((time.perf_counter() - t2) * 1000.0 / steps, 3)
-    del f2
-    if cpu:
-        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
-        import oracle as O
-        c = S
-        xy = fm.unit_grid(c, c, bw, bw)
-        jobs = np.zeros(len(xy) * nc, O.INTRA_JOB)
-        for u, (x, y) in enumerate(xy.tolist()):
-            nt, ntr, nl, nbl = fm.raster_availability(x, y, bw, bw, c)
-            for m, (mode, delta, fi) in enumerate(cands):
-                jobs[u * nc + m] = (x, y, bw, bw, mode, delta, fi, 1, 0, 0, nt, ntr, nl, nbl, 0, 0)
-        threads = min(os.cpu_count() or 1, 16)
-        os.environ['OMP_NUM_THREADS'] = str(threads)
-        s_c, r_c = np.ascontiguousarray(src[:c, :c]), np.ascontiguousarray(recon[:c, :c])
-        tc0 = time.perf_counter()
-        O.md_funnel(s_c, r_c, 10, jobs, nc, rates, lam, k, f.tx_size, 0, qi)
-        tc1 = time.perf_counter()
-        out['cpu_baseline'] = {'value': round(c * c / 1e6 / (tc1 - tc0), 4), 'unit': 'MP/s', 'cores': threads,
-                               'kind': 'port', 'sample': '%dx%d (whole) frame, same chain (oracle C, OpenMP '
-                                                         'over units)' % (c, c),
-                               'seconds': round(tc1 - tc0, 2)}
-    return out
-
-
 def rdoq_stage(zav1, synth, args, cpu=True):
     """RDOQ (zav1_rdoq_batch) of every tx-type candidate of every 16x16 block
     of the config-2 frame (2048x2048 10-bit): 16 384 blocks x 16 types =
@@ -443,21 +390,35 @@ def rdoq_stage(zav1, synth, args, cpu=True):
         zav1.rdoq_batch(ts, qi, 10, d_coef, d_q, d_dq, d_eob, d_jobs, nj, d_costs, d_scans, stream=stream)
 
     steps = max(5, args.steps // 5)
-    for _ in range(2):
-        restore()
-        launch()
-    zav1.sync(stream)
-    tot_ms = 0.0
-    for _ in range(steps):
-        restore()
-        L.zav1_event_record(ev[0].value, stream)
-        launch()
-        L.zav1_event_record(ev[1].value, stream)
-        zav1.sync(stream)
-        ms = ctypes.c_float()
-        L.zav1_event_elapsed_ms(ev[0].value, ev[1].value, ctypes.byref(ms))
-        tot_ms += ms.value
-    ms_step = tot_ms / steps
+
+    def time_form(threshold):
+        # the trellis form by its batch-size threshold (0: lane form; above
+        # the batch: wave form), restored afterwards
+        prev = L.zav1_rdoq_wave_max(ts, threshold)
+        try:
+            for _ in range(2):
+                restore()
+                launch()
+            zav1.sync(stream)
+            tot = 0.0
+            for _ in range(steps):
+                restore()
+                L.zav1_event_record(ev[0].value, stream)
+                launch()
+                L.zav1_event_record(ev[1].value, stream)
+                zav1.sync(stream)
+                ms = ctypes.c_float()
+                L.zav1_event_elapsed_ms(ev[0].value, ev[1].value, ctypes.byref(ms))
+                tot += ms.value
+            return tot / steps, d_q.to_host()
+        finally:
+            L.zav1_rdoq_wave_max(ts, prev)
+
+    wave_ms, q_wave = time_form(nj)
+    lane_ms, q_lane = time_form(0)
+    assert np.array_equal(q_wave, q_lane), 'trellis forms disagree'
+    form = 'wave' if wave_ms < lane_ms else 'lane'
+    ms_step = min(wave_ms, lane_ms)
     # coefficient rate of every candidate after the trellis (one launch)
     rjobs = np.zeros(nj, zav1.RATE_JOB)
     rjobs['coeff'] = jobs['coeff']
@@ -488,7 +449,9 @@ def rdoq_stage(zav1, synth, args, cpu=True):
            'mean_eob_in': round(float(eob0.mean()), 2), 'mean_eob_out': round(float(eob1.mean()), 2),
            'levels_changed_frac': round(changed / max(1, int((q0 != 0).sum())), 4),
            'coeff_rate_ms': round(rate_ms, 4), 'coeff_rate_blocks_per_s': round(nj / (rate_ms / 1000.0)),
-           'roofline': {'bound': 'latency', 'kernel': 'k_rdoq<TX_16X16>', 'unit': 'GB/s',
+           'lane_form_ms': round(lane_ms, 4), 'wave_form_ms': round(wave_ms, 4),
+           'roofline': {'bound': 'latency', 'kernel': ('k_rdoq_wave<TX_16X16>' if form == 'wave' else
+                                                       'k_rdoq<TX_16X16>'), 'unit': 'GB/s',
                         'achieved': round(alg_bytes / (ms_step / 1000.0) / 1e9, 2), 'peak': 8000.0,
                         'frac': round(alg_bytes / (ms_step / 1000.0) / 1e9 / 8000.0, 4),
                         'traffic': measured_traffic('k_rdoq<2>'), 'avg_launch_us': round(ms_step * 1000.0, 2),
@@ -570,22 +533,260 @@ def serving_stage(zav1, synth, tsm, args):
                                    '(one quantizer)'}}
 
 
+def cpu_info():
+    """Host CPU model and the threads this process may run on."""
+    model = ''
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, sorted(os.sched_getaffinity(0))
+
+
+def oracle_cflags():
+    try:
+        for line in open(os.path.join(ROOT, 'oracle', 'Makefile')):
+            if line.startswith('CFLAGS'):
+                return line.split('=', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args):
+    """The oracle's sequential restatement (or_wf_frame) of the same decision
+    on a bounded sample: 16 crops of 512x512 (the top-left 2048x2048 of the
+    first image), each decided as its own frame, one per thread, the
+    threads pinned to the first CPUs this process may use; one untimed
+    warm-up, then the median of 5 timed runs."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle as O
+    C = 512
+    crops = [np.ascontiguousarray(imgs[0, y:y + C, x:x + C]) for y in range(0, 4 * C, C) for x in range(0, 4 * C, C)]
+    cfg = wf.make_cfg(C, C, 10, qidx, lam, n_images=1, leaf=16, k1=15, k3=4, rdoq_level=1,
+                      txt_satd_th=wf.qp_scaled_satd_th(15, qp))
+    model, cpus = cpu_info()
+    threads = min(16, len(cpus), len(crops))
+    keep = os.sched_getaffinity(0)
+    os.sched_setaffinity(0, set(cpus[:threads]))
+    try:
+        def one(c):
+            return O.wf_frame(cfg, c[None])
+        times = []
+        with ThreadPoolExecutor(threads) as ex:
+            for rep in range(6):
+                t0 = time.perf_counter()
+                list(ex.map(one, crops))
+                if rep:
+                    times.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, keep)
+    med = float(np.median(times))
+    return {'value': round(len(crops) * C * C / 1e6 / med, 4), 'unit': 'MP/s', 'cores': threads, 'kind': 'port',
+            'sample': '%d crops of %dx%d (the top-left 2048x2048 of image 0), each decided as its own frame by '
+                      'the oracle C restatement, one crop per thread; 1 warm-up + median of 5' % (len(crops), C, C),
+            'seconds_median': round(med, 3), 'seconds_all': [round(t, 3) for t in times], 'cpu_model': model,
+            'pinned_cpus': cpus[:threads], 'compiler': 'gcc ' + (oracle_cflags() or '?')}
+
+
+def wavefront_headline(zav1, synth, args):
+    wf = _load('zav1_wavefront', os.path.join(PKG, 'wavefront.py'))
+    L = zav1.lib()
+    S, n_img, qp = args.size, args.images, 40
+    qidx = synth.qp_to_qindex(qp)
+    lam = int(L.zav1_kf_full_lambda(qidx, qp, 10))
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
+        imgs = np.stack(list(ex.map(lambda i: synth.config2_luma(S, S, seed=2000 + i), range(n_img))))
+    cfg = wf.make_cfg(S, S, 10, qidx, lam, n_images=n_img, leaf=16, k1=15, k3=4, rdoq_level=1,
+                      txt_satd_th=wf.qp_scaled_satd_th(15, qp))
+    md = wf.WavefrontMD(zav1, cfg)
+    md.upload(imgs)
+    st = zav1.stream_create()
+    if args.profile_pass:  # one eager pass and nothing else, for profilers
+        md.run(False, st)
+        zav1.stream_sync(st)
+        out = {'profile_pass': 'wavefront MD, %d x %dx%d 10-bit, eager' % (n_img, S, S), 'steps': md.info.n_steps,
+               'leaves': md.info.n_leaves}
+        md.close()
+        return out
+    use_graph = not args.no_graph
+    for _ in range(max(1, args.warmup)):
+        md.run(use_graph, st)
+    zav1.stream_sync(st)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        md.run(use_graph, st)
+    zav1.stream_sync(st)
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1000.0 / args.steps
+    mp = n_img * S * S / 1e6
+    value = mp / (ms_step / 1000.0)
+    rec, dec = md.recon(), md.raw_decisions()
+    # the eager form of the same chain must give the same bytes (not run
+    # with --no-graph, which keeps graph capture out of profiled runs)
+    same_forms = None
+    if use_graph:
+        md.run(False, st)
+        zav1.stream_sync(st)
+        same_forms = bool(np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec))
+    # per-stage device time of one eager pass (HIP events on the chain's stream)
+    stage_ms = md.run_timed(st)
+    assert np.array_equal(md.raw_decisions(), dec)
+    info = md.info
+    n_launch = info.n_steps  # one launch of each kernel per step (every step holds leaves)
+    # plain per-call form on a bounded sample: the first P leaves in coding
+    # order, recomputed from the decided neighbours; their records must come
+    # back equal to the batched form's
+    P = args.plain_leaves
+    mark = dec.copy()
+    mark.view(np.uint8)[:] = 0xFF
+    md.set_raw_decisions(mark)
+    tp0 = time.perf_counter()
+    md.run_plain(0, P, st)
+    zav1.stream_sync(st)
+    tp1 = time.perf_counter()
+    got = md.raw_decisions()
+    redone = np.nonzero(got.view(np.uint8).reshape(len(got), -1).min(axis=1) != 0xFF)[0]
+    plain_same = bool(len(redone) == P and np.array_equal(got[redone], dec[redone])
+                      and np.array_equal(md.recon(), rec))
+    md.set_raw_decisions(dec)
+    plain_s_leaf = (tp1 - tp0) / P
+    plain_mp_s = 16 * 16 / 1e6 / plain_s_leaf
+    # roofline of the dominant stage's kernel
+    dom = max(stage_ms, key=stage_ms.get)
+    dom_s = stage_ms[dom] / 1000.0 / n_launch
+    n_leaves = info.n_leaves
+    n_types = info.n_types
+    if dom == 'mds0_satd':
+        ops = md_fast_ops_per_unit(wf.preset_candidates(), 16, 16) * n_leaves / n_launch
+        roof = {'bound': 'valu', 'kernel': 'k_md_fast_satd<uint16_t,16,16>', 'unit': 'Tops/s',
+                'achieved': round(ops / dom_s / 1e12, 4), 'peak': VALU_PEAK_TOPS,
+                'frac': round(ops / dom_s / 1e12 / VALU_PEAK_TOPS, 5),
+                'traffic': measured_traffic('k_md_fast_satd<unsigned short, 16, 16>'),
+                'algorithmic_ops_per_launch': int(ops)}
+    else:
+        # the trellis: per job the block's transform coefficients, levels and
+        # dequantized values are read (12 B per coefficient) and the job
+        # record and eob (26 B); jobs = leaves x survivors x tx types
+        jobs = n_leaves * 4 * n_types
+        per_launch = jobs * (256 * 12 + 26) / n_launch
+        gbs = per_launch / dom_s / 1e9
+        roof = {'bound': 'hbm', 'kernel': 'k_rdoq_wave<TX_16X16>' if dom == 'mds3_rdoq' else dom, 'unit': 'GB/s',
+                'achieved': round(gbs, 3), 'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 6),
+                'traffic': measured_traffic('k_rdoq_wave<2>'), 'algorithmic_bytes_per_launch': int(per_launch),
+                'note': 'latency-bound: one sequential trellis walk per block, ~%d blocks per launch'
+                        % (jobs // n_launch)}
+    roof['stage'] = dom
+    roof['avg_launch_us'] = round(dom_s * 1e6, 2)
+    roof['launches'] = n_launch
+    out = {
+        'metric': 'MP/s through the superblock-wavefront intra mode decision (luma), batched vs plain per-call GPU',
+        'value': round(value, 2), 'unit': 'MP/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(ms_step, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'int32', 'data': 'synthetic',
+        'config': {'workload': 'wavefront MD: %d x %dx%d 10-bit luma in flight, qp 40, preset-4 funnel '
+                               '(62 candidates, 15 -> 4 survivors, 6 tx-type groups, RDOQ), 16x16 leaves, SB64, '
+                               '1 tile; %s' % (n_img, S, S, 'one HIP graph per batch' if use_graph else 'eager'),
+                   'images_in_flight': n_img, 'width': S, 'height': S, 'bit_depth': 10, 'qp': qp, 'qindex': qidx,
+                   'lambda': lam, 'leaf': 16, 'candidates': 62, 'k1': 15, 'k3': 4, 'tx_types': n_types,
+                   'wavefront_steps': info.n_steps, 'max_leaves_per_step': info.max_leaves_per_step,
+                   'device_bytes': int(info.bytes)},
+        'forms_identical': {'graph_vs_eager': same_forms, 'plain_sample_vs_batched': plain_same},
+        'stage_ms': {k: round(v, 3) for k, v in stage_ms.items()},
+        'eager_timed_pass_ms': round(sum(stage_ms.values()), 3),
+        'roofline': roof,
+        'plain_gpu': {'value': round(plain_mp_s, 4), 'unit': 'MP/s',
+                      'sample': 'the first %d leaves in coding order (one launch per elementary call, selections '
+                                'on the host)' % P,
+                      'ms_per_leaf': round(plain_s_leaf * 1e3, 3),
+                      'batched_over_plain': round(value / plain_mp_s, 1)},
+    }
+    if not args.no_cpu:
+        out['cpu_baseline'] = wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args)
+    md.close()
+    zav1.stream_destroy(st)
+    return out
+
+
+def txsearch_stage(zav1, synth, tsm, opsd, args):
+    """Exhaustive DC-predicted tx-type search over the 2048x2048 10-bit
+    config-2 frame at qp 20 and 40 (19 sizes x the intra tx-type sets, one
+    fused launch per size and quantizer pair)."""
+    L = zav1.lib()
+    S = 2048
+    plane = synth.config2_luma(S, S, seed=2000)
+    qps = (20, 40)
+    qis = [synth.qp_to_qindex(q) for q in qps]
+    sweep = tsm.TxSearchSweep(zav1, plane, 10, qis)
+    n_cand = sweep.candidates_per_step()
+    stream = zav1.stream_create()
+    for _ in range(3):
+        sweep.run(stream)
+    zav1.sync(stream)
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        L.zav1_event_create(ctypes.byref(e))
+    per_size_ms = {}
+    steps = 10
+    for ts in range(19):
+        L.zav1_event_record(ev[0].value, stream)
+        for _ in range(steps):
+            sweep.run(stream, sizes=(ts,))
+        L.zav1_event_record(ev[1].value, stream)
+        zav1.sync(stream)
+        ms = ctypes.c_float()
+        L.zav1_event_elapsed_ms(ev[0].value, ev[1].value, ctypes.byref(ms))
+        per_size_ms[ts] = ms.value / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sweep.run(stream)
+    zav1.sync(stream)
+    ms_step = (time.perf_counter() - t0) * 1000.0 / steps
+    dom = max(per_size_ms, key=per_size_ms.get)
+    types = tsm.legal_types(dom)
+    w, h = tsm.TX_DIMS[dom]
+    nblk = (S // w) * (S // h)
+    n_launch = tsm.launches(dom, len(qps))
+    q_per_launch = len(qps) // n_launch
+    avg_launch_s = per_size_ms[dom] / 1000.0 / n_launch
+    ops_launch = nblk * block_ops(tsm, dom, types, opsd, q_per_launch)
+    kname = {'lane': 'k_txsearch_lane', 'rows': 'k_txsearch_rows', 'group': 'k_txsearch'}[tsm.kernel_form(dom)]
+    return {'workload': 'exhaustive DC-predicted tx-type search: 2048x2048 10-bit luma (config 2), qp 20 + 40, 19 '
+                        'tx sizes x the intra ext-tx sets, fused DC-pred/fwd/fold/fp-quant/dist',
+            'value': round(S * S / 1e6 / (ms_step / 1000.0), 2), 'unit': 'MP/s (both quantizers per pixel)',
+            'ms_per_step': round(ms_step, 3), 'candidates_per_step': n_cand,
+            'per_size_ms': {str(tsm.TX_DIMS[t]): round(per_size_ms[t], 4) for t in range(19)},
+            'roofline': {'bound': 'valu', 'kernel': '%s<%d,%d>' % (kname, w, h), 'unit': 'Tops/s',
+                         'achieved': round(ops_launch / avg_launch_s / 1e12, 3), 'peak': VALU_PEAK_TOPS,
+                         'frac': round(ops_launch / avg_launch_s / 1e12 / VALU_PEAK_TOPS, 4),
+                         'avg_launch_us': round(avg_launch_s * 1e6, 2),
+                         'algorithmic_ops_per_launch': int(ops_launch)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=50)
-    ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--size', type=int, default=2048)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--size', type=int, default=4096)
+    ap.add_argument('--images', type=int, default=64)
+    ap.add_argument('--no-graph', action='store_true', help='eager launches instead of the graph (profiling)')
+    ap.add_argument('--profile-pass', action='store_true',
+                    help='one eager pass of the headline workload only (for rocprofv3 runs)')
+    ap.add_argument('--plain-leaves', type=int, default=48)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--no-plain', action='store_true')
-    ap.add_argument('--no-lf', action='store_true')
-    ap.add_argument('--no-funnel', action='store_true')
+    ap.add_argument('--extras', default='lf,rdoq', help='comma list of lf, rdoq, txsearch, config4, serving')
     ap.add_argument('--copy-calibration', action='store_true')
+    ap.add_argument('--pmc-summary', nargs=4, metavar=('FETCH_DIR', 'WRITE_DIR', 'CAL_FETCH_DIR', 'CAL_WRITE_DIR'),
+                    help='print the per-kernel HBM traffic summary of rocprofv3 --pmc CSV runs (merged over '
+                         'profiles/pmc_traffic.json)')
     ap.add_argument('--lf-steps', type=int, default=10)
     ap.add_argument('--streams', type=int, default=2)
-    ap.add_argument('--no-config4', action='store_true')
-    ap.add_argument('--no-serving', action='store_true')
-    ap.add_argument('--no-rdoq', action='store_true')
     ap.add_argument('--serve-n', type=int, default=96)
     ap.add_argument('--serve-ctx', type=int, default=4)
     ap.add_argument('--serve-load', type=float, default=0.6)
@@ -594,198 +795,42 @@ def main():
         print('bench.py: this build measures one GPU; multi-GPU runs are not part of it '
               '(requested --gpus %d)' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.size % 64 or args.size < 512 or args.images < 1 or args.steps < 1 or args.plain_leaves < 1:
+        print('bench.py: --size must be a multiple of 64 from 512, --images/--steps/--plain-leaves >= 1',
+              file=sys.stderr)
+        sys.exit(2)
 
+    if args.pmc_summary:
+        prev = os.path.join(ROOT, 'profiles', 'pmc_traffic.json')
+        print(json.dumps(pmc_summary(*args.pmc_summary, merge_into=prev if os.path.exists(prev) else None),
+                         indent=1))
+        return
     zav1 = _load('zav1', os.path.join(PKG, '__init__.py'))
     if args.copy_calibration:
         zav1.init(0)
         copy_calibration(zav1)
         return
     synth = _load('zav1_synth', os.path.join(PKG, 'synth.py'))
-    tsm = _load('zav1_txsearch', os.path.join(PKG, 'txsearch.py'))
-    opsd = json.load(open(os.path.join(PKG, 'csrc', 'txfm1d_ops.json')))
     zav1.init(0)
-    L = zav1.lib()
-
-    S = args.size
-    plane = synth.config2_luma(S, S, seed=2000)
-    qps = (20, 40)
-    qis = [synth.qp_to_qindex(q) for q in qps]
-    sweep = tsm.TxSearchSweep(zav1, plane, 10, qis)
-    n_cand = sweep.candidates_per_step()
-
-    # explicit stream: kernels and events share it
-    st = ctypes.c_void_p()
-    zav1.check(L.zav1_stream_create(ctypes.byref(st)), 'stream_create')
-    stream = st.value
-
-    for _ in range(args.warmup):
-        sweep.run(stream)
-    zav1.sync(stream)
-
-    def event():
-        e = ctypes.c_void_p()
-        zav1.check(L.zav1_event_create(ctypes.byref(e)), 'event_create')
-        return e.value
-
-    def elapsed(a, b):
-        ms = ctypes.c_float()
-        zav1.check(L.zav1_event_elapsed_ms(a, b, ctypes.byref(ms)), 'event_elapsed')
-        return ms.value
-
-    # per-size breakdown (untimed pass, HIP events around each size's launch)
-    cal_steps = 3
-    evs = {ts: (event(), event()) for ts in range(19)}
-    per_size_ms = {ts: 0.0 for ts in range(19)}
-    for _ in range(cal_steps):
-        for ts in range(19):
-            L.zav1_event_record(evs[ts][0], stream)
-            sweep.run(stream, sizes=(ts,))
-            L.zav1_event_record(evs[ts][1], stream)
-        zav1.sync(stream)
-        for ts in range(19):
-            per_size_ms[ts] += elapsed(*evs[ts]) / cal_steps
-    dom = max(per_size_ms, key=per_size_ms.get)
-
-    # timed region: the steps back to back, the other sizes spread over
-    # args.streams streams (kernels of different sizes overlap, filling each
-    # other's tails).  The dominant size runs alone at the start of every
-    # step (all streams join before it and fork after it), bracketed by HIP
-    # events on its stream, so its measured duration is its own (read after
-    # the region).
-    streams = [stream]
-    for _ in range(args.streams - 1):
-        sx = ctypes.c_void_p()
-        zav1.check(L.zav1_stream_create(ctypes.byref(sx)), 'stream_create')
-        streams.append(sx.value)
-    dom_ev = [(event(), event()) for _ in range(args.steps)]
-    join_ev = [[event() for _ in streams] for _ in range(args.steps)]
-    for sx in streams:
-        zav1.sync(sx)
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        for j, sx in enumerate(streams[1:], 1):  # join: the stream of the dominant waits for the others
-            L.zav1_event_record(join_ev[k][j], sx)
-            L.zav1_stream_wait_event(stream, join_ev[k][j])
-        L.zav1_event_record(dom_ev[k][0], stream)
-        sweep.run(stream, sizes=(dom,))
-        L.zav1_event_record(dom_ev[k][1], stream)
-        for sx in streams[1:]:  # fork: the others start after it
-            L.zav1_stream_wait_event(sx, dom_ev[k][1])
-        for ts in range(19):
-            if ts != dom:
-                sweep.run(streams, sizes=(ts,))
-    for sx in streams:
-        zav1.sync(sx)
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    mp = S * S / 1e6 * len(qps)
-    value = mp / (ms_per_step / 1000.0)
-    dom_ms_total = sum(elapsed(*e) for e in dom_ev)
-
-    # dominant kernel: the tx size with the largest event time.  Every form
-    # evaluates both quantizers in one launch (transforms shared).
-    types = tsm.legal_types(dom)
-    w, h = tsm.TX_DIMS[dom]
-    nblk = (S // w) * (S // h)
-    form = tsm.kernel_form(dom)
-    n_launch = tsm.launches(dom, len(qps))
-    q_per_launch = len(qps) // n_launch
-    avg_launch_s = dom_ms_total / 1000.0 / (args.steps * n_launch)
-    ops_launch = nblk * block_ops(tsm, dom, types, opsd, q_per_launch)
-    ops_launch_per_candidate = nblk * block_ops(tsm, dom, types, opsd) * q_per_launch
-    bytes_launch = nblk * (w * h * 2 + (w + h) * 2 + len(types) * 16 * q_per_launch)
-    achieved_tops = ops_launch / avg_launch_s / 1e12
-    kname = {'lane': 'k_txsearch_lane', 'rows': 'k_txsearch_rows', 'group': 'k_txsearch'}[form]
-
-    dom_pmc = pmc_kernel('%s<%d, %d, ' % (kname, w, h), bd=10, nq=q_per_launch)
-    out = {
-        'metric': 'MP/s (luma megapixels x qps) through batched candidate RD, byte-identical to the oracle',
-        'value': round(value, 2), 'unit': 'MP/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': round(ms_per_step, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'int32', 'data': 'synthetic',
-        'config': {'workload': 'config2 tx-type search: %dx%d 10-bit luma, qp 20+40, 19 tx sizes x legal '
-                               'types, fused DC-pred/fwd/fold/fp-quant/dist' % (S, S),
-                   'candidates_per_step': n_cand,
-                   'launches_per_step': sum(tsm.launches(t, len(qps)) for t in range(19)),
-                   'streams': args.streams},
-        'per_size_ms_per_step': {str(tsm.TX_DIMS[t]): round(per_size_ms[t], 4) for t in range(19)},
-        'roofline': {'bound': 'valu', 'kernel': '%s<%d,%d>' % (kname, w, h), 'achieved': round(achieved_tops, 3),
-                     'peak': VALU_PEAK_TOPS, 'unit': 'Tops/s', 'frac': round(achieved_tops / VALU_PEAK_TOPS, 4),
-                     'traffic': measured_traffic(dom_pmc),
-                     'valu_issue_frac': (None if measured_valu_insts(dom_pmc) is None else
-                                         round(measured_valu_insts(dom_pmc) * 64 / avg_launch_s / VALU_ISSUE_PEAK,
-                                               4)),
-                     'pmc_kernel': dom_pmc,
-                     'quantizers_per_launch': q_per_launch,
-                     'avg_launch_us': round(avg_launch_s * 1e6, 2),
-                     'algorithmic_ops_per_launch': int(ops_launch),
-                     'ops_per_launch_transform_per_quantizer': int(ops_launch_per_candidate),
-                     'hbm_algorithmic_GBs': round(bytes_launch / avg_launch_s / 1e9, 2),
-                     'hbm_peak_GBs': HBM_PEAK_GBS},
-    }
-
-    if not args.no_plain:
-        # plain per-call form: function table with host pointers, one launch per call
-        tab = zav1.dsp_table()
-        rng = np.random.default_rng(1)
-        n = 64
-        blocks = rng.integers(-1023, 1024, (n, 16, 16)).astype(np.int32)
-        row = zav1.quant_row(qis[0], 10)
-        rounding = np.array(list(row.round_fp), np.int32)
-        qfp = np.array(list(row.quant_fp), np.int32)
-        deq = np.array(list(row.dequant), np.int32)
-        P = ctypes.POINTER(ctypes.c_int32)
-        co = np.zeros(256, np.int32)
-        q = np.zeros(256, np.int32)
-        dq = np.zeros(256, np.int32)
-        res = ctypes.c_uint64()
-        tp0 = time.perf_counter()
-        calls = 0
-        for j in range(n):
-            for t in range(16):
-                tab.fwd_txfm2d(blocks[j].ctypes.data_as(P), co.ctypes.data_as(P), 16, 2, t)
-                tab.quantize_fp_raster(co.ctypes.data_as(P), q.ctypes.data_as(P), dq.ctypes.data_as(P), 256,
-                                       rounding.ctypes.data_as(P), qfp.ctypes.data_as(P),
-                                       deq.ctypes.data_as(P), 0)
-                tab.sse_i32(co.ctypes.data_as(P), dq.ctypes.data_as(P), 256, ctypes.byref(res))
-                calls += 3
-        tp1 = time.perf_counter()
-        per_cand = (tp1 - tp0) / (n * 16)
-        plain_ms_step = per_cand * n_cand * 1000.0
-        out['plain_gpu'] = {'value': round(mp / (plain_ms_step / 1000.0), 4), 'unit': 'MP/s',
-                            'sample': '%d 16x16 blocks x 16 types, 3 table calls each (%d calls), '
-                                      'extrapolated per candidate to the step' % (n, calls),
-                            'us_per_candidate': round(per_cand * 1e6, 2),
-                            'batched_over_plain': round(value / (mp / (plain_ms_step / 1000.0)), 1)}
-
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
-        import oracle as O
-        crop = S
-        sub = np.ascontiguousarray(plane[:crop, :crop])
-        threads = min(os.cpu_count() or 1, 16)
-        os.environ['OMP_NUM_THREADS'] = str(threads)
-        tc0 = time.perf_counter()
-        for qi in qis:
-            for ts in range(19):
-                xy = tsm.tile_xy(crop, crop, ts)
-                O.txsearch_plane(sub, 10, ts, xy, np.array(tsm.legal_types(ts), np.uint8), qi)
-        tc1 = time.perf_counter()
-        cpu_mp = crop * crop / 1e6 * len(qps) / (tc1 - tc0)
-        out['cpu_baseline'] = {'value': round(cpu_mp, 4), 'unit': 'MP/s', 'cores': threads, 'kind': 'port',
-                               'sample': '%dx%d (whole) frame, qp 20+40, all sizes/types '
-                                         '(oracle C, OpenMP over blocks)' % (crop, crop),
-                               'seconds': round(tc1 - tc0, 2)}
-    if not args.no_config4:
-        out['config4_tile'] = config4_stage(zav1, synth, tsm, args, streams)
-    if not args.no_serving:
+    out = wavefront_headline(zav1, synth, args)
+    if args.profile_pass:
+        print(json.dumps(out))
+        return
+    extras = [e for e in args.extras.split(',') if e]
+    cpu = not args.no_cpu
+    if {'txsearch', 'config4', 'serving'} & set(extras):
+        tsm = _load('zav1_txsearch', os.path.join(PKG, 'txsearch.py'))
+    if 'lf' in extras:
+        out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=cpu)
+    if 'rdoq' in extras:
+        out['rdoq'] = rdoq_stage(zav1, synth, args, cpu=cpu)
+    if 'txsearch' in extras:
+        opsd = json.load(open(os.path.join(PKG, 'csrc', 'txfm1d_ops.json')))
+        out['txsearch_stage'] = txsearch_stage(zav1, synth, tsm, opsd, args)
+    if 'config4' in extras:
+        out['config4_tile'] = config4_stage(zav1, synth, tsm, args, [zav1.stream_create()])
+    if 'serving' in extras:
         out['serving'] = serving_stage(zav1, synth, tsm, args)
-    if not args.no_lf:
-        out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=not args.no_cpu)
-    if not args.no_funnel:
-        out['md_funnel'] = md_funnel_stage(zav1, synth, args, cpu=not args.no_cpu)
-    if not args.no_rdoq:
-        out['rdoq'] = rdoq_stage(zav1, synth, args, cpu=not args.no_cpu)
     print(json.dumps(out))
 
 

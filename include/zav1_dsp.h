@@ -298,13 +298,22 @@ typedef struct {
  * 0 / 0 for the full trellis.  A job with a field out of range is left as it
  * is.  d_dqcoeff must hold the fp quantizer's dequantized values,
  * sign((|q| * dequant) >> log_scale), which the trellis recomputes from the
- * levels; the call takes a temporary of n_jobs * 8.5 bytes per coefficient
- * from a library-owned stream-ordered pool (one per device, keeping up to
- * 1 GiB between calls until the process ends). */
+ * levels.  d_eob[j] must be the quantizer's eob: positions at or past it
+ * hold level 0 (a block whose level at scan[eob - 1] is 0 is left as it
+ * is).  No allocation: the trellis works in on-chip memory. */
 int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, int32_t *d_qcoeff,
                     int32_t *d_dqcoeff, uint16_t *d_eob, const zav1_rdoq_job *d_jobs, int n_jobs,
                     const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,
                     uint32_t cut_off_denum, zav1_stream_t s);
+
+/* The trellis runs in one of two forms with identical results: one wave per
+ * block (short latency; for batches that cannot fill the chip) or one lane
+ * per block (throughput).  A batch of tx_size with at most the size's
+ * threshold of jobs takes the wave form.  Sets the threshold to n_jobs when
+ * n_jobs >= 0 (0: always the lane form) and returns the previous one, or
+ * ZAV1_EINVAL.  Process-wide; zav1_autotune sets it from measurements.  No
+ * counterpart in the reference (a scheduling choice of this library). */
+int zav1_rdoq_wave_max(int tx_size, int n_jobs);
 
 /* One block of a coefficient-rate batch (cost_coeffs_txb's arguments,
  * rust/crates/svtav1-encoder/src/leaf_funnel/coeff_rate.rs:60). */
@@ -484,11 +493,6 @@ int zav1_txunit_rd_batch_noalloc(const void *d_src, int stride, int plane_w, int
                                  uint16_t *d_recon, zav1_stream_t s);
 size_t zav1_noalloc_scratch_bytes(void);
 
-/* Returns the memory zav1_rdoq_batch's per-device pool keeps between calls
- * (up to 1 GiB) to the device.  No counterpart in the reference: the
- * trellis's temporaries are this library's. */
-int zav1_rdoq_pool_release(void);
-
 /* ------------------------------------------------------------------------ */
 /* Mode-decision funnel, MDS0 (batched).                                     */
 /* ------------------------------------------------------------------------ */
@@ -623,6 +627,20 @@ int zav1_wf_info(const zav1_wf *wf, zav1_wf_info_t *info);
  * captures them once into a HIP graph (s must be a created stream) and
  * replays it. */
 int zav1_wf_run(zav1_wf *wf, int use_graph, zav1_stream_t s);
+/* The eager run with HIP events at the stage boundaries of every step, the
+ * stream synchronized after each step; stage_ms[k] (n_stages >=
+ * ZAV1_WF_STAGES) receives stage k's total device time over the run:
+ *   0 candidate rates and contexts    1 MDS0 prediction + SATD
+ *   2 MDS0 selection (fast cost, exchange sort)
+ *   3 survivor prediction             4 MDS1 job build
+ *   5 MDS1 transform units (quantize_b, rate, distortion)
+ *   6 MDS3 job build (tx-type gates)  7 MDS3 forward transform + quantize
+ *   8 MDS3 RDOQ trellis               9 MDS3 rate, distortion, reconstruction
+ *  10 winner and reconstruction commit
+ * The decisions equal zav1_wf_run's.  No counterpart in the reference
+ * (measurement). */
+#define ZAV1_WF_STAGES 11
+int zav1_wf_run_timed(zav1_wf *wf, zav1_stream_t s, double *stage_ms, int n_stages);
 /* The plain per-call form of the same decisions for leaves
  * [first_leaf, first_leaf + n_leaves) in leaf order (earlier leaves must be
  * decided): one launch per candidate prediction + SATD, per survivor

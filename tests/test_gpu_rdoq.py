@@ -11,7 +11,16 @@ import rdoq_tables as RT
 pytestmark = pytest.mark.gpu
 
 
-def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70):
+FORMS = {'lane': 0, 'wave': 1 << 30}
+
+
+def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70, form=None):
+    if form is not None:
+        prev = z.lib().zav1_rdoq_wave_max(tx_size, FORMS[form])
+        try:
+            return run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class)
+        finally:
+            z.lib().zav1_rdoq_wave_max(tx_size, prev)
     w, h = RC.coded_dims(tx_size)
     n = w * h
     costs = RT.coeff_costs(seed)
@@ -58,16 +67,20 @@ def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70):
     assert changed > len(jobs) // 4  # the trellis did work on most blocks
 
 
+@pytest.mark.parametrize('form', ['lane', 'wave'])
 @pytest.mark.parametrize('tx_size', list(range(19)))
-def test_rdoq_batch_bitexact(gpu, oracle, tx_size):
-    run_case(gpu, oracle, tx_size, 8, 100, (0, 0), 500 + tx_size)
+def test_rdoq_batch_bitexact(gpu, oracle, tx_size, form):
+    """Both trellis forms (one lane per block, one wave per block) against
+    the oracle."""
+    run_case(gpu, oracle, tx_size, 8, 100, (0, 0), 500 + tx_size, form=form)
 
 
 @pytest.mark.parametrize('tx_size,bd,qindex,cut', [(0, 10, 40, (60, 100)), (2, 10, 200, (80, 100)),
                                                    (3, 8, 20, (60, 100)), (9, 10, 120, (0, 0)),
                                                    (16, 8, 255, (80, 100)), (4, 10, 0, (0, 0))])
-def test_rdoq_batch_depths_and_cutoffs(gpu, oracle, tx_size, bd, qindex, cut):
-    run_case(gpu, oracle, tx_size, bd, qindex, cut, 900 + tx_size * 7 + bd)
+@pytest.mark.parametrize('form', ['lane', 'wave'])
+def test_rdoq_batch_depths_and_cutoffs(gpu, oracle, tx_size, bd, qindex, cut, form):
+    run_case(gpu, oracle, tx_size, bd, qindex, cut, 900 + tx_size * 7 + bd, form=form)
 
 
 @pytest.mark.parametrize('tx_size', list(range(19)))
@@ -112,7 +125,8 @@ def test_coeff_rate_batch_bitexact(gpu, oracle, tx_size):
         assert int(got[b]) == want, (tx_size, b, int(got[b]), want)
 
 
-def test_rdoq_and_rate_leave_invalid_jobs_alone(gpu, oracle):
+@pytest.mark.parametrize('form', ['lane', 'wave'])
+def test_rdoq_and_rate_leave_invalid_jobs_alone(gpu, oracle, form):
     """Blocks whose job fields are out of range (tx class, plane type,
     contexts) are left as they are by the trellis and priced 0 by the rate;
     their neighbours in the batch are processed normally."""
@@ -135,7 +149,11 @@ def test_rdoq_and_rate_leave_invalid_jobs_alone(gpu, oracle):
     D = z.DeviceArray
     dq_, ddq, de = D.from_host(q), D.from_host(dq), D.from_host(eob.astype(np.uint16))
     dcosts, dscans = D.from_host(costs), D.from_host(scans)
-    z.rdoq_batch(tx_size, 40, 8, D.from_host(t), dq_, ddq, de, D.from_host(jobs), 8, dcosts, dscans)
+    prev = z.lib().zav1_rdoq_wave_max(tx_size, FORMS[form])
+    try:
+        z.rdoq_batch(tx_size, 40, 8, D.from_host(t), dq_, ddq, de, D.from_host(jobs), 8, dcosts, dscans)
+    finally:
+        z.lib().zav1_rdoq_wave_max(tx_size, prev)
     gq, ge = dq_.to_host(), de.to_host()
     d_rate = D((8,), np.int32)
     z.coeff_rate_batch(tx_size, dq_, de, D.from_host(rjobs), 8, dcosts, dscans, d_rate)

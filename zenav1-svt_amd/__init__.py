@@ -129,6 +129,8 @@ EXPORTS = {
     'zav1_inv_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _i32, _vp],
     'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
     'zav1_coeff_rate_batch': [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp],
+    'zav1_rdoq_wave_max': [_i32, _i32],
+    'zav1_wf_run_timed': [_vp, _vp, _vp, _i32],
     'zav1_rdoq_batch': [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                         _vp],
     'zav1_sse_i32_batch': [_vp, _vp, _vp, _i32, _i32, _vp],
@@ -171,7 +173,6 @@ EXPORTS = {
     'zav1_txunit_rd_batch': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32, ctypes.c_uint,
                              ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp,
                              _vp, _vp],
-    'zav1_rdoq_pool_release': [],
     'zav1_txunit_rd_batch_noalloc': [_vp, _i32, _i32, _i32, _i32, _i32, _vp, ctypes.c_size_t, _vp, _i32, _i32,
                                      ctypes.c_uint, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp,
                                      _vp, _vp],

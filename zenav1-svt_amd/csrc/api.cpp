@@ -1071,6 +1071,13 @@ int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, in
   return ZAV1_OK;
 }
 
+int zav1_rdoq_wave_max(int tx_size, int n_jobs) {
+  if (tx_size < 0 || tx_size >= 19) return ZAV1_EINVAL;
+  const int prev = zav1::rdoq_wave_max(tx_size);
+  if (n_jobs >= 0) zav1::rdoq_set_wave_max(tx_size, n_jobs);
+  return prev;
+}
+
 int zav1_sse_i32_batch(const int32_t* d_a, const int32_t* d_b, uint64_t* d_out, int n_per_job, int n_jobs,
                        zav1_stream_t s) {
   if (n_per_job <= 0 || n_jobs < 0) return ZAV1_EINVAL;
@@ -1233,11 +1240,6 @@ int zav1_txunit_rd_batch_noalloc(const void* d_src, int stride, int plane_w, int
 }
 
 size_t zav1_noalloc_scratch_bytes(void) { return zav1::noalloc_scratch_bytes(); }
-
-int zav1_rdoq_pool_release(void) {
-  ZCHECK(zav1::rdoq_pool_trim());
-  return ZAV1_OK;
-}
 
 static bool funnel_dim(int v) { return v == 4 || v == 8 || v == 16 || v == 32 || v == 64; }
 

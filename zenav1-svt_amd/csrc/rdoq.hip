@@ -3,20 +3,50 @@
 // quantization matrices, with the rate helpers of quant.rs:636-:906 and the
 // coefficient contexts of rust/crates/svtav1-entropy/src/coeff_c.rs.
 //
-// The trellis is a sequential reverse scan per transform block (each step
+// The trellis is a sequential reverse scan per transform block: each step
 // prices a coefficient against the levels of the coefficients after it,
-// which earlier steps may have lowered), so the parallel axis is the block:
-// one lane per block, 64 blocks per workgroup.  A lane's level map lives in
-// LDS as 4-bit levels min(|q|, 15), eight per dword, dwords interleaved
-// across the 64 lanes.  Capping at 15 instead of the reference's 127 leaves
-// every context unchanged: nz_mag clips each level at 3, and br_ctx's
-// min((l1 + l2 + l3 + 1) >> 1, 6) saturates once the sum reaches 11.
+// which earlier steps may have lowered.  Two forms, bit-identical:
+//
+// Lane form (large batches): one lane per block, 64 blocks per workgroup.
+// A lane's level map lives in LDS as 4-bit levels min(|q|, 15), eight per
+// dword, dwords interleaved across the 64 lanes.  Capping at 15 instead of
+// the reference's 127 leaves every context unchanged: nz_mag clips each
+// level at 3, and br_ctx's min((l1 + l2 + l3 + 1) >> 1, 6) saturates once
+// the sum reaches 11.  The map starts at zero and a coefficient's level is
+// written when the trellis decides it.  That is the reference's map
+// (txb_init_levels, then lowered in place) wherever the trellis reads it:
+// every context neighbour of a position, (r, c+1..4), (r+1..4, c) and
+// (r+1, c+1), comes later in the scan of each class (diagonal scans go by
+// anti-diagonal r + c, the vertical class's row scan by r then c, the
+// horizontal class's column scan by c then r), so it has been decided before
+// the position is priced; positions at or past the eob hold level 0.  All
+// lanes walk one scan step per iteration of one loop (each through its own
+// phase), so they refill a per-lane LDS ring of RK (coefficient, level)
+// pairs at the same iterations: the refill's 2 x RK loads are in flight
+// together and the wave waits for memory once per RK steps.
+//
+// Wave form (small batches, where a few hundred blocks cannot keep the chip
+// busy and the lane form's sequential walk is the whole latency): one wave
+// per block, the block's coefficients and levels staged in LDS.  The wave
+// runs the sequential part in lockstep, the eob phase (update_coeff_eob with
+// its running rate and distortion, until five nonzero levels are kept) and
+// the skip decision, stepping only at nonzero levels (the zeros between are
+// priced across the lanes).  The rest of the walk, update_coeff_simple down to the
+// cut-off, carries no running state: each position's level is a function
+// of the final levels of the positions after it.  The wave solves that
+// system as a fixpoint: every lane re-decides its positions against the
+// current map until a pass changes nothing.  The dependencies run strictly
+// from later to earlier scan positions, so the solution is unique and equals
+// the sequential walk's, and every pass settles at least the latest
+// unsettled position, so the passes end.  The wave then prices the DC
+// position (update_coeff_general) when the walk reaches it.
+//
 // Reads past the block's right or bottom edge return 0, which is what the
 // reference's zero padding (4 columns, >= 6 rows) holds.  The cost tables of
 // the launch's size (both plane types) and its three scans are staged in
 // LDS once per workgroup.
 #include <algorithm>
-#include <mutex>
+#include <cstdint>
 
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
@@ -107,14 +137,38 @@ __device__ int br_ctx_of(const LV& lv, int ci, int cls) {
   return mag + 14;
 }
 
-template <int TXS>
+// Level map and coefficient source of the lane form: 4-bit levels in
+// interleaved LDS words, (t, q) pairs from the lane's ring
+struct LaneMap {
+  uint32_t* lv;  // the lane's word 0; word stride 64
+  int2* ring;    // the lane's entry 0; entry stride 64
+  int ring_hi;   // scan index of entry 0 (entry k holds ring_hi - k)
+  __device__ __forceinline__ int get(int p) const { return (lv[(p >> 3) * 64] >> ((p & 7) * 4)) & 15; }
+  __device__ __forceinline__ void set(int p, int v) {
+    uint32_t& w = lv[(p >> 3) * 64];
+    const int sh = (p & 7) * 4;
+    w = (w & ~(15u << sh)) | ((uint32_t)v << sh);
+  }
+  __device__ __forceinline__ int2 coef(int si, int) const { return ring[(ring_hi - si) * 64]; }
+};
+// ... of the wave form: byte levels and the raster block, all in LDS
+struct WaveMap {
+  uint8_t* lv;
+  const int32_t* ts;
+  const int32_t* qs;
+  __device__ __forceinline__ int get(int p) const { return lv[p]; }
+  __device__ __forceinline__ void set(int p, int v) { lv[p] = (uint8_t)v; }
+  __device__ __forceinline__ int2 coef(int, int ci) const { return make_int2(ts[ci], qs[ci]); }
+};
+
+template <int TXS, class Map>
 struct Trellis {
   static constexpr int AW = c_txw(TXS), AH = c_txh(TXS), BWL = c_log2(AW), N = AW * AH, NW = (N + 7) / 8;
   static constexpr int SHIFT = c_scale(TXS);
-  const int2* work;    // this block's (t, q) pairs in scan order
+  static constexpr int RK = N < 32 ? N : 32;
+  Map m;
   int32_t* qc;         // levels and dequantized values, raster (written on change)
   int32_t* dq;
-  uint32_t* lv;        // level words of lane 0; lane stride 1, word stride 64
   const int32_t* cost; // TxbCosts of the block's plane type
   const int32_t* ecost;
   const uint16_t* scan;
@@ -124,14 +178,9 @@ struct Trellis {
 
   __device__ __forceinline__ int level(int r, int c) const {
     if (r >= AH || c >= AW) return 0;
-    const int p = r * AW + c;
-    return (lv[(p >> 3) * 64] >> ((p & 7) * 4)) & 15;
+    return m.get(r * AW + c);
   }
-  __device__ __forceinline__ void set_level(int p, int v) {
-    uint32_t& w = lv[(p >> 3) * 64];
-    const int sh = (p & 7) * 4;
-    w = (w & ~(15u << sh)) | ((uint32_t)(v < 15 ? v : 15) << sh);
-  }
+  __device__ __forceinline__ void set_level(int p, int v) { m.set(p, v < 15 ? v : 15); }
   __device__ __forceinline__ int lower_ctx(int ci, int si, bool is_last) const {
     return nz_ctx<TXS>([this](int r, int c) { return level(r, c); }, ci, si, is_last, cls);
   }
@@ -211,12 +260,11 @@ struct Trellis {
     }
     return c;
   }
-  // coefficient of scan index si (a placeholder for si < 0); the
-  // dequantized value follows the level as fp quantization sets it,
-  // sign((|q| * dequant) >> log_scale)
+  // coefficient of scan index si; the dequantized value follows the level
+  // as fp quantization sets it, sign((|q| * dequant) >> log_scale)
   __device__ __forceinline__ void fetch(int si, int& ci, int32_t& t, int32_t& q, int32_t& d) const {
     ci = scan[si];
-    const int2 v = work[si];
+    const int2 v = m.coef(si, ci);
     t = v.x, q = v.y;
     const int32_t a = (int32_t)(((int64_t)abs(q) * dqv[ci != 0]) >> SHIFT);
     d = q < 0 ? -a : a;
@@ -262,41 +310,41 @@ struct Trellis {
       *accu_rate += rate_low;
       *accu_dist += dist_low - dist0;
     } else {
+      set_level(ci, a);
       *accu_rate += rate;
       *accu_dist += dist - dist0;
     }
   }
-  // quant.rs:1056 update_coeff_simple
-  __device__ void update_simple(int* accu_rate, int si) {
-    int ci;
-    int32_t t, qv, d;
-    fetch(si, ci, t, qv, d);
+  // quant.rs:1056 update_coeff_simple's decision: the level kept at scan
+  // index si (|q| or |q| - 1, q != 0), from the levels after it
+  __device__ int simple_level(int ci, int si, int32_t t, int32_t qv) const {
     const int dv = dqv[ci != 0];
     const int ctx = lower_ctx(ci, si, false);
-    if (qv == 0) {
-      *accu_rate += base_cost(ctx, 0);
-      return;
-    }
     const int a = qv < 0 ? -qv : qv;
-    const int32_t at = abs(t), ad = abs(d);
+    const int32_t at = abs(t), ad = (int32_t)(((int64_t)a * dv) >> SHIFT);
     int b = 0;
     const int rate = cost_simple(ci, a, ctx, &b);
-    if (ad < at) {
-      *accu_rate += rate;
-      return;
-    }
+    if (ad < at) return a;
     const int rate_low = cost_simple_low(a, ctx, b, rate);
     const int64_t rd = rdcost(rdmult, rate, coeff_dist(at, ad, SHIFT));
     const int32_t ad_low = (int32_t)(((int64_t)(a - 1) * dv) >> SHIFT);
-    if (rdcost(rdmult, rate_low, coeff_dist(at, ad_low, SHIFT)) < rd) {
-      const int sign = qv < 0;
-      qc[ci] = (-sign ^ (a - 1)) + sign;
-      dq[ci] = (-sign ^ ad_low) + sign;
-      set_level(ci, a - 1);
-      *accu_rate += rate_low;
-    } else {
-      *accu_rate += rate;
-    }
+    return rdcost(rdmult, rate_low, coeff_dist(at, ad_low, SHIFT)) < rd ? a - 1 : a;
+  }
+  // level a - 1 at ci in place of a (sign 1 for a negative level)
+  __device__ __forceinline__ void store_low(int ci, int a, int sign) {
+    const int32_t ad_low = (int32_t)(((int64_t)(a - 1) * dqv[ci != 0]) >> SHIFT);
+    qc[ci] = (-sign ^ (a - 1)) + sign;
+    dq[ci] = (-sign ^ ad_low) + sign;
+  }
+  // update_coeff_simple (its running rate is never read again: dropped)
+  __device__ void update_simple(int si) {
+    int ci;
+    int32_t t, qv, d;
+    fetch(si, ci, t, qv, d);
+    if (qv == 0) return;
+    const int a = qv < 0 ? -qv : qv, l = simple_level(ci, si, t, qv);
+    if (l != a) store_low(ci, a, qv < 0);
+    set_level(ci, l);
   }
   // quant.rs:1120 update_coeff_eob; nz_ci[0 .. nz_num) are the nonzero
   // positions since the eob (at most 5)
@@ -372,6 +420,8 @@ struct Trellis {
       qc[ci] = q_low;
       dq[ci] = d_low;
       set_level(ci, a - 1);
+    } else {
+      set_level(ci, a);
     }
     if (lower ? q_low != 0 : true) {  // the level kept is nonzero
 #pragma unroll
@@ -383,72 +433,53 @@ struct Trellis {
 };
 }  // namespace
 
+// The part of optimize_b (quant.rs:1312) before the simple walk, for a wave
+// that holds one block (every lane runs it with the same values): the last
+// coefficient, the eob phase and the skip decision.  Returns false for a
+// block the trellis is not defined on (level 0 at scan[eob - 1]: left as it
+// is).  On return si is the first scan index of the simple walk and si_end
+// its cut-off (rdoq_cutoffs, quant.rs:1551).  A zero level in the eob phase only adds base_cost(ctx, 0)
+// to the running rate, with ctx from levels that no zero can change, so the
+// zeros down to the next nonzero level are priced by the wave at once (64
+// positions a pass, found by ballot) and only nonzero levels take a
+// sequential step.
 template <int TXS>
-__global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, const uint32_t* __restrict__ lvwork,
-                                             int32_t* __restrict__ qcoeff,
-                                             int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
-                                             const RdoqJob* __restrict__ jobs, int n_jobs,
-                                             const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans,
-                                             int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
-  using T = Trellis<TXS>;
-  constexpr int N = T::N;
-  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
-  __shared__ uint16_t sc[3 * N];
-  __shared__ uint32_t lv[T::NW * 64];
-  const int lane = threadIdx.x;
-  // CoeffCostTables (quant.rs:475): txb[txs_ctx * 2 + plane], then eob[ms][plane]
-  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
-  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
-  for (int i = lane; i < 2 * kTxbWords; i += 64) tab[i] = txb_src[i];
-  for (int i = lane; i < 2 * kEobWords; i += 64) tab[2 * kTxbWords + i] = eob_src[i];
-  for (int i = lane; i < 3 * N; i += 64) sc[i] = scans[i];
-  __syncthreads();
-  const int j = blockIdx.x * 64 + lane;
-  if (j >= n_jobs) return;
-  const RdoqJob job = jobs[j];
-  int eob = eobs[j];
-  // out-of-range fields would index past the staged tables: such blocks are left as they are
-  if (eob <= 0 || eob > N || job.tx_class > 2 || job.plane_type > 1 || job.txb_skip_ctx >= 13 ||
-      job.dc_sign_ctx >= 3)
-    return;
-  T s;
-  s.work = work + (size_t)j * N;
-  s.qc = qcoeff + job.coeff;
-  s.dq = dqcoeff + job.coeff;
-  s.lv = lv + lane;
-  s.cost = tab + (job.plane_type ? kTxbWords : 0);
-  s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
-  // scan classes: 0 diagonal (2D), 1 row (vertical class), 2 column (horizontal class)
-  s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
-  s.rdmult = job.rdmult;
-  s.dqv[0] = dq0, s.dqv[1] = dq1;
-  s.cls = job.tx_class;
-  s.dc_ctx = job.dc_sign_ctx;
-  const bool sharp = job.sharpness != 0;
-  // txb_init_levels (coeff_c.rs:278), packed by the gather
-  for (int w = 0; w < T::NW; w++) s.lv[w * 64] = lvwork[(long)j * T::NW + w];
-  const int non_skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2], skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2 + 1];
+__device__ bool trellis_head_wave(Trellis<TXS, WaveMap>& s, int lane, int& eob, int& si, int& si_end,
+                                  int non_skip_cost, int skip_cost, bool sharp, uint32_t cut_num, uint32_t cut_den) {
+  using T = Trellis<TXS, WaveMap>;
   int accu_rate = s.eob_cost(eob);
   int64_t accu_dist = 0;
-  int si = eob - 1;
+  si = eob - 1;
+  int ci;
+  int32_t t, qv, d;
+  s.fetch(si, ci, t, qv, d);
+  if (qv == 0) return false;
   {
-    int ci;
-    int32_t t, qv, d;
-    s.fetch(si, ci, t, qv, d);
-    // an eob that is not tight (level 0 at scan[eob - 1]) is not a block the
-    // trellis is defined on: left as it is
-    if (qv == 0) return;
     const int a = qv < 0 ? -qv : qv, sign = qv < 0;
     if (a >= 2) {
       s.update_general(&accu_rate, &accu_dist, si, eob);
     } else {
       accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
       accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
+      s.set_level(ci, a);
     }
-    si--;
   }
-  int nz_num = 1, nz_ci[5] = {s.scan[eob - 1], 0, 0, 0, 0};
+  si--;
+  int nz_num = 1, nz_ci[5] = {ci, 0, 0, 0, 0};
   while (si >= 0 && nz_num <= 4) {
+    const int i = si - lane;
+    int cz = 0;
+    bool nz = false;
+    if (i >= 0) {
+      cz = s.scan[i];
+      nz = s.m.qs[cz] != 0;  // below si: still the quantizer's level
+    }
+    const uint64_t m = __ballot(nz);
+    const int nzero = m ? (int)__builtin_ctzll(m) : min(64, si + 1);
+    const int r = lane < nzero ? s.base_cost(s.lower_ctx(cz, i, false), 0) : 0;
+    accu_rate += group_sum_i32<64>(r);
+    si -= nzero;
+    if (m == 0) continue;
     s.update_eob(&accu_rate, &accu_dist, &eob, &nz_num, nz_ci, si, sharp);
     si--;
   }
@@ -458,21 +489,245 @@ __global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, cons
 #pragma unroll
       for (int k = 0; k < 5; k++)
         if (k < nz_num) s.qc[nz_ci[k]] = 0, s.dq[nz_ci[k]] = 0;
-      accu_rate = 0;
       eob = 0;
     }
   }
-  int si_end = 1;
+  si_end = 1;
   if (cut_num != 0) {
-    const uint32_t cut = max((uint32_t)(N >> 7), (uint32_t)eob * cut_num / cut_den);
+    const uint32_t cut = max((uint32_t)(T::N >> 7), (uint32_t)eob * cut_num / cut_den);
     si_end = max(eob - (int)cut, 1);
   }
-  for (; si >= si_end; si--) s.update_simple(&accu_rate, si);
-  if (si == 0) {
+  return true;
+}
+
+template <int TXS>
+__device__ __forceinline__ bool job_ok(const RdoqJob& job, int eob) {
+  constexpr int N = c_txw(TXS) * c_txh(TXS);
+  // out-of-range fields would index past the staged tables: such blocks are left as they are
+  return eob > 0 && eob <= N && job.tx_class <= 2 && job.plane_type <= 1 && job.txb_skip_ctx < 13 &&
+         job.dc_sign_ctx < 3;
+}
+
+template <int TXS>
+__global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff,
+                                             int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
+                                             const RdoqJob* __restrict__ jobs, int n_jobs,
+                                             const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans,
+                                             int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
+  using T = Trellis<TXS, LaneMap>;
+  constexpr int N = T::N, RK = T::RK;
+  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
+  __shared__ uint16_t sc[3 * N];
+  __shared__ uint32_t lv[T::NW * 64];
+  __shared__ int2 ring[RK * 64];
+  const int lane = threadIdx.x;
+  // CoeffCostTables (quant.rs:475): txb[txs_ctx * 2 + plane], then eob[ms][plane]
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = lane; i < 2 * kTxbWords; i += 64) tab[i] = txb_src[i];
+  for (int i = lane; i < 2 * kEobWords; i += 64) tab[2 * kTxbWords + i] = eob_src[i];
+  for (int i = lane; i < 3 * N; i += 64) sc[i] = scans[i];
+  __syncthreads();
+  const int j = blockIdx.x * 64 + lane;
+  const bool live = j < n_jobs;
+  const RdoqJob job = live ? jobs[j] : RdoqJob{};
+  int eob = live ? eobs[j] : 0;
+  const bool ok = live && job_ok<TXS>(job, eob);
+  T s;
+  const int32_t* tc = tcoeff + (ok ? job.coeff : 0);
+  s.m.lv = lv + lane;
+  s.m.ring = ring + lane;
+  s.m.ring_hi = 0;
+  s.qc = qcoeff + (ok ? job.coeff : 0);
+  s.dq = dqcoeff + (ok ? job.coeff : 0);
+  s.cost = tab + (job.plane_type ? kTxbWords : 0);
+  s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
+  // scan classes: 0 diagonal (2D), 1 row (vertical class), 2 column (horizontal class)
+  s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
+  s.rdmult = job.rdmult;
+  s.dqv[0] = dq0, s.dqv[1] = dq1;
+  s.cls = ok ? job.tx_class : 0;
+  s.dc_ctx = ok ? job.dc_sign_ctx : 0;
+  const bool sharp = job.sharpness != 0;
+  for (int w = 0; w < T::NW; w++) s.m.lv[w * 64] = 0;
+  const int skip_ctx = ok ? job.txb_skip_ctx : 0;
+  const int non_skip_cost = s.cost[kSkip + skip_ctx * 2], skip_cost = s.cost[kSkip + skip_ctx * 2 + 1];
+  // phases: 0 the last coefficient, 1 the eob phase, 2 the simple walk, 3 done
+  int ph = ok ? 0 : 3, si = eob - 1, si_end = 1, accu_rate = 0, nz_num = 0, nz_ci[5] = {0, 0, 0, 0, 0};
+  int64_t accu_dist = 0;
+  bool write_eob = ok;
+  for (int it = 0; __ballot(ph != 3) != 0; it++) {
+    if ((it & (RK - 1)) == 0) {  // every lane at once: its next RK scan positions
+      int32_t tv[RK], qv[RK];
+#pragma unroll
+      for (int k = 0; k < RK; k++) {
+        const int i = si - k;
+        tv[k] = qv[k] = 0;
+        if (ph != 3 && i >= 0) {
+          const int ci = s.scan[i];
+          tv[k] = tc[ci];
+          qv[k] = s.qc[ci];  // not decided yet: still the quantizer's level
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RK; k++) s.m.ring[k * 64] = make_int2(tv[k], qv[k]);
+      s.m.ring_hi = si;
+    }
+    if (ph == 0) {
+      int ci;
+      int32_t t, qv, d;
+      s.fetch(si, ci, t, qv, d);
+      if (qv == 0) {  // not tight: left as it is
+        ph = 3;
+        write_eob = false;
+      } else {
+        accu_rate = s.eob_cost(eob);
+        const int a = qv < 0 ? -qv : qv, sign = qv < 0;
+        if (a >= 2) {
+          s.update_general(&accu_rate, &accu_dist, si, eob);
+        } else {
+          accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
+          accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
+          s.set_level(ci, a);
+        }
+        nz_num = 1;
+        nz_ci[0] = ci;
+        si--;
+        ph = 1;
+      }
+    } else if (ph != 3) {
+      if (ph == 1 && (si < 0 || nz_num > 4)) {
+        if (si == -1 && nz_num <= 4) {  // update_skip (quant.rs:1281)
+          const int64_t rd = rdcost(s.rdmult, accu_rate + non_skip_cost, accu_dist);
+          if (!sharp && rdcost(s.rdmult, skip_cost, 0) < rd) {
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+              if (k < nz_num) s.qc[nz_ci[k]] = 0, s.dq[nz_ci[k]] = 0;
+            eob = 0;
+          }
+        }
+        if (cut_num != 0) {
+          const uint32_t cut = max((uint32_t)(N >> 7), (uint32_t)eob * cut_num / cut_den);
+          si_end = max(eob - (int)cut, 1);
+        }
+        ph = 2;
+      }
+      if (ph == 1) {
+        s.update_eob(&accu_rate, &accu_dist, &eob, &nz_num, nz_ci, si, sharp);
+        si--;
+      } else if (si >= si_end) {
+        s.update_simple(si);
+        si--;
+      } else {
+        if (si == 0) {
+          int64_t dummy = 0;
+          s.update_general(&accu_rate, &dummy, 0, eob);
+        }
+        ph = 3;
+      }
+    }
+  }
+  if (write_eob) eobs[j] = (uint16_t)eob;
+}
+
+// The wave form: kWaveBlocks blocks per workgroup, one wave each.
+constexpr int kWaveBlocks = 4;
+template <int TXS>
+__global__ __launch_bounds__(64 * kWaveBlocks) void k_rdoq_wave(
+    const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
+    uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, int n_jobs, const int32_t* __restrict__ costs,
+    const uint16_t* __restrict__ scans, int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
+  using T = Trellis<TXS, WaveMap>;
+  constexpr int N = T::N, K = (N + 63) / 64;
+  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
+  __shared__ uint16_t sc[3 * N];
+  __shared__ int32_t ts[kWaveBlocks][N], qs[kWaveBlocks][N];
+  __shared__ uint8_t lvb[kWaveBlocks][N];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = tid; i < 2 * kTxbWords; i += 64 * kWaveBlocks) tab[i] = txb_src[i];
+  for (int i = tid; i < 2 * kEobWords; i += 64 * kWaveBlocks) tab[2 * kTxbWords + i] = eob_src[i];
+  for (int i = tid; i < 3 * N; i += 64 * kWaveBlocks) sc[i] = scans[i];
+  __syncthreads();
+  const int j = blockIdx.x * kWaveBlocks + wv;
+  if (j >= n_jobs) return;  // wave-uniform; no barrier follows
+  const RdoqJob job = jobs[j];
+  int eob = eobs[j];
+  if (!job_ok<TXS>(job, eob)) return;
+  // the block, raster, and its levels (txb_init_levels, coeff_c.rs:278)
+  for (int p = lane; p < N; p += 64) {
+    const int32_t t = tcoeff[job.coeff + p], q = qcoeff[job.coeff + p];
+    ts[wv][p] = t;
+    qs[wv][p] = q;
+    const int a = q < 0 ? -q : q;
+    lvb[wv][p] = (uint8_t)(a < 15 ? a : 15);
+  }
+  wave_sync();
+  T s;
+  s.m.lv = lvb[wv];
+  s.m.ts = ts[wv];
+  s.m.qs = qs[wv];
+  s.qc = qcoeff + job.coeff;
+  s.dq = dqcoeff + job.coeff;
+  s.cost = tab + (job.plane_type ? kTxbWords : 0);
+  s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
+  s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
+  s.rdmult = job.rdmult;
+  s.dqv[0] = dq0, s.dqv[1] = dq1;
+  s.cls = job.tx_class;
+  s.dc_ctx = job.dc_sign_ctx;
+  int si = 0, si_end = 1;
+  if (!trellis_head_wave(s, lane, eob, si, si_end, s.cost[kSkip + job.txb_skip_ctx * 2],
+                         s.cost[kSkip + job.txb_skip_ctx * 2 + 1], job.sharpness != 0, cut_num, cut_den))
+    return;
+  wave_sync();
+  // the simple walk over scan indices [si_end, si] as a fixpoint; lane l
+  // holds positions si - l - 64 k
+  int ci[K];
+  int32_t tt[K], qq[K];
+  bool low[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int i = si - lane - 64 * k;
+    low[k] = false;
+    qq[k] = 0, tt[k] = 0, ci[k] = 0;
+    if (i >= si_end) {
+      ci[k] = s.scan[i];
+      tt[k] = s.m.ts[ci[k]];
+      qq[k] = s.m.qs[ci[k]];
+    }
+  }
+  if (si >= si_end) {
+    for (;;) {
+      bool changed = false;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (qq[k] != 0) {
+          const int a = qq[k] < 0 ? -qq[k] : qq[k];
+          const int l = s.simple_level(ci[k], si - lane - 64 * k, tt[k], qq[k]);
+          if ((l != a) != low[k]) {
+            low[k] = l != a;
+            s.set_level(ci[k], l);
+            changed = true;
+          }
+        }
+      }
+      wave_sync();
+      if (__ballot(changed) == 0) break;
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (low[k]) s.store_low(ci[k], qq[k] < 0 ? -qq[k] : qq[k], qq[k] < 0);
+  }
+  // the walk's last step: the DC position when the cut-off reaches it
+  const int last = si >= si_end ? si_end - 1 : si;
+  if (last == 0) {
+    int accu_rate = 0;
     int64_t dummy = 0;
     s.update_general(&accu_rate, &dummy, 0, eob);
   }
-  eobs[j] = (uint16_t)eob;
+  if (lane == 0) eobs[j] = (uint16_t)eob;
 }
 
 // Coefficient rate (cost_coeffs_txb, lf/coeff_rate.rs:60, full middle loop)
@@ -588,119 +843,34 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
   return hipGetLastError();
 }
 
-// (t, q) of every scan index of every block, in scan order: the trellis then
-// reads consecutive pairs as its scan index walks down, instead of one
-// cache line of a raster row per step.  A workgroup reads whole blocks in
-// raster order (coalesced) into LDS and writes them out permuted.
-template <int TXS>
-__global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__ tcoeff,
-                                                     const int32_t* __restrict__ qcoeff,
-                                                     const uint16_t* __restrict__ eobs,
-                                                     const RdoqJob* __restrict__ jobs, int n_jobs,
-                                                     const uint16_t* __restrict__ scans, int2* __restrict__ work,
-                                                     uint32_t* __restrict__ lvwork) {
-  // 1024 coefficients per workgroup (four per thread): enough loads in
-  // flight per workgroup to hide their latency
-  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = N < 1024 ? 1024 / N : 1;
-  __shared__ uint16_t sc[3 * N];
-  __shared__ int32_t st[BPW * N], sq[BPW * N];
-  __shared__ uint32_t boff[BPW];  // per block: coefficient offset, or ~0 when it is not read
-  __shared__ int bscan[BPW];      // per block: its scan's offset in sc
-  const int tid = threadIdx.x, j0 = blockIdx.x * BPW;
-  for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
-  if (tid < BPW) {
-    const int j = j0 + tid;
-    // a block the trellis leaves alone (eob 0) is not read
-    const bool use = j < n_jobs && eobs[j] != 0;
-    const RdoqJob job = use ? jobs[j] : RdoqJob{};
-    const int cls = job.tx_class > 2 ? 0 : job.tx_class;
-    boff[tid] = use ? job.coeff : ~0u;
-    bscan[tid] = (cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N;
-  }
-  __syncthreads();
-  for (int k = tid; k < BPW * N; k += 256) {
-    const int b = k / N, p = k - b * N;
-    const uint32_t off = boff[b];
-    if (off != ~0u) {
-      st[k] = tcoeff[off + p];
-      sq[k] = qcoeff[off + p];
-    }
-  }
-  __syncthreads();
-  for (int k = tid; k < BPW * N; k += 256) {
-    const int b = k / N, i = k - b * N;
-    if (boff[b] != ~0u) {
-      const int ci = sc[bscan[b] + i];
-      work[(long)(j0 + b) * N + i] = make_int2(st[b * N + ci], sq[b * N + ci]);
-    }
-  }
-  // the blocks' level maps, as the trellis keeps them: min(|q|, 15) in
-  // nibbles, eight raster positions per word
-  constexpr int NW = (N + 7) / 8;
-  for (int k = tid; k < BPW * NW; k += 256) {
-    const int b = k / NW, w = k - b * NW;
-    if (boff[b] != ~0u) {
-      uint32_t v = 0;
-#pragma unroll
-      for (int e = 0; e < 8; e++)
-        if (8 * w + e < N) v |= (uint32_t)min(abs(sq[b * N + 8 * w + e]), 15) << (4 * e);
-      lvwork[(long)(j0 + b) * NW + w] = v;
-    }
-  }
+// Largest batch per tx size that takes the wave form (set by the
+// autotuner).  Default: every batch -- measured on MI355X the wave form is
+// ahead at every batch size up to 262 144 dense 16x16 blocks (2.06 vs
+// 2.32 ms), and far ahead on the few hundred blocks of a wavefront step.
+static int g_wave_max[19] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX,
+                             INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX,
+                             INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+
+int rdoq_wave_max(int tx_size) { return tx_size >= 0 && tx_size < 19 ? g_wave_max[tx_size] : 0; }
+void rdoq_set_wave_max(int tx_size, int n) {
+  if (tx_size >= 0 && tx_size < 19) g_wave_max[tx_size] = n < 0 ? 0 : n;
 }
 
-// The library's stream-ordered pool for the trellis's temporaries, one per
-// device, keeping up to 1 GiB between calls (an encoder calls the trellis
-// wave after wave; without it every call would map its temporary afresh).
-// Null if the runtime refuses a pool: the default pool serves then.
-static hipMemPool_t rdoq_pool() {
-  static std::mutex m;
-  static hipMemPool_t pools[64] = {};
-  static bool tried[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  std::lock_guard<std::mutex> g(m);
-  if (!tried[dev]) {
-    tried[dev] = true;
-    hipMemPoolProps props{};
-    props.allocType = hipMemAllocationTypePinned;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = dev;
-    hipMemPool_t pool = nullptr;
-    if (hipMemPoolCreate(&pool, &props) == hipSuccess) {
-      uint64_t keep = 1ull << 30;
-      hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-      pools[dev] = pool;
-    }
-  }
-  return pools[dev];
-}
-
-size_t rdoq_workspace(int tx_size, int n_jobs) {
-  const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
-  const int nw = (n + 7) / 8;
-  return (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t));
-}
-
-// The trellis over a caller's workspace of rdoq_workspace(tx_size, n_jobs)
-// bytes (scan-order pairs, then the packed level maps).
-hipError_t rdoq_batch_ws(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
-                         const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                         int32_t dq1, uint32_t cut_num, uint32_t cut_den, void* ws, hipStream_t st) {
+hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                      const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, int form) {
   if (n_jobs <= 0) return hipSuccess;
-  const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
-  int2* work = static_cast<int2*>(ws);
-  uint32_t* lvwork = reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n);
-  const dim3 grid((unsigned)((n_jobs + 63) / 64));
-  const int bpw = n < 1024 ? 1024 / n : 1;
-  const dim3 ggrid((unsigned)((n_jobs + bpw - 1) / bpw));
-#define ZR(T)                                                                                                   \
-  case T:                                                                                                       \
-    hipLaunchKernelGGL((k_rdoq_gather<T>), ggrid, dim3(256), 0, st, tcoeff, qcoeff, eobs, jobs, n_jobs, scans,  \
-                       work, lvwork);                                                                       \
-    hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, work, lvwork, qcoeff, dqcoeff, eobs, jobs, n_jobs,  \
-                       costs,                                                                               \
-                       scans, dq0, dq1, cut_num, cut_den);                                                      \
+  if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
+  const bool wave = form == kRdoqWave || (form == kRdoqAuto && n_jobs <= g_wave_max[tx_size]);
+  const dim3 grid((unsigned)((n_jobs + 63) / 64)), wgrid((unsigned)((n_jobs + kWaveBlocks - 1) / kWaveBlocks));
+#define ZR(T)                                                                                                      \
+  case T:                                                                                                          \
+    if (wave)                                                                                                      \
+      hipLaunchKernelGGL((k_rdoq_wave<T>), wgrid, dim3(64 * kWaveBlocks), 0, st, tcoeff, qcoeff, dqcoeff, eobs,    \
+                         jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);                                  \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs,    \
+                         scans, dq0, dq1, cut_num, cut_den);                                                       \
     break;
   switch (tx_size) {
     ZR(0) ZR(1) ZR(2) ZR(3) ZR(4) ZR(5) ZR(6) ZR(7) ZR(8) ZR(9) ZR(10) ZR(11) ZR(12) ZR(13) ZR(14) ZR(15) ZR(16)
@@ -710,29 +880,6 @@ hipError_t rdoq_batch_ws(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, in
   }
 #undef ZR
   return hipGetLastError();
-}
-
-hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
-                      const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
-  if (n_jobs <= 0) return hipSuccess;
-  if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
-  void* work = nullptr;
-  const size_t bytes = rdoq_workspace(tx_size, n_jobs);
-  hipMemPool_t pool = rdoq_pool();
-  hipError_t e = pool ? hipMallocFromPoolAsync(&work, bytes, pool, st) : hipMallocAsync(&work, bytes, st);
-  if (e != hipSuccess) return e;
-  e = rdoq_batch_ws(tx_size, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
-                    work, st);
-  const hipError_t f = hipFreeAsync(work, st);
-  return e != hipSuccess ? e : f;
-}
-
-// Returns the trellis pool's cached memory to the device (memory of calls
-// still in flight stays reserved until they complete).
-hipError_t rdoq_pool_trim() {
-  hipMemPool_t pool = rdoq_pool();
-  return pool ? hipMemPoolTrimTo(pool, 0) : hipSuccess;
 }
 
 }  // namespace zav1

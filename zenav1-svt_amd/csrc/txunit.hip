@@ -366,7 +366,7 @@ size_t txunit_rd_workspace(int tx_size, int n_jobs) {
   if (tx_size < 0 || tx_size > 18 || n_jobs <= 0) return 0;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
   return 3 * align256(n * np * 4) + align256(n * 2) + align256(n * 8) + align256(n * sizeof(TxUnitJob)) +
-         align256(n * sizeof(RdoqJob)) + align256(n * sizeof(RateJob)) + align256(rdoq_workspace(tx_size, n_jobs));
+         align256(n * sizeof(RdoqJob)) + align256(n * sizeof(RateJob));
 }
 
 template <int S>
@@ -393,7 +393,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
-                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd) {
+                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd, hipEvent_t* rdoq_ev) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
@@ -413,17 +413,20 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   TxUnitJob* tj = reinterpret_cast<TxUnitJob*>(carve(n * sizeof(TxUnitJob)));
   RdoqJob* rj = reinterpret_cast<RdoqJob*>(carve(n * sizeof(RdoqJob)));
   RateJob* cj = reinterpret_cast<RateJob*>(carve(n * sizeof(RateJob)));
-  void* rdoq_ws = carve(rdoq_workspace(tx_size, n_jobs));
   hipLaunchKernelGGL(k_rd_prep, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, jobs, n_jobs, (int)np, tj,
                      rj, cj);
   hipError_t e = txunit_batch(src, stride, plane_w, plane_h, bd, tx_size, pred, pred_len, tj, n_jobs, row, iscan,
                               flags & kTxuQuantB, dist, eob, cul, tq, nullptr, st, keep);
   if (e != hipSuccess) return e;
+  // rdoq_ev: two events recorded around the trellis (timing; recorded
+  // also when the flags skip it)
+  if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;
   if (flags & kTxuRdoq) {
-    e = rdoq_batch_ws(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num,
-                      cut_den, rdoq_ws, st);
+    e = rdoq_batch(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
+                   st);
     if (e != hipSuccess) return e;
   }
+  if (rdoq_ev && (e = hipEventRecord(rdoq_ev[1], st)) != hipSuccess) return e;
   e = coeff_rate_batch(tx_size, tq, keep.eob16, cj, n_jobs, costs, scans, rate, st);
   if (e != hipSuccess) return e;
   // the tx size's entropy context: mean of its square-down and square-up sizes

@@ -454,37 +454,60 @@ static size_t footprint(const Wavefront& w) {
   return b;
 }
 
-static hipError_t step_chain(Wavefront* w, int s, hipStream_t st) {
+constexpr int kWfStages = ZAV1_WF_STAGES;
+// Events at the stage boundaries of a step (the timed run): ev[k] opens
+// stage k, ev[kWfStages] closes the last.
+struct StageClock {
+  hipEvent_t ev[kWfStages + 1];
+};
+#define MARK(k)                                                              \
+  do {                                                                       \
+    if (clk) {                                                               \
+      const hipError_t em = hipEventRecord(clk->ev[k], st);                  \
+      if (em != hipSuccess) return em;                                       \
+    }                                                                        \
+  } while (0)
+
+static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* clk = nullptr) {
   const int u0 = w->step_off[s], U = w->step_off[s + 1] - u0;
   if (U == 0) return hipSuccess;
   const zav1_wf_cfg& c = w->cfg;
   const int L = w->L, W = c.width, H = w->stacked_h;
   MdUnit* units = w->units + u0;
+  MARK(0);
   hipLaunchKernelGGL(k_wf_prep, dim3((unsigned)((U * w->nc + 255) / 256)), dim3(256), 0, st, units, w->nbr + 2 * u0,
                      U, w->mode_map, w->rate_tab, w->nc, w->rate);
+  MARK(1);
   hipError_t e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, units, U, w->cl, w->satd, st);
   if (e != hipSuccess) return e;
+  MARK(2);
   TypeList dct{};
   dct.n = 1;
   e = fast_select(w->satd, w->rate, c.lambda, U, w->nc, w->k1, reinterpret_cast<const int32_t*>(units), 5, L, L, dct,
                   1, w->sel, w->sel_cost, w->fs_jobs, st);
   if (e != hipSuccess) return e;
+  MARK(3);
   e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, units, U, w->cl, w->sel, w->k1, w->pred, st);
   if (e != hipSuccess) return e;
+  MARK(4);
   hipLaunchKernelGGL(k_wf_mds1_jobs, dim3((unsigned)((U * w->k1 + 255) / 256)), dim3(256), 0, st, units, U, w->sel,
                      w->k1, w->cands, w->txt, L, w->rdmult, w->jobs1);
+  MARK(5);
   e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs1, U * w->k1, w->row,
                       w->iscan, w->fscan, w->costs, kTxuQuantB, 0, 0, w->ws1, w->dist1, w->eob1, w->rate1, w->cul1,
                       nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
   if (e != hipSuccess) return e;
+  MARK(6);
   hipLaunchKernelGGL(k_wf_mds3_jobs, dim3((unsigned)U), dim3(64), 0, st, units, U, w->sel, w->k1, w->k3,
                      w->cands, w->rate, w->nc, w->dist1, w->rate1, c.lambda, w->d_types, w->nt, w->txt, L, w->rdmult,
                      w->m3, w->jobs3);
   const int n3 = U * w->k3 * w->nt;
+  MARK(7);
   e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs3, n3, w->row, w->iscan,
                       w->fscan, w->costs, w->flags3, w->cut_num, w->cut_den, w->ws3, w->dist3, w->eob3, w->rate3,
-                      w->cul3, nullptr, w->rec3, st, w->dq0, w->dq1, w->csatd3);
+                      w->cul3, nullptr, w->rec3, st, w->dq0, w->dq1, w->csatd3, clk ? &clk->ev[8] : nullptr);
   if (e != hipSuccess) return e;
+  MARK(10);
   if (c.bd == 8)
     hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, w->m3,
                        w->sel, w->k1, w->k3, w->cands, w->rate, w->nc, w->d_types, w->nt, w->txt, w->dist3, w->rate3,
@@ -495,8 +518,10 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st) {
                        w->sel, w->k1, w->k3, w->cands, w->rate, w->nc, w->d_types, w->nt, w->txt, w->dist3, w->rate3,
                        w->eob3, w->csatd3, w->rec3, c.lambda, w->rate_th, w->satd_th, (uint16_t*)w->recon, W, L,
                        w->dec, w->mode_map);
+  MARK(kWfStages);
   return hipGetLastError();
 }
+#undef MARK
 
 // The plain form of one unit: one launch per elementary call with one job
 // each (every candidate's prediction + SATD, every survivor's prediction and
@@ -829,6 +854,31 @@ int zav1_wf_run(zav1_wf* wf, int use_graph, zav1_stream_t s) {
     w->graph_stream = st;
   }
   WCHECK(hipGraphLaunch(w->graph, st));
+  return ZAV1_OK;
+}
+
+int zav1_wf_run_timed(zav1_wf* wf, zav1_stream_t s, double* stage_ms, int n_stages) {
+  if (!wf || !stage_ms || n_stages < ZAV1_WF_STAGES) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
+  zav1::StageClock clk{};
+  int made = 0;
+  hipError_t e = hipSuccess;
+  for (; made <= ZAV1_WF_STAGES && e == hipSuccess; made++) e = hipEventCreate(&clk.ev[made]);
+  for (int k = 0; k < ZAV1_WF_STAGES; k++) stage_ms[k] = 0.0;
+  for (int i = 0; i < w->n_steps && e == hipSuccess; i++) {
+    if (w->step_off[i + 1] == w->step_off[i]) continue;
+    e = zav1::step_chain(w, i, st, &clk);
+    if (e == hipSuccess) e = hipEventSynchronize(clk.ev[ZAV1_WF_STAGES]);
+    for (int k = 0; k < ZAV1_WF_STAGES && e == hipSuccess; k++) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, clk.ev[k], clk.ev[k + 1]);
+      stage_ms[k] += ms;
+    }
+  }
+  for (int k = 0; k < made; k++)
+    if (clk.ev[k]) (void)hipEventDestroy(clk.ev[k]);
+  if (e != hipSuccess) return wf_fail(e, "zav1_wf_run_timed");
   return ZAV1_OK;
 }
 

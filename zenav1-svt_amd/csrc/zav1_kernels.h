@@ -209,14 +209,13 @@ struct RateJob {
 static_assert(sizeof(RateJob) == 12, "RateJob layout");
 hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* eobs, const RateJob* jobs,
                             int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t* rate, hipStream_t st);
+// trellis forms: by batch size (the autotuned threshold), lane, wave
+constexpr int kRdoqAuto = 0, kRdoqLane = 1, kRdoqWave = 2;
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st);
-size_t rdoq_workspace(int tx_size, int n_jobs);
-hipError_t rdoq_batch_ws(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
-                         const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                         int32_t dq1, uint32_t cut_num, uint32_t cut_den, void* ws, hipStream_t st);
-hipError_t rdoq_pool_trim();
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, int form = kRdoqAuto);
+int rdoq_wave_max(int tx_size);
+void rdoq_set_wave_max(int tx_size, int n);
 hipError_t quantize_batch(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff,
                           const QuantRow* rows, const uint8_t* qidx, int n_per_job, int n_jobs,
                           int log_scale, int bd, int kind, hipStream_t st);
@@ -337,7 +336,8 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
-                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr);
+                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr,
+                           hipEvent_t* rdoq_ev = nullptr);
 // The current device's quantizer row `row_index` (bd index * 256 + qindex),
 // inverse scans and forward-scan base, as built by zav1_init (no building).
 int device_tables(const QuantRow** row, const uint16_t** iscan, const uint16_t** fscan_base, int row_index);

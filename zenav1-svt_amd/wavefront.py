@@ -105,6 +105,17 @@ class WavefrontMD:
         if rc:
             raise self.z.Zav1Error(rc, 'zav1_wf_run (%s)' % self.z.lib().zav1_wf_last_error().decode())
 
+    STAGES = ('rates', 'mds0_satd', 'mds0_select', 'predict', 'mds1_jobs', 'mds1_units', 'mds3_jobs',
+              'mds3_fwd_quant', 'mds3_rdoq', 'mds3_rate_dist_recon', 'commit')
+
+    def run_timed(self, stream=None):
+        """Eager run with HIP events at every stage boundary: {stage: total ms}."""
+        out = (ctypes.c_double * len(self.STAGES))()
+        rc = self.z.lib().zav1_wf_run_timed(self.h, stream, out, len(self.STAGES))
+        if rc:
+            raise self.z.Zav1Error(rc, 'zav1_wf_run_timed (%s)' % self.z.lib().zav1_wf_last_error().decode())
+        return dict(zip(self.STAGES, list(out)))
+
     def run_plain(self, first, count, stream=None):
         rc = self.z.lib().zav1_wf_run_plain(self.h, first, count, stream)
         if rc:
@@ -120,6 +131,17 @@ class WavefrontMD:
         out = np.zeros(n, MD_UNIT)
         self.z.check(self.z.lib().zav1_memcpy_d2h(out.ctypes.data, self.info.d_units, out.nbytes, None), 'units')
         return out
+
+    def raw_decisions(self):
+        """The device's decision records in storage order (one per leaf)."""
+        d = np.zeros(self.info.n_leaves, DECISION)
+        self.z.check(self.z.lib().zav1_memcpy_d2h(d.ctypes.data, self.info.d_decisions, d.nbytes, None), 'dec')
+        return d
+
+    def set_raw_decisions(self, d):
+        d = np.ascontiguousarray(d, DECISION)
+        assert d.shape == (self.info.n_leaves,)
+        self.z.check(self.z.lib().zav1_memcpy_h2d(self.info.d_decisions, d.ctypes.data, d.nbytes, None), 'dec')
 
     def decisions(self):
         """Per leaf grid position: (n_images, height / leaf, width / leaf) decision records."""
