@@ -843,6 +843,27 @@ int zav1_cdef_search(const void *d_rec_y, const void *d_rec_u, const void *d_rec
                      int subsampling, const uint8_t *d_skip, int mi_cols, int mi_rows, int32_t *d_dirvar,
                      uint64_t *d_mse_y, uint64_t *d_mse_uv, zav1_stream_t s);
 
+/* The CDEF frame pass, decoder-exact: apply_cdef_frame / apply_cdef_frame_hbd
+ * (rust/crates/svtav1-encoder/src/cdef.rs:343 / :556), out of place.  The
+ * d_in_* planes are the post-deblock reconstruction (read only: every read
+ * is of the pre-CDEF value, as the reference's snapshot gives), d_out_*
+ * receive the filtered planes; every output pixel is written (the input
+ * where nothing filters), so outputs must not alias inputs.  Per 64x64
+ * filter block (raster, ceil(w/64) x ceil(h/64)) d_fb_strength[2 fb] and
+ * [2 fb + 1] are the luma and chroma strength indices the block applies
+ * (CdefPick::fb_strengths: primary idx / 4, secondary idx % 4 with 3 ->
+ * 4); damping is the frame's cdef damping (3..6; + bd - 8 inside).  d_skip
+ * (may be NULL = no skip) is the per-mi signaled skip map, mi_rows x
+ * mi_cols: 8x8s whose mi are all skip are not filtered.  Chroma (d_in_u /
+ * d_in_v, both or neither) is 4:2:0 with the luma directions.  width and
+ * height multiples of 8.  d_stats (may be NULL): CdefStats, [0] pixels of
+ * every filter invocation, [1] of those the pixels that changed (at both
+ * bit depths).  The frame-level gate (CdefPick::any) is the caller's: with
+ * all strengths 0 the outputs equal the inputs. */
+int zav1_cdef_apply(const void *d_in_y, const void *d_in_u, const void *d_in_v, void *d_out_y, void *d_out_u,
+                    void *d_out_v, int width, int height, int bd, int damping, const uint8_t *d_fb_strength,
+                    const uint8_t *d_skip, int mi_cols, int mi_rows, uint64_t *d_stats, zav1_stream_t s);
+
 /* Restoration-unit grid of a plane (count_units_in_tile,
  * rust/crates/svtav1-dsp/src/restoration.rs:1388).  Host only. */
 int zav1_rest_unit_grid(int unit_size, int plane_w, int plane_h, int *n_h, int *n_v);
@@ -857,10 +878,46 @@ int zav1_rest_unit_grid(int unit_size, int plane_w, int plane_h, int *n_h, int *
 int zav1_wiener_stats(const void *d_dgd, const void *d_src, int plane_w, int plane_h, int bd, int win, int unit_size,
                       int ss_y, int64_t *d_M, int64_t *d_H, zav1_stream_t s);
 
-/* SSE of two planes over every restoration unit (d_out[u]).  Replaces
- * sse_region / sse_region_hbd (restoration.rs:1666 / :1991). */
+/* SSE of two planes over every restoration unit (d_out[u]); at 10 bits the
+ * sum of highbd_variance's parts, each truncated to 32 bits (the right
+ * strip of width % 16 columns, the bottom strip left of it, the 16x16
+ * blocks).  Replaces sse_region / sse_region_hbd (restoration.rs:1666 /
+ * :1991). */
 int zav1_unit_sse(const void *d_a, const void *d_b, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
                   uint64_t *d_out, zav1_stream_t s);
+
+/* One restoration unit's Wiener filter (WienerInfo, restoration.rs:95):
+ * the 8-tap rows as signaled (taps 0..2 mirrored, centre -2 (t0 + t1 + t2),
+ * tap 7 = 0) and the unit's type (0 RESTORE_NONE, 1 RESTORE_WIENER). */
+typedef struct {
+  int16_t hfilter[8];
+  int16_t vfilter[8];
+  uint8_t type;
+  uint8_t pad[7];
+} zav1_wiener_unit;
+
+/* The Wiener unit filter over every restoration unit of a plane (grid and
+ * limits as zav1_rest_unit_grid / zav1_wiener_stats; d_units[u] in raster
+ * order): loop_restoration_filter_unit without stripe boundaries
+ * (rust/crates/svtav1-dsp/src/restoration.rs:1288 with need_boundaries = 0,
+ * its 10-bit twin loop_restoration_filter_unit_search_hbd :1921, over
+ * wiener_convolve_add_src(_hbd) :138 / :1820), reading d_dgd edge-extended
+ * by 3 (extend_frame) and writing every pixel of d_out (a RESTORE_NONE unit
+ * is copied).  unit_size 32, 64, 128 or 256; ss_y the plane's vertical
+ * subsampling. */
+int zav1_wiener_apply(const void *d_dgd, void *d_out, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
+                      const zav1_wiener_unit *d_units, zav1_stream_t s);
+
+/* finer_tile_search_wiener (rust/crates/svtav1-encoder/src/restoration.rs:
+ * 513) for every unit of a plane at once: from each unit's taps in d_units
+ * (the solver's filter), the SSE of the unit filtered (as
+ * zav1_wiener_apply) against d_src, then, with use_refinement, the +-step
+ * hill climb over taps (7 - win) / 2 .. 2 of hfilter, then vfilter (steps
+ * 4, 2, 1; 4 only with max_one_step).  Writes the refined taps back into
+ * d_units and the final SSE into d_err[u].  One workgroup per unit. */
+int zav1_wiener_refine(const void *d_dgd, const void *d_src, int plane_w, int plane_h, int bd, int win, int unit_size,
+                       int ss_y, int use_refinement, int max_one_step, zav1_wiener_unit *d_units, int64_t *d_err,
+                       zav1_stream_t s);
 
 /* Self-guided projection statistics of every restoration unit and each of
  * the 16 AV1 parameter sets: d_out[(u*16 + set)*5 + {0..4}] = H00, H01,

@@ -453,6 +453,96 @@ void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, 
   free(pad);
 }
 
+/* enc/cdef.rs:343 apply_cdef_frame / :556 apply_cdef_frame_hbd: the
+ * decoder-exact CDEF frame pass, out of place (every read is of the
+ * post-deblock, pre-CDEF planes in_*; every pixel of out_* is written: the
+ * filtered value, or the input where nothing filters).  Per 64x64 fb
+ * (raster): the dlist of 8x8s not all-skip (skip NULL: every 8x8); none ->
+ * the fb is copied.  Strength index per fb fb_str[2 fb] (luma),
+ * [2 fb + 1] (chroma): primary idx / 4, secondary idx % 4 with 3 -> 4, both
+ * << coeff_shift; damping + coeff_shift (chroma one less).  Luma: the
+ * direction search always runs (chroma reuses it); a block filters when its
+ * variance-adjusted primary or the secondary is nonzero, along its
+ * direction when the fb's primary is nonzero, else direction 0.  Chroma
+ * (in_u non-NULL, 4:2:0): 4x4 blocks of the same dlist, unadjusted primary.
+ * stats (may be NULL): [0] pixels of every filter invocation, [1] of those
+ * the pixels that changed. */
+void or_cdef_apply_frame(const void *in_y, const void *in_u, const void *in_v, void *out_y, void *out_u,
+                         void *out_v, int width, int height, int bd, int damping, const uint8_t *fb_str,
+                         const uint8_t *skip, int mi_cols, int mi_rows, uint64_t *stats) {
+  const int hbd = bd > 8, cs = bd - 8;
+  const int nhfb = (width + 63) / 64, nvfb = (height + 63) / 64;
+  const int cw = width / 2, ch = height / 2;
+  uint16_t *pad = (uint16_t *)malloc(sizeof(uint16_t) * BSTRIDE * (64 + 2 * VB));
+  const int base = VB * BSTRIDE + HB;
+  int32_t blk[64];
+  uint64_t filtered = 0, changed = 0;
+  /* start from the input: positions nothing filters keep it */
+  for (long i = 0; i < (long)width * height; i++) st(out_y, hbd, i, ld(in_y, hbd, i));
+  if (in_u)
+    for (long i = 0; i < (long)cw * ch; i++) st(out_u, hbd, i, ld(in_u, hbd, i)), st(out_v, hbd, i, ld(in_v, hbd, i));
+  for (int fbr = 0; fbr < nvfb; fbr++)
+    for (int fbc = 0; fbc < nhfb; fbc++) {
+      const int vsize = height - fbr * 64 < 64 ? height - fbr * 64 : 64;
+      const int hsize = width - fbc * 64 < 64 ? width - fbc * 64 : 64;
+      int act[8][8], dir[8][8], var[8][8], n_act = 0;
+      for (int by = 0; by < 8; by++)
+        for (int bx = 0; bx < 8; bx++) {
+          act[by][bx] = by < vsize / 8 && bx < hsize / 8 &&
+                        (!skip || !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + by * 2, fbc * 16 + bx * 2));
+          n_act += act[by][bx];
+        }
+      if (!n_act) continue;
+      const int fb = fbr * nhfb + fbc, sy = fb_str[2 * fb], suv = fb_str[2 * fb + 1];
+      const int lvl_y = (sy / 4) << cs, sec_y = ((sy % 4) + (sy % 4 == 3)) << cs;
+      const int lvl_uv = in_u ? (suv / 4) << cs : 0, sec_uv = in_u ? ((suv % 4) + (suv % 4 == 3)) << cs : 0;
+      const int dmp = damping + cs;
+      build_src(pad, in_y, hbd, width, height, fbr * 64, fbc * 64, vsize, hsize);
+      for (int by = 0; by < 8; by++)
+        for (int bx = 0; bx < 8; bx++)
+          if (act[by][bx]) dir[by][bx] = or_cdef_find_dir(pad + base + by * 8 * BSTRIDE + bx * 8, BSTRIDE, cs, &var[by][bx]);
+      if (lvl_y || sec_y)
+        for (int by = 0; by < 8; by++)
+          for (int bx = 0; bx < 8; bx++) {
+            if (!act[by][bx]) continue;
+            const int t = or_cdef_adjust_strength(lvl_y, var[by][bx]);
+            if (t == 0 && sec_y == 0) continue;
+            const uint16_t *in = pad + base + by * 8 * BSTRIDE + bx * 8;
+            or_cdef_filter_block(blk, 8, in, t, sec_y, lvl_y ? dir[by][bx] : 0, dmp, dmp, 8, 8, cs, 1);
+            for (int i = 0; i < 8; i++)
+              for (int j = 0; j < 8; j++) {
+                const long o = (long)(fbr * 64 + by * 8 + i) * width + fbc * 64 + bx * 8 + j;
+                changed += blk[i * 8 + j] != (int32_t)in[i * BSTRIDE + j];
+                st(out_y, hbd, o, blk[i * 8 + j]);
+              }
+            filtered += 64;
+          }
+      if (in_u && (lvl_uv || sec_uv)) {
+        const void *ins[2] = {in_u, in_v};
+        void *outs[2] = {out_u, out_v};
+        for (int pl = 0; pl < 2; pl++) {
+          build_src(pad, ins[pl], hbd, cw, ch, fbr * 32, fbc * 32, vsize / 2, hsize / 2);
+          for (int by = 0; by < 8; by++)
+            for (int bx = 0; bx < 8; bx++) {
+              if (!act[by][bx]) continue;
+              const uint16_t *in = pad + base + by * 4 * BSTRIDE + bx * 4;
+              or_cdef_filter_block(blk, 4, in, lvl_uv, sec_uv, lvl_uv ? dir[by][bx] : 0, dmp - 1, dmp - 1, 4, 4, cs,
+                                   1);
+              for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                  const long o = (long)(fbr * 32 + by * 4 + i) * cw + fbc * 32 + bx * 4 + j;
+                  changed += blk[i * 4 + j] != (int32_t)in[i * BSTRIDE + j];
+                  st(outs[pl], hbd, o, blk[i * 4 + j]);
+                }
+              filtered += 16;
+            }
+        }
+      }
+    }
+  free(pad);
+  if (stats) stats[0] = filtered, stats[1] = changed;
+}
+
 /* enc/deblock.rs:208 plane_sse + :228 try_filter_plane: SSE of the source
  * against a scratch copy of the recon filtered at (level, level). */
 static int64_t try_level(const void *recon, const void *src, int pw, int ph, int bd, int plane, int ss, int sharpness,

@@ -399,6 +399,9 @@ def _bind_lr(L):
     L.or_wiener_stats.argtypes = [ctypes.c_int, _vp, _vp] + [ctypes.c_int] * 7 + [_vp, _vp]
     L.or_sse_region.argtypes = [_vp, _vp] + [ctypes.c_int] * 6
     L.or_sse_region.restype = ctypes.c_uint64
+    L.or_wiener_filter_plane.argtypes = [_vp, _vp] + [ctypes.c_int] * 5 + [_vp, _vp]
+    L.or_wiener_refine_unit.argtypes = [_vp, _vp] + [ctypes.c_int] * 10 + [_vp, _vp]
+    L.or_wiener_refine_unit.restype = ctypes.c_int64
     L.or_sgr_param.argtypes = [ctypes.c_int, ctypes.c_int]
     L.or_sgr_unit_stats.argtypes = [_vp, _vp] + [ctypes.c_int] * 8 + [_vp]
 
@@ -413,6 +416,27 @@ def unit_limits(unit_size, pw, ph, ss_y, ux, uy):
     v = [ctypes.c_int() for _ in range(4)]
     lrlib().or_unit_limits(unit_size, pw, ph, ss_y, ux, uy, *[ctypes.byref(x) for x in v])
     return tuple(x.value for x in v)
+
+
+def wiener_filter_plane(dgd, bd, unit_size, ss_y, types, filters):
+    """or_wiener_filter_plane: types (n_units,) 0/1, filters (n_units, 2, 8)
+    int16 (hfilter, vfilter); returns the filtered plane."""
+    d = np.ascontiguousarray(dgd)
+    out = np.zeros_like(d)
+    t = np.ascontiguousarray(types, np.uint8)
+    f = np.ascontiguousarray(filters, np.int16)
+    lrlib().or_wiener_filter_plane(d.ctypes.data, out.ctypes.data, d.shape[1], d.shape[0], bd, unit_size, ss_y,
+                                   t.ctypes.data, f.ctypes.data)
+    return out
+
+
+def wiener_refine_unit(dgd, src, bd, win, lim, hf, vf, use_refinement=True, max_one_step=False):
+    """or_wiener_refine_unit: returns (sse, hfilter, vfilter)."""
+    d, s_ = np.ascontiguousarray(dgd), np.ascontiguousarray(src)
+    h, v = np.array(hf, np.int16), np.array(vf, np.int16)
+    e = lrlib().or_wiener_refine_unit(d.ctypes.data, s_.ctypes.data, d.shape[1], d.shape[0], bd, win, *lim,
+                                      int(use_refinement), int(max_one_step), h.ctypes.data, v.ctypes.data)
+    return int(e), h, v
 
 
 def wiener_stats(win, dgd, src, bd, lim):
@@ -521,6 +545,30 @@ def intra_tx_type_used(tx_size, tx_type):
     L = lib()
     L.or_intra_tx_type_used.argtypes = [ctypes.c_int] * 2
     return bool(L.or_intra_tx_type_used(tx_size, tx_type))
+
+
+def cdef_apply_frame(y, u, v, bd, damping, fb_str, skip=None):
+    """or_cdef_apply_frame: the CDEF frame pass of planes y (h x w) and u, v
+    (h/2 x w/2, or None) with per-fb strength pairs fb_str (n_fb x 2) and
+    the per-mi skip map (mi_rows x mi_cols, or None); returns
+    (out_y, out_u, out_v, [filtered_px, changed_px])."""
+    L = lib()
+    L.or_cdef_apply_frame.argtypes = [_vp] * 6 + [ctypes.c_int] * 4 + [_vp, _vp, ctypes.c_int, ctypes.c_int, _vp]
+    y = np.ascontiguousarray(y)
+    h, w = y.shape
+    oy = np.zeros_like(y)
+    ou = ov = None
+    if u is not None:
+        u, v = np.ascontiguousarray(u, y.dtype), np.ascontiguousarray(v, y.dtype)
+        ou, ov = np.zeros_like(u), np.zeros_like(v)
+    fs = np.ascontiguousarray(fb_str, np.uint8)
+    sk = None if skip is None else np.ascontiguousarray(skip, np.uint8)
+    mr, mc = (0, 0) if sk is None else sk.shape
+    stats = np.zeros(2, np.uint64)
+    ptr = lambda a: None if a is None else a.ctypes.data
+    L.or_cdef_apply_frame(ptr(y), ptr(u), ptr(v), ptr(oy), ptr(ou), ptr(ov), w, h, bd, damping, fs.ctypes.data,
+                          ptr(sk), mc, mr, stats.ctypes.data)
+    return oy, ou, ov, stats
 
 
 def wf_frame(cfg, images):

@@ -10,6 +10,10 @@
  *   dgd border                enc/restoration.rs:740 extend_frame (edge
  *                             replication)
  *   sse_region(_hbd)          dsp/restoration.rs:1666 / :1991
+ *   Wiener unit filter        dsp/restoration.rs:138 / :1820 wiener_convolve_add_src(_hbd),
+ *                             :1288 loop_restoration_filter_unit (search path),
+ *                             :1921 loop_restoration_filter_unit_search_hbd
+ *   finer tap search          enc/restoration.rs:513 finer_tile_search_wiener
  *
  * Self-guided statistics have NO counterpart in the reference
  * (enc/restoration.rs:5-9); the restatement below follows the algorithm
@@ -87,15 +91,148 @@ void or_wiener_stats(int win, const void *dgd, const void *src, int pw, int ph, 
 }
 
 /* sse_region(_hbd): SSE of the unfiltered reconstruction over the unit. */
-uint64_t or_sse_region(const void *a, const void *b, int pw, int bd, int hs, int he, int vs, int ve) {
-  const int hbd = bd > 8;
+static uint64_t sse_rect(const void *a, const void *b, int hbd, int pw, int x0, int x1, int y0, int y1) {
   uint64_t d = 0;
-  for (int i = vs; i < ve; i++)
-    for (int j = hs; j < he; j++) {
+  for (int i = y0; i < y1; i++)
+    for (int j = x0; j < x1; j++) {
       int64_t e = (int64_t)S(a, hbd, (long)i * pw + j) - S(b, hbd, (long)i * pw + j);
       d += (uint64_t)(e * e);
     }
   return d;
+}
+
+/* sse_region: the plain sum at 8 bits; at 10 bits highbd_variance's parts,
+ * each truncated to 32 bits before the sum (dsp/restoration.rs:1991): the
+ * right strip of width % 16 columns over every row, the bottom strip of
+ * height % 16 rows left of it, and the 16x16 blocks. */
+uint64_t or_sse_region(const void *a, const void *b, int pw, int bd, int hs, int he, int vs, int ve) {
+  const int hbd = bd > 8;
+  if (!hbd) return sse_rect(a, b, 0, pw, hs, he, vs, ve);
+  const int w = he - hs, h = ve - vs, dw = w % 16, dh = h % 16;
+  uint64_t t = 0;
+  if (dw) t += (uint32_t)sse_rect(a, b, 1, pw, he - dw, he, vs, ve);
+  if (dh) t += (uint32_t)sse_rect(a, b, 1, pw, hs, he - dw, ve - dh, ve);
+  for (int y = 0; y < h / 16; y++)
+    for (int x = 0; x < w / 16; x++)
+      t += (uint32_t)sse_rect(a, b, 1, pw, hs + 16 * x, hs + 16 * x + 16, vs + 16 * y, vs + 16 * y + 16);
+  return t;
+}
+
+/* ------------------------------------------------------------------ */
+/* Wiener unit filter and the finer tap search                          */
+/* ------------------------------------------------------------------ */
+/* wiener_convolve_add_src(_hbd) (dsp/restoration.rs:138 / :1820) at one
+ * output pixel of the edge-extended plane: the horizontal pass over rows
+ * y - 3 .. y + 3 (round 3, clamped to [0, 2^(bd + 5) - 1]), then the
+ * vertical pass (round 11, clamped to the pixel range).  The 8th taps are
+ * 0.  Per output pixel this is what the stripe / processing-unit loops of
+ * loop_restoration_filter_unit (:1288, need_boundaries = 0) and
+ * loop_restoration_filter_unit_search_hbd (:1921) compute. */
+static int wiener_px(const void *dgd, int hbd, int bd, int pw, int ph, int x, int y, const int16_t *hf,
+                     const int16_t *vf) {
+  const int clamp_h = (1 << (bd + 1 + 7 - 3)) - 1, mx = (1 << bd) - 1;
+  int32_t t[7];
+  for (int r = 0; r < 7; r++) {
+    int32_t sum = (E(dgd, hbd, pw, ph, x, y + r - 3) << 7) + (1 << (bd + 7 - 1));
+    for (int k = 0; k < 7; k++) sum += E(dgd, hbd, pw, ph, x + k - 3, y + r - 3) * hf[k];
+    int32_t v = (sum + 4) >> 3;
+    t[r] = v < 0 ? 0 : (v > clamp_h ? clamp_h : v);
+  }
+  int32_t sum = (t[3] << 7) - (1 << (bd + 11 - 1));
+  for (int k = 0; k < 7; k++) sum += t[k] * vf[k];
+  int32_t v = (sum + 1024) >> 11;
+  return v < 0 ? 0 : (v > mx ? mx : v);
+}
+
+/* The unit filter over every restoration unit of a plane
+ * (loop_restoration_filter_unit without stripe boundaries, as the search
+ * runs it): types[u] 0 copies the unit, 1 filters it with filters[u * 16]
+ * (hfilter[8]) and [u * 16 + 8] (vfilter[8]). */
+void or_wiener_filter_plane(const void *dgd, void *out, int pw, int ph, int bd, int unit_size, int ss_y,
+                            const uint8_t *types, const int16_t *filters) {
+  const int hbd = bd > 8, nh = or_count_units(unit_size, pw), nv = or_count_units(unit_size, ph);
+  for (int uy = 0; uy < nv; uy++)
+    for (int ux = 0; ux < nh; ux++) {
+      int hs, he, vs, ve;
+      or_unit_limits(unit_size, pw, ph, ss_y, ux, uy, &hs, &he, &vs, &ve);
+      const int u = uy * nh + ux;
+      for (int y = vs; y < ve; y++)
+        for (int x = hs; x < he; x++) {
+          const int v = types[u] ? wiener_px(dgd, hbd, bd, pw, ph, x, y, filters + u * 16, filters + u * 16 + 8)
+                                 : S(dgd, hbd, (long)y * pw + x);
+          if (hbd) ((uint16_t *)out)[(long)y * pw + x] = (uint16_t)v;
+          else ((uint8_t *)out)[(long)y * pw + x] = (uint8_t)v;
+        }
+    }
+}
+
+/* try_restoration_unit (enc/restoration.rs:472): the unit filtered with
+ * (hf, vf), SSE against the source over the unit. */
+static int64_t try_unit(const void *dgd, const void *src, int hbd, int bd, int pw, int ph, int hs, int he, int vs,
+                        int ve, const int16_t *hf, const int16_t *vf, void *trial) {
+  for (int y = vs; y < ve; y++)
+    for (int x = hs; x < he; x++) {
+      const int v = wiener_px(dgd, hbd, bd, pw, ph, x, y, hf, vf);
+      if (hbd) ((uint16_t *)trial)[(long)y * pw + x] = (uint16_t)v;
+      else ((uint8_t *)trial)[(long)y * pw + x] = (uint8_t)v;
+    }
+  return (int64_t)or_sse_region(src, trial, pw, bd, hs, he, vs, ve);
+}
+
+/* finer_tile_search_wiener (enc/restoration.rs:513): the unit's SSE with
+ * (hf, vf), then (use_refinement) the +-step hill climb over the taps
+ * plane_off .. 2 of hfilter, then of vfilter, steps 4, 2, 1 (4 only with
+ * max_one_step), each tap moved with its mirror and the centre rebalanced,
+ * kept when the SSE does not grow; at the start step a kept move is
+ * repeated, and a kept minus move ends that pass over the taps (the
+ * reference's break out of the tap loop).  Returns the final SSE; hf / vf
+ * are updated in place. */
+int64_t or_wiener_refine_unit(const void *dgd, const void *src, int pw, int ph, int bd, int win, int hs, int he,
+                              int vs, int ve, int use_refinement, int max_one_step, int16_t *hf, int16_t *vf) {
+  static const int16_t tmin[3] = {3 - 8, -7 - 16, 15 - 32}, tmax[3] = {3 - 1 + 8, -7 - 1 + 16, 15 - 1 + 32};
+  const int hbd = bd > 8;
+  void *trial = calloc((size_t)pw * ph, hbd ? 2 : 1);
+  int64_t err = try_unit(dgd, src, hbd, bd, pw, ph, hs, he, vs, ve, hf, vf, trial);
+  if (use_refinement) {
+    const int plane_off = (7 - win) >> 1, start = 4, end = max_one_step ? 4 : 1;
+    for (int s = start; s >= end; s >>= 1)
+      for (int pass = 0; pass < 2; pass++) {
+        int16_t *f = pass ? vf : hf;
+        for (int p = plane_off; p < 3; p++) {
+          int skip = 0;
+          for (;;) { /* minus */
+            if (f[p] - s >= tmin[p]) {
+              f[p] -= s, f[6 - p] -= s, f[3] += 2 * s;
+              const int64_t e2 = try_unit(dgd, src, hbd, bd, pw, ph, hs, he, vs, ve, hf, vf, trial);
+              if (e2 > err) {
+                f[p] += s, f[6 - p] += s, f[3] -= 2 * s;
+              } else {
+                err = e2;
+                skip = 1;
+                if (s == start && !max_one_step) continue;
+              }
+            }
+            break;
+          }
+          if (skip) break; /* a kept minus move ends the pass over the taps */
+          for (;;) { /* plus */
+            if (f[p] + s <= tmax[p]) {
+              f[p] += s, f[6 - p] += s, f[3] -= 2 * s;
+              const int64_t e2 = try_unit(dgd, src, hbd, bd, pw, ph, hs, he, vs, ve, hf, vf, trial);
+              if (e2 > err) {
+                f[p] -= s, f[6 - p] -= s, f[3] += 2 * s;
+              } else {
+                err = e2;
+                if (s == start && !max_one_step) continue;
+              }
+            }
+            break;
+          }
+        }
+      }
+  }
+  free(trial);
+  return err;
 }
 
 /* ------------------------------------------------------------------ */

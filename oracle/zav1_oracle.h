@@ -198,6 +198,10 @@ uint64_t or_cdef_dist(const void *plane, int hbd, long plane_off, int pstride, c
                       const uint8_t *dlist, int n, int bsize, int coeff_shift, int sub);
 void or_cdef_filter_block(int32_t *dst, int dstride, const uint16_t *in, int pri, int sec, int dir, int pdamp,
                           int sdamp, int bw, int bh, int coeff_shift, int sub);
+/* enc/cdef.rs:343 apply_cdef_frame / :556 apply_cdef_frame_hbd (out of place) */
+void or_cdef_apply_frame(const void *in_y, const void *in_u, const void *in_v, void *out_y, void *out_u,
+                         void *out_v, int width, int height, int bd, int damping, const uint8_t *fb_str,
+                         const uint8_t *skip, int mi_cols, int mi_rows, uint64_t *stats);
 void or_cdef_search_fb(const void *rec_y, const void *rec_u, const void *rec_v, const void *src_y,
                        const void *src_u, const void *src_v, int width, int height, int bd, int qindex,
                        int subsampling, int fbr, int fbc, const int *fs, int n_fs, uint64_t *mse_y,
@@ -209,6 +213,13 @@ void or_unit_limits(int unit_size, int pw, int ph, int ss_y, int ux, int uy, int
 void or_wiener_stats(int win, const void *dgd, const void *src, int pw, int ph, int bd, int hs, int he, int vs,
                      int ve, int64_t *M, int64_t *H);
 uint64_t or_sse_region(const void *a, const void *b, int pw, int bd, int hs, int he, int vs, int ve);
+/* dsp/restoration.rs:1288 / :1921 the Wiener unit filter (no stripe
+ * boundaries) over every unit of a plane; enc/restoration.rs:513 the finer
+ * tap search of one unit */
+void or_wiener_filter_plane(const void *dgd, void *out, int pw, int ph, int bd, int unit_size, int ss_y,
+                            const uint8_t *types, const int16_t *filters);
+int64_t or_wiener_refine_unit(const void *dgd, const void *src, int pw, int ph, int bd, int win, int hs, int he,
+                              int vs, int ve, int use_refinement, int max_one_step, int16_t *hf, int16_t *vf);
 int or_sgr_param(int set, int k);
 int or_sgr_table(int which, int i);
 void or_sgr_unit_stats(const void *dgd, const void *src, int pw, int ph, int bd, int hs, int he, int vs, int ve,

@@ -130,6 +130,9 @@ EXPORTS = {
     'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
     'zav1_coeff_rate_batch': [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp],
     'zav1_rdoq_wave_max': [_i32, _i32],
+    'zav1_cdef_apply': [_vp] * 6 + [_i32] * 4 + [_vp, _vp, _i32, _i32, _vp, _vp],
+    'zav1_wiener_apply': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
+    'zav1_wiener_refine': [_vp, _vp] + [_i32] * 8 + [_vp, _vp, _vp],
     'zav1_wf_run_timed': [_vp, _vp, _vp, _i32],
     'zav1_rdoq_batch': [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                         _vp],
@@ -429,6 +432,31 @@ def cdef_search(rec, src, width, height, bd, qindex, subsampling, d_dirvar, d_ms
     check(lib().zav1_cdef_search(ptr(rec[0]), ptr(rec[1]), ptr(rec[2]), ptr(src[0]), ptr(src[1]), ptr(src[2]), width,
                                  height, bd, qindex, subsampling, ptr(d_skip), mi_cols, mi_rows, ptr(d_dirvar),
                                  d_mse_y.ptr, ptr(d_mse_uv), stream), 'cdef_search')
+
+
+def cdef_apply(d_in, d_out, width, height, bd, damping, d_fb_strength, d_skip=None, mi_cols=0, mi_rows=0,
+               d_stats=None, stream=None):
+    """The CDEF frame pass (zav1_cdef_apply): d_in / d_out (y, u, v)
+    DeviceArrays (u, v may be None), d_fb_strength (n_fb, 2) u8."""
+    ptr = lambda a: a.ptr if a is not None else None  # noqa: E731
+    check(lib().zav1_cdef_apply(ptr(d_in[0]), ptr(d_in[1]), ptr(d_in[2]), ptr(d_out[0]), ptr(d_out[1]),
+                                ptr(d_out[2]), width, height, bd, damping, d_fb_strength.ptr, ptr(d_skip), mi_cols,
+                                mi_rows, ptr(d_stats), stream), 'cdef_apply')
+
+
+WIENER_UNIT = np.dtype([('hfilter', '<i2', 8), ('vfilter', '<i2', 8), ('type', 'u1'), ('pad', 'u1', 7)])
+
+
+def wiener_apply(d_dgd, d_out, plane_w, plane_h, bd, unit_size, ss_y, d_units, stream=None):
+    check(lib().zav1_wiener_apply(d_dgd.ptr, d_out.ptr, plane_w, plane_h, bd, unit_size, ss_y, d_units.ptr, stream),
+          'wiener_apply')
+
+
+def wiener_refine(d_dgd, d_src, plane_w, plane_h, bd, win, unit_size, ss_y, d_units, d_err, use_refinement=True,
+                  max_one_step=False, stream=None):
+    check(lib().zav1_wiener_refine(d_dgd.ptr, d_src.ptr, plane_w, plane_h, bd, win, unit_size, ss_y,
+                                   int(use_refinement), int(max_one_step), d_units.ptr, d_err.ptr, stream),
+          'wiener_refine')
 
 
 def rest_unit_grid(unit_size, plane_w, plane_h):

@@ -1467,6 +1467,48 @@ int zav1_cdef_search(const void* d_rec_y, const void* d_rec_u, const void* d_rec
   return ZAV1_OK;
 }
 
+int zav1_cdef_apply(const void* d_in_y, const void* d_in_u, const void* d_in_v, void* d_out_y, void* d_out_u,
+                    void* d_out_v, int width, int height, int bd, int damping, const uint8_t* d_fb_strength,
+                    const uint8_t* d_skip, int mi_cols, int mi_rows, uint64_t* d_stats, zav1_stream_t s) {
+  if ((bd != 8 && bd != 10) || width <= 0 || height <= 0 || (width & 7) || (height & 7) || damping < 3 ||
+      damping > 6 || !d_in_y || !d_out_y || !d_fb_strength || d_in_y == d_out_y)
+    return ZAV1_EINVAL;
+  if ((d_in_u == nullptr) != (d_in_v == nullptr) || (d_in_u && (!d_out_u || !d_out_v || d_in_u == d_out_u ||
+                                                                d_in_v == d_out_v)))
+    return ZAV1_EINVAL;
+  if (d_skip && (mi_cols < (width + 3) / 4 || mi_rows < (height + 3) / 4)) return ZAV1_EINVAL;
+  ZCHECK(zav1::cdef_apply(d_in_y, d_in_u, d_in_v, d_out_y, d_out_u, d_out_v, width, height, bd, damping,
+                          d_fb_strength, d_skip, mi_cols, mi_rows, d_stats, S(s)));
+  return ZAV1_OK;
+}
+
+static bool lr_unit_size(int u) { return u == 32 || u == 64 || u == 128 || u == 256; }
+
+int zav1_wiener_apply(const void* d_dgd, void* d_out, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
+                      const zav1_wiener_unit* d_units, zav1_stream_t s) {
+  if (!d_dgd || !d_out || d_dgd == d_out || !d_units || plane_w <= 0 || plane_h <= 0 || (bd != 8 && bd != 10) ||
+      !lr_unit_size(unit_size) || (ss_y != 0 && ss_y != 1))
+    return ZAV1_EINVAL;
+  int nh = 0, nv = 0;
+  zav1_rest_unit_grid(unit_size, plane_w, plane_h, &nh, &nv);
+  ZCHECK(zav1::wiener_apply(d_dgd, d_out, plane_w, plane_h, bd, unit_size, ss_y, nh, nv, d_units, S(s)));
+  return ZAV1_OK;
+}
+
+int zav1_wiener_refine(const void* d_dgd, const void* d_src, int plane_w, int plane_h, int bd, int win, int unit_size,
+                       int ss_y, int use_refinement, int max_one_step, zav1_wiener_unit* d_units, int64_t* d_err,
+                       zav1_stream_t s) {
+  if (!d_dgd || !d_src || !d_units || !d_err || plane_w <= 0 || plane_h <= 0 || (bd != 8 && bd != 10) ||
+      (win != 5 && win != 7) || !lr_unit_size(unit_size) || (ss_y != 0 && ss_y != 1) ||
+      (use_refinement != 0 && use_refinement != 1) || (max_one_step != 0 && max_one_step != 1))
+    return ZAV1_EINVAL;
+  int nh = 0, nv = 0;
+  zav1_rest_unit_grid(unit_size, plane_w, plane_h, &nh, &nv);
+  ZCHECK(zav1::wiener_refine(d_dgd, d_src, plane_w, plane_h, bd, win, unit_size, ss_y, nh, nv, use_refinement,
+                             max_one_step, d_units, d_err, S(s)));
+  return ZAV1_OK;
+}
+
 int zav1_rest_unit_grid(int unit_size, int plane_w, int plane_h, int* n_h, int* n_v) {
   if (unit_size < 8 || unit_size > 256 || (unit_size & (unit_size - 1)) || plane_w <= 0 || plane_h <= 0 || !n_h ||
       !n_v)

@@ -363,19 +363,20 @@ struct PixTaps {
   int32_t mn, mx;
 };
 
-__device__ __forceinline__ void gather_taps(const uint16_t* c, int dir, PixTaps& pt) {
+__device__ __forceinline__ void gather_taps(const uint16_t* c, int dir, PixTaps& pt,
+                                            const int (*dirs)[2] = kCdefDir) {
   const int32_t x = (int16_t)c[0];
   int32_t mn = x, mx = x;
 #pragma unroll
   for (int k = 0; k < 2; k++) {
-    const int dk = kCdefDir[dir + 2][k];
+    const int dk = dirs[dir + 2][k];
     const int32_t p0 = (int16_t)c[dk], p1 = (int16_t)c[-dk];
     pt.d_p[2 * k] = p0 - x;
     pt.d_p[2 * k + 1] = p1 - x;
     if ((uint16_t)p0 != kLarge) mx = max(mx, p0);
     if ((uint16_t)p1 != kLarge) mx = max(mx, p1);
     mn = min(mn, min(p0, p1));
-    const int da = kCdefDir[dir + 4][k], db = kCdefDir[dir][k];
+    const int da = dirs[dir + 4][k], db = dirs[dir][k];
     const int32_t s[4] = {(int16_t)c[da], (int16_t)c[-da], (int16_t)c[db], (int16_t)c[-db]};
 #pragma unroll
     for (int m = 0; m < 4; m++) {
@@ -691,6 +692,160 @@ __global__ __launch_bounds__(256) void k_cdef_chroma(const Pix* __restrict__ rec
   if (tid < 64) mse[(long)fb * 64 + tid] = total[tid];
 }
 
+// The CDEF frame pass (apply_cdef_frame / _hbd, enc/cdef.rs:343 / :556),
+// out of place: one workgroup per 64x64 filter block reads the pre-CDEF
+// planes only, so filter blocks are independent.  The fb's luma tile (with
+// the 3-row / 8-column border, kLarge outside the plane) is staged in LDS,
+// the 8x8 directions are found, luma filters, then each chroma plane's tile
+// replaces it and its 4x4 blocks filter along the luma directions.  Every
+// pixel of the fb is written (the input where nothing filters).
+namespace {
+__device__ __forceinline__ int32_t cdef_pixel(const uint16_t* cp, int pri, int sec, int dir, int dmp, int cs,
+                                              const int (*dirs)[2]) {
+  PixTaps pt;
+  gather_taps(cp, dir, pt, dirs);
+  int32_t sum = pri ? pri_sum(pt, pri, (pri >> cs) & 1, dmp) : 0;
+  if (sec) {
+    const int32_t sh = max(0, dmp - msb32((uint32_t)sec));
+#pragma unroll
+    for (int m = 0; m < 8; m++) sum += (m < 4 ? 2 : 1) * constrain(pt.d_s[m], sec, sh);
+  }
+  return cdef_out((int16_t)cp[0], sum, pt.mn, pt.mx);
+}
+}  // namespace
+
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_cdef_apply(const Pix* __restrict__ in_y, const Pix* __restrict__ in_u,
+                                                    const Pix* __restrict__ in_v, Pix* __restrict__ out_y,
+                                                    Pix* __restrict__ out_u, Pix* __restrict__ out_v, int width,
+                                                    int height, int bd, int damping, int nhfb,
+                                                    const uint8_t* __restrict__ fb_str,
+                                                    const uint8_t* __restrict__ skip, int mi_cols, int mi_rows,
+                                                    unsigned long long* __restrict__ stats) {
+  __shared__ uint16_t tile[70 * kTS];
+  __shared__ int32_t sdir[64], svar[64];
+  __shared__ uint8_t sact[64];
+  __shared__ int sdirs[12][2];
+  __shared__ unsigned long long red[2];
+  const int fb = xcd_chunked(blockIdx.x, gridDim.x);
+  const int fbr = fb / nhfb, fbc = fb - fbr * nhfb;
+  const int cs = bd - 8;
+  const int y0 = fbr * 64, x0 = fbc * 64;
+  const int vsize = min(64, height - y0), hsize = min(64, width - x0);
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 70 * kTS; i += 256) {
+    const int r = i / kTS, c = i - r * kTS;
+    const int gy = y0 + r - kVB, gx = x0 + c - kHB;
+    const bool in = (r < vsize + 2 * kVB) && (c < hsize + 2 * kHB) && gy >= 0 && gy < height && gx >= 0 &&
+                    gx < width;
+    tile[i] = in ? (uint16_t)in_y[(long)gy * width + gx] : kLarge;
+  }
+  if (tid < 24) sdirs[tid >> 1][tid & 1] = kCdefDir[tid >> 1][tid & 1];
+  if (tid < 2) red[tid] = 0;
+  __syncthreads();
+  const int nbx = hsize >> 3, nby = vsize >> 3;
+  if (tid < 64) {
+    const int by = tid >> 3, bx = tid & 7;
+    int d = 0, v = 0;
+    const bool act = by < nby && bx < nbx && !all_skip_8x8(skip, mi_cols, mi_rows, fbr * 16 + by * 2, fbc * 16 + bx * 2);
+    if (act) d = find_dir(tile + (kVB + by * 8) * kTS + kHB + bx * 8, cs, &v);
+    sdir[tid] = d;
+    svar[tid] = v;
+    sact[tid] = act;
+  }
+  __syncthreads();
+  // the fb's strengths: primary idx / 4, secondary idx % 4 (3 -> 4), << cs
+  const int sy = fb_str[2 * fb], suv = fb_str[2 * fb + 1];
+  const int lvl_y = (sy >> 2) << cs, sec_y = ((sy & 3) + ((sy & 3) == 3)) << cs;
+  const int lvl_uv = in_u ? (suv >> 2) << cs : 0, sec_uv = in_u ? ((suv & 3) + ((suv & 3) == 3)) << cs : 0;
+  const int dmp = damping + cs;
+  uint32_t n_f = 0, n_c = 0;
+  for (int p = tid; p < 4096; p += 256) {
+    const int r = p >> 6, c = p & 63;
+    if (r >= vsize || c >= hsize) continue;
+    const int blk = (r >> 3) * 8 + (c >> 3);
+    const uint16_t* cp = tile + (kVB + r) * kTS + kHB + c;
+    int32_t v = cp[0];
+    if ((lvl_y || sec_y) && sact[blk]) {
+      const int t = adjust_strength(lvl_y, svar[blk]);
+      if (t || sec_y) {
+        const int32_t f = cdef_pixel(cp, t, sec_y, lvl_y ? sdir[blk] : 0, dmp, cs, sdirs);
+        n_f++;
+        n_c += f != v;
+        v = f;
+      }
+    }
+    out_y[(long)(y0 + r) * width + x0 + c] = (Pix)v;
+  }
+  if (in_u) {
+    const int cw = width >> 1, chh = height >> 1, cy0 = fbr * 32, cx0 = fbc * 32;
+    const int cv = vsize >> 1, chs = hsize >> 1;
+    for (int pl = 0; pl < 2; pl++) {
+      const Pix* in = pl ? in_v : in_u;
+      Pix* out = pl ? out_v : out_u;
+      __syncthreads();  // the previous tile is consumed
+      for (int i = tid; i < 38 * kTS; i += 256) {
+        const int r = i / kTS, c = i - r * kTS;
+        const int gy = cy0 + r - kVB, gx = cx0 + c - kHB;
+        const bool ok = (r < cv + 2 * kVB) && (c < chs + 2 * kHB) && gy >= 0 && gy < chh && gx >= 0 && gx < cw;
+        tile[i] = ok ? (uint16_t)in[(long)gy * cw + gx] : kLarge;
+      }
+      __syncthreads();
+      for (int p = tid; p < 1024; p += 256) {
+        const int r = p >> 5, c = p & 31;
+        if (r >= cv || c >= chs) continue;
+        const int blk = (r >> 2) * 8 + (c >> 2);
+        const uint16_t* cp = tile + (kVB + r) * kTS + kHB + c;
+        int32_t v = cp[0];
+        if ((lvl_uv || sec_uv) && sact[blk]) {
+          const int32_t f = cdef_pixel(cp, lvl_uv, sec_uv, lvl_uv ? sdir[blk] : 0, dmp - 1, cs, sdirs);
+          n_f++;
+          n_c += f != v;
+          v = f;
+        }
+        out[(long)(cy0 + r) * cw + cx0 + c] = (Pix)v;
+      }
+    }
+  }
+  if (stats) {
+    uint64_t a = n_f, b = n_c;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) a += __shfl_xor(a, m, 64), b += __shfl_xor(b, m, 64);
+    if ((tid & 63) == 0) {
+      atomicAdd(&red[0], (unsigned long long)a);
+      atomicAdd(&red[1], (unsigned long long)b);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&stats[0], red[0]);
+      atomicAdd(&stats[1], red[1]);
+    }
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+hipError_t cdef_apply(const void* in_y, const void* in_u, const void* in_v, void* out_y, void* out_u, void* out_v,
+                      int width, int height, int bd, int damping, const uint8_t* fb_str, const uint8_t* skip,
+                      int mi_cols, int mi_rows, uint64_t* stats, hipStream_t st) {
+  const int nhfb = (width + 63) / 64, nvfb = (height + 63) / 64;
+  auto* s = reinterpret_cast<unsigned long long*>(stats);
+  if (s) hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, s, 2);
+  const dim3 grid((unsigned)(nhfb * nvfb));
+  if (bd == 8)
+    hipLaunchKernelGGL((k_cdef_apply<uint8_t>), grid, dim3(256), 0, st, (const uint8_t*)in_y, (const uint8_t*)in_u,
+                       (const uint8_t*)in_v, (uint8_t*)out_y, (uint8_t*)out_u, (uint8_t*)out_v, width, height, bd,
+                       damping, nhfb, fb_str, skip, mi_cols, mi_rows, s);
+  else
+    hipLaunchKernelGGL((k_cdef_apply<uint16_t>), grid, dim3(256), 0, st, (const uint16_t*)in_y,
+                       (const uint16_t*)in_u, (const uint16_t*)in_v, (uint16_t*)out_y, (uint16_t*)out_u,
+                       (uint16_t*)out_v, width, height, bd, damping, nhfb, fb_str, skip, mi_cols, mi_rows, s);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Restoration units
 // ---------------------------------------------------------------------------
@@ -946,16 +1101,225 @@ __global__ __launch_bounds__(256) void k_unit_sse(const Pix* __restrict__ a, con
   const int u = xcd_chunked(blockIdx.x, gridDim.x), uy = u / nh, ux = u - uy * nh;
   int hs, he, vs, ve;
   unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
-  const int uw = he - hs, n = uw * (ve - vs);
-  uint64_t s = 0;
+  const int uw = he - hs, uh = ve - vs, n = uw * uh, dw = uw % 16, dh = uh % 16;
+  // sse_region_hbd's parts (each truncated to 32 bits): the right strip of
+  // uw % 16 columns, the bottom strip left of it, the 16x16 blocks
+  uint64_t sb = 0, sr = 0, st = 0;
   for (int i = threadIdx.x; i < n; i += 256) {
     const int r = i / uw, c = i - r * uw;
     const long o = (long)(vs + r) * pw + hs + c;
     const int64_t e = (int64_t)a[o] - b[o];
-    s += (uint64_t)(e * e);
+    const uint64_t e2 = (uint64_t)(e * e);
+    if (sizeof(Pix) == 1 || (c < uw - dw && r < uh - dh)) sb += e2;
+    else if (c >= uw - dw) sr += e2;
+    else st += e2;
   }
-  const uint64_t t = wg_sum_u64(s, red);
+  uint64_t t = wg_sum_u64(sb, red);
+  if (sizeof(Pix) == 2) t += (uint32_t)wg_sum_u64(sr, red) + (uint64_t)(uint32_t)wg_sum_u64(st, red);
   if (threadIdx.x == 0) out[u] = t;
+}
+
+// ---------------------------------------------------------------------------
+// Wiener unit filter (loop_restoration_filter_unit without stripe
+// boundaries, restoration.rs:1288 / :1921) and the finer tap search
+// (finer_tile_search_wiener, enc/restoration.rs:513)
+// ---------------------------------------------------------------------------
+// Per output pixel the unit filter is wiener_convolve_add_src (:138 / :1820)
+// on the edge-extended plane: the stripe and processing-unit split of the
+// reference changes no pixel of the unit.  A workgroup filters a tile of at
+// most 64 x 64 pixels of one unit: the input with a 3-pixel border (edge
+// replicated) into LDS, the horizontal pass over the tile's rows + 6 into
+// LDS (round 3, clamp to [0, 2^(bd + 5) - 1], fits u16), then the vertical
+// pass (round 11, clamp to the pixel range).  The 8th taps are 0 and are
+// not read.
+struct WienerUnitDev {
+  int16_t hfilter[8], vfilter[8];
+  uint8_t type, pad[7];
+};
+static_assert(sizeof(WienerUnitDev) == 40, "zav1_wiener_unit layout");
+constexpr int kWtS = 72;  // input tile stride (64 + 6, rounded up)
+
+template <typename Pix, class Emit>
+__device__ __forceinline__ void wiener_tile(const Pix* __restrict__ dgd, int pw, int ph, int bd, int tx0, int ty0,
+                                            int tw, int th, const int* hf, const int* vf, uint16_t* in,
+                                            uint16_t* mid, Emit&& emit) {
+  const int tid = threadIdx.x, ir = th + 6, ic = tw + 6;
+  for (int i = tid; i < ir * ic; i += 256) {
+    const int r = i / ic, c = i - r * ic;
+    const int gy = min(max(ty0 + r - 3, 0), ph - 1), gx = min(max(tx0 + c - 3, 0), pw - 1);
+    in[r * kWtS + c] = (uint16_t)dgd[(long)gy * pw + gx];
+  }
+  __syncthreads();
+  const int clamp_h = (1 << (bd + 5)) - 1;
+  for (int i = tid; i < ir * tw; i += 256) {
+    const int r = i / tw, c = i - r * tw;
+    const uint16_t* q = in + r * kWtS + c;
+    int32_t sum = ((int32_t)q[3] << 7) + (1 << (bd + 6));
+#pragma unroll
+    for (int k = 0; k < 7; k++) sum += (int32_t)q[k] * hf[k];
+    mid[r * 64 + c] = (uint16_t)min(max((sum + 4) >> 3, 0), clamp_h);
+  }
+  __syncthreads();
+  const int mx = (1 << bd) - 1;
+  for (int i = tid; i < th * tw; i += 256) {
+    const int r = i / tw, c = i - r * tw;
+    const uint16_t* q = mid + r * 64 + c;
+    int32_t sum = ((int32_t)q[3 * 64] << 7) - (1 << (bd + 10));
+#pragma unroll
+    for (int k = 0; k < 7; k++) sum += (int32_t)q[k * 64] * vf[k];
+    emit(r, c, min(max((sum + 1024) >> 11, 0), mx));
+  }
+  __syncthreads();
+}
+
+// The plane by tiles of T = min(64, unit size) aligned to the unit grid
+// (rows from 0, then T k - (8 >> ss_y)), so each tile lies in one unit.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_wiener_apply(const Pix* __restrict__ dgd, Pix* __restrict__ out, int pw,
+                                                      int ph, int bd, int unit_size, int ss_y, int nh, int nv, int T,
+                                                      int ntc, const WienerUnitDev* __restrict__ units) {
+  __shared__ uint16_t in[70 * kWtS], mid[70 * 64];
+  const int t = blockIdx.x, tr = t / ntc, tc = t - tr * ntc, voff = 8 >> ss_y;
+  const int ty0 = tr == 0 ? 0 : tr * T - voff, ty1 = min((tr + 1) * T - voff, ph);
+  if (ty0 >= ph) return;
+  const int tx0 = tc * T, tw = min(T, pw - tx0), th = ty1 - ty0;
+  const int ux = min(tx0 / unit_size, nh - 1), uy = min((ty0 + voff) / unit_size, nv - 1);
+  const WienerUnitDev& wu = units[uy * nh + ux];
+  if (wu.type == 0) {
+    for (int i = threadIdx.x; i < tw * th; i += 256) {
+      const long o = (long)(ty0 + i / tw) * pw + tx0 + i % tw;
+      out[o] = dgd[o];
+    }
+    return;
+  }
+  int hf[7], vf[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) hf[k] = wu.hfilter[k], vf[k] = wu.vfilter[k];
+  wiener_tile(dgd, pw, ph, bd, tx0, ty0, tw, th, hf, vf, in, mid,
+              [&](int r, int c, int v) { out[(long)(ty0 + r) * pw + tx0 + c] = (Pix)v; });
+}
+
+// One trial of the search: the unit [hs, he) x [vs, ve) filtered with taps
+// (LDS, the workgroup's current filter) and its sse_region against src
+// (the 10-bit parts truncated as the reference does).  Every thread gets
+// the result.
+template <typename Pix>
+__device__ __noinline__ uint64_t wiener_trial(const Pix* __restrict__ dgd, const Pix* __restrict__ src, int pw,
+                                              int ph, int bd, int hs, int he, int vs, int ve, const int* taps) {
+  __shared__ uint16_t in[70 * kWtS], mid[70 * 64];
+  __shared__ uint64_t red[4];
+  int hf[7], vf[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) hf[k] = taps[k], vf[k] = taps[8 + k];
+  const int uw = he - hs, uh = ve - vs, dw = uw % 16, dh = uh % 16;
+  uint64_t sb = 0, sr = 0, st = 0;
+  for (int ty = vs; ty < ve; ty += 64)
+    for (int tx = hs; tx < he; tx += 64)
+      wiener_tile(dgd, pw, ph, bd, tx, ty, min(64, he - tx), min(64, ve - ty), hf, vf, in, mid,
+                  [&](int r, int c, int v) {
+                    const int64_t e = (int64_t)src[(long)(ty + r) * pw + tx + c] - v;
+                    const uint64_t e2 = (uint64_t)(e * e);
+                    const int x = tx + c - hs, y = ty + r - vs;
+                    if (sizeof(Pix) == 1 || (x < uw - dw && y < uh - dh)) sb += e2;
+                    else if (x >= uw - dw) sr += e2;
+                    else st += e2;
+                  });
+  uint64_t t = wg_sum_u64(sb, red);
+  if (sizeof(Pix) == 2) t += (uint32_t)wg_sum_u64(sr, red) + (uint64_t)(uint32_t)wg_sum_u64(st, red);
+  return t;
+}
+
+// One workgroup per unit runs the unit's whole hill climb; the decisions
+// are workgroup-uniform (every thread sees each trial's SSE), thread 0
+// moves the taps in LDS between trials.
+template <typename Pix>
+__global__ __launch_bounds__(256) void k_wiener_refine(const Pix* __restrict__ dgd, const Pix* __restrict__ src,
+                                                       int pw, int ph, int bd, int win, int unit_size, int ss_y,
+                                                       int nh, int use_refinement, int max_one_step,
+                                                       WienerUnitDev* __restrict__ units,
+                                                       int64_t* __restrict__ err_out) {
+  __shared__ int taps[16];
+  const int u = blockIdx.x, uy = u / nh, ux = u - uy * nh, tid = threadIdx.x;
+  int hs, he, vs, ve;
+  unit_limits(unit_size, pw, ph, ss_y, ux, uy, hs, he, vs, ve);
+  if (tid < 8) taps[tid] = units[u].hfilter[tid], taps[8 + tid] = units[u].vfilter[tid];
+  __syncthreads();
+  int64_t err = (int64_t)wiener_trial(dgd, src, pw, ph, bd, hs, he, vs, ve, taps);
+  constexpr int tmin[3] = {3 - 8, -7 - 16, 15 - 32}, tmax[3] = {3 - 1 + 8, -7 - 1 + 16, 15 - 1 + 32};
+  auto move = [&](int f, int p, int d) {
+    __syncthreads();
+    if (tid == 0) taps[f * 8 + p] += d, taps[f * 8 + 6 - p] += d, taps[f * 8 + 3] -= 2 * d;
+    __syncthreads();
+  };
+  if (use_refinement) {
+    const int start = 4, end = max_one_step ? 4 : 1, plane_off = (7 - win) >> 1;
+    for (int s = start; s >= end; s >>= 1)
+      for (int f = 0; f < 2; f++)
+        for (int p = plane_off; p < 3; p++) {
+          bool skip = false;
+          for (;;) {  // minus
+            if (taps[f * 8 + p] - s >= tmin[p]) {
+              move(f, p, -s);
+              const int64_t e2 = (int64_t)wiener_trial(dgd, src, pw, ph, bd, hs, he, vs, ve, taps);
+              if (e2 > err) {
+                move(f, p, s);
+              } else {
+                err = e2;
+                skip = true;
+                if (s == start && !max_one_step) continue;
+              }
+            }
+            break;
+          }
+          if (skip) break;  // a kept minus move ends this pass over the taps
+          for (;;) {        // plus
+            if (taps[f * 8 + p] + s <= tmax[p]) {
+              move(f, p, s);
+              const int64_t e2 = (int64_t)wiener_trial(dgd, src, pw, ph, bd, hs, he, vs, ve, taps);
+              if (e2 > err) {
+                move(f, p, -s);
+              } else {
+                err = e2;
+                if (s == start && !max_one_step) continue;
+              }
+            }
+            break;
+          }
+        }
+  }
+  __syncthreads();
+  if (tid < 8) units[u].hfilter[tid] = (int16_t)taps[tid], units[u].vfilter[tid] = (int16_t)taps[8 + tid];
+  if (tid == 0) err_out[u] = err;
+}
+
+hipError_t wiener_apply(const void* dgd, void* out, int pw, int ph, int bd, int unit_size, int ss_y, int nh, int nv,
+                        const void* units, hipStream_t st) {
+  const int T = unit_size < 64 ? unit_size : 64, voff = 8 >> ss_y;
+  const int ntc = (pw + T - 1) / T, ntr = (ph + voff + T - 1) / T;
+  const dim3 grid((unsigned)(ntc * ntr));
+  const auto* wu = static_cast<const WienerUnitDev*>(units);
+  if (bd == 8)
+    hipLaunchKernelGGL((k_wiener_apply<uint8_t>), grid, dim3(256), 0, st, (const uint8_t*)dgd, (uint8_t*)out, pw, ph,
+                       bd, unit_size, ss_y, nh, nv, T, ntc, wu);
+  else
+    hipLaunchKernelGGL((k_wiener_apply<uint16_t>), grid, dim3(256), 0, st, (const uint16_t*)dgd, (uint16_t*)out, pw,
+                       ph, bd, unit_size, ss_y, nh, nv, T, ntc, wu);
+  return hipGetLastError();
+}
+
+hipError_t wiener_refine(const void* dgd, const void* src, int pw, int ph, int bd, int win, int unit_size, int ss_y,
+                         int nh, int nv, int use_refinement, int max_one_step, void* units, int64_t* err,
+                         hipStream_t st) {
+  auto* wu = static_cast<WienerUnitDev*>(units);
+  const dim3 grid((unsigned)(nh * nv));
+  if (bd == 8)
+    hipLaunchKernelGGL((k_wiener_refine<uint8_t>), grid, dim3(256), 0, st, (const uint8_t*)dgd, (const uint8_t*)src,
+                       pw, ph, bd, win, unit_size, ss_y, nh, use_refinement, max_one_step, wu, err);
+  else
+    hipLaunchKernelGGL((k_wiener_refine<uint16_t>), grid, dim3(256), 0, st, (const uint16_t*)dgd,
+                       (const uint16_t*)src, pw, ph, bd, win, unit_size, ss_y, nh, use_refinement, max_one_step, wu,
+                       err);
+  return hipGetLastError();
 }
 
 // Self-guided parameter sets {r0, s0, r1, s1} (AV1 Sgr_Params).

@@ -408,6 +408,14 @@ hipError_t deblock_trials(const void* recon, const void* src, void* work, int pw
 // CDEF strength search over every 64x64 filter block: mse_y / mse_uv rows of
 // 64 strengths per block (raster of blocks); dirvar [fb][64][2] (dir, var).
 // skip: per-mi signaled skip (mi_rows x mi_cols) or nullptr.
+hipError_t wiener_apply(const void* dgd, void* out, int pw, int ph, int bd, int unit_size, int ss_y, int nh, int nv,
+                        const void* units, hipStream_t st);
+hipError_t wiener_refine(const void* dgd, const void* src, int pw, int ph, int bd, int win, int unit_size, int ss_y,
+                         int nh, int nv, int use_refinement, int max_one_step, void* units, int64_t* err,
+                         hipStream_t st);
+hipError_t cdef_apply(const void* in_y, const void* in_u, const void* in_v, void* out_y, void* out_u, void* out_v,
+                      int width, int height, int bd, int damping, const uint8_t* fb_str, const uint8_t* skip,
+                      int mi_cols, int mi_rows, uint64_t* stats, hipStream_t st);
 hipError_t cdef_search(const void* rec_y, const void* rec_u, const void* rec_v, const void* src_y,
                        const void* src_u, const void* src_v, int width, int height, int bd, int damping, int sub_y,
                        int sub_uv, const uint8_t* skip, int mi_cols, int mi_rows, int32_t* dirvar, uint64_t* mse_y,
