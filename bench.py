@@ -502,35 +502,43 @@ def config4_stage(zav1, synth, tsm, args, streams):
                                    'qp 28, 19 tx sizes x legal types', 'streams': len(streams)}}
 
 
-def serving_stage(zav1, synth, tsm, args):
-    """Serving mix at a fixed Poisson arrival rate: per-request latency of
-    the tx-type search stage (upload, grids, every size at the request's
-    qp, results resident in HBM) with an HBM admission pool of contexts.
-    The rate is set to args.serve_load of the pool's measured back-to-back
-    throughput."""
+def serving_stage(zav1, synth, args):
+    """Serving mix (config 5) through the wavefront MD at a fixed Poisson
+    arrival rate: requests of log-uniform area 256^2..4096x3072, 8 / 10
+    bits, qp 20..50, preset 4 / 6 / 8, admitted against an HBM budget,
+    continuous batching at wavefront-step granularity.  The rate is
+    args.serve_load of the loop's measured back-to-back throughput (all
+    requests at once)."""
+    wf = _load('zav1_wavefront', os.path.join(PKG, 'wavefront.py'))
     srv = _load('zav1_serving', os.path.join(PKG, 'serving.py'))
     reqs = synth.serving_mix(args.serve_n, seed=5000)
     imgs = synth.serving_images(reqs, seed=5000)
-    loop = srv.ServingLoop(zav1, tsm, synth, n_ctx=args.serve_ctx)
-    # back-to-back throughput (every request arrives at once), twice: the
-    # first pass warms up
-    for _ in range(2):
-        lat0, mp0, wall0 = loop.run(reqs, imgs, rate_hz=1e9)
-    mp_per_s = mp0 / wall0
-    rate = args.serve_load * mp_per_s / (mp0 / len(reqs))
-    lat, mp, wall = loop.run(reqs, imgs, rate_hz=rate, seed=5002)
-    loop.close()
-    return {'metric': 'per-image latency (ms) of the tx-type search stage, Poisson arrivals',
-            'p50_ms': round(float(np.percentile(lat, 50)) * 1e3, 3),
-            'p99_ms': round(float(np.percentile(lat, 99)) * 1e3, 3),
-            'mean_ms': round(float(lat.mean()) * 1e3, 3),
-            'value': round(mp / wall, 2), 'unit': 'MP/s', 'requests': len(reqs), 'megapixels': round(mp, 2),
-            'arrival_rate_hz': round(rate, 1), 'load': args.serve_load,
-            'back_to_back_MPs': round(mp_per_s, 2),
-            'admission': {'contexts': args.serve_ctx, 'hbm_per_context_MB': round(loop.budget / 1e6, 1)},
+    specs = [(img, r[2], r[3], r[4]) for img, r in zip(imgs, reqs)]
+    budget = args.serve_budget_gb << 30
+    server = srv.WavefrontServer(zav1, wf, synth, budget, max_active=args.serve_active, chunk=args.serve_chunk)
+    _, st0 = server.run(specs, [0.0] * len(specs), keep_results=False)  # back to back
+    mp = st0['megapixels']
+    b2b = mp / st0['wall_s']
+    rate = args.serve_load * b2b / (mp / len(specs))
+    rng = np.random.default_rng(5002)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(specs)))
+    _, st = server.run(specs, arrivals, keep_results=False)
+    server.close()
+    lat = st['latency_s']
+    return {'metric': 'per-image latency (ms) of the wavefront mode decision, Poisson arrivals',
+            'p50_ms': round(float(np.percentile(lat, 50)) * 1e3, 2),
+            'p99_ms': round(float(np.percentile(lat, 99)) * 1e3, 2),
+            'mean_ms': round(float(lat.mean()) * 1e3, 2),
+            'value': round(st['megapixels'] / st['wall_s'], 2), 'unit': 'MP/s', 'requests': len(specs),
+            'megapixels': round(st['megapixels'], 2), 'arrival_rate_hz': round(rate, 2), 'load': args.serve_load,
+            'back_to_back_MPs': round(b2b, 2),
+            'admission': {'hbm_budget_GB': args.serve_budget_gb, 'max_active': args.serve_active,
+                          'chunk_steps': args.serve_chunk, 'peak_active': st['peak_active'],
+                          'peak_MB': round(st['peak_bytes'] / 1e6, 1),
+                          'deferred_by_memory': st['deferred_by_memory']},
             'config': {'workload': 'serving mix: area log-uniform 256^2..4096x3072, aspect 1:1..16:9, 50/50 '
-                                   '8/10-bit, qp U{20..50}; per request upload + tx-type search over every size '
-                                   '(one quantizer)'}}
+                                   '8/10-bit, qp U{20..50}, preset U{4,6,8}; per request upload + the wavefront '
+                                   'mode decision (16x16 leaves), results resident until read back'}}
 
 
 def cpu_info():
@@ -780,15 +788,17 @@ def main():
                     help='one eager pass of the headline workload only (for rocprofv3 runs)')
     ap.add_argument('--plain-leaves', type=int, default=48)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--extras', default='lf,rdoq', help='comma list of lf, rdoq, txsearch, config4, serving')
+    ap.add_argument('--extras', default='lf,rdoq,serving', help='comma list of lf, rdoq, txsearch, config4, serving')
     ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--pmc-summary', nargs=4, metavar=('FETCH_DIR', 'WRITE_DIR', 'CAL_FETCH_DIR', 'CAL_WRITE_DIR'),
                     help='print the per-kernel HBM traffic summary of rocprofv3 --pmc CSV runs (merged over '
                          'profiles/pmc_traffic.json)')
     ap.add_argument('--lf-steps', type=int, default=10)
     ap.add_argument('--streams', type=int, default=2)
-    ap.add_argument('--serve-n', type=int, default=96)
-    ap.add_argument('--serve-ctx', type=int, default=4)
+    ap.add_argument('--serve-n', type=int, default=48)
+    ap.add_argument('--serve-active', type=int, default=8)
+    ap.add_argument('--serve-chunk', type=int, default=32)
+    ap.add_argument('--serve-budget-gb', type=int, default=8)
     ap.add_argument('--serve-load', type=float, default=0.6)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -818,7 +828,7 @@ def main():
         return
     extras = [e for e in args.extras.split(',') if e]
     cpu = not args.no_cpu
-    if {'txsearch', 'config4', 'serving'} & set(extras):
+    if {'txsearch', 'config4'} & set(extras):
         tsm = _load('zav1_txsearch', os.path.join(PKG, 'txsearch.py'))
     if 'lf' in extras:
         out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=cpu)
@@ -830,7 +840,7 @@ def main():
     if 'config4' in extras:
         out['config4_tile'] = config4_stage(zav1, synth, tsm, args, [zav1.stream_create()])
     if 'serving' in extras:
-        out['serving'] = serving_stage(zav1, synth, tsm, args)
+        out['serving'] = serving_stage(zav1, synth, args)
     print(json.dumps(out))
 
 

@@ -627,6 +627,10 @@ int zav1_wf_info(const zav1_wf *wf, zav1_wf_info_t *info);
  * captures them once into a HIP graph (s must be a created stream) and
  * replays it. */
 int zav1_wf_run(zav1_wf *wf, int use_graph, zav1_stream_t s);
+/* Steps [first_step, first_step + n_steps) of the eager chain (earlier steps
+ * must have been enqueued before): the unit a serving loop advances each
+ * request by, so that requests on shared hardware queues interleave. */
+int zav1_wf_run_steps(zav1_wf *wf, int first_step, int n_steps, zav1_stream_t s);
 /* The eager run with HIP events at the stage boundaries of every step, the
  * stream synchronized after each step; stage_ms[k] (n_stages >=
  * ZAV1_WF_STAGES) receives stage k's total device time over the run:

@@ -174,12 +174,14 @@ def test_serving_mix_properties():
     spec.loader.exec_module(synth)
     reqs = synth.serving_mix(400, seed=5000)
     assert reqs == synth.serving_mix(400, seed=5000)  # deterministic
-    for h, w, bd, qp in reqs:
+    for h, w, bd, qp, preset in reqs:
         assert h % 64 == 0 and w % 64 == 0 and 64 <= h <= 4096 and 64 <= w <= 4096
-        assert bd in (8, 10) and 20 <= qp <= 50
+        assert bd in (8, 10) and 20 <= qp <= 50 and preset in (4, 6, 8)
     bds = [r[2] for r in reqs]
     assert 0.4 < bds.count(8) / len(bds) < 0.6
-    areas = np.log([h * w for h, w, _, _ in reqs])
+    presets = [r[4] for r in reqs]
+    assert all(0.25 < presets.count(p) / len(presets) < 0.42 for p in (4, 6, 8))
+    areas = np.log([r[0] * r[1] for r in reqs])
     assert areas.min() < np.log(300 * 300) and areas.max() > np.log(3000 * 2500)
 
 
@@ -327,3 +329,26 @@ def test_kf_full_lambda_matches_instrumented_values(zav1):
     l10 = L.zav1_kf_full_lambda(160, 40, 10)
     assert l10 % 16 == 0 and 8 < l10 / 248207 < 24
     assert L.zav1_kf_full_lambda(256, 40, 8) == 0
+
+
+def test_preset_funnel_settings():
+    """FunnelCfg::for_preset -> the wavefront's funnel settings for 16x16
+    leaves: candidates (62 at preset 4; DC/V/H/SMOOTH (+ FILTER_DC at 6)),
+    NIC counts max(2, round(64 * 6 / 16)) = 24 and 6, qp-scaled by
+    round(n * qp / 63) below qp 46 (24 -> 15, 6 -> 4 at qp 40), 1 / 1 at
+    preset 8 (nic nums 0), the tx-type groups and gates."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location('zav1_wf_t', os.path.join(ROOT, 'zenav1-svt_amd', 'wavefront.py'))
+    wf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(wf)
+    f = wf.preset_funnel(4, 40)
+    assert (len(f['cands']), f['k1'], f['k3'], f['txt_groups'], f['txt_rate_th'], f['rdoq_level']) == (62, 15, 4, 6, 250, 1)
+    assert f['txt_satd_th'] == (15 * 40 + 31) // 63
+    f = wf.preset_funnel(6, 20)
+    assert [c[0] for c in f['cands']] == [0, 1, 2, 9, 0] and f['cands'][-1][2] == 0
+    assert (f['k1'], f['k3'], f['txt_groups'], f['txt_rate_th'], f['rdoq_level']) == (5, 2, 4, 100, 3)
+    f = wf.preset_funnel(8, 50)
+    assert [c[0] for c in f['cands']] == [0, 1, 2, 9] and (f['k1'], f['k3'], f['txt_groups']) == (1, 1, 2)
+    import pytest
+    with pytest.raises(ValueError):
+        wf.preset_funnel(5, 30)

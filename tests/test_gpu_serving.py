@@ -1,6 +1,8 @@
-"""GPU checks of the serving loop: every request completes with a latency,
-and a request's arena results equal the oracle's search at its qp."""
-import ctypes
+"""GPU checks of the wavefront serving loop (continuous batching with HBM
+admission): requests of mixed size, bit depth, qp and preset arrive faster
+than they complete, overlap on the device, are held back when the budget is
+full, and every request's reconstruction and decisions in the mix equal
+the same request run alone, and the oracle's sequential restatement."""
 import importlib.util
 import os
 import sys
@@ -21,42 +23,54 @@ def _mod(name):
     return m
 
 
-def test_serving_loop_latencies_and_results(gpu, oracle):
-    zav1 = gpu
-    synth, tsm, srv = _mod('synth'), _mod('txsearch'), _mod('serving')
-    reqs = [(128, 192, 10, 20), (64, 128, 8, 45), (192, 64, 10, 33), (256, 256, 8, 28), (64, 4096, 10, 50)]
-    imgs = synth.serving_images(reqs, seed=77)
-    loop = srv.ServingLoop(zav1, tsm, synth, n_ctx=2, ctx_budget=srv.request_bytes(256, 4096, tsm))
-    lat, mp, wall = loop.run(reqs, imgs, rate_hz=2000.0)
-    assert lat.shape == (5,) and np.all(lat > 0) and wall > 0
-    assert abs(mp - sum(h * w for h, w, _, _ in reqs) / 1e6) < 1e-9
-    # one request through a context, results read back from the arena
-    h, w, bd, qp = reqs[1]
-    fs = loop.submit(0, imgs[1], bd, synth.qp_to_qindex(qp))
-    zav1.sync(loop.ctx[0].stream())
-    L = zav1.lib()
-    for ts in range(19):
-        nb = fs.n_blocks[ts]
-        types = np.array(tsm.legal_types(ts), np.uint8)
-        assert fs.n_types[ts] == types.size and list(fs.types[ts])[:types.size] == list(types)
-        assert nb == (w // tsm.TX_DIMS[ts][0]) * (h // tsm.TX_DIMS[ts][1])
-        if nb == 0:
-            assert not fs.dist[ts]
-            continue
-        dd, de, dl = fs.dist[ts], fs.eob[ts], fs.lsum[ts]
-        n = nb * types.size
-        d, e, l = np.zeros(n, np.uint64), np.zeros(n, np.int32), np.zeros(n, np.uint32)
-        for host, dev in ((d, dd), (e, de), (l, dl)):
-            zav1.check(L.zav1_memcpy_d2h(host.ctypes.data, dev, host.nbytes, None), 'd2h')
-        xy = tsm.tile_xy(w, h, ts)
-        rd, re, rl = oracle.txsearch_plane(imgs[1], bd, ts, xy, types, synth.qp_to_qindex(qp))
-        assert np.array_equal(d, rd) and np.array_equal(e, re) and np.array_equal(l, rl), ts
-    # an arena too small for a request: ENOMEM before anything is enqueued
-    small = zav1.Context(0, 1 << 16)
-    d_plane = small.upload(imgs[3][:64, :64].copy(), 10)
-    q = np.array([80], np.int32)
-    f2 = zav1.FrameSearch()
-    assert L.zav1_txsearch_frame(small.h, d_plane, 256, 256, 8, q.ctypes.data, 1, ctypes.byref(f2)) == zav1.ZAV1_ENOMEM
-    assert small.usage()[0] == 64 * 64 * 2
-    small.close()
-    loop.close()
+FIELDS = ('cand', 'tx_type', 'eob', 'rate', 'mode_rate', 'dist', 'cost')
+REQS = [(128, 192, 10, 20, 4), (64, 128, 8, 45, 6), (192, 64, 10, 33, 8), (256, 256, 8, 28, 4), (64, 512, 10, 50, 6),
+        (128, 128, 8, 38, 8), (320, 192, 10, 24, 6), (64, 64, 8, 30, 4), (256, 128, 10, 42, 4), (128, 320, 8, 22, 8)]
+
+
+def test_serving_mix_alone_equals_in_mix(gpu, oracle):
+    z = gpu
+    synth, wf, srv = _mod('synth'), _mod('wavefront'), _mod('serving')
+    imgs = synth.serving_images(REQS, seed=77)
+    specs = [(img, bd, qp, p) for img, (h, w, bd, qp, p) in zip(imgs, REQS)]
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(0.002, len(REQS)))
+    probe = srv.WavefrontServer(z, wf, synth, 1 << 40)
+    fps = sorted(wf.WavefrontMD.footprint(z, probe.request_cfg(h, w, bd, qp, p)) for h, w, bd, qp, p in REQS)
+    budget = fps[-1] + fps[-2]  # the two largest fit together, not three
+    server = srv.WavefrontServer(z, wf, synth, budget, max_active=4, chunk=8, depth=2)
+    done, st = server.run(specs, arrivals)
+    server.close()
+    assert st['peak_active'] >= 2, st
+    assert st['deferred_by_memory'] >= 1, st
+    assert st['peak_bytes'] <= budget
+    lat = st['latency_s']
+    assert lat.shape == (len(REQS),) and np.all(lat > 0)
+    for r in done:
+        md = wf.WavefrontMD(z, r.cfg)
+        try:
+            md.upload(r.img[None])
+            md.run(use_graph=False)
+            z.sync()
+            assert np.array_equal(md.recon()[0], r.recon), r.idx
+            alone = md.decisions()[0]
+            for f in FIELDS:
+                assert np.array_equal(alone[f], r.decisions[f]), (r.idx, f)
+        finally:
+            md.close()
+    # the oracle for the 8-bit and 10-bit requests at each preset
+    for i in (0, 1, 2, 5, 7):
+        r = done[i]
+        orec, odec = oracle.wf_frame(r.cfg, r.img[None].astype(np.uint8 if r.bd == 8 else np.uint16))
+        assert np.array_equal(orec[0], r.recon), i
+        for f in FIELDS:
+            assert np.array_equal(odec[0][f], r.decisions[f]), (i, f)
+
+
+def test_serving_rejects_a_request_over_budget(gpu):
+    z = gpu
+    synth, wf, srv = _mod('synth'), _mod('wavefront'), _mod('serving')
+    img = synth.serving_images([(256, 256, 10, 30, 4)], seed=3)[0]
+    server = srv.WavefrontServer(z, wf, synth, 1 << 16)  # below the plane's own 128 KiB
+    with pytest.raises(ValueError):
+        server.run([(img, 10, 30, 4)], [0.0])

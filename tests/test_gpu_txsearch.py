@@ -128,3 +128,40 @@ def test_txsearch_multi_quantizer_matches_single(gpu, oracle, bd):
             assert np.array_equal(md[sl_], rd) and np.array_equal(sd.to_host(), rd), (tx_size, qi)
             assert np.array_equal(me[sl_], re) and np.array_equal(se.to_host(), re), (tx_size, qi)
             assert np.array_equal(ml[sl_], rl) and np.array_equal(sl.to_host(), rl), (tx_size, qi)
+
+
+def test_txsearch_frame_in_an_arena_and_enomem(gpu, oracle):
+    """zav1_txsearch_frame: grids and outputs carved from a context's arena
+    equal the oracle's search; an arena too small for a request gives
+    ZAV1_ENOMEM with nothing enqueued (the arena keeps only the plane)."""
+    import ctypes
+    zav1 = gpu
+    synth, tsm = _mod('synth'), _mod('txsearch')
+    img = synth.serving_images([(64, 128, 8, 45, 6)], seed=77)[0]
+    h, w, bd, qidx = 64, 128, 8, synth.qp_to_qindex(45)
+    ctx = zav1.Context(0, 64 << 20)
+    d_plane = ctx.upload(img, 10)  # host samples as u16 (the request's 8-bit content in 16-bit words)
+    q = np.array([qidx], np.int32)
+    fs = zav1.FrameSearch()
+    L = zav1.lib()
+    zav1.check(L.zav1_txsearch_frame(ctx.h, d_plane, w, h, bd, q.ctypes.data, 1, ctypes.byref(fs)), 'txsearch_frame')
+    zav1.sync(ctx.stream())
+    for ts in range(19):
+        nb = fs.n_blocks[ts]
+        types = np.array(tsm.legal_types(ts), np.uint8)
+        assert fs.n_types[ts] == types.size and nb == (w // tsm.TX_DIMS[ts][0]) * (h // tsm.TX_DIMS[ts][1])
+        if nb == 0:
+            continue
+        n = nb * types.size
+        d, e, l = np.zeros(n, np.uint64), np.zeros(n, np.int32), np.zeros(n, np.uint32)
+        for host, dev in ((d, fs.dist[ts]), (e, fs.eob[ts]), (l, fs.lsum[ts])):
+            zav1.check(L.zav1_memcpy_d2h(host.ctypes.data, dev, host.nbytes, None), 'd2h')
+        rd, re, rl = oracle.txsearch_plane(img, bd, ts, tsm.tile_xy(w, h, ts), types, qidx)
+        assert np.array_equal(d, rd) and np.array_equal(e, re) and np.array_equal(l, rl), ts
+    ctx.close()
+    small = zav1.Context(0, 1 << 16)
+    d_plane = small.upload(img[:64, :64].copy(), 10)
+    f2 = zav1.FrameSearch()
+    assert L.zav1_txsearch_frame(small.h, d_plane, 256, 256, 8, q.ctypes.data, 1, ctypes.byref(f2)) == zav1.ZAV1_ENOMEM
+    assert small.usage()[0] == 64 * 64 * 2
+    small.close()

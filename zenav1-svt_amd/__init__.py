@@ -186,6 +186,7 @@ EXPORTS = {
     'zav1_wf_create': [_vp, ctypes.POINTER(_vp)],
     'zav1_wf_info': [_vp, _vp],
     'zav1_wf_run': [_vp, _i32, _vp],
+    'zav1_wf_run_steps': [_vp, _i32, _i32, _vp],
     'zav1_wf_run_plain': [_vp, _i32, _i32, _vp],
     'zav1_wf_destroy': [_vp],
     'zav1_wf_last_error': [],
@@ -309,6 +310,25 @@ def stream_sync(s):
 
 def stream_destroy(s):
     check(lib().zav1_stream_destroy(s), 'stream_destroy')
+
+
+def event_create():
+    e = ctypes.c_void_p()
+    check(lib().zav1_event_create(ctypes.byref(e)), 'event_create')
+    return e.value
+
+
+def event_record(e, stream):
+    check(lib().zav1_event_record(e, stream), 'event_record')
+
+
+def event_query(e):
+    """True once the event's recorded work has completed."""
+    return check(lib().zav1_event_query(e), 'event_query') == 1
+
+
+def event_destroy(e):
+    check(lib().zav1_event_destroy(e), 'event_destroy')
 
 
 def quant_row(qindex, bd):

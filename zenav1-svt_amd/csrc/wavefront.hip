@@ -857,6 +857,15 @@ int zav1_wf_run(zav1_wf* wf, int use_graph, zav1_stream_t s) {
   return ZAV1_OK;
 }
 
+int zav1_wf_run_steps(zav1_wf* wf, int first_step, int n_steps, zav1_stream_t s) {
+  if (!wf) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  if (first_step < 0 || n_steps < 0 || first_step + n_steps > w->n_steps) return ZAV1_EINVAL;
+  hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
+  for (int i = first_step; i < first_step + n_steps; i++) WCHECK(zav1::step_chain(w, i, st));
+  return ZAV1_OK;
+}
+
 int zav1_wf_run_timed(zav1_wf* wf, zav1_stream_t s, double* stage_ms, int n_stages) {
   if (!wf || !stage_ms || n_stages < ZAV1_WF_STAGES) return ZAV1_EINVAL;
   auto* w = reinterpret_cast<zav1::Wavefront*>(wf);

@@ -1,120 +1,123 @@
-"""Serving loop for the tx-type search stage.
+"""Serving loop over the superblock-wavefront mode decision (zav1_wf_*).
 
-Requests (images of the serving mix: log-uniform area, 8 or 10 bits, a qp
-each) arrive at Poisson times.  A request is admitted when one of a fixed
-pool of contexts is free -- each context is one stream and one HBM arena of
-a fixed budget, so the pool bounds the HBM the stage may hold (admission
-control).  An admitted request is copied into its context's pinned host
-staging buffer, uploaded into the context's arena (an asynchronous DMA),
-and searched over every tx size and legal type at its qindex by one call
-(zav1_txsearch_frame: grids and outputs from the arena, all on the
-context's stream); its results stay resident in HBM for the next stage.
-Its latency runs from its arrival to the completion of its last kernel,
-observed by polling the completion event.
+Requests -- still luma planes of mixed size, bit depth, qp and preset --
+arrive at Poisson times.  A request is admitted when its device footprint
+(zav1_wf_footprint: source, reconstruction, decisions and its step chain's
+buffers, all in one allocation) fits what is left of the HBM budget and an
+active slot is free; its plane is uploaded and its decision planned
+(zav1_wf_create) on a stream of its own.
+
+Continuous batching: every active request keeps up to `depth` chunks of
+`chunk` wavefront steps in flight (zav1_wf_run_steps, an event after each
+chunk).  A request admitted while others run starts at once, and its chunks
+interleave with theirs on the device's hardware queues -- no request waits
+for another to finish.  A request completes when its last chunk's event has
+fired; its reconstruction and decisions are read back, its device memory is
+released, and the next arrivals are admitted against the freed budget.
+Latency runs from a request's arrival to its completion.  Requests share no
+device state, so each result is the one the request gives alone.
 """
-import ctypes
+import collections
 import time
 
 import numpy as np
 
 
-def request_bytes(h, w, tsm):
-    """Arena bytes one request takes: the u16 plane and, per tx size, the
-    block grid and the per-candidate outputs (each allocation rounded up to
-    the arena's 256-byte alignment)."""
-    def r(n):
-        return (n + 255) // 256 * 256
-    tot = r(2 * h * w)
-    for ts in range(19):
-        bw, bh = tsm.TX_DIMS[ts]
-        nb = (w // bw) * (h // bh)
-        n = nb * len(tsm.legal_types(ts))
-        tot += r(8 * nb) + r(8 * n) + 2 * r(4 * n)
-    return tot
+class Request:
+    __slots__ = ('idx', 'arrival', 'img', 'bd', 'qp', 'preset', 'cfg', 'footprint', 'md', 'stream', 'next_step',
+                 'events', 't_admit', 't_done', 'recon', 'decisions', 'deferred')
+
+    def __init__(self, idx, arrival, img, bd, qp, preset):
+        self.idx, self.arrival, self.img, self.bd, self.qp, self.preset = idx, arrival, img, bd, qp, preset
+        self.md = self.stream = self.t_admit = self.t_done = self.recon = self.decisions = None
+        self.next_step, self.events, self.deferred = 0, collections.deque(), False
 
 
-class ServingLoop:
-    def __init__(self, zav1, tsm, synth, n_ctx=4, ctx_budget=None, max_hw=(4096, 3072)):
-        self.zav1, self.tsm, self.synth = zav1, tsm, synth
+class WavefrontServer:
+    def __init__(self, zav1, wf, synth, budget_bytes, max_active=8, chunk=64, depth=2, leaf=16):
+        self.z, self.wf, self.synth = zav1, wf, synth
         self.L = zav1.lib()
-        budget = ctx_budget or request_bytes(max_hw[0], max_hw[1], tsm)
-        self.ctx = [zav1.Context(0, budget) for _ in range(n_ctx)]
-        self.budget = budget
-        # pinned staging per context, large enough for any request the
-        # budget admits (u16 samples)
-        self.stage_bytes = 2 * max_hw[0] * max_hw[1] if ctx_budget is None else budget
-        self.stage = []
-        for _ in self.ctx:
-            p = ctypes.c_void_p()
-            zav1.check(self.L.zav1_host_alloc(ctypes.byref(p), self.stage_bytes), 'host_alloc')
-            self.stage.append(p.value)
-        self.ev = []
-        for _ in self.ctx:
-            e = ctypes.c_void_p()
-            zav1.check(self.L.zav1_event_create(ctypes.byref(e)), 'event_create')
-            self.ev.append(e.value)
-        self.types = [np.array(tsm.legal_types(t), np.uint8) for t in range(19)]
+        self.budget, self.max_active, self.chunk, self.depth, self.leaf = budget_bytes, max_active, chunk, depth, leaf
+        self.pool_ev = []
 
-    def submit(self, c, img, bd, qindex):
-        """Enqueue one request on context c (staging copy, upload, search of
-        every size); returns the FrameSearch record of its outputs."""
-        ctx, L = self.ctx[c], self.L
-        h, w = img.shape
-        if 2 * h * w > self.stage_bytes:
-            raise self.zav1.Zav1Error('request larger than the staging buffer')
-        dst = np.ctypeslib.as_array((ctypes.c_uint16 * (h * w)).from_address(self.stage[c])).reshape(h, w)
-        np.copyto(dst, img)
-        ctx.reset()
-        p = ctypes.c_void_p()
-        self.zav1.check(L.zav1_image_upload(ctx.h, self.stage[c], w, w, h, 10, ctypes.byref(p)), 'image_upload')
-        fs = ctx.txsearch_frame(p.value, w, h, bd, [qindex])
-        self.zav1.check(L.zav1_event_record(self.ev[c], ctx.stream()), 'event_record')
-        return fs
+    def request_cfg(self, h, w, bd, qp, preset):
+        qidx = self.synth.qp_to_qindex(qp)
+        lam = int(self.L.zav1_kf_full_lambda(qidx, qp, bd))
+        f = self.wf.preset_funnel(preset, qp, self.leaf)
+        return self.wf.make_cfg(w, h, bd, qidx, lam, leaf=self.leaf, k1=f['k1'], k3=f['k3'],
+                                rdoq_level=f['rdoq_level'], txt_groups=f['txt_groups'],
+                                txt_rate_th=f['txt_rate_th'], txt_satd_th=f['txt_satd_th'], cands=f['cands'])
 
-    def run(self, requests, images, rate_hz, seed=5001):
-        """Serve `requests` [(h, w, bd, qp)] with host images `images` at
-        Poisson arrivals of `rate_hz`; returns the per-request latencies (s),
-        the megapixels served and the wall time from the first arrival to
-        the last completion."""
-        rng = np.random.default_rng(seed)
-        gaps = rng.exponential(1.0 / rate_hz, len(requests))
-        busy = [None] * len(self.ctx)   # request index on each context
-        done_t = [0.0] * len(requests)
-        arrive_t = [0.0] * len(requests)
+    def _event(self):
+        if self.pool_ev:
+            return self.pool_ev.pop()
+        return self.z.event_create()
 
-        def poll():
-            now = time.perf_counter()
-            for c, k in enumerate(busy):
-                if k is not None and self.L.zav1_event_query(self.ev[c]) == 1:
-                    done_t[k] = now
-                    busy[c] = None
-
-        for c in range(len(self.ctx)):
-            self.zav1.sync(self.ctx[c].stream())
+    def run(self, specs, arrivals, keep_results=True):
+        """specs: (img, bd, qp, preset) per request; arrivals: seconds after
+        the start.  Returns (requests, stats)."""
+        reqs = [Request(i, float(a), img, bd, qp, p) for i, ((img, bd, qp, p), a) in enumerate(zip(specs, arrivals))]
+        for r in reqs:
+            h, w = r.img.shape
+            r.cfg = self.request_cfg(h, w, r.bd, r.qp, r.preset)
+            r.footprint = self.wf.WavefrontMD.footprint(self.z, r.cfg)
+            if r.footprint > self.budget:
+                raise ValueError('request %d needs %d bytes, over the budget %d' % (r.idx, r.footprint, self.budget))
+        pending = collections.deque(sorted(reqs, key=lambda r: r.arrival))
+        active, used, peak_used, peak_active, deferred = [], 0, 0, 0, 0
         t0 = time.perf_counter()
-        t_next = t0
-        for k, (h, w, bd, qp) in enumerate(requests):
-            t_next += gaps[k]
-            while time.perf_counter() < t_next:
-                poll()
-            arrive_t[k] = t_next
-            while None not in busy:  # admission: wait for a free context
-                poll()
-            c = busy.index(None)
-            busy[c] = k
-            self.submit(c, images[k], bd, self.synth.qp_to_qindex(qp))
-        while any(b is not None for b in busy):
-            poll()
-        lat = np.array([done_t[k] - arrive_t[k] for k in range(len(requests))])
-        mp = sum(h * w for h, w, _, _ in requests) / 1e6
-        return lat, mp, max(done_t) - t0
+        while pending or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                r = pending[0]
+                if used + r.footprint > self.budget:
+                    if not r.deferred:
+                        r.deferred = True
+                        deferred += 1
+                    break
+                pending.popleft()
+                r.stream = self.z.stream_create()
+                r.md = self.wf.WavefrontMD(self.z, r.cfg)
+                r.md.upload(r.img[None], r.stream)
+                r.t_admit = time.perf_counter() - t0
+                used += r.footprint
+                active.append(r)
+            peak_used, peak_active = max(peak_used, used), max(peak_active, len(active))
+            progressed = False
+            for r in active:
+                while r.events and self.z.event_query(r.events[0]):
+                    self.pool_ev.append(r.events.popleft())
+                    progressed = True
+                n = r.md.info.n_steps
+                while len(r.events) < self.depth and r.next_step < n:
+                    k = min(self.chunk, n - r.next_step)
+                    r.md.run_steps(r.next_step, k, r.stream)
+                    r.next_step += k
+                    e = self._event()
+                    self.z.event_record(e, r.stream)
+                    r.events.append(e)
+                    progressed = True
+            done = [r for r in active if r.next_step == r.md.info.n_steps and not r.events]
+            for r in done:
+                r.t_done = time.perf_counter() - t0
+                if keep_results:
+                    r.recon, r.decisions = r.md.recon()[0], r.md.decisions()[0]
+                r.md.close()
+                self.z.stream_destroy(r.stream)
+                r.md = r.stream = None
+                used -= r.footprint
+                active.remove(r)
+                progressed = True
+            if not progressed:
+                time.sleep(50e-6)
+        wall = max(r.t_done for r in reqs) - min(r.arrival for r in reqs)
+        lat = np.array([r.t_done - r.arrival for r in reqs])
+        mp = sum(r.img.size for r in reqs) / 1e6
+        stats = {'wall_s': wall, 'megapixels': mp, 'latency_s': lat, 'peak_active': peak_active,
+                 'peak_bytes': peak_used, 'deferred_by_memory': deferred}
+        return reqs, stats
 
     def close(self):
-        for c in self.ctx:
-            self.zav1.sync(c.stream())
-        for p in self.stage:
-            self.L.zav1_host_free(p)
-        for e in self.ev:
-            self.L.zav1_event_destroy(e)
-        for c in self.ctx:
-            c.close()
+        for e in self.pool_ev:
+            self.z.event_destroy(e)
+        self.pool_ev = []

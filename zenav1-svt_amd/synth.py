@@ -151,9 +151,10 @@ def config4_tile_column(h=4096, w=2048, seed=4000):
 
 
 def serving_mix(n, seed=5000):
-    """Requests of the serving mix: (height, width, bd, qp), area log-uniform
-    in 256^2 .. 4096x3072, aspect 1:1 .. 16:9 (either orientation), 50/50
-    8 / 10 bits, qp uniform in 20..50; sides rounded to multiples of 64."""
+    """Requests of the serving mix: (height, width, bd, qp, preset), area
+    log-uniform in 256^2 .. 4096x3072, aspect 1:1 .. 16:9 (either
+    orientation), 50/50 8 / 10 bits, qp uniform in 20..50, preset uniform in
+    {4, 6, 8}; sides rounded to multiples of 64."""
     rnd = XorShift64Star(seed)
     out = []
     lo, hi = np.log(256.0 * 256.0), np.log(4096.0 * 3072.0)
@@ -168,7 +169,8 @@ def serving_mix(n, seed=5000):
         h = int(max(64, min(4096, round(h / 64) * 64)))
         bd = 8 if rnd.uniform() < 0.5 else 10
         qp = rnd.randint(20, 50)
-        out.append((h, w, bd, qp))
+        preset = (4, 6, 8)[rnd.randint(0, 2)]
+        out.append((h, w, bd, qp, preset))
     return out
 
 
@@ -179,7 +181,8 @@ def serving_images(requests, seed=5000):
     pool = config2_luma(4096, 4096, seed=seed)
     rnd = XorShift64Star(seed + 7)
     out = []
-    for h, w, bd, _ in requests:
+    for req in requests:
+        h, w, bd = req[:3]
         y0, x0 = rnd.randint(0, 4096 - h), rnd.randint(0, 4096 - w)
         img = np.ascontiguousarray(pool[y0:y0 + h, x0:x0 + w])
         out.append((img >> 2).astype(np.uint16) if bd == 8 else img)

@@ -57,6 +57,54 @@ def qp_scaled_satd_th(th, cli_qp):
     return (th * w + d // 2) // d
 
 
+def _div_round(a, b):
+    return (a + b // 2) // b
+
+
+def _qp_factors(cli_qp):
+    """svt_aom_get_qp_based_th_scaling_factors (pd0.rs:185)."""
+    import math
+    if cli_qp >= 46:
+        return int((1.05 - math.exp(-(max(cli_qp, 40) - 35.0) / 10.0)) * 10000.0), 10000
+    return max(cli_qp, 10), 63
+
+
+def preset_funnel(preset, cli_qp, leaf=16):
+    """The wavefront MD's funnel settings at a preset for one leaf size, from
+    FunnelCfg::for_preset (rust/crates/svtav1-encoder/src/leaf_funnel/
+    rate_tables.rs:660): the candidate list (mode_end, angular level,
+    filter intra, mod.rs:1940-1975), the NIC counts of MDS1 / MDS3
+    (nic_counts, rate_tables.rs:1054: bases 64 / 16 x num / 16, at least 2,
+    qp-scaled; capped at the candidates), the tx-type groups (lt16 / ge16),
+    the rate and SATD gates (the SATD one qp-scaled) and the RDOQ level
+    (rdoq_level_allintra, quant.rs:1535: 1 up to preset 5, above it by the
+    picture's coefficient level -- taken as Normal, level 3).  Presets 4, 6
+    and 8."""
+    if preset not in (4, 6, 8):
+        raise ValueError('preset %r: 4, 6 or 8' % (preset,))
+    qw, qd = _qp_factors(cli_qp)
+    fi = leaf <= 32
+    if preset == 4:
+        cands, nic, groups, satd, rate, rdoq = preset_candidates(fi), 6, 6, 15, 250, 1
+    else:
+        # mode_end SMOOTH, angular level 4: D45..D67 masked, no deltas
+        cands = [(m, 0, 5) for m in (0, 1, 2, 9)]
+        if preset == 6 and fi:
+            cands.append((0, 0, 0))  # filter intra, FILTER_DC only
+        nic, groups, satd, rate, rdoq = (6, 4, 10, 100, 3) if preset == 6 else (0, 2, 10, 50, 3)
+        if leaf < 16:
+            groups = 5 if preset == 6 else 3
+
+    def scale(base, num):
+        lo = 2 if num else 1
+        n = max(lo, _div_round(base * num, 16))
+        return max(lo, _div_round(n * qw, qd))
+    k1 = min(scale(64, nic), len(cands))
+    k3 = min(scale(16, nic), k1)
+    return dict(cands=cands, k1=k1, k3=k3, txt_groups=groups, txt_rate_th=rate,
+                txt_satd_th=qp_scaled_satd_th(satd, cli_qp), rdoq_level=rdoq)
+
+
 def make_cfg(width, height, bd, qindex, lam, *, n_images=1, leaf=16, sb=64, tiles=(1, 1), k1=15, k3=4,
              rdoq_level=1, txt_groups=6, txt_rate_th=250, txt_satd_th=0, cands=None):
     cands = preset_candidates() if cands is None else cands
@@ -115,6 +163,11 @@ class WavefrontMD:
         if rc:
             raise self.z.Zav1Error(rc, 'zav1_wf_run_timed (%s)' % self.z.lib().zav1_wf_last_error().decode())
         return dict(zip(self.STAGES, list(out)))
+
+    def run_steps(self, first, count, stream=None):
+        rc = self.z.lib().zav1_wf_run_steps(self.h, first, count, stream)
+        if rc:
+            raise self.z.Zav1Error(rc, 'zav1_wf_run_steps (%s)' % self.z.lib().zav1_wf_last_error().decode())
 
     def run_plain(self, first, count, stream=None):
         rc = self.z.lib().zav1_wf_run_plain(self.h, first, count, stream)
