@@ -23,8 +23,10 @@ bounded sample, multi-threaded over crops.
 
 Extra stages (--extras): lf (loop-filter stage on the 4096x4096 8-bit
 reconstruction), rdoq (the trellis over 262 144 16x16 candidates), txsearch
-(exhaustive DC-predicted tx-type search, 2048x2048 10-bit), config4 (one
-tile column of the 8192x4096 frame through the tx-type search), serving.
+(exhaustive DC-predicted tx-type search, 2048x2048 10-bit), config4 (the
+8192x4096 10-bit frame, SB128, 4x4 tiles, preset 2, through the
+wavefront decision), serving (the config-5 mix through the wavefront
+decision, footprint admission, p50 / p99 latency).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--images N] [--no-graph]
 """
@@ -478,28 +480,47 @@ def rdoq_stage(zav1, synth, args, cpu=True):
     return out
 
 
-def config4_stage(zav1, synth, tsm, args, streams):
-    """Config 4 at its per-GPU shape: one tile column (2048 wide, 4096 high)
-    of the 8192x4096 10-bit frame with screen regions, tx-type search at
-    qp 28 (one quantizer), sizes spread over the bench's streams."""
-    plane = synth.config4_tile_column(4096, 2048, seed=4000)
-    sw = tsm.TxSearchSweep(zav1, plane, 10, [synth.qp_to_qindex(28)])
-    for _ in range(2):
-        sw.run(streams)
-    for sx in streams:
-        zav1.sync(sx)
-    steps = max(5, args.steps // 5)
+def config4_stage(zav1, synth, args):
+    """Config 4: the 8192x4096 10-bit frame (four seeded 2048-wide tile
+    columns of 1/f noise with 25% screen regions), SB128, 4x4 tiles,
+    preset 2, qp 28, through the wavefront mode decision with all 16
+    tiles' wavefronts in the same launches (one HIP graph), timed over
+    graph replays; graph == eager checked."""
+    wf = _load('zav1_wavefront', os.path.join(PKG, 'wavefront.py'))
+    L = zav1.lib()
+    W, H, qp, preset = 8192, 4096, 28, 2
+    img = np.concatenate([synth.config4_tile_column(H, 2048, seed=4000 + k) for k in range(4)], axis=1)
+    qidx = synth.qp_to_qindex(qp)
+    lam = int(L.zav1_kf_full_lambda(qidx, qp, 10))
+    f = wf.preset_funnel(preset, qp)
+    cfg = wf.make_cfg(W, H, 10, qidx, lam, sb=128, tiles=(4, 4), k1=f['k1'], k3=f['k3'],
+                      rdoq_level=f['rdoq_level'], txt_groups=f['txt_groups'], txt_rate_th=f['txt_rate_th'],
+                      txt_satd_th=f['txt_satd_th'], cands=f['cands'])
+    md = wf.WavefrontMD(zav1, cfg)
+    md.upload(img[None])
+    st = zav1.stream_create()
+    md.run(True, st)
+    zav1.stream_sync(st)
+    steps = max(2, args.steps // 2)
     t0 = time.perf_counter()
     for _ in range(steps):
-        sw.run(streams)
-    for sx in streams:
-        zav1.sync(sx)
+        md.run(True, st)
+    zav1.stream_sync(st)
     ms = (time.perf_counter() - t0) * 1000.0 / steps
-    mp = plane.shape[0] * plane.shape[1] / 1e6
-    return {'metric': 'MP/s (luma megapixels x qps) through the tx-type search', 'value': round(mp / ms * 1000.0, 2),
-            'unit': 'MP/s', 'ms_per_step': round(ms, 3), 'steps': steps,
-            'config': {'workload': 'config4 tile column: 2048x4096 10-bit luma (1/f noise, 25% screen regions), '
-                                   'qp 28, 19 tx sizes x legal types', 'streams': len(streams)}}
+    rec, dec = md.recon(), md.raw_decisions()
+    md.run(False, st)
+    zav1.stream_sync(st)
+    same = bool(np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec))
+    info = md.info
+    md.close()
+    zav1.stream_destroy(st)
+    return {'metric': 'MP/s through the wavefront mode decision, 16 tiles co-scheduled',
+            'value': round(W * H / 1e6 / (ms / 1000.0), 2), 'unit': 'MP/s', 'ms_per_step': round(ms, 2),
+            'steps': steps, 'graph_vs_eager_identical': same, 'wavefront_steps': info.n_steps,
+            'max_leaves_per_step': info.max_leaves_per_step,
+            'config': {'workload': 'config4: 8192x4096 10-bit luma (1/f noise + 25%% screen regions), SB128, 4x4 '
+                                   'tiles, preset 2 (%d candidates, %d -> %d survivors), qp 28, 16x16 leaves'
+                                   % (len(f['cands']), f['k1'], f['k3'])}}
 
 
 def serving_stage(zav1, synth, args):
@@ -788,7 +809,8 @@ def main():
                     help='one eager pass of the headline workload only (for rocprofv3 runs)')
     ap.add_argument('--plain-leaves', type=int, default=48)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--extras', default='lf,rdoq,serving', help='comma list of lf, rdoq, txsearch, config4, serving')
+    ap.add_argument('--extras', default='lf,rdoq,serving,config4',
+                    help='comma list of lf, rdoq, txsearch, config4, serving')
     ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--pmc-summary', nargs=4, metavar=('FETCH_DIR', 'WRITE_DIR', 'CAL_FETCH_DIR', 'CAL_WRITE_DIR'),
                     help='print the per-kernel HBM traffic summary of rocprofv3 --pmc CSV runs (merged over '
@@ -828,7 +850,7 @@ def main():
         return
     extras = [e for e in args.extras.split(',') if e]
     cpu = not args.no_cpu
-    if {'txsearch', 'config4'} & set(extras):
+    if 'txsearch' in extras:
         tsm = _load('zav1_txsearch', os.path.join(PKG, 'txsearch.py'))
     if 'lf' in extras:
         out['loopfilter_stage'] = loopfilter_stage(zav1, synth, args, cpu=cpu)
@@ -838,7 +860,7 @@ def main():
         opsd = json.load(open(os.path.join(PKG, 'csrc', 'txfm1d_ops.json')))
         out['txsearch_stage'] = txsearch_stage(zav1, synth, tsm, opsd, args)
     if 'config4' in extras:
-        out['config4_tile'] = config4_stage(zav1, synth, tsm, args, [zav1.stream_create()])
+        out['config4'] = config4_stage(zav1, synth, args)
     if 'serving' in extras:
         out['serving'] = serving_stage(zav1, synth, args)
     print(json.dumps(out))

@@ -349,6 +349,11 @@ def test_preset_funnel_settings():
     assert (f['k1'], f['k3'], f['txt_groups'], f['txt_rate_th'], f['rdoq_level']) == (5, 2, 4, 100, 3)
     f = wf.preset_funnel(8, 50)
     assert [c[0] for c in f['cands']] == [0, 1, 2, 9] and (f['k1'], f['k3'], f['txt_groups']) == (1, 1, 2)
+    assert wf.preset_candidates() == wf.angle_candidates(1, 12, (0,))
+    f = wf.preset_funnel(5, 30)
+    assert len(f['cands']) == 5 + 8 * 3 + 1 and (-2 not in [c[1] for c in f['cands']])
+    f = wf.preset_funnel(2, 28)   # nic 12: 48 -> round(48 * 28 / 63) = 21, 12 -> 5
+    assert (f['k1'], f['k3'], f['txt_satd_th']) == (21, 5, (20 * 28 + 31) // 63)
     import pytest
     with pytest.raises(ValueError):
-        wf.preset_funnel(5, 30)
+        wf.preset_funnel(0, 30)

@@ -1,0 +1,131 @@
+"""The named synthetic workloads on the GPU against the oracle.
+
+Config 1 (the plain method's still): 512x512 8-bit, qp 32, preset 8 --
+(a) the plain function-table chain residual -> forward DCT -> quantize_b ->
+SSE, one call each, over every 16x16 block of the frame against a flat
+prediction; (b) the superblock-wavefront mode decision at the preset-8
+funnel.
+
+Config 4 (large batched encode): a 1024x512 crop of the 10-bit tile column
+(1/f noise with 25% screen regions: flat palette fills and 1-px glyph
+strokes), SB128, 4x4 tiles, preset 2, qp 28 -- the wavefront mode decision
+with every tile's wavefront co-scheduled in the same launches, against
+the oracle; and each tile decided as a frame of its own gives the same
+decisions and reconstruction (tiles share nothing).
+"""
+import ctypes
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIELDS = ('cand', 'tx_type', 'eob', 'rate', 'mode_rate', 'dist', 'cost')
+
+
+def _mod(name):
+    path = os.path.join(ROOT, 'zenav1-svt_amd', name + '.py')
+    spec = importlib.util.spec_from_file_location('zav1_' + name, path)
+    m = importlib.util.module_from_spec(spec)
+    sys.modules['zav1_' + name] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def funnel_cfg(z, wf, synth, w, h, bd, qp, preset, **kw):
+    qidx = synth.qp_to_qindex(qp)
+    lam = int(z.lib().zav1_kf_full_lambda(qidx, qp, bd))
+    f = wf.preset_funnel(preset, qp, kw.get('leaf', 16))
+    return wf.make_cfg(w, h, bd, qidx, lam, k1=f['k1'], k3=f['k3'], rdoq_level=f['rdoq_level'],
+                       txt_groups=f['txt_groups'], txt_rate_th=f['txt_rate_th'], txt_satd_th=f['txt_satd_th'],
+                       cands=f['cands'], **kw)
+
+
+def decide(z, wf, cfg, imgs):
+    md = wf.WavefrontMD(z, cfg)
+    try:
+        md.upload(imgs)
+        md.run(use_graph=False)
+        z.sync()
+        return md.recon(), md.decisions()
+    finally:
+        md.close()
+
+
+def same(rec, dec, orec, odec, what):
+    assert np.array_equal(rec, orec), what
+    for f in FIELDS:
+        bad = np.argwhere(dec[f] != odec[f])
+        assert bad.size == 0, (what, f, bad[:3].tolist())
+
+
+def test_config1_plain_table_chain(gpu, oracle):
+    z = gpu
+    synth = _mod('synth')
+    img = synth.config1_luma(512, 512, seed=1000)
+    qidx = synth.qp_to_qindex(32)
+    row = oracle.quant_row(qidx, 8)
+    tab = z.dsp_table()
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    pred = np.full((16, 16), 128, np.uint8)
+    zb = np.array(list(row.zbin), np.int32)
+    rd = np.array(list(row.round), np.int32)
+    qu, qs = np.array(list(row.quant), np.int32), np.array(list(row.quant_shift), np.int32)
+    deq = np.array(list(row.dequant), np.int32)
+    res, co, q, dq = (np.zeros(256, np.int32) for _ in range(4))
+    sse = ctypes.c_uint64()
+    nz = 0
+    for by in range(0, 512, 16):
+        for bx in range(0, 512, 16):
+            blk = np.ascontiguousarray(img[by:by + 16, bx:bx + 16])
+            assert tab.residual_i32(ctypes.c_void_p(blk.ctypes.data), 16, ctypes.c_void_p(pred.ctypes.data), 16, 16,
+                                    16, res.ctypes.data_as(i32)) == 0
+            assert tab.fwd_txfm2d(res.ctypes.data_as(i32), co.ctypes.data_as(i32), 16, 2, 0) == 1
+            assert tab.quantize_b_raster(co.ctypes.data_as(i32), q.ctypes.data_as(i32), dq.ctypes.data_as(i32), 256,
+                                         zb.ctypes.data_as(i32), rd.ctypes.data_as(i32), qu.ctypes.data_as(i32),
+                                         qs.ctypes.data_as(i32), deq.ctypes.data_as(i32), 0) == 0
+            assert tab.sse_i32(co.ctypes.data_as(i32), dq.ctypes.data_as(i32), 256, ctypes.byref(sse)) == 0
+            r = blk.astype(np.int32) - 128
+            rc = oracle.fwd_txfm2d(r, 2, 0).reshape(-1)
+            rq, rdq = oracle.quantize('b', rc, row, 0, 8)
+            assert np.array_equal(res, r.reshape(-1)) and np.array_equal(co, rc), (by, bx)
+            assert np.array_equal(q, rq) and np.array_equal(dq, rdq), (by, bx)
+            assert sse.value == oracle.sse_i32(rc, rdq), (by, bx)
+            nz += int(np.count_nonzero(q))
+    assert nz > 0
+
+
+def test_config1_wavefront_preset8(gpu, oracle):
+    z = gpu
+    synth, wf = _mod('synth'), _mod('wavefront')
+    img = synth.config1_luma(512, 512, seed=1000)
+    cfg = funnel_cfg(z, wf, synth, 512, 512, 8, 32, 8)
+    rec, dec = decide(z, wf, cfg, img[None])
+    orec, odec = oracle.wf_frame(cfg, img[None])
+    same(rec, dec, orec, odec, 'config1 p8')
+    assert len(np.unique(dec['mode'])) >= 3
+
+
+def test_config4_tiles_preset2(gpu, oracle):
+    z = gpu
+    synth, wf = _mod('synth'), _mod('wavefront')
+    col = synth.config4_tile_column(4096, 2048, seed=4000)
+    img = np.ascontiguousarray(col[1024:1536, 512:1536])  # 1024 wide x 512 high, screen regions inside
+    W, H, T = 1024, 512, 4
+    cfg = funnel_cfg(z, wf, synth, W, H, 10, 28, 2, sb=128, tiles=(T, T))
+    rec, dec = decide(z, wf, cfg, img[None])
+    orec, odec = oracle.wf_frame(cfg, img[None])
+    same(rec, dec, orec, odec, 'config4 p2 4x4 tiles')
+    # every tile alone: a 256 x 128 frame of one tile
+    tw, th = W // T, H // T
+    for ty in range(T):
+        for tx in range(T):
+            sub = np.ascontiguousarray(img[ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw])
+            c1 = funnel_cfg(z, wf, synth, tw, th, 10, 28, 2, sb=128)
+            r1, d1 = decide(z, wf, c1, sub[None])
+            L = 16
+            same(rec[:, ty * th:(ty + 1) * th, tx * tw:(tx + 1) * tw], dec[:, ty * th // L:(ty + 1) * th // L,
+                 tx * tw // L:(tx + 1) * tw // L], r1, d1, ('tile', ty, tx))

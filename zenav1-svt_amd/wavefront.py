@@ -34,16 +34,11 @@ class WfInfo(ctypes.Structure):
 
 
 def preset_candidates(fi=True):
-    """The intra candidates of the mode decision at presets 0-4 (mode_end
+    """The intra candidates of the mode decision at presets 1-4 (mode_end
     PAETH, angular level 1: every delta of the 8 directional modes) with
-    FILTER_DC the only filter-intra mode (fi_max 0): 13 + 48 + 1 = 62."""
-    c = [(m, 0, 5) for m in range(13)]
-    for m in range(1, 9):
-        for d in (-3, -2, -1, 1, 2, 3):
-            c.append((m, d, 5))
-    if fi:
-        c.append((0, 0, 0))
-    return c
+    FILTER_DC the only filter-intra mode (fi_max 0): 13 + 48 + 1 = 62, in
+    the reference's injection order (angle_candidates)."""
+    return angle_candidates(1, 12, (0,) if fi else ())
 
 
 def qp_scaled_satd_th(th, cli_qp):
@@ -69,31 +64,48 @@ def _qp_factors(cli_qp):
     return max(cli_qp, 10), 63
 
 
+def angle_candidates(angular_level, mode_end=12, fi_modes=(), use_angle=True):
+    """inject_intra_candidates (rust/crates/svtav1-encoder/src/leaf_funnel/
+    mod.rs:1940-1975), in its order: modes DC..mode_end, each directional
+    mode with its deltas in place; at angular level >= 4 the D45..D67 modes
+    are masked; at levels <= 2 (and use_angle: blocks of 8x8 and up) every
+    directional mode takes the deltas -3..3 (level 2: only -3, 0, 3); then
+    the filter-intra modes (DC_PRED blocks carrying them)."""
+    c = []
+    for m in range(mode_end + 1):
+        if 3 <= m <= 8 and angular_level >= 4:
+            continue
+        if 1 <= m <= 8 and angular_level <= 2 and use_angle:
+            c += [(m, d, 5) for d in range(-3, 4) if not (angular_level >= 2 and d in (-2, -1, 1, 2))]
+        else:
+            c.append((m, 0, 5))
+    return c + [(0, 0, f) for f in fi_modes]
+
+
+# FunnelCfg::for_preset (rust/crates/svtav1-encoder/src/leaf_funnel/
+# rate_tables.rs:660): preset -> (mode_end, angular level, filter-intra
+# modes, nic num, txt groups < 16 / >= 16, SATD gate, rate gate)
+_PRESETS = {1: (12, 1, (0,), 12, 6, 6, 20, 250), 2: (12, 1, (0,), 12, 6, 6, 20, 250),
+            3: (12, 1, (0,), 6, 6, 6, 20, 250), 4: (12, 1, (0,), 6, 6, 6, 15, 250),
+            5: (12, 2, (0,), 6, 6, 6, 15, 250), 6: (9, 4, (0,), 6, 5, 4, 10, 100),
+            7: (9, 4, (), 4, 3, 2, 10, 50), 8: (9, 4, (), 0, 3, 2, 10, 50)}
+
+
 def preset_funnel(preset, cli_qp, leaf=16):
     """The wavefront MD's funnel settings at a preset for one leaf size, from
-    FunnelCfg::for_preset (rust/crates/svtav1-encoder/src/leaf_funnel/
-    rate_tables.rs:660): the candidate list (mode_end, angular level,
-    filter intra, mod.rs:1940-1975), the NIC counts of MDS1 / MDS3
-    (nic_counts, rate_tables.rs:1054: bases 64 / 16 x num / 16, at least 2,
-    qp-scaled; capped at the candidates), the tx-type groups (lt16 / ge16),
-    the rate and SATD gates (the SATD one qp-scaled) and the RDOQ level
-    (rdoq_level_allintra, quant.rs:1535: 1 up to preset 5, above it by the
-    picture's coefficient level -- taken as Normal, level 3).  Presets 4, 6
-    and 8."""
-    if preset not in (4, 6, 8):
-        raise ValueError('preset %r: 4, 6 or 8' % (preset,))
+    FunnelCfg::for_preset: the candidate list (angle_candidates; filter
+    intra only on leaves up to 32), the NIC counts of MDS1 / MDS3
+    (nic_counts, rate_tables.rs:1054: bases 64 / 16 x num / 16, at least 2
+    when num > 0 else 1, qp-scaled; capped at the candidates), the tx-type
+    groups, the rate and SATD gates (the SATD one qp-scaled) and the RDOQ
+    level (rdoq_level_allintra, quant.rs:1535: 1 up to preset 5, above it by
+    the picture's coefficient level -- taken as Normal, level 3).  Presets
+    1..8 (preset 0's 66 candidates exceed the configuration's 64)."""
+    if preset not in _PRESETS:
+        raise ValueError('preset %r: 1..8' % (preset,))
+    mode_end, ang, fi_modes, nic, g_lt16, g_ge16, satd, rate = _PRESETS[preset]
+    cands = angle_candidates(ang, mode_end, fi_modes if leaf <= 32 else (), use_angle=leaf >= 8)
     qw, qd = _qp_factors(cli_qp)
-    fi = leaf <= 32
-    if preset == 4:
-        cands, nic, groups, satd, rate, rdoq = preset_candidates(fi), 6, 6, 15, 250, 1
-    else:
-        # mode_end SMOOTH, angular level 4: D45..D67 masked, no deltas
-        cands = [(m, 0, 5) for m in (0, 1, 2, 9)]
-        if preset == 6 and fi:
-            cands.append((0, 0, 0))  # filter intra, FILTER_DC only
-        nic, groups, satd, rate, rdoq = (6, 4, 10, 100, 3) if preset == 6 else (0, 2, 10, 50, 3)
-        if leaf < 16:
-            groups = 5 if preset == 6 else 3
 
     def scale(base, num):
         lo = 2 if num else 1
@@ -101,8 +113,8 @@ def preset_funnel(preset, cli_qp, leaf=16):
         return max(lo, _div_round(n * qw, qd))
     k1 = min(scale(64, nic), len(cands))
     k3 = min(scale(16, nic), k1)
-    return dict(cands=cands, k1=k1, k3=k3, txt_groups=groups, txt_rate_th=rate,
-                txt_satd_th=qp_scaled_satd_th(satd, cli_qp), rdoq_level=rdoq)
+    return dict(cands=cands, k1=k1, k3=k3, txt_groups=g_ge16 if leaf >= 16 else g_lt16, txt_rate_th=rate,
+                txt_satd_th=qp_scaled_satd_th(satd, cli_qp), rdoq_level=1 if preset <= 5 else 3)
 
 
 def make_cfg(width, height, bd, qindex, lam, *, n_images=1, leaf=16, sb=64, tiles=(1, 1), k1=15, k3=4,
