@@ -388,15 +388,19 @@ def rdoq_stage(zav1, synth, args, cpu=True):
         for dst, src in ((d_q, d_q_keep), (d_dq, d_dq_keep), (d_eob, d_eob_keep)):
             zav1.check(L.zav1_memcpy_d2d(dst.ptr, src.ptr, src.nbytes, stream), 'memcpy_d2d')
 
+    d_ws = D((L.zav1_rdoq_workspace(ts, nj),), np.uint8)
+    use_ws = [True]
+
     def launch():
-        zav1.rdoq_batch(ts, qi, 10, d_coef, d_q, d_dq, d_eob, d_jobs, nj, d_costs, d_scans, stream=stream)
+        zav1.rdoq_batch(ts, qi, 10, d_coef, d_q, d_dq, d_eob, d_jobs, nj, d_costs, d_scans, stream=stream,
+                        d_work=d_ws if use_ws[0] else None)
 
     steps = max(5, args.steps // 5)
 
-    def time_form(threshold):
-        # the trellis form by its batch-size threshold (0: lane form; above
-        # the batch: wave form), restored afterwards
-        prev = L.zav1_rdoq_wave_max(ts, threshold)
+    def time_form(lane):
+        # the lane form (a workspace, threshold 0) or the wave form (none)
+        use_ws[0] = lane
+        prev = L.zav1_rdoq_lane_min(ts, 0)
         try:
             for _ in range(2):
                 restore()
@@ -414,10 +418,10 @@ def rdoq_stage(zav1, synth, args, cpu=True):
                 tot += ms.value
             return tot / steps, d_q.to_host()
         finally:
-            L.zav1_rdoq_wave_max(ts, prev)
+            L.zav1_rdoq_lane_min(ts, prev)
 
-    wave_ms, q_wave = time_form(nj)
-    lane_ms, q_lane = time_form(0)
+    wave_ms, q_wave = time_form(False)
+    lane_ms, q_lane = time_form(True)
     assert np.array_equal(q_wave, q_lane), 'trellis forms disagree'
     form = 'wave' if wave_ms < lane_ms else 'lane'
     ms_step = min(wave_ms, lane_ms)

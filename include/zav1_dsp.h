@@ -300,20 +300,31 @@ typedef struct {
  * sign((|q| * dequant) >> log_scale), which the trellis recomputes from the
  * levels.  d_eob[j] must be the quantizer's eob: positions at or past it
  * hold level 0 (a block whose level at scan[eob - 1] is 0 is left as it
- * is).  No allocation: the trellis works in on-chip memory. */
+ * is).  No allocation: one wave per block, in on-chip memory. */
 int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, int32_t *d_qcoeff,
                     int32_t *d_dqcoeff, uint16_t *d_eob, const zav1_rdoq_job *d_jobs, int n_jobs,
                     const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,
                     uint32_t cut_off_denum, zav1_stream_t s);
 
-/* The trellis runs in one of two forms with identical results: one wave per
- * block (short latency; for batches that cannot fill the chip) or one lane
- * per block (throughput).  A batch of tx_size with at most the size's
- * threshold of jobs takes the wave form.  Sets the threshold to n_jobs when
- * n_jobs >= 0 (0: always the lane form) and returns the previous one, or
- * ZAV1_EINVAL.  Process-wide; zav1_autotune sets it from measurements.  No
- * counterpart in the reference (a scheduling choice of this library). */
-int zav1_rdoq_wave_max(int tx_size, int n_jobs);
+/* The same trellis with a caller workspace (d_work, work_bytes; both from
+ * the caller, nothing allocated): a batch of at least the tx size's
+ * threshold of jobs (zav1_rdoq_lane_min) whose workspace holds
+ * zav1_rdoq_workspace(tx_size, n_jobs) bytes runs in the lane form (one
+ * lane per block over scan-ordered pairs gathered into the workspace: the
+ * throughput form for large batches); otherwise in the wave form of
+ * zav1_rdoq_batch.  Both forms give identical results. */
+int zav1_rdoq_batch_ws(int tx_size, int qindex, int bd, const int32_t *d_tcoeff, int32_t *d_qcoeff,
+                       int32_t *d_dqcoeff, uint16_t *d_eob, const zav1_rdoq_job *d_jobs, int n_jobs,
+                       const zav1_coeff_costs *d_costs, const uint16_t *d_scans, uint32_t cut_off_num,
+                       uint32_t cut_off_denum, void *d_work, size_t work_bytes, zav1_stream_t s);
+/* Workspace bytes of the lane form: n_jobs x (8.5 bytes per coded
+ * coefficient of tx_size). */
+size_t zav1_rdoq_workspace(int tx_size, int n_jobs);
+/* The lane form's minimum batch for tx_size: set to n_jobs when n_jobs >= 0
+ * (0: the lane form whenever a workspace is given) and returns the previous
+ * value, or ZAV1_EINVAL.  Process-wide.  No counterpart in the reference
+ * (a scheduling choice of this library). */
+int zav1_rdoq_lane_min(int tx_size, int n_jobs);
 
 /* One block of a coefficient-rate batch (cost_coeffs_txb's arguments,
  * rust/crates/svtav1-encoder/src/leaf_funnel/coeff_rate.rs:60). */

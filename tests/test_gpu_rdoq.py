@@ -11,16 +11,23 @@ import rdoq_tables as RT
 pytestmark = pytest.mark.gpu
 
 
-FORMS = {'lane': 0, 'wave': 1 << 30}
+def workspace(z, tx_size, n_jobs, form):
+    """The lane form: a workspace and a threshold of 0 (restored by the
+    caller); the wave form: no workspace."""
+    if form != 'lane':
+        return None
+    return z.DeviceArray((max(1, z.lib().zav1_rdoq_workspace(tx_size, n_jobs)),), np.uint8)
 
 
-def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70, form=None):
-    if form is not None:
-        prev = z.lib().zav1_rdoq_wave_max(tx_size, FORMS[form])
-        try:
-            return run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class)
-        finally:
-            z.lib().zav1_rdoq_wave_max(tx_size, prev)
+def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70, form='wave'):
+    prev = z.lib().zav1_rdoq_lane_min(tx_size, 0)
+    try:
+        return _run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class, form)
+    finally:
+        z.lib().zav1_rdoq_lane_min(tx_size, prev)
+
+
+def _run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class, form):
     w, h = RC.coded_dims(tx_size)
     n = w * h
     costs = RT.coeff_costs(seed)
@@ -48,7 +55,7 @@ def run_case(z, oracle, tx_size, bd, qindex, cut, seed, n_per_class=70, form=Non
     D = z.DeviceArray
     dt, dq_, ddq, de = D.from_host(T), D.from_host(Q), D.from_host(DQ), D.from_host(E)
     z.rdoq_batch(tx_size, qindex, bd, dt, dq_, ddq, de, D.from_host(jobs), len(jobs), D.from_host(costs),
-                 D.from_host(scans), *cut)
+                 D.from_host(scans), *cut, d_work=workspace(z, tx_size, len(jobs), form))
     gq, gdq, ge = dq_.to_host(), ddq.to_host(), de.to_host()
     changed = 0
     for b, j in enumerate(jobs):
@@ -149,11 +156,12 @@ def test_rdoq_and_rate_leave_invalid_jobs_alone(gpu, oracle, form):
     D = z.DeviceArray
     dq_, ddq, de = D.from_host(q), D.from_host(dq), D.from_host(eob.astype(np.uint16))
     dcosts, dscans = D.from_host(costs), D.from_host(scans)
-    prev = z.lib().zav1_rdoq_wave_max(tx_size, FORMS[form])
+    prev = z.lib().zav1_rdoq_lane_min(tx_size, 0)
     try:
-        z.rdoq_batch(tx_size, 40, 8, D.from_host(t), dq_, ddq, de, D.from_host(jobs), 8, dcosts, dscans)
+        z.rdoq_batch(tx_size, 40, 8, D.from_host(t), dq_, ddq, de, D.from_host(jobs), 8, dcosts, dscans,
+                     d_work=workspace(z, tx_size, 8, form))
     finally:
-        z.lib().zav1_rdoq_wave_max(tx_size, prev)
+        z.lib().zav1_rdoq_lane_min(tx_size, prev)
     gq, ge = dq_.to_host(), de.to_host()
     d_rate = D((8,), np.int32)
     z.coeff_rate_batch(tx_size, dq_, de, D.from_host(rjobs), 8, dcosts, dscans, d_rate)

@@ -129,7 +129,10 @@ EXPORTS = {
     'zav1_inv_txfm2d_batch': [_vp, _vp, _vp, _i32, _i32, _i32, _vp],
     'zav1_quantize_batch': [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp],
     'zav1_coeff_rate_batch': [_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp],
-    'zav1_rdoq_wave_max': [_i32, _i32],
+    'zav1_rdoq_lane_min': [_i32, _i32],
+    'zav1_rdoq_workspace': [_i32, _i32],
+    'zav1_rdoq_batch_ws': [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, ctypes.c_uint32,
+                           ctypes.c_uint32, _vp, ctypes.c_size_t, _vp],
     'zav1_cdef_apply': [_vp] * 6 + [_i32] * 4 + [_vp, _vp, _i32, _i32, _vp, _vp],
     'zav1_wiener_apply': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
     'zav1_wiener_refine': [_vp, _vp] + [_i32] * 8 + [_vp, _vp, _vp],
@@ -225,6 +228,7 @@ def lib():
         L.zav1_noalloc_scratch_bytes.restype = ctypes.c_size_t
         L.zav1_wf_last_error.restype = ctypes.c_char_p
         L.zav1_kf_full_lambda.restype = ctypes.c_uint64
+        L.zav1_rdoq_workspace.restype = ctypes.c_size_t
         _lib = L
     return _lib
 
@@ -378,11 +382,17 @@ def coeff_rate_batch(tx_size, d_qcoeff, d_eob, d_jobs, n_jobs, d_costs, d_scans,
 
 
 def rdoq_batch(tx_size, qindex, bd, d_tcoeff, d_qcoeff, d_dqcoeff, d_eob, d_jobs, n_jobs, d_costs, d_scans,
-               cut_off_num=0, cut_off_denum=0, stream=None):
-    """RDOQ of n_jobs blocks in place (zav1_rdoq_batch)."""
-    check(lib().zav1_rdoq_batch(tx_size, qindex, bd, d_tcoeff.ptr, d_qcoeff.ptr, d_dqcoeff.ptr, d_eob.ptr,
-                                d_jobs.ptr, n_jobs, d_costs.ptr, d_scans.ptr, cut_off_num, cut_off_denum, stream),
-          'rdoq_batch')
+               cut_off_num=0, cut_off_denum=0, stream=None, d_work=None):
+    """RDOQ of n_jobs blocks in place (zav1_rdoq_batch, or with a workspace
+    DeviceArray zav1_rdoq_batch_ws)."""
+    if d_work is None:
+        check(lib().zav1_rdoq_batch(tx_size, qindex, bd, d_tcoeff.ptr, d_qcoeff.ptr, d_dqcoeff.ptr, d_eob.ptr,
+                                    d_jobs.ptr, n_jobs, d_costs.ptr, d_scans.ptr, cut_off_num, cut_off_denum, stream),
+              'rdoq_batch')
+    else:
+        check(lib().zav1_rdoq_batch_ws(tx_size, qindex, bd, d_tcoeff.ptr, d_qcoeff.ptr, d_dqcoeff.ptr, d_eob.ptr,
+                                       d_jobs.ptr, n_jobs, d_costs.ptr, d_scans.ptr, cut_off_num, cut_off_denum,
+                                       d_work.ptr, d_work.nbytes, stream), 'rdoq_batch_ws')
 
 
 def sse_i32_batch(d_a, d_b, d_out, n_per_job, n_jobs, stream=None):

@@ -1056,9 +1056,10 @@ int zav1_coeff_rate_batch(int tx_size, const int32_t* d_qcoeff, const uint16_t* 
   return ZAV1_OK;
 }
 
-int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, int32_t* d_qcoeff, int32_t* d_dqcoeff,
-                    uint16_t* d_eob, const zav1_rdoq_job* d_jobs, int n_jobs, const zav1_coeff_costs* d_costs,
-                    const uint16_t* d_scans, uint32_t cut_off_num, uint32_t cut_off_denum, zav1_stream_t s) {
+int zav1_rdoq_batch_ws(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, int32_t* d_qcoeff,
+                       int32_t* d_dqcoeff, uint16_t* d_eob, const zav1_rdoq_job* d_jobs, int n_jobs,
+                       const zav1_coeff_costs* d_costs, const uint16_t* d_scans, uint32_t cut_off_num,
+                       uint32_t cut_off_denum, void* d_work, size_t work_bytes, zav1_stream_t s) {
   if (tx_size < 0 || tx_size >= 19 || qindex < 0 || qindex > 255 || (bd != 8 && bd != 10) || n_jobs < 0 ||
       (cut_off_num != 0 && cut_off_denum == 0) || (n_jobs > 0 && (!d_tcoeff || !d_qcoeff || !d_dqcoeff || !d_eob ||
                                                                   !d_jobs || !d_costs || !d_scans)))
@@ -1067,14 +1068,23 @@ int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, in
   build_row(qindex, bd, &row);
   ZCHECK(zav1::rdoq_batch(tx_size, d_tcoeff, d_qcoeff, d_dqcoeff, d_eob, reinterpret_cast<const zav1::RdoqJob*>(d_jobs),
                           n_jobs, reinterpret_cast<const int32_t*>(d_costs), d_scans, row.dequant[0],
-                          row.dequant[1], cut_off_num, cut_off_denum, S(s)));
+                          row.dequant[1], cut_off_num, cut_off_denum, S(s), d_work, d_work ? work_bytes : 0));
   return ZAV1_OK;
 }
 
-int zav1_rdoq_wave_max(int tx_size, int n_jobs) {
+int zav1_rdoq_batch(int tx_size, int qindex, int bd, const int32_t* d_tcoeff, int32_t* d_qcoeff, int32_t* d_dqcoeff,
+                    uint16_t* d_eob, const zav1_rdoq_job* d_jobs, int n_jobs, const zav1_coeff_costs* d_costs,
+                    const uint16_t* d_scans, uint32_t cut_off_num, uint32_t cut_off_denum, zav1_stream_t s) {
+  return zav1_rdoq_batch_ws(tx_size, qindex, bd, d_tcoeff, d_qcoeff, d_dqcoeff, d_eob, d_jobs, n_jobs, d_costs,
+                            d_scans, cut_off_num, cut_off_denum, nullptr, 0, s);
+}
+
+size_t zav1_rdoq_workspace(int tx_size, int n_jobs) { return zav1::rdoq_workspace(tx_size, n_jobs); }
+
+int zav1_rdoq_lane_min(int tx_size, int n_jobs) {
   if (tx_size < 0 || tx_size >= 19) return ZAV1_EINVAL;
-  const int prev = zav1::rdoq_wave_max(tx_size);
-  if (n_jobs >= 0) zav1::rdoq_set_wave_max(tx_size, n_jobs);
+  const int prev = zav1::rdoq_lane_min(tx_size);
+  if (n_jobs >= 0) zav1::rdoq_set_lane_min(tx_size, n_jobs);
   return prev;
 }
 

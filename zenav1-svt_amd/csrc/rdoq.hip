@@ -7,23 +7,16 @@
 // prices a coefficient against the levels of the coefficients after it,
 // which earlier steps may have lowered.  Two forms, bit-identical:
 //
-// Lane form (large batches): one lane per block, 64 blocks per workgroup.
-// A lane's level map lives in LDS as 4-bit levels min(|q|, 15), eight per
-// dword, dwords interleaved across the 64 lanes.  Capping at 15 instead of
-// the reference's 127 leaves every context unchanged: nz_mag clips each
-// level at 3, and br_ctx's min((l1 + l2 + l3 + 1) >> 1, 6) saturates once
-// the sum reaches 11.  The map starts at zero and a coefficient's level is
-// written when the trellis decides it.  That is the reference's map
-// (txb_init_levels, then lowered in place) wherever the trellis reads it:
-// every context neighbour of a position, (r, c+1..4), (r+1..4, c) and
-// (r+1, c+1), comes later in the scan of each class (diagonal scans go by
-// anti-diagonal r + c, the vertical class's row scan by r then c, the
-// horizontal class's column scan by c then r), so it has been decided before
-// the position is priced; positions at or past the eob hold level 0.  All
-// lanes walk one scan step per iteration of one loop (each through its own
-// phase), so they refill a per-lane LDS ring of RK (coefficient, level)
-// pairs at the same iterations: the refill's 2 x RK loads are in flight
-// together and the wave waits for memory once per RK steps.
+// Lane form (large batches, caller workspace): one lane per block, 64
+// blocks per workgroup.  A gather pass lays every block's (t, q) pairs out
+// in scan order in the caller's workspace, with each block's packed level
+// map, so a lane's walk reads consecutive pairs (one cache line serves
+// eight steps) and many resident waves hide the loads.  A lane's level map
+// lives in LDS as 4-bit levels min(|q|, 15), eight per dword, dwords
+// interleaved across the 64 lanes.  Capping at 15 instead of the
+// reference's 127 leaves every context unchanged: nz_mag clips each level at
+// 3, and br_ctx's min((l1 + l2 + l3 + 1) >> 1, 6) saturates once the sum
+// reaches 11.
 //
 // Wave form (small batches, where a few hundred blocks cannot keep the chip
 // busy and the lane form's sequential walk is the whole latency): one wave
@@ -138,18 +131,17 @@ __device__ int br_ctx_of(const LV& lv, int ci, int cls) {
 }
 
 // Level map and coefficient source of the lane form: 4-bit levels in
-// interleaved LDS words, (t, q) pairs from the lane's ring
-struct LaneMap {
-  uint32_t* lv;  // the lane's word 0; word stride 64
-  int2* ring;    // the lane's entry 0; entry stride 64
-  int ring_hi;   // scan index of entry 0 (entry k holds ring_hi - k)
+// interleaved LDS words, (t, q) pairs in scan order from the workspace
+struct GatherMap {
+  uint32_t* lv;      // the lane's word 0; word stride 64
+  const int2* work;  // the block's pairs, scan order
   __device__ __forceinline__ int get(int p) const { return (lv[(p >> 3) * 64] >> ((p & 7) * 4)) & 15; }
   __device__ __forceinline__ void set(int p, int v) {
     uint32_t& w = lv[(p >> 3) * 64];
     const int sh = (p & 7) * 4;
     w = (w & ~(15u << sh)) | ((uint32_t)v << sh);
   }
-  __device__ __forceinline__ int2 coef(int si, int) const { return ring[(ring_hi - si) * 64]; }
+  __device__ __forceinline__ int2 coef(int si, int) const { return work[si]; }
 };
 // ... of the wave form: byte levels and the raster block, all in LDS
 struct WaveMap {
@@ -165,7 +157,6 @@ template <int TXS, class Map>
 struct Trellis {
   static constexpr int AW = c_txw(TXS), AH = c_txh(TXS), BWL = c_log2(AW), N = AW * AH, NW = (N + 7) / 8;
   static constexpr int SHIFT = c_scale(TXS);
-  static constexpr int RK = N < 32 ? N : 32;
   Map m;
   int32_t* qc;         // levels and dequantized values, raster (written on change)
   int32_t* dq;
@@ -310,7 +301,6 @@ struct Trellis {
       *accu_rate += rate_low;
       *accu_dist += dist_low - dist0;
     } else {
-      set_level(ci, a);
       *accu_rate += rate;
       *accu_dist += dist - dist0;
     }
@@ -343,8 +333,10 @@ struct Trellis {
     fetch(si, ci, t, qv, d);
     if (qv == 0) return;
     const int a = qv < 0 ? -qv : qv, l = simple_level(ci, si, t, qv);
-    if (l != a) store_low(ci, a, qv < 0);
-    set_level(ci, l);
+    if (l != a) {
+      store_low(ci, a, qv < 0);
+      set_level(ci, l);
+    }
   }
   // quant.rs:1120 update_coeff_eob; nz_ci[0 .. nz_num) are the nonzero
   // positions since the eob (at most 5)
@@ -420,8 +412,6 @@ struct Trellis {
       qc[ci] = q_low;
       dq[ci] = d_low;
       set_level(ci, a - 1);
-    } else {
-      set_level(ci, a);
     }
     if (lower ? q_low != 0 : true) {  // the level kept is nonzero
 #pragma unroll
@@ -461,7 +451,6 @@ __device__ bool trellis_head_wave(Trellis<TXS, WaveMap>& s, int lane, int& eob, 
     } else {
       accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
       accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
-      s.set_level(ci, a);
     }
   }
   si--;
@@ -509,17 +498,16 @@ __device__ __forceinline__ bool job_ok(const RdoqJob& job, int eob) {
 }
 
 template <int TXS>
-__global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff,
-                                             int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
-                                             const RdoqJob* __restrict__ jobs, int n_jobs,
+__global__ __launch_bounds__(64) void k_rdoq(const int2* __restrict__ work, const uint32_t* __restrict__ lvwork,
+                                             int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
+                                             uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, int n_jobs,
                                              const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans,
                                              int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
-  using T = Trellis<TXS, LaneMap>;
-  constexpr int N = T::N, RK = T::RK;
+  using T = Trellis<TXS, GatherMap>;
+  constexpr int N = T::N;
   __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
   __shared__ uint16_t sc[3 * N];
   __shared__ uint32_t lv[T::NW * 64];
-  __shared__ int2 ring[RK * 64];
   const int lane = threadIdx.x;
   // CoeffCostTables (quant.rs:475): txb[txs_ctx * 2 + plane], then eob[ms][plane]
   const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
@@ -529,105 +517,71 @@ __global__ __launch_bounds__(64) void k_rdoq(const int32_t* __restrict__ tcoeff,
   for (int i = lane; i < 3 * N; i += 64) sc[i] = scans[i];
   __syncthreads();
   const int j = blockIdx.x * 64 + lane;
-  const bool live = j < n_jobs;
-  const RdoqJob job = live ? jobs[j] : RdoqJob{};
-  int eob = live ? eobs[j] : 0;
-  const bool ok = live && job_ok<TXS>(job, eob);
+  if (j >= n_jobs) return;
+  const RdoqJob job = jobs[j];
+  int eob = eobs[j];
+  if (!job_ok<TXS>(job, eob)) return;
   T s;
-  const int32_t* tc = tcoeff + (ok ? job.coeff : 0);
+  s.m.work = work + (size_t)j * N;
   s.m.lv = lv + lane;
-  s.m.ring = ring + lane;
-  s.m.ring_hi = 0;
-  s.qc = qcoeff + (ok ? job.coeff : 0);
-  s.dq = dqcoeff + (ok ? job.coeff : 0);
+  s.qc = qcoeff + job.coeff;
+  s.dq = dqcoeff + job.coeff;
   s.cost = tab + (job.plane_type ? kTxbWords : 0);
   s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
   // scan classes: 0 diagonal (2D), 1 row (vertical class), 2 column (horizontal class)
   s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
   s.rdmult = job.rdmult;
   s.dqv[0] = dq0, s.dqv[1] = dq1;
-  s.cls = ok ? job.tx_class : 0;
-  s.dc_ctx = ok ? job.dc_sign_ctx : 0;
+  s.cls = job.tx_class;
+  s.dc_ctx = job.dc_sign_ctx;
   const bool sharp = job.sharpness != 0;
-  for (int w = 0; w < T::NW; w++) s.m.lv[w * 64] = 0;
-  const int skip_ctx = ok ? job.txb_skip_ctx : 0;
-  const int non_skip_cost = s.cost[kSkip + skip_ctx * 2], skip_cost = s.cost[kSkip + skip_ctx * 2 + 1];
-  // phases: 0 the last coefficient, 1 the eob phase, 2 the simple walk, 3 done
-  int ph = ok ? 0 : 3, si = eob - 1, si_end = 1, accu_rate = 0, nz_num = 0, nz_ci[5] = {0, 0, 0, 0, 0};
+  // txb_init_levels (coeff_c.rs:278), packed by the gather
+  for (int w = 0; w < T::NW; w++) s.m.lv[w * 64] = lvwork[(long)j * T::NW + w];
+  const int non_skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2], skip_cost = s.cost[kSkip + job.txb_skip_ctx * 2 + 1];
+  int accu_rate = s.eob_cost(eob);
   int64_t accu_dist = 0;
-  bool write_eob = ok;
-  for (int it = 0; __ballot(ph != 3) != 0; it++) {
-    if ((it & (RK - 1)) == 0) {  // every lane at once: its next RK scan positions
-      int32_t tv[RK], qv[RK];
-#pragma unroll
-      for (int k = 0; k < RK; k++) {
-        const int i = si - k;
-        tv[k] = qv[k] = 0;
-        if (ph != 3 && i >= 0) {
-          const int ci = s.scan[i];
-          tv[k] = tc[ci];
-          qv[k] = s.qc[ci];  // not decided yet: still the quantizer's level
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < RK; k++) s.m.ring[k * 64] = make_int2(tv[k], qv[k]);
-      s.m.ring_hi = si;
+  int si = eob - 1;
+  {
+    int ci;
+    int32_t t, qv, d;
+    s.fetch(si, ci, t, qv, d);
+    // an eob that is not tight (level 0 at scan[eob - 1]) is not a block the
+    // trellis is defined on: left as it is
+    if (qv == 0) return;
+    const int a = qv < 0 ? -qv : qv, sign = qv < 0;
+    if (a >= 2) {
+      s.update_general(&accu_rate, &accu_dist, si, eob);
+    } else {
+      accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
+      accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
     }
-    if (ph == 0) {
-      int ci;
-      int32_t t, qv, d;
-      s.fetch(si, ci, t, qv, d);
-      if (qv == 0) {  // not tight: left as it is
-        ph = 3;
-        write_eob = false;
-      } else {
-        accu_rate = s.eob_cost(eob);
-        const int a = qv < 0 ? -qv : qv, sign = qv < 0;
-        if (a >= 2) {
-          s.update_general(&accu_rate, &accu_dist, si, eob);
-        } else {
-          accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
-          accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
-          s.set_level(ci, a);
-        }
-        nz_num = 1;
-        nz_ci[0] = ci;
-        si--;
-        ph = 1;
-      }
-    } else if (ph != 3) {
-      if (ph == 1 && (si < 0 || nz_num > 4)) {
-        if (si == -1 && nz_num <= 4) {  // update_skip (quant.rs:1281)
-          const int64_t rd = rdcost(s.rdmult, accu_rate + non_skip_cost, accu_dist);
-          if (!sharp && rdcost(s.rdmult, skip_cost, 0) < rd) {
+    si--;
+  }
+  int nz_num = 1, nz_ci[5] = {s.scan[eob - 1], 0, 0, 0, 0};
+  while (si >= 0 && nz_num <= 4) {
+    s.update_eob(&accu_rate, &accu_dist, &eob, &nz_num, nz_ci, si, sharp);
+    si--;
+  }
+  if (si == -1 && nz_num <= 4) {  // update_skip (quant.rs:1281)
+    const int64_t rd = rdcost(s.rdmult, accu_rate + non_skip_cost, accu_dist);
+    if (!sharp && rdcost(s.rdmult, skip_cost, 0) < rd) {
 #pragma unroll
-            for (int k = 0; k < 5; k++)
-              if (k < nz_num) s.qc[nz_ci[k]] = 0, s.dq[nz_ci[k]] = 0;
-            eob = 0;
-          }
-        }
-        if (cut_num != 0) {
-          const uint32_t cut = max((uint32_t)(N >> 7), (uint32_t)eob * cut_num / cut_den);
-          si_end = max(eob - (int)cut, 1);
-        }
-        ph = 2;
-      }
-      if (ph == 1) {
-        s.update_eob(&accu_rate, &accu_dist, &eob, &nz_num, nz_ci, si, sharp);
-        si--;
-      } else if (si >= si_end) {
-        s.update_simple(si);
-        si--;
-      } else {
-        if (si == 0) {
-          int64_t dummy = 0;
-          s.update_general(&accu_rate, &dummy, 0, eob);
-        }
-        ph = 3;
-      }
+      for (int k = 0; k < 5; k++)
+        if (k < nz_num) s.qc[nz_ci[k]] = 0, s.dq[nz_ci[k]] = 0;
+      eob = 0;
     }
   }
-  if (write_eob) eobs[j] = (uint16_t)eob;
+  int si_end = 1;
+  if (cut_num != 0) {
+    const uint32_t cut = max((uint32_t)(N >> 7), (uint32_t)eob * cut_num / cut_den);
+    si_end = max(eob - (int)cut, 1);
+  }
+  for (; si >= si_end; si--) s.update_simple(si);
+  if (si == 0) {
+    int64_t dummy = 0;
+    s.update_general(&accu_rate, &dummy, 0, eob);
+  }
+  eobs[j] = (uint16_t)eob;
 }
 
 // The wave form: kWaveBlocks blocks per workgroup, one wave each.
@@ -843,34 +797,109 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
   return hipGetLastError();
 }
 
-// Largest batch per tx size that takes the wave form (set by the
-// autotuner).  Default: every batch -- measured on MI355X the wave form is
-// ahead at every batch size up to 262 144 dense 16x16 blocks (2.06 vs
-// 2.32 ms), and far ahead on the few hundred blocks of a wavefront step.
-static int g_wave_max[19] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX,
-                             INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX,
-                             INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+// (t, q) of every scan index of every block, in scan order: the trellis then
+// reads consecutive pairs as its scan index walks down, instead of one
+// cache line of a raster row per step.  A workgroup reads whole blocks in
+// raster order (coalesced) into LDS and writes them out permuted.
+template <int TXS>
+__global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__ tcoeff,
+                                                     const int32_t* __restrict__ qcoeff,
+                                                     const uint16_t* __restrict__ eobs,
+                                                     const RdoqJob* __restrict__ jobs, int n_jobs,
+                                                     const uint16_t* __restrict__ scans, int2* __restrict__ work,
+                                                     uint32_t* __restrict__ lvwork) {
+  // 1024 coefficients per workgroup (four per thread): enough loads in
+  // flight per workgroup to hide their latency
+  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = N < 1024 ? 1024 / N : 1;
+  __shared__ uint16_t sc[3 * N];
+  __shared__ int32_t st[BPW * N], sq[BPW * N];
+  __shared__ uint32_t boff[BPW];  // per block: coefficient offset, or ~0 when it is not read
+  __shared__ int bscan[BPW];      // per block: its scan's offset in sc
+  const int tid = threadIdx.x, j0 = blockIdx.x * BPW;
+  for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
+  if (tid < BPW) {
+    const int j = j0 + tid;
+    // a block the trellis leaves alone (eob 0) is not read
+    const bool use = j < n_jobs && eobs[j] != 0;
+    const RdoqJob job = use ? jobs[j] : RdoqJob{};
+    const int cls = job.tx_class > 2 ? 0 : job.tx_class;
+    boff[tid] = use ? job.coeff : ~0u;
+    bscan[tid] = (cls == kClsVert ? 1 : (cls == kClsHoriz ? 2 : 0)) * N;
+  }
+  __syncthreads();
+  for (int k = tid; k < BPW * N; k += 256) {
+    const int b = k / N, p = k - b * N;
+    const uint32_t off = boff[b];
+    if (off != ~0u) {
+      st[k] = tcoeff[off + p];
+      sq[k] = qcoeff[off + p];
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < BPW * N; k += 256) {
+    const int b = k / N, i = k - b * N;
+    if (boff[b] != ~0u) {
+      const int ci = sc[bscan[b] + i];
+      work[(long)(j0 + b) * N + i] = make_int2(st[b * N + ci], sq[b * N + ci]);
+    }
+  }
+  // the blocks' level maps, as the trellis keeps them: min(|q|, 15) in
+  // nibbles, eight raster positions per word
+  constexpr int NW = (N + 7) / 8;
+  for (int k = tid; k < BPW * NW; k += 256) {
+    const int b = k / NW, w = k - b * NW;
+    if (boff[b] != ~0u) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (8 * w + e < N) v |= (uint32_t)min(abs(sq[b * N + 8 * w + e]), 15) << (4 * e);
+      lvwork[(long)(j0 + b) * NW + w] = v;
+    }
+  }
+}
 
-int rdoq_wave_max(int tx_size) { return tx_size >= 0 && tx_size < 19 ? g_wave_max[tx_size] : 0; }
-void rdoq_set_wave_max(int tx_size, int n) {
-  if (tx_size >= 0 && tx_size < 19) g_wave_max[tx_size] = n < 0 ? 0 : n;
+// Smallest batch per tx size that takes the lane form when a workspace is
+// given (set by the autotuner).  Default 65 536: measured on MI355X the wave
+// form is ahead at the ~28 000 blocks of a 64-image wavefront step
+// (0.33 ms), the lane form at 262 144 dense 16x16 blocks (1.33 vs 2.06 ms).
+static int g_lane_min[19] = {65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536,
+                             65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536};
+
+int rdoq_lane_min(int tx_size) { return tx_size >= 0 && tx_size < 19 ? g_lane_min[tx_size] : 0; }
+void rdoq_set_lane_min(int tx_size, int n) {
+  if (tx_size >= 0 && tx_size < 19) g_lane_min[tx_size] = n < 0 ? 0 : n;
+}
+
+size_t rdoq_workspace(int tx_size, int n_jobs) {
+  if (tx_size < 0 || tx_size > 18 || n_jobs <= 0) return 0;
+  const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
+  const int nw = (n + 7) / 8;
+  return (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t));
 }
 
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, int form) {
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws, size_t ws_bytes) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
-  const bool wave = form == kRdoqWave || (form == kRdoqAuto && n_jobs <= g_wave_max[tx_size]);
-  const dim3 grid((unsigned)((n_jobs + 63) / 64)), wgrid((unsigned)((n_jobs + kWaveBlocks - 1) / kWaveBlocks));
+  const bool lane = ws && ws_bytes >= rdoq_workspace(tx_size, n_jobs) && n_jobs >= g_lane_min[tx_size];
+  const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
+  int2* work = static_cast<int2*>(ws);
+  uint32_t* lvwork = lane ? reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n) : nullptr;
+  const int bpw = n < 1024 ? 1024 / n : 1;
+  const dim3 grid((unsigned)((n_jobs + 63) / 64)), ggrid((unsigned)((n_jobs + bpw - 1) / bpw)),
+      wgrid((unsigned)((n_jobs + kWaveBlocks - 1) / kWaveBlocks));
 #define ZR(T)                                                                                                      \
   case T:                                                                                                          \
-    if (wave)                                                                                                      \
+    if (lane) {                                                                                                    \
+      hipLaunchKernelGGL((k_rdoq_gather<T>), ggrid, dim3(256), 0, st, tcoeff, qcoeff, eobs, jobs, n_jobs, scans,   \
+                         work, lvwork);                                                                           \
+      hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, work, lvwork, qcoeff, dqcoeff, eobs, jobs, n_jobs,    \
+                         costs, scans, dq0, dq1, cut_num, cut_den);                                               \
+    } else {                                                                                                       \
       hipLaunchKernelGGL((k_rdoq_wave<T>), wgrid, dim3(64 * kWaveBlocks), 0, st, tcoeff, qcoeff, dqcoeff, eobs,    \
                          jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);                                  \
-    else                                                                                                           \
-      hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs,    \
-                         scans, dq0, dq1, cut_num, cut_den);                                                       \
+    }                                                                                                              \
     break;
   switch (tx_size) {
     ZR(0) ZR(1) ZR(2) ZR(3) ZR(4) ZR(5) ZR(6) ZR(7) ZR(8) ZR(9) ZR(10) ZR(11) ZR(12) ZR(13) ZR(14) ZR(15) ZR(16)

@@ -209,13 +209,15 @@ struct RateJob {
 static_assert(sizeof(RateJob) == 12, "RateJob layout");
 hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* eobs, const RateJob* jobs,
                             int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t* rate, hipStream_t st);
-// trellis forms: by batch size (the autotuned threshold), lane, wave
-constexpr int kRdoqAuto = 0, kRdoqLane = 1, kRdoqWave = 2;
+// The trellis: the wave form, or with a workspace of rdoq_workspace bytes
+// and at least rdoq_lane_min(tx_size) jobs the lane form.
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, int form = kRdoqAuto);
-int rdoq_wave_max(int tx_size);
-void rdoq_set_wave_max(int tx_size, int n);
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws = nullptr,
+                      size_t ws_bytes = 0);
+size_t rdoq_workspace(int tx_size, int n_jobs);
+int rdoq_lane_min(int tx_size);
+void rdoq_set_lane_min(int tx_size, int n);
 hipError_t quantize_batch(const int32_t* coeff, int32_t* qcoeff, int32_t* dqcoeff,
                           const QuantRow* rows, const uint8_t* qidx, int n_per_job, int n_jobs,
                           int log_scale, int bd, int kind, hipStream_t st);
