@@ -1205,7 +1205,7 @@ int zav1_txunit_rd_batch(const void* d_src, int stride, int plane_w, int plane_h
                                reinterpret_cast<const zav1::TxUnitRdJob*>(d_jobs), n_jobs, drow, iscan, fscan,
                                reinterpret_cast<const int32_t*>(d_costs), flags, cut_off_num, cut_off_denum, d_work,
                                d_dist, d_eob, d_rate, d_cul, d_qcoeff, d_recon, S(s), row.dequant[0],
-                               row.dequant[1]));
+                               row.dequant[1], nullptr, nullptr, work_bytes));
   return ZAV1_OK;
 }
 

@@ -859,11 +859,12 @@ __global__ __launch_bounds__(256) void k_rdoq_gather(const int32_t* __restrict__
 }
 
 // Smallest batch per tx size that takes the lane form when a workspace is
-// given (set by the autotuner).  Default 65 536: measured on MI355X the wave
-// form is ahead at the ~28 000 blocks of a 64-image wavefront step
-// (0.33 ms), the lane form at 262 144 dense 16x16 blocks (1.33 vs 2.06 ms).
-static int g_lane_min[19] = {65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536,
-                             65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536, 65536};
+// given.  Default 131 072: measured on MI355X the wave form is ahead inside
+// the 4096x4096 wavefront up to its largest steps (~41 000 mostly sparse
+// blocks at 64 images: 1.05 vs 1.66 s per pass), the lane form at 262 144
+// dense 16x16 blocks (1.49 vs 2.07 ms).
+static int g_lane_min[19] = {131072, 131072, 131072, 131072, 131072, 131072, 131072, 131072, 131072, 131072,
+                             131072, 131072, 131072, 131072, 131072, 131072, 131072, 131072, 131072};
 
 int rdoq_lane_min(int tx_size) { return tx_size >= 0 && tx_size < 19 ? g_lane_min[tx_size] : 0; }
 void rdoq_set_lane_min(int tx_size, int n) {

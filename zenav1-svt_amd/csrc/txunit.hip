@@ -393,7 +393,8 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            const QuantRow* row, const uint16_t* iscan, const uint16_t* scans, const int32_t* costs,
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
-                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd, hipEvent_t* rdoq_ev) {
+                           hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd, hipEvent_t* rdoq_ev,
+                           size_t work_bytes) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
@@ -413,6 +414,11 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   TxUnitJob* tj = reinterpret_cast<TxUnitJob*>(carve(n * sizeof(TxUnitJob)));
   RdoqJob* rj = reinterpret_cast<RdoqJob*>(carve(n * sizeof(RdoqJob)));
   RateJob* cj = reinterpret_cast<RateJob*>(carve(n * sizeof(RateJob)));
+  // workspace past the unit's own temporaries goes to the trellis (its lane
+  // form when the batch is large enough; rdoq_batch checks the size)
+  const size_t used = (size_t)(w - static_cast<char*>(work));
+  void* rws = work_bytes > used ? static_cast<void*>(w) : nullptr;
+  const size_t rws_bytes = work_bytes > used ? work_bytes - used : 0;
   hipLaunchKernelGGL(k_rd_prep, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, jobs, n_jobs, (int)np, tj,
                      rj, cj);
   hipError_t e = txunit_batch(src, stride, plane_w, plane_h, bd, tx_size, pred, pred_len, tj, n_jobs, row, iscan,
@@ -423,7 +429,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;
   if (flags & kTxuRdoq) {
     e = rdoq_batch(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
-                   st);
+                   st, rws, rws_bytes);
     if (e != hipSuccess) return e;
   }
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[1], st)) != hipSuccess) return e;

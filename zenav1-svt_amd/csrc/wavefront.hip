@@ -298,6 +298,7 @@ struct Wavefront {
   int32_t *eob1 = nullptr, *rate1 = nullptr, *eob3 = nullptr, *rate3 = nullptr;
   uint8_t *cul1 = nullptr, *cul3 = nullptr;
   void *ws1 = nullptr, *ws3 = nullptr;
+  size_t ws3_bytes = 0;  // MDS3's workspace, with room for the trellis's lane form
   // plain per-call form scratch
   uint8_t* one_sel = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -439,6 +440,12 @@ static int plan(Wavefront* w) {
   return ZAV1_OK;
 }
 
+// MDS3's transform-unit workspace plus the trellis's lane-form workspace,
+// so that a batch large enough for the lane form (rdoq_lane_min) can take it
+static size_t ws3_bytes(int tx, int n3) {
+  return ((txunit_rd_workspace(tx, n3) + 255) & ~(size_t)255) + rdoq_workspace(tx, n3);
+}
+
 static size_t footprint(const Wavefront& w) {
   const zav1_wf_cfg& c = w.cfg;
   const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w.stacked_h * es, U = (size_t)w.max_u;
@@ -450,7 +457,7 @@ static size_t footprint(const Wavefront& w) {
        al(U * w.k1);
   const size_t n3 = U * w.k3 * w.nt;
   b += al(n3 * sizeof(TxUnitRdJob)) + 2 * al(n3 * 8) + 2 * al(n3 * 4) + al(n3) + al(n3 * LL * 2);
-  b += al(txunit_rd_workspace(w.tx, (int)(U * w.k1))) + al(txunit_rd_workspace(w.tx, (int)n3)) + al(64);
+  b += al(txunit_rd_workspace(w.tx, (int)(U * w.k1))) + al(ws3_bytes(w.tx, (int)n3)) + al(64);
   return b;
 }
 
@@ -505,7 +512,8 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   MARK(7);
   e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs3, n3, w->row, w->iscan,
                       w->fscan, w->costs, w->flags3, w->cut_num, w->cut_den, w->ws3, w->dist3, w->eob3, w->rate3,
-                      w->cul3, nullptr, w->rec3, st, w->dq0, w->dq1, w->csatd3, clk ? &clk->ev[8] : nullptr);
+                      w->cul3, nullptr, w->rec3, st, w->dq0, w->dq1, w->csatd3, clk ? &clk->ev[8] : nullptr,
+                      w->ws3_bytes);
   if (e != hipSuccess) return e;
   MARK(10);
   if (c.bd == 8)
@@ -782,7 +790,8 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   w->cul3 = (uint8_t*)take(n3);
   w->rec3 = (uint16_t*)take(n3 * LL * 2);
   w->ws1 = take(zav1::txunit_rd_workspace(w->tx, (int)(U * w->k1)));
-  w->ws3 = take(zav1::txunit_rd_workspace(w->tx, (int)n3));
+  w->ws3_bytes = zav1::ws3_bytes(w->tx, (int)n3);
+  w->ws3 = take(w->ws3_bytes);
   std::vector<int32_t> uid(n);
   for (size_t i = 0; i < n; i++) uid[i] = (int32_t)i;
   hipError_t e = hipSuccess;

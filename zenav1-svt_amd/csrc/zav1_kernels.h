@@ -339,7 +339,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
                            hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr,
-                           hipEvent_t* rdoq_ev = nullptr);
+                           hipEvent_t* rdoq_ev = nullptr, size_t work_bytes = 0);
 // The current device's quantizer row `row_index` (bd index * 256 + qindex),
 // inverse scans and forward-scan base, as built by zav1_init (no building).
 int device_tables(const QuantRow** row, const uint16_t** iscan, const uint16_t** fscan_base, int row_index);
