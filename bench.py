@@ -496,10 +496,7 @@ def config4_stage(zav1, synth, args):
     img = np.concatenate([synth.config4_tile_column(H, 2048, seed=4000 + k) for k in range(4)], axis=1)
     qidx = synth.qp_to_qindex(qp)
     lam = int(L.zav1_kf_full_lambda(qidx, qp, 10))
-    f = wf.preset_funnel(preset, qp)
-    cfg = wf.make_cfg(W, H, 10, qidx, lam, sb=128, tiles=(4, 4), k1=f['k1'], k3=f['k3'],
-                      rdoq_level=f['rdoq_level'], txt_groups=f['txt_groups'], txt_rate_th=f['txt_rate_th'],
-                      txt_satd_th=f['txt_satd_th'], cands=f['cands'])
+    cfg = wf.funnel_cfg(W, H, 10, qidx, lam, preset, qp, sb=128, tiles=(4, 4))
     md = wf.WavefrontMD(zav1, cfg)
     md.upload(img[None])
     st = zav1.stream_create()
@@ -600,8 +597,7 @@ def wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args):
     import oracle as O
     C = 512
     crops = [np.ascontiguousarray(imgs[0, y:y + C, x:x + C]) for y in range(0, 4 * C, C) for x in range(0, 4 * C, C)]
-    cfg = wf.make_cfg(C, C, 10, qidx, lam, n_images=1, leaf=16, k1=15, k3=4, rdoq_level=1,
-                      txt_satd_th=wf.qp_scaled_satd_th(15, qp))
+    cfg = wf.funnel_cfg(C, C, 10, qidx, lam, 4, qp, n_images=1, leaf=16)
     model, cpus = cpu_info()
     threads = min(16, len(cpus), len(crops))
     keep = os.sched_getaffinity(0)
@@ -635,8 +631,7 @@ def wavefront_headline(zav1, synth, args):
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
         imgs = np.stack(list(ex.map(lambda i: synth.config2_luma(S, S, seed=2000 + i), range(n_img))))
-    cfg = wf.make_cfg(S, S, 10, qidx, lam, n_images=n_img, leaf=16, k1=15, k3=4, rdoq_level=1,
-                      txt_satd_th=wf.qp_scaled_satd_th(15, qp))
+    cfg = wf.funnel_cfg(S, S, 10, qidx, lam, 4, qp, n_images=n_img, leaf=16)  # preset 4: k1 15, k3 4
     md = wf.WavefrontMD(zav1, cfg)
     md.upload(imgs)
     st = zav1.stream_create()

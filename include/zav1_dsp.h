@@ -462,8 +462,12 @@ int zav1_txunit_batch(const void *d_src, int stride, int plane_w, int plane_h, i
  * the caller's (the call allocates nothing); calls sharing a workspace must
  * be ordered on one stream.  Outputs per job: d_dist, d_eob (-1 no kernel
  * for the type, -22 unit out of bounds: then rate 0), d_rate (1/512 bit),
- * d_cul; optional d_qcoeff (final packed levels) and d_recon (w*h u16). */
+ * d_cul; optional d_qcoeff (final packed levels) and d_recon (w*h u16).
+ * ZAV1_TXU_RATE_EST2: the luma rate of a unit whose eob is below
+ * (w*h) >> 6 is the fast approximation 6000 + 1000 * eob instead of the
+ * exact one (coeff_rate_est_lvl 2 of presets 7 and 8, tx_pipeline.rs:605). */
 #define ZAV1_TXU_RDOQ 8u
+#define ZAV1_TXU_RATE_EST2 16u
 typedef struct zav1_txunit_rd_job {
   int32_t x, y;           /* unit origin in d_src (samples) */
   uint32_t pred_off;      /* d_pred offset (samples) of the w*h prediction */
@@ -589,7 +593,8 @@ int zav1_md_predict_survivors(const void *d_recon, int stride, int plane_w, int 
  * (the intra set's tx types in TX_TYPE_GROUPS order, quantize_fp + RDOQ at
  * rdoq_level or quantize_b at level 0, spatial distortion; the tx-type rate
  * gate txt_rate_th, the coefficient-SATD early exit txt_satd_th
- * (qp-scaled), no signalable eob-0 non-DCT), the least full cost (mode rate
+ * (qp-scaled), no signalable eob-0 non-DCT), in MDS1 and MDS3 the luma rate
+ * at coeff_rate_est_lvl 2 approximated for small eobs, the least full cost (mode rate
  * + coefficient rate, distortion) first in order; its reconstruction is
  * committed before the leaves that read it.  Superblocks run in
  * anti-diagonal order within each tile (every tile and image of the batch
@@ -608,6 +613,7 @@ typedef struct zav1_wf_cfg {
   int32_t txt_groups;             /* TX_TYPE_GROUPS searched (1..6) */
   int32_t txt_rate_th;            /* txt_rate_cost_th */
   int32_t txt_satd_th;            /* qp-scaled SATD early-exit threshold (0 = off) */
+  int32_t coeff_rate_est_lvl;     /* 0, 1 exact luma rate; 2 the fast approximation (ZAV1_TXU_RATE_EST2) */
   uint64_t lambda;                /* full lambda */
   int32_t n_cands;                /* <= 64 */
   int32_t pad;

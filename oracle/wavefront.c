@@ -66,10 +66,12 @@ typedef struct {
 /* One transform unit of the funnel (tx_unit_inner with do_rdoq and the exact
  * rate): quantize_b (rdoq level 0) or quantize_fp + optimize_b, rate with the
  * tx-type rate (skip cost at eob 0), distortion frequency-domain or spatial
- * with the reconstruction. */
+ * with the reconstruction.  est2: coeff_rate_est_lvl 2, the luma rate of an
+ * eob below L*L/64 is 6000 + 1000 * eob
+ * (rust/crates/svtav1-encoder/src/leaf_funnel/tx_pipeline.rs:605-616). */
 static void rd_unit(const void *src, int stride, int hbd, int bd, long off, const int32_t *pred, int L, int tx,
                     int type, int qidx, const or_txb_costs *txb, const or_eob_costs *eobc, int use_rdoq,
-                    int cut_num, int cut_den, int64_t rdmult, int txt_rate, int spatial, rd_out *o) {
+                    int cut_num, int cut_den, int64_t rdmult, int txt_rate, int spatial, int est2, rd_out *o) {
   const int n = L * L, ls = or_tx_scale(tx);
   int32_t res[32 * 32], coef[32 * 32], q[32 * 32], dq[32 * 32], inv[32 * 32];
   for (int r = 0; r < L; r++)
@@ -95,6 +97,7 @@ static void rd_unit(const void *src, int stride, int hbd, int bd, long off, cons
   o->eob = eob;
   o->rate = eob ? or_cost_coeffs_txb(q, eob, scan, &txb[txs * 2], &eobc[ems * 2], tx, cls, 0, 0, 0, txt_rate)
                 : txb[txs * 2].txb_skip_cost[0][1];
+  if (est2 && eob < (n >> 6)) o->rate = 6000 + 1000 * eob;
   if (eob)
     or_inv_txfm2d(dq, inv, L, tx, type, bd);
   else
@@ -192,7 +195,7 @@ void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decis
                 const int dir = cd->fi_mode < 5 ? FI_DIR[cd->fi_mode] : cd->mode;
                 rd_out o;
                 rd_unit(s, W, hbd, bd, off, pred + (size_t)m * L * L, L, tx, 0, cfg->qindex, txb, &eobc[0][0], 0, 0,
-                        0, rdmult, or_intra_txt_rate(tx, dir, 0), 0, &o);
+                        0, rdmult, or_intra_txt_rate(tx, dir, 0), 0, cfg->coeff_rate_est_lvl == 2, &o);
                 c1[rr] = or_rdcost(cfg->lambda, (uint64_t)mrate[m] + (uint32_t)o.rate, o.dist);
               }
               or_exchange_sort(c1, k1, ord3);
@@ -210,7 +213,7 @@ void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decis
                   rd_out *o = &o3[ii * 16 + t];
                   const int txr = or_intra_txt_rate(tx, dir, types[t]);
                   rd_unit(s, W, hbd, bd, off, pred + (size_t)m * L * L, L, tx, types[t], cfg->qindex, txb,
-                          &eobc[0][0], use_rdoq, cut_num, cut_den, rdmult, txr, 1, o);
+                          &eobc[0][0], use_rdoq, cut_num, cut_den, rdmult, txr, 1, cfg->coeff_rate_est_lvl == 2, o);
                   if (types[t] != 0 && dct_cost != ~0ull &&
                       or_rdcost(cfg->lambda, (uint64_t)txr, 0) * 1000 > dct_cost * (uint64_t)cfg->txt_rate_th)
                     continue;

@@ -292,7 +292,7 @@ typedef struct {
 } or_wf_cand;
 typedef struct { /* same layout as zav1_wf_cfg */
   int32_t width, height, n_images, bd, leaf, sb, tile_cols, tile_rows, qindex, k1, k3, rdoq_level, txt_groups,
-      txt_rate_th, txt_satd_th;
+      txt_rate_th, txt_satd_th, coeff_rate_est_lvl;
   uint64_t lambda;
   int32_t n_cands, pad;
   or_wf_cand cands[64];

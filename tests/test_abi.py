@@ -248,7 +248,7 @@ def test_txunit_rd_validation_without_gpu(zav1):
                                                              None, None)
     big = 1 << 30
     assert L.zav1_txunit_rd_batch(*args(1 | 8, f, big)) == zav1.ZAV1_EINVAL
-    assert L.zav1_txunit_rd_batch(*args(16, f, big)) == zav1.ZAV1_EINVAL
+    assert L.zav1_txunit_rd_batch(*args(32, f, big)) == zav1.ZAV1_EINVAL
     assert L.zav1_txunit_rd_batch(*args(8, f, big, (60, 0))) == zav1.ZAV1_EINVAL
     assert L.zav1_txunit_rd_batch(*args(8, f, big, costs=None)) == zav1.ZAV1_EINVAL
     assert L.zav1_txunit_rd_batch(*args(8, None, big)) == zav1.ZAV1_EINVAL
@@ -343,7 +343,8 @@ def test_preset_funnel_settings():
     spec.loader.exec_module(wf)
     f = wf.preset_funnel(4, 40)
     assert (len(f['cands']), f['k1'], f['k3'], f['txt_groups'], f['txt_rate_th'], f['rdoq_level']) == (62, 15, 4, 6, 250, 1)
-    assert f['txt_satd_th'] == (15 * 40 + 31) // 63
+    assert f['txt_satd_th'] == (15 * 40 + 31) // 63 and f['coeff_rate_est_lvl'] == 1
+    assert [wf.preset_funnel(p, 30)['coeff_rate_est_lvl'] for p in range(1, 9)] == [1] * 6 + [2, 2]
     f = wf.preset_funnel(6, 20)
     assert [c[0] for c in f['cands']] == [0, 1, 2, 9, 0] and f['cands'][-1][2] == 0
     assert (f['k1'], f['k3'], f['txt_groups'], f['txt_rate_th'], f['rdoq_level']) == (5, 2, 4, 100, 3)

@@ -38,10 +38,7 @@ def _mod(name):
 def funnel_cfg(z, wf, synth, w, h, bd, qp, preset, **kw):
     qidx = synth.qp_to_qindex(qp)
     lam = int(z.lib().zav1_kf_full_lambda(qidx, qp, bd))
-    f = wf.preset_funnel(preset, qp, kw.get('leaf', 16))
-    return wf.make_cfg(w, h, bd, qidx, lam, k1=f['k1'], k3=f['k3'], rdoq_level=f['rdoq_level'],
-                       txt_groups=f['txt_groups'], txt_rate_th=f['txt_rate_th'], txt_satd_th=f['txt_satd_th'],
-                       cands=f['cands'], **kw)
+    return wf.funnel_cfg(w, h, bd, qidx, lam, preset, qp, **kw)
 
 
 def decide(z, wf, cfg, imgs):

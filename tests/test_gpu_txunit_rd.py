@@ -104,6 +104,39 @@ def test_txunit_rd_bitexact(gpu, oracle, bd, mode):
                 assert np.array_equal(got['rec'][j], rec), ctx
 
 
+def test_txunit_rd_rate_est2(gpu):
+    """ZAV1_TXU_RATE_EST2 (coeff_rate_est_lvl 2, tx_pipeline.rs:605): a luma
+    unit with eob < (w*h) >> 6 is priced 6000 + 1000 * eob, every other unit
+    (and every other output) is the exact run's."""
+    z = gpu
+    rng = np.random.default_rng(31)
+    PH, PW = 128, 128
+    plane = np.clip(rng.normal(512, 40, (PH, PW)), 0, 1023).astype(np.uint16)
+    d_src = z.DeviceArray.from_host(plane)
+    qidx = 180
+    d_costs = z.DeviceArray.from_host(z.build_coeff_costs(qidx))
+    for ts in (1, 2, 3, 4, 9):
+        w, h = TX_DIMS[ts]
+        n = 24
+        jobs = np.zeros(n, z.TXUNIT_RD_JOB)
+        preds = []
+        for j in range(n):
+            x, y = int(rng.integers(0, PW - w + 1)), int(rng.integers(0, PH - h + 1))
+            amp = 4 + 12 * (j % 6)
+            preds.append(np.clip(plane[y:y + h, x:x + w].astype(np.int64) + rng.integers(-amp, amp + 1, (h, w)),
+                                 0, 1023).astype(np.uint16))
+            jobs[j] = (x, y, j * w * h, 0, w, h, j % 3 == 2, 0, 0, 0, 0, 40, 100000)
+        for flags in (1, 8 | 2):
+            a = run_rd(z, d_src, PW, PW, PH, 10, ts, preds, jobs, qidx, flags, (0, 0), d_costs)
+            b = run_rd(z, d_src, PW, PW, PH, 10, ts, preds, jobs, qidx, flags | 16, (0, 0), d_costs)
+            small = (a['eob'] < (w * h) >> 6) & (jobs['plane_type'] == 0)
+            assert small.any() and (~small).any(), (ts, flags)
+            assert np.array_equal(b['rate'][small], 6000 + 1000 * a['eob'][small]), (ts, flags)
+            assert np.array_equal(b['rate'][~small], a['rate'][~small]), (ts, flags)
+            for k in ('eob', 'dist', 'cul', 'q', 'rec'):
+                assert np.array_equal(a[k], b[k]), (ts, flags, k)
+
+
 # --- the C encoder's records at the frame origin, through the device --------
 
 RDOQ_LEVEL = {('gradient_64x64', 20): 0, ('gradient_64x64', 40): 3, ('gradient_64x64', 55): 3,

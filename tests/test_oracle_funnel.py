@@ -4,6 +4,9 @@ properties its reference documentation states: it yields a permutation with
 non-decreasing costs, equals a stable ascending sort when all costs differ,
 and on a tie displaces the first member of a tie group behind the rest of
 its group when a strictly smaller cost follows the group."""
+import importlib.util
+import os
+
 import numpy as np
 import pytest
 
@@ -149,3 +152,38 @@ def test_origin_block_satd_matches_c_captures(oracle):
                 assert oracle.block_satd(blk, p, False) == dist, (f['name'], mode, fi)
                 seen += 1
     assert seen >= 20
+
+
+def _pkg_mod(name):
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location('zav1_t_' + name, os.path.join(root, 'zenav1-svt_amd', name + '.py'))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_wf_rate_est2_closed_form(oracle):
+    """coeff_rate_est_lvl 2 (presets 7, 8; leaf_funnel/tx_pipeline.rs:605):
+    every committed 16x16 leaf whose eob is below 256 >> 6 = 4 is priced
+    6000 + 1000 * eob, the others keep the exact rate of level 1; the cost
+    is the full cost of that rate."""
+    wf, synth = _pkg_mod('wavefront'), _pkg_mod('synth')
+    img = synth.config1_luma(128, 128, seed=7)
+    qp = 50
+    qidx = synth.qp_to_qindex(qp)
+    lam = 1500000
+    c2 = wf.funnel_cfg(128, 128, 8, qidx, lam, 8, qp)
+    assert int(c2[0]['coeff_rate_est_lvl']) == 2
+    c1 = c2.copy()
+    c1[0]['coeff_rate_est_lvl'] = 1
+    _, d2 = oracle.wf_frame(c2, img[None])
+    _, d1 = oracle.wf_frame(c1, img[None])
+    d2, d1 = d2.ravel(), d1.ravel()
+    small = d2['eob'] < 4
+    assert small.any() and (~small).any()
+    assert np.array_equal(d2['rate'][small], 6000 + 1000 * d2['eob'][small])
+    assert np.all(d2['cost'] == [oracle.rdcost(lam, int(m) + int(r), int(d))
+                                 for m, r, d in zip(d2['mode_rate'], d2['rate'], d2['dist'])])
+    # level 1 never prices the closed form on these leaves
+    s1 = d1['eob'] < 4
+    assert not np.array_equal(d1['rate'][s1], 6000 + 1000 * d1['eob'][s1])

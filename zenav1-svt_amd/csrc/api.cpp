@@ -1183,7 +1183,7 @@ int zav1_txunit_rd_batch(const void* d_src, int stride, int plane_w, int plane_h
                          zav1_stream_t s) {
   static_assert(sizeof(zav1_txunit_rd_job) == sizeof(zav1::TxUnitRdJob), "rd job layout");
   if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_jobs < 0 || qindex < 0 || qindex > 255 ||
-      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~15u) ||
+      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~31u) ||
       ((flags & ZAV1_TXU_RDOQ) && (flags & ZAV1_TXU_QUANT_B)) || (cut_off_num != 0 && cut_off_denum == 0) ||
       (n_jobs && (!d_src || !d_pred || !d_jobs || !d_costs || !d_work || !d_dist || !d_eob || !d_rate || !d_cul)))
     return ZAV1_EINVAL;
@@ -1216,7 +1216,7 @@ int zav1_txunit_rd_batch_noalloc(const void* d_src, int stride, int plane_w, int
                                  int32_t* d_eob, int32_t* d_rate, uint8_t* d_cul, int32_t* d_qcoeff,
                                  uint16_t* d_recon, zav1_stream_t s) {
   if (tx_size < 0 || tx_size >= 19 || (bd != 8 && bd != 10) || n_jobs < 0 || qindex < 0 || qindex > 255 ||
-      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~15u) ||
+      plane_w <= 0 || plane_h <= 0 || stride < plane_w || (flags & ~31u) ||
       ((flags & ZAV1_TXU_RDOQ) && (flags & ZAV1_TXU_QUANT_B)) || (cut_off_num != 0 && cut_off_denum == 0) ||
       (n_jobs && (!d_src || !d_pred || !d_jobs || !d_costs || !d_dist || !d_eob || !d_rate || !d_cul)))
     return ZAV1_EINVAL;

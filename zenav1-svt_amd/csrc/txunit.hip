@@ -353,6 +353,9 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
   out_eob[job] = eob;
   // cost_skip_txb: txb_skip_cost[ctx][1] of the unit's (txs_ctx, plane) tables
   if (eob == 0) out_rate[job] = costs[(txs_ctx * 2 + (jb.plane_type ? 1 : 0)) * 944 + jb.txb_skip_ctx * 2 + 1];
+  // coeff_rate_est_lvl 2 (tx_pipeline.rs:605): small-eob luma units take the
+  // closed form in place of the exact rate (eob 0 included)
+  if ((flags & kTxuRateEst2) && jb.plane_type == 0 && eob < ((W * H) >> 6)) out_rate[job] = 6000 + 1000 * eob;
   uint32_t cl = (uint32_t)(lsum < 63 ? lsum : 63);
   const int32_t dcq = tq[base];
   if (dcq < 0) cl |= 1u << 6;

@@ -271,7 +271,7 @@ struct Wavefront {
   std::vector<WfCand> h_cands;
   uint64_t rate_th = 0;
   int64_t satd_th = 0, rdmult = 0;
-  unsigned flags3 = 0;
+  unsigned flags1 = 0, flags3 = 0;
   uint32_t cut_num = 0, cut_den = 0;
   int32_t dq0 = 0, dq1 = 0;
   const QuantRow* row = nullptr;
@@ -501,7 +501,7 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
                      w->k1, w->cands, w->txt, L, w->rdmult, w->jobs1);
   MARK(5);
   e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs1, U * w->k1, w->row,
-                      w->iscan, w->fscan, w->costs, kTxuQuantB, 0, 0, w->ws1, w->dist1, w->eob1, w->rate1, w->cul1,
+                      w->iscan, w->fscan, w->costs, w->flags1, 0, 0, w->ws1, w->dist1, w->eob1, w->rate1, w->cul1,
                       nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
   if (e != hipSuccess) return e;
   MARK(6);
@@ -574,7 +574,7 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
                      w->jobs1);
   for (int r = 0; r < k1; r++) {
     e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)k1 * L * L, w->jobs1 + r, 1, w->row, w->iscan,
-                        w->fscan, w->costs, kTxuQuantB, 0, 0, w->ws1, w->dist1 + r, w->eob1 + r, w->rate1 + r,
+                        w->fscan, w->costs, w->flags1, 0, 0, w->ws1, w->dist1 + r, w->eob1 + r, w->rate1 + r,
                         w->cul1 + r, nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
     if (e != hipSuccess) return e;
   }
@@ -672,7 +672,8 @@ int check_cfg(const zav1_wf_cfg* c) {
       c->height % c->leaf || c->n_images < 1 || c->tile_cols < 1 || c->tile_rows < 1 || c->qindex < 0 ||
       c->qindex > 255 || c->n_cands < 1 || c->n_cands > 64 || c->k1 < 1 || c->k1 > c->n_cands || c->k3 < 1 ||
       c->k3 > c->k1 || c->rdoq_level < 0 || c->rdoq_level > 3 || c->txt_groups < 1 || c->txt_groups > 6 ||
-      c->txt_rate_th < 0 || c->txt_satd_th < 0 || c->lambda == 0 ||
+      c->txt_rate_th < 0 || c->txt_satd_th < 0 || c->coeff_rate_est_lvl < 0 || c->coeff_rate_est_lvl > 2 ||
+      c->lambda == 0 ||
       (long)c->width * c->height * c->n_images > (1L << 31))
     return ZAV1_EINVAL;
   for (int i = 0; i < c->n_cands; i++) {
@@ -724,6 +725,12 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     w->flags3 = ZAV1_TXU_RDOQ | ZAV1_TXU_SPATIAL | ZAV1_TXU_RECON;
     w->cut_num = c.rdoq_level == 1 ? 0 : (c.rdoq_level == 2 ? 80 : 60);
     w->cut_den = c.rdoq_level == 1 ? 0 : 100;
+  }
+  // the luma rate approximation prices MDS1 and MDS3 alike
+  w->flags1 = ZAV1_TXU_QUANT_B;
+  if (c.coeff_rate_est_lvl == 2) {
+    w->flags1 |= ZAV1_TXU_RATE_EST2;
+    w->flags3 |= ZAV1_TXU_RATE_EST2;
   }
   w->cl.n = c.n_cands;
   static const uint8_t kFiDir[5] = {0, 1, 2, 6, 0};  // FIMODE_TO_INTRADIR

@@ -272,7 +272,7 @@ struct TxUnitJob {
   uint32_t pred_off;
   uint8_t tx_type, crop_w, crop_h, pad;
 };
-enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u, kTxuRdoq = 8u };
+enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u, kTxuRdoq = 8u, kTxuRateEst2 = 16u };
 // Intermediates a transform unit keeps for the RDOQ trellis and the
 // finishing pass (all null for the plain fused unit): the packed pre-quant
 // coefficients and dequantized values (min(w,32) x min(h,32) per job), the

@@ -43,10 +43,7 @@ class WavefrontServer:
     def request_cfg(self, h, w, bd, qp, preset):
         qidx = self.synth.qp_to_qindex(qp)
         lam = int(self.L.zav1_kf_full_lambda(qidx, qp, bd))
-        f = self.wf.preset_funnel(preset, qp, self.leaf)
-        return self.wf.make_cfg(w, h, bd, qidx, lam, leaf=self.leaf, k1=f['k1'], k3=f['k3'],
-                                rdoq_level=f['rdoq_level'], txt_groups=f['txt_groups'],
-                                txt_rate_th=f['txt_rate_th'], txt_satd_th=f['txt_satd_th'], cands=f['cands'])
+        return self.wf.funnel_cfg(w, h, bd, qidx, lam, preset, qp, leaf=self.leaf)
 
     def _event(self):
         if self.pool_ev:

@@ -17,7 +17,7 @@ CAND = np.dtype([('mode', 'u1'), ('delta', 'i1'), ('fi_mode', 'u1'), ('pad', 'u1
 CFG = np.dtype([('width', '<i4'), ('height', '<i4'), ('n_images', '<i4'), ('bd', '<i4'), ('leaf', '<i4'),
                 ('sb', '<i4'), ('tile_cols', '<i4'), ('tile_rows', '<i4'), ('qindex', '<i4'), ('k1', '<i4'),
                 ('k3', '<i4'), ('rdoq_level', '<i4'), ('txt_groups', '<i4'), ('txt_rate_th', '<i4'),
-                ('txt_satd_th', '<i4'), ('pad0', '<i4'), ('lambda', '<u8'), ('n_cands', '<i4'), ('pad', '<i4'),
+                ('txt_satd_th', '<i4'), ('coeff_rate_est_lvl', '<i4'), ('lambda', '<u8'), ('n_cands', '<i4'), ('pad', '<i4'),
                 ('cands', CAND, 64)])
 DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
                      ('rate', '<i4'), ('mode_rate', '<i4'), ('dist', '<u8'), ('cost', '<u8')])
@@ -99,7 +99,9 @@ def preset_funnel(preset, cli_qp, leaf=16):
     when num > 0 else 1, qp-scaled; capped at the candidates), the tx-type
     groups, the rate and SATD gates (the SATD one qp-scaled) and the RDOQ
     level (rdoq_level_allintra, quant.rs:1535: 1 up to preset 5, above it by
-    the picture's coefficient level -- taken as Normal, level 3).  Presets
+    the picture's coefficient level -- taken as Normal, level 3) and the luma
+    rate estimate (coeff_rate_est_lvl: 2, the small-eob closed form, at
+    presets 7 and 8, rate_tables.rs:964).  Presets
     1..8 (preset 0's 66 candidates exceed the configuration's 64)."""
     if preset not in _PRESETS:
         raise ValueError('preset %r: 1..8' % (preset,))
@@ -114,11 +116,19 @@ def preset_funnel(preset, cli_qp, leaf=16):
     k1 = min(scale(64, nic), len(cands))
     k3 = min(scale(16, nic), k1)
     return dict(cands=cands, k1=k1, k3=k3, txt_groups=g_ge16 if leaf >= 16 else g_lt16, txt_rate_th=rate,
-                txt_satd_th=qp_scaled_satd_th(satd, cli_qp), rdoq_level=1 if preset <= 5 else 3)
+                txt_satd_th=qp_scaled_satd_th(satd, cli_qp), rdoq_level=1 if preset <= 5 else 3,
+                coeff_rate_est_lvl=2 if preset >= 7 else 1)
+
+
+def funnel_cfg(width, height, bd, qindex, lam, preset, cli_qp, **kw):
+    """make_cfg with every funnel setting of the preset (preset_funnel) at
+    the configuration's leaf size."""
+    f = preset_funnel(preset, cli_qp, kw.get('leaf', 16))
+    return make_cfg(width, height, bd, qindex, lam, **f, **kw)
 
 
 def make_cfg(width, height, bd, qindex, lam, *, n_images=1, leaf=16, sb=64, tiles=(1, 1), k1=15, k3=4,
-             rdoq_level=1, txt_groups=6, txt_rate_th=250, txt_satd_th=0, cands=None):
+             rdoq_level=1, txt_groups=6, txt_rate_th=250, txt_satd_th=0, coeff_rate_est_lvl=1, cands=None):
     cands = preset_candidates() if cands is None else cands
     cfg = np.zeros(1, CFG)
     c = cfg[0]
@@ -126,6 +136,7 @@ def make_cfg(width, height, bd, qindex, lam, *, n_images=1, leaf=16, sb=64, tile
     c['tile_cols'], c['tile_rows'], c['qindex'], c['k1'], c['k3'] = tiles[0], tiles[1], qindex, k1, k3
     c['rdoq_level'], c['txt_groups'], c['txt_rate_th'], c['txt_satd_th'] = rdoq_level, txt_groups, txt_rate_th, \
         txt_satd_th
+    c['coeff_rate_est_lvl'] = coeff_rate_est_lvl
     c['lambda'], c['n_cands'] = lam, len(cands)
     for i, (m, d, f) in enumerate(cands):
         c['cands'][i] = (m, d, f, 0)
