@@ -521,7 +521,7 @@ def config4_stage(zav1, synth, args):
             'max_leaves_per_step': info.max_leaves_per_step,
             'config': {'workload': 'config4: 8192x4096 10-bit luma (1/f noise + 25%% screen regions), SB128, 4x4 '
                                    'tiles, preset 2 (%d candidates, %d -> %d survivors), qp 28, 16x16 leaves'
-                                   % (len(f['cands']), f['k1'], f['k3'])}}
+                                   % (int(cfg[0]['n_cands']), int(cfg[0]['k1']), int(cfg[0]['k3']))}}
 
 
 def serving_stage(zav1, synth, args):
