@@ -587,3 +587,131 @@ def wf_frame(cfg, images):
 
 WF_DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
                         ('rate', '<i4'), ('mode_rate', '<i4'), ('dist', '<u8'), ('cost', '<u8')])
+
+
+# ---- partition pre-pass (pd0.c) ----------------------------------------------
+class Pd0Cfg(ctypes.Structure):
+    _fields_ = [('level', ctypes.c_int), ('qp', ctypes.c_int), ('qindex', ctypes.c_int), ('lambda_', ctypes.c_uint64),
+                ('ires', ctypes.c_int), ('cap_max_block', ctypes.c_int), ('max_tx', ctypes.c_int),
+                ('min_sq', ctypes.c_int), ('coeff_rate_est_lvl', ctypes.c_int), ('tile_w', ctypes.c_int),
+                ('tile_h', ctypes.c_int), ('split_bits', ctypes.c_int32 * 4), ('none_bits', ctypes.c_int32),
+                ('skip0_bits', ctypes.c_int32)]
+
+
+PD0_NODE = np.dtype([('cost', '<u8'), ('flags', 'u1'), ('pad', 'u1', 7)])
+PD0_VISITED, PD0_TESTED, PD0_SPLIT = 1, 2, 8
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags='C_CONTIGUOUS')
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags='C_CONTIGUOUS')
+
+
+def _bind_pd0(L):
+    if getattr(L, '_pd0', False):
+        return
+    L.or_b64_variance.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u16p]
+    L.or_qp_th_scaling.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    L.or_kf_full_lambda_8bit.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.or_kf_full_lambda_8bit.restype = ctypes.c_uint32
+    L.or_max_block_size_allintra.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.or_input_resolution_factor.argtypes = [ctypes.c_long]
+    L.or_pd0_detector_demotes.argtypes = [_u16p, ctypes.c_int]
+    L.or_lvl6_cost.argtypes = [_u16p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.or_lvl6_cost.restype = ctypes.c_uint64
+    L.or_pd0_default_rates.argtypes = [ctypes.POINTER(ctypes.c_int32 * 4), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32)]
+    L.or_pd0_block_cost_sb.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Pd0Cfg),
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int)]
+    L.or_pd0_block_cost_sb.restype = ctypes.c_uint64
+    L.or_pd0_sb.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Pd0Cfg), _vp, _u8p,
+                            ctypes.POINTER(ctypes.c_int)]
+    L.or_pd0_frame.argtypes = [_u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Pd0Cfg), _vp, _u8p,
+                               _u8p]
+    L._pd0 = True
+
+
+def pd0lib():
+    L = lib()
+    _bind_pd0(L)
+    return L
+
+
+def b64_variance(plane, ox, oy):
+    p = np.ascontiguousarray(plane, np.uint8)
+    v = np.zeros(85, np.uint16)
+    pd0lib().or_b64_variance(p, p.shape[1], ox, oy, v)
+    return v
+
+
+def qp_th_scaling(qp):
+    a, b = ctypes.c_uint32(), ctypes.c_uint32()
+    pd0lib().or_qp_th_scaling(qp, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def pd0_default_rates():
+    s, n, k = (ctypes.c_int32 * 4)(), ctypes.c_int32(), ctypes.c_int32()
+    pd0lib().or_pd0_default_rates(ctypes.byref(s), ctypes.byref(n), ctypes.byref(k))
+    return list(s), n.value, k.value
+
+
+def pd0_cfg(level, qp, qindex, *, ires=0, cap_max_block=0, max_tx=64, min_sq=8, coeff_rate_est_lvl=1, tile_w=0,
+            tile_h=0, lam=None):
+    L = pd0lib()
+    c = Pd0Cfg()
+    c.level, c.qp, c.qindex = level, qp, qindex
+    c.lambda_ = int(L.or_kf_full_lambda_8bit(qindex, qp)) if lam is None else lam
+    c.ires, c.cap_max_block, c.max_tx, c.min_sq = ires, cap_max_block, max_tx, min_sq
+    c.coeff_rate_est_lvl, c.tile_w, c.tile_h = coeff_rate_est_lvl, tile_w, tile_h
+    sb, nb, k0 = pd0_default_rates()
+    for i in range(4):
+        c.split_bits[i] = sb[i]
+    c.none_bits, c.skip0_bits = nb, k0
+    return c
+
+
+def pd0_block_cost(plane, cfg, mode, sq, ox, oy, sb_x=0, sb_y=0, subres=255):
+    p = np.ascontiguousarray(plane, np.uint8)
+    st = ctypes.c_int(subres)
+    v = pd0lib().or_pd0_block_cost_sb(p, p.shape[1], sb_x, sb_y, ctypes.byref(cfg), mode, sq, ox, oy,
+                                      ctypes.byref(st))
+    return int(v), st.value
+
+
+def pd0_sb(plane, cfg, sb_x=0, sb_y=0):
+    p = np.ascontiguousarray(plane, np.uint8)
+    nodes = np.zeros(341, PD0_NODE)
+    lm = np.zeros(256, np.uint8)
+    m = ctypes.c_int()
+    rc = pd0lib().or_pd0_sb(p, p.shape[1], sb_x, sb_y, ctypes.byref(cfg), nodes.ctypes.data, lm, ctypes.byref(m))
+    assert rc == 0
+    return nodes, lm.reshape(16, 16), m.value
+
+
+def pd0_frame(plane, cfg):
+    p = np.ascontiguousarray(plane, np.uint8)
+    h, w = p.shape
+    n = (w // 64) * (h // 64)
+    nodes = np.zeros(n * 341, PD0_NODE)
+    lm = np.zeros(n * 256, np.uint8)
+    modes = np.zeros(n, np.uint8)
+    rc = pd0lib().or_pd0_frame(p, w, w, h, ctypes.byref(cfg), nodes.ctypes.data, lm, modes)
+    assert rc == 0
+    return nodes.reshape(n, 341), lm.reshape(n, 16, 16), modes
+
+
+def leaf_sizes(leaf_map):
+    """Leaf sizes of one 64x64 unit in coding (Z) order from its 16x16 map of
+    log2 sizes per 4x4 cell."""
+    out = []
+
+    def walk(sq, x, y):
+        l = int(leaf_map[y // 4, x // 4])
+        if (1 << l) == sq:
+            out.append(sq)
+            return
+        h = sq // 2
+        for i in range(4):
+            walk(h, x + (i & 1) * h, y + (i >> 1) * h)
+
+    walk(64, 0, 0)
+    return out

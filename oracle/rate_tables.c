@@ -37,7 +37,7 @@ int or_cost_symbol(unsigned p15) {
 
 /* Costs of the symbols of one inverse CDF; stops after the symbol whose
  * stored value is 0 (the last one).  Probabilities floored at 4. */
-static void syntax_rate(int32_t *cost, const uint16_t *icdf, int max_syms) {
+void or_syntax_rate(int32_t *cost, const uint16_t *icdf, int max_syms) {
   unsigned below = 0;
   for (int i = 0; i < max_syms; i++) {
     unsigned cum = 32768u - icdf[i];
@@ -55,22 +55,22 @@ void or_build_coeff_costs(int base_qindex, or_txb_costs txb[10], or_eob_costs eo
   memset(eob, 0, sizeof(or_eob_costs) * 14);
   for (int plane = 0; plane < 2; plane++)
     for (int ctx = 0; ctx < 2; ctx++) {
-      syntax_rate(eob[0][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF16[q][plane][ctx], 6);
-      syntax_rate(eob[1][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF32[q][plane][ctx], 7);
-      syntax_rate(eob[2][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF64[q][plane][ctx], 8);
-      syntax_rate(eob[3][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF128[q][plane][ctx], 9);
-      syntax_rate(eob[4][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF256[q][plane][ctx], 10);
-      syntax_rate(eob[5][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF512[q][plane][ctx], 11);
-      syntax_rate(eob[6][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF1024[q][plane][ctx], 12);
+      or_syntax_rate(eob[0][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF16[q][plane][ctx], 6);
+      or_syntax_rate(eob[1][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF32[q][plane][ctx], 7);
+      or_syntax_rate(eob[2][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF64[q][plane][ctx], 8);
+      or_syntax_rate(eob[3][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF128[q][plane][ctx], 9);
+      or_syntax_rate(eob[4][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF256[q][plane][ctx], 10);
+      or_syntax_rate(eob[5][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF512[q][plane][ctx], 11);
+      or_syntax_rate(eob[6][plane].eob_cost[ctx], AV1_EOB_FLAG_CDF1024[q][plane][ctx], 12);
     }
   for (int t = 0; t < 5; t++)
     for (int plane = 0; plane < 2; plane++) {
       or_txb_costs *c = &txb[t * 2 + plane];
-      for (int k = 0; k < 13; k++) syntax_rate(c->txb_skip_cost[k], AV1_TXB_SKIP_CDF[q][t][k], 3);
-      for (int k = 0; k < 4; k++) syntax_rate(c->base_eob_cost[k], AV1_COEFF_BASE_EOB_CDF[q][t][plane][k], 4);
+      for (int k = 0; k < 13; k++) or_syntax_rate(c->txb_skip_cost[k], AV1_TXB_SKIP_CDF[q][t][k], 3);
+      for (int k = 0; k < 4; k++) or_syntax_rate(c->base_eob_cost[k], AV1_COEFF_BASE_EOB_CDF[q][t][plane][k], 4);
       for (int k = 0; k < 42; k++) {
         int32_t *b = c->base_cost[k];
-        syntax_rate(b, AV1_COEFF_BASE_CDF[q][t][plane][k], 5);
+        or_syntax_rate(b, AV1_COEFF_BASE_CDF[q][t][plane][k], 5);
         /* derived entries: [4] unused, [5] level-1 cost incl. the sign bit
          * relative to level 0, [6] / [7] steps 1->2 and 2->3 */
         b[4] = 0;
@@ -78,12 +78,12 @@ void or_build_coeff_costs(int base_qindex, or_txb_costs txb[10], or_eob_costs eo
         b[6] = b[2] - b[1];
         b[7] = b[3] - b[2];
       }
-      for (int k = 0; k < 9; k++) syntax_rate(c->eob_extra_cost[k], AV1_EOB_EXTRA_CDF[q][t][plane][k], 3);
-      for (int k = 0; k < 3; k++) syntax_rate(c->dc_sign_cost[k], AV1_DC_SIGN_CDF[q][plane][k], 3);
+      for (int k = 0; k < 9; k++) or_syntax_rate(c->eob_extra_cost[k], AV1_EOB_EXTRA_CDF[q][t][plane][k], 3);
+      for (int k = 0; k < 3; k++) or_syntax_rate(c->dc_sign_cost[k], AV1_DC_SIGN_CDF[q][plane][k], 3);
       const int tb = t < 3 ? t : 3;
       for (int k = 0; k < 21; k++) {
         int32_t br[5] = {0};
-        syntax_rate(br, AV1_COEFF_BR_CDF[q][tb][plane][k], 5);
+        or_syntax_rate(br, AV1_COEFF_BR_CDF[q][tb][plane][k], 5);
         int32_t *l = c->lps_cost[k];
         /* lps_cost[i] for base-range increments i = 0..12: every group of
          * three continues with one more br symbol; the fourth br symbol
@@ -131,7 +131,7 @@ int or_intra_txt_rate(int tx_size, int intra_dir, int tx_type) {
   if (eset <= 0) return 0;
   if (!EXT_TX_USED[set][tx_type]) return 0;
   int32_t cost[17] = {0};
-  syntax_rate(cost, AV1_INTRA_EXT_TX_CDF[eset][SQR[tx_size]][intra_dir], 17);
+  or_syntax_rate(cost, AV1_INTRA_EXT_TX_CDF[eset][SQR[tx_size]][intra_dir], 17);
   return cost[EXT_TX_IND[set][tx_type]];
 }
 
@@ -152,19 +152,19 @@ int or_bsize_index(int w, int h) {
  * (+ its mode). */
 int or_intra_mode_rate(int w, int h, int actx, int lctx, int mode, int delta, int fi) {
   int32_t c[16] = {0};
-  syntax_rate(c, AV1_KF_Y_CDF[actx][lctx], 13);
+  or_syntax_rate(c, AV1_KF_Y_CDF[actx][lctx], 13);
   int r = c[mode];
   if (mode >= 1 && mode <= 8 && w >= 8 && h >= 8) {
     int32_t a[8] = {0};
-    syntax_rate(a, AV1_ANGLE_DELTA_CDF[mode - 1], 7);
+    or_syntax_rate(a, AV1_ANGLE_DELTA_CDF[mode - 1], 7);
     r += a[delta + 3];
   }
   if (mode == 0 && w <= 32 && h <= 32) {
     int32_t f[2] = {0}, m[5] = {0};
-    syntax_rate(f, AV1_FILTER_INTRA_CDF[or_bsize_index(w, h)], 2);
+    or_syntax_rate(f, AV1_FILTER_INTRA_CDF[or_bsize_index(w, h)], 2);
     r += f[fi != 5];
     if (fi != 5) {
-      syntax_rate(m, AV1_FILTER_INTRA_MODE_CDF, 5);
+      or_syntax_rate(m, AV1_FILTER_INTRA_MODE_CDF, 5);
       r += m[fi];
     }
   }

@@ -478,6 +478,17 @@ int or_optimize_b(const int32_t *tcoeff, int32_t *qcoeff, int32_t *dqcoeff, int 
 int or_cost_coeffs_txb(const int32_t *qcoeff, int eob, const uint16_t *scan, const or_txb_costs *c,
                        const or_eob_costs *e, int tx_size, int tx_class, int plane_type, int txb_skip_ctx,
                        int dc_sign_ctx, int txt_rate) {
+  return or_cost_coeffs_txb_fast(qcoeff, eob, scan, c, e, tx_size, tx_class, plane_type, txb_skip_ctx, dc_sign_ctx,
+                                 txt_rate, 1);
+}
+
+/* The same with C's mds_fast_coeff_est_level (the partition pre-pass's
+ * svt_aom_txb_estimate_coeff_bits_pd0 passes 2, enc/pd0.rs:976-1001): the
+ * middle loop prices only scan indices min(eob - 2, eob / fast_level) down
+ * to 1; fast_level 1 is the full loop. */
+int or_cost_coeffs_txb_fast(const int32_t *qcoeff, int eob, const uint16_t *scan, const or_txb_costs *c,
+                            const or_eob_costs *e, int tx_size, int tx_class, int plane_type, int txb_skip_ctx,
+                            int dc_sign_ctx, int txt_rate, int fast_level) {
   const int adj = or_adjusted_tx_size(tx_size);
   const int w = TXW[adj], h = TXH[adj], bwl = log2i(w);
   static __thread uint8_t lv[LEVELS_LEN];
@@ -523,7 +534,8 @@ int or_cost_coeffs_txb(const int32_t *qcoeff, int eob, const uint16_t *scan, con
     LEVEL_COST(0, qcoeff[0], 0, 1, ctx);
   }
 #undef LEVEL_COST
-  for (int i = eob - 2; i >= 1; i--) {
+  const int mid = eob / fast_level < eob - 2 ? eob / fast_level : eob - 2;
+  for (int i = mid; i >= 1; i--) {
     const int pos = scan[i], v = qcoeff[pos], lvl = v < 0 ? -v : v;
     const int ctx = lower_ctx(lv, pos, bwl, h, i, 0, tx_size, tx_class);
     if (v != 0) cost += 512;

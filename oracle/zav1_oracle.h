@@ -306,6 +306,56 @@ typedef struct {
  * dec[img][leaf row][leaf col]. */
 void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec);
 
+
+/* Coefficient rate with C's mds_fast_coeff_est_level (1 = full loop). */
+int or_cost_coeffs_txb_fast(const int32_t *qcoeff, int eob, const uint16_t *scan, const or_txb_costs *c,
+                            const or_eob_costs *e, int tx_size, int tx_class, int plane_type, int txb_skip_ctx,
+                            int dc_sign_ctx, int txt_rate, int fast_level);
+/* enc/quant.rs:494 syntax_rate_from_cdf over one inverse CDF. */
+void or_syntax_rate(int32_t *cost, const uint16_t *icdf, int max_syms);
+
+/* ---- partition pre-pass (pd0.c; enc/pd0.rs) ----------------------------- */
+typedef struct {
+  int level;              /* 0 LVL_0, 1 LVL_1, 5 eff-M9 (detector: LVL_5 / LVL_6 per superblock) */
+  int qp, qindex;         /* CLI qp (threshold scaling, lambda weight), frame qindex */
+  uint64_t lambda;        /* kf_full_lambda_8bit(qindex, qp) */
+  int ires;               /* input_resolution_factor (LVL_0 / LVL_5 closed-form rate) */
+  int cap_max_block;      /* LVL_1: the 64x64 variance cap (preset >= 8) */
+  int max_tx;             /* 64, or 32 (max_tx_size) */
+  int min_sq;             /* 8, or 4 when 4x4 blocks are allowed */
+  int coeff_rate_est_lvl; /* LVL_1: 1 exact, 2 approximation for small eobs */
+  int tile_w, tile_h;     /* LVL_1 tile size in pixels (0 = one tile) */
+  int32_t split_bits[4];  /* PARTITION_SPLIT at 8, 16, 32, 64 */
+  int32_t none_bits, skip0_bits;
+} or_pd0_cfg;
+enum { OR_PD0_VISITED = 1, OR_PD0_TESTED = 2, OR_PD0_SPLIT = 8 };
+typedef struct {
+  uint64_t cost;          /* the node's PART_N cost when tested */
+  uint8_t flags, pad[7];
+} or_pd0_node;
+void or_b64_variance(const uint8_t *src, int stride, int ox, int oy, uint16_t v[85]);
+void or_qp_th_scaling(int qp, uint32_t *qw, uint32_t *qwd);
+uint32_t or_kf_full_lambda_8bit(int qindex, int cli_qp);
+int or_max_block_size_allintra(int var64, int qp);
+int or_input_resolution_factor(long pixels);
+int or_pd0_detector_demotes(const uint16_t v[85], int qp);
+uint64_t or_lvl6_cost(const uint16_t v[85], int sq, int ox, int oy, int qp);
+void or_pd0_default_rates(int32_t split_bits[4], int32_t *none_bits, int32_t *skip0_bits);
+/* One block's cost at a given level (mode 0, 1, 5, 6) of the superblock at
+ * (sb_x, sb_y); *subres_state carries the LVL_5 subres decision (255
+ * undetermined) between calls like the reference's Pd0Ctx. */
+uint64_t or_pd0_block_cost_sb(const uint8_t *src, int stride, int sb_x, int sb_y, const or_pd0_cfg *cfg, int mode,
+                              int sq, int ox, int oy, int *subres_state);
+/* One 64x64 unit: node records (index 0 the 64x64, 1..4 the 32x32s,
+ * 5..20 the 16x16s, 21..84 the 8x8s, 85..340 the 4x4s, raster within each
+ * level), leaf map (log2 of the leaf size per 4x4 cell, raster 16 x 16) and
+ * the level used (0, 1, 5, 6). */
+int or_pd0_sb(const uint8_t *src, int stride, int sb_x, int sb_y, const or_pd0_cfg *cfg, or_pd0_node nodes[341],
+              uint8_t leaf_map[256], int *mode_out);
+/* Every 64x64 unit of a frame (sides multiples of 64), raster order. */
+int or_pd0_frame(const uint8_t *src, int stride, int width, int height, const or_pd0_cfg *cfg, or_pd0_node *nodes,
+                 uint8_t *leaf_map, uint8_t *modes);
+
 #ifdef __cplusplus
 }
 #endif
