@@ -949,6 +949,73 @@ int zav1_wiener_refine(const void *d_dgd, const void *d_src, int plane_w, int pl
 int zav1_sgr_stats(const void *d_dgd, const void *d_src, int plane_w, int plane_h, int bd, int unit_size, int ss_y,
                    int64_t *d_out, zav1_stream_t s);
 
+
+/* ------------------------------------------------------------------------ */
+/* Partition pre-pass (light PD0 of the allintra presets).                   */
+/* ------------------------------------------------------------------------ */
+/* The partition tree of every 64x64 unit of a batch of stills, as the
+ * reference's PD0 pass decides it (rust/crates/svtav1-encoder/src/pd0.rs):
+ * the 85-entry variance map (compute_b64_variance :81), the pass level per
+ * unit, each tested square block's cost -- DC prediction from the source
+ * neighbours, DCT_DCT, quantize_b, frequency distortion, coefficient rate --
+ * and the parent-first walk with its early exits (Pd0Ctx::pick :1702).
+ * Replaces pd0_pick_sb_partition (:1890, eff-M9: level 5),
+ * pd0_pick_sb_partition_lvl0 (:1990, level 0) and
+ * pd0_pick_sb_partition_m6_eval (:2144, presets 2..8: level 1) for every
+ * unit of the batch in one call.  The plane is stacked like the wavefront's
+ * (n_images x height rows, stride width); samples are 8-bit, a 10-bit plane
+ * enters MSB-truncated (v >> 2) as the reference's pass reads it.  Image
+ * sides must be multiples of 64. */
+typedef struct zav1_pd0_cfg {
+  int32_t width, height;          /* per image, multiples of 64 */
+  int32_t n_images;               /* images stacked vertically */
+  int32_t bd;                     /* samples of the plane: 8 (u8) or 10 (u16) */
+  int32_t level;                  /* 0 LVL_0, 1 LVL_1, 5 eff-M9 (LVL_5 or LVL_6 per unit by the detector) */
+  int32_t qp, qindex;             /* CLI qp (threshold scaling), frame qindex */
+  int32_t ires;                   /* input_resolution_factor (LVL_0 / LVL_5 rate, pd0.rs:371) */
+  int32_t cap_max_block;          /* LVL_1: the 64x64 variance cap (preset >= 8) */
+  int32_t max_tx;                 /* 64, or 32 (max_tx_size) */
+  int32_t min_sq;                 /* 8, or 4 where 4x4 blocks are allowed (presets <= 3) */
+  int32_t coeff_rate_est_lvl;     /* LVL_1: 1 exact rate, 2 the approximation for small eobs */
+  int32_t tile_w, tile_h;         /* LVL_1 tile size in pixels for neighbour availability; 0 = one tile */
+  int32_t split_bits[4];          /* PARTITION_SPLIT rate at 8, 16, 32, 64 (sub-context 0) */
+  int32_t none_bits, skip0_bits;  /* PARTITION_NONE at context 0, skip = 0 at context 0 */
+  int32_t txt_dc[3];              /* DCT_DCT intra tx-type rate at DC_PRED: 4x4, 8x8, 16x16 */
+  int32_t pad;
+  uint64_t lambda;                /* kf_full_lambda_8bit(qindex, qp) (pd0.rs:224) */
+} zav1_pd0_cfg;
+typedef struct zav1_pd0_sb {
+  uint16_t var[85];               /* 64x64, 32x32 (2x2), 16x16 (4x4), 8x8 (8x8), raster per level */
+  uint8_t mode;                   /* level used: 0, 1, 5, 6 */
+  uint8_t max_sq;                 /* largest tested square */
+  uint8_t subres;                 /* LVL_5: 1 when the 64x64 passed the odd/even check */
+  uint8_t pad[3];
+} zav1_pd0_sb;
+/* A node of the walk (the record the depth refinement reads): index 0 the
+ * 64x64, 1..4 the 32x32s, 5..20 the 16x16s, 21..84 the 8x8s, 85..340 the
+ * 4x4s (min_sq 4 only), raster within a level. */
+#define ZAV1_PD0_VISITED 1
+#define ZAV1_PD0_TESTED 2
+#define ZAV1_PD0_SPLIT 8
+typedef struct zav1_pd0_node {
+  uint64_t cost;                  /* the PART_N cost when tested, else 0 */
+  uint8_t flags, pad[7];
+} zav1_pd0_node;
+/* Fills the rate fields, lambda and ires of cfg from the AV1 default CDFs
+ * (build_m6_pd0_tables, pd0.rs:1295; partition_symbol_cost, skip0_bits
+ * :1460-1489); width, height, qp and qindex must be set.  Host only. */
+int zav1_pd0_rates(zav1_pd0_cfg *cfg);
+/* Nodes per unit: 85, or 341 with min_sq 4. */
+int zav1_pd0_nodes_per_unit(const zav1_pd0_cfg *cfg);
+/* Runs the pass: d_sb[u], d_nodes[u * nodes_per_unit + i] and d_leaf_map
+ * [u * 256 + r * 16 + c] (log2 of the leaf size covering 4x4 cell (r, c))
+ * for every unit u in raster order of the stacked plane.  d_costs: the
+ * frame's coefficient cost tables (zav1_build_coeff_costs at qindex; level 1
+ * only, may be null otherwise).  Needs zav1_init on the device; allocates
+ * nothing. */
+int zav1_pd0_run(const zav1_pd0_cfg *cfg, const void *d_plane, const zav1_coeff_costs *d_costs, zav1_pd0_sb *d_sb,
+                 zav1_pd0_node *d_nodes, uint8_t *d_leaf_map, zav1_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -379,8 +379,8 @@ int or_pd0_sb(const uint8_t *src, int stride, int sb_x, int sb_y, const or_pd0_c
   Pd0 c;
   memset(&c, 0, sizeof(c));
   c.src = src, c.stride = stride, c.sb_x = sb_x, c.sb_y = sb_y, c.cfg = cfg;
-  if (cfg->tile_w > 0) c.tile_x0 = sb_x - sb_x % cfg->tile_w;
-  if (cfg->tile_h > 0) c.tile_y0 = sb_y - sb_y % cfg->tile_h;
+  if (cfg->level == 1 && cfg->tile_w > 0) c.tile_x0 = sb_x - sb_x % cfg->tile_w;
+  if (cfg->level == 1 && cfg->tile_h > 0) c.tile_y0 = sb_y - sb_y % cfg->tile_h;
   static __thread or_txb_costs txb[10];
   static __thread or_eob_costs eob[7][2];
   static __thread int built = -1;

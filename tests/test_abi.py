@@ -205,16 +205,23 @@ def test_header_is_plain_c_and_layouts_match(tmp_path, zav1):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     src = tmp_path / 'probe.c'
     src.write_text('#include <stdio.h>\n#include "zav1_dsp.h"\nint main(void) {\n'
-                   '  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(zav1_rdoq_job), sizeof(zav1_rate_job),\n'
-                   '         sizeof(zav1_txb_costs), sizeof(zav1_eob_costs), sizeof(zav1_coeff_costs),\n'
-                   '         sizeof(zav1_txunit_rd_job), sizeof(zav1_dsp_v1));\n'
+                   '  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(zav1_rdoq_job),\n'
+                   '         sizeof(zav1_rate_job), sizeof(zav1_txb_costs), sizeof(zav1_eob_costs),\n'
+                   '         sizeof(zav1_coeff_costs), sizeof(zav1_txunit_rd_job), sizeof(zav1_dsp_v1),\n'
+                   '         sizeof(zav1_pd0_cfg), sizeof(zav1_pd0_sb), sizeof(zav1_pd0_node));\n'
                    '  return 0;\n}\n')
     exe = tmp_path / 'probe'
     subprocess.run([cc, '-std=c99', '-Wall', '-Werror', '-I', os.path.join(root, 'include'), str(src), '-o', str(exe)],
                    check=True, capture_output=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        'zav1_partition', os.path.join(os.path.dirname(zav1.__file__), 'partition.py'))
+    part = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(part)
     assert sizes == [zav1.RDOQ_JOB.itemsize, zav1.RATE_JOB.itemsize, 944 * 4, 22 * 4, zav1.COEFF_COSTS_WORDS * 4,
-                     zav1.TXUNIT_RD_JOB.itemsize, ctypes.sizeof(zav1.DspV1)]
+                     zav1.TXUNIT_RD_JOB.itemsize, ctypes.sizeof(zav1.DspV1), part.PD0_CFG.itemsize,
+                     part.PD0_SB.itemsize, part.PD0_NODE.itemsize]
 
 
 def test_coeff_cost_tables_host_match_oracle(zav1, oracle):
@@ -358,3 +365,25 @@ def test_preset_funnel_settings():
     import pytest
     with pytest.raises(ValueError):
         wf.preset_funnel(0, 30)
+
+
+def test_pd0_rates_host_match_oracle(zav1, oracle):
+    """zav1_pd0_rates (host) fills the default-CDF rates, lambda and the
+    resolution factor the oracle's partition pass uses (values pinned by the
+    C captures in test_oracle_pd0.py)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        'zav1_partition', os.path.join(os.path.dirname(zav1.__file__), 'partition.py'))
+    part = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(part)
+    for (w, h, qp, qidx) in ((64, 64, 20, 80), (512, 512, 32, 128), (4096, 4096, 55, 220)):
+        c = part.pd0_config(zav1, w, h, 8, qp, qidx, 8)[0]
+        split, none, skip0 = oracle.pd0_default_rates()
+        assert list(c['split_bits']) == split and c['none_bits'] == none and c['skip0_bits'] == skip0
+        assert int(c['lambda']) == oracle.pd0lib().or_kf_full_lambda_8bit(qidx, qp)
+        assert int(c['ires']) == oracle.pd0lib().or_input_resolution_factor(w * h)
+        assert list(c['txt_dc']) == [oracle.intra_txt_rate(t, 0, 0) for t in range(3)]
+    # out-of-range fields are refused before anything runs
+    bad = part.pd0_config(zav1, 64, 64, 8, 20, 80, 8)
+    bad[0]['width'] = 96
+    assert zav1.lib().zav1_pd0_run(bad.ctypes.data, 1, 1, 1, 1, 1, None) == zav1.ZAV1_EINVAL

@@ -193,6 +193,9 @@ EXPORTS = {
     'zav1_wf_run_plain': [_vp, _i32, _i32, _vp],
     'zav1_wf_destroy': [_vp],
     'zav1_wf_last_error': [],
+    'zav1_pd0_rates': [_vp],
+    'zav1_pd0_nodes_per_unit': [_vp],
+    'zav1_pd0_run': [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 
 # numpy layouts of the job structs (match include/zav1_dsp.h)

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -1770,6 +1771,66 @@ int zav1_wave_collect(zav1_ctx* c) {
   const int rc = c->err;
   c->err = ZAV1_OK;
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Partition pre-pass (pd0.hip)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int zav1_pd0_nodes_per_unit(const zav1_pd0_cfg* c) {
+  if (!c) return ZAV1_EINVAL;
+  return c->min_sq <= 4 ? 341 : 85;
+}
+
+int zav1_pd0_run(const zav1_pd0_cfg* c, const void* d_plane, const zav1_coeff_costs* d_costs, zav1_pd0_sb* d_sb,
+                 zav1_pd0_node* d_nodes, uint8_t* d_leaf_map, zav1_stream_t s) {
+  static_assert(sizeof(zav1_pd0_sb) == sizeof(zav1::Pd0SbInfo) && sizeof(zav1_pd0_sb) == 176, "pd0 sb layout");
+  static_assert(sizeof(zav1_pd0_node) == sizeof(zav1::Pd0Node) && sizeof(zav1_pd0_node) == 16, "pd0 node layout");
+  if (!c || c->width <= 0 || c->height <= 0 || c->width % 64 || c->height % 64 || c->n_images < 1 ||
+      (c->bd != 8 && c->bd != 10) || (c->level != 0 && c->level != 1 && c->level != 5) || c->qp < 0 || c->qp > 63 ||
+      c->qindex < 0 || c->qindex > 255 || c->ires < 0 || c->ires > 4 || (c->max_tx != 32 && c->max_tx != 64) ||
+      (c->min_sq != 4 && c->min_sq != 8) || (c->min_sq == 4 && c->level != 1) || c->coeff_rate_est_lvl < 0 ||
+      c->coeff_rate_est_lvl > 2 || c->tile_w < 0 || c->tile_h < 0 || c->tile_w % 64 || c->tile_h % 64 ||
+      c->lambda == 0 || (long)c->width * c->height * c->n_images > (1L << 31) || !d_plane || !d_sb || !d_nodes ||
+      !d_leaf_map || (c->level == 1 && !d_costs))
+    return ZAV1_EINVAL;
+  for (int i = 0; i < 4; i++)
+    if (c->split_bits[i] < 0) return ZAV1_EINVAL;
+  const zav1::QuantRow* rows = nullptr;
+  const uint16_t *iscan = nullptr, *fscan = nullptr;
+  int rc = quant_rows(&rows);
+  if (!rc) rc = iscan_table(&iscan);
+  if (!rc) rc = fscan_table(0, &fscan);
+  if (rc) return rc;
+  zav1::Pd0Params p{};
+  p.width = c->width, p.height = c->height, p.bd = c->bd, p.level = c->level;
+  p.n_sb = (c->width / 64) * (c->height / 64) * c->n_images;
+  p.min_sq = c->min_sq, p.max_tx = c->max_tx, p.cap_max_block = c->cap_max_block;
+  p.coeff_rate_est_lvl = c->coeff_rate_est_lvl, p.tile_w = c->tile_w, p.tile_h = c->tile_h, p.ires = c->ires;
+  p.nps = c->min_sq <= 4 ? 341 : 85;
+  // qp-based threshold scaling (pd0.rs:185)
+  if (c->qp >= 46) {
+    const double ex = -((double)(c->qp > 40 ? c->qp : 40) - 35.0) / 10.0;
+    p.qw = (uint32_t)((1.05 - std::exp(ex)) * 10000.0);
+    p.qwd = 10000;
+  } else {
+    p.qw = (uint32_t)(c->qp > 10 ? c->qp : 10);
+    p.qwd = 63;
+  }
+  for (int i = 0; i < 4; i++) p.split_bits[i] = c->split_bits[i];
+  p.none_bits = c->none_bits, p.skip0_bits = c->skip0_bits;
+  for (int i = 0; i < 3; i++) p.txt_dc[i] = c->txt_dc[i];
+  p.lambda = c->lambda;
+  // the pass quantizes with the 8-bit rows: qindex at LVL_1, qindex + 8 at
+  // LVL_0 / LVL_5 (lpd0_qp_offset)
+  const int qi = c->level == 1 ? c->qindex : std::min(c->qindex + 8, 255);
+  ZCHECK(zav1::pd0_run(d_plane, p, rows + qi, iscan, fscan, reinterpret_cast<const int32_t*>(d_costs),
+                       reinterpret_cast<zav1::Pd0SbInfo*>(d_sb), reinterpret_cast<zav1::Pd0Node*>(d_nodes),
+                       d_leaf_map, S(s)));
+  return ZAV1_OK;
 }
 
 }  // extern "C"

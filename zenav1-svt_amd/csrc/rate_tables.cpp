@@ -15,6 +15,7 @@
 
 #include "../../include/zav1_dsp.h"
 #include "av1_coef_cdfs.h"
+#include "av1_part_cdfs.h"
 
 namespace {
 
@@ -168,4 +169,31 @@ extern "C" uint64_t zav1_kf_full_lambda(int qindex, int cli_qp, int bd) {
   const uint64_t weight = cli_qp >= 56 ? 175 : (cli_qp >= 16 ? 150 : 0);
   if (weight) lambda = (lambda * weight) >> 7;
   return bd == 10 ? lambda * 16 : lambda;
+}
+
+extern "C" int zav1_pd0_rates(zav1_pd0_cfg* c) {
+  if (!c || c->qindex < 0 || c->qindex > 255 || c->qp < 0 || c->qp > 63 || c->width <= 0 || c->height <= 0)
+    return ZAV1_EINVAL;
+  // PARTITION_SPLIT (symbol 3) of each square size's first context row; the
+  // 8x8 row has 4 symbols, PARTITION_NONE read from it for every size
+  for (int s = 0; s < 4; s++) {
+    int32_t cost[10] = {};
+    const uint16_t(&row)[11] = AV1_PARTITION_CDF[s * 4];
+    cdf_costs(cost, row, s == 0 ? 4 : 10);
+    c->split_bits[s] = cost[3];
+    if (s == 0) c->none_bits = cost[0];
+  }
+  int32_t sk[2] = {};
+  cdf_costs(sk, AV1_SKIP_CDF[0], 2);
+  c->skip0_bits = sk[0];
+  for (int t = 0; t < 3; t++) c->txt_dc[t] = zav1_intra_txt_rate(t, 0, 0);
+  c->lambda = zav1_kf_full_lambda(c->qindex, c->qp, 8);
+  // input_resolution_factor (pd0.rs:371) of the picture's pixel count
+  const long px = (long)c->width * c->height;
+  static const int kFactor[7] = {0, 1, 2, 3, 4, 4, 4};
+  const long th[6] = {0x28500, 0x4CE00, 0xA1400, 0x16DA00, 0x535200, 0x140A000};
+  int r = 0;
+  while (r < 6 && px >= th[r]) r++;
+  c->ires = kFactor[r];
+  return ZAV1_OK;
 }

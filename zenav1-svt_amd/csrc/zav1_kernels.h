@@ -457,4 +457,25 @@ hipError_t plain_intra(const PlainIntra& p, const int32_t* row, const int32_t* c
 // CfL: out = clip(pred + round(alpha * ac / 64)) over n packed samples
 hipError_t plain_cfl(const int16_t* ac, const void* pred, int alpha_q3, int n, int bd, void* out, hipStream_t st);
 
+
+// Partition pre-pass (pd0.hip).  Layouts of zav1_pd0_sb / zav1_pd0_node.
+struct Pd0SbInfo {
+  uint16_t var[85];
+  uint8_t mode, max_sq, subres, pad[3];
+};
+struct Pd0Node {
+  uint64_t cost;
+  uint8_t flags, pad[7];
+};
+constexpr uint8_t kPd0Visited = 1, kPd0Tested = 2, kPd0Split = 8;
+struct Pd0Params {
+  int width, height, n_sb, bd, level, min_sq, max_tx, cap_max_block, coeff_rate_est_lvl, tile_w, tile_h, ires, nps;
+  uint32_t qw, qwd;
+  int32_t split_bits[4], none_bits, skip0_bits, txt_dc[3];
+  uint64_t lambda;
+};
+hipError_t pd0_run(const void* plane, const Pd0Params& p, const QuantRow* row, const uint16_t* iscan,
+                   const uint16_t* fscan, const int32_t* costs, Pd0SbInfo* info, Pd0Node* nodes, uint8_t* leaf_map,
+                   hipStream_t st);
+
 }  // namespace zav1
