@@ -581,9 +581,12 @@ int zav1_md_predict_survivors(const void *d_recon, int stride, int plane_w, int 
 /* ------------------------------------------------------------------------ */
 /* Superblock-wavefront mode decision (batched form of the leaf funnel).     */
 /* ------------------------------------------------------------------------ */
-/* Every leaf of a uniform partition (leaf x leaf blocks, transform = block)
- * of n_images equal-size luma planes, stacked vertically in one device
- * plane, decided with the reference's funnel (evaluate_leaf,
+/* Every leaf of a partition (transform = block) -- uniform leaf x leaf
+ * blocks, or the leaves of a partition map such as the partition pre-pass
+ * decides (the fixed tree the reference codes at presets 6..8,
+ * pipeline.rs:8000-8060 encode_fixed_tree) -- of n_images equal-size luma
+ * planes, stacked vertically in one device plane, decided with the
+ * reference's funnel (evaluate_leaf,
  * rust/crates/svtav1-encoder/src/leaf_funnel/mod.rs:1007): MDS0 over the
  * candidate list predicted from the reconstruction with AV1 neighbour
  * availability (enc/intra_edge.rs:355 / :436 within the tile), fast cost
@@ -596,15 +599,18 @@ int zav1_md_predict_survivors(const void *d_recon, int stride, int plane_w, int 
  * (qp-scaled), no signalable eob-0 non-DCT), in MDS1 and MDS3 the luma rate
  * at coeff_rate_est_lvl 2 approximated for small eobs, the least full cost (mode rate
  * + coefficient rate, distortion) first in order; its reconstruction is
- * committed before the leaves that read it.  Superblocks run in
- * anti-diagonal order within each tile (every tile and image of the batch
- * together), leaves in Z-order within a superblock.  Rate tables are the
+ * committed before the leaves that read it.  A leaf runs one step after the
+ * latest leaf whose reconstruction or mode it reads (the coding order's
+ * dependencies within its tile), every tile and image of the batch together,
+ * one chain of launches per (step, leaf size); 64x64 leaves take no filter
+ * intra (the configuration's filter-intra candidates, which must come last,
+ * are dropped for them).  Rate tables are the
  * frame's default-CDF tables (no per-superblock CDF adaptation); luma only. */
 typedef struct zav1_wf_cfg {
   int32_t width, height;          /* per image, multiples of leaf */
   int32_t n_images;               /* images of the batch, stacked in one plane */
   int32_t bd;                     /* 8 or 10 */
-  int32_t leaf;                   /* 8, 16 or 32 */
+  int32_t leaf;                   /* 8, 16, 32 or 64 (uniform leaves) */
   int32_t sb;                     /* superblock 64 or 128 */
   int32_t tile_cols, tile_rows;   /* uniform tiles (in superblocks) */
   int32_t qindex;
@@ -618,6 +624,12 @@ typedef struct zav1_wf_cfg {
   int32_t n_cands;                /* <= 64 */
   int32_t pad;
   zav1_md_cand cands[64];
+  int32_t txt_groups_lt16;        /* with a partition map: tx-type groups of leaves below 16x16 */
+  int32_t pad2;
+  const uint8_t *leaf_map;        /* NULL: uniform leaves of `leaf`; else a partition map (host memory, read by
+                                     zav1_wf_create / zav1_wf_footprint only): per image, per 64x64 unit in
+                                     raster order, 256 log2 leaf sizes of its 4x4 cells (zav1_pd0_run's
+                                     d_leaf_map layout); leaves 8..64, sides multiples of 64 */
 } zav1_wf_cfg;
 typedef struct zav1_wf_decision {
   uint8_t cand, tx_type, mode, fi_mode; /* candidate index, its tx type, mode and filter-intra mode */

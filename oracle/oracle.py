@@ -571,17 +571,30 @@ def cdef_apply_frame(y, u, v, bd, damping, fb_str, skip=None):
     return oy, ou, ov, stats
 
 
-def wf_frame(cfg, images):
+def wf_frame(cfg, images, leaf_map=None):
     """or_wf_frame over (n_images, height, width) samples with a zav1_wf_cfg
-    record (numpy, same layout); returns (recon, decisions[img][row][col])."""
+    record (numpy, same layout); returns (recon, decisions[img][row][col]):
+    rows / columns of leaves for uniform leaves, of 8x8 cells (each leaf's
+    record at its top-left cell) with a partition map (n_images x units x
+    256 log2 leaf sizes, the partition pre-pass's layout)."""
     L = lib()
     L.or_wf_frame.argtypes = [_vp, _vp, _vp, _vp]
+    cfg = np.array(cfg, copy=True)
     c = cfg[0]
     n, H, W, leaf = int(c['n_images']), int(c['height']), int(c['width']), int(c['leaf'])
     src = np.ascontiguousarray(images)
     rec = np.zeros_like(src)
-    dec = np.zeros((n, H // leaf, W // leaf), WF_DECISION)
+    lm = None
+    if leaf_map is not None:
+        lm = np.ascontiguousarray(leaf_map, np.uint8)
+        c['leaf_map'] = lm.ctypes.data
+        dec = np.zeros((n, H // 8, W // 8), WF_DECISION)
+    else:
+        c['leaf_map'] = 0
+        dec = np.zeros((n, H // leaf, W // leaf), WF_DECISION)
+    cfg[0] = c
     L.or_wf_frame(np.ascontiguousarray(cfg).ctypes.data, src.ctypes.data, rec.ctypes.data, dec.ctypes.data)
+    del lm
     return rec, dec
 
 

@@ -296,6 +296,9 @@ typedef struct { /* same layout as zav1_wf_cfg */
   uint64_t lambda;
   int32_t n_cands, pad;
   or_wf_cand cands[64];
+  int32_t txt_groups_lt16, pad2; /* tx-type groups of leaves below 16x16 (partition map only) */
+  const uint8_t *leaf_map;       /* NULL: uniform leaves; else per image, per 64x64 unit (raster), 256 log2
+                                    leaf sizes (16 x 16 cells of 4x4): the partition pre-pass's map */
 } or_wf_cfg;
 typedef struct {
   uint8_t cand, tx_type, mode, fi_mode;
@@ -303,7 +306,8 @@ typedef struct {
   uint64_t dist, cost;
 } or_wf_decision;
 /* Every leaf of n_images stacked planes (src, recon: n_images * height rows);
- * dec[img][leaf row][leaf col]. */
+ * uniform leaves: dec[img][leaf row][leaf col]; with a partition map:
+ * dec[img][y / 8][x / 8] at each leaf's top-left 8x8 cell. */
 void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec);
 
 

@@ -258,36 +258,19 @@ __global__ __launch_bounds__(256) void k_wf_commit(const MdUnit* __restrict__ un
 // Host plan
 // ---------------------------------------------------------------------------
 
-struct Wavefront {
-  zav1_wf_cfg cfg{};
-  int L = 0, nl = 0, lps = 0, tx = 0, nt = 0, nc = 0, k1 = 0, k3 = 0;
-  int stacked_h = 0, n_units = 0, n_steps = 0, max_u = 0;
-  size_t bytes = 0;
-  std::vector<int> step_off;
-  std::vector<MdUnit> h_units;
-  std::vector<int32_t> h_nbr;
+// Per leaf size: the funnel's candidates, counts and tx types at that size
+// and the stage buffers, sized for the size's largest step.
+struct SizePlan {
+  int L = 0, tx = 0, nt = 0, nc = 0, k1 = 0, k3 = 0, max_u = 0;
   uint8_t types[16] = {};
   MdCandList cl{};
   std::vector<WfCand> h_cands;
-  uint64_t rate_th = 0;
-  int64_t satd_th = 0, rdmult = 0;
-  unsigned flags1 = 0, flags3 = 0;
-  uint32_t cut_num = 0, cut_den = 0;
-  int32_t dq0 = 0, dq1 = 0;
-  const QuantRow* row = nullptr;
-  const uint16_t *iscan = nullptr, *fscan = nullptr;
-  // device memory (one allocation, carved)
-  char* base = nullptr;
-  void *src = nullptr, *recon = nullptr;
-  MdUnit* units = nullptr;
-  int32_t *nbr = nullptr, *uid = nullptr;
-  WfDecision* dec = nullptr;
-  uint8_t* mode_map = nullptr;
+  int64_t satd_th = 0;
+  const uint16_t* fscan = nullptr;
   uint32_t* rate_tab = nullptr;
   int32_t* txt = nullptr;
   WfCand* cands = nullptr;
   uint8_t* d_types = nullptr;
-  int32_t* costs = nullptr;
   uint32_t *rate = nullptr, *satd = nullptr;
   uint8_t *sel = nullptr, *m3 = nullptr;
   uint64_t* sel_cost = nullptr;
@@ -299,8 +282,34 @@ struct Wavefront {
   uint8_t *cul1 = nullptr, *cul3 = nullptr;
   void *ws1 = nullptr, *ws3 = nullptr;
   size_t ws3_bytes = 0;  // MDS3's workspace, with room for the trellis's lane form
-  // plain per-call form scratch
-  uint8_t* one_sel = nullptr;
+};
+
+struct Wavefront {
+  zav1_wf_cfg cfg{};
+  int stacked_h = 0, n_units = 0, n_steps = 0, max_u = 0, n_layers = 0;
+  size_t bytes = 0;
+  std::vector<int> step_off;      // per step: first unit (n_steps + 1 entries)
+  std::vector<uint8_t> step_sz;   // per step: its leaf size class (8, 16, 32, 64 -> 0..3)
+  std::vector<uint8_t> h_sz;      // per unit: size class
+  std::vector<MdUnit> h_units;
+  std::vector<int32_t> h_nbr;
+  std::vector<uint8_t> leaf_map;  // the caller's partition map (empty: uniform leaves)
+  SizePlan sp[4];
+  uint64_t rate_th = 0;
+  int64_t rdmult = 0;
+  unsigned flags1 = 0, flags3 = 0;
+  uint32_t cut_num = 0, cut_den = 0;
+  int32_t dq0 = 0, dq1 = 0;
+  const QuantRow* row = nullptr;
+  const uint16_t *iscan = nullptr, *fscan_base = nullptr;
+  // device memory (one allocation, carved)
+  char* base = nullptr;
+  void *src = nullptr, *recon = nullptr;
+  MdUnit* units = nullptr;
+  int32_t *nbr = nullptr, *uid = nullptr;
+  WfDecision* dec = nullptr;
+  uint8_t* mode_map = nullptr;
+  int32_t* costs = nullptr;
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
 };
@@ -314,6 +323,7 @@ int zorder(int i, int j) {  // bit interleave: row bits odd, column bits even
 }
 
 size_t al(size_t v) { return (v + 255) & ~(size_t)255; }
+int size_class(int L) { return L == 8 ? 0 : (L == 16 ? 1 : (L == 32 ? 2 : 3)); }
 
 }  // namespace
 
@@ -341,17 +351,43 @@ static int search_types(int tx_size, int groups, uint8_t* out) {
   return n;
 }
 
+// The funnel at one leaf size (host part): the configuration's candidates
+// without filter intra on 64x64 leaves (filter-intra candidates come last),
+// the survivor counts capped by them, the size's tx types.
+static void size_setup(const zav1_wf_cfg& c, bool tree, int L, SizePlan* p) {
+  static const uint8_t kFiDir[5] = {0, 1, 2, 6, 0};  // FIMODE_TO_INTRADIR
+  p->L = L;
+  for (int s = 0; s < 19; s++)
+    if (tx_w(s) == L && tx_h(s) == L) p->tx = s;
+  int nc = c.n_cands;
+  while (L > 32 && nc > 0 && c.cands[nc - 1].fi_mode < 5) nc--;
+  p->nc = nc;
+  p->k1 = std::min(c.k1, nc);
+  p->k3 = std::min(c.k3, p->k1);
+  p->nt = search_types(p->tx, (tree && L < 16) ? c.txt_groups_lt16 : c.txt_groups, p->types);
+  p->satd_th = p->nt > 1 ? c.txt_satd_th : 0;
+  p->cl.n = nc;
+  p->h_cands.clear();
+  for (int i = 0; i < nc; i++) {
+    const zav1_md_cand m = c.cands[i];
+    p->cl.c[i] = MdCand{m.mode, m.delta, m.fi_mode, 0};
+    p->h_cands.push_back(WfCand{m.mode, m.delta, m.fi_mode, (uint8_t)(m.fi_mode < 5 ? kFiDir[m.fi_mode] : m.mode)});
+  }
+}
+
+// Leaves in the reference's coding order (tiles in order, superblocks in
+// raster order within a tile, leaves in Z-order within a superblock: of a
+// uniform partition, or of the partition map's 64x64 units in Z-order), each
+// with its AV1 neighbour availability; then the steps.  A leaf's step is one
+// past the latest step of the leaves whose reconstruction or mode it reads
+// (the row above from the top-left sample through the top-right extent, the
+// column left through the bottom-left extent), so every step's leaves are
+// independent and each leaf's inputs are final when it runs: the decisions
+// equal the sequential order's.  A step is split by leaf size, one chain of
+// launches per (step, size).
 static int plan(Wavefront* w) {
   const zav1_wf_cfg& c = w->cfg;
-  w->L = c.leaf;
-  w->nl = c.sb / c.leaf;
-  w->lps = w->nl * w->nl;
-  for (int s = 0; s < 19; s++)
-    if (tx_w(s) == c.leaf && tx_h(s) == c.leaf) w->tx = s;
-  w->nt = search_types(w->tx, c.txt_groups, w->types);
-  w->nc = c.n_cands;
-  w->k1 = c.k1;
-  w->k3 = c.k3;
+  const bool tree = !w->leaf_map.empty();
   w->stacked_h = c.height * c.n_images;
   const int sbc = (c.width + c.sb - 1) / c.sb, sbr = (c.height + c.sb - 1) / c.sb;
   const int tcols = std::max(1, c.tile_cols), trows = std::max(1, c.tile_rows);
@@ -365,10 +401,10 @@ static int plan(Wavefront* w) {
   tile_edges(sbc, tcols, tc);
   tile_edges(sbr, trows, tr);
   struct Leaf {
-    int step, img, x, y, tile, tx0, tx1, ty0, ty1;
+    int img, x, y, L, tx0, tx1, ty0, ty1, layer;
   };
   std::vector<Leaf> leaves;
-  const int L = w->L, nl = w->nl;
+  const int uw = c.width / 64, uh = c.height / 64;
   for (int img = 0; img < c.n_images; img++)
     for (int ty = 0; ty < trows; ty++)
       for (int tx = 0; tx < tcols; tx++) {
@@ -376,67 +412,137 @@ static int plan(Wavefront* w) {
         const int y0 = tr[ty] * c.sb, y1 = std::min(c.height, tr[ty + 1] * c.sb);
         for (int r = tr[ty]; r < tr[ty + 1]; r++)
           for (int q = tc[tx]; q < tc[tx + 1]; q++) {
-            const int d = (q - tc[tx]) + 2 * (r - tr[ty]);
-            for (int z = 0; z < w->lps; z++) {
-              int i = 0, j = 0;
-              for (int b = 0; b < 8; b++) {
-                j |= ((z >> (2 * b)) & 1) << b;
-                i |= ((z >> (2 * b + 1)) & 1) << b;
+            if (tree) {
+              const int nu = (c.sb / 64) * (c.sb / 64);
+              for (int u = 0; u < nu; u++) {
+                const int ux = q * c.sb + (u & 1) * 64, uy = r * c.sb + (u >> 1) * 64;
+                if (ux >= c.width || uy >= c.height) continue;
+                const uint8_t* m = w->leaf_map.data() + ((size_t)img * uw * uh + (size_t)(uy / 64) * uw + ux / 64) * 256;
+                // Z-order DFS of the unit's quadtree
+                struct Node {
+                  int sq, ox, oy;
+                };
+                Node stack[64];
+                int sp = 0;
+                stack[sp++] = {64, 0, 0};
+                while (sp) {
+                  const Node nd = stack[--sp];
+                  if ((1 << m[(nd.oy / 4) * 16 + nd.ox / 4]) == nd.sq) {
+                    leaves.push_back({img, ux + nd.ox, uy + nd.oy, nd.sq, x0, x1, y0, y1, 0});
+                    continue;
+                  }
+                  const int h = nd.sq / 2;
+                  for (int i = 3; i >= 0; i--) stack[sp++] = {h, nd.ox + (i & 1) * h, nd.oy + (i >> 1) * h};
+                }
               }
-              const int x = q * c.sb + j * L, y = r * c.sb + i * L;
-              if (x >= c.width || y >= c.height) continue;
-              leaves.push_back({d * w->lps + z, img, x, y, ty * tcols + tx, x0, x1, y0, y1});
+            } else {
+              const int L = c.leaf, nl = c.sb / L;
+              for (int z = 0; z < nl * nl; z++) {
+                int i = 0, j = 0;
+                for (int b = 0; b < 8; b++) {
+                  j |= ((z >> (2 * b)) & 1) << b;
+                  i |= ((z >> (2 * b + 1)) & 1) << b;
+                }
+                const int x = q * c.sb + j * L, y = r * c.sb + i * L;
+                if (x >= c.width || y >= c.height) continue;
+                leaves.push_back({img, x, y, L, x0, x1, y0, y1, 0});
+              }
             }
           }
       }
-  std::stable_sort(leaves.begin(), leaves.end(), [](const Leaf& a, const Leaf& b) { return a.step < b.step; });
   const int n = (int)leaves.size();
   w->n_units = n;
-  // leaf lookup for neighbour ids: (img, y / L, x / L) -> unit index
-  const int gw = c.width / L, gh = c.height / L;
-  std::vector<int32_t> at((size_t)c.n_images * gw * gh, -1);
-  for (int u = 0; u < n; u++) at[((size_t)leaves[u].img * gh + leaves[u].y / L) * gw + leaves[u].x / L] = u;
-  w->h_units.resize(n);
-  w->h_nbr.resize(2 * (size_t)n);
-  w->step_off.clear();
-  int last = -1;
-  for (int u = 0; u < n; u++) {
-    const Leaf& f = leaves[u];
-    while (last < f.step) {
-      w->step_off.push_back(u);
-      last++;
-    }
+  // 8x8 cells -> leaf (coding-order index), per image
+  const int gw = c.width / 8, gh = c.height / 8;
+  std::vector<int32_t> cell((size_t)c.n_images * gw * gh, -1);
+  std::vector<MdUnit> units(n);
+  std::vector<int32_t> nbr(2 * (size_t)n);
+  int n_layers = 0;
+  for (int k = 0; k < n; k++) {
+    Leaf& f = leaves[k];
+    const int L = f.L, nl = c.sb / L;
     const int i = (f.y % c.sb) / L, j = (f.x % c.sb) / L, zi = zorder(i, j);
     const bool up = f.y > f.ty0, lf = f.x > f.tx0;
-    // top-right: the row above; inside the superblock only when that leaf
+    // top-right: the row above; inside the superblock only when that square
     // precedes this one in Z-order, never in the superblock to the right
     bool tr_ok;
     if (!up) tr_ok = false;
     else if (i == 0) tr_ok = true;
     else tr_ok = j + 1 < nl && zorder(i - 1, j + 1) < zi;
     // bottom-left: the left superblock's rows of this superblock row, or an
-    // earlier leaf of this superblock
+    // earlier square of this superblock
     bool bl_ok;
     if (!lf) bl_ok = false;
     else if (j == 0) bl_ok = i + 1 < nl;
     else bl_ok = i + 1 < nl && zorder(i + 1, j - 1) < zi;
+    const int ntr = tr_ok ? std::max(0, std::min(L, f.tx1 - (f.x + L))) : 0;
+    const int nbl = bl_ok ? std::max(0, std::min(L, f.ty1 - (f.y + L))) : 0;
     MdUnit mu{};
     mu.x = f.x;
     mu.y = f.y + f.img * c.height;
     mu.n_top_px = (int16_t)(up ? L : 0);
-    mu.n_topright_px = (int16_t)(tr_ok ? std::max(0, std::min(L, f.tx1 - (f.x + L))) : 0);
+    mu.n_topright_px = (int16_t)ntr;
     mu.n_left_px = (int16_t)(lf ? L : 0);
-    mu.n_bottomleft_px = (int16_t)(bl_ok ? std::max(0, std::min(L, f.ty1 - (f.y + L))) : 0);
+    mu.n_bottomleft_px = (int16_t)nbl;
     mu.edge_filter = 1;
-    w->h_units[u] = mu;
+    units[k] = mu;
     const size_t g = (size_t)f.img * gw * gh;
-    w->h_nbr[2 * u] = up ? at[g + (size_t)(f.y / L - 1) * gw + f.x / L] : -1;
-    w->h_nbr[2 * u + 1] = lf ? at[g + (size_t)(f.y / L) * gw + f.x / L - 1] : -1;
+    auto at = [&](int x, int y) { return cell[g + (size_t)(y / 8) * gw + x / 8]; };
+    nbr[2 * k] = up ? at(f.x, f.y - 1) : -1;
+    nbr[2 * k + 1] = lf ? at(f.x - 1, f.y) : -1;
+    int layer = 0;
+    auto dep = [&](int x, int y) {
+      const int id = at(x, y);
+      if (id >= 0) layer = std::max(layer, leaves[id].layer + 1);
+    };
+    if (up)
+      for (int x = lf ? f.x - 1 : f.x; x < f.x + L + ntr; x += 8) dep(x, f.y - 1);
+    if (lf)
+      for (int y = f.y; y < f.y + L + nbl; y += 8) dep(f.x - 1, y);
+    f.layer = layer;
+    n_layers = std::max(n_layers, layer + 1);
+    for (int y = f.y; y < f.y + L; y += 8)
+      for (int x = f.x; x < f.x + L; x += 8) cell[g + (size_t)(y / 8) * gw + x / 8] = k;
+  }
+  w->n_layers = n_layers;
+  // storage order: by (step, leaf size), coding order within
+  std::vector<int> perm(n), pos(n);
+  for (int k = 0; k < n; k++) perm[k] = k;
+  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+    const Leaf &la = leaves[a], &lb = leaves[b];
+    return la.layer != lb.layer ? la.layer < lb.layer : size_class(la.L) < size_class(lb.L);
+  });
+  for (int s = 0; s < n; s++) pos[perm[s]] = s;
+  w->h_units.resize(n);
+  w->h_nbr.resize(2 * (size_t)n);
+  w->h_sz.resize(n);
+  w->step_off.clear();
+  w->step_sz.clear();
+  for (int k = 0; k < 4; k++) w->sp[k].max_u = 0;
+  for (int s = 0; s < n; s++) {
+    const int k = perm[s];
+    const Leaf& f = leaves[k];
+    w->h_units[s] = units[k];
+    w->h_nbr[2 * s] = nbr[2 * k] >= 0 ? pos[nbr[2 * k]] : -1;
+    w->h_nbr[2 * s + 1] = nbr[2 * k + 1] >= 0 ? pos[nbr[2 * k + 1]] : -1;
+    const int sc = size_class(f.L);
+    w->h_sz[s] = (uint8_t)sc;
+    if (s == 0 || f.layer != leaves[perm[s - 1]].layer || sc != w->h_sz[s - 1]) {
+      w->step_off.push_back(s);
+      w->step_sz.push_back((uint8_t)sc);
+    }
   }
   w->step_off.push_back(n);
-  w->n_steps = (int)w->step_off.size() - 1;
+  w->n_steps = (int)w->step_sz.size();
   w->max_u = 0;
-  for (int s = 0; s < w->n_steps; s++) w->max_u = std::max(w->max_u, w->step_off[s + 1] - w->step_off[s]);
+  for (int s = 0; s < w->n_steps; s++) {
+    const int u = w->step_off[s + 1] - w->step_off[s];
+    SizePlan& p = w->sp[w->step_sz[s]];
+    p.max_u = std::max(p.max_u, u);
+    w->max_u = std::max(w->max_u, u);
+  }
+  for (int k = 0; k < 4; k++)
+    if (w->sp[k].max_u) size_setup(c, tree, 8 << k, &w->sp[k]);
   return ZAV1_OK;
 }
 
@@ -446,18 +552,25 @@ static size_t ws3_bytes(int tx, int n3) {
   return ((txunit_rd_workspace(tx, n3) + 255) & ~(size_t)255) + rdoq_workspace(tx, n3);
 }
 
+static size_t size_bytes(const SizePlan& p) {
+  if (!p.max_u) return 0;
+  const size_t U = (size_t)p.max_u, LL = (size_t)p.L * p.L, n3 = U * p.k3 * p.nt;
+  size_t b = al(25 * p.nc * 4) + al(13 * 16 * 4) + al(64 * sizeof(WfCand)) + al(16);
+  b += 2 * al(U * p.nc * 4) + 2 * al(U * p.k1) + al(U * p.k1 * 8) + al(U * p.k1 * sizeof(TxUnitJob));
+  b += al(U * p.k1 * LL * 2) + al(U * p.k1 * sizeof(TxUnitRdJob)) + al(U * p.k1 * 8) + 2 * al(U * p.k1 * 4) +
+       al(U * p.k1);
+  b += al(n3 * sizeof(TxUnitRdJob)) + 2 * al(n3 * 8) + 2 * al(n3 * 4) + al(n3) + al(n3 * LL * 2);
+  b += al(txunit_rd_workspace(p.tx, (int)(U * p.k1))) + al(ws3_bytes(p.tx, (int)n3)) + al(64);
+  return b;
+}
+
 static size_t footprint(const Wavefront& w) {
   const zav1_wf_cfg& c = w.cfg;
-  const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w.stacked_h * es, U = (size_t)w.max_u;
-  const size_t n = (size_t)w.n_units, LL = (size_t)w.L * w.L;
+  const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w.stacked_h * es;
+  const size_t n = (size_t)w.n_units;
   size_t b = 2 * al(plane) + al(n * sizeof(MdUnit)) + al(n * 8) + al(n * 4) + al(n * sizeof(WfDecision)) + al(n);
-  b += al(25 * w.nc * 4) + al(13 * 16 * 4) + al(64 * sizeof(WfCand)) + al(16) + al(sizeof(zav1_coeff_costs));
-  b += 2 * al(U * w.nc * 4) + 2 * al(U * w.k1) + al(U * w.k1 * 8) + al(U * w.k1 * sizeof(TxUnitJob));
-  b += al(U * w.k1 * LL * 2) + al(U * w.k1 * sizeof(TxUnitRdJob)) + al(U * w.k1 * 8) + 2 * al(U * w.k1 * 4) +
-       al(U * w.k1);
-  const size_t n3 = U * w.k3 * w.nt;
-  b += al(n3 * sizeof(TxUnitRdJob)) + 2 * al(n3 * 8) + 2 * al(n3 * 4) + al(n3) + al(n3 * LL * 2);
-  b += al(txunit_rd_workspace(w.tx, (int)(U * w.k1))) + al(ws3_bytes(w.tx, (int)n3)) + al(64);
+  b += al(sizeof(zav1_coeff_costs));
+  for (int k = 0; k < 4; k++) b += size_bytes(w.sp[k]);
   return b;
 }
 
@@ -479,53 +592,51 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   const int u0 = w->step_off[s], U = w->step_off[s + 1] - u0;
   if (U == 0) return hipSuccess;
   const zav1_wf_cfg& c = w->cfg;
-  const int L = w->L, W = c.width, H = w->stacked_h;
+  SizePlan& P = w->sp[w->step_sz[s]];
+  const int L = P.L, W = c.width, H = w->stacked_h;
   MdUnit* units = w->units + u0;
   MARK(0);
-  hipLaunchKernelGGL(k_wf_prep, dim3((unsigned)((U * w->nc + 255) / 256)), dim3(256), 0, st, units, w->nbr + 2 * u0,
-                     U, w->mode_map, w->rate_tab, w->nc, w->rate);
+  hipLaunchKernelGGL(k_wf_prep, dim3((unsigned)((U * P.nc + 255) / 256)), dim3(256), 0, st, units, w->nbr + 2 * u0,
+                     U, w->mode_map, P.rate_tab, P.nc, P.rate);
   MARK(1);
-  hipError_t e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, units, U, w->cl, w->satd, st);
+  hipError_t e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, units, U, P.cl, P.satd, st);
   if (e != hipSuccess) return e;
   MARK(2);
   TypeList dct{};
   dct.n = 1;
-  e = fast_select(w->satd, w->rate, c.lambda, U, w->nc, w->k1, reinterpret_cast<const int32_t*>(units), 5, L, L, dct,
-                  1, w->sel, w->sel_cost, w->fs_jobs, st);
+  e = fast_select(P.satd, P.rate, c.lambda, U, P.nc, P.k1, reinterpret_cast<const int32_t*>(units), 5, L, L, dct, 1,
+                  P.sel, P.sel_cost, P.fs_jobs, st);
   if (e != hipSuccess) return e;
   MARK(3);
-  e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, units, U, w->cl, w->sel, w->k1, w->pred, st);
+  e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, units, U, P.cl, P.sel, P.k1, P.pred, st);
   if (e != hipSuccess) return e;
   MARK(4);
-  hipLaunchKernelGGL(k_wf_mds1_jobs, dim3((unsigned)((U * w->k1 + 255) / 256)), dim3(256), 0, st, units, U, w->sel,
-                     w->k1, w->cands, w->txt, L, w->rdmult, w->jobs1);
+  hipLaunchKernelGGL(k_wf_mds1_jobs, dim3((unsigned)((U * P.k1 + 255) / 256)), dim3(256), 0, st, units, U, P.sel,
+                     P.k1, P.cands, P.txt, L, w->rdmult, P.jobs1);
   MARK(5);
-  e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs1, U * w->k1, w->row,
-                      w->iscan, w->fscan, w->costs, w->flags1, 0, 0, w->ws1, w->dist1, w->eob1, w->rate1, w->cul1,
-                      nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
+  e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs1, U * P.k1, w->row,
+                      w->iscan, P.fscan, w->costs, w->flags1, 0, 0, P.ws1, P.dist1, P.eob1, P.rate1, P.cul1, nullptr,
+                      nullptr, st, w->dq0, w->dq1, nullptr);
   if (e != hipSuccess) return e;
   MARK(6);
-  hipLaunchKernelGGL(k_wf_mds3_jobs, dim3((unsigned)U), dim3(64), 0, st, units, U, w->sel, w->k1, w->k3,
-                     w->cands, w->rate, w->nc, w->dist1, w->rate1, c.lambda, w->d_types, w->nt, w->txt, L, w->rdmult,
-                     w->m3, w->jobs3);
-  const int n3 = U * w->k3 * w->nt;
+  hipLaunchKernelGGL(k_wf_mds3_jobs, dim3((unsigned)U), dim3(64), 0, st, units, U, P.sel, P.k1, P.k3, P.cands, P.rate,
+                     P.nc, P.dist1, P.rate1, c.lambda, P.d_types, P.nt, P.txt, L, w->rdmult, P.m3, P.jobs3);
+  const int n3 = U * P.k3 * P.nt;
   MARK(7);
-  e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)U * w->k1 * L * L, w->jobs3, n3, w->row, w->iscan,
-                      w->fscan, w->costs, w->flags3, w->cut_num, w->cut_den, w->ws3, w->dist3, w->eob3, w->rate3,
-                      w->cul3, nullptr, w->rec3, st, w->dq0, w->dq1, w->csatd3, clk ? &clk->ev[8] : nullptr,
-                      w->ws3_bytes);
+  e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs3, n3, w->row, w->iscan,
+                      P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3, P.eob3, P.rate3, P.cul3,
+                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes);
   if (e != hipSuccess) return e;
   MARK(10);
   if (c.bd == 8)
-    hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, w->m3,
-                       w->sel, w->k1, w->k3, w->cands, w->rate, w->nc, w->d_types, w->nt, w->txt, w->dist3, w->rate3,
-                       w->eob3, w->csatd3, w->rec3, c.lambda, w->rate_th, w->satd_th, (uint8_t*)w->recon, W, L, w->dec,
-                       w->mode_map);
+    hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, P.m3, P.sel,
+                       P.k1, P.k3, P.cands, P.rate, P.nc, P.d_types, P.nt, P.txt, P.dist3, P.rate3, P.eob3, P.csatd3,
+                       P.rec3, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W, L, w->dec, w->mode_map);
   else
-    hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, w->m3,
-                       w->sel, w->k1, w->k3, w->cands, w->rate, w->nc, w->d_types, w->nt, w->txt, w->dist3, w->rate3,
-                       w->eob3, w->csatd3, w->rec3, c.lambda, w->rate_th, w->satd_th, (uint16_t*)w->recon, W, L,
-                       w->dec, w->mode_map);
+    hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, P.m3,
+                       P.sel, P.k1, P.k3, P.cands, P.rate, P.nc, P.d_types, P.nt, P.txt, P.dist3, P.rate3, P.eob3,
+                       P.csatd3, P.rec3, c.lambda, w->rate_th, P.satd_th, (uint16_t*)w->recon, W, L, w->dec,
+                       w->mode_map);
   MARK(kWfStages);
   return hipGetLastError();
 }
@@ -538,21 +649,22 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
 // same kernels does.
 static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
   const zav1_wf_cfg& c = w->cfg;
-  const int L = w->L, W = c.width, H = w->stacked_h, nc = w->nc, k1 = w->k1, k3 = w->k3, nt = w->nt;
+  SizePlan& P = w->sp[w->h_sz[u]];
+  const int L = P.L, W = c.width, H = w->stacked_h, nc = P.nc, k1 = P.k1, k3 = P.k3, nt = P.nt;
   MdUnit* unit = w->units + u;
   hipLaunchKernelGGL(k_wf_prep, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, unit, w->nbr + 2 * u, 1,
-                     w->mode_map, w->rate_tab, nc, w->rate);
+                     w->mode_map, P.rate_tab, nc, P.rate);
   hipError_t e;
   for (int m = 0; m < nc; m++) {
     MdCandList one{};
     one.n = 1;
-    one.c[0] = w->cl.c[m];
-    e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, unit, 1, one, w->satd + m, st);
+    one.c[0] = P.cl.c[m];
+    e = md_fast_satd(w->src, w->recon, W, W, H, c.bd, L, L, unit, 1, one, P.satd + m, st);
     if (e != hipSuccess) return e;
   }
   std::vector<uint32_t> satd(nc), rate(nc);
-  if ((e = hipMemcpyAsync(satd.data(), w->satd, 4 * nc, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(rate.data(), w->rate, 4 * nc, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(satd.data(), P.satd, 4 * nc, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(rate.data(), P.rate, 4 * nc, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   std::vector<uint64_t> fc(nc);
   for (int m = 0; m < nc; m++)
@@ -560,26 +672,26 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
   uint8_t ord[64];
   exchange_first_k(fc.data(), nc, k1, ord);
   std::vector<uint8_t> sel(ord, ord + k1);
-  if ((e = hipMemcpyAsync(w->sel, sel.data(), k1, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(P.sel, sel.data(), k1, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
   MdUnit hu = w->h_units[u];
   std::vector<uint64_t> d1(k1);
   std::vector<int32_t> r1(k1);
   for (int r = 0; r < k1; r++) {
-    e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, unit, 1, w->cl, w->sel + r, 1, w->pred + r * L * L, st);
+    e = md_predict_survivors(w->recon, W, W, H, c.bd, L, L, unit, 1, P.cl, P.sel + r, 1, P.pred + r * L * L, st);
     if (e != hipSuccess) return e;
   }
   // MDS1 jobs on the device as the batched chain builds them, then one
   // transform-unit call per survivor
-  hipLaunchKernelGGL(k_wf_mds1_jobs, dim3(1), dim3(256), 0, st, unit, 1, w->sel, k1, w->cands, w->txt, L, w->rdmult,
-                     w->jobs1);
+  hipLaunchKernelGGL(k_wf_mds1_jobs, dim3(1), dim3(256), 0, st, unit, 1, P.sel, k1, P.cands, P.txt, L, w->rdmult,
+                     P.jobs1);
   for (int r = 0; r < k1; r++) {
-    e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)k1 * L * L, w->jobs1 + r, 1, w->row, w->iscan,
-                        w->fscan, w->costs, w->flags1, 0, 0, w->ws1, w->dist1 + r, w->eob1 + r, w->rate1 + r,
-                        w->cul1 + r, nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
+    e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)k1 * L * L, P.jobs1 + r, 1, w->row, w->iscan,
+                        P.fscan, w->costs, w->flags1, 0, 0, P.ws1, P.dist1 + r, P.eob1 + r, P.rate1 + r,
+                        P.cul1 + r, nullptr, nullptr, st, w->dq0, w->dq1, nullptr);
     if (e != hipSuccess) return e;
   }
-  if ((e = hipMemcpyAsync(d1.data(), w->dist1, 8 * k1, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(r1.data(), w->rate1, 4 * k1, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(d1.data(), P.dist1, 8 * k1, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(r1.data(), P.rate1, 4 * k1, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   std::vector<uint64_t> c1(k1);
   for (int r = 0; r < k1; r++) c1[r] = rdc(c.lambda, (uint64_t)rate[sel[r]] + (uint32_t)r1[r], d1[r]);
@@ -587,34 +699,34 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
   exchange_first_k(c1.data(), k1, k3, ord3);
   std::vector<TxUnitRdJob> j3((size_t)k3 * nt);
   std::vector<int32_t> htxt(13 * 16);
-  if ((e = hipMemcpyAsync(htxt.data(), w->txt, 4 * 13 * 16, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(htxt.data(), P.txt, 4 * 13 * 16, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   for (int i = 0; i < k3; i++) {
-    const WfCand cd = w->h_cands[sel[ord3[i]]];
+    const WfCand cd = P.h_cands[sel[ord3[i]]];
     for (int t = 0; t < nt; t++) {
       TxUnitRdJob& j = j3[(size_t)i * nt + t];
       j = TxUnitRdJob{};
-      j.x = hu.x, j.y = hu.y, j.pred_off = (uint32_t)(ord3[i] * L * L), j.tx_type = w->types[t];
-      j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.txt_rate = htxt[cd.dir * 16 + w->types[t]], j.rdmult = w->rdmult;
+      j.x = hu.x, j.y = hu.y, j.pred_off = (uint32_t)(ord3[i] * L * L), j.tx_type = P.types[t];
+      j.crop_w = (uint8_t)L, j.crop_h = (uint8_t)L, j.txt_rate = htxt[cd.dir * 16 + P.types[t]], j.rdmult = w->rdmult;
     }
   }
-  if ((e = hipMemcpyAsync(w->jobs3, j3.data(), j3.size() * sizeof(TxUnitRdJob), hipMemcpyHostToDevice, st)) !=
+  if ((e = hipMemcpyAsync(P.jobs3, j3.data(), j3.size() * sizeof(TxUnitRdJob), hipMemcpyHostToDevice, st)) !=
       hipSuccess)
     return e;
   for (int q = 0; q < k3 * nt; q++) {
-    e = txunit_rd_batch(w->src, W, W, H, c.bd, w->tx, w->pred, (long)k1 * L * L, w->jobs3 + q, 1, w->row, w->iscan,
-                        w->fscan, w->costs, w->flags3, w->cut_num, w->cut_den, w->ws3, w->dist3 + q, w->eob3 + q,
-                        w->rate3 + q, w->cul3 + q, nullptr, w->rec3 + (size_t)q * L * L, st, w->dq0, w->dq1,
-                        w->csatd3 + q);
+    e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)k1 * L * L, P.jobs3 + q, 1, w->row, w->iscan,
+                        P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3 + q, P.eob3 + q,
+                        P.rate3 + q, P.cul3 + q, nullptr, P.rec3 + (size_t)q * L * L, st, w->dq0, w->dq1,
+                        P.csatd3 + q);
     if (e != hipSuccess) return e;
   }
   // the winner from the results, then one call storing its reconstruction
   std::vector<uint64_t> d3((size_t)k3 * nt), s3((size_t)k3 * nt);
   std::vector<int32_t> r3((size_t)k3 * nt), e3((size_t)k3 * nt);
-  if ((e = hipMemcpyAsync(d3.data(), w->dist3, 8 * d3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(s3.data(), w->csatd3, 8 * s3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(r3.data(), w->rate3, 4 * r3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(e3.data(), w->eob3, 4 * e3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(d3.data(), P.dist3, 8 * d3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(s3.data(), P.csatd3, 8 * s3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(r3.data(), P.rate3, 4 * r3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(e3.data(), P.eob3, 4 * e3.size(), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
   uint64_t best_full = ~0ull;
   int bi = 0;
@@ -622,8 +734,8 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
     const int cand = sel[ord3[i]];
     uint64_t tc;
     const size_t j0 = (size_t)i * nt;
-    const int t = txt_pick(nt, w->types, htxt.data() + w->h_cands[cand].dir * 16, d3.data() + j0, r3.data() + j0,
-                           e3.data() + j0, s3.data() + j0, c.lambda, w->rate_th, w->satd_th, &tc);
+    const int t = txt_pick(nt, P.types, htxt.data() + P.h_cands[cand].dir * 16, d3.data() + j0, r3.data() + j0,
+                           e3.data() + j0, s3.data() + j0, c.lambda, w->rate_th, P.satd_th, &tc);
     const uint64_t full = rdc(c.lambda, (uint64_t)rate[cand] + (uint32_t)r3[j0 + t], d3[j0 + t]);
     if (full < best_full) {
       best_full = full;
@@ -633,17 +745,17 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
   // the store: the commit kernel restricted to the chosen candidate (its
   // search over that candidate's types repeats the host's)
   uint8_t m3one = ord3[bi];
-  if ((e = hipMemcpyAsync(w->m3, &m3one, 1, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(P.m3, &m3one, 1, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
   const size_t j0 = (size_t)bi * nt;
   if (c.bd == 8)
-    hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, w->m3, w->sel, k1, 1,
-                       w->cands, w->rate, nc, w->d_types, nt, w->txt, w->dist3 + j0, w->rate3 + j0, w->eob3 + j0,
-                       w->csatd3 + j0, w->rec3 + j0 * L * L, c.lambda, w->rate_th, w->satd_th, (uint8_t*)w->recon, W,
+    hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, P.m3, P.sel, k1, 1,
+                       P.cands, P.rate, nc, P.d_types, nt, P.txt, P.dist3 + j0, P.rate3 + j0, P.eob3 + j0,
+                       P.csatd3 + j0, P.rec3 + j0 * L * L, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W,
                        L, w->dec, w->mode_map);
   else
-    hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, w->m3, w->sel, k1, 1,
-                       w->cands, w->rate, nc, w->d_types, nt, w->txt, w->dist3 + j0, w->rate3 + j0, w->eob3 + j0,
-                       w->csatd3 + j0, w->rec3 + j0 * L * L, c.lambda, w->rate_th, w->satd_th, (uint16_t*)w->recon,
+    hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, P.m3, P.sel, k1, 1,
+                       P.cands, P.rate, nc, P.d_types, nt, P.txt, P.dist3 + j0, P.rate3 + j0, P.eob3 + j0,
+                       P.csatd3 + j0, P.rec3 + j0 * L * L, c.lambda, w->rate_th, P.satd_th, (uint16_t*)w->recon,
                        W, L, w->dec, w->mode_map);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return hipStreamSynchronize(st);
@@ -666,22 +778,77 @@ int wf_fail(hipError_t e, const char* where) {
     if (e_ != hipSuccess) return wf_fail(e_, #call);   \
   } while (0)
 
+// A partition map: per 64x64 unit 16 x 16 cells of log2 leaf sizes forming
+// a quadtree of squares 8..64 (ENOTSUP for 4x4 leaves, EINVAL otherwise).
+int check_map(const uint8_t* m) {
+  struct Node {
+    int sq, ox, oy;
+  };
+  Node st[64];
+  int sp = 0;
+  st[sp++] = {64, 0, 0};
+  while (sp) {
+    const Node nd = st[--sp];
+    const int l = m[(nd.oy / 4) * 16 + nd.ox / 4];
+    if (l < 2 || l > 6 || (1 << l) > nd.sq) return ZAV1_EINVAL;
+    if ((1 << l) == nd.sq) {
+      if (l < 3) return ZAV1_ENOTSUP;
+      for (int r = nd.oy / 4; r < (nd.oy + nd.sq) / 4; r++)
+        for (int k = nd.ox / 4; k < (nd.ox + nd.sq) / 4; k++)
+          if (m[r * 16 + k] != l) return ZAV1_EINVAL;
+      continue;
+    }
+    const int h = nd.sq / 2;
+    for (int i = 3; i >= 0; i--) st[sp++] = {h, nd.ox + (i & 1) * h, nd.oy + (i >> 1) * h};
+  }
+  return ZAV1_OK;
+}
+
 int check_cfg(const zav1_wf_cfg* c) {
-  if (!c || (c->bd != 8 && c->bd != 10) || (c->leaf != 8 && c->leaf != 16 && c->leaf != 32) ||
-      (c->sb != 64 && c->sb != 128) || c->width <= 0 || c->height <= 0 || c->width % c->leaf ||
-      c->height % c->leaf || c->n_images < 1 || c->tile_cols < 1 || c->tile_rows < 1 || c->qindex < 0 ||
-      c->qindex > 255 || c->n_cands < 1 || c->n_cands > 64 || c->k1 < 1 || c->k1 > c->n_cands || c->k3 < 1 ||
-      c->k3 > c->k1 || c->rdoq_level < 0 || c->rdoq_level > 3 || c->txt_groups < 1 || c->txt_groups > 6 ||
-      c->txt_rate_th < 0 || c->txt_satd_th < 0 || c->coeff_rate_est_lvl < 0 || c->coeff_rate_est_lvl > 2 ||
-      c->lambda == 0 ||
+  if (!c) return ZAV1_EINVAL;
+  const bool tree = c->leaf_map != nullptr;
+  if ((c->bd != 8 && c->bd != 10) || (c->sb != 64 && c->sb != 128) || c->width <= 0 || c->height <= 0 ||
+      c->n_images < 1 || c->tile_cols < 1 || c->tile_rows < 1 || c->qindex < 0 || c->qindex > 255 ||
+      c->n_cands < 1 || c->n_cands > 64 || c->k1 < 1 || c->k1 > c->n_cands || c->k3 < 1 || c->k3 > c->k1 ||
+      c->rdoq_level < 0 || c->rdoq_level > 3 || c->txt_groups < 1 || c->txt_groups > 6 || c->txt_rate_th < 0 ||
+      c->txt_satd_th < 0 || c->coeff_rate_est_lvl < 0 || c->coeff_rate_est_lvl > 2 || c->lambda == 0 ||
       (long)c->width * c->height * c->n_images > (1L << 31))
     return ZAV1_EINVAL;
+  if (tree) {
+    if (c->width % 64 || c->height % 64 || c->txt_groups_lt16 < 1 || c->txt_groups_lt16 > 6) return ZAV1_EINVAL;
+  } else if ((c->leaf != 8 && c->leaf != 16 && c->leaf != 32 && c->leaf != 64) || c->width % c->leaf ||
+             c->height % c->leaf || c->leaf > c->sb) {
+    return ZAV1_EINVAL;
+  }
+  bool fi_seen = false;
   for (int i = 0; i < c->n_cands; i++) {
     const zav1_md_cand m = c->cands[i];
     if (m.mode > 12 || m.delta < -3 || m.delta > 3 || m.fi_mode > 5) return ZAV1_EINVAL;
-    if (m.fi_mode < 5 && (m.mode != 0 || c->leaf > 32)) return ZAV1_EINVAL;
+    if (m.fi_mode < 5) {
+      if (m.mode != 0 || (!tree && c->leaf > 32)) return ZAV1_EINVAL;
+      fi_seen = true;
+    } else if (fi_seen) {
+      return ZAV1_EINVAL;  // filter-intra candidates come last
+    }
+  }
+  if (tree) {
+    const size_t nu = (size_t)(c->width / 64) * (c->height / 64) * c->n_images;
+    for (size_t u = 0; u < nu; u++) {
+      const int rc = check_map(c->leaf_map + u * 256);
+      if (rc) return rc;
+    }
   }
   return ZAV1_OK;
+}
+
+int plan_cfg(zav1::Wavefront* w, const zav1_wf_cfg* cfg) {
+  w->cfg = *cfg;
+  if (cfg->leaf_map) {
+    const size_t nu = (size_t)(cfg->width / 64) * (cfg->height / 64) * cfg->n_images;
+    w->leaf_map.assign(cfg->leaf_map, cfg->leaf_map + nu * 256);
+  }
+  w->cfg.leaf_map = nullptr;  // the plan keeps its own copy
+  return zav1::plan(w);
 }
 }  // namespace
 
@@ -692,8 +859,7 @@ int zav1_wf_footprint(const zav1_wf_cfg* cfg, size_t* bytes) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
   zav1::Wavefront w;
-  w.cfg = *cfg;
-  zav1::plan(&w);
+  plan_cfg(&w, cfg);
   *bytes = zav1::footprint(w);
   return ZAV1_OK;
 }
@@ -704,22 +870,19 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   int rc = check_cfg(cfg);
   if (rc) return rc;
   auto* w = new zav1::Wavefront();
-  w->cfg = *cfg;
-  zav1::plan(w);
+  plan_cfg(w, cfg);
   const zav1_wf_cfg& c = w->cfg;
   // tables of the current device (built by zav1_init) and the host-side rates
-  rc = zav1::device_tables(&w->row, &w->iscan, &w->fscan, c.bd == 10 ? 256 + c.qindex : c.qindex);
+  rc = zav1::device_tables(&w->row, &w->iscan, &w->fscan_base, c.bd == 10 ? 256 + c.qindex : c.qindex);
   if (rc) {
     delete w;
     return rc;
   }
-  w->fscan += w->tx * 3072;
   zav1_quant_row qr;
   zav1_build_quant_row(c.qindex, c.bd, &qr);
   w->dq0 = qr.dequant[0], w->dq1 = qr.dequant[1];
   w->rdmult = ((int64_t)c.lambda * 17 + 2) >> 2;  // rdoq_rdmult_full, luma, sharpness <= 2
   w->rate_th = (uint64_t)c.txt_rate_th;
-  w->satd_th = w->nt > 1 ? c.txt_satd_th : 0;
   if (c.rdoq_level == 0) w->flags3 = ZAV1_TXU_QUANT_B | ZAV1_TXU_SPATIAL | ZAV1_TXU_RECON;
   else {
     w->flags3 = ZAV1_TXU_RDOQ | ZAV1_TXU_SPATIAL | ZAV1_TXU_RECON;
@@ -732,22 +895,6 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     w->flags1 |= ZAV1_TXU_RATE_EST2;
     w->flags3 |= ZAV1_TXU_RATE_EST2;
   }
-  w->cl.n = c.n_cands;
-  static const uint8_t kFiDir[5] = {0, 1, 2, 6, 0};  // FIMODE_TO_INTRADIR
-  for (int i = 0; i < c.n_cands; i++) {
-    const zav1_md_cand m = c.cands[i];
-    w->cl.c[i] = zav1::MdCand{m.mode, m.delta, m.fi_mode, 0};
-    w->h_cands.push_back(zav1::WfCand{m.mode, m.delta, m.fi_mode, (uint8_t)(m.fi_mode < 5 ? kFiDir[m.fi_mode] : m.mode)});
-  }
-  std::vector<uint32_t> rtab(25 * (size_t)c.n_cands);
-  for (int a = 0; a < 5; a++)
-    for (int l = 0; l < 5; l++)
-      for (int i = 0; i < c.n_cands; i++)
-        rtab[(a * 5 + l) * c.n_cands + i] = (uint32_t)zav1_intra_mode_rate(c.leaf, c.leaf, a, l, c.cands[i].mode,
-                                                                           c.cands[i].delta, c.cands[i].fi_mode);
-  std::vector<int32_t> txt(13 * 16);
-  for (int d = 0; d < 13; d++)
-    for (int t = 0; t < 16; t++) txt[d * 16 + t] = zav1_intra_txt_rate(w->tx, d, t);
   zav1_coeff_costs costs;
   zav1_build_coeff_costs(c.qindex, &costs);
   // one device allocation, carved
@@ -763,8 +910,8 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     p += zav1::al(b);
     return (void*)q;
   };
-  const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w->stacked_h * es, U = (size_t)w->max_u;
-  const size_t n = (size_t)w->n_units, LL = (size_t)w->L * w->L, n3 = U * w->k3 * w->nt;
+  const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w->stacked_h * es;
+  const size_t n = (size_t)w->n_units;
   w->src = take(plane);
   w->recon = take(plane);
   w->units = (zav1::MdUnit*)take(n * sizeof(zav1::MdUnit));
@@ -772,33 +919,7 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   w->uid = (int32_t*)take(n * 4);
   w->dec = (zav1::WfDecision*)take(n * sizeof(zav1::WfDecision));
   w->mode_map = (uint8_t*)take(n);
-  w->rate_tab = (uint32_t*)take(25 * w->nc * 4);
-  w->txt = (int32_t*)take(13 * 16 * 4);
-  w->cands = (zav1::WfCand*)take(64 * sizeof(zav1::WfCand));
-  w->d_types = (uint8_t*)take(16);
   w->costs = (int32_t*)take(sizeof(zav1_coeff_costs));
-  w->rate = (uint32_t*)take(U * w->nc * 4);
-  w->satd = (uint32_t*)take(U * w->nc * 4);
-  w->sel = (uint8_t*)take(U * w->k1);
-  w->m3 = (uint8_t*)take(U * w->k1);
-  w->sel_cost = (uint64_t*)take(U * w->k1 * 8);
-  w->fs_jobs = (zav1::TxUnitJob*)take(U * w->k1 * sizeof(zav1::TxUnitJob));
-  w->pred = (uint16_t*)take(U * w->k1 * LL * 2);
-  w->jobs1 = (zav1::TxUnitRdJob*)take(U * w->k1 * sizeof(zav1::TxUnitRdJob));
-  w->dist1 = (uint64_t*)take(U * w->k1 * 8);
-  w->eob1 = (int32_t*)take(U * w->k1 * 4);
-  w->rate1 = (int32_t*)take(U * w->k1 * 4);
-  w->cul1 = (uint8_t*)take(U * w->k1);
-  w->jobs3 = (zav1::TxUnitRdJob*)take(n3 * sizeof(zav1::TxUnitRdJob));
-  w->dist3 = (uint64_t*)take(n3 * 8);
-  w->csatd3 = (uint64_t*)take(n3 * 8);
-  w->eob3 = (int32_t*)take(n3 * 4);
-  w->rate3 = (int32_t*)take(n3 * 4);
-  w->cul3 = (uint8_t*)take(n3);
-  w->rec3 = (uint16_t*)take(n3 * LL * 2);
-  w->ws1 = take(zav1::txunit_rd_workspace(w->tx, (int)(U * w->k1)));
-  w->ws3_bytes = zav1::ws3_bytes(w->tx, (int)n3);
-  w->ws3 = take(w->ws3_bytes);
   std::vector<int32_t> uid(n);
   for (size_t i = 0; i < n; i++) uid[i] = (int32_t)i;
   hipError_t e = hipSuccess;
@@ -808,11 +929,52 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   up(w->units, w->h_units.data(), n * sizeof(zav1::MdUnit));
   up(w->nbr, w->h_nbr.data(), n * 8);
   up(w->uid, uid.data(), n * 4);
-  up(w->rate_tab, rtab.data(), rtab.size() * 4);
-  up(w->txt, txt.data(), txt.size() * 4);
-  up(w->cands, w->h_cands.data(), w->h_cands.size() * sizeof(zav1::WfCand));
-  up(w->d_types, w->types, 16);
   up(w->costs, &costs, sizeof(costs));
+  for (int k = 0; k < 4; k++) {
+    zav1::SizePlan& P = w->sp[k];
+    if (!P.max_u) continue;
+    const size_t U = (size_t)P.max_u, LL = (size_t)P.L * P.L, n3 = U * P.k3 * P.nt;
+    P.fscan = w->fscan_base + P.tx * 3072;
+    P.rate_tab = (uint32_t*)take(25 * P.nc * 4);
+    P.txt = (int32_t*)take(13 * 16 * 4);
+    P.cands = (zav1::WfCand*)take(64 * sizeof(zav1::WfCand));
+    P.d_types = (uint8_t*)take(16);
+    P.rate = (uint32_t*)take(U * P.nc * 4);
+    P.satd = (uint32_t*)take(U * P.nc * 4);
+    P.sel = (uint8_t*)take(U * P.k1);
+    P.m3 = (uint8_t*)take(U * P.k1);
+    P.sel_cost = (uint64_t*)take(U * P.k1 * 8);
+    P.fs_jobs = (zav1::TxUnitJob*)take(U * P.k1 * sizeof(zav1::TxUnitJob));
+    P.pred = (uint16_t*)take(U * P.k1 * LL * 2);
+    P.jobs1 = (zav1::TxUnitRdJob*)take(U * P.k1 * sizeof(zav1::TxUnitRdJob));
+    P.dist1 = (uint64_t*)take(U * P.k1 * 8);
+    P.eob1 = (int32_t*)take(U * P.k1 * 4);
+    P.rate1 = (int32_t*)take(U * P.k1 * 4);
+    P.cul1 = (uint8_t*)take(U * P.k1);
+    P.jobs3 = (zav1::TxUnitRdJob*)take(n3 * sizeof(zav1::TxUnitRdJob));
+    P.dist3 = (uint64_t*)take(n3 * 8);
+    P.csatd3 = (uint64_t*)take(n3 * 8);
+    P.eob3 = (int32_t*)take(n3 * 4);
+    P.rate3 = (int32_t*)take(n3 * 4);
+    P.cul3 = (uint8_t*)take(n3);
+    P.rec3 = (uint16_t*)take(n3 * LL * 2);
+    P.ws1 = take(zav1::txunit_rd_workspace(P.tx, (int)(U * P.k1)));
+    P.ws3_bytes = zav1::ws3_bytes(P.tx, (int)n3);
+    P.ws3 = take(P.ws3_bytes);
+    std::vector<uint32_t> rtab(25 * (size_t)P.nc);
+    for (int a = 0; a < 5; a++)
+      for (int l = 0; l < 5; l++)
+        for (int i = 0; i < P.nc; i++)
+          rtab[(a * 5 + l) * P.nc + i] = (uint32_t)zav1_intra_mode_rate(P.L, P.L, a, l, c.cands[i].mode,
+                                                                         c.cands[i].delta, c.cands[i].fi_mode);
+    std::vector<int32_t> txt(13 * 16);
+    for (int d = 0; d < 13; d++)
+      for (int t = 0; t < 16; t++) txt[d * 16 + t] = zav1_intra_txt_rate(P.tx, d, t);
+    up(P.rate_tab, rtab.data(), rtab.size() * 4);
+    up(P.txt, txt.data(), txt.size() * 4);
+    up(P.cands, P.h_cands.data(), P.h_cands.size() * sizeof(zav1::WfCand));
+    up(P.d_types, P.types, 16);
+  }
   if (e != hipSuccess) {
     const int r = wf_fail(e, "zav1_wf_create upload");
     (void)hipFree(w->base);
@@ -833,10 +995,16 @@ int zav1_wf_info(const zav1_wf* wf, zav1_wf_info_t* info) {
   info->n_leaves = w->n_units;
   info->n_steps = w->n_steps;
   info->max_leaves_per_step = w->max_u;
-  info->n_types = w->nt;
   info->stacked_height = w->stacked_h;
   info->bytes = w->bytes;
-  for (int i = 0; i < 16; i++) info->types[i] = w->types[i];
+  // the tx types of the configuration's leaf size (uniform leaves), else of
+  // the largest leaf size present
+  int k = w->cfg.leaf == 8 ? 0 : (w->cfg.leaf == 16 ? 1 : (w->cfg.leaf == 32 ? 2 : 3));
+  if (!w->sp[k].max_u)
+    for (k = 3; k > 0 && !w->sp[k].max_u; k--) {
+    }
+  info->n_types = w->sp[k].nt;
+  for (int i = 0; i < 16; i++) info->types[i] = w->sp[k].types[i];
   return ZAV1_OK;
 }
 

@@ -18,7 +18,7 @@ CFG = np.dtype([('width', '<i4'), ('height', '<i4'), ('n_images', '<i4'), ('bd',
                 ('sb', '<i4'), ('tile_cols', '<i4'), ('tile_rows', '<i4'), ('qindex', '<i4'), ('k1', '<i4'),
                 ('k3', '<i4'), ('rdoq_level', '<i4'), ('txt_groups', '<i4'), ('txt_rate_th', '<i4'),
                 ('txt_satd_th', '<i4'), ('coeff_rate_est_lvl', '<i4'), ('lambda', '<u8'), ('n_cands', '<i4'), ('pad', '<i4'),
-                ('cands', CAND, 64)])
+                ('cands', CAND, 64), ('txt_groups_lt16', '<i4'), ('pad2', '<i4'), ('leaf_map', '<u8')])
 DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
                      ('rate', '<i4'), ('mode_rate', '<i4'), ('dist', '<u8'), ('cost', '<u8')])
 MD_UNIT = np.dtype([('x', '<i4'), ('y', '<i4'), ('n_top_px', '<i2'), ('n_topright_px', '<i2'),
@@ -127,6 +127,21 @@ def funnel_cfg(width, height, bd, qindex, lam, preset, cli_qp, **kw):
     return make_cfg(width, height, bd, qindex, lam, **f, **kw)
 
 
+def tree_cfg(width, height, bd, qindex, lam, preset, cli_qp, leaf_map, **kw):
+    """funnel_cfg for the leaves of a partition map (the partition pre-pass's
+    leaf maps, n_images x units x 256): the candidates with filter intra
+    (64x64 leaves drop it), the tx-type groups of leaves from 16x16 and of
+    the smaller ones.  The map must stay alive until the plan is created."""
+    f = preset_funnel(preset, cli_qp, 16)
+    f_small = preset_funnel(preset, cli_qp, 8)
+    cfg = make_cfg(width, height, bd, qindex, lam, **f, **kw)
+    cfg[0]['txt_groups_lt16'] = f_small['txt_groups']
+    if not (isinstance(leaf_map, np.ndarray) and leaf_map.dtype == np.uint8 and leaf_map.flags['C_CONTIGUOUS']):
+        raise TypeError('leaf_map: a C-contiguous uint8 array (its address goes into the configuration)')
+    cfg[0]['leaf_map'] = leaf_map.ctypes.data
+    return cfg
+
+
 def make_cfg(width, height, bd, qindex, lam, *, n_images=1, leaf=16, sb=64, tiles=(1, 1), k1=15, k3=4,
              rdoq_level=1, txt_groups=6, txt_rate_th=250, txt_satd_th=0, coeff_rate_est_lvl=1, cands=None):
     cands = preset_candidates() if cands is None else cands
@@ -218,6 +233,18 @@ class WavefrontMD:
         d = np.ascontiguousarray(d, DECISION)
         assert d.shape == (self.info.n_leaves,)
         self.z.check(self.z.lib().zav1_memcpy_h2d(self.info.d_decisions, d.ctypes.data, d.nbytes, None), 'dec')
+
+    def decisions_cells(self):
+        """Per 8x8 cell: (n_images, height / 8, width / 8) records, each leaf's
+        at its top-left cell (zeros elsewhere): the layout for partition maps."""
+        n = self.info.n_leaves
+        d = np.zeros(n, DECISION)
+        self.z.check(self.z.lib().zav1_memcpy_d2h(d.ctypes.data, self.info.d_decisions, d.nbytes, None), 'dec')
+        u = self.units()
+        grid = np.zeros((self.n, self.H // 8, self.W // 8), DECISION)
+        img = u['y'] // self.H
+        grid[img, (u['y'] - img * self.H) // 8, u['x'] // 8] = d
+        return grid
 
     def decisions(self):
         """Per leaf grid position: (n_images, height / leaf, width / leaf) decision records."""
