@@ -625,7 +625,7 @@ typedef struct zav1_wf_cfg {
   int32_t pad;
   zav1_md_cand cands[64];
   int32_t txt_groups_lt16;        /* with a partition map: tx-type groups of leaves below 16x16 */
-  int32_t pad2;
+  int32_t keep_levels;            /* 1: keep every winner's quantized levels (for the entropy coder) */
   const uint8_t *leaf_map;        /* NULL: uniform leaves of `leaf`; else a partition map (host memory, read by
                                      zav1_wf_create / zav1_wf_footprint only): per image, per 64x64 unit in
                                      raster order, 256 log2 leaf sizes of its 4x4 cells (zav1_pd0_run's
@@ -644,6 +644,9 @@ typedef struct zav1_wf_info_t {
   int32_t n_leaves, n_steps, max_leaves_per_step, n_types, stacked_height;
   uint8_t types[16];              /* MDS3 tx types in search order */
   size_t bytes;                   /* device memory of the object */
+  int32_t *d_levels;              /* keep_levels: the winners' levels, packed min(leaf, 32)^2 raster per leaf */
+  const uint64_t *d_level_off;    /* keep_levels: per leaf (storage order) its offset in d_levels */
+  size_t n_levels;
 } zav1_wf_info_t;
 typedef struct zav1_wf zav1_wf;
 /* Device bytes zav1_wf_create would take for cfg (admission by footprint). */
@@ -1027,6 +1030,55 @@ int zav1_pd0_nodes_per_unit(const zav1_pd0_cfg *cfg);
  * nothing. */
 int zav1_pd0_run(const zav1_pd0_cfg *cfg, const void *d_plane, const zav1_coeff_costs *d_costs, zav1_pd0_sb *d_sb,
                  zav1_pd0_node *d_nodes, uint8_t *d_leaf_map, zav1_stream_t s);
+
+
+/* ------------------------------------------------------------------------ */
+/* AV1 bitstream of a still image's luma decisions (host).                   */
+/* ------------------------------------------------------------------------ */
+/* One leaf of a still: a square block whose transform is the block, its
+ * intra mode info and its quantized levels (packed min(size, 32)^2 raster,
+ * the coded top-left 32x32 of a 64x64). */
+typedef struct zav1_still_leaf {
+  int32_t x, y;                   /* luma position, pixels */
+  uint8_t size;                   /* 8, 16, 32 or 64 */
+  uint8_t mode;                   /* intra mode 0..12 */
+  int8_t delta;                   /* angle delta -3..3 (directional modes) */
+  uint8_t fi_mode;                /* filter-intra mode 0..4, 5 = none */
+  uint8_t tx_type;                /* in the size's intra ext-tx set */
+  uint8_t pad[3];
+  int32_t eob;                    /* scan positions coded (the last nonzero); 0 = skip */
+  int32_t pad2;
+  uint64_t level_off;             /* its levels at levels + level_off */
+} zav1_still_leaf;
+typedef struct zav1_still_desc {
+  int32_t width, height;          /* multiples of the superblock size */
+  int32_t bd;                     /* 8 or 10 */
+  int32_t sb;                     /* 64 or 128 */
+  int32_t tile_cols, tile_rows;   /* uniform tiles, powers of two within AV1's tile limits */
+  int32_t qindex;                 /* 1..255 */
+  int32_t n_leaves;
+  int32_t threads;                /* host threads coding tiles in parallel; 0 = all cores */
+  int32_t pad;
+  const zav1_still_leaf *leaves;  /* any order; together they cover the frame once */
+  const int32_t *levels;
+} zav1_still_desc;
+/* The AV1 stream (temporal delimiter, reduced still-picture sequence header,
+ * one frame OBU) of a monochrome key frame coding these leaves: partition
+ * (NONE / SPLIT quadtrees), skip, key-frame y mode, angle delta, filter
+ * intra, tx type and coefficients with adaptive CDFs, each tile with fresh
+ * CDFs and contexts, TX_MODE_LARGEST, no loop filter / CDEF / restoration.
+ * Replaces the entropy walk and OBU packing of the reference
+ * (rust/crates/svtav1-encoder/src/pipeline.rs:2542-2811 over
+ * rust/crates/svtav1-entropy/src/range_coder.rs, coeff_c.rs:924, obu.rs) for
+ * the luma plane.  Writes *size bytes into out; ZAV1_ENOMEM (with *size the
+ * stream's length) when cap is smaller.  Host only. */
+int zav1_write_still(const zav1_still_desc *d, uint8_t *out, size_t cap, size_t *size);
+/* The streams of images [first, first + n) of a wavefront batch run with
+ * keep_levels, back to back in out; sizes[i] each.  The batch's tiles must
+ * be powers of two and its sides multiples of the superblock.  Host-side
+ * coding after the device results are read back (synchronous). */
+int zav1_wf_write_stills(zav1_wf *wf, int first, int n, int threads, uint8_t *out, size_t cap, size_t *sizes,
+                         zav1_stream_t s);
 
 #ifdef __cplusplus
 }

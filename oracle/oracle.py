@@ -571,14 +571,16 @@ def cdef_apply_frame(y, u, v, bd, damping, fb_str, skip=None):
     return oy, ou, ov, stats
 
 
-def wf_frame(cfg, images, leaf_map=None):
+def wf_frame(cfg, images, leaf_map=None, levels=False):
     """or_wf_frame over (n_images, height, width) samples with a zav1_wf_cfg
     record (numpy, same layout); returns (recon, decisions[img][row][col]):
     rows / columns of leaves for uniform leaves, of 8x8 cells (each leaf's
     record at its top-left cell) with a partition map (n_images x units x
-    256 log2 leaf sizes, the partition pre-pass's layout)."""
+    256 log2 leaf sizes, the partition pre-pass's layout).  levels=True adds
+    the winners' quantized levels: (n_images, height, width) int32, each
+    leaf's min(L, 32)^2 raster at its top-left corner."""
     L = lib()
-    L.or_wf_frame.argtypes = [_vp, _vp, _vp, _vp]
+    L.or_wf_frame_levels.argtypes = [_vp, _vp, _vp, _vp, _vp]
     cfg = np.array(cfg, copy=True)
     c = cfg[0]
     n, H, W, leaf = int(c['n_images']), int(c['height']), int(c['width']), int(c['leaf'])
@@ -593,9 +595,11 @@ def wf_frame(cfg, images, leaf_map=None):
         c['leaf_map'] = 0
         dec = np.zeros((n, H // leaf, W // leaf), WF_DECISION)
     cfg[0] = c
-    L.or_wf_frame(np.ascontiguousarray(cfg).ctypes.data, src.ctypes.data, rec.ctypes.data, dec.ctypes.data)
+    lev = np.zeros((n, H, W), np.int32) if levels else None
+    L.or_wf_frame_levels(np.ascontiguousarray(cfg).ctypes.data, src.ctypes.data, rec.ctypes.data, dec.ctypes.data,
+                         lev.ctypes.data if levels else None)
     del lm
-    return rec, dec
+    return (rec, dec, lev) if levels else (rec, dec)
 
 
 WF_DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
@@ -728,3 +732,33 @@ def leaf_sizes(leaf_map):
 
     walk(64, 0, 0)
     return out
+
+
+# ---- AV1 still decoder (av1dec.c) --------------------------------------------
+DEC_BLOCK = np.dtype([('x', '<i4'), ('y', '<i4'), ('size', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('skip', 'u1'),
+                      ('delta', 'i1'), ('tx_type', 'u1'), ('pad', 'u1', 2), ('eob', '<i4'), ('pad2', '<i4'),
+                      ('level_off', '<u8')])
+
+
+class DecInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ('width', 'height', 'bd', 'sb', 'qindex', 'tile_cols', 'tile_rows',
+                                              'n_blocks')] + [('n_levels', ctypes.c_int64)]
+
+
+def av1_decode_still(stream, max_w, max_h, bd):
+    """Decode a still of the library's subset: (frame, info, blocks, levels)."""
+    L = lib()
+    L.or_av1_decode_still.argtypes = [_vp, ctypes.c_size_t, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(DecInfo),
+                                      _vp, ctypes.c_int, _vp, ctypes.c_size_t]
+    data = np.frombuffer(bytes(stream), np.uint8)
+    frame = np.zeros(max_w * max_h, np.uint16 if bd > 8 else np.uint8)  # decoded rows are `width` apart
+    nb = (max_w // 8) * (max_h // 8)
+    blocks = np.zeros(nb, DEC_BLOCK)
+    levels = np.zeros(max_w * max_h, np.int32)
+    info = DecInfo()
+    rc = L.or_av1_decode_still(data.ctypes.data, data.size, frame.ctypes.data, max_w, max_h, ctypes.byref(info),
+                               blocks.ctypes.data, nb, levels.ctypes.data, levels.size)
+    if rc:
+        raise ValueError('or_av1_decode_still: %d' % rc)
+    img = frame[:info.width * info.height].reshape(info.height, info.width)
+    return img, info, blocks[:info.n_blocks], levels[:info.n_levels]

@@ -61,6 +61,7 @@ typedef struct {
   uint64_t dist, csatd;
   int eob, rate;
   int32_t rec[64 * 64];
+  int32_t lev[32 * 32]; /* quantized levels, min(L, 32)^2 raster */
 } rd_out;
 
 /* One transform unit of the funnel (tx_unit_inner with do_rdoq and the exact
@@ -104,6 +105,7 @@ static void rd_unit(const void *src, int stride, int hbd, int bd, long off, cons
     eob = or_optimize_b(pk, q, dq, eob, scan, row.dequant, &txb[txs * 2], &eobc[ems * 2], rdmult, tx, cls, 0, 0,
                         0, cut_num, cut_den);
   o->eob = eob;
+  memcpy(o->lev, q, sizeof(int32_t) * np);
   o->rate = eob ? or_cost_coeffs_txb(q, eob, scan, &txb[txs * 2], &eobc[ems * 2], tx, cls, 0, 0, 0, txt_rate)
                 : txb[txs * 2].txb_skip_cost[0][1];
   if (est2 && eob < (n >> 6)) o->rate = 6000 + 1000 * eob;
@@ -148,6 +150,11 @@ static int unit_leaves(const uint8_t *map, int sq, int ox, int oy, int ux, int u
 }
 
 void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec) {
+  or_wf_frame_levels(cfg, src, recon, dec, NULL);
+}
+
+void or_wf_frame_levels(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec,
+                        int32_t *levels) {
   const int W = cfg->width, H = cfg->height, SB = cfg->sb, bd = cfg->bd, hbd = bd > 8;
   const int tree = cfg->leaf_map != NULL;
   const int gw8 = W / 8, gh8 = H / 8, uw = W / 64;
@@ -304,6 +311,12 @@ void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decis
               d->cand = (uint8_t)m, d->tx_type = (uint8_t)types[win_t], d->mode = cfg->cands[m].mode;
               d->fi_mode = cfg->cands[m].fi_mode, d->eob = ow->eob, d->rate = ow->rate, d->mode_rate = mrate[m];
               d->dist = ow->dist, d->cost = best_full;
+              if (levels) {
+                const int P = L < 32 ? L : 32;
+                int32_t *lp = levels + (size_t)img * W * H;
+                for (int rr = 0; rr < P; rr++)
+                  for (int cc = 0; cc < P; cc++) lp[(size_t)(y + rr) * W + x + cc] = ow->lev[rr * P + cc];
+              }
               for (int rr = y / 8; rr < (y + L) / 8; rr++)
                 for (int cc = x / 8; cc < (x + L) / 8; cc++) modes[rr * gw8 + cc] = cfg->cands[m].mode;
             }

@@ -296,7 +296,7 @@ typedef struct { /* same layout as zav1_wf_cfg */
   uint64_t lambda;
   int32_t n_cands, pad;
   or_wf_cand cands[64];
-  int32_t txt_groups_lt16, pad2; /* tx-type groups of leaves below 16x16 (partition map only) */
+  int32_t txt_groups_lt16, keep_levels; /* tx-type groups of leaves below 16x16 (partition map only) */
   const uint8_t *leaf_map;       /* NULL: uniform leaves; else per image, per 64x64 unit (raster), 256 log2
                                     leaf sizes (16 x 16 cells of 4x4): the partition pre-pass's map */
 } or_wf_cfg;
@@ -309,6 +309,11 @@ typedef struct {
  * uniform leaves: dec[img][leaf row][leaf col]; with a partition map:
  * dec[img][y / 8][x / 8] at each leaf's top-left 8x8 cell. */
 void or_wf_frame(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec);
+/* or_wf_frame that also returns the winners' quantized levels: per image a
+ * width x height int32 plane, each leaf's min(L, 32)^2 levels (raster) at
+ * its top-left corner (the leaf's own area, so leaves never overlap). */
+void or_wf_frame_levels(const or_wf_cfg *cfg, const void *src, void *recon, or_wf_decision *dec,
+                        int32_t *levels);
 
 
 /* Coefficient rate with C's mds_fast_coeff_est_level (1 = full loop). */
@@ -359,6 +364,26 @@ int or_pd0_sb(const uint8_t *src, int stride, int sb_x, int sb_y, const or_pd0_c
 /* Every 64x64 unit of a frame (sides multiples of 64), raster order. */
 int or_pd0_frame(const uint8_t *src, int stride, int width, int height, const or_pd0_cfg *cfg, or_pd0_node *nodes,
                  uint8_t *leaf_map, uint8_t *modes);
+
+
+/* ---- AV1 still decoder (av1dec.c; the specification's decoding process) --- */
+typedef struct {
+  int32_t x, y;                       /* luma position */
+  uint8_t size, mode, fi_mode, skip;  /* square block, y mode, filter intra (5 none), skip */
+  int8_t delta;                       /* angle delta */
+  uint8_t tx_type, pad[2];
+  int32_t eob, pad2;
+  uint64_t level_off;                 /* its levels (min(size, 32)^2, raster) in the level array */
+} or_dec_block;
+typedef struct {
+  int32_t width, height, bd, sb, qindex, tile_cols, tile_rows, n_blocks;
+  int64_t n_levels;
+} or_dec_info;
+/* Decodes a monochrome reduced-still-picture stream of the library's subset
+ * into frame (u8 at 8 bits, u16 at 10, width x height); blocks in decoding
+ * order.  0 ok, -1 malformed / too small, -2 outside the subset. */
+int or_av1_decode_still(const uint8_t *data, size_t size, void *frame, int max_w, int max_h, or_dec_info *info,
+                        or_dec_block *blocks, int max_blocks, int32_t *levels, size_t max_levels);
 
 #ifdef __cplusplus
 }

@@ -196,6 +196,8 @@ EXPORTS = {
     'zav1_pd0_rates': [_vp],
     'zav1_pd0_nodes_per_unit': [_vp],
     'zav1_pd0_run': [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'zav1_write_still': [_vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
+    'zav1_wf_write_stills': [_vp, _i32, _i32, _i32, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _vp],
 }
 
 # numpy layouts of the job structs (match include/zav1_dsp.h)

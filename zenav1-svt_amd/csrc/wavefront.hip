@@ -208,7 +208,9 @@ __global__ __launch_bounds__(256) void k_wf_commit(const MdUnit* __restrict__ un
                                                    const uint16_t* __restrict__ rec3, uint64_t lambda,
                                                    uint64_t rate_th, int64_t satd_th, Pix* __restrict__ recon,
                                                    int stride, int L, WfDecision* __restrict__ dec,
-                                                   uint8_t* __restrict__ mode_map) {
+                                                   uint8_t* __restrict__ mode_map, const int32_t* __restrict__ q3,
+                                                   int32_t* __restrict__ levels,
+                                                   const uint64_t* __restrict__ level_off) {
   __shared__ int win_job;
   const int u = blockIdx.x;
   if (u >= n) return;
@@ -250,6 +252,14 @@ __global__ __launch_bounds__(256) void k_wf_commit(const MdUnit* __restrict__ un
     const int r = i / L, c = i - r * L;
     recon[(y0 + r) * stride + x0 + c] = (Pix)src[i];
   }
+  // the winner's quantized levels (packed min(L, 32)^2, raster), for the
+  // entropy coder
+  if (levels) {
+    const int np = L < 32 ? L * L : 1024;
+    const int32_t* qs = q3 + (long)win_job * np;
+    int32_t* dst = levels + level_off[unit_id[u]];
+    for (int i = threadIdx.x; i < np; i += 256) dst[i] = qs[i];
+  }
 }
 
 }  // namespace
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(256) void k_wf_commit(const MdUnit* __restrict__ un
 // Per leaf size: the funnel's candidates, counts and tx types at that size
 // and the stage buffers, sized for the size's largest step.
 struct SizePlan {
-  int L = 0, tx = 0, nt = 0, nc = 0, k1 = 0, k3 = 0, max_u = 0;
+  int L = 0, tx = 0, nt = 0, nc = 0, k1 = 0, k3 = 0, max_u = 0, np = 0;
   uint8_t types[16] = {};
   MdCandList cl{};
   std::vector<WfCand> h_cands;
@@ -279,6 +289,7 @@ struct SizePlan {
   TxUnitRdJob *jobs1 = nullptr, *jobs3 = nullptr;
   uint64_t *dist1 = nullptr, *dist3 = nullptr, *csatd3 = nullptr;
   int32_t *eob1 = nullptr, *rate1 = nullptr, *eob3 = nullptr, *rate3 = nullptr;
+  int32_t* q3 = nullptr;  // MDS3 levels per job (kept levels only)
   uint8_t *cul1 = nullptr, *cul3 = nullptr;
   void *ws1 = nullptr, *ws3 = nullptr;
   size_t ws3_bytes = 0;  // MDS3's workspace, with room for the trellis's lane form
@@ -294,6 +305,9 @@ struct Wavefront {
   std::vector<MdUnit> h_units;
   std::vector<int32_t> h_nbr;
   std::vector<uint8_t> leaf_map;  // the caller's partition map (empty: uniform leaves)
+  std::vector<uint64_t> h_loff;   // per unit: offset of its kept levels
+  std::vector<uint64_t> img_loff; // per image: first kept level (n_images + 1 entries)
+  size_t n_levels = 0;
   SizePlan sp[4];
   uint64_t rate_th = 0;
   int64_t rdmult = 0;
@@ -310,6 +324,8 @@ struct Wavefront {
   WfDecision* dec = nullptr;
   uint8_t* mode_map = nullptr;
   int32_t* costs = nullptr;
+  int32_t* levels = nullptr;  // kept levels (cfg.keep_levels), per unit at h_loff
+  uint64_t* loff = nullptr;
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
 };
@@ -357,6 +373,7 @@ static int search_types(int tx_size, int groups, uint8_t* out) {
 static void size_setup(const zav1_wf_cfg& c, bool tree, int L, SizePlan* p) {
   static const uint8_t kFiDir[5] = {0, 1, 2, 6, 0};  // FIMODE_TO_INTRADIR
   p->L = L;
+  p->np = L < 32 ? L * L : 1024;
   for (int s = 0; s < 19; s++)
     if (tx_w(s) == L && tx_h(s) == L) p->tx = s;
   int nc = c.n_cands;
@@ -534,6 +551,26 @@ static int plan(Wavefront* w) {
   }
   w->step_off.push_back(n);
   w->n_steps = (int)w->step_sz.size();
+  // kept levels image-major (an image's levels are one range), storage
+  // order within an image
+  w->h_loff.assign(n, 0);
+  w->n_levels = 0;
+  w->img_loff.assign(c.n_images + 1, 0);
+  if (c.keep_levels) {
+    for (int s = 0; s < n; s++) {
+      const int L = 8 << w->h_sz[s];
+      w->img_loff[w->h_units[s].y / c.height + 1] += L < 32 ? (size_t)L * L : 1024;
+    }
+    for (int i = 0; i < c.n_images; i++) w->img_loff[i + 1] += w->img_loff[i];
+    std::vector<uint64_t> next(w->img_loff.begin(), w->img_loff.end() - 1);
+    for (int s = 0; s < n; s++) {
+      const int L = 8 << w->h_sz[s];
+      uint64_t& o = next[w->h_units[s].y / c.height];
+      w->h_loff[s] = o;
+      o += L < 32 ? (uint64_t)L * L : 1024;
+    }
+    w->n_levels = w->img_loff[c.n_images];
+  }
   w->max_u = 0;
   for (int s = 0; s < w->n_steps; s++) {
     const int u = w->step_off[s + 1] - w->step_off[s];
@@ -552,7 +589,7 @@ static size_t ws3_bytes(int tx, int n3) {
   return ((txunit_rd_workspace(tx, n3) + 255) & ~(size_t)255) + rdoq_workspace(tx, n3);
 }
 
-static size_t size_bytes(const SizePlan& p) {
+static size_t size_bytes(const SizePlan& p, bool keep) {
   if (!p.max_u) return 0;
   const size_t U = (size_t)p.max_u, LL = (size_t)p.L * p.L, n3 = U * p.k3 * p.nt;
   size_t b = al(25 * p.nc * 4) + al(13 * 16 * 4) + al(64 * sizeof(WfCand)) + al(16);
@@ -561,6 +598,7 @@ static size_t size_bytes(const SizePlan& p) {
        al(U * p.k1);
   b += al(n3 * sizeof(TxUnitRdJob)) + 2 * al(n3 * 8) + 2 * al(n3 * 4) + al(n3) + al(n3 * LL * 2);
   b += al(txunit_rd_workspace(p.tx, (int)(U * p.k1))) + al(ws3_bytes(p.tx, (int)n3)) + al(64);
+  if (keep) b += al(n3 * p.np * 4);
   return b;
 }
 
@@ -570,7 +608,8 @@ static size_t footprint(const Wavefront& w) {
   const size_t n = (size_t)w.n_units;
   size_t b = 2 * al(plane) + al(n * sizeof(MdUnit)) + al(n * 8) + al(n * 4) + al(n * sizeof(WfDecision)) + al(n);
   b += al(sizeof(zav1_coeff_costs));
-  for (int k = 0; k < 4; k++) b += size_bytes(w.sp[k]);
+  if (c.keep_levels) b += al(w.n_levels * 4) + al(n * 8);
+  for (int k = 0; k < 4; k++) b += size_bytes(w.sp[k], c.keep_levels != 0);
   return b;
 }
 
@@ -625,18 +664,19 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   MARK(7);
   e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs3, n3, w->row, w->iscan,
                       P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3, P.eob3, P.rate3, P.cul3,
-                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes);
+                      P.q3, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes);
   if (e != hipSuccess) return e;
   MARK(10);
   if (c.bd == 8)
     hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, P.m3, P.sel,
                        P.k1, P.k3, P.cands, P.rate, P.nc, P.d_types, P.nt, P.txt, P.dist3, P.rate3, P.eob3, P.csatd3,
-                       P.rec3, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W, L, w->dec, w->mode_map);
+                       P.rec3, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W, L, w->dec, w->mode_map, P.q3,
+                       w->levels, w->loff);
   else
     hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, P.m3,
                        P.sel, P.k1, P.k3, P.cands, P.rate, P.nc, P.d_types, P.nt, P.txt, P.dist3, P.rate3, P.eob3,
                        P.csatd3, P.rec3, c.lambda, w->rate_th, P.satd_th, (uint16_t*)w->recon, W, L, w->dec,
-                       w->mode_map);
+                       w->mode_map, P.q3, w->levels, w->loff);
   MARK(kWfStages);
   return hipGetLastError();
 }
@@ -716,7 +756,8 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
   for (int q = 0; q < k3 * nt; q++) {
     e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)k1 * L * L, P.jobs3 + q, 1, w->row, w->iscan,
                         P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3 + q, P.eob3 + q,
-                        P.rate3 + q, P.cul3 + q, nullptr, P.rec3 + (size_t)q * L * L, st, w->dq0, w->dq1,
+                        P.rate3 + q, P.cul3 + q, P.q3 ? P.q3 + (size_t)q * P.np : nullptr, P.rec3 + (size_t)q * L * L,
+                        st, w->dq0, w->dq1,
                         P.csatd3 + q);
     if (e != hipSuccess) return e;
   }
@@ -751,12 +792,12 @@ static hipError_t plain_unit(Wavefront* w, int u, hipStream_t st) {
     hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, P.m3, P.sel, k1, 1,
                        P.cands, P.rate, nc, P.d_types, nt, P.txt, P.dist3 + j0, P.rate3 + j0, P.eob3 + j0,
                        P.csatd3 + j0, P.rec3 + j0 * L * L, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W,
-                       L, w->dec, w->mode_map);
+                       L, w->dec, w->mode_map, P.q3 ? P.q3 + j0 * P.np : nullptr, w->levels, w->loff);
   else
     hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3(1), dim3(256), 0, st, unit, 1, w->uid + u, P.m3, P.sel, k1, 1,
                        P.cands, P.rate, nc, P.d_types, nt, P.txt, P.dist3 + j0, P.rate3 + j0, P.eob3 + j0,
                        P.csatd3 + j0, P.rec3 + j0 * L * L, c.lambda, w->rate_th, P.satd_th, (uint16_t*)w->recon,
-                       W, L, w->dec, w->mode_map);
+                       W, L, w->dec, w->mode_map, P.q3 ? P.q3 + j0 * P.np : nullptr, w->levels, w->loff);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return hipStreamSynchronize(st);
 }
@@ -814,6 +855,7 @@ int check_cfg(const zav1_wf_cfg* c) {
       c->txt_satd_th < 0 || c->coeff_rate_est_lvl < 0 || c->coeff_rate_est_lvl > 2 || c->lambda == 0 ||
       (long)c->width * c->height * c->n_images > (1L << 31))
     return ZAV1_EINVAL;
+  if (c->keep_levels != 0 && c->keep_levels != 1) return ZAV1_EINVAL;
   if (tree) {
     if (c->width % 64 || c->height % 64 || c->txt_groups_lt16 < 1 || c->txt_groups_lt16 > 6) return ZAV1_EINVAL;
   } else if ((c->leaf != 8 && c->leaf != 16 && c->leaf != 32 && c->leaf != 64) || c->width % c->leaf ||
@@ -920,6 +962,10 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   w->dec = (zav1::WfDecision*)take(n * sizeof(zav1::WfDecision));
   w->mode_map = (uint8_t*)take(n);
   w->costs = (int32_t*)take(sizeof(zav1_coeff_costs));
+  if (c.keep_levels) {
+    w->levels = (int32_t*)take(w->n_levels * 4);
+    w->loff = (uint64_t*)take(n * 8);
+  }
   std::vector<int32_t> uid(n);
   for (size_t i = 0; i < n; i++) uid[i] = (int32_t)i;
   hipError_t e = hipSuccess;
@@ -930,6 +976,7 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   up(w->nbr, w->h_nbr.data(), n * 8);
   up(w->uid, uid.data(), n * 4);
   up(w->costs, &costs, sizeof(costs));
+  if (c.keep_levels) up(w->loff, w->h_loff.data(), n * 8);
   for (int k = 0; k < 4; k++) {
     zav1::SizePlan& P = w->sp[k];
     if (!P.max_u) continue;
@@ -961,6 +1008,7 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     P.ws1 = take(zav1::txunit_rd_workspace(P.tx, (int)(U * P.k1)));
     P.ws3_bytes = zav1::ws3_bytes(P.tx, (int)n3);
     P.ws3 = take(P.ws3_bytes);
+    if (c.keep_levels) P.q3 = (int32_t*)take(n3 * P.np * 4);
     std::vector<uint32_t> rtab(25 * (size_t)P.nc);
     for (int a = 0; a < 5; a++)
       for (int l = 0; l < 5; l++)
@@ -1005,6 +1053,9 @@ int zav1_wf_info(const zav1_wf* wf, zav1_wf_info_t* info) {
     }
   info->n_types = w->sp[k].nt;
   for (int i = 0; i < 16; i++) info->types[i] = w->sp[k].types[i];
+  info->d_levels = w->levels;
+  info->d_level_off = w->loff;
+  info->n_levels = w->n_levels;
   return ZAV1_OK;
 }
 
@@ -1091,6 +1142,48 @@ int zav1_wf_destroy(zav1_wf* wf) {
   if (w->base) (void)hipFree(w->base);
   delete w;
   return ZAV1_OK;
+}
+
+int zav1_wf_write_stills(zav1_wf* wf, int first, int n, int threads, uint8_t* out, size_t cap, size_t* sizes,
+                         zav1_stream_t s) {
+  if (!wf || !sizes || first < 0 || n < 1 || threads < 0) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  const zav1_wf_cfg& c = w->cfg;
+  if (!c.keep_levels || first + n > c.n_images) return ZAV1_EINVAL;
+  hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
+  const int nu = w->n_units;
+  std::vector<zav1::WfDecision> dec(nu);
+  WCHECK(hipMemcpyAsync(dec.data(), w->dec, sizeof(zav1::WfDecision) * nu, hipMemcpyDeviceToHost, st));
+  const uint64_t l0 = w->img_loff[first], l1 = w->img_loff[first + n];
+  std::vector<int32_t> lev(l1 - l0);
+  WCHECK(hipMemcpyAsync(lev.data(), w->levels + l0, 4 * (l1 - l0), hipMemcpyDeviceToHost, st));
+  WCHECK(hipStreamSynchronize(st));
+  std::vector<std::vector<zav1_still_leaf>> per(n);
+  for (int u = 0; u < nu; u++) {
+    const int img = w->h_units[u].y / c.height;
+    if (img < first || img >= first + n) continue;
+    const zav1::WfDecision& d = dec[u];
+    zav1_still_leaf l{};
+    l.x = w->h_units[u].x, l.y = w->h_units[u].y - img * c.height;
+    l.size = (uint8_t)(8 << w->h_sz[u]);
+    l.mode = d.mode, l.delta = c.cands[d.cand].delta, l.fi_mode = d.fi, l.tx_type = d.tx_type;
+    l.eob = d.eob, l.level_off = w->h_loff[u] - l0;
+    per[img - first].push_back(l);
+  }
+  size_t used = 0;
+  int rc = ZAV1_OK;
+  for (int i = 0; i < n; i++) {
+    zav1_still_desc sd{};
+    sd.width = c.width, sd.height = c.height, sd.bd = c.bd, sd.sb = c.sb;
+    sd.tile_cols = c.tile_cols, sd.tile_rows = c.tile_rows, sd.qindex = c.qindex;
+    sd.n_leaves = (int32_t)per[i].size(), sd.threads = threads;
+    sd.leaves = per[i].data(), sd.levels = lev.data();
+    const int r = zav1_write_still(&sd, out ? out + used : nullptr, out && cap > used ? cap - used : 0, &sizes[i]);
+    if (r == ZAV1_ENOMEM) rc = ZAV1_ENOMEM;
+    else if (r) return r;
+    used += sizes[i];
+  }
+  return rc;
 }
 
 const char* zav1_wf_last_error(void) { return g_wf_err.c_str(); }

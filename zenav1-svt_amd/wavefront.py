@@ -18,7 +18,7 @@ CFG = np.dtype([('width', '<i4'), ('height', '<i4'), ('n_images', '<i4'), ('bd',
                 ('sb', '<i4'), ('tile_cols', '<i4'), ('tile_rows', '<i4'), ('qindex', '<i4'), ('k1', '<i4'),
                 ('k3', '<i4'), ('rdoq_level', '<i4'), ('txt_groups', '<i4'), ('txt_rate_th', '<i4'),
                 ('txt_satd_th', '<i4'), ('coeff_rate_est_lvl', '<i4'), ('lambda', '<u8'), ('n_cands', '<i4'), ('pad', '<i4'),
-                ('cands', CAND, 64), ('txt_groups_lt16', '<i4'), ('pad2', '<i4'), ('leaf_map', '<u8')])
+                ('cands', CAND, 64), ('txt_groups_lt16', '<i4'), ('keep_levels', '<i4'), ('leaf_map', '<u8')])
 DECISION = np.dtype([('cand', 'u1'), ('tx_type', 'u1'), ('mode', 'u1'), ('fi_mode', 'u1'), ('eob', '<i4'),
                      ('rate', '<i4'), ('mode_rate', '<i4'), ('dist', '<u8'), ('cost', '<u8')])
 MD_UNIT = np.dtype([('x', '<i4'), ('y', '<i4'), ('n_top_px', '<i2'), ('n_topright_px', '<i2'),
@@ -30,7 +30,8 @@ class WfInfo(ctypes.Structure):
     _fields_ = [('d_src', ctypes.c_void_p), ('d_recon', ctypes.c_void_p), ('d_decisions', ctypes.c_void_p),
                 ('d_units', ctypes.c_void_p), ('n_leaves', ctypes.c_int32), ('n_steps', ctypes.c_int32),
                 ('max_leaves_per_step', ctypes.c_int32), ('n_types', ctypes.c_int32),
-                ('stacked_height', ctypes.c_int32), ('types', ctypes.c_uint8 * 16), ('bytes', ctypes.c_size_t)]
+                ('stacked_height', ctypes.c_int32), ('types', ctypes.c_uint8 * 16), ('bytes', ctypes.c_size_t),
+                ('d_levels', ctypes.c_void_p), ('d_level_off', ctypes.c_void_p), ('n_levels', ctypes.c_size_t)]
 
 
 def preset_candidates(fi=True):
@@ -257,6 +258,25 @@ class WavefrontMD:
         img = u['y'] // self.H
         grid[img, (u['y'] - img * self.H) // L, u['x'] // L] = d
         return grid
+
+    def write_stills(self, first=0, n=None, threads=0, stream=None):
+        """The AV1 streams of images [first, first + n) (a plan made with
+        keep_levels): a list of bytes objects."""
+        n = self.n - first if n is None else n
+        sizes = (ctypes.c_size_t * n)()
+        L = self.z.lib()
+        rc = L.zav1_wf_write_stills(self.h, first, n, threads, None, 0, sizes, stream)
+        if rc not in (0, self.z.ZAV1_ENOMEM):
+            raise self.z.Zav1Error(rc, 'zav1_wf_write_stills (%s)' % L.zav1_wf_last_error().decode())
+        out = np.zeros(max(1, sum(sizes)), np.uint8)
+        rc = L.zav1_wf_write_stills(self.h, first, n, threads, out.ctypes.data, out.size, sizes, stream)
+        if rc:
+            raise self.z.Zav1Error(rc, 'zav1_wf_write_stills (%s)' % L.zav1_wf_last_error().decode())
+        res, o = [], 0
+        for k in range(n):
+            res.append(out[o:o + sizes[k]].tobytes())
+            o += sizes[k]
+        return res
 
     def close(self):
         if self.h:
