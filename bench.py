@@ -586,12 +586,13 @@ def oracle_cflags():
     return None
 
 
-def wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args):
-    """The oracle's sequential restatement (or_wf_frame) of the same decision
-    on a bounded sample: 16 crops of 512x512 (the top-left 2048x2048 of the
-    first image), each decided as its own frame, one per thread, the
-    threads pinned to the first CPUs this process may use; one untimed
-    warm-up, then the median of 5 timed runs."""
+def wavefront_cpu_baseline(zav1, wf, bsm, synth, imgs, S, qidx, qp, lam, args):
+    """The same encode on the CPU over a bounded sample: 16 crops of 512x512
+    (the top-left 2048x2048 of the first image), each decided as its own
+    frame by the oracle's sequential restatement (or_wf_frame_levels) and
+    written by the host still writer (one tile, one thread), one crop per
+    thread, the threads pinned to the first CPUs this process may use; one
+    untimed warm-up, then the median of 5 timed runs."""
     from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle as O
@@ -604,7 +605,12 @@ def wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args):
     os.sched_setaffinity(0, set(cpus[:threads]))
     try:
         def one(c):
-            return O.wf_frame(cfg, c[None])
+            _, dec, lev = O.wf_frame(cfg, c[None], None, levels=True)
+            leaves, levels = bsm.leaves_uniform(dec[0], lev[0], 16, cfg[0]['cands'])
+            rc, stream = bsm.write_still(zav1, C, C, 10, 64, (1, 1), qidx, leaves, levels, threads=1)
+            if rc:
+                raise RuntimeError('zav1_write_still: %d' % rc)
+            return stream
         times = []
         with ThreadPoolExecutor(threads) as ex:
             for rep in range(6):
@@ -617,7 +623,8 @@ def wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args):
     med = float(np.median(times))
     return {'value': round(len(crops) * C * C / 1e6 / med, 4), 'unit': 'MP/s', 'cores': threads, 'kind': 'port',
             'sample': '%d crops of %dx%d (the top-left 2048x2048 of image 0), each decided as its own frame by '
-                      'the oracle C restatement, one crop per thread; 1 warm-up + median of 5' % (len(crops), C, C),
+                      'the oracle C restatement and written by the host still writer, one crop per thread; '
+                      '1 warm-up + median of 5' % (len(crops), C, C),
             'seconds_median': round(med, 3), 'seconds_all': [round(t, 3) for t in times], 'cpu_model': model,
             'pinned_cpus': cpus[:threads], 'compiler': 'gcc ' + (oracle_cflags() or '?')}
 
@@ -631,7 +638,9 @@ def wavefront_headline(zav1, synth, args):
     from concurrent.futures import ThreadPoolExecutor
     with ThreadPoolExecutor(min(16, len(os.sched_getaffinity(0)))) as ex:
         imgs = np.stack(list(ex.map(lambda i: synth.config2_luma(S, S, seed=2000 + i), range(n_img))))
-    cfg = wf.funnel_cfg(S, S, 10, qidx, lam, 4, qp, n_images=n_img, leaf=16)  # preset 4: k1 15, k3 4
+    bsm = _load('zav1_bitstream', os.path.join(PKG, 'bitstream.py'))
+    tiles = (bsm.legal_tile_cols(S, S, 64), 1)  # AV1's MAX_TILE_AREA: 2 tile columns at 4096 x 4096
+    cfg = wf.funnel_cfg(S, S, 10, qidx, lam, 4, qp, n_images=n_img, leaf=16, tiles=tiles)  # preset 4: k1 15, k3 4
     md = wf.WavefrontMD(zav1, cfg)
     md.upload(imgs)
     st = zav1.stream_create()
@@ -713,17 +722,24 @@ def wavefront_headline(zav1, synth, args):
     roof['avg_launch_us'] = round(dom_s * 1e6, 2)
     roof['launches'] = n_launch
     out = {
-        'metric': 'MP/s through the superblock-wavefront intra mode decision (luma), batched vs plain per-call GPU',
-        'value': round(value, 2), 'unit': 'MP/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': round(ms_step, 3), 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'metric': 'MP/s encoded (AV1 still streams of the luma plane: superblock-wavefront intra mode decision on '
+                  'the device, range coder on host threads, pipelined), batched vs plain per-call GPU',
+        'value': None, 'unit': 'MP/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'int32', 'data': 'synthetic',
-        'config': {'workload': 'wavefront MD: %d x %dx%d 10-bit luma in flight, qp 40, preset-4 funnel '
+        'config': {'workload': 'encode: %d x %dx%d 10-bit luma in flight, qp 40, preset-4 funnel '
                                '(62 candidates, 15 -> 4 survivors, 6 tx-type groups, RDOQ), 16x16 leaves, SB64, '
-                               '1 tile; %s' % (n_img, S, S, 'one HIP graph per batch' if use_graph else 'eager'),
+                               '%dx%d tiles (the fewest within AV1\'s tile-area limit); mode decision as %s, '
+                               'monochrome AV1 still streams written on host threads while the device decides the '
+                               'next batch' % (n_img, S, S, tiles[0], tiles[1],
+                                               'one HIP graph per batch' if use_graph else 'eager'),
                    'images_in_flight': n_img, 'width': S, 'height': S, 'bit_depth': 10, 'qp': qp, 'qindex': qidx,
                    'lambda': lam, 'leaf': 16, 'candidates': 62, 'k1': 15, 'k3': 4, 'tx_types': n_types,
+                   'tile_cols': tiles[0], 'tile_rows': tiles[1],
                    'wavefront_steps': info.n_steps, 'max_leaves_per_step': info.max_leaves_per_step,
                    'device_bytes': int(info.bytes)},
+        'md_only': {'value': round(value, 2), 'unit': 'MP/s', 'ms_per_step': round(ms_step, 3),
+                    'what': 'the mode decision alone (graph replays of one plan, no streams)'},
         'forms_identical': {'graph_vs_eager': same_forms, 'plain_sample_vs_batched': plain_same},
         'stage_ms': {k: round(v, 3) for k, v in stage_ms.items()},
         'eager_timed_pass_ms': round(sum(stage_ms.values()), 3),
@@ -735,10 +751,80 @@ def wavefront_headline(zav1, synth, args):
                       'batched_over_plain': round(value / plain_mp_s, 1)},
     }
     if not args.no_cpu:
-        out['cpu_baseline'] = wavefront_cpu_baseline(wf, synth, imgs, S, qidx, qp, lam, args)
+        out['cpu_baseline'] = wavefront_cpu_baseline(zav1, wf, bsm, synth, imgs, S, qidx, qp, lam, args)
     md.close()
     zav1.stream_destroy(st)
+    if args.no_encode:  # decision only: the headline is the mode decision's rate
+        out['metric'] = 'MP/s through the superblock-wavefront intra mode decision (luma), batched vs plain ' \
+                        'per-call GPU (no streams: --no-encode)'
+        out['value'], out['ms_per_step'] = round(value, 2), round(ms_step, 3)
+        return out
+    enc = encoded_stage(zav1, wf, imgs, cfg, args)
+    out['value'], out['ms_per_step'] = enc.pop('value'), enc.pop('ms_per_step')
+    out['encode'] = enc
     return out
+
+
+def encoded_stage(zav1, wf, imgs, cfg, args):
+    """The headline batch encoded: the same wavefront mode decision with the
+    winners' levels kept, then the AV1 still stream of every image written
+    on the host (zav1_wf_write_stills: the serial range coder per tile,
+    whole images on parallel host threads).  Two plans alternate so the
+    host writes batch k while the device decides batch k + 1 (the next
+    batch's graph waits on an event for the previous one, so the device
+    runs one batch at a time); a step is one batch decided and written.
+    The two plans' streams must be byte-identical, and every timed batch's
+    to the first."""
+    c = np.array(cfg, copy=True)
+    c[0]['keep_levels'] = 1
+    L = zav1.lib()
+    n_img, S = int(c[0]['n_images']), int(c[0]['width'])
+    threads = min(16, len(os.sched_getaffinity(0)))
+    plans = [wf.WavefrontMD(zav1, c) for _ in range(2)]
+    sts = [zav1.stream_create() for _ in range(2)]
+    evs = [zav1.event_create() for _ in range(2)]
+    for p, st in zip(plans, sts):
+        p.upload(imgs, st)
+        zav1.stream_sync(st)
+    ref = []
+    for p, st in zip(plans, sts):
+        for _ in range(max(1, args.warmup)):
+            p.run(True, st)
+            got = p.write_stills(threads=threads, stream=st)
+        ref.append(got)
+    nbytes = sum(len(x) for x in ref[0])
+    plans_identical = ref[0] == ref[1]
+    cap = 2 * nbytes + (1 << 20)
+    # host-only cost of one batch's read-back and writing (device idle)
+    tw0 = time.perf_counter()
+    plans[0].write_stills(threads=threads, stream=sts[0], cap=cap)
+    write_ms = (time.perf_counter() - tw0) * 1000.0
+    K = args.steps
+    repeat_identical = True
+    t0 = time.perf_counter()
+    plans[0].run(True, sts[0])
+    zav1.event_record(evs[0], sts[0])
+    for k in range(K):
+        p, q = k % 2, (k + 1) % 2
+        if k + 1 < K:
+            zav1.check(L.zav1_stream_wait_event(sts[q], evs[p]), 'stream_wait_event')
+            plans[q].run(True, sts[q])
+            zav1.event_record(evs[q], sts[q])
+        got = plans[p].write_stills(threads=threads, stream=sts[p], cap=cap)
+        repeat_identical &= got == ref[0]
+    t1 = time.perf_counter()
+    for p in plans:
+        p.close()
+    for st in sts:
+        zav1.stream_destroy(st)
+    for e in evs:
+        zav1.event_destroy(e)
+    ms = (t1 - t0) * 1000.0 / K
+    mp = n_img * S * S / 1e6
+    return {'value': round(mp / (ms / 1000.0), 2), 'ms_per_step': round(ms, 3), 'steps': K,
+            'host_threads': threads, 'host_write_ms_per_batch': round(write_ms, 2),
+            'stream_bytes_per_batch': nbytes, 'bits_per_pixel': round(8.0 * nbytes / (n_img * S * S), 4),
+            'streams_identical': {'two_plans': plans_identical, 'every_timed_batch': bool(repeat_identical)}}
 
 
 def txsearch_stage(zav1, synth, tsm, opsd, args):
@@ -808,6 +894,7 @@ def main():
                     help='one eager pass of the headline workload only (for rocprofv3 runs)')
     ap.add_argument('--plain-leaves', type=int, default=48)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-encode', action='store_true', help='skip the encoded (bitstream) leg')
     ap.add_argument('--extras', default='lf,rdoq,serving,config4',
                     help='comma list of lf, rdoq, txsearch, config4, serving')
     ap.add_argument('--copy-calibration', action='store_true')

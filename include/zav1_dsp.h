@@ -1076,7 +1076,11 @@ int zav1_write_still(const zav1_still_desc *d, uint8_t *out, size_t cap, size_t 
 /* The streams of images [first, first + n) of a wavefront batch run with
  * keep_levels, back to back in out; sizes[i] each.  The batch's tiles must
  * be powers of two and its sides multiples of the superblock.  Host-side
- * coding after the device results are read back (synchronous). */
+ * coding after the device results are read back on stream s (synchronous):
+ * whole images on `threads` host threads (0 = all cores), an image's tiles
+ * split among the threads left over when n < threads.  A cap short of the
+ * streams' total returns ZAV1_ENOMEM with every sizes[i] set (nothing
+ * written; call again with a larger buffer). */
 int zav1_wf_write_stills(zav1_wf *wf, int first, int n, int threads, uint8_t *out, size_t cap, size_t *sizes,
                          zav1_stream_t s);
 

@@ -14,7 +14,6 @@ oracle's reconstruction: the stream carries what the funnel decided, and
 the decoder's prediction and inverse transforms (the specification's)
 agree with the funnel's.  The same check over the device's decisions runs
 on the GPU in test_gpu_bitstream.py."""
-import ctypes
 import importlib.util
 import os
 import sys
@@ -22,16 +21,20 @@ import sys
 import numpy as np
 import pytest
 
-STILL_LEAF = np.dtype([('x', '<i4'), ('y', '<i4'), ('size', 'u1'), ('mode', 'u1'), ('delta', 'i1'), ('fi_mode', 'u1'),
-                       ('tx_type', 'u1'), ('pad', 'u1', 3), ('eob', '<i4'), ('pad2', '<i4'), ('level_off', '<u8')])
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-class StillDesc(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int32) for n in ('width', 'height', 'bd', 'sb', 'tile_cols', 'tile_rows', 'qindex',
-                                              'n_leaves', 'threads', 'pad')] + [('leaves', ctypes.c_void_p),
-                                                                                ('levels', ctypes.c_void_p)]
+def _mod(name):
+    path = os.path.join(ROOT, 'zenav1-svt_amd', name + '.py')
+    spec = importlib.util.spec_from_file_location('zav1_' + name, path)
+    m = importlib.util.module_from_spec(spec)
+    sys.modules['zav1_' + name] = m
+    spec.loader.exec_module(m)
+    return m
 
 
+bs = _mod('bitstream')
+STILL_LEAF = bs.STILL_LEAF
 SET7 = [0, 1, 2, 3, 9, 10, 11]
 SET5 = [0, 1, 2, 3, 9]
 
@@ -92,18 +95,7 @@ def random_still(oracle, rng, W, H, sb, big=False):
 
 
 def write(zav1, W, H, bd, sb, tiles, qidx, leaves, levels, threads=0):
-    d = StillDesc()
-    d.width, d.height, d.bd, d.sb = W, H, bd, sb
-    d.tile_cols, d.tile_rows, d.qindex, d.n_leaves, d.threads = tiles[0], tiles[1], qidx, len(leaves), threads
-    d.leaves, d.levels = leaves.ctypes.data, levels.ctypes.data
-    size = ctypes.c_size_t()
-    L = zav1.lib()
-    rc = L.zav1_write_still(ctypes.byref(d), None, 0, ctypes.byref(size))
-    if rc != zav1.ZAV1_ENOMEM:
-        return rc, None
-    out = np.zeros(size.value, np.uint8)
-    rc = L.zav1_write_still(ctypes.byref(d), out.ctypes.data, out.size, ctypes.byref(size))
-    return rc, out.tobytes()
+    return bs.write_still(zav1, W, H, bd, sb, tiles, qidx, leaves, levels, threads=threads)
 
 
 @pytest.mark.parametrize('W,H,bd,sb,tiles,qidx,seed', [
@@ -157,21 +149,10 @@ def test_writer_refuses_bad_input(zav1, oracle):
     assert write(zav1, 128, 128, 8, 64, (1, 1), 100, loose, lv2)[0] == zav1.ZAV1_EINVAL       # eob not tight
 
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-
-
-def _mod(name):
-    path = os.path.join(ROOT, 'zenav1-svt_amd', name + '.py')
-    spec = importlib.util.spec_from_file_location('zav1_' + name, path)
-    m = importlib.util.module_from_spec(spec)
-    sys.modules['zav1_' + name] = m
-    spec.loader.exec_module(m)
-    return m
-
-
-def leaves_of_decisions(dec, lev, cells, leaf=None, leaf_map=None):
+def leaves_of_decisions(dec, lev, cells, leaf=None, leaf_map=None, cands=None):
     """Still leaves (raster order of their top-left cells, not coding order)
-    from one image's oracle decisions and level plane."""
+    from one image's oracle decisions and level plane; the angle delta is the
+    winning candidate's (cands: the configuration's candidate records)."""
     H, W = lev.shape
     out, chunks, off = [], [], 0
     if leaf_map is not None:
@@ -188,7 +169,8 @@ def leaves_of_decisions(dec, lev, cells, leaf=None, leaf_map=None):
             s = leaf
         p = min(s, 32)
         q = np.ascontiguousarray(lev[y:y + p, x:x + p]).ravel()
-        out.append((x, y, s, d['mode'], 0, d['fi_mode'], d['tx_type'], (0, 0, 0), d['eob'], 0, off))
+        delta = int(cands[d['cand']]['delta'])
+        out.append((x, y, s, d['mode'], delta, d['fi_mode'], d['tx_type'], (0, 0, 0), d['eob'], 0, off))
         chunks.append(q)
         off += q.size
     arr = np.zeros(len(out), STILL_LEAF)
@@ -197,7 +179,7 @@ def leaves_of_decisions(dec, lev, cells, leaf=None, leaf_map=None):
     return arr, np.concatenate(chunks)
 
 
-@pytest.mark.parametrize('case', ['uniform16_10bit_tiles', 'map_8bit', 'uniform32_sb128'])
+@pytest.mark.parametrize('case', ['uniform16_10bit_tiles', 'map_8bit', 'uniform32_sb128', 'preset4_angles'])
 def test_oracle_decisions_decode_to_the_reconstruction(zav1, oracle, case):
     wf = _mod('wavefront')
     synth = _mod('synth')
@@ -207,6 +189,9 @@ def test_oracle_decisions_decode_to_the_reconstruction(zav1, oracle, case):
     elif case == 'uniform32_sb128':
         W, H, bd, qp, preset, sb, tiles, leaf = 256, 128, 8, 28, 8, 128, (1, 1), 32
         img = synth.config1_luma(H, W, seed=1300)
+    elif case == 'preset4_angles':
+        W, H, bd, qp, preset, sb, tiles, leaf = 128, 128, 10, 28, 4, 64, (1, 1), 8
+        img = synth.config2_luma(H, W, seed=2301)
     else:
         W, H, bd, qp, preset, sb, tiles, leaf = 128, 128, 8, 32, 8, 64, (1, 1), 16
         img = synth.config1_luma(H, W, seed=1301)
@@ -221,10 +206,15 @@ def test_oracle_decisions_decode_to_the_reconstruction(zav1, oracle, case):
     else:
         cfg = wf.funnel_cfg(W, H, bd, qidx, lam, preset, qp, sb=sb, tiles=tiles, leaf=leaf)
     rec, dec, lev = oracle.wf_frame(cfg, img[None], lm, levels=True)
-    leaves, levels = leaves_of_decisions(dec[0], lev[0], lm is not None, leaf, lm)
+    leaves, levels = leaves_of_decisions(dec[0], lev[0], lm is not None, leaf, lm, cfg[0]['cands'])
     # the tight eob of each leaf's levels is its decided eob
     rc, stream = write(zav1, W, H, bd, sb, tiles, qidx, leaves, levels)
     assert rc == 0, rc
     frame, info, blocks, _ = oracle.av1_decode_still(stream, W, H, bd)
     assert info.n_blocks == len(leaves)
     assert np.array_equal(frame, rec[0]), 'decoded frame differs from the funnel reconstruction'
+    if lm is None:  # the package's vectorised helper builds the same stream
+        ul, uv = bs.leaves_uniform(dec[0], lev[0], leaf, cfg[0]['cands'])
+        assert write(zav1, W, H, bd, sb, tiles, qidx, ul, uv) == (0, stream)
+    if case == 'preset4_angles':  # nonzero angle deltas and filter intra were decided and coded
+        assert (leaves['delta'] != 0).any() and (leaves['fi_mode'] < 5).any()

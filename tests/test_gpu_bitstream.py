@@ -3,9 +3,11 @@ zav1_wf_write_stills (host AV1 writer) -> the oracle's decoder written from
 the specification.  The decoded frame must equal the device reconstruction
 sample for sample and the decoded syntax the device decisions (mode, filter
 intra, tx type, eob per leaf): the stream is a valid AV1 monochrome still of
-exactly what the mode decision chose.  Uniform 16x16 leaves at 10 bits
-(preset 4 funnel, 2 images), the PD0 trees at preset 8 (config 1), SB128
-with 2x2 tiles at preset 6."""
+exactly what the mode decision chose.  And the streams are byte-identical
+to the ones the same writer makes from the oracle's sequential mode
+decision (or_wf_frame_levels) of the same images.  Uniform 16x16 leaves at
+10 bits (preset 4 funnel, 2 images), the PD0 trees at preset 8 (config 1),
+SB128 with 2x2 tiles at preset 6 and 32x32 leaves."""
 import importlib.util
 import os
 import sys
@@ -26,9 +28,26 @@ def _mod(name):
     return m
 
 
+import test_bitstream as tb  # noqa: E402
+
 wf = _mod('wavefront')
 part = _mod('partition')
 synth = _mod('synth')
+
+
+def oracle_streams(z, O, cfg, images, leaf_map):
+    c = cfg[0]
+    W, H, bd, sb, leaf = int(c['width']), int(c['height']), int(c['bd']), int(c['sb']), int(c['leaf'])
+    tiles = (int(c['tile_cols']), int(c['tile_rows']))
+    _, dec, lev = O.wf_frame(cfg, np.stack(images), leaf_map, levels=True)
+    out = []
+    for i in range(len(images)):
+        lm = None if leaf_map is None else leaf_map[i * (W // 64) * (H // 64) * 256:(i + 1) * (W // 64) * (H // 64) * 256]
+        leaves, levels = tb.leaves_of_decisions(dec[i], lev[i], leaf_map is not None, leaf, lm, c['cands'])
+        rc, s = tb.write(z, W, H, bd, sb, tiles, int(c['qindex']), leaves, levels)
+        assert rc == 0, rc
+        out.append(s)
+    return out
 
 
 def check_streams(z, O, md, images, bd):
@@ -61,8 +80,16 @@ def test_uniform_leaves_10bit(gpu, oracle):
     cfg[0]['keep_levels'] = 1
     md = wf.WavefrontMD(gpu, cfg)
     md.upload(np.stack(imgs))
-    check_streams(gpu, oracle, md, imgs, 10)
+    streams = check_streams(gpu, oracle, md, imgs, 10)
+    # the plain per-call form keeps the same levels: cleared, every leaf
+    # redone one call at a time, the streams come back byte-identical
+    gpu.check(gpu.lib().zav1_memset(md.info.d_levels, 0, 4 * md.info.n_levels, None), 'memset')
+    md.set_raw_decisions(np.zeros_like(md.raw_decisions()))
+    md.run_plain(0, md.info.n_leaves)
+    gpu.sync()
+    assert md.write_stills(0, 2, threads=2) == streams
     md.close()
+    assert streams == oracle_streams(gpu, oracle, cfg, imgs, None)
 
 
 def test_pd0_tree_config1(gpu, oracle):
@@ -82,6 +109,7 @@ def test_pd0_tree_config1(gpu, oracle):
     md.upload(img[None])
     streams = check_streams(gpu, oracle, md, [img], 8)
     md.close()
+    assert streams == oracle_streams(gpu, oracle, cfg, [img], lm)
     # every leaf size of the tree appears in the stream
     _, _, blocks, _ = oracle.av1_decode_still(streams[0], 512, 512, 8)
     assert len(set(int(s) for s in blocks['size'])) >= 3
@@ -95,5 +123,6 @@ def test_sb128_tiles(gpu, oracle):
     cfg[0]['keep_levels'] = 1
     md = wf.WavefrontMD(gpu, cfg)
     md.upload(img[None])
-    check_streams(gpu, oracle, md, [img], 10)
+    streams = check_streams(gpu, oracle, md, [img], 10)
     md.close()
+    assert streams == oracle_streams(gpu, oracle, cfg, [img], None)

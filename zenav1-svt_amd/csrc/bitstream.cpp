@@ -558,9 +558,13 @@ std::vector<uint8_t> frame_obu(const Frame& f, const std::vector<std::vector<uin
 
 }  // namespace
 
-extern "C" int zav1_write_still(const zav1_still_desc* d, uint8_t* out, size_t cap, size_t* size) {
+namespace zav1 {
+
+// One still into `stream` (ZAV1_OK) or an error code; shared by
+// zav1_write_still and the wavefront's batch writer.
+int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream) {
   static_assert(sizeof(zav1_still_leaf) == 32, "still leaf layout");
-  if (!d || !size || (d->bd != 8 && d->bd != 10) || (d->sb != 64 && d->sb != 128) || d->width <= 0 ||
+  if (!d || (d->bd != 8 && d->bd != 10) || (d->sb != 64 && d->sb != 128) || d->width <= 0 ||
       d->height <= 0 || d->width % d->sb || d->height % d->sb || d->qindex < 1 || d->qindex > 255 ||
       d->n_leaves < 1 || !d->leaves || !d->levels || d->tile_cols < 1 || d->tile_rows < 1 ||
       (d->tile_cols & (d->tile_cols - 1)) || (d->tile_rows & (d->tile_rows - 1)) || d->width > 65536 ||
@@ -636,10 +640,20 @@ extern "C" int zav1_write_still(const zav1_still_desc* d, uint8_t* out, size_t c
       });
     for (auto& th : pool) th.join();
   }
-  std::vector<uint8_t> stream;
+  stream.clear();
   obu(stream, 2, {});  // temporal delimiter
   obu(stream, 1, sequence_header(f));
   obu(stream, 6, frame_obu(f, tiles));
+  return ZAV1_OK;
+}
+
+}  // namespace zav1
+
+extern "C" int zav1_write_still(const zav1_still_desc* d, uint8_t* out, size_t cap, size_t* size) {
+  if (!size) return ZAV1_EINVAL;
+  std::vector<uint8_t> stream;
+  const int rc = zav1::write_still_vec(d, stream);
+  if (rc) return rc;
   *size = stream.size();
   if (!out || cap < stream.size()) return ZAV1_ENOMEM;
   std::memcpy(out, stream.data(), stream.size());

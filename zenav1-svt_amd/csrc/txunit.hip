@@ -365,6 +365,9 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// the levels are the workspace's first carve (txunit_rd_batch)
+int32_t* txunit_rd_levels(void* work) { return static_cast<int32_t*>(work); }
+
 size_t txunit_rd_workspace(int tx_size, int n_jobs) {
   if (tx_size < 0 || tx_size > 18 || n_jobs <= 0) return 0;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
@@ -408,7 +411,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
     return p;
   };
   TuKeep keep;
-  int32_t* tq = reinterpret_cast<int32_t*>(carve(n * np * 4));
+  int32_t* tq = reinterpret_cast<int32_t*>(carve(n * np * 4));  // first: txunit_rd_levels
   keep.t = reinterpret_cast<int32_t*>(carve(n * np * 4));
   keep.dq = reinterpret_cast<int32_t*>(carve(n * np * 4));
   keep.eob16 = reinterpret_cast<uint16_t*>(carve(n * 2));

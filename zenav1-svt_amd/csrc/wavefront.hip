@@ -30,11 +30,17 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/zav1_dsp.h"
 #include "zav1_dev.h"
 #include "zav1_kernels.h"
+
+namespace zav1 {
+// the host AV1 still writer (bitstream.cpp): one still into `stream`
+int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream);
+}  // namespace zav1
 
 namespace zav1 {
 namespace {
@@ -289,7 +295,7 @@ struct SizePlan {
   TxUnitRdJob *jobs1 = nullptr, *jobs3 = nullptr;
   uint64_t *dist1 = nullptr, *dist3 = nullptr, *csatd3 = nullptr;
   int32_t *eob1 = nullptr, *rate1 = nullptr, *eob3 = nullptr, *rate3 = nullptr;
-  int32_t* q3 = nullptr;  // MDS3 levels per job (kept levels only)
+  int32_t* q3 = nullptr;  // the plain form's MDS3 levels of one leaf (kept levels only)
   uint8_t *cul1 = nullptr, *cul3 = nullptr;
   void *ws1 = nullptr, *ws3 = nullptr;
   size_t ws3_bytes = 0;  // MDS3's workspace, with room for the trellis's lane form
@@ -328,6 +334,10 @@ struct Wavefront {
   uint64_t* loff = nullptr;
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
+  // pinned host staging of the decisions and kept levels (the batch
+  // writer's read-back), allocated on its first call
+  WfDecision* h_dec_pin = nullptr;
+  int32_t* h_lev_pin = nullptr;
 };
 
 namespace {
@@ -598,7 +608,7 @@ static size_t size_bytes(const SizePlan& p, bool keep) {
        al(U * p.k1);
   b += al(n3 * sizeof(TxUnitRdJob)) + 2 * al(n3 * 8) + 2 * al(n3 * 4) + al(n3) + al(n3 * LL * 2);
   b += al(txunit_rd_workspace(p.tx, (int)(U * p.k1))) + al(ws3_bytes(p.tx, (int)n3)) + al(64);
-  if (keep) b += al(n3 * p.np * 4);
+  if (keep) b += al((size_t)p.k3 * p.nt * p.np * 4);
   return b;
 }
 
@@ -664,19 +674,21 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   MARK(7);
   e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs3, n3, w->row, w->iscan,
                       P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3, P.eob3, P.rate3, P.cul3,
-                      P.q3, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes);
+                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes);
   if (e != hipSuccess) return e;
+  // the commit copies the winners' levels straight from the batch's workspace
+  const int32_t* q3 = c.keep_levels ? txunit_rd_levels(P.ws3) : nullptr;
   MARK(10);
   if (c.bd == 8)
     hipLaunchKernelGGL((k_wf_commit<uint8_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, P.m3, P.sel,
                        P.k1, P.k3, P.cands, P.rate, P.nc, P.d_types, P.nt, P.txt, P.dist3, P.rate3, P.eob3, P.csatd3,
-                       P.rec3, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W, L, w->dec, w->mode_map, P.q3,
+                       P.rec3, c.lambda, w->rate_th, P.satd_th, (uint8_t*)w->recon, W, L, w->dec, w->mode_map, q3,
                        w->levels, w->loff);
   else
     hipLaunchKernelGGL((k_wf_commit<uint16_t>), dim3((unsigned)U), dim3(256), 0, st, units, U, w->uid + u0, P.m3,
                        P.sel, P.k1, P.k3, P.cands, P.rate, P.nc, P.d_types, P.nt, P.txt, P.dist3, P.rate3, P.eob3,
                        P.csatd3, P.rec3, c.lambda, w->rate_th, P.satd_th, (uint16_t*)w->recon, W, L, w->dec,
-                       w->mode_map, P.q3, w->levels, w->loff);
+                       w->mode_map, q3, w->levels, w->loff);
   MARK(kWfStages);
   return hipGetLastError();
 }
@@ -1008,7 +1020,7 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     P.ws1 = take(zav1::txunit_rd_workspace(P.tx, (int)(U * P.k1)));
     P.ws3_bytes = zav1::ws3_bytes(P.tx, (int)n3);
     P.ws3 = take(P.ws3_bytes);
-    if (c.keep_levels) P.q3 = (int32_t*)take(n3 * P.np * 4);
+    if (c.keep_levels) P.q3 = (int32_t*)take((size_t)P.k3 * P.nt * P.np * 4);
     std::vector<uint32_t> rtab(25 * (size_t)P.nc);
     for (int a = 0; a < 5; a++)
       for (int l = 0; l < 5; l++)
@@ -1140,6 +1152,8 @@ int zav1_wf_destroy(zav1_wf* wf) {
   auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
   if (w->graph) (void)hipGraphExecDestroy(w->graph);
   if (w->base) (void)hipFree(w->base);
+  if (w->h_dec_pin) (void)hipHostFree(w->h_dec_pin);
+  if (w->h_lev_pin) (void)hipHostFree(w->h_lev_pin);
   delete w;
   return ZAV1_OK;
 }
@@ -1152,11 +1166,24 @@ int zav1_wf_write_stills(zav1_wf* wf, int first, int n, int threads, uint8_t* ou
   if (!c.keep_levels || first + n > c.n_images) return ZAV1_EINVAL;
   hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
   const int nu = w->n_units;
-  std::vector<zav1::WfDecision> dec(nu);
-  WCHECK(hipMemcpyAsync(dec.data(), w->dec, sizeof(zav1::WfDecision) * nu, hipMemcpyDeviceToHost, st));
+  if (!w->h_dec_pin) {
+    void *pd = nullptr, *pl = nullptr;
+    if (hipHostMalloc(&pd, sizeof(zav1::WfDecision) * nu, hipHostMallocDefault) != hipSuccess) {
+      g_wf_err = "zav1_wf_write_stills: pinned staging of the decisions";
+      return ZAV1_ENOMEM;
+    }
+    if (hipHostMalloc(&pl, 4 * std::max<size_t>(1, w->n_levels), hipHostMallocDefault) != hipSuccess) {
+      (void)hipHostFree(pd);
+      g_wf_err = "zav1_wf_write_stills: pinned staging of the levels";
+      return ZAV1_ENOMEM;
+    }
+    w->h_dec_pin = (zav1::WfDecision*)pd, w->h_lev_pin = (int32_t*)pl;
+  }
+  const zav1::WfDecision* dec = w->h_dec_pin;
+  WCHECK(hipMemcpyAsync(w->h_dec_pin, w->dec, sizeof(zav1::WfDecision) * nu, hipMemcpyDeviceToHost, st));
   const uint64_t l0 = w->img_loff[first], l1 = w->img_loff[first + n];
-  std::vector<int32_t> lev(l1 - l0);
-  WCHECK(hipMemcpyAsync(lev.data(), w->levels + l0, 4 * (l1 - l0), hipMemcpyDeviceToHost, st));
+  const int32_t* lev = w->h_lev_pin + l0;
+  WCHECK(hipMemcpyAsync(w->h_lev_pin + l0, w->levels + l0, 4 * (l1 - l0), hipMemcpyDeviceToHost, st));
   WCHECK(hipStreamSynchronize(st));
   std::vector<std::vector<zav1_still_leaf>> per(n);
   for (int u = 0; u < nu; u++) {
@@ -1170,20 +1197,44 @@ int zav1_wf_write_stills(zav1_wf* wf, int first, int n, int threads, uint8_t* ou
     l.eob = d.eob, l.level_off = w->h_loff[u] - l0;
     per[img - first].push_back(l);
   }
-  size_t used = 0;
-  int rc = ZAV1_OK;
-  for (int i = 0; i < n; i++) {
+  // the images' streams on `threads` host threads (0: the host's
+  // concurrency): whole images per thread, the tiles of an image split
+  // among the threads left over when there are fewer images than threads
+  int nth = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  const int img_threads = std::min(nth, n), tile_threads = std::max(1, nth / img_threads);
+  std::vector<std::vector<uint8_t>> streams(n);
+  std::vector<int> rcs(n, ZAV1_OK);
+  auto one = [&](int i) {
     zav1_still_desc sd{};
     sd.width = c.width, sd.height = c.height, sd.bd = c.bd, sd.sb = c.sb;
     sd.tile_cols = c.tile_cols, sd.tile_rows = c.tile_rows, sd.qindex = c.qindex;
-    sd.n_leaves = (int32_t)per[i].size(), sd.threads = threads;
-    sd.leaves = per[i].data(), sd.levels = lev.data();
-    const int r = zav1_write_still(&sd, out ? out + used : nullptr, out && cap > used ? cap - used : 0, &sizes[i]);
-    if (r == ZAV1_ENOMEM) rc = ZAV1_ENOMEM;
-    else if (r) return r;
+    sd.n_leaves = (int32_t)per[i].size(), sd.threads = tile_threads;
+    sd.leaves = per[i].data(), sd.levels = lev;
+    rcs[i] = zav1::write_still_vec(&sd, streams[i]);
+  };
+  if (img_threads <= 1) {
+    for (int i = 0; i < n; i++) one(i);
+  } else {
+    std::vector<std::thread> pool;
+    for (int k = 0; k < img_threads; k++)
+      pool.emplace_back([&, k] {
+        for (int i = k; i < n; i += img_threads) one(i);
+      });
+    for (auto& th : pool) th.join();
+  }
+  size_t total = 0;
+  for (int i = 0; i < n; i++) {
+    if (rcs[i]) return rcs[i];
+    sizes[i] = streams[i].size();
+    total += sizes[i];
+  }
+  if (!out || cap < total) return ZAV1_ENOMEM;
+  size_t used = 0;
+  for (int i = 0; i < n; i++) {
+    std::memcpy(out + used, streams[i].data(), sizes[i]);
     used += sizes[i];
   }
-  return rc;
+  return ZAV1_OK;
 }
 
 const char* zav1_wf_last_error(void) { return g_wf_err.c_str(); }

@@ -340,6 +340,10 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
                            hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr,
                            hipEvent_t* rdoq_ev = nullptr, size_t work_bytes = 0);
+// The final levels of a txunit_rd_batch call stay in its workspace until the
+// workspace is used again: job j's packed levels at txunit_rd_levels(work) +
+// j * min(w, 32) * min(h, 32).
+int32_t* txunit_rd_levels(void* work);
 // The current device's quantizer row `row_index` (bd index * 256 + qindex),
 // inverse scans and forward-scan base, as built by zav1_init (no building).
 int device_tables(const QuantRow** row, const uint16_t** iscan, const uint16_t** fscan_base, int row_index);

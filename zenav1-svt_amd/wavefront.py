@@ -259,17 +259,20 @@ class WavefrontMD:
         grid[img, (u['y'] - img * self.H) // L, u['x'] // L] = d
         return grid
 
-    def write_stills(self, first=0, n=None, threads=0, stream=None):
-        """The AV1 streams of images [first, first + n) (a plan made with
-        keep_levels): a list of bytes objects."""
+    def write_stills(self, first=0, n=None, threads=0, stream=None, cap=None):
+        """The AV1 streams of images [first, first + n) of a plan made with
+        keep_levels, written on `threads` host threads (0: all): a list of
+        bytes objects.  `cap` is the first output buffer's size (default half
+        a byte per sample); a short buffer costs a second write."""
         n = self.n - first if n is None else n
         sizes = (ctypes.c_size_t * n)()
         L = self.z.lib()
-        rc = L.zav1_wf_write_stills(self.h, first, n, threads, None, 0, sizes, stream)
-        if rc not in (0, self.z.ZAV1_ENOMEM):
-            raise self.z.Zav1Error(rc, 'zav1_wf_write_stills (%s)' % L.zav1_wf_last_error().decode())
-        out = np.zeros(max(1, sum(sizes)), np.uint8)
+        cap = max(1 << 16, n * self.W * self.H // 2) if cap is None else cap
+        out = np.empty(cap, np.uint8)
         rc = L.zav1_wf_write_stills(self.h, first, n, threads, out.ctypes.data, out.size, sizes, stream)
+        if rc == self.z.ZAV1_ENOMEM:
+            out = np.empty(max(1, sum(sizes)), np.uint8)
+            rc = L.zav1_wf_write_stills(self.h, first, n, threads, out.ctypes.data, out.size, sizes, stream)
         if rc:
             raise self.z.Zav1Error(rc, 'zav1_wf_write_stills (%s)' % L.zav1_wf_last_error().decode())
         res, o = [], 0
