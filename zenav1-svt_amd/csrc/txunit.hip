@@ -363,6 +363,21 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
   out_cul[job] = (uint8_t)cl;
 }
 
+// The pre-trellis SATD gate (TuSatdGate): per job against its group's first.
+__global__ __launch_bounds__(256) void k_tu_satd_gate(int n_jobs, int group, const uint64_t* __restrict__ csatd,
+                                                      int64_t th, uint16_t* __restrict__ eob16,
+                                                      int32_t* __restrict__ eob) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n_jobs) return;
+  const int t = j % group;
+  if (t == 0 || eob[j] < 0) return;
+  const int64_t s0 = (int64_t)csatd[j - t], sd = (int64_t)csatd[j];
+  if ((sd - s0) * 100 > s0 * th) {
+    eob16[j] = 0;
+    eob[j] = kTuGated;
+  }
+}
+
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // the levels are the workspace's first carve (txunit_rd_batch)
@@ -400,7 +415,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
                            hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd, hipEvent_t* rdoq_ev,
-                           size_t work_bytes) {
+                           size_t work_bytes, const TuSatdGate* gate) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
@@ -430,6 +445,9 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   hipError_t e = txunit_batch(src, stride, plane_w, plane_h, bd, tx_size, pred, pred_len, tj, n_jobs, row, iscan,
                               flags & kTxuQuantB, dist, eob, cul, tq, nullptr, st, keep);
   if (e != hipSuccess) return e;
+  if (gate && csatd && gate->group > 1 && gate->th > 0)
+    hipLaunchKernelGGL(k_tu_satd_gate, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, n_jobs, gate->group,
+                       csatd, gate->th, keep.eob16, eob);
   // rdoq_ev: two events recorded around the trellis (timing; recorded
   // also when the flags skip it)
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;

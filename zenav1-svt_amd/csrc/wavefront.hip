@@ -672,9 +672,13 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
                      P.nc, P.dist1, P.rate1, c.lambda, P.d_types, P.nt, P.txt, L, w->rdmult, P.m3, P.jobs3);
   const int n3 = U * P.k3 * P.nt;
   MARK(7);
+  // types the commit's coefficient-SATD gate skips whatever the rate gate
+  // does (more than satd_th percent above the DCT's: the running minimum it
+  // compares against is at most the DCT's) skip the trellis and the rest
+  const TuSatdGate gate{P.nt, P.satd_th};
   e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs3, n3, w->row, w->iscan,
                       P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3, P.eob3, P.rate3, P.cul3,
-                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes);
+                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes, &gate);
   if (e != hipSuccess) return e;
   // the commit copies the winners' levels straight from the batch's workspace
   const int32_t* q3 = c.keep_levels ? txunit_rd_levels(P.ws3) : nullptr;

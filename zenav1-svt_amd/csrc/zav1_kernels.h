@@ -328,6 +328,15 @@ hipError_t txunit_batch(const void* src, int stride, int plane_w, int plane_h, i
 // txunit_rd_workspace(tx_size, n_jobs) bytes; scans are tx_size's three
 // forward scans, costs the frame's coefficient cost tables.
 size_t txunit_rd_workspace(int tx_size, int n_jobs);
+// A pre-trellis gate over jobs in groups of `group` (one candidate's tx
+// types in search order, the reference type first): a job whose coefficient
+// SATD exceeds its group's first by more than `th` percent gets no trellis,
+// rate, reconstruction or distortion (its eob becomes kTuGated).
+struct TuSatdGate {
+  int group;
+  int64_t th;
+};
+constexpr int32_t kTuGated = -3;
 // The fixed per-device scratch of the no-allocation entry points
 // (noalloc.hip): set up once per device with that device current.
 hipError_t noalloc_setup(int dev);
@@ -339,7 +348,8 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
                            hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr,
-                           hipEvent_t* rdoq_ev = nullptr, size_t work_bytes = 0);
+                           hipEvent_t* rdoq_ev = nullptr, size_t work_bytes = 0,
+                           const TuSatdGate* gate = nullptr);
 // The final levels of a txunit_rd_batch call stay in its workspace until the
 // workspace is used again: job j's packed levels at txunit_rd_levels(work) +
 // j * min(w, 32) * min(h, 32).
