@@ -127,15 +127,47 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
   __syncthreads();
 
   // ---- forward row pass + fold + quantize: lane t < H owns row t ----
-  // The row's dequantized values replace it in LDS (0 where folded) as the
-  // inverse transform's input.
+  // The arrays the later kernels read (coefficients, levels, dequantized
+  // values; packed PW x PH per job) leave through the job's LDS block, so
+  // each store instruction writes a contiguous run of the job's array
+  // instead of one word per row; the dequantized values stay in LDS (0
+  // where folded) as the inverse transform's input.
   uint64_t fdist = 0, fold = 0, csatd = 0;
   int32_t eob = 0, lsum = 0, dcq = 0;
-  if (active && t < H) {
-    int32_t v[W];
+  int32_t v[W];
+  const bool row_lane = active && t < H;
+  if (row_lane) {
 #pragma unroll
     for (int c = 0; c < W; c++) v[c] = blk[t * G::LS + c];
     fwd1d<W, CBR, kBRow>(v, cfg.row);
+#pragma unroll
+    for (int c = 0; c < W; c++) {
+      int32_t x = rshift_round32(v[c], FS.s2);
+      // 2:1 rectangles: x NewSqrt2 / 2^12; |x| stays below 2^18 (the pinned
+      // coefficient bound before this scaling), so the product fits 32 bits
+      if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
+      v[c] = x;
+    }
+  }
+  const long pbase = (long)job * (G::PW * G::PH);
+  // packed element e of the job: lane t moves e = t, t + T, ...
+  auto store_packed = [&](int32_t* dst) {
+    __syncthreads();
+    if (active) {
+      for (int e = t; e < G::PW * G::PH; e += G::T) dst[pbase + e] = blk[(e / G::PW) * G::LS + e % G::PW];
+    }
+    __syncthreads();
+  };
+  if (keep.t) {
+    if (row_lane) {
+#pragma unroll
+      for (int c = 0; c < G::PW; c++)
+        if (t < G::PH) blk[t * G::LS + c] = v[c];
+    }
+    store_packed(keep.t);
+  }
+  int32_t dk[G::PW];
+  if (row_lane) {
     const QuantRow& qr = *row_p;
     const int cls = (jb.tx_type >= 10) ? (((jb.tx_type - 10) & 1) ? 2 : 1) : 0;
     const uint16_t* iscan = iscan_all + (tx_size * 3 + cls) * 1024 + (t < 32 ? t : 0) * G::PW;
@@ -145,16 +177,12 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
     const bool use_b = flags & kTxuQuantB;
 #pragma unroll
     for (int c = 0; c < W; c++) {
-      int32_t x = rshift_round32(v[c], FS.s2);
-      // 2:1 rectangles: x NewSqrt2 / 2^12; |x| stays below 2^18 (the pinned
-      // coefficient bound before this scaling), so the product fits 32 bits
-      if constexpr (G::RECT2) x = (__mul24(x, 5793) + 2048) >> 12;
+      const int32_t x = v[c];
       if (keep.csatd) csatd += (uint64_t)(x < 0 ? -(int64_t)x : (int64_t)x);
       if (t >= 32 || c >= 32) {
         const uint64_t e2 = (uint64_t)((int64_t)x * x);  // energy of the folded-away coefficients
         fdist += e2;
         fold += e2;
-        blk[t * G::LS + c] = 0;
         continue;
       }
       const int iz = (t | c) != 0;
@@ -171,15 +199,17 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       if (t == 0 && c == 0) dcq = qs;
       const int64_t e = (int64_t)x - dq;
       fdist += (uint64_t)e * (uint64_t)e;
-      blk[t * G::LS + c] = dq;
-      const long pi = (long)job * (G::PW * G::PH) + t * G::PW + c;
-      if (out_q) out_q[pi] = qs;
-      if (keep.t) {
-        keep.t[pi] = x;
-        keep.dq[pi] = dq;
-      }
+      dk[c] = dq;
+      blk[t * G::LS + c] = qs;  // levels first through the block
     }
   }
+  if (out_q) store_packed(out_q);
+  else __syncthreads();
+  if (row_lane) {
+#pragma unroll
+    for (int c = 0; c < W; c++) blk[t * G::LS + c] = (t >= 32 || c >= 32) ? 0 : dk[c < G::PW ? c : 0];
+  }
+  if (keep.t) store_packed(keep.dq);
   fdist = group_sum_u64<G::T>(fdist);
   if constexpr (W == 64 || H == 64) fold = group_sum_u64<G::T>(fold);
   if (keep.csatd) csatd = group_sum_u64<G::T>(csatd);
@@ -316,20 +346,23 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
   const long base = (long)job * NP;
   uint64_t fd = 0;
   int32_t lsum = 0;
-  if (active && t < H) {
-#pragma unroll
-    for (int c = 0; c < W; c++) {
-      int32_t d = 0;
-      if (t < G::PH && c < G::PW) {
-        const long pi = base + t * G::PW + c;
-        d = tdq[pi];
-        const int64_t e = (int64_t)tt[pi] - d;
-        fd += (uint64_t)(e * e);
-        const int32_t q = tq[pi];
-        lsum += q < 0 ? -q : q;
-        if (out_q) out_q[pi] = q;
+  // the job's packed arrays read as contiguous runs (lane t: elements t,
+  // t + T, ...), the dequantized block laid out in LDS for the inverse
+  if (active) {
+    for (int e = t; e < NP; e += G::T) {
+      const int32_t d = tdq[base + e];
+      const int64_t df = (int64_t)tt[base + e] - d;
+      fd += (uint64_t)(df * df);
+      const int32_t q = tq[base + e];
+      lsum += q < 0 ? -q : q;
+      if (out_q) out_q[base + e] = q;
+      blk[(e / G::PW) * G::LS + e % G::PW] = d;
+    }
+    if constexpr (W > G::PW || H > G::PH) {  // the folded region reads as 0
+      for (int e = t; e < W * H; e += G::T) {
+        const int r = e / W, c = e % W;
+        if (r >= G::PH || c >= G::PW) blk[r * G::LS + c] = 0;
       }
-      blk[t * G::LS + c] = d;
     }
   }
   fd = group_sum_u64<G::T>(fd);
