@@ -542,34 +542,28 @@ __global__ __launch_bounds__(64 * kWaveBlocks) void k_rdoq_wave(
     return;
   wave_sync();
   // the simple walk over scan indices [si_end, si] as a fixpoint; lane l
-  // holds positions si - l - 64 k
-  int ci[K];
-  int32_t tt[K], qq[K];
-  bool low[K];
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    const int i = si - lane - 64 * k;
-    low[k] = false;
-    qq[k] = 0, tt[k] = 0, ci[k] = 0;
-    if (i >= si_end) {
-      ci[k] = s.scan[i];
-      tt[k] = s.m.ts[ci[k]];
-      qq[k] = s.m.qs[ci[k]];
-    }
-  }
+  // holds positions si - l - 64 k, re-read from LDS each pass (the
+  // quantizer's levels there are never changed by the walk), its lowered
+  // positions as bits of one word
+  static_assert(K <= 32, "one bit per position of a lane");
   if (si >= si_end) {
+    uint32_t lowm = 0;
     for (;;) {
       bool changed = false;
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        if (qq[k] != 0) {
-          const int a = qq[k] < 0 ? -qq[k] : qq[k];
-          const int l = s.simple_level(ci[k], si - lane - 64 * k, tt[k], qq[k]);
-          if ((l != a) != low[k]) {
-            low[k] = l != a;
-            s.set_level(ci[k], l);
-            changed = true;
-          }
+        const int i = si - lane - 64 * k;
+        if (i < si_end) break;
+        const int ci = s.scan[i];
+        const int32_t qv = s.m.qs[ci];
+        if (qv == 0) continue;
+        const int a = qv < 0 ? -qv : qv;
+        const int l = s.simple_level(ci, i, s.m.ts[ci], qv);
+        const bool lo = l != a;
+        if (lo != ((lowm >> k) & 1)) {
+          lowm ^= 1u << k;
+          s.set_level(ci, l);
+          changed = true;
         }
       }
       wave_sync();
@@ -577,7 +571,11 @@ __global__ __launch_bounds__(64 * kWaveBlocks) void k_rdoq_wave(
     }
 #pragma unroll
     for (int k = 0; k < K; k++)
-      if (low[k]) s.store_low(ci[k], qq[k] < 0 ? -qq[k] : qq[k], qq[k] < 0);
+      if ((lowm >> k) & 1) {
+        const int ci = s.scan[si - lane - 64 * k];
+        const int32_t qv = s.m.qs[ci];
+        s.store_low(ci, qv < 0 ? -qv : qv, qv < 0);
+      }
   }
   // the walk's last step: the DC position when the cut-off reaches it
   const int last = si >= si_end ? si_end - 1 : si;
