@@ -34,6 +34,11 @@
 // unsettled position, so the passes end.  The wave then prices the DC
 // position (update_coeff_general) when the walk reaches it.
 //
+// Group form (the latency form up to 256 coefficients): the wave form with
+// 16 lanes per block and four blocks per wave, so one instruction stream
+// advances four blocks' sequential eob phases (the phase that dominates:
+// about twelve sequential steps per block on dense 16x16 candidates).
+//
 // Reads past the block's right or bottom edge return 0, which is what the
 // reference's zero padding (4 columns, >= 6 rows) holds.  The cost tables of
 // the launch's size (both plane types) and its three scans are staged in
@@ -587,6 +592,177 @@ __global__ __launch_bounds__(64 * kWaveBlocks) void k_rdoq_wave(
   if (lane == 0) eobs[j] = (uint16_t)eob;
 }
 
+// The group form: G lanes per block (64 / G blocks per wave, 4 waves per
+// workgroup).  The eob phase is sequential per block and runs the same
+// instructions on every lane of its group, so a wave advances 64 / G
+// blocks' walks at once instead of one; the zeros between nonzero levels
+// are priced G at a time, the fixpoint gives each lane N / G positions.
+// Everything else is the wave form's.  Collective steps (ballots, group
+// sums) run in wave-uniform control flow; a group whose block is done or
+// not defined keeps taking part with no effect.
+template <int TXS, int G>
+__global__ __launch_bounds__(256) void k_rdoq_group(
+    const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
+    uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, int n_jobs, const int32_t* __restrict__ costs,
+    const uint16_t* __restrict__ scans, int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
+  using T = Trellis<TXS, WaveMap>;
+  constexpr int N = T::N, BPW = 64 / G, BPG = 4 * BPW, K = N / G;
+  static_assert(N % G == 0 && K <= 32, "positions of a lane as bits of one word");
+  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
+  __shared__ uint16_t sc[3 * N];
+  __shared__ int32_t ts[BPG][N], qs[BPG][N];
+  __shared__ uint8_t lvb[BPG][N];
+  const int tid = threadIdx.x, lane = tid & 63, g = lane % G, grp = lane / G, b = tid / G;
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = tid; i < 2 * kTxbWords; i += 256) tab[i] = txb_src[i];
+  for (int i = tid; i < 2 * kEobWords; i += 256) tab[2 * kTxbWords + i] = eob_src[i];
+  for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
+  __syncthreads();
+  const int j = blockIdx.x * BPG + b;
+  RdoqJob job{};
+  int eob = 0;
+  bool act = false;
+  if (j < n_jobs) {
+    job = jobs[j];
+    eob = eobs[j];
+    act = job_ok<TXS>(job, eob);
+  }
+  // the block, raster, and its levels (txb_init_levels, coeff_c.rs:278)
+  if (act)
+    for (int p = g; p < N; p += G) {
+      const int32_t t = tcoeff[job.coeff + p], q = qcoeff[job.coeff + p];
+      ts[b][p] = t;
+      qs[b][p] = q;
+      const int a = q < 0 ? -q : q;
+      lvb[b][p] = (uint8_t)(a < 15 ? a : 15);
+    }
+  wave_sync();
+  T s;
+  s.m.lv = lvb[b];
+  s.m.ts = ts[b];
+  s.m.qs = qs[b];
+  s.qc = qcoeff + job.coeff;
+  s.dq = dqcoeff + job.coeff;
+  s.cost = tab + (job.plane_type ? kTxbWords : 0);
+  s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
+  s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
+  s.rdmult = job.rdmult;
+  s.dqv[0] = dq0, s.dqv[1] = dq1;
+  s.cls = job.tx_class;
+  s.dc_ctx = job.dc_sign_ctx;
+  const bool sharp = job.sharpness != 0;
+  // ---- the last coefficient (as the wave form's head) ----
+  int accu_rate = 0, si = 0, nz_num = 0, nz_ci[5] = {0, 0, 0, 0, 0};
+  int64_t accu_dist = 0;
+  if (act) {
+    accu_rate = s.eob_cost(eob);
+    si = eob - 1;
+    int ci;
+    int32_t t, qv, d;
+    s.fetch(si, ci, t, qv, d);
+    if (qv == 0) {
+      act = false;  // not defined: the block is left as it is
+    } else {
+      const int a = qv < 0 ? -qv : qv, sign = qv < 0;
+      if (a >= 2) {
+        s.update_general(&accu_rate, &accu_dist, si, eob);
+      } else {
+        accu_rate += s.cost_eob(ci, a, sign, s.lower_ctx(ci, si, true));
+        accu_dist += coeff_dist(t, d, T::SHIFT) - coeff_dist(t, 0, T::SHIFT);
+      }
+      si--;
+      nz_num = 1, nz_ci[0] = ci;
+    }
+  }
+  // ---- the eob phase, G positions of zeros at a time ----
+  for (;;) {
+    const bool walking = act && si >= 0 && nz_num <= 4;
+    if (__ballot(walking) == 0) break;
+    const int i = si - g;
+    int cz = 0;
+    bool nz = false;
+    if (walking && i >= 0) {
+      cz = s.scan[i];
+      nz = s.m.qs[cz] != 0;  // below si: still the quantizer's level
+    }
+    const uint32_t m = (uint32_t)(__ballot(nz) >> (grp * G)) & (G == 32 ? 0xffffffffu : ((1u << G) - 1));
+    const int nzero = m ? (int)__builtin_ctz(m) : min(G, si + 1);
+    const int r = (walking && g < nzero) ? s.base_cost(s.lower_ctx(cz, i, false), 0) : 0;
+    const int rs = group_sum_i32<G>(r);
+    if (walking) {
+      accu_rate += rs;
+      si -= nzero;
+      if (m != 0) {
+        s.update_eob(&accu_rate, &accu_dist, &eob, &nz_num, nz_ci, si, sharp);
+        si--;
+      }
+    }
+  }
+  int si_end = 1;
+  if (act) {
+    if (si == -1 && nz_num <= 4) {  // update_skip (quant.rs:1281)
+      const int64_t rd = rdcost(s.rdmult, accu_rate + s.cost[kSkip + job.txb_skip_ctx * 2], accu_dist);
+      if (!sharp && rdcost(s.rdmult, s.cost[kSkip + job.txb_skip_ctx * 2 + 1], 0) < rd) {
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+          if (k < nz_num) s.qc[nz_ci[k]] = 0, s.dq[nz_ci[k]] = 0;
+        eob = 0;
+      }
+    }
+    if (cut_num != 0) {
+      const uint32_t cut = max((uint32_t)(N >> 7), (uint32_t)eob * cut_num / cut_den);
+      si_end = max(eob - (int)cut, 1);
+    }
+  }
+  wave_sync();
+  // ---- the simple walk over [si_end, si] as a fixpoint (the wave form's),
+  // lane g of the group holding positions si - g - G k ----
+  const bool fix = act && si >= si_end;
+  uint32_t lowm = 0;
+  for (;;) {
+    bool changed = false;
+    if (fix) {
+#pragma unroll 4
+      for (int k = 0; k < K; k++) {
+        const int i = si - g - G * k;
+        if (i < si_end) break;
+        const int ci = s.scan[i];
+        const int32_t qv = s.m.qs[ci];
+        if (qv == 0) continue;
+        const int a = qv < 0 ? -qv : qv;
+        const int l = s.simple_level(ci, i, s.m.ts[ci], qv);
+        const bool lo = l != a;
+        if (lo != ((lowm >> k) & 1)) {
+          lowm ^= 1u << k;
+          s.set_level(ci, l);
+          changed = true;
+        }
+      }
+    }
+    wave_sync();
+    if (__ballot(changed) == 0) break;
+  }
+  if (fix) {
+    for (int k = 0; k < K; k++)
+      if ((lowm >> k) & 1) {
+        const int ci = s.scan[si - g - G * k];
+        const int32_t qv = s.m.qs[ci];
+        s.store_low(ci, qv < 0 ? -qv : qv, qv < 0);
+      }
+  }
+  if (act) {
+    // the walk's last step: the DC position when the cut-off reaches it
+    const int last = si >= si_end ? si_end - 1 : si;
+    if (last == 0) {
+      int ar = 0;
+      int64_t dummy = 0;
+      s.update_general(&ar, &dummy, 0, eob);
+    }
+    if (g == 0) eobs[j] = (uint16_t)eob;
+  }
+}
+
 // Coefficient rate (cost_coeffs_txb, lf/coeff_rate.rs:60, full middle loop)
 // of quantized blocks.  Without a trellis every scanned position's cost is a
 // function of the final level map, so the positions of a block are priced in
@@ -781,6 +957,25 @@ size_t rdoq_workspace(int tx_size, int n_jobs) {
   return (size_t)n_jobs * (n * sizeof(int2) + nw * sizeof(uint32_t));
 }
 
+// The latency form of a size: the group form up to 256 coefficients (16
+// lanes per block), the wave form above (a workgroup of group-form blocks
+// would not fit its LDS).
+template <int TXS>
+static void launch_latency_form(const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
+                                const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans,
+                                int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
+  constexpr int N = c_txw(TXS) * c_txh(TXS);
+  if constexpr (N <= 256) {
+    constexpr int BPG = 4 * (64 / 16);
+    hipLaunchKernelGGL((k_rdoq_group<TXS, 16>), dim3((unsigned)((n_jobs + BPG - 1) / BPG)), dim3(256), 0, st,
+                       tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);
+  } else {
+    hipLaunchKernelGGL((k_rdoq_wave<TXS>), dim3((unsigned)((n_jobs + kWaveBlocks - 1) / kWaveBlocks)),
+                       dim3(64 * kWaveBlocks), 0, st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0,
+                       dq1, cut_num, cut_den);
+  }
+}
+
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws, size_t ws_bytes) {
@@ -791,8 +986,7 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
   int2* work = static_cast<int2*>(ws);
   uint32_t* lvwork = lane ? reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n) : nullptr;
   const int bpw = n < 1024 ? 1024 / n : 1;
-  const dim3 grid((unsigned)((n_jobs + 63) / 64)), ggrid((unsigned)((n_jobs + bpw - 1) / bpw)),
-      wgrid((unsigned)((n_jobs + kWaveBlocks - 1) / kWaveBlocks));
+  const dim3 grid((unsigned)((n_jobs + 63) / 64)), ggrid((unsigned)((n_jobs + bpw - 1) / bpw));
 #define ZR(T)                                                                                                      \
   case T:                                                                                                          \
     if (lane) {                                                                                                    \
@@ -801,8 +995,8 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
       hipLaunchKernelGGL((k_rdoq<T>), grid, dim3(64), 0, st, work, lvwork, qcoeff, dqcoeff, eobs, jobs, n_jobs,    \
                          costs, scans, dq0, dq1, cut_num, cut_den);                                               \
     } else {                                                                                                       \
-      hipLaunchKernelGGL((k_rdoq_wave<T>), wgrid, dim3(64 * kWaveBlocks), 0, st, tcoeff, qcoeff, dqcoeff, eobs,    \
-                         jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);                                  \
+      launch_latency_form<T>(tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den, \
+                             st);                                                                                  \
     }                                                                                                              \
     break;
   switch (tx_size) {
