@@ -671,8 +671,15 @@ def wavefront_headline(zav1, synth, args):
         md.run(False, st)
         zav1.stream_sync(st)
         same_forms = bool(np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec))
-    # per-stage device time of one eager pass (HIP events on the chain's stream)
+    # per-stage device time of one eager pass (HIP events on the chain's stream),
+    # and the trellis jobs it ran (the plan's counter, read around the pass)
+    runs = np.zeros(2, np.uint64)
+    zav1.check(L.zav1_memcpy_d2h(runs[0:].ctypes.data, md.info.d_trellis_runs, 8, st), 'runs')
+    zav1.stream_sync(st)
     stage_ms = md.run_timed(st)
+    zav1.check(L.zav1_memcpy_d2h(runs[1:].ctypes.data, md.info.d_trellis_runs, 8, st), 'runs')
+    zav1.stream_sync(st)
+    trellis_jobs = int(runs[1] - runs[0])
     assert np.array_equal(md.raw_decisions(), dec)
     info = md.info
     n_launch = info.n_steps  # one launch of each kernel per step (every step holds leaves)
@@ -707,17 +714,20 @@ def wavefront_headline(zav1, synth, args):
                 'traffic': measured_traffic('k_md_fast_satd<unsigned short, 16, 16>'),
                 'algorithmic_ops_per_launch': int(ops)}
     else:
-        # the trellis: per job the block's transform coefficients, levels and
-        # dequantized values are read (12 B per coefficient) and the job
-        # record and eob (26 B); jobs = leaves x survivors x tx types
+        # the trellis: per block it walks, the block's transform coefficients,
+        # levels and dequantized values are read (12 B per coefficient); per
+        # job handed to it, the job record and eob (26 B).  Jobs = leaves x
+        # survivors x tx types; the blocks walked are those the coefficient-
+        # SATD gate passes with a nonzero eob (the plan's counter)
         jobs = n_leaves * 4 * n_types
-        per_launch = jobs * (256 * 12 + 26) / n_launch
+        per_launch = (trellis_jobs * 256 * 12 + jobs * 26) / n_launch
         gbs = per_launch / dom_s / 1e9
-        roof = {'bound': 'hbm', 'kernel': 'k_rdoq_wave<TX_16X16>' if dom == 'mds3_rdoq' else dom, 'unit': 'GB/s',
-                'achieved': round(gbs, 3), 'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 6),
-                'traffic': measured_traffic('k_rdoq_wave<2>'), 'algorithmic_bytes_per_launch': int(per_launch),
-                'note': 'latency-bound: one sequential trellis walk per block, ~%d blocks per launch'
-                        % (jobs // n_launch)}
+        roof = {'bound': 'hbm', 'kernel': 'k_rdoq_group<TX_16X16, 16>' if dom == 'mds3_rdoq' else dom,
+                'unit': 'GB/s', 'achieved': round(gbs, 3), 'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 6),
+                'traffic': measured_traffic('k_rdoq_group<2, 16>'), 'algorithmic_bytes_per_launch': int(per_launch),
+                'blocks_walked_per_launch': round(trellis_jobs / n_launch, 1),
+                'jobs_per_launch': round(jobs / n_launch, 1),
+                'note': 'latency-bound: a sequential trellis walk per block (the eob phase), 16 lanes per block'}
     roof['stage'] = dom
     roof['avg_launch_us'] = round(dom_s * 1e6, 2)
     roof['launches'] = n_launch

@@ -647,6 +647,8 @@ typedef struct zav1_wf_info_t {
   int32_t *d_levels;              /* keep_levels: the winners' levels, packed min(leaf, 32)^2 raster per leaf */
   const uint64_t *d_level_off;    /* keep_levels: per leaf (storage order) its offset in d_levels */
   size_t n_levels;
+  const uint64_t *d_trellis_runs; /* MDS3 jobs handed to the RDOQ trellis with a nonzero eob, summed over every
+                                   * run of the object (measurement: read it before and after a run) */
 } zav1_wf_info_t;
 typedef struct zav1_wf zav1_wf;
 /* Device bytes zav1_wf_create would take for cfg (admission by footprint). */

@@ -84,7 +84,7 @@ def test_wavefront_graph_and_plain_equal_eager(gpu):
     imgs = content(rng, 2, H, W, bd)
     cfg = wf.make_cfg(W, H, bd, 110, 300000, n_images=2, leaf=16, tiles=(1, 1), k1=6, k3=2, rdoq_level=2,
                       txt_satd_th=10)
-    outs = []
+    outs, runs = [], {}
     for mode in ('eager', 'graph', 'graph2', 'plain'):
         md = wf.WavefrontMD(z, cfg)
         try:
@@ -102,10 +102,18 @@ def test_wavefront_graph_and_plain_equal_eager(gpu):
                 md.run_plain(0, md.info.n_leaves)
             z.sync()
             outs.append((md.recon(), md.decisions()))
+            c = np.zeros(1, np.uint64)
+            z.check(z.lib().zav1_memcpy_d2h(c.ctypes.data, md.info.d_trellis_runs, 8, None), 'runs')
+            runs[mode] = int(c[0])
         finally:
             md.close()
     for r, d in outs[1:]:
         assert np.array_equal(r, outs[0][0]) and np.array_equal(d, outs[0][1])
+    # the trellis-run counter: the same per run, summed over replays, within
+    # the MDS3 jobs (leaves x survivors x types); the plain form does not count
+    jobs = md.info.n_leaves * 2 * md.info.n_types
+    assert 0 < runs['eager'] <= jobs and runs['graph'] == runs['eager'] and runs['graph2'] == 2 * runs['eager']
+    assert runs['plain'] == 0
 
 
 def test_wavefront_rejects_bad_configs(gpu):

@@ -399,15 +399,25 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
 // The pre-trellis SATD gate (TuSatdGate): per job against its group's first.
 __global__ __launch_bounds__(256) void k_tu_satd_gate(int n_jobs, int group, const uint64_t* __restrict__ csatd,
                                                       int64_t th, uint16_t* __restrict__ eob16,
-                                                      int32_t* __restrict__ eob) {
+                                                      int32_t* __restrict__ eob, unsigned long long* runs) {
   const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= n_jobs) return;
-  const int t = j % group;
-  if (t == 0 || eob[j] < 0) return;
-  const int64_t s0 = (int64_t)csatd[j - t], sd = (int64_t)csatd[j];
-  if ((sd - s0) * 100 > s0 * th) {
-    eob16[j] = 0;
-    eob[j] = kTuGated;
+  bool run = false;
+  if (j < n_jobs && eob[j] >= 0) {
+    const int t = j % group;
+    bool gated = false;
+    if (t != 0 && th > 0) {
+      const int64_t s0 = (int64_t)csatd[j - t], sd = (int64_t)csatd[j];
+      gated = (sd - s0) * 100 > s0 * th;
+    }
+    if (gated) {
+      eob16[j] = 0;
+      eob[j] = kTuGated;
+    }
+    run = !gated && eob16[j] > 0;
+  }
+  if (runs) {
+    const uint64_t m = __ballot(run);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(runs, (unsigned long long)__popcll(m));
   }
 }
 
@@ -478,9 +488,9 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   hipError_t e = txunit_batch(src, stride, plane_w, plane_h, bd, tx_size, pred, pred_len, tj, n_jobs, row, iscan,
                               flags & kTxuQuantB, dist, eob, cul, tq, nullptr, st, keep);
   if (e != hipSuccess) return e;
-  if (gate && csatd && gate->group > 1 && gate->th > 0)
+  if (gate && csatd && gate->group > 0 && (gate->runs || (gate->group > 1 && gate->th > 0)))
     hipLaunchKernelGGL(k_tu_satd_gate, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, n_jobs, gate->group,
-                       csatd, gate->th, keep.eob16, eob);
+                       csatd, gate->group > 1 ? gate->th : (int64_t)0, keep.eob16, eob, gate->runs);
   // rdoq_ev: two events recorded around the trellis (timing; recorded
   // also when the flags skip it)
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;

@@ -331,6 +331,7 @@ struct Wavefront {
   uint8_t* mode_map = nullptr;
   int32_t* costs = nullptr;
   int32_t* levels = nullptr;  // kept levels (cfg.keep_levels), per unit at h_loff
+  unsigned long long* trellis_runs = nullptr;  // MDS3 jobs handed to the trellis, over every run
   uint64_t* loff = nullptr;
   hipGraphExec_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -617,7 +618,7 @@ static size_t footprint(const Wavefront& w) {
   const size_t es = c.bd > 8 ? 2 : 1, plane = (size_t)c.width * w.stacked_h * es;
   const size_t n = (size_t)w.n_units;
   size_t b = 2 * al(plane) + al(n * sizeof(MdUnit)) + al(n * 8) + al(n * 4) + al(n * sizeof(WfDecision)) + al(n);
-  b += al(sizeof(zav1_coeff_costs));
+  b += al(sizeof(zav1_coeff_costs)) + al(8);
   if (c.keep_levels) b += al(w.n_levels * 4) + al(n * 8);
   for (int k = 0; k < 4; k++) b += size_bytes(w.sp[k], c.keep_levels != 0);
   return b;
@@ -675,7 +676,7 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   // types the commit's coefficient-SATD gate skips whatever the rate gate
   // does (more than satd_th percent above the DCT's: the running minimum it
   // compares against is at most the DCT's) skip the trellis and the rest
-  const TuSatdGate gate{P.nt, P.satd_th};
+  const TuSatdGate gate{P.nt, P.satd_th, w->trellis_runs};
   e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs3, n3, w->row, w->iscan,
                       P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3, P.eob3, P.rate3, P.cul3,
                       nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes, &gate);
@@ -978,6 +979,7 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   w->dec = (zav1::WfDecision*)take(n * sizeof(zav1::WfDecision));
   w->mode_map = (uint8_t*)take(n);
   w->costs = (int32_t*)take(sizeof(zav1_coeff_costs));
+  w->trellis_runs = (unsigned long long*)take(8);
   if (c.keep_levels) {
     w->levels = (int32_t*)take(w->n_levels * 4);
     w->loff = (uint64_t*)take(n * 8);
@@ -992,6 +994,8 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
   up(w->nbr, w->h_nbr.data(), n * 8);
   up(w->uid, uid.data(), n * 4);
   up(w->costs, &costs, sizeof(costs));
+  const unsigned long long zero = 0;
+  up(w->trellis_runs, &zero, 8);
   if (c.keep_levels) up(w->loff, w->h_loff.data(), n * 8);
   for (int k = 0; k < 4; k++) {
     zav1::SizePlan& P = w->sp[k];
@@ -1072,6 +1076,7 @@ int zav1_wf_info(const zav1_wf* wf, zav1_wf_info_t* info) {
   info->d_levels = w->levels;
   info->d_level_off = w->loff;
   info->n_levels = w->n_levels;
+  info->d_trellis_runs = reinterpret_cast<const uint64_t*>(w->trellis_runs);
   return ZAV1_OK;
 }
 

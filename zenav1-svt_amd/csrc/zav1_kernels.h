@@ -331,10 +331,13 @@ size_t txunit_rd_workspace(int tx_size, int n_jobs);
 // A pre-trellis gate over jobs in groups of `group` (one candidate's tx
 // types in search order, the reference type first): a job whose coefficient
 // SATD exceeds its group's first by more than `th` percent gets no trellis,
-// rate, reconstruction or distortion (its eob becomes kTuGated).
+// rate, reconstruction or distortion (its eob becomes kTuGated).  `runs`
+// (may be null) accumulates the jobs then handed to the trellis with a
+// nonzero eob (measurement).
 struct TuSatdGate {
   int group;
   int64_t th;
+  unsigned long long* runs;
 };
 constexpr int32_t kTuGated = -3;
 // The fixed per-device scratch of the no-allocation entry points

@@ -31,7 +31,8 @@ class WfInfo(ctypes.Structure):
                 ('d_units', ctypes.c_void_p), ('n_leaves', ctypes.c_int32), ('n_steps', ctypes.c_int32),
                 ('max_leaves_per_step', ctypes.c_int32), ('n_types', ctypes.c_int32),
                 ('stacked_height', ctypes.c_int32), ('types', ctypes.c_uint8 * 16), ('bytes', ctypes.c_size_t),
-                ('d_levels', ctypes.c_void_p), ('d_level_off', ctypes.c_void_p), ('n_levels', ctypes.c_size_t)]
+                ('d_levels', ctypes.c_void_p), ('d_level_off', ctypes.c_void_p), ('n_levels', ctypes.c_size_t),
+                ('d_trellis_runs', ctypes.c_void_p)]
 
 
 def preset_candidates(fi=True):
