@@ -398,7 +398,8 @@ def rdoq_stage(zav1, synth, args, cpu=True):
     steps = max(5, args.steps // 5)
 
     def time_form(lane):
-        # the lane form (a workspace, threshold 0) or the wave form (none)
+        # the lane form (a workspace, threshold 0) or the latency form (none: the
+        # group form at this size)
         use_ws[0] = lane
         prev = L.zav1_rdoq_lane_min(ts, 0)
         try:
@@ -420,11 +421,11 @@ def rdoq_stage(zav1, synth, args, cpu=True):
         finally:
             L.zav1_rdoq_lane_min(ts, prev)
 
-    wave_ms, q_wave = time_form(False)
+    group_ms, q_group = time_form(False)
     lane_ms, q_lane = time_form(True)
-    assert np.array_equal(q_wave, q_lane), 'trellis forms disagree'
-    form = 'wave' if wave_ms < lane_ms else 'lane'
-    ms_step = min(wave_ms, lane_ms)
+    assert np.array_equal(q_group, q_lane), 'trellis forms disagree'
+    form = 'group' if group_ms < lane_ms else 'lane'
+    ms_step = min(group_ms, lane_ms)
     # coefficient rate of every candidate after the trellis (one launch)
     rjobs = np.zeros(nj, zav1.RATE_JOB)
     rjobs['coeff'] = jobs['coeff']
@@ -455,8 +456,8 @@ def rdoq_stage(zav1, synth, args, cpu=True):
            'mean_eob_in': round(float(eob0.mean()), 2), 'mean_eob_out': round(float(eob1.mean()), 2),
            'levels_changed_frac': round(changed / max(1, int((q0 != 0).sum())), 4),
            'coeff_rate_ms': round(rate_ms, 4), 'coeff_rate_blocks_per_s': round(nj / (rate_ms / 1000.0)),
-           'lane_form_ms': round(lane_ms, 4), 'wave_form_ms': round(wave_ms, 4),
-           'roofline': {'bound': 'latency', 'kernel': ('k_rdoq_wave<TX_16X16>' if form == 'wave' else
+           'lane_form_ms': round(lane_ms, 4), 'group_form_ms': round(group_ms, 4),
+           'roofline': {'bound': 'latency', 'kernel': ('k_rdoq_group<TX_16X16, 16>' if form == 'group' else
                                                        'k_rdoq<TX_16X16>'), 'unit': 'GB/s',
                         'achieved': round(alg_bytes / (ms_step / 1000.0) / 1e9, 2), 'peak': 8000.0,
                         'frac': round(alg_bytes / (ms_step / 1000.0) / 1e9 / 8000.0, 4),
