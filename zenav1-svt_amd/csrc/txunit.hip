@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
       out_dist[job] = 0;
       out_eob[job] = !inside ? -22 : -1;
       out_cul[job] = 0;
-      if (keep.t) {
+      if (keep.eob16) {
         keep.eob16[job] = 0;
         keep.fold[job] = 0;
       }
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
     constexpr int SHF = (1 - LSC) * 2;
     out_dist[job] = spatial ? (sdist << 4) : (SHF < 0 ? (fdist << (-SHF)) : (fdist >> SHF));
     out_eob[job] = eob;
-    if (keep.t) {
+    if (keep.eob16) {
       keep.eob16[job] = (uint16_t)eob;
       keep.fold[job] = fold;
     }
@@ -346,6 +346,24 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
   const long base = (long)job * NP;
   uint64_t fd = 0;
   int32_t lsum = 0;
+  // Without the trellis and the reconstruction the forward pass's
+  // distortion, eob and cul level are final (it wrote them, and the
+  // coefficients it would otherwise keep were not written): only the skip
+  // cost and the rate estimate are left to apply.
+  const bool light = !(flags & (kTxuRdoq | kTxuRecon | kTxuSpatial));
+  if (light) {
+    if (active && out_q)
+      for (int e = t; e < NP; e += G::T) out_q[base + e] = tq[base + e];
+    if (!live || t != 0) return;
+    if (!active) {
+      out_rate[job] = 0;
+      return;
+    }
+    const int eob = eob16[job];
+    if (eob == 0) out_rate[job] = costs[(txs_ctx * 2 + (jb.plane_type ? 1 : 0)) * 944 + jb.txb_skip_ctx * 2 + 1];
+    if ((flags & kTxuRateEst2) && jb.plane_type == 0 && eob < ((W * H) >> 6)) out_rate[job] = 6000 + 1000 * eob;
+    return;
+  }
   // the job's packed arrays read as contiguous runs (lane t: elements t,
   // t + T, ...), the dequantized block laid out in LDS for the inverse
   if (active) {
@@ -475,6 +493,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   keep.eob16 = reinterpret_cast<uint16_t*>(carve(n * 2));
   keep.fold = reinterpret_cast<uint64_t*>(carve(n * 8));
   keep.csatd = csatd;  // written by the forward pass straight into the caller's output
+  if (!(flags & (kTxuRdoq | kTxuRecon | kTxuSpatial))) keep.t = keep.dq = nullptr;  // the finish does not read them
   TxUnitJob* tj = reinterpret_cast<TxUnitJob*>(carve(n * sizeof(TxUnitJob)));
   RdoqJob* rj = reinterpret_cast<RdoqJob*>(carve(n * sizeof(RdoqJob)));
   RateJob* cj = reinterpret_cast<RateJob*>(carve(n * sizeof(RateJob)));

@@ -278,8 +278,8 @@ enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u, kTxuRdoq = 
 // coefficients and dequantized values (min(w,32) x min(h,32) per job), the
 // eob as u16 and the 64-point fold's discarded energy per job.
 struct TuKeep {
-  int32_t* t = nullptr;
-  int32_t* dq = nullptr;
+  int32_t* t = nullptr;   // coefficients and dequantized values (nullable: the
+  int32_t* dq = nullptr;  // eob and fold are still kept when eob16 is set)
   uint16_t* eob16 = nullptr;
   uint64_t* fold = nullptr;
   uint64_t* csatd = nullptr;  // sum of |coefficient| over the whole transform (nullable)
