@@ -766,11 +766,13 @@ __global__ __launch_bounds__(256) void k_rdoq_group(
 // Coefficient rate (cost_coeffs_txb, lf/coeff_rate.rs:60, full middle loop)
 // of quantized blocks.  Without a trellis every scanned position's cost is a
 // function of the final level map, so the positions of a block are priced in
-// parallel: a group of G = min(64, N) lanes per block (64 / G blocks per
-// wave) walks the scan indices below the eob with stride G, reading levels
-// straight from the block (min(|q|, 127), 0 past its edges), and the group's
-// sum is reduced across its lanes; the group's first lane adds the skip-flag,
-// tx-type and eob costs.
+// parallel: a group of G = min(16, N) lanes per block up to 256 coefficients
+// (16 blocks per workgroup, so the cost tables staged in LDS serve 16
+// blocks), 64 above (the blocks' LDS), walks the
+// scan indices below the eob with stride G, reading levels straight from the
+// block (min(|q|, 127), 0 past its edges), and the group's sum is reduced
+// across its lanes; the group's first lane adds the skip-flag, tx-type and
+// eob costs.
 template <int TXS>
 __global__ __launch_bounds__(256) void k_coeff_rate(const int32_t* __restrict__ qcoeff,
                                                     const uint16_t* __restrict__ eobs,
@@ -778,7 +780,7 @@ __global__ __launch_bounds__(256) void k_coeff_rate(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ costs,
                                                     const uint16_t* __restrict__ scans, int32_t* __restrict__ rate) {
   constexpr int AW = c_txw(TXS), AH = c_txh(TXS), N = AW * AH;
-  constexpr int G = N < 64 ? N : 64, PER_WG = 256 / G;
+  constexpr int G = N < 16 ? N : (N <= 256 ? 16 : 64), PER_WG = 256 / G;  // LDS: PER_WG blocks
   __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
   __shared__ uint16_t sc[3 * N];
   __shared__ int32_t qs[PER_WG * N];  // each group's block, raster
@@ -861,7 +863,7 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
   if (n_jobs <= 0) return hipSuccess;
 #define ZC(T)                                                                                                \
   case T: {                                                                                                  \
-    constexpr int N = c_txw(T) * c_txh(T), PER_WG = 256 / (N < 64 ? N : 64);                               \
+    constexpr int N = c_txw(T) * c_txh(T), PER_WG = 256 / (N < 16 ? N : (N <= 256 ? 16 : 64));              \
     hipLaunchKernelGGL((k_coeff_rate<T>), dim3((unsigned)((n_jobs + PER_WG - 1) / PER_WG)), dim3(256), 0, st, \
                        qcoeff, eobs, jobs, n_jobs, costs, scans, rate);                                      \
     break;                                                                                                   \

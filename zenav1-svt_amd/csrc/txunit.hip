@@ -365,12 +365,17 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
     return;
   }
   // the job's packed arrays read as contiguous runs (lane t: elements t,
-  // t + T, ...), the dequantized block laid out in LDS for the inverse
+  // t + T, ...), the dequantized block laid out in LDS for the inverse; the
+  // coefficients only for the frequency distortion (not read when the
+  // distortion is spatial)
+  const bool spatial = flags & kTxuSpatial;
   if (active) {
     for (int e = t; e < NP; e += G::T) {
       const int32_t d = tdq[base + e];
-      const int64_t df = (int64_t)tt[base + e] - d;
-      fd += (uint64_t)(df * df);
+      if (!spatial) {
+        const int64_t df = (int64_t)tt[base + e] - d;
+        fd += (uint64_t)(df * df);
+      }
       const int32_t q = tq[base + e];
       lsum += q < 0 ? -q : q;
       if (out_q) out_q[base + e] = q;
@@ -386,7 +391,6 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
   fd = group_sum_u64<G::T>(fd);
   lsum = group_sum_i32<G::T>(lsum);
   uint64_t sdist = 0;
-  const bool spatial = flags & kTxuSpatial;
   if (flags & (kTxuRecon | kTxuSpatial)) {
     __syncthreads();
     sdist = tu_recon<W, H>(blk, t, active, cfg, bd, pred + jb.pred_off, src + (long)jb.y * stride + jb.x, stride,
