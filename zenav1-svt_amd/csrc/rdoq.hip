@@ -110,6 +110,36 @@ struct Trellis {
   __device__ __forceinline__ int br_ctx(int ci) const {
     return br_ctx_of<TXS>([this](int r, int c) { return level(r, c); }, ci, cls);
   }
+  // nz_ctx (not last) and br_ctx of ci from one read of its five template
+  // levels (the base-range template is the first three of the nz-map one in
+  // every class), so the reads issue together
+  __device__ __forceinline__ void ctx_pair(int ci, int* nzc, int* brc) const {
+    const int r = ci >> BWL, c = ci & (AW - 1);
+    const bool d2 = cls == kCls2D, hz = cls == kClsHoriz;
+    const int r1 = d2 ? r + 1 : (hz ? r : r + 2), c1 = d2 ? c + 1 : (hz ? c + 2 : c);
+    const int r2 = d2 ? r : (hz ? r : r + 3), c2 = d2 ? c + 2 : (hz ? c + 3 : c);
+    const int r3 = d2 ? r + 2 : (hz ? r : r + 4), c3 = d2 ? c : (hz ? c + 4 : c);
+    const int la = level(r, c + 1), lb = level(r + 1, c), l1 = level(r1, c1), l2 = level(r2, c2),
+              l3 = level(r3, c3);
+    const int mag = min(la, 3) + min(lb, 3) + min(l1, 3) + min(l2, 3) + min(l3, 3);
+    const int bm = min((la + lb + l1 + 1) >> 1, 6);
+    if (ci == 0) {
+      *nzc = 0;  // (cls | ci) == 0 for the 2D class; the 1D classes at ci 0 below
+      *brc = bm;
+      if (!d2) *nzc = min((mag + 1) >> 1, 4) + 26;
+      return;
+    }
+    const int ctx = min((mag + 1) >> 1, 4);
+    if (d2) {
+      if (tx_w(TXS) < tx_h(TXS) && r < 2) *nzc = ctx + 11;
+      else if (tx_w(TXS) > tx_h(TXS) && c < 2) *nzc = ctx + 16;
+      else *nzc = ctx + (r + c < 2 ? 1 : (r + c < 4 ? 6 : 21));
+    } else {
+      const int pos = hz ? c : r;
+      *nzc = ctx + (pos == 0 ? 26 : (pos == 1 ? 31 : 36));
+    }
+    *brc = bm + (((d2 && r < 2 && c < 2) || (hz && c == 0) || (!d2 && !hz && r == 0)) ? 7 : 14);
+  }
   __device__ __forceinline__ int base_cost(int ctx, int k) const { return cost[kBase + ctx * 8 + k]; }
   __device__ __forceinline__ int lps(int ctx, int k) const { return cost[kLps + ctx * 26 + k]; }
   __device__ __forceinline__ int br_cost(int level_, int ctx) const {
@@ -250,44 +280,62 @@ struct Trellis {
   }
   // quant.rs:1120 update_coeff_eob; nz_ci[0 .. nz_num) are the nonzero
   // positions since the eob (at most 5)
+  // The step's contexts come from one read of the template levels
+  // (ctx_pair) and the eob context and eob cost from the position alone, so
+  // its table reads issue together: the sequential walk's chain is scan ->
+  // coefficient -> levels -> tables.
   __device__ void update_eob(int* accu_rate, int64_t* accu_dist, int* eob, int* nz_num, int* nz_ci, int si,
                              bool sharp) {
     int ci;
     int32_t t, qv, d;
     fetch(si, ci, t, qv, d);
     const int dv = dqv[ci != 0];
-    const int ctx = lower_ctx(ci, si, false);
+    int ctx, brc;
+    ctx_pair(ci, &ctx, &brc);
     if (qv == 0) {
       *accu_rate += base_cost(ctx, 0);
       return;
     }
     bool lower = false;
     const int a = qv < 0 ? -qv : qv, sign = qv < 0;
+    const int ctx_ne = si == 0 ? 0 : (si <= N / 8 ? 1 : (si <= N / 4 ? 2 : 3));
+    const int brc_ne = br_ctx_eob(ci);
+    // every cost term of the four candidates (a, a - 1; before the eob, last)
+    const int sgn = ci == 0 ? cost[kDcSign + dc_ctx * 2 + sign] : 512;
+    const int b_hi = base_cost(ctx, min(a, 3)), b_lo = base_cost(ctx, min(a - 1, 3));
+    const int e_hi = cost[kBaseEob + ctx_ne * 3 + min(a, 3) - 1], e_lo = cost[kBaseEob + ctx_ne * 3 + max(min(a - 1, 3) - 1, 0)];
+    const int p_hi = lps(brc, min(max(a - 3, 0), 12)), p_lo = lps(brc, min(max(a - 4, 0), 12));
+    const int pe_hi = lps(brc_ne, min(max(a - 3, 0), 12)), pe_lo = lps(brc_ne, min(max(a - 4, 0), 12));
+    const int ne_cost = eob_cost(si + 1);
+    const int g_hi = golomb_cost(a), g_lo = golomb_cost(a - 1);
+    // cost_general(false, ci, a / a - 1) and cost_eob(ci, a / a - 1) (quant.rs:758 / :795)
+    const int c_hi = b_hi + sgn + (a > 2 ? p_hi + g_hi : 0);
+    const int c_lo = a - 1 == 0 ? b_lo : b_lo + sgn + (a - 1 > 2 ? p_lo + g_lo : 0);
+    const int ce_hi = e_hi + sgn + (a > 2 ? pe_hi + g_hi : 0);
+    const int ce_lo = e_lo + sgn + (a - 1 > 2 ? pe_lo + g_lo : 0);
     const int64_t dist0 = coeff_dist(t, 0, SHIFT);
     int64_t dist = coeff_dist(t, d, SHIFT) - dist0;
-    int rate = cost_general(false, ci, a, sign, ctx);
+    int rate = c_hi;
     const int64_t rd = rdcost(rdmult, *accu_rate + rate, *accu_dist + dist);
     int32_t q_low = 0, d_low = 0;
     int rate_low;
     int64_t dist_low, rd_low;
     if (a == 1) {
       dist_low = 0;
-      rate_low = base_cost(ctx, 0);
+      rate_low = b_lo;  // base_cost(ctx, 0)
       rd_low = rdcost(rdmult, *accu_rate + rate_low, *accu_dist);
     } else {
       low(a, sign, dv, &q_low, &d_low);
       dist_low = coeff_dist(t, d_low, SHIFT) - dist0;
-      rate_low = cost_general(false, ci, a - 1, sign, ctx);
+      rate_low = c_lo;
       rd_low = rdcost(rdmult, *accu_rate + rate_low, *accu_dist + dist_low);
     }
     bool lower_ne = false;
-    const int ctx_ne = lower_ctx(ci, si, true);
-    const int ne_cost = eob_cost(si + 1);
-    int rate_ce = ne_cost + cost_eob(ci, a, sign, ctx_ne);
+    int rate_ce = ne_cost + ce_hi;
     int64_t dist_ne = dist;
     int64_t rd_ne = rdcost(rdmult, rate_ce, dist_ne);
     if (a - 1 > 0) {
-      const int rate_ce_low = ne_cost + cost_eob(ci, a - 1, sign, ctx_ne);
+      const int rate_ce_low = ne_cost + ce_lo;
       const int64_t rd_ne_low = rdcost(rdmult, rate_ce_low, dist_low);
       if (rd_ne_low < rd_ne) {
         lower_ne = true;
@@ -608,6 +656,7 @@ __global__ __launch_bounds__(256) void k_rdoq_group(
   using T = Trellis<TXS, WaveMap>;
   constexpr int N = T::N, BPW = 64 / G, BPG = 4 * BPW, K = N / G;
   static_assert(N % G == 0 && K <= 32, "positions of a lane as bits of one word");
+  static_assert(G <= 16, "group sums by DPP within a row of 16 lanes");
   __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
   __shared__ uint16_t sc[3 * N];
   __shared__ int32_t ts[BPG][N], qs[BPG][N];
@@ -689,7 +738,7 @@ __global__ __launch_bounds__(256) void k_rdoq_group(
     const uint32_t m = (uint32_t)(__ballot(nz) >> (grp * G)) & (G == 32 ? 0xffffffffu : ((1u << G) - 1));
     const int nzero = m ? (int)__builtin_ctz(m) : min(G, si + 1);
     const int r = (walking && g < nzero) ? s.base_cost(s.lower_ctx(cz, i, false), 0) : 0;
-    const int rs = group_sum_i32<G>(r);
+    const int rs = dpp_sum_i32<G>(r);  // DPP row rotations: no LDS round trip
     if (walking) {
       accu_rate += rs;
       si -= nzero;
