@@ -45,6 +45,7 @@
 // LDS once per workgroup.
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "coeff_ctx.h"
 #include "zav1_dev.h"
@@ -69,14 +70,18 @@ struct GatherMap {
   __device__ __forceinline__ int2 coef(int si, int) const { return work[si]; }
 };
 // ... of the wave form: byte levels and the raster block, all in LDS
-struct WaveMap {
+// (QT: the staged quantizer levels' type -- 16 bits in the compact group
+// form, where the caller guarantees they fit)
+template <class QT>
+struct WaveMapQ {
   uint8_t* lv;
   const int32_t* ts;
-  const int32_t* qs;
+  const QT* qs;
   __device__ __forceinline__ int get(int p) const { return lv[p]; }
   __device__ __forceinline__ void set(int p, int v) { lv[p] = (uint8_t)v; }
-  __device__ __forceinline__ int2 coef(int, int ci) const { return make_int2(ts[ci], qs[ci]); }
+  __device__ __forceinline__ int2 coef(int, int ci) const { return make_int2(ts[ci], (int32_t)qs[ci]); }
 };
+using WaveMap = WaveMapQ<int32_t>;
 
 template <int TXS, class Map>
 struct Trellis {
@@ -648,25 +653,32 @@ __global__ __launch_bounds__(64 * kWaveBlocks) void k_rdoq_wave(
 // Everything else is the wave form's.  Collective steps (ballots, group
 // sums) run in wave-uniform control flow; a group whose block is done or
 // not defined keeps taking part with no effect.
-template <int TXS, int G>
-__global__ __launch_bounds__(256) void k_rdoq_group(
+// CMP (compact): every job is luma and every level fits 16 bits (the
+// caller's guarantee), so the workgroup stages one plane's cost tables and
+// 16-bit levels -- less LDS per block, more blocks resident per CU.
+template <int TXS, int G, int WPG, bool CMP>
+__global__ __launch_bounds__(64 * WPG) void k_rdoq_group(
     const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
     uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, int n_jobs, const int32_t* __restrict__ costs,
     const uint16_t* __restrict__ scans, int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
-  using T = Trellis<TXS, WaveMap>;
-  constexpr int N = T::N, BPW = 64 / G, BPG = 4 * BPW, K = N / G;
+  using QT = typename std::conditional<CMP, int16_t, int32_t>::type;
+  using T = Trellis<TXS, WaveMapQ<QT>>;
+  constexpr int N = T::N, BPW = 64 / G, BPG = WPG * BPW, K = N / G, NT = 64 * WPG;
+  constexpr int TXB = CMP ? kTxbWords : 2 * kTxbWords, EOBW = CMP ? kEobWords : 2 * kEobWords;
   static_assert(N % G == 0 && K <= 32, "positions of a lane as bits of one word");
   static_assert(G <= 16, "group sums by DPP within a row of 16 lanes");
-  __shared__ int32_t tab[2 * kTxbWords + 2 * kEobWords];
+  __shared__ int32_t tab[TXB + EOBW];
   __shared__ uint16_t sc[3 * N];
-  __shared__ int32_t ts[BPG][N], qs[BPG][N];
+  __shared__ int32_t ts[BPG][N];
+  __shared__ QT qs[BPG][N];
   __shared__ uint8_t lvb[BPG][N];
   const int tid = threadIdx.x, lane = tid & 63, g = lane % G, grp = lane / G, b = tid / G;
+  // the size's tables: both planes, or the luma plane's (the first of each pair)
   const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
   const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
-  for (int i = tid; i < 2 * kTxbWords; i += 256) tab[i] = txb_src[i];
-  for (int i = tid; i < 2 * kEobWords; i += 256) tab[2 * kTxbWords + i] = eob_src[i];
-  for (int i = tid; i < 3 * N; i += 256) sc[i] = scans[i];
+  for (int i = tid; i < TXB; i += NT) tab[i] = txb_src[i];
+  for (int i = tid; i < EOBW; i += NT) tab[TXB + i] = eob_src[i];
+  for (int i = tid; i < 3 * N; i += NT) sc[i] = scans[i];
   __syncthreads();
   const int j = blockIdx.x * BPG + b;
   RdoqJob job{};
@@ -675,14 +687,14 @@ __global__ __launch_bounds__(256) void k_rdoq_group(
   if (j < n_jobs) {
     job = jobs[j];
     eob = eobs[j];
-    act = job_ok<TXS>(job, eob);
+    act = job_ok<TXS>(job, eob) && (!CMP || job.plane_type == 0);
   }
   // the block, raster, and its levels (txb_init_levels, coeff_c.rs:278)
   if (act)
     for (int p = g; p < N; p += G) {
       const int32_t t = tcoeff[job.coeff + p], q = qcoeff[job.coeff + p];
       ts[b][p] = t;
-      qs[b][p] = q;
+      qs[b][p] = (QT)q;
       const int a = q < 0 ? -q : q;
       lvb[b][p] = (uint8_t)(a < 15 ? a : 15);
     }
@@ -694,7 +706,7 @@ __global__ __launch_bounds__(256) void k_rdoq_group(
   s.qc = qcoeff + job.coeff;
   s.dq = dqcoeff + job.coeff;
   s.cost = tab + (job.plane_type ? kTxbWords : 0);
-  s.ecost = tab + 2 * kTxbWords + (job.plane_type ? kEobWords : 0);
+  s.ecost = tab + TXB + (job.plane_type ? kEobWords : 0);
   s.scan = sc + (job.tx_class == kClsVert ? 1 : (job.tx_class == kClsHoriz ? 2 : 0)) * N;
   s.rdmult = job.rdmult;
   s.dqv[0] = dq0, s.dqv[1] = dq1;
@@ -1014,12 +1026,21 @@ size_t rdoq_workspace(int tx_size, int n_jobs) {
 template <int TXS>
 static void launch_latency_form(const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                                 const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans,
-                                int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st) {
+                                int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st,
+                                bool compact) {
   constexpr int N = c_txw(TXS) * c_txh(TXS);
   if constexpr (N <= 256) {
-    constexpr int BPG = 4 * (64 / 16);
-    hipLaunchKernelGGL((k_rdoq_group<TXS, 16>), dim3((unsigned)((n_jobs + BPG - 1) / BPG)), dim3(256), 0, st,
-                       tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);
+    // 12 blocks per workgroup in the compact form (26.9 KB of LDS at 16x16:
+    // six workgroups per CU), 16 otherwise (46.1 KB: three)
+    if (compact) {
+      constexpr int BPG = 3 * (64 / 16);
+      hipLaunchKernelGGL((k_rdoq_group<TXS, 16, 3, true>), dim3((unsigned)((n_jobs + BPG - 1) / BPG)), dim3(192), 0,
+                         st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);
+    } else {
+      constexpr int BPG = 4 * (64 / 16);
+      hipLaunchKernelGGL((k_rdoq_group<TXS, 16, 4, false>), dim3((unsigned)((n_jobs + BPG - 1) / BPG)), dim3(256), 0,
+                         st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);
+    }
   } else {
     hipLaunchKernelGGL((k_rdoq_wave<TXS>), dim3((unsigned)((n_jobs + kWaveBlocks - 1) / kWaveBlocks)),
                        dim3(64 * kWaveBlocks), 0, st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0,
@@ -1029,7 +1050,8 @@ static void launch_latency_form(const int32_t* tcoeff, int32_t* qcoeff, int32_t*
 
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
-                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws, size_t ws_bytes) {
+                      int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws, size_t ws_bytes,
+                      bool compact) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const bool lane = ws && ws_bytes >= rdoq_workspace(tx_size, n_jobs) && n_jobs >= g_lane_min[tx_size];
@@ -1047,7 +1069,7 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
                          costs, scans, dq0, dq1, cut_num, cut_den);                                               \
     } else {                                                                                                       \
       launch_latency_form<T>(tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den, \
-                             st);                                                                                  \
+                             st, compact);                                                                         \
     }                                                                                                              \
     break;
   switch (tx_size) {

@@ -518,8 +518,12 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   // also when the flags skip it)
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;
   if (flags & kTxuRdoq) {
+    // levels fit 16 bits when both dequantizers are at least 8: the forward
+    // transform's coefficients stay below 2^17 at up to 10 bits, and
+    // quantize_fp's level is at most (|c| + dq / 2) / dq
+    const bool compact = (flags & kTxuLuma) && bd <= 10 && dq0 >= 8 && dq1 >= 8;
     e = rdoq_batch(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
-                   st, rws, rws_bytes);
+                   st, rws, rws_bytes, compact);
     if (e != hipSuccess) return e;
   }
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[1], st)) != hipSuccess) return e;

@@ -948,8 +948,9 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     w->cut_num = c.rdoq_level == 1 ? 0 : (c.rdoq_level == 2 ? 80 : 60);
     w->cut_den = c.rdoq_level == 1 ? 0 : 100;
   }
-  // the luma rate approximation prices MDS1 and MDS3 alike
+  w->flags3 |= zav1::kTxuLuma;  // every MDS3 unit is luma
   w->flags1 = ZAV1_TXU_QUANT_B;
+  // the luma rate approximation prices MDS1 and MDS3 alike
   if (c.coeff_rate_est_lvl == 2) {
     w->flags1 |= ZAV1_TXU_RATE_EST2;
     w->flags3 |= ZAV1_TXU_RATE_EST2;

@@ -209,12 +209,14 @@ struct RateJob {
 static_assert(sizeof(RateJob) == 12, "RateJob layout");
 hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* eobs, const RateJob* jobs,
                             int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t* rate, hipStream_t st);
-// The trellis: the wave form, or with a workspace of rdoq_workspace bytes
-// and at least rdoq_lane_min(tx_size) jobs the lane form.
+// The trellis: the group / wave form, or with a workspace of rdoq_workspace
+// bytes and at least rdoq_lane_min(tx_size) jobs the lane form.  compact:
+// the caller guarantees every job is luma and every |level| < 2^15 (the
+// group form then stages one plane's tables and 16-bit levels).
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws = nullptr,
-                      size_t ws_bytes = 0);
+                      size_t ws_bytes = 0, bool compact = false);
 size_t rdoq_workspace(int tx_size, int n_jobs);
 int rdoq_lane_min(int tx_size);
 void rdoq_set_lane_min(int tx_size, int n);
@@ -272,7 +274,8 @@ struct TxUnitJob {
   uint32_t pred_off;
   uint8_t tx_type, crop_w, crop_h, pad;
 };
-enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u, kTxuRdoq = 8u, kTxuRateEst2 = 16u };
+// kTxuLuma: every job is a luma unit (lets the trellis stage one plane's tables)
+enum : unsigned { kTxuQuantB = 1u, kTxuSpatial = 2u, kTxuRecon = 4u, kTxuRdoq = 8u, kTxuRateEst2 = 16u, kTxuLuma = 32u };
 // Intermediates a transform unit keeps for the RDOQ trellis and the
 // finishing pass (all null for the plain fused unit): the packed pre-quant
 // coefficients and dequantized values (min(w,32) x min(h,32) per job), the
