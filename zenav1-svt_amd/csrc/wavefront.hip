@@ -218,24 +218,34 @@ __global__ __launch_bounds__(256) void k_wf_commit(const MdUnit* __restrict__ un
                                                    int32_t* __restrict__ levels,
                                                    const uint64_t* __restrict__ level_off) {
   __shared__ int win_job;
+  __shared__ uint64_t s_full[64];
+  __shared__ int s_t[64];
   const int u = blockIdx.x;
   if (u >= n) return;
+  // each survivor's tx-type pick on its own lane (k3 <= 64), then the
+  // first-in-order least full cost
+  if (threadIdx.x < k3) {
+    const int i = threadIdx.x;
+    const int r = m3[u * k3 + i];
+    const int c = sel[u * k1 + r];
+    const long j0 = (long)(u * k3 + i) * nt;
+    uint64_t tc;
+    const int t = txt_pick(nt, types, txt + cands[c].dir * 16, dist3 + j0, rate3 + j0, eob3 + j0, csatd3 + j0,
+                           lambda, rate_th, satd_th, &tc);
+    s_full[i] = rdc(lambda, (uint64_t)mode_rate[u * nc + c] + (uint32_t)rate3[j0 + t], dist3[j0 + t]);
+    s_t[i] = t;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     uint64_t best_full = ~0ull;
     int bi = 0, bt = 0;
     uint64_t bcost = 0;
     for (int i = 0; i < k3; i++) {
-      const int r = m3[u * k3 + i];
-      const int c = sel[u * k1 + r];
-      const long j0 = (long)(u * k3 + i) * nt;
-      uint64_t tc;
-      const int t = txt_pick(nt, types, txt + cands[c].dir * 16, dist3 + j0, rate3 + j0, eob3 + j0, csatd3 + j0,
-                             lambda, rate_th, satd_th, &tc);
-      const uint64_t full = rdc(lambda, (uint64_t)mode_rate[u * nc + c] + (uint32_t)rate3[j0 + t], dist3[j0 + t]);
+      const uint64_t full = s_full[i];
       if (full < best_full) {
         best_full = full;
         bi = i;
-        bt = t;
+        bt = s_t[i];
         bcost = full;
       }
     }
