@@ -209,7 +209,8 @@ __global__ __launch_bounds__(256) void k_txunit(const Pix* __restrict__ src, int
 #pragma unroll
     for (int c = 0; c < W; c++) blk[t * G::LS + c] = (t >= 32 || c >= 32) ? 0 : dk[c < G::PW ? c : 0];
   }
-  if (keep.t) store_packed(keep.dq);
+  // (the dequantized values are not kept: the finish and the trellis
+  // recompute them from the levels, sign * ((|q| * dequant) >> log_scale))
   fdist = group_sum_u64<G::T>(fdist);
   if constexpr (W == 64 || H == 64) fold = group_sum_u64<G::T>(fold);
   if (keep.csatd) csatd = group_sum_u64<G::T>(csatd);
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
                                                      const uint16_t* __restrict__ pred,
                                                      const TxUnitRdJob* __restrict__ jobs, int n_jobs,
                                                      const int32_t* __restrict__ tq, const int32_t* __restrict__ tt,
-                                                     const int32_t* __restrict__ tdq,
+                                                     const QuantRow* __restrict__ row_p,
                                                      const uint16_t* __restrict__ eob16,
                                                      const uint64_t* __restrict__ fold,
                                                      const int32_t* __restrict__ costs, int txs_ctx, unsigned flags,
@@ -370,13 +371,15 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
   // distortion is spatial)
   const bool spatial = flags & kTxuSpatial;
   if (active) {
+    const QuantRow& qr = *row_p;
     for (int e = t; e < NP; e += G::T) {
-      const int32_t d = tdq[base + e];
+      const int32_t q = tq[base + e];
+      const int32_t dm = dequant_level(q < 0 ? -q : q, qr, e != 0, LSC);
+      const int32_t d = q < 0 ? -dm : dm;
       if (!spatial) {
         const int64_t df = (int64_t)tt[base + e] - d;
         fd += (uint64_t)(df * df);
       }
-      const int32_t q = tq[base + e];
       lsum += q < 0 ? -q : q;
       if (out_q) out_q[base + e] = q;
       blk[(e / G::PW) * G::LS + e % G::PW] = d;
@@ -457,7 +460,7 @@ size_t txunit_rd_workspace(int tx_size, int n_jobs) {
 
 template <int S>
 static hipError_t launch_finish(const void* src, int stride, int bd, const uint16_t* pred, const TxUnitRdJob* jobs,
-                                int n_jobs, const int32_t* tq, const int32_t* tt, const int32_t* tdq,
+                                int n_jobs, const int32_t* tq, const int32_t* tt, const QuantRow* row,
                                 const uint16_t* eob16, const uint64_t* fold, const int32_t* costs, int txs_ctx,
                                 unsigned flags, uint64_t* dist, int32_t* eob, int32_t* rate, uint8_t* cul,
                                 int32_t* q, uint16_t* rec, hipStream_t st) {
@@ -465,11 +468,11 @@ static hipError_t launch_finish(const void* src, int stride, int bd, const uint1
   const int grid = (n_jobs + TuGeom<W, H>::JPW - 1) / TuGeom<W, H>::JPW;
   if (bd == 8)
     hipLaunchKernelGGL((k_unit_finish<W, H, uint8_t>), dim3(grid), dim3(256), 0, st, (const uint8_t*)src, stride, bd,
-                       pred, jobs, n_jobs, tq, tt, tdq, eob16, fold, costs, txs_ctx, flags, dist, eob, rate, cul, q,
+                       pred, jobs, n_jobs, tq, tt, row, eob16, fold, costs, txs_ctx, flags, dist, eob, rate, cul, q,
                        rec);
   else
     hipLaunchKernelGGL((k_unit_finish<W, H, uint16_t>), dim3(grid), dim3(256), 0, st, (const uint16_t*)src, stride,
-                       bd, pred, jobs, n_jobs, tq, tt, tdq, eob16, fold, costs, txs_ctx, flags, dist, eob, rate, cul,
+                       bd, pred, jobs, n_jobs, tq, tt, row, eob16, fold, costs, txs_ctx, flags, dist, eob, rate, cul,
                        q, rec);
   return hipGetLastError();
 }
@@ -536,7 +539,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   switch (tx_size) {
 #define X(S)                                                                                                        \
   case S:                                                                                                           \
-    return launch_finish<S>(src, stride, bd, pred, jobs, n_jobs, tq, keep.t, keep.dq, keep.eob16, keep.fold, costs, \
+    return launch_finish<S>(src, stride, bd, pred, jobs, n_jobs, tq, keep.t, row, keep.eob16, keep.fold, costs,     \
                             txs_ctx, flags, dist, eob, rate, cul, qcoeff, recon, st);
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
 #undef X
