@@ -656,35 +656,26 @@ __global__ __launch_bounds__(64 * kWaveBlocks) void k_rdoq_wave(
 // CMP (compact): every job is luma and every level fits 16 bits (the
 // caller's guarantee), so the workgroup stages one plane's cost tables and
 // 16-bit levels -- less LDS per block, more blocks resident per CU.
-template <int TXS, int G, int WPG, bool CMP>
-__global__ __launch_bounds__(64 * WPG) void k_rdoq_group(
-    const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
-    uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, int n_jobs, const int32_t* __restrict__ costs,
-    const uint16_t* __restrict__ scans, int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
-  using QT = typename std::conditional<CMP, int16_t, int32_t>::type;
+// One block of the group form on the G lanes of its group (lane g of the
+// group, group grp of the wave): stage it into the group's LDS slot, the
+// eob phase, the skip decision, the fixpoint walk and the DC step.  live:
+// j names a job.  Every lane of the wave calls it together (ballots, DPP
+// sums, wave syncs); a group without a live, defined job takes part with
+// no effect.
+template <int TXS, int G, bool CMP, class QT>
+__device__ __forceinline__ void group_trellis(int j, bool live, int lane, int g, int grp,
+                                              const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff,
+                                              int32_t* __restrict__ dqcoeff, uint16_t* __restrict__ eobs,
+                                              const RdoqJob* __restrict__ jobs, const int32_t* tab,
+                                              const uint16_t* sc, int32_t* tsb, QT* qsb, uint8_t* lvbb, int32_t dq0,
+                                              int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
   using T = Trellis<TXS, WaveMapQ<QT>>;
-  constexpr int N = T::N, BPW = 64 / G, BPG = WPG * BPW, K = N / G, NT = 64 * WPG;
-  constexpr int TXB = CMP ? kTxbWords : 2 * kTxbWords, EOBW = CMP ? kEobWords : 2 * kEobWords;
-  static_assert(N % G == 0 && K <= 32, "positions of a lane as bits of one word");
-  static_assert(G <= 16, "group sums by DPP within a row of 16 lanes");
-  __shared__ int32_t tab[TXB + EOBW];
-  __shared__ uint16_t sc[3 * N];
-  __shared__ int32_t ts[BPG][N];
-  __shared__ QT qs[BPG][N];
-  __shared__ uint8_t lvb[BPG][N];
-  const int tid = threadIdx.x, lane = tid & 63, g = lane % G, grp = lane / G, b = tid / G;
-  // the size's tables: both planes, or the luma plane's (the first of each pair)
-  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
-  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
-  for (int i = tid; i < TXB; i += NT) tab[i] = txb_src[i];
-  for (int i = tid; i < EOBW; i += NT) tab[TXB + i] = eob_src[i];
-  for (int i = tid; i < 3 * N; i += NT) sc[i] = scans[i];
-  __syncthreads();
-  const int j = blockIdx.x * BPG + b;
+  constexpr int N = T::N, K = N / G;
+  constexpr int TXB = CMP ? kTxbWords : 2 * kTxbWords;
   RdoqJob job{};
   int eob = 0;
   bool act = false;
-  if (j < n_jobs) {
+  if (live) {
     job = jobs[j];
     eob = eobs[j];
     act = job_ok<TXS>(job, eob) && (!CMP || job.plane_type == 0);
@@ -693,16 +684,16 @@ __global__ __launch_bounds__(64 * WPG) void k_rdoq_group(
   if (act)
     for (int p = g; p < N; p += G) {
       const int32_t t = tcoeff[job.coeff + p], q = qcoeff[job.coeff + p];
-      ts[b][p] = t;
-      qs[b][p] = (QT)q;
+      tsb[p] = t;
+      qsb[p] = (QT)q;
       const int a = q < 0 ? -q : q;
-      lvb[b][p] = (uint8_t)(a < 15 ? a : 15);
+      lvbb[p] = (uint8_t)(a < 15 ? a : 15);
     }
   wave_sync();
   T s;
-  s.m.lv = lvb[b];
-  s.m.ts = ts[b];
-  s.m.qs = qs[b];
+  s.m.lv = lvbb;
+  s.m.ts = tsb;
+  s.m.qs = qsb;
   s.qc = qcoeff + job.coeff;
   s.dq = dqcoeff + job.coeff;
   s.cost = tab + (job.plane_type ? kTxbWords : 0);
@@ -821,6 +812,76 @@ __global__ __launch_bounds__(64 * WPG) void k_rdoq_group(
       s.update_general(&ar, &dummy, 0, eob);
     }
     if (g == 0) eobs[j] = (uint16_t)eob;
+  }
+}
+
+template <int TXS, int G, int WPG, bool CMP>
+__global__ __launch_bounds__(64 * WPG) void k_rdoq_group(
+    const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
+    uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, int n_jobs, const int32_t* __restrict__ costs,
+    const uint16_t* __restrict__ scans, int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
+  using QT = typename std::conditional<CMP, int16_t, int32_t>::type;
+  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = 64 / G, BPG = WPG * BPW, K = N / G, NT = 64 * WPG;
+  constexpr int TXB = CMP ? kTxbWords : 2 * kTxbWords, EOBW = CMP ? kEobWords : 2 * kEobWords;
+  static_assert(N % G == 0 && K <= 32, "positions of a lane as bits of one word");
+  static_assert(G <= 16, "group sums by DPP within a row of 16 lanes");
+  __shared__ int32_t tab[TXB + EOBW];
+  __shared__ uint16_t sc[3 * N];
+  __shared__ int32_t ts[BPG][N];
+  __shared__ QT qs[BPG][N];
+  __shared__ uint8_t lvb[BPG][N];
+  const int tid = threadIdx.x, lane = tid & 63, g = lane % G, grp = lane / G, b = tid / G;
+  // the size's tables: both planes, or the luma plane's (the first of each pair)
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = tid; i < TXB; i += NT) tab[i] = txb_src[i];
+  for (int i = tid; i < EOBW; i += NT) tab[TXB + i] = eob_src[i];
+  for (int i = tid; i < 3 * N; i += NT) sc[i] = scans[i];
+  __syncthreads();
+  const int j = blockIdx.x * BPG + b;
+  group_trellis<TXS, G, CMP, QT>(j, j < n_jobs, lane, g, grp, tcoeff, qcoeff, dqcoeff, eobs, jobs, tab, sc, ts[b],
+                                 qs[b], lvb[b], dq0, dq1, cut_num, cut_den);
+}
+
+// The persistent compact group form fed by a device work queue: queue[0 ..
+// ctr[0]) holds the jobs to walk (written by the pre-trellis gate: the jobs
+// it passes with a nonzero eob), ctr[1] is the queue head.  A grid of about
+// the device's resident capacity stages the cost tables once per workgroup,
+// then each wave takes the next 64 / G jobs (one atomic per wave) until the
+// queue is empty -- every wave reaches that exit.  Jobs outside the queue
+// are left as they are, as the direct form leaves the jobs it skips.
+template <int TXS, int G, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_rdoq_queue(
+    const int32_t* __restrict__ tcoeff, int32_t* __restrict__ qcoeff, int32_t* __restrict__ dqcoeff,
+    uint16_t* __restrict__ eobs, const RdoqJob* __restrict__ jobs, const int32_t* __restrict__ queue,
+    unsigned* __restrict__ ctr, const int32_t* __restrict__ costs, const uint16_t* __restrict__ scans, int32_t dq0,
+    int32_t dq1, uint32_t cut_num, uint32_t cut_den) {
+  using QT = int16_t;
+  constexpr int N = c_txw(TXS) * c_txh(TXS), BPW = 64 / G, BPG = WPG * BPW, K = N / G, NT = 64 * WPG;
+  static_assert(N % G == 0 && K <= 32 && G <= 16, "group form shape");
+  __shared__ int32_t tab[kTxbWords + kEobWords];
+  __shared__ uint16_t sc[3 * N];
+  __shared__ int32_t ts[BPG][N];
+  __shared__ QT qs[BPG][N];
+  __shared__ uint8_t lvb[BPG][N];
+  const int tid = threadIdx.x, lane = tid & 63, g = lane % G, grp = lane / G, b = tid / G;
+  const int32_t* txb_src = costs + c_txs_ctx(TXS) * 2 * kTxbWords;
+  const int32_t* eob_src = costs + 10 * kTxbWords + c_eob_ms(TXS) * 2 * kEobWords;
+  for (int i = tid; i < kTxbWords; i += NT) tab[i] = txb_src[i];
+  for (int i = tid; i < kEobWords; i += NT) tab[kTxbWords + i] = eob_src[i];
+  for (int i = tid; i < 3 * N; i += NT) sc[i] = scans[i];
+  __syncthreads();
+  const unsigned qlen = ctr[0];
+  for (;;) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&ctr[1], (unsigned)BPW);
+    base = (unsigned)__shfl((int)base, 0, 64);
+    if (base >= qlen) break;  // wave-uniform
+    const unsigned qi = base + (unsigned)grp;
+    const int j = qi < qlen ? queue[qi] : -1;
+    group_trellis<TXS, G, true, QT>(j, j >= 0, lane, g, grp, tcoeff, qcoeff, dqcoeff, eobs, jobs, tab, sc, ts[b],
+                                    qs[b], lvb[b], dq0, dq1, cut_num, cut_den);
+    wave_sync();  // the group's LDS slot is rewritten by the next job
   }
 }
 
@@ -1027,12 +1088,23 @@ template <int TXS>
 static void launch_latency_form(const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                                 const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans,
                                 int32_t dq0, int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st,
-                                bool compact) {
+                                bool compact, const int32_t* queue, unsigned* qctr) {
   constexpr int N = c_txw(TXS) * c_txh(TXS);
   if constexpr (N <= 256) {
     // 12 blocks per workgroup in the compact form (26.9 KB of LDS at 16x16:
     // six workgroups per CU), 16 otherwise (46.1 KB: three)
-    if (compact) {
+    if (compact && queue) {
+      // persistent: about the resident capacity (six workgroups per CU),
+      // never more workgroups than the jobs could fill
+      constexpr int BPG = 3 * (64 / 16);
+      int dev = 0, cus = 256;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+      const int grid = std::max(1, std::min((n_jobs + BPG - 1) / BPG, cus * 6));
+      hipLaunchKernelGGL((k_rdoq_queue<TXS, 16, 3>), dim3((unsigned)grid), dim3(192), 0, st, tcoeff, qcoeff, dqcoeff,
+                         eobs, jobs, queue, qctr, costs, scans, dq0, dq1, cut_num, cut_den);
+    } else if (compact) {
       constexpr int BPG = 3 * (64 / 16);
       hipLaunchKernelGGL((k_rdoq_group<TXS, 16, 3, true>), dim3((unsigned)((n_jobs + BPG - 1) / BPG)), dim3(192), 0,
                          st, tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den);
@@ -1051,7 +1123,7 @@ static void launch_latency_form(const int32_t* tcoeff, int32_t* qcoeff, int32_t*
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws, size_t ws_bytes,
-                      bool compact) {
+                      bool compact, const int32_t* queue, unsigned* qctr) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const bool lane = ws && ws_bytes >= rdoq_workspace(tx_size, n_jobs) && n_jobs >= g_lane_min[tx_size];
@@ -1069,7 +1141,7 @@ hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32
                          costs, scans, dq0, dq1, cut_num, cut_den);                                               \
     } else {                                                                                                       \
       launch_latency_form<T>(tcoeff, qcoeff, dqcoeff, eobs, jobs, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den, \
-                             st, compact);                                                                         \
+                             st, compact, queue, qctr);                                                            \
     }                                                                                                              \
     break;
   switch (tx_size) {

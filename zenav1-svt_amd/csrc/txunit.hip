@@ -297,8 +297,9 @@ __device__ inline uint8_t tx_class_of(uint8_t tt) { return tt >= 10 ? ((tt & 1) 
 
 __global__ __launch_bounds__(256) void k_rd_prep(const TxUnitRdJob* __restrict__ jobs, int n_jobs, int np,
                                                  TxUnitJob* __restrict__ tj, RdoqJob* __restrict__ rj,
-                                                 RateJob* __restrict__ cj) {
+                                                 RateJob* __restrict__ cj, unsigned* __restrict__ qctr) {
   const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j == 0) qctr[0] = qctr[1] = 0;  // the trellis queue's length and head, for this batch
   if (j >= n_jobs) return;
   const TxUnitRdJob a = jobs[j];
   TxUnitJob u;
@@ -422,9 +423,12 @@ __global__ __launch_bounds__(256) void k_unit_finish(const Pix* __restrict__ src
 }
 
 // The pre-trellis SATD gate (TuSatdGate): per job against its group's first.
+// With `queue`, the jobs it hands to the trellis are also appended to the
+// trellis's work queue (queue[0 .. qctr[0]); order within it is free).
 __global__ __launch_bounds__(256) void k_tu_satd_gate(int n_jobs, int group, const uint64_t* __restrict__ csatd,
                                                       int64_t th, uint16_t* __restrict__ eob16,
-                                                      int32_t* __restrict__ eob, unsigned long long* runs) {
+                                                      int32_t* __restrict__ eob, unsigned long long* runs,
+                                                      int32_t* __restrict__ queue, unsigned* __restrict__ qctr) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   bool run = false;
   if (j < n_jobs && eob[j] >= 0) {
@@ -440,9 +444,14 @@ __global__ __launch_bounds__(256) void k_tu_satd_gate(int n_jobs, int group, con
     }
     run = !gated && eob16[j] > 0;
   }
-  if (runs) {
-    const uint64_t m = __ballot(run);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(runs, (unsigned long long)__popcll(m));
+  const uint64_t m = __ballot(run);
+  const int lane = threadIdx.x & 63;
+  if (runs && lane == 0 && m) atomicAdd(runs, (unsigned long long)__popcll(m));
+  if (queue && m) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&qctr[0], (unsigned)__popcll(m));
+    base = (unsigned)__shfl((int)base, 0, 64);
+    if (run) queue[base + (unsigned)__popcll(m & ((1ull << lane) - 1))] = j;
   }
 }
 
@@ -455,7 +464,7 @@ size_t txunit_rd_workspace(int tx_size, int n_jobs) {
   if (tx_size < 0 || tx_size > 18 || n_jobs <= 0) return 0;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
   return 3 * align256(n * np * 4) + align256(n * 2) + align256(n * 8) + align256(n * sizeof(TxUnitJob)) +
-         align256(n * sizeof(RdoqJob)) + align256(n * sizeof(RateJob));
+         align256(n * sizeof(RdoqJob)) + align256(n * sizeof(RateJob)) + align256(n * 4) + 256;
 }
 
 template <int S>
@@ -504,29 +513,34 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   TxUnitJob* tj = reinterpret_cast<TxUnitJob*>(carve(n * sizeof(TxUnitJob)));
   RdoqJob* rj = reinterpret_cast<RdoqJob*>(carve(n * sizeof(RdoqJob)));
   RateJob* cj = reinterpret_cast<RateJob*>(carve(n * sizeof(RateJob)));
+  int32_t* queue = reinterpret_cast<int32_t*>(carve(n * 4));      // the trellis's work queue
+  unsigned* qctr = reinterpret_cast<unsigned*>(carve(16));        // its length and head
   // workspace past the unit's own temporaries goes to the trellis (its lane
   // form when the batch is large enough; rdoq_batch checks the size)
   const size_t used = (size_t)(w - static_cast<char*>(work));
   void* rws = work_bytes > used ? static_cast<void*>(w) : nullptr;
   const size_t rws_bytes = work_bytes > used ? work_bytes - used : 0;
   hipLaunchKernelGGL(k_rd_prep, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, jobs, n_jobs, (int)np, tj,
-                     rj, cj);
+                     rj, cj, qctr);
   hipError_t e = txunit_batch(src, stride, plane_w, plane_h, bd, tx_size, pred, pred_len, tj, n_jobs, row, iscan,
                               flags & kTxuQuantB, dist, eob, cul, tq, nullptr, st, keep);
   if (e != hipSuccess) return e;
-  if (gate && csatd && gate->group > 0 && (gate->runs || (gate->group > 1 && gate->th > 0)))
+  // levels fit 16 bits when both dequantizers are at least 8: the forward
+  // transform's coefficients stay below 2^17 at up to 10 bits, and
+  // quantize_fp's level is at most (|c| + dq / 2) / dq
+  const bool compact = (flags & kTxuRdoq) && (flags & kTxuLuma) && bd <= 10 && dq0 >= 8 && dq1 >= 8;
+  // the gate also fills the trellis's work queue when the trellis drains one
+  const bool queued = compact && gate && csatd && gate->group > 0;
+  if (gate && csatd && gate->group > 0 && (queued || gate->runs || (gate->group > 1 && gate->th > 0)))
     hipLaunchKernelGGL(k_tu_satd_gate, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, n_jobs, gate->group,
-                       csatd, gate->group > 1 ? gate->th : (int64_t)0, keep.eob16, eob, gate->runs);
+                       csatd, gate->group > 1 ? gate->th : (int64_t)0, keep.eob16, eob, gate->runs,
+                       queued ? queue : nullptr, qctr);
   // rdoq_ev: two events recorded around the trellis (timing; recorded
   // also when the flags skip it)
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;
   if (flags & kTxuRdoq) {
-    // levels fit 16 bits when both dequantizers are at least 8: the forward
-    // transform's coefficients stay below 2^17 at up to 10 bits, and
-    // quantize_fp's level is at most (|c| + dq / 2) / dq
-    const bool compact = (flags & kTxuLuma) && bd <= 10 && dq0 >= 8 && dq1 >= 8;
     e = rdoq_batch(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
-                   st, rws, rws_bytes, compact);
+                   st, rws, rws_bytes, compact, queued ? queue : nullptr, qctr);
     if (e != hipSuccess) return e;
   }
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[1], st)) != hipSuccess) return e;

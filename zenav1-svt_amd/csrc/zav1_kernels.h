@@ -212,11 +212,15 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
 // The trellis: the group / wave form, or with a workspace of rdoq_workspace
 // bytes and at least rdoq_lane_min(tx_size) jobs the lane form.  compact:
 // the caller guarantees every job is luma and every |level| < 2^15 (the
-// group form then stages one plane's tables and 16-bit levels).
+// group form then stages one plane's tables and 16-bit levels).  queue /
+// qctr (compact only): a device work queue of the jobs to walk (queue[0 ..
+// qctr[0]), head qctr[1] = 0), which the persistent form drains; jobs
+// outside it are left as they are.
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws = nullptr,
-                      size_t ws_bytes = 0, bool compact = false);
+                      size_t ws_bytes = 0, bool compact = false, const int32_t* queue = nullptr,
+                      unsigned* qctr = nullptr);
 size_t rdoq_workspace(int tx_size, int n_jobs);
 int rdoq_lane_min(int tx_size);
 void rdoq_set_lane_min(int tx_size, int n);
