@@ -503,6 +503,9 @@ def config4_stage(zav1, synth, args):
     st = zav1.stream_create()
     md.run(True, st)
     zav1.stream_sync(st)
+    tuned = md.autotune(stream=st)
+    md.run(True, st)
+    zav1.stream_sync(st)
     steps = max(2, args.steps // 2)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -518,7 +521,7 @@ def config4_stage(zav1, synth, args):
     zav1.stream_destroy(st)
     return {'metric': 'MP/s through the wavefront mode decision, 16 tiles co-scheduled',
             'value': round(W * H / 1e6 / (ms / 1000.0), 2), 'unit': 'MP/s', 'ms_per_step': round(ms, 2),
-            'steps': steps, 'graph_vs_eager_identical': same, 'wavefront_steps': info.n_steps,
+            'steps': steps, 'graph_vs_eager_identical': same, 'wavefront_steps': info.n_steps, 'autotune': tuned,
             'max_leaves_per_step': info.max_leaves_per_step,
             'config': {'workload': 'config4: 8192x4096 10-bit luma (1/f noise + 25%% screen regions), SB128, 4x4 '
                                    'tiles, preset 2 (%d candidates, %d -> %d survivors), qp 28, 16x16 leaves'
@@ -656,6 +659,11 @@ def wavefront_headline(zav1, synth, args):
     for _ in range(max(1, args.warmup)):
         md.run(use_graph, st)
     zav1.stream_sync(st)
+    # the plan's autotuner: the trellis form per leaf size, timed on a window
+    # of its own steps (forms give identical levels); then the graph again
+    tuned = md.autotune(stream=st)
+    md.run(use_graph, st)
+    zav1.stream_sync(st)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         md.run(use_graph, st)
@@ -752,6 +760,7 @@ def wavefront_headline(zav1, synth, args):
         'md_only': {'value': round(value, 2), 'unit': 'MP/s', 'ms_per_step': round(ms_step, 3),
                     'what': 'the mode decision alone (graph replays of one plan, no streams)'},
         'forms_identical': {'graph_vs_eager': same_forms, 'plain_sample_vs_batched': plain_same},
+        'autotune': tuned,
         'stage_ms': {k: round(v, 3) for k, v in stage_ms.items()},
         'eager_timed_pass_ms': round(sum(stage_ms.values()), 3),
         'roofline': roof,
@@ -799,6 +808,8 @@ def encoded_stage(zav1, wf, imgs, cfg, args):
         zav1.stream_sync(st)
     ref = []
     for p, st in zip(plans, sts):
+        p.run(True, st)
+        p.autotune(stream=st)  # the trellis form per leaf size (identical results)
         for _ in range(max(1, args.warmup)):
             p.run(True, st)
             got = p.write_stills(threads=threads, stream=st)

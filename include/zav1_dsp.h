@@ -678,6 +678,17 @@ int zav1_wf_run_steps(zav1_wf *wf, int first_step, int n_steps, zav1_stream_t s)
  * The decisions equal zav1_wf_run's.  No counterpart in the reference
  * (measurement). */
 #define ZAV1_WF_STAGES 11
+/* The plan's autotuner: times the RDOQ trellis of steps [first_step,
+ * first_step + n_steps) under each of its forms -- 0 the persistent form fed
+ * by a device work queue, 1 the direct group / wave form, 2 the lane form --
+ * and keeps the fastest per leaf size (8, 16, 32, 64 -> forms[0..3]; -1
+ * where the size has no step in the window or no trellis runs; ms[k * 3 +
+ * f], when not NULL, the measured milliseconds).  Every form gives the same
+ * levels, so the decisions do not change; the window's leaves are decided
+ * again (run the plan once first so their neighbours are decided).  A graph
+ * captured before is dropped and captured again by the next graph run.  No
+ * counterpart in the reference (tuning). */
+int zav1_wf_autotune(zav1_wf *wf, int first_step, int n_steps, zav1_stream_t s, int32_t *forms, double *ms);
 int zav1_wf_run_timed(zav1_wf *wf, zav1_stream_t s, double *stage_ms, int n_stages);
 /* The plain per-call form of the same decisions for leaves
  * [first_leaf, first_leaf + n_leaves) in leaf order (earlier leaves must be

@@ -128,3 +128,32 @@ def test_every_leaf_size_sb128(gpu, oracle):
     decide(gpu, oracle, [img8], 8, 28, 4, lm, sb=128, graph=False)
     img10 = synth.config2_luma(256, 256, seed=2003)
     decide(gpu, oracle, [img10], 10, 50, 8, lm, sb=128, graph=False)
+
+
+def test_autotune_keeps_the_decisions(gpu, oracle):
+    """The autotuner times every trellis form per leaf size and keeps the
+    fastest; the forms give the same levels, so the decisions, the
+    reconstruction and a graph captured after tuning equal the untuned
+    run's (config-2 content, 10 bits, PD0 tree at preset 4: four sizes)."""
+    imgs = [synth.config2_luma(256, 256, seed=2400 + k) for k in range(2)]
+    lm = pd0_maps(gpu, imgs, 10, 28, 4)
+    qidx = synth.qp_to_qindex(28)
+    cfg = wf.tree_cfg(256, 256, 10, qidx, int(gpu.lib().zav1_kf_full_lambda(qidx, 28, 10)), 4, 28, lm, n_images=2)
+    md = wf.WavefrontMD(gpu, cfg)
+    md.upload(np.stack(imgs))
+    st = gpu.stream_create()
+    md.run(True, st)
+    gpu.stream_sync(st)
+    rec, dec = md.recon(), md.raw_decisions()
+    tuned = md.autotune(0, md.info.n_steps, st)
+    print('autotune:', tuned, flush=True)
+    assert tuned['forms'] and all(f in wf.WavefrontMD.FORMS for f in tuned['forms'].values())
+    assert np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec)
+    md.run(True, st)   # a new graph with the tuned forms
+    gpu.stream_sync(st)
+    assert np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec)
+    md.run(False, st)
+    gpu.stream_sync(st)
+    assert np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec)
+    md.close()
+    gpu.stream_destroy(st)

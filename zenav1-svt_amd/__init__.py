@@ -137,6 +137,7 @@ EXPORTS = {
     'zav1_wiener_apply': [_vp, _vp] + [_i32] * 5 + [_vp, _vp],
     'zav1_wiener_refine': [_vp, _vp] + [_i32] * 8 + [_vp, _vp, _vp],
     'zav1_wf_run_timed': [_vp, _vp, _vp, _i32],
+    'zav1_wf_autotune': [_vp, _i32, _i32, _vp, _vp, _vp],
     'zav1_rdoq_batch': [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                         _vp],
     'zav1_sse_i32_batch': [_vp, _vp, _vp, _i32, _i32, _vp],

@@ -1123,10 +1123,12 @@ static void launch_latency_form(const int32_t* tcoeff, int32_t* qcoeff, int32_t*
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws, size_t ws_bytes,
-                      bool compact, const int32_t* queue, unsigned* qctr) {
+                      bool compact, const int32_t* queue, unsigned* qctr, int form) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
-  const bool lane = ws && ws_bytes >= rdoq_workspace(tx_size, n_jobs) && n_jobs >= g_lane_min[tx_size];
+  const bool ws_ok = ws && ws_bytes >= rdoq_workspace(tx_size, n_jobs);
+  const bool lane = form == 2 ? ws_ok : (form < 0 && ws_ok && n_jobs >= g_lane_min[tx_size]);
+  if (form == 1) queue = nullptr;  // the direct latency form
   const int n = std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32);
   int2* work = static_cast<int2*>(ws);
   uint32_t* lvwork = lane ? reinterpret_cast<uint32_t*>(work + (size_t)n_jobs * n) : nullptr;

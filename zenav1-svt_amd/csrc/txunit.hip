@@ -492,7 +492,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            unsigned flags, uint32_t cut_num, uint32_t cut_den, void* work, uint64_t* dist,
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
                            hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd, hipEvent_t* rdoq_ev,
-                           size_t work_bytes, const TuSatdGate* gate) {
+                           size_t work_bytes, const TuSatdGate* gate, int rdoq_form) {
   if (n_jobs <= 0) return hipSuccess;
   if (tx_size < 0 || tx_size > 18) return hipErrorInvalidValue;
   const size_t np = (size_t)std::min(tx_w(tx_size), 32) * std::min(tx_h(tx_size), 32), n = (size_t)n_jobs;
@@ -530,7 +530,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   // quantize_fp's level is at most (|c| + dq / 2) / dq
   const bool compact = (flags & kTxuRdoq) && (flags & kTxuLuma) && bd <= 10 && dq0 >= 8 && dq1 >= 8;
   // the gate also fills the trellis's work queue when the trellis drains one
-  const bool queued = compact && gate && csatd && gate->group > 0;
+  const bool queued = compact && gate && csatd && gate->group > 0 && rdoq_form != 1 && rdoq_form != 2;
   if (gate && csatd && gate->group > 0 && (queued || gate->runs || (gate->group > 1 && gate->th > 0)))
     hipLaunchKernelGGL(k_tu_satd_gate, dim3((unsigned)((n_jobs + 255) / 256)), dim3(256), 0, st, n_jobs, gate->group,
                        csatd, gate->group > 1 ? gate->th : (int64_t)0, keep.eob16, eob, gate->runs,
@@ -540,7 +540,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[0], st)) != hipSuccess) return e;
   if (flags & kTxuRdoq) {
     e = rdoq_batch(tx_size, keep.t, tq, keep.dq, keep.eob16, rj, n_jobs, costs, scans, dq0, dq1, cut_num, cut_den,
-                   st, rws, rws_bytes, compact, queued ? queue : nullptr, qctr);
+                   st, rws, rws_bytes, compact, queued ? queue : nullptr, qctr, rdoq_form);
     if (e != hipSuccess) return e;
   }
   if (rdoq_ev && (e = hipEventRecord(rdoq_ev[1], st)) != hipSuccess) return e;

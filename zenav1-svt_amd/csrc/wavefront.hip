@@ -309,6 +309,7 @@ struct SizePlan {
   uint8_t *cul1 = nullptr, *cul3 = nullptr;
   void *ws1 = nullptr, *ws3 = nullptr;
   size_t ws3_bytes = 0;  // MDS3's workspace, with room for the trellis's lane form
+  int rdoq_form = -1;    // the trellis form (rdoq_batch's form; -1 its defaults), set by the autotuner
 };
 
 struct Wavefront {
@@ -689,7 +690,8 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   const TuSatdGate gate{P.nt, P.satd_th, w->trellis_runs};
   e = txunit_rd_batch(w->src, W, W, H, c.bd, P.tx, P.pred, (long)U * P.k1 * L * L, P.jobs3, n3, w->row, w->iscan,
                       P.fscan, w->costs, w->flags3, w->cut_num, w->cut_den, P.ws3, P.dist3, P.eob3, P.rate3, P.cul3,
-                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes, &gate);
+                      nullptr, P.rec3, st, w->dq0, w->dq1, P.csatd3, clk ? &clk->ev[8] : nullptr, P.ws3_bytes, &gate,
+                      P.rdoq_form);
   if (e != hipSuccess) return e;
   // the commit copies the winners' levels straight from the batch's workspace
   const int32_t* q3 = c.keep_levels ? txunit_rd_levels(P.ws3) : nullptr;
@@ -1155,6 +1157,52 @@ int zav1_wf_run_timed(zav1_wf* wf, zav1_stream_t s, double* stage_ms, int n_stag
   for (int k = 0; k < made; k++)
     if (clk.ev[k]) (void)hipEventDestroy(clk.ev[k]);
   if (e != hipSuccess) return wf_fail(e, "zav1_wf_run_timed");
+  return ZAV1_OK;
+}
+
+int zav1_wf_autotune(zav1_wf* wf, int first_step, int n_steps, zav1_stream_t s, int32_t* forms, double* ms) {
+  if (!wf || !forms) return ZAV1_EINVAL;
+  auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
+  if (first_step < 0 || n_steps < 1 || first_step + n_steps > w->n_steps) return ZAV1_EINVAL;
+  hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
+  const bool rdoq = (w->flags3 & ZAV1_TXU_RDOQ) != 0;
+  constexpr int kForms = 3;  // queued group, direct latency, lane
+  double t[4][kForms] = {};
+  zav1::StageClock clk{};
+  int made = 0;
+  hipError_t e = hipSuccess;
+  for (; made <= ZAV1_WF_STAGES && e == hipSuccess; made++) e = hipEventCreate(&clk.ev[made]);
+  // the window's trellis time per leaf size under each form (re-deciding the
+  // window's leaves gives the results they already have)
+  for (int f = 0; f < kForms && rdoq && e == hipSuccess; f++) {
+    for (int k = 0; k < 4; k++) w->sp[k].rdoq_form = f;
+    for (int i = first_step; i < first_step + n_steps && e == hipSuccess; i++) {
+      if (w->step_off[i + 1] == w->step_off[i]) continue;
+      e = zav1::step_chain(w, i, st, &clk);
+      if (e == hipSuccess) e = hipEventSynchronize(clk.ev[ZAV1_WF_STAGES]);
+      float tm = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&tm, clk.ev[8], clk.ev[9]);
+      t[w->step_sz[i]][f] += tm;
+    }
+  }
+  for (int k = 0; k < made; k++)
+    if (clk.ev[k]) (void)hipEventDestroy(clk.ev[k]);
+  for (int k = 0; k < 4; k++) {
+    zav1::SizePlan& P = w->sp[k];
+    int best = -1;
+    if (P.max_u && rdoq && t[k][0] + t[k][1] + t[k][2] > 0.0)  // the size ran in the window
+      for (int f = 0; f < kForms; f++)
+        if (best < 0 || t[k][f] < t[k][best]) best = f;
+    P.rdoq_form = best;
+    forms[k] = best;
+    if (ms)
+      for (int f = 0; f < kForms; f++) ms[k * kForms + f] = t[k][f];
+  }
+  if (w->graph) {  // captured with the previous forms
+    (void)hipGraphExecDestroy(w->graph);
+    w->graph = nullptr;
+  }
+  if (e != hipSuccess) return wf_fail(e, "zav1_wf_autotune");
   return ZAV1_OK;
 }
 

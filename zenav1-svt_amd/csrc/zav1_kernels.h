@@ -215,12 +215,14 @@ hipError_t coeff_rate_batch(int tx_size, const int32_t* qcoeff, const uint16_t* 
 // group form then stages one plane's tables and 16-bit levels).  queue /
 // qctr (compact only): a device work queue of the jobs to walk (queue[0 ..
 // qctr[0]), head qctr[1] = 0), which the persistent form drains; jobs
-// outside it are left as they are.
+// outside it are left as they are.  form: -1 the defaults above; 0 the
+// queued form where it applies; 1 the direct latency form; 2 the lane form
+// (with the workspace).  Every form gives the same levels.
 hipError_t rdoq_batch(int tx_size, const int32_t* tcoeff, int32_t* qcoeff, int32_t* dqcoeff, uint16_t* eobs,
                       const RdoqJob* jobs, int n_jobs, const int32_t* costs, const uint16_t* scans, int32_t dq0,
                       int32_t dq1, uint32_t cut_num, uint32_t cut_den, hipStream_t st, void* ws = nullptr,
                       size_t ws_bytes = 0, bool compact = false, const int32_t* queue = nullptr,
-                      unsigned* qctr = nullptr);
+                      unsigned* qctr = nullptr, int form = -1);
 size_t rdoq_workspace(int tx_size, int n_jobs);
 int rdoq_lane_min(int tx_size);
 void rdoq_set_lane_min(int tx_size, int n);
@@ -359,7 +361,7 @@ hipError_t txunit_rd_batch(const void* src, int stride, int plane_w, int plane_h
                            int32_t* eob, int32_t* rate, uint8_t* cul, int32_t* qcoeff, uint16_t* recon,
                            hipStream_t st, int32_t dq0, int32_t dq1, uint64_t* csatd = nullptr,
                            hipEvent_t* rdoq_ev = nullptr, size_t work_bytes = 0,
-                           const TuSatdGate* gate = nullptr);
+                           const TuSatdGate* gate = nullptr, int rdoq_form = -1);
 // The final levels of a txunit_rd_batch call stay in its workspace until the
 // workspace is used again: job j's packed levels at txunit_rd_levels(work) +
 // j * min(w, 32) * min(h, 32).

@@ -204,6 +204,28 @@ class WavefrontMD:
             raise self.z.Zav1Error(rc, 'zav1_wf_run_timed (%s)' % self.z.lib().zav1_wf_last_error().decode())
         return dict(zip(self.STAGES, list(out)))
 
+    FORMS = ('queued', 'direct', 'lane')
+
+    def autotune(self, first=None, count=None, stream=None):
+        """The plan's trellis form per leaf size, timed on steps [first,
+        first + count) (default: up to 256 steps from the middle of the
+        plan): {'forms': {leaf: form name}, 'ms': {leaf: {form: ms}}}.  The
+        decisions are unchanged (every form gives the same levels)."""
+        n = self.info.n_steps
+        count = min(256, n) if count is None else count
+        first = max(0, (n - count) // 2) if first is None else first
+        forms = (ctypes.c_int32 * 4)()
+        ms = (ctypes.c_double * 12)()
+        rc = self.z.lib().zav1_wf_autotune(self.h, first, count, stream, forms, ms)
+        if rc:
+            raise self.z.Zav1Error(rc, 'zav1_wf_autotune (%s)' % self.z.lib().zav1_wf_last_error().decode())
+        out = {'forms': {}, 'ms': {}}
+        for k, leaf in enumerate((8, 16, 32, 64)):
+            if forms[k] >= 0:
+                out['forms'][leaf] = self.FORMS[forms[k]]
+                out['ms'][leaf] = {self.FORMS[f]: round(ms[k * 3 + f], 3) for f in range(3)}
+        return out
+
     def run_steps(self, first, count, stream=None):
         rc = self.z.lib().zav1_wf_run_steps(self.h, first, count, stream)
         if rc:
