@@ -74,3 +74,37 @@ def test_serving_rejects_a_request_over_budget(gpu):
     server = srv.WavefrontServer(z, wf, synth, 1 << 16)  # below the plane's own 128 KiB
     with pytest.raises(ValueError):
         server.run([(img, 10, 30, 4)], [0.0])
+
+
+def test_serving_encode_streams_alone_equal_in_mix(gpu, oracle):
+    """Encode mode: every request's AV1 stream, written on the host pool
+    while other requests run, equals the stream of the same request decided
+    and written alone, and decodes (the oracle's decoder) to its
+    reconstruction; a 4096x3072 request takes two tile columns (AV1's tile
+    area limit)."""
+    z = gpu
+    synth, wf, srv = _mod('synth'), _mod('wavefront'), _mod('serving')
+    reqs = REQS[:6] + [(3072, 4096, 10, 36, 8)]
+    imgs = synth.serving_images(reqs, seed=78)
+    specs = [(img, bd, qp, p) for img, (h, w, bd, qp, p) in zip(imgs, reqs)]
+    rng = np.random.default_rng(6)
+    arrivals = np.cumsum(rng.exponential(0.002, len(reqs)))
+    server = srv.WavefrontServer(z, wf, synth, 8 << 30, max_active=4, chunk=16, depth=2, encode=True)
+    done, st = server.run(specs, arrivals)
+    server.close()
+    assert st['stream_bytes'] == sum(len(r.stream_bytes) for r in done) > 0
+    assert int(done[-1].cfg[0]['tile_cols']) == 2
+    for r in done:
+        md = wf.WavefrontMD(z, r.cfg)
+        try:
+            md.upload(r.img[None])
+            md.run(use_graph=False)
+            z.sync()
+            alone = md.write_stills(0, 1, threads=2)[0]
+        finally:
+            md.close()
+        assert alone == r.stream_bytes, r.idx
+        h, w = r.img.shape
+        frame, info, _, _ = oracle.av1_decode_still(r.stream_bytes, w, h, r.bd)
+        assert np.array_equal(frame, r.recon), r.idx
+    print('%d requests, %d stream bytes' % (len(done), st['stream_bytes']), flush=True)

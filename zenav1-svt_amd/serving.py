@@ -16,34 +16,69 @@ fired; its reconstruction and decisions are read back, its device memory is
 released, and the next arrivals are admitted against the freed budget.
 Latency runs from a request's arrival to its completion.  Requests share no
 device state, so each result is the one the request gives alone.
+
+Encode mode (`encode=True`): every plan keeps its winners' levels and uses
+the fewest uniform tile columns within AV1's tile-area limit; when a
+request's decision completes, its AV1 still stream is written on a host
+thread pool (zav1_wf_write_stills reads the decisions back on the
+request's stream, then codes the tiles) while the loop keeps feeding the
+device; the request completes -- and its device memory is released -- when
+its stream is written.  Latency then runs from arrival to the stream.
 """
 import collections
+import importlib.util
+import os
+import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 
+def _bitstream_module():
+    name = 'zav1_bitstream'
+    if name not in sys.modules:
+        spec = importlib.util.spec_from_file_location(name, os.path.join(os.path.dirname(__file__), 'bitstream.py'))
+        m = importlib.util.module_from_spec(spec)
+        sys.modules[name] = m
+        spec.loader.exec_module(m)
+    return sys.modules[name]
+
+
 class Request:
     __slots__ = ('idx', 'arrival', 'img', 'bd', 'qp', 'preset', 'cfg', 'footprint', 'md', 'stream', 'next_step',
-                 'events', 't_admit', 't_done', 'recon', 'decisions', 'deferred')
+                 'events', 't_admit', 't_done', 'recon', 'decisions', 'deferred', 'write', 'stream_bytes')
 
     def __init__(self, idx, arrival, img, bd, qp, preset):
         self.idx, self.arrival, self.img, self.bd, self.qp, self.preset = idx, arrival, img, bd, qp, preset
         self.md = self.stream = self.t_admit = self.t_done = self.recon = self.decisions = None
+        self.write = self.stream_bytes = None
         self.next_step, self.events, self.deferred = 0, collections.deque(), False
 
 
 class WavefrontServer:
-    def __init__(self, zav1, wf, synth, budget_bytes, max_active=8, chunk=64, depth=2, leaf=16):
+    def __init__(self, zav1, wf, synth, budget_bytes, max_active=8, chunk=64, depth=2, leaf=16, encode=False,
+                 writers=4, write_threads=4):
         self.z, self.wf, self.synth = zav1, wf, synth
         self.L = zav1.lib()
         self.budget, self.max_active, self.chunk, self.depth, self.leaf = budget_bytes, max_active, chunk, depth, leaf
         self.pool_ev = []
+        self.encode, self.write_threads = encode, write_threads
+        self.writers = ThreadPoolExecutor(writers) if encode else None
+        self.bs = _bitstream_module() if encode else None
 
     def request_cfg(self, h, w, bd, qp, preset):
         qidx = self.synth.qp_to_qindex(qp)
         lam = int(self.L.zav1_kf_full_lambda(qidx, qp, bd))
-        return self.wf.funnel_cfg(w, h, bd, qidx, lam, preset, qp, leaf=self.leaf)
+        if not self.encode:
+            return self.wf.funnel_cfg(w, h, bd, qidx, lam, preset, qp, leaf=self.leaf)
+        cfg = self.wf.funnel_cfg(w, h, bd, qidx, lam, preset, qp, leaf=self.leaf,
+                                 tiles=(self.bs.legal_tile_cols(w, h, 64), 1))
+        cfg[0]['keep_levels'] = 1
+        return cfg
+
+    def _write(self, r):
+        return r.md.write_stills(0, 1, threads=self.write_threads, stream=r.stream)[0]
 
     def _event(self):
         if self.pool_ev:
@@ -94,7 +129,17 @@ class WavefrontServer:
                     self.z.event_record(e, r.stream)
                     r.events.append(e)
                     progressed = True
-            done = [r for r in active if r.next_step == r.md.info.n_steps and not r.events]
+            decided = [r for r in active if r.next_step == r.md.info.n_steps and not r.events]
+            done = []
+            for r in decided:
+                if not self.encode:
+                    done.append(r)
+                elif r.write is None:
+                    r.write = self.writers.submit(self._write, r)  # the stream, off the feeding thread
+                    progressed = True
+                elif r.write.done():
+                    r.stream_bytes = r.write.result()
+                    done.append(r)
             for r in done:
                 r.t_done = time.perf_counter() - t0
                 if keep_results:
@@ -112,9 +157,14 @@ class WavefrontServer:
         mp = sum(r.img.size for r in reqs) / 1e6
         stats = {'wall_s': wall, 'megapixels': mp, 'latency_s': lat, 'peak_active': peak_active,
                  'peak_bytes': peak_used, 'deferred_by_memory': deferred}
+        if self.encode:
+            stats['stream_bytes'] = sum(len(r.stream_bytes) for r in reqs)
         return reqs, stats
 
     def close(self):
         for e in self.pool_ev:
             self.z.event_destroy(e)
         self.pool_ev = []
+        if self.writers:
+            self.writers.shutdown()
+            self.writers = None
