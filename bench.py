@@ -518,10 +518,32 @@ def config4_stage(zav1, synth, args):
     same = bool(np.array_equal(md.recon(), rec) and np.array_equal(md.raw_decisions(), dec))
     info = md.info
     md.close()
+    # the frame encoded: the same decision with the levels kept, then its
+    # stream (16 tiles on 16 host threads) -- one frame, not pipelined
+    cfg[0]['keep_levels'] = 1
+    mde = wf.WavefrontMD(zav1, cfg)
+    mde.upload(img[None])
+    mde.run(True, st)
+    mde.autotune(stream=st)
+    mde.run(True, st)
+    first = mde.write_stills(0, 1, threads=16, stream=st)[0]
+    enc_ms, stream_same = [], True
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        mde.run(True, st)
+        got = mde.write_stills(0, 1, threads=16, stream=st)[0]
+        enc_ms.append((time.perf_counter() - t0) * 1000.0)
+        stream_same &= got == first
+    mde.close()
     zav1.stream_destroy(st)
+    enc = float(np.median(enc_ms))
     return {'metric': 'MP/s through the wavefront mode decision, 16 tiles co-scheduled',
             'value': round(W * H / 1e6 / (ms / 1000.0), 2), 'unit': 'MP/s', 'ms_per_step': round(ms, 2),
             'steps': steps, 'graph_vs_eager_identical': same, 'wavefront_steps': info.n_steps, 'autotune': tuned,
+            'encoded': {'value': round(W * H / 1e6 / (enc / 1000.0), 2), 'unit': 'MP/s', 'ms_per_frame': round(enc, 2),
+                        'stream_bytes': len(first), 'streams_identical': bool(stream_same),
+                        'what': 'decision + AV1 stream of the frame (16 tiles on 16 host threads), one frame at a '
+                                'time, median of the timed frames'},
             'max_leaves_per_step': info.max_leaves_per_step,
             'config': {'workload': 'config4: 8192x4096 10-bit luma (1/f noise + 25%% screen regions), SB128, 4x4 '
                                    'tiles, preset 2 (%d candidates, %d -> %d survivors), qp 28, 16x16 leaves'
