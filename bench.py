@@ -550,6 +550,116 @@ def config4_stage(zav1, synth, args):
                                    % (int(cfg[0]['n_cands']), int(cfg[0]['k1']), int(cfg[0]['k3']))}}
 
 
+def preset8_stage(zav1, synth, args):
+    """The luma path at preset 8 end to end, as the reference runs it there
+    (the partition pre-pass decides the tree, the mode decision codes its
+    leaves, pipeline.rs:8000-8060): per batch of images, the PD0 pass
+    (level 1, variance cap, small-eob rate) on the device, its leaf maps
+    read back, a wavefront plan made from them on the host (leaves 8x8 ..
+    64x64, their coding-order dependencies), the preset-8 funnel over the
+    leaves on the device (eager: a plan serves one batch), each image's
+    stream written on host threads.  Timed once part by part, then
+    pipelined: the next batch's PD0 and plan overlap this batch's decision,
+    the previous batch's streams and the release of its plan.  Every
+    pipelined batch's streams must equal the part-by-part batch's."""
+    from concurrent.futures import ThreadPoolExecutor
+    wf = _load('zav1_wavefront', os.path.join(PKG, 'wavefront.py'))
+    part = _load('zav1_partition', os.path.join(PKG, 'partition.py'))
+    bsm = _load('zav1_bitstream', os.path.join(PKG, 'bitstream.py'))
+    L = zav1.lib()
+    S, n, qp, preset, bd = args.size, args.p8_images, 40, 8, 10
+    qidx = synth.qp_to_qindex(qp)
+    lam = int(L.zav1_kf_full_lambda(qidx, qp, bd))
+    cores = min(16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(cores) as ex:
+        imgs = np.stack(list(ex.map(lambda i: synth.config2_luma(S, S, seed=2000 + i), range(n))))
+    tc = bsm.legal_tile_cols(S, S, 64)
+    tiles = (tc, 1)
+    # the pass's tiles are given in 64x64 units: the frame's tile columns
+    pcfg = part.pd0_config(zav1, S, S, bd, qp, qidx, preset, n_images=n, tiles=(-(-(S // 64) // tc), S // 64))
+    d_plane = zav1.DeviceArray.from_host(np.ascontiguousarray(imgs.reshape(n * S, S), np.uint16))
+    pp = part.PartitionPass(zav1, pcfg)
+    st_p = zav1.stream_create()
+    sts = [zav1.stream_create() for _ in range(2)]
+
+    def pd0():
+        pp.run(d_plane, st_p)
+        return np.ascontiguousarray(pp.d_leaf.to_host(st_p).reshape(-1))  # synchronous read-back
+
+    def make_plan(lm, st):
+        cfg = wf.tree_cfg(S, S, bd, qidx, lam, preset, qp, lm, n_images=n, tiles=tiles)
+        cfg[0]['keep_levels'] = 1
+        md = wf.WavefrontMD(zav1, cfg)
+        zav1.check(L.zav1_memcpy_d2d(md.info.d_src, d_plane.ptr, d_plane.nbytes, st), 'memcpy_d2d')
+        return md
+
+    def write_and_release(md, st):
+        got = md.write_stills(threads=cores, stream=st)
+        md.close()
+        return got
+
+    # part by part, one batch (after one untimed batch)
+    md = make_plan(pd0(), sts[0])
+    md.run(False, sts[0])
+    write_and_release(md, sts[0])
+    part_ms = {}
+    t = time.perf_counter()
+    lm = pd0()
+    part_ms['pd0_and_map_readback'] = time.perf_counter() - t
+    t = time.perf_counter()
+    md = make_plan(lm, sts[0])
+    zav1.stream_sync(sts[0])
+    part_ms['plan'] = time.perf_counter() - t
+    info = md.info
+    n_leaves, n_steps = info.n_leaves, info.n_steps
+    t = time.perf_counter()
+    md.run(False, sts[0])
+    zav1.stream_sync(sts[0])
+    part_ms['decision'] = time.perf_counter() - t
+    t = time.perf_counter()
+    ref = md.write_stills(threads=cores, stream=sts[0])
+    part_ms['streams'] = time.perf_counter() - t
+    t = time.perf_counter()
+    md.close()
+    part_ms['release'] = time.perf_counter() - t
+    seq_ms = sum(part_ms.values()) * 1000.0
+    cnt = np.bincount(lm, minlength=7)
+    leaves_by_size = {'%dx%d' % (1 << v, 1 << v): int(cnt[v] // ((1 << v) // 4) ** 2) for v in range(3, 7)}
+    # pipelined
+    K = max(2, args.steps)
+    same = True
+    with ThreadPoolExecutor(1) as planner, ThreadPoolExecutor(1) as writer:
+        t0 = time.perf_counter()
+        fut = planner.submit(make_plan, pd0(), sts[0])
+        wfut = None
+        for k in range(K):
+            md = fut.result()
+            md.run(False, sts[k % 2])
+            if k + 1 < K:
+                fut = planner.submit(make_plan, pd0(), sts[(k + 1) % 2])
+            if wfut is not None:
+                same &= wfut.result() == ref
+            wfut = writer.submit(write_and_release, md, sts[k % 2])
+        same &= wfut.result() == ref
+        t1 = time.perf_counter()
+    pp.close()
+    d_plane.free()
+    for s in [st_p] + sts:
+        zav1.stream_destroy(s)
+    ms = (t1 - t0) * 1000.0 / K
+    mp = n * S * S / 1e6
+    nbytes = sum(len(x) for x in ref)
+    return {'metric': 'MP/s encoded at preset 8 (PD0 tree + mode decision of its leaves + streams)',
+            'value': round(mp / (ms / 1000.0), 2), 'unit': 'MP/s', 'ms_per_batch': round(ms, 2), 'batches': K,
+            'part_by_part': {'value': round(mp / (seq_ms / 1000.0), 2), 'ms_per_batch': round(seq_ms, 2),
+                             'ms': {k: round(v * 1000.0, 2) for k, v in part_ms.items()}},
+            'streams_identical': bool(same), 'stream_bytes_per_batch': nbytes,
+            'bits_per_pixel': round(8.0 * nbytes / (n * S * S), 4),
+            'leaves': n_leaves, 'leaves_by_size': leaves_by_size, 'wavefront_steps': n_steps, 'host_threads': cores,
+            'config': {'workload': 'preset 8: %d x %dx%d 10-bit luma (the headline content), qp 40, PD0 level 1 '
+                                   'tree, preset-8 funnel over its leaves, SB64, %dx%d tiles' % (n, S, S, tc, 1)}}
+
+
 def serving_stage(zav1, synth, args):
     """Serving mix (config 5) encoded at a fixed Poisson arrival rate:
     requests of log-uniform area 256^2..4096x3072, 8 / 10 bits, qp 20..50,
@@ -941,8 +1051,9 @@ def main():
     ap.add_argument('--plain-leaves', type=int, default=48)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-encode', action='store_true', help='skip the encoded (bitstream) leg')
-    ap.add_argument('--extras', default='lf,rdoq,serving,config4',
-                    help='comma list of lf, rdoq, txsearch, config4, serving')
+    ap.add_argument('--extras', default='lf,rdoq,serving,config4,preset8',
+                    help='comma list of lf, rdoq, txsearch, config4, serving, preset8')
+    ap.add_argument('--p8-images', type=int, default=16, help='images per batch of the preset-8 leg')
     ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--pmc-summary', nargs=4, metavar=('FETCH_DIR', 'WRITE_DIR', 'CAL_FETCH_DIR', 'CAL_WRITE_DIR'),
                     help='print the per-kernel HBM traffic summary of rocprofv3 --pmc CSV runs (merged over '
@@ -995,6 +1106,8 @@ def main():
         out['config4'] = config4_stage(zav1, synth, args)
     if 'serving' in extras:
         out['serving'] = serving_stage(zav1, synth, args)
+    if 'preset8' in extras:
+        out['preset8'] = preset8_stage(zav1, synth, args)
     print(json.dumps(out))
 
 

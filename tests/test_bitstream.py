@@ -141,12 +141,14 @@ def test_writer_refuses_bad_input(zav1, oracle):
     assert write(zav1, 128, 128, 8, 64, (1, 1), 0, leaves, levels)[0] == zav1.ZAV1_EINVAL     # lossless
     gap = leaves[1:].copy()
     assert write(zav1, 128, 128, 8, 64, (1, 1), 100, gap, levels)[0] == zav1.ZAV1_EINVAL     # uncovered cells
+    assert 'cells no leaf covers' in zav1.last_error()
     loose = leaves.copy()
     k = int(np.nonzero(loose['eob'] > 1)[0][0])
     sc = scan_of(oracle, int(loose[k]['size']), int(loose[k]['tx_type']))
     lv2 = levels.copy()
     lv2[int(loose[k]['level_off']) + int(sc[int(loose[k]['eob']) - 1])] = 0
     assert write(zav1, 128, 128, 8, 64, (1, 1), 100, loose, lv2)[0] == zav1.ZAV1_EINVAL       # eob not tight
+    assert 'last nonzero is not at the eob' in zav1.last_error()
 
 
 def leaves_of_decisions(dec, lev, cells, leaf=None, leaf_map=None, cands=None):

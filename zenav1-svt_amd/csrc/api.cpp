@@ -21,6 +21,14 @@ namespace {
 
 thread_local std::string g_err;
 
+}  // namespace
+
+namespace zav1 {
+void set_last_error(const std::string& what) { g_err = what; }
+}  // namespace zav1
+
+namespace {
+
 int fail(hipError_t e, const char* where) {
   g_err = std::string(where) + ": " + hipGetErrorString(e);
   return e == hipErrorOutOfMemory ? ZAV1_ENOMEM : ZAV1_EIO;

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -562,14 +563,18 @@ namespace zav1 {
 
 // One still into `stream` (ZAV1_OK) or an error code; shared by
 // zav1_write_still and the wavefront's batch writer.
-int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream) {
+int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream, std::string* why) {
+  auto fail = [why](int rc, const char* what) {
+    if (why) *why = what;
+    return rc;
+  };
   static_assert(sizeof(zav1_still_leaf) == 32, "still leaf layout");
   if (!d || (d->bd != 8 && d->bd != 10) || (d->sb != 64 && d->sb != 128) || d->width <= 0 ||
       d->height <= 0 || d->width % d->sb || d->height % d->sb || d->qindex < 1 || d->qindex > 255 ||
       d->n_leaves < 1 || !d->leaves || !d->levels || d->tile_cols < 1 || d->tile_rows < 1 ||
       (d->tile_cols & (d->tile_cols - 1)) || (d->tile_rows & (d->tile_rows - 1)) || d->width > 65536 ||
       d->height > 65536 || d->threads < 0)
-    return ZAV1_EINVAL;
+    return fail(ZAV1_EINVAL, "frame description");
   Frame f;
   f.W = d->width, f.H = d->height, f.bd = d->bd, f.sb = d->sb, f.qindex = d->qindex;
   f.mi_cols = f.W / 4, f.mi_rows = f.H / 4;
@@ -582,7 +587,7 @@ int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream) {
   const int min_tiles = std::max(min_c, tile_log2((4096 * 2304) >> (2 * sb_size), sb_rows * sb_cols));
   const int min_r = std::max(min_tiles - f.tile_cols_log2, 0);
   if (f.tile_cols_log2 < min_c || f.tile_cols_log2 > max_c || f.tile_rows_log2 < min_r || f.tile_rows_log2 > max_r)
-    return ZAV1_ENOTSUP;
+    return fail(ZAV1_ENOTSUP, "tile layout outside the frame's tile limits");
   const int tws = (sb_cols + (1 << f.tile_cols_log2) - 1) >> f.tile_cols_log2;
   const int ths = (sb_rows + (1 << f.tile_rows_log2) - 1) >> f.tile_rows_log2;
   for (int s = 0; s < sb_cols; s += tws) f.col_starts.push_back(s << sb_shift);
@@ -603,26 +608,26 @@ int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream) {
         l.y % l.size || l.x + l.size > f.W || l.y + l.size > f.H || l.mode > 12 || l.delta < -3 || l.delta > 3 ||
         (l.delta != 0 && (l.mode < 1 || l.mode > 8)) || l.fi > 5 || (l.fi < 5 && (l.mode != 0 || l.size > 32)) ||
         l.tx_type > 15 || l.eob < 0 || l.eob > n * n)
-      return ZAV1_EINVAL;
+      return fail(ZAV1_EINVAL, "leaf position, size, mode or eob");
     const int eset = intra_eset(l.size);
     if (l.eob > 0 && (eset == 0 ? l.tx_type != 0 : (eset == 1 ? kSym7[l.tx_type] : kSym5[l.tx_type]) < 0))
-      return ZAV1_EINVAL;
+      return fail(ZAV1_EINVAL, "transform type outside the leaf size's intra set");
     for (int y = l.y / 8; y < (l.y + l.size) / 8; y++)
       for (int x = l.x / 8; x < (l.x + l.size) / 8; x++) {
         int32_t& a = f.leaf_at[(size_t)y * cw + x];
-        if (a >= 0) return ZAV1_EINVAL;
+        if (a >= 0) return fail(ZAV1_EINVAL, "overlapping leaves");
         a = i;
       }
   }
   for (int32_t a : f.leaf_at)
-    if (a < 0) return ZAV1_EINVAL;
+    if (a < 0) return fail(ZAV1_EINVAL, "cells no leaf covers");
   // each leaf's levels must end at its eob (a tight eob: the last coded
   // position nonzero)
   static const ScanSet scans;
   for (const Leaf& l : leaves)
     if (l.eob > 0) {
       const uint16_t* sc = scans.s[lg2i(l.size) - 2][scan_class(l.tx_type)].data();
-      if (l.q[sc[l.eob - 1]] == 0) return ZAV1_EINVAL;
+      if (l.q[sc[l.eob - 1]] == 0) return fail(ZAV1_EINVAL, "levels whose last nonzero is not at the eob");
     }
   f.leaves = &leaves;
   f.scans = &scans;
@@ -649,11 +654,19 @@ int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream) {
 
 }  // namespace zav1
 
+namespace zav1 {
+void set_last_error(const std::string& what);  // zav1_last_error()'s text on this thread
+}
+
 extern "C" int zav1_write_still(const zav1_still_desc* d, uint8_t* out, size_t cap, size_t* size) {
   if (!size) return ZAV1_EINVAL;
   std::vector<uint8_t> stream;
-  const int rc = zav1::write_still_vec(d, stream);
-  if (rc) return rc;
+  std::string why;
+  const int rc = zav1::write_still_vec(d, stream, &why);
+  if (rc) {
+    zav1::set_last_error("zav1_write_still: " + why);
+    return rc;
+  }
   *size = stream.size();
   if (!out || cap < stream.size()) return ZAV1_ENOMEM;
   std::memcpy(out, stream.data(), stream.size());

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -39,7 +40,7 @@
 
 namespace zav1 {
 // the host AV1 still writer (bitstream.cpp): one still into `stream`
-int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream);
+int write_still_vec(const zav1_still_desc* d, std::vector<uint8_t>& stream, std::string* why = nullptr);
 }  // namespace zav1
 
 namespace zav1 {
@@ -318,6 +319,11 @@ struct Wavefront {
   size_t bytes = 0;
   std::vector<int> step_off;      // per step: first unit (n_steps + 1 entries)
   std::vector<uint8_t> step_sz;   // per step: its leaf size class (8, 16, 32, 64 -> 0..3)
+  std::vector<int> step_layer;    // per step: its dependency layer (a layer's steps are independent)
+  // a layer's other leaf sizes run on these streams beside the caller's
+  // (plans with more than one step in a layer), forked and joined by events
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[3] = {nullptr, nullptr, nullptr};
   std::vector<uint8_t> h_sz;      // per unit: size class
   std::vector<MdUnit> h_units;
   std::vector<int32_t> h_nbr;
@@ -440,11 +446,13 @@ static int plan(Wavefront* w) {
   tile_edges(sbc, tcols, tc);
   tile_edges(sbr, trows, tr);
   struct Leaf {
-    int img, x, y, L, tx0, tx1, ty0, ty1, layer;
+    int img, x, y, L, tx0, tx1, ty0, ty1, layer, ntr, nbl;
   };
   std::vector<Leaf> leaves;
+  std::vector<int> img_k0(c.n_images + 1, 0);  // each image's leaves are one range
   const int uw = c.width / 64, uh = c.height / 64;
-  for (int img = 0; img < c.n_images; img++)
+  for (int img = 0; img < c.n_images; img++) {
+    img_k0[img] = (int)leaves.size();
     for (int ty = 0; ty < trows; ty++)
       for (int tx = 0; tx < tcols; tx++) {
         const int x0 = tc[tx] * c.sb, x1 = std::min(c.width, tc[tx + 1] * c.sb);
@@ -467,7 +475,7 @@ static int plan(Wavefront* w) {
                 while (sp) {
                   const Node nd = stack[--sp];
                   if ((1 << m[(nd.oy / 4) * 16 + nd.ox / 4]) == nd.sq) {
-                    leaves.push_back({img, ux + nd.ox, uy + nd.oy, nd.sq, x0, x1, y0, y1, 0});
+                    leaves.push_back({img, ux + nd.ox, uy + nd.oy, nd.sq, x0, x1, y0, y1, 0, 0, 0});
                     continue;
                   }
                   const int h = nd.sq / 2;
@@ -484,79 +492,141 @@ static int plan(Wavefront* w) {
                 }
                 const int x = q * c.sb + j * L, y = r * c.sb + i * L;
                 if (x >= c.width || y >= c.height) continue;
-                leaves.push_back({img, x, y, L, x0, x1, y0, y1, 0});
+                leaves.push_back({img, x, y, L, x0, x1, y0, y1, 0, 0, 0});
               }
             }
           }
       }
+  }
   const int n = (int)leaves.size();
+  img_k0[c.n_images] = n;
   w->n_units = n;
   // 8x8 cells -> leaf (coding-order index), per image
   const int gw = c.width / 8, gh = c.height / 8;
   std::vector<int32_t> cell((size_t)c.n_images * gw * gh, -1);
   std::vector<MdUnit> units(n);
   std::vector<int32_t> nbr(2 * (size_t)n);
-  int n_layers = 0;
-  for (int k = 0; k < n; k++) {
-    Leaf& f = leaves[k];
-    const int L = f.L, nl = c.sb / L;
-    const int i = (f.y % c.sb) / L, j = (f.x % c.sb) / L, zi = zorder(i, j);
-    const bool up = f.y > f.ty0, lf = f.x > f.tx0;
-    // top-right: the row above; inside the superblock only when that square
-    // precedes this one in Z-order, never in the superblock to the right
-    bool tr_ok;
-    if (!up) tr_ok = false;
-    else if (i == 0) tr_ok = true;
-    else tr_ok = j + 1 < nl && zorder(i - 1, j + 1) < zi;
-    // bottom-left: the left superblock's rows of this superblock row, or an
-    // earlier square of this superblock
-    bool bl_ok;
-    if (!lf) bl_ok = false;
-    else if (j == 0) bl_ok = i + 1 < nl;
-    else bl_ok = i + 1 < nl && zorder(i + 1, j - 1) < zi;
-    const int ntr = tr_ok ? std::max(0, std::min(L, f.tx1 - (f.x + L))) : 0;
-    const int nbl = bl_ok ? std::max(0, std::min(L, f.ty1 - (f.y + L))) : 0;
-    MdUnit mu{};
-    mu.x = f.x;
-    mu.y = f.y + f.img * c.height;
-    mu.n_top_px = (int16_t)(up ? L : 0);
-    mu.n_topright_px = (int16_t)ntr;
-    mu.n_left_px = (int16_t)(lf ? L : 0);
-    mu.n_bottomleft_px = (int16_t)nbl;
-    mu.edge_filter = 1;
-    units[k] = mu;
-    const size_t g = (size_t)f.img * gw * gh;
-    auto at = [&](int x, int y) { return cell[g + (size_t)(y / 8) * gw + x / 8]; };
-    nbr[2 * k] = up ? at(f.x, f.y - 1) : -1;
-    nbr[2 * k + 1] = lf ? at(f.x - 1, f.y) : -1;
-    int layer = 0;
-    auto dep = [&](int x, int y) {
-      const int id = at(x, y);
-      if (id >= 0) layer = std::max(layer, leaves[id].layer + 1);
+  // images are independent: their leaves' neighbours and layers are found on
+  // host threads, one image range each
+  const int n_th = std::max(1, std::min({c.n_images, 16, (int)std::max(1u, std::thread::hardware_concurrency())}));
+  std::vector<int> th_layers(n_th, 0);
+  auto layer_images = [&](int th) {
+    int& n_layers = th_layers[th];
+    for (int k = img_k0[th * c.n_images / n_th]; k < img_k0[(th + 1) * c.n_images / n_th]; k++) {
+      Leaf& f = leaves[k];
+      const int L = f.L, nl = c.sb / L;
+      const int i = (f.y % c.sb) / L, j = (f.x % c.sb) / L, zi = zorder(i, j);
+      const bool up = f.y > f.ty0, lf = f.x > f.tx0;
+      // top-right: the row above; inside the superblock only when that square
+      // precedes this one in Z-order, never in the superblock to the right
+      bool tr_ok;
+      if (!up) tr_ok = false;
+      else if (i == 0) tr_ok = true;
+      else tr_ok = j + 1 < nl && zorder(i - 1, j + 1) < zi;
+      // bottom-left: the left superblock's rows of this superblock row, or an
+      // earlier square of this superblock
+      bool bl_ok;
+      if (!lf) bl_ok = false;
+      else if (j == 0) bl_ok = i + 1 < nl;
+      else bl_ok = i + 1 < nl && zorder(i + 1, j - 1) < zi;
+      const int ntr = tr_ok ? std::max(0, std::min(L, f.tx1 - (f.x + L))) : 0;
+      const int nbl = bl_ok ? std::max(0, std::min(L, f.ty1 - (f.y + L))) : 0;
+      f.ntr = ntr, f.nbl = nbl;
+      MdUnit mu{};
+      mu.x = f.x;
+      mu.y = f.y + f.img * c.height;
+      mu.n_top_px = (int16_t)(up ? L : 0);
+      mu.n_topright_px = (int16_t)ntr;
+      mu.n_left_px = (int16_t)(lf ? L : 0);
+      mu.n_bottomleft_px = (int16_t)nbl;
+      mu.edge_filter = 1;
+      units[k] = mu;
+      const size_t g = (size_t)f.img * gw * gh;
+      auto at = [&](int x, int y) { return cell[g + (size_t)(y / 8) * gw + x / 8]; };
+      nbr[2 * k] = up ? at(f.x, f.y - 1) : -1;
+      nbr[2 * k + 1] = lf ? at(f.x - 1, f.y) : -1;
+      int layer = 0;
+      auto dep = [&](int x, int y) {
+        const int id = at(x, y);
+        if (id >= 0) layer = std::max(layer, leaves[id].layer + 1);
+      };
+      if (up)
+        for (int x = lf ? f.x - 1 : f.x; x < f.x + L + ntr; x += 8) dep(x, f.y - 1);
+      if (lf)
+        for (int y = f.y; y < f.y + L + nbl; y += 8) dep(f.x - 1, y);
+      f.layer = layer;
+      n_layers = std::max(n_layers, layer + 1);
+      for (int y = f.y; y < f.y + L; y += 8)
+        for (int x = f.x; x < f.x + L; x += 8) cell[g + (size_t)(y / 8) * gw + x / 8] = k;
+    }
+  };
+  if (n_th == 1) {
+    layer_images(0);
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < n_th; t++) pool.emplace_back(layer_images, t);
+    for (auto& t : pool) t.join();
+  }
+  int n_layers = *std::max_element(th_layers.begin(), th_layers.end());
+  if (tree) {
+    // the layers above are as soon as possible; a step is one (layer, leaf
+    // size) pair, so leaves of a size with slack are moved to a layer that
+    // already holds a step of their size.  Each leaf's latest layer (its
+    // dependents' latest less one) bounds the move, so no dependency breaks
+    // and the layer count does not grow; any order that keeps the
+    // dependencies gives the same decisions as coding order.
+    auto for_deps = [&](const Leaf& f, auto&& fn) {
+      const size_t g = (size_t)f.img * gw * gh;
+      auto at = [&](int x, int y) { return cell[g + (size_t)(y / 8) * gw + x / 8]; };
+      const bool up = f.y > f.ty0, lf = f.x > f.tx0;
+      if (up)
+        for (int x = lf ? f.x - 1 : f.x; x < f.x + f.L + f.ntr; x += 8) fn(at(x, f.y - 1));
+      if (lf)
+        for (int y = f.y; y < f.y + f.L + f.nbl; y += 8) fn(at(f.x - 1, y));
     };
-    if (up)
-      for (int x = lf ? f.x - 1 : f.x; x < f.x + L + ntr; x += 8) dep(x, f.y - 1);
-    if (lf)
-      for (int y = f.y; y < f.y + L + nbl; y += 8) dep(f.x - 1, y);
-    f.layer = layer;
-    n_layers = std::max(n_layers, layer + 1);
-    for (int y = f.y; y < f.y + L; y += 8)
-      for (int x = f.x; x < f.x + L; x += 8) cell[g + (size_t)(y / 8) * gw + x / 8] = k;
+    std::vector<int> late(n, n_layers - 1);
+    for (int k = n - 1; k >= 0; k--)
+      for_deps(leaves[k], [&](int id) {
+        if (id >= 0) late[id] = std::min(late[id], late[k] - 1);
+      });
+    std::set<int> open[4];  // per size class: the layers holding a step of it
+    n_layers = 0;
+    for (int k = 0; k < n; k++) {
+      Leaf& f = leaves[k];
+      int lo = 0;
+      for_deps(f, [&](int id) {
+        if (id >= 0) lo = std::max(lo, leaves[id].layer + 1);
+      });
+      const int sc = size_class(f.L), hi = late[k];
+      const auto it = open[sc].lower_bound(lo);
+      int at;
+      if (it != open[sc].end() && *it <= hi) {
+        at = *it;
+      } else {
+        at = hi;
+        open[sc].insert(at);
+      }
+      f.layer = at;
+      n_layers = std::max(n_layers, at + 1);
+    }
   }
   w->n_layers = n_layers;
-  // storage order: by (step, leaf size), coding order within
+  // storage order: by (step, leaf size), coding order within -- a counting
+  // sort over the keys layer x 4 + size class (stable)
   std::vector<int> perm(n), pos(n);
-  for (int k = 0; k < n; k++) perm[k] = k;
-  std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
-    const Leaf &la = leaves[a], &lb = leaves[b];
-    return la.layer != lb.layer ? la.layer < lb.layer : size_class(la.L) < size_class(lb.L);
-  });
+  {
+    std::vector<int> start((size_t)n_layers * 4 + 1, 0);
+    for (int k = 0; k < n; k++) start[(size_t)leaves[k].layer * 4 + size_class(leaves[k].L) + 1]++;
+    for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
+    for (int k = 0; k < n; k++) perm[start[(size_t)leaves[k].layer * 4 + size_class(leaves[k].L)]++] = k;
+  }
   for (int s = 0; s < n; s++) pos[perm[s]] = s;
   w->h_units.resize(n);
   w->h_nbr.resize(2 * (size_t)n);
   w->h_sz.resize(n);
   w->step_off.clear();
   w->step_sz.clear();
+  w->step_layer.clear();
   for (int k = 0; k < 4; k++) w->sp[k].max_u = 0;
   for (int s = 0; s < n; s++) {
     const int k = perm[s];
@@ -569,6 +639,7 @@ static int plan(Wavefront* w) {
     if (s == 0 || f.layer != leaves[perm[s - 1]].layer || sc != w->h_sz[s - 1]) {
       w->step_off.push_back(s);
       w->step_sz.push_back((uint8_t)sc);
+      w->step_layer.push_back(f.layer);
     }
   }
   w->step_off.push_back(n);
@@ -710,6 +781,36 @@ static hipError_t step_chain(Wavefront* w, int s, hipStream_t st, StageClock* cl
   return hipGetLastError();
 }
 #undef MARK
+
+// Steps [s0, s1) in order on `st`; the steps of one layer (one per leaf
+// size present) are independent, so when the plan has side streams the
+// layer's first step runs on `st` and the others on side streams, forked
+// from and joined back into `st` by events (captured as parallel branches
+// when `st` is capturing).
+static hipError_t run_range(Wavefront* w, int s0, int s1, hipStream_t st) {
+  for (int s = s0; s < s1;) {
+    int e = s + 1;
+    while (e < s1 && w->step_layer[e] == w->step_layer[s]) e++;
+    if (e - s == 1 || !w->side[0]) {
+      for (; s < e; s++) {
+        const hipError_t r = step_chain(w, s, st);
+        if (r != hipSuccess) return r;
+      }
+      continue;
+    }
+    hipError_t r = hipEventRecord(w->fork, st);
+    for (int k = 1; k < e - s && r == hipSuccess; k++) {
+      r = hipStreamWaitEvent(w->side[k - 1], w->fork, 0);
+      if (r == hipSuccess) r = step_chain(w, s + k, w->side[k - 1]);
+      if (r == hipSuccess) r = hipEventRecord(w->join[k - 1], w->side[k - 1]);
+    }
+    if (r == hipSuccess) r = step_chain(w, s, st);
+    for (int k = 1; k < e - s && r == hipSuccess; k++) r = hipStreamWaitEvent(st, w->join[k - 1], 0);
+    if (r != hipSuccess) return r;
+    s = e;
+  }
+  return hipSuccess;
+}
 
 // The plain form of one unit: one launch per elementary call with one job
 // each (every candidate's prediction + SATD, every survivor's prediction and
@@ -1056,10 +1157,23 @@ int zav1_wf_create(const zav1_wf_cfg* cfg, zav1_wf** out) {
     up(P.cands, P.h_cands.data(), P.h_cands.size() * sizeof(zav1::WfCand));
     up(P.d_types, P.types, 16);
   }
+  // side streams when a layer holds steps of several leaf sizes (default
+  // priority, non-blocking with respect to the null stream)
+  int per_layer = 1;
+  for (int s0 = 0, s1 = 0; s0 < w->n_steps; s0 = s1) {
+    for (s1 = s0 + 1; s1 < w->n_steps && w->step_layer[s1] == w->step_layer[s0];) s1++;
+    per_layer = std::max(per_layer, s1 - s0);
+  }
+  if (e == hipSuccess && per_layer > 1) {
+    e = hipEventCreateWithFlags(&w->fork, hipEventDisableTiming);
+    for (int k = 0; k < per_layer - 1 && e == hipSuccess; k++) {
+      e = hipStreamCreateWithFlags(&w->side[k], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&w->join[k], hipEventDisableTiming);
+    }
+  }
   if (e != hipSuccess) {
     const int r = wf_fail(e, "zav1_wf_create upload");
-    (void)hipFree(w->base);
-    delete w;
+    (void)zav1_wf_destroy(reinterpret_cast<zav1_wf*>(w));
     return r;
   }
   *out = reinterpret_cast<zav1_wf*>(w);
@@ -1098,7 +1212,7 @@ int zav1_wf_run(zav1_wf* wf, int use_graph, zav1_stream_t s) {
   auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
   hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
   if (!use_graph) {
-    for (int i = 0; i < w->n_steps; i++) WCHECK(zav1::step_chain(w, i, st));
+    WCHECK(zav1::run_range(w, 0, w->n_steps, st));
     return ZAV1_OK;
   }
   if (st == hipStreamPerThread) {
@@ -1113,7 +1227,7 @@ int zav1_wf_run(zav1_wf* wf, int use_graph, zav1_stream_t s) {
     hipGraph_t g = nullptr;
     WCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int i = 0; i < w->n_steps && e == hipSuccess; i++) e = zav1::step_chain(w, i, st);
+    e = zav1::run_range(w, 0, w->n_steps, st);
     const hipError_t ec = hipStreamEndCapture(st, &g);
     if (e != hipSuccess) return wf_fail(e, "zav1_wf_run capture");
     if (ec != hipSuccess) return wf_fail(ec, "hipStreamEndCapture");
@@ -1131,7 +1245,7 @@ int zav1_wf_run_steps(zav1_wf* wf, int first_step, int n_steps, zav1_stream_t s)
   auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
   if (first_step < 0 || n_steps < 0 || first_step + n_steps > w->n_steps) return ZAV1_EINVAL;
   hipStream_t st = s ? (hipStream_t)s : hipStreamPerThread;
-  for (int i = first_step; i < first_step + n_steps; i++) WCHECK(zav1::step_chain(w, i, st));
+  WCHECK(zav1::run_range(w, first_step, first_step + n_steps, st));
   return ZAV1_OK;
 }
 
@@ -1219,6 +1333,11 @@ int zav1_wf_destroy(zav1_wf* wf) {
   if (!wf) return ZAV1_EINVAL;
   auto* w = reinterpret_cast<zav1::Wavefront*>(wf);
   if (w->graph) (void)hipGraphExecDestroy(w->graph);
+  for (int k = 0; k < 3; k++) {
+    if (w->side[k]) (void)hipStreamDestroy(w->side[k]);
+    if (w->join[k]) (void)hipEventDestroy(w->join[k]);
+  }
+  if (w->fork) (void)hipEventDestroy(w->fork);
   if (w->base) (void)hipFree(w->base);
   if (w->h_dec_pin) (void)hipHostFree(w->h_dec_pin);
   if (w->h_lev_pin) (void)hipHostFree(w->h_lev_pin);
@@ -1272,13 +1391,14 @@ int zav1_wf_write_stills(zav1_wf* wf, int first, int n, int threads, uint8_t* ou
   const int img_threads = std::min(nth, n), tile_threads = std::max(1, nth / img_threads);
   std::vector<std::vector<uint8_t>> streams(n);
   std::vector<int> rcs(n, ZAV1_OK);
+  std::vector<std::string> why(n);
   auto one = [&](int i) {
     zav1_still_desc sd{};
     sd.width = c.width, sd.height = c.height, sd.bd = c.bd, sd.sb = c.sb;
     sd.tile_cols = c.tile_cols, sd.tile_rows = c.tile_rows, sd.qindex = c.qindex;
     sd.n_leaves = (int32_t)per[i].size(), sd.threads = tile_threads;
     sd.leaves = per[i].data(), sd.levels = lev;
-    rcs[i] = zav1::write_still_vec(&sd, streams[i]);
+    rcs[i] = zav1::write_still_vec(&sd, streams[i], &why[i]);
   };
   if (img_threads <= 1) {
     for (int i = 0; i < n; i++) one(i);
@@ -1292,7 +1412,10 @@ int zav1_wf_write_stills(zav1_wf* wf, int first, int n, int threads, uint8_t* ou
   }
   size_t total = 0;
   for (int i = 0; i < n; i++) {
-    if (rcs[i]) return rcs[i];
+    if (rcs[i]) {
+      g_wf_err = "zav1_wf_write_stills: image " + std::to_string(first + i) + ": " + why[i];
+      return rcs[i];
+    }
     sizes[i] = streams[i].size();
     total += sizes[i];
   }
