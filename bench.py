@@ -865,12 +865,18 @@ def wavefront_headline(zav1, synth, args):
         jobs = n_leaves * 4 * n_types
         per_launch = (trellis_jobs * 256 * 12 + jobs * 26) / n_launch
         gbs = per_launch / dom_s / 1e9
-        roof = {'bound': 'hbm', 'kernel': 'k_rdoq_group<TX_16X16, 16>' if dom == 'mds3_rdoq' else dom,
+        # the trellis kernel the autotuner kept for the 16x16 leaves
+        form = tuned['forms'].get(16, 'queued')
+        kname, kprefix = {'queued': ('k_rdoq_queue<TX_16X16, 16, 3>', 'k_rdoq_queue<2, 16, 3>'),
+                          'direct': ('k_rdoq_group<TX_16X16, 16, 3, true>', 'k_rdoq_group<2, 16, 3, true>'),
+                          'lane': ('k_rdoq<TX_16X16>', 'k_rdoq<2>')}[form]
+        roof = {'bound': 'hbm', 'kernel': kname if dom == 'mds3_rdoq' else dom,
                 'unit': 'GB/s', 'achieved': round(gbs, 3), 'peak': HBM_PEAK_GBS, 'frac': round(gbs / HBM_PEAK_GBS, 6),
-                'traffic': measured_traffic('k_rdoq_group<2, 16>'), 'algorithmic_bytes_per_launch': int(per_launch),
+                'traffic': measured_traffic(kprefix), 'algorithmic_bytes_per_launch': int(per_launch),
                 'blocks_walked_per_launch': round(trellis_jobs / n_launch, 1),
                 'jobs_per_launch': round(jobs / n_launch, 1),
-                'note': 'latency-bound: a sequential trellis walk per block (the eob phase), 16 lanes per block'}
+                'note': 'latency-bound: a sequential trellis walk per block (the eob phase), 16 lanes per block; '
+                        'the %s form, kept by the autotuner' % form}
     roof['stage'] = dom
     roof['avg_launch_us'] = round(dom_s * 1e6, 2)
     roof['launches'] = n_launch
