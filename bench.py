@@ -559,9 +559,11 @@ def preset8_stage(zav1, synth, args):
     64x64, their coding-order dependencies), the preset-8 funnel over the
     leaves on the device (eager: a plan serves one batch), each image's
     stream written on host threads.  Timed once part by part, then
-    pipelined: the next batch's PD0 and plan overlap this batch's decision,
-    the previous batch's streams and the release of its plan.  Every
-    pipelined batch's streams must equal the part-by-part batch's."""
+    pipelined: every batch's plan, decision, streams and release on a
+    worker thread with its own stream, up to three batches in flight (their
+    small steps overlap on the device), the PD0 passes on the calling
+    thread.  Every pipelined batch's streams must equal the part-by-part
+    batch's."""
     from concurrent.futures import ThreadPoolExecutor
     wf = _load('zav1_wavefront', os.path.join(PKG, 'wavefront.py'))
     part = _load('zav1_partition', os.path.join(PKG, 'partition.py'))
@@ -625,23 +627,35 @@ def preset8_stage(zav1, synth, args):
     seq_ms = sum(part_ms.values()) * 1000.0
     cnt = np.bincount(lm, minlength=7)
     leaves_by_size = {'%dx%d' % (1 << v, 1 << v): int(cnt[v] // ((1 << v) // 4) ** 2) for v in range(3, 7)}
-    # pipelined
-    K = max(2, args.steps)
+    # pipelined: each batch's plan, decision and streams on a worker thread
+    # of its own (its own stream), up to `depth` batches in flight; the PD0
+    # passes (shared buffers) stay on this thread, one batch ahead
+    import threading
+    K, depth = max(2, args.steps), 3
+    local = threading.local()
+    wsts = []
+
+    def batch(lm):
+        if not hasattr(local, 'st'):
+            local.st = zav1.stream_create()
+            wsts.append(local.st)
+        m = make_plan(lm, local.st)
+        m.run(False, local.st)
+        return write_and_release(m, local.st)
+
     same = True
-    with ThreadPoolExecutor(1) as planner, ThreadPoolExecutor(1) as writer:
+    with ThreadPoolExecutor(depth) as pool:
         t0 = time.perf_counter()
-        fut = planner.submit(make_plan, pd0(), sts[0])
-        wfut = None
+        futs = []
         for k in range(K):
-            md = fut.result()
-            md.run(False, sts[k % 2])
-            if k + 1 < K:
-                fut = planner.submit(make_plan, pd0(), sts[(k + 1) % 2])
-            if wfut is not None:
-                same &= wfut.result() == ref
-            wfut = writer.submit(write_and_release, md, sts[k % 2])
-        same &= wfut.result() == ref
+            if len(futs) >= depth:
+                same &= futs[len(futs) - depth].result() == ref
+            futs.append(pool.submit(batch, pd0()))
+        for f in futs[max(0, len(futs) - depth):]:
+            same &= f.result() == ref
         t1 = time.perf_counter()
+    for s_ in wsts:
+        zav1.stream_destroy(s_)
     pp.close()
     d_plane.free()
     for s in [st_p] + sts:
