@@ -126,3 +126,37 @@ def test_sb128_tiles(gpu, oracle):
     streams = check_streams(gpu, oracle, md, [img], 10)
     md.close()
     assert streams == oracle_streams(gpu, oracle, cfg, [img], None)
+
+
+def test_pd0_trees_10bit_two_images_graph(gpu, oracle):
+    """The preset-8 path on two 10-bit images at once: the device PD0 maps
+    (every leaf size from 8x8 to 64x64, so a layer's sizes run as parallel
+    stream branches), the plan's decision eager and as a graph, the streams
+    -- decoded to the device's reconstruction and byte-identical to the
+    streams of the oracle's decisions over the same maps."""
+    imgs = [synth.config2_luma(256, 256, seed=2400 + k) for k in range(2)]
+    qidx = synth.qp_to_qindex(40)
+    pc = part.pd0_config(gpu, 256, 256, 10, 40, qidx, 8, n_images=2, tiles=(2, 2))  # 128-pixel tiles
+    d = gpu.DeviceArray.from_host(np.ascontiguousarray(np.concatenate(imgs, axis=0)))
+    pp = part.PartitionPass(gpu, pc)
+    pp.run(d)
+    gpu.sync()
+    lm = np.ascontiguousarray(pp.results()[2].reshape(-1), np.uint8)
+    pp.close()
+    d.free()
+    sizes = set(int(v) for v in np.unique(lm))
+    print('leaf sizes (log2):', sorted(sizes), flush=True)
+    assert len(sizes) >= 3
+    cfg = wf.tree_cfg(256, 256, 10, qidx, int(gpu.lib().zav1_kf_full_lambda(qidx, 40, 10)), 8, 40, lm, n_images=2,
+                      tiles=(2, 2))
+    cfg[0]['keep_levels'] = 1
+    md = wf.WavefrontMD(gpu, cfg)
+    md.upload(np.stack(imgs))
+    streams = check_streams(gpu, oracle, md, imgs, 10)
+    st = gpu.stream_create()
+    md.run(True, st)
+    gpu.stream_sync(st)
+    assert md.write_stills(0, 2, threads=2, stream=st) == streams
+    gpu.stream_destroy(st)
+    md.close()
+    assert streams == oracle_streams(gpu, oracle, cfg, imgs, lm)
