@@ -600,6 +600,16 @@ def preset8_stage(zav1, synth, args):
         md.close()
         return got
 
+    if args.p8_profile_pass:  # one batch and nothing else, for profilers
+        md = make_plan(pd0(), sts[0])
+        md.run(False, sts[0])
+        got = write_and_release(md, sts[0])
+        pp.close()
+        d_plane.free()
+        for s_ in [st_p] + sts:
+            zav1.stream_destroy(s_)
+        return {'profile_pass': 'preset 8, %d x %dx%d 10-bit: PD0, plan, eager decision, streams' % (n, S, S),
+                'stream_bytes': sum(len(x) for x in got)}
     # part by part, one batch (after one untimed batch)
     md = make_plan(pd0(), sts[0])
     md.run(False, sts[0])
@@ -1074,6 +1084,8 @@ def main():
     ap.add_argument('--extras', default='lf,rdoq,serving,config4,preset8',
                     help='comma list of lf, rdoq, txsearch, config4, serving, preset8')
     ap.add_argument('--p8-images', type=int, default=16, help='images per batch of the preset-8 leg')
+    ap.add_argument('--p8-profile-pass', action='store_true',
+                    help='one batch of the preset-8 leg only (for rocprofv3 runs)')
     ap.add_argument('--copy-calibration', action='store_true')
     ap.add_argument('--pmc-summary', nargs=4, metavar=('FETCH_DIR', 'WRITE_DIR', 'CAL_FETCH_DIR', 'CAL_WRITE_DIR'),
                     help='print the per-kernel HBM traffic summary of rocprofv3 --pmc CSV runs (merged over '
@@ -1107,6 +1119,9 @@ def main():
         return
     synth = _load('zav1_synth', os.path.join(PKG, 'synth.py'))
     zav1.init(0)
+    if args.p8_profile_pass:
+        print(json.dumps(preset8_stage(zav1, synth, args)))
+        return
     out = wavefront_headline(zav1, synth, args)
     if args.profile_pass:
         print(json.dumps(out))
