@@ -387,3 +387,43 @@ def test_pd0_rates_host_match_oracle(zav1, oracle):
     bad = part.pd0_config(zav1, 64, 64, 8, 20, 80, 8)
     bad[0]['width'] = 96
     assert zav1.lib().zav1_pd0_run(bad.ctypes.data, 1, 1, 1, 1, 1, None) == zav1.ZAV1_EINVAL
+
+
+def test_partition_map_plans_on_the_host(zav1):
+    """The wavefront plan of partition maps is made on the host (no device
+    call): random quadtree maps over several images, SB64 and SB128, with
+    tiles, plan (images on host threads, same-size steps merged through
+    slack) without error and to the same footprint on every call; a map
+    that is not a quadtree of 8..64 squares is refused."""
+    import importlib.util
+    import pytest
+    spec = importlib.util.spec_from_file_location('zav1_wavefront_abi',
+                                                  os.path.join(ROOT, 'zenav1-svt_amd', 'wavefront.py'))
+    wfm = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(wfm)
+    rng = np.random.default_rng(5)
+
+    def unit_map():
+        m = np.zeros((16, 16), np.uint8)
+
+        def walk(sq, x, y):
+            if sq == 8 or rng.random() < 0.4:
+                m[y // 4:(y + sq) // 4, x // 4:(x + sq) // 4] = int(np.log2(sq))
+                return
+            h = sq // 2
+            for i in range(4):
+                walk(h, x + (i & 1) * h, y + (i >> 1) * h)
+        walk(64, 0, 0)
+        return m.reshape(-1)
+
+    for W, H, n, sb, tiles in ((512, 256, 3, 64, (2, 1)), (512, 512, 2, 128, (2, 2)), (256, 192, 5, 64, (1, 1))):
+        lm = np.ascontiguousarray(np.concatenate([unit_map() for _ in range(n * (W // 64) * (H // 64))]))
+        cfg = wfm.tree_cfg(W, H, 10, 100, 1000, 8, 40, lm, n_images=n, sb=sb, tiles=tiles)
+        cfg[0]['keep_levels'] = 1
+        b1 = wfm.WavefrontMD.footprint(zav1, cfg)
+        assert b1 > 0 and wfm.WavefrontMD.footprint(zav1, cfg) == b1
+    bad = lm.copy()
+    bad[:16] = 6  # a 64x64 leaf over part of a unit only
+    cfg = wfm.tree_cfg(W, H, 10, 100, 1000, 8, 40, bad, n_images=n, sb=64)
+    with pytest.raises(zav1.Zav1Error):
+        wfm.WavefrontMD.footprint(zav1, cfg)
